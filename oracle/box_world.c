@@ -1,0 +1,841 @@
+/* CPU oracle: box rigid-body step, restated from the reference's engine/src/phys.rs (World::step :152-243,
+ * warm_start :245-262, solve :264-324, sleep_islands :326-354, broadphase_and_collide :356-409, wake :411-416,
+ * collide_ground :418-432, collide_pair :434-448, push_manifold :450-498, tangents :501-504, reduce4 :507-527,
+ * collide_boxes :531-628), generalised to compound bodies (one body, several box shapes at local poses, as in
+ * gpu/gpu_world.js buildBodies :2212-2249) and per-shape friction mu = sqrt(mu_a mu_b) (gpu_world.js:601-603).
+ * With one shape per body at the identity local pose and equal frictions every f32 result equals phys.rs.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): tests, the smoke check and the benchmark's CPU-baseline leg use it. */
+#include "sf_oracle.h"
+#include "om_math.h"
+#include <stdlib.h>
+#include <stdio.h>
+
+#define NONE_U 0xFFFFFFFFu
+
+typedef struct { V3 ra, rb, la, raxn, rbxn, iaxn, ibxn; float adj, nm, jn; } Cp;
+typedef struct {
+    int32_t a, b;   /* shapes (b = -1: ground) */
+    int32_t ba, bb; /* bodies (bb = -1: ground) */
+    int32_t sa, sb; /* solver slots: body + 1 when the body is in the solver, 0 = static dummy */
+    V3 n, t1, t2; float mu; int cnt; Cp cp[4];
+    V3 fa, fb, la_f, lb_f, e0;
+    float tm1, tm2, twm, rad, jt1, jt2, jtw; int stick;
+    V3 ca1, ca2, cb1, cb2, ia1, ia2, ib1, ib2, ian, ibn; float ima, imb;
+} Man;
+
+typedef struct {
+    V3 p; Quat q; V3 v, w; float inv_m; V3 inv_i; int awake; float sleep_t; V3 sleep_p; Quat sleep_q;
+    M3 rot;
+} Body;
+typedef struct { int body; V3 lp; Quat lq; M3 lrot; V3 h; float mu; V3 wp; M3 wrot; V3 lo, hi; } Shape;
+
+struct ob_world {
+    int order;
+    float prm[OB_NPARAM];
+    int nb, ns;
+    Body* bd; Shape* sh;
+    /* solver arrays indexed by slot (0 = static dummy, body i -> i + 1) */
+    V3 *sv, *sw, *sdp, *sth, *sp0; float* sim; M3* sii; Quat* sq; int32_t* sidx;
+    Man *man, *old; int nman, nold, capman, capold;
+    int32_t* old_sorted; /* indices of old manifolds sorted by key */
+    int32_t* cand; int ncand, capcand;
+    int32_t* colour; int capcolour;
+    int32_t* island;
+    int32_t stats[8];
+    uint8_t* wake_flag;
+};
+
+/* parameter indices (same as SF_P_* in include/sf.h) */
+enum { P_GX, P_GY, P_GZ, P_SUBSTEPS, P_HERTZ, P_DAMPING, P_MAXPUSH, P_FRICTION, P_SPEC, P_SLEEP_LIN, P_SLEEP_ANG,
+       P_SLEEP_TIME, P_SLEEP_DIST, P_SLEEP_ROT, P_ALLOW_SLEEP, P_CELL, P_LEAK, P_ANCHOR, P_ANCHOR_PUSH, P_ANCHOR_SLOP,
+       P_COL_LIMIT, P_COL_ROUNDS, P_COL_SPREAD, P_RESERVED };
+
+static void default_params(float* p) {
+    /* phys.rs:101-105 */
+    p[P_GX] = 0.0f; p[P_GY] = -9.81f; p[P_GZ] = 0.0f; p[P_SUBSTEPS] = 12; p[P_HERTZ] = 0.0f; p[P_DAMPING] = 10.0f;
+    p[P_MAXPUSH] = 3.0f; p[P_FRICTION] = 0.6f; p[P_SPEC] = 0.02f; p[P_SLEEP_LIN] = 0.05f; p[P_SLEEP_ANG] = 0.08f;
+    p[P_SLEEP_TIME] = 0.5f; p[P_SLEEP_DIST] = 0.01f; p[P_SLEEP_ROT] = 0.02f; p[P_ALLOW_SLEEP] = 1.0f; p[P_CELL] = 1.0f;
+    p[P_LEAK] = 1.0f; p[P_ANCHOR] = 1.0f; p[P_ANCHOR_PUSH] = 3.0f; p[P_ANCHOR_SLOP] = 0.0f;
+    p[P_COL_LIMIT] = 64; p[P_COL_ROUNDS] = 32; p[P_COL_SPREAD] = 4; p[P_RESERVED] = 0;
+}
+
+ob_world* ob_create(int order) {
+    ob_world* w = (ob_world*)calloc(1, sizeof(ob_world));
+    w->order = order;
+    default_params(w->prm);
+    return w;
+}
+void ob_destroy(ob_world* w) {
+    if (!w) return;
+    free(w->bd); free(w->sh); free(w->sv); free(w->sw); free(w->sdp); free(w->sth); free(w->sp0); free(w->sim);
+    free(w->sii); free(w->sq); free(w->sidx); free(w->man); free(w->old); free(w->old_sorted); free(w->cand);
+    free(w->colour); free(w->island); free(w->wake_flag); free(w);
+}
+void ob_set_params(ob_world* w, const float* prm, int n) { for (int i = 0; i < n && i < OB_NPARAM; i++) w->prm[i] = prm[i]; }
+int ob_nbody(const ob_world* w) { return w->nb; }
+
+int ob_load(ob_world* w, int nbody, const float* bodies, int nshape, const int32_t* shape_body, const float* shapes) {
+    w->nb = nbody; w->ns = nshape;
+    w->bd = (Body*)realloc(w->bd, sizeof(Body) * (size_t)(nbody > 0 ? nbody : 1));
+    w->sh = (Shape*)realloc(w->sh, sizeof(Shape) * (size_t)(nshape > 0 ? nshape : 1));
+    for (int i = 0; i < nbody; i++) {
+        const float* f = bodies + (size_t)i * OB_BODY_F;
+        Body* b = &w->bd[i];
+        b->p = v3(f[0], f[1], f[2]); b->q = quat(f[3], f[4], f[5], f[6]); b->v = v3(f[7], f[8], f[9]);
+        b->w = v3(f[10], f[11], f[12]); b->inv_m = f[13]; b->inv_i = v3(f[14], f[15], f[16]);
+        b->awake = f[17] > 0.5f; b->sleep_t = f[18]; b->sleep_p = b->p; b->sleep_q = b->q;
+        b->rot = q_to_m3(b->q);
+    }
+    for (int s = 0; s < nshape; s++) {
+        const float* f = shapes + (size_t)s * OB_SHAPE_F;
+        Shape* sh = &w->sh[s];
+        sh->body = shape_body[s];
+        if (sh->body < 0 || sh->body >= nbody) return -1;
+        if (s > 0 && shape_body[s] < shape_body[s - 1]) return -2;
+        sh->lp = v3(f[0], f[1], f[2]); sh->lq = quat(f[3], f[4], f[5], f[6]); sh->h = v3(f[7], f[8], f[9]); sh->mu = f[10];
+        sh->lrot = q_to_m3(sh->lq);
+    }
+    size_t n1 = (size_t)nbody + 1;
+    w->sv = (V3*)realloc(w->sv, sizeof(V3) * n1); w->sw = (V3*)realloc(w->sw, sizeof(V3) * n1);
+    w->sdp = (V3*)realloc(w->sdp, sizeof(V3) * n1); w->sth = (V3*)realloc(w->sth, sizeof(V3) * n1);
+    w->sp0 = (V3*)realloc(w->sp0, sizeof(V3) * n1); w->sim = (float*)realloc(w->sim, sizeof(float) * n1);
+    w->sii = (M3*)realloc(w->sii, sizeof(M3) * n1); w->sq = (Quat*)realloc(w->sq, sizeof(Quat) * n1);
+    w->sidx = (int32_t*)realloc(w->sidx, sizeof(int32_t) * (size_t)(nbody > 0 ? nbody : 1));
+    w->island = (int32_t*)realloc(w->island, sizeof(int32_t) * (size_t)(nbody > 0 ? nbody : 1));
+    w->wake_flag = (uint8_t*)realloc(w->wake_flag, (size_t)(nbody > 0 ? nbody : 1));
+    for (int i = 0; i < nbody; i++) w->island[i] = i;
+    w->nman = 0; w->nold = 0;
+    return 0;
+}
+
+void ob_read_bodies(const ob_world* w, float* out) {
+    for (int i = 0; i < w->nb; i++) {
+        const Body* b = &w->bd[i]; float* f = out + (size_t)i * OB_BODY_F;
+        f[0] = b->p.x; f[1] = b->p.y; f[2] = b->p.z; f[3] = b->q.x; f[4] = b->q.y; f[5] = b->q.z; f[6] = b->q.w;
+        f[7] = b->v.x; f[8] = b->v.y; f[9] = b->v.z; f[10] = b->w.x; f[11] = b->w.y; f[12] = b->w.z;
+        f[13] = b->inv_m; f[14] = b->inv_i.x; f[15] = b->inv_i.y; f[16] = b->inv_i.z;
+        f[17] = b->awake ? 1.0f : 0.0f; f[18] = b->sleep_t; f[19] = 0.0f;
+    }
+}
+void ob_stats(const ob_world* w, int32_t* out8) { for (int i = 0; i < 8; i++) out8[i] = w->stats[i]; }
+
+/* ---------------------------------------------------------------- geometry helpers */
+static void tangents(V3 n, V3* t1, V3* t2) {
+    /* phys.rs:501-504 */
+    V3 t = om_fabs(n.x) > 0.57f ? v3(n.y, -n.x, 0.0f) : v3(0.0f, n.z, -n.y);
+    *t1 = vnorm(t);
+    *t2 = vcross(n, *t1);
+}
+
+typedef struct { V3 p; float s; } Pt;
+
+/* phys.rs:507-527 */
+static int reduce4(Pt* pts, int cnt, V3 n) {
+    if (cnt <= 4) return cnt;
+    int i0 = 0; for (int k = 1; k < cnt; k++) if (pts[k].s < pts[i0].s) i0 = k;
+    V3 p0 = pts[i0].p;
+    int i1 = 0; float best = -1.0f;
+    for (int k = 0; k < cnt; k++) { float d = vlen2(vsub(pts[k].p, p0)); if (d > best) { best = d; i1 = k; } }
+    V3 p1 = pts[i1].p; V3 e = vsub(p1, p0);
+    int i2 = 0; float b2 = -3.40282347e38f;
+    for (int k = 0; k < cnt; k++) { float a = vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b2) { b2 = a; i2 = k; } }
+    int i3 = 0; float b3 = -3.40282347e38f;
+    for (int k = 0; k < cnt; k++) { float a = -vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b3) { b3 = a; i3 = k; } }
+    int idx[4] = {i0, i1, i2, i3};
+    Pt sel[4] = {pts[i0], pts[i1], pts[i2], pts[i3]};
+    int out = 0;
+    for (int k = 0; k < 4; k++) {
+        int dup = 0; for (int q = 0; q < k; q++) if (idx[q] == idx[k]) dup = 1;
+        if (dup) continue;
+        pts[out++] = sel[k];
+    }
+    return out;
+}
+
+/* phys.rs:531-628. Returns the point count (0 = no contact); *n_out the normal from A to B. */
+static int collide_boxes(V3 pa, const M3* ra, V3 ha, V3 pb, const M3* rb, V3 hb, float margin, Pt* out, V3* n_out) {
+    V3 t = vsub(pb, pa);
+    float ac[3][3];
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) ac[i][j] = om_fabs(vdot(ra->c[i], rb->c[j])) + 1e-6f;
+    float hav[3] = {ha.x, ha.y, ha.z}, hbv[3] = {hb.x, hb.y, hb.z};
+    float sa = -3.40282347e38f; int ia = 0;
+    for (int i = 0; i < 3; i++) {
+        float s = om_fabs(vdot(t, ra->c[i])) - (hav[i] + hbv[0] * ac[i][0] + hbv[1] * ac[i][1] + hbv[2] * ac[i][2]);
+        if (s > margin) return 0;
+        if (s > sa) { sa = s; ia = i; }
+    }
+    float sbm = -3.40282347e38f; int jb = 0;
+    for (int j = 0; j < 3; j++) {
+        float s = om_fabs(vdot(t, rb->c[j])) - (hbv[j] + hav[0] * ac[0][j] + hav[1] * ac[1][j] + hav[2] * ac[2][j]);
+        if (s > margin) return 0;
+        if (s > sbm) { sbm = s; jb = j; }
+    }
+    float se = -3.40282347e38f; int ei = 0, ej = 0; V3 el = v3(0, 0, 0);
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) {
+        V3 l = vcross(ra->c[i], rb->c[j]);
+        float len2 = vlen2(l);
+        if (len2 < 1e-8f) continue;
+        l = vmul(l, 1.0f / om_sqrt(len2));
+        float r = hav[0] * om_fabs(vdot(ra->c[0], l)) + hav[1] * om_fabs(vdot(ra->c[1], l)) + hav[2] * om_fabs(vdot(ra->c[2], l))
+                + hbv[0] * om_fabs(vdot(rb->c[0], l)) + hbv[1] * om_fabs(vdot(rb->c[1], l)) + hbv[2] * om_fabs(vdot(rb->c[2], l));
+        float s = om_fabs(vdot(t, l)) - r;
+        if (s > margin) return 0;
+        if (s > se) { se = s; ei = i; ej = j; el = l; }
+    }
+    const float face_tol = 0.005f;
+    int use_b = sbm > sa + face_tol;
+    float face_best = use_b ? sbm : sa;
+    if (se > face_best + 0.01f) {
+        V3 n = vdot(t, el) < 0.0f ? vneg(el) : el;
+        V3 ca = pa, cb = pb;
+        for (int k = 0; k < 3; k++) {
+            if (k != ei) { float s = vdot(ra->c[k], n) > 0.0f ? 1.0f : -1.0f; ca = vadd(ca, vmul(ra->c[k], s * hav[k])); }
+            if (k != ej) { float s = vdot(rb->c[k], n) > 0.0f ? -1.0f : 1.0f; cb = vadd(cb, vmul(rb->c[k], s * hbv[k])); }
+        }
+        V3 d1 = ra->c[ei], d2 = rb->c[ej];
+        V3 r = vsub(ca, cb);
+        float bb = vdot(d1, d2), c = vdot(d1, r), f = vdot(d2, r);
+        float den = 1.0f - bb * bb;
+        float s = den > 1e-6f ? (bb * f - c) / den : 0.0f;
+        s = om_min(om_max(s, -hav[ei]), hav[ei]);
+        float tt = bb * s + f;
+        tt = om_min(om_max(tt, -hbv[ej]), hbv[ej]);
+        s = om_min(om_max(bb * tt - c, -hav[ei]), hav[ei]);
+        V3 p1 = vadd(ca, vmul(d1, s)), p2 = vadd(cb, vmul(d2, tt));
+        float sep = vdot(vsub(p2, p1), n);
+        out[0].p = vmul(vadd(p1, p2), 0.5f); out[0].s = sep;
+        *n_out = n;
+        return 1;
+    }
+    V3 pr, pi_; const M3 *rr, *ri; float hr[3], hi[3]; int k, flip;
+    if (use_b) { pr = pb; rr = rb; pi_ = pa; ri = ra; k = jb; flip = 1; for (int q = 0; q < 3; q++) { hr[q] = hbv[q]; hi[q] = hav[q]; } }
+    else { pr = pa; rr = ra; pi_ = pb; ri = rb; k = ia; flip = 0; for (int q = 0; q < 3; q++) { hr[q] = hav[q]; hi[q] = hbv[q]; } }
+    V3 n = rr->c[k];
+    if (vdot(vsub(pi_, pr), n) < 0.0f) n = vneg(n);
+    int j = 0; float bd = -1.0f;
+    for (int q = 0; q < 3; q++) { float d = om_fabs(vdot(ri->c[q], n)); if (d > bd) { bd = d; j = q; } }
+    float sgn = vdot(ri->c[j], n) > 0.0f ? -1.0f : 1.0f;
+    V3 fc = vadd(pi_, vmul(ri->c[j], sgn * hi[j]));
+    int u = (j + 1) % 3, v = (j + 2) % 3;
+    V3 eu = vmul(ri->c[u], hi[u]), ev = vmul(ri->c[v], hi[v]);
+    V3 poly[8], tmp[8];
+    poly[0] = vadd(vadd(fc, eu), ev); poly[1] = vadd(vsub(fc, eu), ev); poly[2] = vsub(vsub(fc, eu), ev); poly[3] = vsub(vadd(fc, eu), ev);
+    int pn = 4;
+    int u2 = (k + 1) % 3, v2 = (k + 2) % 3;
+    int axs[4] = {u2, u2, v2, v2}; float sgs[4] = {1.0f, -1.0f, 1.0f, -1.0f};
+    for (int pl = 0; pl < 4; pl++) {
+        V3 dir = vmul(rr->c[axs[pl]], sgs[pl]);
+        float off = vdot(dir, pr) + hr[axs[pl]];
+        int tn = 0;
+        for (int q = 0; q < pn; q++) {
+            V3 p0 = poly[q], p1 = poly[(q + 1) % pn];
+            float d0 = vdot(p0, dir) - off, d1 = vdot(p1, dir) - off;
+            if (d0 <= 0.0f) { if (tn < 8) tmp[tn++] = p0; }
+            if ((d0 <= 0.0f) != (d1 <= 0.0f)) {
+                float tt = d0 / (d0 - d1);
+                if (tn < 8) tmp[tn++] = vadd(p0, vmul(vsub(p1, p0), tt));
+            }
+        }
+        pn = tn; for (int q = 0; q < pn; q++) poly[q] = tmp[q];
+        if (pn == 0) return 0;
+    }
+    V3 rc = vadd(pr, vmul(n, hr[k]));
+    int cnt = 0;
+    for (int q = 0; q < pn; q++) {
+        V3 p = poly[q];
+        float sep = vdot(vsub(p, rc), n);
+        if (sep <= margin) { out[cnt].p = vsub(p, vmul(n, 0.5f * sep)); out[cnt].s = sep; cnt++; }
+    }
+    if (cnt == 0) return 0;
+    *n_out = flip ? vneg(n) : n;
+    return cnt;
+}
+
+/* ---------------------------------------------------------------- per-step preparation */
+static float shape_reach(const Shape* s) { return vlen(s->lp) + vlen(s->h); }
+
+/* phys.rs:158-172 per shape: world pose, AABB with the speculative margin */
+static void prep_shapes(ob_world* w, float dt) {
+    const float spec = w->prm[P_SPEC];
+    for (int i = 0; i < w->nb; i++) w->bd[i].rot = q_to_m3(w->bd[i].q);
+    for (int s = 0; s < w->ns; s++) {
+        Shape* sh = &w->sh[s]; const Body* b = &w->bd[sh->body];
+        sh->wrot = m3_mul(&b->rot, &sh->lrot);
+        sh->wp = vadd(b->p, m3_mulv(&b->rot, sh->lp));
+        const M3* r = &sh->wrot; V3 h = sh->h;
+        V3 e = v3(om_fabs(r->c[0].x) * h.x + om_fabs(r->c[1].x) * h.y + om_fabs(r->c[2].x) * h.z,
+                  om_fabs(r->c[0].y) * h.x + om_fabs(r->c[1].y) * h.y + om_fabs(r->c[2].y) * h.z,
+                  om_fabs(r->c[0].z) * h.x + om_fabs(r->c[1].z) * h.y + om_fabs(r->c[2].z) * h.z);
+        float m = b->awake ? (vlen(b->v) + vlen(b->w) * shape_reach(sh)) * dt + 0.5f * spec : 0.5f * spec;
+        e = vadd(e, v3(m, m, m));
+        sh->lo = vsub(sh->wp, e); sh->hi = vadd(sh->wp, e);
+    }
+}
+
+static int pair_ok(const ob_world* w, int ia, int ib) {
+    const Shape *a = &w->sh[ia], *b = &w->sh[ib];
+    if (ia == ib || a->body == b->body) return 0;
+    const Body *ba = &w->bd[a->body], *bb = &w->bd[b->body];
+    if (!ba->awake && !bb->awake) return 0;
+    if (ba->inv_m == 0.0f && bb->inv_m == 0.0f) return 0;
+    if (a->lo.x > b->hi.x || b->lo.x > a->hi.x || a->lo.y > b->hi.y || b->lo.y > a->hi.y || a->lo.z > b->hi.z || b->lo.z > a->hi.z) return 0;
+    return 1;
+}
+
+/* phys.rs:366-404: hashed cells (every cell a shape's box overlaps), counting sort by slot in insertion order,
+ * pairs reported in the cell of the max of the two min corners. Returns pairs (a < b) in slot order. */
+typedef struct { uint32_t h; int32_t i; int32_t c[3]; } Ent;
+static int32_t* hashed_pairs(ob_world* w, int* npairs) {
+    const float inv_cell = 1.0f / w->prm[P_CELL];
+    size_t ne = 0, cap = 1024;
+    Ent* tmp = (Ent*)malloc(sizeof(Ent) * cap);
+    for (int i = 0; i < w->ns; i++) {
+        const Shape* s = &w->sh[i];
+        int c0[3] = {(int32_t)om_floor(s->lo.x * inv_cell), (int32_t)om_floor(s->lo.y * inv_cell), (int32_t)om_floor(s->lo.z * inv_cell)};
+        int c1[3] = {(int32_t)om_floor(s->hi.x * inv_cell), (int32_t)om_floor(s->hi.y * inv_cell), (int32_t)om_floor(s->hi.z * inv_cell)};
+        for (int x = c0[0]; x <= c1[0]; x++) for (int y = c0[1]; y <= c1[1]; y++) for (int z = c0[2]; z <= c1[2]; z++) {
+            if (ne == cap) { cap *= 2; tmp = (Ent*)realloc(tmp, sizeof(Ent) * cap); }
+            uint32_t h = (uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ (uint32_t)z * 83492791u;
+            tmp[ne].h = h; tmp[ne].i = i; tmp[ne].c[0] = x; tmp[ne].c[1] = y; tmp[ne].c[2] = z; ne++;
+        }
+    }
+    size_t tbl = 1; while (tbl < ne * 2) tbl <<= 1;
+    uint32_t mask = (uint32_t)(tbl - 1);
+    uint32_t* cnt = (uint32_t*)calloc(tbl + 1, sizeof(uint32_t));
+    for (size_t e = 0; e < ne; e++) cnt[(tmp[e].h & mask) + 1]++;
+    for (size_t k = 0; k < tbl; k++) cnt[k + 1] += cnt[k];
+    Ent* ent = (Ent*)malloc(sizeof(Ent) * (ne ? ne : 1));
+    uint32_t* fill = (uint32_t*)malloc(sizeof(uint32_t) * tbl);
+    for (size_t k = 0; k < tbl; k++) fill[k] = cnt[k];
+    for (size_t e = 0; e < ne; e++) { uint32_t s = tmp[e].h & mask; ent[fill[s]++] = tmp[e]; }
+    size_t pcap = 1024, np = 0;
+    int32_t* pairs = (int32_t*)malloc(sizeof(int32_t) * 2 * pcap);
+    for (size_t s = 0; s < tbl; s++) {
+        uint32_t lo = cnt[s], hi = cnt[s + 1];
+        if (hi - lo < 2) continue;
+        for (uint32_t x = lo; x < hi; x++) for (uint32_t y = x + 1; y < hi; y++) {
+            const Ent *ea = &ent[x], *eb = &ent[y];
+            if (ea->c[0] != eb->c[0] || ea->c[1] != eb->c[1] || ea->c[2] != eb->c[2] || ea->i == eb->i) continue;
+            if (!pair_ok(w, ea->i, eb->i)) continue;
+            V3 m = vmaxv(w->sh[ea->i].lo, w->sh[eb->i].lo);
+            if ((int32_t)om_floor(m.x * inv_cell) != ea->c[0] || (int32_t)om_floor(m.y * inv_cell) != ea->c[1] ||
+                (int32_t)om_floor(m.z * inv_cell) != ea->c[2]) continue;
+            if (np == pcap) { pcap *= 2; pairs = (int32_t*)realloc(pairs, sizeof(int32_t) * 2 * pcap); }
+            pairs[2 * np] = ea->i < eb->i ? ea->i : eb->i; pairs[2 * np + 1] = ea->i < eb->i ? eb->i : ea->i; np++;
+        }
+    }
+    free(tmp); free(cnt); free(ent); free(fill);
+    *npairs = (int)np;
+    return pairs;
+}
+
+static int cmp_pair(const void* x, const void* y) {
+    const int32_t *a = (const int32_t*)x, *b = (const int32_t*)y;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    uint32_t ua = (uint32_t)a[1] + 1u, ub = (uint32_t)b[1] + 1u; /* ground (-1) first */
+    return ua < ub ? -1 : (ua > ub ? 1 : 0);
+}
+
+int ob_bruteforce_pairs(ob_world* w, float dt, int32_t* out, int cap) {
+    prep_shapes(w, dt);
+    int np = 0;
+    for (int a = 0; a < w->ns; a++) {
+        const Shape* s = &w->sh[a];
+        if (w->bd[s->body].awake && s->lo.y < 0.0f) { if (np < cap) { out[2 * np] = a; out[2 * np + 1] = -1; } np++; }
+        for (int b = a + 1; b < w->ns; b++) if (pair_ok(w, a, b)) { if (np < cap) { out[2 * np] = a; out[2 * np + 1] = b; } np++; }
+    }
+    return np;
+}
+
+/* ---------------------------------------------------------------- manifolds */
+static uint64_t mkey64(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint64_t)((uint32_t)b + 1u); }
+
+static int find_old(const ob_world* w, int32_t a, int32_t b) {
+    uint64_t k = mkey64(a, b);
+    int lo = 0, hi = w->nold - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        const Man* m = &w->old[w->old_sorted[mid]];
+        uint64_t km = mkey64(m->a, m->b);
+        if (km == k) return w->old_sorted[mid];
+        if (km < k) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+static Man* new_man(ob_world* w) {
+    if (w->nman == w->capman) { w->capman = w->capman ? w->capman * 2 : 1024; w->man = (Man*)realloc(w->man, sizeof(Man) * (size_t)w->capman); }
+    Man* m = &w->man[w->nman++];
+    memset(m, 0, sizeof(Man));
+    return m;
+}
+
+static float mix_mu(float a, float b) { return a == b ? a : om_sqrt(a * b); }
+
+/* phys.rs:450-498 */
+static void push_manifold(ob_world* w, int a, int b, V3 n, const Pt* pts, int cnt) {
+    const Shape* sa_ = &w->sh[a];
+    const Body* ba = &w->bd[sa_->body];
+    V3 pa = ba->p; const M3* ra = &ba->rot;
+    const Body* bb = b >= 0 ? &w->bd[w->sh[b].body] : NULL;
+    V3 pb = bb ? bb->p : v3(0, 0, 0);
+    int32_t sa = w->sidx[sa_->body], sb = bb ? w->sidx[w->sh[b].body] : 0;
+    if (sa == 0 && sb == 0) return;
+    Man* m = new_man(w);
+    m->a = a; m->b = b; m->ba = sa_->body; m->bb = bb ? w->sh[b].body : -1; m->sa = sa; m->sb = sb;
+    m->n = n; tangents(n, &m->t1, &m->t2);
+    m->mu = mix_mu(sa_->mu, b >= 0 ? w->sh[b].mu : w->prm[P_FRICTION]);
+    m->cnt = cnt; m->stick = w->prm[P_ANCHOR] > 0.5f;
+    int oi = find_old(w, a, b);
+    const Man* o = oi >= 0 ? &w->old[oi] : NULL;
+    M3 rbm = bb ? bb->rot : m3_identity();
+    V3 cen = v3(0, 0, 0); for (int k = 0; k < cnt; k++) cen = vadd(cen, pts[k].p); cen = vmul(cen, 1.0f / (float)cnt);
+    float rad = 0.0f; for (int k = 0; k < cnt; k++) rad += vlen(vsub(pts[k].p, cen)); m->rad = rad / (float)cnt;
+    int anchored = 0;
+    if (o) {
+        if (o->stick) {
+            V3 wa = vadd(pa, m3_mulv(ra, o->la_f));
+            V3 wb = bb ? vadd(pb, m3_mulv(&rbm, o->lb_f)) : o->lb_f;
+            float lim = om_max(0.05f, m->rad);
+            if (vlen2(vsub(wa, cen)) < lim * lim && vlen2(vsub(wb, wa)) < 0.03f * 0.03f) {
+                m->la_f = o->la_f; m->lb_f = o->lb_f; m->fa = vsub(wa, pa); m->fb = vsub(wb, pb); m->e0 = vsub(wb, wa); anchored = 1;
+            }
+        }
+        V3 f = vadd(vmul(o->t1, o->jt1), vmul(o->t2, o->jt2));
+        m->jt1 = vdot(f, m->t1); m->jt2 = vdot(f, m->t2); m->jtw = o->jtw * vdot(o->n, n);
+    }
+    if (!anchored) {
+        m->fa = vsub(cen, pa); m->fb = vsub(cen, pb); m->la_f = m3_tmulv(ra, m->fa);
+        m->lb_f = bb ? m3_tmulv(&rbm, m->fb) : cen;
+        m->e0 = v3(0, 0, 0);
+    }
+    for (int k = 0; k < cnt; k++) {
+        Cp* c = &m->cp[k];
+        c->ra = vsub(pts[k].p, pa);
+        c->rb = bb ? vsub(pts[k].p, pb) : pts[k].p;
+        c->adj = pts[k].s;
+        c->la = m3_tmulv(ra, c->ra);
+        if (o) {
+            float best = 0.03f * 0.03f; int bi = -1;
+            for (int q = 0; q < o->cnt; q++) { float d = vlen2(vsub(o->cp[q].la, c->la)); if (d < best) { best = d; bi = q; } }
+            if (bi >= 0) c->jn = o->cp[bi].jn;
+        }
+    }
+}
+
+/* phys.rs:411-416 */
+static void wake_body(ob_world* w, int bi) {
+    Body* b = &w->bd[bi];
+    if (b->awake || b->inv_m == 0.0f) return;
+    b->awake = 1; b->sleep_t = 0.0f; b->sleep_p = b->p; b->sleep_q = b->q;
+}
+
+/* phys.rs:418-432 */
+static void collide_ground(ob_world* w, int i) {
+    const Shape* s = &w->sh[i]; const Body* b = &w->bd[s->body];
+    Pt pts[8]; int cnt = 0;
+    float margin = w->prm[P_SPEC] + om_max(-b->v.y, 0.0f) * (1.0f / 60.0f);
+    for (int k = 0; k < 8; k++) {
+        V3 sg = v3((k & 1) ? 1.0f : -1.0f, (k & 2) ? 1.0f : -1.0f, (k & 4) ? 1.0f : -1.0f);
+        V3 c = vadd(s->wp, m3_mulv(&s->wrot, v3(sg.x * s->h.x, sg.y * s->h.y, sg.z * s->h.z)));
+        if (c.y < margin) { pts[cnt].p = v3(c.x, 0.5f * c.y, c.z); pts[cnt].s = c.y; cnt++; }
+    }
+    if (cnt == 0) return;
+    V3 n = v3(0.0f, -1.0f, 0.0f);
+    cnt = reduce4(pts, cnt, n);
+    push_manifold(w, i, -1, n, pts, cnt);
+}
+
+/* phys.rs:434-448; `snap` holds the step-start awake/still flags in the coloured order (NULL: live flags) */
+static void collide_pair(ob_world* w, int ia, int ib, const uint8_t* snap) {
+    const Shape *sa = &w->sh[ia], *sb = &w->sh[ib];
+    const Body *a = &w->bd[sa->body], *b = &w->bd[sb->body];
+    int a_awake, b_awake, a_still, b_still;
+    if (snap) { a_awake = snap[sa->body] & 1; b_awake = snap[sb->body] & 1; a_still = (snap[sa->body] >> 1) & 1; b_still = (snap[sb->body] >> 1) & 1; }
+    else { a_awake = a->awake; b_awake = b->awake; a_still = a->sleep_t != 0.0f; b_still = b->sleep_t != 0.0f; }
+    float rel = vlen(vsub(b->v, a->v)) + vlen(a->w) * shape_reach(sa) + vlen(b->w) * shape_reach(sb);
+    float margin = w->prm[P_SPEC] + rel * (1.0f / 60.0f);
+    Pt pts[8]; V3 n;
+    int cnt = collide_boxes(sa->wp, &sa->wrot, sa->h, sb->wp, &sb->wrot, sb->h, margin, pts, &n);
+    if (!cnt) return;
+    if (snap) {
+        if (a_awake && !b_awake && !a_still) w->wake_flag[sb->body] = 1;
+        if (b_awake && !a_awake && !b_still) w->wake_flag[sa->body] = 1;
+    } else {
+        if (a_awake && !b_awake && !a_still) wake_body(w, sb->body);
+        if (b_awake && !a_awake && !b_still) wake_body(w, sa->body);
+    }
+    cnt = reduce4(pts, cnt, n);
+    push_manifold(w, ia, ib, n, pts, cnt);
+}
+
+/* ---------------------------------------------------------------- colouring (device order) */
+/* Deterministic claim rounds. Each round every uncoloured manifold picks one of the lowest `spread` colours free at
+ * both its moving bodies (by a hash of its key and the round) and claims it at each with a unique priority; it takes
+ * the colour where its claim is the highest at both. Manifolds with no free colour, or none after the last round,
+ * go to the overflow (solved one by one in manifold order). */
+static inline uint32_t mix32(uint32_t h) { h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16; return h; }
+static inline uint32_t key_hash(int32_t a, int32_t b) { return mix32((uint32_t)a * 0x9E3779B9u + mix32((uint32_t)b ^ 0x5BD1E995u)); }
+
+static void colour_manifolds(ob_world* w) {
+    int K = (int)w->prm[P_COL_LIMIT]; if (K < 1) K = 1; if (K > 64) K = 64;
+    int R = (int)w->prm[P_COL_ROUNDS]; int spread = (int)w->prm[P_COL_SPREAD]; if (spread < 1) spread = 1;
+    int M = w->nman;
+    if (w->capcolour < M) { w->capcolour = M; w->colour = (int32_t*)realloc(w->colour, sizeof(int32_t) * (size_t)M); }
+    uint64_t* mask = (uint64_t*)calloc((size_t)w->nb + 1, sizeof(uint64_t));
+    uint64_t* claim = (uint64_t*)calloc(((size_t)w->nb + 1) * (size_t)K, sizeof(uint64_t));
+    int32_t* cand = (int32_t*)malloc(sizeof(int32_t) * (size_t)(M ? M : 1));
+    uint64_t* pri = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(M ? M : 1));
+    const uint64_t all = K == 64 ? ~0ull : ((1ull << K) - 1ull);
+    for (int m = 0; m < M; m++) w->colour[m] = -1;
+    for (int r = 0; r < R; r++) {
+        int any = 0;
+        memset(claim, 0, sizeof(uint64_t) * ((size_t)w->nb + 1) * (size_t)K);
+        for (int m = 0; m < M; m++) {
+            cand[m] = -1;
+            if (w->colour[m] != -1) continue;
+            const Man* mm = &w->man[m];
+            uint64_t used = (mm->sa ? mask[mm->sa] : 0) | (mm->sb ? mask[mm->sb] : 0);
+            uint64_t fr = all & ~used;
+            if (!fr) { w->colour[m] = K; continue; }
+            int nf = __builtin_popcountll(fr);
+            uint32_t kh = key_hash(mm->a, mm->b);
+            int lim = nf < spread ? nf : spread;
+            int pick = (int)(mix32(kh + (uint32_t)r * 0x165667B1u) % (uint32_t)lim);
+            for (int q = 0; q < pick; q++) fr &= fr - 1;
+            int c = __builtin_ctzll(fr);
+            cand[m] = c;
+            pri[m] = ((uint64_t)(mix32(kh ^ ((uint32_t)r * 0x27D4EB2Fu)) >> 8) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)m);
+            if (mm->sa) { uint64_t* cl = &claim[(size_t)mm->sa * K + c]; if (pri[m] > *cl) *cl = pri[m]; }
+            if (mm->sb) { uint64_t* cl = &claim[(size_t)mm->sb * K + c]; if (pri[m] > *cl) *cl = pri[m]; }
+            any = 1;
+        }
+        if (!any) break;
+        for (int m = 0; m < M; m++) {
+            if (cand[m] < 0) continue;
+            const Man* mm = &w->man[m]; int c = cand[m];
+            if (mm->sa && claim[(size_t)mm->sa * K + c] != pri[m]) continue;
+            if (mm->sb && claim[(size_t)mm->sb * K + c] != pri[m]) continue;
+            w->colour[m] = c;
+        }
+        for (int m = 0; m < M; m++) {
+            if (cand[m] < 0 || w->colour[m] != cand[m]) continue;
+            const Man* mm = &w->man[m];
+            if (mm->sa) mask[mm->sa] |= 1ull << cand[m];
+            if (mm->sb) mask[mm->sb] |= 1ull << cand[m];
+        }
+    }
+    int used = 0, ovf = 0;
+    for (int m = 0; m < M; m++) {
+        if (w->colour[m] == -1) w->colour[m] = K;
+        if (w->colour[m] == K) ovf++; else if (w->colour[m] + 1 > used) used = w->colour[m] + 1;
+    }
+    w->stats[4] = used; w->stats[5] = ovf;
+    free(mask); free(claim); free(cand); free(pri);
+}
+
+/* ---------------------------------------------------------------- solver */
+/* phys.rs:245-262 */
+static void warm_start(ob_world* w) {
+    for (int mi = 0; mi < w->nman; mi++) {
+        const Man* m = &w->man[mi];
+        int sa = m->sa, sb = m->sb;
+        V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
+        float pn = 0.0f;
+        for (int k = 0; k < m->cnt; k++) {
+            const Cp* c = &m->cp[k];
+            pn += c->jn; wa = vsub(wa, vmul(c->iaxn, c->jn)); wb = vadd(wb, vmul(c->ibxn, c->jn));
+        }
+        V3 p = vadd(vadd(vmul(m->n, pn), vmul(m->t1, m->jt1)), vmul(m->t2, m->jt2));
+        va = vsub(va, vmul(p, m->ima)); vb = vadd(vb, vmul(p, m->imb));
+        wa = vsub(wa, vadd(vadd(vmul(m->ia1, m->jt1), vmul(m->ia2, m->jt2)), vmul(m->ian, m->jtw)));
+        wb = vadd(wb, vadd(vadd(vmul(m->ib1, m->jt1), vmul(m->ib2, m->jt2)), vmul(m->ibn, m->jtw)));
+        w->sv[sa] = va; w->sw[sa] = wa; w->sv[sb] = vb; w->sw[sb] = wb;
+        w->sv[0] = v3(0, 0, 0); w->sw[0] = v3(0, 0, 0);
+    }
+}
+
+typedef struct { int use_bias; float inv_h, bias_rate, mass_scale, imp_scale, max_push, anchor_push, anchor_slop; } SolveK;
+
+static inline float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
+
+/* phys.rs:268-323, one manifold */
+static void solve_one(ob_world* w, Man* m, const SolveK* K) {
+    int sa = m->sa, sb = m->sb;
+    V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
+    V3 tha = w->sth[sa], thb = w->sth[sb];
+    V3 n = m->n, t1 = m->t1, t2 = m->t2;
+    V3 dpab = vsub(w->sdp[sb], w->sdp[sa]);
+    float dn = vdot(dpab, n);
+    float ima = m->ima, imb = m->imb;
+    float tot = 0.0f;
+    for (int k = 0; k < m->cnt; k++) {
+        Cp* c = &m->cp[k];
+        float s = c->adj + dn + vdot(thb, c->rbxn) - vdot(tha, c->raxn);
+        float bias, ms, is;
+        if (s > 0.0f) { bias = s * K->inv_h; ms = 1.0f; is = 0.0f; }
+        else if (K->use_bias) { bias = om_max(K->bias_rate * s, -K->max_push); ms = K->mass_scale; is = K->imp_scale; }
+        else { bias = 0.0f; ms = 1.0f; is = 0.0f; }
+        float vn = vdot(vsub(vb, va), n) + vdot(wb, c->rbxn) - vdot(wa, c->raxn);
+        float imp = -c->nm * ms * (vn + bias) - is * c->jn;
+        float nj = om_max(c->jn + imp, 0.0f);
+        imp = nj - c->jn; c->jn = nj; tot += nj;
+        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(c->iaxn, imp));
+        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(c->ibxn, imp));
+    }
+    float lim = m->mu * tot;
+    V3 dv = vsub(vb, va);
+    float v1 = vdot(dv, t1) + vdot(wb, m->cb1) - vdot(wa, m->ca1);
+    float v2 = vdot(dv, t2) + vdot(wb, m->cb2) - vdot(wa, m->ca2);
+    int anchored = K->use_bias && m->stick;
+    float b1, b2, ms, is;
+    if (anchored) {
+        float e1 = vdot(m->e0, t1) + vdot(dpab, t1) + vdot(thb, m->cb1) - vdot(tha, m->ca1);
+        float e2 = vdot(m->e0, t2) + vdot(dpab, t2) + vdot(thb, m->cb2) - vdot(tha, m->ca2);
+        float ap = K->anchor_push, sl = K->anchor_slop;
+        b1 = om_min(om_max(K->bias_rate * slop_fn(e1, sl), -ap), ap);
+        b2 = om_min(om_max(K->bias_rate * slop_fn(e2, sl), -ap), ap);
+        ms = K->mass_scale; is = K->imp_scale;
+    } else { b1 = 0.0f; b2 = 0.0f; ms = 1.0f; is = 0.0f; }
+    float o1 = m->jt1, o2 = m->jt2;
+    float j1 = o1 - m->tm1 * ms * (v1 + b1) - is * o1;
+    float j2 = o2 - m->tm2 * ms * (v2 + b2) - is * o2;
+    float l2 = j1 * j1 + j2 * j2;
+    if (l2 > lim * lim) { float kk = lim / om_sqrt(l2); j1 *= kk; j2 *= kk; m->stick = 0; }
+    m->jt1 = j1; m->jt2 = j2;
+    float d1 = j1 - o1, d2 = j2 - o2;
+    V3 p = vadd(vmul(t1, d1), vmul(t2, d2));
+    va = vsub(va, vmul(p, ima)); wa = vsub(wa, vadd(vmul(m->ia1, d1), vmul(m->ia2, d2)));
+    vb = vadd(vb, vmul(p, imb)); wb = vadd(wb, vadd(vmul(m->ib1, d1), vmul(m->ib2, d2)));
+    float wl = lim * m->rad;
+    float ot = m->jtw;
+    float jt = om_min(om_max(ot - m->twm * vdot(vsub(wb, wa), n), -wl), wl);
+    m->jtw = jt;
+    float dtw = jt - ot;
+    wa = vsub(wa, vmul(m->ian, dtw)); wb = vadd(wb, vmul(m->ibn, dtw));
+    w->sv[sa] = va; w->sw[sa] = wa; w->sv[sb] = vb; w->sw[sb] = wb;
+    w->sv[0] = v3(0, 0, 0); w->sw[0] = v3(0, 0, 0);
+}
+
+static void solve_pass(ob_world* w, const SolveK* K, const int32_t* order) {
+    for (int i = 0; i < w->nman; i++) solve_one(w, &w->man[order ? order[i] : i], K);
+}
+
+/* phys.rs:326-354, plus island labels = minimum body index of each island */
+static int uf_find(int32_t* uf, int x) { while (uf[x] != x) { int p = uf[uf[x]]; uf[x] = p; x = p; } return x; }
+static void sleep_islands(ob_world* w, float dt) {
+    int n = w->nb;
+    int32_t* uf = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+    for (int i = 0; i < n; i++) uf[i] = i;
+    for (int mi = 0; mi < w->nman; mi++) {
+        const Man* m = &w->man[mi];
+        if (m->sa == 0 || m->sb == 0) continue;
+        int ra = uf_find(uf, m->ba), rb = uf_find(uf, m->bb);
+        if (ra != rb) uf[ra] = rb;
+    }
+    for (int i = 0; i < n; i++) w->island[i] = n;
+    for (int i = 0; i < n; i++) { int r = uf_find(uf, i); if (i < w->island[r]) w->island[r] = i; }
+    for (int i = 0; i < n; i++) { int r = uf_find(uf, i); w->island[i] = w->island[r]; }
+    /* (the root's slot is rewritten by its own turn in the loop above; islands[r] then already holds the minimum) */
+    float l2 = w->prm[P_SLEEP_LIN] * w->prm[P_SLEEP_LIN], a2 = w->prm[P_SLEEP_ANG] * w->prm[P_SLEEP_ANG];
+    for (int i = 0; i < n; i++) {
+        Body* b = &w->bd[i];
+        if (!b->awake) continue;
+        Quat dq = qmul(b->q, qconj(b->sleep_q));
+        float ang = 2.0f * om_sqrt(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z);
+        int near = vlen2(vsub(b->p, b->sleep_p)) < w->prm[P_SLEEP_DIST] * w->prm[P_SLEEP_DIST] && ang < w->prm[P_SLEEP_ROT];
+        if (near && vlen2(b->v) < 100.0f * l2 && vlen2(b->w) < 100.0f * a2) b->sleep_t += dt;
+        else { b->sleep_t = 0.0f; b->sleep_p = b->p; b->sleep_q = b->q; }
+    }
+    if (w->prm[P_ALLOW_SLEEP] > 0.5f) {
+        float* min_t = (float*)malloc(sizeof(float) * (size_t)(n ? n : 1));
+        for (int i = 0; i < n; i++) min_t[i] = 3.40282347e38f;
+        for (int i = 0; i < n; i++) if (w->bd[i].awake) { int r = uf_find(uf, i); min_t[r] = om_min(min_t[r], w->bd[i].sleep_t); }
+        for (int i = 0; i < n; i++) {
+            if (!w->bd[i].awake) continue;
+            int r = uf_find(uf, i);
+            if (min_t[r] >= w->prm[P_SLEEP_TIME]) { Body* b = &w->bd[i]; b->awake = 0; b->v = v3(0, 0, 0); b->w = v3(0, 0, 0); }
+        }
+        free(min_t);
+    }
+    free(uf);
+}
+
+static const Man* g_sort_base;
+static int cmp_old(const void* x, const void* y) {
+    const Man *a = &g_sort_base[*(const int32_t*)x], *b = &g_sort_base[*(const int32_t*)y];
+    uint64_t ka = mkey64(a->a, a->b), kb = mkey64(b->a, b->b);
+    return ka < kb ? -1 : (ka > kb ? 1 : 0);
+}
+
+/* phys.rs:152-243 */
+void ob_step(ob_world* w, float dt) {
+    const float* P = w->prm;
+    const int coloured = w->order == OB_ORDER_COLOURED;
+    int n = w->nb;
+    /* ---- prepare bodies, AABBs, solver slots for awake ones */
+    prep_shapes(w, dt);
+    w->sv[0] = v3(0, 0, 0); w->sw[0] = v3(0, 0, 0); w->sdp[0] = v3(0, 0, 0); w->sth[0] = v3(0, 0, 0);
+    w->sim[0] = 0.0f; w->sii[0] = m3_diag(v3(0, 0, 0)); w->sq[0] = Q_ID; w->sp0[0] = v3(0, 0, 0);
+    int nsolver = 0;
+    for (int i = 0; i < n; i++) {
+        Body* b = &w->bd[i];
+        if (b->awake) {
+            int s = i + 1;
+            w->sidx[i] = s; nsolver++;
+            w->sv[s] = b->v; w->sw[s] = b->w; w->sdp[s] = v3(0, 0, 0); w->sth[s] = v3(0, 0, 0);
+            w->sim[s] = b->inv_m; w->sii[s] = m3_sandwich(&b->rot, b->inv_i); w->sq[s] = b->q; w->sp0[s] = b->p;
+        } else w->sidx[i] = 0;
+    }
+    /* ---- broadphase + narrowphase */
+    Man* t = w->old; w->old = w->man; w->man = t;
+    int tc = w->capold; w->capold = w->capman; w->capman = tc;
+    w->nold = w->nman; w->nman = 0;
+    w->old_sorted = (int32_t*)realloc(w->old_sorted, sizeof(int32_t) * (size_t)(w->nold ? w->nold : 1));
+    for (int i = 0; i < w->nold; i++) w->old_sorted[i] = i;
+    g_sort_base = w->old;
+    qsort(w->old_sorted, (size_t)w->nold, sizeof(int32_t), cmp_old);
+
+    int np = 0;
+    int32_t* pairs = hashed_pairs(w, &np);
+    w->stats[1] = np;
+    int ngc = 0;
+    uint8_t* snap = NULL;
+    if (coloured) {
+        snap = (uint8_t*)malloc((size_t)(n ? n : 1));
+        for (int i = 0; i < n; i++) { snap[i] = (uint8_t)((w->bd[i].awake ? 1 : 0) | (w->bd[i].sleep_t != 0.0f ? 2 : 0)); w->wake_flag[i] = 0; }
+        /* canonical candidates: per shape, the ground first, then partners by index */
+        int nc = np;
+        for (int s = 0; s < w->ns; s++) if (w->bd[w->sh[s].body].awake && w->sh[s].lo.y < 0.0f) nc++;
+        if (w->capcand < nc) { w->capcand = nc; w->cand = (int32_t*)realloc(w->cand, sizeof(int32_t) * 2 * (size_t)nc); }
+        int k = 0;
+        for (int s = 0; s < w->ns; s++) if (w->bd[w->sh[s].body].awake && w->sh[s].lo.y < 0.0f) { w->cand[2 * k] = s; w->cand[2 * k + 1] = -1; k++; ngc++; }
+        memcpy(w->cand + 2 * k, pairs, sizeof(int32_t) * 2 * (size_t)np);
+        qsort(w->cand, (size_t)nc, 2 * sizeof(int32_t), cmp_pair);
+        w->ncand = nc;
+        for (int c = 0; c < nc; c++) {
+            if (w->cand[2 * c + 1] < 0) collide_ground(w, w->cand[2 * c]);
+            else collide_pair(w, w->cand[2 * c], w->cand[2 * c + 1], snap);
+        }
+        int nw = 0;
+        for (int i = 0; i < n; i++) if (w->wake_flag[i]) { if (!w->bd[i].awake && w->bd[i].inv_m != 0.0f) nw++; wake_body(w, i); }
+        w->stats[7] = nw;
+        free(snap);
+    } else {
+        int nc = np;
+        for (int s = 0; s < w->ns; s++) if (w->bd[w->sh[s].body].awake && w->sh[s].lo.y < 0.0f) nc++;
+        if (w->capcand < nc) { w->capcand = nc; w->cand = (int32_t*)realloc(w->cand, sizeof(int32_t) * 2 * (size_t)nc); }
+        int k = 0;
+        for (int s = 0; s < w->ns; s++) if (w->bd[w->sh[s].body].awake && w->sh[s].lo.y < 0.0f) { w->cand[2 * k] = s; w->cand[2 * k + 1] = -1; k++; ngc++; collide_ground(w, s); }
+        for (int p = 0; p < np; p++) { w->cand[2 * k] = pairs[2 * p]; w->cand[2 * k + 1] = pairs[2 * p + 1]; k++; collide_pair(w, pairs[2 * p], pairs[2 * p + 1], NULL); }
+        w->ncand = nc;
+        w->stats[7] = 0;
+    }
+    w->stats[6] = ngc;
+    free(pairs);
+    /* ---- constraint masses (phys.rs:180-211) */
+    int sub = (int)P[P_SUBSTEPS]; if (sub < 1) sub = 1;
+    float h = dt / (float)sub;
+    float inv_h = 1.0f / h;
+    float hz = P[P_HERTZ] > 0.0f ? P[P_HERTZ] : inv_h / 6.0f;
+    float omega = 2.0f * 3.14159265358979323846f * hz;
+    float a1 = 2.0f * P[P_DAMPING] + h * omega;
+    float a2 = h * omega * a1;
+    float a3 = 1.0f / (1.0f + a2);
+    float bias_rate = omega / a1, mass_scale = a2 * a3, imp_scale = a3 * P[P_LEAK];
+    int pts = 0;
+    for (int mi = 0; mi < w->nman; mi++) {
+        Man* m = &w->man[mi];
+        float ima = w->sim[m->sa]; M3 iia = w->sii[m->sa];
+        float imb = w->sim[m->sb]; M3 iib = w->sii[m->sb];
+        m->ima = ima; m->imb = imb;
+        V3 nn = m->n;
+        for (int k = 0; k < m->cnt; k++) {
+            Cp* c = &m->cp[k];
+            c->raxn = vcross(c->ra, nn); c->rbxn = vcross(c->rb, nn);
+            c->iaxn = m3_mulv(&iia, c->raxn); c->ibxn = m3_mulv(&iib, c->rbxn);
+            float kk = ima + imb + vdot(c->raxn, c->iaxn) + vdot(c->rbxn, c->ibxn);
+            c->nm = kk > 0.0f ? 1.0f / kk : 0.0f;
+        }
+        m->ca1 = vcross(m->fa, m->t1); m->ca2 = vcross(m->fa, m->t2); m->cb1 = vcross(m->fb, m->t1); m->cb2 = vcross(m->fb, m->t2);
+        m->ia1 = m3_mulv(&iia, m->ca1); m->ia2 = m3_mulv(&iia, m->ca2); m->ib1 = m3_mulv(&iib, m->cb1); m->ib2 = m3_mulv(&iib, m->cb2);
+        float k1 = ima + imb + vdot(m->ca1, m->ia1) + vdot(m->cb1, m->ib1);
+        float k2 = ima + imb + vdot(m->ca2, m->ia2) + vdot(m->cb2, m->ib2);
+        m->tm1 = k1 > 0.0f ? 1.0f / k1 : 0.0f; m->tm2 = k2 > 0.0f ? 1.0f / k2 : 0.0f;
+        m->ian = m3_mulv(&iia, nn); m->ibn = m3_mulv(&iib, nn);
+        float kt = vdot(nn, m->ian) + vdot(nn, m->ibn);
+        m->twm = kt > 0.0f ? 1.0f / kt : 0.0f;
+        pts += m->cnt;
+    }
+    w->stats[2] = w->nman; w->stats[3] = pts;
+    /* ---- solve order */
+    int32_t* order = NULL;
+    if (coloured) {
+        colour_manifolds(w);
+        int K = (int)P[P_COL_LIMIT]; if (K < 1) K = 1; if (K > 64) K = 64;
+        order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(w->nman ? w->nman : 1));
+        int k = 0;
+        for (int c = 0; c <= K; c++) for (int mi = 0; mi < w->nman; mi++) if (w->colour[mi] == c) order[k++] = mi;
+    } else { w->stats[4] = 0; w->stats[5] = 0; }
+    /* ---- sub-steps (phys.rs:214-230) */
+    V3 g = v3(P[P_GX], P[P_GY], P[P_GZ]);
+    float wmax = 0.25f * 3.14159265358979323846f * inv_h;
+    SolveK kb = {1, inv_h, bias_rate, mass_scale, imp_scale, P[P_MAXPUSH], P[P_ANCHOR_PUSH], P[P_ANCHOR_SLOP]};
+    SolveK kr = kb; kr.use_bias = 0;
+    for (int s = 0; s < sub; s++) {
+        for (int i = 0; i < n; i++) {
+            int si = w->sidx[i]; if (!si) continue;
+            w->sv[si] = vadd(w->sv[si], vmul(g, h));
+            V3 ww = w->sw[si]; float wl = vlen2(ww);
+            if (wl > wmax * wmax) w->sw[si] = vmul(ww, wmax / om_sqrt(wl));
+        }
+        warm_start(w);
+        solve_pass(w, &kb, order);
+        for (int i = 0; i < n; i++) {
+            int si = w->sidx[i]; if (!si) continue;
+            V3 v = w->sv[si], ww = w->sw[si];
+            w->sdp[si] = vadd(w->sdp[si], vmul(v, h)); w->sth[si] = vadd(w->sth[si], vmul(ww, h));
+            w->sq[si] = qintegrate(w->sq[si], ww, h);
+        }
+        solve_pass(w, &kr, order);
+    }
+    free(order);
+    /* ---- write back */
+    for (int i = 0; i < n; i++) {
+        int si = w->sidx[i]; if (!si) continue;
+        Body* b = &w->bd[i];
+        b->v = w->sv[si]; b->w = w->sw[si]; b->p = vadd(w->sp0[si], w->sdp[si]); b->q = w->sq[si];
+    }
+    sleep_islands(w, dt);
+    w->stats[0] = nsolver;
+}
+
+int ob_candidates(const ob_world* w, int32_t* out, int cap) {
+    for (int i = 0; i < w->ncand && i < cap; i++) { out[2 * i] = w->cand[2 * i]; out[2 * i + 1] = w->cand[2 * i + 1]; }
+    return w->ncand;
+}
+
+int ob_manifolds(const ob_world* w, int32_t* keys, float* data, int cap) {
+    for (int i = 0; i < w->nman && i < cap; i++) {
+        const Man* m = &w->man[i];
+        keys[3 * i] = m->a; keys[3 * i + 1] = m->b; keys[3 * i + 2] = m->cnt;
+        float* f = data + (size_t)i * OB_MAN_F;
+        memset(f, 0, sizeof(float) * OB_MAN_F);
+        f[0] = m->n.x; f[1] = m->n.y; f[2] = m->n.z; f[3] = (float)m->cnt;
+        for (int k = 0; k < m->cnt; k++) {
+            const Cp* c = &m->cp[k]; float* q = f + 4 + 8 * k;
+            q[0] = c->ra.x; q[1] = c->ra.y; q[2] = c->ra.z; q[3] = c->rb.x; q[4] = c->rb.y; q[5] = c->rb.z; q[6] = c->adj; q[7] = c->jn;
+        }
+        f[36] = m->jt1; f[37] = m->jt2; f[38] = m->jtw; f[39] = (float)m->stick;
+    }
+    return w->nman;
+}
+
+int ob_colours(const ob_world* w, int32_t* out, int cap) {
+    if (w->order != OB_ORDER_COLOURED) return 0;
+    for (int i = 0; i < w->nman && i < cap; i++) out[i] = w->colour[i];
+    return w->nman;
+}
+
+void ob_islands(const ob_world* w, int32_t* out) { for (int i = 0; i < w->nb; i++) out[i] = w->island[i]; }

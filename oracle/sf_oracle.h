@@ -1,0 +1,88 @@
+/* CPU oracle for the physics hot path: C restatement of the reference's box core (engine/src/phys.rs) extended to
+ * compound bodies and per-shape friction as in gpu/gpu_world.js (buildBodies :2212-2249, emit :601-603), the
+ * colouring and island labelling the device path uses, and the legacy load-path solve of src/lib.rs
+ * (sor_brick :1005-1026, cond_w :1142-1151, stress_brick :1153-1227).
+ *
+ * TEST INFRASTRUCTURE ONLY: tests, the smoke check and the benchmark's CPU-baseline leg load it as the checker.
+ * The product library never links or calls it. */
+#ifndef SF_ORACLE_H
+#define SF_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- box world (engine/src/phys.rs) ---- */
+typedef struct ob_world ob_world;
+
+/* Solve/pair order. OB_ORDER_REFERENCE reproduces phys.rs exactly: pairs in hash-table slot order
+ * (phys.rs:376-404), ground manifolds first, sequential Gauss-Seidel in manifold order, wakes applied at once
+ * (phys.rs:441-444). OB_ORDER_COLOURED is the device order: candidates per shape (ground, then partners by index),
+ * colour-bucketed solve in colour order then the overflow in manifold order, wakes from the step-start snapshot. */
+enum { OB_ORDER_REFERENCE = 0, OB_ORDER_COLOURED = 1 };
+
+/* Parameter block, phys.rs:74-106 defaults plus the colouring limits. Index names in sf.h (SF_P_*). */
+#define OB_NPARAM 24
+
+/* body record: p[3] q[4] v[3] w[3] inv_m inv_i[3] awake sleep_t reserved  (20 floats) */
+#define OB_BODY_F 20
+/* shape record: lp[3] lq[4] h[3] mu  (11 floats); shape_body[] gives the owning body */
+#define OB_SHAPE_F 11
+
+ob_world* ob_create(int order);
+void ob_destroy(ob_world* w);
+void ob_set_params(ob_world* w, const float* prm, int n);
+/* Bodies and shapes; shapes of one body must be contiguous and in body order. */
+int ob_load(ob_world* w, int nbody, const float* bodies, int nshape, const int32_t* shape_body, const float* shapes);
+void ob_step(ob_world* w, float dt);
+int ob_nbody(const ob_world* w);
+/* read back body state (OB_BODY_F floats per body) */
+void ob_read_bodies(const ob_world* w, float* out);
+/* stats: [awake solver bodies, pairs, manifolds, points, colours used, overflow, ground candidates, wakes] */
+void ob_stats(const ob_world* w, int32_t* out8);
+/* last step's candidate list in canonical order: (a, b) with b = -1 for the ground; returns the count */
+int ob_candidates(const ob_world* w, int32_t* out, int cap);
+/* last step's manifolds: keys (a, b) and point counts; plus per manifold: normal[3], npts, and up to 4 points of
+ * (ra[3], rb[3], adj, jn) after the solve, jt1, jt2, jtw */
+int ob_manifolds(const ob_world* w, int32_t* keys, float* data, int cap);
+#define OB_MAN_F 40
+/* colour per manifold (K = overflow) for the last step (coloured order only) */
+int ob_colours(const ob_world* w, int32_t* out, int cap);
+/* island label per body (min body index of its island, -1 for bodies not awake at the step start) */
+void ob_islands(const ob_world* w, int32_t* out);
+/* brute-force O(n^2) candidate pairs for the current pose (test pin for the broadphase) */
+int ob_bruteforce_pairs(ob_world* w, float dt, int32_t* out, int cap);
+
+/* ---- load path (src/lib.rs structures / SOR) ---- */
+/* A structure: n bricks on the lattice with material codes 1..5 (concrete, masonry, steel, glass, timber), lattice
+ * coordinates, per-brick bond capacities bond[4] (+x,+y,+z, ground) and external load ext. nb6[6] neighbour brick
+ * index or -1 (DIRS order +x,-x,+y,-y,+z,-z). */
+typedef struct {
+    int n;
+    const uint8_t* mat;
+    const int32_t* xyz;   /* 3 per brick */
+    const int32_t* nb6;   /* 6 per brick */
+    const float* bond;    /* 4 per brick */
+    const float* ext;     /* 1 per brick (may be NULL) */
+} ol_struct;
+
+/* One lexicographic SOR sweep in brick order (sor_struct, lib.rs:1029-1040); returns max|d| / max(phi, 1). */
+float ol_sor_sweep(const ol_struct* s, float* phi, float omega);
+/* FP64 solve of the same linear system by conjugate gradients to rel_tol; returns iterations (or -1). */
+int ol_solve64(const ol_struct* s, double* phi, double rel_tol, int max_iter);
+/* FP64 residual max|b - A phi| / max|b| */
+double ol_residual64(const ol_struct* s, const double* phi);
+/* Joint verdicts from a field (stress_brick, lib.rs:1153-1227, static bricks): per brick 4 bonds (+x,+y,+z, ground):
+ * ratio (FP64), mode (0 none, 1 compression, 2 tension, 3 shear, 4 ground) and fail = ratio > 1. */
+void ol_verdicts64(const ol_struct* s, const double* phi, double* ratio4, uint8_t* mode4, uint8_t* fail4);
+/* The reference's f32 verdict from an f32 field (same function, f32 arithmetic as the reference). */
+void ol_verdicts32(const ol_struct* s, const float* phi, float* ratio4, uint8_t* mode4);
+/* grounded connectivity: 1 for bricks reachable from a grounded brick through linked bonds (conn_struct :1085-1098) */
+int ol_grounded(const ol_struct* s, uint8_t* out);
+/* neighbour table and bond capacities of one standing structure (finalize_brick :668-684, bond_cap :646-649) */
+void ol_finalize(int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, int32_t* nb6, float* bond);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
