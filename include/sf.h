@@ -1,0 +1,162 @@
+/* libsf_hip: MI355X-native physics hot path of the Shatterforge engine behind a plain C ABI.
+ *
+ * Three groups of entry points:
+ *   1. the box world (rigid step: broadphase, SAT contacts, colouring, coloured soft-step solve, union-find sleep),
+ *      handle based (sf_world_*) plus the reference's singleton exports (sf_pile_setup ... sf_pose);
+ *   2. the load path (per-structure scalar-potential SOR, joint verdicts) (sf_load_*);
+ *   3. seeded scene generators used by tests and the benchmark (sf_scene_*).
+ * Plain pointers and sizes only. Functions never abort the process: failures return a negative code (or NULL) and
+ * leave a message for sf_last_error_message(). Calls on one handle are not thread safe. Host pointers returned by
+ * the library stay valid until the next mutating call on the same handle. */
+#ifndef SF_H
+#define SF_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors and device */
+#define SF_OK 0
+#define SF_E_ARG (-1)      /* bad argument (size, pointer, NaN input) */
+#define SF_E_HIP (-2)      /* a HIP runtime call failed */
+#define SF_E_CAP (-3)      /* input larger than the handle's capacity */
+#define SF_E_NODEV (-4)    /* no GPU */
+int sf_last_error(void);
+const char* sf_last_error_message(void);
+int sf_device_count(void);
+
+/* ---------------------------------------------------------------- box world */
+/* Parameter block (floats). Defaults: engine/src/phys.rs:101-105 (Params::default) plus the colouring limits. */
+enum {
+    SF_P_GX, SF_P_GY, SF_P_GZ,      /* gravity (0, -9.81, 0) */
+    SF_P_SUBSTEPS,                  /* 12 */
+    SF_P_HERTZ,                     /* contact stiffness, 0 = a sixth of the sub-step rate */
+    SF_P_DAMPING,                   /* 10 */
+    SF_P_MAXPUSH,                   /* 3 m/s */
+    SF_P_FRICTION,                  /* 0.6: the ground's friction (shapes carry their own) */
+    SF_P_SPEC,                      /* speculative distance 0.02 m */
+    SF_P_SLEEP_LIN, SF_P_SLEEP_ANG, SF_P_SLEEP_TIME, SF_P_SLEEP_DIST, SF_P_SLEEP_ROT, SF_P_ALLOW_SLEEP,
+    SF_P_CELL,                      /* broadphase cell 1 m */
+    SF_P_LEAK, SF_P_ANCHOR, SF_P_ANCHOR_PUSH, SF_P_ANCHOR_SLOP,
+    SF_P_COL_LIMIT,                 /* colours (<= 64); contacts beyond go to the overflow, solved in order */
+    SF_P_COL_ROUNDS,                /* colouring rounds (32) */
+    SF_P_COL_SPREAD,                /* a contact picks among the lowest this-many free colours (4) */
+    SF_P_RESERVED,
+    SF_NPARAM
+};
+/* body record, SF_BODY_F floats: p[3] q[4] v[3] w[3] inv_m inv_i[3] (principal, local) awake sleep_t reserved */
+#define SF_BODY_F 20
+/* shape record, SF_SHAPE_F floats: local position[3] local rotation q[4] half extents[3] friction */
+#define SF_SHAPE_F 11
+/* statistics of the last step */
+enum { SF_S_AWAKE, SF_S_PAIRS, SF_S_MANIFOLDS, SF_S_POINTS, SF_S_COLOURS, SF_S_OVERFLOW, SF_S_GROUND, SF_S_WAKES,
+       SF_S_ENTRIES, SF_S_CANDIDATES, SF_S_DROPPED, SF_S_ROUNDS, SF_NSTAT };
+/* per-manifold record from sf_world_manifolds, SF_MAN_F floats: n[3] count, then 4 points of
+ * (raxn[3], rbxn[3], separation, normal impulse), then jt1 jt2 jtw stick */
+#define SF_MAN_F 40
+
+typedef struct sf_world sf_world;
+/* Capacity in bodies and shapes; params may be NULL (defaults). Replaces engine/src/phys.rs World::new (:132). */
+sf_world* sf_world_create(int body_cap, int shape_cap, const float* params);
+void sf_world_destroy(sf_world* w);
+int sf_world_set_params(sf_world* w, const float* params, int n);
+/* Load bodies and shapes (shapes of a body contiguous, bodies in order). Replaces World::add_box (phys.rs:139) and
+ * GpuWorld.addBodies (gpu/gpu_world.js:2656). Rejects NaN/inf input like addBodies (:2657-2659). */
+int sf_world_load(sf_world* w, int nbody, const float* bodies, int nshape, const int32_t* shape_body, const float* shapes);
+/* One 60 Hz step; returns the number of bodies in the solver (phys.rs World::step :152, wasm step :39) */
+int sf_world_step(sf_world* w, float dt);
+/* Enqueue `nsteps` steps without host synchronisation in between (the colour plan of each step is read on the
+ * device); returns 0 or an error */
+int sf_world_step_many(sf_world* w, float dt, int nsteps);
+int sf_world_sync(sf_world* w);
+int sf_world_read_bodies(sf_world* w, float* out);
+int sf_world_stats(sf_world* w, int32_t* out);
+int sf_world_nbody(sf_world* w);
+/* last step's candidates (a, b; b = -1 ground) in canonical order, count returned */
+int sf_world_candidates(sf_world* w, int32_t* out, int cap);
+/* last step's manifolds: keys (a, b, count) and SF_MAN_F floats each */
+int sf_world_manifolds(sf_world* w, int32_t* keys, float* data, int cap);
+/* colour of each manifold of the last step (SF_P_COL_LIMIT = overflow) */
+int sf_world_colours(sf_world* w, int32_t* out, int cap);
+/* island label (minimum body index of its island) of each body after the last step */
+int sf_world_islands(sf_world* w, int32_t* out);
+/* 10 floats per body (position, rotation, half extents of the first shape), like engine/src/wasm.rs pose (:47-53) */
+const float* sf_world_pose(sf_world* w);
+/* device time of the last step's stages in ms: broad, narrow, colour, solve, sleep (when timing is on) */
+int sf_world_set_timing(sf_world* w, int on);
+int sf_world_stage_ms(sf_world* w, float* out5);
+/* live timing of the dominant kernel (the colour-bucket solve): an event pair around each launch; stats are the
+ * summed device ms, the launches and the manifold solves they performed since timing was switched on */
+int sf_world_set_kernel_timing(sf_world* w, int on);
+int sf_world_kernel_stats(sf_world* w, double* ms, int64_t* launches, int64_t* items);
+
+/* ---- the reference's box-core exports (engine/src/wasm.rs:27-53) on one default world */
+void sf_pile_setup(uint32_t n, uint32_t substeps);              /* wasm.rs:27 pile_setup */
+void sf_stack_setup(uint32_t n, uint32_t substeps, uint32_t sleep); /* wasm.rs:33 stack_setup */
+uint32_t sf_step(float dt);                                     /* wasm.rs:39 step */
+uint32_t sf_body_count(void);                                   /* wasm.rs:40 */
+uint32_t sf_manifolds(void);                                    /* wasm.rs:41 */
+uint32_t sf_points(void);                                       /* wasm.rs:42 */
+float sf_max_speed(void);                                       /* wasm.rs:43 */
+float sf_body_pos(uint32_t i, uint32_t axis);                   /* wasm.rs:44 */
+const float* sf_pose(void);                                     /* wasm.rs:47 */
+
+/* ---------------------------------------------------------------- scenes (host only, no GPU needed) */
+/* Each generator is called twice: with NULL outputs it returns the body count and writes the shape count to
+ * *nshape; then with arrays of those sizes it fills them. */
+/* engine/src/phys.rs:632-646 pile(n): brick-sized boxes in a loose grid (LCG seed 1) */
+int sf_scene_pile(int n, float* bodies, int32_t* shape_body, float* shapes, int* nshape);
+/* phys.rs:654-661 / wasm.rs:33-38: n 0.5 m cubes stacked exactly */
+int sf_scene_stack(int n, float* bodies, int32_t* shape_body, float* shapes, int* nshape);
+/* Rubble pile: nbody polycubes of 0.5 m bricks, sizes ~ Geometric(p) truncated at maxsize, grown on the lattice,
+ * materials uniform over concrete/masonry/steel/glass/timber, placed without overlap in a box of the given size
+ * (m) on the ground, velocities N(0, vsig), angular N(0, wsig). Compound mass/COM/inertia in the principal frame. */
+int sf_scene_rubble(int nbody, float p, int maxsize, float bx, float by, float bz, float vsig, float wsig, uint32_t seed,
+                    float* bodies, int32_t* shape_body, float* shapes, int* nshape);
+/* Compound body from bricks: nb bricks at lattice cells xyz (0.5 m), materials 1..5; writes one body record and nb
+ * shape records (principal frame, gpu/gpu_world.js buildBodies :2212-2249). Returns 0. */
+int sf_build_compound(int nb, const int32_t* xyz, const uint8_t* mat, float* body, float* shapes);
+
+/* ---------------------------------------------------------------- load path (src/lib.rs structures, SOR) */
+/* A standing structure of bricks on the 0.5 m lattice: material 1..5 per brick, lattice cell, infill flag.
+ * Bonds and neighbours are built as finalize_brick does (src/lib.rs:668-684). */
+#define SF_MODE_NONE 0
+#define SF_MODE_COMPRESSION 1
+#define SF_MODE_TENSION 2
+#define SF_MODE_SHEAR 3
+#define SF_MODE_GROUND 4
+typedef struct sf_load sf_load;
+/* one batch of independent structures solved together */
+sf_load* sf_load_create(void);
+void sf_load_destroy(sf_load* L);
+/* add a structure; ext (external load per brick, kg) may be NULL; returns the structure id */
+int sf_load_add(sf_load* L, int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, const float* ext);
+/* Solve every structure to max|b - A phi| / max|b| <= rel_tol in FP64 (FP32 sweeps, FP64 residual and
+ * refinement). omega <= 0 picks the over-relaxation per structure. Replaces sor_struct / warm_struct
+ * (src/lib.rs:1029-1079). Returns the number of structures that converged. */
+int sf_load_solve(sf_load* L, float omega, double rel_tol, int max_sweeps);
+/* per structure: sweeps done, final relative residual, grounded bricks */
+int sf_load_info(sf_load* L, int sid, int32_t* sweeps, double* resid, int32_t* grounded);
+/* the FP64 field of a structure (n doubles) */
+int sf_load_field(sf_load* L, int sid, double* phi);
+/* bond verdicts of a structure: 4 bonds per brick (+x,+y,+z,ground): ratio, mode, fail (stress_brick,
+ * src/lib.rs:1153-1227) */
+int sf_load_verdicts(sf_load* L, int sid, double* ratio4, uint8_t* mode4, uint8_t* fail4);
+/* neighbour table (6 per brick, -1 none) and bonds (4 per brick) of a structure as built */
+int sf_load_topology(sf_load* L, int sid, int32_t* nb6, float* bond4);
+int sf_load_count(sf_load* L);
+/* device time of the last sf_load_solve in ms and the FP32 / FP64 sweep counts summed over structures */
+int sf_load_last_timing(sf_load* L, float* ms, int64_t* sweeps32, int64_t* sweeps64, int64_t* bricks);
+
+/* Frame/wall building for load tests: footprint wx x wz bricks, storeys of 7 brick layers, columns every
+ * `bay` bricks, slabs, perimeter walls of masonry with glass infill; `remove` columns removed at random.
+ * Returns the brick count; call with NULL outputs for the size. */
+int sf_scene_building(int wx, int wz, int storeys, int bay, int remove, uint32_t seed, uint8_t* mat, int32_t* xyz, uint8_t* infill);
+/* the tower of the plain-method workload: 12x12 footprint x 40 high, 2x2 concrete corner columns, a concrete
+ * slab every 6th layer, masonry elsewhere */
+int sf_scene_tower(uint8_t* mat, int32_t* xyz, uint8_t* infill);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,113 @@
+"""ctypes binding of libsf_hip.so (the C ABI declared in include/sf.h).
+
+The library is built in-tree (``shatterforge_amd/libsf_hip.so``); importing this module without it raises, so no
+code path can silently fall back to anything else.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsf_hip.so")
+
+c_int, c_uint, c_float, c_double, c_void_p, c_char_p = (ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.c_double,
+                                                        ctypes.c_void_p, ctypes.c_char_p)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+
+# name: (restype, argtypes)
+_PROTOS = {
+    "sf_last_error": (c_int, []),
+    "sf_last_error_message": (c_char_p, []),
+    "sf_device_count": (c_int, []),
+    "sf_world_create": (c_void_p, [c_int, c_int, c_void_p]),
+    "sf_world_destroy": (None, [c_void_p]),
+    "sf_world_set_params": (c_int, [c_void_p, c_void_p, c_int]),
+    "sf_world_load": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "sf_world_step": (c_int, [c_void_p, c_float]),
+    "sf_world_step_many": (c_int, [c_void_p, c_float, c_int]),
+    "sf_world_sync": (c_int, [c_void_p]),
+    "sf_world_read_bodies": (c_int, [c_void_p, c_void_p]),
+    "sf_world_stats": (c_int, [c_void_p, c_void_p]),
+    "sf_world_nbody": (c_int, [c_void_p]),
+    "sf_world_candidates": (c_int, [c_void_p, c_void_p, c_int]),
+    "sf_world_manifolds": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+    "sf_world_colours": (c_int, [c_void_p, c_void_p, c_int]),
+    "sf_world_islands": (c_int, [c_void_p, c_void_p]),
+    "sf_world_pose": (ctypes.POINTER(c_float), [c_void_p]),
+    "sf_world_set_timing": (c_int, [c_void_p, c_int]),
+    "sf_world_stage_ms": (c_int, [c_void_p, c_void_p]),
+    "sf_world_set_kernel_timing": (c_int, [c_void_p, c_int]),
+    "sf_world_kernel_stats": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_pile_setup": (None, [c_uint, c_uint]),
+    "sf_stack_setup": (None, [c_uint, c_uint, c_uint]),
+    "sf_step": (c_uint, [c_float]),
+    "sf_body_count": (c_uint, []),
+    "sf_manifolds": (c_uint, []),
+    "sf_points": (c_uint, []),
+    "sf_max_speed": (c_float, []),
+    "sf_body_pos": (c_float, [c_uint, c_uint]),
+    "sf_pose": (ctypes.POINTER(c_float), []),
+    "sf_scene_pile": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_scene_stack": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_scene_rubble": (c_int, [c_int, c_float, c_int, c_float, c_float, c_float, c_float, c_float, c_uint,
+                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_build_compound": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_load_create": (c_void_p, []),
+    "sf_load_destroy": (None, [c_void_p]),
+    "sf_load_add": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_load_solve": (c_int, [c_void_p, c_float, c_double, c_int]),
+    "sf_load_info": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "sf_load_field": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_load_verdicts": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "sf_load_topology": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "sf_load_count": (c_int, [c_void_p]),
+    "sf_load_last_timing": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_scene_building": (c_int, [c_int, c_int, c_int, c_int, c_int, c_uint, c_void_p, c_void_p, c_void_p]),
+    "sf_scene_tower": (c_int, [c_void_p, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (loaded once). Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class SfError(RuntimeError):
+    pass
+
+
+def check(rc):
+    """Raise SfError with the library's message for a negative return code."""
+    if rc is None or (isinstance(rc, int) and rc < 0):
+        L = lib()
+        raise SfError(f"libsf_hip error {L.sf_last_error()}: {L.sf_last_error_message().decode()}")
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a contiguous numpy array (or None)."""
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def declared_symbols():
+    """Function names declared in include/sf.h (the drop-in boundary)."""
+    import re
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "sf.h")
+    names = []
+    with open(hdr) as f:
+        for line in f:
+            m = re.match(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(sf_[a-z_0-9]+)\s*\(", line)
+            if m:
+                names.append(m.group(1))
+    return names

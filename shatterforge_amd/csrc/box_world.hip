@@ -1,0 +1,1436 @@
+// Box world on the GPU: the rigid step of the reference's box core (engine/src/phys.rs World::step :152-243) with
+// compound bodies (gpu/gpu_world.js :2212-2249), as hand-written gfx950 kernels.
+//
+// One step, all on one stream, counts kept on the device:
+//   prep_bodies / prep_shapes    world poses, AABBs with the speculative margin, solver slots (phys.rs:158-172)
+//   broadphase                   every cell a shape's AABB covers goes into a hashed counting-sort table; each
+//                                shape then finds its partners of higher index in its cells, reporting a pair only
+//                                in the cell of the max of the two min corners (phys.rs:366-404), ground first,
+//                                partners sorted: the candidate list is canonical and deterministic
+//   narrow                       one lane per candidate: 15-axis SAT with face clipping or ground corners
+//                                (phys.rs:418-432, :531-628), reduce4, wake requests (:441-444)
+//   build                        compacted manifolds, warm start from last step's sorted keys, anchored patch
+//                                friction, effective masses (:180-211, :450-498)
+//   colour                       deterministic claim rounds (one of the lowest free colours at both bodies, the
+//                                highest priority claim wins); contacts left over go to the overflow
+//   sub-steps                    gravity + spin clamp + warm-start gather per body (its contacts in manifold order),
+//                                coloured soft-step solve, integrate, relax (:214-230, :245-324)
+//   write back, islands, sleep   union-find by atomic hooking of the larger root under the smaller (labels = the
+//                                island's minimum body index), rest timers, island minimum (:231-237, :326-354)
+// Layouts are struct-of-arrays of float4 so every lane's loads are coalesced.
+#include "sf_dev.h"
+#include "sf_host.h"
+#include "sf_scan.h"
+#include "../../include/sf.h"
+#include <vector>
+#include <string>
+#include <cstring>
+#include <cmath>
+#include <algorithm>
+
+namespace {
+
+constexpr int NT = 256;  // threads per block of the grid-stride kernels
+
+// device counters
+enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_ACTIVE,
+       C_ROUNDS, C_NOVF2, C_NCTR = 16 };
+
+// manifold rows (each row is a float4 array of mcap entries)
+enum { MR_N, MR_T1, MR_T2, MR_E0, MR_J, MR_CA1, MR_CA2, MR_CB1, MR_CB2, MR_IA1, MR_IA2, MR_IB1, MR_IB2, MR_IAN, MR_IBN,
+       MR_LAF, MR_LBF, MR_P0 };
+constexpr int MR_PER_PT = 5;  // raxn|adj, rbxn|nm, iaxn|jn, ibxn|-, la|-
+constexpr int MR_ROWS = MR_P0 + 4 * MR_PER_PT;
+
+struct ManBuf {
+    int4* hdr;        // shape a, shape b (-1 ground), slot a, slot b
+    float4* row;      // MR_ROWS x mcap
+    uint64_t* key;    // sort key (a, b + 1)
+};
+
+struct StepK {
+    float dt, h, inv_h, bias_rate, mass_scale, imp_scale, max_push, anchor_push, anchor_slop;
+    float spec, inv_cell, friction, sleep_lin2, sleep_ang2, sleep_time, sleep_dist2, sleep_rot, wmax;
+    float gx, gy, gz;
+    int anchor, allow_sleep, K, spread, use_bias;
+};
+
+struct Dev {
+    int nb, ns;
+    uint32_t ecap, tcap, tmask, ccap, mcap;
+    // bodies
+    float4 *bp, *bq, *bv, *bw, *bii, *bsp, *bsq, *brot;  // bp: p|inv_m, bii: inv_i|sleep_t, brot: 3 per body
+    int32_t *bflag, *bslot;
+    uint32_t* wake;
+    // solver slots (nb + 1, slot 0 = the static dummy, all zero)
+    float4 *sv, *sw, *sdp, *sth, *sq, *sp0, *sii;  // sv: v|inv_m, sii: 3 per slot
+    // shapes
+    int32_t* shb;
+    float4 *slp, *slr, *shh, *swp, *swr, *slo, *shi;  // slp: lp|reach, slr/swr: 3 per shape, shh: h|mu
+    // broadphase
+    uint32_t *ecnt, *eoff, *ekey;
+    int32_t* eshape;
+    uint32_t *tcnt, *tstart, *tfill;
+    int32_t* tent;
+    uint32_t *ccnt, *coff;
+    int2* cand;
+    float4* nar;  // 5 per candidate: n|count, points p|sep
+    uint32_t *hit, *hoff;
+    ManBuf cur, old;
+    // colouring
+    int32_t *mcol, *mcand;
+    unsigned long long *mpri, *bmask, *claim;
+    uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff;
+    int32_t* order;
+    // per-slot manifold lists (m * 2 + side), in manifold order
+    uint32_t *blcnt, *bloff, *blfill, *blist;
+    // islands
+    int32_t* parent;
+    uint32_t* mint;
+    uint32_t* ctr;
+};
+
+#define GRID_STRIDE(i, n) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+__device__ __forceinline__ M3 load_m3(const float4* p) { M3 r; r.c[0] = v3f4(p[0]); r.c[1] = v3f4(p[1]); r.c[2] = v3f4(p[2]); return r; }
+__device__ __forceinline__ void store_m3(float4* p, const M3& r) { p[0] = f4(r.c[0], 0.0f); p[1] = f4(r.c[1], 0.0f); p[2] = f4(r.c[2], 0.0f); }
+__device__ __forceinline__ float4& MROW(const Dev& D, const ManBuf& M, int r, uint32_t m) { return M.row[(size_t)r * D.mcap + m]; }
+__device__ __forceinline__ uint32_t cell_key(int x, int y, int z) { return ((uint32_t)x & 1023u) | (((uint32_t)y & 1023u) << 10) | (((uint32_t)z & 1023u) << 20); }
+__device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) { return mix32(key * 0x9E3779B1u) & mask; }
+
+// Cells a box covers. A box over more than 8 cells along an axis (or 512 in all, or not finite) is left out of the
+// grid and counted as dropped: no sane body is that large or that fast, and it bounds every lane's loop.
+constexpr int MAX_CELLS_AXIS = 8;
+struct CellRange { int x0, y0, z0, x1, y1, z1; uint32_t n; };
+__device__ __forceinline__ CellRange cell_range(V3 lo, V3 hi, float inv_cell) {
+    CellRange c;
+    c.n = 0;
+    c.x0 = c.y0 = c.z0 = 0; c.x1 = c.y1 = c.z1 = -1;
+    float fl[3] = {lo.x * inv_cell, lo.y * inv_cell, lo.z * inv_cell}, fh[3] = {hi.x * inv_cell, hi.y * inv_cell, hi.z * inv_cell};
+    for (int a = 0; a < 3; a++) if (!(fabsf(fl[a]) < 1.0e8f) || !(fabsf(fh[a]) < 1.0e8f) || !(fh[a] - fl[a] < (float)MAX_CELLS_AXIS)) return c;
+    c.x0 = (int32_t)sf_floor(fl[0]); c.y0 = (int32_t)sf_floor(fl[1]); c.z0 = (int32_t)sf_floor(fl[2]);
+    c.x1 = (int32_t)sf_floor(fh[0]); c.y1 = (int32_t)sf_floor(fh[1]); c.z1 = (int32_t)sf_floor(fh[2]);
+    int nx = c.x1 - c.x0 + 1, ny = c.y1 - c.y0 + 1, nz = c.z1 - c.z0 + 1;
+    if (nx < 1 || ny < 1 || nz < 1 || nx > MAX_CELLS_AXIS || ny > MAX_CELLS_AXIS || nz > MAX_CELLS_AXIS) { c.x1 = c.x0 - 1; return c; }
+    c.n = (uint32_t)(nx * ny * nz);
+    return c;
+}
+
+// ---------------------------------------------------------------- prepare
+__global__ void __launch_bounds__(NT) k_prep_bodies(Dev D) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        float4 q4 = D.bq[i];
+        M3 R = q_to_m3(qf4(q4));
+        store_m3(D.brot + 3 * (size_t)i, R);
+        int awake = D.bflag[i] & 1;
+        int slot = awake ? (int)i + 1 : 0;
+        D.bslot[i] = slot;
+        D.wake[i] = 0u;
+        D.parent[i] = (int32_t)i;
+        D.mint[i] = 0x7F800000u;
+        if (awake) {
+            float4 p = D.bp[i];
+            D.sv[slot] = make_float4(D.bv[i].x, D.bv[i].y, D.bv[i].z, p.w);
+            D.sw[slot] = make_float4(D.bw[i].x, D.bw[i].y, D.bw[i].z, 0.0f);
+            D.sdp[slot] = make_float4(0, 0, 0, 0);
+            D.sth[slot] = make_float4(0, 0, 0, 0);
+            D.sq[slot] = q4;
+            D.sp0[slot] = make_float4(p.x, p.y, p.z, 0.0f);
+            store_m3(D.sii + 3 * (size_t)slot, sandwich(R, v3f4(D.bii[i])));
+            atomicAdd(&D.ctr[C_AWAKE], 1u);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_prep_shapes(Dev D, StepK K) {
+    GRID_STRIDE(s, (uint32_t)D.ns) {
+        int b = D.shb[s];
+        M3 R = load_m3(D.brot + 3 * (size_t)b);
+        M3 Rs = m3mul(R, load_m3(D.slr + 3 * (size_t)s));
+        float4 lp4 = D.slp[s];
+        V3 p = v3f4(D.bp[b]);
+        V3 wp = vadd(p, mulv(R, v3f4(lp4)));
+        V3 h = v3f4(D.shh[s]);
+        V3 e = v3(sf_abs(Rs.c[0].x) * h.x + sf_abs(Rs.c[1].x) * h.y + sf_abs(Rs.c[2].x) * h.z,
+                  sf_abs(Rs.c[0].y) * h.x + sf_abs(Rs.c[1].y) * h.y + sf_abs(Rs.c[2].y) * h.z,
+                  sf_abs(Rs.c[0].z) * h.x + sf_abs(Rs.c[1].z) * h.y + sf_abs(Rs.c[2].z) * h.z);
+        float m = (D.bflag[b] & 1) ? (vlen(v3f4(D.bv[b])) + vlen(v3f4(D.bw[b])) * lp4.w) * K.dt + 0.5f * K.spec : 0.5f * K.spec;
+        e = vadd(e, v3(m, m, m));
+        V3 lo = vsub(wp, e), hi = vadd(wp, e);
+        D.swp[s] = f4(wp, 0.0f);
+        store_m3(D.swr + 3 * (size_t)s, Rs);
+        D.slo[s] = f4(lo, 0.0f);
+        D.shi[s] = f4(hi, 0.0f);
+        CellRange cr = cell_range(lo, hi, K.inv_cell);
+        D.ecnt[s] = cr.n;
+        if (!cr.n) atomicAdd(&D.ctr[C_DROP], 1u);
+    }
+}
+
+// ---------------------------------------------------------------- broadphase
+__global__ void __launch_bounds__(NT) k_emit_entries(Dev D, StepK K) {
+    GRID_STRIDE(s, (uint32_t)D.ns) {
+        CellRange cr = cell_range(v3f4(D.slo[s]), v3f4(D.shi[s]), K.inv_cell);
+        if (!cr.n) continue;
+        uint32_t e = D.eoff[s];
+        for (int x = cr.x0; x <= cr.x1; x++) for (int y = cr.y0; y <= cr.y1; y++) for (int z = cr.z0; z <= cr.z1; z++, e++) {
+            if (e >= D.ecap) { atomicAdd(&D.ctr[C_DROP], 1u); continue; }
+            uint32_t key = cell_key(x, y, z);
+            D.ekey[e] = key;
+            D.eshape[e] = (int32_t)s;
+            atomicAdd(&D.tcnt[slot_of(key, D.tmask)], 1u);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_fill_table(Dev D) {
+    uint32_t ne = min(D.ctr[C_NE], D.ecap);
+    GRID_STRIDE(e, ne) {
+        uint32_t sl = slot_of(D.ekey[e], D.tmask);
+        uint32_t pos = D.tstart[sl] + atomicAdd(&D.tfill[sl], 1u);
+        D.tent[pos] = (int32_t)e;
+    }
+}
+
+__device__ __forceinline__ bool pair_ok(const Dev& D, int a, int b, V3 loa, V3 hia) {
+    int ba = D.shb[a], bb = D.shb[b];
+    if (ba == bb) return false;
+    int fa = D.bflag[ba] & 1, fb = D.bflag[bb] & 1;
+    if (!fa && !fb) return false;
+    if (D.bp[ba].w == 0.0f && D.bp[bb].w == 0.0f) return false;
+    V3 lob = v3f4(D.slo[b]), hib = v3f4(D.shi[b]);
+    if (loa.x > hib.x || lob.x > hia.x || loa.y > hib.y || lob.y > hia.y || loa.z > hib.z || lob.z > hia.z) return false;
+    return true;
+}
+
+// Partners of shape a of higher index, found in the cell where the pair's home is; `out` (may be null) receives them.
+__device__ uint32_t partners(const Dev& D, const StepK& K, int a, int2* out, uint32_t cap) {
+    V3 lo = v3f4(D.slo[a]), hi = v3f4(D.shi[a]);
+    CellRange cr = cell_range(lo, hi, K.inv_cell);
+    uint32_t n = 0;
+    if (!cr.n) return 0;
+    for (int x = cr.x0; x <= cr.x1; x++) for (int y = cr.y0; y <= cr.y1; y++) for (int z = cr.z0; z <= cr.z1; z++) {
+        uint32_t key = cell_key(x, y, z);
+        uint32_t sl = slot_of(key, D.tmask);
+        uint32_t s0 = D.tstart[sl], s1 = s0 + D.tcnt[sl];
+        for (uint32_t q = s0; q < s1; q++) {
+            int32_t e = D.tent[q];
+            if (D.ekey[e] != key) continue;
+            int b = D.eshape[e];
+            if (b <= a) continue;
+            if (!pair_ok(D, a, b, lo, hi)) continue;
+            V3 m = vmaxv(lo, v3f4(D.slo[b]));
+            if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
+            if (out && n < cap) out[n] = make_int2(a, b);
+            n++;
+        }
+    }
+    return n;
+}
+
+__global__ void __launch_bounds__(NT) k_count_cand(Dev D, StepK K) {
+    GRID_STRIDE(a, (uint32_t)D.ns) {
+        int ground = (D.bflag[D.shb[a]] & 1) && D.slo[a].y < 0.0f;
+        uint32_t n = partners(D, K, (int)a, nullptr, 0);
+        D.ccnt[a] = n + (uint32_t)ground;
+        if (ground) atomicAdd(&D.ctr[C_NGROUND], 1u);
+        atomicAdd(&D.ctr[C_NPAIR], n);
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_write_cand(Dev D, StepK K) {
+    GRID_STRIDE(a, (uint32_t)D.ns) {
+        uint32_t o = D.coff[a], n = D.ccnt[a];
+        if (n == 0) continue;
+        if (o + n > D.ccap) { atomicAdd(&D.ctr[C_DROP], n); continue; }
+        int ground = (D.bflag[D.shb[a]] & 1) && D.slo[a].y < 0.0f;
+        if (ground) D.cand[o++] = make_int2((int)a, -1);
+        uint32_t np = min(partners(D, K, (int)a, D.cand + o, n - (uint32_t)ground), n - (uint32_t)ground);
+        // partners by index (insertion sort; a shape has a handful)
+        for (uint32_t i = 1; i < np; i++) {
+            int2 v = D.cand[o + i];
+            uint32_t j = i;
+            while (j > 0 && D.cand[o + j - 1].y > v.y) { D.cand[o + j] = D.cand[o + j - 1]; j--; }
+            D.cand[o + j] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- narrow phase
+struct Pt { V3 p; float s; };
+
+__device__ __forceinline__ void tangents(V3 n, V3& t1, V3& t2) {
+    V3 t = sf_abs(n.x) > 0.57f ? v3(n.y, -n.x, 0.0f) : v3(0.0f, n.z, -n.y);
+    t1 = vnorm(t);
+    t2 = vcross(n, t1);
+}
+
+// keep the deepest point, the farthest from it and the two spanning the most area (phys.rs:507-527)
+__device__ int reduce4(Pt* pts, int cnt, V3 n) {
+    if (cnt <= 4) return cnt;
+    int i0 = 0;
+    for (int k = 1; k < cnt; k++) if (pts[k].s < pts[i0].s) i0 = k;
+    V3 p0 = pts[i0].p;
+    int i1 = 0; float best = -1.0f;
+    for (int k = 0; k < cnt; k++) { float d = vlen2(vsub(pts[k].p, p0)); if (d > best) { best = d; i1 = k; } }
+    V3 e = vsub(pts[i1].p, p0);
+    int i2 = 0; float b2 = -3.40282347e38f;
+    for (int k = 0; k < cnt; k++) { float a = vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b2) { b2 = a; i2 = k; } }
+    int i3 = 0; float b3 = -3.40282347e38f;
+    for (int k = 0; k < cnt; k++) { float a = -vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b3) { b3 = a; i3 = k; } }
+    int idx[4] = {i0, i1, i2, i3};
+    Pt sel[4] = {pts[i0], pts[i1], pts[i2], pts[i3]};
+    int out = 0;
+    for (int k = 0; k < 4; k++) {
+        bool dup = false;
+        for (int q = 0; q < k; q++) dup |= idx[q] == idx[k];
+        if (!dup) pts[out++] = sel[k];
+    }
+    return out;
+}
+
+// 15-axis SAT; face contact clips the incident face against the reference face's side planes; edge contact gives
+// the closest points of the two edges (phys.rs:531-628). Returns the point count, normal from A to B.
+__device__ int collide_boxes(V3 pa, const M3& ra, V3 ha, V3 pb, const M3& rb, V3 hb, float margin, Pt* out, V3& n_out) {
+    V3 t = vsub(pb, pa);
+    float ac[3][3];
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) ac[i][j] = sf_abs(vdot(ra.c[i], rb.c[j])) + 1e-6f;
+    float hav[3] = {ha.x, ha.y, ha.z}, hbv[3] = {hb.x, hb.y, hb.z};
+    float sa = -3.40282347e38f; int ia = 0;
+    for (int i = 0; i < 3; i++) {
+        float s = sf_abs(vdot(t, ra.c[i])) - (hav[i] + hbv[0] * ac[i][0] + hbv[1] * ac[i][1] + hbv[2] * ac[i][2]);
+        if (s > margin) return 0;
+        if (s > sa) { sa = s; ia = i; }
+    }
+    float sbm = -3.40282347e38f; int jb = 0;
+    for (int j = 0; j < 3; j++) {
+        float s = sf_abs(vdot(t, rb.c[j])) - (hbv[j] + hav[0] * ac[0][j] + hav[1] * ac[1][j] + hav[2] * ac[2][j]);
+        if (s > margin) return 0;
+        if (s > sbm) { sbm = s; jb = j; }
+    }
+    float se = -3.40282347e38f; int ei = 0, ej = 0; V3 el = v3(0, 0, 0);
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) {
+        V3 l = vcross(ra.c[i], rb.c[j]);
+        float len2 = vlen2(l);
+        if (len2 < 1e-8f) continue;
+        l = vmul(l, 1.0f / sf_sqrt(len2));
+        float r = hav[0] * sf_abs(vdot(ra.c[0], l)) + hav[1] * sf_abs(vdot(ra.c[1], l)) + hav[2] * sf_abs(vdot(ra.c[2], l))
+                + hbv[0] * sf_abs(vdot(rb.c[0], l)) + hbv[1] * sf_abs(vdot(rb.c[1], l)) + hbv[2] * sf_abs(vdot(rb.c[2], l));
+        float s = sf_abs(vdot(t, l)) - r;
+        if (s > margin) return 0;
+        if (s > se) { se = s; ei = i; ej = j; el = l; }
+    }
+    const float face_tol = 0.005f;
+    bool use_b = sbm > sa + face_tol;
+    float face_best = use_b ? sbm : sa;
+    if (se > face_best + 0.01f) {
+        V3 n = vdot(t, el) < 0.0f ? vneg(el) : el;
+        V3 ca = pa, cb = pb;
+        for (int k = 0; k < 3; k++) {
+            if (k != ei) { float s = vdot(ra.c[k], n) > 0.0f ? 1.0f : -1.0f; ca = vadd(ca, vmul(ra.c[k], s * hav[k])); }
+            if (k != ej) { float s = vdot(rb.c[k], n) > 0.0f ? -1.0f : 1.0f; cb = vadd(cb, vmul(rb.c[k], s * hbv[k])); }
+        }
+        V3 d1 = ra.c[ei], d2 = rb.c[ej];
+        V3 r = vsub(ca, cb);
+        float bb = vdot(d1, d2), c = vdot(d1, r), f = vdot(d2, r);
+        float den = 1.0f - bb * bb;
+        float s = den > 1e-6f ? (bb * f - c) / den : 0.0f;
+        s = sf_min(sf_max(s, -hav[ei]), hav[ei]);
+        float tt = bb * s + f;
+        tt = sf_min(sf_max(tt, -hbv[ej]), hbv[ej]);
+        s = sf_min(sf_max(bb * tt - c, -hav[ei]), hav[ei]);
+        V3 p1 = vadd(ca, vmul(d1, s)), p2 = vadd(cb, vmul(d2, tt));
+        out[0].p = vmul(vadd(p1, p2), 0.5f);
+        out[0].s = vdot(vsub(p2, p1), n);
+        n_out = n;
+        return 1;
+    }
+    V3 pr = use_b ? pb : pa, pi = use_b ? pa : pb;
+    const M3& rr = use_b ? rb : ra;
+    const M3& ri = use_b ? ra : rb;
+    float hr[3], hi[3];
+    for (int q = 0; q < 3; q++) { hr[q] = use_b ? hbv[q] : hav[q]; hi[q] = use_b ? hav[q] : hbv[q]; }
+    int k = use_b ? jb : ia;
+    V3 n = rr.c[k];
+    if (vdot(vsub(pi, pr), n) < 0.0f) n = vneg(n);
+    int j = 0; float bd = -1.0f;
+    for (int q = 0; q < 3; q++) { float d = sf_abs(vdot(ri.c[q], n)); if (d > bd) { bd = d; j = q; } }
+    float sgn = vdot(ri.c[j], n) > 0.0f ? -1.0f : 1.0f;
+    V3 fc = vadd(pi, vmul(ri.c[j], sgn * hi[j]));
+    int u = (j + 1) % 3, v = (j + 2) % 3;
+    V3 eu = vmul(ri.c[u], hi[u]), ev = vmul(ri.c[v], hi[v]);
+    V3 poly[8], tmp[8];
+    poly[0] = vadd(vadd(fc, eu), ev); poly[1] = vadd(vsub(fc, eu), ev); poly[2] = vsub(vsub(fc, eu), ev); poly[3] = vsub(vadd(fc, eu), ev);
+    int pn = 4;
+    int u2 = (k + 1) % 3, v2 = (k + 2) % 3;
+    for (int pl = 0; pl < 4; pl++) {
+        int ax = pl < 2 ? u2 : v2;
+        float sg = (pl & 1) ? -1.0f : 1.0f;
+        V3 dir = vmul(rr.c[ax], sg);
+        float off = vdot(dir, pr) + hr[ax];
+        int tn = 0;
+        for (int q = 0; q < pn; q++) {
+            V3 p0 = poly[q], p1 = poly[(q + 1) % pn];
+            float d0 = vdot(p0, dir) - off, d1 = vdot(p1, dir) - off;
+            if (d0 <= 0.0f) { if (tn < 8) tmp[tn++] = p0; }
+            if ((d0 <= 0.0f) != (d1 <= 0.0f)) {
+                float tt = d0 / (d0 - d1);
+                if (tn < 8) tmp[tn++] = vadd(p0, vmul(vsub(p1, p0), tt));
+            }
+        }
+        pn = tn;
+        for (int q = 0; q < pn; q++) poly[q] = tmp[q];
+        if (pn == 0) return 0;
+    }
+    V3 rc = vadd(pr, vmul(n, hr[k]));
+    int cnt = 0;
+    for (int q = 0; q < pn; q++) {
+        float sep = vdot(vsub(poly[q], rc), n);
+        if (sep <= margin) { out[cnt].p = vsub(poly[q], vmul(n, 0.5f * sep)); out[cnt].s = sep; cnt++; }
+    }
+    if (cnt == 0) return 0;
+    n_out = use_b ? vneg(n) : n;
+    return cnt;
+}
+
+__global__ void __launch_bounds__(NT) k_narrow(Dev D, StepK K) {
+    uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
+    GRID_STRIDE(i, nc) {
+        int2 c = D.cand[i];
+        Pt pts[8];
+        V3 n = v3(0, 0, 0);
+        int cnt = 0;
+        int ba = D.shb[c.x];
+        int slot_b = 0;
+        if (c.y < 0) {
+            // ground: the shape's corners under the speculative margin (phys.rs:418-432)
+            V3 wp = v3f4(D.swp[c.x]);
+            M3 R = load_m3(D.swr + 3 * (size_t)c.x);
+            V3 h = v3f4(D.shh[c.x]);
+            float margin = K.spec + sf_max(-D.bv[ba].y, 0.0f) * (1.0f / 60.0f);
+            for (int k = 0; k < 8; k++) {
+                V3 sg = v3((k & 1) ? 1.0f : -1.0f, (k & 2) ? 1.0f : -1.0f, (k & 4) ? 1.0f : -1.0f);
+                V3 cp = vadd(wp, mulv(R, v3(sg.x * h.x, sg.y * h.y, sg.z * h.z)));
+                if (cp.y < margin) { pts[cnt].p = v3(cp.x, 0.5f * cp.y, cp.z); pts[cnt].s = cp.y; cnt++; }
+            }
+            n = v3(0.0f, -1.0f, 0.0f);
+        } else {
+            int bb = D.shb[c.y];
+            slot_b = D.bslot[bb];
+            V3 va = v3f4(D.bv[ba]), vb = v3f4(D.bv[bb]);
+            float rel = vlen(vsub(vb, va)) + vlen(v3f4(D.bw[ba])) * D.slp[c.x].w + vlen(v3f4(D.bw[bb])) * D.slp[c.y].w;
+            float margin = K.spec + rel * (1.0f / 60.0f);
+            M3 Ra = load_m3(D.swr + 3 * (size_t)c.x), Rb = load_m3(D.swr + 3 * (size_t)c.y);
+            cnt = collide_boxes(v3f4(D.swp[c.x]), Ra, v3f4(D.shh[c.x]), v3f4(D.swp[c.y]), Rb, v3f4(D.shh[c.y]), margin, pts, n);
+            if (cnt) {
+                // wakes from the step-start flags: a moving awake body wakes a resting one it touches
+                int fa = D.bflag[ba], fb = D.bflag[bb];
+                bool a_awake = fa & 1, b_awake = fb & 1;
+                bool a_still = D.bii[ba].w != 0.0f, b_still = D.bii[bb].w != 0.0f;
+                if (a_awake && !b_awake && !a_still) D.wake[bb] = 1u;
+                if (b_awake && !a_awake && !b_still) D.wake[ba] = 1u;
+            }
+        }
+        if (cnt) cnt = reduce4(pts, cnt, n);
+        bool hit = cnt > 0 && !(D.bslot[ba] == 0 && slot_b == 0);
+        D.hit[i] = hit ? 1u : 0u;
+        float4* o = D.nar + 5 * (size_t)i;
+        o[0] = f4(n, (float)cnt);
+        for (int k = 0; k < cnt; k++) o[1 + k] = f4(pts[k].p, pts[k].s);
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_apply_wake(Dev D) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        if (!D.wake[i]) continue;
+        if ((D.bflag[i] & 1) || D.bp[i].w == 0.0f) continue;
+        D.bflag[i] |= 1;
+        float4 ii = D.bii[i]; ii.w = 0.0f; D.bii[i] = ii;
+        D.bsp[i] = D.bp[i];
+        D.bsq[i] = D.bq[i];
+        atomicAdd(&D.ctr[C_NWAKE], 1u);
+    }
+}
+
+__device__ int find_old(const Dev& D, uint64_t key, uint32_t nold) {
+    int lo = 0, hi = (int)nold - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        uint64_t k = D.old.key[mid];
+        if (k == key) return mid;
+        if (k < key) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ float mix_mu(float a, float b) { return a == b ? a : sf_sqrt(a * b); }
+
+// push_manifold (phys.rs:450-498) and the constraint masses (:190-211), one manifold per lane
+__global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
+    uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
+    GRID_STRIDE(i, nc) {
+        if (!D.hit[i]) continue;
+        uint32_t m = D.hoff[i];
+        if (m >= D.mcap) { atomicAdd(&D.ctr[C_DROP], 1u); continue; }
+        int2 c = D.cand[i];
+        int a = c.x, b = c.y;
+        int ba = D.shb[a];
+        int bb = b >= 0 ? D.shb[b] : -1;
+        V3 pa = v3f4(D.bp[ba]);
+        M3 ra = load_m3(D.brot + 3 * (size_t)ba);
+        V3 pb = bb >= 0 ? v3f4(D.bp[bb]) : v3(0, 0, 0);
+        M3 rbm;
+        if (bb >= 0) rbm = load_m3(D.brot + 3 * (size_t)bb);
+        int sa = D.bslot[ba], sb = bb >= 0 ? D.bslot[bb] : 0;
+        const float4* nr = D.nar + 5 * (size_t)i;
+        float4 n4 = nr[0];
+        V3 n = v3f4(n4);
+        int cnt = (int)n4.w;
+        Pt pts[4];
+        for (int k = 0; k < cnt; k++) { pts[k].p = v3f4(nr[1 + k]); pts[k].s = nr[1 + k].w; }
+        V3 t1, t2;
+        tangents(n, t1, t2);
+        float mu = mix_mu(D.shh[a].w, b >= 0 ? D.shh[b].w : K.friction);
+        bool stick = K.anchor != 0;
+        uint64_t key = man_key(a, b);
+        int oi = find_old(D, key, nold);
+        V3 cen = v3(0, 0, 0);
+        for (int k = 0; k < cnt; k++) cen = vadd(cen, pts[k].p);
+        cen = vmul(cen, 1.0f / (float)cnt);
+        float rad = 0.0f;
+        for (int k = 0; k < cnt; k++) rad += vlen(vsub(pts[k].p, cen));
+        rad = rad / (float)cnt;
+        V3 fa, fb, la_f, lb_f, e0 = v3(0, 0, 0);
+        float jt1 = 0.0f, jt2 = 0.0f, jtw = 0.0f;
+        bool anchored = false;
+        if (oi >= 0) {
+            const ManBuf& O = D.old;
+            float4 oj = MROW(D, O, MR_J, oi);
+            bool ostick = MROW(D, O, MR_CB2, oi).w != 0.0f;
+            if (ostick) {
+                V3 olaf = v3f4(MROW(D, O, MR_LAF, oi)), olbf = v3f4(MROW(D, O, MR_LBF, oi));
+                V3 wa = vadd(pa, mulv(ra, olaf));
+                V3 wb = bb >= 0 ? vadd(pb, mulv(rbm, olbf)) : olbf;
+                float lim = sf_max(0.05f, rad);
+                if (vlen2(vsub(wa, cen)) < lim * lim && vlen2(vsub(wb, wa)) < 0.03f * 0.03f) {
+                    la_f = olaf; lb_f = olbf; fa = vsub(wa, pa); fb = vsub(wb, pb); e0 = vsub(wb, wa); anchored = true;
+                }
+            }
+            V3 ot1 = v3f4(MROW(D, O, MR_T1, oi)), ot2 = v3f4(MROW(D, O, MR_T2, oi)), on = v3f4(MROW(D, O, MR_N, oi));
+            V3 f = vadd(vmul(ot1, oj.x), vmul(ot2, oj.y));
+            jt1 = vdot(f, t1); jt2 = vdot(f, t2); jtw = oj.z * vdot(on, n);
+        }
+        if (!anchored) {
+            fa = vsub(cen, pa); fb = vsub(cen, pb); la_f = tmulv(ra, fa);
+            lb_f = bb >= 0 ? tmulv(rbm, fb) : cen;
+            e0 = v3(0, 0, 0);
+        }
+        // masses (the static side: slot 0, zero inverse mass and inertia)
+        float ima = D.sv[sa].w, imb = D.sv[sb].w;
+        M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
+        const ManBuf& C = D.cur;
+        int ocnt = 0;
+        if (oi >= 0) ocnt = min(max((int)MROW(D, D.old, MR_CB1, oi).w, 0), 4);
+        for (int k = 0; k < cnt; k++) {
+            V3 rap = vsub(pts[k].p, pa);
+            V3 rbp = bb >= 0 ? vsub(pts[k].p, pb) : pts[k].p;
+            V3 la = tmulv(ra, rap);
+            float jn = 0.0f;
+            if (oi >= 0) {
+                float best = 0.03f * 0.03f; int bi = -1;
+                for (int q = 0; q < ocnt; q++) {
+                    float d = vlen2(vsub(v3f4(MROW(D, D.old, MR_P0 + MR_PER_PT * q + 4, oi)), la));
+                    if (d < best) { best = d; bi = q; }
+                }
+                if (bi >= 0) jn = MROW(D, D.old, MR_P0 + MR_PER_PT * bi + 2, oi).w;
+            }
+            V3 raxn = vcross(rap, n), rbxn = vcross(rbp, n);
+            V3 iaxn = mulv(iia, raxn), ibxn = mulv(iib, rbxn);
+            float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn);
+            float nm = kk > 0.0f ? 1.0f / kk : 0.0f;
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 0, m) = f4(raxn, pts[k].s);
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 1, m) = f4(rbxn, nm);
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m) = f4(iaxn, jn);
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m) = f4(ibxn, 0.0f);
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 4, m) = f4(la, 0.0f);
+        }
+        V3 ca1 = vcross(fa, t1), ca2 = vcross(fa, t2), cb1 = vcross(fb, t1), cb2 = vcross(fb, t2);
+        V3 ia1 = mulv(iia, ca1), ia2 = mulv(iia, ca2), ib1 = mulv(iib, cb1), ib2 = mulv(iib, cb2);
+        float k1 = ima + imb + vdot(ca1, ia1) + vdot(cb1, ib1);
+        float k2 = ima + imb + vdot(ca2, ia2) + vdot(cb2, ib2);
+        float tm1 = k1 > 0.0f ? 1.0f / k1 : 0.0f, tm2 = k2 > 0.0f ? 1.0f / k2 : 0.0f;
+        V3 ian = mulv(iia, n), ibn = mulv(iib, n);
+        float kt = vdot(n, ian) + vdot(n, ibn);
+        float twm = kt > 0.0f ? 1.0f / kt : 0.0f;
+        C.hdr[m] = make_int4(a, b, sa, sb);
+        C.key[m] = key;
+        MROW(D, C, MR_N, m) = f4(n, mu);
+        MROW(D, C, MR_T1, m) = f4(t1, tm1);
+        MROW(D, C, MR_T2, m) = f4(t2, tm2);
+        MROW(D, C, MR_E0, m) = f4(e0, twm);
+        MROW(D, C, MR_J, m) = make_float4(jt1, jt2, jtw, rad);
+        MROW(D, C, MR_CA1, m) = f4(ca1, ima);
+        MROW(D, C, MR_CA2, m) = f4(ca2, imb);
+        MROW(D, C, MR_CB1, m) = f4(cb1, (float)cnt);
+        MROW(D, C, MR_CB2, m) = f4(cb2, stick ? 1.0f : 0.0f);
+        MROW(D, C, MR_IA1, m) = f4(ia1, 0.0f);
+        MROW(D, C, MR_IA2, m) = f4(ia2, 0.0f);
+        MROW(D, C, MR_IB1, m) = f4(ib1, 0.0f);
+        MROW(D, C, MR_IB2, m) = f4(ib2, 0.0f);
+        MROW(D, C, MR_IAN, m) = f4(ian, 0.0f);
+        MROW(D, C, MR_IBN, m) = f4(ibn, 0.0f);
+        MROW(D, C, MR_LAF, m) = f4(la_f, 0.0f);
+        MROW(D, C, MR_LBF, m) = f4(lb_f, 0.0f);
+        atomicAdd(&D.ctr[C_NPTS], (uint32_t)cnt);
+        if (sa) atomicAdd(&D.blcnt[sa], 1u);
+        if (sb) atomicAdd(&D.blcnt[sb], 1u);
+    }
+}
+
+// ---------------------------------------------------------------- colouring
+__global__ void __launch_bounds__(NT) k_col_init(Dev D) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(m, nm) D.mcol[m] = -1;
+}
+
+__global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int r) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    const unsigned long long all = K == 64 ? ~0ull : ((1ull << K) - 1ull);
+    GRID_STRIDE(m, nm) {
+        if (D.mcol[m] != -1) continue;
+        int4 hd = D.cur.hdr[m];
+        int sa = hd.z, sb = hd.w;
+        unsigned long long used = (sa ? D.bmask[sa] : 0ull) | (sb ? D.bmask[sb] : 0ull);
+        unsigned long long fr = all & ~used;
+        if (!fr) { D.mcol[m] = K; continue; }
+        int nf = __popcll(fr);
+        uint32_t kh = key_hash(hd.x, hd.y);
+        int lim = nf < spread ? nf : spread;
+        int pick = (int)(mix32(kh + (uint32_t)r * 0x165667B1u) % (uint32_t)lim);
+        for (int q = 0; q < pick; q++) fr &= fr - 1;
+        int c = __ffsll((long long)fr) - 1;
+        unsigned long long pri = ((unsigned long long)(r + 1) << 56) |
+                                 ((unsigned long long)(mix32(kh ^ ((uint32_t)r * 0x27D4EB2Fu)) >> 8) << 32) |
+                                 (unsigned long long)(0xFFFFFFFFu - m);
+        D.mcand[m] = c;
+        D.mpri[m] = pri;
+        if (sa) atomicMax(&D.claim[(size_t)sa * K + c], pri);
+        if (sb) atomicMax(&D.claim[(size_t)sb * K + c], pri);
+        atomicAdd(&D.ctr[C_ACTIVE], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(m, nm) {
+        if (D.mcol[m] != -1) continue;
+        int4 hd = D.cur.hdr[m];
+        int sa = hd.z, sb = hd.w, c = D.mcand[m];
+        unsigned long long pri = D.mpri[m];
+        if (sa && D.claim[(size_t)sa * K + c] != pri) continue;
+        if (sb && D.claim[(size_t)sb * K + c] != pri) continue;
+        D.mcol[m] = c;
+        if (sa) atomicOr(&D.bmask[sa], 1ull << c);
+        if (sb) atomicOr(&D.bmask[sb], 1ull << c);
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(m, nm) {
+        int c = D.mcol[m];
+        if (c < 0) { c = K; D.mcol[m] = K; }
+        atomicAdd(&D.colcnt[c], 1u);
+        D.ovflag[m] = c == K ? 1u : 0u;
+        if (c < K) atomicMax(&D.ctr[C_NCOL], (uint32_t)c + 1u);
+        else atomicAdd(&D.ctr[C_NOVF], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(m, nm) {
+        int c = D.mcol[m];
+        if (c < K) D.order[D.coloff[c] + atomicAdd(&D.colfill[c], 1u)] = (int32_t)m;
+        else D.order[D.coloff[K] + D.ovoff[m]] = (int32_t)m;  // the overflow in manifold order
+    }
+}
+
+// ---------------------------------------------------------------- per-body contact lists
+__global__ void __launch_bounds__(NT) k_bl_fill(Dev D) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(m, nm) {
+        int4 hd = D.cur.hdr[m];
+        if (hd.z) D.blist[D.bloff[hd.z] + atomicAdd(&D.blfill[hd.z], 1u)] = m * 2u;
+        if (hd.w) D.blist[D.bloff[hd.w] + atomicAdd(&D.blfill[hd.w], 1u)] = m * 2u + 1u;
+    }
+}
+__global__ void __launch_bounds__(NT) k_bl_sort(Dev D) {
+    GRID_STRIDE(s, (uint32_t)D.nb + 1u) {
+        uint32_t o = D.bloff[s], n = D.blcnt[s];
+        for (uint32_t i = 1; i < n; i++) {
+            uint32_t v = D.blist[o + i];
+            uint32_t j = i;
+            while (j > 0 && D.blist[o + j - 1] > v) { D.blist[o + j] = D.blist[o + j - 1]; j--; }
+            D.blist[o + j] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- solver
+// gravity, spin clamp, then this body's warm start gathered over its contacts in manifold order: the same
+// sequence of f32 updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262)
+__global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        int s = D.bslot[i];
+        if (!s) continue;
+        float4 v4 = D.sv[s];
+        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
+        v = vadd(v, vmul(v3(K.gx, K.gy, K.gz), K.h));
+        float wl = vlen2(w);
+        if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
+        const ManBuf& C = D.cur;
+        uint32_t o = D.bloff[s], n = D.blcnt[s];
+        for (uint32_t e = 0; e < n; e++) {
+            uint32_t ent = D.blist[o + e];
+            uint32_t m = ent >> 1;
+            bool side_b = ent & 1u;
+            int cnt = (int)MROW(D, C, MR_CB1, m).w;
+            float pn = 0.0f;
+            for (int k = 0; k < cnt; k++) {
+                float4 r2 = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
+                float jn = r2.w;
+                pn += jn;
+                if (!side_b) w = vsub(w, vmul(v3f4(r2), jn));
+                else w = vadd(w, vmul(v3f4(MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m)), jn));
+            }
+            float4 j4 = MROW(D, C, MR_J, m);
+            V3 p = vadd(vadd(vmul(v3f4(MROW(D, C, MR_N, m)), pn), vmul(v3f4(MROW(D, C, MR_T1, m)), j4.x)), vmul(v3f4(MROW(D, C, MR_T2, m)), j4.y));
+            if (!side_b) {
+                v = vsub(v, vmul(p, MROW(D, C, MR_CA1, m).w));
+                w = vsub(w, vadd(vadd(vmul(v3f4(MROW(D, C, MR_IA1, m)), j4.x), vmul(v3f4(MROW(D, C, MR_IA2, m)), j4.y)), vmul(v3f4(MROW(D, C, MR_IAN, m)), j4.z)));
+            } else {
+                v = vadd(v, vmul(p, MROW(D, C, MR_CA2, m).w));
+                w = vadd(w, vadd(vadd(vmul(v3f4(MROW(D, C, MR_IB1, m)), j4.x), vmul(v3f4(MROW(D, C, MR_IB2, m)), j4.y)), vmul(v3f4(MROW(D, C, MR_IBN, m)), j4.z)));
+            }
+        }
+        D.sv[s] = f4(v, v4.w);
+        D.sw[s] = f4(w, 0.0f);
+    }
+}
+
+__device__ __forceinline__ float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
+
+// one manifold of the soft-step solve (phys.rs:268-323)
+__device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
+    const ManBuf& C = D.cur;
+    int4 hd = C.hdr[m];
+    int sa = hd.z, sb = hd.w;
+    float4 va4 = D.sv[sa], vb4 = D.sv[sb];
+    V3 va = v3f4(va4), wa = v3f4(D.sw[sa]), vb = v3f4(vb4), wb = v3f4(D.sw[sb]);
+    V3 tha = v3f4(D.sth[sa]), thb = v3f4(D.sth[sb]);
+    float4 n4 = MROW(D, C, MR_N, m), t14 = MROW(D, C, MR_T1, m), t24 = MROW(D, C, MR_T2, m);
+    V3 n = v3f4(n4), t1 = v3f4(t14), t2 = v3f4(t24);
+    V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
+    float dn = vdot(dpab, n);
+    float4 ca14 = MROW(D, C, MR_CA1, m), ca24 = MROW(D, C, MR_CA2, m), cb14 = MROW(D, C, MR_CB1, m), cb24 = MROW(D, C, MR_CB2, m);
+    float ima = ca14.w, imb = ca24.w;
+    int cnt = (int)cb14.w;
+    float tot = 0.0f;
+    for (int k = 0; k < cnt; k++) {
+        float4 r0 = MROW(D, C, MR_P0 + MR_PER_PT * k + 0, m);
+        float4 r1 = MROW(D, C, MR_P0 + MR_PER_PT * k + 1, m);
+        float4 r2 = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
+        float4 r3 = MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m);
+        V3 raxn = v3f4(r0), rbxn = v3f4(r1);
+        float jn = r2.w;
+        float s = r0.w + dn + vdot(thb, rbxn) - vdot(tha, raxn);
+        float bias, ms, is;
+        if (s > 0.0f) { bias = s * K.inv_h; ms = 1.0f; is = 0.0f; }
+        else if (K.use_bias) { bias = sf_max(K.bias_rate * s, -K.max_push); ms = K.mass_scale; is = K.imp_scale; }
+        else { bias = 0.0f; ms = 1.0f; is = 0.0f; }
+        float vn = vdot(vsub(vb, va), n) + vdot(wb, rbxn) - vdot(wa, raxn);
+        float imp = -r1.w * ms * (vn + bias) - is * jn;
+        float nj = sf_max(jn + imp, 0.0f);
+        imp = nj - jn;
+        tot += nj;
+        reinterpret_cast<float*>(&MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m))[3] = nj;
+        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(v3f4(r2), imp));
+        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(v3f4(r3), imp));
+    }
+    float lim = n4.w * tot;
+    V3 ca1 = v3f4(ca14), ca2 = v3f4(ca24), cb1 = v3f4(cb14), cb2 = v3f4(cb24);
+    V3 dv = vsub(vb, va);
+    float v1 = vdot(dv, t1) + vdot(wb, cb1) - vdot(wa, ca1);
+    float v2 = vdot(dv, t2) + vdot(wb, cb2) - vdot(wa, ca2);
+    bool stick = cb24.w != 0.0f;
+    bool anchored = K.use_bias && stick;
+    float b1, b2, ms, is;
+    if (anchored) {
+        V3 e0 = v3f4(MROW(D, C, MR_E0, m));
+        float e1 = vdot(e0, t1) + vdot(dpab, t1) + vdot(thb, cb1) - vdot(tha, ca1);
+        float e2 = vdot(e0, t2) + vdot(dpab, t2) + vdot(thb, cb2) - vdot(tha, ca2);
+        float ap = K.anchor_push, sl = K.anchor_slop;
+        b1 = sf_min(sf_max(K.bias_rate * slop_fn(e1, sl), -ap), ap);
+        b2 = sf_min(sf_max(K.bias_rate * slop_fn(e2, sl), -ap), ap);
+        ms = K.mass_scale; is = K.imp_scale;
+    } else { b1 = 0.0f; b2 = 0.0f; ms = 1.0f; is = 0.0f; }
+    float4 j4 = MROW(D, C, MR_J, m);
+    float o1 = j4.x, o2 = j4.y;
+    float j1 = o1 - t14.w * ms * (v1 + b1) - is * o1;
+    float j2 = o2 - t24.w * ms * (v2 + b2) - is * o2;
+    float l2 = j1 * j1 + j2 * j2;
+    if (l2 > lim * lim) {
+        float kk = lim / sf_sqrt(l2); j1 *= kk; j2 *= kk;
+        reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;
+    }
+    float d1 = j1 - o1, d2 = j2 - o2;
+    V3 p = vadd(vmul(t1, d1), vmul(t2, d2));
+    va = vsub(va, vmul(p, ima)); wa = vsub(wa, vadd(vmul(v3f4(MROW(D, C, MR_IA1, m)), d1), vmul(v3f4(MROW(D, C, MR_IA2, m)), d2)));
+    vb = vadd(vb, vmul(p, imb)); wb = vadd(wb, vadd(vmul(v3f4(MROW(D, C, MR_IB1, m)), d1), vmul(v3f4(MROW(D, C, MR_IB2, m)), d2)));
+    float wl = lim * j4.w;
+    float ot = j4.z;
+    float twm = MROW(D, C, MR_E0, m).w;
+    float jt = sf_min(sf_max(ot - twm * vdot(vsub(wb, wa), n), -wl), wl);
+    float dtw = jt - ot;
+    MROW(D, C, MR_J, m) = make_float4(j1, j2, jt, j4.w);
+    wa = vsub(wa, vmul(v3f4(MROW(D, C, MR_IAN, m)), dtw));
+    wb = vadd(wb, vmul(v3f4(MROW(D, C, MR_IBN, m)), dtw));
+    if (sa) { D.sv[sa] = f4(va, va4.w); D.sw[sa] = f4(wa, 0.0f); }
+    if (sb) { D.sv[sb] = f4(vb, vb4.w); D.sw[sb] = f4(wb, 0.0f); }
+}
+
+// one colour bucket: its manifolds share no moving body, so every lane solves one
+__global__ void __launch_bounds__(NT) k_solve_bucket(Dev D, StepK K, int c) {
+    uint32_t n = D.colcnt[c], o = D.coloff[c];
+    GRID_STRIDE(i, n) solve_one(D, (uint32_t)D.order[o + i], K);
+}
+
+// the small colours from c0 on, one after another in one workgroup, then the overflow one by one in order
+__global__ void __launch_bounds__(NT) k_solve_tail(Dev D, StepK K, int c0) {
+    int ncol = (int)D.ctr[C_NCOL];
+    for (int c = c0; c < ncol; c++) {
+        uint32_t n = D.colcnt[c], o = D.coloff[c];
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) solve_one(D, (uint32_t)D.order[o + i], K);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
+        for (uint32_t i = 0; i < n; i++) solve_one(D, (uint32_t)D.order[o + i], K);
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_integrate(Dev D, StepK K) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        int s = D.bslot[i];
+        if (!s) continue;
+        V3 v = v3f4(D.sv[s]), w = v3f4(D.sw[s]);
+        D.sdp[s] = f4(vadd(v3f4(D.sdp[s]), vmul(v, K.h)), 0.0f);
+        D.sth[s] = f4(vadd(v3f4(D.sth[s]), vmul(w, K.h)), 0.0f);
+        D.sq[s] = f4q(qintegrate(qf4(D.sq[s]), w, K.h));
+    }
+}
+
+// ---------------------------------------------------------------- write back, islands, sleep
+__global__ void __launch_bounds__(NT) k_writeback(Dev D) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        int s = D.bslot[i];
+        if (!s) continue;
+        float4 p = D.bp[i];
+        V3 np = vadd(v3f4(D.sp0[s]), v3f4(D.sdp[s]));
+        D.bp[i] = f4(np, p.w);
+        D.bv[i] = f4(v3f4(D.sv[s]), 0.0f);
+        D.bw[i] = f4(v3f4(D.sw[s]), 0.0f);
+        D.bq[i] = D.sq[s];
+    }
+}
+
+__device__ __forceinline__ int uf_find(int32_t* par, int x) {
+    while (true) {
+        int p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+        if (p == x) return x;
+        int g = __atomic_load_n(&par[p], __ATOMIC_RELAXED);
+        if (g != p) atomicCAS(&par[x], p, g);  // path halving
+        x = g;
+    }
+}
+// hook the larger root under the smaller: every island's root ends as its minimum body index
+__global__ void __launch_bounds__(NT) k_uf_hook(Dev D) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(m, nm) {
+        int4 hd = D.cur.hdr[m];
+        if (hd.z == 0 || hd.w == 0) continue;
+        int a = hd.z - 1, b = hd.w - 1;
+        while (true) {
+            a = uf_find(D.parent, a);
+            b = uf_find(D.parent, b);
+            if (a == b) break;
+            int hi = a > b ? a : b, lo = a > b ? b : a;
+            if (atomicCAS(&D.parent[hi], hi, lo) == hi) break;
+        }
+    }
+}
+__global__ void __launch_bounds__(NT) k_uf_flatten_sleep(Dev D, StepK K) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        int r = uf_find(D.parent, (int)i);
+        D.parent[i] = r;
+        if (!(D.bflag[i] & 1)) continue;
+        // rest timer: still = within a small distance and angle of the pose where the timer last restarted
+        Qt q = qf4(D.bq[i]);
+        Qt dq = qmul(q, qconj(qf4(D.bsq[i])));
+        float ang = 2.0f * sf_sqrt(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z);
+        V3 p = v3f4(D.bp[i]);
+        float4 ii = D.bii[i];
+        bool near = vlen2(vsub(p, v3f4(D.bsp[i]))) < K.sleep_dist2 && ang < K.sleep_rot;
+        if (near && vlen2(v3f4(D.bv[i])) < 100.0f * K.sleep_lin2 && vlen2(v3f4(D.bw[i])) < 100.0f * K.sleep_ang2) ii.w += K.dt;
+        else { ii.w = 0.0f; D.bsp[i] = D.bp[i]; D.bsq[i] = D.bq[i]; }
+        D.bii[i] = ii;
+        atomicMin(&D.mint[r], __float_as_uint(ii.w));
+    }
+}
+__global__ void __launch_bounds__(NT) k_sleep_apply(Dev D, StepK K) {
+    if (!K.allow_sleep) return;
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        if (!(D.bflag[i] & 1)) continue;
+        int r = D.parent[i];
+        if (__uint_as_float(D.mint[r]) >= K.sleep_time) {
+            D.bflag[i] &= ~1;
+            D.bv[i] = make_float4(0, 0, 0, 0);
+            D.bw[i] = make_float4(0, 0, 0, 0);
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) { for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v; }
+__global__ void k_fill_u64(unsigned long long* p, size_t n) { for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull; }
+
+// ---------------------------------------------------------------- host
+struct HostWorld {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int body_cap = 0, shape_cap = 0;
+    float prm[SF_NPARAM];
+    Dev D{};
+    std::vector<void*> allocs;
+    uint32_t nold = 0;
+    int32_t stats[SF_NSTAT] = {0};
+    std::vector<float> pose;
+    uint32_t* scan_tmp = nullptr;
+    uint32_t* h_ctr = nullptr;     // pinned
+    uint32_t* h_col = nullptr;     // pinned colour counts
+    bool timing = false;
+    // live timing of the solve-bucket kernel: an event pair around each launch
+    bool ktime = false;
+    std::vector<hipEvent_t> kev;
+    size_t kev_used = 0;
+    double ksolve_ms = 0.0;
+    int64_t ksolve_launches = 0, ksolve_items = 0;
+    hipEvent_t ev[6] = {};
+    float stage_ms[5] = {0};
+    std::vector<int32_t> h_shb;
+    std::vector<float> h_h;  // half extents of the first shape of each body (pose())
+
+    ~HostWorld() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void* p : allocs) (void)hipFree(p);
+        if (h_ctr) (void)hipHostFree(h_ctr);
+        if (h_col) (void)hipHostFree(h_col);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        for (auto& e : kev) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    template <class T> int alloc(T*& p, size_t n) {
+        void* q = nullptr;
+        SF_HIP(hipMalloc(&q, sizeof(T) * (n ? n : 1)));
+        SF_HIP(hipMemset(q, 0, sizeof(T) * (n ? n : 1)));
+        allocs.push_back(q);
+        p = (T*)q;
+        return 0;
+    }
+};
+
+void default_params(float* p) {
+    p[SF_P_GX] = 0.0f; p[SF_P_GY] = -9.81f; p[SF_P_GZ] = 0.0f; p[SF_P_SUBSTEPS] = 12; p[SF_P_HERTZ] = 0.0f;
+    p[SF_P_DAMPING] = 10.0f; p[SF_P_MAXPUSH] = 3.0f; p[SF_P_FRICTION] = 0.6f; p[SF_P_SPEC] = 0.02f;
+    p[SF_P_SLEEP_LIN] = 0.05f; p[SF_P_SLEEP_ANG] = 0.08f; p[SF_P_SLEEP_TIME] = 0.5f; p[SF_P_SLEEP_DIST] = 0.01f;
+    p[SF_P_SLEEP_ROT] = 0.02f; p[SF_P_ALLOW_SLEEP] = 1.0f; p[SF_P_CELL] = 1.0f; p[SF_P_LEAK] = 1.0f;
+    p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 64;
+    p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_RESERVED] = 0;
+}
+
+int world_alloc(HostWorld* w) {
+    Dev& D = w->D;
+    int nb = w->body_cap, ns = w->shape_cap;
+    D.ecap = (uint32_t)std::min<uint64_t>(8ull * ns + 4096, 0x7FFFFFFFull);
+    uint32_t t = 1; while (t < 2 * D.ecap) t <<= 1;
+    D.tcap = t; D.tmask = t - 1;
+    D.ccap = (uint32_t)(12ull * ns + 4096);
+    D.mcap = D.ccap;
+    size_t n1 = (size_t)nb + 1;
+    int r = 0;
+    r |= w->alloc(D.bp, nb); r |= w->alloc(D.bq, nb); r |= w->alloc(D.bv, nb); r |= w->alloc(D.bw, nb);
+    r |= w->alloc(D.bii, nb); r |= w->alloc(D.bsp, nb); r |= w->alloc(D.bsq, nb); r |= w->alloc(D.brot, 3 * (size_t)nb);
+    r |= w->alloc(D.bflag, nb); r |= w->alloc(D.bslot, nb); r |= w->alloc(D.wake, nb);
+    r |= w->alloc(D.sv, n1); r |= w->alloc(D.sw, n1); r |= w->alloc(D.sdp, n1); r |= w->alloc(D.sth, n1);
+    r |= w->alloc(D.sq, n1); r |= w->alloc(D.sp0, n1); r |= w->alloc(D.sii, 3 * n1);
+    r |= w->alloc(D.shb, ns); r |= w->alloc(D.slp, ns); r |= w->alloc(D.slr, 3 * (size_t)ns); r |= w->alloc(D.shh, ns);
+    r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
+    r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
+    r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap);
+    r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
+    r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
+    for (ManBuf* M : {&D.cur, &D.old}) {
+        r |= w->alloc(M->hdr, D.mcap); r |= w->alloc(M->row, (size_t)MR_ROWS * D.mcap); r |= w->alloc(M->key, D.mcap);
+    }
+    r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
+    r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
+    r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
+    r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap); r |= w->alloc(D.order, D.mcap);
+    r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap);
+    r |= w->alloc(D.parent, nb); r |= w->alloc(D.mint, nb); r |= w->alloc(D.ctr, C_NCTR);
+    uint32_t big = std::max<uint32_t>(std::max<uint32_t>(D.tcap, D.ccap), (uint32_t)n1);
+    r |= w->alloc(w->scan_tmp, sf::scan_scratch_words(big));
+    if (r) return r;
+    SF_HIP(hipHostMalloc((void**)&w->h_ctr, sizeof(uint32_t) * C_NCTR, hipHostMallocDefault));
+    SF_HIP(hipHostMalloc((void**)&w->h_col, sizeof(uint32_t) * 72, hipHostMallocDefault));
+    for (auto& e : w->ev) SF_HIP(hipEventCreate(&e));
+    return 0;
+}
+
+StepK make_step(const float* P, float dt) {
+    StepK K{};
+    int sub = (int)P[SF_P_SUBSTEPS]; if (sub < 1) sub = 1;
+    float h = dt / (float)sub;
+    float inv_h = 1.0f / h;
+    float hz = P[SF_P_HERTZ] > 0.0f ? P[SF_P_HERTZ] : inv_h / 6.0f;
+    float omega = 2.0f * 3.14159265358979323846f * hz;
+    float a1 = 2.0f * P[SF_P_DAMPING] + h * omega;
+    float a2 = h * omega * a1;
+    float a3 = 1.0f / (1.0f + a2);
+    K.dt = dt; K.h = h; K.inv_h = inv_h;
+    K.bias_rate = omega / a1; K.mass_scale = a2 * a3; K.imp_scale = a3 * P[SF_P_LEAK];
+    K.max_push = P[SF_P_MAXPUSH]; K.anchor_push = P[SF_P_ANCHOR_PUSH]; K.anchor_slop = P[SF_P_ANCHOR_SLOP];
+    K.spec = P[SF_P_SPEC]; K.inv_cell = 1.0f / P[SF_P_CELL]; K.friction = P[SF_P_FRICTION];
+    K.sleep_lin2 = P[SF_P_SLEEP_LIN] * P[SF_P_SLEEP_LIN]; K.sleep_ang2 = P[SF_P_SLEEP_ANG] * P[SF_P_SLEEP_ANG];
+    K.sleep_time = P[SF_P_SLEEP_TIME]; K.sleep_dist2 = P[SF_P_SLEEP_DIST] * P[SF_P_SLEEP_DIST]; K.sleep_rot = P[SF_P_SLEEP_ROT];
+    K.wmax = 0.25f * 3.14159265358979323846f * inv_h;
+    K.gx = P[SF_P_GX]; K.gy = P[SF_P_GY]; K.gz = P[SF_P_GZ];
+    K.anchor = P[SF_P_ANCHOR] > 0.5f; K.allow_sleep = P[SF_P_ALLOW_SLEEP] > 0.5f;
+    K.K = (int)P[SF_P_COL_LIMIT]; if (K.K < 1) K.K = 1; if (K.K > 64) K.K = 64;
+    K.spread = (int)P[SF_P_COL_SPREAD]; if (K.spread < 1) K.spread = 1;
+    K.use_bias = 1;
+    return K;
+}
+
+constexpr int TAIL_MIN = 2048;  // colour buckets smaller than this go to the one-workgroup tail pass
+
+// Everything up to the colour counts; the sub-steps are planned from the counts read back by the caller.
+int step_front(HostWorld* w, const StepK& K) {
+    Dev& D = w->D;
+    hipStream_t s = w->st;
+    const unsigned gb = sf_grid(D.nb), gs = sf_grid(D.ns), gc = sf_grid(D.ccap), gm = sf_grid(D.mcap);
+    // old manifolds = last step's
+    std::swap(D.cur, D.old);
+    // counters and per-step tables
+    SF_HIP(hipMemsetAsync(D.ctr, 0, sizeof(uint32_t) * C_NCTR, s));
+    SF_HIP(hipMemsetAsync(D.tcnt, 0, sizeof(uint32_t) * D.tcap, s));
+    SF_HIP(hipMemsetAsync(D.tfill, 0, sizeof(uint32_t) * D.tcap, s));
+    SF_HIP(hipMemsetAsync(D.blcnt, 0, sizeof(uint32_t) * ((size_t)D.nb + 1), s));
+    SF_HIP(hipMemsetAsync(D.blfill, 0, sizeof(uint32_t) * ((size_t)D.nb + 1), s));
+    SF_HIP(hipMemsetAsync(D.bmask, 0, sizeof(unsigned long long) * ((size_t)D.nb + 1), s));
+    SF_HIP(hipMemsetAsync(D.claim, 0, sizeof(unsigned long long) * ((size_t)D.nb + 1) * K.K, s));
+    SF_HIP(hipMemsetAsync(D.colcnt, 0, sizeof(uint32_t) * 72, s));
+    SF_HIP(hipMemsetAsync(D.colfill, 0, sizeof(uint32_t) * 72, s));
+    if (w->timing) SF_HIP(hipEventRecord(w->ev[0], s));
+    hipLaunchKernelGGL(k_prep_bodies, dim3(gb), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_prep_shapes, dim3(gs), dim3(NT), 0, s, D, K);
+    sf::scan_u32(D.ecnt, D.eoff, D.ctr + C_NE, nullptr, (uint32_t)D.ns, (uint32_t)D.ns, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_emit_entries, dim3(gs), dim3(NT), 0, s, D, K);
+    sf::scan_u32(D.tcnt, D.tstart, nullptr, nullptr, D.tcap, D.tcap, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_fill_table, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_count_cand, dim3(gs), dim3(NT), 0, s, D, K);
+    sf::scan_u32(D.ccnt, D.coff, D.ctr + C_NCAND, nullptr, (uint32_t)D.ns, (uint32_t)D.ns, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_write_cand, dim3(gs), dim3(NT), 0, s, D, K);
+    if (w->timing) SF_HIP(hipEventRecord(w->ev[1], s));
+    hipLaunchKernelGGL(k_narrow, dim3(gc), dim3(NT), 0, s, D, K);
+    sf::scan_u32(D.hit, D.hoff, D.ctr + C_NMAN, D.ctr + C_NCAND, 0, D.ccap, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_apply_wake, dim3(gb), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_build, dim3(gc), dim3(NT), 0, s, D, K, w->nold);
+    if (w->timing) SF_HIP(hipEventRecord(w->ev[2], s));
+    // colouring rounds (checked for progress every few rounds)
+    int R = (int)w->prm[SF_P_COL_ROUNDS];
+    hipLaunchKernelGGL(k_col_init, dim3(gm), dim3(NT), 0, s, D);
+    int rounds = 0;
+    for (int r = 0; r < R; r++) {
+        hipLaunchKernelGGL(k_col_claim, dim3(gm), dim3(NT), 0, s, D, K.K, K.spread, r);
+        hipLaunchKernelGGL(k_col_resolve, dim3(gm), dim3(NT), 0, s, D, K.K);
+        rounds++;
+        if ((r & 3) == 3 && r + 1 < R) {
+            SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
+            SF_HIP(hipStreamSynchronize(s));
+            if (w->h_ctr[C_ACTIVE] == 0) break;
+            SF_HIP(hipMemsetAsync(D.ctr + C_ACTIVE, 0, sizeof(uint32_t), s));
+        }
+    }
+    w->stats[SF_S_ROUNDS] = rounds;
+    hipLaunchKernelGGL(k_col_count, dim3(gm), dim3(NT), 0, s, D, K.K);
+    sf::scan_u32(D.colcnt, D.coloff, nullptr, nullptr, (uint32_t)K.K + 1, 72, w->scan_tmp, s);
+    sf::scan_u32(D.ovflag, D.ovoff, nullptr, D.ctr + C_NMAN, 0, D.mcap, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
+    // per-slot contact lists in manifold order
+    sf::scan_u32(D.blcnt, D.bloff, nullptr, nullptr, (uint32_t)D.nb + 1, (uint32_t)D.nb + 1, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_bl_fill, dim3(gm), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_bl_sort, dim3(sf_grid((uint64_t)D.nb + 1)), dim3(NT), 0, s, D);
+    if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
+    SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
+    SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int launch_bucket(HostWorld* w, const StepK& K, int c) {
+    hipStream_t s = w->st;
+    if (w->ktime) {
+        while (w->kev.size() < w->kev_used + 2) { hipEvent_t e; SF_HIP(hipEventCreate(&e)); w->kev.push_back(e); }
+        SF_HIP(hipEventRecord(w->kev[w->kev_used], s));
+    }
+    hipLaunchKernelGGL(k_solve_bucket, dim3(sf_grid(w->h_col[c])), dim3(NT), 0, s, w->D, K, c);
+    if (w->ktime) {
+        SF_HIP(hipEventRecord(w->kev[w->kev_used + 1], s));
+        w->kev_used += 2;
+        w->ksolve_launches++;
+        w->ksolve_items += w->h_col[c];
+    }
+    return 0;
+}
+
+int step_back(HostWorld* w, StepK K) {
+    Dev& D = w->D;
+    hipStream_t s = w->st;
+    const unsigned gb = sf_grid(D.nb), gm = sf_grid(D.mcap);
+    int ncol = (int)w->h_ctr[C_NCOL];
+    int big = 0;
+    while (big < ncol && (int)w->h_col[big] >= TAIL_MIN) big++;
+    bool tail = big < ncol || w->h_col[K.K] > 0;
+    int sub = (int)w->prm[SF_P_SUBSTEPS]; if (sub < 1) sub = 1;
+    StepK Kb = K; Kb.use_bias = 1;
+    StepK Kr = K; Kr.use_bias = 0;
+    for (int it = 0; it < sub; it++) {
+        hipLaunchKernelGGL(k_warm, dim3(gb), dim3(NT), 0, s, D, K);
+        for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kb, c); if (r) return r; }
+        if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kb, big);
+        hipLaunchKernelGGL(k_integrate, dim3(gb), dim3(NT), 0, s, D, K);
+        for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kr, c); if (r) return r; }
+        if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
+    }
+    if (w->timing) SF_HIP(hipEventRecord(w->ev[4], s));
+    hipLaunchKernelGGL(k_writeback, dim3(gb), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_uf_hook, dim3(gm), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_uf_flatten_sleep, dim3(gb), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_sleep_apply, dim3(gb), dim3(NT), 0, s, D, K);
+    if (w->timing) SF_HIP(hipEventRecord(w->ev[5], s));
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+
+int finish_stats(HostWorld* w) {
+    uint32_t* c = w->h_ctr;
+    w->nold = std::min(c[C_NMAN], w->D.mcap);
+    w->stats[SF_S_AWAKE] = (int32_t)c[C_AWAKE];
+    w->stats[SF_S_PAIRS] = (int32_t)c[C_NPAIR];
+    w->stats[SF_S_MANIFOLDS] = (int32_t)std::min(c[C_NMAN], w->D.mcap);
+    w->stats[SF_S_POINTS] = (int32_t)c[C_NPTS];
+    w->stats[SF_S_COLOURS] = (int32_t)c[C_NCOL];
+    w->stats[SF_S_OVERFLOW] = (int32_t)c[C_NOVF];
+    w->stats[SF_S_GROUND] = (int32_t)c[C_NGROUND];
+    w->stats[SF_S_WAKES] = (int32_t)c[C_NWAKE];
+    w->stats[SF_S_ENTRIES] = (int32_t)c[C_NE];
+    w->stats[SF_S_CANDIDATES] = (int32_t)c[C_NCAND];
+    w->stats[SF_S_DROPPED] = (int32_t)c[C_DROP];
+    return 0;
+}
+
+int world_step(HostWorld* w, float dt) {
+    if (!(dt >= 0.0f) || dt > 1.0f) { sf::set_error(SF_E_ARG, "dt out of range"); return SF_E_ARG; }
+    SF_HIP(hipSetDevice(w->dev));
+    StepK K = make_step(w->prm, dt);
+    int r = step_front(w, K);
+    if (r) return r;
+    r = step_back(w, K);
+    if (r) return r;
+    SF_HIP(hipStreamSynchronize(w->st));
+    finish_stats(w);
+    if (w->timing) {
+        for (int k = 0; k < 5; k++) { float ms = 0; (void)hipEventElapsedTime(&ms, w->ev[k], w->ev[k + 1]); w->stage_ms[k] = ms; }
+    }
+    if (w->ktime) {
+        for (size_t k = 0; k < w->kev_used; k += 2) { float ms = 0; SF_HIP(hipEventElapsedTime(&ms, w->kev[k], w->kev[k + 1])); w->ksolve_ms += ms; }
+        w->kev_used = 0;
+    }
+    if (w->h_ctr[C_DROP]) { sf::set_error(SF_E_CAP, "capacity exceeded: candidates or manifolds dropped"); }
+    return w->stats[SF_S_AWAKE];
+}
+
+bool finite_all(const float* p, size_t n) { for (size_t i = 0; i < n; i++) if (!std::isfinite(p[i])) return false; return true; }
+
+int world_load(HostWorld* w, int nb, const float* bodies, int ns, const int32_t* shb, const float* shapes) {
+    if (nb < 0 || ns < 0 || nb > w->body_cap || ns > w->shape_cap) { sf::set_error(SF_E_CAP, "more bodies or shapes than the world's capacity"); return SF_E_CAP; }
+    if ((nb && !bodies) || (ns && (!shb || !shapes))) { sf::set_error(SF_E_ARG, "null input"); return SF_E_ARG; }
+    if (!finite_all(bodies, (size_t)nb * SF_BODY_F) || !finite_all(shapes, (size_t)ns * SF_SHAPE_F)) { sf::set_error(SF_E_ARG, "NaN or inf in the input"); return SF_E_ARG; }
+    for (int s = 0; s < ns; s++) {
+        if (shb[s] < 0 || shb[s] >= nb || (s > 0 && shb[s] < shb[s - 1])) { sf::set_error(SF_E_ARG, "shapes must name bodies in order"); return SF_E_ARG; }
+    }
+    SF_HIP(hipSetDevice(w->dev));
+    Dev& D = w->D;
+    D.nb = nb; D.ns = ns;
+    std::vector<float4> bp(nb), bq(nb), bv(nb), bw(nb), bii(nb);
+    std::vector<int32_t> fl(nb);
+    for (int i = 0; i < nb; i++) {
+        const float* f = bodies + (size_t)i * SF_BODY_F;
+        bp[i] = make_float4(f[0], f[1], f[2], f[13]);
+        bq[i] = make_float4(f[3], f[4], f[5], f[6]);
+        bv[i] = make_float4(f[7], f[8], f[9], 0.0f);
+        bw[i] = make_float4(f[10], f[11], f[12], 0.0f);
+        bii[i] = make_float4(f[14], f[15], f[16], f[18]);
+        fl[i] = f[17] > 0.5f ? 1 : 0;
+    }
+    std::vector<float4> slp(ns), slr(3 * (size_t)ns), shh(ns);
+    w->h_h.assign(3 * (size_t)nb, 0.0f);
+    std::vector<int> seen(nb, 0);
+    for (int s = 0; s < ns; s++) {
+        const float* f = shapes + (size_t)s * SF_SHAPE_F;
+        // reach = |lp| + |h| (the shape's farthest extent from the body centre, for speculative margins)
+        float lpx = f[0], lpy = f[1], lpz = f[2], hx = f[7], hy = f[8], hz = f[9];
+        float l1 = lpx * lpx + lpy * lpy + lpz * lpz, l2 = hx * hx + hy * hy + hz * hz;
+        float reach = (l1 <= 0.0f ? 0.0f : sqrtf(l1)) + (l2 <= 0.0f ? 0.0f : sqrtf(l2));
+        slp[s] = make_float4(lpx, lpy, lpz, reach);
+        float x = f[3], y = f[4], z = f[5], ww = f[6];
+        slr[3 * s + 0] = make_float4(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + z * ww), 2.0f * (x * z - y * ww), 0.0f);
+        slr[3 * s + 1] = make_float4(2.0f * (x * y - z * ww), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + x * ww), 0.0f);
+        slr[3 * s + 2] = make_float4(2.0f * (x * z + y * ww), 2.0f * (y * z - x * ww), 1.0f - 2.0f * (x * x + y * y), 0.0f);
+        shh[s] = make_float4(hx, hy, hz, f[10]);
+        if (!seen[shb[s]]) { seen[shb[s]] = 1; w->h_h[3 * shb[s]] = hx; w->h_h[3 * shb[s] + 1] = hy; w->h_h[3 * shb[s] + 2] = hz; }
+    }
+    hipStream_t s = w->st;
+    SF_HIP(hipMemcpyAsync(D.bp, bp.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.bq, bq.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.bv, bv.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.bw, bw.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.bii, bii.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.bsp, bp.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.bsq, bq.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.bflag, fl.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.shb, shb, sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.slp, slp.data(), sizeof(float4) * ns, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.slr, slr.data(), sizeof(float4) * 3 * ns, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.shh, shh.data(), sizeof(float4) * ns, hipMemcpyHostToDevice, s));
+    SF_HIP(hipStreamSynchronize(s));
+    w->nold = 0;
+    w->h_shb.assign(shb, shb + ns);
+    return 0;
+}
+
+int world_read_bodies(HostWorld* w, float* out) {
+    Dev& D = w->D;
+    int nb = D.nb;
+    std::vector<float4> bp(nb), bq(nb), bv(nb), bw(nb), bii(nb);
+    std::vector<int32_t> fl(nb);
+    SF_HIP(hipSetDevice(w->dev));
+    SF_HIP(hipStreamSynchronize(w->st));
+    SF_HIP(hipMemcpy(bp.data(), D.bp, sizeof(float4) * nb, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(bq.data(), D.bq, sizeof(float4) * nb, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(bv.data(), D.bv, sizeof(float4) * nb, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(bw.data(), D.bw, sizeof(float4) * nb, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(bii.data(), D.bii, sizeof(float4) * nb, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(fl.data(), D.bflag, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; i++) {
+        float* f = out + (size_t)i * SF_BODY_F;
+        f[0] = bp[i].x; f[1] = bp[i].y; f[2] = bp[i].z; f[3] = bq[i].x; f[4] = bq[i].y; f[5] = bq[i].z; f[6] = bq[i].w;
+        f[7] = bv[i].x; f[8] = bv[i].y; f[9] = bv[i].z; f[10] = bw[i].x; f[11] = bw[i].y; f[12] = bw[i].z;
+        f[13] = bp[i].w; f[14] = bii[i].x; f[15] = bii[i].y; f[16] = bii[i].z; f[17] = (fl[i] & 1) ? 1.0f : 0.0f;
+        f[18] = bii[i].w; f[19] = 0.0f;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+struct sf_world { HostWorld h; };
+
+extern "C" {
+
+sf_world* sf_world_create(int body_cap, int shape_cap, const float* params) {
+    if (body_cap <= 0 || shape_cap <= 0) { sf::set_error(SF_E_ARG, "capacities must be positive"); return nullptr; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { sf::set_error(SF_E_NODEV, "no GPU visible"); return nullptr; }
+    sf_world* w = new sf_world();
+    HostWorld* h = &w->h;
+    SF_HIP_NULL(hipGetDevice(&h->dev));
+    default_params(h->prm);
+    if (params) for (int i = 0; i < SF_NPARAM; i++) h->prm[i] = params[i];
+    h->body_cap = body_cap; h->shape_cap = shape_cap;
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) { delete w; sf::set_error(SF_E_HIP, "stream"); return nullptr; }
+    if (world_alloc(h) != 0) { delete w; return nullptr; }
+    return w;
+}
+void sf_world_destroy(sf_world* w) { delete w; }
+int sf_world_set_params(sf_world* w, const float* params, int n) {
+    if (!w || !params || n < 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    for (int i = 0; i < n && i < SF_NPARAM; i++) w->h.prm[i] = params[i];
+    return 0;
+}
+int sf_world_load(sf_world* w, int nbody, const float* bodies, int nshape, const int32_t* shape_body, const float* shapes) {
+    if (!w) { sf::set_error(SF_E_ARG, "null world"); return SF_E_ARG; }
+    return world_load(&w->h, nbody, bodies, nshape, shape_body, shapes);
+}
+int sf_world_step(sf_world* w, float dt) {
+    if (!w) { sf::set_error(SF_E_ARG, "null world"); return SF_E_ARG; }
+    return world_step(&w->h, dt);
+}
+int sf_world_step_many(sf_world* w, float dt, int nsteps) {
+    if (!w || nsteps < 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    for (int i = 0; i < nsteps; i++) { int r = world_step(&w->h, dt); if (r < 0) return r; }
+    return 0;
+}
+int sf_world_sync(sf_world* w) { if (!w) return SF_E_ARG; SF_HIP(hipStreamSynchronize(w->h.st)); return 0; }
+int sf_world_read_bodies(sf_world* w, float* out) {
+    if (!w || !out) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    return world_read_bodies(&w->h, out);
+}
+int sf_world_stats(sf_world* w, int32_t* out) {
+    if (!w || !out) return SF_E_ARG;
+    for (int i = 0; i < SF_NSTAT; i++) out[i] = w->h.stats[i];
+    return 0;
+}
+int sf_world_nbody(sf_world* w) { return w ? w->h.D.nb : SF_E_ARG; }
+int sf_world_candidates(sf_world* w, int32_t* out, int cap) {
+    if (!w) return SF_E_ARG;
+    HostWorld* h = &w->h;
+    int n = (int)std::min(h->h_ctr[C_NCAND], h->D.ccap);
+    if (out && cap > 0) SF_HIP(hipMemcpy(out, h->D.cand, sizeof(int2) * std::min(n, cap), hipMemcpyDeviceToHost));
+    return n;
+}
+int sf_world_manifolds(sf_world* w, int32_t* keys, float* data, int cap) {
+    if (!w) return SF_E_ARG;
+    HostWorld* h = &w->h;
+    Dev& D = h->D;
+    int n = (int)std::min(h->h_ctr[C_NMAN], D.mcap);
+    if (!keys || !data || cap <= 0) return n;
+    int k = std::min(n, cap);
+    std::vector<int4> hdr(k);
+    std::vector<float4> rows((size_t)MR_ROWS * k);
+    SF_HIP(hipMemcpy(hdr.data(), D.cur.hdr, sizeof(int4) * k, hipMemcpyDeviceToHost));
+    for (int r = 0; r < MR_ROWS; r++) SF_HIP(hipMemcpy(rows.data() + (size_t)r * k, D.cur.row + (size_t)r * D.mcap, sizeof(float4) * k, hipMemcpyDeviceToHost));
+    auto R = [&](int r, int m) { return rows[(size_t)r * k + m]; };
+    for (int m = 0; m < k; m++) {
+        int cnt = (int)R(MR_CB1, m).w;
+        keys[3 * m] = hdr[m].x; keys[3 * m + 1] = hdr[m].y; keys[3 * m + 2] = cnt;
+        float* f = data + (size_t)m * SF_MAN_F;
+        std::memset(f, 0, sizeof(float) * SF_MAN_F);
+        float4 n4 = R(MR_N, m);
+        f[0] = n4.x; f[1] = n4.y; f[2] = n4.z; f[3] = (float)cnt;
+        for (int q = 0; q < cnt; q++) {
+            float4 a = R(MR_P0 + MR_PER_PT * q, m), b = R(MR_P0 + MR_PER_PT * q + 1, m), c = R(MR_P0 + MR_PER_PT * q + 2, m);
+            float* o = f + 4 + 8 * q;
+            o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = b.x; o[4] = b.y; o[5] = b.z; o[6] = a.w; o[7] = c.w;
+        }
+        float4 j = R(MR_J, m);
+        f[36] = j.x; f[37] = j.y; f[38] = j.z; f[39] = R(MR_CB2, m).w;
+    }
+    return n;
+}
+int sf_world_colours(sf_world* w, int32_t* out, int cap) {
+    if (!w) return SF_E_ARG;
+    HostWorld* h = &w->h;
+    int n = (int)std::min(h->h_ctr[C_NMAN], h->D.mcap);
+    if (out && cap > 0) SF_HIP(hipMemcpy(out, h->D.mcol, sizeof(int32_t) * std::min(n, cap), hipMemcpyDeviceToHost));
+    return n;
+}
+int sf_world_islands(sf_world* w, int32_t* out) {
+    if (!w || !out) return SF_E_ARG;
+    SF_HIP(hipMemcpy(out, w->h.D.parent, sizeof(int32_t) * w->h.D.nb, hipMemcpyDeviceToHost));
+    return 0;
+}
+const float* sf_world_pose(sf_world* w) {
+    if (!w) return nullptr;
+    HostWorld* h = &w->h;
+    int nb = h->D.nb;
+    std::vector<float> b((size_t)nb * SF_BODY_F);
+    if (world_read_bodies(h, b.data()) != 0) return nullptr;
+    h->pose.resize((size_t)nb * 10);
+    for (int i = 0; i < nb; i++) {
+        const float* f = b.data() + (size_t)i * SF_BODY_F;
+        float* o = h->pose.data() + (size_t)i * 10;
+        for (int k = 0; k < 7; k++) o[k] = f[k];
+        o[7] = h->h_h[3 * i]; o[8] = h->h_h[3 * i + 1]; o[9] = h->h_h[3 * i + 2];
+    }
+    return h->pose.data();
+}
+int sf_world_set_timing(sf_world* w, int on) { if (!w) return SF_E_ARG; w->h.timing = on != 0; return 0; }
+int sf_world_set_kernel_timing(sf_world* w, int on) {
+    if (!w) return SF_E_ARG;
+    w->h.ktime = on != 0;
+    w->h.ksolve_ms = 0.0; w->h.ksolve_launches = 0; w->h.ksolve_items = 0;
+    return 0;
+}
+int sf_world_kernel_stats(sf_world* w, double* ms, int64_t* launches, int64_t* items) {
+    if (!w) return SF_E_ARG;
+    if (ms) *ms = w->h.ksolve_ms;
+    if (launches) *launches = w->h.ksolve_launches;
+    if (items) *items = w->h.ksolve_items;
+    return 0;
+}
+int sf_world_stage_ms(sf_world* w, float* out5) { if (!w || !out5) return SF_E_ARG; for (int k = 0; k < 5; k++) out5[k] = w->h.stage_ms[k]; return 0; }
+
+// ---- the reference's singleton box-core exports (engine/src/wasm.rs:27-53)
+static sf_world* g_world = nullptr;
+static uint32_t g_substeps = 12;
+static void replace_default(const std::vector<float>& bodies, const std::vector<int32_t>& shb, const std::vector<float>& shapes, const float* prm) {
+    if (g_world) sf_world_destroy(g_world);
+    int nb = (int)(bodies.size() / SF_BODY_F), ns = (int)shb.size();
+    g_world = sf_world_create(nb > 0 ? nb : 1, ns > 0 ? ns : 1, prm);
+    if (g_world) sf_world_load(g_world, nb, bodies.data(), ns, shb.data(), shapes.data());
+}
+void sf_pile_setup(uint32_t n, uint32_t substeps) {
+    int ns = 0;
+    int nb = sf_scene_pile((int)n, nullptr, nullptr, nullptr, &ns);
+    std::vector<float> b((size_t)nb * SF_BODY_F), s((size_t)ns * SF_SHAPE_F);
+    std::vector<int32_t> sb(ns);
+    sf_scene_pile((int)n, b.data(), sb.data(), s.data(), &ns);
+    float prm[SF_NPARAM]; default_params(prm);
+    prm[SF_P_ALLOW_SLEEP] = 0.0f; prm[SF_P_SUBSTEPS] = (float)substeps;
+    g_substeps = substeps;
+    replace_default(b, sb, s, prm);
+}
+void sf_stack_setup(uint32_t n, uint32_t substeps, uint32_t sleep) {
+    int ns = 0;
+    int nb = sf_scene_stack((int)n, nullptr, nullptr, nullptr, &ns);
+    std::vector<float> b((size_t)nb * SF_BODY_F), s((size_t)ns * SF_SHAPE_F);
+    std::vector<int32_t> sb(ns);
+    sf_scene_stack((int)n, b.data(), sb.data(), s.data(), &ns);
+    float prm[SF_NPARAM]; default_params(prm);
+    prm[SF_P_ALLOW_SLEEP] = sleep ? 1.0f : 0.0f; prm[SF_P_SUBSTEPS] = (float)substeps;
+    replace_default(b, sb, s, prm);
+}
+uint32_t sf_step(float dt) { if (!g_world) return 0; int r = sf_world_step(g_world, dt); return r < 0 ? 0u : (uint32_t)r; }
+uint32_t sf_body_count(void) { return g_world ? (uint32_t)g_world->h.D.nb : 0u; }
+uint32_t sf_manifolds(void) { return g_world ? (uint32_t)g_world->h.stats[SF_S_MANIFOLDS] : 0u; }
+uint32_t sf_points(void) { return g_world ? (uint32_t)g_world->h.stats[SF_S_POINTS] : 0u; }
+float sf_max_speed(void) {
+    if (!g_world) return 0.0f;
+    int nb = g_world->h.D.nb;
+    std::vector<float> b((size_t)nb * SF_BODY_F);
+    if (world_read_bodies(&g_world->h, b.data()) != 0) return 0.0f;
+    float m = 0.0f;
+    for (int i = 0; i < nb; i++) { const float* f = b.data() + (size_t)i * SF_BODY_F; float v = f[7] * f[7] + f[8] * f[8] + f[9] * f[9]; v = v <= 0.0f ? 0.0f : sqrtf(v); if (v > m) m = v; }
+    return m;
+}
+float sf_body_pos(uint32_t i, uint32_t axis) {
+    const float* p = g_world ? sf_world_pose(g_world) : nullptr;
+    if (!p || i >= (uint32_t)g_world->h.D.nb || axis > 2) return 0.0f;
+    return p[(size_t)i * 10 + axis];
+}
+const float* sf_pose(void) { return g_world ? sf_world_pose(g_world) : nullptr; }
+
+}  // extern "C"

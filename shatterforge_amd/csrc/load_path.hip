@@ -1,0 +1,468 @@
+// Load path on the GPU: the per-structure scalar-potential solve of the reference's legacy engine
+// (src/lib.rs sor_brick :1005-1026, sor_struct :1029-1040, warm_struct/warm_step :1061-1079) and the joint
+// verdicts of stress_brick (:1153-1227), for a batch of independent structures in one launch.
+//
+// Each structure is the linear system sum_j k_ij (phi_i - phi_j) + g_i phi_i = MASS_i + ext_i over its grounded
+// bricks (k = min stiffness x 1 vertical / 0.3 lateral, g = 4 x stiffness on footings). Bricks are reordered red
+// (x+y+z even) then black: lattice neighbours always differ in colour, so a half-sweep updates one colour from the
+// other and is order-independent. Precision: the FP64 field phi is corrected by FP32 red-black SOR sweeps on
+// A delta = r, where r = b - A phi is evaluated in FP64 (iterative refinement); convergence is judged on the FP64
+// residual max|r| / max|b|. One workgroup owns one structure for the whole solve, so the only barriers are
+// workgroup barriers, and every structure of the batch runs in the same launch.
+#include "sf_host.h"
+#include "../../include/sf.h"
+#include <hip/hip_runtime.h>
+#include <vector>
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <unordered_map>
+#include <chrono>
+
+namespace {
+
+const float MASS[9] = {0.0f, 300.0f, 230.0f, 250.0f, 60.0f, 70.0f, 2000.0f, 8.0f, 44.0f};
+const float CAPV[9] = {0.0f, 140000.0f, 50000.0f, 150000.0f, 1500.0f, 9000.0f, 1e12f, 1e12f, 1e12f};
+const float CAPH[9] = {0.0f, 12000.0f, 12000.0f, 90000.0f, 400.0f, 5000.0f, 1e12f, 1e12f, 1e12f};
+const float STIFF[9] = {0.0f, 1.0f, 0.8f, 2.0f, 0.02f, 0.4f, 1.0f, 1.0f, 1.0f};
+const float GROUNDK = 4.0f;
+const int DIRS[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+constexpr int LT = 1024;  // threads per structure workgroup
+
+// one structure as stored on the host (input order) plus its device placement
+struct Struct {
+    int n = 0;
+    std::vector<uint8_t> mat, infill;
+    std::vector<int32_t> xyz, nb6;
+    std::vector<float> bond, ext;
+    std::vector<uint8_t> grounded;
+    int ngrounded = 0;
+    // device placement: bricks base..base+n in device order (red, then black); perm[device k] = input index
+    int base = 0, nred = 0;
+    std::vector<int32_t> perm, inv;
+    float omega = 1.85f;
+    // results
+    int sweeps = 0;
+    double resid = 1.0;
+};
+
+struct DevInfo { int base, n, nred, max_sweeps; float omega; double bmax; };
+
+struct LoadDev {
+    int nstruct;
+    const DevInfo* info;
+    const int32_t* nbr;    // 6 x N: device index of the linked neighbour or -1
+    const float* kc;       // 6 x N: conductance of that bond (f32, as the reference computes it)
+    const float* sk32;     // diagonal in f32 (sum of the bonds' conductances in order, plus the footing)
+    const double* sk64;    // the same diagonal summed in f64
+    const float* b;        // MASS + ext (0 for bricks not grounded)
+    const uint8_t* act;    // 1 = grounded (solved)
+    double* phi;           // FP64 field
+    float* delta;          // FP32 correction
+    float* r32;            // FP32 copy of the FP64 residual
+    int* out_sweeps;
+    double* out_resid;
+    size_t N;
+    double tol;
+    int inner;             // FP32 sweeps between FP64 residual evaluations
+};
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) { double t = __shfl_xor(v, o, 64); v = t > v ? t : v; }
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double m = 0.0;
+    for (int k = 0; k < LT / 64; k++) m = sh[k] > m ? sh[k] : m;
+    __syncthreads();
+    return m;
+}
+
+// the whole solve of one structure per workgroup
+__global__ void __launch_bounds__(LT) k_load_solve(LoadDev L) {
+    __shared__ double sh[LT / 64];
+    const int s = blockIdx.x;
+    if (s >= L.nstruct) return;
+    const DevInfo I = L.info[s];
+    const int base = I.base, n = I.n, nred = I.nred;
+    const size_t N = L.N;
+    const float om = I.omega;
+    int sweeps = 0;
+    double rel = 1.0;
+    while (true) {
+        // FP64 residual r = b - A phi, kept in f32 for the correction sweeps
+        double rmax = 0.0;
+        for (int i = threadIdx.x; i < n; i += LT) {
+            size_t g = (size_t)base + i;
+            double r = 0.0;
+            if (L.act[g]) {
+                double acc = L.sk64[g] * L.phi[g];
+                for (int d = 0; d < 6; d++) {
+                    int j = L.nbr[d * N + g];
+                    if (j >= 0) acc -= (double)L.kc[d * N + g] * L.phi[j];
+                }
+                r = (double)L.b[g] - acc;
+            }
+            L.r32[g] = (float)r;
+            L.delta[g] = 0.0f;
+            double a = fabs(r);
+            rmax = a > rmax ? a : rmax;
+        }
+        rmax = block_max(rmax, sh);
+        rel = I.bmax > 0.0 ? rmax / I.bmax : 0.0;
+        if (rel <= L.tol || sweeps >= I.max_sweeps) break;
+        // FP32 red-black SOR on A delta = r
+        for (int it = 0; it < L.inner && sweeps < I.max_sweeps; it++, sweeps++) {
+            for (int half = 0; half < 2; half++) {
+                int i0 = half ? nred : 0, i1 = half ? n : nred;
+                for (int i = i0 + threadIdx.x; i < i1; i += LT) {
+                    size_t g = (size_t)base + i;
+                    if (!L.act[g]) continue;
+                    float skp = 0.0f;
+                    for (int d = 0; d < 6; d++) {
+                        int j = L.nbr[d * N + g];
+                        if (j >= 0) skp += L.kc[d * N + g] * L.delta[j];
+                    }
+                    float target = (L.r32[g] + skp) / L.sk32[g];
+                    L.delta[g] += om * (target - L.delta[g]);
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = threadIdx.x; i < n; i += LT) {
+            size_t g = (size_t)base + i;
+            L.phi[g] += (double)L.delta[g];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { L.out_sweeps[s] = sweeps; L.out_resid[s] = rel; }
+}
+
+struct Verd { double* ratio; uint8_t* mode; uint8_t* fail; };
+
+// stress_brick (src/lib.rs:1153-1227) for standing bricks, from the FP64 field: +x, +y, +z bonds and the footing
+__global__ void k_verdicts(const int32_t* nbr, const float* kc, const float* hp, const uint8_t* mat, const uint8_t* footing,
+                           const double* phi, const float* gk, size_t N, Verd V) {
+    const float TENSd[9] = {0.0f, 0.35f, 0.1f, 0.9f, 0.5f, 0.7f, 1.0f, 1.0f, 1.0f};
+    for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < N; g += (size_t)gridDim.x * blockDim.x) {
+        for (int a = 0; a < 3; a++) {
+            int j = nbr[(2 * a) * N + g];
+            double r = 0.0; uint8_t mode = 0, fail = 0;
+            if (j >= 0) {
+                double f = (double)kc[(2 * a) * N + g] * (phi[g] - phi[j]);
+                double u = a == 1 ? 1.0 : 0.0;
+                double up = f > 0.0 ? (u > 0.0 ? u : 0.0) : (-u > 0.0 ? -u : 0.0);
+                float ti = TENSd[mat[g]], tj = TENSd[mat[j]];
+                double tens = (double)(ti < tj ? ti : tj);
+                double capm = 1.0 - (1.0 - tens) * up;
+                r = fabs(f) / ((double)hp[4 * g + a] * capm);
+                mode = a == 1 ? (f > 0.0 ? SF_MODE_TENSION : SF_MODE_COMPRESSION) : SF_MODE_SHEAR;
+                fail = r > 1.0;
+            }
+            V.ratio[4 * g + a] = r; V.mode[4 * g + a] = mode; V.fail[4 * g + a] = fail;
+        }
+        double r = 0.0; uint8_t mode = 0, fail = 0;
+        if (footing[g]) {
+            r = (double)gk[g] * phi[g] / (double)hp[4 * g + 3];
+            mode = SF_MODE_GROUND; fail = r > 1.0;
+        }
+        V.ratio[4 * g + 3] = r; V.mode[4 * g + 3] = mode; V.fail[4 * g + 3] = fail;
+    }
+}
+
+float bond_cap(int mi, int mj, int axis) { float a = axis == 1 ? CAPV[mi] : CAPH[mi], b = axis == 1 ? CAPV[mj] : CAPH[mj]; return a < b ? a : b; }
+float cond(int mi, int mj, int axis) { float a = STIFF[mi], b = STIFF[mj]; return (a < b ? a : b) * (axis == 1 ? 1.0f : 0.3f); }
+
+// finalize_brick (src/lib.rs:668-684) for a standing structure: neighbour table and bond capacities
+void build_topology(Struct& S) {
+    const int n = S.n;
+    std::unordered_map<int64_t, int32_t> at;
+    at.reserve((size_t)n * 2);
+    auto pack = [](int x, int y, int z) { return (((int64_t)x + 4096) << 32) | (((int64_t)y + 4096) << 16) | ((int64_t)z + 4096); };
+    for (int i = 0; i < n; i++) at[pack(S.xyz[3 * i], S.xyz[3 * i + 1], S.xyz[3 * i + 2])] = i;
+    S.nb6.assign(6 * (size_t)n, -1);
+    S.bond.assign(4 * (size_t)n, 0.0f);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 6; d++) {
+            auto it = at.find(pack(S.xyz[3 * i] + DIRS[d][0], S.xyz[3 * i + 1] + DIRS[d][1], S.xyz[3 * i + 2] + DIRS[d][2]));
+            if (it != at.end()) S.nb6[6 * i + d] = it->second;
+        }
+    for (int i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) {
+            int j = S.nb6[6 * i + 2 * a];
+            bool gap = j >= 0 && S.infill[i] != S.infill[j] && (a != 1 || S.infill[i] == 1);
+            S.bond[4 * i + a] = (j >= 0 && !gap) ? bond_cap(S.mat[i], S.mat[j], a) : 0.0f;
+        }
+        S.bond[4 * i + 3] = S.xyz[3 * i + 1] == 0 ? CAPV[S.mat[i]] * 2.0f : 0.0f;
+    }
+}
+// linked (src/lib.rs:378-383): the neighbour in direction d joined by an intact bond
+inline int linked(const Struct& S, int i, int d) {
+    int j = S.nb6[6 * i + d];
+    if (j < 0) return -1;
+    float hp = (d & 1) == 0 ? S.bond[4 * i + (d >> 1)] : S.bond[4 * j + (d >> 1)];
+    return hp > 0.0f ? j : -1;
+}
+inline bool footing(const Struct& S, int i) { return S.xyz[3 * i + 1] == 0 && S.bond[4 * i + 3] > 0.0f; }
+// conn_struct's grounded search (src/lib.rs:1085-1098)
+void grounded(Struct& S) {
+    S.grounded.assign(S.n, 0);
+    std::vector<int> q;
+    q.reserve(S.n);
+    for (int i = 0; i < S.n; i++) if (footing(S, i)) { S.grounded[i] = 1; q.push_back(i); }
+    for (size_t h = 0; h < q.size(); h++) {
+        int i = q[h];
+        for (int d = 0; d < 6; d++) { int j = linked(S, i, d); if (j >= 0 && !S.grounded[j]) { S.grounded[j] = 1; q.push_back(j); } }
+    }
+    S.ngrounded = (int)q.size();
+}
+
+struct HostLoad {
+    int dev = 0;
+    std::vector<Struct> st;
+    bool dirty = true;
+    size_t N = 0;
+    // device arrays
+    int32_t* nbr = nullptr; float* kc = nullptr; float* sk32 = nullptr; double* sk64 = nullptr; float* b = nullptr;
+    uint8_t *act = nullptr, *mat = nullptr, *foot = nullptr; float* hp = nullptr; float* gk = nullptr;
+    double* phi = nullptr; float* delta = nullptr; float* r32 = nullptr;
+    DevInfo* info = nullptr; int* osw = nullptr; double* ores = nullptr;
+    double* vratio = nullptr; uint8_t *vmode = nullptr, *vfail = nullptr;
+    bool verdicts_fresh = false;
+    float last_ms = 0.0f;
+    int64_t last_sweeps32 = 0, last_sweeps64 = 0;
+    std::vector<void*> allocs;
+    ~HostLoad() { free_dev(); }
+    void free_dev() { for (void* p : allocs) (void)hipFree(p); allocs.clear(); }
+    template <class T> int alloc(T*& p, size_t n) {
+        void* q = nullptr;
+        SF_HIP(hipMalloc(&q, sizeof(T) * (n ? n : 1)));
+        allocs.push_back(q);
+        p = (T*)q;
+        return 0;
+    }
+};
+
+// lay every structure out on the device: red bricks then black, neighbour indices remapped
+int upload(HostLoad* L) {
+    L->free_dev();
+    size_t N = 0;
+    for (auto& S : L->st) { S.base = (int)N; N += (size_t)S.n; }
+    L->N = N;
+    std::vector<int32_t> nbr(6 * N, -1);
+    std::vector<float> kc(6 * N, 0.0f), sk32(N, 0.0f), b(N, 0.0f), hp(4 * N, 0.0f), gk(N, 0.0f);
+    std::vector<double> sk64(N, 0.0);
+    std::vector<uint8_t> act(N, 0), mat(N, 0), foot(N, 0);
+    std::vector<DevInfo> info(L->st.size());
+    for (size_t s = 0; s < L->st.size(); s++) {
+        Struct& S = L->st[s];
+        S.perm.clear(); S.inv.assign(S.n, 0);
+        for (int i = 0; i < S.n; i++) if (((S.xyz[3 * i] + S.xyz[3 * i + 1] + S.xyz[3 * i + 2]) & 1) == 0) S.perm.push_back(i);
+        S.nred = (int)S.perm.size();
+        for (int i = 0; i < S.n; i++) if (((S.xyz[3 * i] + S.xyz[3 * i + 1] + S.xyz[3 * i + 2]) & 1) != 0) S.perm.push_back(i);
+        for (int k = 0; k < S.n; k++) S.inv[S.perm[k]] = k;
+        double bmax = 0.0;
+        int xmax = 0;
+        for (int k = 0; k < S.n; k++) {
+            int i = S.perm[k];
+            size_t g = (size_t)S.base + k;
+            int mi = S.mat[i];
+            mat[g] = (uint8_t)mi;
+            act[g] = S.grounded[i];
+            for (int a = 0; a < 4; a++) hp[4 * g + a] = S.bond[4 * i + a];
+            foot[g] = footing(S, i);
+            gk[g] = GROUNDK * STIFF[mi];
+            float s32 = 0.0f; double s64 = 0.0;
+            for (int d = 0; d < 6; d++) {
+                int j = linked(S, i, d);
+                if (j < 0) continue;
+                float k = cond(mi, S.mat[j], d >> 1);
+                nbr[d * N + g] = S.base + S.inv[j];
+                kc[d * N + g] = k;
+                s32 += k; s64 += (double)k;
+            }
+            if (foot[g]) { s32 += GROUNDK * STIFF[mi]; s64 += (double)(GROUNDK * STIFF[mi]); }
+            sk32[g] = s32; sk64[g] = s64;
+            if (act[g]) {
+                b[g] = MASS[mi] + (S.ext.empty() ? 0.0f : S.ext[i]);
+                bmax = std::max(bmax, (double)std::fabs(b[g]));
+            }
+            for (int a = 0; a < 3; a++) xmax = std::max(xmax, S.xyz[3 * i + a] + 1);
+        }
+        info[s].base = S.base; info[s].n = S.n; info[s].nred = S.nred; info[s].bmax = bmax;
+        info[s].omega = S.omega;
+        info[s].max_sweeps = 0;
+    }
+    int r = 0;
+    r |= L->alloc(L->nbr, 6 * N); r |= L->alloc(L->kc, 6 * N); r |= L->alloc(L->sk32, N); r |= L->alloc(L->sk64, N);
+    r |= L->alloc(L->b, N); r |= L->alloc(L->act, N); r |= L->alloc(L->mat, N); r |= L->alloc(L->foot, N);
+    r |= L->alloc(L->hp, 4 * N); r |= L->alloc(L->gk, N); r |= L->alloc(L->phi, N); r |= L->alloc(L->delta, N);
+    r |= L->alloc(L->r32, N); r |= L->alloc(L->info, info.size()); r |= L->alloc(L->osw, info.size());
+    r |= L->alloc(L->ores, info.size()); r |= L->alloc(L->vratio, 4 * N); r |= L->alloc(L->vmode, 4 * N); r |= L->alloc(L->vfail, 4 * N);
+    if (r) return r;
+    SF_HIP(hipMemcpy(L->nbr, nbr.data(), sizeof(int32_t) * 6 * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->kc, kc.data(), sizeof(float) * 6 * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->sk32, sk32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->sk64, sk64.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->b, b.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->act, act.data(), N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->mat, mat.data(), N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->foot, foot.data(), N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->hp, hp.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->gk, gk.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemset(L->phi, 0, sizeof(double) * N));
+    SF_HIP(hipMemcpy(L->info, info.data(), sizeof(DevInfo) * info.size(), hipMemcpyHostToDevice));
+    L->dirty = false;
+    return 0;
+}
+
+}  // namespace
+
+struct sf_load { HostLoad h; };
+
+extern "C" {
+
+// (structures can be added and their topology read without a GPU; solving needs one)
+sf_load* sf_load_create(void) {
+    sf_load* L = new sf_load();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) (void)hipGetDevice(&L->h.dev);
+    return L;
+}
+void sf_load_destroy(sf_load* L) { delete L; }
+int sf_load_count(sf_load* L) { return L ? (int)L->h.st.size() : SF_E_ARG; }
+
+int sf_load_add(sf_load* L, int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, const float* ext) {
+    if (!L || n <= 0 || !mat || !xyz || !infill) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    Struct S;
+    S.n = n;
+    S.mat.assign(mat, mat + n); S.infill.assign(infill, infill + n); S.xyz.assign(xyz, xyz + 3 * (size_t)n);
+    for (int i = 0; i < n; i++) if (S.mat[i] < 1 || S.mat[i] > 5) { sf::set_error(SF_E_ARG, "material must be 1..5"); return SF_E_ARG; }
+    if (ext) {
+        S.ext.assign(ext, ext + n);
+        for (float e : S.ext) if (!std::isfinite(e)) { sf::set_error(SF_E_ARG, "non-finite load"); return SF_E_ARG; }
+    }
+    build_topology(S);
+    grounded(S);
+    // over-relaxation for a lattice of this height and span: 2 / (1 + sin(pi / L)), held to [1.6, 1.95]
+    int ext_max = 1;
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (int i = 0; i < n; i++) for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], S.xyz[3 * i + a]); hi[a] = std::max(hi[a], S.xyz[3 * i + a]); }
+    for (int a = 0; a < 3; a++) ext_max = std::max(ext_max, hi[a] - lo[a] + 1);
+    double om = 2.0 / (1.0 + std::sin(3.14159265358979 / (double)ext_max));
+    S.omega = (float)std::min(1.95, std::max(1.6, om));
+    L->h.st.push_back(std::move(S));
+    L->h.dirty = true;
+    return (int)L->h.st.size() - 1;
+}
+
+int sf_load_solve(sf_load* L, float omega, double rel_tol, int max_sweeps) {
+    if (!L || !(rel_tol > 0.0) || max_sweeps <= 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    HostLoad* H = &L->h;
+    if (H->st.empty()) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { sf::set_error(SF_E_NODEV, "no GPU visible"); return SF_E_NODEV; }
+    SF_HIP(hipSetDevice(H->dev));
+    if (omega > 0.0f) for (auto& S : H->st) S.omega = omega;
+    if (H->dirty) { int r = upload(H); if (r) return r; }
+    std::vector<DevInfo> info(H->st.size());
+    for (size_t s = 0; s < H->st.size(); s++) {
+        const Struct& S = H->st[s];
+        double bmax = 0.0;
+        for (int i = 0; i < S.n; i++) if (S.grounded[i]) bmax = std::max(bmax, (double)std::fabs(MASS[S.mat[i]] + (S.ext.empty() ? 0.0f : S.ext[i])));
+        info[s] = DevInfo{S.base, S.n, S.nred, max_sweeps, S.omega, bmax};
+    }
+    SF_HIP(hipMemcpy(H->info, info.data(), sizeof(DevInfo) * info.size(), hipMemcpyHostToDevice));
+    SF_HIP(hipMemset(H->phi, 0, sizeof(double) * H->N));
+    LoadDev D;
+    D.nstruct = (int)H->st.size(); D.info = H->info; D.nbr = H->nbr; D.kc = H->kc; D.sk32 = H->sk32; D.sk64 = H->sk64;
+    D.b = H->b; D.act = H->act; D.phi = H->phi; D.delta = H->delta; D.r32 = H->r32; D.out_sweeps = H->osw;
+    D.out_resid = H->ores; D.N = H->N; D.tol = rel_tol; D.inner = 32;
+    hipEvent_t e0, e1;
+    SF_HIP(hipEventCreate(&e0)); SF_HIP(hipEventCreate(&e1));
+    SF_HIP(hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(k_load_solve, dim3(D.nstruct), dim3(LT), 0, 0, D);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipEventRecord(e1, 0));
+    SF_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    H->last_ms = ms;
+    std::vector<int> sw(H->st.size());
+    std::vector<double> rs(H->st.size());
+    SF_HIP(hipMemcpy(sw.data(), H->osw, sizeof(int) * sw.size(), hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(rs.data(), H->ores, sizeof(double) * rs.size(), hipMemcpyDeviceToHost));
+    int ok = 0;
+    H->last_sweeps32 = 0; H->last_sweeps64 = 0;
+    for (size_t s = 0; s < H->st.size(); s++) {
+        H->st[s].sweeps = sw[s]; H->st[s].resid = rs[s];
+        if (rs[s] <= rel_tol) ok++;
+        H->last_sweeps32 += (int64_t)sw[s] * H->st[s].n;
+        H->last_sweeps64 += (int64_t)((sw[s] + D.inner - 1) / D.inner + 1) * H->st[s].n;
+    }
+    // verdicts from the converged field
+    Verd V{H->vratio, H->vmode, H->vfail};
+    hipLaunchKernelGGL(k_verdicts, dim3(sf_grid(H->N)), dim3(256), 0, 0, H->nbr, H->kc, H->hp, H->mat, H->foot, H->phi, H->gk, H->N, V);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipDeviceSynchronize());
+    H->verdicts_fresh = true;
+    return ok;
+}
+
+int sf_load_info(sf_load* L, int sid, int32_t* sweeps, double* resid, int32_t* grounded_out) {
+    if (!L || sid < 0 || sid >= (int)L->h.st.size()) { sf::set_error(SF_E_ARG, "bad structure id"); return SF_E_ARG; }
+    const Struct& S = L->h.st[sid];
+    if (sweeps) *sweeps = S.sweeps;
+    if (resid) *resid = S.resid;
+    if (grounded_out) *grounded_out = S.ngrounded;
+    return 0;
+}
+
+int sf_load_field(sf_load* L, int sid, double* phi) {
+    if (!L || !phi || sid < 0 || sid >= (int)L->h.st.size() || L->h.dirty) { sf::set_error(SF_E_ARG, "bad structure id or not solved"); return SF_E_ARG; }
+    const Struct& S = L->h.st[sid];
+    std::vector<double> d(S.n);
+    SF_HIP(hipMemcpy(d.data(), L->h.phi + S.base, sizeof(double) * S.n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < S.n; k++) phi[S.perm[k]] = d[k];
+    return 0;
+}
+
+int sf_load_verdicts(sf_load* L, int sid, double* ratio4, uint8_t* mode4, uint8_t* fail4) {
+    if (!L || sid < 0 || sid >= (int)L->h.st.size() || !L->h.verdicts_fresh) { sf::set_error(SF_E_ARG, "bad structure id or not solved"); return SF_E_ARG; }
+    const Struct& S = L->h.st[sid];
+    std::vector<double> r(4 * (size_t)S.n);
+    std::vector<uint8_t> m(4 * (size_t)S.n), f(4 * (size_t)S.n);
+    SF_HIP(hipMemcpy(r.data(), L->h.vratio + 4 * (size_t)S.base, sizeof(double) * 4 * S.n, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(m.data(), L->h.vmode + 4 * (size_t)S.base, 4 * (size_t)S.n, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(f.data(), L->h.vfail + 4 * (size_t)S.base, 4 * (size_t)S.n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < S.n; k++) {
+        int i = S.perm[k];
+        for (int a = 0; a < 4; a++) {
+            if (ratio4) ratio4[4 * i + a] = r[4 * k + a];
+            if (mode4) mode4[4 * i + a] = m[4 * k + a];
+            if (fail4) fail4[4 * i + a] = f[4 * k + a];
+        }
+    }
+    return 0;
+}
+
+int sf_load_topology(sf_load* L, int sid, int32_t* nb6, float* bond4) {
+    if (!L || sid < 0 || sid >= (int)L->h.st.size()) { sf::set_error(SF_E_ARG, "bad structure id"); return SF_E_ARG; }
+    const Struct& S = L->h.st[sid];
+    if (nb6) std::memcpy(nb6, S.nb6.data(), sizeof(int32_t) * S.nb6.size());
+    if (bond4) std::memcpy(bond4, S.bond.data(), sizeof(float) * S.bond.size());
+    return 0;
+}
+
+int sf_load_last_timing(sf_load* L, float* ms, int64_t* sweeps32, int64_t* sweeps64, int64_t* bricks) {
+    if (!L) return SF_E_ARG;
+    if (ms) *ms = L->h.last_ms;
+    if (sweeps32) *sweeps32 = L->h.last_sweeps32;
+    if (sweeps64) *sweeps64 = L->h.last_sweeps64;
+    if (bricks) *bricks = (int64_t)L->h.N;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+"""Seeded scenes as numpy arrays (thin wrappers over the library's host-side generators).
+
+A box scene is ``(bodies, shape_body, shapes)``: ``bodies`` float32 [nb, 20] (p, q, v, w, inv_m, inv_i, awake,
+sleep_t, reserved), ``shape_body`` int32 [ns], ``shapes`` float32 [ns, 11] (local p, local q, half extents, mu).
+A structure is ``(mat, xyz, infill)`` on the 0.5 m lattice.
+"""
+import numpy as np
+
+from ._native import lib, check, ptr
+
+BODY_F = 20
+SHAPE_F = 11
+
+
+def _box_scene(fn, *args):
+    L = lib()
+    ns = np.zeros(1, np.int32)
+    nb = check(fn(*args, None, None, None, ptr(ns)))
+    bodies = np.zeros((nb, BODY_F), np.float32)
+    shape_body = np.zeros(int(ns[0]), np.int32)
+    shapes = np.zeros((int(ns[0]), SHAPE_F), np.float32)
+    check(fn(*args, ptr(bodies), ptr(shape_body), ptr(shapes), ptr(ns)))
+    return bodies, shape_body, shapes
+
+
+def pile(n):
+    """The reference's throughput scene (engine/src/phys.rs:632-646): n brick-sized boxes in a loose grid."""
+    return _box_scene(lib().sf_scene_pile, int(n))
+
+
+def stack(n):
+    """n 0.5 m cubes stacked exactly (engine/src/wasm.rs:33-38)."""
+    return _box_scene(lib().sf_scene_stack, int(n))
+
+
+def rubble(nbody=50000, p=0.3, maxsize=64, box=(40.0, 20.0, 40.0), vsig=3.0, wsig=2.0, seed=1):
+    """Rubble pile of polycubes (sizes ~ Geometric(p) truncated at maxsize), materials uniform over five,
+    packed into `box` (m) on the ground, velocities N(0, vsig) per axis, angular N(0, wsig)."""
+    return _box_scene(lib().sf_scene_rubble, int(nbody), float(p), int(maxsize), float(box[0]), float(box[1]),
+                      float(box[2]), float(vsig), float(wsig), int(seed) & 0xFFFFFFFF)
+
+
+def compound(xyz, mat):
+    """One compound body from bricks at lattice cells: returns (body[20], shapes[n, 11])."""
+    xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+    mat = np.ascontiguousarray(mat, np.uint8).reshape(-1)
+    body = np.zeros(BODY_F, np.float32)
+    shapes = np.zeros((len(mat), SHAPE_F), np.float32)
+    check(lib().sf_build_compound(len(mat), ptr(xyz), ptr(mat), ptr(body), ptr(shapes)))
+    return body, shapes
+
+
+def box(p, h, q=(0.0, 0.0, 0.0, 1.0), density=1900.0, mu=0.6, v=(0, 0, 0), w=(0, 0, 0)):
+    """A single box body as World::add_box builds it (engine/src/phys.rs:139-148), in f32 arithmetic."""
+    f = np.float32
+    hx, hy, hz = f(h[0]), f(h[1]), f(h[2])
+    m = f(f(f(f(density) * f(8.0)) * hx) * hy) * hz
+    body = np.zeros(BODY_F, np.float32)
+    body[0:3] = p
+    body[3:7] = q
+    body[7:10] = v
+    body[10:13] = w
+    if m > 0:
+        x2, y2, z2 = f(4.0) * hx * hx, f(4.0) * hy * hy, f(4.0) * hz * hz
+        body[13] = f(1.0) / m
+        body[14:17] = [f(12.0) / (m * (y2 + z2)), f(12.0) / (m * (x2 + z2)), f(12.0) / (m * (x2 + y2))]
+        body[17] = 1.0
+    shape = np.zeros(SHAPE_F, np.float32)
+    shape[6] = 1.0
+    shape[7:10] = h
+    shape[10] = mu
+    return body, shape
+
+
+def boxes(items):
+    """Scene from a list of (body, shape) pairs."""
+    bodies = np.stack([b for b, _ in items]).astype(np.float32)
+    shapes = np.stack([s for _, s in items]).astype(np.float32)
+    return bodies, np.arange(len(items), dtype=np.int32), shapes
+
+
+def building(wx, wz, storeys, bay=6, remove=0, seed=1):
+    """Frame/wall building: returns (mat, xyz, infill)."""
+    L = lib()
+    n = check(L.sf_scene_building(int(wx), int(wz), int(storeys), int(bay), int(remove), int(seed), None, None, None))
+    mat = np.zeros(n, np.uint8)
+    xyz = np.zeros((n, 3), np.int32)
+    infill = np.zeros(n, np.uint8)
+    check(L.sf_scene_building(int(wx), int(wz), int(storeys), int(bay), int(remove), int(seed), ptr(mat), ptr(xyz), ptr(infill)))
+    return mat, xyz, infill
+
+
+def building_for(n_target, seed=1):
+    """A seeded building with about n_target bricks: square footprint and storey count chosen to fit, 1-8 ground
+    columns removed."""
+    rng = np.random.default_rng(seed)
+    best = None
+    for storeys in range(1, 40):
+        for w in range(6, 120):
+            n = lib().sf_scene_building(w, w, storeys, 6, 0, 1, None, None, None)
+            if n >= n_target:
+                err = abs(n - n_target) / n_target
+                aspect = abs(storeys * 7 - w) / max(w, storeys * 7)
+                score = err + 0.2 * aspect
+                if best is None or score < best[0]:
+                    best = (score, w, storeys)
+                break
+    _, w, storeys = best
+    remove = int(rng.integers(1, 9))
+    return building(w, w, storeys, 6, remove, seed)
+
+
+def tower():
+    """The 12 x 12 x 40 brick tower of the plain-method workload."""
+    L = lib()
+    n = 5760
+    mat = np.zeros(n, np.uint8)
+    xyz = np.zeros((n, 3), np.int32)
+    infill = np.zeros(n, np.uint8)
+    check(L.sf_scene_tower(ptr(mat), ptr(xyz), ptr(infill)))
+    return mat, xyz, infill
+
+
+def column(height, mat=1):
+    """A vertical column of `height` bricks of one material."""
+    n = int(height)
+    return (np.full(n, mat, np.uint8), np.stack([np.zeros(n), np.arange(n), np.zeros(n)], 1).astype(np.int32),
+            np.zeros(n, np.uint8))

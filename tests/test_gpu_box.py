@@ -1,0 +1,140 @@
+"""Box step on the GPU against the oracle (oracle/box_world.c in the device's colour order) through the C ABI.
+
+The bar: candidate lists, manifold keys, colours and island labels equal exactly; contact data and body states equal
+bit for bit (the kernels perform the oracle's f32 operations in the same order, with no FMA contraction).
+"""
+import numpy as np
+import pytest
+
+import shatterforge_amd as sf
+from shatterforge_amd import scenes, world
+from shatterforge_amd.world import params
+
+pytestmark = pytest.mark.gpu
+DT = np.float32(1.0 / 60.0)
+
+
+def _assert_same_step(w, o, tag):
+    assert np.array_equal(w.candidates(), o.candidates()), f"{tag}: candidates"
+    gk, gd = w.manifolds()
+    ok, od = o.manifolds()
+    assert np.array_equal(gk, ok), f"{tag}: manifold keys"
+    assert np.array_equal(gd, od), f"{tag}: manifold data max diff {np.abs(gd - od).max() if len(gd) else 0}"
+    assert np.array_equal(w.colours(), o.colours()), f"{tag}: colours"
+    assert np.array_equal(w.islands(), o.islands()), f"{tag}: islands"
+    gb, ob = w.bodies(), o.bodies()
+    assert np.array_equal(gb, ob), f"{tag}: bodies max diff {np.abs(gb - ob).max()}"
+
+
+def test_pile_steps_bit_exact(oracle):
+    print("case: pile(600) substeps 4, 8 steps", flush=True)
+    b, sb, s = scenes.pile(600)
+    prm = params(substeps=4, sleep=False)
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    for k in range(8):
+        w.step(DT)
+        o.step(DT)
+        _assert_same_step(w, o, f"step {k}")
+    st, ost = w.stats(), o.stats()
+    assert st["manifolds"] == ost["manifolds"] and st["points"] == ost["points"] and st["colours"] == ost["colours"]
+
+
+def test_rubble_compound_steps_bit_exact_with_sleep(oracle):
+    print("case: rubble(300) compound bodies, 60 steps, sleep on", flush=True)
+    b, sb, s = scenes.rubble(300, box=(7.0, 5.0, 7.0), seed=5)
+    prm = params(substeps=6)
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    for k in range(60):
+        w.step(DT)
+        o.step(DT)
+        gb, ob = w.bodies(), o.bodies()
+        assert np.array_equal(gb, ob), f"step {k}: max diff {np.abs(gb - ob).max()}"
+    _assert_same_step(w, o, "final")
+
+
+def test_first_frame_candidates_of_the_reference_pile():
+    # docs/GPU.md:89: 5,871 candidates on the first frame of the 2,000-box pile
+    b, sb, s = scenes.pile(2000)
+    w = sf.World.from_scene(b, sb, s, params(substeps=4, sleep=False))
+    w.step(DT)
+    st = w.stats()
+    assert st["pairs"] + st["ground"] == 5871 == len(w.candidates())
+
+
+def test_stack_gate_and_reference_exports():
+    # engine/src/phys.rs:672-683 through the reference's own exports (engine/src/wasm.rs:33-47)
+    world.stack_setup(20, 12, 0)
+    p0 = world.pose()
+    top0 = p0[191]
+    maxv = 0.0
+    for f in range(600):
+        world.step(1.0 / 60.0)
+        if f >= 540:
+            maxv = max(maxv, world.max_speed())
+    p = world.pose()
+    drift = float(np.hypot(np.hypot(p[190], p[191] - top0), p[192]))
+    assert drift < 0.01 and maxv < 0.01, (drift, maxv)
+    assert world.body_count() == 20 and world.manifolds() >= 20
+    assert world.body_pos(19, 1) == p[191]
+
+
+def test_pile_two_seconds_within_reference_tolerance():
+    # gpu/test_gpu.cjs:1049-1061 against the CPU reference heights of docs/GPU.md:158 (y 0.117-4.828 m)
+    world.pile_setup(2000, 4)
+    for _ in range(120):
+        world.step(1.0 / 60.0)
+    y = world.pose().reshape(-1, 10)[:, 1]
+    assert np.isfinite(y).all()
+    assert abs(y.min() - 0.117) < 0.02 and abs(y.max() - 4.828) < 0.1, (y.min(), y.max())
+
+
+def test_determinism_run_twice():
+    b, sb, s = scenes.rubble(2000, box=(12.0, 8.0, 12.0), seed=9)
+    out = []
+    for _ in range(2):
+        w = sf.World.from_scene(b, sb, s, params())
+        for _ in range(20):
+            w.step(DT)
+        out.append((w.bodies(), w.colours(), w.islands()))
+        w.close()
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
+
+
+def test_full_size_rubble_step_matches_oracle_and_invariants(oracle):
+    # the rubble workload at full size (50,000 polycube bodies): one step bit for bit, then invariants
+    print("case: rubble 50k bodies, 1 step vs oracle", flush=True)
+    b, sb, s = scenes.rubble(50000, seed=1)
+    prm = params()
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    w.step(DT)
+    o.step(DT)
+    _assert_same_step(w, o, "rubble 50k step 0")
+    for _ in range(4):
+        pre = w.bodies()
+        w.step(DT)
+        keys, _ = w.manifolds()
+        col = w.colours()
+        K = int(prm[world.P["COL_LIMIT"]])
+        # canonical candidates: sorted, unique
+        c = w.candidates()
+        kc = (c[:, 0].astype(np.int64) << 32) + (c[:, 1].astype(np.int64) + 1)
+        assert (np.diff(kc) > 0).all()
+        # colour validity on moving bodies
+        awake0 = pre[:, 17] > 0.5
+        ba = sb[keys[:, 0]]
+        bb = np.where(keys[:, 1] >= 0, sb[np.maximum(keys[:, 1], 0)], -1)
+        for side in (ba, bb):
+            sel = (col < K) & (side >= 0)
+            sel &= awake0[np.maximum(side, 0)]
+        pairs = np.concatenate([np.stack([ba, col], 1)[(col < K) & awake0[ba]],
+                                np.stack([bb, col], 1)[(col < K) & (bb >= 0) & awake0[np.maximum(bb, 0)]]])
+        assert len(np.unique(pairs, axis=0)) == len(pairs)
+        # island labels are roots: label[label[i]] == label[i] <= i
+        isl = w.islands()
+        assert (isl[isl] == isl).all() and (isl <= np.arange(len(isl))).all()
+        assert np.isfinite(w.bodies()).all()
+    assert w.stats()["dropped"] == 0

@@ -1,0 +1,70 @@
+"""Load path on the GPU against the FP64 oracle field (oracle/load_path.c conjugate gradients) and the oracle's
+verdicts. Tolerance: the device reaches max|b - A phi| / max|b| <= 1e-6 (checked by the oracle's own FP64 residual);
+the field's relative error to the oracle's is stated per structure and must stay under 1e-5; bond modes equal
+exactly and pass/fail verdicts equal exactly for every bond whose ratio is not within 1e-5 of 1."""
+import numpy as np
+import pytest
+
+import shatterforge_amd as sf
+from shatterforge_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(L, sid, mat, xyz, inf, oracle, tag):
+    o = oracle.LoadOracle(mat, xyz, inf)
+    ref, it = o.solve64(rel_tol=1e-13)
+    assert it >= 0
+    phi = L.field(sid)
+    info = L.info(sid)
+    assert info["resid"] <= 1e-6, (tag, info)
+    assert o.residual64(phi) <= 1e-6, tag
+    g = o.grounded().astype(bool)
+    err = float(np.abs(phi[g] - ref[g]).max() / np.abs(ref[g]).max())
+    assert err < 1e-5, (tag, err)
+    r, m, f = L.verdicts(sid)
+    ro, mo, fo = o.verdicts64(ref)
+    assert np.array_equal(m, mo), tag
+    clear = np.abs(ro - 1.0) > 1e-5
+    assert np.array_equal(f[clear], fo[clear]), tag
+    assert np.allclose(r, ro, rtol=1e-5, atol=1e-9), tag
+    return err
+
+
+def test_column_closed_form_on_device():
+    mat, xyz, inf = scenes.column(40, 1)
+    L = sf.LoadBatch()
+    sid = L.add(mat, xyz, inf)
+    assert L.solve(rel_tol=1e-6) == 1
+    phi = L.field(sid)
+    expect = np.zeros(40)
+    expect[0] = 40 * 300.0 / 4.0
+    for i in range(39):
+        expect[i + 1] = expect[i] + 300.0 * (39 - i)
+    assert np.allclose(phi, expect, rtol=1e-6)
+    r, m, f = L.verdicts(sid)
+    assert (m[:39, 1] == 1).all()
+    assert np.allclose(r[:39, 1], 300.0 * (39 - np.arange(39)) / 140000.0, rtol=1e-6)
+
+
+def test_tower_against_fp64_oracle(oracle):
+    print("case: tower 5760 bricks", flush=True)
+    mat, xyz, inf = scenes.tower()
+    L = sf.LoadBatch()
+    sid = L.add(mat, xyz, inf)
+    assert L.solve(rel_tol=1e-6) == 1
+    _check(L, sid, mat, xyz, inf, oracle, "tower")
+
+
+def test_batch_of_buildings_in_one_launch(oracle):
+    print("case: 12 buildings 2k-12k bricks in one launch", flush=True)
+    rng = np.random.default_rng(7)
+    L = sf.LoadBatch()
+    sts = []
+    for k in range(12):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(12000))))
+        st = scenes.building_for(n, seed=100 + k)
+        sts.append((L.add(*st), st))
+    assert L.solve(rel_tol=1e-6) == len(sts)
+    errs = [_check(L, sid, *st, oracle, f"building {sid}") for sid, st in sts]
+    print("per-structure field error vs FP64 oracle:", ["%.1e" % e for e in errs])
