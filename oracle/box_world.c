@@ -825,7 +825,8 @@ int ob_manifolds(const ob_world* w, int32_t* keys, float* data, int cap) {
         f[0] = m->n.x; f[1] = m->n.y; f[2] = m->n.z; f[3] = (float)m->cnt;
         for (int k = 0; k < m->cnt; k++) {
             const Cp* c = &m->cp[k]; float* q = f + 4 + 8 * k;
-            q[0] = c->ra.x; q[1] = c->ra.y; q[2] = c->ra.z; q[3] = c->rb.x; q[4] = c->rb.y; q[5] = c->rb.z; q[6] = c->adj; q[7] = c->jn;
+            q[0] = c->raxn.x; q[1] = c->raxn.y; q[2] = c->raxn.z; q[3] = c->rbxn.x; q[4] = c->rbxn.y; q[5] = c->rbxn.z;
+            q[6] = c->adj; q[7] = c->jn;
         }
         f[36] = m->jt1; f[37] = m->jt2; f[38] = m->jtw; f[39] = (float)m->stick;
     }

@@ -135,7 +135,7 @@ class LoadOracle:
     def sor_sweep(self, phi32, omega):
         return lib().ol_sor_sweep(self._sp(), _p(phi32), ctypes.c_float(omega))
 
-    def solve64(self, rel_tol=1e-13, max_iter=200000):
+    def solve64(self, rel_tol=1e-12, max_iter=200000):
         phi = np.zeros(self.n, np.float64)
         it = lib().ol_solve64(self._sp(), _p(phi), ctypes.c_double(rel_tol), max_iter)
         return phi, it

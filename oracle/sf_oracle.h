@@ -43,7 +43,7 @@ void ob_stats(const ob_world* w, int32_t* out8);
 /* last step's candidate list in canonical order: (a, b) with b = -1 for the ground; returns the count */
 int ob_candidates(const ob_world* w, int32_t* out, int cap);
 /* last step's manifolds: keys (a, b) and point counts; plus per manifold: normal[3], npts, and up to 4 points of
- * (ra[3], rb[3], adj, jn) after the solve, jt1, jt2, jtw */
+ * (ra x n [3], rb x n [3], adj, jn) after the solve, jt1, jt2, jtw, stick */
 int ob_manifolds(const ob_world* w, int32_t* keys, float* data, int cap);
 #define OB_MAN_F 40
 /* colour per manifold (K = overflow) for the last step (coloured order only) */

@@ -128,7 +128,7 @@ class World:
     def stats(self):
         out = np.zeros(NSTAT, np.int32)
         check(lib().sf_world_stats(self._h, ptr(out)))
-        return {k: int(out[v]) for k, v in S.items()}
+        return {k.lower(): int(out[v]) for k, v in S.items()}
 
     def candidates(self):
         n = check(lib().sf_world_candidates(self._h, None, 0))

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 def _check(L, sid, mat, xyz, inf, oracle, tag):
     o = oracle.LoadOracle(mat, xyz, inf)
-    ref, it = o.solve64(rel_tol=1e-13)
+    ref, it = o.solve64(rel_tol=1e-12)
     assert it >= 0
     phi = L.field(sid)
     info = L.info(sid)
@@ -27,7 +27,9 @@ def _check(L, sid, mat, xyz, inf, oracle, tag):
     assert np.array_equal(m, mo), tag
     clear = np.abs(ro - 1.0) > 1e-5
     assert np.array_equal(f[clear], fo[clear]), tag
-    assert np.allclose(r, ro, rtol=1e-5, atol=1e-9), tag
+    # ratios: within 1e-6 of the FP64 oracle's (lateral bonds carry differences of nearly equal potentials, so a
+    # relative bound on tiny ratios means nothing; 1e-6 absolute is far below any verdict margin)
+    assert np.abs(r - ro).max() < 1e-6, (tag, np.abs(r - ro).max())
     return err
 
 
