@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bodies", type=int, default=50000)
     ap.add_argument("--substeps", type=int, default=12)
+    ap.add_argument("--colours", type=int, default=None, help="colour cap (default: the library's)")
+    ap.add_argument("--tail-min", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -63,6 +65,7 @@ def main():
 
     import shatterforge_amd as sf
     from shatterforge_amd import scenes
+    from shatterforge_amd import world
     from shatterforge_amd.world import params
 
     if sf.lib().sf_device_count() < 1:
@@ -78,6 +81,10 @@ def main():
     nbricks = len(shapes)
     log(f"scene: {len(bodies)} bodies, {nbricks} bricks ({time.perf_counter() - t:.1f} s)")
     prm = params(substeps=args.substeps, sleep=True)
+    if args.colours is not None:
+        prm[world.P["COL_LIMIT"]] = args.colours
+    if args.tail_min is not None:
+        prm[world.P["TAIL_MIN"]] = args.tail_min
     w = sf.World.from_scene(bodies, shape_body, shapes, prm)
     for k in range(args.warmup):
         w.step(1.0 / 60.0)
@@ -119,7 +126,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded rubble pile)",
         "config": {"workload": "rubble-pile stress, plain step on one scene", "bodies": len(bodies), "bricks": nbricks,
-                   "substeps": args.substeps, "dt": "1/60", "mode": "plain"},
+                   "substeps": args.substeps, "dt": "1/60", "mode": "plain",
+                   "colour_limit": int(prm[world.P["COL_LIMIT"]]), "tail_min": int(prm[world.P["TAIL_MIN"]])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_solve_bucket", "launches": klaunch, "avg_us": ksum_ms / max(klaunch, 1) * 1e3,

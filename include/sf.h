@@ -41,7 +41,7 @@ enum {
     SF_P_COL_LIMIT,                 /* colours (<= 64); contacts beyond go to the overflow, solved in order */
     SF_P_COL_ROUNDS,                /* colouring rounds (32) */
     SF_P_COL_SPREAD,                /* a contact picks among the lowest this-many free colours (4) */
-    SF_P_RESERVED,
+    SF_P_TAIL_MIN,                  /* colour buckets smaller than this are solved in one workgroup (1024) */
     SF_NPARAM
 };
 /* body record, SF_BODY_F floats: p[3] q[4] v[3] w[3] inv_m inv_i[3] (principal, local) awake sleep_t reserved */

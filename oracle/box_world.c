@@ -560,29 +560,43 @@ typedef struct { int use_bias; float inv_h, bias_rate, mass_scale, imp_scale, ma
 
 static inline float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
 
-/* phys.rs:268-323, one manifold */
-static void solve_one(ob_world* w, Man* m, const SolveK* K) {
-    int sa = m->sa, sb = m->sb;
-    V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
-    V3 tha = w->sth[sa], thb = w->sth[sb];
+/* phys.rs:268-323 on velocity copies of the two bodies. fa, fb = 0: the manifold as the reference solves it.
+ * fa, fb > 0: mass splitting for the overflow pass: each side's inverse mass and inertia are scaled by the number of
+ * overflow manifolds at that body (each manifold acts on its own copy of the body), and the effective masses are
+ * recomputed from the scaled ones. */
+static void solve_core(Man* m, const SolveK* K, V3* pva, V3* pwa, V3* pvb, V3* pwb, V3 tha, V3 thb, V3 dpab, float fa, float fb) {
+    V3 va = *pva, wa = *pwa, vb = *pvb, wb = *pwb;
+    const int split = fa > 0.0f;
     V3 n = m->n, t1 = m->t1, t2 = m->t2;
-    V3 dpab = vsub(w->sdp[sb], w->sdp[sa]);
     float dn = vdot(dpab, n);
-    float ima = m->ima, imb = m->imb;
+    float ima = split ? m->ima * fa : m->ima, imb = split ? m->imb * fb : m->imb;
     float tot = 0.0f;
     for (int k = 0; k < m->cnt; k++) {
         Cp* c = &m->cp[k];
+        V3 iaxn = split ? vmul(c->iaxn, fa) : c->iaxn, ibxn = split ? vmul(c->ibxn, fb) : c->ibxn;
+        float nm = c->nm;
+        if (split) { float kk = ima + imb + vdot(c->raxn, iaxn) + vdot(c->rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
         float s = c->adj + dn + vdot(thb, c->rbxn) - vdot(tha, c->raxn);
         float bias, ms, is;
         if (s > 0.0f) { bias = s * K->inv_h; ms = 1.0f; is = 0.0f; }
         else if (K->use_bias) { bias = om_max(K->bias_rate * s, -K->max_push); ms = K->mass_scale; is = K->imp_scale; }
         else { bias = 0.0f; ms = 1.0f; is = 0.0f; }
         float vn = vdot(vsub(vb, va), n) + vdot(wb, c->rbxn) - vdot(wa, c->raxn);
-        float imp = -c->nm * ms * (vn + bias) - is * c->jn;
+        float imp = -nm * ms * (vn + bias) - is * c->jn;
         float nj = om_max(c->jn + imp, 0.0f);
         imp = nj - c->jn; c->jn = nj; tot += nj;
-        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(c->iaxn, imp));
-        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(c->ibxn, imp));
+        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(iaxn, imp));
+        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(ibxn, imp));
+    }
+    V3 ia1 = m->ia1, ia2 = m->ia2, ib1 = m->ib1, ib2 = m->ib2, ian = m->ian, ibn = m->ibn;
+    float tm1 = m->tm1, tm2 = m->tm2, twm = m->twm;
+    if (split) {
+        ia1 = vmul(ia1, fa); ia2 = vmul(ia2, fa); ian = vmul(ian, fa); ib1 = vmul(ib1, fb); ib2 = vmul(ib2, fb); ibn = vmul(ibn, fb);
+        float k1 = ima + imb + vdot(m->ca1, ia1) + vdot(m->cb1, ib1);
+        float k2 = ima + imb + vdot(m->ca2, ia2) + vdot(m->cb2, ib2);
+        tm1 = k1 > 0.0f ? 1.0f / k1 : 0.0f; tm2 = k2 > 0.0f ? 1.0f / k2 : 0.0f;
+        float kt = vdot(n, ian) + vdot(n, ibn);
+        twm = kt > 0.0f ? 1.0f / kt : 0.0f;
     }
     float lim = m->mu * tot;
     V3 dv = vsub(vb, va);
@@ -599,27 +613,72 @@ static void solve_one(ob_world* w, Man* m, const SolveK* K) {
         ms = K->mass_scale; is = K->imp_scale;
     } else { b1 = 0.0f; b2 = 0.0f; ms = 1.0f; is = 0.0f; }
     float o1 = m->jt1, o2 = m->jt2;
-    float j1 = o1 - m->tm1 * ms * (v1 + b1) - is * o1;
-    float j2 = o2 - m->tm2 * ms * (v2 + b2) - is * o2;
+    float j1 = o1 - tm1 * ms * (v1 + b1) - is * o1;
+    float j2 = o2 - tm2 * ms * (v2 + b2) - is * o2;
     float l2 = j1 * j1 + j2 * j2;
     if (l2 > lim * lim) { float kk = lim / om_sqrt(l2); j1 *= kk; j2 *= kk; m->stick = 0; }
     m->jt1 = j1; m->jt2 = j2;
     float d1 = j1 - o1, d2 = j2 - o2;
     V3 p = vadd(vmul(t1, d1), vmul(t2, d2));
-    va = vsub(va, vmul(p, ima)); wa = vsub(wa, vadd(vmul(m->ia1, d1), vmul(m->ia2, d2)));
-    vb = vadd(vb, vmul(p, imb)); wb = vadd(wb, vadd(vmul(m->ib1, d1), vmul(m->ib2, d2)));
+    va = vsub(va, vmul(p, ima)); wa = vsub(wa, vadd(vmul(ia1, d1), vmul(ia2, d2)));
+    vb = vadd(vb, vmul(p, imb)); wb = vadd(wb, vadd(vmul(ib1, d1), vmul(ib2, d2)));
     float wl = lim * m->rad;
     float ot = m->jtw;
-    float jt = om_min(om_max(ot - m->twm * vdot(vsub(wb, wa), n), -wl), wl);
+    float jt = om_min(om_max(ot - twm * vdot(vsub(wb, wa), n), -wl), wl);
     m->jtw = jt;
     float dtw = jt - ot;
-    wa = vsub(wa, vmul(m->ian, dtw)); wb = vadd(wb, vmul(m->ibn, dtw));
+    wa = vsub(wa, vmul(ian, dtw)); wb = vadd(wb, vmul(ibn, dtw));
+    *pva = va; *pwa = wa; *pvb = vb; *pwb = wb;
+}
+
+/* one manifold of the sequential (or colour-ordered) solve: the reference's manifold loop body */
+static void solve_one(ob_world* w, Man* m, const SolveK* K) {
+    int sa = m->sa, sb = m->sb;
+    V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
+    V3 dpab = vsub(w->sdp[sb], w->sdp[sa]);
+    solve_core(m, K, &va, &wa, &vb, &wb, w->sth[sa], w->sth[sb], dpab, 0.0f, 0.0f);
     w->sv[sa] = va; w->sw[sa] = wa; w->sv[sb] = vb; w->sw[sb] = wb;
     w->sv[0] = v3(0, 0, 0); w->sw[0] = v3(0, 0, 0);
 }
 
-static void solve_pass(ob_world* w, const SolveK* K, const int32_t* order) {
-    for (int i = 0; i < w->nman; i++) solve_one(w, &w->man[order ? order[i] : i], K);
+/* The overflow pass (manifolds the colouring left over): Jacobi with mass splitting. Every overflow manifold solves
+ * against its own copies of its bodies, all starting from the velocities before the pass; each body then takes the
+ * average of its copies, its copies' changes summed in manifold order. */
+static void overflow_pass(ob_world* w, const SolveK* K, const int32_t* ovf, int novf, const int32_t* ocnt, V3* dv) {
+    size_t n1 = (size_t)w->nb + 1;
+    V3* v0 = (V3*)malloc(sizeof(V3) * n1);
+    V3* w0 = (V3*)malloc(sizeof(V3) * n1);
+    memcpy(v0, w->sv, sizeof(V3) * n1); memcpy(w0, w->sw, sizeof(V3) * n1);
+    for (int i = 0; i < novf; i++) {
+        Man* m = &w->man[ovf[i]];
+        int sa = m->sa, sb = m->sb;
+        V3 va = v0[sa], wa = w0[sa], vb = v0[sb], wb = w0[sb];
+        V3 dpab = vsub(w->sdp[sb], w->sdp[sa]);
+        float fa = sa ? (float)ocnt[sa] : 1.0f, fb = sb ? (float)ocnt[sb] : 1.0f;
+        solve_core(m, K, &va, &wa, &vb, &wb, w->sth[sa], w->sth[sb], dpab, fa, fb);
+        dv[4 * i + 0] = vsub(va, v0[sa]); dv[4 * i + 1] = vsub(wa, w0[sa]);
+        dv[4 * i + 2] = vsub(vb, v0[sb]); dv[4 * i + 3] = vsub(wb, w0[sb]);
+    }
+    V3* sv = (V3*)calloc(n1, sizeof(V3));
+    V3* sw = (V3*)calloc(n1, sizeof(V3));
+    for (int i = 0; i < novf; i++) {
+        const Man* m = &w->man[ovf[i]];
+        if (m->sa) { sv[m->sa] = vadd(sv[m->sa], dv[4 * i + 0]); sw[m->sa] = vadd(sw[m->sa], dv[4 * i + 1]); }
+        if (m->sb) { sv[m->sb] = vadd(sv[m->sb], dv[4 * i + 2]); sw[m->sb] = vadd(sw[m->sb], dv[4 * i + 3]); }
+    }
+    for (size_t s = 1; s < n1; s++) {
+        if (!ocnt[s]) continue;
+        float inv = 1.0f / (float)ocnt[s];
+        w->sv[s] = vadd(v0[s], vmul(sv[s], inv));
+        w->sw[s] = vadd(w0[s], vmul(sw[s], inv));
+    }
+    free(v0); free(w0); free(sv); free(sw);
+}
+
+static void solve_pass(ob_world* w, const SolveK* K, const int32_t* order, int ncol_order, const int32_t* ovf, int novf,
+                       const int32_t* ocnt, V3* dv) {
+    for (int i = 0; i < ncol_order; i++) solve_one(w, &w->man[order ? order[i] : i], K);
+    if (novf) overflow_pass(w, K, ovf, novf, ocnt, dv);
 }
 
 /* phys.rs:326-354, plus island labels = minimum body index of each island */
@@ -771,12 +830,24 @@ void ob_step(ob_world* w, float dt) {
     w->stats[2] = w->nman; w->stats[3] = pts;
     /* ---- solve order */
     int32_t* order = NULL;
+    int ncol_order = w->nman, novf = 0;
+    int32_t *ovf = NULL, *ocnt = NULL;
+    V3* dv = NULL;
     if (coloured) {
         colour_manifolds(w);
         int K = (int)P[P_COL_LIMIT]; if (K < 1) K = 1; if (K > 64) K = 64;
         order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(w->nman ? w->nman : 1));
+        ovf = (int32_t*)malloc(sizeof(int32_t) * (size_t)(w->nman ? w->nman : 1));
+        ocnt = (int32_t*)calloc((size_t)w->nb + 1, sizeof(int32_t));
         int k = 0;
-        for (int c = 0; c <= K; c++) for (int mi = 0; mi < w->nman; mi++) if (w->colour[mi] == c) order[k++] = mi;
+        for (int c = 0; c < K; c++) for (int mi = 0; mi < w->nman; mi++) if (w->colour[mi] == c) order[k++] = mi;
+        ncol_order = k;
+        for (int mi = 0; mi < w->nman; mi++) if (w->colour[mi] == K) {
+            ovf[novf++] = mi;
+            if (w->man[mi].sa) ocnt[w->man[mi].sa]++;
+            if (w->man[mi].sb) ocnt[w->man[mi].sb]++;
+        }
+        dv = (V3*)malloc(sizeof(V3) * 4 * (size_t)(novf ? novf : 1));
     } else { w->stats[4] = 0; w->stats[5] = 0; }
     /* ---- sub-steps (phys.rs:214-230) */
     V3 g = v3(P[P_GX], P[P_GY], P[P_GZ]);
@@ -791,16 +862,16 @@ void ob_step(ob_world* w, float dt) {
             if (wl > wmax * wmax) w->sw[si] = vmul(ww, wmax / om_sqrt(wl));
         }
         warm_start(w);
-        solve_pass(w, &kb, order);
+        solve_pass(w, &kb, order, ncol_order, ovf, novf, ocnt, dv);
         for (int i = 0; i < n; i++) {
             int si = w->sidx[i]; if (!si) continue;
             V3 v = w->sv[si], ww = w->sw[si];
             w->sdp[si] = vadd(w->sdp[si], vmul(v, h)); w->sth[si] = vadd(w->sth[si], vmul(ww, h));
             w->sq[si] = qintegrate(w->sq[si], ww, h);
         }
-        solve_pass(w, &kr, order);
+        solve_pass(w, &kr, order, ncol_order, ovf, novf, ocnt, dv);
     }
-    free(order);
+    free(order); free(ovf); free(ocnt); free(dv);
     /* ---- write back */
     for (int i = 0; i < n; i++) {
         int si = w->sidx[i]; if (!si) continue;

@@ -42,10 +42,14 @@ enum { MR_N, MR_T1, MR_T2, MR_E0, MR_J, MR_CA1, MR_CA2, MR_CB1, MR_CB2, MR_IA1, 
 constexpr int MR_PER_PT = 5;  // raxn|adj, rbxn|nm, iaxn|jn, ibxn|-, la|-
 constexpr int MR_ROWS = MR_P0 + 4 * MR_PER_PT;
 
+// A step's manifolds. Canonical index m (candidate order): header, sort key, and pos[m], the manifold's position
+// in colour order. The rows are stored by position, so a colour bucket is one contiguous, coalesced range.
 struct ManBuf {
-    int4* hdr;        // shape a, shape b (-1 ground), slot a, slot b
-    float4* row;      // MR_ROWS x mcap
-    uint64_t* key;    // sort key (a, b + 1)
+    int4* hdr;        // by m: shape a, shape b (-1 ground), slot a, slot b
+    uint64_t* key;    // by m: sort key (a, b + 1)
+    uint32_t* pos;    // by m: position in colour order
+    int4* hdrp;       // by position: the header again
+    float4* row;      // by position: MR_ROWS x mcap
 };
 
 struct StepK {
@@ -70,18 +74,21 @@ struct Dev {
     // broadphase
     uint32_t *ecnt, *eoff, *ekey;
     int32_t* eshape;
-    uint32_t *tcnt, *tstart, *tfill;
-    int32_t* tent;
+    uint32_t *tcnt, *tstart, *tfill, *tkey;
+    int32_t* tent;    // shape of each table entry
+    int32_t* cslot;   // per-shape partner slots
     uint32_t *ccnt, *coff;
     int2* cand;
     float4* nar;  // 5 per candidate: n|count, points p|sep
     uint32_t *hit, *hoff;
     ManBuf cur, old;
+    float4* srow;  // rows as built, by canonical index (MR_ROWS x mcap), before the move into colour order
     // colouring
     int32_t *mcol, *mcand;
     unsigned long long *mpri, *bmask, *claim;
-    uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff;
+    uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff, *ovcnt;
     int32_t* order;
+    float4* ovdv;  // overflow velocity changes: 4 per overflow manifold (v, w of side a; v, w of side b)
     // per-slot manifold lists (m * 2 + side), in manifold order
     uint32_t *blcnt, *bloff, *blfill, *blist;
     // islands
@@ -94,7 +101,8 @@ struct Dev {
 
 __device__ __forceinline__ M3 load_m3(const float4* p) { M3 r; r.c[0] = v3f4(p[0]); r.c[1] = v3f4(p[1]); r.c[2] = v3f4(p[2]); return r; }
 __device__ __forceinline__ void store_m3(float4* p, const M3& r) { p[0] = f4(r.c[0], 0.0f); p[1] = f4(r.c[1], 0.0f); p[2] = f4(r.c[2], 0.0f); }
-__device__ __forceinline__ float4& MROW(const Dev& D, const ManBuf& M, int r, uint32_t m) { return M.row[(size_t)r * D.mcap + m]; }
+__device__ __forceinline__ float4& MROW(const Dev& D, const ManBuf& M, int r, uint32_t p) { return M.row[(size_t)r * D.mcap + p]; }
+__device__ __forceinline__ float4& SROW(const Dev& D, int r, uint32_t m) { return D.srow[(size_t)r * D.mcap + m]; }
 __device__ __forceinline__ uint32_t cell_key(int x, int y, int z) { return ((uint32_t)x & 1023u) | (((uint32_t)y & 1023u) << 10) | (((uint32_t)z & 1023u) << 20); }
 __device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) { return mix32(key * 0x9E3779B1u) & mask; }
 
@@ -159,8 +167,10 @@ __global__ void __launch_bounds__(NT) k_prep_shapes(Dev D, StepK K) {
         V3 lo = vsub(wp, e), hi = vadd(wp, e);
         D.swp[s] = f4(wp, 0.0f);
         store_m3(D.swr + 3 * (size_t)s, Rs);
-        D.slo[s] = f4(lo, 0.0f);
-        D.shi[s] = f4(hi, 0.0f);
+        // (the AABB's w carries what the pair test needs: the body, and its awake / static flags)
+        int flags = (D.bflag[b] & 1) | (D.bp[b].w == 0.0f ? 2 : 0);
+        D.slo[s] = f4(lo, __int_as_float(b));
+        D.shi[s] = f4(hi, __int_as_float(flags));
         CellRange cr = cell_range(lo, hi, K.inv_cell);
         D.ecnt[s] = cr.n;
         if (!cr.n) atomicAdd(&D.ctr[C_DROP], 1u);
@@ -186,26 +196,21 @@ __global__ void __launch_bounds__(NT) k_emit_entries(Dev D, StepK K) {
 __global__ void __launch_bounds__(NT) k_fill_table(Dev D) {
     uint32_t ne = min(D.ctr[C_NE], D.ecap);
     GRID_STRIDE(e, ne) {
-        uint32_t sl = slot_of(D.ekey[e], D.tmask);
+        uint32_t key = D.ekey[e];
+        uint32_t sl = slot_of(key, D.tmask);
         uint32_t pos = D.tstart[sl] + atomicAdd(&D.tfill[sl], 1u);
-        D.tent[pos] = (int32_t)e;
+        D.tkey[pos] = key;
+        D.tent[pos] = D.eshape[e];
     }
 }
 
-__device__ __forceinline__ bool pair_ok(const Dev& D, int a, int b, V3 loa, V3 hia) {
-    int ba = D.shb[a], bb = D.shb[b];
-    if (ba == bb) return false;
-    int fa = D.bflag[ba] & 1, fb = D.bflag[bb] & 1;
-    if (!fa && !fb) return false;
-    if (D.bp[ba].w == 0.0f && D.bp[bb].w == 0.0f) return false;
-    V3 lob = v3f4(D.slo[b]), hib = v3f4(D.shi[b]);
-    if (loa.x > hib.x || lob.x > hia.x || loa.y > hib.y || lob.y > hia.y || loa.z > hib.z || lob.z > hia.z) return false;
-    return true;
-}
-
-// Partners of shape a of higher index, found in the cell where the pair's home is; `out` (may be null) receives them.
-__device__ uint32_t partners(const Dev& D, const StepK& K, int a, int2* out, uint32_t cap) {
-    V3 lo = v3f4(D.slo[a]), hi = v3f4(D.shi[a]);
+// Partners of shape a of higher index, found in the cell where the pair's home is (phys.rs:388-404): different
+// bodies, not both resting, not both static, overlapping boxes. Written to out[0..cap) sorted by index; returns the
+// count (more than cap is an overflow the caller reports).
+__device__ uint32_t partners(const Dev& D, const StepK& K, int a, int32_t* out, uint32_t cap) {
+    float4 lo4 = D.slo[a], hi4 = D.shi[a];
+    V3 lo = v3f4(lo4), hi = v3f4(hi4);
+    int ba = __float_as_int(lo4.w), fa = __float_as_int(hi4.w);
     CellRange cr = cell_range(lo, hi, K.inv_cell);
     uint32_t n = 0;
     if (!cr.n) return 0;
@@ -214,24 +219,38 @@ __device__ uint32_t partners(const Dev& D, const StepK& K, int a, int2* out, uin
         uint32_t sl = slot_of(key, D.tmask);
         uint32_t s0 = D.tstart[sl], s1 = s0 + D.tcnt[sl];
         for (uint32_t q = s0; q < s1; q++) {
-            int32_t e = D.tent[q];
-            if (D.ekey[e] != key) continue;
-            int b = D.eshape[e];
+            if (D.tkey[q] != key) continue;
+            int b = D.tent[q];
             if (b <= a) continue;
-            if (!pair_ok(D, a, b, lo, hi)) continue;
-            V3 m = vmaxv(lo, v3f4(D.slo[b]));
+            float4 lob4 = D.slo[b], hib4 = D.shi[b];
+            int bb = __float_as_int(lob4.w), fb = __float_as_int(hib4.w);
+            if (bb == ba) continue;
+            if (!((fa | fb) & 1)) continue;          // both resting
+            if ((fa & fb & 2)) continue;             // both static
+            V3 lob = v3f4(lob4), hib = v3f4(hib4);
+            if (lo.x > hib.x || lob.x > hi.x || lo.y > hib.y || lob.y > hi.y || lo.z > hib.z || lob.z > hi.z) continue;
+            V3 m = vmaxv(lo, lob);
             if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
-            if (out && n < cap) out[n] = make_int2(a, b);
+            if (n < cap) {
+                // insertion into the sorted prefix
+                uint32_t j = n;
+                while (j > 0 && out[j - 1] > b) { out[j] = out[j - 1]; j--; }
+                out[j] = b;
+            }
             n++;
         }
     }
     return n;
 }
 
-__global__ void __launch_bounds__(NT) k_count_cand(Dev D, StepK K) {
+constexpr uint32_t CAND_SLOTS = 48;  // partners a shape may have (a brick in a dense pile has about 13)
+
+// one pass over the grid: each shape's ground candidate and sorted partners into its own slots
+__global__ void __launch_bounds__(NT) k_find_cand(Dev D, StepK K) {
     GRID_STRIDE(a, (uint32_t)D.ns) {
-        int ground = (D.bflag[D.shb[a]] & 1) && D.slo[a].y < 0.0f;
-        uint32_t n = partners(D, K, (int)a, nullptr, 0);
+        int ground = (__float_as_int(D.shi[a].w) & 1) && D.slo[a].y < 0.0f;
+        uint32_t n = partners(D, K, (int)a, D.cslot + (size_t)a * CAND_SLOTS, CAND_SLOTS);
+        if (n > CAND_SLOTS) { atomicAdd(&D.ctr[C_DROP], n - CAND_SLOTS); n = CAND_SLOTS; }
         D.ccnt[a] = n + (uint32_t)ground;
         if (ground) atomicAdd(&D.ctr[C_NGROUND], 1u);
         atomicAdd(&D.ctr[C_NPAIR], n);
@@ -243,16 +262,10 @@ __global__ void __launch_bounds__(NT) k_write_cand(Dev D, StepK K) {
         uint32_t o = D.coff[a], n = D.ccnt[a];
         if (n == 0) continue;
         if (o + n > D.ccap) { atomicAdd(&D.ctr[C_DROP], n); continue; }
-        int ground = (D.bflag[D.shb[a]] & 1) && D.slo[a].y < 0.0f;
+        int ground = (__float_as_int(D.shi[a].w) & 1) && D.slo[a].y < 0.0f;
         if (ground) D.cand[o++] = make_int2((int)a, -1);
-        uint32_t np = min(partners(D, K, (int)a, D.cand + o, n - (uint32_t)ground), n - (uint32_t)ground);
-        // partners by index (insertion sort; a shape has a handful)
-        for (uint32_t i = 1; i < np; i++) {
-            int2 v = D.cand[o + i];
-            uint32_t j = i;
-            while (j > 0 && D.cand[o + j - 1].y > v.y) { D.cand[o + j] = D.cand[o + j - 1]; j--; }
-            D.cand[o + j] = v;
-        }
+        const int32_t* sl = D.cslot + (size_t)a * CAND_SLOTS;
+        for (uint32_t k = 0; k < n - (uint32_t)ground; k++) D.cand[o + k] = make_int2((int)a, sl[k]);
     }
 }
 
@@ -494,6 +507,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         bool stick = K.anchor != 0;
         uint64_t key = man_key(a, b);
         int oi = find_old(D, key, nold);
+        uint32_t opos = oi >= 0 ? D.old.pos[oi] : 0u;
         V3 cen = v3(0, 0, 0);
         for (int k = 0; k < cnt; k++) cen = vadd(cen, pts[k].p);
         cen = vmul(cen, 1.0f / (float)cnt);
@@ -505,10 +519,10 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         bool anchored = false;
         if (oi >= 0) {
             const ManBuf& O = D.old;
-            float4 oj = MROW(D, O, MR_J, oi);
-            bool ostick = MROW(D, O, MR_CB2, oi).w != 0.0f;
+            float4 oj = MROW(D, O, MR_J, opos);
+            bool ostick = MROW(D, O, MR_CB2, opos).w != 0.0f;
             if (ostick) {
-                V3 olaf = v3f4(MROW(D, O, MR_LAF, oi)), olbf = v3f4(MROW(D, O, MR_LBF, oi));
+                V3 olaf = v3f4(MROW(D, O, MR_LAF, opos)), olbf = v3f4(MROW(D, O, MR_LBF, opos));
                 V3 wa = vadd(pa, mulv(ra, olaf));
                 V3 wb = bb >= 0 ? vadd(pb, mulv(rbm, olbf)) : olbf;
                 float lim = sf_max(0.05f, rad);
@@ -516,7 +530,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
                     la_f = olaf; lb_f = olbf; fa = vsub(wa, pa); fb = vsub(wb, pb); e0 = vsub(wb, wa); anchored = true;
                 }
             }
-            V3 ot1 = v3f4(MROW(D, O, MR_T1, oi)), ot2 = v3f4(MROW(D, O, MR_T2, oi)), on = v3f4(MROW(D, O, MR_N, oi));
+            V3 ot1 = v3f4(MROW(D, O, MR_T1, opos)), ot2 = v3f4(MROW(D, O, MR_T2, opos)), on = v3f4(MROW(D, O, MR_N, opos));
             V3 f = vadd(vmul(ot1, oj.x), vmul(ot2, oj.y));
             jt1 = vdot(f, t1); jt2 = vdot(f, t2); jtw = oj.z * vdot(on, n);
         }
@@ -530,7 +544,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
         const ManBuf& C = D.cur;
         int ocnt = 0;
-        if (oi >= 0) ocnt = min(max((int)MROW(D, D.old, MR_CB1, oi).w, 0), 4);
+        if (oi >= 0) ocnt = min(max((int)MROW(D, D.old, MR_CB1, opos).w, 0), 4);
         for (int k = 0; k < cnt; k++) {
             V3 rap = vsub(pts[k].p, pa);
             V3 rbp = bb >= 0 ? vsub(pts[k].p, pb) : pts[k].p;
@@ -539,20 +553,20 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             if (oi >= 0) {
                 float best = 0.03f * 0.03f; int bi = -1;
                 for (int q = 0; q < ocnt; q++) {
-                    float d = vlen2(vsub(v3f4(MROW(D, D.old, MR_P0 + MR_PER_PT * q + 4, oi)), la));
+                    float d = vlen2(vsub(v3f4(MROW(D, D.old, MR_P0 + MR_PER_PT * q + 4, opos)), la));
                     if (d < best) { best = d; bi = q; }
                 }
-                if (bi >= 0) jn = MROW(D, D.old, MR_P0 + MR_PER_PT * bi + 2, oi).w;
+                if (bi >= 0) jn = MROW(D, D.old, MR_P0 + MR_PER_PT * bi + 2, opos).w;
             }
             V3 raxn = vcross(rap, n), rbxn = vcross(rbp, n);
             V3 iaxn = mulv(iia, raxn), ibxn = mulv(iib, rbxn);
             float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn);
             float nm = kk > 0.0f ? 1.0f / kk : 0.0f;
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 0, m) = f4(raxn, pts[k].s);
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 1, m) = f4(rbxn, nm);
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m) = f4(iaxn, jn);
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m) = f4(ibxn, 0.0f);
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 4, m) = f4(la, 0.0f);
+            SROW(D, MR_P0 + MR_PER_PT * k + 0, m) = f4(raxn, pts[k].s);
+            SROW(D, MR_P0 + MR_PER_PT * k + 1, m) = f4(rbxn, nm);
+            SROW(D, MR_P0 + MR_PER_PT * k + 2, m) = f4(iaxn, jn);
+            SROW(D, MR_P0 + MR_PER_PT * k + 3, m) = f4(ibxn, 0.0f);
+            SROW(D, MR_P0 + MR_PER_PT * k + 4, m) = f4(la, 0.0f);
         }
         V3 ca1 = vcross(fa, t1), ca2 = vcross(fa, t2), cb1 = vcross(fb, t1), cb2 = vcross(fb, t2);
         V3 ia1 = mulv(iia, ca1), ia2 = mulv(iia, ca2), ib1 = mulv(iib, cb1), ib2 = mulv(iib, cb2);
@@ -564,23 +578,23 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         float twm = kt > 0.0f ? 1.0f / kt : 0.0f;
         C.hdr[m] = make_int4(a, b, sa, sb);
         C.key[m] = key;
-        MROW(D, C, MR_N, m) = f4(n, mu);
-        MROW(D, C, MR_T1, m) = f4(t1, tm1);
-        MROW(D, C, MR_T2, m) = f4(t2, tm2);
-        MROW(D, C, MR_E0, m) = f4(e0, twm);
-        MROW(D, C, MR_J, m) = make_float4(jt1, jt2, jtw, rad);
-        MROW(D, C, MR_CA1, m) = f4(ca1, ima);
-        MROW(D, C, MR_CA2, m) = f4(ca2, imb);
-        MROW(D, C, MR_CB1, m) = f4(cb1, (float)cnt);
-        MROW(D, C, MR_CB2, m) = f4(cb2, stick ? 1.0f : 0.0f);
-        MROW(D, C, MR_IA1, m) = f4(ia1, 0.0f);
-        MROW(D, C, MR_IA2, m) = f4(ia2, 0.0f);
-        MROW(D, C, MR_IB1, m) = f4(ib1, 0.0f);
-        MROW(D, C, MR_IB2, m) = f4(ib2, 0.0f);
-        MROW(D, C, MR_IAN, m) = f4(ian, 0.0f);
-        MROW(D, C, MR_IBN, m) = f4(ibn, 0.0f);
-        MROW(D, C, MR_LAF, m) = f4(la_f, 0.0f);
-        MROW(D, C, MR_LBF, m) = f4(lb_f, 0.0f);
+        SROW(D, MR_N, m) = f4(n, mu);
+        SROW(D, MR_T1, m) = f4(t1, tm1);
+        SROW(D, MR_T2, m) = f4(t2, tm2);
+        SROW(D, MR_E0, m) = f4(e0, twm);
+        SROW(D, MR_J, m) = make_float4(jt1, jt2, jtw, rad);
+        SROW(D, MR_CA1, m) = f4(ca1, ima);
+        SROW(D, MR_CA2, m) = f4(ca2, imb);
+        SROW(D, MR_CB1, m) = f4(cb1, (float)cnt);
+        SROW(D, MR_CB2, m) = f4(cb2, stick ? 1.0f : 0.0f);
+        SROW(D, MR_IA1, m) = f4(ia1, 0.0f);
+        SROW(D, MR_IA2, m) = f4(ia2, 0.0f);
+        SROW(D, MR_IB1, m) = f4(ib1, 0.0f);
+        SROW(D, MR_IB2, m) = f4(ib2, 0.0f);
+        SROW(D, MR_IAN, m) = f4(ian, 0.0f);
+        SROW(D, MR_IBN, m) = f4(ibn, 0.0f);
+        SROW(D, MR_LAF, m) = f4(la_f, 0.0f);
+        SROW(D, MR_LBF, m) = f4(lb_f, 0.0f);
         atomicAdd(&D.ctr[C_NPTS], (uint32_t)cnt);
         if (sa) atomicAdd(&D.blcnt[sa], 1u);
         if (sb) atomicAdd(&D.blcnt[sb], 1u);
@@ -635,24 +649,62 @@ __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
     }
 }
 
+// colour histogram: per-block counts in LDS, one global add per colour per block (a few colours take every
+// manifold, so per-manifold global atomics would serialise on a handful of addresses)
 __global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
+    __shared__ uint32_t hist[72];
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
-    GRID_STRIDE(m, nm) {
-        int c = D.mcol[m];
-        if (c < 0) { c = K; D.mcol[m] = K; }
-        atomicAdd(&D.colcnt[c], 1u);
-        D.ovflag[m] = c == K ? 1u : 0u;
-        if (c < K) atomicMax(&D.ctr[C_NCOL], (uint32_t)c + 1u);
-        else atomicAdd(&D.ctr[C_NOVF], 1u);
+    for (uint32_t b0 = blockIdx.x * NT; b0 < nm; b0 += gridDim.x * NT) {
+        for (int c = threadIdx.x; c <= K; c += NT) hist[c] = 0u;
+        __syncthreads();
+        uint32_t m = b0 + threadIdx.x;
+        if (m < nm) {
+            int c = D.mcol[m];
+            if (c < 0) { c = K; D.mcol[m] = K; }
+            atomicAdd(&hist[c], 1u);
+            D.ovflag[m] = c == K ? 1u : 0u;
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c <= K; c += NT) {
+            uint32_t h = hist[c];
+            if (!h) continue;
+            atomicAdd(&D.colcnt[c], h);
+            if (c < K) atomicMax(&D.ctr[C_NCOL], (uint32_t)c + 1u);
+            else atomicAdd(&D.ctr[C_NOVF], h);
+        }
+        __syncthreads();
     }
 }
 
+// colour-ordered list: a block reserves its range in each colour once, lanes take ranks inside it from LDS
 __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
+    __shared__ uint32_t cnt[72], base[72];
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
-    GRID_STRIDE(m, nm) {
-        int c = D.mcol[m];
-        if (c < K) D.order[D.coloff[c] + atomicAdd(&D.colfill[c], 1u)] = (int32_t)m;
-        else D.order[D.coloff[K] + D.ovoff[m]] = (int32_t)m;  // the overflow in manifold order
+    for (uint32_t b0 = blockIdx.x * NT; b0 < nm; b0 += gridDim.x * NT) {
+        for (int c = threadIdx.x; c <= K; c += NT) cnt[c] = 0u;
+        __syncthreads();
+        uint32_t m = b0 + threadIdx.x;
+        int c = m < nm ? D.mcol[m] : -1;
+        uint32_t rank = 0;
+        if (c >= 0 && c < K) rank = atomicAdd(&cnt[c], 1u);
+        __syncthreads();
+        for (int q = threadIdx.x; q < K; q += NT) if (cnt[q]) base[q] = atomicAdd(&D.colfill[q], cnt[q]);
+        __syncthreads();
+        if (c >= 0 && c < K) D.order[D.coloff[c] + base[c] + rank] = (int32_t)m;
+        else if (c == K) D.order[D.coloff[K] + D.ovoff[m]] = (int32_t)m;  // the overflow in manifold order
+        __syncthreads();
+    }
+}
+
+// move the built rows into colour order: the manifold at position p of the colour-ordered list
+__global__ void __launch_bounds__(NT) k_permute(Dev D) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(p, nm) {
+        uint32_t m = (uint32_t)D.order[p];
+        D.cur.pos[m] = p;
+        D.cur.hdrp[p] = D.cur.hdr[m];
+#pragma unroll 4
+        for (int r = 0; r < MR_ROWS; r++) MROW(D, D.cur, r, p) = SROW(D, r, m);
     }
 }
 
@@ -665,23 +717,82 @@ __global__ void __launch_bounds__(NT) k_bl_fill(Dev D) {
         if (hd.w) D.blist[D.bloff[hd.w] + atomicAdd(&D.blfill[hd.w], 1u)] = m * 2u + 1u;
     }
 }
+// Each body's contact list into manifold order: one wavefront per body, a 64-lane bitonic sort in registers (lists
+// longer than a wavefront are sorted by k_bl_sort_big).
 __global__ void __launch_bounds__(NT) k_bl_sort(Dev D) {
-    GRID_STRIDE(s, (uint32_t)D.nb + 1u) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t s = wave; s < (uint32_t)D.nb + 1u; s += nwaves) {
         uint32_t o = D.bloff[s], n = D.blcnt[s];
-        for (uint32_t i = 1; i < n; i++) {
-            uint32_t v = D.blist[o + i];
-            uint32_t j = i;
-            while (j > 0 && D.blist[o + j - 1] > v) { D.blist[o + j] = D.blist[o + j - 1]; j--; }
-            D.blist[o + j] = v;
+        if (n < 2) continue;
+        if (n <= 64) {
+            uint32_t v = (uint32_t)lane < n ? D.blist[o + lane] : 0xFFFFFFFFu;
+            for (int k = 2; k <= 64; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    uint32_t u = __shfl_xor(v, j, 64);
+                    bool up = (lane & k) == 0, lower = (lane & j) == 0;
+                    v = (lower == up) ? min(v, u) : max(v, u);
+                }
+            }
+            if ((uint32_t)lane < n) D.blist[o + lane] = v;
         }
+    }
+}
+
+// lists longer than a wavefront (the largest bodies): one workgroup each, bitonic sort in LDS
+__global__ void __launch_bounds__(NT) k_bl_sort_big(Dev D) {
+    __shared__ uint32_t sh[2048];
+    for (uint32_t s = blockIdx.x; s < (uint32_t)D.nb + 1u; s += gridDim.x) {
+        uint32_t o = D.bloff[s], n = D.blcnt[s];
+        if (n <= 64) continue;
+        if (n > 2048) {
+            if (threadIdx.x == 0) {
+                for (uint32_t i = 1; i < n; i++) {
+                    uint32_t v = D.blist[o + i];
+                    uint32_t j = i;
+                    while (j > 0 && D.blist[o + j - 1] > v) { D.blist[o + j] = D.blist[o + j - 1]; j--; }
+                    D.blist[o + j] = v;
+                }
+            }
+            continue;
+        }
+        uint32_t n2 = 128;
+        while (n2 < n) n2 <<= 1;
+        for (uint32_t i = threadIdx.x; i < n2; i += NT) sh[i] = i < n ? D.blist[o + i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < n2; i += NT) {
+                    uint32_t l = i ^ j;
+                    if (l > i) {
+                        uint32_t x = sh[i], y = sh[l];
+                        bool up = (i & k) == 0;
+                        if ((x > y) == up) { sh[i] = y; sh[l] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < n; i += NT) D.blist[o + i] = sh[i];
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------- solver
 // gravity, spin clamp, then this body's warm start gathered over its contacts in manifold order: the same
 // sequence of f32 updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262)
+__device__ __forceinline__ float rdl(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+__device__ __forceinline__ V3 rdl3(V3 a, int l) { return v3(rdl(a.x, l), rdl(a.y, l), rdl(a.z, l)); }
+
+// Gravity, spin clamp, then this body's warm start over its contacts in manifold order: the same sequence of f32
+// updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262). One wavefront per body: the lanes
+// load up to 64 contacts at once and form their update vectors; every lane then applies them in order (values read
+// across lanes), so the loads overlap and the order of the additions is the reference's.
 __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
-    GRID_STRIDE(i, (uint32_t)D.nb) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    const ManBuf& C = D.cur;
+    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
         int s = D.bslot[i];
         if (!s) continue;
         float4 v4 = D.sv[s];
@@ -689,77 +800,120 @@ __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
         v = vadd(v, vmul(v3(K.gx, K.gy, K.gz), K.h));
         float wl = vlen2(w);
         if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
-        const ManBuf& C = D.cur;
         uint32_t o = D.bloff[s], n = D.blcnt[s];
-        for (uint32_t e = 0; e < n; e++) {
-            uint32_t ent = D.blist[o + e];
-            uint32_t m = ent >> 1;
-            bool side_b = ent & 1u;
-            int cnt = (int)MROW(D, C, MR_CB1, m).w;
-            float pn = 0.0f;
-            for (int k = 0; k < cnt; k++) {
-                float4 r2 = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
-                float jn = r2.w;
-                pn += jn;
-                if (!side_b) w = vsub(w, vmul(v3f4(r2), jn));
-                else w = vadd(w, vmul(v3f4(MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m)), jn));
+        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+            uint32_t e = e0 + lane;
+            V3 t[4], tv = v3(0, 0, 0), tf = v3(0, 0, 0);
+            int cnt = 0, side = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) t[k] = v3(0, 0, 0);
+            if (e < n) {
+                uint32_t ent = D.blist[o + e];
+                uint32_t m = C.pos[ent >> 1];
+                side = (int)(ent & 1u);
+                cnt = (int)MROW(D, C, MR_CB1, m).w;
+                float4 rn = MROW(D, C, MR_N, m), rt1 = MROW(D, C, MR_T1, m), rt2 = MROW(D, C, MR_T2, m), j4 = MROW(D, C, MR_J, m);
+                float4 im4 = side ? MROW(D, C, MR_CA2, m) : MROW(D, C, MR_CA1, m);
+                float4 f1 = side ? MROW(D, C, MR_IB1, m) : MROW(D, C, MR_IA1, m);
+                float4 f2 = side ? MROW(D, C, MR_IB2, m) : MROW(D, C, MR_IA2, m);
+                float4 fn = side ? MROW(D, C, MR_IBN, m) : MROW(D, C, MR_IAN, m);
+                float pn = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float4 r2 = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
+                    float4 r3 = MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m);
+                    if (k < cnt) {
+                        pn += r2.w;
+                        t[k] = vmul(v3f4(side ? r3 : r2), r2.w);
+                    }
+                }
+                V3 p = vadd(vadd(vmul(v3f4(rn), pn), vmul(v3f4(rt1), j4.x)), vmul(v3f4(rt2), j4.y));
+                tv = vmul(p, im4.w);
+                tf = vadd(vadd(vmul(v3f4(f1), j4.x), vmul(v3f4(f2), j4.y)), vmul(v3f4(fn), j4.z));
             }
-            float4 j4 = MROW(D, C, MR_J, m);
-            V3 p = vadd(vadd(vmul(v3f4(MROW(D, C, MR_N, m)), pn), vmul(v3f4(MROW(D, C, MR_T1, m)), j4.x)), vmul(v3f4(MROW(D, C, MR_T2, m)), j4.y));
-            if (!side_b) {
-                v = vsub(v, vmul(p, MROW(D, C, MR_CA1, m).w));
-                w = vsub(w, vadd(vadd(vmul(v3f4(MROW(D, C, MR_IA1, m)), j4.x), vmul(v3f4(MROW(D, C, MR_IA2, m)), j4.y)), vmul(v3f4(MROW(D, C, MR_IAN, m)), j4.z)));
-            } else {
-                v = vadd(v, vmul(p, MROW(D, C, MR_CA2, m).w));
-                w = vadd(w, vadd(vadd(vmul(v3f4(MROW(D, C, MR_IB1, m)), j4.x), vmul(v3f4(MROW(D, C, MR_IB2, m)), j4.y)), vmul(v3f4(MROW(D, C, MR_IBN, m)), j4.z)));
+            int nl = (int)min(64u, n - e0);
+            for (int l = 0; l < nl; l++) {
+                int c = __builtin_amdgcn_readlane(cnt, l);
+                bool sb = __builtin_amdgcn_readlane(side, l) != 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (k < c) { V3 tk = rdl3(t[k], l); w = sb ? vadd(w, tk) : vsub(w, tk); }
+                }
+                V3 a = rdl3(tv, l), f = rdl3(tf, l);
+                v = sb ? vadd(v, a) : vsub(v, a);
+                w = sb ? vadd(w, f) : vsub(w, f);
             }
         }
-        D.sv[s] = f4(v, v4.w);
-        D.sw[s] = f4(w, 0.0f);
+        if (lane == 0) {
+            D.sv[s] = f4(v, v4.w);
+            D.sw[s] = f4(w, 0.0f);
+        }
     }
 }
 
 __device__ __forceinline__ float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
 
-// one manifold of the soft-step solve (phys.rs:268-323)
-__device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
+// The soft-step solve of one manifold (phys.rs:268-323) on velocity copies of its two bodies. fa, fb = 0: as the
+// reference solves it. fa, fb > 0 (the overflow pass): mass splitting, each side's inverse mass and inertia scaled
+// by the number of overflow manifolds at that body and the effective masses recomputed from the scaled ones.
+__device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK& K, V3& va, V3& wa, V3& vb, V3& wb,
+                                           V3 tha, V3 thb, V3 dpab, float fa, float fb) {
     const ManBuf& C = D.cur;
-    int4 hd = C.hdr[m];
-    int sa = hd.z, sb = hd.w;
-    float4 va4 = D.sv[sa], vb4 = D.sv[sb];
-    V3 va = v3f4(va4), wa = v3f4(D.sw[sa]), vb = v3f4(vb4), wb = v3f4(D.sw[sb]);
-    V3 tha = v3f4(D.sth[sa]), thb = v3f4(D.sth[sb]);
+    const bool split = fa > 0.0f;
+    // every load of the record up front (they are independent; the stores come at the end)
     float4 n4 = MROW(D, C, MR_N, m), t14 = MROW(D, C, MR_T1, m), t24 = MROW(D, C, MR_T2, m);
-    V3 n = v3f4(n4), t1 = v3f4(t14), t2 = v3f4(t24);
-    V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
-    float dn = vdot(dpab, n);
     float4 ca14 = MROW(D, C, MR_CA1, m), ca24 = MROW(D, C, MR_CA2, m), cb14 = MROW(D, C, MR_CB1, m), cb24 = MROW(D, C, MR_CB2, m);
-    float ima = ca14.w, imb = ca24.w;
+    float4 e04 = MROW(D, C, MR_E0, m), j4 = MROW(D, C, MR_J, m);
+    float4 ia14 = MROW(D, C, MR_IA1, m), ia24 = MROW(D, C, MR_IA2, m), ib14 = MROW(D, C, MR_IB1, m), ib24 = MROW(D, C, MR_IB2, m);
+    float4 ian4 = MROW(D, C, MR_IAN, m), ibn4 = MROW(D, C, MR_IBN, m);
+    float4 r0[4], r1[4], r2[4], r3[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        r0[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 0, m);
+        r1[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 1, m);
+        r2[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
+        r3[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m);
+    }
+    V3 n = v3f4(n4), t1 = v3f4(t14), t2 = v3f4(t24);
+    float dn = vdot(dpab, n);
+    float ima = split ? ca14.w * fa : ca14.w, imb = split ? ca24.w * fb : ca24.w;
     int cnt = (int)cb14.w;
     float tot = 0.0f;
-    for (int k = 0; k < cnt; k++) {
-        float4 r0 = MROW(D, C, MR_P0 + MR_PER_PT * k + 0, m);
-        float4 r1 = MROW(D, C, MR_P0 + MR_PER_PT * k + 1, m);
-        float4 r2 = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
-        float4 r3 = MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m);
-        V3 raxn = v3f4(r0), rbxn = v3f4(r1);
-        float jn = r2.w;
-        float s = r0.w + dn + vdot(thb, rbxn) - vdot(tha, raxn);
+    float njs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k >= cnt) continue;
+        V3 raxn = v3f4(r0[k]), rbxn = v3f4(r1[k]);
+        V3 iaxn = split ? vmul(v3f4(r2[k]), fa) : v3f4(r2[k]), ibxn = split ? vmul(v3f4(r3[k]), fb) : v3f4(r3[k]);
+        float nm = r1[k].w;
+        if (split) { float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
+        float jn = r2[k].w;
+        float s = r0[k].w + dn + vdot(thb, rbxn) - vdot(tha, raxn);
         float bias, ms, is;
         if (s > 0.0f) { bias = s * K.inv_h; ms = 1.0f; is = 0.0f; }
         else if (K.use_bias) { bias = sf_max(K.bias_rate * s, -K.max_push); ms = K.mass_scale; is = K.imp_scale; }
         else { bias = 0.0f; ms = 1.0f; is = 0.0f; }
         float vn = vdot(vsub(vb, va), n) + vdot(wb, rbxn) - vdot(wa, raxn);
-        float imp = -r1.w * ms * (vn + bias) - is * jn;
+        float imp = -nm * ms * (vn + bias) - is * jn;
         float nj = sf_max(jn + imp, 0.0f);
         imp = nj - jn;
         tot += nj;
-        reinterpret_cast<float*>(&MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m))[3] = nj;
-        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(v3f4(r2), imp));
-        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(v3f4(r3), imp));
+        njs[k] = nj;
+        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(iaxn, imp));
+        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(ibxn, imp));
+    }
+    V3 ia1 = v3f4(ia14), ia2 = v3f4(ia24), ib1 = v3f4(ib14), ib2 = v3f4(ib24), ian = v3f4(ian4), ibn = v3f4(ibn4);
+    V3 ca1 = v3f4(ca14), ca2 = v3f4(ca24), cb1 = v3f4(cb14), cb2 = v3f4(cb24);
+    float tm1 = t14.w, tm2 = t24.w, twm = e04.w;
+    if (split) {
+        ia1 = vmul(ia1, fa); ia2 = vmul(ia2, fa); ian = vmul(ian, fa); ib1 = vmul(ib1, fb); ib2 = vmul(ib2, fb); ibn = vmul(ibn, fb);
+        float k1 = ima + imb + vdot(ca1, ia1) + vdot(cb1, ib1);
+        float k2 = ima + imb + vdot(ca2, ia2) + vdot(cb2, ib2);
+        tm1 = k1 > 0.0f ? 1.0f / k1 : 0.0f; tm2 = k2 > 0.0f ? 1.0f / k2 : 0.0f;
+        float kt = vdot(n, ian) + vdot(n, ibn);
+        twm = kt > 0.0f ? 1.0f / kt : 0.0f;
     }
     float lim = n4.w * tot;
-    V3 ca1 = v3f4(ca14), ca2 = v3f4(ca24), cb1 = v3f4(cb14), cb2 = v3f4(cb24);
     V3 dv = vsub(vb, va);
     float v1 = vdot(dv, t1) + vdot(wb, cb1) - vdot(wa, ca1);
     float v2 = vdot(dv, t2) + vdot(wb, cb2) - vdot(wa, ca2);
@@ -767,7 +921,7 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
     bool anchored = K.use_bias && stick;
     float b1, b2, ms, is;
     if (anchored) {
-        V3 e0 = v3f4(MROW(D, C, MR_E0, m));
+        V3 e0 = v3f4(e04);
         float e1 = vdot(e0, t1) + vdot(dpab, t1) + vdot(thb, cb1) - vdot(tha, ca1);
         float e2 = vdot(e0, t2) + vdot(dpab, t2) + vdot(thb, cb2) - vdot(tha, ca2);
         float ap = K.anchor_push, sl = K.anchor_slop;
@@ -775,27 +929,37 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
         b2 = sf_min(sf_max(K.bias_rate * slop_fn(e2, sl), -ap), ap);
         ms = K.mass_scale; is = K.imp_scale;
     } else { b1 = 0.0f; b2 = 0.0f; ms = 1.0f; is = 0.0f; }
-    float4 j4 = MROW(D, C, MR_J, m);
     float o1 = j4.x, o2 = j4.y;
-    float j1 = o1 - t14.w * ms * (v1 + b1) - is * o1;
-    float j2 = o2 - t24.w * ms * (v2 + b2) - is * o2;
+    float j1 = o1 - tm1 * ms * (v1 + b1) - is * o1;
+    float j2 = o2 - tm2 * ms * (v2 + b2) - is * o2;
     float l2 = j1 * j1 + j2 * j2;
-    if (l2 > lim * lim) {
-        float kk = lim / sf_sqrt(l2); j1 *= kk; j2 *= kk;
-        reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;
-    }
+    bool unstick = false;
+    if (l2 > lim * lim) { float kk = lim / sf_sqrt(l2); j1 *= kk; j2 *= kk; unstick = true; }
     float d1 = j1 - o1, d2 = j2 - o2;
     V3 p = vadd(vmul(t1, d1), vmul(t2, d2));
-    va = vsub(va, vmul(p, ima)); wa = vsub(wa, vadd(vmul(v3f4(MROW(D, C, MR_IA1, m)), d1), vmul(v3f4(MROW(D, C, MR_IA2, m)), d2)));
-    vb = vadd(vb, vmul(p, imb)); wb = vadd(wb, vadd(vmul(v3f4(MROW(D, C, MR_IB1, m)), d1), vmul(v3f4(MROW(D, C, MR_IB2, m)), d2)));
+    va = vsub(va, vmul(p, ima)); wa = vsub(wa, vadd(vmul(ia1, d1), vmul(ia2, d2)));
+    vb = vadd(vb, vmul(p, imb)); wb = vadd(wb, vadd(vmul(ib1, d1), vmul(ib2, d2)));
     float wl = lim * j4.w;
     float ot = j4.z;
-    float twm = MROW(D, C, MR_E0, m).w;
     float jt = sf_min(sf_max(ot - twm * vdot(vsub(wb, wa), n), -wl), wl);
     float dtw = jt - ot;
+    wa = vsub(wa, vmul(ian, dtw));
+    wb = vadd(wb, vmul(ibn, dtw));
+    // stores: impulses and the stick flag
+#pragma unroll
+    for (int k = 0; k < 4; k++) if (k < cnt) reinterpret_cast<float*>(&MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m))[3] = njs[k];
     MROW(D, C, MR_J, m) = make_float4(j1, j2, jt, j4.w);
-    wa = vsub(wa, vmul(v3f4(MROW(D, C, MR_IAN, m)), dtw));
-    wb = vadd(wb, vmul(v3f4(MROW(D, C, MR_IBN, m)), dtw));
+    if (unstick) reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;
+}
+
+// one manifold of the colour-ordered solve (p: its position in colour order)
+__device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
+    int4 hd = D.cur.hdrp[m];
+    int sa = hd.z, sb = hd.w;
+    float4 va4 = D.sv[sa], vb4 = D.sv[sb];
+    V3 va = v3f4(va4), wa = v3f4(D.sw[sa]), vb = v3f4(vb4), wb = v3f4(D.sw[sb]);
+    V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
+    solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, 0.0f, 0.0f);
     if (sa) { D.sv[sa] = f4(va, va4.w); D.sw[sa] = f4(wa, 0.0f); }
     if (sb) { D.sv[sb] = f4(vb, vb4.w); D.sw[sb] = f4(wb, 0.0f); }
 }
@@ -803,20 +967,84 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
 // one colour bucket: its manifolds share no moving body, so every lane solves one
 __global__ void __launch_bounds__(NT) k_solve_bucket(Dev D, StepK K, int c) {
     uint32_t n = D.colcnt[c], o = D.coloff[c];
-    GRID_STRIDE(i, n) solve_one(D, (uint32_t)D.order[o + i], K);
+    GRID_STRIDE(i, n) solve_one(D, o + i, K);
 }
 
-// the small colours from c0 on, one after another in one workgroup, then the overflow one by one in order
+// the small colours from c0 on, one after another in one workgroup
 __global__ void __launch_bounds__(NT) k_solve_tail(Dev D, StepK K, int c0) {
     int ncol = (int)D.ctr[C_NCOL];
     for (int c = c0; c < ncol; c++) {
         uint32_t n = D.colcnt[c], o = D.coloff[c];
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) solve_one(D, (uint32_t)D.order[o + i], K);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) solve_one(D, o + i, K);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
-        for (uint32_t i = 0; i < n; i++) solve_one(D, (uint32_t)D.order[o + i], K);
+}
+
+// overflow (Jacobi with mass splitting): every overflow manifold against copies of its bodies taken before the pass
+__global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
+    uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
+    GRID_STRIDE(i, n) {
+        uint32_t m = o + i;
+        int4 hd = D.cur.hdrp[m];
+        int sa = hd.z, sb = hd.w;
+        V3 va0 = v3f4(D.sv[sa]), wa0 = v3f4(D.sw[sa]), vb0 = v3f4(D.sv[sb]), wb0 = v3f4(D.sw[sb]);
+        V3 va = va0, wa = wa0, vb = vb0, wb = wb0;
+        V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
+        float fa = sa ? (float)D.ovcnt[sa] : 1.0f, fb = sb ? (float)D.ovcnt[sb] : 1.0f;
+        solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, fa, fb);
+        float4* dv = D.ovdv + 4 * (size_t)i;
+        dv[0] = f4(vsub(va, va0), 0.0f); dv[1] = f4(vsub(wa, wa0), 0.0f);
+        dv[2] = f4(vsub(vb, vb0), 0.0f); dv[3] = f4(vsub(wb, wb0), 0.0f);
+    }
+}
+// each body takes the average of its copies: the changes summed in manifold order (one wavefront per body, the
+// lanes load up to 64 contacts at once and the sum runs over them in order)
+__global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
+        int s = D.bslot[i];
+        if (!s) continue;
+        uint32_t cnt = D.ovcnt[s];
+        if (!cnt) continue;
+        V3 sv = v3(0, 0, 0), sw = v3(0, 0, 0);
+        uint32_t o = D.bloff[s], n = D.blcnt[s];
+        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+            uint32_t e = e0 + lane;
+            int use = 0;
+            V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0);
+            if (e < n) {
+                uint32_t ent = D.blist[o + e];
+                uint32_t m = ent >> 1;
+                if (D.mcol[m] == K.K) {
+                    use = 1;
+                    const float4* dv = D.ovdv + 4 * (size_t)D.ovoff[m] + ((ent & 1u) ? 2 : 0);
+                    dvv = v3f4(dv[0]);
+                    dww = v3f4(dv[1]);
+                }
+            }
+            int nl = (int)min(64u, n - e0);
+            for (int l = 0; l < nl; l++) {
+                if (!__builtin_amdgcn_readlane(use, l)) continue;
+                sv = vadd(sv, rdl3(dvv, l));
+                sw = vadd(sw, rdl3(dww, l));
+            }
+        }
+        if (lane == 0) {
+            float inv = 1.0f / (float)cnt;
+            float4 v4 = D.sv[s];
+            D.sv[s] = f4(vadd(v3f4(v4), vmul(sv, inv)), v4.w);
+            D.sw[s] = f4(vadd(v3f4(D.sw[s]), vmul(sw, inv)), 0.0f);
+        }
+    }
+}
+__global__ void __launch_bounds__(NT) k_ovf_count(Dev D, int K) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(m, nm) {
+        if (D.mcol[m] != K) continue;
+        int4 hd = D.cur.hdr[m];
+        if (hd.z) atomicAdd(&D.ovcnt[hd.z], 1u);
+        if (hd.w) atomicAdd(&D.ovcnt[hd.w], 1u);
     }
 }
 
@@ -849,9 +1077,7 @@ __device__ __forceinline__ int uf_find(int32_t* par, int x) {
     while (true) {
         int p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
         if (p == x) return x;
-        int g = __atomic_load_n(&par[p], __ATOMIC_RELAXED);
-        if (g != p) atomicCAS(&par[x], p, g);  // path halving
-        x = g;
+        x = p;
     }
 }
 // hook the larger root under the smaller: every island's root ends as its minimum body index
@@ -865,8 +1091,11 @@ __global__ void __launch_bounds__(NT) k_uf_hook(Dev D) {
             a = uf_find(D.parent, a);
             b = uf_find(D.parent, b);
             if (a == b) break;
-            int hi = a > b ? a : b, lo = a > b ? b : a;
-            if (atomicCAS(&D.parent[hi], hi, lo) == hi) break;
+            if (a < b) { int t = a; a = b; b = t; }
+            // lower the larger root's parent; if it had been hooked elsewhere meanwhile, join that parent next
+            int old = atomicMin(&D.parent[a], b);
+            if (old == a) break;
+            a = old;
         }
     }
 }
@@ -955,7 +1184,7 @@ void default_params(float* p) {
     p[SF_P_SLEEP_LIN] = 0.05f; p[SF_P_SLEEP_ANG] = 0.08f; p[SF_P_SLEEP_TIME] = 0.5f; p[SF_P_SLEEP_DIST] = 0.01f;
     p[SF_P_SLEEP_ROT] = 0.02f; p[SF_P_ALLOW_SLEEP] = 1.0f; p[SF_P_CELL] = 1.0f; p[SF_P_LEAK] = 1.0f;
     p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 64;
-    p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_RESERVED] = 0;
+    p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_TAIL_MIN] = 1024;
 }
 
 int world_alloc(HostWorld* w) {
@@ -976,16 +1205,20 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.shb, ns); r |= w->alloc(D.slp, ns); r |= w->alloc(D.slr, 3 * (size_t)ns); r |= w->alloc(D.shh, ns);
     r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
     r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
-    r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap);
+    r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
+    r |= w->alloc(D.cslot, (size_t)ns * CAND_SLOTS);
     r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
     r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
     for (ManBuf* M : {&D.cur, &D.old}) {
         r |= w->alloc(M->hdr, D.mcap); r |= w->alloc(M->row, (size_t)MR_ROWS * D.mcap); r |= w->alloc(M->key, D.mcap);
+        r |= w->alloc(M->pos, D.mcap); r |= w->alloc(M->hdrp, D.mcap);
     }
+    r |= w->alloc(D.srow, (size_t)MR_ROWS * D.mcap);
     r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
     r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap); r |= w->alloc(D.order, D.mcap);
+    r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovdv, 4 * (size_t)D.mcap);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap);
     r |= w->alloc(D.parent, nb); r |= w->alloc(D.mint, nb); r |= w->alloc(D.ctr, C_NCTR);
     uint32_t big = std::max<uint32_t>(std::max<uint32_t>(D.tcap, D.ccap), (uint32_t)n1);
@@ -1022,7 +1255,6 @@ StepK make_step(const float* P, float dt) {
     return K;
 }
 
-constexpr int TAIL_MIN = 2048;  // colour buckets smaller than this go to the one-workgroup tail pass
 
 // Everything up to the colour counts; the sub-steps are planned from the counts read back by the caller.
 int step_front(HostWorld* w, const StepK& K) {
@@ -1041,6 +1273,7 @@ int step_front(HostWorld* w, const StepK& K) {
     SF_HIP(hipMemsetAsync(D.claim, 0, sizeof(unsigned long long) * ((size_t)D.nb + 1) * K.K, s));
     SF_HIP(hipMemsetAsync(D.colcnt, 0, sizeof(uint32_t) * 72, s));
     SF_HIP(hipMemsetAsync(D.colfill, 0, sizeof(uint32_t) * 72, s));
+    SF_HIP(hipMemsetAsync(D.ovcnt, 0, sizeof(uint32_t) * ((size_t)D.nb + 1), s));
     if (w->timing) SF_HIP(hipEventRecord(w->ev[0], s));
     hipLaunchKernelGGL(k_prep_bodies, dim3(gb), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_prep_shapes, dim3(gs), dim3(NT), 0, s, D, K);
@@ -1048,7 +1281,7 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_emit_entries, dim3(gs), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.tcnt, D.tstart, nullptr, nullptr, D.tcap, D.tcap, w->scan_tmp, s);
     hipLaunchKernelGGL(k_fill_table, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_count_cand, dim3(gs), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_find_cand, dim3(gs), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.ccnt, D.coff, D.ctr + C_NCAND, nullptr, (uint32_t)D.ns, (uint32_t)D.ns, w->scan_tmp, s);
     hipLaunchKernelGGL(k_write_cand, dim3(gs), dim3(NT), 0, s, D, K);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[1], s));
@@ -1077,10 +1310,13 @@ int step_front(HostWorld* w, const StepK& K) {
     sf::scan_u32(D.colcnt, D.coloff, nullptr, nullptr, (uint32_t)K.K + 1, 72, w->scan_tmp, s);
     sf::scan_u32(D.ovflag, D.ovoff, nullptr, D.ctr + C_NMAN, 0, D.mcap, w->scan_tmp, s);
     hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
+    hipLaunchKernelGGL(k_ovf_count, dim3(gm), dim3(NT), 0, s, D, K.K);
+    hipLaunchKernelGGL(k_permute, dim3(gm), dim3(NT), 0, s, D);
     // per-slot contact lists in manifold order
     sf::scan_u32(D.blcnt, D.bloff, nullptr, nullptr, (uint32_t)D.nb + 1, (uint32_t)D.nb + 1, w->scan_tmp, s);
     hipLaunchKernelGGL(k_bl_fill, dim3(gm), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_bl_sort, dim3(sf_grid((uint64_t)D.nb + 1)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_bl_sort, dim3(sf_grid(((uint64_t)D.nb + 1) * 64, NT, 65536)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_bl_sort_big, dim3(1024), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
     SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
     SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
@@ -1107,21 +1343,31 @@ int launch_bucket(HostWorld* w, const StepK& K, int c) {
 int step_back(HostWorld* w, StepK K) {
     Dev& D = w->D;
     hipStream_t s = w->st;
-    const unsigned gb = sf_grid(D.nb), gm = sf_grid(D.mcap);
+    const unsigned gb = sf_grid(D.nb), gm = sf_grid(D.mcap), gw = sf_grid((uint64_t)D.nb * 64, NT, 65536);
     int ncol = (int)w->h_ctr[C_NCOL];
     int big = 0;
-    while (big < ncol && (int)w->h_col[big] >= TAIL_MIN) big++;
-    bool tail = big < ncol || w->h_col[K.K] > 0;
+    const int tail_min = (int)w->prm[SF_P_TAIL_MIN];
+    while (big < ncol && (int)w->h_col[big] >= tail_min) big++;
+    bool tail = big < ncol;
+    uint32_t novf = w->h_col[K.K];
     int sub = (int)w->prm[SF_P_SUBSTEPS]; if (sub < 1) sub = 1;
     StepK Kb = K; Kb.use_bias = 1;
     StepK Kr = K; Kr.use_bias = 0;
     for (int it = 0; it < sub; it++) {
-        hipLaunchKernelGGL(k_warm, dim3(gb), dim3(NT), 0, s, D, K);
+        hipLaunchKernelGGL(k_warm, dim3(gw), dim3(NT), 0, s, D, K);
         for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kb, c); if (r) return r; }
         if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kb, big);
+        if (novf) {
+            hipLaunchKernelGGL(k_ovf_solve, dim3(sf_grid(novf)), dim3(NT), 0, s, D, Kb);
+            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kb);
+        }
         hipLaunchKernelGGL(k_integrate, dim3(gb), dim3(NT), 0, s, D, K);
         for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kr, c); if (r) return r; }
         if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
+        if (novf) {
+            hipLaunchKernelGGL(k_ovf_solve, dim3(sf_grid(novf)), dim3(NT), 0, s, D, Kr);
+            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
+        }
     }
     if (w->timing) SF_HIP(hipEventRecord(w->ev[4], s));
     hipLaunchKernelGGL(k_writeback, dim3(gb), dim3(NT), 0, s, D);
@@ -1319,10 +1565,12 @@ int sf_world_manifolds(sf_world* w, int32_t* keys, float* data, int cap) {
     if (!keys || !data || cap <= 0) return n;
     int k = std::min(n, cap);
     std::vector<int4> hdr(k);
-    std::vector<float4> rows((size_t)MR_ROWS * k);
+    std::vector<uint32_t> pos(n);
+    std::vector<float4> rows((size_t)MR_ROWS * n);
     SF_HIP(hipMemcpy(hdr.data(), D.cur.hdr, sizeof(int4) * k, hipMemcpyDeviceToHost));
-    for (int r = 0; r < MR_ROWS; r++) SF_HIP(hipMemcpy(rows.data() + (size_t)r * k, D.cur.row + (size_t)r * D.mcap, sizeof(float4) * k, hipMemcpyDeviceToHost));
-    auto R = [&](int r, int m) { return rows[(size_t)r * k + m]; };
+    SF_HIP(hipMemcpy(pos.data(), D.cur.pos, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    for (int r = 0; r < MR_ROWS; r++) SF_HIP(hipMemcpy(rows.data() + (size_t)r * n, D.cur.row + (size_t)r * D.mcap, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    auto R = [&](int r, int m) { return rows[(size_t)r * n + pos[m]]; };
     for (int m = 0; m < k; m++) {
         int cnt = (int)R(MR_CB1, m).w;
         keys[3 * m] = hdr[m].x; keys[3 * m + 1] = hdr[m].y; keys[3 * m + 2] = cnt;

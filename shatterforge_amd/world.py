@@ -14,7 +14,7 @@ from ._native import lib, check, ptr
 # parameter indices (include/sf.h SF_P_*)
 P = dict(GX=0, GY=1, GZ=2, SUBSTEPS=3, HERTZ=4, DAMPING=5, MAXPUSH=6, FRICTION=7, SPEC=8, SLEEP_LIN=9, SLEEP_ANG=10,
          SLEEP_TIME=11, SLEEP_DIST=12, SLEEP_ROT=13, ALLOW_SLEEP=14, CELL=15, LEAK=16, ANCHOR=17, ANCHOR_PUSH=18,
-         ANCHOR_SLOP=19, COL_LIMIT=20, COL_ROUNDS=21, COL_SPREAD=22, RESERVED=23)
+         ANCHOR_SLOP=19, COL_LIMIT=20, COL_ROUNDS=21, COL_SPREAD=22, TAIL_MIN=23)
 NPARAM = 24
 # statistics (SF_S_*)
 S = dict(AWAKE=0, PAIRS=1, MANIFOLDS=2, POINTS=3, COLOURS=4, OVERFLOW=5, GROUND=6, WAKES=7, ENTRIES=8, CANDIDATES=9,
@@ -45,6 +45,7 @@ def default_params():
     p[P["COL_LIMIT"]] = 64
     p[P["COL_ROUNDS"]] = 32
     p[P["COL_SPREAD"]] = 4
+    p[P["TAIL_MIN"]] = 1024
     return p
 
 

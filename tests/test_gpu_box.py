@@ -54,6 +54,24 @@ def test_rubble_compound_steps_bit_exact_with_sleep(oracle):
     _assert_same_step(w, o, "final")
 
 
+@pytest.mark.parametrize("limit", [2, 5])
+def test_overflow_jacobi_bit_exact(oracle, limit):
+    # a tiny colour cap sends most contacts to the mass-splitting overflow pass; device == oracle bit for bit
+    print(f"case: rubble(400) colour limit {limit}", flush=True)
+    b, sb, s = scenes.rubble(400, box=(8.0, 6.0, 8.0), seed=21)
+    prm = params(substeps=4, col_limit=limit, tail_min=64)
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    for k in range(15):
+        w.step(DT)
+        o.step(DT)
+        gb, ob = w.bodies(), o.bodies()
+        assert np.array_equal(gb, ob), f"step {k}: max diff {np.abs(gb - ob).max()}"
+    st = w.stats()
+    assert st["overflow"] > 0 and st["overflow"] == o.stats()["overflow"]
+    _assert_same_step(w, o, "final")
+
+
 def test_first_frame_candidates_of_the_reference_pile():
     # docs/GPU.md:89: 5,871 candidates on the first frame of the 2,000-box pile
     b, sb, s = scenes.pile(2000)
