@@ -44,6 +44,7 @@ struct ob_world {
     int32_t* island;
     int32_t stats[8];
     uint8_t* wake_flag;
+    int32_t *bl_off, *bl_ent, *ovf_index;  /* device order: per-slot contact lists, overflow position per manifold */
 };
 
 /* parameter indices (same as SF_P_* in include/sf.h) */
@@ -70,7 +71,7 @@ void ob_destroy(ob_world* w) {
     if (!w) return;
     free(w->bd); free(w->sh); free(w->sv); free(w->sw); free(w->sdp); free(w->sth); free(w->sp0); free(w->sim);
     free(w->sii); free(w->sq); free(w->sidx); free(w->man); free(w->old); free(w->old_sorted); free(w->cand);
-    free(w->colour); free(w->island); free(w->wake_flag); free(w);
+    free(w->colour); free(w->island); free(w->wake_flag); free(w->bl_off); free(w->bl_ent); free(w->ovf_index); free(w);
 }
 void ob_set_params(ob_world* w, const float* prm, int n) { for (int i = 0; i < n && i < OB_NPARAM; i++) w->prm[i] = prm[i]; }
 int ob_nbody(const ob_world* w) { return w->nb; }
@@ -556,6 +557,70 @@ static void warm_start(ob_world* w) {
     }
 }
 
+/* Device-order sums: a body's contributions (one per contact, its contacts in manifold order) are added in chunks of
+ * 64 by a fixed halving tree (x[i] += x[i + h] for h = 32, 16, ..., 1, empty slots zero), the chunk totals in order.
+ * The device forms the same tree across the 64 lanes of a wavefront. */
+static V3 tree64(V3* x) {
+    for (int h = 32; h >= 1; h >>= 1) for (int i = 0; i < h; i++) x[i] = vadd(x[i], x[i + h]);
+    return x[0];
+}
+
+/* per-slot contact lists in manifold order (entries m * 2 + side) */
+static void build_lists(const ob_world* w, int32_t* off, int32_t* ent) {
+    size_t n1 = (size_t)w->nb + 1;
+    for (size_t s = 0; s <= n1; s++) off[s] = 0;
+    for (int m = 0; m < w->nman; m++) { if (w->man[m].sa) off[w->man[m].sa + 1]++; if (w->man[m].sb) off[w->man[m].sb + 1]++; }
+    for (size_t s = 0; s < n1; s++) off[s + 1] += off[s];
+    int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * n1);
+    for (size_t s = 0; s < n1; s++) fill[s] = off[s];
+    for (int m = 0; m < w->nman; m++) {
+        if (w->man[m].sa) ent[fill[w->man[m].sa]++] = 2 * m;
+        if (w->man[m].sb) ent[fill[w->man[m].sb]++] = 2 * m + 1;
+    }
+    free(fill);
+}
+
+/* the warm-start contribution of one contact to one side's body: the same vectors phys.rs:245-262 applies, the
+ * side's w terms summed in point order then the friction/twist term */
+static void warm_contrib(const Man* m, int side, V3* dv, V3* dw) {
+    float pn = 0.0f;
+    V3 acc = v3(0, 0, 0);
+    for (int k = 0; k < m->cnt; k++) {
+        const Cp* c = &m->cp[k];
+        pn += c->jn;
+        V3 t = side ? vmul(c->ibxn, c->jn) : vneg(vmul(c->iaxn, c->jn));
+        acc = k == 0 ? t : vadd(acc, t);
+    }
+    V3 p = vadd(vadd(vmul(m->n, pn), vmul(m->t1, m->jt1)), vmul(m->t2, m->jt2));
+    if (side) {
+        *dv = vmul(p, m->imb);
+        *dw = vadd(acc, vadd(vadd(vmul(m->ib1, m->jt1), vmul(m->ib2, m->jt2)), vmul(m->ibn, m->jtw)));
+    } else {
+        *dv = vneg(vmul(p, m->ima));
+        *dw = vadd(acc, vneg(vadd(vadd(vmul(m->ia1, m->jt1), vmul(m->ia2, m->jt2)), vmul(m->ian, m->jtw))));
+    }
+}
+
+/* device-order warm start: each moving body adds the tree sum of its contacts' contributions */
+static void warm_start_tree(ob_world* w, const int32_t* off, const int32_t* ent) {
+    V3 xv[64], xw[64];
+    for (int s = 1; s <= w->nb; s++) {
+        int o = off[s], n = off[s + 1] - off[s];
+        if (!n) continue;
+        V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            for (int i = 0; i < 64; i++) {
+                if (c0 + i < n) { int e = ent[o + c0 + i]; warm_contrib(&w->man[e >> 1], e & 1, &xv[i], &xw[i]); }
+                else { xv[i] = v3(0, 0, 0); xw[i] = v3(0, 0, 0); }
+            }
+            av = vadd(av, tree64(xv));
+            aw = vadd(aw, tree64(xw));
+        }
+        w->sv[s] = vadd(w->sv[s], av);
+        w->sw[s] = vadd(w->sw[s], aw);
+    }
+}
+
 typedef struct { int use_bias; float inv_h, bias_rate, mass_scale, imp_scale, max_push, anchor_push, anchor_slop; } SolveK;
 
 static inline float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
@@ -659,20 +724,31 @@ static void overflow_pass(ob_world* w, const SolveK* K, const int32_t* ovf, int 
         dv[4 * i + 0] = vsub(va, v0[sa]); dv[4 * i + 1] = vsub(wa, w0[sa]);
         dv[4 * i + 2] = vsub(vb, v0[sb]); dv[4 * i + 3] = vsub(wb, w0[sb]);
     }
-    V3* sv = (V3*)calloc(n1, sizeof(V3));
-    V3* sw = (V3*)calloc(n1, sizeof(V3));
-    for (int i = 0; i < novf; i++) {
-        const Man* m = &w->man[ovf[i]];
-        if (m->sa) { sv[m->sa] = vadd(sv[m->sa], dv[4 * i + 0]); sw[m->sa] = vadd(sw[m->sa], dv[4 * i + 1]); }
-        if (m->sb) { sv[m->sb] = vadd(sv[m->sb], dv[4 * i + 2]); sw[m->sb] = vadd(sw[m->sb], dv[4 * i + 3]); }
-    }
+    /* each body: the tree sum (over its full contact list, zero for contacts not in the overflow) of its copies'
+     * changes, averaged */
+    V3 xv[64], xw[64];
     for (size_t s = 1; s < n1; s++) {
         if (!ocnt[s]) continue;
+        int o = w->bl_off[s], n = w->bl_off[s + 1] - w->bl_off[s];
+        V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            for (int i = 0; i < 64; i++) {
+                xv[i] = v3(0, 0, 0); xw[i] = v3(0, 0, 0);
+                if (c0 + i >= n) continue;
+                int e = w->bl_ent[o + c0 + i], mi = e >> 1;
+                int oi = w->ovf_index[mi];
+                if (oi < 0) continue;
+                xv[i] = dv[4 * oi + ((e & 1) ? 2 : 0)];
+                xw[i] = dv[4 * oi + ((e & 1) ? 3 : 1)];
+            }
+            av = vadd(av, tree64(xv));
+            aw = vadd(aw, tree64(xw));
+        }
         float inv = 1.0f / (float)ocnt[s];
-        w->sv[s] = vadd(v0[s], vmul(sv[s], inv));
-        w->sw[s] = vadd(w0[s], vmul(sw[s], inv));
+        w->sv[s] = vadd(v0[s], vmul(av, inv));
+        w->sw[s] = vadd(w0[s], vmul(aw, inv));
     }
-    free(v0); free(w0); free(sv); free(sw);
+    free(v0); free(w0);
 }
 
 static void solve_pass(ob_world* w, const SolveK* K, const int32_t* order, int ncol_order, const int32_t* ovf, int novf,
@@ -848,6 +924,12 @@ void ob_step(ob_world* w, float dt) {
             if (w->man[mi].sb) ocnt[w->man[mi].sb]++;
         }
         dv = (V3*)malloc(sizeof(V3) * 4 * (size_t)(novf ? novf : 1));
+        w->bl_off = (int32_t*)realloc(w->bl_off, sizeof(int32_t) * ((size_t)w->nb + 2));
+        w->bl_ent = (int32_t*)realloc(w->bl_ent, sizeof(int32_t) * 2 * (size_t)(w->nman ? w->nman : 1));
+        w->ovf_index = (int32_t*)realloc(w->ovf_index, sizeof(int32_t) * (size_t)(w->nman ? w->nman : 1));
+        build_lists(w, w->bl_off, w->bl_ent);
+        for (int mi = 0; mi < w->nman; mi++) w->ovf_index[mi] = -1;
+        for (int i = 0; i < novf; i++) w->ovf_index[ovf[i]] = i;
     } else { w->stats[4] = 0; w->stats[5] = 0; }
     /* ---- sub-steps (phys.rs:214-230) */
     V3 g = v3(P[P_GX], P[P_GY], P[P_GZ]);
@@ -861,7 +943,7 @@ void ob_step(ob_world* w, float dt) {
             V3 ww = w->sw[si]; float wl = vlen2(ww);
             if (wl > wmax * wmax) w->sw[si] = vmul(ww, wmax / om_sqrt(wl));
         }
-        warm_start(w);
+        if (coloured) warm_start_tree(w, w->bl_off, w->bl_ent); else warm_start(w);
         solve_pass(w, &kb, order, ncol_order, ovf, novf, ocnt, dv);
         for (int i = 0; i < n; i++) {
             int si = w->sidx[i]; if (!si) continue;

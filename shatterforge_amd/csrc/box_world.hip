@@ -83,6 +83,7 @@ struct Dev {
     uint32_t *hit, *hoff;
     ManBuf cur, old;
     float4* srow;  // rows as built, by canonical index (MR_ROWS x mcap), before the move into colour order
+    float4* wrec;  // warm-start records by position: 16 float4 (two sides of 8)
     // colouring
     int32_t *mcol, *mcand;
     unsigned long long *mpri, *bmask, *claim;
@@ -90,7 +91,7 @@ struct Dev {
     int32_t* order;
     float4* ovdv;  // overflow velocity changes: 4 per overflow manifold (v, w of side a; v, w of side b)
     // per-slot manifold lists (m * 2 + side), in manifold order
-    uint32_t *blcnt, *bloff, *blfill, *blist;
+    uint32_t *blcnt, *bloff, *blfill, *blist, *blpos;
     // islands
     int32_t* parent;
     uint32_t* mint;
@@ -696,6 +697,32 @@ __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
     }
 }
 
+// The warm-start record of a manifold (at position p), one 128-byte half per side: the vectors the warm start adds
+// to that side's body, w += t_k (k < count) then v += tv, w += tf (signs by side), formed from the accumulated
+// impulses with the true masses, exactly as phys.rs:245-262 forms them.
+__device__ __forceinline__ void write_warm(const Dev& D, uint32_t p, int cnt, float4 n4, float4 t14, float4 t24,
+                                           float j1, float j2, float jt, const float* jn, const float4* r2, const float4* r3,
+                                           float4 ia14, float4 ia24, float4 ian4, float4 ib14, float4 ib24, float4 ibn4,
+                                           float ima, float imb) {
+    float pn = 0.0f;
+    V3 ta[4], tb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k < cnt) { pn += jn[k]; ta[k] = vmul(v3f4(r2[k]), jn[k]); tb[k] = vmul(v3f4(r3[k]), jn[k]); }
+        else { ta[k] = v3(0, 0, 0); tb[k] = v3(0, 0, 0); }
+    }
+    V3 pp = vadd(vadd(vmul(v3f4(n4), pn), vmul(v3f4(t14), j1)), vmul(v3f4(t24), j2));
+    V3 tfa = vadd(vadd(vmul(v3f4(ia14), j1), vmul(v3f4(ia24), j2)), vmul(v3f4(ian4), jt));
+    V3 tfb = vadd(vadd(vmul(v3f4(ib14), j1), vmul(v3f4(ib24), j2)), vmul(v3f4(ibn4), jt));
+    float4* w = D.wrec + (size_t)p * 16;
+    float c = (float)cnt;
+    // side a subtracts: stored negated
+    w[0] = f4(vneg(ta[0]), c); w[1] = f4(vneg(ta[1]), 0.0f); w[2] = f4(vneg(ta[2]), 0.0f); w[3] = f4(vneg(ta[3]), 0.0f);
+    w[4] = f4(vneg(vmul(pp, ima)), 0.0f); w[5] = f4(vneg(tfa), 0.0f);
+    w[8] = f4(tb[0], c); w[9] = f4(tb[1], 0.0f); w[10] = f4(tb[2], 0.0f); w[11] = f4(tb[3], 0.0f);
+    w[12] = f4(vmul(pp, imb), 0.0f); w[13] = f4(tfb, 0.0f);
+}
+
 // move the built rows into colour order: the manifold at position p of the colour-ordered list
 __global__ void __launch_bounds__(NT) k_permute(Dev D) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
@@ -705,6 +732,17 @@ __global__ void __launch_bounds__(NT) k_permute(Dev D) {
         D.cur.hdrp[p] = D.cur.hdr[m];
 #pragma unroll 4
         for (int r = 0; r < MR_ROWS; r++) MROW(D, D.cur, r, p) = SROW(D, r, m);
+        // the first sub-step's warm start, from the impulses carried over from the last step
+        float4 r2[4], r3[4];
+        float jn[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            r2[k] = SROW(D, MR_P0 + MR_PER_PT * k + 2, m); r3[k] = SROW(D, MR_P0 + MR_PER_PT * k + 3, m); jn[k] = r2[k].w;
+        }
+        float4 j4 = SROW(D, MR_J, m);
+        write_warm(D, p, (int)SROW(D, MR_CB1, m).w, SROW(D, MR_N, m), SROW(D, MR_T1, m), SROW(D, MR_T2, m), j4.x, j4.y, j4.z,
+                   jn, r2, r3, SROW(D, MR_IA1, m), SROW(D, MR_IA2, m), SROW(D, MR_IAN, m), SROW(D, MR_IB1, m),
+                   SROW(D, MR_IB2, m), SROW(D, MR_IBN, m), SROW(D, MR_CA1, m).w, SROW(D, MR_CA2, m).w);
     }
 }
 
@@ -778,6 +816,16 @@ __global__ void __launch_bounds__(NT) k_bl_sort_big(Dev D) {
     }
 }
 
+// each list entry's warm-start record (position x 2 + side), once the positions are known
+__global__ void __launch_bounds__(NT) k_bl_pos(Dev D) {
+    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(e, 2 * nm) {
+        if (e >= D.bloff[D.nb] + D.blcnt[D.nb]) continue;
+        uint32_t ent = D.blist[e];
+        D.blpos[e] = D.cur.pos[ent >> 1] * 2u + (ent & 1u);
+    }
+}
+
 // ---------------------------------------------------------------- solver
 // gravity, spin clamp, then this body's warm start gathered over its contacts in manifold order: the same
 // sequence of f32 updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262)
@@ -788,65 +836,50 @@ __device__ __forceinline__ V3 rdl3(V3 a, int l) { return v3(rdl(a.x, l), rdl(a.y
 // updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262). One wavefront per body: the lanes
 // load up to 64 contacts at once and form their update vectors; every lane then applies them in order (values read
 // across lanes), so the loads overlap and the order of the additions is the reference's.
+// Fixed halving tree across a wavefront (x_i += x_{i+h}, h = 32 ... 1); lane 0's total is returned to every lane.
+// The oracle sums in exactly this tree.
+__device__ __forceinline__ V3 wave_tree(V3 x) {
+#pragma unroll
+    for (int h = 32; h >= 1; h >>= 1) x = vadd(x, v3(__shfl_down(x.x, h, 64), __shfl_down(x.y, h, 64), __shfl_down(x.z, h, 64)));
+    return rdl3(x, 0);
+}
+
+// Gravity, spin clamp, then the warm start (phys.rs:217-222, :245-262): one wavefront per body, each lane one contact
+// (the body's contacts in manifold order, 64 at a time); a contact's contribution is its record's terms summed in
+// point order, and the body adds the tree sum of its contacts' contributions.
 __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    const ManBuf& C = D.cur;
+    const V3 gh = vmul(v3(K.gx, K.gy, K.gz), K.h);
     for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
         int s = D.bslot[i];
         if (!s) continue;
         float4 v4 = D.sv[s];
-        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
-        v = vadd(v, vmul(v3(K.gx, K.gy, K.gz), K.h));
+        V3 v = vadd(v3f4(v4), gh), w = v3f4(D.sw[s]);
         float wl = vlen2(w);
         if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
         uint32_t o = D.bloff[s], n = D.blcnt[s];
+        V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
         for (uint32_t e0 = 0; e0 < n; e0 += 64) {
             uint32_t e = e0 + lane;
-            V3 t[4], tv = v3(0, 0, 0), tf = v3(0, 0, 0);
-            int cnt = 0, side = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) t[k] = v3(0, 0, 0);
+            V3 dv = v3(0, 0, 0), dw = v3(0, 0, 0);
             if (e < n) {
-                uint32_t ent = D.blist[o + e];
-                uint32_t m = C.pos[ent >> 1];
-                side = (int)(ent & 1u);
-                cnt = (int)MROW(D, C, MR_CB1, m).w;
-                float4 rn = MROW(D, C, MR_N, m), rt1 = MROW(D, C, MR_T1, m), rt2 = MROW(D, C, MR_T2, m), j4 = MROW(D, C, MR_J, m);
-                float4 im4 = side ? MROW(D, C, MR_CA2, m) : MROW(D, C, MR_CA1, m);
-                float4 f1 = side ? MROW(D, C, MR_IB1, m) : MROW(D, C, MR_IA1, m);
-                float4 f2 = side ? MROW(D, C, MR_IB2, m) : MROW(D, C, MR_IA2, m);
-                float4 fn = side ? MROW(D, C, MR_IBN, m) : MROW(D, C, MR_IAN, m);
-                float pn = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    float4 r2 = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
-                    float4 r3 = MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m);
-                    if (k < cnt) {
-                        pn += r2.w;
-                        t[k] = vmul(v3f4(side ? r3 : r2), r2.w);
-                    }
-                }
-                V3 p = vadd(vadd(vmul(v3f4(rn), pn), vmul(v3f4(rt1), j4.x)), vmul(v3f4(rt2), j4.y));
-                tv = vmul(p, im4.w);
-                tf = vadd(vadd(vmul(v3f4(f1), j4.x), vmul(v3f4(f2), j4.y)), vmul(v3f4(fn), j4.z));
+                const float4* wr = D.wrec + (size_t)D.blpos[o + e] * 8;
+                float4 w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4], w5 = wr[5];
+                int c = (int)w0.w;
+                dw = v3f4(w0);
+                if (c > 1) dw = vadd(dw, v3f4(w1));
+                if (c > 2) dw = vadd(dw, v3f4(w2));
+                if (c > 3) dw = vadd(dw, v3f4(w3));
+                dw = vadd(dw, v3f4(w5));
+                dv = v3f4(w4);
             }
-            int nl = (int)min(64u, n - e0);
-            for (int l = 0; l < nl; l++) {
-                int c = __builtin_amdgcn_readlane(cnt, l);
-                bool sb = __builtin_amdgcn_readlane(side, l) != 0;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    if (k < c) { V3 tk = rdl3(t[k], l); w = sb ? vadd(w, tk) : vsub(w, tk); }
-                }
-                V3 a = rdl3(tv, l), f = rdl3(tf, l);
-                v = sb ? vadd(v, a) : vsub(v, a);
-                w = sb ? vadd(w, f) : vsub(w, f);
-            }
+            av = vadd(av, wave_tree(dv));
+            aw = vadd(aw, wave_tree(dw));
         }
         if (lane == 0) {
-            D.sv[s] = f4(v, v4.w);
-            D.sw[s] = f4(w, 0.0f);
+            D.sv[s] = f4(vadd(v, av), v4.w);
+            D.sw[s] = f4(vadd(w, aw), 0.0f);
         }
     }
 }
@@ -950,6 +983,9 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     for (int k = 0; k < 4; k++) if (k < cnt) reinterpret_cast<float*>(&MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m))[3] = njs[k];
     MROW(D, C, MR_J, m) = make_float4(j1, j2, jt, j4.w);
     if (unstick) reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;
+    // the relax pass ends the sub-step: its impulses are the next sub-step's warm start
+    if (!K.use_bias)
+        write_warm(D, m, cnt, n4, t14, t24, j1, j2, jt, njs, r2, r3, ia14, ia24, ian4, ib14, ib24, ibn4, ca14.w, ca24.w);
 }
 
 // one manifold of the colour-ordered solve (p: its position in colour order)
@@ -997,8 +1033,8 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
         dv[2] = f4(vsub(vb, vb0), 0.0f); dv[3] = f4(vsub(wb, wb0), 0.0f);
     }
 }
-// each body takes the average of its copies: the changes summed in manifold order (one wavefront per body, the
-// lanes load up to 64 contacts at once and the sum runs over them in order)
+// each body takes the average of its copies: the tree sum of the changes over its contacts (zero for contacts not
+// in the overflow), one wavefront per body
 __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
@@ -1007,34 +1043,28 @@ __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
         if (!s) continue;
         uint32_t cnt = D.ovcnt[s];
         if (!cnt) continue;
-        V3 sv = v3(0, 0, 0), sw = v3(0, 0, 0);
+        V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
         uint32_t o = D.bloff[s], n = D.blcnt[s];
         for (uint32_t e0 = 0; e0 < n; e0 += 64) {
             uint32_t e = e0 + lane;
-            int use = 0;
             V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0);
             if (e < n) {
                 uint32_t ent = D.blist[o + e];
                 uint32_t m = ent >> 1;
                 if (D.mcol[m] == K.K) {
-                    use = 1;
                     const float4* dv = D.ovdv + 4 * (size_t)D.ovoff[m] + ((ent & 1u) ? 2 : 0);
                     dvv = v3f4(dv[0]);
                     dww = v3f4(dv[1]);
                 }
             }
-            int nl = (int)min(64u, n - e0);
-            for (int l = 0; l < nl; l++) {
-                if (!__builtin_amdgcn_readlane(use, l)) continue;
-                sv = vadd(sv, rdl3(dvv, l));
-                sw = vadd(sw, rdl3(dww, l));
-            }
+            av = vadd(av, wave_tree(dvv));
+            aw = vadd(aw, wave_tree(dww));
         }
         if (lane == 0) {
             float inv = 1.0f / (float)cnt;
             float4 v4 = D.sv[s];
-            D.sv[s] = f4(vadd(v3f4(v4), vmul(sv, inv)), v4.w);
-            D.sw[s] = f4(vadd(v3f4(D.sw[s]), vmul(sw, inv)), 0.0f);
+            D.sv[s] = f4(vadd(v3f4(v4), vmul(av, inv)), v4.w);
+            D.sw[s] = f4(vadd(v3f4(D.sw[s]), vmul(aw, inv)), 0.0f);
         }
     }
 }
@@ -1073,14 +1103,59 @@ __global__ void __launch_bounds__(NT) k_writeback(Dev D) {
     }
 }
 
+// root of x with path halving (every parent is a smaller index of the same island, so any value stored here is
+// still an ancestor: a racing store can undo a shortcut, never a union)
 __device__ __forceinline__ int uf_find(int32_t* par, int x) {
     while (true) {
         int p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
         if (p == x) return x;
-        x = p;
+        int g = __atomic_load_n(&par[p], __ATOMIC_RELAXED);
+        if (g != p) __atomic_store_n(&par[x], g, __ATOMIC_RELAXED);
+        x = g;
     }
 }
 // hook the larger root under the smaller: every island's root ends as its minimum body index
+// A first forest without atomics: every moving body points at its smallest-index neighbour across a contact between
+// two moving bodies (a wavefront per body, min over its contacts); the hooking pass then finds most pairs joined.
+__global__ void __launch_bounds__(NT) k_uf_init(Dev D) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
+        int s = D.bslot[i];
+        if (!s) continue;
+        uint32_t o = D.bloff[s], n = D.blcnt[s];
+        int best = (int)i;
+        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+            uint32_t e = e0 + lane;
+            int nbr = (int)i;
+            if (e < n) {
+                uint32_t ent = D.blist[o + e];
+                int4 hd = D.cur.hdr[ent >> 1];
+                int other = (ent & 1u) ? hd.z : hd.w;
+                if (hd.z && hd.w) nbr = other - 1;
+            }
+#pragma unroll
+            for (int h = 32; h >= 1; h >>= 1) nbr = min(nbr, __shfl_xor(nbr, h, 64));
+            best = min(best, nbr);
+        }
+        if (lane == 0) D.parent[i] = best;
+    }
+}
+
+// pointer jumping: every body's parent replaced by its grandparent until it reaches a root (each store is an
+// ancestor, so the lanes may run it concurrently)
+__global__ void __launch_bounds__(NT) k_uf_jump(Dev D) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        int p = D.parent[i];
+        for (int it = 0; it < 64; it++) {
+            int g = __atomic_load_n(&D.parent[p], __ATOMIC_RELAXED);
+            if (g == p) break;
+            p = g;
+            __atomic_store_n(&D.parent[i], p, __ATOMIC_RELAXED);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(NT) k_uf_hook(Dev D) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(m, nm) {
@@ -1214,12 +1289,13 @@ int world_alloc(HostWorld* w) {
         r |= w->alloc(M->pos, D.mcap); r |= w->alloc(M->hdrp, D.mcap);
     }
     r |= w->alloc(D.srow, (size_t)MR_ROWS * D.mcap);
+    r |= w->alloc(D.wrec, (size_t)16 * D.mcap);
     r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
     r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap); r |= w->alloc(D.order, D.mcap);
     r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovdv, 4 * (size_t)D.mcap);
-    r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap);
+    r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blpos, 2 * (size_t)D.mcap);
     r |= w->alloc(D.parent, nb); r |= w->alloc(D.mint, nb); r |= w->alloc(D.ctr, C_NCTR);
     uint32_t big = std::max<uint32_t>(std::max<uint32_t>(D.tcap, D.ccap), (uint32_t)n1);
     r |= w->alloc(w->scan_tmp, sf::scan_scratch_words(big));
@@ -1317,6 +1393,7 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_bl_fill, dim3(gm), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort, dim3(sf_grid(((uint64_t)D.nb + 1) * 64, NT, 65536)), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort_big, dim3(1024), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_bl_pos, dim3(sf_grid(2ull * D.mcap)), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
     SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
     SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
@@ -1371,6 +1448,8 @@ int step_back(HostWorld* w, StepK K) {
     }
     if (w->timing) SF_HIP(hipEventRecord(w->ev[4], s));
     hipLaunchKernelGGL(k_writeback, dim3(gb), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_uf_init, dim3(gw), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_uf_jump, dim3(gb), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_uf_hook, dim3(gm), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_uf_flatten_sleep, dim3(gb), dim3(NT), 0, s, D, K);
     hipLaunchKernelGGL(k_sleep_apply, dim3(gb), dim3(NT), 0, s, D, K);
