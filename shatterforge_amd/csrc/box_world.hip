@@ -94,7 +94,7 @@ struct Dev {
     uint32_t *blcnt, *bloff, *blfill, *blist, *blpos;
     // islands
     int32_t* parent;
-    uint32_t* mint;
+    uint32_t* sst;     // sleep state bits per body (SS_*)
     uint32_t* ctr;
 };
 
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(NT) k_prep_bodies(Dev D) {
         D.bslot[i] = slot;
         D.wake[i] = 0u;
         D.parent[i] = (int32_t)i;
-        D.mint[i] = 0x7F800000u;
+        D.sst[i] = 0u;
         if (awake) {
             float4 p = D.bp[i];
             D.sv[slot] = make_float4(D.bv[i].x, D.bv[i].y, D.bv[i].z, p.w);
@@ -1103,28 +1103,61 @@ __global__ void __launch_bounds__(NT) k_writeback(Dev D) {
     }
 }
 
-// root of x with path halving (every parent is a smaller index of the same island, so any value stored here is
-// still an ancestor: a racing store can undo a shortcut, never a union)
+// ---------------------------------------------------------------- islands and sleep
+// Union-find over the contact graph of moving bodies; an island's root is its smallest body index. Parents only move
+// to smaller indices of the same island, so whatever a racing store leaves behind is still an ancestor. Loads and
+// stores are device-scope relaxed atomics: coherent across the XCDs' L2s without the cost of system scope.
+__device__ __forceinline__ int par_load(const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void par_store(int32_t* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// root of x with path halving
 __device__ __forceinline__ int uf_find(int32_t* par, int x) {
     while (true) {
-        int p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+        int p = par_load(&par[x]);
         if (p == x) return x;
-        int g = __atomic_load_n(&par[p], __ATOMIC_RELAXED);
-        if (g != p) __atomic_store_n(&par[x], g, __ATOMIC_RELAXED);
+        int g = par_load(&par[p]);
+        if (g != p) par_store(&par[x], g);
         x = g;
     }
 }
-// hook the larger root under the smaller: every island's root ends as its minimum body index
-// A first forest without atomics: every moving body points at its smallest-index neighbour across a contact between
-// two moving bodies (a wavefront per body, min over its contacts); the hooking pass then finds most pairs joined.
-__global__ void __launch_bounds__(NT) k_uf_init(Dev D) {
+
+// Sleep state of a body (sst): an island sleeps when every member's rest timer has reached the sleep time. Bodies
+// whose timer has not are "restless"; the others are "sleepy". An island with a restless member is connected, so each
+// of its sleepy components (under the contacts between two sleepy bodies) has a contact with a restless body, and an
+// island without one is a single sleepy component with no such contact. The step therefore unites only sleepy
+// bodies and taints the components that touch a restless body: the same decisions as a minimum of the timers over
+// full islands, without the union-find work (and its hot-root atomics) over islands that cannot sleep.
+enum : uint32_t { SS_SLEEPY = 1u, SS_RESTLESS_NBR = 2u, SS_TAINT = 4u };
+
+// rest timer: still = within a small distance and angle of the pose where the timer last restarted
+__global__ void __launch_bounds__(NT) k_sleep_timer(Dev D, StepK K) {
+    GRID_STRIDE(i, (uint32_t)D.nb) {
+        if (!(D.bflag[i] & 1)) continue;
+        Qt q = qf4(D.bq[i]);
+        Qt dq = qmul(q, qconj(qf4(D.bsq[i])));
+        float ang = 2.0f * sf_sqrt(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z);
+        V3 p = v3f4(D.bp[i]);
+        float4 ii = D.bii[i];
+        bool near = vlen2(vsub(p, v3f4(D.bsp[i]))) < K.sleep_dist2 && ang < K.sleep_rot;
+        if (near && vlen2(v3f4(D.bv[i])) < 100.0f * K.sleep_lin2 && vlen2(v3f4(D.bw[i])) < 100.0f * K.sleep_ang2) ii.w += K.dt;
+        else { ii.w = 0.0f; D.bsp[i] = D.bp[i]; D.bsq[i] = D.bq[i]; }
+        D.bii[i] = ii;
+        if (K.allow_sleep && ii.w >= K.sleep_time) D.sst[i] = SS_SLEEPY;
+    }
+}
+
+// A first forest without atomics: each body points at its smallest-index neighbour across a contact between two
+// moving bodies (a wavefront per body, min over its contact list). With all == 0 only sleepy bodies and sleepy
+// neighbours take part, and a sleepy body with a restless neighbour is marked.
+__global__ void __launch_bounds__(NT) k_uf_init(Dev D, int all) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
         int s = D.bslot[i];
         if (!s) continue;
+        if (!all && !(D.sst[i] & SS_SLEEPY)) continue;
         uint32_t o = D.bloff[s], n = D.blcnt[s];
         int best = (int)i;
+        bool restless = false;
         for (uint32_t e0 = 0; e0 < n; e0 += 64) {
             uint32_t e = e0 + lane;
             int nbr = (int)i;
@@ -1132,36 +1165,46 @@ __global__ void __launch_bounds__(NT) k_uf_init(Dev D) {
                 uint32_t ent = D.blist[o + e];
                 int4 hd = D.cur.hdr[ent >> 1];
                 int other = (ent & 1u) ? hd.z : hd.w;
-                if (hd.z && hd.w) nbr = other - 1;
+                if (hd.z && hd.w) {
+                    if (all || (D.sst[other - 1] & SS_SLEEPY)) nbr = other - 1;
+                    else restless = true;
+                }
             }
 #pragma unroll
             for (int h = 32; h >= 1; h >>= 1) nbr = min(nbr, __shfl_xor(nbr, h, 64));
             best = min(best, nbr);
         }
-        if (lane == 0) D.parent[i] = best;
+        bool any_restless = __ballot(restless) != 0ull;
+        if (lane == 0) {
+            D.parent[i] = best;
+            if (any_restless) D.sst[i] |= SS_RESTLESS_NBR;
+        }
     }
 }
 
 // pointer jumping: every body's parent replaced by its grandparent until it reaches a root (each store is an
 // ancestor, so the lanes may run it concurrently)
-__global__ void __launch_bounds__(NT) k_uf_jump(Dev D) {
+__global__ void __launch_bounds__(NT) k_uf_jump(Dev D, int all) {
     GRID_STRIDE(i, (uint32_t)D.nb) {
-        int p = D.parent[i];
+        if (!all && !(D.sst[i] & SS_SLEEPY)) continue;
+        int p = par_load(&D.parent[i]);
         for (int it = 0; it < 64; it++) {
-            int g = __atomic_load_n(&D.parent[p], __ATOMIC_RELAXED);
+            int g = par_load(&D.parent[p]);
             if (g == p) break;
             p = g;
-            __atomic_store_n(&D.parent[i], p, __ATOMIC_RELAXED);
+            par_store(&D.parent[i], p);
         }
     }
 }
 
-__global__ void __launch_bounds__(NT) k_uf_hook(Dev D) {
+// hook the larger root under the smaller, for every contact between two moving (all == 0: two sleepy) bodies
+__global__ void __launch_bounds__(NT) k_uf_hook(Dev D, int all) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(m, nm) {
         int4 hd = D.cur.hdr[m];
         if (hd.z == 0 || hd.w == 0) continue;
         int a = hd.z - 1, b = hd.w - 1;
+        if (!all && !(D.sst[a] & D.sst[b] & SS_SLEEPY)) continue;
         while (true) {
             a = uf_find(D.parent, a);
             b = uf_find(D.parent, b);
@@ -1174,30 +1217,25 @@ __global__ void __launch_bounds__(NT) k_uf_hook(Dev D) {
         }
     }
 }
-__global__ void __launch_bounds__(NT) k_uf_flatten_sleep(Dev D, StepK K) {
+
+// every body's parent made its root (all == 0: sleepy bodies only, each carrying its restless mark to the root)
+__global__ void __launch_bounds__(NT) k_uf_flatten(Dev D, int all) {
     GRID_STRIDE(i, (uint32_t)D.nb) {
+        uint32_t ss = all ? 0u : D.sst[i];
+        if (!all && !(ss & SS_SLEEPY)) continue;
         int r = uf_find(D.parent, (int)i);
         D.parent[i] = r;
-        if (!(D.bflag[i] & 1)) continue;
-        // rest timer: still = within a small distance and angle of the pose where the timer last restarted
-        Qt q = qf4(D.bq[i]);
-        Qt dq = qmul(q, qconj(qf4(D.bsq[i])));
-        float ang = 2.0f * sf_sqrt(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z);
-        V3 p = v3f4(D.bp[i]);
-        float4 ii = D.bii[i];
-        bool near = vlen2(vsub(p, v3f4(D.bsp[i]))) < K.sleep_dist2 && ang < K.sleep_rot;
-        if (near && vlen2(v3f4(D.bv[i])) < 100.0f * K.sleep_lin2 && vlen2(v3f4(D.bw[i])) < 100.0f * K.sleep_ang2) ii.w += K.dt;
-        else { ii.w = 0.0f; D.bsp[i] = D.bp[i]; D.bsq[i] = D.bq[i]; }
-        D.bii[i] = ii;
-        atomicMin(&D.mint[r], __float_as_uint(ii.w));
+        if ((ss & SS_RESTLESS_NBR) && !(__hip_atomic_load(&D.sst[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & SS_TAINT))
+            atomicOr(&D.sst[r], SS_TAINT);
     }
 }
-__global__ void __launch_bounds__(NT) k_sleep_apply(Dev D, StepK K) {
-    if (!K.allow_sleep) return;
+
+__global__ void __launch_bounds__(NT) k_uf_reset(Dev D) { GRID_STRIDE(i, (uint32_t)D.nb) D.parent[i] = (int32_t)i; }
+
+__global__ void __launch_bounds__(NT) k_sleep_apply(Dev D) {
     GRID_STRIDE(i, (uint32_t)D.nb) {
-        if (!(D.bflag[i] & 1)) continue;
-        int r = D.parent[i];
-        if (__uint_as_float(D.mint[r]) >= K.sleep_time) {
+        if (!(D.sst[i] & SS_SLEEPY)) continue;
+        if (!(D.sst[D.parent[i]] & SS_TAINT)) {
             D.bflag[i] &= ~1;
             D.bv[i] = make_float4(0, 0, 0, 0);
             D.bw[i] = make_float4(0, 0, 0, 0);
@@ -1211,6 +1249,7 @@ __global__ void k_fill_u64(unsigned long long* p, size_t n) { for (size_t i = bl
 // ---------------------------------------------------------------- host
 struct HostWorld {
     int dev = 0;
+    long steps = 0;   // steps since the last load
     hipStream_t st = nullptr;
     int body_cap = 0, shape_cap = 0;
     float prm[SF_NPARAM];
@@ -1296,7 +1335,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap); r |= w->alloc(D.order, D.mcap);
     r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovdv, 4 * (size_t)D.mcap);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blpos, 2 * (size_t)D.mcap);
-    r |= w->alloc(D.parent, nb); r |= w->alloc(D.mint, nb); r |= w->alloc(D.ctr, C_NCTR);
+    r |= w->alloc(D.parent, nb); r |= w->alloc(D.sst, nb); r |= w->alloc(D.ctr, C_NCTR);
     uint32_t big = std::max<uint32_t>(std::max<uint32_t>(D.tcap, D.ccap), (uint32_t)n1);
     r |= w->alloc(w->scan_tmp, sf::scan_scratch_words(big));
     if (r) return r;
@@ -1448,11 +1487,14 @@ int step_back(HostWorld* w, StepK K) {
     }
     if (w->timing) SF_HIP(hipEventRecord(w->ev[4], s));
     hipLaunchKernelGGL(k_writeback, dim3(gb), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_uf_init, dim3(gw), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_uf_jump, dim3(gb), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_uf_hook, dim3(gm), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_uf_flatten_sleep, dim3(gb), dim3(NT), 0, s, D, K);
-    hipLaunchKernelGGL(k_sleep_apply, dim3(gb), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_sleep_timer, dim3(gb), dim3(NT), 0, s, D, K);
+    if (K.allow_sleep) {
+        hipLaunchKernelGGL(k_uf_init, dim3(gw), dim3(NT), 0, s, D, 0);
+        hipLaunchKernelGGL(k_uf_jump, dim3(gb), dim3(NT), 0, s, D, 0);
+        hipLaunchKernelGGL(k_uf_hook, dim3(gm), dim3(NT), 0, s, D, 0);
+        hipLaunchKernelGGL(k_uf_flatten, dim3(gb), dim3(NT), 0, s, D, 0);
+        hipLaunchKernelGGL(k_sleep_apply, dim3(gb), dim3(NT), 0, s, D);
+    }
     if (w->timing) SF_HIP(hipEventRecord(w->ev[5], s));
     SF_HIP(hipGetLastError());
     return 0;
@@ -1484,6 +1526,7 @@ int world_step(HostWorld* w, float dt) {
     r = step_back(w, K);
     if (r) return r;
     SF_HIP(hipStreamSynchronize(w->st));
+    w->steps++;
     finish_stats(w);
     if (w->timing) {
         for (int k = 0; k < 5; k++) { float ms = 0; (void)hipEventElapsedTime(&ms, w->ev[k], w->ev[k + 1]); w->stage_ms[k] = ms; }
@@ -1502,6 +1545,7 @@ int world_load(HostWorld* w, int nb, const float* bodies, int ns, const int32_t*
     if (nb < 0 || ns < 0 || nb > w->body_cap || ns > w->shape_cap) { sf::set_error(SF_E_CAP, "more bodies or shapes than the world's capacity"); return SF_E_CAP; }
     if ((nb && !bodies) || (ns && (!shb || !shapes))) { sf::set_error(SF_E_ARG, "null input"); return SF_E_ARG; }
     if (!finite_all(bodies, (size_t)nb * SF_BODY_F) || !finite_all(shapes, (size_t)ns * SF_SHAPE_F)) { sf::set_error(SF_E_ARG, "NaN or inf in the input"); return SF_E_ARG; }
+    w->steps = 0;
     for (int s = 0; s < ns; s++) {
         if (shb[s] < 0 || shb[s] >= nb || (s > 0 && shb[s] < shb[s - 1])) { sf::set_error(SF_E_ARG, "shapes must name bodies in order"); return SF_E_ARG; }
     }
@@ -1674,9 +1718,28 @@ int sf_world_colours(sf_world* w, int32_t* out, int cap) {
     if (out && cap > 0) SF_HIP(hipMemcpy(out, h->D.mcol, sizeof(int32_t) * std::min(n, cap), hipMemcpyDeviceToHost));
     return n;
 }
+// Island labels of the last step (smallest body index of each island of moving bodies in contact, a body's own
+// index otherwise). The step itself unites only bodies that could fall asleep; the full labels are computed here,
+// from the last step's contacts, on demand.
 int sf_world_islands(sf_world* w, int32_t* out) {
     if (!w || !out) return SF_E_ARG;
-    SF_HIP(hipMemcpy(out, w->h.D.parent, sizeof(int32_t) * w->h.D.nb, hipMemcpyDeviceToHost));
+    HostWorld* h = &w->h;
+    Dev& D = h->D;
+    if (h->steps == 0) {
+        for (int i = 0; i < D.nb; i++) out[i] = i;
+        return 0;
+    }
+    SF_HIP(hipSetDevice(h->dev));
+    hipStream_t s = h->st;
+    const unsigned gb = sf_grid(D.nb), gm = sf_grid(D.mcap), gw = sf_grid((uint64_t)D.nb * 64, NT, 65536);
+    hipLaunchKernelGGL(k_uf_reset, dim3(gb), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_uf_init, dim3(gw), dim3(NT), 0, s, D, 1);
+    hipLaunchKernelGGL(k_uf_jump, dim3(gb), dim3(NT), 0, s, D, 1);
+    hipLaunchKernelGGL(k_uf_hook, dim3(gm), dim3(NT), 0, s, D, 1);
+    hipLaunchKernelGGL(k_uf_flatten, dim3(gb), dim3(NT), 0, s, D, 1);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipMemcpyAsync(out, D.parent, sizeof(int32_t) * D.nb, hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
     return 0;
 }
 const float* sf_world_pose(sf_world* w) {

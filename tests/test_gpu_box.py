@@ -156,3 +156,36 @@ def test_full_size_rubble_step_matches_oracle_and_invariants(oracle):
         assert (isl[isl] == isl).all() and (isl <= np.arange(len(isl))).all()
         assert np.isfinite(w.bodies()).all()
     assert w.stats()["dropped"] == 0
+
+
+def _sleep_scene():
+    # three 6-box stacks: A at rest, B with a top box sliding (its island stays restless until it stops), C hit by a
+    # box sliding in along the ground after the others fell asleep
+    items = []
+    for x in (-4.0, 0.0, 4.0):
+        for k in range(6):
+            v = (1.0, 0.0, 0.0) if (x == 0.0 and k == 5) else (0, 0, 0)
+            items.append(scenes.box((x, 0.25 + 0.5 * k, 0.0), (0.25, 0.25, 0.25), v=v))
+    items.append(scenes.box((6.8, 0.25, 0.0), (0.25, 0.25, 0.25), v=(-6.0, 0.0, 0.0)))
+    return scenes.boxes(items)
+
+
+def test_island_sleep_bit_exact(oracle):
+    # islands fall asleep one at a time; a sleepy body in an island with a restless member must stay awake
+    print("case: three stacks with sleep on, 150 steps", flush=True)
+    b, sb, s = _sleep_scene()
+    prm = params(substeps=4)
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    awake = []
+    for k in range(150):
+        w.step(DT)
+        o.step(DT)
+        gb, ob = w.bodies(), o.bodies()
+        assert np.array_equal(gb, ob), f"step {k}: max diff {np.abs(gb - ob).max()}"
+        awake.append(int(gb[:, 17].sum()))
+        if k % 10 == 0:
+            assert np.array_equal(w.islands(), o.islands()), f"step {k}: islands"
+    n = len(b)
+    assert max(awake) == n and min(awake) < n, awake
+    assert len(set(awake)) >= 3, awake
