@@ -156,6 +156,53 @@ int sf_scene_building(int wx, int wz, int storeys, int bay, int remove, uint32_t
  * slab every 6th layer, masonry elsewhere */
 int sf_scene_tower(uint8_t* mat, int32_t* xyz, uint8_t* infill);
 
+
+/* ---------------------------------------------------------------- lattice world (the legacy brick engine)
+ * One handle holds one or many independent scenes (a standing structure of lattice bricks plus loose bodies made of
+ * bricks) that step together: sf_lat_step advances every scene by one frame of the reference's step
+ * (src/lib.rs:3639-3721: two substeps of sphere-per-brick contacts and a 10-iteration impulse solve, then the
+ * structural pass: resting loads, g - a loading, SOR load field, joint fatigue and failure, fracture bookkeeping).
+ * Indices given and returned are scene local. Calls on one handle are not thread safe. */
+typedef struct sf_lat sf_lat;
+#define SF_LAT_BODY_F 22
+#define SF_LAT_BRICK_F 13
+#define SF_LAT_CONTACT_F 18
+/* capacities are totals over the scenes; colours: colour limit of the contact solve (1..32), rounds: claim rounds */
+sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, int rounds);
+void sf_lat_destroy(sf_lat* L);
+/* a scene with its own brick, body and structure capacity and random seed (rnd(), src/lib.rs:323); returns its index */
+int sf_lat_scene_add(sf_lat* L, int brick_cap, int body_cap, int max_structs, uint32_t seed);
+/* empty a scene for reuse (same capacities) */
+int sf_lat_scene_clear(sf_lat* L, int scene, uint32_t seed);
+/* a standing structure (add_brick, src/lib.rs:629-645): lattice cells xyz[3n], materials 1..8, infill flags */
+int sf_lat_add_structure(sf_lat* L, int scene, int n, const int32_t* xyz, const uint8_t* mat, const uint8_t* infill);
+/* a loose body of lattice bricks with velocity v3 and spin w3 (setup_body, src/lib.rs:844-891); returns its slot */
+int sf_lat_add_body(sf_lat* L, int scene, int n, const int32_t* xyz, const uint8_t* mat, const float* v3, const float* w3);
+/* neighbour tables, bonds (finalize_brick, src/lib.rs:668-696) and upload, once every brick of the scene is in */
+int sf_lat_finalize(sf_lat* L, int scene);
+/* ask every structure of the scene for its load field (warm_struct, src/lib.rs:1061); the sweeps run next frame */
+int sf_lat_warm_all(sf_lat* L, int scene);
+/* a charge (explode_at, src/lib.rs:2697-2716) */
+int sf_lat_explode(sf_lat* L, int scene, float x, float y, float z, float r, float power);
+/* one frame of every scene (step, src/lib.rs:3639) */
+int sf_lat_step(sf_lat* L, float dt);
+/* ptr_stats layout (src/lib.rs:3718-3720): 0 static bricks, 1 bricks in bodies, 2 bodies, 3 awake, 4 contacts,
+ * 6 worst joint ratio, 13 structures with a live load field, 14 structures never solved, 15 structures */
+int sf_lat_stats(sf_lat* L, int scene, float* out16);
+/* bricks, body slots, structures of a scene */
+int sf_lat_counts(sf_lat* L, int scene, int32_t* out3);
+int sf_lat_bodies(sf_lat* L, int scene, float* out);
+int sf_lat_bricks(sf_lat* L, int scene, float* out);
+/* 12 floats per structure (ptr_structs, src/lib.rs:3733-3748) */
+int sf_lat_structs(sf_lat* L, int scene, float* out);
+/* per dominant material: pieces, bricks, largest piece, pieces >= 50 bricks, pieces <= 2 (dbg_matstats,
+ * tools/debug_exports.rs:41-56) */
+int sf_lat_matstats(sf_lat* L, int scene, float* out50);
+int sf_lat_contacts(sf_lat* L, int scene, float* out, int cap);
+int sf_lat_rng(sf_lat* L, int scene, uint32_t* out);
+/* wall time of the last sf_lat_step in ms and the host-device synchronisations it made */
+int sf_lat_timing(sf_lat* L, double* ms_frame, int32_t* syncs);
+
 #ifdef __cplusplus
 }
 #endif

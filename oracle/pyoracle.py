@@ -163,3 +163,116 @@ class LoadOracle:
         out = np.zeros(self.n, np.uint8)
         lib().ol_grounded(self._sp(), _p(out))
         return out
+
+
+ORDER_SEQ = 0
+ORDER_DEV = 1
+LL_BODY_F = 22
+LL_BRICK_F = 13
+LL_CONTACT_F = 18
+
+
+def _lattice_protos():
+    L = lib()
+    if getattr(L, "_ll_done", False):
+        return L
+    vp, i32, f32, u32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_uint32
+    L.ll_create.restype = vp
+    L.ll_create.argtypes = [i32, i32, i32, i32, i32, u32]
+    L.ll_destroy.argtypes = [vp]
+    L.ll_add_structure.argtypes = [vp, i32, vp, vp, vp]
+    L.ll_add_body.argtypes = [vp, i32, vp, vp, vp, vp]
+    for f in ("ll_finalize", "ll_warm_all"):
+        getattr(L, f).argtypes = [vp]
+    L.ll_step.argtypes = [vp, f32]
+    L.ll_explode.argtypes = [vp, f32, f32, f32, f32, f32]
+    for f in ("ll_stats", "ll_bodies", "ll_bricks", "ll_structs", "ll_matstats"):
+        getattr(L, f).argtypes = [vp, vp]
+    for f in ("ll_nbricks", "ll_nbodies", "ll_nstructs"):
+        getattr(L, f).argtypes = [vp]
+    L.ll_contacts.argtypes = [vp, vp, i32]
+    L.ll_rng.argtypes = [vp]
+    L.ll_rng.restype = u32
+    L._ll_done = True
+    return L
+
+
+class LatticeOracle:
+    """The legacy brick engine's physics frame on the CPU (src/lib.rs restated; oracle/lattice.c)."""
+
+    def __init__(self, order=ORDER_DEV, brick_cap=1 << 16, body_cap=1 << 14, colours=16, rounds=8, seed=0x9E3779B9):
+        L = _lattice_protos()
+        self._h = L.ll_create(int(order), int(brick_cap), int(body_cap), int(colours), int(rounds), int(seed))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ll_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_structure(self, mat, xyz, infill=None):
+        mat = np.ascontiguousarray(mat, np.uint8)
+        xyz = np.ascontiguousarray(xyz, np.int32)
+        infill = np.zeros(len(mat), np.uint8) if infill is None else np.ascontiguousarray(infill, np.uint8)
+        return lib().ll_add_structure(self._h, len(mat), _p(xyz), _p(mat), _p(infill))
+
+    def add_body(self, mat, xyz, v=(0, 0, 0), w=(0, 0, 0)):
+        mat = np.ascontiguousarray(mat, np.uint8)
+        xyz = np.ascontiguousarray(xyz, np.int32)
+        v = np.asarray(v, np.float32)
+        w = np.asarray(w, np.float32)
+        return lib().ll_add_body(self._h, len(mat), _p(xyz), _p(mat), _p(v), _p(w))
+
+    def finalize(self):
+        lib().ll_finalize(self._h)
+
+    def warm_all(self):
+        lib().ll_warm_all(self._h)
+
+    def step(self, dt=1.0 / 60.0):
+        lib().ll_step(self._h, ctypes.c_float(dt))
+
+    def explode(self, x, y, z, r, power):
+        lib().ll_explode(self._h, *(ctypes.c_float(v) for v in (x, y, z, r, power)))
+
+    def stats(self):
+        out = np.zeros(16, np.float32)
+        lib().ll_stats(self._h, _p(out))
+        return out
+
+    def bodies(self):
+        n = lib().ll_nbodies(self._h)
+        out = np.zeros((max(n, 1), LL_BODY_F), np.float32)
+        lib().ll_bodies(self._h, _p(out))
+        return out[:n]
+
+    def bricks(self):
+        n = lib().ll_nbricks(self._h)
+        out = np.zeros((max(n, 1), LL_BRICK_F), np.float32)
+        lib().ll_bricks(self._h, _p(out))
+        return out[:n]
+
+    def structs(self):
+        n = lib().ll_nstructs(self._h)
+        out = np.zeros((max(n, 1), 12), np.float32)
+        lib().ll_structs(self._h, _p(out))
+        return out[:n]
+
+    def matstats(self):
+        out = np.zeros(50, np.float32)
+        lib().ll_matstats(self._h, _p(out))
+        return out
+
+    def contacts(self):
+        n = lib().ll_contacts(self._h, None, 0)
+        out = np.zeros((max(n, 1), LL_CONTACT_F), np.float32)
+        lib().ll_contacts(self._h, _p(out), n)
+        return out[:n]
+
+    def rng(self):
+        return lib().ll_rng(self._h)

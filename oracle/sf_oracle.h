@@ -82,6 +82,36 @@ int ol_grounded(const ol_struct* s, uint8_t* out);
 /* neighbour table and bond capacities of one standing structure (finalize_brick :668-684, bond_cap :646-649) */
 void ol_finalize(int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, int32_t* nb6, float* bond);
 
+/* ---------------------------------------------------------------- lattice world (lattice.c): the legacy brick
+ * engine's physics frame (src/lib.rs), in the reference's sequential orders or the GPU's orders. */
+typedef struct LW LW;
+enum { LL_ORDER_SEQ = 0, LL_ORDER_DEV = 1 };
+#define LL_BODY_F 22
+#define LL_BRICK_F 13
+#define LL_CONTACT_F 18
+LW* ll_create(int order, int brick_cap, int body_cap, int ncolour, int rounds, uint32_t seed);
+void ll_destroy(LW* w);
+/* a standing structure of lattice bricks (add_brick, lib.rs:629-645); returns its index */
+int ll_add_structure(LW* w, int n, const int32_t* xyz, const uint8_t* mat, const uint8_t* infill);
+/* a loose body of lattice bricks with the given velocity and spin (setup_body, lib.rs:844-891); returns its slot */
+int ll_add_body(LW* w, int n, const int32_t* xyz, const uint8_t* mat, const float* v3, const float* w3);
+/* neighbour tables, bonds and structure counts once every brick is in (finalize_brick, lib.rs:668-696) */
+void ll_finalize(LW* w);
+/* ask every structure for its load field (warm_struct, lib.rs:1061-1066); the sweeps run in the next frame */
+void ll_warm_all(LW* w);
+void ll_step(LW* w, float dt);
+void ll_explode(LW* w, float x, float y, float z, float r, float power);
+void ll_stats(const LW* w, float* out16);
+int ll_nbricks(const LW* w);
+int ll_nbodies(const LW* w);
+int ll_nstructs(const LW* w);
+void ll_bodies(const LW* w, float* out);
+void ll_bricks(const LW* w, float* out);
+void ll_structs(const LW* w, float* out);
+void ll_matstats(const LW* w, float* out50);
+int ll_contacts(const LW* w, float* out, int cap);
+uint32_t ll_rng(const LW* w);
+
 #ifdef __cplusplus
 }
 #endif

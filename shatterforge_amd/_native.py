@@ -63,6 +63,25 @@ _PROTOS = {
     "sf_load_last_timing": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_scene_building": (c_int, [c_int, c_int, c_int, c_int, c_int, c_uint, c_void_p, c_void_p, c_void_p]),
     "sf_scene_tower": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sf_lat_create": (c_void_p, [c_int, c_int, c_int, c_int, c_int]),
+    "sf_lat_destroy": (None, [c_void_p]),
+    "sf_lat_scene_add": (c_int, [c_void_p, c_int, c_int, c_int, c_uint]),
+    "sf_lat_scene_clear": (c_int, [c_void_p, c_int, c_uint]),
+    "sf_lat_add_structure": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "sf_lat_add_body": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_lat_finalize": (c_int, [c_void_p, c_int]),
+    "sf_lat_warm_all": (c_int, [c_void_p, c_int]),
+    "sf_lat_explode": (c_int, [c_void_p, c_int, c_float, c_float, c_float, c_float, c_float]),
+    "sf_lat_step": (c_int, [c_void_p, c_float]),
+    "sf_lat_stats": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_counts": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_bodies": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_bricks": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_structs": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_matstats": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_contacts": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "sf_lat_rng": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_timing": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None

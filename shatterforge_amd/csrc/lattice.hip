@@ -1,0 +1,2572 @@
+// Lattice world on the GPU: the legacy brick engine's physics frame (src/lib.rs step :3639-3721) for one or many
+// independent scenes at once, as hand-written gfx950 kernels plus host bookkeeping for the bursty topology changes.
+//
+// A frame (all scenes of a world together):
+//   2 substeps of h = dt/2 (substep, src/lib.rs:1769-1783):
+//     cache           rotation, world inertia, gravity per awake body; world brick positions (update_cache :1368)
+//     hash            every live body brick into a counting-sort table keyed by (scene, cell); a cell's bricks in
+//                     body order (build_hash :1383)
+//     contacts        one lane per awake brick, counted then written: ground corners, spheres against the 27 lattice
+//                     cells of the standing structure, spheres against bricks of other bodies in the 27 cells
+//                     (gen_contacts :1396-1471); wake requests, support stamps, contact counts
+//     plough          candidates flagged on the device, applied by the host in contact order (plough :1493-1550)
+//     prep            effective masses, tangents, bias (prep_contacts :1551-1577)
+//     colour          claim rounds: each contact proposes the lowest colour free at its moving bodies; the highest
+//                     priority claim wins; the rest go to a Jacobi pass with mass splitting
+//     solve x 10      colour after colour, Gauss-Seidel within a colour is race free (solve :1578-1601); then the
+//                     overflow pass: private body copies with split masses, per-body sums as a 64-lane tree
+//     pinch           velocity-change limits, pulverised small pieces (pinch_check :1746-1768)
+//     impact          candidates flagged on the device, crushed by the host in contact order (impact_damage :1610)
+//     integrate       drag, rolling resistance, per-body sleep, out-of-bounds (integrate :1703-1745), wakes
+//   host: rebuild split bodies, detach unsupported parts of structures (rebuild_body :930, connectivity :1138)
+//   structural (structural :1228-1365): resting loads onto structures, g - a loading of loaded pieces, red-black
+//     SOR sweeps (one workgroup per structure or piece, adaptive count), joint judgement with fatigue; crushes and
+//     cracks go back to the host as events
+//   warm-up sweeps of newly solved structures (warm_step :1068), connectivity again
+//
+// Scenes never interact: every table is keyed by scene, every order and priority is scene local, so a scene's
+// result does not depend on which other scenes share the world.
+#include "lattice_dev.h"
+#include "sf_host.h"
+#include "sf_scan.h"
+#include "../../include/sf.h"
+#include <vector>
+#include <string>
+#include <cstring>
+#include <cmath>
+#include <algorithm>
+#include <chrono>
+
+using namespace lat;
+
+namespace {
+
+#define GS(i, n) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)(n); i += gridDim.x * blockDim.x)
+
+__device__ __forceinline__ LM load_m(const float4* p) {
+    float4 a = p[0], b = p[1], c = p[2];
+    LM r;
+    r.m[0] = a.x; r.m[1] = a.y; r.m[2] = a.z; r.m[3] = b.x; r.m[4] = b.y; r.m[5] = b.z; r.m[6] = c.x; r.m[7] = c.y; r.m[8] = c.z;
+    return r;
+}
+__device__ __forceinline__ void store_m(float4* p, const LM& m) {
+    p[0] = make_float4(m.m[0], m.m[1], m.m[2], 0.0f);
+    p[1] = make_float4(m.m[3], m.m[4], m.m[5], 0.0f);
+    p[2] = make_float4(m.m[6], m.m[7], m.m[8], 0.0f);
+}
+__device__ __forceinline__ float fmax_bits(float a) { return a; }
+__device__ __forceinline__ void atomic_max_pos(uint32_t* p, float v) { if (v > 0.0f) atomicMax(p, __float_as_uint(v)); }
+
+// ---------------------------------------------------------------- cache
+__global__ void __launch_bounds__(NT) k_cache(LD D, int gravity, float h) {
+    GS(b, D.nbd) {
+        if (!D.alive[b] || D.asleep[b]) continue;
+        LM r = q_to_m(lq4(D.q[b]));
+        store_m(D.rot + 3 * (size_t)b, r);
+        LM iil = load_m(D.iil + 3 * (size_t)b);
+        LM t = mmul(r, iil), rt = mt(r);
+        store_m(D.iiw + 3 * (size_t)b, mmul(t, rt));
+        if (gravity) { float4 v = D.v[b]; v.y -= GR * h; D.v[b] = v; }
+    }
+}
+__global__ void __launch_bounds__(NT) k_wpos(LD D) {
+    GS(k, D.ndyn) {
+        int b = D.dyn_body[k];
+        if (!D.alive[b] || D.asleep[b]) continue;
+        uint32_t i = D.dyn[k];
+        LM r = load_m(D.rot + 3 * (size_t)b);
+        D.wpos[i] = f4(add(lv4(D.pos[b]), mv(r, lv4(D.local[i]))));
+    }
+}
+
+// ---------------------------------------------------------------- hash of body bricks by (scene, cell)
+__device__ __forceinline__ unsigned long long cell_key(int scene, int x, int y, int z) {
+    return ((unsigned long long)(uint32_t)scene << 33) | ((unsigned long long)((x + 1024) & 2047) << 22) |
+           ((unsigned long long)((y + 1024) & 2047) << 11) | (unsigned long long)((z + 1024) & 2047);
+}
+__device__ __forceinline__ uint32_t key_slot(unsigned long long k, uint32_t mask) {
+    return mix32((uint32_t)k ^ mix32((uint32_t)(k >> 32) * 0x9E3779B9u + 0x7F4A7C15u)) & mask;
+}
+__global__ void __launch_bounds__(NT) k_hash_count(LD D) {
+    GS(k, D.ndyn) {
+        uint32_t i = D.dyn[k];
+        int b = D.dyn_body[k];
+        if (!D.alive[b] || D.mat[i] == 0) { D.dslot[k] = NONE; continue; }
+        LV p = lv4(D.wpos[i]);
+        unsigned long long key = cell_key(D.bd_scene[b], fcell(p.x), fcell(p.y), fcell(p.z));
+        uint32_t s = key_slot(key, D.tmask);
+        D.dkey[k] = key;
+        D.dslot[k] = s;
+        atomicAdd(&D.hcnt[s], 1u);
+    }
+}
+__global__ void __launch_bounds__(NT) k_hash_fill(LD D) {
+    GS(k, D.ndyn) {
+        uint32_t s = D.dslot[k];
+        if (s == NONE) continue;
+        D.hent[D.hoff[s] + atomicAdd(&D.hfill[s], 1u)] = k;
+    }
+}
+
+// ---------------------------------------------------------------- segmented sort (ascending u32 values)
+// segments of at most a wavefront: bitonic within one wave; longer ones: k_segsort_big
+__global__ void __launch_bounds__(NT) k_segsort(const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t s = wave; s < nseg; s += nwaves) {
+        uint32_t o = off[s], n = cnt[s];
+        if (n < 2 || n > 64) continue;
+        uint32_t v = (uint32_t)lane < n ? val[o + lane] : 0xFFFFFFFFu;
+        for (int k = 2; k <= 64; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                uint32_t u = __shfl_xor(v, j, 64);
+                bool up = (lane & k) == 0, lower = (lane & j) == 0;
+                v = (lower == up) ? min(v, u) : max(v, u);
+            }
+        if ((uint32_t)lane < n) val[o + lane] = v;
+    }
+}
+__global__ void __launch_bounds__(NT) k_segsort_big(const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
+    __shared__ uint32_t sh[4096];
+    for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+        uint32_t o = off[s], n = cnt[s];
+        if (n <= 64) continue;
+        if (n > 4096) {
+            if (threadIdx.x == 0)
+                for (uint32_t i = 1; i < n; i++) {
+                    uint32_t v = val[o + i], j = i;
+                    while (j > 0 && val[o + j - 1] > v) { val[o + j] = val[o + j - 1]; j--; }
+                    val[o + j] = v;
+                }
+            __syncthreads();
+            continue;
+        }
+        uint32_t n2 = 128;
+        while (n2 < n) n2 <<= 1;
+        for (uint32_t i = threadIdx.x; i < n2; i += NT) sh[i] = i < n ? val[o + i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < n2; i += NT) {
+                    uint32_t l = i ^ j;
+                    if (l > i) {
+                        uint32_t x = sh[i], y = sh[l];
+                        bool up = (i & k) == 0;
+                        if ((x > y) == up) { sh[i] = y; sh[l] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t i = threadIdx.x; i < n; i += NT) val[o + i] = sh[i];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- contacts
+__device__ __forceinline__ int static_brick(const LD& D, const SceneK& S, int x, int y, int z) {
+    x -= S.gx0; y -= S.gy0; z -= S.gz0;
+    if (x < 0 || y < 0 || z < 0 || x >= S.gnx || y >= S.gny || z >= S.gnz) return -1;
+    int32_t g = D.grid[S.goff + ((size_t)y * S.gnz + z) * S.gnx + x];
+    if (!g) return -1;
+    int j = g - 1;
+    return (D.mat[j] != 0 && D.body[j] < 0) ? j : -1;
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(NT) k_contacts(LD D) {
+    GS(k, D.ndyn) {
+        int a = D.dyn_body[k];
+        if (!D.alive[a] || D.asleep[a]) { if (!WRITE) D.ccnt[k] = 0; continue; }
+        uint32_t i = D.dyn[k];
+        int sc = D.bd_scene[a];
+        const SceneK S = D.sc[sc];
+        uint32_t out = WRITE ? D.coff[k] : 0u, n = 0;
+        LM rot = load_m(D.rot + 3 * (size_t)a);
+        float speed = len(lv4(D.v[a])) + len(lv4(D.w[a])) * D.rad[a] * 0.5f;
+        LV p = lv4(D.wpos[i]);
+        auto push = [&](int b, int bo, uint32_t bb, LV cp, LV nn, float depth) {
+            if (WRITE) {
+                uint32_t c = out + n;
+                if (c < D.ccap) {
+                    D.ca[c] = a; D.cb[c] = b; D.cbo[c] = bo; D.cba[c] = i; D.cbb[c] = bb; D.cscene[c] = sc;
+                    D.cp[c] = f4(cp); D.cn[c] = f4(nn); D.cdepth[c] = depth;
+                    D.con[c] = 1; D.ccol[c] = -1; D.cjn[c] = 0.0f; D.cjt1[c] = 0.0f; D.cjt2[c] = 0.0f; D.cvn0[c] = 0.0f;
+                }
+                atomicAdd(&D.nc[a], 1u);
+                if (b >= 0) atomicAdd(&D.nc[b], 1u);
+            }
+            n++;
+        };
+        if (p.y < 0.5f) {
+            for (int c = 0; c < 8; c++) {
+                LV l = lv((c & 1) == 0 ? -H : H, (c & 2) == 0 ? -H : H, (c & 4) == 0 ? -H : H);
+                LV cp = add(p, mv(rot, l));
+                if (cp.y < 0.0f) { push(-1, -1, NONE, cp, lv(0.0f, 1.0f, 0.0f), -cp.y); if (WRITE) D.sup[i] = S.frame; }
+            }
+        }
+        int cx = fcell(p.x) + NX2, cy = fcell(p.y), cz = fcell(p.z) + NX2;
+        for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
+            int x = cx + dx, y = cy + dy, z = cz + dz;
+            int j = static_brick(D, S, x, y, z);
+            if (j < 0) continue;
+            LV c = lat_pos(x, y, z);
+            LV q = lv(clampf(p.x, c.x - H, c.x + H), clampf(p.y, c.y - H, c.y + H), clampf(p.z, c.z - H, c.z + H));
+            LV d = sub(p, q);
+            float dist = len(d);
+            if (dist >= RB) continue;
+            if (dist > 1e-4f) {
+                if (WRITE && d.y > 0.3f * dist) D.sup[i] = S.frame;
+                push(-1, -1, (uint32_t)j, q, mul(d, 1.0f / dist), RB - dist);
+            } else {
+                LV r = sub(p, c);
+                float rv[3] = {r.x, r.y, r.z};
+                int ax = 0;
+                for (int t = 1; t < 3; t++) if (fabsf(rv[t]) > fabsf(rv[ax])) ax = t;
+                float nv[3] = {0.0f, 0.0f, 0.0f};
+                nv[ax] = rv[ax] >= 0.0f ? 1.0f : -1.0f;
+                push(-1, -1, (uint32_t)j, p, lv(nv[0], nv[1], nv[2]), H + RB - fabsf(rv[ax]));
+            }
+        }
+        int hx = fcell(p.x), hy = fcell(p.y), hz = fcell(p.z);
+        for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
+            unsigned long long key = cell_key(sc, hx + dx, hy + dy, hz + dz);
+            uint32_t s = key_slot(key, D.tmask);
+            uint32_t e0 = D.hoff[s], e1 = e0 + D.hcnt[s];
+            for (uint32_t e = e0; e < e1; e++) {
+                uint32_t k2 = D.hent[e];
+                if (D.dkey[k2] != key) continue;
+                uint32_t j = D.dyn[k2];
+                int bj = D.body[j];
+                if (bj == a || D.mat[j] == 0) continue;
+                bool other_awake = !D.asleep[bj];
+                if (other_awake && j < i) continue;
+                LV d = sub(p, lv4(D.wpos[j]));
+                float dd = dot(d, d);
+                if (dd >= 4.0f * RB * RB || dd < 1e-10f) continue;
+                float dist = sqrtf(dd);
+                LV nn = mul(d, 1.0f / dist);
+                LV cp = madd(lv4(D.wpos[j]), nn, dist * 0.5f);
+                if (WRITE) { if (nn.y > 0.3f) D.sup[i] = S.frame; else if (nn.y < -0.3f) D.sup[j] = S.frame; }
+                if (other_awake) push(bj, bj, j, cp, nn, 2.0f * RB - dist);
+                else {
+                    if (WRITE && speed > 0.7f) D.wake[bj] = 1;
+                    push(-1, bj, j, cp, nn, 2.0f * RB - dist);
+                }
+            }
+        }
+        if (!WRITE) D.ccnt[k] = n;
+    }
+}
+// first contact of every scene (its first dynamic slot's offset)
+__global__ void k_scene_c0(LD D) {
+    GS(s, D.nscene) {
+        SceneK S = D.sc[s];
+        D.sc_c0[s] = S.dyn0 < D.ndyn ? D.coff[S.dyn0] : D.ctr[C_NCS];
+    }
+}
+
+__device__ __forceinline__ LV vel_at(const LD& D, int b, LV r) { return b < 0 ? lv(0.0f, 0.0f, 0.0f) : add(lv4(D.v[b]), cross(lv4(D.w[b]), r)); }
+
+__global__ void __launch_bounds__(NT) k_plough_flag(LD D) {
+    uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    const Tables& T = tables();
+    GS(c, ncs) {
+        if (!D.con[c] || D.cb[c] >= 0 || D.cbb[c] == NONE || D.cbo[c] >= 0) continue;
+        uint32_t j = D.cbb[c];
+        bool cand = D.mat[j] == 0 || D.body[j] >= 0;
+        if (!cand) {
+            int a = D.ca[c];
+            LV ra = sub(lv4(D.cp[c]), lv4(D.pos[a]));
+            float vn = -dot(vel_at(D, a, ra), lv4(D.cn[c]));
+            if (vn < 2.0f) continue;
+            float meff = D.m[a] * 3.0f / maxf((float)D.nc[a], 3.0f);
+            float mom = minf(meff, D.m[a]) * vn;
+            int mj = D.mat[j];
+            if (mj >= SLUG || mom < T.capv[mj] * 0.35f) continue;
+        }
+        uint32_t slot = atomicAdd(&D.ctr[C_NPLOUGH], 1u);
+        if (slot < D.pcap) D.plist[slot] = c;
+    }
+}
+
+__device__ __forceinline__ float eff_k(const LD& D, int a, LV r, LV n) {
+    LM iiw = load_m(D.iiw + 3 * (size_t)a);
+    return D.im[a] + dot(n, cross(mv(iiw, cross(r, n)), r));
+}
+__device__ __forceinline__ float eff_mass(const LD& D, int a, int b, LV ra, LV rb, LV n) {
+    float k = eff_k(D, a, ra, n);
+    if (b >= 0) k += eff_k(D, b, rb, n);
+    return k > 1e-12f ? 1.0f / k : 0.0f;
+}
+__device__ __forceinline__ float eff_mass_split(const LD& D, int a, float na, int b, float nb, LV ra, LV rb, LV n) {
+    float k = eff_k(D, a, ra, n) * na;
+    if (b >= 0) k += eff_k(D, b, rb, n) * nb;
+    return k > 1e-12f ? 1.0f / k : 0.0f;
+}
+
+__global__ void __launch_bounds__(NT) k_prep(LD D, float h) {
+    uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    GS(c, ncs) {
+        if (!D.con[c]) continue;
+        int a = D.ca[c], b = D.cb[c];
+        LV p = lv4(D.cp[c]), n = lv4(D.cn[c]);
+        LV ra = sub(p, lv4(D.pos[a]));
+        LV rb = b >= 0 ? sub(p, lv4(D.pos[b])) : lv(0.0f, 0.0f, 0.0f);
+        D.cra[c] = f4(ra); D.crb[c] = f4(rb);
+        D.cmn[c] = eff_mass(D, a, b, ra, rb, n);
+        LV dv = sub(vel_at(D, a, ra), vel_at(D, b, rb));
+        float vn = dot(dv, n);
+        D.cvn0[c] = maxf(-vn, 0.0f);
+        LV vt = sub(dv, mul(n, vn));
+        float lt = len(vt);
+        LV t1;
+        if (lt > 1e-4f) t1 = mul(vt, 1.0f / lt);
+        else {
+            LV t = fabsf(n.x) < 0.9f ? lv(1.0f, 0.0f, 0.0f) : lv(0.0f, 1.0f, 0.0f);
+            LV x = cross(n, t);
+            t1 = mul(x, 1.0f / len(x));
+        }
+        LV t2 = cross(n, t1);
+        D.ct1[c] = f4(t1); D.ct2[c] = f4(t2);
+        D.cmt1[c] = eff_mass(D, a, b, ra, rb, t1);
+        D.cmt2[c] = eff_mass(D, a, b, ra, rb, t2);
+        float bias = minf(0.2f / h * maxf(D.cdepth[c] - 0.01f, 0.0f), 1.0f);
+        if (vn < -2.0f) bias = maxf(bias, -0.12f * vn);
+        D.cbias[c] = bias;
+    }
+}
+__global__ void __launch_bounds__(NT) k_vpre(LD D) {
+    GS(b, D.nbd) if (D.alive[b]) { D.vpre[b] = D.v[b]; D.wpre[b] = D.w[b]; }
+}
+
+// ---------------------------------------------------------------- colouring
+__global__ void __launch_bounds__(NT) k_col_clear(LD D) {
+    GS(x, (size_t)D.nbd * D.K) {
+        D.claim[x] = 0ull;
+        if (x < D.nbd) D.mask[x] = 0u;
+    }
+}
+__global__ void __launch_bounds__(NT) k_col_claim(LD D, int r) {
+    uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    const uint32_t full = D.K >= 32 ? 0xFFFFFFFFu : ((1u << D.K) - 1u);
+    GS(c, ncs) {
+        D.cprop[c] = -1;
+        if (!D.con[c] || D.ccol[c] >= 0) continue;
+        int a = D.ca[c], b = D.cb[c];
+        uint32_t used = D.mask[a] | (b >= 0 ? D.mask[b] : 0u);
+        uint32_t fr = ~used & full;
+        if (!fr) continue;
+        int col = __ffs(fr) - 1;
+        D.cprop[c] = col;
+        unsigned long long pr = claim_prio(c - D.sc_c0[D.cscene[c]], r);
+        atomicMax(&D.claim[(size_t)a * D.K + col], pr);
+        if (b >= 0) atomicMax(&D.claim[(size_t)b * D.K + col], pr);
+    }
+}
+__global__ void __launch_bounds__(NT) k_col_resolve(LD D, int r) {
+    uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    GS(c, ncs) {
+        int col = D.cprop[c];
+        if (col < 0) continue;
+        int a = D.ca[c], b = D.cb[c];
+        unsigned long long pr = claim_prio(c - D.sc_c0[D.cscene[c]], r);
+        if (D.claim[(size_t)a * D.K + col] != pr) continue;
+        if (b >= 0 && D.claim[(size_t)b * D.K + col] != pr) continue;
+        D.ccol[c] = col;
+        atomicOr(&D.mask[a], 1u << col);
+        if (b >= 0) atomicOr(&D.mask[b], 1u << col);
+    }
+}
+// contacts bucketed by colour (overflow last), one atomic per bucket per block
+__global__ void __launch_bounds__(NT) k_col_count(LD D) {
+    __shared__ uint32_t h[KMAX + 1];
+    uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    for (uint32_t t = threadIdx.x; t <= D.K; t += NT) h[t] = 0;
+    __syncthreads();
+    GS(c, ncs) if (D.con[c]) atomicAdd(&h[D.ccol[c] < 0 ? D.K : (uint32_t)D.ccol[c]], 1u);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t <= D.K; t += NT) if (h[t]) atomicAdd(&D.ctr[C_COLCNT + t], h[t]);
+}
+__global__ void k_col_scan(LD D) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t o = 0;
+        for (uint32_t t = 0; t <= D.K; t++) { D.coloff[t] = o; o += D.ctr[C_COLCNT + t]; D.colfill[t] = 0; }
+    }
+}
+__global__ void __launch_bounds__(NT) k_col_fill(LD D) {
+    __shared__ uint32_t cnt[KMAX + 1], base[KMAX + 1];
+    uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    for (uint32_t c0 = blockIdx.x * NT; c0 < ncs; c0 += gridDim.x * NT) {
+        for (uint32_t t = threadIdx.x; t <= D.K; t += NT) cnt[t] = 0;
+        __syncthreads();
+        uint32_t c = c0 + threadIdx.x;
+        bool ok = c < ncs && D.con[c];
+        uint32_t bin = 0, r = 0;
+        if (ok) { bin = D.ccol[c] < 0 ? D.K : (uint32_t)D.ccol[c]; r = atomicAdd(&cnt[bin], 1u); }
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t <= D.K; t += NT) base[t] = cnt[t] ? atomicAdd(&D.colfill[t], cnt[t]) : 0u;
+        __syncthreads();
+        if (ok) D.collist[D.coloff[bin] + base[bin] + r] = c;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- overflow lists (per body, in contact order)
+__global__ void __launch_bounds__(NT) k_ovf_count(LD D) {
+    uint32_t n = D.ctr[C_COLCNT + D.K];
+    GS(t, n) {
+        uint32_t c = D.collist[D.coloff[D.K] + t];
+        atomicAdd(&D.ovcnt[D.ca[c]], 1u);
+        if (D.cb[c] >= 0) atomicAdd(&D.ovcnt[D.cb[c]], 1u);
+    }
+}
+__global__ void __launch_bounds__(NT) k_ovf_fill(LD D) {
+    uint32_t n = D.ctr[C_COLCNT + D.K];
+    GS(t, n) {
+        uint32_t c = D.collist[D.coloff[D.K] + t];
+        int a = D.ca[c], b = D.cb[c];
+        D.ovent[D.ovoff[a] + atomicAdd(&D.ovfill[a], 1u)] = 2u * c;
+        if (b >= 0) D.ovent[D.ovoff[b] + atomicAdd(&D.ovfill[b], 1u)] = 2u * c + 1u;
+    }
+}
+__global__ void __launch_bounds__(NT) k_ovf_prep(LD D) {
+    uint32_t n = D.ctr[C_COLCNT + D.K];
+    GS(t, n) {
+        uint32_t c = D.collist[D.coloff[D.K] + t];
+        int a = D.ca[c], b = D.cb[c];
+        float na = (float)D.ovcnt[a], nb = b >= 0 ? (float)D.ovcnt[b] : 0.0f;
+        LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]);
+        D.csmn[c] = eff_mass_split(D, a, na, b, nb, ra, rb, lv4(D.cn[c]));
+        D.csmt1[c] = eff_mass_split(D, a, na, b, nb, ra, rb, lv4(D.ct1[c]));
+        D.csmt2[c] = eff_mass_split(D, a, na, b, nb, ra, rb, lv4(D.ct2[c]));
+    }
+}
+
+// ---------------------------------------------------------------- solve
+__device__ __forceinline__ void apply_imp(LD& D, int a, int b, LV ra, LV rb, LV p) {
+    D.v[a] = f4(madd(lv4(D.v[a]), p, D.im[a]));
+    D.w[a] = f4(add(lv4(D.w[a]), mv(load_m(D.iiw + 3 * (size_t)a), cross(ra, p))));
+    if (b >= 0) {
+        D.v[b] = f4(madd(lv4(D.v[b]), p, -D.im[b]));
+        D.w[b] = f4(sub(lv4(D.w[b]), mv(load_m(D.iiw + 3 * (size_t)b), cross(rb, p))));
+    }
+}
+__global__ void __launch_bounds__(NT) k_solve_colour(LD D, int col) {
+    uint32_t n = D.ctr[C_COLCNT + col], o = D.coloff[col];
+    GS(t, n) {
+        uint32_t c = D.collist[o + t];
+        int a = D.ca[c], b = D.cb[c];
+        LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
+        LV dv = sub(vel_at(D, a, ra), vel_at(D, b, rb));
+        float vn = dot(dv, nn);
+        float lam = D.cmn[c] * (-vn + D.cbias[c]);
+        float old = D.cjn[c];
+        float jn = maxf(old + lam, 0.0f);
+        float d = jn - old;
+        apply_imp(D, a, b, ra, rb, mul(nn, d));
+        float lim = MU * jn;
+        dv = sub(vel_at(D, a, ra), vel_at(D, b, rb));
+        float l1 = -D.cmt1[c] * dot(dv, t1);
+        float o1 = D.cjt1[c], j1 = clampf(o1 + l1, -lim, lim);
+        float l2 = -D.cmt2[c] * dot(dv, t2);
+        float o2 = D.cjt2[c], j2 = clampf(o2 + l2, -lim, lim);
+        LV p = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
+        apply_imp(D, a, b, ra, rb, p);
+        D.cjn[c] = jn; D.cjt1[c] = j1; D.cjt2[c] = j2;
+    }
+}
+// overflow contacts, Jacobi: each works on private copies of its bodies with split inverse masses
+__global__ void __launch_bounds__(NT) k_ovf_solve(LD D) {
+    uint32_t n = D.ctr[C_COLCNT + D.K], o = D.coloff[D.K];
+    GS(t, n) {
+        uint32_t c = D.collist[o + t];
+        int a = D.ca[c], b = D.cb[c];
+        float na = (float)D.ovcnt[a], nb = b >= 0 ? (float)D.ovcnt[b] : 0.0f;
+        LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
+        LV va = lv4(D.v[a]), wa = lv4(D.w[a]), vb = lv(0.0f, 0.0f, 0.0f), wb = lv(0.0f, 0.0f, 0.0f);
+        if (b >= 0) { vb = lv4(D.v[b]); wb = lv4(D.w[b]); }
+        LM iia = load_m(D.iiw + 3 * (size_t)a);
+        LM iib;
+        if (b >= 0) iib = load_m(D.iiw + 3 * (size_t)b);
+        float ima = D.im[a] * na, imb = b >= 0 ? D.im[b] * nb : 0.0f;
+        LV dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+        float vn = dot(dv, nn);
+        float lam = D.csmn[c] * (-vn + D.cbias[c]);
+        float old = D.cjn[c];
+        float jn = maxf(old + lam, 0.0f);
+        LV pn = mul(nn, jn - old);
+        va = madd(va, pn, ima);
+        wa = add(wa, mul(mv(iia, cross(ra, pn)), na));
+        if (b >= 0) { vb = madd(vb, pn, -imb); wb = sub(wb, mul(mv(iib, cross(rb, pn)), nb)); }
+        float lim = MU * jn;
+        dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+        float l1 = -D.csmt1[c] * dot(dv, t1);
+        float o1 = D.cjt1[c], j1 = clampf(o1 + l1, -lim, lim);
+        float l2 = -D.csmt2[c] * dot(dv, t2);
+        float o2 = D.cjt2[c], j2 = clampf(o2 + l2, -lim, lim);
+        LV pt = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
+        LV p = add(pn, pt);
+        D.cjn[c] = jn; D.cjt1[c] = j1; D.cjt2[c] = j2;
+        D.odv[2 * c] = f4(mul(p, D.im[a]));
+        D.odw[2 * c] = f4(mv(iia, cross(ra, p)));
+        if (b >= 0) {
+            D.odv[2 * c + 1] = f4(mul(p, -D.im[b]));
+            D.odw[2 * c + 1] = f4(neg(mv(iib, cross(rb, p))));
+        }
+    }
+}
+__device__ __forceinline__ LV wave_tree(LV x) {
+    for (int h = 32; h >= 1; h >>= 1) {
+        float ux = __shfl_down(x.x, h, 64), uy = __shfl_down(x.y, h, 64), uz = __shfl_down(x.z, h, 64);
+        x = add(x, lv(ux, uy, uz));
+    }
+    return lv(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64), __shfl(x.z, 0, 64));
+}
+__global__ void __launch_bounds__(NT) k_ovf_apply(LD D) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t b = wave; b < D.nbd; b += nwaves) {
+        uint32_t n = D.ovcnt[b];
+        if (!n) continue;
+        uint32_t o = D.ovoff[b];
+        LV av = lv(0.0f, 0.0f, 0.0f), aw = lv(0.0f, 0.0f, 0.0f);
+        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+            LV xv = lv(0.0f, 0.0f, 0.0f), xw = lv(0.0f, 0.0f, 0.0f);
+            if (e0 + lane < n) { uint32_t ent = D.ovent[o + e0 + lane]; xv = lv4(D.odv[ent]); xw = lv4(D.odw[ent]); }
+            av = add(av, wave_tree(xv));
+            aw = add(aw, wave_tree(xw));
+        }
+        if (lane == 0) { D.v[b] = f4(add(lv4(D.v[b]), av)); D.w[b] = f4(add(lv4(D.w[b]), aw)); }
+    }
+}
+
+// ---------------------------------------------------------------- pinch, impact, integrate
+__device__ void kill_body_dev(LD& D, int b) {
+    uint32_t o = D.boff[b], n = D.bcnt[b];
+    for (uint32_t t = 0; t < n; t++) {
+        uint32_t i = D.dyn[o + t];
+        if (D.body[i] == b) { D.mat[i] = 0; D.body[i] = -1; }
+    }
+    D.alive[b] = 0;
+    uint32_t s = atomicAdd(&D.ctr[C_NKILL], 1u);
+    if (s < D.nbd) D.klist[s] = b;
+}
+__global__ void __launch_bounds__(NT) k_pinch(LD D) {
+    GS(b, D.nbd) {
+        if (!D.alive[b] || D.asleep[b]) continue;
+        LV w = lv4(D.w[b]), wpre = lv4(D.wpre[b]);
+        LV dw = sub(w, wpre);
+        float tip = len(dw) * D.rad[b];
+        if (tip > 6.0f) D.w[b] = f4(madd(wpre, dw, 6.0f / tip));
+        LV v = lv4(D.v[b]), vpre = lv4(D.vpre[b]);
+        LV dv = sub(v, vpre);
+        float m = len(dv);
+        if (m < 8.0f) continue;
+        if (D.bcnt[b] == 0) continue;
+        uint32_t f = D.dyn[D.boff[b]];
+        if (D.mat[f] >= SLUG) continue;
+        if (D.count[b] <= 6u && D.sc[D.bd_scene[b]].blast_t <= 0.0f) kill_body_dev(D, b);
+        else D.v[b] = f4(madd(vpre, dv, 8.0f / m));
+    }
+}
+__global__ void __launch_bounds__(NT) k_impact_flag(LD D) {
+    uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    GS(c, ncs) {
+        if (!D.con[c]) continue;
+        int ma = D.mat[D.cba[c]];
+        if (ma == 0 || ma >= SLUG) continue;
+        float jn = D.cjn[c], vn0 = D.cvn0[c];
+        bool hit = D.cbb[c] != NONE && D.cbo[c] < 0 && jn * vn0 > 4000.0f;
+        bool crush = !(vn0 < 1.0f) && !(0.5f * jn * vn0 < 150.0f);
+        if (!hit && !crush) continue;
+        uint32_t slot = atomicAdd(&D.ctr[C_NIMPACT], 1u);
+        if (slot < D.pcap) D.ilist[slot] = c;
+    }
+}
+__global__ void __launch_bounds__(NT) k_integrate(LD D, float h) {
+    GS(b, D.nbd) {
+        if (!D.alive[b] || D.asleep[b]) continue;
+        LV v = lv4(D.v[b]), w = lv4(D.w[b]);
+        float vl = len(v);
+        if (vl > 60.0f) v = mul(v, 60.0f / vl);
+        float a = len(w);
+        float amax = minf(25.0f, 16.0f / maxf(D.rad[b], 0.3f));
+        if (a > amax) w = mul(w, amax / a);
+        if (D.cool[b] > 0.0f) D.cool[b] -= h;
+        LV pos = madd(lv4(D.pos[b]), v, h);
+        D.q[b] = f4q(q_int(lq4(D.q[b]), w, h));
+        v = mul(v, 1.0f - 0.01f * h);
+        w = mul(w, 1.0f - 0.05f * h);
+        bool small = D.count[b] <= 3u;
+        if (D.nc[b] > 0u && small) {
+            w = mul(w, 1.0f - minf(7.0f * h, 0.5f));
+            v = mul(v, 1.0f - minf(2.0f * h, 0.3f));
+        }
+        float v2 = dot(v, v);
+        float rr = D.rad[b];
+        float w2 = dot(w, w) * rr * rr;
+        float vt = small ? 0.16f : 0.06f, wt = small ? 0.3f : 0.04f;
+        if (v2 < vt && w2 < wt && D.nc[b] > 0u && D.maxr[b] < 0.95f) {
+            D.sleep[b] += h;
+            if (D.sleep[b] > 0.45f) { D.asleep[b] = 1; v = lv(0.0f, 0.0f, 0.0f); w = lv(0.0f, 0.0f, 0.0f); }
+        } else D.sleep[b] = 0.0f;
+        D.pos[b] = f4(pos); D.v[b] = f4(v); D.w[b] = f4(w);
+        if (pos.y < -15.0f || fabsf(pos.x) > 190.0f || fabsf(pos.z) > 190.0f || pos.y > 120.0f) kill_body_dev(D, b);
+    }
+}
+__global__ void __launch_bounds__(NT) k_wakes(LD D) {
+    GS(b, D.nbd) if (D.wake[b]) { D.wake[b] = 0; if (D.alive[b]) { D.asleep[b] = 0; D.sleep[b] = 0.0f; } }
+}
+
+// ---------------------------------------------------------------- load field
+__device__ __forceinline__ int linked(const LD& D, uint32_t i, int d, float* hp_out) {
+    uint32_t j = D.nb6[6 * (size_t)i + d];
+    if (j == NONE || D.mat[j] == 0 || D.body[j] != D.body[i]) return -1;
+    float4 bi = (d & 1) == 0 ? D.bond[i] : D.bond[j];
+    int ax = d >> 1;
+    float hp = ax == 0 ? bi.x : (ax == 1 ? bi.y : bi.z);
+    if (hp > 0.0f) { if (hp_out) *hp_out = hp; return (int)j; }
+    return -1;
+}
+__device__ __forceinline__ float rot_el(const LD& D, int b, int row, int col) {
+    float4 r = D.rot[3 * (size_t)b + row];
+    return col == 0 ? r.x : (col == 1 ? r.y : r.z);
+}
+__device__ __forceinline__ float cond_w(const LD& D, uint32_t i, uint32_t j, int a) {
+    const Tables& T = tables();
+    float k = minf(T.stiff[D.mat[i]], T.stiff[D.mat[j]]);
+    int b = D.body[i];
+    if (b < 0) return k * (a == 1 ? 1.0f : 0.3f);
+    float u = fabsf(rot_el(D, b, 1, a));
+    return k * (0.3f + 0.7f * u);
+}
+__device__ __forceinline__ float axis_up(const LD& D, uint32_t i, int a) {
+    int b = D.body[i];
+    if (b < 0) return a == 1 ? 1.0f : 0.0f;
+    return rot_el(D, b, 1, a);
+}
+__device__ float sor_brick(LD& D, uint32_t i, float omega, uint32_t frame) {
+    const Tables& T = tables();
+    int mi = D.mat[i];
+    float sk = 0.0f, skp = 0.0f;
+    for (int d = 0; d < 6; d++) {
+        int j = linked(D, i, d, nullptr);
+        if (j < 0) continue;
+        float k = cond_w(D, i, (uint32_t)j, d >> 1);
+        sk += k; skp += k * D.phi[j];
+    }
+    if (D.body[i] < 0) { if (D.bxyz[i].y == 0 && D.bond[i].w > 0.0f) sk += GROUNDK * T.stiff[mi]; }
+    else if (frame - D.sup[i] < 10u) sk += GROUNDK * T.stiff[mi];
+    if (sk > 0.0f) {
+        float target = (T.mass[mi] + D.ext[i] + skp) / sk;
+        float d = omega * (target - D.phi[i]);
+        D.phi[i] += d;
+        return fabsf(d);
+    }
+    return 0.0f;
+}
+__device__ __forceinline__ bool in_struct(const LD& D, uint32_t i, int s_local) {
+    int m = D.mat[i];
+    return D.sid[i] == s_local && m != 0 && m < SLUG && D.body[i] < 0;
+}
+// block max of two values; every thread gets the result
+template <int BT>
+__device__ __forceinline__ void block_max2(float& a, float& b, float* sh) {
+    for (int h = 32; h >= 1; h >>= 1) { a = fmaxf(a, __shfl_xor(a, h, 64)); b = fmaxf(b, __shfl_xor(b, h, 64)); }
+    int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { sh[wv] = a; sh[BT / 64 + wv] = b; }
+    __syncthreads();
+    a = sh[0]; b = sh[BT / 64];
+    for (int k = 1; k < BT / 64; k++) { a = fmaxf(a, sh[k]); b = fmaxf(b, sh[BT / 64 + k]); }
+    __syncthreads();
+}
+
+// one workgroup per structure; mode 0: the frame's adaptive sweeps (nmin > 0), mode 1: warm-up (nmin sweeps)
+constexpr int ST_NT = 1024;
+__global__ void __launch_bounds__(ST_NT) k_sor_struct(LD D, int mode) {
+    __shared__ float sh[2 * ST_NT / 64];
+    for (uint32_t s = blockIdx.x; s < D.nst; s += gridDim.x) {
+        int nmin = D.st_nmin[s];
+        if (nmin <= 0) continue;
+        if (mode == 0 && !D.st_act[s]) continue;
+        int sc = D.st_scene[s];
+        uint32_t frame = D.sc[sc].frame;
+        int sl = (int)s - D.sc[sc].st0;
+        int4 rb = D.st_rb[s];
+        float resid = D.st_resid[s];
+        int k = 0;
+        for (;;) {
+            float om = mode == 0 ? 1.85f : (nmin - k > 60 ? 1.9f : 1.0f);
+            float md = 0.0f, mp = 1.0f;
+            for (int par = 0; par < 2; par++) {
+                uint32_t o = par == 0 ? rb.x : rb.z, n = par == 0 ? rb.y : rb.w;
+                for (uint32_t t = threadIdx.x; t < n; t += ST_NT) {
+                    uint32_t i = D.st_red[o + t];
+                    if (!in_struct(D, i, sl)) continue;
+                    float d = sor_brick(D, i, om, frame);
+                    md = fmaxf(md, d);
+                    mp = fmaxf(mp, D.phi[i]);
+                }
+                __syncthreads();
+            }
+            block_max2<ST_NT>(md, mp, sh);
+            resid = md / mp;
+            k++;
+            if (mode == 0) { if (k >= nmin && (resid < 4e-4f || k >= 90)) break; }
+            else if (k >= nmin) break;
+        }
+        if (threadIdx.x == 0) D.st_resid[s] = resid;
+        __syncthreads();
+    }
+}
+
+// structures: clear the new resting loads, then mix them in
+__global__ void __launch_bounds__(NT) k_extn_clear(LD D) {
+    GS(i, D.nb) {
+        int sc = D.bxyz[i].w;
+        int s = D.sid[i];
+        const SceneK& S = D.sc[sc];
+        if (s < S.nst && D.st_act[S.st0 + s]) D.extn[i] = 0.0f;
+    }
+}
+__global__ void __launch_bounds__(NT) k_ext_mix(LD D) {
+    GS(i, D.nb) {
+        int sc = D.bxyz[i].w;
+        int s = D.sid[i];
+        const SceneK& S = D.sc[sc];
+        if (s < S.nst && D.st_act[S.st0 + s]) D.ext[i] = D.ext[i] * 0.8f + D.extn[i] * 0.2f;
+    }
+}
+// resting loads: a body's weight shared between the bricks that rest on a structure brick or on the ground
+__global__ void __launch_bounds__(NT) k_rest_count(LD D) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t b = wave; b < D.nbd; b += nwaves) {
+        if (!D.alive[b]) continue;
+        const SceneK S = D.sc[D.bd_scene[b]];
+        uint32_t o = D.boff[b], n = D.bcnt[b];
+        uint32_t ns = 0, ng = 0;
+        for (uint32_t t = lane; t < n; t += 64) {
+            uint32_t i = D.dyn[o + t];
+            LV p = lv4(D.wpos[i]);
+            if (p.y < RB + 0.05f) ng++;
+            float yb = p.y - RB - 0.02f;
+            int j = static_brick(D, S, fcell(p.x) + NX2, fcell(yb), fcell(p.z) + NX2);
+            bool hit = j >= 0 && D.sid[j] < S.nst && D.st_on[S.st0 + D.sid[j]];
+            D.ltarget[o + t] = hit ? j : -1;
+            if (hit) ns++;
+        }
+        for (int h = 32; h >= 1; h >>= 1) { ns += __shfl_xor(ns, h, 64); ng += __shfl_xor(ng, h, 64); }
+        if (lane == 0) D.bshare[b] = ns == 0 ? -1.0f : D.m[b] / (float)(ns + ng);
+    }
+}
+__global__ void __launch_bounds__(NT) k_rest_emit(LD D, int fill) {
+    GS(k, D.ndyn) {
+        int b = D.dyn_body[k];
+        if (!D.alive[b] || D.bshare[b] < 0.0f) continue;
+        int j = D.ltarget[k];
+        if (j < 0) continue;
+        if (!fill) atomicAdd(&D.lcnt[j], 1u);
+        else D.lent[D.loff[j] + atomicAdd(&D.lfill[j], 1u)] = k;
+    }
+}
+__global__ void __launch_bounds__(NT) k_rest_sum(LD D) {
+    GS(j, D.nb) {
+        uint32_t n = D.lcnt[j];
+        if (!n) continue;
+        float s = D.extn[j];
+        for (uint32_t e = 0; e < n; e++) s += D.bshare[D.dyn_body[D.lent[D.loff[j] + e]]];
+        D.extn[j] = s;
+    }
+}
+// pieces of 8+ bricks with no support stamped for 10 frames start their load field over
+__global__ void __launch_bounds__(NT) k_unsupported(LD D) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t b = wave; b < D.nbd; b += nwaves) {
+        if (!D.alive[b] || D.count[b] < 8u) continue;
+        uint32_t frame = D.sc[D.bd_scene[b]].frame;
+        uint32_t o = D.boff[b], n = D.bcnt[b];
+        bool any = false;
+        for (uint32_t t = lane; t < n; t += 64) if (frame - D.sup[D.dyn[o + t]] < 10u) any = true;
+        if (__ballot(any)) continue;
+        for (uint32_t t = lane; t < n; t += 64) D.phi[D.dyn[o + t]] = 0.0f;
+    }
+}
+__device__ __forceinline__ bool body_loaded(const LD& D, uint32_t b) {
+    return D.alive[b] && D.count[b] >= 8u && !D.asleep[b] && D.nc[b] > 0u && D.bcnt[b] > 0u && D.mat[D.dyn[D.boff[b]]] < SLUG;
+}
+// g - a loading of loaded pieces (src/lib.rs:1284-1308); velocities remembered for every live body
+__global__ void __launch_bounds__(NT) k_ga(LD D, float dt) {
+    const Tables& T = tables();
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    float idt = 1.0f / maxf(dt, 1e-3f);
+    for (uint32_t b = wave; b < D.nbd; b += nwaves) {
+        if (!D.alive[b]) continue;
+        LV v = lv4(D.v[b]), w = lv4(D.w[b]);
+        LV al = mul(sub(v, lv4(D.vlast[b])), idt);
+        LV aa = mul(sub(w, lv4(D.wlast[b])), idt);
+        bool loaded = body_loaded(D, b);
+        if (lane == 0) { D.vlast[b] = f4(v); D.wlast[b] = f4(w); }
+        if (!loaded) continue;
+        LV pb = lv4(D.pos[b]);
+        uint32_t o = D.boff[b], n = D.bcnt[b];
+        float nm = 0.0f;
+        for (uint32_t t = lane; t < n; t += 64) {
+            uint32_t i = D.dyn[o + t];
+            if (D.mat[i] == 0) continue;
+            LV a = add(al, cross(aa, sub(lv4(D.wpos[i]), pb)));
+            LV ge = lv(-a.x, -GR - a.y, -a.z);
+            float nn = minf(len(ge) / GR, 150.0f);
+            if (nn > nm) nm = nn;
+            D.ext[i] = T.mass[D.mat[i]] * (nn - 1.0f);
+        }
+        for (int h = 32; h >= 1; h >>= 1) nm = fmaxf(nm, __shfl_xor(nm, h, 64));
+        if (lane == 0) D.nmax[b] = nm;
+    }
+}
+// which pieces sweep their load field this frame
+__global__ void __launch_bounds__(NT) k_body_sor_prep(LD D) {
+    GS(b, D.nbd) {
+        uint8_t go = 0;
+        if (body_loaded(D, b) && D.count[b] <= 60000u) {
+            const SceneK& S = D.sc[D.bd_scene[b]];
+            uint32_t lb = b - (uint32_t)S.bd0;
+            go = !(D.resid[b] < 2e-4f && D.nmax[b] < 1.5f && ((S.frame + lb) % 4u) != 0u);
+        }
+        D.bd_sor[b] = go;
+    }
+}
+constexpr int BD_NT = 256;
+__global__ void __launch_bounds__(BD_NT) k_sor_body(LD D) {
+    __shared__ float sh[2 * BD_NT / 64];
+    for (uint32_t b = blockIdx.x; b < D.nbd; b += gridDim.x) {
+        if (!D.bd_sor[b]) continue;
+        uint32_t frame = D.sc[D.bd_scene[b]].frame;
+        uint32_t o = D.boff[b], n = D.bcnt[b];
+        int k = 0;
+        for (;;) {
+            float md = 0.0f, mp = 1.0f;
+            for (int par = 0; par < 2; par++) {
+                for (uint32_t t = threadIdx.x; t < n; t += BD_NT) {
+                    uint32_t i = D.dyn[o + t];
+                    if (D.mat[i] == 0) continue;
+                    int4 c = D.bxyz[i];
+                    if (((c.x + c.y + c.z) & 1) != par) continue;
+                    float d = sor_brick(D, i, 1.8f, frame);
+                    md = fmaxf(md, d);
+                    mp = fmaxf(mp, D.phi[i]);
+                }
+                __syncthreads();
+            }
+            block_max2<BD_NT>(md, mp, sh);
+            float r = md / mp;
+            k++;
+            if (threadIdx.x == 0) D.resid[b] = r;
+            if (k >= 2 && (r < 4e-4f || k >= 60)) break;
+        }
+        __syncthreads();
+    }
+}
+
+// joints: clear stress where it is judged, decide which pieces are judged
+__global__ void __launch_bounds__(NT) k_stress_clear(LD D) {
+    GS(i, D.nb) {
+        int sc = D.bxyz[i].w;
+        const SceneK& S = D.sc[sc];
+        int s = D.sid[i];
+        if (D.body[i] < 0) {
+            if (s < S.nst && D.st_act[S.st0 + s] && in_struct(D, i, s)) D.stress[i] = 0.0f;
+        } else {
+            int b = D.body[i];
+            if (D.mat[i] != 0 && body_loaded(D, (uint32_t)b) && D.count[b] <= 60000u) D.stress[i] = 0.0f;
+        }
+    }
+}
+__global__ void __launch_bounds__(NT) k_body_judge_prep(LD D, float dt) {
+    GS(b, D.nbd) {
+        uint8_t judge = 0;
+        if (body_loaded(D, b) && D.count[b] <= 60000u) {
+            const SceneK& S = D.sc[D.bd_scene[b]];
+            uint32_t lb = b - (uint32_t)S.bd0;
+            D.lt[b] += dt;
+            bool settled = D.resid[b] < 2e-4f && D.nmax[b] < 1.5f;
+            if (!(settled && ((S.frame + lb) % 4u) != 0u)) {
+                judge = 1;
+                D.bd_dt[b] = settled ? dt * 4.0f : dt;
+                D.bd_live[b] = (D.lt[b] > 0.3f && (D.resid[b] < 1e-2f || D.lt[b] > 0.6f)) || D.nmax[b] > 2.5f;
+                D.maxr[b] = 0.0f;
+            }
+        }
+        D.bd_judge[b] = judge;
+    }
+}
+// stress of brick i's three positive-axis bonds and its footing (src/lib.rs:1153-1227), judged on the field as it
+// stands; a bearing joint failing in compression is reported for the host to crush
+__device__ float stress_eval(LD& D, uint32_t i, float dt, bool live, unsigned long long key, int gst) {
+    const Tables& T = tables();
+    int mi = D.mat[i];
+    int bi = D.body[i];
+    bool dynb = bi >= 0;
+    float maxr = 0.0f;
+    float4 bd = D.bond[i];
+    float4 bm = D.bmax[i];
+    for (int a = 0; a < 3; a++) {
+        float hp = 0.0f;
+        int j = linked(D, i, a * 2, &hp);
+        if (j < 0) continue;
+        float f = cond_w(D, i, (uint32_t)j, a) * (D.phi[i] - D.phi[j]);
+        float u = axis_up(D, i, a);
+        float up = f > 0.0f ? maxf(u, 0.0f) : maxf(-u, 0.0f);
+        float tens = minf(T.tens[mi], T.tens[D.mat[j]]);
+        float capm = 1.0f - (1.0f - tens) * up;
+        float hpw = hp;
+        if (dynb) {
+            float cv = minf(T.capv[mi], T.capv[D.mat[j]]), ch = minf(T.caph[mi], T.caph[D.mat[j]]);
+            float lat = a == 1 ? cv : ch;
+            float au = fabsf(u);
+            hpw = hp * (cv * au + ch * (1.0f - au)) / maxf(lat, 1.0f);
+        }
+        float r = fabsf(f) / (hpw * capm);
+        atomic_max_pos((uint32_t*)&D.stress[i], r);
+        atomic_max_pos((uint32_t*)&D.stress[j], r);
+        if (r > maxr) maxr = r;
+        if (live && r > 1.0f) {
+            float bmaxa = a == 0 ? bm.x : (a == 1 ? bm.y : bm.z);
+            float nhp = dynb ? hp - hp * dt * T.duct[mi] * minf(r - 0.85f, 40.0f) : hp - hp * dt * 2.2f * minf(r - 0.85f, 2.5f);
+            if (nhp < bmaxa * 0.08f) {
+                if (up == 0.0f && fabsf(u) > 0.5f) {
+                    int k = T.capv[D.mat[j]] < T.capv[mi] ? j : (int)i;
+                    uint32_t e = atomicAdd(&D.ctr[C_NEV], 1u);
+                    if (e < D.evcap) { D.evkey[e] = key; D.ev[e] = make_int4((int)i, a, k, gst); }
+                    return maxr > r ? maxr : r;
+                }
+                nhp = 0.0f;
+                if (dynb) D.bd_dirty[bi] = 1;
+                else atomicOr(&D.st_flags[gst], 3u);
+            }
+            if (a == 0) bd.x = nhp; else if (a == 1) bd.y = nhp; else bd.z = nhp;
+            D.bond[i] = bd;
+            uint32_t e = atomicAdd(&D.ctr[C_NBW], 1u);
+            if (e < D.bwcap) { D.bwi[e] = 4u * i + (uint32_t)a; D.bwv[e] = nhp; }
+        }
+    }
+    if (!dynb && D.bxyz[i].y == 0 && bd.w > 0.0f) {
+        float f = GROUNDK * T.stiff[mi] * D.phi[i];
+        float r = f / bd.w;
+        atomic_max_pos((uint32_t*)&D.stress[i], r);
+        if (r > maxr) maxr = r;
+        if (live && r > 1.0f) {
+            float g = bd.w - bd.w * dt * 2.2f * (r - 0.85f);
+            if (g < bm.w * 0.08f) { g = 0.0f; atomicOr(&D.st_flags[gst], 3u); }
+            bd.w = g;
+            D.bond[i] = bd;
+            uint32_t e = atomicAdd(&D.ctr[C_NBW], 1u);
+            if (e < D.bwcap) { D.bwi[e] = 4u * i + 3u; D.bwv[e] = g; }
+        }
+    }
+    return maxr;
+}
+__global__ void __launch_bounds__(NT) k_stress_static(LD D, float dt) {
+    GS(i, D.nb) {
+        if (D.body[i] >= 0) continue;
+        int sc = D.bxyz[i].w;
+        const SceneK& S = D.sc[sc];
+        int s = D.sid[i];
+        if (s >= S.nst) continue;
+        int gst = S.st0 + s;
+        if (!D.st_act[gst] || !in_struct(D, i, s)) continue;
+        unsigned long long key = ((unsigned long long)(uint32_t)gst << 40) | (unsigned long long)i;
+        float r = stress_eval(D, i, dt, D.st_live[gst] != 0, key, gst);
+        atomic_max_pos(&D.st_maxr[gst], r);
+        atomic_max_pos(&D.sc_gmax[sc], r);
+    }
+}
+__global__ void __launch_bounds__(NT) k_stress_body(LD D) {
+    GS(k, D.ndyn) {
+        int b = D.dyn_body[k];
+        if (!D.bd_judge[b]) continue;
+        uint32_t i = D.dyn[k];
+        if (D.mat[i] == 0) continue;
+        unsigned long long key = (1ull << 63) | ((unsigned long long)(uint32_t)b << 40) | (unsigned long long)k;
+        float r = stress_eval(D, i, D.bd_dt[b], D.bd_live[b] != 0, key, -1);
+        atomic_max_pos((uint32_t*)&D.maxr[b], r);
+        atomic_max_pos(&D.sc_gmax[D.bd_scene[b]], r);
+    }
+}
+__global__ void __launch_bounds__(NT) k_lt_decay(LD D, float dt) {
+    GS(b, D.nbd) if (D.alive[b] && !body_loaded(D, b)) D.lt[b] = maxf(D.lt[b] - 2.0f * dt, 0.0f);
+}
+
+// per-scene counts: bodies, bricks in bodies, awake bodies
+__global__ void __launch_bounds__(NT) k_scene_stats(LD D, uint32_t* out) {
+    GS(b, D.nbd) {
+        if (!D.alive[b]) continue;
+        int s = D.bd_scene[b];
+        atomicAdd(&out[3 * s], 1u);
+        atomicAdd(&out[3 * s + 1], D.count[b]);
+        if (!D.asleep[b]) atomicAdd(&out[3 * s + 2], 1u);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) { GS(i, n) p[i] = v; }
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, size_t n) { GS(i, n) p[i] = v; }
+
+}  // namespace
+
+// ====================================================================== host
+namespace {
+
+struct HScene {
+    int b0 = 0, bcap = 0, nb = 0;          // bricks [b0, b0 + nb), capacity bcap
+    int bd0 = 0, dcap = 0, hw = 0, cursor = 0;
+    int st0 = 0, nst = 0;
+    uint32_t rng = 0x9E3779B9u, frame = 0, stamp_id = 0;
+    float time = 0.0f, blast_t = 0.0f, blast_r = 0.0f, blast_p = 0.0f;
+    LV blast_c{0.0f, 0.0f, 0.0f};
+    int crush_n = 0;
+    bool conn_any = false, finalized = false, active = true;
+    int gx0 = 0, gy0 = 0, gz0 = 0, gnx = 1, gny = 1, gnz = 1;
+    size_t goff = 0;
+    float stats[16] = {};
+    uint32_t dyn0 = 0, ndyn = 0;
+    uint32_t c0 = 0, ncs = 0;              // contact range of the last substep
+    bool host_dirty = false;               // host mirror changed since the last push
+};
+struct HStruct {
+    int scene = 0, lo = 0, hi = 0, mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, top0 = 0, top = 0, warm = 0;
+    uint32_t init = 0, now = 0;
+    float active = 0.0f, settle = 0.0f, maxr = 0.0f, resid = 0.0f;
+    bool solved = false, conn = false, broke = false;
+    int4 rb{0, 0, 0, 0};
+};
+
+// candidate contact records gathered on the device (plough / impact)
+struct CRec { uint32_t c; int32_t a, b, bo; uint32_t ba, bb; int32_t on; float p[3], n[3], jn, vn0; };
+
+__global__ void k_gather(LD D, const uint32_t* list, uint32_t n, CRec* out) {
+    GS(t, n) {
+        uint32_t c = list[t];
+        CRec r;
+        r.c = c; r.a = D.ca[c]; r.b = D.cb[c]; r.bo = D.cbo[c]; r.ba = D.cba[c]; r.bb = D.cbb[c]; r.on = D.con[c];
+        float4 p = D.cp[c], nn = D.cn[c];
+        r.p[0] = p.x; r.p[1] = p.y; r.p[2] = p.z; r.n[0] = nn.x; r.n[1] = nn.y; r.n[2] = nn.z;
+        r.jn = D.cjn[c]; r.vn0 = D.cvn0[c];
+        out[t] = r;
+    }
+}
+// contacts switched off by the host: contact k itself, and (j >= 0) every later contact of the scene against brick j
+__global__ void k_disable(LD D, const int2* kj, uint32_t n, uint32_t c_end) {
+    GS(c, c_end) {
+        for (uint32_t t = 0; t < n; t++) {
+            int2 e = kj[t];
+            if (c == (uint32_t)e.x || (e.y >= 0 && c > (uint32_t)e.x && D.cbb[c] == (uint32_t)e.y && D.cscene[c] == D.cscene[e.x])) D.con[c] = 0;
+        }
+    }
+}
+
+template <class T> struct DBuf {
+    T* p = nullptr; size_t n = 0;
+    int alloc(size_t cnt) {
+        if (p) (void)hipFree(p);
+        p = nullptr; n = cnt;
+        if (!cnt) cnt = 1;
+        if (hipMalloc(&p, sizeof(T) * cnt) != hipSuccess) { p = nullptr; return -1; }
+        return hipMemset(p, 0, sizeof(T) * cnt) == hipSuccess ? 0 : -1;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+struct Lat {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int K = 16, R = 8;
+    int bcap = 0, dcap = 0, scap = 0;
+    int nb = 0, nbd = 0;                   // bricks / body slots handed out to scenes
+    std::vector<HScene> sc;
+    std::vector<HStruct> sts;
+    // host mirror: bricks
+    std::vector<uint8_t> mat, sid, infill;
+    std::vector<int32_t> body;
+    std::vector<int4> bxyz;
+    std::vector<uint32_t> next, nb6, stamp, q;
+    std::vector<float4> bond, bmax, local, wpos;
+    std::vector<float> phi, ext, extn;
+    std::vector<int32_t> grid;
+    // host mirror: bodies
+    std::vector<uint8_t> alive, asleep, dirty, wake;
+    std::vector<uint32_t> first, count, nc;
+    std::vector<float4> pos, qq, v, w, vlast, wlast;
+    std::vector<LM> rot, iil, iiw;
+    std::vector<float> m, im, sleep, rad, cool, lt, resid, maxr, nmax;
+    // scratch
+    std::vector<uint32_t> lst, lst2, comp, cnt;
+    // dynamic slots
+    std::vector<uint32_t> dyn, boff, bcnt;
+    std::vector<int32_t> dyn_body;
+    uint32_t ndyn = 0;
+    bool lists_dirty = true;
+    // device
+    LD D{};
+    std::vector<void*> owned;
+    DBuf<SceneK> d_sc;
+    DBuf<int32_t> d_grid;
+    DBuf<CRec> d_rec;
+    DBuf<int2> d_dis;
+    DBuf<uint32_t> d_scstat;
+    uint32_t h_ctr[C_NCTR];
+    uint32_t ccap = 0, tsize = 0;
+    // timing
+    double ms_frame = 0.0, ms_host = 0.0;
+    int syncs = 0;
+
+    ~Lat() {
+        for (void* p : owned) if (p) (void)hipFree(p);
+        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release();
+        if (st) (void)hipStreamDestroy(st);
+    }
+    template <class T> int dalloc(T*& p, size_t n) {
+        p = nullptr;
+        if (hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)) != hipSuccess) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
+        owned.push_back((void*)p);
+        if (hipMemset(p, 0, sizeof(T) * (n ? n : 1)) != hipSuccess) return -1;
+        return 0;
+    }
+    template <class T> void dfree(T*& p) {
+        for (auto& o : owned) if (o == (void*)p) { (void)hipFree(o); o = nullptr; }
+        p = nullptr;
+    }
+};
+
+int lat_alloc(Lat* L) {
+    const size_t B = L->bcap, N = L->dcap, S = L->scap, NS = 256 * (size_t)S;
+    LD& D = L->D;
+    int r = 0;
+    r |= L->dalloc(D.mat, B); r |= L->dalloc(D.sid, B); r |= L->dalloc(D.body, B); r |= L->dalloc(D.bxyz, B);
+    r |= L->dalloc(D.nb6, 6 * B); r |= L->dalloc(D.bond, B); r |= L->dalloc(D.bmax, B); r |= L->dalloc(D.local, B);
+    r |= L->dalloc(D.wpos, B); r |= L->dalloc(D.phi, B); r |= L->dalloc(D.ext, B); r |= L->dalloc(D.extn, B);
+    r |= L->dalloc(D.stress, B); r |= L->dalloc(D.sup, B);
+    r |= L->dalloc(D.bd_scene, N); r |= L->dalloc(D.alive, N); r |= L->dalloc(D.asleep, N); r |= L->dalloc(D.wake, N);
+    r |= L->dalloc(D.pos, N); r |= L->dalloc(D.q, N); r |= L->dalloc(D.v, N); r |= L->dalloc(D.w, N);
+    r |= L->dalloc(D.vpre, N); r |= L->dalloc(D.wpre, N); r |= L->dalloc(D.vlast, N); r |= L->dalloc(D.wlast, N);
+    r |= L->dalloc(D.rot, 3 * N); r |= L->dalloc(D.iil, 3 * N); r |= L->dalloc(D.iiw, 3 * N);
+    r |= L->dalloc(D.m, N); r |= L->dalloc(D.im, N); r |= L->dalloc(D.sleep, N); r |= L->dalloc(D.rad, N);
+    r |= L->dalloc(D.cool, N); r |= L->dalloc(D.lt, N); r |= L->dalloc(D.resid, N); r |= L->dalloc(D.maxr, N);
+    r |= L->dalloc(D.nmax, N); r |= L->dalloc(D.nc, N); r |= L->dalloc(D.count, N); r |= L->dalloc(D.boff, N);
+    r |= L->dalloc(D.bcnt, N);
+    r |= L->dalloc(D.dyn, B); r |= L->dalloc(D.dyn_body, B);
+    r |= L->dalloc(D.st_scene, NS); r |= L->dalloc(D.st_lo, NS); r |= L->dalloc(D.st_hi, NS); r |= L->dalloc(D.st_act, NS);
+    r |= L->dalloc(D.st_on, NS); r |= L->dalloc(D.st_live, NS); r |= L->dalloc(D.st_nmin, NS); r |= L->dalloc(D.st_resid, NS);
+    r |= L->dalloc(D.st_maxr, NS); r |= L->dalloc(D.st_flags, NS); r |= L->dalloc(D.st_red, B); r |= L->dalloc(D.st_rb, NS);
+    r |= L->dalloc(D.sc_c0, S + 1); r |= L->dalloc(D.sc_gmax, S);
+    r |= L->dalloc(D.dslot, B); r |= L->dalloc(D.dkey, B); r |= L->dalloc(D.hent, B);
+    r |= L->dalloc(D.ccnt, B); r |= L->dalloc(D.coff, B);
+    r |= L->dalloc(D.mask, N); r |= L->dalloc(D.claim, N * (size_t)L->K);
+    r |= L->dalloc(D.colfill, KMAX + 1); r |= L->dalloc(D.coloff, KMAX + 1);
+    r |= L->dalloc(D.ovcnt, N); r |= L->dalloc(D.ovoff, N); r |= L->dalloc(D.ovfill, N);
+    r |= L->dalloc(D.lcnt, B); r |= L->dalloc(D.loff, B); r |= L->dalloc(D.lfill, B); r |= L->dalloc(D.lent, B);
+    r |= L->dalloc(D.ltarget, B); r |= L->dalloc(D.lshare, B); r |= L->dalloc(D.bshare, N);
+    r |= L->dalloc(D.bd_sor, N); r |= L->dalloc(D.bd_judge, N); r |= L->dalloc(D.bd_live, N); r |= L->dalloc(D.bd_dt, N);
+    r |= L->dalloc(D.bd_dirty, N);
+    r |= L->dalloc(D.ctr, C_NCTR);
+    D.pcap = (uint32_t)std::max<size_t>(4096, B);
+    r |= L->dalloc(D.plist, D.pcap); r |= L->dalloc(D.ilist, D.pcap); r |= L->dalloc(D.klist, N);
+    D.evcap = (uint32_t)std::max<size_t>(4096, B); D.bwcap = (uint32_t)std::max<size_t>(4096, 4 * B);
+    r |= L->dalloc(D.evkey, D.evcap); r |= L->dalloc(D.ev, D.evcap); r |= L->dalloc(D.bwi, D.bwcap); r |= L->dalloc(D.bwv, D.bwcap);
+    r |= L->dalloc(D.scan_tmp, sf::scan_scratch_words((uint32_t)std::max<size_t>(std::max(2 * B + 4096, N), 1u << 16)));
+    r |= L->d_sc.alloc(S); r |= L->d_scstat.alloc(3 * S); r |= L->d_rec.alloc(D.pcap); r |= L->d_dis.alloc(D.pcap);
+    D.sc = L->d_sc.p;
+    D.K = (uint32_t)L->K; D.R = (uint32_t)L->R;
+    return r ? -1 : 0;
+}
+
+// contact arrays sized to `cap`
+int lat_contacts_alloc(Lat* L, uint32_t cap) {
+    LD& D = L->D;
+    auto fr = [&](auto*& p) { L->dfree(p); };
+    fr(D.ca); fr(D.cb); fr(D.cbo); fr(D.cscene); fr(D.ccol); fr(D.cprop); fr(D.cba); fr(D.cbb); fr(D.con);
+    fr(D.cn); fr(D.cp); fr(D.cra); fr(D.crb); fr(D.ct1); fr(D.ct2);
+    fr(D.cdepth); fr(D.cmn); fr(D.cmt1); fr(D.cmt2); fr(D.cjn); fr(D.cjt1); fr(D.cjt2); fr(D.cbias); fr(D.cvn0);
+    fr(D.csmn); fr(D.csmt1); fr(D.csmt2); fr(D.collist); fr(D.ovent); fr(D.odv); fr(D.odw);
+    int r = 0;
+    r |= L->dalloc(D.ca, cap); r |= L->dalloc(D.cb, cap); r |= L->dalloc(D.cbo, cap); r |= L->dalloc(D.cscene, cap);
+    r |= L->dalloc(D.ccol, cap); r |= L->dalloc(D.cprop, cap); r |= L->dalloc(D.cba, cap); r |= L->dalloc(D.cbb, cap);
+    r |= L->dalloc(D.con, cap);
+    r |= L->dalloc(D.cn, cap); r |= L->dalloc(D.cp, cap); r |= L->dalloc(D.cra, cap); r |= L->dalloc(D.crb, cap);
+    r |= L->dalloc(D.ct1, cap); r |= L->dalloc(D.ct2, cap);
+    r |= L->dalloc(D.cdepth, cap); r |= L->dalloc(D.cmn, cap); r |= L->dalloc(D.cmt1, cap); r |= L->dalloc(D.cmt2, cap);
+    r |= L->dalloc(D.cjn, cap); r |= L->dalloc(D.cjt1, cap); r |= L->dalloc(D.cjt2, cap); r |= L->dalloc(D.cbias, cap);
+    r |= L->dalloc(D.cvn0, cap); r |= L->dalloc(D.csmn, cap); r |= L->dalloc(D.csmt1, cap); r |= L->dalloc(D.csmt2, cap);
+    r |= L->dalloc(D.collist, cap); r |= L->dalloc(D.ovent, 2 * (size_t)cap);
+    r |= L->dalloc(D.odv, 2 * (size_t)cap); r |= L->dalloc(D.odw, 2 * (size_t)cap);
+    D.ccap = cap;
+    L->ccap = cap;
+    return r ? -1 : 0;
+}
+int lat_hash_alloc(Lat* L, uint32_t ndyn) {
+    uint32_t t = 1024;
+    while (t < 2 * ndyn) t <<= 1;
+    if (t <= L->tsize) return 0;
+    LD& D = L->D;
+    L->dfree(D.hcnt); L->dfree(D.hoff); L->dfree(D.hfill);
+    int r = 0;
+    r |= L->dalloc(D.hcnt, t); r |= L->dalloc(D.hoff, t); r |= L->dalloc(D.hfill, t);
+    L->tsize = t;
+    D.tmask = t - 1;
+    return r ? -1 : 0;
+}
+
+// ---------------------------------------------------------------- host mirror helpers (the reference's bookkeeping)
+inline LV hv(float4 a) { return lv(a.x, a.y, a.z); }
+inline float4 hf(LV a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+inline float& bond_ref(Lat* L, int i, int a) { float4& b = L->bond[i]; return a == 0 ? b.x : (a == 1 ? b.y : (a == 2 ? b.z : b.w)); }
+inline float bmax_of(Lat* L, int i, int a) { float4 b = L->bmax[i]; return a == 0 ? b.x : (a == 1 ? b.y : (a == 2 ? b.z : b.w)); }
+inline float rnd(HScene& S) { uint32_t x = S.rng; x ^= x << 13; x ^= x >> 17; x ^= x << 5; S.rng = x; return (float)(x >> 8) / 16777216.0f; }
+LV rndv(HScene& S) {
+    for (;;) {
+        float a = rnd(S) * 2.0f - 1.0f, b = rnd(S) * 2.0f - 1.0f, c = rnd(S) * 2.0f - 1.0f;
+        LV x = lv(a, b, c);
+        float l = dot(x, x);
+        if (l > 0.01f && l <= 1.0f) return mul(x, 1.0f / sqrtf(l));
+    }
+}
+inline int hnbr(Lat* L, int i, int d) { uint32_t j = L->nb6[6 * (size_t)i + d]; return j == NONE ? -1 : (int)j; }
+int hlinked(Lat* L, int i, int d) {
+    int j = hnbr(L, i, d);
+    if (j < 0 || L->mat[j] == 0 || L->body[j] != L->body[i]) return -1;
+    float hp = (d & 1) == 0 ? bond_ref(L, i, d >> 1) : bond_ref(L, j, d >> 1);
+    return hp > 0.0f ? j : -1;
+}
+inline LV hbpos(Lat* L, int i) { return L->body[i] < 0 ? lat_pos(L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z) : hv(L->wpos[i]); }
+inline bool hin_struct(Lat* L, int i, int sl) { int m = L->mat[i]; return L->sid[i] == sl && m != 0 && m < SLUG && L->body[i] < 0; }
+int32_t hgget(Lat* L, const HScene& S, int x, int y, int z) {
+    x -= S.gx0; y -= S.gy0; z -= S.gz0;
+    if (x < 0 || y < 0 || z < 0 || x >= S.gnx || y >= S.gny || z >= S.gnz) return 0;
+    return L->grid[S.goff + ((size_t)y * S.gnz + z) * S.gnx + x];
+}
+
+void st_warm(Lat* L, int g) { HStruct& T = L->sts[g]; if (T.solved) return; T.solved = true; T.warm = 560; T.settle = 1e9f; }
+void st_activate(Lat* L, HScene& S, int sl) {
+    if (sl < 0 || sl >= S.nst) return;
+    HStruct& T = L->sts[S.st0 + sl];
+    if (!T.solved) st_warm(L, S.st0 + sl);
+    if (T.active < 4.0f) T.active = 4.0f;
+}
+void st_static_change(Lat* L, HScene& S, int i) { int s = L->sid[i]; if (s < S.nst) st_activate(L, S, s); }
+void remove_brick(Lat* L, HScene& S, int i) {
+    if (L->mat[i] == 0) return;
+    L->mat[i] = 0;
+    if (L->body[i] < 0) {
+        int s = L->sid[i];
+        if (s < S.nst) { L->sts[S.st0 + s].conn = true; S.conn_any = true; }
+        st_static_change(L, S, i);
+    } else L->dirty[L->body[i]] = 1;
+}
+void remove_brick_raw(Lat* L, int i) { L->mat[i] = 0; L->body[i] = -1; }
+uint32_t damage_brick(Lat* L, HScene& S, int i, float frac, float absolute) {
+    uint32_t broke = 0;
+    for (int d = 0; d < 6; d++) {
+        int j = hnbr(L, i, d);
+        if (j < 0 || L->mat[j] == 0) continue;
+        int o = (d & 1) == 0 ? i : j, a = d >> 1;
+        float& bd = bond_ref(L, o, a);
+        if (bd > 0.0f) {
+            bd -= bmax_of(L, o, a) * frac + absolute;
+            if (bd <= bmax_of(L, o, a) * 0.03f) { bd = 0.0f; broke++; }
+        }
+    }
+    float& g = bond_ref(L, i, 3);
+    if (g > 0.0f) {
+        g -= bmax_of(L, i, 3) * frac * 0.5f + absolute * 0.5f;
+        if (g <= 0.0f) { g = 0.0f; broke++; }
+    }
+    if (L->body[i] < 0) {
+        st_static_change(L, S, i);
+        if (broke > 0) { int s = L->sid[i]; if (s < S.nst) { HStruct& T = L->sts[S.st0 + s]; T.conn = true; S.conn_any = true; T.broke = true; } }
+    } else if (broke > 0) L->dirty[L->body[i]] = 1;
+    return broke;
+}
+void sever_brick(Lat* L, int i) {
+    L->bond[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int ds[3] = {1, 3, 5};
+    for (int t = 0; t < 3; t++) { int j = hnbr(L, i, ds[t]); if (j >= 0) bond_ref(L, j, ds[t] >> 1) = 0.0f; }
+}
+void wake_body(Lat* L, int b) { L->asleep[b] = 0; L->sleep[b] = 0.0f; }
+void kill_body_host(Lat* L, int b) {
+    uint32_t k = L->first[b];
+    while (k != NONE) {
+        uint32_t nx = L->next[k];
+        if (L->body[k] == b) { L->mat[k] = 0; L->body[k] = -1; }
+        k = nx;
+    }
+    L->alive[b] = 0; L->first[b] = NONE;
+    L->lists_dirty = true;
+}
+void clamp_vel(Lat* L, int b) {
+    LV vv = hv(L->v[b]);
+    float vl = len(vv);
+    if (vl > 60.0f) L->v[b] = hf(mul(vv, 60.0f / vl));
+    LV ww = hv(L->w[b]);
+    float a = len(ww);
+    float amax = minf(25.0f, 16.0f / maxf(L->rad[b], 0.3f));
+    if (a > amax) L->w[b] = hf(mul(ww, amax / a));
+}
+void impulse_at(Lat* L, int b, LV p, LV j) {
+    L->v[b] = hf(madd(hv(L->v[b]), j, L->im[b]));
+    LV r = sub(p, hv(L->pos[b]));
+    L->w[b] = hf(add(hv(L->w[b]), mv(L->iiw[b], cross(r, j))));
+}
+void apply_blast_to_body(Lat* L, HScene& S, int b) {
+    if (S.blast_t <= 0.0f) return;
+    for (uint32_t k = L->first[b]; k != NONE; k = L->next[k]) {
+        LV p = hv(L->wpos[k]);
+        LV d = sub(p, S.blast_c);
+        float dist = len(d);
+        if (dist < S.blast_r) {
+            float f = S.blast_p * 1600.0f * (1.0f - dist / S.blast_r);
+            LV dir = dist > 1e-3f ? mul(d, 1.0f / dist) : lv(0.0f, 1.0f, 0.0f);
+            dir = add(dir, lv(0.0f, 0.35f, 0.0f));
+            float mm = maxf(tables().mass[L->mat[k]], 25.0f);
+            impulse_at(L, b, p, mul(dir, f * mm / 300.0f));
+        }
+    }
+    clamp_vel(L, b);
+}
+void wake_near(Lat* L, HScene& S, LV p, float r) {
+    for (int lb = 0; lb < S.hw; lb++) {
+        int b = S.bd0 + lb;
+        if (L->alive[b] && L->asleep[b] && len(sub(hv(L->pos[b]), p)) < r + L->rad[b]) wake_body(L, b);
+    }
+}
+int alloc_body(Lat* L, HScene& S) {
+    for (int k = 0; k < S.dcap; k++) {
+        int lb = (S.cursor + k) % S.dcap;
+        int b = S.bd0 + lb;
+        if (!L->alive[b]) {
+            S.cursor = (lb + 1) % S.dcap;
+            if (lb + 1 > S.hw) S.hw = lb + 1;
+            return b;
+        }
+    }
+    return -1;
+}
+const LM IDM = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+void setup_body(Lat* L, int b, int s, int e, LV ppos, const LM& prot, LQ pq, LV pv, LV pw) {
+    const Tables& T = tables();
+    L->lt[b] = 0.0f; L->resid[b] = 1.0f; L->maxr[b] = 0.0f;
+    float mass = 0.0f;
+    LV cl = lv(0.0f, 0.0f, 0.0f);
+    for (int t = s; t < e; t++) { int i = L->lst2[t]; float mm = T.mass[L->mat[i]]; mass += mm; cl = madd(cl, hv(L->local[i]), mm); }
+    cl = mul(cl, 1.0f / mass);
+    float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float radv = 0.0f;
+    const float cube = S * S / 6.0f;
+    for (int t = s; t < e; t++) {
+        int i = L->lst2[t];
+        float mm = T.mass[L->mat[i]];
+        LV r = sub(hv(L->local[i]), cl);
+        L->local[i] = hf(r);
+        float rr = dot(r, r);
+        radv = maxf(radv, rr);
+        it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
+        it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
+        L->body[i] = b;
+        L->next[i] = t + 1 < e ? L->lst2[t + 1] : NONE;
+    }
+    it[3] = it[1]; it[6] = it[2]; it[7] = it[5];
+    LM itm;
+    for (int k = 0; k < 9; k++) itm.m[k] = it[k];
+    LM iilm = minv(itm);
+    LV off = mv(prot, cl);
+    L->alive[b] = 1;
+    L->first[b] = L->lst2[s];
+    L->count[b] = (uint32_t)(e - s);
+    L->pos[b] = hf(add(ppos, off));
+    L->qq[b] = f4q(pq); L->rot[b] = prot;
+    L->v[b] = hf(add(pv, cross(pw, off)));
+    L->w[b] = hf(pw);
+    L->vlast[b] = L->v[b]; L->wlast[b] = hf(pw); L->nmax[b] = 1.0f;
+    L->m[b] = mass; L->im[b] = 1.0f / mass;
+    L->iil[b] = iilm;
+    LM t1 = mmul(prot, iilm), pt = mt(prot);
+    L->iiw[b] = mmul(t1, pt);
+    L->sleep[b] = 0.0f; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0; L->cool[b] = 0.15f;
+    L->rad[b] = sqrtf(radv) + H * 1.8f;
+    for (int t = s; t < e; t++) { int i = L->lst2[t]; L->wpos[i] = hf(add(hv(L->pos[b]), mv(prot, hv(L->local[i])))); }
+    L->lists_dirty = true;
+}
+void rebuild_body(Lat* L, HScene& S, int id) {
+    L->dirty[id] = 0;
+    int n = 0;
+    uint32_t k = L->first[id];
+    while (k != NONE) {
+        uint32_t nx = L->next[k];
+        if (L->body[k] == id) { if (L->mat[k] != 0) L->lst[n++] = k; else L->body[k] = -1; }
+        k = nx;
+    }
+    L->lists_dirty = true;
+    if (n == 0) { L->alive[id] = 0; L->first[id] = NONE; return; }
+    uint32_t sid = ++S.stamp_id;
+    uint32_t ncomp = 0;
+    for (int t = 0; t < n; t++) {
+        uint32_t s0 = L->lst[t];
+        if (L->stamp[s0] == sid) continue;
+        L->stamp[s0] = sid; L->comp[s0] = ncomp;
+        int qh = 0, qt = 0;
+        L->q[qt++] = s0;
+        while (qh < qt) {
+            int i = (int)L->q[qh++];
+            for (int d = 0; d < 6; d++) {
+                int j = hlinked(L, i, d);
+                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->comp[j] = ncomp; L->q[qt++] = (uint32_t)j; }
+            }
+        }
+        ncomp++;
+    }
+    int ncmp = (int)ncomp;
+    for (int c = 0; c <= ncmp; c++) L->cnt[c] = 0;
+    for (int t = 0; t < n; t++) L->cnt[L->comp[L->lst[t]] + 1]++;
+    for (int c = 0; c < ncmp; c++) L->cnt[c + 1] += L->cnt[c];
+    for (int t = 0; t < n; t++) { int i = L->lst[t]; uint32_t c = L->comp[i]; L->lst2[L->cnt[c]++] = (uint32_t)i; }
+    LV ppos = hv(L->pos[id]), pv = hv(L->v[id]), pw = hv(L->w[id]);
+    LM prot = L->rot[id];
+    LQ pq = lq4(L->qq[id]);
+    for (int c = 0; c < ncmp; c++) {
+        int s0 = c == 0 ? 0 : (int)L->cnt[c - 1];
+        int e = (int)L->cnt[c];
+        if (c > 0 && e - s0 == 1 && S.stats[3] > 1500.0f) {
+            int i = L->lst2[s0];
+            int mt_ = L->mat[i];
+            if (mt_ != STEEL && rnd(S) < minf((S.stats[3] - 1500.0f) / 2000.0f, 0.85f)) { remove_brick_raw(L, i); continue; }
+        }
+        int b = c == 0 ? id : alloc_body(L, S);
+        if (b < 0) { for (int t = s0; t < e; t++) remove_brick_raw(L, L->lst2[t]); continue; }
+        float keep_cool = L->cool[id], keep_lt = L->lt[id];
+        setup_body(L, b, s0, e, ppos, prot, pq, pv, pw);
+        if (c == 0) { L->cool[b] = keep_cool; L->lt[b] = keep_lt; }
+    }
+}
+void conn_struct(Lat* L, HScene& S, int sl) {
+    HStruct& T = L->sts[S.st0 + sl];
+    uint32_t sid = ++S.stamp_id;
+    int qh = 0, qt = 0;
+    for (int i = T.lo; i < T.hi; i++)
+        if (hin_struct(L, i, sl) && L->bxyz[i].y == 0 && L->bond[i].w > 0.0f) { L->stamp[i] = sid; L->q[qt++] = (uint32_t)i; }
+    while (qh < qt) {
+        int i = (int)L->q[qh++];
+        for (int d = 0; d < 6; d++) {
+            int j = hlinked(L, i, d);
+            if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->q[qt++] = (uint32_t)j; }
+        }
+    }
+    bool retry = false;
+    for (int s0 = T.lo; s0 < T.hi; s0++) {
+        if (!hin_struct(L, s0, sl) || L->stamp[s0] == sid) continue;
+        int n = 0;
+        L->stamp[s0] = sid; L->lst2[n++] = (uint32_t)s0;
+        int h = 0;
+        while (h < n) {
+            int i = (int)L->lst2[h++];
+            for (int d = 0; d < 6; d++) {
+                int j = hlinked(L, i, d);
+                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->lst2[n++] = (uint32_t)j; }
+            }
+        }
+        int b = alloc_body(L, S);
+        if (b < 0) { retry = true; continue; }
+        for (int t = 0; t < n; t++) {
+            int i = L->lst2[t];
+            L->local[i] = hf(lat_pos(L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z));
+            L->phi[i] = 0.0f; L->ext[i] = 0.0f; L->extn[i] = 0.0f;
+        }
+        LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
+        setup_body(L, b, 0, n, lv(0.0f, 0.0f, 0.0f), IDM, idq, lv(0.0f, 0.0f, 0.0f), lv(0.0f, 0.0f, 0.0f));
+        apply_blast_to_body(L, S, b);
+        wake_near(L, S, hv(L->pos[b]), L->rad[b] + 1.5f);
+    }
+    if (retry) { T.conn = true; S.conn_any = true; }
+    uint32_t cntv = 0; int top = 0;
+    for (int i = T.lo; i < T.hi; i++) if (hin_struct(L, i, sl)) { cntv++; if (L->bxyz[i].y + 1 > top) top = L->bxyz[i].y + 1; }
+    T.now = cntv; T.top = top;
+}
+void connectivity(Lat* L, HScene& S) {
+    if (!S.conn_any) return;
+    S.conn_any = false;
+    for (int sl = 0; sl < S.nst; sl++) { HStruct& T = L->sts[S.st0 + sl]; if (T.conn) { T.conn = false; conn_struct(L, S, sl); } }
+}
+uint32_t crush_at(Lat* L, HScene& S, int start, float e, LV p, uint32_t cap) {
+    const Tables& T = tables();
+    int m0 = L->mat[start];
+    if (m0 == 0 || m0 >= SLUG) return 0;
+    if (e < T.eb[m0] * 0.15f) return 0;
+    int bodyv = L->body[start];
+    int sidv = L->sid[start];
+    uint32_t st = ++S.stamp_id;
+    L->q[0] = (uint32_t)start;
+    int qn = 1; L->stamp[start] = st;
+    int head = 0;
+    uint32_t crushed = 0;
+    uint32_t maxn = cap < 600 ? cap : 600;
+    int ncr = 0;
+    while (head < qn && crushed < maxn) {
+        int best = head; float bd = 1e18f;
+        int lim = head + 24 < qn ? head + 24 : qn;
+        for (int t = head; t < lim; t++) { int i = (int)L->q[t]; LV d = sub(hbpos(L, i), p); float dd = dot(d, d); if (dd < bd) { bd = dd; best = t; } }
+        std::swap(L->q[head], L->q[best]);
+        int i = (int)L->q[head]; head++;
+        int mm = L->mat[i];
+        if (mm == 0) continue;
+        float cost = T.eb[mm];
+        if (e < cost) { damage_brick(L, S, i, 0.6f * e / cost, 0.0f); break; }
+        e -= cost;
+        L->lst2[ncr++] = (uint32_t)i; crushed++;
+        for (int d = 0; d < 6; d++) {
+            int j = hlinked(L, i, d);
+            if (j < 0 || L->stamp[j] == st || L->body[j] != bodyv || L->sid[j] != sidv) continue;
+            if (qn >= S.bcap) break;
+            L->stamp[j] = st; L->q[qn++] = (uint32_t)j;
+        }
+    }
+    if (ncr == 0) return 0;
+    for (int t = 0; t < ncr; t++) {
+        int i = (int)L->lst2[t];
+        int mm = L->mat[i];
+        if (mm == 0) continue;
+        if (rnd(S) < T.pulv[mm]) remove_brick(L, S, i);
+        else {
+            sever_brick(L, i);
+            if (bodyv < 0) { int s = L->sid[i]; if (s < S.nst) { L->sts[S.st0 + s].conn = true; S.conn_any = true; } st_static_change(L, S, i); }
+            else L->dirty[bodyv] = 1;
+        }
+    }
+    return crushed;
+}
+}  // namespace
+
+namespace {
+
+enum : int { G_MAT = 1, G_BOND = 2, G_BRICK = 4, G_BODY = 8, G_ALL = 15 };
+
+template <class T> int cp_down(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) SF_HIP(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    return 0;
+}
+template <class T> int cp_up(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) SF_HIP(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return 0;
+}
+std::vector<float4> g_m3;  // staging for matrices
+
+int pull(Lat* L, int si, int groups) {
+    HScene& S = L->sc[si];
+    LD& D = L->D;
+    hipStream_t s = L->st;
+    size_t b0 = S.b0, nb = S.nb, d0 = S.bd0, nd = S.hw;
+    int r = 0;
+    if (groups & G_MAT) { r |= cp_down(&L->mat[b0], D.mat + b0, nb, s); r |= cp_down(&L->body[b0], D.body + b0, nb, s); }
+    if (groups & G_BOND) r |= cp_down(&L->bond[b0], D.bond + b0, nb, s);
+    if (groups & G_BRICK) {
+        r |= cp_down(&L->wpos[b0], D.wpos + b0, nb, s); r |= cp_down(&L->phi[b0], D.phi + b0, nb, s);
+        r |= cp_down(&L->ext[b0], D.ext + b0, nb, s); r |= cp_down(&L->extn[b0], D.extn + b0, nb, s);
+    }
+    std::vector<float4> rot(3 * nd), iiw(3 * nd);
+    if ((groups & G_BODY) && nd) {
+        r |= cp_down(&L->alive[d0], D.alive + d0, nd, s); r |= cp_down(&L->asleep[d0], D.asleep + d0, nd, s);
+        r |= cp_down(&L->wake[d0], D.wake + d0, nd, s);
+        r |= cp_down(&L->pos[d0], D.pos + d0, nd, s); r |= cp_down(&L->qq[d0], D.q + d0, nd, s);
+        r |= cp_down(&L->v[d0], D.v + d0, nd, s); r |= cp_down(&L->w[d0], D.w + d0, nd, s);
+        r |= cp_down(&L->vlast[d0], D.vlast + d0, nd, s); r |= cp_down(&L->wlast[d0], D.wlast + d0, nd, s);
+        r |= cp_down(rot.data(), D.rot + 3 * d0, 3 * nd, s); r |= cp_down(iiw.data(), D.iiw + 3 * d0, 3 * nd, s);
+        r |= cp_down(&L->sleep[d0], D.sleep + d0, nd, s); r |= cp_down(&L->cool[d0], D.cool + d0, nd, s);
+        r |= cp_down(&L->lt[d0], D.lt + d0, nd, s); r |= cp_down(&L->resid[d0], D.resid + d0, nd, s);
+        r |= cp_down(&L->maxr[d0], D.maxr + d0, nd, s); r |= cp_down(&L->nmax[d0], D.nmax + d0, nd, s);
+        r |= cp_down(&L->nc[d0], D.nc + d0, nd, s);
+    }
+    if (r) return r;
+    SF_HIP(hipStreamSynchronize(s));
+    L->syncs++;
+    if ((groups & G_BODY) && nd)
+        for (size_t b = 0; b < nd; b++) {
+            LM a, c;
+            for (int k = 0; k < 3; k++) {
+                float4 x = rot[3 * b + k], y = iiw[3 * b + k];
+                a.m[3 * k] = x.x; a.m[3 * k + 1] = x.y; a.m[3 * k + 2] = x.z;
+                c.m[3 * k] = y.x; c.m[3 * k + 1] = y.y; c.m[3 * k + 2] = y.z;
+            }
+            L->rot[d0 + b] = a; L->iiw[d0 + b] = c;
+        }
+    return 0;
+}
+int push(Lat* L, int si, int groups) {
+    HScene& S = L->sc[si];
+    LD& D = L->D;
+    hipStream_t s = L->st;
+    size_t b0 = S.b0, nb = S.nb, d0 = S.bd0, nd = S.hw;
+    int r = 0;
+    if (groups & G_MAT) { r |= cp_up(D.mat + b0, &L->mat[b0], nb, s); r |= cp_up(D.body + b0, &L->body[b0], nb, s); }
+    if (groups & G_BOND) r |= cp_up(D.bond + b0, &L->bond[b0], nb, s);
+    if (groups & G_BRICK) {
+        r |= cp_up(D.local + b0, &L->local[b0], nb, s); r |= cp_up(D.wpos + b0, &L->wpos[b0], nb, s);
+        r |= cp_up(D.phi + b0, &L->phi[b0], nb, s); r |= cp_up(D.ext + b0, &L->ext[b0], nb, s);
+        r |= cp_up(D.extn + b0, &L->extn[b0], nb, s);
+    }
+    std::vector<float4> rot(3 * nd), iil(3 * nd), iiw(3 * nd);
+    if ((groups & G_BODY) && nd) {
+        for (size_t b = 0; b < nd; b++)
+            for (int k = 0; k < 3; k++) {
+                const LM &a = L->rot[d0 + b], &c = L->iil[d0 + b], &e = L->iiw[d0 + b];
+                rot[3 * b + k] = make_float4(a.m[3 * k], a.m[3 * k + 1], a.m[3 * k + 2], 0.0f);
+                iil[3 * b + k] = make_float4(c.m[3 * k], c.m[3 * k + 1], c.m[3 * k + 2], 0.0f);
+                iiw[3 * b + k] = make_float4(e.m[3 * k], e.m[3 * k + 1], e.m[3 * k + 2], 0.0f);
+            }
+        r |= cp_up(D.alive + d0, &L->alive[d0], nd, s); r |= cp_up(D.asleep + d0, &L->asleep[d0], nd, s);
+        r |= cp_up(D.wake + d0, &L->wake[d0], nd, s);
+        r |= cp_up(D.pos + d0, &L->pos[d0], nd, s); r |= cp_up(D.q + d0, &L->qq[d0], nd, s);
+        r |= cp_up(D.v + d0, &L->v[d0], nd, s); r |= cp_up(D.w + d0, &L->w[d0], nd, s);
+        r |= cp_up(D.vlast + d0, &L->vlast[d0], nd, s); r |= cp_up(D.wlast + d0, &L->wlast[d0], nd, s);
+        r |= cp_up(D.rot + 3 * d0, rot.data(), 3 * nd, s); r |= cp_up(D.iil + 3 * d0, iil.data(), 3 * nd, s);
+        r |= cp_up(D.iiw + 3 * d0, iiw.data(), 3 * nd, s);
+        r |= cp_up(D.m + d0, &L->m[d0], nd, s); r |= cp_up(D.im + d0, &L->im[d0], nd, s);
+        r |= cp_up(D.sleep + d0, &L->sleep[d0], nd, s); r |= cp_up(D.rad + d0, &L->rad[d0], nd, s);
+        r |= cp_up(D.cool + d0, &L->cool[d0], nd, s); r |= cp_up(D.lt + d0, &L->lt[d0], nd, s);
+        r |= cp_up(D.resid + d0, &L->resid[d0], nd, s); r |= cp_up(D.maxr + d0, &L->maxr[d0], nd, s);
+        r |= cp_up(D.nmax + d0, &L->nmax[d0], nd, s); r |= cp_up(D.count + d0, &L->count[d0], nd, s);
+        r |= cp_up(D.nc + d0, &L->nc[d0], nd, s);
+    }
+    if (r) return r;
+    SF_HIP(hipStreamSynchronize(s));  // staging vectors go out of scope
+    L->syncs++;
+    return 0;
+}
+
+int upload_scenes(Lat* L);
+// dynamic slots: every live body's bricks in list order, bodies in index order, scenes in order
+int rebuild_csr(Lat* L) {
+    if (!L->lists_dirty) return 0;
+    L->lists_dirty = false;
+    uint32_t k = 0;
+    for (int b = 0; b < L->nbd; b++) { L->boff[b] = 0; L->bcnt[b] = 0; }
+    for (size_t si = 0; si < L->sc.size(); si++) {
+        HScene& S = L->sc[si];
+        S.dyn0 = k;
+        for (int lb = 0; lb < S.hw; lb++) {
+            int b = S.bd0 + lb;
+            if (!L->alive[b]) continue;
+            L->boff[b] = k;
+            for (uint32_t i = L->first[b]; i != NONE; i = L->next[i]) { L->dyn[k] = i; L->dyn_body[k] = b; k++; }
+            L->bcnt[b] = k - L->boff[b];
+        }
+        S.ndyn = k - S.dyn0;
+    }
+    L->ndyn = k;
+    L->D.ndyn = k;
+    if (lat_hash_alloc(L, k)) return -1;
+    hipStream_t s = L->st;
+    int r = 0;
+    r |= cp_up(L->D.dyn, L->dyn.data(), k, s); r |= cp_up(L->D.dyn_body, L->dyn_body.data(), k, s);
+    r |= cp_up(L->D.boff, L->boff.data(), L->nbd, s); r |= cp_up(L->D.bcnt, L->bcnt.data(), L->nbd, s);
+    if (r) return r;
+    SF_HIP(hipStreamSynchronize(s));
+    return upload_scenes(L);  // the scenes' first dynamic slots moved
+}
+
+int upload_scenes(Lat* L) {
+    std::vector<SceneK> k(L->sc.size());
+    for (size_t si = 0; si < L->sc.size(); si++) {
+        const HScene& S = L->sc[si];
+        SceneK& x = k[si];
+        x.frame = S.frame; x.blast_t = S.blast_t;
+        x.gx0 = S.gx0; x.gy0 = S.gy0; x.gz0 = S.gz0; x.gnx = S.gnx; x.gny = S.gny; x.gnz = S.gnz; x.goff = (uint32_t)S.goff;
+        x.bd0 = S.bd0; x.nbd = S.dcap; x.dyn0 = S.dyn0; x.ndyn = S.ndyn; x.st0 = S.st0; x.nst = S.nst; x.b0 = S.b0; x.nb = S.nb;
+    }
+    SF_HIP(hipMemcpyAsync(L->d_sc.p, k.data(), sizeof(SceneK) * k.size(), hipMemcpyHostToDevice, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    return 0;
+}
+
+#define LAUNCH(kern, grid, block, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, L->st, __VA_ARGS__)
+
+int fill_u32(Lat* L, uint32_t* p, uint32_t v, size_t n) { if (n) LAUNCH(k_fill_u32, sf_grid(n), NT, p, v, n); return 0; }
+// every counter but the killed-body count, which accumulates until the host has drained the list
+void reset_ctr(Lat* L) {
+    fill_u32(L, L->D.ctr, 0, C_NKILL);
+    fill_u32(L, L->D.ctr + C_NKILL + 1, 0, C_NCTR - C_NKILL - 1);
+}
+int read_ctr(Lat* L) {
+    SF_HIP(hipMemcpyAsync(L->h_ctr, L->D.ctr, sizeof(L->h_ctr), hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    return 0;
+}
+
+// bodies killed on the device (pinch, out of bounds): the host mirror follows
+int drain_kills(Lat* L) {
+    uint32_t n = L->h_ctr[C_NKILL];
+    if (!n) return 0;
+    n = std::min<uint32_t>(n, (uint32_t)L->nbd);
+    std::vector<int32_t> k(n);
+    SF_HIP(hipMemcpyAsync(k.data(), L->D.klist, sizeof(int32_t) * n, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    for (uint32_t t = 0; t < n; t++) kill_body_host(L, k[t]);
+    L->h_ctr[C_NKILL] = 0;
+    uint32_t z = 0;
+    SF_HIP(hipMemcpyAsync(L->D.ctr + C_NKILL, &z, 4, hipMemcpyHostToDevice, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    return 0;
+}
+
+int scene_of_contact(Lat* L, uint32_t c) {
+    for (size_t si = 0; si < L->sc.size(); si++) { const HScene& S = L->sc[si]; if (c >= S.c0 && c < S.c0 + S.ncs) return (int)si; }
+    return -1;
+}
+
+int gather(Lat* L, const uint32_t* list, uint32_t n, std::vector<CRec>& out) {
+    out.resize(n);
+    if (!n) return 0;
+    LAUNCH(k_gather, sf_grid(n), NT, L->D, list, n, L->d_rec.p);
+    SF_HIP(hipMemcpyAsync(out.data(), L->d_rec.p, sizeof(CRec) * n, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    std::sort(out.begin(), out.end(), [](const CRec& a, const CRec& b) { return a.c < b.c; });
+    return 0;
+}
+
+// (src/lib.rs:1493-1550) applied in contact order on the host mirror
+int plough_phase(Lat* L, uint32_t n) {
+    n = std::min(n, L->D.pcap);
+    std::vector<CRec> rec;
+    if (gather(L, L->D.plist, n, rec)) return -1;
+    std::vector<int2> dis;
+    std::vector<uint8_t> pulled(L->sc.size(), 0);
+    const Tables& T = tables();
+    for (const CRec& c : rec) {
+        int si = scene_of_contact(L, c.c);
+        if (si < 0) continue;
+        HScene& S = L->sc[si];
+        if (!pulled[si]) { if (pull(L, si, G_ALL)) return -1; pulled[si] = 1; }
+        int j = (int)c.bb;
+        if (L->mat[j] == 0 || L->body[j] >= 0) { dis.push_back(make_int2((int)c.c, -1)); continue; }
+        int a = c.a;
+        LV p = lv(c.p[0], c.p[1], c.p[2]), nn = lv(c.n[0], c.n[1], c.n[2]);
+        LV ra = sub(p, hv(L->pos[a]));
+        float vn = -dot(add(hv(L->v[a]), cross(hv(L->w[a]), ra)), nn);
+        if (vn < 2.0f) continue;
+        float meff = L->m[a] * 3.0f / maxf((float)L->nc[a], 3.0f);
+        float mom = minf(meff, L->m[a]) * vn;
+        int mj = L->mat[j];
+        if (mj >= SLUG || mom < T.capv[mj] * 0.35f) continue;
+        if (L->count[a] > 40) {
+            float f = clampf(mom / (T.capv[mj] * 1.5f), 0.15f, 0.9f);
+            damage_brick(L, S, j, f, 0.0f);
+            continue;
+        }
+        sever_brick(L, j);
+        int sl = L->sid[j];
+        if (sl < S.nst) { L->sts[S.st0 + sl].conn = true; S.conn_any = true; }
+        st_static_change(L, S, j);
+        int b = alloc_body(L, S);
+        if (b < 0) { dis.push_back(make_int2((int)c.c, -1)); continue; }
+        L->local[j] = hf(lat_pos(L->bxyz[j].x, L->bxyz[j].y, L->bxyz[j].z));
+        L->lst2[0] = (uint32_t)j;
+        LV rv = rndv(S);
+        LV vv = madd(mul(hv(L->v[a]), 0.75f), rv, 2.0f);
+        LV ww = mul(rndv(S), 6.0f);
+        LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
+        setup_body(L, b, 0, 1, lv(0.0f, 0.0f, 0.0f), IDM, idq, vv, lv(0.0f, 0.0f, 0.0f));
+        L->w[b] = hf(ww);
+        float mm = L->m[a];
+        float f = mm / (mm + T.mass[mj] * 1.5f);
+        L->v[a] = hf(mul(hv(L->v[a]), f));
+        dis.push_back(make_int2((int)c.c, j));
+    }
+    for (size_t si = 0; si < pulled.size(); si++) if (pulled[si] && push(L, (int)si, G_ALL)) return -1;
+    if (!dis.empty()) {
+        if (dis.size() > L->D.pcap) { sf::set_error(SF_E_CAP, "too many plough events"); return SF_E_CAP; }
+        SF_HIP(hipMemcpyAsync(L->d_dis.p, dis.data(), sizeof(int2) * dis.size(), hipMemcpyHostToDevice, L->st));
+        LAUNCH(k_disable, sf_grid(L->h_ctr[C_NCS]), NT, L->D, L->d_dis.p, (uint32_t)dis.size(), L->h_ctr[C_NCS]);
+    }
+    return rebuild_csr(L);
+}
+
+// (src/lib.rs:1610-1642) in contact order, a budget of 2500 crushed bricks per scene and substep
+int impact_phase(Lat* L, uint32_t n) {
+    n = std::min(n, L->D.pcap);
+    std::vector<CRec> rec;
+    if (gather(L, L->D.ilist, n, rec)) return -1;
+    std::vector<uint8_t> pulled(L->sc.size(), 0);
+    std::vector<uint32_t> budget(L->sc.size(), 2500u);
+    const Tables& T = tables();
+    for (const CRec& c : rec) {
+        if (!c.on) continue;
+        int si = scene_of_contact(L, c.c);
+        if (si < 0) continue;
+        HScene& S = L->sc[si];
+        if (!pulled[si]) { if (pull(L, si, G_ALL)) return -1; pulled[si] = 1; }
+        int ia = (int)c.ba;
+        int ma = L->mat[ia];
+        if (ma == 0 || ma >= SLUG) continue;
+        if (c.bb != NONE && c.bo < 0 && c.jn * c.vn0 > 4000.0f) {
+            int j = (int)c.bb;
+            if (L->mat[j] != 0 && L->body[j] < 0) st_activate(L, S, L->sid[j]);
+        }
+        if (L->mat[ia] == 0) continue;
+        if (c.vn0 < 1.0f) continue;
+        float e = 0.5f * c.jn * c.vn0;
+        if (e < 150.0f) continue;
+        int jb = c.bb != NONE ? (int)c.bb : -1;
+        int mb = jb >= 0 ? L->mat[jb] : 0;
+        float soft_a = 1.0f / T.eb[ma];
+        float soft_b = (jb < 0 || mb == 0 || mb >= SLUG) ? 0.0f : 1.0f / T.eb[mb];
+        if (soft_a + soft_b <= 0.0f) continue;
+        float ea = e * soft_a / (soft_a + soft_b);
+        LV p = lv(c.p[0], c.p[1], c.p[2]);
+        uint32_t& bud = budget[si];
+        if (ea > 0.0f && bud > 0) { uint32_t k = crush_at(L, S, ia, ea, p, bud); bud -= k < bud ? k : bud; }
+        if (soft_b > 0.0f && bud > 0) { uint32_t k = crush_at(L, S, jb, e - ea, p, bud); bud -= k < bud ? k : bud; }
+    }
+    for (size_t si = 0; si < pulled.size(); si++) if (pulled[si] && push(L, (int)si, G_ALL)) return -1;
+    return rebuild_csr(L);
+}
+
+int scan(Lat* L, const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t n) {
+    sf::scan_u32(in, out, total, nullptr, n, n, L->D.scan_tmp, L->st);
+    return 0;
+}
+void segsort(Lat* L, const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
+    if (!nseg) return;
+    LAUNCH(k_segsort, sf_grid((uint64_t)nseg * 64, NT, 65536), NT, off, cnt, val, nseg);
+    LAUNCH(k_segsort_big, std::min<uint32_t>(nseg, 1024), NT, off, cnt, val, nseg);
+}
+
+int substep(Lat* L, float h) {
+    LD& D = L->D;
+    const unsigned gb = sf_grid(L->nbd), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
+    LAUNCH(k_cache, gb, NT, D, 1, h);
+    LAUNCH(k_wpos, gd, NT, D);
+    // hash
+    fill_u32(L, D.hcnt, 0, L->tsize);
+    fill_u32(L, D.hfill, 0, L->tsize);
+    LAUNCH(k_hash_count, gd, NT, D);
+    scan(L, D.hcnt, D.hoff, D.ctr + C_NCTR0, L->tsize);
+    LAUNCH(k_hash_fill, gd, NT, D);
+    segsort(L, D.hoff, D.hcnt, D.hent, L->tsize);
+    // contacts: count, size, write
+    reset_ctr(L);
+    LAUNCH(k_contacts<false>, gd, NT, D);
+    scan(L, D.ccnt, D.coff, D.ctr + C_NCS, L->ndyn);
+    if (read_ctr(L)) return -1;
+    if (drain_kills(L)) return -1;
+    uint32_t ncs = L->h_ctr[C_NCS];
+    if (ncs > L->ccap) {
+        if (lat_contacts_alloc(L, std::max<uint32_t>(ncs + ncs / 2, 1u << 16))) return -1;
+    }
+    fill_u32(L, D.nc, 0, L->nbd);
+    LAUNCH(k_contacts<true>, gd, NT, D);
+    LAUNCH(k_scene_c0, 1, 64, D);
+    LAUNCH(k_plough_flag, sf_grid(ncs), NT, D);
+    std::vector<uint32_t> c0(L->sc.size());
+    SF_HIP(hipMemcpyAsync(c0.data(), D.sc_c0, sizeof(uint32_t) * c0.size(), hipMemcpyDeviceToHost, L->st));
+    if (read_ctr(L)) return -1;
+    for (size_t si = 0; si < L->sc.size(); si++) {
+        L->sc[si].c0 = c0[si];
+        uint32_t end = si + 1 < L->sc.size() ? c0[si + 1] : ncs;
+        L->sc[si].ncs = end >= c0[si] ? end - c0[si] : 0;
+    }
+    if (L->h_ctr[C_NPLOUGH] && plough_phase(L, L->h_ctr[C_NPLOUGH])) return -1;
+    const unsigned gc = sf_grid(std::max<uint32_t>(ncs, 1));
+    LAUNCH(k_prep, gc, NT, D, h);
+    LAUNCH(k_vpre, gb, NT, D);
+    // colour
+    LAUNCH(k_col_clear, sf_grid((uint64_t)L->nbd * L->K), NT, D);
+    for (int r = 0; r < L->R; r++) {
+        LAUNCH(k_col_claim, gc, NT, D, r);
+        LAUNCH(k_col_resolve, gc, NT, D, r);
+    }
+    LAUNCH(k_col_count, gc, NT, D);
+    LAUNCH(k_col_scan, 1, 64, D);
+    LAUNCH(k_col_fill, gc, NT, D);
+    // overflow lists
+    fill_u32(L, D.ovcnt, 0, L->nbd);
+    fill_u32(L, D.ovfill, 0, L->nbd);
+    LAUNCH(k_ovf_count, gc, NT, D);
+    scan(L, D.ovcnt, D.ovoff, D.ctr + C_NCTR0, L->nbd);
+    LAUNCH(k_ovf_fill, gc, NT, D);
+    segsort(L, D.ovoff, D.ovcnt, D.ovent, L->nbd);
+    LAUNCH(k_ovf_prep, gc, NT, D);
+    if (read_ctr(L)) return -1;
+    const uint32_t* cc = L->h_ctr + C_COLCNT;
+    for (int it = 0; it < 10; it++) {
+        for (int c = 0; c < L->K; c++) if (cc[c]) LAUNCH(k_solve_colour, sf_grid(cc[c]), NT, D, c);
+        if (cc[L->K]) {
+            LAUNCH(k_ovf_solve, sf_grid(cc[L->K]), NT, D);
+            LAUNCH(k_ovf_apply, sf_grid((uint64_t)L->nbd * 64, NT, 65536), NT, D);
+        }
+    }
+    LAUNCH(k_pinch, gb, NT, D);
+    LAUNCH(k_impact_flag, gc, NT, D);
+    if (read_ctr(L)) return -1;
+    if (drain_kills(L)) return -1;
+    if (L->h_ctr[C_NIMPACT] && impact_phase(L, L->h_ctr[C_NIMPACT])) return -1;
+    LAUNCH(k_integrate, gb, NT, D, h);
+    LAUNCH(k_wakes, gb, NT, D);
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+
+int structures_upload(Lat* L, bool warm) {
+    size_t ns = L->sts.size();
+    if (!ns) return 0;
+    std::vector<uint8_t> act(ns), on(ns), live(ns);
+    std::vector<int32_t> nmin(ns);
+    for (size_t g = 0; g < ns; g++) {
+        const HStruct& T = L->sts[g];
+        const HScene& S = L->sc[T.scene];
+        int sl = (int)g - S.st0;
+        act[g] = T.active > 0.0f && T.solved;
+        on[g] = T.active > 0.0f;
+        live[g] = T.settle <= 0.0f;
+        if (warm) nmin[g] = T.warm;
+        else if (!act[g]) nmin[g] = 0;
+        else {
+            int size = T.hi - T.lo;
+            nmin[g] = T.resid < 1e-4f ? (((S.frame + (uint32_t)sl) % 3u) == 0 ? 1 : 0) : (size > 15000 ? 1 : 6);
+        }
+    }
+    if (!warm) {
+        for (HScene& S : L->sc) {
+            int na = 0;
+            for (int sl = 0; sl < S.nst; sl++) na += act[S.st0 + sl];
+            S.stats[13] = (float)na;
+        }
+    }
+    LD& D = L->D;
+    int r = 0;
+    r |= cp_up(D.st_act, act.data(), ns, L->st); r |= cp_up(D.st_on, on.data(), ns, L->st);
+    r |= cp_up(D.st_live, live.data(), ns, L->st); r |= cp_up(D.st_nmin, nmin.data(), ns, L->st);
+    if (r) return r;
+    SF_HIP(hipStreamSynchronize(L->st));
+    return 0;
+}
+
+int structural(Lat* L, float dt) {
+    LD& D = L->D;
+    const unsigned gb = sf_grid(L->nbd), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1)), gbr = sf_grid(L->nb);
+    const unsigned gw = sf_grid((uint64_t)L->nbd * 64, NT, 65536);
+    if (structures_upload(L, false)) return -1;
+    size_t ns = L->sts.size();
+    fill_u32(L, D.st_maxr, 0, ns);
+    fill_u32(L, D.st_flags, 0, ns);
+    fill_u32(L, D.sc_gmax, 0, L->sc.size());
+    LAUNCH(k_extn_clear, gbr, NT, D);
+    LAUNCH(k_rest_count, gw, NT, D);
+    fill_u32(L, D.lcnt, 0, L->nb);
+    fill_u32(L, D.lfill, 0, L->nb);
+    LAUNCH(k_rest_emit, gd, NT, D, 0);
+    scan(L, D.lcnt, D.loff, D.ctr + C_NCTR0, L->nb);
+    LAUNCH(k_rest_emit, gd, NT, D, 1);
+    segsort(L, D.loff, D.lcnt, D.lent, L->nb);
+    LAUNCH(k_rest_sum, gbr, NT, D);
+    LAUNCH(k_ext_mix, gbr, NT, D);
+    LAUNCH(k_unsupported, gw, NT, D);
+    if (ns) LAUNCH(k_sor_struct, std::min<size_t>(ns, 2048), ST_NT, D, 0);
+    LAUNCH(k_ga, gw, NT, D, dt);
+    LAUNCH(k_body_sor_prep, gb, NT, D);
+    LAUNCH(k_sor_body, std::min<int>(std::max(L->nbd, 1), 8192), BD_NT, D);
+    LAUNCH(k_stress_clear, gbr, NT, D);
+    LAUNCH(k_body_judge_prep, gb, NT, D, dt);
+    reset_ctr(L);
+    LAUNCH(k_stress_static, gbr, NT, D, dt);
+    LAUNCH(k_stress_body, gd, NT, D);
+    LAUNCH(k_lt_decay, gb, NT, D, dt);
+    SF_HIP(hipGetLastError());
+    // results back: per structure resid / max ratio / flags, per scene max ratio, events
+    std::vector<float> resid(ns);
+    std::vector<uint32_t> maxr(ns), flags(ns), gmax(L->sc.size());
+    std::vector<uint8_t> bdirty(L->nbd);
+    int r = 0;
+    r |= cp_down(resid.data(), D.st_resid, ns, L->st); r |= cp_down(maxr.data(), D.st_maxr, ns, L->st);
+    r |= cp_down(flags.data(), D.st_flags, ns, L->st); r |= cp_down(gmax.data(), D.sc_gmax, L->sc.size(), L->st);
+    r |= cp_down(bdirty.data(), D.bd_dirty, L->nbd, L->st);
+    if (r) return r;
+    if (read_ctr(L)) return -1;
+    if (drain_kills(L)) return -1;
+    if (L->nbd) LAUNCH(k_fill_u8, gb, NT, D.bd_dirty, (uint8_t)0, (size_t)L->nbd);
+    uint32_t nev = std::min(L->h_ctr[C_NEV], D.evcap), nbw = L->h_ctr[C_NBW];
+    std::vector<unsigned long long> ekey(nev);
+    std::vector<int4> ev(nev);
+    std::vector<uint32_t> bwi;
+    std::vector<float> bwv;
+    bool bw_overflow = nbw > D.bwcap;
+    if (nev) { r |= cp_down(ekey.data(), D.evkey, nev, L->st); r |= cp_down(ev.data(), D.ev, nev, L->st); }
+    if (nbw && !bw_overflow) { bwi.resize(nbw); bwv.resize(nbw); r |= cp_down(bwi.data(), D.bwi, nbw, L->st); r |= cp_down(bwv.data(), D.bwv, nbw, L->st); }
+    if (r) return r;
+    SF_HIP(hipStreamSynchronize(L->st));
+    if (nev > 0 && L->h_ctr[C_NEV] > D.evcap) { sf::set_error(SF_E_CAP, "structural event buffer overflow"); return SF_E_CAP; }
+    if (bw_overflow) { for (size_t si = 0; si < L->sc.size(); si++) if (pull(L, (int)si, G_BOND)) return -1; }
+    else for (uint32_t e = 0; e < nbw; e++) bond_ref(L, (int)(bwi[e] >> 2), (int)(bwi[e] & 3u)) = bwv[e];
+    for (int b = 0; b < L->nbd; b++) if (bdirty[b]) L->dirty[b] = 1;
+    std::vector<uint32_t> order(nev);
+    for (uint32_t e = 0; e < nev; e++) order[e] = e;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ekey[a] < ekey[b]; });
+    std::vector<uint8_t> changed(L->sc.size(), 0);
+    auto apply_event = [&](const int4& e) {
+        int i = e.x, a = e.y, k = e.z;
+        int si = L->bxyz[i].w;
+        HScene& S = L->sc[si];
+        changed[si] = 1;
+        if (S.crush_n < 300) { S.crush_n++; remove_brick(L, S, k); return; }
+        bond_ref(L, i, a) = 0.0f;
+        if (L->body[i] >= 0) L->dirty[L->body[i]] = 1;
+        else { int s = L->sid[i]; if (s < S.nst) { HStruct& T = L->sts[S.st0 + s]; T.conn = true; S.conn_any = true; T.broke = true; } }
+    };
+    // structures in order: their crushes, then their timers; then the pieces' crushes
+    uint32_t e = 0;
+    for (size_t g = 0; g < ns; g++) {
+        HStruct& T = L->sts[g];
+        HScene& S = L->sc[T.scene];
+        T.resid = resid[g];
+        if (!(T.active > 0.0f && T.solved)) continue;
+        if (flags[g] & 1u) { T.conn = true; S.conn_any = true; }
+        if (flags[g] & 2u) T.broke = true;
+        while (e < nev && !(ekey[order[e]] >> 63) && (int)(ekey[order[e]] >> 40) == (int)g) apply_event(ev[order[e++]]);
+        float mr; memcpy(&mr, &maxr[g], 4);
+        T.maxr = mr;
+        if (T.settle > 0.0f && T.warm == 0) T.settle -= dt;
+        if (T.warm == 0) T.active -= dt;
+        if (T.broke) { T.broke = false; T.active = 4.0f; }
+        if (mr > 0.98f && T.active < 1.0f) T.active = 1.0f;
+    }
+    for (; e < nev; e++) apply_event(ev[order[e]]);
+    for (size_t si = 0; si < L->sc.size(); si++) {
+        HScene& S = L->sc[si];
+        memcpy(&S.stats[6], &gmax[si], 4);
+        if (changed[si] && push(L, (int)si, G_MAT | G_BOND)) return -1;
+    }
+    return 0;
+}
+}  // namespace
+
+namespace {
+
+int scene_stats(Lat* L) {
+    size_t S = L->sc.size();
+    fill_u32(L, L->d_scstat.p, 0, 3 * S);
+    LAUNCH(k_scene_stats, sf_grid(L->nbd), NT, L->D, L->d_scstat.p);
+    std::vector<uint32_t> o(3 * S);
+    SF_HIP(hipMemcpyAsync(o.data(), L->d_scstat.p, sizeof(uint32_t) * 3 * S, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    for (size_t si = 0; si < S; si++) {
+        HScene& sc = L->sc[si];
+        uint32_t stat = 0, unsolved = 0;
+        for (int sl = 0; sl < sc.nst; sl++) { const HStruct& T = L->sts[sc.st0 + sl]; stat += T.now; if (!T.solved) unsolved++; }
+        sc.stats[0] = (float)stat; sc.stats[1] = (float)o[3 * si + 1]; sc.stats[2] = (float)o[3 * si]; sc.stats[3] = (float)o[3 * si + 2];
+        sc.stats[4] = (float)sc.ncs; sc.stats[14] = (float)unsolved; sc.stats[15] = (float)sc.nst;
+    }
+    return 0;
+}
+
+int host_topology(Lat* L, bool rebuild) {
+    for (size_t si = 0; si < L->sc.size(); si++) {
+        HScene& S = L->sc[si];
+        bool need = S.conn_any;
+        if (rebuild) for (int lb = 0; lb < S.hw && !need; lb++) { int b = S.bd0 + lb; need = L->alive[b] && L->dirty[b]; }
+        if (!need) continue;
+        if (pull(L, (int)si, G_ALL)) return -1;
+        if (rebuild) for (int lb = 0; lb < S.hw; lb++) { int b = S.bd0 + lb; if (L->alive[b] && L->dirty[b]) rebuild_body(L, S, b); }
+        connectivity(L, S);
+        if (push(L, (int)si, G_ALL)) return -1;
+    }
+    if (rebuild_csr(L)) return -1;
+    return upload_scenes(L);
+}
+
+int frame(Lat* L, float dt) {
+    auto t0 = std::chrono::steady_clock::now();
+    L->syncs = 0;
+    dt = clampf(dt, 0.0f, 1.0f / 30.0f);
+    for (HScene& S : L->sc) {
+        S.time += dt; S.crush_n = 0; S.frame += 1;
+        if (S.blast_t > 0.0f) S.blast_t -= dt;
+    }
+    if (rebuild_csr(L) || upload_scenes(L)) return -1;
+    for (int k = 0; k < 2; k++) if (substep(L, dt * 0.5f)) return -1;
+    if (read_ctr(L) || drain_kills(L)) return -1;
+    if (host_topology(L, true)) return -1;
+    LD& D = L->D;
+    const unsigned gb = sf_grid(L->nbd), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
+    LAUNCH(k_cache, gb, NT, D, 0, 0.0f);
+    LAUNCH(k_wpos, gd, NT, D);
+    if (structural(L, dt)) return -1;
+    // warm-up sweeps of structures asked for a load field
+    bool any_warm = false;
+    for (const HStruct& T : L->sts) if (T.warm > 0) { any_warm = true; break; }
+    if (any_warm) {
+        if (structures_upload(L, true)) return -1;
+        LAUNCH(k_sor_struct, std::min<size_t>(L->sts.size(), 2048), ST_NT, D, 1);
+        std::vector<float> resid(L->sts.size());
+        if (cp_down(resid.data(), D.st_resid, resid.size(), L->st)) return -1;
+        SF_HIP(hipStreamSynchronize(L->st));
+        for (size_t g = 0; g < L->sts.size(); g++) {
+            HStruct& T = L->sts[g];
+            if (T.warm <= 0) continue;
+            T.resid = resid[g]; T.warm = 0; T.settle = 0.3f;
+        }
+    }
+    if (host_topology(L, false)) return -1;
+    LAUNCH(k_cache, gb, NT, D, 0, 0.0f);
+    LAUNCH(k_wpos, sf_grid(std::max<uint32_t>(L->ndyn, 1)), NT, D);
+    if (scene_stats(L)) return -1;
+    SF_HIP(hipGetLastError());
+    L->ms_frame = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// ---------------------------------------------------------------- scene set-up
+int add_brick(Lat* L, HScene& S, int x, int y, int z, int m, int sid, int infill, int si) {
+    if (S.nb >= S.bcap) { sf::set_error(SF_E_CAP, "scene brick capacity exceeded"); return -1; }
+    int i = S.b0 + S.nb++;
+    L->mat[i] = (uint8_t)m; L->bxyz[i] = make_int4(x, y, z, si);
+    L->sid[i] = (uint8_t)sid; L->infill[i] = (uint8_t)((m == GLASS || infill) ? 1 : 0);
+    L->body[i] = -1; L->next[i] = NONE;
+    L->local[i] = hf(lat_pos(x, y, z)); L->wpos[i] = L->local[i];
+    L->bond[i] = make_float4(0, 0, 0, 0); L->bmax[i] = make_float4(0, 0, 0, 0);
+    L->phi[i] = 0.0f; L->ext[i] = 0.0f; L->extn[i] = 0.0f;
+    return i;
+}
+void finalize_brick(Lat* L, const HScene& S, int i) {
+    if (L->mat[i] == 0) return;
+    int4 c = L->bxyz[i];
+    for (int d = 0; d < 6; d++) {
+        int32_t g = hgget(L, S, c.x + DIRS[d][0], c.y + DIRS[d][1], c.z + DIRS[d][2]);
+        L->nb6[6 * (size_t)i + d] = g ? (uint32_t)(g - 1) : NONE;
+    }
+    float bd[4] = {0, 0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        int j = hnbr(L, i, a * 2);
+        bool gap = j >= 0 && L->infill[i] != L->infill[j] && (a != 1 || L->infill[i] == 1);
+        if (j >= 0 && !gap && L->mat[j] != 0 && L->body[j] == L->body[i] && L->sid[j] == L->sid[i]) bd[a] = bond_cap(L->mat[i], L->mat[j], a);
+    }
+    if (c.y == 0 && L->body[i] < 0) bd[3] = tables().capv[L->mat[i]] * 2.0f;
+    L->bond[i] = make_float4(bd[0], bd[1], bd[2], bd[3]);
+    L->bmax[i] = L->bond[i];
+}
+
+int finalize(Lat* L, int si) {
+    HScene& S = L->sc[si];
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (int i = S.b0; i < S.b0 + S.nb; i++) {
+        int c[3] = {L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z};
+        for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], c[k]); hi[k] = std::max(hi[k], c[k]); }
+    }
+    if (S.nb == 0) { lo[0] = lo[1] = lo[2] = 0; hi[0] = hi[1] = hi[2] = 0; }
+    S.gx0 = lo[0] - 1; S.gy0 = lo[1] - 1; S.gz0 = lo[2] - 1;
+    S.gnx = hi[0] - lo[0] + 3; S.gny = hi[1] - lo[1] + 3; S.gnz = hi[2] - lo[2] + 3;
+    size_t gsz = (size_t)S.gnx * S.gny * S.gnz;
+    S.goff = L->grid.size();
+    L->grid.resize(S.goff + gsz, 0);
+    for (int i = S.b0; i < S.b0 + S.nb; i++) {
+        if (!L->mat[i]) continue;
+        int4 c = L->bxyz[i];
+        size_t idx = S.goff + ((size_t)(c.y - S.gy0) * S.gnz + (c.z - S.gz0)) * S.gnx + (c.x - S.gx0);
+        if (!L->grid[idx]) L->grid[idx] = i + 1;
+    }
+    for (int i = S.b0; i < S.b0 + S.nb; i++) finalize_brick(L, S, i);
+    int red_at = S.b0;
+    std::vector<int32_t> stsc(S.nst), stlo(S.nst), sthi(S.nst);
+    std::vector<int4> strb(S.nst);
+    for (int sl = 0; sl < S.nst; sl++) {
+        HStruct& T = L->sts[S.st0 + sl];
+        uint32_t cnt = 0; int top = 0;
+        int mn[3] = {1 << 30, 1 << 30, 1 << 30}, mx[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        int r0 = red_at;
+        for (int par = 0; par < 2; par++) {
+            int p0 = red_at;
+            for (int i = T.lo; i < T.hi; i++) {
+                if (!hin_struct(L, i, sl)) continue;
+                int4 c = L->bxyz[i];
+                if (((c.x + c.y + c.z) & 1) != par) continue;
+                L->lst[red_at - S.b0] = (uint32_t)i;
+                red_at++;
+            }
+            if (par == 0) { T.rb.x = p0; T.rb.y = red_at - p0; } else { T.rb.z = p0; T.rb.w = red_at - p0; }
+        }
+        (void)r0;
+        for (int i = T.lo; i < T.hi; i++) {
+            if (!hin_struct(L, i, sl)) continue;
+            cnt++; top = std::max(top, L->bxyz[i].y + 1);
+            int c[3] = {L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z};
+            for (int k = 0; k < 3; k++) { mn[k] = std::min(mn[k], c[k]); mx[k] = std::max(mx[k], c[k]); }
+        }
+        T.init = T.now = cnt; T.top0 = T.top = top;
+        for (int k = 0; k < 3; k++) { T.mn[k] = mn[k]; T.mx[k] = mx[k]; }
+        stsc[sl] = si; stlo[sl] = T.lo; sthi[sl] = T.hi; strb[sl] = T.rb;
+    }
+    LD& D = L->D;
+    hipStream_t s = L->st;
+    size_t b0 = S.b0, nb = S.nb;
+    int r = 0;
+    r |= cp_up(D.mat + b0, &L->mat[b0], nb, s); r |= cp_up(D.sid + b0, &L->sid[b0], nb, s); r |= cp_up(D.body + b0, &L->body[b0], nb, s);
+    r |= cp_up(D.bxyz + b0, &L->bxyz[b0], nb, s); r |= cp_up(D.nb6 + 6 * b0, &L->nb6[6 * b0], 6 * nb, s);
+    r |= cp_up(D.bond + b0, &L->bond[b0], nb, s); r |= cp_up(D.bmax + b0, &L->bmax[b0], nb, s);
+    r |= cp_up(D.local + b0, &L->local[b0], nb, s); r |= cp_up(D.wpos + b0, &L->wpos[b0], nb, s);
+    r |= cp_up(D.phi + b0, &L->phi[b0], nb, s); r |= cp_up(D.ext + b0, &L->ext[b0], nb, s); r |= cp_up(D.extn + b0, &L->extn[b0], nb, s);
+    std::vector<uint32_t> zero(nb, 0u);
+    r |= cp_up(D.sup + b0, zero.data(), nb, s);
+    r |= cp_up((uint32_t*)D.stress + b0, zero.data(), nb, s);
+    if (red_at > S.b0) r |= cp_up(D.st_red + S.b0, L->lst.data(), (size_t)(red_at - S.b0), s);
+    if (S.nst) {
+        r |= cp_up(D.st_scene + S.st0, stsc.data(), S.nst, s); r |= cp_up(D.st_lo + S.st0, stlo.data(), S.nst, s);
+        r |= cp_up(D.st_hi + S.st0, sthi.data(), S.nst, s); r |= cp_up(D.st_rb + S.st0, strb.data(), S.nst, s);
+        std::vector<float> res(S.nst, 0.0f);
+        r |= cp_up(D.st_resid + S.st0, res.data(), S.nst, s);
+    }
+    if (r) return -1;
+    SF_HIP(hipStreamSynchronize(s));
+    // the grid: all scenes' regions
+    if (L->d_grid.n < L->grid.size()) { if (L->d_grid.alloc(L->grid.size() + L->grid.size() / 2)) return -1; }
+    SF_HIP(hipMemcpy(L->d_grid.p, L->grid.data(), sizeof(int32_t) * L->grid.size(), hipMemcpyHostToDevice));
+    D.grid = L->d_grid.p;
+    std::vector<int32_t> bsc(S.dcap, si);
+    r |= cp_up(D.bd_scene + S.bd0, bsc.data(), S.dcap, s);
+    if (r) return -1;
+    if (push(L, si, G_BODY)) return -1;
+    S.finalized = true;
+    L->lists_dirty = true;
+    if (rebuild_csr(L) || upload_scenes(L)) return -1;
+    return 0;
+}
+
+// ---------------------------------------------------------------- blast (src/lib.rs:2679-2716)
+void blast_brick(Lat* L, HScene& S, int i, LV c, float r, float power) {
+    int m = L->mat[i];
+    if (m == 0 || m >= SLUG) return;
+    LV p = hbpos(L, i);
+    float d = len(sub(p, c));
+    float rr = r * tables().blast[m];
+    if (d < rr * 0.5f) remove_brick(L, S, i);
+    else if (d < rr * 1.35f) { float f = 1.0f - d / (rr * 1.35f); damage_brick(L, S, i, power * f * f * 1.3f, 0.0f); }
+}
+int explode(Lat* L, int si, float x, float y, float z, float r, float power) {
+    HScene& S = L->sc[si];
+    if (pull(L, si, G_ALL)) return -1;
+    LV c = lv(x, y, z);
+    S.blast_c = c; S.blast_r = r * 2.2f; S.blast_p = power; S.blast_t = 0.25f;
+    float reach = r * 1.4f * 1.35f + 0.5f;
+    for (int sl = 0; sl < S.nst; sl++) {
+        HStruct& T = L->sts[S.st0 + sl];
+        LV a = sub(c, lv(reach, reach, reach)), b = add(c, lv(reach, reach, reach));
+        int x0 = fcell(a.x) + NX2, y0 = fcell(a.y), z0 = fcell(a.z) + NX2;
+        int x1 = fcell(b.x) + NX2, y1 = fcell(b.y), z1 = fcell(b.z) + NX2;
+        if (x1 < T.mn[0] || x0 > T.mx[0] || y1 < T.mn[1] || y0 > T.mx[1] || z1 < T.mn[2] || z0 > T.mx[2]) continue;
+        for (int i = T.lo; i < T.hi; i++) if (hin_struct(L, i, sl)) blast_brick(L, S, i, c, r, power);
+    }
+    for (int lb = 0; lb < S.hw; lb++) {
+        int b = S.bd0 + lb;
+        if (!(L->alive[b] && len(sub(hv(L->pos[b]), c)) < reach + L->rad[b])) continue;
+        uint32_t k = L->first[b];
+        while (k != NONE) { uint32_t nx = L->next[k]; blast_brick(L, S, (int)k, c, r, power); k = nx; }
+    }
+    for (int lb = 0; lb < S.hw; lb++) {
+        int b = S.bd0 + lb;
+        if (!(L->alive[b] && len(sub(hv(L->pos[b]), c)) < S.blast_r + L->rad[b])) continue;
+        wake_body(L, b);
+        apply_blast_to_body(L, S, b);
+    }
+    if (push(L, si, G_ALL)) return -1;
+    return upload_scenes(L);
+}
+
+int add_body(Lat* L, int si, int n, const int32_t* xyz, const uint8_t* mat, const float* v3, const float* w3) {
+    HScene& S = L->sc[si];
+    if (n <= 0) return SF_E_ARG;
+    int b = alloc_body(L, S);
+    if (b < 0) { sf::set_error(SF_E_CAP, "scene body capacity exceeded"); return SF_E_CAP; }
+    if (S.nb + n > S.bcap) { sf::set_error(SF_E_CAP, "scene brick capacity exceeded"); return SF_E_CAP; }
+    for (int k = 0; k < n; k++) {
+        int i = add_brick(L, S, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], mat[k], 255, 0, si);
+        L->lst2[k] = (uint32_t)i;
+    }
+    LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
+    setup_body(L, b, 0, n, lv(0.0f, 0.0f, 0.0f), IDM, idq, lv(v3[0], v3[1], v3[2]), lv(w3[0], w3[1], w3[2]));
+    return b - S.bd0;
+}
+
+void clear_scene(Lat* L, int si, uint32_t seed) {
+    HScene& S = L->sc[si];
+    for (int i = S.b0; i < S.b0 + S.bcap; i++) { L->mat[i] = 0; L->body[i] = -1; L->next[i] = NONE; L->sid[i] = 255; }
+    for (int b = S.bd0; b < S.bd0 + S.dcap; b++) { L->alive[b] = 0; L->first[b] = NONE; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0; L->nc[b] = 0; }
+    int st0 = S.st0, smax = (int)(L->sts.size()) ;
+    (void)smax;
+    for (int sl = 0; sl < S.nst; sl++) { HStruct fresh; fresh.scene = si; L->sts[st0 + sl] = fresh; }
+    int b0 = S.b0, bcap = S.bcap, bd0 = S.bd0, dcap = S.dcap, stc = S.st0;
+    size_t goff = S.goff;
+    int gnx = S.gnx, gny = S.gny, gnz = S.gnz;
+    bool had_grid = S.finalized;
+    S = HScene();
+    S.b0 = b0; S.bcap = bcap; S.bd0 = bd0; S.dcap = dcap; S.st0 = stc;
+    S.rng = seed ? seed : 0x9E3779B9u;
+    if (had_grid) { for (size_t k = 0; k < (size_t)gnx * gny * gnz; k++) L->grid[goff + k] = 0; }
+    L->lists_dirty = true;
+}
+
+struct Handle {
+    Lat L;
+    std::vector<int> smax;  // structure capacity per scene
+};
+
+}  // namespace
+
+struct sf_lat { Handle h; };
+
+extern "C" {
+
+sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, int rounds) {
+    if (brick_cap <= 0 || body_cap <= 0 || max_scenes <= 0 || colours < 1 || colours > KMAX || rounds < 0) {
+        sf::set_error(SF_E_ARG, "bad capacities or colouring parameters"); return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { sf::set_error(SF_E_NODEV, "no GPU visible"); return nullptr; }
+    sf_lat* h = new sf_lat();
+    Lat* L = &h->h.L;
+    SF_HIP_NULL(hipGetDevice(&L->dev));
+    L->bcap = brick_cap; L->dcap = body_cap; L->scap = max_scenes; L->K = colours; L->R = rounds;
+    if (hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) != hipSuccess) { delete h; sf::set_error(SF_E_HIP, "stream"); return nullptr; }
+    if (lat_alloc(L) || lat_contacts_alloc(L, (uint32_t)std::max<int>(1 << 16, 4 * brick_cap)) || lat_hash_alloc(L, 1024) || L->d_grid.alloc(1)) { delete h; return nullptr; }
+    L->D.grid = L->d_grid.p;
+    size_t B = brick_cap, N = body_cap;
+    L->mat.assign(B, 0); L->sid.assign(B, 255); L->infill.assign(B, 0); L->body.assign(B, -1); L->bxyz.assign(B, make_int4(0, 0, 0, 0));
+    L->next.assign(B, NONE); L->nb6.assign(6 * B, NONE); L->stamp.assign(B, 0); L->q.assign(B, 0);
+    L->bond.assign(B, make_float4(0, 0, 0, 0)); L->bmax = L->bond; L->local = L->bond; L->wpos = L->bond;
+    L->phi.assign(B, 0.0f); L->ext = L->phi; L->extn = L->phi;
+    L->alive.assign(N, 0); L->asleep.assign(N, 0); L->dirty.assign(N, 0); L->wake.assign(N, 0);
+    L->first.assign(N, NONE); L->count.assign(N, 0); L->nc.assign(N, 0);
+    L->pos.assign(N, make_float4(0, 0, 0, 0)); L->qq.assign(N, make_float4(0, 0, 0, 1)); L->v = L->pos; L->w = L->pos; L->vlast = L->pos; L->wlast = L->pos;
+    LM z; for (int k = 0; k < 9; k++) z.m[k] = 0.0f;
+    L->rot.assign(N, z); L->iil = L->rot; L->iiw = L->rot;
+    L->m.assign(N, 0.0f); L->im = L->m; L->sleep = L->m; L->rad = L->m; L->cool = L->m; L->lt = L->m; L->resid = L->m; L->maxr = L->m; L->nmax = L->m;
+    L->lst.assign(B + 1, 0); L->lst2.assign(B + 1, 0); L->comp.assign(B + 1, 0); L->cnt.assign(B + 2, 0);
+    L->dyn.assign(B, 0); L->dyn_body.assign(B, 0); L->boff.assign(N, 0); L->bcnt.assign(N, 0);
+    return h;
+}
+void sf_lat_destroy(sf_lat* h) { delete h; }
+
+int sf_lat_scene_add(sf_lat* h, int brick_cap, int body_cap, int max_structs, uint32_t seed) {
+    if (!h || brick_cap <= 0 || body_cap <= 0 || max_structs < 0 || max_structs > 255) { sf::set_error(SF_E_ARG, "bad scene capacities"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    if ((int)L->sc.size() >= L->scap || L->nb + brick_cap > L->bcap || L->nbd + body_cap > L->dcap || (L->sts.size() + max_structs) > 256 * (size_t)L->scap) {
+        sf::set_error(SF_E_CAP, "world capacity exceeded"); return SF_E_CAP;
+    }
+    HScene S;
+    S.b0 = L->nb; S.bcap = brick_cap; S.bd0 = L->nbd; S.dcap = body_cap; S.st0 = (int)L->sts.size();
+    S.rng = seed ? seed : 0x9E3779B9u;
+    int si = (int)L->sc.size();
+    L->sc.push_back(S);
+    h->h.smax.push_back(max_structs);
+    for (int k = 0; k < max_structs; k++) { HStruct T; T.scene = si; L->sts.push_back(T); }
+    L->nb += brick_cap; L->nbd += body_cap;
+    LD& D = L->D;
+    D.nb = (uint32_t)L->nb; D.nbd = (uint32_t)L->nbd; D.nscene = (uint32_t)L->sc.size(); D.nst = (uint32_t)L->sts.size();
+    std::vector<int32_t> bsc(body_cap, si);
+    if (cp_up(D.bd_scene + S.bd0, bsc.data(), body_cap, L->st)) return SF_E_HIP;
+    std::vector<int4> bx(brick_cap, make_int4(0, 0, 0, si));
+    if (cp_up(D.bxyz + S.b0, bx.data(), brick_cap, L->st)) return SF_E_HIP;
+    std::vector<uint8_t> sids(brick_cap, 255);
+    if (cp_up(D.sid + S.b0, sids.data(), brick_cap, L->st)) return SF_E_HIP;
+    for (int k = 0; k < brick_cap; k++) L->bxyz[S.b0 + k] = make_int4(0, 0, 0, si);
+    SF_HIP(hipStreamSynchronize(L->st));
+    if (upload_scenes(L)) return SF_E_HIP;
+    return si;
+}
+int sf_lat_scene_clear(sf_lat* h, int scene, uint32_t seed) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    HScene& S = L->sc[scene];
+    size_t b0 = S.b0, bc = S.bcap, d0 = S.bd0, dc = S.dcap;
+    clear_scene(L, scene, seed);
+    LD& D = L->D;
+    int r = 0;
+    r |= cp_up(D.mat + b0, &L->mat[b0], bc, L->st); r |= cp_up(D.body + b0, &L->body[b0], bc, L->st);
+    r |= cp_up(D.sid + b0, &L->sid[b0], bc, L->st);
+    r |= cp_up(D.alive + d0, &L->alive[d0], dc, L->st); r |= cp_up(D.asleep + d0, &L->asleep[d0], dc, L->st);
+    r |= cp_up(D.wake + d0, &L->wake[d0], dc, L->st);
+    if (r) return SF_E_HIP;
+    SF_HIP(hipStreamSynchronize(L->st));
+    if (rebuild_csr(L) || upload_scenes(L)) return SF_E_HIP;
+    return 0;
+}
+int sf_lat_add_structure(sf_lat* h, int scene, int n, const int32_t* xyz, const uint8_t* mat, const uint8_t* infill) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size() || n < 0 || (n && (!xyz || !mat))) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    HScene& S = L->sc[scene];
+    if (S.finalized) { sf::set_error(SF_E_ARG, "scene already finalized"); return SF_E_ARG; }
+    if (S.nst >= h->h.smax[scene]) { sf::set_error(SF_E_CAP, "scene structure capacity exceeded"); return SF_E_CAP; }
+    if (S.nb + n > S.bcap) { sf::set_error(SF_E_CAP, "scene brick capacity exceeded"); return SF_E_CAP; }
+    for (int k = 0; k < n; k++) if (mat[k] < 1 || mat[k] > 8) { sf::set_error(SF_E_ARG, "material out of range"); return SF_E_ARG; }
+    int sl = S.nst++;
+    HStruct& T = L->sts[S.st0 + sl];
+    T = HStruct(); T.scene = scene;
+    T.lo = S.b0 + S.nb;
+    for (int k = 0; k < n; k++) add_brick(L, S, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], mat[k], sl, infill ? infill[k] : 0, scene);
+    T.hi = S.b0 + S.nb;
+    return sl;
+}
+int sf_lat_add_body(sf_lat* h, int scene, int n, const int32_t* xyz, const uint8_t* mat, const float* v3, const float* w3) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size() || n <= 0 || !xyz || !mat || !v3 || !w3) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    if (h->h.L.sc[scene].finalized) { sf::set_error(SF_E_ARG, "scene already finalized"); return SF_E_ARG; }
+    for (int k = 0; k < n; k++) if (mat[k] < 1 || mat[k] > 8) { sf::set_error(SF_E_ARG, "material out of range"); return SF_E_ARG; }
+    return add_body(&h->h.L, scene, n, xyz, mat, v3, w3);
+}
+int sf_lat_finalize(sf_lat* h, int scene) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
+    return finalize(&h->h.L, scene) ? SF_E_HIP : 0;
+}
+int sf_lat_warm_all(sf_lat* h, int scene) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    HScene& S = L->sc[scene];
+    for (int sl = 0; sl < S.nst; sl++) st_warm(L, S.st0 + sl);
+    return 0;
+}
+int sf_lat_explode(sf_lat* h, int scene, float x, float y, float z, float r, float power) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size() || !(r > 0.0f)) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    return explode(&h->h.L, scene, x, y, z, r, power) ? SF_E_HIP : 0;
+}
+int sf_lat_step(sf_lat* h, float dt) {
+    if (!h) return SF_E_ARG;
+    if (!(dt >= 0.0f)) { sf::set_error(SF_E_ARG, "dt out of range"); return SF_E_ARG; }
+    SF_HIP(hipSetDevice(h->h.L.dev));
+    int r = frame(&h->h.L, dt);
+    return r ? (sf::last_error() ? sf::last_error() : SF_E_HIP) : 0;
+}
+int sf_lat_stats(sf_lat* h, int scene, float* out16) {
+    if (!h || !out16 || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    memcpy(out16, h->h.L.sc[scene].stats, sizeof(float) * 16);
+    return 0;
+}
+int sf_lat_counts(sf_lat* h, int scene, int32_t* out3) {
+    if (!h || !out3 || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    const HScene& S = h->h.L.sc[scene];
+    out3[0] = S.nb; out3[1] = S.hw; out3[2] = S.nst;
+    return 0;
+}
+// per body slot of the scene: alive, asleep, count, nc, pos[3], q[4], v[3], w[3], m, sleep, maxr, resid, nmax
+int sf_lat_bodies(sf_lat* h, int scene, float* out) {
+    if (!h || !out || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    Lat* L = &h->h.L;
+    if (pull(L, scene, G_BODY)) return SF_E_HIP;
+    const HScene& S = L->sc[scene];
+    for (int lb = 0; lb < S.hw; lb++) {
+        int b = S.bd0 + lb;
+        float* o = out + (size_t)lb * SF_LAT_BODY_F;
+        o[0] = L->alive[b]; o[1] = L->asleep[b]; o[2] = (float)L->count[b]; o[3] = (float)L->nc[b];
+        o[4] = L->pos[b].x; o[5] = L->pos[b].y; o[6] = L->pos[b].z;
+        o[7] = L->qq[b].x; o[8] = L->qq[b].y; o[9] = L->qq[b].z; o[10] = L->qq[b].w;
+        o[11] = L->v[b].x; o[12] = L->v[b].y; o[13] = L->v[b].z; o[14] = L->w[b].x; o[15] = L->w[b].y; o[16] = L->w[b].z;
+        o[17] = L->m[b]; o[18] = L->sleep[b]; o[19] = L->maxr[b]; o[20] = L->resid[b]; o[21] = L->nmax[b];
+    }
+    return S.hw;
+}
+// per brick: mat, body (scene-local, -1 static), wpos[3], phi, ext, bond[4], stress, frames since support
+int sf_lat_bricks(sf_lat* h, int scene, float* out) {
+    if (!h || !out || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    Lat* L = &h->h.L;
+    if (pull(L, scene, G_MAT | G_BOND | G_BRICK)) return SF_E_HIP;
+    const HScene& S = L->sc[scene];
+    std::vector<float> stress(S.nb);
+    std::vector<uint32_t> sup(S.nb);
+    if (cp_down(stress.data(), L->D.stress + S.b0, S.nb, L->st) || cp_down(sup.data(), L->D.sup + S.b0, S.nb, L->st)) return SF_E_HIP;
+    SF_HIP(hipStreamSynchronize(L->st));
+    for (int k = 0; k < S.nb; k++) {
+        int i = S.b0 + k;
+        float* o = out + (size_t)k * SF_LAT_BRICK_F;
+        o[0] = L->mat[i]; o[1] = L->body[i] < 0 ? -1.0f : (float)(L->body[i] - S.bd0);
+        o[2] = L->wpos[i].x; o[3] = L->wpos[i].y; o[4] = L->wpos[i].z;
+        o[5] = L->phi[i]; o[6] = L->ext[i];
+        o[7] = L->bond[i].x; o[8] = L->bond[i].y; o[9] = L->bond[i].z; o[10] = L->bond[i].w;
+        o[11] = stress[k]; o[12] = (float)(S.frame - sup[k]);
+    }
+    return S.nb;
+}
+int sf_lat_structs(sf_lat* h, int scene, float* out) {
+    if (!h || !out || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    Lat* L = &h->h.L;
+    const HScene& S = L->sc[scene];
+    for (int sl = 0; sl < S.nst; sl++) {
+        const HStruct& T = L->sts[S.st0 + sl];
+        float* o = out + 12 * sl;
+        LV pa = lat_pos(T.mn[0], T.mn[1], T.mn[2]), pb = lat_pos(T.mx[0], T.mx[1], T.mx[2]);
+        o[0] = pa.x - H; o[1] = pa.z - H; o[2] = pb.x + H; o[3] = pb.z + H;
+        o[4] = (float)T.init; o[5] = (float)T.now; o[6] = (float)T.top0; o[7] = (float)T.top;
+        o[8] = 0.0f; o[9] = 0.0f; o[10] = T.active > 0.0f ? 1.0f : 0.0f; o[11] = (float)T.top0 * lat::S;
+    }
+    return S.nst;
+}
+int sf_lat_matstats(sf_lat* h, int scene, float* out50) {
+    if (!h || !out50 || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    Lat* L = &h->h.L;
+    if (pull(L, scene, G_MAT | G_BODY)) return SF_E_HIP;
+    const HScene& S = L->sc[scene];
+    for (int k = 0; k < 50; k++) out50[k] = 0.0f;
+    for (int lb = 0; lb < S.hw; lb++) {
+        int b = S.bd0 + lb;
+        if (!L->alive[b]) continue;
+        uint32_t cnt[9] = {0}, tot = 0;
+        for (uint32_t k = L->first[b]; k != NONE; k = L->next[k]) if (L->mat[k] != 0 && L->mat[k] < 9) { cnt[L->mat[k]]++; tot++; }
+        if (!tot) continue;
+        int m = 1;
+        for (int q = 1; q < 9; q++) if (cnt[q] > cnt[m]) m = q;
+        if (m * 5 + 4 >= 50) continue;
+        out50[m * 5] += 1.0f; out50[m * 5 + 1] += (float)tot;
+        if ((float)tot > out50[m * 5 + 2]) out50[m * 5 + 2] = (float)tot;
+        if (tot >= 50) out50[m * 5 + 3] += 1.0f;
+        if (tot <= 2) out50[m * 5 + 4] += 1.0f;
+    }
+    return 0;
+}
+// contacts of the last substep: a, b, bo, ba, bb (scene-local; -1 none), on, colour, n[3], p[3], depth, jn, jt1, jt2, vn0
+int sf_lat_contacts(sf_lat* h, int scene, float* out, int cap) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    Lat* L = &h->h.L;
+    const HScene& S = L->sc[scene];
+    int n = (int)S.ncs;
+    if (!out || cap <= 0) return n;
+    int m = std::min(n, cap);
+    LD& D = L->D;
+    size_t c0 = S.c0;
+    std::vector<int32_t> a(m), b(m), bo(m), col(m);
+    std::vector<uint32_t> ba(m), bb(m);
+    std::vector<uint8_t> on(m);
+    std::vector<float4> nn(m), pp(m);
+    std::vector<float> depth(m), jn(m), jt1(m), jt2(m), vn0(m);
+    int r = 0;
+    r |= cp_down(a.data(), D.ca + c0, m, L->st); r |= cp_down(b.data(), D.cb + c0, m, L->st); r |= cp_down(bo.data(), D.cbo + c0, m, L->st);
+    r |= cp_down(col.data(), D.ccol + c0, m, L->st); r |= cp_down(ba.data(), D.cba + c0, m, L->st); r |= cp_down(bb.data(), D.cbb + c0, m, L->st);
+    r |= cp_down(on.data(), D.con + c0, m, L->st); r |= cp_down(nn.data(), D.cn + c0, m, L->st); r |= cp_down(pp.data(), D.cp + c0, m, L->st);
+    r |= cp_down(depth.data(), D.cdepth + c0, m, L->st); r |= cp_down(jn.data(), D.cjn + c0, m, L->st);
+    r |= cp_down(jt1.data(), D.cjt1 + c0, m, L->st); r |= cp_down(jt2.data(), D.cjt2 + c0, m, L->st); r |= cp_down(vn0.data(), D.cvn0 + c0, m, L->st);
+    if (r) return SF_E_HIP;
+    SF_HIP(hipStreamSynchronize(L->st));
+    for (int k = 0; k < m; k++) {
+        float* o = out + (size_t)k * SF_LAT_CONTACT_F;
+        o[0] = (float)(a[k] - S.bd0); o[1] = b[k] < 0 ? -1.0f : (float)(b[k] - S.bd0); o[2] = bo[k] < 0 ? -1.0f : (float)(bo[k] - S.bd0);
+        o[3] = (float)((int)ba[k] - S.b0); o[4] = bb[k] == NONE ? -1.0f : (float)((int)bb[k] - S.b0);
+        o[5] = on[k]; o[6] = (float)col[k];
+        o[7] = nn[k].x; o[8] = nn[k].y; o[9] = nn[k].z; o[10] = pp[k].x; o[11] = pp[k].y; o[12] = pp[k].z;
+        o[13] = depth[k]; o[14] = jn[k]; o[15] = jt1[k]; o[16] = jt2[k]; o[17] = vn0[k];
+    }
+    return n;
+}
+int sf_lat_rng(sf_lat* h, int scene, uint32_t* out) {
+    if (!h || !out || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    *out = h->h.L.sc[scene].rng;
+    return 0;
+}
+int sf_lat_timing(sf_lat* h, double* ms_frame, int32_t* syncs) {
+    if (!h) return SF_E_ARG;
+    if (ms_frame) *ms_frame = h->h.L.ms_frame;
+    if (syncs) *syncs = h->h.L.syncs;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+"""Lattice world (the legacy brick engine's frame) on the GPU against the oracle (oracle/lattice.c) in the device's
+orders, through the C ABI.
+
+The bar: every body (alive, asleep, brick count, contact count, position, rotation, velocities, mass, rest timer,
+joint ratio, load-field residual, g - a factor), every brick (material, body, position, load potential, external
+load, bonds, stress, support age), the frame statistics and the random stream equal bit for bit, frame after frame:
+the kernels perform the oracle's f32 operations in the same order and the host bookkeeping follows it.
+"""
+import numpy as np
+import pytest
+
+import lattice_cases as lc
+from shatterforge_amd.lattice import Lattice
+
+pytestmark = pytest.mark.gpu
+DT = np.float32(1.0 / 60.0)
+
+
+def _first_diff(a, b):
+    d = np.argwhere(~((a == b) | (np.isnan(a) & np.isnan(b))))
+    return d[:5].tolist()
+
+
+def _assert_same(G, s, O, tag):
+    gb, ob = G.bodies(s), O.bodies()
+    assert gb.shape == ob.shape, f"{tag}: body slots {gb.shape} vs {ob.shape}"
+    assert np.array_equal(gb, ob, equal_nan=True), f"{tag}: bodies differ at {_first_diff(gb, ob)}"
+    gk, ok = G.bricks(s), O.bricks()
+    assert np.array_equal(gk, ok, equal_nan=True), f"{tag}: bricks differ at {_first_diff(gk, ok)}"
+    gs, os_ = G.stats(s), O.stats()
+    assert np.array_equal(gs, os_), f"{tag}: stats {gs} vs {os_}"
+    assert G.rng(s) == O.rng(), f"{tag}: random stream"
+
+
+def _run(oracle, case, frames, check_every=1):
+    nb = lc.brick_count(case) + 8
+    nd = 4096
+    G = Lattice(nb, nd, scenes=1)
+    s = G.add_scene(nb, nd, structs=4)
+    O = oracle.LatticeOracle(order=oracle.ORDER_DEV, brick_cap=nb, body_cap=nd)
+    lc.build(G, s, case)
+    lc.build(O, None, case)
+    G.step(DT)
+    O.step(DT)
+    _assert_same(G, s, O, "warm-up frame")
+    for (x, y, z, r, p) in lc.charges(case):
+        G.explode(s, x, y, z, r, p)
+        O.explode(x, y, z, r, p)
+    _assert_same(G, s, O, "after charges")
+    for f in range(frames):
+        G.step(DT)
+        O.step(DT)
+        if f % check_every == 0 or f == frames - 1:
+            _assert_same(G, s, O, f"frame {f}")
+    return G, s, O
+
+
+def test_drop_bodies_bit_exact(oracle):
+    print("case: 24 loose bodies dropped around a column, 90 frames", flush=True)
+    G, s, O = _run(oracle, dict(kind="drop", seed=3, nbody=24), 90)
+    st = O.stats()
+    assert st[2] > 10 and st[4] > 0
+
+
+def test_tower_blast_bit_exact(oracle):
+    print("case: 8x8x16 tower, 4 charges at the column bases, 120 frames", flush=True)
+    G, s, O = _run(oracle, dict(kind="tower", w=8, h=16, nch=4, r=2.2, p=1.4), 120)
+    assert O.stats()[2] > 0
+
+
+def test_building_collapse_bit_exact(oracle):
+    print("case: 2500-brick frame building, 4 charges r=3, 150 frames", flush=True)
+    G, s, O = _run(oracle, dict(kind="building", n=2500, seed=4, nch=4, r=3.0, p=1.4), 150, check_every=5)
+    assert O.stats()[2] > 0
+
+
+MIX = [dict(kind="drop", seed=3, nbody=24), dict(kind="tower", w=8, h=16, nch=4, r=2.2, p=1.4),
+       dict(kind="building", n=2500, seed=4, nch=4, r=3.0, p=1.4), dict(kind="drop", seed=11, nbody=30)]
+
+
+def test_batched_scenes_same_alone_and_in_mix(oracle):
+    # four scenes of uneven size stepped together in one world: each equals the oracle run of that scene alone and
+    # the GPU run of that scene alone, bit for bit
+    print("case: 4 scenes batched, 100 frames, vs each alone (GPU and oracle)", flush=True)
+    caps = [lc.brick_count(c) + 8 for c in MIX]
+    G = Lattice(sum(caps), 4096 * len(MIX), scenes=len(MIX))
+    sc = [G.add_scene(caps[k], 4096, structs=4, seed=0x9E3779B9) for k in range(len(MIX))]
+    alone = []
+    for k, c in enumerate(MIX):
+        A = Lattice(caps[k], 4096, scenes=1)
+        a = A.add_scene(caps[k], 4096, structs=4)
+        alone.append((A, a))
+    Os = [oracle.LatticeOracle(order=oracle.ORDER_DEV, brick_cap=caps[k], body_cap=4096) for k in range(len(MIX))]
+    for k, c in enumerate(MIX):
+        lc.build(G, sc[k], c)
+        lc.build(alone[k][0], alone[k][1], c)
+        lc.build(Os[k], None, c)
+    def step_all():
+        G.step(DT)
+        for A, _ in alone:
+            A.step(DT)
+        for O in Os:
+            O.step(DT)
+    step_all()
+    for k, c in enumerate(MIX):
+        for ch in lc.charges(c):
+            G.explode(sc[k], *ch)
+            alone[k][0].explode(alone[k][1], *ch)
+            Os[k].explode(*ch)
+    for f in range(100):
+        step_all()
+        if f % 10 == 9:
+            for k in range(len(MIX)):
+                _assert_same(G, sc[k], Os[k], f"frame {f} scene {k} vs oracle")
+                A, a = alone[k]
+                assert np.array_equal(G.bodies(sc[k]), A.bodies(a), equal_nan=True), f"frame {f} scene {k}: mix vs alone bodies"
+                assert np.array_equal(G.bricks(sc[k]), A.bricks(a), equal_nan=True), f"frame {f} scene {k}: mix vs alone bricks"
