@@ -182,6 +182,8 @@ int sf_lat_add_body(sf_lat* L, int scene, int n, const int32_t* xyz, const uint8
 int sf_lat_finalize(sf_lat* L, int scene);
 /* ask every structure of the scene for its load field (warm_struct, src/lib.rs:1061); the sweeps run next frame */
 int sf_lat_warm_all(sf_lat* L, int scene);
+/* every structure's load field and joints live (dbg_wake_all, tools/debug_exports.rs:107) */
+int sf_lat_wake_all(sf_lat* L, int scene);
 /* a charge (explode_at, src/lib.rs:2697-2716) */
 int sf_lat_explode(sf_lat* L, int scene, float x, float y, float z, float r, float power);
 /* one frame of every scene (step, src/lib.rs:3639) */

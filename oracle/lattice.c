@@ -22,9 +22,9 @@
  *                 a Jacobi pass with mass splitting (per-body sums as a 64-lane halving tree in contact order); SOR
  *                 sweeps red-black by lattice parity; stress judged on the field at the start of the pass, crushes
  *                 applied afterwards in brick order.
- * Common to both (deviations from the reference, all explicit): contact and SOR budgets are lifted (no MAXC cap, no
- * 160k brick-update budget, warm-up sweeps all run in the frame they are asked for); sphere neighbours are found in
- * the 27 cells around a brick by exact cell (no hash-bucket collisions or duplicates).
+ * Common to both (deviations from the reference, all explicit): no contact cap (MAXC); the load-field sweeps of
+ * loose pieces have no per-frame budget (standing structures keep the reference's 160k and 120k brick-update budgets);
+ * sphere neighbours are found in the 27 cells around a brick by exact cell (no hash-bucket collisions or duplicates).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -309,6 +309,8 @@ static void st_activate(LW* w, int s) {
     if (w->st_active[s] < 4.0f) w->st_active[s] = 4.0f;
 }
 void ll_warm_all(LW* w) { for (int s = 0; s < w->st_n; s++) warm_struct(w, s); }
+/* dbg_wake_all (tools/debug_exports.rs:107): every structure's load field and joints live */
+void ll_wake_all(LW* w) { for (int s = 0; s < w->st_n; s++) st_activate(w, s); }
 static void st_static_change(LW* w, int i) { int s = w->sid[i]; if (s < w->st_n) st_activate(w, s); }
 
 /* ------------------------------------------------------------------ topology changes (src/lib.rs:697-769, :816-843) */
@@ -1206,6 +1208,7 @@ static void structural(LW* w, float dt) {
         if (!any) for (uint32_t k = w->first[b]; k != NONE; k = w->next[k]) w->phi[k] = 0.0f;
     }
     uint32_t* list = w->lst;
+    int64_t budget = 160000;
     for (int t = 0; t < na; t++) {
         int s = act[t];
         int size = w->st_hi[s] - w->st_lo[s];
@@ -1216,7 +1219,8 @@ static void structural(LW* w, float dt) {
         for (;;) {
             w->st_resid[s] = sweep_list(w, list, n, 1.85f);
             k++;
-            if (k >= nmin && (w->st_resid[s] < 4e-4f || k >= 90)) break;
+            budget -= size;
+            if (k >= nmin && (w->st_resid[s] < 4e-4f || k >= 90 || budget <= 0)) break;
         }
     }
     float idt = 1.0f / lg_maxf(dt, 1e-3f);
@@ -1282,15 +1286,18 @@ static void structural(LW* w, float dt) {
     w->stats[13] = (float)na;
 }
 
-/* warm-up sweeps (src/lib.rs:1068-1079), all in the frame they are asked for */
+/* warm-up sweeps spread over frames, 120k brick updates per frame (warm_step, src/lib.rs:1068-1079) */
 static void warm_step(LW* w) {
+    int64_t budget = 120000;
     for (int s = 0; s < w->st_n; s++) {
-        if (w->st_warm[s] <= 0) continue;
+        if (w->st_warm[s] <= 0 || budget <= 0) continue;
         int n = struct_list(w, s, w->lst);
-        while (w->st_warm[s] > 0) {
+        int size = w->st_hi[s] - w->st_lo[s];
+        while (w->st_warm[s] > 0 && budget > 0) {
             float om = w->st_warm[s] > 60 ? 1.9f : 1.0f;
             w->st_resid[s] = sweep_list(w, w->lst, n, om);
             w->st_warm[s]--;
+            budget -= size;
             if (w->st_warm[s] == 0) w->st_settle[s] = 0.3f;
         }
     }

@@ -182,7 +182,7 @@ def _lattice_protos():
     L.ll_destroy.argtypes = [vp]
     L.ll_add_structure.argtypes = [vp, i32, vp, vp, vp]
     L.ll_add_body.argtypes = [vp, i32, vp, vp, vp, vp]
-    for f in ("ll_finalize", "ll_warm_all"):
+    for f in ("ll_finalize", "ll_warm_all", "ll_wake_all"):
         getattr(L, f).argtypes = [vp]
     L.ll_step.argtypes = [vp, f32]
     L.ll_explode.argtypes = [vp, f32, f32, f32, f32, f32]
@@ -233,6 +233,9 @@ class LatticeOracle:
 
     def warm_all(self):
         lib().ll_warm_all(self._h)
+
+    def wake_all(self):
+        lib().ll_wake_all(self._h)
 
     def step(self, dt=1.0 / 60.0):
         lib().ll_step(self._h, ctypes.c_float(dt))

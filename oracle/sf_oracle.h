@@ -99,6 +99,8 @@ int ll_add_body(LW* w, int n, const int32_t* xyz, const uint8_t* mat, const floa
 void ll_finalize(LW* w);
 /* ask every structure for its load field (warm_struct, lib.rs:1061-1066); the sweeps run in the next frame */
 void ll_warm_all(LW* w);
+/* every structure's load field and joints live (dbg_wake_all, tools/debug_exports.rs:107) */
+void ll_wake_all(LW* w);
 void ll_step(LW* w, float dt);
 void ll_explode(LW* w, float x, float y, float z, float r, float power);
 void ll_stats(const LW* w, float* out16);

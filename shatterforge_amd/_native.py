@@ -71,6 +71,7 @@ _PROTOS = {
     "sf_lat_add_body": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_lat_finalize": (c_int, [c_void_p, c_int]),
     "sf_lat_warm_all": (c_int, [c_void_p, c_int]),
+    "sf_lat_wake_all": (c_int, [c_void_p, c_int]),
     "sf_lat_explode": (c_int, [c_void_p, c_int, c_float, c_float, c_float, c_float, c_float]),
     "sf_lat_step": (c_int, [c_void_p, c_float]),
     "sf_lat_stats": (c_int, [c_void_p, c_int, c_void_p]),

@@ -1342,6 +1342,8 @@ int world_alloc(HostWorld* w) {
     SF_HIP(hipHostMalloc((void**)&w->h_ctr, sizeof(uint32_t) * C_NCTR, hipHostMallocDefault));
     SF_HIP(hipHostMalloc((void**)&w->h_col, sizeof(uint32_t) * 72, hipHostMallocDefault));
     for (auto& e : w->ev) SF_HIP(hipEventCreate(&e));
+    // the zero fills ran on the null stream, which the world's non-blocking stream does not wait for
+    SF_HIP(hipDeviceSynchronize());
     return 0;
 }
 

@@ -681,42 +681,59 @@ __device__ __forceinline__ void block_max2(float& a, float& b, float* sh) {
     __syncthreads();
 }
 
-// one workgroup per structure; mode 0: the frame's adaptive sweeps (nmin > 0), mode 1: warm-up (nmin sweeps)
+// One workgroup per scene: its structures in order sharing the frame's brick-update budget (src/lib.rs:1268-1281,
+// :1068-1079). mode 0: adaptive sweeps of live structures (st_nmin = sweeps asked for); mode 1: warm-up sweeps
+// (st_nmin = warm-up sweeps left, written back).
 constexpr int ST_NT = 1024;
-__global__ void __launch_bounds__(ST_NT) k_sor_struct(LD D, int mode) {
-    __shared__ float sh[2 * ST_NT / 64];
-    for (uint32_t s = blockIdx.x; s < D.nst; s += gridDim.x) {
-        int nmin = D.st_nmin[s];
-        if (nmin <= 0) continue;
-        if (mode == 0 && !D.st_act[s]) continue;
-        int sc = D.st_scene[s];
-        uint32_t frame = D.sc[sc].frame;
-        int sl = (int)s - D.sc[sc].st0;
-        int4 rb = D.st_rb[s];
-        float resid = D.st_resid[s];
-        int k = 0;
-        for (;;) {
-            float om = mode == 0 ? 1.85f : (nmin - k > 60 ? 1.9f : 1.0f);
-            float md = 0.0f, mp = 1.0f;
-            for (int par = 0; par < 2; par++) {
-                uint32_t o = par == 0 ? rb.x : rb.z, n = par == 0 ? rb.y : rb.w;
-                for (uint32_t t = threadIdx.x; t < n; t += ST_NT) {
-                    uint32_t i = D.st_red[o + t];
-                    if (!in_struct(D, i, sl)) continue;
-                    float d = sor_brick(D, i, om, frame);
-                    md = fmaxf(md, d);
-                    mp = fmaxf(mp, D.phi[i]);
-                }
-                __syncthreads();
-            }
-            block_max2<ST_NT>(md, mp, sh);
-            resid = md / mp;
-            k++;
-            if (mode == 0) { if (k >= nmin && (resid < 4e-4f || k >= 90)) break; }
-            else if (k >= nmin) break;
+__device__ float sweep_struct(LD& D, uint32_t g, int sl, float om, uint32_t frame, float* sh) {
+    int4 rb = D.st_rb[g];
+    float md = 0.0f, mp = 1.0f;
+    for (int par = 0; par < 2; par++) {
+        uint32_t o = par == 0 ? rb.x : rb.z, n = par == 0 ? rb.y : rb.w;
+        for (uint32_t t = threadIdx.x; t < n; t += ST_NT) {
+            uint32_t i = D.st_red[o + t];
+            if (!in_struct(D, i, sl)) continue;
+            float d = sor_brick(D, i, om, frame);
+            md = fmaxf(md, d);
+            mp = fmaxf(mp, D.phi[i]);
         }
-        if (threadIdx.x == 0) D.st_resid[s] = resid;
         __syncthreads();
+    }
+    block_max2<ST_NT>(md, mp, sh);
+    return md / mp;
+}
+__global__ void __launch_bounds__(ST_NT) k_sor_scene(LD D, int mode) {
+    __shared__ float sh[2 * ST_NT / 64];
+    for (uint32_t sc = blockIdx.x; sc < D.nscene; sc += gridDim.x) {
+        const SceneK S = D.sc[sc];
+        long long budget = mode == 0 ? 160000 : 120000;
+        for (int sl = 0; sl < S.nst; sl++) {
+            uint32_t g = (uint32_t)(S.st0 + sl);
+            int nmin = D.st_nmin[g];
+            if (nmin <= 0) continue;
+            if (mode == 0 && !D.st_act[g]) continue;
+            long long size = D.st_hi[g] - D.st_lo[g];
+            float resid = D.st_resid[g];
+            if (mode == 0) {
+                int k = 0;
+                for (;;) {
+                    resid = sweep_struct(D, g, sl, 1.85f, S.frame, sh);
+                    k++;
+                    budget -= size;
+                    if (k >= nmin && (resid < 4e-4f || k >= 90 || budget <= 0)) break;
+                }
+            } else {
+                int warm = nmin;
+                while (warm > 0 && budget > 0) {
+                    resid = sweep_struct(D, g, sl, warm > 60 ? 1.9f : 1.0f, S.frame, sh);
+                    warm--;
+                    budget -= size;
+                }
+                if (threadIdx.x == 0) D.st_nmin[g] = warm;
+            }
+            if (threadIdx.x == 0) D.st_resid[g] = resid;
+            __syncthreads();
+        }
     }
 }
 
@@ -968,31 +985,61 @@ __device__ float stress_eval(LD& D, uint32_t i, float dt, bool live, unsigned lo
     }
     return maxr;
 }
+// max of v over the lanes sharing each distinct key (key < 0: none), one atomic per key per wave
+__device__ __forceinline__ void wave_keyed_max(int key, float v, uint32_t* dst, uint32_t* dst2, const int32_t* key2of) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long active = __ballot(key >= 0);
+    while (active) {
+        int leader = __ffsll((long long)active) - 1;
+        int k = __shfl(key, leader, 64);
+        bool mine = key == k;
+        float x = mine ? v : 0.0f;
+        for (int h = 32; h >= 1; h >>= 1) x = fmaxf(x, __shfl_xor(x, h, 64));
+        if (lane == leader) {
+            atomic_max_pos(&dst[k], x);
+            if (dst2) atomic_max_pos(&dst2[key2of ? key2of[k] : 0], x);
+        }
+        active &= ~__ballot(mine);
+    }
+}
 __global__ void __launch_bounds__(NT) k_stress_static(LD D, float dt) {
-    GS(i, D.nb) {
-        if (D.body[i] >= 0) continue;
-        int sc = D.bxyz[i].w;
-        const SceneK& S = D.sc[sc];
-        int s = D.sid[i];
-        if (s >= S.nst) continue;
-        int gst = S.st0 + s;
-        if (!D.st_act[gst] || !in_struct(D, i, s)) continue;
-        unsigned long long key = ((unsigned long long)(uint32_t)gst << 40) | (unsigned long long)i;
-        float r = stress_eval(D, i, dt, D.st_live[gst] != 0, key, gst);
-        atomic_max_pos(&D.st_maxr[gst], r);
-        atomic_max_pos(&D.sc_gmax[sc], r);
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t base = wave * 64; base < D.nb; base += nwaves * 64) {
+        uint32_t i = base + lane;
+        int gst = -1;
+        float r = 0.0f;
+        if (i < D.nb && D.body[i] < 0) {
+            int sc = D.bxyz[i].w;
+            const SceneK& S = D.sc[sc];
+            int s = D.sid[i];
+            if (s < S.nst && D.st_act[S.st0 + s] && in_struct(D, i, s)) {
+                gst = S.st0 + s;
+                bool live = D.st_live[gst] != 0 && D.st_resid[gst] < 1.5e-3f;
+                unsigned long long key = ((unsigned long long)(uint32_t)gst << 40) | (unsigned long long)i;
+                r = stress_eval(D, i, dt, live, key, gst);
+            }
+        }
+        wave_keyed_max(gst, r, D.st_maxr, D.sc_gmax, D.st_scene);
     }
 }
 __global__ void __launch_bounds__(NT) k_stress_body(LD D) {
-    GS(k, D.ndyn) {
-        int b = D.dyn_body[k];
-        if (!D.bd_judge[b]) continue;
-        uint32_t i = D.dyn[k];
-        if (D.mat[i] == 0) continue;
-        unsigned long long key = (1ull << 63) | ((unsigned long long)(uint32_t)b << 40) | (unsigned long long)k;
-        float r = stress_eval(D, i, D.bd_dt[b], D.bd_live[b] != 0, key, -1);
-        atomic_max_pos((uint32_t*)&D.maxr[b], r);
-        atomic_max_pos(&D.sc_gmax[D.bd_scene[b]], r);
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t base = wave * 64; base < D.ndyn; base += nwaves * 64) {
+        uint32_t k = base + lane;
+        int b = -1;
+        float r = 0.0f;
+        if (k < D.ndyn) {
+            int bb = D.dyn_body[k];
+            uint32_t i = D.dyn[k];
+            if (D.bd_judge[bb] && D.mat[i] != 0) {
+                b = bb;
+                unsigned long long key = (1ull << 63) | ((unsigned long long)(uint32_t)b << 40) | (unsigned long long)k;
+                r = stress_eval(D, i, D.bd_dt[b], D.bd_live[b] != 0, key, -1);
+            }
+        }
+        wave_keyed_max(b, r, (uint32_t*)D.maxr, D.sc_gmax, D.bd_scene);
     }
 }
 __global__ void __launch_bounds__(NT) k_lt_decay(LD D, float dt) {
@@ -1073,7 +1120,9 @@ template <class T> struct DBuf {
         p = nullptr; n = cnt;
         if (!cnt) cnt = 1;
         if (hipMalloc(&p, sizeof(T) * cnt) != hipSuccess) { p = nullptr; return -1; }
-        return hipMemset(p, 0, sizeof(T) * cnt) == hipSuccess ? 0 : -1;
+        // the zero fill runs on the null stream, which the world's non-blocking stream does not wait for
+        if (hipMemset(p, 0, sizeof(T) * cnt) != hipSuccess) return -1;
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
     }
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
@@ -1117,6 +1166,10 @@ struct Lat {
     DBuf<uint32_t> d_scstat;
     uint32_t h_ctr[C_NCTR];
     uint32_t ccap = 0, tsize = 0;
+    // bricks whose material, body or bonds the host changed since the last flush
+    std::vector<uint8_t> touched;
+    std::vector<uint32_t> tlist;
+    void touch(int i) { if (!touched[i]) { touched[i] = 1; tlist.push_back((uint32_t)i); } }
     // timing
     double ms_frame = 0.0, ms_host = 0.0;
     int syncs = 0;
@@ -1178,6 +1231,8 @@ int lat_alloc(Lat* L) {
     r |= L->d_sc.alloc(S); r |= L->d_scstat.alloc(3 * S); r |= L->d_rec.alloc(D.pcap); r |= L->d_dis.alloc(D.pcap);
     D.sc = L->d_sc.p;
     D.K = (uint32_t)L->K; D.R = (uint32_t)L->R;
+    // the zero fills above ran on the null stream, which the world's non-blocking stream does not wait for
+    if (hipDeviceSynchronize() != hipSuccess) r = -1;
     return r ? -1 : 0;
 }
 
@@ -1202,6 +1257,7 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     r |= L->dalloc(D.odv, 2 * (size_t)cap); r |= L->dalloc(D.odw, 2 * (size_t)cap);
     D.ccap = cap;
     L->ccap = cap;
+    if (hipDeviceSynchronize() != hipSuccess) r = -1;
     return r ? -1 : 0;
 }
 int lat_hash_alloc(Lat* L, uint32_t ndyn) {
@@ -1214,6 +1270,7 @@ int lat_hash_alloc(Lat* L, uint32_t ndyn) {
     r |= L->dalloc(D.hcnt, t); r |= L->dalloc(D.hoff, t); r |= L->dalloc(D.hfill, t);
     L->tsize = t;
     D.tmask = t - 1;
+    if (hipDeviceSynchronize() != hipSuccess) r = -1;
     return r ? -1 : 0;
 }
 
@@ -1257,13 +1314,14 @@ void st_static_change(Lat* L, HScene& S, int i) { int s = L->sid[i]; if (s < S.n
 void remove_brick(Lat* L, HScene& S, int i) {
     if (L->mat[i] == 0) return;
     L->mat[i] = 0;
+    L->touch(i);
     if (L->body[i] < 0) {
         int s = L->sid[i];
         if (s < S.nst) { L->sts[S.st0 + s].conn = true; S.conn_any = true; }
         st_static_change(L, S, i);
     } else L->dirty[L->body[i]] = 1;
 }
-void remove_brick_raw(Lat* L, int i) { L->mat[i] = 0; L->body[i] = -1; }
+void remove_brick_raw(Lat* L, int i) { L->mat[i] = 0; L->body[i] = -1; L->touch(i); }
 uint32_t damage_brick(Lat* L, HScene& S, int i, float frac, float absolute) {
     uint32_t broke = 0;
     for (int d = 0; d < 6; d++) {
@@ -1272,12 +1330,14 @@ uint32_t damage_brick(Lat* L, HScene& S, int i, float frac, float absolute) {
         int o = (d & 1) == 0 ? i : j, a = d >> 1;
         float& bd = bond_ref(L, o, a);
         if (bd > 0.0f) {
+            L->touch(o);
             bd -= bmax_of(L, o, a) * frac + absolute;
             if (bd <= bmax_of(L, o, a) * 0.03f) { bd = 0.0f; broke++; }
         }
     }
     float& g = bond_ref(L, i, 3);
     if (g > 0.0f) {
+        L->touch(i);
         g -= bmax_of(L, i, 3) * frac * 0.5f + absolute * 0.5f;
         if (g <= 0.0f) { g = 0.0f; broke++; }
     }
@@ -1289,8 +1349,9 @@ uint32_t damage_brick(Lat* L, HScene& S, int i, float frac, float absolute) {
 }
 void sever_brick(Lat* L, int i) {
     L->bond[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    L->touch(i);
     const int ds[3] = {1, 3, 5};
-    for (int t = 0; t < 3; t++) { int j = hnbr(L, i, ds[t]); if (j >= 0) bond_ref(L, j, ds[t] >> 1) = 0.0f; }
+    for (int t = 0; t < 3; t++) { int j = hnbr(L, i, ds[t]); if (j >= 0) { bond_ref(L, j, ds[t] >> 1) = 0.0f; L->touch(j); } }
 }
 void wake_body(Lat* L, int b) { L->asleep[b] = 0; L->sleep[b] = 0.0f; }
 void kill_body_host(Lat* L, int b) {
@@ -1546,7 +1607,7 @@ uint32_t crush_at(Lat* L, HScene& S, int start, float e, LV p, uint32_t cap) {
 
 namespace {
 
-enum : int { G_MAT = 1, G_BOND = 2, G_BRICK = 4, G_BODY = 8, G_ALL = 15 };
+enum : int { G_MAT = 1, G_BOND = 2, G_BRICK = 4, G_BODY = 8, G_ALL = 15, G_WPOS = 16 };
 
 template <class T> int cp_down(T* dst, const T* src, size_t n, hipStream_t s) {
     if (n) SF_HIP(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, s));
@@ -1566,6 +1627,7 @@ int pull(Lat* L, int si, int groups) {
     int r = 0;
     if (groups & G_MAT) { r |= cp_down(&L->mat[b0], D.mat + b0, nb, s); r |= cp_down(&L->body[b0], D.body + b0, nb, s); }
     if (groups & G_BOND) r |= cp_down(&L->bond[b0], D.bond + b0, nb, s);
+    if ((groups & G_WPOS) && !(groups & G_BRICK)) r |= cp_down(&L->wpos[b0], D.wpos + b0, nb, s);
     if (groups & G_BRICK) {
         r |= cp_down(&L->wpos[b0], D.wpos + b0, nb, s); r |= cp_down(&L->phi[b0], D.phi + b0, nb, s);
         r |= cp_down(&L->ext[b0], D.ext + b0, nb, s); r |= cp_down(&L->extn[b0], D.extn + b0, nb, s);
@@ -1732,6 +1794,29 @@ int gather(Lat* L, const uint32_t* list, uint32_t n, std::vector<CRec>& out) {
     return 0;
 }
 
+struct BrickDelta { uint32_t i; int32_t body; uint32_t mat; uint32_t pad; float4 bond; };
+__global__ void k_brick_delta(LD D, const BrickDelta* d, uint32_t n) {
+    GS(t, n) { BrickDelta x = d[t]; D.mat[x.i] = (uint8_t)x.mat; D.body[x.i] = x.body; D.bond[x.i] = x.bond; }
+}
+DBuf<BrickDelta> g_delta;
+// upload the bricks the host changed (material, body, bonds)
+int flush_delta(Lat* L) {
+    size_t n = L->tlist.size();
+    if (!n) return 0;
+    std::vector<BrickDelta> d(n);
+    for (size_t t = 0; t < n; t++) {
+        uint32_t i = L->tlist[t];
+        d[t].i = i; d[t].body = L->body[i]; d[t].mat = L->mat[i]; d[t].pad = 0; d[t].bond = L->bond[i];
+        L->touched[i] = 0;
+    }
+    L->tlist.clear();
+    if (g_delta.n < n && g_delta.alloc(std::max<size_t>(n, 4096))) return -1;
+    SF_HIP(hipMemcpyAsync(g_delta.p, d.data(), sizeof(BrickDelta) * n, hipMemcpyHostToDevice, L->st));
+    LAUNCH(k_brick_delta, sf_grid(n), NT, L->D, g_delta.p, (uint32_t)n);
+    SF_HIP(hipStreamSynchronize(L->st));
+    return 0;
+}
+
 // (src/lib.rs:1493-1550) applied in contact order on the host mirror
 int plough_phase(Lat* L, uint32_t n) {
     n = std::min(n, L->D.pcap);
@@ -1785,6 +1870,7 @@ int plough_phase(Lat* L, uint32_t n) {
         if (dis.size() > L->D.pcap) { sf::set_error(SF_E_CAP, "too many plough events"); return SF_E_CAP; }
         SF_HIP(hipMemcpyAsync(L->d_dis.p, dis.data(), sizeof(int2) * dis.size(), hipMemcpyHostToDevice, L->st));
         LAUNCH(k_disable, sf_grid(L->h_ctr[C_NCS]), NT, L->D, L->d_dis.p, (uint32_t)dis.size(), L->h_ctr[C_NCS]);
+        SF_HIP(hipStreamSynchronize(L->st));
     }
     return rebuild_csr(L);
 }
@@ -1802,7 +1888,7 @@ int impact_phase(Lat* L, uint32_t n) {
         int si = scene_of_contact(L, c.c);
         if (si < 0) continue;
         HScene& S = L->sc[si];
-        if (!pulled[si]) { if (pull(L, si, G_ALL)) return -1; pulled[si] = 1; }
+        if (!pulled[si]) { if (pull(L, si, G_WPOS)) return -1; pulled[si] = 1; }
         int ia = (int)c.ba;
         int ma = L->mat[ia];
         if (ma == 0 || ma >= SLUG) continue;
@@ -1825,7 +1911,7 @@ int impact_phase(Lat* L, uint32_t n) {
         if (ea > 0.0f && bud > 0) { uint32_t k = crush_at(L, S, ia, ea, p, bud); bud -= k < bud ? k : bud; }
         if (soft_b > 0.0f && bud > 0) { uint32_t k = crush_at(L, S, jb, e - ea, p, bud); bud -= k < bud ? k : bud; }
     }
-    for (size_t si = 0; si < pulled.size(); si++) if (pulled[si] && push(L, (int)si, G_ALL)) return -1;
+    if (flush_delta(L)) return -1;
     return rebuild_csr(L);
 }
 
@@ -1969,7 +2055,7 @@ int structural(Lat* L, float dt) {
     LAUNCH(k_rest_sum, gbr, NT, D);
     LAUNCH(k_ext_mix, gbr, NT, D);
     LAUNCH(k_unsupported, gw, NT, D);
-    if (ns) LAUNCH(k_sor_struct, std::min<size_t>(ns, 2048), ST_NT, D, 0);
+    if (ns) LAUNCH(k_sor_scene, std::min<size_t>(L->sc.size(), 2048), ST_NT, D, 0);
     LAUNCH(k_ga, gw, NT, D, dt);
     LAUNCH(k_body_sor_prep, gb, NT, D);
     LAUNCH(k_sor_body, std::min<int>(std::max(L->nbd, 1), 8192), BD_NT, D);
@@ -2017,6 +2103,7 @@ int structural(Lat* L, float dt) {
         changed[si] = 1;
         if (S.crush_n < 300) { S.crush_n++; remove_brick(L, S, k); return; }
         bond_ref(L, i, a) = 0.0f;
+        L->touch(i);
         if (L->body[i] >= 0) L->dirty[L->body[i]] = 1;
         else { int s = L->sid[i]; if (s < S.nst) { HStruct& T = L->sts[S.st0 + s]; T.conn = true; S.conn_any = true; T.broke = true; } }
     };
@@ -2041,9 +2128,9 @@ int structural(Lat* L, float dt) {
     for (size_t si = 0; si < L->sc.size(); si++) {
         HScene& S = L->sc[si];
         memcpy(&S.stats[6], &gmax[si], 4);
-        if (changed[si] && push(L, (int)si, G_MAT | G_BOND)) return -1;
     }
-    return 0;
+    (void)changed;
+    return flush_delta(L);
 }
 }  // namespace
 
@@ -2103,14 +2190,17 @@ int frame(Lat* L, float dt) {
     for (const HStruct& T : L->sts) if (T.warm > 0) { any_warm = true; break; }
     if (any_warm) {
         if (structures_upload(L, true)) return -1;
-        LAUNCH(k_sor_struct, std::min<size_t>(L->sts.size(), 2048), ST_NT, D, 1);
+        LAUNCH(k_sor_scene, std::min<size_t>(L->sc.size(), 2048), ST_NT, D, 1);
         std::vector<float> resid(L->sts.size());
-        if (cp_down(resid.data(), D.st_resid, resid.size(), L->st)) return -1;
+        std::vector<int32_t> left(L->sts.size());
+        if (cp_down(resid.data(), D.st_resid, resid.size(), L->st) || cp_down(left.data(), D.st_nmin, left.size(), L->st)) return -1;
         SF_HIP(hipStreamSynchronize(L->st));
         for (size_t g = 0; g < L->sts.size(); g++) {
             HStruct& T = L->sts[g];
             if (T.warm <= 0) continue;
-            T.resid = resid[g]; T.warm = 0; T.settle = 0.3f;
+            T.resid = resid[g];
+            T.warm = left[g];
+            if (T.warm == 0) T.settle = 0.3f;
         }
     }
     if (host_topology(L, false)) return -1;
@@ -2345,6 +2435,7 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     LM z; for (int k = 0; k < 9; k++) z.m[k] = 0.0f;
     L->rot.assign(N, z); L->iil = L->rot; L->iiw = L->rot;
     L->m.assign(N, 0.0f); L->im = L->m; L->sleep = L->m; L->rad = L->m; L->cool = L->m; L->lt = L->m; L->resid = L->m; L->maxr = L->m; L->nmax = L->m;
+    L->touched.assign(B, 0);
     L->lst.assign(B + 1, 0); L->lst2.assign(B + 1, 0); L->comp.assign(B + 1, 0); L->cnt.assign(B + 2, 0);
     L->dyn.assign(B, 0); L->dyn_body.assign(B, 0); L->boff.assign(N, 0); L->bcnt.assign(N, 0);
     return h;
@@ -2426,6 +2517,13 @@ int sf_lat_warm_all(sf_lat* h, int scene) {
     Lat* L = &h->h.L;
     HScene& S = L->sc[scene];
     for (int sl = 0; sl < S.nst; sl++) st_warm(L, S.st0 + sl);
+    return 0;
+}
+int sf_lat_wake_all(sf_lat* h, int scene) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    HScene& S = L->sc[scene];
+    for (int sl = 0; sl < S.nst; sl++) st_activate(L, S, sl);
     return 0;
 }
 int sf_lat_explode(sf_lat* h, int scene, float x, float y, float z, float r, float power) {

@@ -64,6 +64,9 @@ class Lattice:
     def warm_all(self, scene):
         check(lib().sf_lat_warm_all(self._h, int(scene)))
 
+    def wake_all(self, scene):
+        check(lib().sf_lat_wake_all(self._h, int(scene)))
+
     def explode(self, scene, x, y, z, r, power):
         check(lib().sf_lat_explode(self._h, int(scene), *(ctypes.c_float(v) for v in (x, y, z, r, power))))
 
