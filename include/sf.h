@@ -202,6 +202,12 @@ int sf_lat_structs(sf_lat* L, int scene, float* out);
 int sf_lat_matstats(sf_lat* L, int scene, float* out50);
 int sf_lat_contacts(sf_lat* L, int scene, float* out, int cap);
 int sf_lat_rng(sf_lat* L, int scene, uint32_t* out);
+/* a structure's load field from a converged solve (sf_load_field, brick order of the structure): the structure is
+ * solved and its warm-up sweeps skipped (warm_struct/warm_step, src/lib.rs:1061-1079) */
+int sf_lat_set_field(sf_lat* L, int scene, int structure, const float* phi);
+/* joints failed under load since the scene was created: compression (crushed), tension, shear, footing
+ * (stress_brick, src/lib.rs:1153-1227) */
+int sf_lat_failures(sf_lat* L, int scene, uint32_t* out4);
 /* wall time of the last sf_lat_step in ms and the host-device synchronisations it made */
 int sf_lat_timing(sf_lat* L, double* ms_frame, int32_t* syncs);
 

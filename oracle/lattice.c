@@ -49,6 +49,7 @@ typedef struct LW {
     float time, blast_t, blast_r, blast_p, hsub;
     LV blast_c;
     int crush_n;
+    uint32_t fail[4];      /* joints failed under load: compression (crushed), tension, shear, footing */
     /* bricks */
     int nb, bcap;
     uint8_t *mat, *sid, *infill;
@@ -1128,10 +1129,12 @@ static float stress_brick(LW* w, int i, float dt, int live, int* ev_crush) {
                     if (ev_crush) { ev_crush[0] = a; ev_crush[1] = k; return maxr > r ? maxr : r; }
                     if (w->crush_n < 300) {
                         w->crush_n++;
+                        w->fail[0]++;
                         remove_brick(w, k);
                         return maxr > r ? maxr : r;
                     }
-                }
+                    w->fail[0]++;
+                } else w->fail[up > 0.0f ? 1 : 2]++;
                 w->bond[4 * i + a] = 0.0f;
                 if (dynb) w->dirty[w->body[i]] = 1;
                 else if (s < w->st_n) { w->st_conn[s] = 1; w->conn_any = 1; w->st_broke[s] = 1; }
@@ -1146,6 +1149,7 @@ static float stress_brick(LW* w, int i, float dt, int live, int* ev_crush) {
         if (live && r > 1.0f) {
             w->bond[4 * i + 3] -= w->bond[4 * i + 3] * dt * 2.2f * (r - 0.85f);
             if (w->bond[4 * i + 3] < w->bmax[4 * i + 3] * 0.08f) {
+                w->fail[3]++;
                 w->bond[4 * i + 3] = 0.0f;
                 if (s < w->st_n) { w->st_conn[s] = 1; w->conn_any = 1; w->st_broke[s] = 1; }
             }
@@ -1166,6 +1170,7 @@ static float stress_list(LW* w, const uint32_t* list, int n, float dt, int live)
     for (int t = 0; t < n; t++) {
         if (ev[2 * t] < 0) continue;
         int i = (int)list[t], a = ev[2 * t], k = ev[2 * t + 1];
+        w->fail[0]++;
         if (w->crush_n < 300) { w->crush_n++; remove_brick(w, k); continue; }
         w->bond[4 * i + a] = 0.0f;
         if (w->body[i] >= 0) w->dirty[w->body[i]] = 1;
@@ -1366,6 +1371,14 @@ void ll_explode(LW* w, float x, float y, float z, float r, float power) {
         apply_blast_to_body(w, b);
     }
 }
+
+/* a structure's load field taken from a converged solve: the warm-up is skipped (warm_struct then warm_step) */
+void ll_set_field(LW* w, int s, const float* phi) {
+    if (s < 0 || s >= w->st_n) return;
+    for (int i = w->st_lo[s]; i < w->st_hi[s]; i++) w->phi[i] = phi[i - w->st_lo[s]];
+    w->st_solved[s] = 1; w->st_warm[s] = 0; w->st_settle[s] = 0.3f; w->st_resid[s] = 0.0f;
+}
+void ll_failures(const LW* w, uint32_t* out4) { for (int k = 0; k < 4; k++) out4[k] = w->fail[k]; }
 
 /* ------------------------------------------------------------------ readers */
 void ll_stats(const LW* w, float* out16) { memcpy(out16, w->stats, sizeof w->stats); }

@@ -193,6 +193,8 @@ def _lattice_protos():
     L.ll_contacts.argtypes = [vp, vp, i32]
     L.ll_rng.argtypes = [vp]
     L.ll_rng.restype = u32
+    L.ll_set_field.argtypes = [vp, i32, vp]
+    L.ll_failures.argtypes = [vp, vp]
     L._ll_done = True
     return L
 
@@ -279,3 +281,12 @@ class LatticeOracle:
 
     def rng(self):
         return lib().ll_rng(self._h)
+
+    def set_field(self, s, phi):
+        phi = np.ascontiguousarray(phi, np.float32)
+        lib().ll_set_field(self._h, int(s), _p(phi))
+
+    def failures(self):
+        out = np.zeros(4, np.uint32)
+        lib().ll_failures(self._h, _p(out))
+        return out

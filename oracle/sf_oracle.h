@@ -113,6 +113,10 @@ void ll_structs(const LW* w, float* out);
 void ll_matstats(const LW* w, float* out50);
 int ll_contacts(const LW* w, float* out, int cap);
 uint32_t ll_rng(const LW* w);
+/* a structure's load field from a converged solve (brick order of the structure); marks it solved */
+void ll_set_field(LW* w, int s, const float* phi);
+/* joints failed under load since creation: compression (crushed), tension, shear, footing */
+void ll_failures(const LW* w, uint32_t* out4);
 
 #ifdef __cplusplus
 }

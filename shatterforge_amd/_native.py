@@ -82,6 +82,8 @@ _PROTOS = {
     "sf_lat_matstats": (c_int, [c_void_p, c_int, c_void_p]),
     "sf_lat_contacts": (c_int, [c_void_p, c_int, c_void_p, c_int]),
     "sf_lat_rng": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_lat_set_field": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "sf_lat_failures": (c_int, [c_void_p, c_int, c_void_p]),
     "sf_lat_timing": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
 

@@ -960,6 +960,7 @@ __device__ float stress_eval(LD& D, uint32_t i, float dt, bool live, unsigned lo
                     return maxr > r ? maxr : r;
                 }
                 nhp = 0.0f;
+                atomicAdd(&D.sc_fail[4 * D.bxyz[i].w + (up > 0.0f ? 1 : 2)], 1u);
                 if (dynb) D.bd_dirty[bi] = 1;
                 else atomicOr(&D.st_flags[gst], 3u);
             }
@@ -976,7 +977,7 @@ __device__ float stress_eval(LD& D, uint32_t i, float dt, bool live, unsigned lo
         if (r > maxr) maxr = r;
         if (live && r > 1.0f) {
             float g = bd.w - bd.w * dt * 2.2f * (r - 0.85f);
-            if (g < bm.w * 0.08f) { g = 0.0f; atomicOr(&D.st_flags[gst], 3u); }
+            if (g < bm.w * 0.08f) { g = 0.0f; atomicOr(&D.st_flags[gst], 3u); atomicAdd(&D.sc_fail[4 * D.bxyz[i].w + 3], 1u); }
             bd.w = g;
             D.bond[i] = bd;
             uint32_t e = atomicAdd(&D.ctr[C_NBW], 1u);
@@ -1075,8 +1076,9 @@ struct HScene {
     int crush_n = 0;
     bool conn_any = false, finalized = false, active = true;
     int gx0 = 0, gy0 = 0, gz0 = 0, gnx = 1, gny = 1, gnz = 1;
-    size_t goff = 0;
+    size_t goff = 0, gcap = 0;             // grid region of the scene and its size
     float stats[16] = {};
+    uint32_t crushed_joints = 0;           // compression failures (applied on the host)
     uint32_t dyn0 = 0, ndyn = 0;
     uint32_t c0 = 0, ncs = 0;              // contact range of the last substep
     bool host_dirty = false;               // host mirror changed since the last push
@@ -1212,7 +1214,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.st_scene, NS); r |= L->dalloc(D.st_lo, NS); r |= L->dalloc(D.st_hi, NS); r |= L->dalloc(D.st_act, NS);
     r |= L->dalloc(D.st_on, NS); r |= L->dalloc(D.st_live, NS); r |= L->dalloc(D.st_nmin, NS); r |= L->dalloc(D.st_resid, NS);
     r |= L->dalloc(D.st_maxr, NS); r |= L->dalloc(D.st_flags, NS); r |= L->dalloc(D.st_red, B); r |= L->dalloc(D.st_rb, NS);
-    r |= L->dalloc(D.sc_c0, S + 1); r |= L->dalloc(D.sc_gmax, S);
+    r |= L->dalloc(D.sc_c0, S + 1); r |= L->dalloc(D.sc_gmax, S); r |= L->dalloc(D.sc_fail, 4 * S);
     r |= L->dalloc(D.dslot, B); r |= L->dalloc(D.dkey, B); r |= L->dalloc(D.hent, B);
     r |= L->dalloc(D.ccnt, B); r |= L->dalloc(D.coff, B);
     r |= L->dalloc(D.mask, N); r |= L->dalloc(D.claim, N * (size_t)L->K);
@@ -2101,6 +2103,7 @@ int structural(Lat* L, float dt) {
         int si = L->bxyz[i].w;
         HScene& S = L->sc[si];
         changed[si] = 1;
+        S.crushed_joints++;
         if (S.crush_n < 300) { S.crush_n++; remove_brick(L, S, k); return; }
         bond_ref(L, i, a) = 0.0f;
         L->touch(i);
@@ -2253,8 +2256,8 @@ int finalize(Lat* L, int si) {
     S.gx0 = lo[0] - 1; S.gy0 = lo[1] - 1; S.gz0 = lo[2] - 1;
     S.gnx = hi[0] - lo[0] + 3; S.gny = hi[1] - lo[1] + 3; S.gnz = hi[2] - lo[2] + 3;
     size_t gsz = (size_t)S.gnx * S.gny * S.gnz;
-    S.goff = L->grid.size();
-    L->grid.resize(S.goff + gsz, 0);
+    if (gsz > S.gcap) { S.goff = L->grid.size(); S.gcap = gsz; L->grid.resize(S.goff + gsz, 0); }
+    std::fill(L->grid.begin() + S.goff, L->grid.begin() + S.goff + S.gcap, 0);
     for (int i = S.b0; i < S.b0 + S.nb; i++) {
         if (!L->mat[i]) continue;
         int4 c = L->bxyz[i];
@@ -2384,17 +2387,19 @@ int add_body(Lat* L, int si, int n, const int32_t* xyz, const uint8_t* mat, cons
 
 void clear_scene(Lat* L, int si, uint32_t seed) {
     HScene& S = L->sc[si];
-    for (int i = S.b0; i < S.b0 + S.bcap; i++) { L->mat[i] = 0; L->body[i] = -1; L->next[i] = NONE; L->sid[i] = 255; }
+    // the search stamps restart with the scene's stamp counter, so the old ones must go
+    for (int i = S.b0; i < S.b0 + S.bcap; i++) { L->mat[i] = 0; L->body[i] = -1; L->next[i] = NONE; L->sid[i] = 255; L->stamp[i] = 0; }
     for (int b = S.bd0; b < S.bd0 + S.dcap; b++) { L->alive[b] = 0; L->first[b] = NONE; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0; L->nc[b] = 0; }
     int st0 = S.st0, smax = (int)(L->sts.size()) ;
     (void)smax;
     for (int sl = 0; sl < S.nst; sl++) { HStruct fresh; fresh.scene = si; L->sts[st0 + sl] = fresh; }
     int b0 = S.b0, bcap = S.bcap, bd0 = S.bd0, dcap = S.dcap, stc = S.st0;
-    size_t goff = S.goff;
+    size_t goff = S.goff, gcap = S.gcap;
     int gnx = S.gnx, gny = S.gny, gnz = S.gnz;
     bool had_grid = S.finalized;
     S = HScene();
     S.b0 = b0; S.bcap = bcap; S.bd0 = bd0; S.dcap = dcap; S.st0 = stc;
+    S.goff = goff; S.gcap = gcap;  // the grid region is kept for the next finalize
     S.rng = seed ? seed : 0x9E3779B9u;
     if (had_grid) { for (size_t k = 0; k < (size_t)gnx * gny * gnz; k++) L->grid[goff + k] = 0; }
     L->lists_dirty = true;
@@ -2477,6 +2482,8 @@ int sf_lat_scene_clear(sf_lat* h, int scene, uint32_t seed) {
     clear_scene(L, scene, seed);
     LD& D = L->D;
     int r = 0;
+    uint32_t zf[4] = {0, 0, 0, 0};
+    r |= cp_up(D.sc_fail + 4 * (size_t)scene, zf, 4, L->st);
     r |= cp_up(D.mat + b0, &L->mat[b0], bc, L->st); r |= cp_up(D.body + b0, &L->body[b0], bc, L->st);
     r |= cp_up(D.sid + b0, &L->sid[b0], bc, L->st);
     r |= cp_up(D.alive + d0, &L->alive[d0], dc, L->st); r |= cp_up(D.asleep + d0, &L->asleep[d0], dc, L->st);
@@ -2654,6 +2661,28 @@ int sf_lat_contacts(sf_lat* h, int scene, float* out, int cap) {
         o[13] = depth[k]; o[14] = jn[k]; o[15] = jt1[k]; o[16] = jt2[k]; o[17] = vn0[k];
     }
     return n;
+}
+int sf_lat_set_field(sf_lat* h, int scene, int sl, const float* phi) {
+    if (!h || !phi || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    HScene& S = L->sc[scene];
+    if (sl < 0 || sl >= S.nst || !S.finalized) { sf::set_error(SF_E_ARG, "bad structure or scene not finalized"); return SF_E_ARG; }
+    HStruct& T = L->sts[S.st0 + sl];
+    for (int i = T.lo; i < T.hi; i++) L->phi[i] = phi[i - T.lo];
+    T.solved = true; T.warm = 0; T.settle = 0.3f; T.resid = 0.0f;
+    float z = 0.0f;
+    if (cp_up(L->D.phi + T.lo, &L->phi[T.lo], (size_t)(T.hi - T.lo), L->st) || cp_up(L->D.st_resid + S.st0 + sl, &z, 1, L->st)) return SF_E_HIP;
+    SF_HIP(hipStreamSynchronize(L->st));
+    return 0;
+}
+int sf_lat_failures(sf_lat* h, int scene, uint32_t* out4) {
+    if (!h || !out4 || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    Lat* L = &h->h.L;
+    uint32_t d[4];
+    if (cp_down(d, L->D.sc_fail + 4 * (size_t)scene, 4, L->st)) return SF_E_HIP;
+    SF_HIP(hipStreamSynchronize(L->st));
+    out4[0] = L->sc[scene].crushed_joints; out4[1] = d[1]; out4[2] = d[2]; out4[3] = d[3];
+    return 0;
 }
 int sf_lat_rng(sf_lat* h, int scene, uint32_t* out) {
     if (!h || !out || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;

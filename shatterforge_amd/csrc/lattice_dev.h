@@ -1,5 +1,5 @@
 // Device state of the lattice (legacy brick) engine: struct-of-arrays for bricks, bodies, contacts and the
-// per-scene tables, shared by the kernels (lattice_kernels.hip) and the host engine (lattice.hip).
+// per-scene tables, shared by the kernels and the host engine in lattice.hip.
 #pragma once
 #include "lat_math.h"
 
@@ -79,6 +79,7 @@ struct LD {
     int4* st_rb;               // red offset, red count, black offset, black count
     uint32_t* sc_c0;           // first contact of each scene this substep
     uint32_t* sc_gmax;         // f32 bits per scene
+    uint32_t* sc_fail;         // per scene: joints failed in tension, shear, at the footing (4 slots, [0] unused)
     // hash
     uint32_t *hcnt, *hoff, *hfill, *hent, *dslot;
     unsigned long long* dkey;

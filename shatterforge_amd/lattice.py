@@ -117,6 +117,15 @@ class Lattice:
         check(lib().sf_lat_rng(self._h, int(scene), ptr(out)))
         return int(out[0])
 
+    def set_field(self, scene, structure, phi):
+        phi = np.ascontiguousarray(phi, np.float32)
+        check(lib().sf_lat_set_field(self._h, int(scene), int(structure), ptr(phi)))
+
+    def failures(self, scene):
+        out = np.zeros(4, np.uint32)
+        check(lib().sf_lat_failures(self._h, int(scene), ptr(out)))
+        return out
+
     def timing(self):
         ms = np.zeros(1, np.float64)
         syncs = np.zeros(1, np.int32)
