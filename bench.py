@@ -46,6 +46,113 @@ def cpu_baseline(bodies, shape_body, shapes, prm, seconds=15.0, max_steps=20):
             "sample": f"oracle box step (reference order), same scene, first {n} steps from the same state, {dt:.1f} s"}
 
 
+def _lattice_mix(n_scenes, seed=11):
+    """what-if-sized scenes: frame buildings of 2k-40k bricks (log-uniform), 1-8 charges at column bases"""
+    from shatterforge_amd import serve
+    return [serve.make_query(k, seed * 1000 + k) for k in range(n_scenes)]
+
+
+def bench_lattice(frames=120, n_scenes=32, plain_scenes=4, plain_frames=40):
+    """legacy brick frame: the plain C1 tower collapse, a batch of what-if scenes in one world, and the same kind of
+    scenes stepped one world (one launch sequence) each"""
+    from shatterforge_amd import scenes, serve
+    from shatterforge_amd.lattice import Lattice
+    from shatterforge_amd.load import LoadBatch
+    out = {}
+    # C1: 12 x 12 x 40 tower, load field converged by the reference's warm-up sweeps, 4 charges at the column bases
+    mat, xyz, infill = scenes.tower()
+    xyz = np.array(xyz, np.int32)
+    xyz[:, 0] += 320 - 6
+    xyz[:, 2] += 320 - 6
+    L = Lattice(len(mat) + 64, 8192)
+    s = L.add_scene(len(mat) + 64, 8192, structs=2)
+    L.add_structure(s, mat, xyz, infill)
+    L.finalize(s)
+    L.warm_all(s)
+    nwarm = 0
+    while L.stats(s)[14] > 0 or nwarm < 30:
+        L.step(1.0 / 60.0)
+        nwarm += 1
+        if nwarm > 200:
+            break
+    for cx, cz in [(1, 1), (1, 11), (11, 1), (11, 11)]:  # the corners of the four 2 x 2 columns
+        L.explode(s, (cx - 6) * 0.5, 0.6, (cz - 6) * 0.5, 2.2, 1.4)
+    t0 = time.perf_counter()
+    for f in range(frames):
+        L.step(1.0 / 60.0)
+    el = time.perf_counter() - t0
+    st = L.stats(s)
+    out["plain_tower"] = {"bricks": int(len(mat)), "frames": frames, "ms_per_frame": el * 1e3 / frames,
+                          "brick_steps_per_s": len(mat) * frames / el, "warmup_frames": nwarm,
+                          "pieces": int(st[2]), "standing": int(st[0])}
+    L.close()
+    log(f"lattice plain tower: {el * 1e3 / frames:.2f} ms/frame")
+    # a batch of what-if scenes in one world vs the same kind of scenes one world each
+    qs = _lattice_mix(n_scenes)
+    def setup(world, slots, qq):
+        lb = LoadBatch()
+        for q in qq:
+            lb.add(q["mat"], q["xyz"], q["infill"])
+        lb.solve(rel_tol=1e-6)
+        for k, (sl, q) in enumerate(zip(slots, qq)):
+            st_ = world.add_structure(sl, q["mat"], q["xyz"], q["infill"])
+            world.finalize(sl)
+            world.set_field(sl, st_, lb.field(k).astype(np.float32))
+            for ch in q["charges"]:
+                world.explode(sl, *ch)
+        lb.close()
+    caps = [q["bricks"] + 64 for q in qs]
+    B = Lattice(sum(caps), 4096 * n_scenes, scenes=n_scenes)
+    slots = [B.add_scene(caps[k], 4096, structs=2, seed=qs[k]["rng_seed"]) for k in range(n_scenes)]
+    setup(B, slots, qs)
+    B.step(1.0 / 60.0)
+    t0 = time.perf_counter()
+    for f in range(frames):
+        B.step(1.0 / 60.0)
+    el = time.perf_counter() - t0
+    nbr = sum(q["bricks"] for q in qs)
+    batched = nbr * frames / el
+    B.close()
+    log(f"lattice batched {n_scenes} scenes ({nbr} bricks): {el * 1e3 / frames:.2f} ms/frame")
+    tot_steps, tot_t = 0, 0.0
+    for q in qs[:plain_scenes]:
+        P = Lattice(q["bricks"] + 64, 4096)
+        ps = P.add_scene(q["bricks"] + 64, 4096, structs=2, seed=q["rng_seed"])
+        setup(P, [ps], [q])
+        P.step(1.0 / 60.0)
+        t0 = time.perf_counter()
+        for f in range(plain_frames):
+            P.step(1.0 / 60.0)
+        tot_t += time.perf_counter() - t0
+        tot_steps += q["bricks"] * plain_frames
+        P.close()
+    plain = tot_steps / tot_t
+    out["batched"] = {"scenes": n_scenes, "bricks": nbr, "frames": frames, "ms_per_frame": el * 1e3 / frames,
+                      "brick_steps_per_s": batched, "plain_same_scenes_brick_steps_per_s": plain,
+                      "batched_over_plain": batched / plain}
+    log(f"lattice plain (one world per scene): {plain:.3e} brick-steps/s; batched/plain {batched / plain:.1f}")
+    return out
+
+
+def bench_whatif(n_queries=64, rate=200.0, frames=300, slots=64):
+    """the served what-if stream: Poisson arrivals, continuous batching, latency from arrival to answer"""
+    from shatterforge_amd import serve
+    qs = [serve.make_query(k, 7000 + k) for k in range(n_queries)]
+    arr = serve.poisson_arrivals(rate, n_queries, seed=5)
+    srv = serve.WhatIfServer(slots=slots, slot_bricks=max(q["bricks"] for q in qs) + 64, slot_bodies=4096, frames=frames)
+    t0 = time.perf_counter()
+    done = srv.run_stream(qs, arr)
+    el = time.perf_counter() - t0
+    lat = np.array([q["latency"] for q in done])
+    res = {"queries": len(done), "rate_per_s": rate, "frames_per_query": frames, "slots": slots,
+           "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+           "answered_per_s": len(done) / el, "brick_steps_per_s": srv.brick_steps / el,
+           "mean_bricks": float(np.mean([q["bricks"] for q in qs])), "batched_frames": srv.frames_run}
+    srv.close()
+    log(f"what-if: p50 {res['p50_ms']:.0f} ms p99 {res['p99_ms']:.0f} ms, {res['answered_per_s']:.1f} q/s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +164,8 @@ def main():
     ap.add_argument("--tail-min", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
+    ap.add_argument("--whatif-queries", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard by scene, see DESIGN.md)",
@@ -135,6 +244,9 @@ def main():
         "stages_ms": stages,
         "stats": st,
     }
+    if not args.no_lattice:
+        out["lattice"] = bench_lattice()
+        out["whatif"] = bench_whatif(n_queries=args.whatif_queries)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(bodies, shape_body, shapes, prm, args.cpu_seconds)

@@ -38,7 +38,7 @@ enum {
     SF_P_SLEEP_LIN, SF_P_SLEEP_ANG, SF_P_SLEEP_TIME, SF_P_SLEEP_DIST, SF_P_SLEEP_ROT, SF_P_ALLOW_SLEEP,
     SF_P_CELL,                      /* broadphase cell 1 m */
     SF_P_LEAK, SF_P_ANCHOR, SF_P_ANCHOR_PUSH, SF_P_ANCHOR_SLOP,
-    SF_P_COL_LIMIT,                 /* colours (<= 64); contacts beyond go to the overflow, solved in order */
+    SF_P_COL_LIMIT,                 /* colours (<= 64, default 16); contacts beyond go to the overflow: a Jacobi pass with mass splitting */
     SF_P_COL_ROUNDS,                /* colouring rounds (32) */
     SF_P_COL_SPREAD,                /* a contact picks among the lowest this-many free colours (4) */
     SF_P_TAIL_MIN,                  /* colour buckets smaller than this are solved in one workgroup (1024) */
@@ -210,6 +210,9 @@ int sf_lat_set_field(sf_lat* L, int scene, int structure, const float* phi);
 int sf_lat_failures(sf_lat* L, int scene, uint32_t* out4);
 /* wall time of the last sf_lat_step in ms and the host-device synchronisations it made */
 int sf_lat_timing(sf_lat* L, double* ms_frame, int32_t* syncs);
+/* wall ms of the last frame by phase: substep launches and syncs, plough, impact, topology (rebuild and
+ * connectivity), structural, warm-up, statistics, all */
+int sf_lat_phase_ms(sf_lat* L, double* out8);
 
 #ifdef __cplusplus
 }

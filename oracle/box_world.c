@@ -58,7 +58,7 @@ static void default_params(float* p) {
     p[P_MAXPUSH] = 3.0f; p[P_FRICTION] = 0.6f; p[P_SPEC] = 0.02f; p[P_SLEEP_LIN] = 0.05f; p[P_SLEEP_ANG] = 0.08f;
     p[P_SLEEP_TIME] = 0.5f; p[P_SLEEP_DIST] = 0.01f; p[P_SLEEP_ROT] = 0.02f; p[P_ALLOW_SLEEP] = 1.0f; p[P_CELL] = 1.0f;
     p[P_LEAK] = 1.0f; p[P_ANCHOR] = 1.0f; p[P_ANCHOR_PUSH] = 3.0f; p[P_ANCHOR_SLOP] = 0.0f;
-    p[P_COL_LIMIT] = 64; p[P_COL_ROUNDS] = 32; p[P_COL_SPREAD] = 4; p[P_RESERVED] = 0;
+    p[P_COL_LIMIT] = 16; p[P_COL_ROUNDS] = 32; p[P_COL_SPREAD] = 4; p[P_RESERVED] = 0;
 }
 
 ob_world* ob_create(int order) {

@@ -85,6 +85,7 @@ _PROTOS = {
     "sf_lat_set_field": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "sf_lat_failures": (c_int, [c_void_p, c_int, c_void_p]),
     "sf_lat_timing": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sf_lat_phase_ms": (c_int, [c_void_p, c_void_p]),
 }
 
 _lib = None

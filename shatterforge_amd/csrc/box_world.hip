@@ -1297,7 +1297,7 @@ void default_params(float* p) {
     p[SF_P_DAMPING] = 10.0f; p[SF_P_MAXPUSH] = 3.0f; p[SF_P_FRICTION] = 0.6f; p[SF_P_SPEC] = 0.02f;
     p[SF_P_SLEEP_LIN] = 0.05f; p[SF_P_SLEEP_ANG] = 0.08f; p[SF_P_SLEEP_TIME] = 0.5f; p[SF_P_SLEEP_DIST] = 0.01f;
     p[SF_P_SLEEP_ROT] = 0.02f; p[SF_P_ALLOW_SLEEP] = 1.0f; p[SF_P_CELL] = 1.0f; p[SF_P_LEAK] = 1.0f;
-    p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 64;
+    p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 16;
     p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_TAIL_MIN] = 1024;
 }
 

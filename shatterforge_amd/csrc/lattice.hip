@@ -1171,9 +1171,16 @@ struct Lat {
     // bricks whose material, body or bonds the host changed since the last flush
     std::vector<uint8_t> touched;
     std::vector<uint32_t> tlist;
-    void touch(int i) { if (!touched[i]) { touched[i] = 1; tlist.push_back((uint32_t)i); } }
+    // what the host wrote of a brick: 1 material, body, bonds; 2 local and world position (body set-up); 4 load
+    // field (detachment zeroes it). Fields not written stay as the device has them.
+    void touch(int i, uint8_t what = 1) { if (!touched[i]) tlist.push_back((uint32_t)i); touched[i] |= what; }
+    // bodies whose state the host changed since the last flush
+    std::vector<uint8_t> btouched;
+    std::vector<uint32_t> btlist;
+    void btouch(int b) { if (!btouched[b]) { btouched[b] = 1; btlist.push_back((uint32_t)b); } }
     // timing
     double ms_frame = 0.0, ms_host = 0.0;
+    double ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int syncs = 0;
 
     ~Lat() {
@@ -1355,7 +1362,7 @@ void sever_brick(Lat* L, int i) {
     const int ds[3] = {1, 3, 5};
     for (int t = 0; t < 3; t++) { int j = hnbr(L, i, ds[t]); if (j >= 0) { bond_ref(L, j, ds[t] >> 1) = 0.0f; L->touch(j); } }
 }
-void wake_body(Lat* L, int b) { L->asleep[b] = 0; L->sleep[b] = 0.0f; }
+void wake_body(Lat* L, int b) { L->asleep[b] = 0; L->sleep[b] = 0.0f; L->btouch(b); }
 void kill_body_host(Lat* L, int b) {
     uint32_t k = L->first[b];
     while (k != NONE) {
@@ -1367,6 +1374,7 @@ void kill_body_host(Lat* L, int b) {
     L->lists_dirty = true;
 }
 void clamp_vel(Lat* L, int b) {
+    L->btouch(b);
     LV vv = hv(L->v[b]);
     float vl = len(vv);
     if (vl > 60.0f) L->v[b] = hf(mul(vv, 60.0f / vl));
@@ -1376,6 +1384,7 @@ void clamp_vel(Lat* L, int b) {
     if (a > amax) L->w[b] = hf(mul(ww, amax / a));
 }
 void impulse_at(Lat* L, int b, LV p, LV j) {
+    L->btouch(b);
     L->v[b] = hf(madd(hv(L->v[b]), j, L->im[b]));
     LV r = sub(p, hv(L->pos[b]));
     L->w[b] = hf(add(hv(L->w[b]), mv(L->iiw[b], cross(r, j))));
@@ -1456,7 +1465,8 @@ void setup_body(Lat* L, int b, int s, int e, LV ppos, const LM& prot, LQ pq, LV 
     L->iiw[b] = mmul(t1, pt);
     L->sleep[b] = 0.0f; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0; L->cool[b] = 0.15f;
     L->rad[b] = sqrtf(radv) + H * 1.8f;
-    for (int t = s; t < e; t++) { int i = L->lst2[t]; L->wpos[i] = hf(add(hv(L->pos[b]), mv(prot, hv(L->local[i])))); }
+    for (int t = s; t < e; t++) { int i = L->lst2[t]; L->wpos[i] = hf(add(hv(L->pos[b]), mv(prot, hv(L->local[i])))); L->touch(i, 3); }
+    L->btouch(b);
     L->lists_dirty = true;
 }
 void rebuild_body(Lat* L, HScene& S, int id) {
@@ -1465,10 +1475,11 @@ void rebuild_body(Lat* L, HScene& S, int id) {
     uint32_t k = L->first[id];
     while (k != NONE) {
         uint32_t nx = L->next[k];
-        if (L->body[k] == id) { if (L->mat[k] != 0) L->lst[n++] = k; else L->body[k] = -1; }
+        if (L->body[k] == id) { if (L->mat[k] != 0) L->lst[n++] = k; else { L->body[k] = -1; L->touch(k); } }
         k = nx;
     }
     L->lists_dirty = true;
+    L->btouch(id);
     if (n == 0) { L->alive[id] = 0; L->first[id] = NONE; return; }
     uint32_t sid = ++S.stamp_id;
     uint32_t ncomp = 0;
@@ -1542,6 +1553,7 @@ void conn_struct(Lat* L, HScene& S, int sl) {
             int i = L->lst2[t];
             L->local[i] = hf(lat_pos(L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z));
             L->phi[i] = 0.0f; L->ext[i] = 0.0f; L->extn[i] = 0.0f;
+            L->touch(i, 7);
         }
         LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
         setup_body(L, b, 0, n, lv(0.0f, 0.0f, 0.0f), IDM, idq, lv(0.0f, 0.0f, 0.0f), lv(0.0f, 0.0f, 0.0f));
@@ -1796,31 +1808,115 @@ int gather(Lat* L, const uint32_t* list, uint32_t n, std::vector<CRec>& out) {
     return 0;
 }
 
-struct BrickDelta { uint32_t i; int32_t body; uint32_t mat; uint32_t pad; float4 bond; };
+struct BrickDelta { uint32_t i; int32_t body; uint32_t mat; uint32_t full; float4 bond, local, wpos; float phi, ext, extn, pad2; };
 __global__ void k_brick_delta(LD D, const BrickDelta* d, uint32_t n) {
-    GS(t, n) { BrickDelta x = d[t]; D.mat[x.i] = (uint8_t)x.mat; D.body[x.i] = x.body; D.bond[x.i] = x.bond; }
+    GS(t, n) {
+        BrickDelta x = d[t];
+        D.mat[x.i] = (uint8_t)x.mat; D.body[x.i] = x.body; D.bond[x.i] = x.bond;
+        if (x.full & 2u) { D.local[x.i] = x.local; D.wpos[x.i] = x.wpos; }
+        if (x.full & 4u) { D.phi[x.i] = x.phi; D.ext[x.i] = x.ext; D.extn[x.i] = x.extn; }
+    }
+}
+// a body's full state, packed for one copy each way
+struct BodyRec {
+    uint32_t b; uint8_t alive, asleep, wake, pad0; uint32_t count, nc;
+    float4 pos, q, v, w, vlast, wlast;
+    float4 rot[3], iil[3], iiw[3];
+    float m, im, sleep, rad, cool, lt, resid, maxr, nmax, pad1, pad2, pad3;
+};
+__global__ void k_body_pack(LD D, uint32_t b0, uint32_t n, BodyRec* out) {
+    GS(t, n) {
+        uint32_t b = b0 + t;
+        BodyRec r;
+        r.b = b; r.alive = D.alive[b]; r.asleep = D.asleep[b]; r.wake = D.wake[b]; r.pad0 = 0; r.count = D.count[b]; r.nc = D.nc[b];
+        r.pos = D.pos[b]; r.q = D.q[b]; r.v = D.v[b]; r.w = D.w[b]; r.vlast = D.vlast[b]; r.wlast = D.wlast[b];
+        for (int k = 0; k < 3; k++) { r.rot[k] = D.rot[3 * (size_t)b + k]; r.iil[k] = D.iil[3 * (size_t)b + k]; r.iiw[k] = D.iiw[3 * (size_t)b + k]; }
+        r.m = D.m[b]; r.im = D.im[b]; r.sleep = D.sleep[b]; r.rad = D.rad[b]; r.cool = D.cool[b]; r.lt = D.lt[b];
+        r.resid = D.resid[b]; r.maxr = D.maxr[b]; r.nmax = D.nmax[b]; r.pad1 = r.pad2 = r.pad3 = 0.0f;
+        out[t] = r;
+    }
+}
+__global__ void k_body_unpack(LD D, const BodyRec* in, uint32_t n) {
+    GS(t, n) {
+        BodyRec r = in[t];
+        uint32_t b = r.b;
+        D.alive[b] = r.alive; D.asleep[b] = r.asleep; D.wake[b] = r.wake; D.count[b] = r.count; D.nc[b] = r.nc;
+        D.pos[b] = r.pos; D.q[b] = r.q; D.v[b] = r.v; D.w[b] = r.w; D.vlast[b] = r.vlast; D.wlast[b] = r.wlast;
+        for (int k = 0; k < 3; k++) { D.rot[3 * (size_t)b + k] = r.rot[k]; D.iil[3 * (size_t)b + k] = r.iil[k]; D.iiw[3 * (size_t)b + k] = r.iiw[k]; }
+        D.m[b] = r.m; D.im[b] = r.im; D.sleep[b] = r.sleep; D.rad[b] = r.rad; D.cool[b] = r.cool; D.lt[b] = r.lt;
+        D.resid[b] = r.resid; D.maxr[b] = r.maxr; D.nmax[b] = r.nmax;
+    }
 }
 DBuf<BrickDelta> g_delta;
-// upload the bricks the host changed (material, body, bonds)
-int flush_delta(Lat* L) {
-    size_t n = L->tlist.size();
+DBuf<BodyRec> g_brec;
+float4 m_row(const LM& m, int k) { return make_float4(m.m[3 * k], m.m[3 * k + 1], m.m[3 * k + 2], 0.0f); }
+void row_m(LM& m, int k, float4 r) { m.m[3 * k] = r.x; m.m[3 * k + 1] = r.y; m.m[3 * k + 2] = r.z; }
+
+// the scene's body slots in use, from the device into the host mirror (one packing kernel, one copy)
+int pull_bodies(Lat* L, int si) {
+    const HScene& S = L->sc[si];
+    uint32_t n = (uint32_t)S.hw;
     if (!n) return 0;
-    std::vector<BrickDelta> d(n);
-    for (size_t t = 0; t < n; t++) {
-        uint32_t i = L->tlist[t];
-        d[t].i = i; d[t].body = L->body[i]; d[t].mat = L->mat[i]; d[t].pad = 0; d[t].bond = L->bond[i];
-        L->touched[i] = 0;
-    }
-    L->tlist.clear();
-    if (g_delta.n < n && g_delta.alloc(std::max<size_t>(n, 4096))) return -1;
-    SF_HIP(hipMemcpyAsync(g_delta.p, d.data(), sizeof(BrickDelta) * n, hipMemcpyHostToDevice, L->st));
-    LAUNCH(k_brick_delta, sf_grid(n), NT, L->D, g_delta.p, (uint32_t)n);
+    if (g_brec.n < n && g_brec.alloc(std::max<size_t>(n, 4096))) return -1;
+    LAUNCH(k_body_pack, sf_grid(n), NT, L->D, (uint32_t)S.bd0, n, g_brec.p);
+    std::vector<BodyRec> r(n);
+    SF_HIP(hipMemcpyAsync(r.data(), g_brec.p, sizeof(BodyRec) * n, hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    for (uint32_t t = 0; t < n; t++) {
+        const BodyRec& x = r[t];
+        int b = (int)x.b;
+        L->alive[b] = x.alive; L->asleep[b] = x.asleep; L->wake[b] = x.wake; L->nc[b] = x.nc;
+        L->pos[b] = x.pos; L->qq[b] = x.q; L->v[b] = x.v; L->w[b] = x.w; L->vlast[b] = x.vlast; L->wlast[b] = x.wlast;
+        for (int k = 0; k < 3; k++) { row_m(L->rot[b], k, x.rot[k]); row_m(L->iiw[b], k, x.iiw[k]); }
+        L->sleep[b] = x.sleep; L->cool[b] = x.cool; L->lt[b] = x.lt; L->resid[b] = x.resid; L->maxr[b] = x.maxr; L->nmax[b] = x.nmax;
+    }
+    return 0;
+}
+// upload what the host changed: bricks (material, body, bonds, local and world position, load field) and bodies
+int flush_delta(Lat* L) {
+    size_t n = L->tlist.size(), nb = L->btlist.size();
+    if (n) {
+        std::vector<BrickDelta> d(n);
+        for (size_t t = 0; t < n; t++) {
+            uint32_t i = L->tlist[t];
+            BrickDelta& x = d[t];
+            x.i = i; x.body = L->body[i]; x.mat = L->mat[i]; x.full = L->touched[i]; x.bond = L->bond[i]; x.local = L->local[i]; x.wpos = L->wpos[i];
+            x.phi = L->phi[i]; x.ext = L->ext[i]; x.extn = L->extn[i]; x.pad2 = 0.0f;
+            L->touched[i] = 0;
+        }
+        L->tlist.clear();
+        if (g_delta.n < n && g_delta.alloc(std::max<size_t>(n, 4096))) return -1;
+        SF_HIP(hipMemcpyAsync(g_delta.p, d.data(), sizeof(BrickDelta) * n, hipMemcpyHostToDevice, L->st));
+        LAUNCH(k_brick_delta, sf_grid(n), NT, L->D, g_delta.p, (uint32_t)n);
+        SF_HIP(hipStreamSynchronize(L->st));
+    }
+    if (nb) {
+        std::vector<BodyRec> r(nb);
+        for (size_t t = 0; t < nb; t++) {
+            uint32_t b = L->btlist[t];
+            BodyRec& x = r[t];
+            x.b = b; x.alive = L->alive[b]; x.asleep = L->asleep[b]; x.wake = L->wake[b]; x.pad0 = 0; x.count = L->count[b]; x.nc = L->nc[b];
+            x.pos = L->pos[b]; x.q = L->qq[b]; x.v = L->v[b]; x.w = L->w[b]; x.vlast = L->vlast[b]; x.wlast = L->wlast[b];
+            for (int k = 0; k < 3; k++) { x.rot[k] = m_row(L->rot[b], k); x.iil[k] = m_row(L->iil[b], k); x.iiw[k] = m_row(L->iiw[b], k); }
+            x.m = L->m[b]; x.im = L->im[b]; x.sleep = L->sleep[b]; x.rad = L->rad[b]; x.cool = L->cool[b]; x.lt = L->lt[b];
+            x.resid = L->resid[b]; x.maxr = L->maxr[b]; x.nmax = L->nmax[b]; x.pad1 = x.pad2 = x.pad3 = 0.0f;
+            L->btouched[b] = 0;
+        }
+        L->btlist.clear();
+        if (g_brec.n < nb && g_brec.alloc(std::max<size_t>(nb, 4096))) return -1;
+        SF_HIP(hipMemcpyAsync(g_brec.p, r.data(), sizeof(BodyRec) * nb, hipMemcpyHostToDevice, L->st));
+        LAUNCH(k_body_unpack, sf_grid(nb), NT, L->D, g_brec.p, (uint32_t)nb);
+        SF_HIP(hipStreamSynchronize(L->st));
+    }
     return 0;
 }
 
 // (src/lib.rs:1493-1550) applied in contact order on the host mirror
+double ms_since(std::chrono::steady_clock::time_point t) { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
 int plough_phase(Lat* L, uint32_t n) {
+    auto t0 = std::chrono::steady_clock::now();
+    struct Acc { Lat* L; std::chrono::steady_clock::time_point t; ~Acc() { L->ph[1] += ms_since(t); } } acc{L, t0};
     n = std::min(n, L->D.pcap);
     std::vector<CRec> rec;
     if (gather(L, L->D.plist, n, rec)) return -1;
@@ -1831,7 +1927,7 @@ int plough_phase(Lat* L, uint32_t n) {
         int si = scene_of_contact(L, c.c);
         if (si < 0) continue;
         HScene& S = L->sc[si];
-        if (!pulled[si]) { if (pull(L, si, G_ALL)) return -1; pulled[si] = 1; }
+        if (!pulled[si]) { if (pull_bodies(L, si)) return -1; pulled[si] = 1; }
         int j = (int)c.bb;
         if (L->mat[j] == 0 || L->body[j] >= 0) { dis.push_back(make_int2((int)c.c, -1)); continue; }
         int a = c.a;
@@ -1862,12 +1958,14 @@ int plough_phase(Lat* L, uint32_t n) {
         LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
         setup_body(L, b, 0, 1, lv(0.0f, 0.0f, 0.0f), IDM, idq, vv, lv(0.0f, 0.0f, 0.0f));
         L->w[b] = hf(ww);
+        L->btouch(b);
         float mm = L->m[a];
         float f = mm / (mm + T.mass[mj] * 1.5f);
         L->v[a] = hf(mul(hv(L->v[a]), f));
+        L->btouch(a);
         dis.push_back(make_int2((int)c.c, j));
     }
-    for (size_t si = 0; si < pulled.size(); si++) if (pulled[si] && push(L, (int)si, G_ALL)) return -1;
+    if (flush_delta(L)) return -1;
     if (!dis.empty()) {
         if (dis.size() > L->D.pcap) { sf::set_error(SF_E_CAP, "too many plough events"); return SF_E_CAP; }
         SF_HIP(hipMemcpyAsync(L->d_dis.p, dis.data(), sizeof(int2) * dis.size(), hipMemcpyHostToDevice, L->st));
@@ -1879,6 +1977,7 @@ int plough_phase(Lat* L, uint32_t n) {
 
 // (src/lib.rs:1610-1642) in contact order, a budget of 2500 crushed bricks per scene and substep
 int impact_phase(Lat* L, uint32_t n) {
+    struct Acc { Lat* L; std::chrono::steady_clock::time_point t; ~Acc() { L->ph[2] += ms_since(t); } } acc{L, std::chrono::steady_clock::now()};
     n = std::min(n, L->D.pcap);
     std::vector<CRec> rec;
     if (gather(L, L->D.ilist, n, rec)) return -1;
@@ -2157,16 +2256,17 @@ int scene_stats(Lat* L) {
 }
 
 int host_topology(Lat* L, bool rebuild) {
+    struct Acc { Lat* L; std::chrono::steady_clock::time_point t; ~Acc() { L->ph[3] += ms_since(t); } } acc{L, std::chrono::steady_clock::now()};
     for (size_t si = 0; si < L->sc.size(); si++) {
         HScene& S = L->sc[si];
         bool need = S.conn_any;
         if (rebuild) for (int lb = 0; lb < S.hw && !need; lb++) { int b = S.bd0 + lb; need = L->alive[b] && L->dirty[b]; }
         if (!need) continue;
-        if (pull(L, (int)si, G_ALL)) return -1;
+        if (pull_bodies(L, (int)si)) return -1;
         if (rebuild) for (int lb = 0; lb < S.hw; lb++) { int b = S.bd0 + lb; if (L->alive[b] && L->dirty[b]) rebuild_body(L, S, b); }
         connectivity(L, S);
-        if (push(L, (int)si, G_ALL)) return -1;
     }
+    if (flush_delta(L)) return -1;
     if (rebuild_csr(L)) return -1;
     return upload_scenes(L);
 }
@@ -2174,20 +2274,26 @@ int host_topology(Lat* L, bool rebuild) {
 int frame(Lat* L, float dt) {
     auto t0 = std::chrono::steady_clock::now();
     L->syncs = 0;
+    for (double& x : L->ph) x = 0.0;
     dt = clampf(dt, 0.0f, 1.0f / 30.0f);
     for (HScene& S : L->sc) {
         S.time += dt; S.crush_n = 0; S.frame += 1;
         if (S.blast_t > 0.0f) S.blast_t -= dt;
     }
     if (rebuild_csr(L) || upload_scenes(L)) return -1;
+    auto ts = std::chrono::steady_clock::now();
     for (int k = 0; k < 2; k++) if (substep(L, dt * 0.5f)) return -1;
+    L->ph[0] = ms_since(ts) - L->ph[1] - L->ph[2];
     if (read_ctr(L) || drain_kills(L)) return -1;
     if (host_topology(L, true)) return -1;
     LD& D = L->D;
     const unsigned gb = sf_grid(L->nbd), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
     LAUNCH(k_cache, gb, NT, D, 0, 0.0f);
     LAUNCH(k_wpos, gd, NT, D);
+    ts = std::chrono::steady_clock::now();
     if (structural(L, dt)) return -1;
+    L->ph[4] = ms_since(ts);
+    ts = std::chrono::steady_clock::now();
     // warm-up sweeps of structures asked for a load field
     bool any_warm = false;
     for (const HStruct& T : L->sts) if (T.warm > 0) { any_warm = true; break; }
@@ -2206,12 +2312,16 @@ int frame(Lat* L, float dt) {
             if (T.warm == 0) T.settle = 0.3f;
         }
     }
+    L->ph[5] = ms_since(ts);
     if (host_topology(L, false)) return -1;
     LAUNCH(k_cache, gb, NT, D, 0, 0.0f);
     LAUNCH(k_wpos, sf_grid(std::max<uint32_t>(L->ndyn, 1)), NT, D);
+    ts = std::chrono::steady_clock::now();
     if (scene_stats(L)) return -1;
+    L->ph[6] = ms_since(ts);
     SF_HIP(hipGetLastError());
     L->ms_frame = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    L->ph[7] = L->ms_frame;
     return 0;
 }
 
@@ -2324,6 +2434,11 @@ int finalize(Lat* L, int si) {
     r |= cp_up(D.bd_scene + S.bd0, bsc.data(), S.dcap, s);
     if (r) return -1;
     if (push(L, si, G_BODY)) return -1;
+    // everything the host set up is on the device now: nothing left to flush
+    for (uint32_t i : L->tlist) L->touched[i] = 0;
+    L->tlist.clear();
+    for (uint32_t b : L->btlist) L->btouched[b] = 0;
+    L->btlist.clear();
     S.finalized = true;
     L->lists_dirty = true;
     if (rebuild_csr(L) || upload_scenes(L)) return -1;
@@ -2342,7 +2457,7 @@ void blast_brick(Lat* L, HScene& S, int i, LV c, float r, float power) {
 }
 int explode(Lat* L, int si, float x, float y, float z, float r, float power) {
     HScene& S = L->sc[si];
-    if (pull(L, si, G_ALL)) return -1;
+    if (pull_bodies(L, si) || pull(L, si, G_WPOS)) return -1;
     LV c = lv(x, y, z);
     S.blast_c = c; S.blast_r = r * 2.2f; S.blast_p = power; S.blast_t = 0.25f;
     float reach = r * 1.4f * 1.35f + 0.5f;
@@ -2366,7 +2481,7 @@ int explode(Lat* L, int si, float x, float y, float z, float r, float power) {
         wake_body(L, b);
         apply_blast_to_body(L, S, b);
     }
-    if (push(L, si, G_ALL)) return -1;
+    if (flush_delta(L)) return -1;
     return upload_scenes(L);
 }
 
@@ -2441,6 +2556,7 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     L->rot.assign(N, z); L->iil = L->rot; L->iiw = L->rot;
     L->m.assign(N, 0.0f); L->im = L->m; L->sleep = L->m; L->rad = L->m; L->cool = L->m; L->lt = L->m; L->resid = L->m; L->maxr = L->m; L->nmax = L->m;
     L->touched.assign(B, 0);
+    L->btouched.assign(N, 0);
     L->lst.assign(B + 1, 0); L->lst2.assign(B + 1, 0); L->comp.assign(B + 1, 0); L->cnt.assign(B + 2, 0);
     L->dyn.assign(B, 0); L->dyn_body.assign(B, 0); L->boff.assign(N, 0); L->bcnt.assign(N, 0);
     return h;
@@ -2687,6 +2803,11 @@ int sf_lat_failures(sf_lat* h, int scene, uint32_t* out4) {
 int sf_lat_rng(sf_lat* h, int scene, uint32_t* out) {
     if (!h || !out || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
     *out = h->h.L.sc[scene].rng;
+    return 0;
+}
+int sf_lat_phase_ms(sf_lat* h, double* out8) {
+    if (!h || !out8) return SF_E_ARG;
+    for (int k = 0; k < 8; k++) out8[k] = h->h.L.ph[k];
     return 0;
 }
 int sf_lat_timing(sf_lat* h, double* ms_frame, int32_t* syncs) {

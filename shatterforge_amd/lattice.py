@@ -126,6 +126,12 @@ class Lattice:
         check(lib().sf_lat_failures(self._h, int(scene), ptr(out)))
         return out
 
+    def phase_ms(self):
+        """wall ms of the last frame: substeps, plough, impact, topology, structural, warm-up, statistics, all"""
+        out = np.zeros(8, np.float64)
+        check(lib().sf_lat_phase_ms(self._h, ptr(out)))
+        return dict(zip(["substeps", "plough", "impact", "topology", "structural", "warmup", "stats", "frame"], out.tolist()))
+
     def timing(self):
         ms = np.zeros(1, np.float64)
         syncs = np.zeros(1, np.int32)

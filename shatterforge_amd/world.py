@@ -42,7 +42,7 @@ def default_params():
     p[P["LEAK"]] = 1.0
     p[P["ANCHOR"]] = 1.0
     p[P["ANCHOR_PUSH"]] = 3.0
-    p[P["COL_LIMIT"]] = 64
+    p[P["COL_LIMIT"]] = 16
     p[P["COL_ROUNDS"]] = 32
     p[P["COL_SPREAD"]] = 4
     p[P["TAIL_MIN"]] = 1024
