@@ -685,17 +685,84 @@ __device__ __forceinline__ void block_max2(float& a, float& b, float* sh) {
 // :1068-1079). mode 0: adaptive sweeps of live structures (st_nmin = sweeps asked for); mode 1: warm-up sweeps
 // (st_nmin = warm-up sweeps left, written back).
 constexpr int ST_NT = 1024;
+// the stencil of every brick of the structures that sweep this frame (mode 0: live ones, mode 1: warming ones);
+// links and conductances do not change during the sweeps, so each brick update becomes a gather of six potentials
+__global__ void __launch_bounds__(NT) k_stencil(LD D, int mode) {
+    const Tables& T = tables();
+    for (uint32_t g = blockIdx.y; g < D.nst; g += gridDim.y) {
+        if (D.st_nmin[g] <= 0) continue;
+        if (mode == 0 && !D.st_act[g]) continue;
+        int sc = D.st_scene[g];
+        int sl = (int)g - D.sc[sc].st0;
+        int4 rb = D.st_rb[g];
+        uint32_t o = rb.x, n = (uint32_t)(rb.y + rb.w);
+        for (uint32_t t = blockIdx.x * NT + threadIdx.x; t < n; t += gridDim.x * NT) {
+            uint32_t p = o + t, i = D.st_red[p];
+            int nbr[6];
+            float kk[6];
+            float sk = 0.0f;
+            bool ok = in_struct(D, i, sl);
+            for (int d = 0; d < 6; d++) {
+                int j = ok ? linked(D, i, d, nullptr) : -1;
+                nbr[d] = j;
+                kk[d] = 0.0f;
+                if (j < 0) continue;
+                float k = cond_w(D, i, (uint32_t)j, d >> 1);
+                kk[d] = k;
+                sk += k;
+            }
+            int mi = D.mat[i];
+            if (ok && D.bxyz[i].y == 0 && D.bond[i].w > 0.0f) sk += GROUNDK * T.stiff[mi];
+            D.sten_na[p] = make_int4(nbr[0], nbr[1], nbr[2], nbr[3]);
+            D.sten_nb[p] = make_int2(nbr[4], nbr[5]);
+            D.sten_ka[p] = make_float4(kk[0], kk[1], kk[2], kk[3]);
+            D.sten_kb[p] = make_float2(kk[4], kk[5]);
+            D.sten_s[p] = make_float2(ok ? sk : -1.0f, ok ? T.mass[mi] + D.ext[i] : 0.0f);
+        }
+    }
+}
+// one brick's update from its stencil, in sor_brick's operation order (src/lib.rs:1005-1026)
+__device__ __forceinline__ float sten_update(LD& D, uint32_t p, float om, float& phi_out, bool& did) {
+    float2 sr = D.sten_s[p];
+    did = false;
+    if (sr.x < 0.0f) return 0.0f;
+    did = true;
+    int4 na = D.sten_na[p];
+    int2 nb = D.sten_nb[p];
+    float4 ka = D.sten_ka[p];
+    float2 kb = D.sten_kb[p];
+    uint32_t i = D.st_red[p];
+    int nbr[6] = {na.x, na.y, na.z, na.w, nb.x, nb.y};
+    float kk[6] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y};
+    float ph[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) ph[d] = nbr[d] >= 0 ? D.phi[nbr[d]] : 0.0f;
+    float skp = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 6; d++) if (nbr[d] >= 0) skp += kk[d] * ph[d];
+    float pi = D.phi[i];
+    phi_out = pi;
+    if (sr.x > 0.0f) {
+        float target = (sr.y + skp) / sr.x;
+        float d = om * (target - pi);
+        phi_out = pi + d;
+        D.phi[i] = phi_out;
+        return fabsf(d);
+    }
+    return 0.0f;
+}
 __device__ float sweep_struct(LD& D, uint32_t g, int sl, float om, uint32_t frame, float* sh) {
     int4 rb = D.st_rb[g];
     float md = 0.0f, mp = 1.0f;
     for (int par = 0; par < 2; par++) {
         uint32_t o = par == 0 ? rb.x : rb.z, n = par == 0 ? rb.y : rb.w;
         for (uint32_t t = threadIdx.x; t < n; t += ST_NT) {
-            uint32_t i = D.st_red[o + t];
-            if (!in_struct(D, i, sl)) continue;
-            float d = sor_brick(D, i, om, frame);
+            float ph;
+            bool did;
+            float d = sten_update(D, o + t, om, ph, did);
+            if (!did) continue;
             md = fmaxf(md, d);
-            mp = fmaxf(mp, D.phi[i]);
+            mp = fmaxf(mp, ph);
         }
         __syncthreads();
     }
@@ -1221,6 +1288,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.st_scene, NS); r |= L->dalloc(D.st_lo, NS); r |= L->dalloc(D.st_hi, NS); r |= L->dalloc(D.st_act, NS);
     r |= L->dalloc(D.st_on, NS); r |= L->dalloc(D.st_live, NS); r |= L->dalloc(D.st_nmin, NS); r |= L->dalloc(D.st_resid, NS);
     r |= L->dalloc(D.st_maxr, NS); r |= L->dalloc(D.st_flags, NS); r |= L->dalloc(D.st_red, B); r |= L->dalloc(D.st_rb, NS);
+    r |= L->dalloc(D.sten_na, B); r |= L->dalloc(D.sten_nb, B); r |= L->dalloc(D.sten_ka, B); r |= L->dalloc(D.sten_kb, B); r |= L->dalloc(D.sten_s, B);
     r |= L->dalloc(D.sc_c0, S + 1); r |= L->dalloc(D.sc_gmax, S); r |= L->dalloc(D.sc_fail, 4 * S);
     r |= L->dalloc(D.dslot, B); r |= L->dalloc(D.dkey, B); r |= L->dalloc(D.hent, B);
     r |= L->dalloc(D.ccnt, B); r |= L->dalloc(D.coff, B);
@@ -2156,7 +2224,10 @@ int structural(Lat* L, float dt) {
     LAUNCH(k_rest_sum, gbr, NT, D);
     LAUNCH(k_ext_mix, gbr, NT, D);
     LAUNCH(k_unsupported, gw, NT, D);
-    if (ns) LAUNCH(k_sor_scene, std::min<size_t>(L->sc.size(), 2048), ST_NT, D, 0);
+    if (ns) {
+        hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(ns, 4096)), dim3(NT), 0, L->st, D, 0);
+        LAUNCH(k_sor_scene, std::min<size_t>(L->sc.size(), 2048), ST_NT, D, 0);
+    }
     LAUNCH(k_ga, gw, NT, D, dt);
     LAUNCH(k_body_sor_prep, gb, NT, D);
     LAUNCH(k_sor_body, std::min<int>(std::max(L->nbd, 1), 8192), BD_NT, D);
@@ -2299,6 +2370,7 @@ int frame(Lat* L, float dt) {
     for (const HStruct& T : L->sts) if (T.warm > 0) { any_warm = true; break; }
     if (any_warm) {
         if (structures_upload(L, true)) return -1;
+        hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(L->sts.size(), 4096)), dim3(NT), 0, L->st, D, 1);
         LAUNCH(k_sor_scene, std::min<size_t>(L->sc.size(), 2048), ST_NT, D, 1);
         std::vector<float> resid(L->sts.size());
         std::vector<int32_t> left(L->sts.size());

@@ -77,6 +77,13 @@ struct LD {
     uint32_t* st_flags;        // bit0 conn, bit1 broke
     uint32_t* st_red;          // per structure: red bricks then black bricks (offsets in st_rb)
     int4* st_rb;               // red offset, red count, black offset, black count
+    // per position of st_red: the brick's linked neighbours and conductances in link order, their sum with the
+    // footing term and the source M + ext (sk < 0: not part of the structure any more)
+    int4* sten_na;             // neighbours of directions 0-3 (-1: not linked)
+    int2* sten_nb;             // directions 4-5
+    float4* sten_ka;
+    float2* sten_kb;
+    float2* sten_s;            // sk, rhs
     uint32_t* sc_c0;           // first contact of each scene this substep
     uint32_t* sc_gmax;         // f32 bits per scene
     uint32_t* sc_fail;         // per scene: joints failed in tension, shear, at the footing (4 slots, [0] unused)
