@@ -153,6 +153,81 @@ def bench_whatif(n_queries=64, rate=200.0, frames=300, slots=64):
     return res
 
 
+def bench_load_batch(n_struct=1024, seed=4):
+    """load-path precision + batching: n_struct seeded frame buildings of 2k-40k bricks (log-uniform, 1-8 ground
+    columns removed), FP32 sweeps with FP64 residual and refinement to max|b - A phi| / max|b| <= 1e-6, all
+    structures in one persistent launch"""
+    from shatterforge_amd import serve
+    from shatterforge_amd.load import LoadBatch
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    L = LoadBatch()
+    nbr = 0
+    for k in range(n_struct):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(40000))))
+        mat, xyz, infill = serve.building_near(n, int(rng.integers(1, 1 << 30)))
+        L.add(mat, xyz, infill)
+        nbr += len(mat)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    conv = L.solve(rel_tol=1e-6)
+    t_solve = time.perf_counter() - t0
+    tm = L.timing()
+    ms, sw32, sw64 = tm["ms"], tm["brick_sweeps32"], tm["brick_sweeps64"]
+    res = np.array([L.info(k)["resid"] for k in range(n_struct)])
+    L.close()
+    # algorithmic bytes: 16 per brick per FP32 sweep (potential read and written, source, packed bonds) and 24 per
+    # brick per FP64 residual/refinement pass
+    alg = 16.0 * sw32 + 24.0 * sw64
+    out = {"structures": n_struct, "bricks": nbr, "converged": int(conv), "max_resid64": float(res.max()),
+           "device_ms": ms, "wall_solve_s": t_solve, "host_topology_s": t_build,
+           "brick_sweeps32": int(sw32), "brick_passes64": int(sw64),
+           "achieved_GBs": alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0}
+    out["roofline_frac"] = out["achieved_GBs"] / HBM_PEAK_GBS
+    log(f"load batch: {n_struct} structures, {nbr} bricks, {ms:.1f} ms device, {conv} converged, max resid {res.max():.2e}")
+    return out
+
+
+def bench_site(frames=1200, seed=1):
+    """site-scale collapse: 37 structures (~380k bricks) in one scene, load fields from the converged solve, eight
+    charge sets of four 2.2 m charges at column bases, `frames` frames"""
+    from shatterforge_amd import scenes
+    from shatterforge_amd.lattice import Lattice
+    from shatterforge_amd.load import LoadBatch
+    st = scenes.site(seed)
+    nbr = sum(len(m) for m, _, _ in st)
+    L = Lattice(nbr + 1024, 32768)
+    s = L.add_scene(nbr + 1024, 32768, structs=len(st))
+    for mat, xyz, infill in st:
+        L.add_structure(s, mat, xyz, infill)
+    L.finalize(s)
+    lb = LoadBatch()
+    for mat, xyz, infill in st:
+        lb.add(mat, xyz, infill)
+    lb.solve(rel_tol=1e-6)
+    for k in range(len(st)):
+        L.set_field(s, k, lb.field(k).astype(np.float32))
+    lb.close()
+    for ch in scenes.site_charges(st, seed):
+        for c in ch:
+            L.explode(s, *c)
+    t0 = time.perf_counter()
+    for f in range(frames):
+        L.step(1.0 / 60.0)
+        if f % 200 == 0:
+            stt = L.stats(s)
+            log(f"site frame {f}: {int(stt[1])} bricks in {int(stt[2])} pieces, {int(stt[4])} contacts")
+    el = time.perf_counter() - t0
+    stt = L.stats(s)
+    ms = L.matstats(s).reshape(10, 5)[1:6]
+    out = {"structures": len(st), "bricks": nbr, "frames": frames, "ms_per_frame": el * 1e3 / frames,
+           "brick_steps_per_s": nbr * frames / el, "standing": int(stt[0]), "pieces": int(stt[2]),
+           "failed_joints": L.failures(s).tolist(), "fragments": ms.tolist()}
+    L.close()
+    log(f"site: {el * 1e3 / frames:.2f} ms/frame")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +241,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
     ap.add_argument("--whatif-queries", type=int, default=64)
+    ap.add_argument("--load-batch", type=int, default=0, help="also run the load-path batch of this many structures")
+    ap.add_argument("--site", type=int, default=0, help="also run the site-scale collapse for this many frames")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard by scene, see DESIGN.md)",
@@ -247,6 +324,10 @@ def main():
     if not args.no_lattice:
         out["lattice"] = bench_lattice()
         out["whatif"] = bench_whatif(n_queries=args.whatif_queries)
+    if args.load_batch:
+        out["load_batch"] = bench_load_batch(args.load_batch)
+    if args.site:
+        out["site"] = bench_site(args.site)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(bodies, shape_body, shapes, prm, args.cpu_seconds)
