@@ -19,6 +19,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_solve_bucket launch on this workload, from FETCH_SIZE / WRITE_SIZE counter runs calibrated by a copy
+# of known size (profiles/k_solve_bucket_rubble50k_traffic.md)
+SOLVE_TRAFFIC_BYTES = 25_834_496
 # algorithmic bytes per manifold solve in one colour pass (box path): the hot contact record read, impulses written,
 # both bodies' v and w read and written
 SOLVE_BYTES_PER_MANIFOLD = 392
@@ -228,40 +231,12 @@ def bench_site(frames=1200, seed=1):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--bodies", type=int, default=50000)
-    ap.add_argument("--substeps", type=int, default=12)
-    ap.add_argument("--colours", type=int, default=None, help="colour cap (default: the library's)")
-    ap.add_argument("--tail-min", type=int, default=None)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
-    ap.add_argument("--whatif-queries", type=int, default=64)
-    ap.add_argument("--load-batch", type=int, default=0, help="also run the load-path batch of this many structures")
-    ap.add_argument("--site", type=int, default=0, help="also run the site-scale collapse for this many frames")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard by scene, see DESIGN.md)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def bench_box(args, sync):
+    """the plain box step on the rubble pile: the JSON line's value, roofline line and stages"""
     import shatterforge_amd as sf
     from shatterforge_amd import scenes
     from shatterforge_amd import world
     from shatterforge_amd.world import params
-
-    if sf.lib().sf_device_count() < 1:
-        raise SystemExit("no GPU visible")
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
-
     t = time.perf_counter()
     bodies, shape_body, shapes = scenes.rubble(args.bodies, seed=1)
     nbricks = len(shapes)
@@ -315,22 +290,89 @@ def main():
                    "substeps": args.substeps, "dt": "1/60", "mode": "plain",
                    "colour_limit": int(prm[world.P["COL_LIMIT"]]), "tail_min": int(prm[world.P["TAIL_MIN"]])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": SOLVE_TRAFFIC_BYTES,
                      "kernel": "k_solve_bucket", "launches": klaunch, "avg_us": ksum_ms / max(klaunch, 1) * 1e3,
                      "bytes_per_unit": SOLVE_BYTES_PER_MANIFOLD, "units": kitems},
         "stages_ms": stages,
         "stats": st,
     }
-    if not args.no_lattice:
+    return out, (bodies, shape_body, shapes, prm)
+
+
+def bench_copy(gib=1.0, reps=5):
+    """a device-to-device copy of known size (the library's copy probe): the reference reading for the HBM byte
+    counters (FETCH_SIZE and WRITE_SIZE per byte moved)"""
+    from shatterforge_amd._native import lib, check
+    nbytes = int(gib * (1 << 30))
+    ms = np.zeros(1, np.float32)
+    check(lib().sf_probe_copy(nbytes, reps, ms.ctypes.data))
+    res = {"kernel": "k_copy_probe", "bytes_per_copy": nbytes, "copies": reps,
+           "GBs_read_plus_write": 2.0 * nbytes * reps / (float(ms[0]) * 1e-3) / 1e9}
+    log(f"copy probe: {res['GBs_read_plus_write']:.0f} GB/s (read + write)")
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--bodies", type=int, default=50000)
+    ap.add_argument("--substeps", type=int, default=12)
+    ap.add_argument("--colours", type=int, default=None, help="colour cap (default: the library's)")
+    ap.add_argument("--tail-min", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
+    ap.add_argument("--parts", default="box,lattice,whatif",
+                    help="comma list of box, lattice, whatif, load, site, copy (a device copy of known size, the "
+                         "reference for HBM byte counters)")
+    ap.add_argument("--whatif-queries", type=int, default=64)
+    ap.add_argument("--load-batch", type=int, default=0, help="also run the load-path batch of this many structures")
+    ap.add_argument("--site", type=int, default=0, help="also run the site-scale collapse for this many frames")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard by scene, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import shatterforge_amd as sf
+
+    if sf.lib().sf_device_count() < 1:
+        raise SystemExit("no GPU visible")
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+
+    parts = set(args.parts.split(","))
+    if args.no_lattice:
+        parts -= {"lattice", "whatif"}
+    out = {"metric": "brick-steps/s (batched vs plain GPU step) + p50/p99 what-if query latency", "value": None,
+           "unit": "brick-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded scenes)", "config": {}}
+    scene = None
+    if "box" in parts:
+        out, scene = bench_box(args, sync)
+    if "copy" in parts:
+        out["copy"] = bench_copy()
+    if "lattice" in parts:
         out["lattice"] = bench_lattice()
+    if "whatif" in parts:
         out["whatif"] = bench_whatif(n_queries=args.whatif_queries)
+    if "load" in parts and not args.load_batch:
+        args.load_batch = 1024
+    if "site" in parts and not args.site:
+        args.site = 1200
     if args.load_batch:
         out["load_batch"] = bench_load_batch(args.load_batch)
     if args.site:
         out["site"] = bench_site(args.site)
-    if not args.no_cpu:
+    if not args.no_cpu and scene is not None:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(bodies, shape_body, shapes, prm, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(*scene, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -214,6 +214,12 @@ int sf_lat_timing(sf_lat* L, double* ms_frame, int32_t* syncs);
  * connectivity), structural, warm-up, statistics, all */
 int sf_lat_phase_ms(sf_lat* L, double* out8);
 
+
+/* ---------------------------------------------------------------- measurement
+ * a device copy of `bytes` (rounded down to 16), `reps` times; ms_out: device time of all copies. The reference
+ * reading for HBM byte counters (no counterpart in the reference). */
+int sf_probe_copy(int64_t bytes, int reps, float* ms_out);
+
 #ifdef __cplusplus
 }
 #endif
