@@ -213,6 +213,10 @@ int sf_lat_timing(sf_lat* L, double* ms_frame, int32_t* syncs);
 /* wall ms of the last frame by phase: substep launches and syncs, plough, impact, topology (rebuild and
  * connectivity), structural, warm-up, statistics, all */
 int sf_lat_phase_ms(sf_lat* L, double* out8);
+/* the contact solve of a substep runs as one workgroup per scene when no scene has more than max_contacts contacts
+ * (default 32768), else as colour-by-colour launches over all scenes; 0: always the latter. Both give the same
+ * bits (no counterpart in the reference: the solve route is a device choice) */
+int sf_lat_set_fused(sf_lat* L, int max_contacts);
 
 
 /* ---------------------------------------------------------------- measurement

@@ -451,68 +451,73 @@ __device__ __forceinline__ void apply_imp(LD& D, int a, int b, LV ra, LV rb, LV 
         D.w[b] = f4(sub(lv4(D.w[b]), mv(load_m(D.iiw + 3 * (size_t)b), cross(rb, p))));
     }
 }
+// one contact of a colour bucket, Gauss-Seidel (solve, src/lib.rs:1579-1601)
+__device__ __forceinline__ void solve_contact(LD& D, uint32_t c) {
+    int a = D.ca[c], b = D.cb[c];
+    LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
+    LV dv = sub(vel_at(D, a, ra), vel_at(D, b, rb));
+    float vn = dot(dv, nn);
+    float lam = D.cmn[c] * (-vn + D.cbias[c]);
+    float old = D.cjn[c];
+    float jn = maxf(old + lam, 0.0f);
+    float d = jn - old;
+    apply_imp(D, a, b, ra, rb, mul(nn, d));
+    float lim = MU * jn;
+    dv = sub(vel_at(D, a, ra), vel_at(D, b, rb));
+    float l1 = -D.cmt1[c] * dot(dv, t1);
+    float o1 = D.cjt1[c], j1 = clampf(o1 + l1, -lim, lim);
+    float l2 = -D.cmt2[c] * dot(dv, t2);
+    float o2 = D.cjt2[c], j2 = clampf(o2 + l2, -lim, lim);
+    LV p = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
+    apply_imp(D, a, b, ra, rb, p);
+    D.cjn[c] = jn; D.cjt1[c] = j1; D.cjt2[c] = j2;
+}
 __global__ void __launch_bounds__(NT) k_solve_colour(LD D, int col) {
     uint32_t n = D.ctr[C_COLCNT + col], o = D.coloff[col];
-    GS(t, n) {
-        uint32_t c = D.collist[o + t];
-        int a = D.ca[c], b = D.cb[c];
-        LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
-        LV dv = sub(vel_at(D, a, ra), vel_at(D, b, rb));
-        float vn = dot(dv, nn);
-        float lam = D.cmn[c] * (-vn + D.cbias[c]);
-        float old = D.cjn[c];
-        float jn = maxf(old + lam, 0.0f);
-        float d = jn - old;
-        apply_imp(D, a, b, ra, rb, mul(nn, d));
-        float lim = MU * jn;
-        dv = sub(vel_at(D, a, ra), vel_at(D, b, rb));
-        float l1 = -D.cmt1[c] * dot(dv, t1);
-        float o1 = D.cjt1[c], j1 = clampf(o1 + l1, -lim, lim);
-        float l2 = -D.cmt2[c] * dot(dv, t2);
-        float o2 = D.cjt2[c], j2 = clampf(o2 + l2, -lim, lim);
-        LV p = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
-        apply_imp(D, a, b, ra, rb, p);
-        D.cjn[c] = jn; D.cjt1[c] = j1; D.cjt2[c] = j2;
+    GS(t, n) solve_contact(D, D.collist[o + t]);
+}
+// one overflow contact, Jacobi: private copies of its bodies with inverse masses scaled by their overflow counts
+// na, nb; the velocity changes go to odv/odw for the per-body sums
+__device__ __forceinline__ void ovf_contact(LD& D, uint32_t c, float na, float nb) {
+    int a = D.ca[c], b = D.cb[c];
+    LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
+    LV va = lv4(D.v[a]), wa = lv4(D.w[a]), vb = lv(0.0f, 0.0f, 0.0f), wb = lv(0.0f, 0.0f, 0.0f);
+    if (b >= 0) { vb = lv4(D.v[b]); wb = lv4(D.w[b]); }
+    LM iia = load_m(D.iiw + 3 * (size_t)a);
+    LM iib;
+    if (b >= 0) iib = load_m(D.iiw + 3 * (size_t)b);
+    float ima = D.im[a] * na, imb = b >= 0 ? D.im[b] * nb : 0.0f;
+    LV dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+    float vn = dot(dv, nn);
+    float lam = D.csmn[c] * (-vn + D.cbias[c]);
+    float old = D.cjn[c];
+    float jn = maxf(old + lam, 0.0f);
+    LV pn = mul(nn, jn - old);
+    va = madd(va, pn, ima);
+    wa = add(wa, mul(mv(iia, cross(ra, pn)), na));
+    if (b >= 0) { vb = madd(vb, pn, -imb); wb = sub(wb, mul(mv(iib, cross(rb, pn)), nb)); }
+    float lim = MU * jn;
+    dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+    float l1 = -D.csmt1[c] * dot(dv, t1);
+    float o1 = D.cjt1[c], j1 = clampf(o1 + l1, -lim, lim);
+    float l2 = -D.csmt2[c] * dot(dv, t2);
+    float o2 = D.cjt2[c], j2 = clampf(o2 + l2, -lim, lim);
+    LV pt = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
+    LV p = add(pn, pt);
+    D.cjn[c] = jn; D.cjt1[c] = j1; D.cjt2[c] = j2;
+    D.odv[2 * c] = f4(mul(p, D.im[a]));
+    D.odw[2 * c] = f4(mv(iia, cross(ra, p)));
+    if (b >= 0) {
+        D.odv[2 * c + 1] = f4(mul(p, -D.im[b]));
+        D.odw[2 * c + 1] = f4(neg(mv(iib, cross(rb, p))));
     }
 }
-// overflow contacts, Jacobi: each works on private copies of its bodies with split inverse masses
 __global__ void __launch_bounds__(NT) k_ovf_solve(LD D) {
     uint32_t n = D.ctr[C_COLCNT + D.K], o = D.coloff[D.K];
     GS(t, n) {
         uint32_t c = D.collist[o + t];
         int a = D.ca[c], b = D.cb[c];
-        float na = (float)D.ovcnt[a], nb = b >= 0 ? (float)D.ovcnt[b] : 0.0f;
-        LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
-        LV va = lv4(D.v[a]), wa = lv4(D.w[a]), vb = lv(0.0f, 0.0f, 0.0f), wb = lv(0.0f, 0.0f, 0.0f);
-        if (b >= 0) { vb = lv4(D.v[b]); wb = lv4(D.w[b]); }
-        LM iia = load_m(D.iiw + 3 * (size_t)a);
-        LM iib;
-        if (b >= 0) iib = load_m(D.iiw + 3 * (size_t)b);
-        float ima = D.im[a] * na, imb = b >= 0 ? D.im[b] * nb : 0.0f;
-        LV dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
-        float vn = dot(dv, nn);
-        float lam = D.csmn[c] * (-vn + D.cbias[c]);
-        float old = D.cjn[c];
-        float jn = maxf(old + lam, 0.0f);
-        LV pn = mul(nn, jn - old);
-        va = madd(va, pn, ima);
-        wa = add(wa, mul(mv(iia, cross(ra, pn)), na));
-        if (b >= 0) { vb = madd(vb, pn, -imb); wb = sub(wb, mul(mv(iib, cross(rb, pn)), nb)); }
-        float lim = MU * jn;
-        dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
-        float l1 = -D.csmt1[c] * dot(dv, t1);
-        float o1 = D.cjt1[c], j1 = clampf(o1 + l1, -lim, lim);
-        float l2 = -D.csmt2[c] * dot(dv, t2);
-        float o2 = D.cjt2[c], j2 = clampf(o2 + l2, -lim, lim);
-        LV pt = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
-        LV p = add(pn, pt);
-        D.cjn[c] = jn; D.cjt1[c] = j1; D.cjt2[c] = j2;
-        D.odv[2 * c] = f4(mul(p, D.im[a]));
-        D.odw[2 * c] = f4(mv(iia, cross(ra, p)));
-        if (b >= 0) {
-            D.odv[2 * c + 1] = f4(mul(p, -D.im[b]));
-            D.odw[2 * c + 1] = f4(neg(mv(iib, cross(rb, p))));
-        }
+        ovf_contact(D, c, (float)D.ovcnt[a], b >= 0 ? (float)D.ovcnt[b] : 0.0f);
     }
 }
 __device__ __forceinline__ LV wave_tree(LV x) {
@@ -522,21 +527,178 @@ __device__ __forceinline__ LV wave_tree(LV x) {
     }
     return lv(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64), __shfl(x.z, 0, 64));
 }
+// one body's overflow sum by one wave: its entries in contact order, 64 at a time as halving trees
+__device__ __forceinline__ void ovf_body(LD& D, uint32_t b, uint32_t n, int lane) {
+    uint32_t o = D.ovoff[b];
+    LV av = lv(0.0f, 0.0f, 0.0f), aw = lv(0.0f, 0.0f, 0.0f);
+    for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+        LV xv = lv(0.0f, 0.0f, 0.0f), xw = lv(0.0f, 0.0f, 0.0f);
+        if (e0 + lane < n) { uint32_t ent = D.ovent[o + e0 + lane]; xv = lv4(D.odv[ent]); xw = lv4(D.odw[ent]); }
+        av = add(av, wave_tree(xv));
+        aw = add(aw, wave_tree(xw));
+    }
+    if (lane == 0) { D.v[b] = f4(add(lv4(D.v[b]), av)); D.w[b] = f4(add(lv4(D.w[b]), aw)); }
+}
 __global__ void __launch_bounds__(NT) k_ovf_apply(LD D) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     for (uint32_t b = wave; b < D.nbd; b += nwaves) {
         uint32_t n = D.ovcnt[b];
-        if (!n) continue;
-        uint32_t o = D.ovoff[b];
-        LV av = lv(0.0f, 0.0f, 0.0f), aw = lv(0.0f, 0.0f, 0.0f);
-        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-            LV xv = lv(0.0f, 0.0f, 0.0f), xw = lv(0.0f, 0.0f, 0.0f);
-            if (e0 + lane < n) { uint32_t ent = D.ovent[o + e0 + lane]; xv = lv4(D.odv[ent]); xw = lv4(D.odw[ent]); }
-            av = add(av, wave_tree(xv));
-            aw = add(aw, wave_tree(xw));
+        if (n) ovf_body(D, b, n, lane);
+    }
+}
+
+// ---------------------------------------------------------------- fused per-scene solve
+// One workgroup per scene runs the whole contact solve of a substep: claim-round colouring, colour buckets, the
+// overflow lists and split masses, then the 10 iterations colour after colour with the overflow pass, separated by
+// workgroup barriers instead of kernel boundaries. The arithmetic and every order are those of the kernels above
+// (k_col_claim ... k_ovf_apply), so both routes give the same bits; this one needs one launch instead of about 150.
+// Locations updated by atomics (claims, masks, overflow counts) are read with device-scope loads.
+constexpr int FS_NT = 1024;
+__device__ __forceinline__ uint32_t ld_dev(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ unsigned long long ld_dev(const unsigned long long* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// exclusive scan of one value per thread over the workgroup (wsum: 17 words of LDS); every thread must call it
+__device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int d = 1; d < 64; d <<= 1) { uint32_t y = __shfl_up(x, d, 64); if (lane >= d) x += y; }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) { uint32_t o = 0; for (int k = 0; k < FS_NT / 64; k++) { uint32_t t = wsum[k]; wsum[k] = o; o += t; } wsum[FS_NT / 64] = o; }
+    __syncthreads();
+    *total = wsum[FS_NT / 64];
+    return wsum[wv] + x - v;
+}
+__global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list) {
+    __shared__ uint32_t cnt[KMAX + 1], off[KMAX + 2], fil[KMAX + 1], wsum[FS_NT / 64 + 1], nov;
+    const int s = list[blockIdx.x];
+    const SceneK S = D.sc[s];
+    const uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    const uint32_t c0 = min(D.sc_c0[s], ncs);
+    const uint32_t c1 = s + 1 < (int)D.nscene ? min(D.sc_c0[s + 1], ncs) : ncs;
+    if (c1 <= c0) return;
+    const uint32_t tid = threadIdx.x, K = D.K;
+    const uint32_t full = K >= 32 ? 0xFFFFFFFFu : ((1u << K) - 1u);
+    const uint32_t bd0 = (uint32_t)S.bd0, hw = (uint32_t)S.hw;
+    // clear the scene's bodies
+    for (uint32_t lb = tid; lb < hw; lb += FS_NT) { uint32_t b = bd0 + lb; D.mask[b] = 0u; D.ovcnt[b] = 0u; D.ovfill[b] = 0u; }
+    for (uint32_t x = tid; x < hw * K; x += FS_NT) D.claim[(size_t)bd0 * K + x] = 0ull;
+    if (tid <= K) { cnt[tid] = 0u; fil[tid] = 0u; }
+    if (tid == 0) nov = 0u;
+    __syncthreads();
+    // colouring: claim rounds (k_col_claim, k_col_resolve)
+    for (int r = 0; r < (int)D.R; r++) {
+        for (uint32_t c = c0 + tid; c < c1; c += FS_NT) {
+            int prop = -1;
+            if (D.con[c] && D.ccol[c] < 0) {
+                int a = D.ca[c], b = D.cb[c];
+                uint32_t used = ld_dev(&D.mask[a]) | (b >= 0 ? ld_dev(&D.mask[b]) : 0u);
+                uint32_t fr = ~used & full;
+                if (fr) {
+                    int col = __ffs(fr) - 1;
+                    prop = col;
+                    unsigned long long pr = claim_prio(c - c0, r);
+                    atomicMax(&D.claim[(size_t)a * K + col], pr);
+                    if (b >= 0) atomicMax(&D.claim[(size_t)b * K + col], pr);
+                }
+            }
+            D.cprop[c] = prop;
         }
-        if (lane == 0) { D.v[b] = f4(add(lv4(D.v[b]), av)); D.w[b] = f4(add(lv4(D.w[b]), aw)); }
+        __syncthreads();
+        for (uint32_t c = c0 + tid; c < c1; c += FS_NT) {
+            int col = D.cprop[c];
+            if (col < 0) continue;
+            int a = D.ca[c], b = D.cb[c];
+            unsigned long long pr = claim_prio(c - c0, r);
+            if (ld_dev(&D.claim[(size_t)a * K + col]) != pr) continue;
+            if (b >= 0 && ld_dev(&D.claim[(size_t)b * K + col]) != pr) continue;
+            D.ccol[c] = col;
+            atomicOr(&D.mask[a], 1u << col);
+            if (b >= 0) atomicOr(&D.mask[b], 1u << col);
+        }
+        __syncthreads();
+    }
+    // colour buckets in the scene's own stretch of collist, overflow last
+    for (uint32_t c = c0 + tid; c < c1; c += FS_NT) if (D.con[c]) atomicAdd(&cnt[D.ccol[c] < 0 ? K : (uint32_t)D.ccol[c]], 1u);
+    __syncthreads();
+    if (tid == 0) { uint32_t o = 0; for (uint32_t t = 0; t <= K; t++) { off[t] = o; o += cnt[t]; } off[K + 1] = o; }
+    __syncthreads();
+    for (uint32_t c = c0 + tid; c < c1; c += FS_NT) {
+        if (!D.con[c]) continue;
+        uint32_t bin = D.ccol[c] < 0 ? K : (uint32_t)D.ccol[c];
+        D.collist[c0 + off[bin] + atomicAdd(&fil[bin], 1u)] = c;
+    }
+    __syncthreads();
+    // overflow lists: per body, its overflow contact sides in contact order (k_ovf_count ... k_ovf_prep)
+    const uint32_t no = cnt[K], oo = c0 + off[K];
+    if (no) {
+        for (uint32_t t = tid; t < no; t += FS_NT) {
+            uint32_t c = D.collist[oo + t];
+            atomicAdd(&D.ovcnt[D.ca[c]], 1u);
+            if (D.cb[c] >= 0) atomicAdd(&D.ovcnt[D.cb[c]], 1u);
+        }
+        __syncthreads();
+        const uint32_t chunk = (hw + FS_NT - 1) / FS_NT, lo = min(tid * chunk, hw), hi = min(lo + chunk, hw);
+        uint32_t sum = 0, nb = 0;
+        for (uint32_t lb = lo; lb < hi; lb++) { uint32_t n = ld_dev(&D.ovcnt[bd0 + lb]); sum += n; nb += n ? 1u : 0u; }
+        uint32_t tot;
+        uint32_t base = wg_excl_scan(sum, wsum, &tot);
+        uint32_t bslot = nb ? atomicAdd(&nov, nb) : 0u;
+        for (uint32_t lb = lo, o = 2 * c0 + base; lb < hi; lb++) {
+            uint32_t n = ld_dev(&D.ovcnt[bd0 + lb]);
+            D.ovoff[bd0 + lb] = o;
+            o += n;
+            if (n) D.ovbody[bd0 + bslot++] = bd0 + lb;
+        }
+        __syncthreads();
+        for (uint32_t t = tid; t < no; t += FS_NT) {
+            uint32_t c = D.collist[oo + t];
+            int a = D.ca[c], b = D.cb[c];
+            D.ovent[D.ovoff[a] + atomicAdd(&D.ovfill[a], 1u)] = 2u * c;
+            if (b >= 0) D.ovent[D.ovoff[b] + atomicAdd(&D.ovfill[b], 1u)] = 2u * c + 1u;
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < nov; k += FS_NT) {
+            uint32_t b = D.ovbody[bd0 + k], o = D.ovoff[b], n = ld_dev(&D.ovcnt[b]);
+            for (uint32_t x = 1; x < n; x++) {
+                uint32_t v = D.ovent[o + x], y = x;
+                while (y > 0 && D.ovent[o + y - 1] > v) { D.ovent[o + y] = D.ovent[o + y - 1]; y--; }
+                D.ovent[o + y] = v;
+            }
+        }
+        for (uint32_t t = tid; t < no; t += FS_NT) {
+            uint32_t c = D.collist[oo + t];
+            int a = D.ca[c], b = D.cb[c];
+            float na = (float)ld_dev(&D.ovcnt[a]), nb2 = b >= 0 ? (float)ld_dev(&D.ovcnt[b]) : 0.0f;
+            LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]);
+            D.csmn[c] = eff_mass_split(D, a, na, b, nb2, ra, rb, lv4(D.cn[c]));
+            D.csmt1[c] = eff_mass_split(D, a, na, b, nb2, ra, rb, lv4(D.ct1[c]));
+            D.csmt2[c] = eff_mass_split(D, a, na, b, nb2, ra, rb, lv4(D.ct2[c]));
+        }
+        __syncthreads();
+    }
+    // 10 iterations: colour after colour, then the overflow pass
+    const int lane = tid & 63;
+    for (int it = 0; it < 10; it++) {
+        for (uint32_t col = 0; col < K; col++) {
+            const uint32_t n = cnt[col], o = c0 + off[col];
+            if (!n) continue;
+            for (uint32_t t = tid; t < n; t += FS_NT) solve_contact(D, D.collist[o + t]);
+            __syncthreads();
+        }
+        if (no) {
+            for (uint32_t t = tid; t < no; t += FS_NT) {
+                uint32_t c = D.collist[oo + t];
+                int a = D.ca[c], b = D.cb[c];
+                ovf_contact(D, c, (float)ld_dev(&D.ovcnt[a]), b >= 0 ? (float)ld_dev(&D.ovcnt[b]) : 0.0f);
+            }
+            __syncthreads();
+            for (uint32_t k = tid >> 6; k < nov; k += FS_NT / 64) {
+                uint32_t b = D.ovbody[bd0 + k];
+                ovf_body(D, b, ld_dev(&D.ovcnt[b]), lane);
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -1200,6 +1362,8 @@ struct Lat {
     int dev = 0;
     hipStream_t st = nullptr;
     int K = 16, R = 8;
+    int fuse_max = 32768;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
+    std::vector<int32_t> slist;
     int bcap = 0, dcap = 0, scap = 0;
     int nb = 0, nbd = 0;                   // bricks / body slots handed out to scenes
     std::vector<HScene> sc;
@@ -1233,6 +1397,7 @@ struct Lat {
     DBuf<CRec> d_rec;
     DBuf<int2> d_dis;
     DBuf<uint32_t> d_scstat;
+    DBuf<int32_t> d_slist;
     uint32_t h_ctr[C_NCTR];
     uint32_t ccap = 0, tsize = 0;
     // bricks whose material, body or bonds the host changed since the last flush
@@ -1252,7 +1417,7 @@ struct Lat {
 
     ~Lat() {
         for (void* p : owned) if (p) (void)hipFree(p);
-        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release();
+        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release();
         if (st) (void)hipStreamDestroy(st);
     }
     template <class T> int dalloc(T*& p, size_t n) {
@@ -1294,7 +1459,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.ccnt, B); r |= L->dalloc(D.coff, B);
     r |= L->dalloc(D.mask, N); r |= L->dalloc(D.claim, N * (size_t)L->K);
     r |= L->dalloc(D.colfill, KMAX + 1); r |= L->dalloc(D.coloff, KMAX + 1);
-    r |= L->dalloc(D.ovcnt, N); r |= L->dalloc(D.ovoff, N); r |= L->dalloc(D.ovfill, N);
+    r |= L->dalloc(D.ovcnt, N); r |= L->dalloc(D.ovoff, N); r |= L->dalloc(D.ovfill, N); r |= L->dalloc(D.ovbody, N);
     r |= L->dalloc(D.lcnt, B); r |= L->dalloc(D.loff, B); r |= L->dalloc(D.lfill, B); r |= L->dalloc(D.lent, B);
     r |= L->dalloc(D.ltarget, B); r |= L->dalloc(D.lshare, B); r |= L->dalloc(D.bshare, N);
     r |= L->dalloc(D.bd_sor, N); r |= L->dalloc(D.bd_judge, N); r |= L->dalloc(D.bd_live, N); r |= L->dalloc(D.bd_dt, N);
@@ -1305,7 +1470,7 @@ int lat_alloc(Lat* L) {
     D.evcap = (uint32_t)std::max<size_t>(4096, B); D.bwcap = (uint32_t)std::max<size_t>(4096, 4 * B);
     r |= L->dalloc(D.evkey, D.evcap); r |= L->dalloc(D.ev, D.evcap); r |= L->dalloc(D.bwi, D.bwcap); r |= L->dalloc(D.bwv, D.bwcap);
     r |= L->dalloc(D.scan_tmp, sf::scan_scratch_words((uint32_t)std::max<size_t>(std::max(2 * B + 4096, N), 1u << 16)));
-    r |= L->d_sc.alloc(S); r |= L->d_scstat.alloc(3 * S); r |= L->d_rec.alloc(D.pcap); r |= L->d_dis.alloc(D.pcap);
+    r |= L->d_sc.alloc(S); r |= L->d_scstat.alloc(3 * S); r |= L->d_slist.alloc(S); r |= L->d_rec.alloc(D.pcap); r |= L->d_dis.alloc(D.pcap);
     D.sc = L->d_sc.p;
     D.K = (uint32_t)L->K; D.R = (uint32_t)L->R;
     // the zero fills above ran on the null stream, which the world's non-blocking stream does not wait for
@@ -1822,7 +1987,7 @@ int upload_scenes(Lat* L) {
         SceneK& x = k[si];
         x.frame = S.frame; x.blast_t = S.blast_t;
         x.gx0 = S.gx0; x.gy0 = S.gy0; x.gz0 = S.gz0; x.gnx = S.gnx; x.gny = S.gny; x.gnz = S.gnz; x.goff = (uint32_t)S.goff;
-        x.bd0 = S.bd0; x.nbd = S.dcap; x.dyn0 = S.dyn0; x.ndyn = S.ndyn; x.st0 = S.st0; x.nst = S.nst; x.b0 = S.b0; x.nb = S.nb;
+        x.bd0 = S.bd0; x.nbd = S.dcap; x.dyn0 = S.dyn0; x.ndyn = S.ndyn; x.st0 = S.st0; x.nst = S.nst; x.b0 = S.b0; x.nb = S.nb; x.hw = S.hw;
     }
     SF_HIP(hipMemcpyAsync(L->d_sc.p, k.data(), sizeof(SceneK) * k.size(), hipMemcpyHostToDevice, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
@@ -2132,30 +2297,45 @@ int substep(Lat* L, float h) {
     const unsigned gc = sf_grid(std::max<uint32_t>(ncs, 1));
     LAUNCH(k_prep, gc, NT, D, h);
     LAUNCH(k_vpre, gb, NT, D);
-    // colour
-    LAUNCH(k_col_clear, sf_grid((uint64_t)L->nbd * L->K), NT, D);
-    for (int r = 0; r < L->R; r++) {
-        LAUNCH(k_col_claim, gc, NT, D, r);
-        LAUNCH(k_col_resolve, gc, NT, D, r);
+    // every scene small enough: one workgroup per scene solves it whole; else the colour-by-colour launches
+    bool fused = L->fuse_max > 0;
+    L->slist.clear();
+    for (size_t si = 0; si < L->sc.size(); si++) {
+        uint32_t n = L->sc[si].ncs;
+        if (n > (uint32_t)L->fuse_max) fused = false;
+        if (n) L->slist.push_back((int32_t)si);
     }
-    LAUNCH(k_col_count, gc, NT, D);
-    LAUNCH(k_col_scan, 1, 64, D);
-    LAUNCH(k_col_fill, gc, NT, D);
-    // overflow lists
-    fill_u32(L, D.ovcnt, 0, L->nbd);
-    fill_u32(L, D.ovfill, 0, L->nbd);
-    LAUNCH(k_ovf_count, gc, NT, D);
-    scan(L, D.ovcnt, D.ovoff, D.ctr + C_NCTR0, L->nbd);
-    LAUNCH(k_ovf_fill, gc, NT, D);
-    segsort(L, D.ovoff, D.ovcnt, D.ovent, L->nbd);
-    LAUNCH(k_ovf_prep, gc, NT, D);
-    if (read_ctr(L)) return -1;
-    const uint32_t* cc = L->h_ctr + C_COLCNT;
-    for (int it = 0; it < 10; it++) {
-        for (int c = 0; c < L->K; c++) if (cc[c]) LAUNCH(k_solve_colour, sf_grid(cc[c]), NT, D, c);
-        if (cc[L->K]) {
-            LAUNCH(k_ovf_solve, sf_grid(cc[L->K]), NT, D);
-            LAUNCH(k_ovf_apply, sf_grid((uint64_t)L->nbd * 64, NT, 65536), NT, D);
+    if (fused) {
+        if (!L->slist.empty()) {
+            SF_HIP(hipMemcpyAsync(L->d_slist.p, L->slist.data(), sizeof(int32_t) * L->slist.size(), hipMemcpyHostToDevice, L->st));
+            LAUNCH(k_scene_solve, (unsigned)L->slist.size(), FS_NT, D, (const int32_t*)L->d_slist.p);
+        }
+    } else {
+        // colour
+        LAUNCH(k_col_clear, sf_grid((uint64_t)L->nbd * L->K), NT, D);
+        for (int r = 0; r < L->R; r++) {
+            LAUNCH(k_col_claim, gc, NT, D, r);
+            LAUNCH(k_col_resolve, gc, NT, D, r);
+        }
+        LAUNCH(k_col_count, gc, NT, D);
+        LAUNCH(k_col_scan, 1, 64, D);
+        LAUNCH(k_col_fill, gc, NT, D);
+        // overflow lists
+        fill_u32(L, D.ovcnt, 0, L->nbd);
+        fill_u32(L, D.ovfill, 0, L->nbd);
+        LAUNCH(k_ovf_count, gc, NT, D);
+        scan(L, D.ovcnt, D.ovoff, D.ctr + C_NCTR0, L->nbd);
+        LAUNCH(k_ovf_fill, gc, NT, D);
+        segsort(L, D.ovoff, D.ovcnt, D.ovent, L->nbd);
+        LAUNCH(k_ovf_prep, gc, NT, D);
+        if (read_ctr(L)) return -1;
+        const uint32_t* cc = L->h_ctr + C_COLCNT;
+        for (int it = 0; it < 10; it++) {
+            for (int c = 0; c < L->K; c++) if (cc[c]) LAUNCH(k_solve_colour, sf_grid(cc[c]), NT, D, c);
+            if (cc[L->K]) {
+                LAUNCH(k_ovf_solve, sf_grid(cc[L->K]), NT, D);
+                LAUNCH(k_ovf_apply, sf_grid((uint64_t)L->nbd * 64, NT, 65536), NT, D);
+            }
         }
     }
     LAUNCH(k_pinch, gb, NT, D);
@@ -2886,6 +3066,11 @@ int sf_lat_timing(sf_lat* h, double* ms_frame, int32_t* syncs) {
     if (!h) return SF_E_ARG;
     if (ms_frame) *ms_frame = h->h.L.ms_frame;
     if (syncs) *syncs = h->h.L.syncs;
+    return 0;
+}
+int sf_lat_set_fused(sf_lat* h, int max_contacts) {
+    if (!h || max_contacts < 0) return SF_E_ARG;
+    h->h.L.fuse_max = max_contacts;
     return 0;
 }
 

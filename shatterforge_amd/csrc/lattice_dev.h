@@ -32,6 +32,7 @@ struct SceneK {
     uint32_t dyn0, ndyn;   // dynamic slot range
     int32_t st0, nst;      // structure range (global indices)
     int32_t b0, nb;        // brick range
+    int32_t hw;            // body slots in use (high-water mark)
 };
 
 struct LD {
@@ -103,6 +104,7 @@ struct LD {
     uint32_t *colfill, *coloff, *collist;
     // overflow lists
     uint32_t *ovcnt, *ovoff, *ovfill, *ovent;
+    uint32_t* ovbody;          // per scene: its bodies with overflow entries (at the scene's first body slot)
     float4 *odv, *odw;
     // structural scratch
     uint32_t *lcnt, *loff, *lfill, *lent;

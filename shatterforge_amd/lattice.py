@@ -132,6 +132,10 @@ class Lattice:
         check(lib().sf_lat_phase_ms(self._h, ptr(out)))
         return dict(zip(["substeps", "plough", "impact", "topology", "structural", "warmup", "stats", "frame"], out.tolist()))
 
+    def set_fused(self, max_contacts):
+        """scenes of at most max_contacts contacts are solved by one workgroup each; 0: colour-by-colour launches"""
+        check(lib().sf_lat_set_fused(self._h, int(max_contacts)))
+
     def timing(self):
         ms = np.zeros(1, np.float64)
         syncs = np.zeros(1, np.int32)

@@ -32,10 +32,12 @@ def _assert_same(G, s, O, tag):
     assert G.rng(s) == O.rng(), f"{tag}: random stream"
 
 
-def _run(oracle, case, frames, check_every=1):
+def _run(oracle, case, frames, check_every=1, fused=True):
     nb = lc.brick_count(case) + 8
     nd = 4096
     G = Lattice(nb, nd, scenes=1)
+    if not fused:
+        G.set_fused(0)
     s = G.add_scene(nb, nd, structs=4)
     O = oracle.LatticeOracle(order=oracle.ORDER_DEV, brick_cap=nb, body_cap=nd)
     lc.build(G, s, case)
@@ -62,9 +64,11 @@ def test_drop_bodies_bit_exact(oracle):
     assert st[2] > 10 and st[4] > 0
 
 
-def test_tower_blast_bit_exact(oracle):
-    print("case: 8x8x16 tower, 4 charges at the column bases, 120 frames", flush=True)
-    G, s, O = _run(oracle, dict(kind="tower", w=8, h=16, nch=4, r=2.2, p=1.4), 120)
+@pytest.mark.parametrize("route", ["workgroup per scene", "colour launches"])
+def test_tower_blast_bit_exact(oracle, route):
+    # the contact solve as one workgroup per scene and as colour-by-colour launches: both equal the oracle
+    print(f"case: 8x8x16 tower, 4 charges at the column bases, 120 frames, solve as {route}", flush=True)
+    G, s, O = _run(oracle, dict(kind="tower", w=8, h=16, nch=4, r=2.2, p=1.4), 120, fused=route.startswith("work"))
     assert O.stats()[2] > 0
 
 
@@ -80,7 +84,8 @@ MIX = [dict(kind="drop", seed=3, nbody=24), dict(kind="tower", w=8, h=16, nch=4,
 
 def test_batched_scenes_same_alone_and_in_mix(oracle):
     # four scenes of uneven size stepped together in one world: each equals the oracle run of that scene alone and
-    # the GPU run of that scene alone, bit for bit
+    # the GPU run of that scene alone, bit for bit (the batch solves one workgroup per scene, the lone worlds by
+    # colour launches)
     print("case: 4 scenes batched, 100 frames, vs each alone (GPU and oracle)", flush=True)
     caps = [lc.brick_count(c) + 8 for c in MIX]
     G = Lattice(sum(caps), 4096 * len(MIX), scenes=len(MIX))
@@ -88,6 +93,7 @@ def test_batched_scenes_same_alone_and_in_mix(oracle):
     alone = []
     for k, c in enumerate(MIX):
         A = Lattice(caps[k], 4096, scenes=1)
+        A.set_fused(0)
         a = A.add_scene(caps[k], 4096, structs=4)
         alone.append((A, a))
     Os = [oracle.LatticeOracle(order=oracle.ORDER_DEV, brick_cap=caps[k], body_cap=4096) for k in range(len(MIX))]
