@@ -213,6 +213,9 @@ int sf_lat_timing(sf_lat* L, double* ms_frame, int32_t* syncs);
 /* wall ms of the last frame by phase: substep launches and syncs, plough, impact, topology (rebuild and
  * connectivity), structural, warm-up, statistics, all */
 int sf_lat_phase_ms(sf_lat* L, double* out8);
+/* finer split of the last frame's host work, ms: topology (body pull, rebuilds, connectivity, delta flush, slot
+ * lists, scene upload, scenes visited), structural (upload, launches, read-back wait, event application, flush) */
+int sf_lat_subphase_ms(sf_lat* L, double* out16);
 /* the contact solve of a substep runs as one workgroup per scene when no scene has more than max_contacts contacts
  * (default 32768), else as colour-by-colour launches over all scenes; 0: always the latter. Both give the same
  * bits (no counterpart in the reference: the solve route is a device choice) */

@@ -36,12 +36,20 @@
 #include <cmath>
 #include <algorithm>
 #include <chrono>
+#include <atomic>
+#include <thread>
+#include <mutex>
+#include <condition_variable>
+#include <functional>
+#include <memory>
 
 using namespace lat;
 
 namespace {
 
 #define GS(i, n) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)(n); i += gridDim.x * blockDim.x)
+// launch on the world's stream (host code with a Lat* L in scope)
+#define LAUNCH(kern, grid, block, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, L->st, __VA_ARGS__)
 
 __device__ __forceinline__ LM load_m(const float4* p) {
     float4 a = p[0], b = p[1], c = p[2];
@@ -59,7 +67,8 @@ __device__ __forceinline__ void atomic_max_pos(uint32_t* p, float v) { if (v > 0
 
 // ---------------------------------------------------------------- cache
 __global__ void __launch_bounds__(NT) k_cache(LD D, int gravity, float h) {
-    GS(b, D.nbd) {
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
         if (!D.alive[b] || D.asleep[b]) continue;
         LM r = q_to_m(lq4(D.q[b]));
         store_m(D.rot + 3 * (size_t)b, r);
@@ -336,7 +345,10 @@ __global__ void __launch_bounds__(NT) k_prep(LD D, float h) {
     }
 }
 __global__ void __launch_bounds__(NT) k_vpre(LD D) {
-    GS(b, D.nbd) if (D.alive[b]) { D.vpre[b] = D.v[b]; D.wpre[b] = D.w[b]; }
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
+        if (D.alive[b]) { D.vpre[b] = D.v[b]; D.wpre[b] = D.w[b]; }
+    }
 }
 
 // ---------------------------------------------------------------- colouring
@@ -714,7 +726,8 @@ __device__ void kill_body_dev(LD& D, int b) {
     if (s < D.nbd) D.klist[s] = b;
 }
 __global__ void __launch_bounds__(NT) k_pinch(LD D) {
-    GS(b, D.nbd) {
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
         if (!D.alive[b] || D.asleep[b]) continue;
         LV w = lv4(D.w[b]), wpre = lv4(D.wpre[b]);
         LV dw = sub(w, wpre);
@@ -746,7 +759,8 @@ __global__ void __launch_bounds__(NT) k_impact_flag(LD D) {
     }
 }
 __global__ void __launch_bounds__(NT) k_integrate(LD D, float h) {
-    GS(b, D.nbd) {
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
         if (!D.alive[b] || D.asleep[b]) continue;
         LV v = lv4(D.v[b]), w = lv4(D.w[b]);
         float vl = len(v);
@@ -777,7 +791,10 @@ __global__ void __launch_bounds__(NT) k_integrate(LD D, float h) {
     }
 }
 __global__ void __launch_bounds__(NT) k_wakes(LD D) {
-    GS(b, D.nbd) if (D.wake[b]) { D.wake[b] = 0; if (D.alive[b]) { D.asleep[b] = 0; D.sleep[b] = 0.0f; } }
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
+        if (D.wake[b]) { D.wake[b] = 0; if (D.alive[b]) { D.asleep[b] = 0; D.sleep[b] = 0.0f; } }
+    }
 }
 
 // ---------------------------------------------------------------- load field
@@ -987,7 +1004,8 @@ __global__ void __launch_bounds__(NT) k_ext_mix(LD D) {
 __global__ void __launch_bounds__(NT) k_rest_count(LD D) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    for (uint32_t b = wave; b < D.nbd; b += nwaves) {
+    for (uint32_t ub = wave; ub < D.nub; ub += nwaves) {
+        const uint32_t b = D.ubody[ub];
         if (!D.alive[b]) continue;
         const SceneK S = D.sc[D.bd_scene[b]];
         uint32_t o = D.boff[b], n = D.bcnt[b];
@@ -1029,7 +1047,8 @@ __global__ void __launch_bounds__(NT) k_rest_sum(LD D) {
 __global__ void __launch_bounds__(NT) k_unsupported(LD D) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    for (uint32_t b = wave; b < D.nbd; b += nwaves) {
+    for (uint32_t ub = wave; ub < D.nub; ub += nwaves) {
+        const uint32_t b = D.ubody[ub];
         if (!D.alive[b] || D.count[b] < 8u) continue;
         uint32_t frame = D.sc[D.bd_scene[b]].frame;
         uint32_t o = D.boff[b], n = D.bcnt[b];
@@ -1048,7 +1067,8 @@ __global__ void __launch_bounds__(NT) k_ga(LD D, float dt) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     float idt = 1.0f / maxf(dt, 1e-3f);
-    for (uint32_t b = wave; b < D.nbd; b += nwaves) {
+    for (uint32_t ub = wave; ub < D.nub; ub += nwaves) {
+        const uint32_t b = D.ubody[ub];
         if (!D.alive[b]) continue;
         LV v = lv4(D.v[b]), w = lv4(D.w[b]);
         LV al = mul(sub(v, lv4(D.vlast[b])), idt);
@@ -1074,7 +1094,8 @@ __global__ void __launch_bounds__(NT) k_ga(LD D, float dt) {
 }
 // which pieces sweep their load field this frame
 __global__ void __launch_bounds__(NT) k_body_sor_prep(LD D) {
-    GS(b, D.nbd) {
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
         uint8_t go = 0;
         if (body_loaded(D, b) && D.count[b] <= 60000u) {
             const SceneK& S = D.sc[D.bd_scene[b]];
@@ -1087,7 +1108,8 @@ __global__ void __launch_bounds__(NT) k_body_sor_prep(LD D) {
 constexpr int BD_NT = 256;
 __global__ void __launch_bounds__(BD_NT) k_sor_body(LD D) {
     __shared__ float sh[2 * BD_NT / 64];
-    for (uint32_t b = blockIdx.x; b < D.nbd; b += gridDim.x) {
+    for (uint32_t ub = blockIdx.x; ub < D.nub; ub += gridDim.x) {
+        const uint32_t b = D.ubody[ub];
         if (!D.bd_sor[b]) continue;
         uint32_t frame = D.sc[D.bd_scene[b]].frame;
         uint32_t o = D.boff[b], n = D.bcnt[b];
@@ -1131,7 +1153,8 @@ __global__ void __launch_bounds__(NT) k_stress_clear(LD D) {
     }
 }
 __global__ void __launch_bounds__(NT) k_body_judge_prep(LD D, float dt) {
-    GS(b, D.nbd) {
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
         uint8_t judge = 0;
         if (body_loaded(D, b) && D.count[b] <= 60000u) {
             const SceneK& S = D.sc[D.bd_scene[b]];
@@ -1273,12 +1296,16 @@ __global__ void __launch_bounds__(NT) k_stress_body(LD D) {
     }
 }
 __global__ void __launch_bounds__(NT) k_lt_decay(LD D, float dt) {
-    GS(b, D.nbd) if (D.alive[b] && !body_loaded(D, b)) D.lt[b] = maxf(D.lt[b] - 2.0f * dt, 0.0f);
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
+        if (D.alive[b] && !body_loaded(D, b)) D.lt[b] = maxf(D.lt[b] - 2.0f * dt, 0.0f);
+    }
 }
 
 // per-scene counts: bodies, bricks in bodies, awake bodies
 __global__ void __launch_bounds__(NT) k_scene_stats(LD D, uint32_t* out) {
-    GS(b, D.nbd) {
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
         if (!D.alive[b]) continue;
         int s = D.bd_scene[b];
         atomicAdd(&out[3 * s], 1u);
@@ -1288,12 +1315,86 @@ __global__ void __launch_bounds__(NT) k_scene_stats(LD D, uint32_t* out) {
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) { GS(i, n) p[i] = v; }
+__global__ void __launch_bounds__(NT) k_csr_set(LD D, const int4* r, uint32_t n) { GS(t, n) { int4 x = r[t]; D.boff[x.x] = (uint32_t)x.y; D.bcnt[x.x] = (uint32_t)x.z; } }
+__global__ void __launch_bounds__(NT) k_nc_clear(LD D) { GS(ub, D.nub) D.nc[D.ubody[ub]] = 0u; }
+// the used slots' dirty flags, in list order, cleared behind
+__global__ void __launch_bounds__(NT) k_take_dirty(LD D) {
+    GS(ub, D.nub) { uint32_t b = D.ubody[ub]; D.bdflag[ub] = D.bd_dirty[b]; D.bd_dirty[b] = 0; }
+}
 __global__ void k_fill_u8(uint8_t* p, uint8_t v, size_t n) { GS(i, n) p[i] = v; }
 
 }  // namespace
 
 // ====================================================================== host
 namespace {
+
+// Host worker pool for per-scene bookkeeping: scenes share no host state but the per-brick and per-body arrays, which
+// are partitioned by scene, so their topology work runs on separate threads with identical results.
+class Pool {
+  public:
+    explicit Pool(int workers) {
+        for (int k = 0; k < workers; k++) th_.emplace_back([this, k] { loop(k); });
+    }
+    ~Pool() {
+        { std::lock_guard<std::mutex> g(m_); stop_ = true; }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    int workers() const { return (int)th_.size(); }
+    // f(i, slot) for i in [0, n): slot 0 is the calling thread, 1.. the workers; returns when all have finished
+    void run(int n, const std::function<void(int, int)>& f) {
+        if (n <= 0) return;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = &f; n_ = n; next_.store(0); busy_ = (int)th_.size(); gen_++;
+        }
+        cv_.notify_all();
+        work(0);
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [this] { return busy_ == 0; });
+        job_ = nullptr;
+    }
+  private:
+    void work(int slot) {
+        for (;;) {
+            int i = next_.fetch_add(1);
+            if (i >= n_) break;
+            (*job_)(i, slot);
+        }
+    }
+    void loop(int k) {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work(k + 1);
+            std::lock_guard<std::mutex> g(m_);
+            if (--busy_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int, int)>* job_ = nullptr;
+    std::atomic<int> next_{0};
+    int n_ = 0, busy_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// per-thread scratch of the topology searches and the thread's lists of touched bricks and bodies
+struct Scratch {
+    std::vector<uint32_t> q, lst, lst2, cnt, tlist, btlist;
+    void reserve(size_t bricks) {
+        if (q.size() >= bricks + 2) return;
+        q.assign(bricks + 2, 0); lst.assign(bricks + 2, 0); lst2.assign(bricks + 2, 0); cnt.assign(bricks + 3, 0);
+    }
+};
+thread_local Scratch* t_scr = nullptr;  // set while a pool worker (or the caller, as slot 0) runs a scene's work
 
 struct HScene {
     int b0 = 0, bcap = 0, nb = 0;          // bricks [b0, b0 + nb), capacity bcap
@@ -1358,6 +1459,28 @@ template <class T> struct DBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+struct BrickDelta { uint32_t i; int32_t body; uint32_t mat; uint32_t full; float4 bond, local, wpos; float phi, ext, extn, pad2; };
+// a body's full state, packed for one copy each way
+struct BodyRec {
+    uint32_t b; uint8_t alive, asleep, wake, pad0; uint32_t count, nc;
+    float4 pos, q, v, w, vlast, wlast;
+    float4 rot[3], iil[3], iiw[3];
+    float m, im, sleep, rad, cool, lt, resid, maxr, nmax, pad1, pad2, pad3;
+};
+// pinned host staging: copies to and from it are asynchronous DMA
+template <class T> struct HBuf {
+    T* p = nullptr; size_t n = 0;
+    int reserve(size_t cnt) {
+        if (cnt <= n) return 0;
+        release();
+        size_t c = std::max<size_t>(cnt, 4096);
+        if (hipHostMalloc((void**)&p, sizeof(T) * c, hipHostMallocDefault) != hipSuccess) { p = nullptr; return -1; }
+        n = c;
+        return 0;
+    }
+    void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }
+};
+
 struct Lat {
     int dev = 0;
     hipStream_t st = nullptr;
@@ -1385,10 +1508,16 @@ struct Lat {
     // scratch
     std::vector<uint32_t> lst, lst2, comp, cnt;
     // dynamic slots
-    std::vector<uint32_t> dyn, boff, bcnt;
+    std::vector<uint32_t> dyn;
+    std::vector<int4> csr;
     std::vector<int32_t> dyn_body;
     uint32_t ndyn = 0;
-    bool lists_dirty = true;
+    std::atomic<bool> lists_dirty{true};
+    std::atomic<bool> ub_dirty{true};  // a scene's body high-water mark moved: the used-slot list is rebuilt
+    // host workers: per-slot scratch (slot 0 is the calling thread's)
+    std::unique_ptr<Pool> pool;
+    std::vector<std::unique_ptr<Scratch>> scr;
+    std::vector<uint32_t> ubody;
     // device
     LD D{};
     std::vector<void*> owned;
@@ -1398,6 +1527,17 @@ struct Lat {
     DBuf<int2> d_dis;
     DBuf<uint32_t> d_scstat;
     DBuf<int32_t> d_slist;
+    DBuf<int4> d_csr;
+    // host phases' transfers: brick deltas and body records up, body records and brick positions down
+    DBuf<BrickDelta> d_delta;
+    DBuf<BodyRec> d_brec;
+    DBuf<uint32_t> d_idx;
+    DBuf<float4> d_wp;
+    HBuf<BrickDelta> h_delta;
+    HBuf<BodyRec> h_brec;
+    HBuf<uint32_t> h_idx;
+    HBuf<float4> h_wp;
+    std::vector<uint8_t> bmark;
     uint32_t h_ctr[C_NCTR];
     uint32_t ccap = 0, tsize = 0;
     // bricks whose material, body or bonds the host changed since the last flush
@@ -1405,19 +1545,22 @@ struct Lat {
     std::vector<uint32_t> tlist;
     // what the host wrote of a brick: 1 material, body, bonds; 2 local and world position (body set-up); 4 load
     // field (detachment zeroes it). Fields not written stay as the device has them.
-    void touch(int i, uint8_t what = 1) { if (!touched[i]) tlist.push_back((uint32_t)i); touched[i] |= what; }
+    void touch(int i, uint8_t what = 1) { if (!touched[i]) (t_scr ? t_scr->tlist : tlist).push_back((uint32_t)i); touched[i] |= what; }
     // bodies whose state the host changed since the last flush
     std::vector<uint8_t> btouched;
     std::vector<uint32_t> btlist;
-    void btouch(int b) { if (!btouched[b]) { btouched[b] = 1; btlist.push_back((uint32_t)b); } }
+    void btouch(int b) { if (!btouched[b]) { btouched[b] = 1; (t_scr ? t_scr->btlist : btlist).push_back((uint32_t)b); } }
     // timing
     double ms_frame = 0.0, ms_host = 0.0;
     double ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double sp[16] = {0};       // finer split of the host phases (sf_lat_subphase_ms)
     int syncs = 0;
 
     ~Lat() {
         for (void* p : owned) if (p) (void)hipFree(p);
-        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release();
+        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release(); d_csr.release();
+        d_delta.release(); d_brec.release(); d_idx.release(); d_wp.release();
+        h_delta.release(); h_brec.release(); h_idx.release(); h_wp.release();
         if (st) (void)hipStreamDestroy(st);
     }
     template <class T> int dalloc(T*& p, size_t n) {
@@ -1448,7 +1591,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.m, N); r |= L->dalloc(D.im, N); r |= L->dalloc(D.sleep, N); r |= L->dalloc(D.rad, N);
     r |= L->dalloc(D.cool, N); r |= L->dalloc(D.lt, N); r |= L->dalloc(D.resid, N); r |= L->dalloc(D.maxr, N);
     r |= L->dalloc(D.nmax, N); r |= L->dalloc(D.nc, N); r |= L->dalloc(D.count, N); r |= L->dalloc(D.boff, N);
-    r |= L->dalloc(D.bcnt, N);
+    r |= L->dalloc(D.bcnt, N); r |= L->dalloc(D.ubody, N); r |= L->dalloc(D.bdflag, N);
     r |= L->dalloc(D.dyn, B); r |= L->dalloc(D.dyn_body, B);
     r |= L->dalloc(D.st_scene, NS); r |= L->dalloc(D.st_lo, NS); r |= L->dalloc(D.st_hi, NS); r |= L->dalloc(D.st_act, NS);
     r |= L->dalloc(D.st_on, NS); r |= L->dalloc(D.st_live, NS); r |= L->dalloc(D.st_nmin, NS); r |= L->dalloc(D.st_resid, NS);
@@ -1517,6 +1660,9 @@ int lat_hash_alloc(Lat* L, uint32_t ndyn) {
 }
 
 // ---------------------------------------------------------------- host mirror helpers (the reference's bookkeeping)
+// the scratch of the running thread: a pool worker's own, else the world's
+struct ScrRef { std::vector<uint32_t> &q, &lst, &lst2, &cnt; };
+inline ScrRef scr(Lat* L) { if (t_scr) return {t_scr->q, t_scr->lst, t_scr->lst2, t_scr->cnt}; return {L->q, L->lst, L->lst2, L->cnt}; }
 inline LV hv(float4 a) { return lv(a.x, a.y, a.z); }
 inline float4 hf(LV a) { return make_float4(a.x, a.y, a.z, 0.0f); }
 inline float& bond_ref(Lat* L, int i, int a) { float4& b = L->bond[i]; return a == 0 ? b.x : (a == 1 ? b.y : (a == 2 ? b.z : b.w)); }
@@ -1650,7 +1796,7 @@ int alloc_body(Lat* L, HScene& S) {
         int b = S.bd0 + lb;
         if (!L->alive[b]) {
             S.cursor = (lb + 1) % S.dcap;
-            if (lb + 1 > S.hw) S.hw = lb + 1;
+            if (lb + 1 > S.hw) { S.hw = lb + 1; L->ub_dirty = true; }
             return b;
         }
     }
@@ -1658,17 +1804,18 @@ int alloc_body(Lat* L, HScene& S) {
 }
 const LM IDM = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
 void setup_body(Lat* L, int b, int s, int e, LV ppos, const LM& prot, LQ pq, LV pv, LV pw) {
+    ScrRef W = scr(L);
     const Tables& T = tables();
     L->lt[b] = 0.0f; L->resid[b] = 1.0f; L->maxr[b] = 0.0f;
     float mass = 0.0f;
     LV cl = lv(0.0f, 0.0f, 0.0f);
-    for (int t = s; t < e; t++) { int i = L->lst2[t]; float mm = T.mass[L->mat[i]]; mass += mm; cl = madd(cl, hv(L->local[i]), mm); }
+    for (int t = s; t < e; t++) { int i = W.lst2[t]; float mm = T.mass[L->mat[i]]; mass += mm; cl = madd(cl, hv(L->local[i]), mm); }
     cl = mul(cl, 1.0f / mass);
     float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     float radv = 0.0f;
     const float cube = S * S / 6.0f;
     for (int t = s; t < e; t++) {
-        int i = L->lst2[t];
+        int i = W.lst2[t];
         float mm = T.mass[L->mat[i]];
         LV r = sub(hv(L->local[i]), cl);
         L->local[i] = hf(r);
@@ -1677,7 +1824,7 @@ void setup_body(Lat* L, int b, int s, int e, LV ppos, const LM& prot, LQ pq, LV 
         it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
         it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
         L->body[i] = b;
-        L->next[i] = t + 1 < e ? L->lst2[t + 1] : NONE;
+        L->next[i] = t + 1 < e ? W.lst2[t + 1] : NONE;
     }
     it[3] = it[1]; it[6] = it[2]; it[7] = it[5];
     LM itm;
@@ -1685,7 +1832,7 @@ void setup_body(Lat* L, int b, int s, int e, LV ppos, const LM& prot, LQ pq, LV 
     LM iilm = minv(itm);
     LV off = mv(prot, cl);
     L->alive[b] = 1;
-    L->first[b] = L->lst2[s];
+    L->first[b] = W.lst2[s];
     L->count[b] = (uint32_t)(e - s);
     L->pos[b] = hf(add(ppos, off));
     L->qq[b] = f4q(pq); L->rot[b] = prot;
@@ -1698,17 +1845,18 @@ void setup_body(Lat* L, int b, int s, int e, LV ppos, const LM& prot, LQ pq, LV 
     L->iiw[b] = mmul(t1, pt);
     L->sleep[b] = 0.0f; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0; L->cool[b] = 0.15f;
     L->rad[b] = sqrtf(radv) + H * 1.8f;
-    for (int t = s; t < e; t++) { int i = L->lst2[t]; L->wpos[i] = hf(add(hv(L->pos[b]), mv(prot, hv(L->local[i])))); L->touch(i, 3); }
+    for (int t = s; t < e; t++) { int i = W.lst2[t]; L->wpos[i] = hf(add(hv(L->pos[b]), mv(prot, hv(L->local[i])))); L->touch(i, 3); }
     L->btouch(b);
     L->lists_dirty = true;
 }
 void rebuild_body(Lat* L, HScene& S, int id) {
+    ScrRef W = scr(L);
     L->dirty[id] = 0;
     int n = 0;
     uint32_t k = L->first[id];
     while (k != NONE) {
         uint32_t nx = L->next[k];
-        if (L->body[k] == id) { if (L->mat[k] != 0) L->lst[n++] = k; else { L->body[k] = -1; L->touch(k); } }
+        if (L->body[k] == id) { if (L->mat[k] != 0) W.lst[n++] = k; else { L->body[k] = -1; L->touch(k); } }
         k = nx;
     }
     L->lists_dirty = true;
@@ -1717,73 +1865,74 @@ void rebuild_body(Lat* L, HScene& S, int id) {
     uint32_t sid = ++S.stamp_id;
     uint32_t ncomp = 0;
     for (int t = 0; t < n; t++) {
-        uint32_t s0 = L->lst[t];
+        uint32_t s0 = W.lst[t];
         if (L->stamp[s0] == sid) continue;
         L->stamp[s0] = sid; L->comp[s0] = ncomp;
         int qh = 0, qt = 0;
-        L->q[qt++] = s0;
+        W.q[qt++] = s0;
         while (qh < qt) {
-            int i = (int)L->q[qh++];
+            int i = (int)W.q[qh++];
             for (int d = 0; d < 6; d++) {
                 int j = hlinked(L, i, d);
-                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->comp[j] = ncomp; L->q[qt++] = (uint32_t)j; }
+                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->comp[j] = ncomp; W.q[qt++] = (uint32_t)j; }
             }
         }
         ncomp++;
     }
     int ncmp = (int)ncomp;
-    for (int c = 0; c <= ncmp; c++) L->cnt[c] = 0;
-    for (int t = 0; t < n; t++) L->cnt[L->comp[L->lst[t]] + 1]++;
-    for (int c = 0; c < ncmp; c++) L->cnt[c + 1] += L->cnt[c];
-    for (int t = 0; t < n; t++) { int i = L->lst[t]; uint32_t c = L->comp[i]; L->lst2[L->cnt[c]++] = (uint32_t)i; }
+    for (int c = 0; c <= ncmp; c++) W.cnt[c] = 0;
+    for (int t = 0; t < n; t++) W.cnt[L->comp[W.lst[t]] + 1]++;
+    for (int c = 0; c < ncmp; c++) W.cnt[c + 1] += W.cnt[c];
+    for (int t = 0; t < n; t++) { int i = W.lst[t]; uint32_t c = L->comp[i]; W.lst2[W.cnt[c]++] = (uint32_t)i; }
     LV ppos = hv(L->pos[id]), pv = hv(L->v[id]), pw = hv(L->w[id]);
     LM prot = L->rot[id];
     LQ pq = lq4(L->qq[id]);
     for (int c = 0; c < ncmp; c++) {
-        int s0 = c == 0 ? 0 : (int)L->cnt[c - 1];
-        int e = (int)L->cnt[c];
+        int s0 = c == 0 ? 0 : (int)W.cnt[c - 1];
+        int e = (int)W.cnt[c];
         if (c > 0 && e - s0 == 1 && S.stats[3] > 1500.0f) {
-            int i = L->lst2[s0];
+            int i = W.lst2[s0];
             int mt_ = L->mat[i];
             if (mt_ != STEEL && rnd(S) < minf((S.stats[3] - 1500.0f) / 2000.0f, 0.85f)) { remove_brick_raw(L, i); continue; }
         }
         int b = c == 0 ? id : alloc_body(L, S);
-        if (b < 0) { for (int t = s0; t < e; t++) remove_brick_raw(L, L->lst2[t]); continue; }
+        if (b < 0) { for (int t = s0; t < e; t++) remove_brick_raw(L, W.lst2[t]); continue; }
         float keep_cool = L->cool[id], keep_lt = L->lt[id];
         setup_body(L, b, s0, e, ppos, prot, pq, pv, pw);
         if (c == 0) { L->cool[b] = keep_cool; L->lt[b] = keep_lt; }
     }
 }
 void conn_struct(Lat* L, HScene& S, int sl) {
+    ScrRef W = scr(L);
     HStruct& T = L->sts[S.st0 + sl];
     uint32_t sid = ++S.stamp_id;
     int qh = 0, qt = 0;
     for (int i = T.lo; i < T.hi; i++)
-        if (hin_struct(L, i, sl) && L->bxyz[i].y == 0 && L->bond[i].w > 0.0f) { L->stamp[i] = sid; L->q[qt++] = (uint32_t)i; }
+        if (hin_struct(L, i, sl) && L->bxyz[i].y == 0 && L->bond[i].w > 0.0f) { L->stamp[i] = sid; W.q[qt++] = (uint32_t)i; }
     while (qh < qt) {
-        int i = (int)L->q[qh++];
+        int i = (int)W.q[qh++];
         for (int d = 0; d < 6; d++) {
             int j = hlinked(L, i, d);
-            if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->q[qt++] = (uint32_t)j; }
+            if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; W.q[qt++] = (uint32_t)j; }
         }
     }
     bool retry = false;
     for (int s0 = T.lo; s0 < T.hi; s0++) {
         if (!hin_struct(L, s0, sl) || L->stamp[s0] == sid) continue;
         int n = 0;
-        L->stamp[s0] = sid; L->lst2[n++] = (uint32_t)s0;
+        L->stamp[s0] = sid; W.lst2[n++] = (uint32_t)s0;
         int h = 0;
         while (h < n) {
-            int i = (int)L->lst2[h++];
+            int i = (int)W.lst2[h++];
             for (int d = 0; d < 6; d++) {
                 int j = hlinked(L, i, d);
-                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->lst2[n++] = (uint32_t)j; }
+                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; W.lst2[n++] = (uint32_t)j; }
             }
         }
         int b = alloc_body(L, S);
         if (b < 0) { retry = true; continue; }
         for (int t = 0; t < n; t++) {
-            int i = L->lst2[t];
+            int i = W.lst2[t];
             L->local[i] = hf(lat_pos(L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z));
             L->phi[i] = 0.0f; L->ext[i] = 0.0f; L->extn[i] = 0.0f;
             L->touch(i, 7);
@@ -1804,6 +1953,7 @@ void connectivity(Lat* L, HScene& S) {
     for (int sl = 0; sl < S.nst; sl++) { HStruct& T = L->sts[S.st0 + sl]; if (T.conn) { T.conn = false; conn_struct(L, S, sl); } }
 }
 uint32_t crush_at(Lat* L, HScene& S, int start, float e, LV p, uint32_t cap) {
+    ScrRef W = scr(L);
     const Tables& T = tables();
     int m0 = L->mat[start];
     if (m0 == 0 || m0 >= SLUG) return 0;
@@ -1811,7 +1961,7 @@ uint32_t crush_at(Lat* L, HScene& S, int start, float e, LV p, uint32_t cap) {
     int bodyv = L->body[start];
     int sidv = L->sid[start];
     uint32_t st = ++S.stamp_id;
-    L->q[0] = (uint32_t)start;
+    W.q[0] = (uint32_t)start;
     int qn = 1; L->stamp[start] = st;
     int head = 0;
     uint32_t crushed = 0;
@@ -1820,25 +1970,25 @@ uint32_t crush_at(Lat* L, HScene& S, int start, float e, LV p, uint32_t cap) {
     while (head < qn && crushed < maxn) {
         int best = head; float bd = 1e18f;
         int lim = head + 24 < qn ? head + 24 : qn;
-        for (int t = head; t < lim; t++) { int i = (int)L->q[t]; LV d = sub(hbpos(L, i), p); float dd = dot(d, d); if (dd < bd) { bd = dd; best = t; } }
-        std::swap(L->q[head], L->q[best]);
-        int i = (int)L->q[head]; head++;
+        for (int t = head; t < lim; t++) { int i = (int)W.q[t]; LV d = sub(hbpos(L, i), p); float dd = dot(d, d); if (dd < bd) { bd = dd; best = t; } }
+        std::swap(W.q[head], W.q[best]);
+        int i = (int)W.q[head]; head++;
         int mm = L->mat[i];
         if (mm == 0) continue;
         float cost = T.eb[mm];
         if (e < cost) { damage_brick(L, S, i, 0.6f * e / cost, 0.0f); break; }
         e -= cost;
-        L->lst2[ncr++] = (uint32_t)i; crushed++;
+        W.lst2[ncr++] = (uint32_t)i; crushed++;
         for (int d = 0; d < 6; d++) {
             int j = hlinked(L, i, d);
             if (j < 0 || L->stamp[j] == st || L->body[j] != bodyv || L->sid[j] != sidv) continue;
             if (qn >= S.bcap) break;
-            L->stamp[j] = st; L->q[qn++] = (uint32_t)j;
+            L->stamp[j] = st; W.q[qn++] = (uint32_t)j;
         }
     }
     if (ncr == 0) return 0;
     for (int t = 0; t < ncr; t++) {
-        int i = (int)L->lst2[t];
+        int i = (int)W.lst2[t];
         int mm = L->mat[i];
         if (mm == 0) continue;
         if (rnd(S) < T.pulv[mm]) remove_brick(L, S, i);
@@ -1955,32 +2105,43 @@ int rebuild_csr(Lat* L) {
     if (!L->lists_dirty) return 0;
     L->lists_dirty = false;
     uint32_t k = 0;
-    for (int b = 0; b < L->nbd; b++) { L->boff[b] = 0; L->bcnt[b] = 0; }
+    std::vector<int4>& csr = L->csr;
+    csr.clear();
     for (size_t si = 0; si < L->sc.size(); si++) {
         HScene& S = L->sc[si];
         S.dyn0 = k;
         for (int lb = 0; lb < S.hw; lb++) {
             int b = S.bd0 + lb;
             if (!L->alive[b]) continue;
-            L->boff[b] = k;
+            uint32_t o = k;
             for (uint32_t i = L->first[b]; i != NONE; i = L->next[i]) { L->dyn[k] = i; L->dyn_body[k] = b; k++; }
-            L->bcnt[b] = k - L->boff[b];
+            csr.push_back(make_int4(b, (int)o, (int)(k - o), 0));
         }
         S.ndyn = k - S.dyn0;
     }
     L->ndyn = k;
     L->D.ndyn = k;
     if (lat_hash_alloc(L, k)) return -1;
+    if (L->d_csr.n < csr.size() && L->d_csr.alloc(std::max<size_t>(csr.size(), 4096))) return -1;
     hipStream_t s = L->st;
     int r = 0;
     r |= cp_up(L->D.dyn, L->dyn.data(), k, s); r |= cp_up(L->D.dyn_body, L->dyn_body.data(), k, s);
-    r |= cp_up(L->D.boff, L->boff.data(), L->nbd, s); r |= cp_up(L->D.bcnt, L->bcnt.data(), L->nbd, s);
+    r |= cp_up(L->d_csr.p, csr.data(), csr.size(), s);
     if (r) return r;
+    // live bodies' brick ranges; the ranges of dead bodies are never read
+    if (!csr.empty()) LAUNCH(k_csr_set, sf_grid(csr.size()), NT, L->D, (const int4*)L->d_csr.p, (uint32_t)csr.size());
     SF_HIP(hipStreamSynchronize(s));
     return upload_scenes(L);  // the scenes' first dynamic slots moved
 }
 
 int upload_scenes(Lat* L) {
+    if (L->ub_dirty) {
+        L->ub_dirty = false;
+        L->ubody.clear();
+        for (const HScene& S : L->sc) for (int lb = 0; lb < S.hw; lb++) L->ubody.push_back((uint32_t)(S.bd0 + lb));
+        L->D.nub = (uint32_t)L->ubody.size();
+        if (cp_up(L->D.ubody, L->ubody.data(), L->ubody.size(), L->st)) return -1;
+    }
     std::vector<SceneK> k(L->sc.size());
     for (size_t si = 0; si < L->sc.size(); si++) {
         const HScene& S = L->sc[si];
@@ -1994,7 +2155,6 @@ int upload_scenes(Lat* L) {
     return 0;
 }
 
-#define LAUNCH(kern, grid, block, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, L->st, __VA_ARGS__)
 
 int fill_u32(Lat* L, uint32_t* p, uint32_t v, size_t n) { if (n) LAUNCH(k_fill_u32, sf_grid(n), NT, p, v, n); return 0; }
 // every counter but the killed-body count, which accumulates until the host has drained the list
@@ -2041,7 +2201,6 @@ int gather(Lat* L, const uint32_t* list, uint32_t n, std::vector<CRec>& out) {
     return 0;
 }
 
-struct BrickDelta { uint32_t i; int32_t body; uint32_t mat; uint32_t full; float4 bond, local, wpos; float phi, ext, extn, pad2; };
 __global__ void k_brick_delta(LD D, const BrickDelta* d, uint32_t n) {
     GS(t, n) {
         BrickDelta x = d[t];
@@ -2050,16 +2209,9 @@ __global__ void k_brick_delta(LD D, const BrickDelta* d, uint32_t n) {
         if (x.full & 4u) { D.phi[x.i] = x.phi; D.ext[x.i] = x.ext; D.extn[x.i] = x.extn; }
     }
 }
-// a body's full state, packed for one copy each way
-struct BodyRec {
-    uint32_t b; uint8_t alive, asleep, wake, pad0; uint32_t count, nc;
-    float4 pos, q, v, w, vlast, wlast;
-    float4 rot[3], iil[3], iiw[3];
-    float m, im, sleep, rad, cool, lt, resid, maxr, nmax, pad1, pad2, pad3;
-};
-__global__ void k_body_pack(LD D, uint32_t b0, uint32_t n, BodyRec* out) {
+__global__ void k_body_pack(LD D, const uint32_t* idx, uint32_t n, BodyRec* out) {
     GS(t, n) {
-        uint32_t b = b0 + t;
+        uint32_t b = idx[t];
         BodyRec r;
         r.b = b; r.alive = D.alive[b]; r.asleep = D.asleep[b]; r.wake = D.wake[b]; r.pad0 = 0; r.count = D.count[b]; r.nc = D.nc[b];
         r.pos = D.pos[b]; r.q = D.q[b]; r.v = D.v[b]; r.w = D.w[b]; r.vlast = D.vlast[b]; r.wlast = D.wlast[b];
@@ -2069,6 +2221,7 @@ __global__ void k_body_pack(LD D, uint32_t b0, uint32_t n, BodyRec* out) {
         out[t] = r;
     }
 }
+__global__ void k_gather_wpos(LD D, const uint32_t* idx, uint32_t n, float4* out) { GS(t, n) out[t] = D.wpos[idx[t]]; }
 __global__ void k_body_unpack(LD D, const BodyRec* in, uint32_t n) {
     GS(t, n) {
         BodyRec r = in[t];
@@ -2080,55 +2233,83 @@ __global__ void k_body_unpack(LD D, const BodyRec* in, uint32_t n) {
         D.resid[b] = r.resid; D.maxr[b] = r.maxr; D.nmax[b] = r.nmax;
     }
 }
-DBuf<BrickDelta> g_delta;
-DBuf<BodyRec> g_brec;
 float4 m_row(const LM& m, int k) { return make_float4(m.m[3 * k], m.m[3 * k + 1], m.m[3 * k + 2], 0.0f); }
 void row_m(LM& m, int k, float4 r) { m.m[3 * k] = r.x; m.m[3 * k + 1] = r.y; m.m[3 * k + 2] = r.z; }
 
 // the scene's body slots in use, from the device into the host mirror (one packing kernel, one copy)
-int pull_bodies(Lat* L, int si) {
-    const HScene& S = L->sc[si];
-    uint32_t n = (uint32_t)S.hw;
+// the body records of the given scenes (every slot up to each scene's high-water mark) into the host mirror: one
+// pack, one copy, one wait for all of them
+int pull_bodies(Lat* L, const std::vector<int>& scenes) {
+    size_t n = 0;
+    for (int si : scenes) n += (size_t)L->sc[si].hw;
     if (!n) return 0;
-    if (g_brec.n < n && g_brec.alloc(std::max<size_t>(n, 4096))) return -1;
-    LAUNCH(k_body_pack, sf_grid(n), NT, L->D, (uint32_t)S.bd0, n, g_brec.p);
-    std::vector<BodyRec> r(n);
-    SF_HIP(hipMemcpyAsync(r.data(), g_brec.p, sizeof(BodyRec) * n, hipMemcpyDeviceToHost, L->st));
+    if (L->h_idx.reserve(n) || L->h_brec.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    if ((L->d_idx.n < n && L->d_idx.alloc(std::max<size_t>(n, 4096))) || (L->d_brec.n < n && L->d_brec.alloc(std::max<size_t>(n, 4096)))) {
+        sf::set_error(SF_E_HIP, "hipMalloc failed");
+        return -1;
+    }
+    size_t k = 0;
+    for (int si : scenes) { const HScene& S = L->sc[si]; for (int lb = 0; lb < S.hw; lb++) L->h_idx.p[k++] = (uint32_t)(S.bd0 + lb); }
+    SF_HIP(hipMemcpyAsync(L->d_idx.p, L->h_idx.p, sizeof(uint32_t) * n, hipMemcpyHostToDevice, L->st));
+    LAUNCH(k_body_pack, sf_grid(n), NT, L->D, (const uint32_t*)L->d_idx.p, (uint32_t)n, L->d_brec.p);
+    SF_HIP(hipMemcpyAsync(L->h_brec.p, L->d_brec.p, sizeof(BodyRec) * n, hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
     L->syncs++;
-    for (uint32_t t = 0; t < n; t++) {
-        const BodyRec& x = r[t];
+    for (size_t t = 0; t < n; t++) {
+        const BodyRec& x = L->h_brec.p[t];
         int b = (int)x.b;
         L->alive[b] = x.alive; L->asleep[b] = x.asleep; L->wake[b] = x.wake; L->nc[b] = x.nc;
         L->pos[b] = x.pos; L->qq[b] = x.q; L->v[b] = x.v; L->w[b] = x.w; L->vlast[b] = x.vlast; L->wlast[b] = x.wlast;
-        for (int k = 0; k < 3; k++) { row_m(L->rot[b], k, x.rot[k]); row_m(L->iiw[b], k, x.iiw[k]); }
+        for (int k2 = 0; k2 < 3; k2++) { row_m(L->rot[b], k2, x.rot[k2]); row_m(L->iiw[b], k2, x.iiw[k2]); }
         L->sleep[b] = x.sleep; L->cool[b] = x.cool; L->lt[b] = x.lt; L->resid[b] = x.resid; L->maxr[b] = x.maxr; L->nmax[b] = x.nmax;
     }
+    return 0;
+}
+int pull_bodies(Lat* L, int si) { return pull_bodies(L, std::vector<int>{si}); }
+// world positions of the bricks of the given bodies into the host mirror: one gather, one copy, one wait
+int pull_body_wpos(Lat* L, const std::vector<int>& bodies) {
+    std::vector<uint32_t>& list = L->lst;
+    size_t n = 0;
+    for (int b : bodies)
+        for (uint32_t i = L->first[b]; i != NONE; i = L->next[i]) list[n++] = i;
+    if (!n) return 0;
+    if (L->h_idx.reserve(n) || L->h_wp.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    if ((L->d_idx.n < n && L->d_idx.alloc(std::max<size_t>(n, 4096))) || (L->d_wp.n < n && L->d_wp.alloc(std::max<size_t>(n, 4096)))) {
+        sf::set_error(SF_E_HIP, "hipMalloc failed");
+        return -1;
+    }
+    memcpy(L->h_idx.p, list.data(), sizeof(uint32_t) * n);
+    SF_HIP(hipMemcpyAsync(L->d_idx.p, L->h_idx.p, sizeof(uint32_t) * n, hipMemcpyHostToDevice, L->st));
+    LAUNCH(k_gather_wpos, sf_grid(n), NT, L->D, (const uint32_t*)L->d_idx.p, (uint32_t)n, L->d_wp.p);
+    SF_HIP(hipMemcpyAsync(L->h_wp.p, L->d_wp.p, sizeof(float4) * n, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    for (size_t t = 0; t < n; t++) L->wpos[L->h_idx.p[t]] = L->h_wp.p[t];
     return 0;
 }
 // upload what the host changed: bricks (material, body, bonds, local and world position, load field) and bodies
 int flush_delta(Lat* L) {
     size_t n = L->tlist.size(), nb = L->btlist.size();
     if (n) {
-        std::vector<BrickDelta> d(n);
+        if (L->h_delta.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+        if (L->d_delta.n < n && L->d_delta.alloc(std::max<size_t>(n, 4096))) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
         for (size_t t = 0; t < n; t++) {
             uint32_t i = L->tlist[t];
-            BrickDelta& x = d[t];
+            BrickDelta& x = L->h_delta.p[t];
             x.i = i; x.body = L->body[i]; x.mat = L->mat[i]; x.full = L->touched[i]; x.bond = L->bond[i]; x.local = L->local[i]; x.wpos = L->wpos[i];
             x.phi = L->phi[i]; x.ext = L->ext[i]; x.extn = L->extn[i]; x.pad2 = 0.0f;
             L->touched[i] = 0;
         }
         L->tlist.clear();
-        if (g_delta.n < n && g_delta.alloc(std::max<size_t>(n, 4096))) return -1;
-        SF_HIP(hipMemcpyAsync(g_delta.p, d.data(), sizeof(BrickDelta) * n, hipMemcpyHostToDevice, L->st));
-        LAUNCH(k_brick_delta, sf_grid(n), NT, L->D, g_delta.p, (uint32_t)n);
-        SF_HIP(hipStreamSynchronize(L->st));
+        SF_HIP(hipMemcpyAsync(L->d_delta.p, L->h_delta.p, sizeof(BrickDelta) * n, hipMemcpyHostToDevice, L->st));
+        LAUNCH(k_brick_delta, sf_grid(n), NT, L->D, (const BrickDelta*)L->d_delta.p, (uint32_t)n);
     }
     if (nb) {
-        std::vector<BodyRec> r(nb);
+        if (L->h_brec.reserve(nb)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+        if (L->d_brec.n < nb && L->d_brec.alloc(std::max<size_t>(nb, 4096))) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
         for (size_t t = 0; t < nb; t++) {
             uint32_t b = L->btlist[t];
-            BodyRec& x = r[t];
+            BodyRec& x = L->h_brec.p[t];
             x.b = b; x.alive = L->alive[b]; x.asleep = L->asleep[b]; x.wake = L->wake[b]; x.pad0 = 0; x.count = L->count[b]; x.nc = L->nc[b];
             x.pos = L->pos[b]; x.q = L->qq[b]; x.v = L->v[b]; x.w = L->w[b]; x.vlast = L->vlast[b]; x.wlast = L->wlast[b];
             for (int k = 0; k < 3; k++) { x.rot[k] = m_row(L->rot[b], k); x.iil[k] = m_row(L->iil[b], k); x.iiw[k] = m_row(L->iiw[b], k); }
@@ -2137,11 +2318,11 @@ int flush_delta(Lat* L) {
             L->btouched[b] = 0;
         }
         L->btlist.clear();
-        if (g_brec.n < nb && g_brec.alloc(std::max<size_t>(nb, 4096))) return -1;
-        SF_HIP(hipMemcpyAsync(g_brec.p, r.data(), sizeof(BodyRec) * nb, hipMemcpyHostToDevice, L->st));
-        LAUNCH(k_body_unpack, sf_grid(nb), NT, L->D, g_brec.p, (uint32_t)nb);
-        SF_HIP(hipStreamSynchronize(L->st));
+        SF_HIP(hipMemcpyAsync(L->d_brec.p, L->h_brec.p, sizeof(BodyRec) * nb, hipMemcpyHostToDevice, L->st));
+        LAUNCH(k_body_unpack, sf_grid(nb), NT, L->D, (const BodyRec*)L->d_brec.p, (uint32_t)nb);
     }
+    // the staging buffers are reused by the next transfer
+    if (n || nb) { SF_HIP(hipStreamSynchronize(L->st)); L->syncs++; }
     return 0;
 }
 
@@ -2154,13 +2335,17 @@ int plough_phase(Lat* L, uint32_t n) {
     std::vector<CRec> rec;
     if (gather(L, L->D.plist, n, rec)) return -1;
     std::vector<int2> dis;
-    std::vector<uint8_t> pulled(L->sc.size(), 0);
+    std::vector<int> scenes;
+    for (const CRec& c : rec) {
+        int si = scene_of_contact(L, c.c);
+        if (si >= 0 && (scenes.empty() || scenes.back() != si)) scenes.push_back(si);  // records are in contact order
+    }
+    if (pull_bodies(L, scenes)) return -1;
     const Tables& T = tables();
     for (const CRec& c : rec) {
         int si = scene_of_contact(L, c.c);
         if (si < 0) continue;
         HScene& S = L->sc[si];
-        if (!pulled[si]) { if (pull_bodies(L, si)) return -1; pulled[si] = 1; }
         int j = (int)c.bb;
         if (L->mat[j] == 0 || L->body[j] >= 0) { dis.push_back(make_int2((int)c.c, -1)); continue; }
         int a = c.a;
@@ -2214,7 +2399,16 @@ int impact_phase(Lat* L, uint32_t n) {
     n = std::min(n, L->D.pcap);
     std::vector<CRec> rec;
     if (gather(L, L->D.ilist, n, rec)) return -1;
-    std::vector<uint8_t> pulled(L->sc.size(), 0);
+    // a crush walks the bricks of the body it starts in: the world positions of the candidates' bodies, all at once
+    std::vector<int> bodies;
+    auto mark = [&](uint32_t i) {
+        if (i == NONE) return;
+        int b = L->body[i];
+        if (b >= 0 && !L->bmark[b]) { L->bmark[b] = 1; bodies.push_back(b); }
+    };
+    for (const CRec& c : rec) if (c.on && scene_of_contact(L, c.c) >= 0) { mark(c.ba); mark(c.bb); }
+    for (int b : bodies) L->bmark[b] = 0;
+    if (pull_body_wpos(L, bodies)) return -1;
     std::vector<uint32_t> budget(L->sc.size(), 2500u);
     const Tables& T = tables();
     for (const CRec& c : rec) {
@@ -2222,7 +2416,6 @@ int impact_phase(Lat* L, uint32_t n) {
         int si = scene_of_contact(L, c.c);
         if (si < 0) continue;
         HScene& S = L->sc[si];
-        if (!pulled[si]) { if (pull(L, si, G_WPOS)) return -1; pulled[si] = 1; }
         int ia = (int)c.ba;
         int ma = L->mat[ia];
         if (ma == 0 || ma >= SLUG) continue;
@@ -2261,7 +2454,7 @@ void segsort(Lat* L, const uint32_t* off, const uint32_t* cnt, uint32_t* val, ui
 
 int substep(Lat* L, float h) {
     LD& D = L->D;
-    const unsigned gb = sf_grid(L->nbd), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
+    const unsigned gb = sf_grid(D.nub), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
     LAUNCH(k_cache, gb, NT, D, 1, h);
     LAUNCH(k_wpos, gd, NT, D);
     // hash
@@ -2281,7 +2474,7 @@ int substep(Lat* L, float h) {
     if (ncs > L->ccap) {
         if (lat_contacts_alloc(L, std::max<uint32_t>(ncs + ncs / 2, 1u << 16))) return -1;
     }
-    fill_u32(L, D.nc, 0, L->nbd);
+    LAUNCH(k_nc_clear, gb, NT, D);
     LAUNCH(k_contacts<true>, gd, NT, D);
     LAUNCH(k_scene_c0, 1, 64, D);
     LAUNCH(k_plough_flag, sf_grid(ncs), NT, D);
@@ -2386,9 +2579,12 @@ int structures_upload(Lat* L, bool warm) {
 
 int structural(Lat* L, float dt) {
     LD& D = L->D;
-    const unsigned gb = sf_grid(L->nbd), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1)), gbr = sf_grid(L->nb);
-    const unsigned gw = sf_grid((uint64_t)L->nbd * 64, NT, 65536);
+    const unsigned gb = sf_grid(D.nub), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1)), gbr = sf_grid(L->nb);
+    const unsigned gw = sf_grid((uint64_t)D.nub * 64, NT, 65536);
+    auto t = std::chrono::steady_clock::now();
     if (structures_upload(L, false)) return -1;
+    L->sp[7] += ms_since(t);
+    t = std::chrono::steady_clock::now();
     size_t ns = L->sts.size();
     fill_u32(L, D.st_maxr, 0, ns);
     fill_u32(L, D.st_flags, 0, ns);
@@ -2410,7 +2606,7 @@ int structural(Lat* L, float dt) {
     }
     LAUNCH(k_ga, gw, NT, D, dt);
     LAUNCH(k_body_sor_prep, gb, NT, D);
-    LAUNCH(k_sor_body, std::min<int>(std::max(L->nbd, 1), 8192), BD_NT, D);
+    LAUNCH(k_sor_body, std::min<uint32_t>(std::max<uint32_t>(D.nub, 1), 8192), BD_NT, D);
     LAUNCH(k_stress_clear, gbr, NT, D);
     LAUNCH(k_body_judge_prep, gb, NT, D, dt);
     reset_ctr(L);
@@ -2421,15 +2617,17 @@ int structural(Lat* L, float dt) {
     // results back: per structure resid / max ratio / flags, per scene max ratio, events
     std::vector<float> resid(ns);
     std::vector<uint32_t> maxr(ns), flags(ns), gmax(L->sc.size());
-    std::vector<uint8_t> bdirty(L->nbd);
+    std::vector<uint8_t> bdirty(D.nub);
+    L->sp[8] += ms_since(t);
+    t = std::chrono::steady_clock::now();
     int r = 0;
     r |= cp_down(resid.data(), D.st_resid, ns, L->st); r |= cp_down(maxr.data(), D.st_maxr, ns, L->st);
     r |= cp_down(flags.data(), D.st_flags, ns, L->st); r |= cp_down(gmax.data(), D.sc_gmax, L->sc.size(), L->st);
-    r |= cp_down(bdirty.data(), D.bd_dirty, L->nbd, L->st);
+    LAUNCH(k_take_dirty, gb, NT, D);
+    r |= cp_down(bdirty.data(), D.bdflag, D.nub, L->st);
     if (r) return r;
     if (read_ctr(L)) return -1;
     if (drain_kills(L)) return -1;
-    if (L->nbd) LAUNCH(k_fill_u8, gb, NT, D.bd_dirty, (uint8_t)0, (size_t)L->nbd);
     uint32_t nev = std::min(L->h_ctr[C_NEV], D.evcap), nbw = L->h_ctr[C_NBW];
     std::vector<unsigned long long> ekey(nev);
     std::vector<int4> ev(nev);
@@ -2440,10 +2638,12 @@ int structural(Lat* L, float dt) {
     if (nbw && !bw_overflow) { bwi.resize(nbw); bwv.resize(nbw); r |= cp_down(bwi.data(), D.bwi, nbw, L->st); r |= cp_down(bwv.data(), D.bwv, nbw, L->st); }
     if (r) return r;
     SF_HIP(hipStreamSynchronize(L->st));
+    L->sp[9] += ms_since(t);
+    t = std::chrono::steady_clock::now();
     if (nev > 0 && L->h_ctr[C_NEV] > D.evcap) { sf::set_error(SF_E_CAP, "structural event buffer overflow"); return SF_E_CAP; }
     if (bw_overflow) { for (size_t si = 0; si < L->sc.size(); si++) if (pull(L, (int)si, G_BOND)) return -1; }
     else for (uint32_t e = 0; e < nbw; e++) bond_ref(L, (int)(bwi[e] >> 2), (int)(bwi[e] & 3u)) = bwv[e];
-    for (int b = 0; b < L->nbd; b++) if (bdirty[b]) L->dirty[b] = 1;
+    for (uint32_t ub = 0; ub < D.nub; ub++) if (bdirty[ub]) L->dirty[L->ubody[ub]] = 1;
     std::vector<uint32_t> order(nev);
     for (uint32_t e = 0; e < nev; e++) order[e] = e;
     std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ekey[a] < ekey[b]; });
@@ -2483,7 +2683,11 @@ int structural(Lat* L, float dt) {
         memcpy(&S.stats[6], &gmax[si], 4);
     }
     (void)changed;
-    return flush_delta(L);
+    L->sp[10] += ms_since(t);
+    t = std::chrono::steady_clock::now();
+    r = flush_delta(L);
+    L->sp[11] += ms_since(t);
+    return r;
 }
 }  // namespace
 
@@ -2492,7 +2696,7 @@ namespace {
 int scene_stats(Lat* L) {
     size_t S = L->sc.size();
     fill_u32(L, L->d_scstat.p, 0, 3 * S);
-    LAUNCH(k_scene_stats, sf_grid(L->nbd), NT, L->D, L->d_scstat.p);
+    LAUNCH(k_scene_stats, sf_grid(L->D.nub), NT, L->D, L->d_scstat.p);
     std::vector<uint32_t> o(3 * S);
     SF_HIP(hipMemcpyAsync(o.data(), L->d_scstat.p, sizeof(uint32_t) * 3 * S, hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
@@ -2506,26 +2710,78 @@ int scene_stats(Lat* L) {
     return 0;
 }
 
+// the world's worker pool, made on first use: SF_HOST_THREADS threads in all (default: the cores, at most 16)
+Pool* host_pool(Lat* L) {
+    if (!L->pool) {
+        int n = (int)std::thread::hardware_concurrency();
+        if (const char* e = getenv("SF_HOST_THREADS")) n = atoi(e);
+        n = std::max(1, std::min(n, 16));
+        if (n < 2) return nullptr;
+        L->pool.reset(new Pool(n - 1));
+        L->scr.clear();
+        for (int k = 0; k < n; k++) L->scr.emplace_back(new Scratch());
+    }
+    return L->pool.get();
+}
+// the workers' touched bricks and bodies join the world's lists
+void merge_touched(Lat* L) {
+    for (auto& w : L->scr) {
+        L->tlist.insert(L->tlist.end(), w->tlist.begin(), w->tlist.end()); w->tlist.clear();
+        L->btlist.insert(L->btlist.end(), w->btlist.begin(), w->btlist.end()); w->btlist.clear();
+    }
+}
+
 int host_topology(Lat* L, bool rebuild) {
     struct Acc { Lat* L; std::chrono::steady_clock::time_point t; ~Acc() { L->ph[3] += ms_since(t); } } acc{L, std::chrono::steady_clock::now()};
+    std::vector<int> need;
     for (size_t si = 0; si < L->sc.size(); si++) {
         HScene& S = L->sc[si];
-        bool need = S.conn_any;
-        if (rebuild) for (int lb = 0; lb < S.hw && !need; lb++) { int b = S.bd0 + lb; need = L->alive[b] && L->dirty[b]; }
-        if (!need) continue;
-        if (pull_bodies(L, (int)si)) return -1;
+        bool nd = S.conn_any;
+        if (rebuild) for (int lb = 0; lb < S.hw && !nd; lb++) { int b = S.bd0 + lb; nd = L->alive[b] && L->dirty[b]; }
+        if (nd) need.push_back((int)si);
+    }
+    auto t = std::chrono::steady_clock::now();
+    if (pull_bodies(L, need)) return -1;
+    L->sp[0] += ms_since(t);
+    t = std::chrono::steady_clock::now();
+    auto scene_work = [&](int si) {
+        HScene& S = L->sc[si];
         if (rebuild) for (int lb = 0; lb < S.hw; lb++) { int b = S.bd0 + lb; if (L->alive[b] && L->dirty[b]) rebuild_body(L, S, b); }
         connectivity(L, S);
+    };
+    Pool* pool = need.size() >= 2 ? host_pool(L) : nullptr;
+    if (pool) {
+        size_t maxb = 0;
+        for (int si : need) maxb = std::max<size_t>(maxb, (size_t)L->sc[si].bcap);
+        for (auto& w : L->scr) w->reserve(maxb);
+        pool->run((int)need.size(), [&](int k, int slot) {
+            t_scr = L->scr[slot].get();
+            scene_work(need[k]);
+            t_scr = nullptr;
+        });
+        merge_touched(L);
+    } else {
+        for (int si : need) scene_work(si);
     }
+    L->sp[2] += ms_since(t);
+    t = std::chrono::steady_clock::now();
     if (flush_delta(L)) return -1;
+    L->sp[3] += ms_since(t);
+    t = std::chrono::steady_clock::now();
     if (rebuild_csr(L)) return -1;
-    return upload_scenes(L);
+    L->sp[4] += ms_since(t);
+    t = std::chrono::steady_clock::now();
+    int r = upload_scenes(L);
+    L->sp[5] += ms_since(t);
+    L->sp[6] += (double)need.size();
+    return r;
 }
 
 int frame(Lat* L, float dt) {
     auto t0 = std::chrono::steady_clock::now();
     L->syncs = 0;
     for (double& x : L->ph) x = 0.0;
+    for (double& x : L->sp) x = 0.0;
     dt = clampf(dt, 0.0f, 1.0f / 30.0f);
     for (HScene& S : L->sc) {
         S.time += dt; S.crush_n = 0; S.frame += 1;
@@ -2538,7 +2794,7 @@ int frame(Lat* L, float dt) {
     if (read_ctr(L) || drain_kills(L)) return -1;
     if (host_topology(L, true)) return -1;
     LD& D = L->D;
-    const unsigned gb = sf_grid(L->nbd), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
+    const unsigned gb = sf_grid(D.nub), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
     LAUNCH(k_cache, gb, NT, D, 0, 0.0f);
     LAUNCH(k_wpos, gd, NT, D);
     ts = std::chrono::steady_clock::now();
@@ -2770,6 +3026,7 @@ void clear_scene(Lat* L, int si, uint32_t seed) {
     S.rng = seed ? seed : 0x9E3779B9u;
     if (had_grid) { for (size_t k = 0; k < (size_t)gnx * gny * gnz; k++) L->grid[goff + k] = 0; }
     L->lists_dirty = true;
+    L->ub_dirty = true;
 }
 
 struct Handle {
@@ -2808,9 +3065,9 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     L->rot.assign(N, z); L->iil = L->rot; L->iiw = L->rot;
     L->m.assign(N, 0.0f); L->im = L->m; L->sleep = L->m; L->rad = L->m; L->cool = L->m; L->lt = L->m; L->resid = L->m; L->maxr = L->m; L->nmax = L->m;
     L->touched.assign(B, 0);
-    L->btouched.assign(N, 0);
+    L->btouched.assign(N, 0); L->bmark.assign(N, 0);
     L->lst.assign(B + 1, 0); L->lst2.assign(B + 1, 0); L->comp.assign(B + 1, 0); L->cnt.assign(B + 2, 0);
-    L->dyn.assign(B, 0); L->dyn_body.assign(B, 0); L->boff.assign(N, 0); L->bcnt.assign(N, 0);
+    L->dyn.assign(B, 0); L->dyn_body.assign(B, 0);
     return h;
 }
 void sf_lat_destroy(sf_lat* h) { delete h; }
@@ -3066,6 +3323,11 @@ int sf_lat_timing(sf_lat* h, double* ms_frame, int32_t* syncs) {
     if (!h) return SF_E_ARG;
     if (ms_frame) *ms_frame = h->h.L.ms_frame;
     if (syncs) *syncs = h->h.L.syncs;
+    return 0;
+}
+int sf_lat_subphase_ms(sf_lat* h, double* out16) {
+    if (!h || !out16) return SF_E_ARG;
+    for (int k = 0; k < 16; k++) out16[k] = h->h.L.sp[k];
     return 0;
 }
 int sf_lat_set_fused(sf_lat* h, int max_contacts) {

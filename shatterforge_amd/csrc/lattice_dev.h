@@ -37,6 +37,8 @@ struct SceneK {
 
 struct LD {
     uint32_t nb, nbd, ndyn, nscene, nst, K, R, ccap, tmask;
+    uint32_t nub;              // body slots in use: every scene's slots below its high-water mark
+    uint32_t* ubody;
     // bricks
     uint8_t* mat;
     uint8_t* sid;
@@ -116,6 +118,7 @@ struct LD {
     uint8_t* bd_live;
     float* bd_dt;
     uint8_t* bd_dirty;
+    uint8_t* bdflag;           // bd_dirty of the used slots, in list order, for the host
     // events
     uint32_t* ctr;
     uint32_t* plist;           // plough candidates

@@ -132,6 +132,14 @@ class Lattice:
         check(lib().sf_lat_phase_ms(self._h, ptr(out)))
         return dict(zip(["substeps", "plough", "impact", "topology", "structural", "warmup", "stats", "frame"], out.tolist()))
 
+    def subphase_ms(self):
+        """finer split of the last frame's host work (ms; "scenes" is a count)"""
+        out = np.zeros(16, np.float64)
+        check(lib().sf_lat_subphase_ms(self._h, ptr(out)))
+        keys = ["topo_pull", "topo_rebuild", "topo_conn", "topo_flush", "topo_csr", "topo_upload", "topo_scenes",
+                "st_upload", "st_launch", "st_wait", "st_apply", "st_flush"]
+        return dict(zip(keys, out.tolist()))
+
     def set_fused(self, max_contacts):
         """scenes of at most max_contacts contacts are solved by one workgroup each; 0: colour-by-colour launches"""
         check(lib().sf_lat_set_fused(self._h, int(max_contacts)))
