@@ -119,55 +119,75 @@ __global__ void __launch_bounds__(NT) k_hash_fill(LD D) {
 
 // ---------------------------------------------------------------- segmented sort (ascending u32 values)
 // segments of at most a wavefront: bitonic within one wave; longer ones: k_segsort_big
+// segments of 2-64 values: each wave takes 64 consecutive segments, reads their sizes at once and sorts the ones in
+// range one after another, a bitonic network across the lanes
 __global__ void __launch_bounds__(NT) k_segsort(const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    for (uint32_t s = wave; s < nseg; s += nwaves) {
-        uint32_t o = off[s], n = cnt[s];
-        if (n < 2 || n > 64) continue;
-        uint32_t v = (uint32_t)lane < n ? val[o + lane] : 0xFFFFFFFFu;
-        for (int k = 2; k <= 64; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                uint32_t u = __shfl_xor(v, j, 64);
-                bool up = (lane & k) == 0, lower = (lane & j) == 0;
-                v = (lower == up) ? min(v, u) : max(v, u);
-            }
-        if ((uint32_t)lane < n) val[o + lane] = v;
+    for (uint32_t base = wave * 64; base < nseg; base += nwaves * 64) {
+        uint32_t s = base + lane;
+        uint32_t my_n = s < nseg ? cnt[s] : 0u, my_o = s < nseg ? off[s] : 0u;
+        unsigned long long todo = __ballot(my_n >= 2u && my_n <= 64u);
+        while (todo) {
+            int l = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            uint32_t o = __shfl(my_o, l, 64), n = __shfl(my_n, l, 64);
+            uint32_t v = (uint32_t)lane < n ? val[o + lane] : 0xFFFFFFFFu;
+            for (int k = 2; k <= 64; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    uint32_t u = __shfl_xor(v, j, 64);
+                    bool up = (lane & k) == 0, lower = (lane & j) == 0;
+                    v = (lower == up) ? min(v, u) : max(v, u);
+                }
+            if ((uint32_t)lane < n) val[o + lane] = v;
+        }
     }
 }
+// segments of more than 64 values: each block reads the sizes of NT consecutive segments at once, lists the large
+// ones in LDS and sorts them one by one (bitonic in LDS up to 4096 values, insertion beyond)
 __global__ void __launch_bounds__(NT) k_segsort_big(const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
     __shared__ uint32_t sh[4096];
-    for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-        uint32_t o = off[s], n = cnt[s];
-        if (n <= 64) continue;
-        if (n > 4096) {
-            if (threadIdx.x == 0)
-                for (uint32_t i = 1; i < n; i++) {
-                    uint32_t v = val[o + i], j = i;
-                    while (j > 0 && val[o + j - 1] > v) { val[o + j] = val[o + j - 1]; j--; }
-                    val[o + j] = v;
-                }
-            __syncthreads();
-            continue;
-        }
-        uint32_t n2 = 128;
-        while (n2 < n) n2 <<= 1;
-        for (uint32_t i = threadIdx.x; i < n2; i += NT) sh[i] = i < n ? val[o + i] : 0xFFFFFFFFu;
+    __shared__ uint32_t big[NT];
+    __shared__ uint32_t nbig;
+    for (uint32_t base = blockIdx.x * NT; base < nseg; base += gridDim.x * NT) {
+        if (threadIdx.x == 0) nbig = 0u;
         __syncthreads();
-        for (uint32_t k = 2; k <= n2; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < n2; i += NT) {
-                    uint32_t l = i ^ j;
-                    if (l > i) {
-                        uint32_t x = sh[i], y = sh[l];
-                        bool up = (i & k) == 0;
-                        if ((x > y) == up) { sh[i] = y; sh[l] = x; }
+        uint32_t s0 = base + threadIdx.x;
+        if (s0 < nseg && cnt[s0] > 64u) big[atomicAdd(&nbig, 1u)] = s0;
+        __syncthreads();
+        const uint32_t nb = nbig;
+        for (uint32_t q = 0; q < nb; q++) {
+            uint32_t s = big[q];
+            uint32_t o = off[s], n = cnt[s];
+            if (n > 4096) {
+                if (threadIdx.x == 0)
+                    for (uint32_t i = 1; i < n; i++) {
+                        uint32_t v = val[o + i], j = i;
+                        while (j > 0 && val[o + j - 1] > v) { val[o + j] = val[o + j - 1]; j--; }
+                        val[o + j] = v;
                     }
-                }
                 __syncthreads();
+                continue;
             }
-        for (uint32_t i = threadIdx.x; i < n; i += NT) val[o + i] = sh[i];
-        __syncthreads();
+            uint32_t n2 = 128;
+            while (n2 < n) n2 <<= 1;
+            for (uint32_t i = threadIdx.x; i < n2; i += NT) sh[i] = i < n ? val[o + i] : 0xFFFFFFFFu;
+            __syncthreads();
+            for (uint32_t k = 2; k <= n2; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t i = threadIdx.x; i < n2; i += NT) {
+                        uint32_t l = i ^ j;
+                        if (l > i) {
+                            uint32_t x = sh[i], y = sh[l];
+                            bool up = (i & k) == 0;
+                            if ((x > y) == up) { sh[i] = y; sh[l] = x; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            for (uint32_t i = threadIdx.x; i < n; i += NT) val[o + i] = sh[i];
+            __syncthreads();
+        }
     }
 }
 
@@ -2448,8 +2468,8 @@ int scan(Lat* L, const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t n)
 }
 void segsort(Lat* L, const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
     if (!nseg) return;
-    LAUNCH(k_segsort, sf_grid((uint64_t)nseg * 64, NT, 65536), NT, off, cnt, val, nseg);
-    LAUNCH(k_segsort_big, std::min<uint32_t>(nseg, 1024), NT, off, cnt, val, nseg);
+    LAUNCH(k_segsort, sf_grid(nseg, NT, 16384), NT, off, cnt, val, nseg);
+    LAUNCH(k_segsort_big, sf_grid(nseg, NT, 4096), NT, off, cnt, val, nseg);
 }
 
 int substep(Lat* L, float h) {
