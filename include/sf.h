@@ -135,6 +135,13 @@ int sf_load_add(sf_load* L, int n, const uint8_t* mat, const int32_t* xyz, const
  * refinement). omega <= 0 picks the over-relaxation per structure. Replaces sor_struct / warm_struct
  * (src/lib.rs:1029-1079). Returns the number of structures that converged. */
 int sf_load_solve(sf_load* L, float omega, double rel_tol, int max_sweeps);
+/* the same solve split so that it overlaps other work: begin launches it on the handle's own stream and returns;
+ * ready is 1 once it has finished (0 before); end waits for it and returns what sf_load_solve returns. Field and
+ * verdicts are readable after end. (No counterpart in the reference: a serving loop admits queries while the
+ * scenes already in flight keep stepping.) */
+int sf_load_solve_begin(sf_load* L, float omega, double rel_tol, int max_sweeps);
+int sf_load_solve_ready(sf_load* L);
+int sf_load_solve_end(sf_load* L);
 /* per structure: sweeps done, final relative residual, grounded bricks */
 int sf_load_info(sf_load* L, int sid, int32_t* sweeps, double* resid, int32_t* grounded);
 /* the FP64 field of a structure (n doubles) */

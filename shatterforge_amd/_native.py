@@ -55,6 +55,9 @@ _PROTOS = {
     "sf_load_destroy": (None, [c_void_p]),
     "sf_load_add": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_load_solve": (c_int, [c_void_p, c_float, c_double, c_int]),
+    "sf_load_solve_begin": (c_int, [c_void_p, c_float, c_double, c_int]),
+    "sf_load_solve_ready": (c_int, [c_void_p]),
+    "sf_load_solve_end": (c_int, [c_void_p]),
     "sf_load_info": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "sf_load_field": (c_int, [c_void_p, c_int, c_void_p]),
     "sf_load_verdicts": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

@@ -1465,18 +1465,23 @@ __global__ void k_disable(LD D, const int2* kj, uint32_t n, uint32_t c_end) {
     }
 }
 
+// a device buffer zero-filled on the world's stream. Growing it keeps the old block until release: freeing memory
+// waits for the whole device, including other streams' work (a load-path solve running beside the frame).
 template <class T> struct DBuf {
     T* p = nullptr; size_t n = 0;
-    int alloc(size_t cnt) {
-        if (p) (void)hipFree(p);
+    std::vector<void*> old;
+    int alloc(size_t cnt, hipStream_t st) {
+        if (p) old.push_back((void*)p);
         p = nullptr; n = cnt;
         if (!cnt) cnt = 1;
-        if (hipMalloc(&p, sizeof(T) * cnt) != hipSuccess) { p = nullptr; return -1; }
-        // the zero fill runs on the null stream, which the world's non-blocking stream does not wait for
-        if (hipMemset(p, 0, sizeof(T) * cnt) != hipSuccess) return -1;
-        return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
+        if (hipMalloc(&p, sizeof(T) * cnt) != hipSuccess) { p = nullptr; n = 0; sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
+        return hipMemsetAsync(p, 0, sizeof(T) * cnt, st) == hipSuccess ? 0 : -1;
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    void release() {
+        if (p) (void)hipFree(p);
+        for (void* q : old) (void)hipFree(q);
+        old.clear(); p = nullptr; n = 0;
+    }
 };
 
 struct BrickDelta { uint32_t i; int32_t body; uint32_t mat; uint32_t full; float4 bond, local, wpos; float phi, ext, extn, pad2; };
@@ -1587,13 +1592,10 @@ struct Lat {
         p = nullptr;
         if (hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)) != hipSuccess) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
         owned.push_back((void*)p);
-        if (hipMemset(p, 0, sizeof(T) * (n ? n : 1)) != hipSuccess) return -1;
-        return 0;
+        return hipMemsetAsync(p, 0, sizeof(T) * (n ? n : 1), st) == hipSuccess ? 0 : -1;
     }
-    template <class T> void dfree(T*& p) {
-        for (auto& o : owned) if (o == (void*)p) { (void)hipFree(o); o = nullptr; }
-        p = nullptr;
-    }
+    // a replaced array stays allocated (in `owned`) until the world is destroyed: freeing waits for the whole device
+    template <class T> void dfree(T*& p) { p = nullptr; }
 };
 
 int lat_alloc(Lat* L) {
@@ -1633,11 +1635,9 @@ int lat_alloc(Lat* L) {
     D.evcap = (uint32_t)std::max<size_t>(4096, B); D.bwcap = (uint32_t)std::max<size_t>(4096, 4 * B);
     r |= L->dalloc(D.evkey, D.evcap); r |= L->dalloc(D.ev, D.evcap); r |= L->dalloc(D.bwi, D.bwcap); r |= L->dalloc(D.bwv, D.bwcap);
     r |= L->dalloc(D.scan_tmp, sf::scan_scratch_words((uint32_t)std::max<size_t>(std::max(2 * B + 4096, N), 1u << 16)));
-    r |= L->d_sc.alloc(S); r |= L->d_scstat.alloc(3 * S); r |= L->d_slist.alloc(S); r |= L->d_rec.alloc(D.pcap); r |= L->d_dis.alloc(D.pcap);
+    r |= L->d_sc.alloc(S, L->st); r |= L->d_scstat.alloc(3 * S, L->st); r |= L->d_slist.alloc(S, L->st); r |= L->d_rec.alloc(D.pcap, L->st); r |= L->d_dis.alloc(D.pcap, L->st);
     D.sc = L->d_sc.p;
     D.K = (uint32_t)L->K; D.R = (uint32_t)L->R;
-    // the zero fills above ran on the null stream, which the world's non-blocking stream does not wait for
-    if (hipDeviceSynchronize() != hipSuccess) r = -1;
     return r ? -1 : 0;
 }
 
@@ -1662,7 +1662,6 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     r |= L->dalloc(D.odv, 2 * (size_t)cap); r |= L->dalloc(D.odw, 2 * (size_t)cap);
     D.ccap = cap;
     L->ccap = cap;
-    if (hipDeviceSynchronize() != hipSuccess) r = -1;
     return r ? -1 : 0;
 }
 int lat_hash_alloc(Lat* L, uint32_t ndyn) {
@@ -1675,7 +1674,6 @@ int lat_hash_alloc(Lat* L, uint32_t ndyn) {
     r |= L->dalloc(D.hcnt, t); r |= L->dalloc(D.hoff, t); r |= L->dalloc(D.hfill, t);
     L->tsize = t;
     D.tmask = t - 1;
-    if (hipDeviceSynchronize() != hipSuccess) r = -1;
     return r ? -1 : 0;
 }
 
@@ -2142,7 +2140,7 @@ int rebuild_csr(Lat* L) {
     L->ndyn = k;
     L->D.ndyn = k;
     if (lat_hash_alloc(L, k)) return -1;
-    if (L->d_csr.n < csr.size() && L->d_csr.alloc(std::max<size_t>(csr.size(), 4096))) return -1;
+    if (L->d_csr.n < csr.size() && L->d_csr.alloc(std::max<size_t>(csr.size(), 4096), L->st)) return -1;
     hipStream_t s = L->st;
     int r = 0;
     r |= cp_up(L->D.dyn, L->dyn.data(), k, s); r |= cp_up(L->D.dyn_body, L->dyn_body.data(), k, s);
@@ -2264,7 +2262,7 @@ int pull_bodies(Lat* L, const std::vector<int>& scenes) {
     for (int si : scenes) n += (size_t)L->sc[si].hw;
     if (!n) return 0;
     if (L->h_idx.reserve(n) || L->h_brec.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-    if ((L->d_idx.n < n && L->d_idx.alloc(std::max<size_t>(n, 4096))) || (L->d_brec.n < n && L->d_brec.alloc(std::max<size_t>(n, 4096)))) {
+    if ((L->d_idx.n < n && L->d_idx.alloc(std::max<size_t>(n, 4096), L->st)) || (L->d_brec.n < n && L->d_brec.alloc(std::max<size_t>(n, 4096), L->st))) {
         sf::set_error(SF_E_HIP, "hipMalloc failed");
         return -1;
     }
@@ -2294,7 +2292,7 @@ int pull_body_wpos(Lat* L, const std::vector<int>& bodies) {
         for (uint32_t i = L->first[b]; i != NONE; i = L->next[i]) list[n++] = i;
     if (!n) return 0;
     if (L->h_idx.reserve(n) || L->h_wp.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-    if ((L->d_idx.n < n && L->d_idx.alloc(std::max<size_t>(n, 4096))) || (L->d_wp.n < n && L->d_wp.alloc(std::max<size_t>(n, 4096)))) {
+    if ((L->d_idx.n < n && L->d_idx.alloc(std::max<size_t>(n, 4096), L->st)) || (L->d_wp.n < n && L->d_wp.alloc(std::max<size_t>(n, 4096), L->st))) {
         sf::set_error(SF_E_HIP, "hipMalloc failed");
         return -1;
     }
@@ -2312,7 +2310,7 @@ int flush_delta(Lat* L) {
     size_t n = L->tlist.size(), nb = L->btlist.size();
     if (n) {
         if (L->h_delta.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-        if (L->d_delta.n < n && L->d_delta.alloc(std::max<size_t>(n, 4096))) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
+        if (L->d_delta.n < n && L->d_delta.alloc(std::max<size_t>(n, 4096), L->st)) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
         for (size_t t = 0; t < n; t++) {
             uint32_t i = L->tlist[t];
             BrickDelta& x = L->h_delta.p[t];
@@ -2326,7 +2324,7 @@ int flush_delta(Lat* L) {
     }
     if (nb) {
         if (L->h_brec.reserve(nb)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-        if (L->d_brec.n < nb && L->d_brec.alloc(std::max<size_t>(nb, 4096))) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
+        if (L->d_brec.n < nb && L->d_brec.alloc(std::max<size_t>(nb, 4096), L->st)) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
         for (size_t t = 0; t < nb; t++) {
             uint32_t b = L->btlist[t];
             BodyRec& x = L->h_brec.p[t];
@@ -2955,8 +2953,13 @@ int finalize(Lat* L, int si) {
     if (r) return -1;
     SF_HIP(hipStreamSynchronize(s));
     // the grid: all scenes' regions
-    if (L->d_grid.n < L->grid.size()) { if (L->d_grid.alloc(L->grid.size() + L->grid.size() / 2)) return -1; }
-    SF_HIP(hipMemcpy(L->d_grid.p, L->grid.data(), sizeof(int32_t) * L->grid.size(), hipMemcpyHostToDevice));
+    if (L->d_grid.n < L->grid.size()) {
+        if (L->d_grid.alloc(L->grid.size() + L->grid.size() / 2, L->st)) return -1;
+        r |= cp_up(L->d_grid.p, L->grid.data(), L->grid.size(), s);
+    } else {
+        r |= cp_up(L->d_grid.p + S.goff, L->grid.data() + S.goff, S.gcap, s);
+    }
+    SF_HIP(hipStreamSynchronize(s));
     D.grid = L->d_grid.p;
     std::vector<int32_t> bsc(S.dcap, si);
     r |= cp_up(D.bd_scene + S.bd0, bsc.data(), S.dcap, s);
@@ -3071,7 +3074,7 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     SF_HIP_NULL(hipGetDevice(&L->dev));
     L->bcap = brick_cap; L->dcap = body_cap; L->scap = max_scenes; L->K = colours; L->R = rounds;
     if (hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) != hipSuccess) { delete h; sf::set_error(SF_E_HIP, "stream"); return nullptr; }
-    if (lat_alloc(L) || lat_contacts_alloc(L, (uint32_t)std::max<int>(1 << 16, 4 * brick_cap)) || lat_hash_alloc(L, 1024) || L->d_grid.alloc(1)) { delete h; return nullptr; }
+    if (lat_alloc(L) || lat_contacts_alloc(L, (uint32_t)std::max<int>(1 << 16, 4 * brick_cap)) || lat_hash_alloc(L, 1024) || L->d_grid.alloc(1, L->st)) { delete h; return nullptr; }
     L->D.grid = L->d_grid.p;
     size_t B = brick_cap, N = body_cap;
     L->mat.assign(B, 0); L->sid.assign(B, 255); L->infill.assign(B, 0); L->body.assign(B, -1); L->bxyz.assign(B, make_int4(0, 0, 0, 0));

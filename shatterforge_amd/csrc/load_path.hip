@@ -66,6 +66,7 @@ struct LoadDev {
     size_t N;
     double tol;
     int inner;             // FP32 sweeps between FP64 residual evaluations
+    int lds_floats;        // structures of up to this many bricks keep their correction in LDS
 };
 
 __device__ __forceinline__ double block_max(double v, double* sh) {
@@ -79,15 +80,26 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return m;
 }
 
-// the whole solve of one structure per workgroup
-__global__ void __launch_bounds__(LT) k_load_solve(LoadDev L) {
-    __shared__ double sh[LT / 64];
-    const int s = blockIdx.x;
-    if (s >= L.nstruct) return;
+// the FP32 correction of a structure: in LDS when it fits (neighbour reads then cost no memory round trip and the
+// stencil loads, which cannot alias it, overlap), else in global memory
+template <bool LDSD> struct Corr;
+template <> struct Corr<true> {
+    float* p; size_t base;
+    __device__ __forceinline__ float& operator()(size_t g) const { return p[(uint32_t)(g - base)]; }
+};
+template <> struct Corr<false> {
+    float* p; size_t base;
+    __device__ __forceinline__ float& operator()(size_t g) const { return p[g]; }
+};
+
+// the whole solve of one structure by one workgroup
+template <bool LDSD>
+__device__ void solve_struct(const LoadDev& L, int s, float* lds, double* sh) {
     const DevInfo I = L.info[s];
     const int base = I.base, n = I.n, nred = I.nred;
     const size_t N = L.N;
     const float om = I.omega;
+    const Corr<LDSD> del{LDSD ? lds : L.delta, (size_t)base};
     int sweeps = 0;
     double rel = 1.0;
     while (true) {
@@ -105,7 +117,7 @@ __global__ void __launch_bounds__(LT) k_load_solve(LoadDev L) {
                 r = (double)L.b[g] - acc;
             }
             L.r32[g] = (float)r;
-            L.delta[g] = 0.0f;
+            del(g) = 0.0f;
             double a = fabs(r);
             rmax = a > rmax ? a : rmax;
         }
@@ -116,27 +128,52 @@ __global__ void __launch_bounds__(LT) k_load_solve(LoadDev L) {
         for (int it = 0; it < L.inner && sweeps < I.max_sweeps; it++, sweeps++) {
             for (int half = 0; half < 2; half++) {
                 int i0 = half ? nred : 0, i1 = half ? n : nred;
-                for (int i = i0 + threadIdx.x; i < i1; i += LT) {
-                    size_t g = (size_t)base + i;
-                    if (!L.act[g]) continue;
-                    float skp = 0.0f;
-                    for (int d = 0; d < 6; d++) {
-                        int j = L.nbr[d * N + g];
-                        if (j >= 0) skp += L.kc[d * N + g] * L.delta[j];
+                // U bricks of one colour per thread at a time: their stencils are loaded together, then the
+                // neighbours (all of the other colour, so no brick of this batch is read after it is written)
+                constexpr int U = 4;
+                for (int i = i0 + threadIdx.x; i < i1; i += LT * U) {
+                    int jj[U][6];
+                    float kk[U][6], rr[U], ss[U];
+                    bool on[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        int ii = i + u * LT;
+                        size_t g = (size_t)base + ii;
+                        on[u] = ii < i1 && L.act[g];
+#pragma unroll
+                        for (int d = 0; d < 6; d++) { jj[u][d] = on[u] ? L.nbr[d * N + g] : -1; kk[u][d] = on[u] ? L.kc[d * N + g] : 0.0f; }
+                        rr[u] = on[u] ? L.r32[g] : 0.0f;
+                        ss[u] = on[u] ? L.sk32[g] : 1.0f;
                     }
-                    float target = (L.r32[g] + skp) / L.sk32[g];
-                    L.delta[g] += om * (target - L.delta[g]);
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (!on[u]) continue;
+                        size_t g = (size_t)base + i + u * LT;
+                        float skp = 0.0f;
+#pragma unroll
+                        for (int d = 0; d < 6; d++) if (jj[u][d] >= 0) skp += kk[u][d] * del((size_t)jj[u][d]);
+                        float target = (rr[u] + skp) / ss[u];
+                        del(g) += om * (target - del(g));
+                    }
                 }
                 __syncthreads();
             }
         }
         for (int i = threadIdx.x; i < n; i += LT) {
             size_t g = (size_t)base + i;
-            L.phi[g] += (double)L.delta[g];
+            L.phi[g] += (double)del(g);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) { L.out_sweeps[s] = sweeps; L.out_resid[s] = rel; }
+}
+__global__ void __launch_bounds__(LT) k_load_solve(LoadDev L) {
+    __shared__ double sh[LT / 64];
+    extern __shared__ float lds[];
+    const int s = blockIdx.x;
+    if (s >= L.nstruct) return;
+    if (L.info[s].n <= L.lds_floats) solve_struct<true>(L, s, lds, sh);
+    else solve_struct<false>(L, s, lds, sh);
 }
 
 struct Verd { double* ratio; uint8_t* mode; uint8_t* fail; };
@@ -233,7 +270,20 @@ struct HostLoad {
     float last_ms = 0.0f;
     int64_t last_sweeps32 = 0, last_sweeps64 = 0;
     std::vector<void*> allocs;
-    ~HostLoad() { free_dev(); }
+    // a solve in flight on the handle's own stream (sf_load_solve_begin ... sf_load_solve_end)
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool pending = false;
+    double tol = 0.0;
+    int inner = 32;
+    std::vector<DevInfo> hinfo;
+    ~HostLoad() {
+        if (pending && e1) (void)hipEventSynchronize(e1);
+        free_dev();
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
     void free_dev() { for (void* p : allocs) (void)hipFree(p); allocs.clear(); }
     template <class T> int alloc(T*& p, size_t n) {
         void* q = nullptr;
@@ -311,7 +361,6 @@ int upload(HostLoad* L) {
     SF_HIP(hipMemcpy(L->foot, foot.data(), N, hipMemcpyHostToDevice));
     SF_HIP(hipMemcpy(L->hp, hp.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice));
     SF_HIP(hipMemcpy(L->gk, gk.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemset(L->phi, 0, sizeof(double) * N));
     SF_HIP(hipMemcpy(L->info, info.data(), sizeof(DevInfo) * info.size(), hipMemcpyHostToDevice));
     L->dirty = false;
     return 0;
@@ -357,38 +406,74 @@ int sf_load_add(sf_load* L, int n, const uint8_t* mat, const int32_t* xyz, const
     return (int)L->h.st.size() - 1;
 }
 
-int sf_load_solve(sf_load* L, float omega, double rel_tol, int max_sweeps) {
+int sf_load_solve_begin(sf_load* L, float omega, double rel_tol, int max_sweeps) {
     if (!L || !(rel_tol > 0.0) || max_sweeps <= 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
     HostLoad* H = &L->h;
+    if (H->pending) { int r = sf_load_solve_end(L); if (r < 0) return r; }
+    H->verdicts_fresh = false;
     if (H->st.empty()) return 0;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { sf::set_error(SF_E_NODEV, "no GPU visible"); return SF_E_NODEV; }
     SF_HIP(hipSetDevice(H->dev));
+    if (!H->stream) {
+        SF_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
+        SF_HIP(hipEventCreate(&H->e0)); SF_HIP(hipEventCreate(&H->e1));
+    }
     if (omega > 0.0f) for (auto& S : H->st) S.omega = omega;
     if (H->dirty) { int r = upload(H); if (r) return r; }
-    std::vector<DevInfo> info(H->st.size());
+    H->hinfo.resize(H->st.size());
     for (size_t s = 0; s < H->st.size(); s++) {
         const Struct& S = H->st[s];
         double bmax = 0.0;
         for (int i = 0; i < S.n; i++) if (S.grounded[i]) bmax = std::max(bmax, (double)std::fabs(MASS[S.mat[i]] + (S.ext.empty() ? 0.0f : S.ext[i])));
-        info[s] = DevInfo{S.base, S.n, S.nred, max_sweeps, S.omega, bmax};
+        H->hinfo[s] = DevInfo{S.base, S.n, S.nred, max_sweeps, S.omega, bmax};
     }
-    SF_HIP(hipMemcpy(H->info, info.data(), sizeof(DevInfo) * info.size(), hipMemcpyHostToDevice));
-    SF_HIP(hipMemset(H->phi, 0, sizeof(double) * H->N));
+    SF_HIP(hipMemcpy(H->info, H->hinfo.data(), sizeof(DevInfo) * H->hinfo.size(), hipMemcpyHostToDevice));
+    SF_HIP(hipMemsetAsync(H->phi, 0, sizeof(double) * H->N, H->stream));
     LoadDev D;
     D.nstruct = (int)H->st.size(); D.info = H->info; D.nbr = H->nbr; D.kc = H->kc; D.sk32 = H->sk32; D.sk64 = H->sk64;
     D.b = H->b; D.act = H->act; D.phi = H->phi; D.delta = H->delta; D.r32 = H->r32; D.out_sweeps = H->osw;
-    D.out_resid = H->ores; D.N = H->N; D.tol = rel_tol; D.inner = 32;
-    hipEvent_t e0, e1;
-    SF_HIP(hipEventCreate(&e0)); SF_HIP(hipEventCreate(&e1));
-    SF_HIP(hipEventRecord(e0, 0));
-    hipLaunchKernelGGL(k_load_solve, dim3(D.nstruct), dim3(LT), 0, 0, D);
+    D.out_resid = H->ores; D.N = H->N; D.tol = rel_tol; D.inner = H->inner;
+    H->tol = rel_tol;
+    // dynamic LDS for the corrections: the largest structure that fits the workgroup limit
+    int lds_max = 0;
+    SF_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, H->dev));
+    int fit = std::max(0, (lds_max - 1024) / (int)sizeof(float)), nmax = 0;
+    for (const Struct& S : H->st) if (S.n <= fit) nmax = std::max(nmax, S.n);
+    D.lds_floats = nmax;
+    size_t lds_bytes = sizeof(float) * (size_t)nmax;
+    if (lds_bytes > 65536) (void)hipFuncSetAttribute((const void*)k_load_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    SF_HIP(hipEventRecord(H->e0, H->stream));
+    hipLaunchKernelGGL(k_load_solve, dim3(D.nstruct), dim3(LT), lds_bytes, H->stream, D);
     SF_HIP(hipGetLastError());
-    SF_HIP(hipEventRecord(e1, 0));
-    SF_HIP(hipEventSynchronize(e1));
+    // verdicts from the converged field
+    Verd V{H->vratio, H->vmode, H->vfail};
+    hipLaunchKernelGGL(k_verdicts, dim3(sf_grid(H->N)), dim3(256), 0, H->stream, H->nbr, H->kc, H->hp, H->mat, H->foot, H->phi, H->gk, H->N, V);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipEventRecord(H->e1, H->stream));
+    H->pending = true;
+    return 0;
+}
+
+int sf_load_solve_ready(sf_load* L) {
+    if (!L) return SF_E_ARG;
+    HostLoad* H = &L->h;
+    if (!H->pending) return 1;
+    hipError_t e = hipEventQuery(H->e1);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    sf::set_error(SF_E_HIP, hipGetErrorString(e));
+    return SF_E_HIP;
+}
+
+int sf_load_solve_end(sf_load* L) {
+    if (!L) return SF_E_ARG;
+    HostLoad* H = &L->h;
+    if (!H->pending) return H->verdicts_fresh ? (int)std::count_if(H->st.begin(), H->st.end(), [&](const Struct& S) { return S.resid <= H->tol; }) : 0;
+    H->pending = false;
+    SF_HIP(hipEventSynchronize(H->e1));
     float ms = 0.0f;
-    SF_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    SF_HIP(hipEventElapsedTime(&ms, H->e0, H->e1));
     H->last_ms = ms;
     std::vector<int> sw(H->st.size());
     std::vector<double> rs(H->st.size());
@@ -398,17 +483,18 @@ int sf_load_solve(sf_load* L, float omega, double rel_tol, int max_sweeps) {
     H->last_sweeps32 = 0; H->last_sweeps64 = 0;
     for (size_t s = 0; s < H->st.size(); s++) {
         H->st[s].sweeps = sw[s]; H->st[s].resid = rs[s];
-        if (rs[s] <= rel_tol) ok++;
+        if (rs[s] <= H->tol) ok++;
         H->last_sweeps32 += (int64_t)sw[s] * H->st[s].n;
-        H->last_sweeps64 += (int64_t)((sw[s] + D.inner - 1) / D.inner + 1) * H->st[s].n;
+        H->last_sweeps64 += (int64_t)((sw[s] + H->inner - 1) / H->inner + 1) * H->st[s].n;
     }
-    // verdicts from the converged field
-    Verd V{H->vratio, H->vmode, H->vfail};
-    hipLaunchKernelGGL(k_verdicts, dim3(sf_grid(H->N)), dim3(256), 0, 0, H->nbr, H->kc, H->hp, H->mat, H->foot, H->phi, H->gk, H->N, V);
-    SF_HIP(hipGetLastError());
-    SF_HIP(hipDeviceSynchronize());
     H->verdicts_fresh = true;
     return ok;
+}
+
+int sf_load_solve(sf_load* L, float omega, double rel_tol, int max_sweeps) {
+    int r = sf_load_solve_begin(L, omega, rel_tol, max_sweeps);
+    if (r < 0) return r;
+    return sf_load_solve_end(L);
 }
 
 int sf_load_info(sf_load* L, int sid, int32_t* sweeps, double* resid, int32_t* grounded_out) {
@@ -421,7 +507,7 @@ int sf_load_info(sf_load* L, int sid, int32_t* sweeps, double* resid, int32_t* g
 }
 
 int sf_load_field(sf_load* L, int sid, double* phi) {
-    if (!L || !phi || sid < 0 || sid >= (int)L->h.st.size() || L->h.dirty) { sf::set_error(SF_E_ARG, "bad structure id or not solved"); return SF_E_ARG; }
+    if (!L || !phi || sid < 0 || sid >= (int)L->h.st.size() || L->h.dirty || L->h.pending) { sf::set_error(SF_E_ARG, "bad structure id or not solved"); return SF_E_ARG; }
     const Struct& S = L->h.st[sid];
     std::vector<double> d(S.n);
     SF_HIP(hipMemcpy(d.data(), L->h.phi + S.base, sizeof(double) * S.n, hipMemcpyDeviceToHost));

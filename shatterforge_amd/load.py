@@ -44,6 +44,18 @@ class LoadBatch:
         of structures that converged."""
         return check(lib().sf_load_solve(self._h, float(omega), float(rel_tol), int(max_sweeps)))
 
+    def solve_begin(self, rel_tol=1e-6, omega=0.0, max_sweeps=200000):
+        """start the same solve on the batch's own stream and return at once"""
+        check(lib().sf_load_solve_begin(self._h, float(omega), float(rel_tol), int(max_sweeps)))
+
+    def ready(self):
+        """True once a started solve has finished"""
+        return check(lib().sf_load_solve_ready(self._h)) == 1
+
+    def solve_end(self):
+        """wait for a started solve; returns the number of structures that converged"""
+        return check(lib().sf_load_solve_end(self._h))
+
     def info(self, sid):
         sw = np.zeros(1, np.int32)
         rs = np.zeros(1, np.float64)

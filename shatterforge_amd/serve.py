@@ -83,8 +83,10 @@ def make_query(qid, seed, n_min=2000, n_max=40000):
 class WhatIfServer:
     """Continuous-batching server over `slots` resident scene slots of `slot_bricks` bricks each."""
 
-    def __init__(self, slots=64, slot_bricks=40960, slot_bodies=8192, frames=300, hbm_budget=None, dt=1.0 / 60.0):
+    def __init__(self, slots=64, slot_bricks=40960, slot_bodies=8192, frames=300, hbm_budget=None, dt=1.0 / 60.0,
+                 max_solving=3):
         self.frames = frames
+        self.max_solving = max_solving
         self.dt = dt
         self.slot_bricks = slot_bricks
         self.lat = Lattice(slots * slot_bricks, slots * slot_bodies, scenes=slots)
@@ -94,6 +96,7 @@ class WhatIfServer:
         self.in_use = 0
         self.live = {}          # slot -> query state
         self.queue = deque()    # admitted to the server, waiting for a slot
+        self.pending = []       # (slot and query list, load batch) whose load-path solve is running
         self.done = []
         self.brick_steps = 0
         self.frames_run = 0
@@ -108,7 +111,11 @@ class WhatIfServer:
         self.queue.append(q)
 
     def _admit(self, now):
-        """slots and HBM bytes permitting, queued queries enter the batch: scene, load-path solve, charges"""
+        """slots and HBM bytes permitting, queued queries take a slot and their load-path solves start on the load
+        batch's own stream; the scenes already in flight keep stepping meanwhile. At most `max_solving` solve
+        batches run at once (each on a stream of its own beside the frame's); later arrivals form the next batch"""
+        if len(self.pending) >= self.max_solving:
+            return
         batch = []
         while self.queue and self.free and self.in_use + self.query_bytes(self.queue[0]) <= self.budget:
             q = self.queue.popleft()
@@ -116,28 +123,38 @@ class WhatIfServer:
                 raise ValueError("query larger than a scene slot")
             s = self.free.popleft()
             self.in_use += self.query_bytes(q)
-            self.lat.clear_scene(s, q["rng_seed"])
-            q["structure"] = self.lat.add_structure(s, q["mat"], q["xyz"], q["infill"])
-            self.lat.finalize(s)
             batch.append((s, q))
         if not batch:
             return
         load = LoadBatch()
         for s, q in batch:
             load.add(q["mat"], q["xyz"], q["infill"])
-        load.solve(rel_tol=1e-6)
-        for k, (s, q) in enumerate(batch):
-            phi = load.field(k).astype(np.float32)
-            inf = load.info(k)
-            q["load_resid"] = inf["resid"]
-            q["load_sweeps"] = inf["sweeps"]
-            self.lat.set_field(s, q["structure"], phi)
-            for ch in q["charges"]:
-                self.lat.explode(s, *ch)
-            q["t_start"] = now
-            q["frame"] = 0
-            self.live[s] = q
-        load.close()
+        load.solve_begin(rel_tol=1e-6)
+        self.pending.append((batch, load))
+
+    def _activate(self, now, wait=False):
+        """queries whose load-path solve has finished join the batch: scene, converged field, charges"""
+        keep = []
+        for batch, load in self.pending:
+            if not (wait or load.ready()):
+                keep.append((batch, load))
+                continue
+            load.solve_end()
+            for k, (s, q) in enumerate(batch):
+                self.lat.clear_scene(s, q["rng_seed"])
+                q["structure"] = self.lat.add_structure(s, q["mat"], q["xyz"], q["infill"])
+                self.lat.finalize(s)
+                inf = load.info(k)
+                q["load_resid"] = inf["resid"]
+                q["load_sweeps"] = inf["sweeps"]
+                self.lat.set_field(s, q["structure"], load.field(k).astype(np.float32))
+                for ch in q["charges"]:
+                    self.lat.explode(s, *ch)
+                q["t_start"] = now
+                q["frame"] = 0
+                self.live[s] = q
+            load.close()
+        self.pending = keep
 
     def _retire(self, now):
         for s in [s for s, q in self.live.items() if q["frame"] >= self.frames]:
@@ -152,12 +169,19 @@ class WhatIfServer:
                 q.pop(k, None)
             self.done.append(q)
             self.in_use -= self.query_bytes(q)
+            self.lat.clear_scene(s, 1)  # an empty slot costs the batched frame nothing
             self.free.append(s)
 
+    def busy(self):
+        """queries waiting for a slot, solving their load path, or stepping"""
+        return bool(self.queue or self.pending or self.live)
+
     def tick(self):
-        """admit what can be admitted, advance every live scene one frame, retire finished queries"""
+        """admit what can be admitted, activate queries whose solve finished, advance every live scene one frame,
+        retire finished queries"""
         now = time.perf_counter()
         self._admit(now)
+        self._activate(now, wait=not self.live)
         if self.live:
             self.lat.step(self.dt)
             self.frames_run += 1
@@ -171,18 +195,21 @@ class WhatIfServer:
         """serve queries arriving at the given times (s, from now) in real time; returns the finished queries"""
         t0 = time.perf_counter()
         pending = deque(sorted(zip(arrivals, range(len(queries)))))
-        while pending or self.queue or self.live:
+        while pending or self.busy():
             now = time.perf_counter() - t0
             while pending and pending[0][0] <= now:
                 ta, i = pending.popleft()
                 self.submit(queries[i], t0 + ta)
-            if not self.live and not self.queue and pending:
+            if not self.busy() and pending:
                 time.sleep(max(0.0, min(pending[0][0] - now, 0.01)))
                 continue
             self.tick()
         return self.done
 
     def close(self):
+        for _, load in self.pending:
+            load.close()
+        self.pending = []
         self.lat.close()
 
 

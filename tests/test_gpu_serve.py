@@ -26,7 +26,7 @@ def test_whatif_same_alone_and_in_mix():
         srv.tick()
     for q in qs[2:]:
         srv.submit(q, 0.0)
-    while srv.queue or srv.live:
+    while srv.busy():
         srv.tick()
     mixed = _answers(srv.done)
     assert len(mixed) == 5
@@ -35,7 +35,7 @@ def test_whatif_same_alone_and_in_mix():
     for q in qs:
         alone = serve.WhatIfServer(slots=1, slot_bricks=6000, slot_bodies=2048, frames=90)
         alone.submit(q, 0.0)
-        while alone.queue or alone.live:
+        while alone.busy():
             alone.tick()
         a = _answers(alone.done)[q["id"]]
         alone.close()
