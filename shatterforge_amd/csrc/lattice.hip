@@ -201,89 +201,108 @@ __device__ __forceinline__ int static_brick(const LD& D, const SceneK& S, int x,
     return (D.mat[j] != 0 && D.body[j] < 0) ? j : -1;
 }
 
+// Contacts of one awake dynamic brick (gen_contacts, src/lib.rs:1396-1471) by one wavefront: lanes 0-7 test the
+// ground corners, lanes 8-34 the 27 lattice cells and lanes 35-61 the 27 hash cells, each in the sequential walk's
+// order; an exclusive prefix sum over the lanes' counts puts every contact where the sequential walk would.
+// WRITE = false counts (ccnt); WRITE = true counts again, then writes from coff with the side effects (support
+// stamps, wake requests, per-body contact counts).
 template <bool WRITE>
 __global__ void __launch_bounds__(NT) k_contacts(LD D) {
-    GS(k, D.ndyn) {
-        int a = D.dyn_body[k];
-        if (!D.alive[a] || D.asleep[a]) { if (!WRITE) D.ccnt[k] = 0; continue; }
-        uint32_t i = D.dyn[k];
-        int sc = D.bd_scene[a];
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t k = wave; k < D.ndyn; k += nwaves) {
+        const int a = D.dyn_body[k];
+        if (!D.alive[a] || D.asleep[a]) { if (!WRITE && lane == 0) D.ccnt[k] = 0; continue; }
+        const uint32_t i = D.dyn[k];
+        const int sc = D.bd_scene[a];
         const SceneK S = D.sc[sc];
-        uint32_t out = WRITE ? D.coff[k] : 0u, n = 0;
-        LM rot = load_m(D.rot + 3 * (size_t)a);
-        float speed = len(lv4(D.v[a])) + len(lv4(D.w[a])) * D.rad[a] * 0.5f;
-        LV p = lv4(D.wpos[i]);
-        auto push = [&](int b, int bo, uint32_t bb, LV cp, LV nn, float depth) {
-            if (WRITE) {
-                uint32_t c = out + n;
-                if (c < D.ccap) {
-                    D.ca[c] = a; D.cb[c] = b; D.cbo[c] = bo; D.cba[c] = i; D.cbb[c] = bb; D.cscene[c] = sc;
-                    D.cp[c] = f4(cp); D.cn[c] = f4(nn); D.cdepth[c] = depth;
-                    D.con[c] = 1; D.ccol[c] = -1; D.cjn[c] = 0.0f; D.cjt1[c] = 0.0f; D.cjt2[c] = 0.0f; D.cvn0[c] = 0.0f;
+        const LV p = lv4(D.wpos[i]);
+        // one lane's probe: emit(b, bo, bb, point, normal, depth) for each contact in order; `side` also performs
+        // the stamps and wake requests
+        auto probe = [&](auto&& emit, bool side) {
+            if (lane < 8) {
+                if (p.y < 0.5f) {
+                    LM rot = load_m(D.rot + 3 * (size_t)a);
+                    int c = lane;
+                    LV l = lv((c & 1) == 0 ? -H : H, (c & 2) == 0 ? -H : H, (c & 4) == 0 ? -H : H);
+                    LV cp = add(p, mv(rot, l));
+                    if (cp.y < 0.0f) { emit(-1, -1, NONE, cp, lv(0.0f, 1.0f, 0.0f), -cp.y); if (side) D.sup[i] = S.frame; }
                 }
-                atomicAdd(&D.nc[a], 1u);
-                if (b >= 0) atomicAdd(&D.nc[b], 1u);
+            } else if (lane < 35) {
+                int c = lane - 8, dz = c / 9 - 1, dy = (c / 3) % 3 - 1, dx = c % 3 - 1;
+                int x = fcell(p.x) + NX2 + dx, y = fcell(p.y) + dy, z = fcell(p.z) + NX2 + dz;
+                int j = static_brick(D, S, x, y, z);
+                if (j >= 0) {
+                    LV cc = lat_pos(x, y, z);
+                    LV q = lv(clampf(p.x, cc.x - H, cc.x + H), clampf(p.y, cc.y - H, cc.y + H), clampf(p.z, cc.z - H, cc.z + H));
+                    LV d = sub(p, q);
+                    float dist = len(d);
+                    if (dist < RB) {
+                        if (dist > 1e-4f) {
+                            if (side && d.y > 0.3f * dist) D.sup[i] = S.frame;
+                            emit(-1, -1, (uint32_t)j, q, mul(d, 1.0f / dist), RB - dist);
+                        } else {
+                            LV r = sub(p, cc);
+                            float rv[3] = {r.x, r.y, r.z};
+                            int ax = 0;
+                            for (int t = 1; t < 3; t++) if (fabsf(rv[t]) > fabsf(rv[ax])) ax = t;
+                            float nv[3] = {0.0f, 0.0f, 0.0f};
+                            nv[ax] = rv[ax] >= 0.0f ? 1.0f : -1.0f;
+                            emit(-1, -1, (uint32_t)j, p, lv(nv[0], nv[1], nv[2]), H + RB - fabsf(rv[ax]));
+                        }
+                    }
+                }
+            } else if (lane < 62) {
+                int c = lane - 35, dz = c / 9 - 1, dy = (c / 3) % 3 - 1, dx = c % 3 - 1;
+                unsigned long long key = cell_key(sc, fcell(p.x) + dx, fcell(p.y) + dy, fcell(p.z) + dz);
+                uint32_t sl = key_slot(key, D.tmask);
+                uint32_t e0 = D.hoff[sl], e1 = e0 + D.hcnt[sl];
+                float speed = 0.0f;
+                if (side) speed = len(lv4(D.v[a])) + len(lv4(D.w[a])) * D.rad[a] * 0.5f;
+                for (uint32_t e = e0; e < e1; e++) {
+                    uint32_t k2 = D.hent[e];
+                    if (D.dkey[k2] != key) continue;
+                    uint32_t j = D.dyn[k2];
+                    int bj = D.body[j];
+                    if (bj == a || D.mat[j] == 0) continue;
+                    bool other_awake = !D.asleep[bj];
+                    if (other_awake && j < i) continue;
+                    LV d = sub(p, lv4(D.wpos[j]));
+                    float dd = dot(d, d);
+                    if (dd >= 4.0f * RB * RB || dd < 1e-10f) continue;
+                    float dist = sqrtf(dd);
+                    LV nn = mul(d, 1.0f / dist);
+                    LV cp = madd(lv4(D.wpos[j]), nn, dist * 0.5f);
+                    if (side) { if (nn.y > 0.3f) D.sup[i] = S.frame; else if (nn.y < -0.3f) D.sup[j] = S.frame; }
+                    if (other_awake) emit(bj, bj, j, cp, nn, 2.0f * RB - dist);
+                    else {
+                        if (side && speed > 0.7f) D.wake[bj] = 1;
+                        emit(-1, bj, j, cp, nn, 2.0f * RB - dist);
+                    }
+                }
             }
-            n++;
         };
-        if (p.y < 0.5f) {
-            for (int c = 0; c < 8; c++) {
-                LV l = lv((c & 1) == 0 ? -H : H, (c & 2) == 0 ? -H : H, (c & 4) == 0 ? -H : H);
-                LV cp = add(p, mv(rot, l));
-                if (cp.y < 0.0f) { push(-1, -1, NONE, cp, lv(0.0f, 1.0f, 0.0f), -cp.y); if (WRITE) D.sup[i] = S.frame; }
-            }
+        uint32_t n = 0;
+        probe([&](int, int, uint32_t, LV, LV, float) { n++; }, false);
+        // exclusive prefix over the lanes, and the wave's total
+        uint32_t x = n;
+        for (int d = 1; d < 64; d <<= 1) { uint32_t y = __shfl_up(x, d, 64); if (lane >= d) x += y; }
+        if (!WRITE) {
+            if (lane == 63) D.ccnt[k] = x;
+            continue;
         }
-        int cx = fcell(p.x) + NX2, cy = fcell(p.y), cz = fcell(p.z) + NX2;
-        for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
-            int x = cx + dx, y = cy + dy, z = cz + dz;
-            int j = static_brick(D, S, x, y, z);
-            if (j < 0) continue;
-            LV c = lat_pos(x, y, z);
-            LV q = lv(clampf(p.x, c.x - H, c.x + H), clampf(p.y, c.y - H, c.y + H), clampf(p.z, c.z - H, c.z + H));
-            LV d = sub(p, q);
-            float dist = len(d);
-            if (dist >= RB) continue;
-            if (dist > 1e-4f) {
-                if (WRITE && d.y > 0.3f * dist) D.sup[i] = S.frame;
-                push(-1, -1, (uint32_t)j, q, mul(d, 1.0f / dist), RB - dist);
-            } else {
-                LV r = sub(p, c);
-                float rv[3] = {r.x, r.y, r.z};
-                int ax = 0;
-                for (int t = 1; t < 3; t++) if (fabsf(rv[t]) > fabsf(rv[ax])) ax = t;
-                float nv[3] = {0.0f, 0.0f, 0.0f};
-                nv[ax] = rv[ax] >= 0.0f ? 1.0f : -1.0f;
-                push(-1, -1, (uint32_t)j, p, lv(nv[0], nv[1], nv[2]), H + RB - fabsf(rv[ax]));
+        uint32_t out = D.coff[k] + x - n, m = 0;
+        probe([&](int b, int bo, uint32_t bb, LV cp, LV nn, float depth) {
+            uint32_t c = out + m;
+            if (c < D.ccap) {
+                D.ca[c] = a; D.cb[c] = b; D.cbo[c] = bo; D.cba[c] = i; D.cbb[c] = bb; D.cscene[c] = sc;
+                D.cp[c] = f4(cp); D.cn[c] = f4(nn); D.cdepth[c] = depth;
+                D.con[c] = 1; D.ccol[c] = -1; D.cjn[c] = 0.0f; D.cjt1[c] = 0.0f; D.cjt2[c] = 0.0f; D.cvn0[c] = 0.0f;
             }
-        }
-        int hx = fcell(p.x), hy = fcell(p.y), hz = fcell(p.z);
-        for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
-            unsigned long long key = cell_key(sc, hx + dx, hy + dy, hz + dz);
-            uint32_t s = key_slot(key, D.tmask);
-            uint32_t e0 = D.hoff[s], e1 = e0 + D.hcnt[s];
-            for (uint32_t e = e0; e < e1; e++) {
-                uint32_t k2 = D.hent[e];
-                if (D.dkey[k2] != key) continue;
-                uint32_t j = D.dyn[k2];
-                int bj = D.body[j];
-                if (bj == a || D.mat[j] == 0) continue;
-                bool other_awake = !D.asleep[bj];
-                if (other_awake && j < i) continue;
-                LV d = sub(p, lv4(D.wpos[j]));
-                float dd = dot(d, d);
-                if (dd >= 4.0f * RB * RB || dd < 1e-10f) continue;
-                float dist = sqrtf(dd);
-                LV nn = mul(d, 1.0f / dist);
-                LV cp = madd(lv4(D.wpos[j]), nn, dist * 0.5f);
-                if (WRITE) { if (nn.y > 0.3f) D.sup[i] = S.frame; else if (nn.y < -0.3f) D.sup[j] = S.frame; }
-                if (other_awake) push(bj, bj, j, cp, nn, 2.0f * RB - dist);
-                else {
-                    if (WRITE && speed > 0.7f) D.wake[bj] = 1;
-                    push(-1, bj, j, cp, nn, 2.0f * RB - dist);
-                }
-            }
-        }
-        if (!WRITE) D.ccnt[k] = n;
+            atomicAdd(&D.nc[a], 1u);
+            if (b >= 0) atomicAdd(&D.nc[b], 1u);
+            m++;
+        }, true);
     }
 }
 // first contact of every scene (its first dynamic slot's offset)
@@ -921,84 +940,116 @@ __global__ void __launch_bounds__(NT) k_stencil(LD D, int mode) {
     }
 }
 // one brick's update from its stencil, in sor_brick's operation order (src/lib.rs:1005-1026)
-__device__ __forceinline__ float sten_update(LD& D, uint32_t p, float om, float& phi_out, bool& did) {
-    float2 sr = D.sten_s[p];
-    did = false;
-    if (sr.x < 0.0f) return 0.0f;
-    did = true;
-    int4 na = D.sten_na[p];
-    int2 nb = D.sten_nb[p];
-    float4 ka = D.sten_ka[p];
-    float2 kb = D.sten_kb[p];
-    uint32_t i = D.st_red[p];
-    int nbr[6] = {na.x, na.y, na.z, na.w, nb.x, nb.y};
-    float kk[6] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y};
-    float ph[6];
-#pragma unroll
-    for (int d = 0; d < 6; d++) ph[d] = nbr[d] >= 0 ? D.phi[nbr[d]] : 0.0f;
-    float skp = 0.0f;
-#pragma unroll
-    for (int d = 0; d < 6; d++) if (nbr[d] >= 0) skp += kk[d] * ph[d];
-    float pi = D.phi[i];
-    phi_out = pi;
-    if (sr.x > 0.0f) {
-        float target = (sr.y + skp) / sr.x;
-        float d = om * (target - pi);
-        phi_out = pi + d;
-        D.phi[i] = phi_out;
-        return fabsf(d);
-    }
-    return 0.0f;
-}
-__device__ float sweep_struct(LD& D, uint32_t g, int sl, float om, uint32_t frame, float* sh) {
+// The potential of the structure being swept: in LDS when the structure fits (the workgroup copies it in before its
+// sweeps and back after), else in global memory.
+template <bool LDSP> struct Phi;
+template <> struct Phi<true> {
+    float* p; int lo; uint32_t n; float* g;
+    __device__ __forceinline__ float ld(int i) const { uint32_t k = (uint32_t)(i - lo); return k < n ? p[k] : g[i]; }
+    __device__ __forceinline__ void st(int i, float v) const { uint32_t k = (uint32_t)(i - lo); if (k < n) p[k] = v; else g[i] = v; }
+};
+template <> struct Phi<false> {
+    float* p; int lo; uint32_t n; float* g;
+    __device__ __forceinline__ float ld(int i) const { return g[i]; }
+    __device__ __forceinline__ void st(int i, float v) const { g[i] = v; }
+};
+// one red-black sweep over a structure's stencil (sor_brick in order within a colour, src/lib.rs:1005-1026); returns
+// max|change| / max(phi, 1) over its bricks. U stencil rows per thread are loaded together before any is applied:
+// rows of one colour only read bricks of the other colour, so the order of application within a colour is free.
+template <bool LDSP>
+__device__ float sweep_struct(LD& D, uint32_t g, float om, const Phi<LDSP>& phi, float* sh) {
     int4 rb = D.st_rb[g];
     float md = 0.0f, mp = 1.0f;
+    constexpr int U = 4;
     for (int par = 0; par < 2; par++) {
         uint32_t o = par == 0 ? rb.x : rb.z, n = par == 0 ? rb.y : rb.w;
-        for (uint32_t t = threadIdx.x; t < n; t += ST_NT) {
-            float ph;
-            bool did;
-            float d = sten_update(D, o + t, om, ph, did);
-            if (!did) continue;
-            md = fmaxf(md, d);
-            mp = fmaxf(mp, ph);
+        for (uint32_t t = threadIdx.x; t < n; t += ST_NT * U) {
+            float2 sr[U];
+            int4 na[U];
+            int2 nb[U];
+            float4 ka[U];
+            float2 kb[U];
+            uint32_t ii[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                uint32_t q = t + u * ST_NT, p = o + q;
+                sr[u] = q < n ? D.sten_s[p] : make_float2(-1.0f, 0.0f);
+                if (sr[u].x >= 0.0f) { na[u] = D.sten_na[p]; nb[u] = D.sten_nb[p]; ka[u] = D.sten_ka[p]; kb[u] = D.sten_kb[p]; ii[u] = D.st_red[p]; }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (sr[u].x < 0.0f) continue;
+                int nbr[6] = {na[u].x, na[u].y, na[u].z, na[u].w, nb[u].x, nb[u].y};
+                float kk[6] = {ka[u].x, ka[u].y, ka[u].z, ka[u].w, kb[u].x, kb[u].y};
+                float skp = 0.0f;
+#pragma unroll
+                for (int d = 0; d < 6; d++) if (nbr[d] >= 0) skp += kk[d] * phi.ld(nbr[d]);
+                int i = (int)ii[u];
+                float pi = phi.ld(i), po = pi;
+                if (sr[u].x > 0.0f) {
+                    float target = (sr[u].y + skp) / sr[u].x;
+                    float d = om * (target - pi);
+                    po = pi + d;
+                    phi.st(i, po);
+                    md = fmaxf(md, fabsf(d));
+                } else {
+                    md = fmaxf(md, 0.0f);
+                }
+                mp = fmaxf(mp, po);
+            }
         }
         __syncthreads();
     }
     block_max2<ST_NT>(md, mp, sh);
     return md / mp;
 }
-__global__ void __launch_bounds__(ST_NT) k_sor_scene(LD D, int mode) {
+// one structure's sweeps of a frame: adaptive (mode 0, structural :1268-1281) or warm-up (mode 1, warm_step :1068)
+template <bool LDSP>
+__device__ void sor_struct_frame(LD& D, uint32_t g, int mode, const Phi<LDSP>& phi, long long& budget, float* sh) {
+    int nmin = D.st_nmin[g];
+    long long size = D.st_hi[g] - D.st_lo[g];
+    float resid = D.st_resid[g];
+    if (LDSP) {
+        for (uint32_t k = threadIdx.x; k < phi.n; k += ST_NT) phi.p[k] = phi.g[phi.lo + (int)k];
+        __syncthreads();
+    }
+    if (mode == 0) {
+        int k = 0;
+        for (;;) {
+            resid = sweep_struct<LDSP>(D, g, 1.85f, phi, sh);
+            k++;
+            budget -= size;
+            if (k >= nmin && (resid < 4e-4f || k >= 90 || budget <= 0)) break;
+        }
+    } else {
+        int warm = nmin;
+        while (warm > 0 && budget > 0) {
+            resid = sweep_struct<LDSP>(D, g, warm > 60 ? 1.9f : 1.0f, phi, sh);
+            warm--;
+            budget -= size;
+        }
+        if (threadIdx.x == 0) D.st_nmin[g] = warm;
+    }
+    if (LDSP) for (uint32_t k = threadIdx.x; k < phi.n; k += ST_NT) phi.g[phi.lo + (int)k] = phi.p[k];
+    if (threadIdx.x == 0) D.st_resid[g] = resid;
+    __syncthreads();
+}
+// one workgroup per scene: its structures in order, within the frame's budget of brick updates; lds_cap floats of
+// dynamic LDS hold the potential of a structure of up to that many bricks
+__global__ void __launch_bounds__(ST_NT) k_sor_scene(LD D, int mode, uint32_t lds_cap) {
     __shared__ float sh[2 * ST_NT / 64];
+    extern __shared__ float lph[];
     for (uint32_t sc = blockIdx.x; sc < D.nscene; sc += gridDim.x) {
         const SceneK S = D.sc[sc];
         long long budget = mode == 0 ? 160000 : 120000;
         for (int sl = 0; sl < S.nst; sl++) {
             uint32_t g = (uint32_t)(S.st0 + sl);
-            int nmin = D.st_nmin[g];
-            if (nmin <= 0) continue;
+            if (D.st_nmin[g] <= 0) continue;
             if (mode == 0 && !D.st_act[g]) continue;
-            long long size = D.st_hi[g] - D.st_lo[g];
-            float resid = D.st_resid[g];
-            if (mode == 0) {
-                int k = 0;
-                for (;;) {
-                    resid = sweep_struct(D, g, sl, 1.85f, S.frame, sh);
-                    k++;
-                    budget -= size;
-                    if (k >= nmin && (resid < 4e-4f || k >= 90 || budget <= 0)) break;
-                }
-            } else {
-                int warm = nmin;
-                while (warm > 0 && budget > 0) {
-                    resid = sweep_struct(D, g, sl, warm > 60 ? 1.9f : 1.0f, S.frame, sh);
-                    warm--;
-                    budget -= size;
-                }
-                if (threadIdx.x == 0) D.st_nmin[g] = warm;
-            }
-            if (threadIdx.x == 0) D.st_resid[g] = resid;
-            __syncthreads();
+            int lo = D.st_lo[g];
+            uint32_t size = (uint32_t)(D.st_hi[g] - lo);
+            if (size <= lds_cap) sor_struct_frame<true>(D, g, mode, Phi<true>{lph, lo, size, D.phi}, budget, sh);
+            else sor_struct_frame<false>(D, g, mode, Phi<false>{lph, lo, 0u, D.phi}, budget, sh);
         }
     }
 }
@@ -1510,7 +1561,9 @@ struct Lat {
     int dev = 0;
     hipStream_t st = nullptr;
     int K = 16, R = 8;
-    int fuse_max = 32768;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
+    int fuse_max = 32768;
+    int lds_max = 0;           // LDS bytes a workgroup may have (device attribute)
+    bool lds_attr = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
     std::vector<int32_t> slist;
     int bcap = 0, dcap = 0, scap = 0;
     int nb = 0, nbd = 0;                   // bricks / body slots handed out to scenes
@@ -2473,6 +2526,7 @@ void segsort(Lat* L, const uint32_t* off, const uint32_t* cnt, uint32_t* val, ui
 int substep(Lat* L, float h) {
     LD& D = L->D;
     const unsigned gb = sf_grid(D.nub), gd = sf_grid(std::max<uint32_t>(L->ndyn, 1));
+    const unsigned gw = sf_grid((uint64_t)std::max<uint32_t>(L->ndyn, 1) * 64, NT, 65536);  // a wave per dynamic brick
     LAUNCH(k_cache, gb, NT, D, 1, h);
     LAUNCH(k_wpos, gd, NT, D);
     // hash
@@ -2484,7 +2538,7 @@ int substep(Lat* L, float h) {
     segsort(L, D.hoff, D.hcnt, D.hent, L->tsize);
     // contacts: count, size, write
     reset_ctr(L);
-    LAUNCH(k_contacts<false>, gd, NT, D);
+    LAUNCH(k_contacts<false>, gw, NT, D);
     scan(L, D.ccnt, D.coff, D.ctr + C_NCS, L->ndyn);
     if (read_ctr(L)) return -1;
     if (drain_kills(L)) return -1;
@@ -2493,7 +2547,7 @@ int substep(Lat* L, float h) {
         if (lat_contacts_alloc(L, std::max<uint32_t>(ncs + ncs / 2, 1u << 16))) return -1;
     }
     LAUNCH(k_nc_clear, gb, NT, D);
-    LAUNCH(k_contacts<true>, gd, NT, D);
+    LAUNCH(k_contacts<true>, gw, NT, D);
     LAUNCH(k_scene_c0, 1, 64, D);
     LAUNCH(k_plough_flag, sf_grid(ncs), NT, D);
     std::vector<uint32_t> c0(L->sc.size());
@@ -2560,6 +2614,20 @@ int substep(Lat* L, float h) {
     return 0;
 }
 
+// the scenes' structure sweeps; the largest structure that fits the workgroup's LDS sizes the potential copy
+void sor_scene_launch(Lat* L, int mode) {
+    if (!L->lds_max) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, L->dev) != hipSuccess || v <= 0) v = 65536;
+        L->lds_max = v;
+    }
+    uint32_t fit = (uint32_t)std::max(0, (L->lds_max - 1024) / (int)sizeof(float)), cap = 0;
+    for (const HStruct& T : L->sts) { uint32_t n = (uint32_t)(T.hi - T.lo); if (n <= fit) cap = std::max(cap, n); }
+    size_t bytes = sizeof(float) * (size_t)cap;
+    if (bytes > 65536 && !L->lds_attr) { (void)hipFuncSetAttribute((const void*)k_sor_scene, hipFuncAttributeMaxDynamicSharedMemorySize, L->lds_max - 1024); L->lds_attr = true; }
+    hipLaunchKernelGGL(k_sor_scene, dim3((unsigned)std::min<size_t>(L->sc.size(), 2048)), dim3(ST_NT), bytes, L->st, L->D, mode, cap);
+}
+
 int structures_upload(Lat* L, bool warm) {
     size_t ns = L->sts.size();
     if (!ns) return 0;
@@ -2620,7 +2688,7 @@ int structural(Lat* L, float dt) {
     LAUNCH(k_unsupported, gw, NT, D);
     if (ns) {
         hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(ns, 4096)), dim3(NT), 0, L->st, D, 0);
-        LAUNCH(k_sor_scene, std::min<size_t>(L->sc.size(), 2048), ST_NT, D, 0);
+        sor_scene_launch(L, 0);
     }
     LAUNCH(k_ga, gw, NT, D, dt);
     LAUNCH(k_body_sor_prep, gb, NT, D);
@@ -2825,7 +2893,7 @@ int frame(Lat* L, float dt) {
     if (any_warm) {
         if (structures_upload(L, true)) return -1;
         hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(L->sts.size(), 4096)), dim3(NT), 0, L->st, D, 1);
-        LAUNCH(k_sor_scene, std::min<size_t>(L->sc.size(), 2048), ST_NT, D, 1);
+        sor_scene_launch(L, 1);
         std::vector<float> resid(L->sts.size());
         std::vector<int32_t> left(L->sts.size());
         if (cp_down(resid.data(), D.st_resid, resid.size(), L->st) || cp_down(left.data(), D.st_nmin, left.size(), L->st)) return -1;
