@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(NT) k_ovf_apply(LD D) {
 // workgroup barriers instead of kernel boundaries. The arithmetic and every order are those of the kernels above
 // (k_col_claim ... k_ovf_apply), so both routes give the same bits; this one needs one launch instead of about 150.
 // Locations updated by atomics (claims, masks, overflow counts) are read with device-scope loads.
-constexpr int FS_NT = 1024;
+constexpr int FS_NT = 512;
 __device__ __forceinline__ uint32_t ld_dev(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ unsigned long long ld_dev(const unsigned long long* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 // exclusive scan of one value per thread over the workgroup (wsum: 17 words of LDS); every thread must call it
@@ -620,8 +620,118 @@ __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t* wsum, uin
     *total = wsum[FS_NT / 64];
     return wsum[wv] + x - v;
 }
-__global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list) {
+// A contact's solve record, in colour-bucket order: its bodies as scene-local slots, arms, normal, tangents,
+// effective masses (split ones for the overflow bucket), bias and accumulated impulses; one load per pass.
+struct SRec {
+    int a, b;               // scene-local body slots (b < 0: static)
+    float4 r0, r1, r2, r3, r4, r5;  // ra|mn, rb|bias, n|mt1, t1|mt2, t2|na, jn jt1 jt2|nb
+};
+__device__ __forceinline__ SRec srec_load(const LD& D, uint32_t q) {
+    SRec r;
+    int2 ab = D.rq_ab[q];
+    r.a = ab.x; r.b = ab.y;
+    r.r0 = D.rq_0[q]; r.r1 = D.rq_1[q]; r.r2 = D.rq_2[q]; r.r3 = D.rq_3[q]; r.r4 = D.rq_4[q]; r.r5 = D.rq_5[q];
+    return r;
+}
+// the scene's bodies in LDS: velocities, angular velocities, world inverse inertia rows, inverse mass
+struct LBody {
+    float4 *v, *w, *ii;
+    float* im;
+    __device__ __forceinline__ LM iiw(int b) const {
+        float4 x = ii[3 * b], y = ii[3 * b + 1], z = ii[3 * b + 2];
+        LM m;
+        m.m[0] = x.x; m.m[1] = x.y; m.m[2] = x.z; m.m[3] = y.x; m.m[4] = y.y; m.m[5] = y.z; m.m[6] = z.x; m.m[7] = z.y; m.m[8] = z.z;
+        return m;
+    }
+};
+// solve_contact on LDS bodies (same operations in the same order)
+__device__ __forceinline__ void srec_solve(const LD& D, const LBody& B, SRec& r, uint32_t q) {
+    const int a = r.a, b = r.b;
+    const LV ra = lv4(r.r0), rb = lv4(r.r1), nn = lv4(r.r2), t1 = lv4(r.r3), t2 = lv4(r.r4);
+    LV va = lv4(B.v[a]), wa = lv4(B.w[a]), vb = lv(0.0f, 0.0f, 0.0f), wb = lv(0.0f, 0.0f, 0.0f);
+    if (b >= 0) { vb = lv4(B.v[b]); wb = lv4(B.w[b]); }
+    const float ima = B.im[a], imb = b >= 0 ? B.im[b] : 0.0f;
+    const LM ia = B.iiw(a);
+    LM ib;
+    if (b >= 0) ib = B.iiw(b);
+    LV dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+    float vn = dot(dv, nn);
+    float lam = r.r0.w * (-vn + r.r1.w);
+    float old = r.r5.x;
+    float jn = maxf(old + lam, 0.0f);
+    float d = jn - old;
+    LV p = mul(nn, d);
+    va = madd(va, p, ima); wa = add(wa, mv(ia, cross(ra, p)));
+    if (b >= 0) { vb = madd(vb, p, -imb); wb = sub(wb, mv(ib, cross(rb, p))); }
+    float lim = MU * jn;
+    dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+    float l1 = -r.r2.w * dot(dv, t1);
+    float o1 = r.r5.y, j1 = clampf(o1 + l1, -lim, lim);
+    float l2 = -r.r3.w * dot(dv, t2);
+    float o2 = r.r5.z, j2 = clampf(o2 + l2, -lim, lim);
+    p = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
+    va = madd(va, p, ima); wa = add(wa, mv(ia, cross(ra, p)));
+    if (b >= 0) { vb = madd(vb, p, -imb); wb = sub(wb, mv(ib, cross(rb, p))); }
+    B.v[a] = f4(va); B.w[a] = f4(wa);
+    if (b >= 0) { B.v[b] = f4(vb); B.w[b] = f4(wb); }
+    r.r5.x = jn; r.r5.y = j1; r.r5.z = j2;
+    D.rq_5[q] = r.r5;
+}
+// ovf_contact on LDS bodies; the velocity changes go to odv/odw under the contact's own index
+__device__ __forceinline__ void srec_ovf(const LD& D, const LBody& B, SRec& r, uint32_t q) {
+    const int a = r.a, b = r.b;
+    const float na = r.r4.w, nb = r.r5.w;
+    const LV ra = lv4(r.r0), rb = lv4(r.r1), nn = lv4(r.r2), t1 = lv4(r.r3), t2 = lv4(r.r4);
+    LV va = lv4(B.v[a]), wa = lv4(B.w[a]), vb = lv(0.0f, 0.0f, 0.0f), wb = lv(0.0f, 0.0f, 0.0f);
+    if (b >= 0) { vb = lv4(B.v[b]); wb = lv4(B.w[b]); }
+    LM iia = B.iiw(a);
+    LM iib;
+    if (b >= 0) iib = B.iiw(b);
+    float ima = B.im[a] * na, imb = b >= 0 ? B.im[b] * nb : 0.0f;
+    LV dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+    float vn = dot(dv, nn);
+    float lam = r.r0.w * (-vn + r.r1.w);
+    float old = r.r5.x;
+    float jn = maxf(old + lam, 0.0f);
+    LV pn = mul(nn, jn - old);
+    va = madd(va, pn, ima);
+    wa = add(wa, mul(mv(iia, cross(ra, pn)), na));
+    if (b >= 0) { vb = madd(vb, pn, -imb); wb = sub(wb, mul(mv(iib, cross(rb, pn)), nb)); }
+    float lim = MU * jn;
+    dv = sub(add(va, cross(wa, ra)), b >= 0 ? add(vb, cross(wb, rb)) : lv(0.0f, 0.0f, 0.0f));
+    float l1 = -r.r2.w * dot(dv, t1);
+    float o1 = r.r5.y, j1 = clampf(o1 + l1, -lim, lim);
+    float l2 = -r.r3.w * dot(dv, t2);
+    float o2 = r.r5.z, j2 = clampf(o2 + l2, -lim, lim);
+    LV pt = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
+    LV p = add(pn, pt);
+    r.r5.x = jn; r.r5.y = j1; r.r5.z = j2;
+    D.rq_5[q] = r.r5;
+    const uint32_t c = D.rq_c[q];
+    D.odv[2 * c] = f4(mul(p, B.im[a]));
+    D.odw[2 * c] = f4(mv(iia, cross(ra, p)));
+    if (b >= 0) {
+        D.odv[2 * c + 1] = f4(mul(p, -B.im[b]));
+        D.odw[2 * c + 1] = f4(neg(mv(iib, cross(rb, p))));
+    }
+}
+// ovf_body into LDS bodies
+__device__ __forceinline__ void lbody_ovf(const LD& D, const LBody& B, uint32_t lb, uint32_t b, uint32_t n, int lane) {
+    uint32_t o = D.ovoff[b];
+    LV av = lv(0.0f, 0.0f, 0.0f), aw = lv(0.0f, 0.0f, 0.0f);
+    for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+        LV xv = lv(0.0f, 0.0f, 0.0f), xw = lv(0.0f, 0.0f, 0.0f);
+        if (e0 + lane < n) { uint32_t ent = D.ovent[o + e0 + lane]; xv = lv4(D.odv[ent]); xw = lv4(D.odw[ent]); }
+        av = add(av, wave_tree(xv));
+        aw = add(aw, wave_tree(xw));
+    }
+    if (lane == 0) { B.v[lb] = f4(add(lv4(B.v[lb]), av)); B.w[lb] = f4(add(lv4(B.w[lb]), aw)); }
+}
+
+__global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list, uint32_t lds_cap) {
     __shared__ uint32_t cnt[KMAX + 1], off[KMAX + 2], fil[KMAX + 1], wsum[FS_NT / 64 + 1], nov;
+    __shared__ uint32_t ps_off[KMAX + 2], ps_cnt[KMAX + 2], ps_type[KMAX + 2], npass;
+    extern __shared__ float4 lbody[];
     const int s = list[blockIdx.x];
     const SceneK S = D.sc[s];
     const uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
@@ -728,8 +838,77 @@ __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list
         }
         __syncthreads();
     }
-    // 10 iterations: colour after colour, then the overflow pass
     const int lane = tid & 63;
+    if (hw <= lds_cap) {
+        // bodies into LDS, contacts into solve records in bucket order
+        const LBody B{lbody, lbody + hw, lbody + 2 * hw, (float*)(lbody + 5 * hw)};
+        for (uint32_t lb = tid; lb < hw; lb += FS_NT) {
+            uint32_t b = bd0 + lb;
+            B.v[lb] = D.v[b]; B.w[lb] = D.w[b];
+            B.ii[3 * lb] = D.iiw[3 * (size_t)b]; B.ii[3 * lb + 1] = D.iiw[3 * (size_t)b + 1]; B.ii[3 * lb + 2] = D.iiw[3 * (size_t)b + 2];
+            B.im[lb] = D.im[b];
+        }
+        const uint32_t nrec = off[K + 1];
+        for (uint32_t q = tid; q < nrec; q += FS_NT) {
+            uint32_t c = D.collist[c0 + q], pos = c0 + q;
+            int a = D.ca[c], b = D.cb[c];
+            const bool ovf = D.ccol[c] < 0;
+            float na = 0.0f, nb2 = 0.0f;
+            if (ovf) { na = (float)ld_dev(&D.ovcnt[a]); if (b >= 0) nb2 = (float)ld_dev(&D.ovcnt[b]); }
+            D.rq_c[pos] = c;
+            D.rq_ab[pos] = make_int2(a - (int)bd0, b >= 0 ? b - (int)bd0 : -1);
+            const LV ra = lv4(D.cra[c]), rb = lv4(D.crb[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
+            D.rq_0[pos] = f4(ra, ovf ? D.csmn[c] : D.cmn[c]);
+            D.rq_1[pos] = f4(rb, D.cbias[c]);
+            D.rq_2[pos] = f4(nn, ovf ? D.csmt1[c] : D.cmt1[c]);
+            D.rq_3[pos] = f4(t1, ovf ? D.csmt2[c] : D.cmt2[c]);
+            D.rq_4[pos] = f4(t2, na);
+            D.rq_5[pos] = make_float4(D.cjn[c], D.cjt1[c], D.cjt2[c], nb2);
+        }
+        if (tid == 0) {
+            uint32_t np = 0;
+            for (uint32_t col = 0; col < K; col++) if (cnt[col]) { ps_type[np] = 0; ps_off[np] = c0 + off[col]; ps_cnt[np] = cnt[col]; np++; }
+            if (no) { ps_type[np] = 1; ps_off[np] = oo; ps_cnt[np] = no; np++; ps_type[np] = 2; ps_off[np] = 0; ps_cnt[np] = nov; np++; }
+            npass = np;
+        }
+        __syncthreads();
+        // 10 iterations of the passes; each thread's record of the next pass is loaded before this pass's barrier
+        const uint32_t np = npass, total = 10 * np;
+        SRec cur, nxt;
+        bool has = false, hasn = false;
+        if (np && ps_type[0] != 2 && tid < ps_cnt[0]) { cur = srec_load(D, ps_off[0] + tid); has = true; }
+        for (uint32_t x = 0; x < total; x++) {
+            const uint32_t ps = x % np, ty = ps_type[ps], po = ps_off[ps], pn = ps_cnt[ps];
+            hasn = false;
+            if (x + 1 < total && np > 1) {  // (with one pass, its records are the ones being updated: kept in cur)
+                uint32_t q = (x + 1) % np;
+                if (ps_type[q] != 2 && tid < ps_cnt[q]) { nxt = srec_load(D, ps_off[q] + tid); hasn = true; }
+            }
+            if (ty == 0) {
+                if (has) srec_solve(D, B, cur, po + tid);
+                for (uint32_t t = tid + FS_NT; t < pn; t += FS_NT) { SRec r = srec_load(D, po + t); srec_solve(D, B, r, po + t); }
+            } else if (ty == 1) {
+                if (has) srec_ovf(D, B, cur, po + tid);
+                for (uint32_t t = tid + FS_NT; t < pn; t += FS_NT) { SRec r = srec_load(D, po + t); srec_ovf(D, B, r, po + t); }
+            } else {
+                for (uint32_t k = tid >> 6; k < pn; k += FS_NT / 64) {
+                    uint32_t b = D.ovbody[bd0 + k];
+                    lbody_ovf(D, B, b - bd0, b, ld_dev(&D.ovcnt[b]), lane);
+                }
+            }
+            __syncthreads();
+            if (np > 1) { cur = nxt; has = hasn; }
+        }
+        // impulses back to the contacts, bodies back to global memory
+        for (uint32_t q = tid; q < nrec; q += FS_NT) {
+            uint32_t pos = c0 + q, c = D.rq_c[pos];
+            float4 j = D.rq_5[pos];
+            D.cjn[c] = j.x; D.cjt1[c] = j.y; D.cjt2[c] = j.z;
+        }
+        for (uint32_t lb = tid; lb < hw; lb += FS_NT) { D.v[bd0 + lb] = B.v[lb]; D.w[bd0 + lb] = B.w[lb]; }
+        return;
+    }
+    // 10 iterations: colour after colour, then the overflow pass
     for (int it = 0; it < 10; it++) {
         for (uint32_t col = 0; col < K; col++) {
             const uint32_t n = cnt[col], o = c0 + off[col];
@@ -1563,7 +1742,7 @@ struct Lat {
     int K = 16, R = 8;
     int fuse_max = 32768;
     int lds_max = 0;           // LDS bytes a workgroup may have (device attribute)
-    bool lds_attr = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
+    bool lds_attr = false, lds_attr_fs = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
     std::vector<int32_t> slist;
     int bcap = 0, dcap = 0, scap = 0;
     int nb = 0, nbd = 0;                   // bricks / body slots handed out to scenes
@@ -1702,6 +1881,7 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     fr(D.cn); fr(D.cp); fr(D.cra); fr(D.crb); fr(D.ct1); fr(D.ct2);
     fr(D.cdepth); fr(D.cmn); fr(D.cmt1); fr(D.cmt2); fr(D.cjn); fr(D.cjt1); fr(D.cjt2); fr(D.cbias); fr(D.cvn0);
     fr(D.csmn); fr(D.csmt1); fr(D.csmt2); fr(D.collist); fr(D.ovent); fr(D.odv); fr(D.odw);
+    fr(D.rq_c); fr(D.rq_ab); fr(D.rq_0); fr(D.rq_1); fr(D.rq_2); fr(D.rq_3); fr(D.rq_4); fr(D.rq_5);
     int r = 0;
     r |= L->dalloc(D.ca, cap); r |= L->dalloc(D.cb, cap); r |= L->dalloc(D.cbo, cap); r |= L->dalloc(D.cscene, cap);
     r |= L->dalloc(D.ccol, cap); r |= L->dalloc(D.cprop, cap); r |= L->dalloc(D.cba, cap); r |= L->dalloc(D.cbb, cap);
@@ -1713,6 +1893,8 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     r |= L->dalloc(D.cvn0, cap); r |= L->dalloc(D.csmn, cap); r |= L->dalloc(D.csmt1, cap); r |= L->dalloc(D.csmt2, cap);
     r |= L->dalloc(D.collist, cap); r |= L->dalloc(D.ovent, 2 * (size_t)cap);
     r |= L->dalloc(D.odv, 2 * (size_t)cap); r |= L->dalloc(D.odw, 2 * (size_t)cap);
+    r |= L->dalloc(D.rq_c, cap); r |= L->dalloc(D.rq_ab, cap); r |= L->dalloc(D.rq_0, cap); r |= L->dalloc(D.rq_1, cap);
+    r |= L->dalloc(D.rq_2, cap); r |= L->dalloc(D.rq_3, cap); r |= L->dalloc(D.rq_4, cap); r |= L->dalloc(D.rq_5, cap);
     D.ccap = cap;
     L->ccap = cap;
     return r ? -1 : 0;
@@ -2573,7 +2755,17 @@ int substep(Lat* L, float h) {
     if (fused) {
         if (!L->slist.empty()) {
             SF_HIP(hipMemcpyAsync(L->d_slist.p, L->slist.data(), sizeof(int32_t) * L->slist.size(), hipMemcpyHostToDevice, L->st));
-            LAUNCH(k_scene_solve, (unsigned)L->slist.size(), FS_NT, D, (const int32_t*)L->d_slist.p);
+            // bodies of scenes up to the LDS capacity (84 bytes each) are solved in LDS
+            if (!L->lds_max) {
+                int v = 0;
+                if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, L->dev) != hipSuccess || v <= 0) v = 65536;
+                L->lds_max = v;
+            }
+            uint32_t fit = (uint32_t)std::max(0, (L->lds_max - 2048) / 84), cap = 0;
+            for (int32_t si : L->slist) { uint32_t hw = (uint32_t)L->sc[si].hw; if (hw <= fit) cap = std::max(cap, hw); }
+            size_t bytes = 84 * (size_t)cap;
+            if (bytes > 65536 && !L->lds_attr_fs) { (void)hipFuncSetAttribute((const void*)k_scene_solve, hipFuncAttributeMaxDynamicSharedMemorySize, L->lds_max - 2048); L->lds_attr_fs = true; }
+            hipLaunchKernelGGL(k_scene_solve, dim3((unsigned)L->slist.size()), dim3(FS_NT), bytes, L->st, D, (const int32_t*)L->d_slist.p, cap);
         }
     } else {
         // colour

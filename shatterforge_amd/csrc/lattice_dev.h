@@ -108,6 +108,10 @@ struct LD {
     uint32_t *ovcnt, *ovoff, *ovfill, *ovent;
     uint32_t* ovbody;          // per scene: its bodies with overflow entries (at the scene's first body slot)
     float4 *odv, *odw;
+    // solve records of the workgroup-per-scene route, in colour-bucket order (see SRec)
+    uint32_t* rq_c;
+    int2* rq_ab;
+    float4 *rq_0, *rq_1, *rq_2, *rq_3, *rq_4, *rq_5;
     // structural scratch
     uint32_t *lcnt, *loff, *lfill, *lent;
     int32_t* ltarget;
