@@ -1795,7 +1795,18 @@ struct Lat {
     HBuf<uint32_t> h_idx;
     HBuf<float4> h_wp;
     std::vector<uint8_t> bmark;
-    uint32_t h_ctr[C_NCTR];
+    // pinned read-back: counters, scene first contacts, killed bodies, the structural pass's results, scene
+    // statistics (read at the next wait), and the scene table (two buffers, each reused once its copy is done)
+    HBuf<uint32_t> h_ctrbuf, h_c0, h_scstat;
+    HBuf<int32_t> h_kl;
+    HBuf<uint8_t> h_rb;
+    uint32_t* h_ctr = nullptr;
+    HBuf<SceneK> h_sk[2];
+    hipEvent_t ev_sk[2] = {nullptr, nullptr}, ev_stat = nullptr;
+    int sk_cur = 0;
+    bool sk_recorded[2] = {false, false}, stat_pending = false;
+    size_t stat_n = 0;
+    std::vector<SceneK> sk_last;
     uint32_t ccap = 0, tsize = 0;
     // bricks whose material, body or bonds the host changed since the last flush
     std::vector<uint8_t> touched;
@@ -1818,6 +1829,8 @@ struct Lat {
         d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release(); d_csr.release();
         d_delta.release(); d_brec.release(); d_idx.release(); d_wp.release();
         h_delta.release(); h_brec.release(); h_idx.release(); h_wp.release();
+        h_ctrbuf.release(); h_c0.release(); h_scstat.release(); h_kl.release(); h_rb.release(); h_sk[0].release(); h_sk[1].release();
+        for (hipEvent_t e : {ev_sk[0], ev_sk[1], ev_stat}) if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
     template <class T> int dalloc(T*& p, size_t n) {
@@ -2403,8 +2416,18 @@ int upload_scenes(Lat* L) {
         x.gx0 = S.gx0; x.gy0 = S.gy0; x.gz0 = S.gz0; x.gnx = S.gnx; x.gny = S.gny; x.gnz = S.gnz; x.goff = (uint32_t)S.goff;
         x.bd0 = S.bd0; x.nbd = S.dcap; x.dyn0 = S.dyn0; x.ndyn = S.ndyn; x.st0 = S.st0; x.nst = S.nst; x.b0 = S.b0; x.nb = S.nb; x.hw = S.hw;
     }
-    SF_HIP(hipMemcpyAsync(L->d_sc.p, k.data(), sizeof(SceneK) * k.size(), hipMemcpyHostToDevice, L->st));
-    SF_HIP(hipStreamSynchronize(L->st));
+    // unchanged since the last upload: nothing to do; else an asynchronous copy from one of two pinned buffers
+    if (k.size() == L->sk_last.size() && !k.empty() && memcmp(k.data(), L->sk_last.data(), sizeof(SceneK) * k.size()) == 0) return 0;
+    L->sk_last = k;
+    if (k.empty()) return 0;
+    int b = L->sk_cur ^= 1;
+    if (!L->ev_sk[b]) SF_HIP(hipEventCreateWithFlags(&L->ev_sk[b], hipEventDisableTiming));
+    if (L->sk_recorded[b]) SF_HIP(hipEventSynchronize(L->ev_sk[b]));
+    if (L->h_sk[b].reserve(k.size())) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    memcpy(L->h_sk[b].p, k.data(), sizeof(SceneK) * k.size());
+    SF_HIP(hipMemcpyAsync(L->d_sc.p, L->h_sk[b].p, sizeof(SceneK) * k.size(), hipMemcpyHostToDevice, L->st));
+    SF_HIP(hipEventRecord(L->ev_sk[b], L->st));
+    L->sk_recorded[b] = true;
     return 0;
 }
 
@@ -2415,11 +2438,13 @@ void reset_ctr(Lat* L) {
     fill_u32(L, L->D.ctr, 0, C_NKILL);
     fill_u32(L, L->D.ctr + C_NKILL + 1, 0, C_NCTR - C_NKILL - 1);
 }
+int finish_stats(Lat* L);
+// the device counters into pinned memory; the wait also completes every read-back queued before it
 int read_ctr(Lat* L) {
-    SF_HIP(hipMemcpyAsync(L->h_ctr, L->D.ctr, sizeof(L->h_ctr), hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipMemcpyAsync(L->h_ctr, L->D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
     L->syncs++;
-    return 0;
+    return finish_stats(L);
 }
 
 // bodies killed on the device (pinch, out of bounds): the host mirror follows
@@ -2427,15 +2452,13 @@ int drain_kills(Lat* L) {
     uint32_t n = L->h_ctr[C_NKILL];
     if (!n) return 0;
     n = std::min<uint32_t>(n, (uint32_t)L->nbd);
-    std::vector<int32_t> k(n);
-    SF_HIP(hipMemcpyAsync(k.data(), L->D.klist, sizeof(int32_t) * n, hipMemcpyDeviceToHost, L->st));
+    if (L->h_kl.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    SF_HIP(hipMemcpyAsync(L->h_kl.p, L->D.klist, sizeof(int32_t) * n, hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
-    for (uint32_t t = 0; t < n; t++) kill_body_host(L, k[t]);
+    L->syncs++;
+    for (uint32_t t = 0; t < n; t++) kill_body_host(L, L->h_kl.p[t]);
     L->h_ctr[C_NKILL] = 0;
-    uint32_t z = 0;
-    SF_HIP(hipMemcpyAsync(L->D.ctr + C_NKILL, &z, 4, hipMemcpyHostToDevice, L->st));
-    SF_HIP(hipStreamSynchronize(L->st));
-    return 0;
+    return fill_u32(L, L->D.ctr + C_NKILL, 0, 1);
 }
 
 int scene_of_contact(Lat* L, uint32_t c) {
@@ -2732,8 +2755,9 @@ int substep(Lat* L, float h) {
     LAUNCH(k_contacts<true>, gw, NT, D);
     LAUNCH(k_scene_c0, 1, 64, D);
     LAUNCH(k_plough_flag, sf_grid(ncs), NT, D);
-    std::vector<uint32_t> c0(L->sc.size());
-    SF_HIP(hipMemcpyAsync(c0.data(), D.sc_c0, sizeof(uint32_t) * c0.size(), hipMemcpyDeviceToHost, L->st));
+    if (L->h_c0.reserve(L->sc.size())) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    const uint32_t* c0 = L->h_c0.p;
+    SF_HIP(hipMemcpyAsync(L->h_c0.p, D.sc_c0, sizeof(uint32_t) * L->sc.size(), hipMemcpyDeviceToHost, L->st));
     if (read_ctr(L)) return -1;
     for (size_t si = 0; si < L->sc.size(); si++) {
         L->sc[si].c0 = c0[si];
@@ -2893,16 +2917,23 @@ int structural(Lat* L, float dt) {
     LAUNCH(k_lt_decay, gb, NT, D, dt);
     SF_HIP(hipGetLastError());
     // results back: per structure resid / max ratio / flags, per scene max ratio, events
-    std::vector<float> resid(ns);
-    std::vector<uint32_t> maxr(ns), flags(ns), gmax(L->sc.size());
-    std::vector<uint8_t> bdirty(D.nub);
+    const size_t nsc = L->sc.size();
     L->sp[8] += ms_since(t);
     t = std::chrono::steady_clock::now();
+    // into pinned memory; the counter read's wait completes them all
+    auto al = [](size_t x) { return (x + 63) & ~(size_t)63; };
+    const size_t o_mr = al(4 * ns), o_fl = o_mr + al(4 * ns), o_gm = o_fl + al(4 * ns), o_bd = o_gm + al(4 * nsc);
+    if (L->h_rb.reserve(o_bd + D.nub + 64)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    float* resid = (float*)L->h_rb.p;
+    uint32_t* maxr = (uint32_t*)(L->h_rb.p + o_mr);
+    uint32_t* flags = (uint32_t*)(L->h_rb.p + o_fl);
+    uint32_t* gmax = (uint32_t*)(L->h_rb.p + o_gm);
+    uint8_t* bdirty = L->h_rb.p + o_bd;
     int r = 0;
-    r |= cp_down(resid.data(), D.st_resid, ns, L->st); r |= cp_down(maxr.data(), D.st_maxr, ns, L->st);
-    r |= cp_down(flags.data(), D.st_flags, ns, L->st); r |= cp_down(gmax.data(), D.sc_gmax, L->sc.size(), L->st);
+    r |= cp_down(resid, D.st_resid, ns, L->st); r |= cp_down(maxr, D.st_maxr, ns, L->st);
+    r |= cp_down(flags, D.st_flags, ns, L->st); r |= cp_down(gmax, D.sc_gmax, nsc, L->st);
     LAUNCH(k_take_dirty, gb, NT, D);
-    r |= cp_down(bdirty.data(), D.bdflag, D.nub, L->st);
+    r |= cp_down(bdirty, D.bdflag, D.nub, L->st);
     if (r) return r;
     if (read_ctr(L)) return -1;
     if (drain_kills(L)) return -1;
@@ -2971,19 +3002,35 @@ int structural(Lat* L, float dt) {
 
 namespace {
 
+// the frame statistics (ptr_stats, src/lib.rs:3718-3720): the host's fields now; the device counts (dynamic bricks,
+// bodies, awake bodies) are copied back asynchronously and taken at the next wait (finish_stats)
 int scene_stats(Lat* L) {
     size_t S = L->sc.size();
+    if (finish_stats(L)) return -1;
     fill_u32(L, L->d_scstat.p, 0, 3 * S);
     LAUNCH(k_scene_stats, sf_grid(L->D.nub), NT, L->D, L->d_scstat.p);
-    std::vector<uint32_t> o(3 * S);
-    SF_HIP(hipMemcpyAsync(o.data(), L->d_scstat.p, sizeof(uint32_t) * 3 * S, hipMemcpyDeviceToHost, L->st));
-    SF_HIP(hipStreamSynchronize(L->st));
+    if (L->h_scstat.reserve(3 * S)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    if (!L->ev_stat) SF_HIP(hipEventCreateWithFlags(&L->ev_stat, hipEventDisableTiming));
+    SF_HIP(hipMemcpyAsync(L->h_scstat.p, L->d_scstat.p, sizeof(uint32_t) * 3 * S, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipEventRecord(L->ev_stat, L->st));
+    L->stat_pending = true;
+    L->stat_n = S;
     for (size_t si = 0; si < S; si++) {
         HScene& sc = L->sc[si];
         uint32_t stat = 0, unsolved = 0;
         for (int sl = 0; sl < sc.nst; sl++) { const HStruct& T = L->sts[sc.st0 + sl]; stat += T.now; if (!T.solved) unsolved++; }
-        sc.stats[0] = (float)stat; sc.stats[1] = (float)o[3 * si + 1]; sc.stats[2] = (float)o[3 * si]; sc.stats[3] = (float)o[3 * si + 2];
-        sc.stats[4] = (float)sc.ncs; sc.stats[14] = (float)unsolved; sc.stats[15] = (float)sc.nst;
+        sc.stats[0] = (float)stat; sc.stats[4] = (float)sc.ncs; sc.stats[14] = (float)unsolved; sc.stats[15] = (float)sc.nst;
+    }
+    return 0;
+}
+int finish_stats(Lat* L) {
+    if (!L->stat_pending) return 0;
+    L->stat_pending = false;
+    SF_HIP(hipEventSynchronize(L->ev_stat));
+    const uint32_t* o = L->h_scstat.p;
+    for (size_t si = 0; si < L->stat_n && si < L->sc.size(); si++) {
+        HScene& sc = L->sc[si];
+        sc.stats[1] = (float)o[3 * si + 1]; sc.stats[2] = (float)o[3 * si]; sc.stats[3] = (float)o[3 * si + 2];
     }
     return 0;
 }
@@ -3336,6 +3383,9 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     if (hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) != hipSuccess) { delete h; sf::set_error(SF_E_HIP, "stream"); return nullptr; }
     if (lat_alloc(L) || lat_contacts_alloc(L, (uint32_t)std::max<int>(1 << 16, 4 * brick_cap)) || lat_hash_alloc(L, 1024) || L->d_grid.alloc(1, L->st)) { delete h; return nullptr; }
     L->D.grid = L->d_grid.p;
+    if (L->h_ctrbuf.reserve(C_NCTR)) { delete h; sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return nullptr; }
+    L->h_ctr = L->h_ctrbuf.p;
+    memset(L->h_ctr, 0, sizeof(uint32_t) * C_NCTR);
     size_t B = brick_cap, N = body_cap;
     L->mat.assign(B, 0); L->sid.assign(B, 255); L->infill.assign(B, 0); L->body.assign(B, -1); L->bxyz.assign(B, make_int4(0, 0, 0, 0));
     L->next.assign(B, NONE); L->nb6.assign(6 * B, NONE); L->stamp.assign(B, 0); L->q.assign(B, 0);
@@ -3454,6 +3504,7 @@ int sf_lat_step(sf_lat* h, float dt) {
 }
 int sf_lat_stats(sf_lat* h, int scene, float* out16) {
     if (!h || !out16 || scene < 0 || scene >= (int)h->h.L.sc.size()) return SF_E_ARG;
+    if (finish_stats(&h->h.L)) return SF_E_HIP;
     memcpy(out16, h->h.L.sc[scene].stats, sizeof(float) * 16);
     return 0;
 }
