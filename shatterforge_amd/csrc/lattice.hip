@@ -1565,6 +1565,7 @@ __global__ void __launch_bounds__(NT) k_scene_stats(LD D, uint32_t* out) {
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) { GS(i, n) p[i] = v; }
+__global__ void k_fill2_u32(uint32_t* p, uint32_t* q, uint32_t v, size_t n) { GS(i, n) { p[i] = v; q[i] = v; } }
 __global__ void __launch_bounds__(NT) k_csr_set(LD D, const int4* r, uint32_t n) { GS(t, n) { int4 x = r[t]; D.boff[x.x] = (uint32_t)x.y; D.bcnt[x.x] = (uint32_t)x.z; } }
 __global__ void __launch_bounds__(NT) k_nc_clear(LD D) { GS(ub, D.nub) D.nc[D.ubody[ub]] = 0u; }
 // the used slots' dirty flags, in list order, cleared behind
@@ -2434,10 +2435,8 @@ int upload_scenes(Lat* L) {
 
 int fill_u32(Lat* L, uint32_t* p, uint32_t v, size_t n) { if (n) LAUNCH(k_fill_u32, sf_grid(n), NT, p, v, n); return 0; }
 // every counter but the killed-body count, which accumulates until the host has drained the list
-void reset_ctr(Lat* L) {
-    fill_u32(L, L->D.ctr, 0, C_NKILL);
-    fill_u32(L, L->D.ctr + C_NKILL + 1, 0, C_NCTR - C_NKILL - 1);
-}
+__global__ void k_reset_ctr(uint32_t* c) { uint32_t i = threadIdx.x; if (i < (uint32_t)C_NCTR && i != (uint32_t)C_NKILL) c[i] = 0u; }
+void reset_ctr(Lat* L) { LAUNCH(k_reset_ctr, 1, 64, L->D.ctr); }
 int finish_stats(Lat* L);
 // the device counters into pinned memory; the wait also completes every read-back queued before it
 int read_ctr(Lat* L) {
@@ -2735,30 +2734,33 @@ int substep(Lat* L, float h) {
     LAUNCH(k_cache, gb, NT, D, 1, h);
     LAUNCH(k_wpos, gd, NT, D);
     // hash
-    fill_u32(L, D.hcnt, 0, L->tsize);
-    fill_u32(L, D.hfill, 0, L->tsize);
+    LAUNCH(k_fill2_u32, sf_grid(L->tsize), NT, D.hcnt, D.hfill, 0u, (size_t)L->tsize);
     LAUNCH(k_hash_count, gd, NT, D);
     scan(L, D.hcnt, D.hoff, D.ctr + C_NCTR0, L->tsize);
     LAUNCH(k_hash_fill, gd, NT, D);
     segsort(L, D.hoff, D.hcnt, D.hent, L->tsize);
-    // contacts: count, size, write
+    // contacts: count, size, write. The write pass runs without waiting for the count: the contact arrays hold four
+    // per brick, and a substep that needs more grows them and writes again (the pass is repeatable: contact counts
+    // restart from zero, stamps and wake flags are idempotent)
     reset_ctr(L);
     LAUNCH(k_contacts<false>, gw, NT, D);
     scan(L, D.ccnt, D.coff, D.ctr + C_NCS, L->ndyn);
-    if (read_ctr(L)) return -1;
-    if (drain_kills(L)) return -1;
-    uint32_t ncs = L->h_ctr[C_NCS];
-    if (ncs > L->ccap) {
-        if (lat_contacts_alloc(L, std::max<uint32_t>(ncs + ncs / 2, 1u << 16))) return -1;
-    }
-    LAUNCH(k_nc_clear, gb, NT, D);
-    LAUNCH(k_contacts<true>, gw, NT, D);
-    LAUNCH(k_scene_c0, 1, 64, D);
-    LAUNCH(k_plough_flag, sf_grid(ncs), NT, D);
     if (L->h_c0.reserve(L->sc.size())) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
     const uint32_t* c0 = L->h_c0.p;
-    SF_HIP(hipMemcpyAsync(L->h_c0.p, D.sc_c0, sizeof(uint32_t) * L->sc.size(), hipMemcpyDeviceToHost, L->st));
-    if (read_ctr(L)) return -1;
+    for (int pass = 0;; pass++) {
+        LAUNCH(k_nc_clear, gb, NT, D);
+        LAUNCH(k_contacts<true>, gw, NT, D);
+        LAUNCH(k_scene_c0, 1, 64, D);
+        LAUNCH(k_plough_flag, sf_grid(L->ccap, NT, 1024), NT, D);
+        SF_HIP(hipMemcpyAsync(L->h_c0.p, D.sc_c0, sizeof(uint32_t) * L->sc.size(), hipMemcpyDeviceToHost, L->st));
+        if (read_ctr(L)) return -1;
+        uint32_t need = L->h_ctr[C_NCS];
+        if (need <= L->ccap) break;
+        if (pass > 0 || lat_contacts_alloc(L, std::max<uint32_t>(need + need / 2, 1u << 16))) { sf::set_error(SF_E_CAP, "contact arrays"); return SF_E_CAP; }
+        LAUNCH(k_fill_u32, 1, 64, D.ctr + C_NPLOUGH, 0u, (size_t)1);
+    }
+    if (drain_kills(L)) return -1;
+    const uint32_t ncs = L->h_ctr[C_NCS];
     for (size_t si = 0; si < L->sc.size(); si++) {
         L->sc[si].c0 = c0[si];
         uint32_t end = si + 1 < L->sc.size() ? c0[si + 1] : ncs;

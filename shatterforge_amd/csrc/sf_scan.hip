@@ -75,10 +75,32 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_tile_scan(const uint32_t* in, 
     for (uint32_t k = 0; k < SCAN_ITEMS; k++) { if (base + k < n) out[base + k] = run; run += v[k]; }
 }
 
+// short arrays: one block walks the tiles in order, carrying the running total (one launch instead of three)
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_one(const uint32_t* in, uint32_t* out, uint32_t* total, const uint32_t* n_dev, uint32_t n_fixed, uint32_t cap) {
+    const uint32_t n = count_of(n_dev, n_fixed, cap);
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < n; t0 += SCAN_TILE) {
+        const uint32_t base = t0 + threadIdx.x * SCAN_ITEMS;
+        uint32_t v[SCAN_ITEMS];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_ITEMS; k++) { v[k] = base + k < n ? in[base + k] : 0u; sum += v[k]; }
+        uint32_t tot;
+        uint32_t run = carry + block_excl_scan(sum, &tot);
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_ITEMS; k++) { if (base + k < n) out[base + k] = run; run += v[k]; }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
 void scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, const uint32_t* n_dev, uint32_t n_fixed, uint32_t cap,
               uint32_t* scratch, hipStream_t s) {
     uint32_t tiles = scan_tiles(cap);
-    if (tiles == 0) tiles = 1;
+    if (tiles <= 4) {
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(SCAN_THREADS), 0, s, in, out, total, n_dev, n_fixed, cap);
+        return;
+    }
     hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(SCAN_THREADS), 0, s, in, scratch, n_dev, n_fixed, cap);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, s, scratch, tiles, total);
     hipLaunchKernelGGL(k_tile_scan, dim3(tiles), dim3(SCAN_THREADS), 0, s, in, out, scratch, n_dev, n_fixed, cap);
