@@ -116,17 +116,20 @@ __global__ void __launch_bounds__(NT) k_hash_fill(LD D) {
         D.hent[D.hoff[s] + atomicAdd(&D.hfill[s], 1u)] = k;
     }
 }
-
-// ---------------------------------------------------------------- segmented sort (ascending u32 values)
-// segments of at most a wavefront: bitonic within one wave; longer ones: k_segsort_big
-// segments of 2-64 values: each wave takes 64 consecutive segments, reads their sizes at once and sorts the ones in
-// range one after another, a bitonic network across the lanes
-__global__ void __launch_bounds__(NT) k_segsort(const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
+// every segment of val[off[s] .. off[s] + cnt[s]) sorted ascending (in-cell and per-brick lists in a fixed order):
+// each block takes NT consecutive segments; its waves sort the segments of 2-64 values by a bitonic network across
+// the lanes, then the block sorts the larger ones one by one (bitonic in LDS up to 4096 values, insertion beyond)
+__global__ void __launch_bounds__(NT) k_segsort_all(const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
+    __shared__ uint32_t sh[4096];
+    __shared__ uint32_t big[NT];
+    __shared__ uint32_t nbig;
     const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    for (uint32_t base = wave * 64; base < nseg; base += nwaves * 64) {
-        uint32_t s = base + lane;
-        uint32_t my_n = s < nseg ? cnt[s] : 0u, my_o = s < nseg ? off[s] : 0u;
+    for (uint32_t base = blockIdx.x * NT; base < nseg; base += gridDim.x * NT) {
+        if (threadIdx.x == 0) nbig = 0u;
+        __syncthreads();
+        uint32_t s0 = base + threadIdx.x;
+        uint32_t my_n = s0 < nseg ? cnt[s0] : 0u, my_o = s0 < nseg ? off[s0] : 0u;
+        if (my_n > 64u) big[atomicAdd(&nbig, 1u)] = s0;
         unsigned long long todo = __ballot(my_n >= 2u && my_n <= 64u);
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
@@ -141,19 +144,6 @@ __global__ void __launch_bounds__(NT) k_segsort(const uint32_t* off, const uint3
                 }
             if ((uint32_t)lane < n) val[o + lane] = v;
         }
-    }
-}
-// segments of more than 64 values: each block reads the sizes of NT consecutive segments at once, lists the large
-// ones in LDS and sorts them one by one (bitonic in LDS up to 4096 values, insertion beyond)
-__global__ void __launch_bounds__(NT) k_segsort_big(const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
-    __shared__ uint32_t sh[4096];
-    __shared__ uint32_t big[NT];
-    __shared__ uint32_t nbig;
-    for (uint32_t base = blockIdx.x * NT; base < nseg; base += gridDim.x * NT) {
-        if (threadIdx.x == 0) nbig = 0u;
-        __syncthreads();
-        uint32_t s0 = base + threadIdx.x;
-        if (s0 < nseg && cnt[s0] > 64u) big[atomicAdd(&nbig, 1u)] = s0;
         __syncthreads();
         const uint32_t nb = nbig;
         for (uint32_t q = 0; q < nb; q++) {
@@ -208,6 +198,7 @@ __device__ __forceinline__ int static_brick(const LD& D, const SceneK& S, int x,
 // stamps, wake requests, per-body contact counts).
 template <bool WRITE>
 __global__ void __launch_bounds__(NT) k_contacts(LD D) {
+    if (!WRITE) GS(ub, D.nub) D.nc[D.ubody[ub]] = 0u;  // the write pass counts contacts per body from zero
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     for (uint32_t k = wave; k < D.ndyn; k += nwaves) {
@@ -305,17 +296,16 @@ __global__ void __launch_bounds__(NT) k_contacts(LD D) {
         }, true);
     }
 }
-// first contact of every scene (its first dynamic slot's offset)
-__global__ void k_scene_c0(LD D) {
-    GS(s, D.nscene) {
-        SceneK S = D.sc[s];
-        D.sc_c0[s] = S.dyn0 < D.ndyn ? D.coff[S.dyn0] : D.ctr[C_NCS];
-    }
-}
 
 __device__ __forceinline__ LV vel_at(const LD& D, int b, LV r) { return b < 0 ? lv(0.0f, 0.0f, 0.0f) : add(lv4(D.v[b]), cross(lv4(D.w[b]), r)); }
 
 __global__ void __launch_bounds__(NT) k_plough_flag(LD D) {
+    // first contact of every scene (its first dynamic slot's offset)
+    if (blockIdx.x == 0)
+        for (uint32_t s = threadIdx.x; s < D.nscene; s += NT) {
+            SceneK S = D.sc[s];
+            D.sc_c0[s] = S.dyn0 < D.ndyn ? D.coff[S.dyn0] : D.ctr[C_NCS];
+        }
     uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
     const Tables& T = tables();
     GS(c, ncs) {
@@ -353,6 +343,8 @@ __device__ __forceinline__ float eff_mass_split(const LD& D, int a, float na, in
 }
 
 __global__ void __launch_bounds__(NT) k_prep(LD D, float h) {
+    // velocities before the solve (prep only reads them)
+    GS(ub, D.nub) { const uint32_t b = D.ubody[ub]; if (D.alive[b]) { D.vpre[b] = D.v[b]; D.wpre[b] = D.w[b]; } }
     uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
     GS(c, ncs) {
         if (!D.con[c]) continue;
@@ -381,12 +373,6 @@ __global__ void __launch_bounds__(NT) k_prep(LD D, float h) {
         float bias = minf(0.2f / h * maxf(D.cdepth[c] - 0.01f, 0.0f), 1.0f);
         if (vn < -2.0f) bias = maxf(bias, -0.12f * vn);
         D.cbias[c] = bias;
-    }
-}
-__global__ void __launch_bounds__(NT) k_vpre(LD D) {
-    GS(ub, D.nub) {
-        const uint32_t b = D.ubody[ub];
-        if (D.alive[b]) { D.vpre[b] = D.v[b]; D.wpre[b] = D.w[b]; }
     }
 }
 
@@ -976,10 +962,18 @@ __global__ void __launch_bounds__(NT) k_impact_flag(LD D) {
         if (slot < D.pcap) D.ilist[slot] = c;
     }
 }
+__device__ __forceinline__ void integrate_body(LD& D, uint32_t b, float h);
+// integrate (src/lib.rs:1703-1745), then the body's wake request from this substep's contacts, both per body
 __global__ void __launch_bounds__(NT) k_integrate(LD D, float h) {
     GS(ub, D.nub) {
         const uint32_t b = D.ubody[ub];
-        if (!D.alive[b] || D.asleep[b]) continue;
+        integrate_body(D, b, h);
+        if (D.wake[b]) { D.wake[b] = 0; if (D.alive[b]) { D.asleep[b] = 0; D.sleep[b] = 0.0f; } }
+    }
+}
+__device__ __forceinline__ void integrate_body(LD& D, uint32_t b, float h) {
+    {
+        if (!D.alive[b] || D.asleep[b]) return;
         LV v = lv4(D.v[b]), w = lv4(D.w[b]);
         float vl = len(v);
         if (vl > 60.0f) v = mul(v, 60.0f / vl);
@@ -1006,12 +1000,6 @@ __global__ void __launch_bounds__(NT) k_integrate(LD D, float h) {
         } else D.sleep[b] = 0.0f;
         D.pos[b] = f4(pos); D.v[b] = f4(v); D.w[b] = f4(w);
         if (pos.y < -15.0f || fabsf(pos.x) > 190.0f || fabsf(pos.z) > 190.0f || pos.y > 120.0f) kill_body_dev(D, b);
-    }
-}
-__global__ void __launch_bounds__(NT) k_wakes(LD D) {
-    GS(ub, D.nub) {
-        const uint32_t b = D.ubody[ub];
-        if (D.wake[b]) { D.wake[b] = 0; if (D.alive[b]) { D.asleep[b] = 0; D.sleep[b] = 0.0f; } }
     }
 }
 
@@ -1234,20 +1222,20 @@ __global__ void __launch_bounds__(ST_NT) k_sor_scene(LD D, int mode, uint32_t ld
 }
 
 // structures: clear the new resting loads, then mix them in
-__global__ void __launch_bounds__(NT) k_extn_clear(LD D) {
-    GS(i, D.nb) {
-        int sc = D.bxyz[i].w;
-        int s = D.sid[i];
-        const SceneK& S = D.sc[sc];
-        if (s < S.nst && D.st_act[S.st0 + s]) D.extn[i] = 0.0f;
-    }
-}
-__global__ void __launch_bounds__(NT) k_ext_mix(LD D) {
-    GS(i, D.nb) {
-        int sc = D.bxyz[i].w;
-        int s = D.sid[i];
-        const SceneK& S = D.sc[sc];
-        if (s < S.nst && D.st_act[S.st0 + s]) D.ext[i] = D.ext[i] * 0.8f + D.extn[i] * 0.2f;
+// the structural pass's clears in one launch: per structure maximum ratio and flags, per scene maximum, resting-list
+// counts, new resting loads of active structures, and the event counters
+__global__ void __launch_bounds__(NT) k_struct_clear(LD D, uint32_t ns, uint32_t nsc) {
+    const uint32_t n = max(max(D.nb, ns), nsc);
+    GS(i, n) {
+        if (i < ns) { D.st_maxr[i] = 0u; D.st_flags[i] = 0u; }
+        if (i < nsc) D.sc_gmax[i] = 0u;
+        if (i < D.nb) {
+            D.lcnt[i] = 0u; D.lfill[i] = 0u;
+            int sc = D.bxyz[i].w, s = D.sid[i];
+            const SceneK& S = D.sc[sc];
+            if (s < S.nst && D.st_act[S.st0 + s]) D.extn[i] = 0.0f;
+        }
+        if (i == 0) { D.ctr[C_NEV] = 0u; D.ctr[C_NBW] = 0u; }
     }
 }
 // resting loads: a body's weight shared between the bricks that rest on a structure brick or on the ground
@@ -1284,13 +1272,20 @@ __global__ void __launch_bounds__(NT) k_rest_emit(LD D, int fill) {
         else D.lent[D.loff[j] + atomicAdd(&D.lfill[j], 1u)] = k;
     }
 }
-__global__ void __launch_bounds__(NT) k_rest_sum(LD D) {
+// resting loads summed per brick from its list, then mixed into the brick's external load (structure bricks only)
+__global__ void __launch_bounds__(NT) k_rest_sum_mix(LD D) {
     GS(j, D.nb) {
         uint32_t n = D.lcnt[j];
-        if (!n) continue;
-        float s = D.extn[j];
-        for (uint32_t e = 0; e < n; e++) s += D.bshare[D.dyn_body[D.lent[D.loff[j] + e]]];
-        D.extn[j] = s;
+        float en = D.extn[j];
+        if (n) {
+            float s = en;
+            for (uint32_t e = 0; e < n; e++) s += D.bshare[D.dyn_body[D.lent[D.loff[j] + e]]];
+            D.extn[j] = s;
+            en = s;
+        }
+        int sc = D.bxyz[j].w, sl = D.sid[j];
+        const SceneK& S = D.sc[sc];
+        if (sl < S.nst && D.st_act[S.st0 + sl]) D.ext[j] = D.ext[j] * 0.8f + en * 0.2f;
     }
 }
 // pieces of 8+ bricks with no support stamped for 10 frames start their load field over
@@ -1565,7 +1560,11 @@ __global__ void __launch_bounds__(NT) k_scene_stats(LD D, uint32_t* out) {
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) { GS(i, n) p[i] = v; }
-__global__ void k_fill2_u32(uint32_t* p, uint32_t* q, uint32_t v, size_t n) { GS(i, n) { p[i] = v; q[i] = v; } }
+// a substep's start: the hash counts and fill cursors, and every counter but the killed-body count
+__global__ void __launch_bounds__(NT) k_hash_clear(LD D) {
+    GS(i, D.tmask + 1u) { D.hcnt[i] = 0u; D.hfill[i] = 0u; }
+    if (blockIdx.x == 0 && threadIdx.x < (uint32_t)C_NCTR && threadIdx.x != (uint32_t)C_NKILL) D.ctr[threadIdx.x] = 0u;
+}
 __global__ void __launch_bounds__(NT) k_csr_set(LD D, const int4* r, uint32_t n) { GS(t, n) { int4 x = r[t]; D.boff[x.x] = (uint32_t)x.y; D.bcnt[x.x] = (uint32_t)x.z; } }
 __global__ void __launch_bounds__(NT) k_nc_clear(LD D) { GS(ub, D.nub) D.nc[D.ubody[ub]] = 0u; }
 // the used slots' dirty flags, in list order, cleared behind
@@ -2723,8 +2722,7 @@ int scan(Lat* L, const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t n)
 }
 void segsort(Lat* L, const uint32_t* off, const uint32_t* cnt, uint32_t* val, uint32_t nseg) {
     if (!nseg) return;
-    LAUNCH(k_segsort, sf_grid(nseg, NT, 16384), NT, off, cnt, val, nseg);
-    LAUNCH(k_segsort_big, sf_grid(nseg, NT, 4096), NT, off, cnt, val, nseg);
+    LAUNCH(k_segsort_all, sf_grid(nseg, NT, 4096), NT, off, cnt, val, nseg);
 }
 
 int substep(Lat* L, float h) {
@@ -2734,7 +2732,7 @@ int substep(Lat* L, float h) {
     LAUNCH(k_cache, gb, NT, D, 1, h);
     LAUNCH(k_wpos, gd, NT, D);
     // hash
-    LAUNCH(k_fill2_u32, sf_grid(L->tsize), NT, D.hcnt, D.hfill, 0u, (size_t)L->tsize);
+    LAUNCH(k_hash_clear, sf_grid(L->tsize), NT, D);
     LAUNCH(k_hash_count, gd, NT, D);
     scan(L, D.hcnt, D.hoff, D.ctr + C_NCTR0, L->tsize);
     LAUNCH(k_hash_fill, gd, NT, D);
@@ -2742,15 +2740,13 @@ int substep(Lat* L, float h) {
     // contacts: count, size, write. The write pass runs without waiting for the count: the contact arrays hold four
     // per brick, and a substep that needs more grows them and writes again (the pass is repeatable: contact counts
     // restart from zero, stamps and wake flags are idempotent)
-    reset_ctr(L);
     LAUNCH(k_contacts<false>, gw, NT, D);
     scan(L, D.ccnt, D.coff, D.ctr + C_NCS, L->ndyn);
     if (L->h_c0.reserve(L->sc.size())) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
     const uint32_t* c0 = L->h_c0.p;
     for (int pass = 0;; pass++) {
-        LAUNCH(k_nc_clear, gb, NT, D);
+        if (pass > 0) LAUNCH(k_nc_clear, gb, NT, D);  // (the count pass cleared them the first time)
         LAUNCH(k_contacts<true>, gw, NT, D);
-        LAUNCH(k_scene_c0, 1, 64, D);
         LAUNCH(k_plough_flag, sf_grid(L->ccap, NT, 1024), NT, D);
         SF_HIP(hipMemcpyAsync(L->h_c0.p, D.sc_c0, sizeof(uint32_t) * L->sc.size(), hipMemcpyDeviceToHost, L->st));
         if (read_ctr(L)) return -1;
@@ -2768,8 +2764,7 @@ int substep(Lat* L, float h) {
     }
     if (L->h_ctr[C_NPLOUGH] && plough_phase(L, L->h_ctr[C_NPLOUGH])) return -1;
     const unsigned gc = sf_grid(std::max<uint32_t>(ncs, 1));
-    LAUNCH(k_prep, gc, NT, D, h);
-    LAUNCH(k_vpre, gb, NT, D);
+    LAUNCH(k_prep, std::max(gc, gb), NT, D, h);
     // every scene small enough: one workgroup per scene solves it whole; else the colour-by-colour launches
     bool fused = L->fuse_max > 0;
     L->slist.clear();
@@ -2827,7 +2822,6 @@ int substep(Lat* L, float h) {
     if (drain_kills(L)) return -1;
     if (L->h_ctr[C_NIMPACT] && impact_phase(L, L->h_ctr[C_NIMPACT])) return -1;
     LAUNCH(k_integrate, gb, NT, D, h);
-    LAUNCH(k_wakes, gb, NT, D);
     SF_HIP(hipGetLastError());
     return 0;
 }
@@ -2890,19 +2884,13 @@ int structural(Lat* L, float dt) {
     L->sp[7] += ms_since(t);
     t = std::chrono::steady_clock::now();
     size_t ns = L->sts.size();
-    fill_u32(L, D.st_maxr, 0, ns);
-    fill_u32(L, D.st_flags, 0, ns);
-    fill_u32(L, D.sc_gmax, 0, L->sc.size());
-    LAUNCH(k_extn_clear, gbr, NT, D);
+    LAUNCH(k_struct_clear, sf_grid(std::max<size_t>(std::max<size_t>(L->nb, ns), L->sc.size())), NT, D, (uint32_t)ns, (uint32_t)L->sc.size());
     LAUNCH(k_rest_count, gw, NT, D);
-    fill_u32(L, D.lcnt, 0, L->nb);
-    fill_u32(L, D.lfill, 0, L->nb);
     LAUNCH(k_rest_emit, gd, NT, D, 0);
     scan(L, D.lcnt, D.loff, D.ctr + C_NCTR0, L->nb);
     LAUNCH(k_rest_emit, gd, NT, D, 1);
     segsort(L, D.loff, D.lcnt, D.lent, L->nb);
-    LAUNCH(k_rest_sum, gbr, NT, D);
-    LAUNCH(k_ext_mix, gbr, NT, D);
+    LAUNCH(k_rest_sum_mix, gbr, NT, D);
     LAUNCH(k_unsupported, gw, NT, D);
     if (ns) {
         hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(ns, 4096)), dim3(NT), 0, L->st, D, 0);
@@ -2913,7 +2901,6 @@ int structural(Lat* L, float dt) {
     LAUNCH(k_sor_body, std::min<uint32_t>(std::max<uint32_t>(D.nub, 1), 8192), BD_NT, D);
     LAUNCH(k_stress_clear, gbr, NT, D);
     LAUNCH(k_body_judge_prep, gb, NT, D, dt);
-    reset_ctr(L);
     LAUNCH(k_stress_static, gbr, NT, D, dt);
     LAUNCH(k_stress_body, gd, NT, D);
     LAUNCH(k_lt_decay, gb, NT, D, dt);
