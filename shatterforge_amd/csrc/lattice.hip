@@ -1065,6 +1065,17 @@ __device__ __forceinline__ void block_max2(float& a, float& b, float* sh) {
     for (int k = 1; k < BT / 64; k++) { a = fmaxf(a, sh[k]); b = fmaxf(b, sh[BT / 64 + k]); }
     __syncthreads();
 }
+// the same maximum with one barrier: it also serves as the barrier that ends the caller's last pass, and the
+// caller's next barrier comes before sh is written again
+template <int BT>
+__device__ __forceinline__ void block_max2_tail(float& a, float& b, float* sh) {
+    for (int h = 32; h >= 1; h >>= 1) { a = fmaxf(a, __shfl_xor(a, h, 64)); b = fmaxf(b, __shfl_xor(b, h, 64)); }
+    int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { sh[wv] = a; sh[BT / 64 + wv] = b; }
+    __syncthreads();
+    a = sh[0]; b = sh[BT / 64];
+    for (int k = 1; k < BT / 64; k++) { a = fmaxf(a, sh[k]); b = fmaxf(b, sh[BT / 64 + k]); }
+}
 
 // One workgroup per scene: its structures in order sharing the frame's brick-update budget (src/lib.rs:1268-1281,
 // :1068-1079). mode 0: adaptive sweeps of live structures (st_nmin = sweeps asked for); mode 1: warm-up sweeps
@@ -1165,9 +1176,9 @@ __device__ float sweep_struct(LD& D, uint32_t g, float om, const Phi<LDSP>& phi,
                 mp = fmaxf(mp, po);
             }
         }
-        __syncthreads();
+        if (par == 0) __syncthreads();
     }
-    block_max2<ST_NT>(md, mp, sh);
+    block_max2_tail<ST_NT>(md, mp, sh);
     return md / mp;
 }
 // one structure's sweeps of a frame: adaptive (mode 0, structural :1268-1281) or warm-up (mode 1, warm_step :1068)
