@@ -1001,9 +1001,16 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
 }
 
 // one colour bucket: its manifolds share no moving body, so every lane solves one
-__global__ void __launch_bounds__(NT) k_solve_bucket(Dev D, StepK K, int c) {
-    uint32_t n = D.colcnt[c], o = D.coloff[c];
-    GRID_STRIDE(i, n) solve_one(D, o + i, K);
+// One colour bucket. A bucket holds tens of thousands of manifolds, so workgroups are one wavefront each to spread
+// them over every CU. Workgroups go to the 8 XCDs round robin (block b to XCD b % 8); block b takes the b-th slice of
+// its XCD's contiguous eighth of the bucket, so an XCD's L2 holds the bodies of one region of the bucket instead of
+// all of them. Within a colour no two manifolds share a moving body, so any assignment gives the same result.
+constexpr int SOLVE_NT = 64;
+__global__ void __launch_bounds__(SOLVE_NT) k_solve_bucket(Dev D, StepK K, int c) {
+    const uint32_t n = D.colcnt[c], o = D.coloff[c];
+    const uint32_t G = gridDim.x, b = blockIdx.x, x = b & 7u, q = G >> 3, r = G & 7u;
+    const uint32_t lb = x * q + min(x, r) + (b >> 3);  // a bijection of [0, G)
+    for (uint32_t i = lb * SOLVE_NT + threadIdx.x; i < n; i += G * SOLVE_NT) solve_one(D, o + i, K);
 }
 
 // the small colours from c0 on, one after another in one workgroup
@@ -1448,7 +1455,7 @@ int launch_bucket(HostWorld* w, const StepK& K, int c) {
         while (w->kev.size() < w->kev_used + 2) { hipEvent_t e; SF_HIP(hipEventCreate(&e)); w->kev.push_back(e); }
         SF_HIP(hipEventRecord(w->kev[w->kev_used], s));
     }
-    hipLaunchKernelGGL(k_solve_bucket, dim3(sf_grid(w->h_col[c])), dim3(NT), 0, s, w->D, K, c);
+    hipLaunchKernelGGL(k_solve_bucket, dim3(sf_grid(w->h_col[c], SOLVE_NT, 16384)), dim3(SOLVE_NT), 0, s, w->D, K, c);
     if (w->ktime) {
         SF_HIP(hipEventRecord(w->kev[w->kev_used + 1], s));
         w->kev_used += 2;
