@@ -88,6 +88,7 @@ struct Dev {
     int32_t *mcol, *mcand;
     unsigned long long *mpri, *bmask, *claim;
     uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff, *ovcnt;
+    uint32_t* chunkc;  // per colour, per chunk of NT manifolds: count, then its exclusive offset within the colour
     int32_t* order;
     float4* ovdv;  // overflow velocity changes: 4 per overflow manifold (v, w of side a; v, w of side b)
     // per-slot manifold lists (m * 2 + side), in manifold order
@@ -206,9 +207,10 @@ __global__ void __launch_bounds__(NT) k_fill_table(Dev D) {
 }
 
 // Partners of shape a of higher index, found in the cell where the pair's home is (phys.rs:388-404): different
-// bodies, not both resting, not both static, overlapping boxes. Written to out[0..cap) sorted by index; returns the
-// count (more than cap is an overflow the caller reports).
-__device__ uint32_t partners(const Dev& D, const StepK& K, int a, int32_t* out, uint32_t cap) {
+// bodies, not both resting, not both static, overlapping boxes. The cap smallest go to out (get/set of slot j) in
+// ascending order whatever order the cell table holds them in; returns the full count.
+template <class Get, class Set>
+__device__ uint32_t partners(const Dev& D, const StepK& K, int a, Get get, Set set, uint32_t cap) {
     float4 lo4 = D.slo[a], hi4 = D.shi[a];
     V3 lo = v3f4(lo4), hi = v3f4(hi4);
     int ba = __float_as_int(lo4.w), fa = __float_as_int(hi4.w);
@@ -232,12 +234,12 @@ __device__ uint32_t partners(const Dev& D, const StepK& K, int a, int32_t* out, 
             if (lo.x > hib.x || lob.x > hi.x || lo.y > hib.y || lob.y > hi.y || lo.z > hib.z || lob.z > hi.z) continue;
             V3 m = vmaxv(lo, lob);
             if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
-            if (n < cap) {
-                // insertion into the sorted prefix
-                uint32_t j = n;
-                while (j > 0 && out[j - 1] > b) { out[j] = out[j - 1]; j--; }
-                out[j] = b;
-            }
+            // insertion into the sorted prefix; when it is full, the largest gives way to a smaller one
+            uint32_t j = n < cap ? n : cap;
+            if (n >= cap && (cap == 0 || b > get(cap - 1))) { n++; continue; }
+            if (n >= cap) j = cap - 1;
+            while (j > 0 && get(j - 1) > b) { set(j, get(j - 1)); j--; }
+            set(j, b);
             n++;
         }
     }
@@ -250,8 +252,8 @@ constexpr uint32_t CAND_SLOTS = 48;  // partners a shape may have (a brick in a 
 __global__ void __launch_bounds__(NT) k_find_cand(Dev D, StepK K) {
     GRID_STRIDE(a, (uint32_t)D.ns) {
         int ground = (__float_as_int(D.shi[a].w) & 1) && D.slo[a].y < 0.0f;
-        uint32_t n = partners(D, K, (int)a, D.cslot + (size_t)a * CAND_SLOTS, CAND_SLOTS);
-        if (n > CAND_SLOTS) { atomicAdd(&D.ctr[C_DROP], n - CAND_SLOTS); n = CAND_SLOTS; }
+        int32_t* sl = D.cslot + (size_t)a * CAND_SLOTS;
+        uint32_t n = partners(D, K, (int)a, [&](uint32_t j) { return sl[j]; }, [&](uint32_t j, int v) { sl[j] = v; }, CAND_SLOTS);
         D.ccnt[a] = n + (uint32_t)ground;
         if (ground) atomicAdd(&D.ctr[C_NGROUND], 1u);
         atomicAdd(&D.ctr[C_NPAIR], n);
@@ -265,8 +267,15 @@ __global__ void __launch_bounds__(NT) k_write_cand(Dev D, StepK K) {
         if (o + n > D.ccap) { atomicAdd(&D.ctr[C_DROP], n); continue; }
         int ground = (__float_as_int(D.shi[a].w) & 1) && D.slo[a].y < 0.0f;
         if (ground) D.cand[o++] = make_int2((int)a, -1);
-        const int32_t* sl = D.cslot + (size_t)a * CAND_SLOTS;
-        for (uint32_t k = 0; k < n - (uint32_t)ground; k++) D.cand[o + k] = make_int2((int)a, sl[k]);
+        const uint32_t np = n - (uint32_t)ground;
+        if (np <= CAND_SLOTS) {
+            const int32_t* sl = D.cslot + (size_t)a * CAND_SLOTS;
+            for (uint32_t k = 0; k < np; k++) D.cand[o + k] = make_int2((int)a, sl[k]);
+        } else {
+            // more partners than slots: walk the cells again, sorting all of them straight into the list
+            int2* c = D.cand + o;
+            partners(D, K, (int)a, [&](uint32_t j) { return c[j].y; }, [&](uint32_t j, int v) { c[j] = make_int2((int)a, v); }, np);
+        }
     }
 }
 
@@ -666,6 +675,8 @@ __global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
             D.ovflag[m] = c == K ? 1u : 0u;
         }
         __syncthreads();
+        const uint32_t nchunk = (D.mcap + NT - 1) / NT, chunk = b0 / NT;
+        for (int c = threadIdx.x; c < K; c += NT) D.chunkc[(size_t)c * nchunk + chunk] = hist[c];
         for (int c = threadIdx.x; c <= K; c += NT) {
             uint32_t h = hist[c];
             if (!h) continue;
@@ -677,22 +688,55 @@ __global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
     }
 }
 
-// colour-ordered list: a block reserves its range in each colour once, lanes take ranks inside it from LDS
+// per colour, the exclusive scan of its chunk counts in chunk order (one block per colour)
+__global__ void __launch_bounds__(NT) k_col_chunk_scan(Dev D, int K) {
+    __shared__ uint32_t ws[NT / 64];
+    const int c = blockIdx.x;
+    if (c >= K) return;
+    const uint32_t nm = min(D.ctr[C_NMAN], D.mcap), nchunk = (D.mcap + NT - 1) / NT, used = (nm + NT - 1) / NT;
+    uint32_t* a = D.chunkc + (size_t)c * nchunk;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < used; b0 += NT) {
+        uint32_t i = b0 + threadIdx.x, v = i < used ? a[i] : 0u, x = v;
+        for (int d = 1; d < 64; d <<= 1) { uint32_t y = __shfl_up(x, d, 64); if (lane >= d) x += y; }
+        if (lane == 63) ws[wv] = x;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        for (int k = 0; k < NT / 64; k++) { if (k < wv) before += ws[k]; all += ws[k]; }
+        if (i < used) a[i] = carry + before + x - v;
+        carry += all;
+        __syncthreads();
+    }
+}
+// colour-ordered list, stable: within a colour, manifolds keep their canonical order (chunk offsets from the scan,
+// ranks inside a chunk from wavefront ballots), so a bucket is spatially ordered like the pair list
 __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
-    __shared__ uint32_t cnt[72], base[72];
-    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    __shared__ uint32_t wc[NT / 64][72];
+    const uint32_t nm = min(D.ctr[C_NMAN], D.mcap), nchunk = (D.mcap + NT - 1) / NT;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (uint32_t b0 = blockIdx.x * NT; b0 < nm; b0 += gridDim.x * NT) {
-        for (int c = threadIdx.x; c <= K; c += NT) cnt[c] = 0u;
+        for (int q = threadIdx.x; q < (NT / 64) * 72; q += NT) wc[q / 72][q % 72] = 0u;
         __syncthreads();
         uint32_t m = b0 + threadIdx.x;
         int c = m < nm ? D.mcol[m] : -1;
+        bool col = c >= 0 && c < K;
         uint32_t rank = 0;
-        if (c >= 0 && c < K) rank = atomicAdd(&cnt[c], 1u);
+        unsigned long long todo = __ballot(col);
+        while (todo) {
+            int l = __ffsll((long long)todo) - 1;
+            int cc = __shfl(c, l, 64);
+            unsigned long long same = __ballot(c == cc);
+            if (c == cc) rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+            if (lane == 0) wc[wv][cc] = (uint32_t)__popcll(same);
+            todo &= ~same;
+        }
         __syncthreads();
-        for (int q = threadIdx.x; q < K; q += NT) if (cnt[q]) base[q] = atomicAdd(&D.colfill[q], cnt[q]);
-        __syncthreads();
-        if (c >= 0 && c < K) D.order[D.coloff[c] + base[c] + rank] = (int32_t)m;
-        else if (c == K) D.order[D.coloff[K] + D.ovoff[m]] = (int32_t)m;  // the overflow in manifold order
+        if (col) {
+            uint32_t before = 0;
+            for (int k = 0; k < wv; k++) before += wc[k][c];
+            D.order[D.coloff[c] + D.chunkc[(size_t)c * nchunk + b0 / NT] + before + rank] = (int32_t)m;
+        } else if (c == K) D.order[D.coloff[K] + D.ovoff[m]] = (int32_t)m;  // the overflow in manifold order
         __syncthreads();
     }
 }
@@ -1339,6 +1383,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
+    r |= w->alloc(D.chunkc, 72 * (size_t)((D.mcap + NT - 1) / NT + 1));
     r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap); r |= w->alloc(D.order, D.mcap);
     r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovdv, 4 * (size_t)D.mcap);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blpos, 2 * (size_t)D.mcap);
@@ -1433,6 +1478,7 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_col_count, dim3(gm), dim3(NT), 0, s, D, K.K);
     sf::scan_u32(D.colcnt, D.coloff, nullptr, nullptr, (uint32_t)K.K + 1, 72, w->scan_tmp, s);
     sf::scan_u32(D.ovflag, D.ovoff, nullptr, D.ctr + C_NMAN, 0, D.mcap, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_ovf_count, dim3(gm), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_permute, dim3(gm), dim3(NT), 0, s, D);
