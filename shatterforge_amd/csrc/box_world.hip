@@ -76,23 +76,24 @@ struct Dev {
     int32_t* eshape;
     uint32_t *tcnt, *tstart, *tfill, *tkey;
     int32_t* tent;    // shape of each table entry
-    int32_t* cslot;   // per-shape partner slots
+    uint32_t *epc, *erel;  // per cell entry: partners homed there, offset within the shape's candidate list
     uint32_t *ccnt, *coff;
     int2* cand;
     float4* nar;  // 5 per candidate: n|count, points p|sep
     uint32_t *hit, *hoff;
     ManBuf cur, old;
     float4* srow;  // rows as built, by canonical index (MR_ROWS x mcap), before the move into colour order
-    float4* wrec;  // warm-start records by position: 16 float4 (two sides of 8)
+    float4* wrec;  // warm-start records by body contact-list entry: 8 float4 each (6 used)
     // colouring
     int32_t *mcol, *mcand;
     unsigned long long *mpri, *bmask, *claim;
     uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff, *ovcnt;
     uint32_t* chunkc;  // per colour, per chunk of NT manifolds: count, then its exclusive offset within the colour
     int32_t* order;
-    float4* ovdv;  // overflow velocity changes: 4 per overflow manifold (v, w of side a; v, w of side b)
+    float4* ovl;   // overflow velocity changes by body contact-list entry: v, w (zero outside the overflow)
     // per-slot manifold lists (m * 2 + side), in manifold order
-    uint32_t *blcnt, *bloff, *blfill, *blist, *blpos;
+    uint32_t *blcnt, *bloff, *blfill, *blist;
+    uint32_t *blinv, *minv;  // the list entry of (colour position, side), of (manifold, side); ~0u: the static world
     // islands
     int32_t* parent;
     uint32_t* sst;     // sleep state bits per body (SS_*)
@@ -206,75 +207,112 @@ __global__ void __launch_bounds__(NT) k_fill_table(Dev D) {
     }
 }
 
-// Partners of shape a of higher index, found in the cell where the pair's home is (phys.rs:388-404): different
-// bodies, not both resting, not both static, overlapping boxes. The cap smallest go to out (get/set of slot j) in
-// ascending order whatever order the cell table holds them in; returns the full count.
-template <class Get, class Set>
-__device__ uint32_t partners(const Dev& D, const StepK& K, int a, Get get, Set set, uint32_t cap) {
+// Candidates by cell entry: one thread per (shape, cell) entry finds the shape's partners whose home is that cell
+// (count, then write), one thread per shape gathers its entries' counts and puts its ground candidate first, then
+// one wavefront per shape sorts its partners ascending: the same canonical list as a walk of every cell per shape.
+__device__ __forceinline__ int ground_cand(const Dev& D, uint32_t a) { return (__float_as_int(D.shi[a].w) & 1) && D.slo[a].y < 0.0f; }
+// the cell of entry e of shape a (entries are emitted x-, y-, z-major over the shape's cell range)
+__device__ __forceinline__ void entry_cell(const Dev& D, const StepK& K, uint32_t a, uint32_t e, int& x, int& y, int& z) {
+    CellRange cr = cell_range(v3f4(D.slo[a]), v3f4(D.shi[a]), K.inv_cell);
+    uint32_t k = e - D.eoff[a];
+    uint32_t ny = (uint32_t)(cr.y1 - cr.y0 + 1), nz = (uint32_t)(cr.z1 - cr.z0 + 1);
+    z = cr.z0 + (int)(k % nz); y = cr.y0 + (int)((k / nz) % ny); x = cr.x0 + (int)(k / (nz * ny));
+}
+// Partners b > a of shape a whose pair's home (the cell holding the larger of the two lower corners) is cell (x, y, z),
+// under the pair rules of phys.rs:388-404: different bodies, not both resting, not both static, overlapping boxes.
+// emit(j, b) for the j-th found (cell-table order); returns the count.
+template <class Emit>
+__device__ __forceinline__ uint32_t cell_partners(const Dev& D, const StepK& K, int a, int x, int y, int z, Emit emit) {
     float4 lo4 = D.slo[a], hi4 = D.shi[a];
     V3 lo = v3f4(lo4), hi = v3f4(hi4);
     int ba = __float_as_int(lo4.w), fa = __float_as_int(hi4.w);
-    CellRange cr = cell_range(lo, hi, K.inv_cell);
-    uint32_t n = 0;
-    if (!cr.n) return 0;
-    for (int x = cr.x0; x <= cr.x1; x++) for (int y = cr.y0; y <= cr.y1; y++) for (int z = cr.z0; z <= cr.z1; z++) {
-        uint32_t key = cell_key(x, y, z);
-        uint32_t sl = slot_of(key, D.tmask);
-        uint32_t s0 = D.tstart[sl], s1 = s0 + D.tcnt[sl];
-        for (uint32_t q = s0; q < s1; q++) {
-            if (D.tkey[q] != key) continue;
-            int b = D.tent[q];
-            if (b <= a) continue;
-            float4 lob4 = D.slo[b], hib4 = D.shi[b];
-            int bb = __float_as_int(lob4.w), fb = __float_as_int(hib4.w);
-            if (bb == ba) continue;
-            if (!((fa | fb) & 1)) continue;          // both resting
-            if ((fa & fb & 2)) continue;             // both static
-            V3 lob = v3f4(lob4), hib = v3f4(hib4);
-            if (lo.x > hib.x || lob.x > hi.x || lo.y > hib.y || lob.y > hi.y || lo.z > hib.z || lob.z > hi.z) continue;
-            V3 m = vmaxv(lo, lob);
-            if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
-            // insertion into the sorted prefix; when it is full, the largest gives way to a smaller one
-            uint32_t j = n < cap ? n : cap;
-            if (n >= cap && (cap == 0 || b > get(cap - 1))) { n++; continue; }
-            if (n >= cap) j = cap - 1;
-            while (j > 0 && get(j - 1) > b) { set(j, get(j - 1)); j--; }
-            set(j, b);
-            n++;
-        }
+    uint32_t key = cell_key(x, y, z), sl = slot_of(key, D.tmask);
+    uint32_t s0 = D.tstart[sl], s1 = s0 + D.tcnt[sl], n = 0;
+    for (uint32_t q = s0; q < s1; q++) {
+        if (D.tkey[q] != key) continue;
+        int b = D.tent[q];
+        if (b <= a) continue;
+        float4 lob4 = D.slo[b], hib4 = D.shi[b];
+        int bb = __float_as_int(lob4.w), fb = __float_as_int(hib4.w);
+        if (bb == ba) continue;
+        if (!((fa | fb) & 1)) continue;          // both resting
+        if ((fa & fb & 2)) continue;             // both static
+        V3 lob = v3f4(lob4), hib = v3f4(hib4);
+        if (lo.x > hib.x || lob.x > hi.x || lo.y > hib.y || lob.y > hi.y || lo.z > hib.z || lob.z > hi.z) continue;
+        V3 m = vmaxv(lo, lob);
+        if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
+        emit(n, b);
+        n++;
     }
     return n;
 }
-
-constexpr uint32_t CAND_SLOTS = 48;  // partners a shape may have (a brick in a dense pile has about 13)
-
-// one pass over the grid: each shape's ground candidate and sorted partners into its own slots
-__global__ void __launch_bounds__(NT) k_find_cand(Dev D, StepK K) {
-    GRID_STRIDE(a, (uint32_t)D.ns) {
-        int ground = (__float_as_int(D.shi[a].w) & 1) && D.slo[a].y < 0.0f;
-        int32_t* sl = D.cslot + (size_t)a * CAND_SLOTS;
-        uint32_t n = partners(D, K, (int)a, [&](uint32_t j) { return sl[j]; }, [&](uint32_t j, int v) { sl[j] = v; }, CAND_SLOTS);
-        D.ccnt[a] = n + (uint32_t)ground;
-        if (ground) atomicAdd(&D.ctr[C_NGROUND], 1u);
-        atomicAdd(&D.ctr[C_NPAIR], n);
+__global__ void __launch_bounds__(NT) k_cand_count(Dev D, StepK K) {
+    uint32_t ne = min(D.ctr[C_NE], D.ecap);
+    GRID_STRIDE(e, ne) {
+        uint32_t a = (uint32_t)D.eshape[e];
+        int x, y, z;
+        entry_cell(D, K, a, e, x, y, z);
+        D.epc[e] = cell_partners(D, K, (int)a, x, y, z, [](uint32_t, int) {});
     }
 }
-
-__global__ void __launch_bounds__(NT) k_write_cand(Dev D, StepK K) {
+// per shape: its candidate count (ground first) and each entry's offset within the shape's list
+__global__ void __launch_bounds__(NT) k_cand_shape(Dev D) {
     GRID_STRIDE(a, (uint32_t)D.ns) {
+        uint32_t g = (uint32_t)ground_cand(D, a), o = g, e0 = D.eoff[a], n = D.ecnt[a];
+        uint32_t ne = min(D.ctr[C_NE], D.ecap);
+        for (uint32_t k = 0; k < n; k++) {
+            uint32_t e = e0 + k;
+            if (e >= ne) break;
+            uint32_t c = D.epc[e];
+            D.erel[e] = o;
+            o += c;
+        }
+        D.ccnt[a] = o;
+        if (g) atomicAdd(&D.ctr[C_NGROUND], 1u);
+        if (o > g) atomicAdd(&D.ctr[C_NPAIR], o - g);
+    }
+}
+__global__ void __launch_bounds__(NT) k_cand_write(Dev D, StepK K) {
+    uint32_t ne = min(D.ctr[C_NE], D.ecap);
+    GRID_STRIDE(e, ne) {
+        uint32_t a = (uint32_t)D.eshape[e];
+        if (!D.epc[e]) continue;
+        if (D.coff[a] + D.ccnt[a] > D.ccap) { atomicAdd(&D.ctr[C_DROP], D.epc[e]); continue; }  // the shape's list is dropped whole
+        uint32_t o = D.coff[a] + D.erel[e];
+        int x, y, z;
+        entry_cell(D, K, a, e, x, y, z);
+        int2* out = D.cand + o;
+        cell_partners(D, K, (int)a, x, y, z, [&](uint32_t j, int b) { out[j] = make_int2((int)a, b); });
+    }
+}
+// per shape (one wavefront each): the ground candidate first, the partners ascending after it
+__global__ void __launch_bounds__(NT) k_cand_finish(Dev D) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t a = wave; a < (uint32_t)D.ns; a += nwaves) {
         uint32_t o = D.coff[a], n = D.ccnt[a];
-        if (n == 0) continue;
-        if (o + n > D.ccap) { atomicAdd(&D.ctr[C_DROP], n); continue; }
-        int ground = (__float_as_int(D.shi[a].w) & 1) && D.slo[a].y < 0.0f;
-        if (ground) D.cand[o++] = make_int2((int)a, -1);
-        const uint32_t np = n - (uint32_t)ground;
-        if (np <= CAND_SLOTS) {
-            const int32_t* sl = D.cslot + (size_t)a * CAND_SLOTS;
-            for (uint32_t k = 0; k < np; k++) D.cand[o + k] = make_int2((int)a, sl[k]);
-        } else {
-            // more partners than slots: walk the cells again, sorting all of them straight into the list
-            int2* c = D.cand + o;
-            partners(D, K, (int)a, [&](uint32_t j) { return c[j].y; }, [&](uint32_t j, int v) { c[j] = make_int2((int)a, v); }, np);
+        uint32_t g = (uint32_t)ground_cand(D, a);
+        if (!n) continue;
+        if (o + n > D.ccap) { if (g && lane == 0) atomicAdd(&D.ctr[C_DROP], 1u); continue; }
+        if (g && lane == 0) D.cand[o] = make_int2((int)a, -1);
+        o += g; n -= g;
+        if (n < 2) continue;
+        if (n <= 64) {
+            int v = (uint32_t)lane < n ? D.cand[o + lane].y : 0x7FFFFFFF;
+            for (int k = 2; k <= 64; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    int u = __shfl_xor(v, j, 64);
+                    bool up = (lane & k) == 0, lower = (lane & j) == 0;
+                    v = (lower == up) ? min(v, u) : max(v, u);
+                }
+            if ((uint32_t)lane < n) D.cand[o + lane].y = v;
+        } else if (lane == 0) {
+            for (uint32_t i = 1; i < n; i++) {
+                int2 v = D.cand[o + i];
+                uint32_t j = i;
+                while (j > 0 && D.cand[o + j - 1].y > v.y) { D.cand[o + j] = D.cand[o + j - 1]; j--; }
+                D.cand[o + j] = v;
+            }
         }
     }
 }
@@ -741,39 +779,52 @@ __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
     }
 }
 
-// The warm-start record of a manifold (at position p), one 128-byte half per side: the vectors the warm start adds
-// to that side's body, w += t_k (k < count) then v += tv, w += tf (signs by side), formed from the accumulated
-// impulses with the true masses, exactly as phys.rs:245-262 forms them.
-__device__ __forceinline__ void write_warm(const Dev& D, uint32_t p, int cnt, float4 n4, float4 t14, float4 t24,
-                                           float j1, float j2, float jt, const float* jn, const float4* r2, const float4* r3,
-                                           float4 ia14, float4 ia24, float4 ian4, float4 ib14, float4 ib24, float4 ibn4,
-                                           float ima, float imb) {
+// The warm-start record of one side of a manifold: the vectors the warm start adds to that side's body, w += t_k
+// (k < count) then v += tv, w += tf (signs by side), formed from the accumulated impulses with the true masses,
+// exactly as phys.rs:245-262 forms them. It goes to the body's contact-list entry (8 float4 each), so a body's warm
+// start reads its records in one contiguous run.
+__device__ __forceinline__ void write_warm_side(float4* w, int side, int cnt, float4 n4, float4 t14, float4 t24,
+                                                float j1, float j2, float jt, const float* jn, const float4* r2,
+                                                const float4* r3, float4 i14, float4 i24, float4 in4, float im) {
     float pn = 0.0f;
-    V3 ta[4], tb[4];
+    V3 t[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (k < cnt) { pn += jn[k]; ta[k] = vmul(v3f4(r2[k]), jn[k]); tb[k] = vmul(v3f4(r3[k]), jn[k]); }
-        else { ta[k] = v3(0, 0, 0); tb[k] = v3(0, 0, 0); }
+        if (k < cnt) { pn += jn[k]; t[k] = vmul(v3f4(side ? r3[k] : r2[k]), jn[k]); }
+        else t[k] = v3(0, 0, 0);
     }
     V3 pp = vadd(vadd(vmul(v3f4(n4), pn), vmul(v3f4(t14), j1)), vmul(v3f4(t24), j2));
-    V3 tfa = vadd(vadd(vmul(v3f4(ia14), j1), vmul(v3f4(ia24), j2)), vmul(v3f4(ian4), jt));
-    V3 tfb = vadd(vadd(vmul(v3f4(ib14), j1), vmul(v3f4(ib24), j2)), vmul(v3f4(ibn4), jt));
-    float4* w = D.wrec + (size_t)p * 16;
+    V3 tf = vadd(vadd(vmul(v3f4(i14), j1), vmul(v3f4(i24), j2)), vmul(v3f4(in4), jt));
     float c = (float)cnt;
-    // side a subtracts: stored negated
-    w[0] = f4(vneg(ta[0]), c); w[1] = f4(vneg(ta[1]), 0.0f); w[2] = f4(vneg(ta[2]), 0.0f); w[3] = f4(vneg(ta[3]), 0.0f);
-    w[4] = f4(vneg(vmul(pp, ima)), 0.0f); w[5] = f4(vneg(tfa), 0.0f);
-    w[8] = f4(tb[0], c); w[9] = f4(tb[1], 0.0f); w[10] = f4(tb[2], 0.0f); w[11] = f4(tb[3], 0.0f);
-    w[12] = f4(vmul(pp, imb), 0.0f); w[13] = f4(tfb, 0.0f);
+    if (side == 0) {  // side a subtracts: stored negated
+        w[0] = f4(vneg(t[0]), c); w[1] = f4(vneg(t[1]), 0.0f); w[2] = f4(vneg(t[2]), 0.0f); w[3] = f4(vneg(t[3]), 0.0f);
+        w[4] = f4(vneg(vmul(pp, im)), 0.0f); w[5] = f4(vneg(tf), 0.0f);
+    } else {
+        w[0] = f4(t[0], c); w[1] = f4(t[1], 0.0f); w[2] = f4(t[2], 0.0f); w[3] = f4(t[3], 0.0f);
+        w[4] = f4(vmul(pp, im), 0.0f); w[5] = f4(tf, 0.0f);
+    }
+}
+// both sides of the manifold at position p (ea, eb: the list entries of its two bodies, ~0u for the static world)
+__device__ __forceinline__ void write_warm(const Dev& D, uint32_t ea, uint32_t eb, int cnt, float4 n4, float4 t14,
+                                           float4 t24, float j1, float j2, float jt, const float* jn, const float4* r2,
+                                           const float4* r3, float4 ia14, float4 ia24, float4 ian4, float4 ib14,
+                                           float4 ib24, float4 ibn4, float ima, float imb) {
+    if (ea != ~0u) write_warm_side(D.wrec + (size_t)ea * 8, 0, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ia14, ia24, ian4, ima);
+    if (eb != ~0u) write_warm_side(D.wrec + (size_t)eb * 8, 1, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ib14, ib24, ibn4, imb);
 }
 
-// move the built rows into colour order: the manifold at position p of the colour-ordered list
+// Move the built rows into colour order (the manifold at position p of the colour-ordered list), note the bodies'
+// list entries of each position, write the first warm-start records into them and clear their overflow changes.
 __global__ void __launch_bounds__(NT) k_permute(Dev D) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(p, nm) {
         uint32_t m = (uint32_t)D.order[p];
         D.cur.pos[m] = p;
-        D.cur.hdrp[p] = D.cur.hdr[m];
+        int4 hd = D.cur.hdr[m];
+        D.cur.hdrp[p] = hd;
+        uint32_t ea = hd.z ? D.minv[2 * m] : ~0u, eb = hd.w ? D.minv[2 * m + 1] : ~0u;
+        D.blinv[2 * p] = ea;
+        D.blinv[2 * p + 1] = eb;
 #pragma unroll 4
         for (int r = 0; r < MR_ROWS; r++) MROW(D, D.cur, r, p) = SROW(D, r, m);
         // the first sub-step's warm start, from the impulses carried over from the last step
@@ -784,9 +835,13 @@ __global__ void __launch_bounds__(NT) k_permute(Dev D) {
             r2[k] = SROW(D, MR_P0 + MR_PER_PT * k + 2, m); r3[k] = SROW(D, MR_P0 + MR_PER_PT * k + 3, m); jn[k] = r2[k].w;
         }
         float4 j4 = SROW(D, MR_J, m);
-        write_warm(D, p, (int)SROW(D, MR_CB1, m).w, SROW(D, MR_N, m), SROW(D, MR_T1, m), SROW(D, MR_T2, m), j4.x, j4.y, j4.z,
-                   jn, r2, r3, SROW(D, MR_IA1, m), SROW(D, MR_IA2, m), SROW(D, MR_IAN, m), SROW(D, MR_IB1, m),
+        write_warm(D, ea, eb, (int)SROW(D, MR_CB1, m).w, SROW(D, MR_N, m), SROW(D, MR_T1, m), SROW(D, MR_T2, m), j4.x, j4.y,
+                   j4.z, jn, r2, r3, SROW(D, MR_IA1, m), SROW(D, MR_IA2, m), SROW(D, MR_IAN, m), SROW(D, MR_IB1, m),
                    SROW(D, MR_IB2, m), SROW(D, MR_IBN, m), SROW(D, MR_CA1, m).w, SROW(D, MR_CA2, m).w);
+        // overflow changes: zero for contacts outside the overflow (the overflow pass overwrites its own)
+        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ea != ~0u) { D.ovl[2 * (size_t)ea] = z; D.ovl[2 * (size_t)ea + 1] = z; }
+        if (eb != ~0u) { D.ovl[2 * (size_t)eb] = z; D.ovl[2 * (size_t)eb + 1] = z; }
     }
 }
 
@@ -860,13 +915,12 @@ __global__ void __launch_bounds__(NT) k_bl_sort_big(Dev D) {
     }
 }
 
-// each list entry's warm-start record (position x 2 + side), once the positions are known
-__global__ void __launch_bounds__(NT) k_bl_pos(Dev D) {
+// the list entry of each (manifold, side): the inverse of the sorted lists
+__global__ void __launch_bounds__(NT) k_bl_inv(Dev D) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(e, 2 * nm) {
         if (e >= D.bloff[D.nb] + D.blcnt[D.nb]) continue;
-        uint32_t ent = D.blist[e];
-        D.blpos[e] = D.cur.pos[ent >> 1] * 2u + (ent & 1u);
+        D.minv[D.blist[e]] = e;
     }
 }
 
@@ -908,7 +962,7 @@ __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
             uint32_t e = e0 + lane;
             V3 dv = v3(0, 0, 0), dw = v3(0, 0, 0);
             if (e < n) {
-                const float4* wr = D.wrec + (size_t)D.blpos[o + e] * 8;
+                const float4* wr = D.wrec + (size_t)(o + e) * 8;
                 float4 w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4], w5 = wr[5];
                 int c = (int)w0.w;
                 dw = v3f4(w0);
@@ -934,7 +988,7 @@ __device__ __forceinline__ float slop_fn(float e, float slop) { return e > slop 
 // reference solves it. fa, fb > 0 (the overflow pass): mass splitting, each side's inverse mass and inertia scaled
 // by the number of overflow manifolds at that body and the effective masses recomputed from the scaled ones.
 __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK& K, V3& va, V3& wa, V3& vb, V3& wb,
-                                           V3 tha, V3 thb, V3 dpab, float fa, float fb) {
+                                           V3 tha, V3 thb, V3 dpab, float fa, float fb, uint32_t ea, uint32_t eb) {
     const ManBuf& C = D.cur;
     const bool split = fa > 0.0f;
     // every load of the record up front (they are independent; the stores come at the end)
@@ -1029,17 +1083,21 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     if (unstick) reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;
     // the relax pass ends the sub-step: its impulses are the next sub-step's warm start
     if (!K.use_bias)
-        write_warm(D, m, cnt, n4, t14, t24, j1, j2, jt, njs, r2, r3, ia14, ia24, ian4, ib14, ib24, ibn4, ca14.w, ca24.w);
+        write_warm(D, ea, eb, cnt, n4, t14, t24, j1, j2, jt, njs, r2, r3, ia14, ia24, ian4, ib14, ib24, ibn4, ca14.w, ca24.w);
 }
 
 // one manifold of the colour-ordered solve (p: its position in colour order)
 __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
+    // (the list entries are loaded beside the header, not after it)
+    uint2 e2 = K.use_bias ? make_uint2(~0u, ~0u) : reinterpret_cast<const uint2*>(D.blinv)[m];
     int4 hd = D.cur.hdrp[m];
     int sa = hd.z, sb = hd.w;
     float4 va4 = D.sv[sa], vb4 = D.sv[sb];
     V3 va = v3f4(va4), wa = v3f4(D.sw[sa]), vb = v3f4(vb4), wb = v3f4(D.sw[sb]);
     V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
-    solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, 0.0f, 0.0f);
+    // the relax pass writes the next warm start into the bodies' list entries
+    uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
+    solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, 0.0f, 0.0f, ea, eb);
     if (sa) { D.sv[sa] = f4(va, va4.w); D.sw[sa] = f4(wa, 0.0f); }
     if (sb) { D.sv[sb] = f4(vb, vb4.w); D.sw[sb] = f4(wb, 0.0f); }
 }
@@ -1072,20 +1130,23 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
     uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
     GRID_STRIDE(i, n) {
         uint32_t m = o + i;
+        uint2 e2 = reinterpret_cast<const uint2*>(D.blinv)[m];
         int4 hd = D.cur.hdrp[m];
         int sa = hd.z, sb = hd.w;
         V3 va0 = v3f4(D.sv[sa]), wa0 = v3f4(D.sw[sa]), vb0 = v3f4(D.sv[sb]), wb0 = v3f4(D.sw[sb]);
         V3 va = va0, wa = wa0, vb = vb0, wb = wb0;
         V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
         float fa = sa ? (float)D.ovcnt[sa] : 1.0f, fb = sb ? (float)D.ovcnt[sb] : 1.0f;
-        solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, fa, fb);
-        float4* dv = D.ovdv + 4 * (size_t)i;
-        dv[0] = f4(vsub(va, va0), 0.0f); dv[1] = f4(vsub(wa, wa0), 0.0f);
-        dv[2] = f4(vsub(vb, vb0), 0.0f); dv[3] = f4(vsub(wb, wb0), 0.0f);
+        uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
+        solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, fa, fb,
+                   K.use_bias ? ~0u : ea, K.use_bias ? ~0u : eb);
+        // each side's change into its body's list entry
+        if (ea != ~0u) { D.ovl[2 * (size_t)ea] = f4(vsub(va, va0), 0.0f); D.ovl[2 * (size_t)ea + 1] = f4(vsub(wa, wa0), 0.0f); }
+        if (eb != ~0u) { D.ovl[2 * (size_t)eb] = f4(vsub(vb, vb0), 0.0f); D.ovl[2 * (size_t)eb + 1] = f4(vsub(wb, wb0), 0.0f); }
     }
 }
 // each body takes the average of its copies: the tree sum of the changes over its contacts (zero for contacts not
-// in the overflow), one wavefront per body
+// in the overflow), one wavefront per body reading its list entries' changes in one contiguous run
 __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
@@ -1100,13 +1161,9 @@ __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
             uint32_t e = e0 + lane;
             V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0);
             if (e < n) {
-                uint32_t ent = D.blist[o + e];
-                uint32_t m = ent >> 1;
-                if (D.mcol[m] == K.K) {
-                    const float4* dv = D.ovdv + 4 * (size_t)D.ovoff[m] + ((ent & 1u) ? 2 : 0);
-                    dvv = v3f4(dv[0]);
-                    dww = v3f4(dv[1]);
-                }
+                const float4* dv = D.ovl + 2 * (size_t)(o + e);
+                dvv = v3f4(dv[0]);
+                dww = v3f4(dv[1]);
             }
             av = vadd(av, wave_tree(dvv));
             aw = vadd(aw, wave_tree(dww));
@@ -1371,7 +1428,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
     r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
     r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
-    r |= w->alloc(D.cslot, (size_t)ns * CAND_SLOTS);
+    r |= w->alloc(D.epc, D.ecap); r |= w->alloc(D.erel, D.ecap);
     r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
     r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
     for (ManBuf* M : {&D.cur, &D.old}) {
@@ -1379,14 +1436,14 @@ int world_alloc(HostWorld* w) {
         r |= w->alloc(M->pos, D.mcap); r |= w->alloc(M->hdrp, D.mcap);
     }
     r |= w->alloc(D.srow, (size_t)MR_ROWS * D.mcap);
-    r |= w->alloc(D.wrec, (size_t)16 * D.mcap);
+    r |= w->alloc(D.wrec, (size_t)16 * D.mcap);  // two list entries per manifold
     r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
     r |= w->alloc(D.chunkc, 72 * (size_t)((D.mcap + NT - 1) / NT + 1));
     r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap); r |= w->alloc(D.order, D.mcap);
-    r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovdv, 4 * (size_t)D.mcap);
-    r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blpos, 2 * (size_t)D.mcap);
+    r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovl, 4 * (size_t)D.mcap);
+    r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blinv, 2 * (size_t)D.mcap); r |= w->alloc(D.minv, 2 * (size_t)D.mcap);
     r |= w->alloc(D.parent, nb); r |= w->alloc(D.sst, nb); r |= w->alloc(D.ctr, C_NCTR);
     uint32_t big = std::max<uint32_t>(std::max<uint32_t>(D.tcap, D.ccap), (uint32_t)n1);
     r |= w->alloc(w->scan_tmp, sf::scan_scratch_words(big));
@@ -1450,9 +1507,11 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_emit_entries, dim3(gs), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.tcnt, D.tstart, nullptr, nullptr, D.tcap, D.tcap, w->scan_tmp, s);
     hipLaunchKernelGGL(k_fill_table, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_find_cand, dim3(gs), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_cand_count, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_cand_shape, dim3(gs), dim3(NT), 0, s, D);
     sf::scan_u32(D.ccnt, D.coff, D.ctr + C_NCAND, nullptr, (uint32_t)D.ns, (uint32_t)D.ns, w->scan_tmp, s);
-    hipLaunchKernelGGL(k_write_cand, dim3(gs), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_cand_write, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_cand_finish, dim3(sf_grid((uint64_t)D.ns * 64, NT, 65536)), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[1], s));
     hipLaunchKernelGGL(k_narrow, dim3(gc), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.hit, D.hoff, D.ctr + C_NMAN, D.ctr + C_NCAND, 0, D.ccap, w->scan_tmp, s);
@@ -1481,13 +1540,13 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_ovf_count, dim3(gm), dim3(NT), 0, s, D, K.K);
-    hipLaunchKernelGGL(k_permute, dim3(gm), dim3(NT), 0, s, D);
-    // per-slot contact lists in manifold order
+    // per-slot contact lists in manifold order, then the rows into colour order
     sf::scan_u32(D.blcnt, D.bloff, nullptr, nullptr, (uint32_t)D.nb + 1, (uint32_t)D.nb + 1, w->scan_tmp, s);
     hipLaunchKernelGGL(k_bl_fill, dim3(gm), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort, dim3(sf_grid(((uint64_t)D.nb + 1) * 64, NT, 65536)), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort_big, dim3(1024), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_bl_pos, dim3(sf_grid(2ull * D.mcap)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_bl_inv, dim3(sf_grid(2ull * D.mcap)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_permute, dim3(gm), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
     SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
     SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
