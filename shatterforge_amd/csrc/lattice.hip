@@ -1576,11 +1576,50 @@ __global__ void __launch_bounds__(NT) k_hash_clear(LD D) {
     GS(i, D.tmask + 1u) { D.hcnt[i] = 0u; D.hfill[i] = 0u; }
     if (blockIdx.x == 0 && threadIdx.x < (uint32_t)C_NCTR && threadIdx.x != (uint32_t)C_NKILL) D.ctr[threadIdx.x] = 0u;
 }
-__global__ void __launch_bounds__(NT) k_csr_set(LD D, const int4* r, uint32_t n) { GS(t, n) { int4 x = r[t]; D.boff[x.x] = (uint32_t)x.y; D.bcnt[x.x] = (uint32_t)x.z; } }
+// the dynamic brick lists from one uploaded block: k brick indices, k owning bodies, then (at o_csr) nc records of
+// (body, first, count) for the live bodies' brick ranges
+__global__ void __launch_bounds__(NT) k_csr_set(LD D, const uint8_t* blk, uint32_t k, uint32_t nc, uint32_t o_csr) {
+    GS(t, max(k, nc)) {
+        if (t < k) { D.dyn[t] = reinterpret_cast<const uint32_t*>(blk)[t]; D.dyn_body[t] = reinterpret_cast<const int32_t*>(blk)[k + t]; }
+        if (t < nc) { int4 x = reinterpret_cast<const int4*>(blk + o_csr)[t]; D.boff[x.x] = (uint32_t)x.y; D.bcnt[x.x] = (uint32_t)x.z; }
+    }
+}
 __global__ void __launch_bounds__(NT) k_nc_clear(LD D) { GS(ub, D.nub) D.nc[D.ubody[ub]] = 0u; }
-// the used slots' dirty flags, in list order, cleared behind
-__global__ void __launch_bounds__(NT) k_take_dirty(LD D) {
-    GS(ub, D.nub) { uint32_t b = D.ubody[ub]; D.bdflag[ub] = D.bd_dirty[b]; D.bd_dirty[b] = 0; }
+// The structural pass's results in one block for one copy to the host: the counters, then per structure residual,
+// maximum ratio and flags, per scene maximum ratio, per used body slot its dirty flag (taken: cleared here), and the
+// first kev crush events and kbw bond writes (the host copies the rest when there are more).
+struct PackRB { uint32_t ns, nsc, o_rs, o_mr, o_fl, o_gm, o_bd, o_ek, o_ev, o_bi, o_bv, kev, kbw; };
+__global__ void __launch_bounds__(NT) k_pack_rb(LD D, uint8_t* out, PackRB P) {
+    const uint32_t ns = P.ns, nsc = P.nsc, o_rs = P.o_rs, o_mr = P.o_mr, o_fl = P.o_fl, o_gm = P.o_gm, o_bd = P.o_bd;
+    const uint32_t nev = min(min(D.ctr[C_NEV], D.evcap), P.kev), nbw = min(min(D.ctr[C_NBW], D.bwcap), P.kbw);
+    const uint32_t n = max(max(max(D.nub, ns), max(nsc, (uint32_t)C_NCTR)), max(P.kev, P.kbw));
+    GS(i, n) {
+        if (i < nev) {
+            reinterpret_cast<unsigned long long*>(out + P.o_ek)[i] = D.evkey[i];
+            reinterpret_cast<int4*>(out + P.o_ev)[i] = D.ev[i];
+        }
+        if (i < nbw) {
+            reinterpret_cast<uint32_t*>(out + P.o_bi)[i] = D.bwi[i];
+            reinterpret_cast<float*>(out + P.o_bv)[i] = D.bwv[i];
+        }
+        if (i < (uint32_t)C_NCTR) reinterpret_cast<uint32_t*>(out)[i] = D.ctr[i];
+        if (i < ns) {
+            reinterpret_cast<float*>(out + o_rs)[i] = D.st_resid[i];
+            reinterpret_cast<uint32_t*>(out + o_mr)[i] = D.st_maxr[i];
+            reinterpret_cast<uint32_t*>(out + o_fl)[i] = D.st_flags[i];
+        }
+        if (i < nsc) reinterpret_cast<uint32_t*>(out + o_gm)[i] = D.sc_gmax[i];
+        if (i < D.nub) { uint32_t b = D.ubody[i]; out[o_bd + i] = D.bd_dirty[b]; D.bd_dirty[b] = 0; }
+    }
+}
+// the structures' sweep controls from one uploaded block: sweeps asked for (int32), then active, on, live (bytes)
+__global__ void __launch_bounds__(NT) k_st_ctl(LD D, const uint8_t* blk, uint32_t ns) {
+    GS(g, ns) {
+        D.st_nmin[g] = reinterpret_cast<const int32_t*>(blk)[g];
+        D.st_act[g] = blk[4 * (size_t)ns + g];
+        D.st_on[g] = blk[5 * (size_t)ns + g];
+        D.st_live[g] = blk[6 * (size_t)ns + g];
+    }
 }
 __global__ void k_fill_u8(uint8_t* p, uint8_t v, size_t n) { GS(i, n) p[i] = v; }
 
@@ -1747,6 +1786,34 @@ template <class T> struct HBuf {
     void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }
 };
 
+// A host-to-device block staged through two pinned buffers: begin() hands out the buffer not in flight (waiting for
+// the copy that last used it), end() queues the copy into the device block and marks the buffer in flight.
+struct Stage {
+    HBuf<uint8_t> h[2];
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool rec[2] = {false, false};
+    int cur = 0;
+    DBuf<uint8_t> d;
+    uint8_t* begin(size_t bytes) {
+        cur ^= 1;
+        if (!ev[cur] && hipEventCreateWithFlags(&ev[cur], hipEventDisableTiming) != hipSuccess) { sf::set_error(SF_E_HIP, "event"); return nullptr; }
+        if (rec[cur] && hipEventSynchronize(ev[cur]) != hipSuccess) { sf::set_error(SF_E_HIP, "event wait"); return nullptr; }
+        if (h[cur].reserve(bytes)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return nullptr; }
+        return h[cur].p;
+    }
+    int end(size_t bytes, hipStream_t st) {
+        if (d.n < bytes && d.alloc(std::max<size_t>(bytes + bytes / 2, 4096), st)) return -1;
+        if (hipMemcpyAsync(d.p, h[cur].p, bytes, hipMemcpyHostToDevice, st) != hipSuccess) { sf::set_error(SF_E_HIP, "copy"); return -1; }
+        if (hipEventRecord(ev[cur], st) != hipSuccess) { sf::set_error(SF_E_HIP, "event"); return -1; }
+        rec[cur] = true;
+        return 0;
+    }
+    void release() {
+        for (int k = 0; k < 2; k++) { h[k].release(); if (ev[k]) (void)hipEventDestroy(ev[k]); ev[k] = nullptr; rec[k] = false; }
+        d.release();
+    }
+};
+
 struct Lat {
     int dev = 0;
     hipStream_t st = nullptr;
@@ -1754,7 +1821,10 @@ struct Lat {
     int fuse_max = 32768;
     int lds_max = 0;           // LDS bytes a workgroup may have (device attribute)
     bool lds_attr = false, lds_attr_fs = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
-    std::vector<int32_t> slist;
+    std::vector<int32_t> slist, slist_up;  // scenes with contacts this substep; the list on the device
+    Stage stg_ctl, stg_csr, stg_delta, stg_brec;  // uploads: sweep controls, dynamic brick lists, brick and body deltas
+    DBuf<uint8_t> d_rbk;     // the structural results packed for one copy
+    uint32_t kev_pack = 64, kbw_pack = 256;  // crush events and bond writes the packed block carries
     int bcap = 0, dcap = 0, scap = 0;
     int nb = 0, nbd = 0;                   // bricks / body slots handed out to scenes
     std::vector<HScene> sc;
@@ -1795,20 +1865,17 @@ struct Lat {
     DBuf<int2> d_dis;
     DBuf<uint32_t> d_scstat;
     DBuf<int32_t> d_slist;
-    DBuf<int4> d_csr;
     // host phases' transfers: brick deltas and body records up, body records and brick positions down
-    DBuf<BrickDelta> d_delta;
     DBuf<BodyRec> d_brec;
     DBuf<uint32_t> d_idx;
     DBuf<float4> d_wp;
-    HBuf<BrickDelta> h_delta;
     HBuf<BodyRec> h_brec;
     HBuf<uint32_t> h_idx;
     HBuf<float4> h_wp;
     std::vector<uint8_t> bmark;
     // pinned read-back: counters, scene first contacts, killed bodies, the structural pass's results, scene
     // statistics (read at the next wait), and the scene table (two buffers, each reused once its copy is done)
-    HBuf<uint32_t> h_ctrbuf, h_c0, h_scstat;
+    HBuf<uint32_t> h_ctrbuf, h_scstat;
     HBuf<int32_t> h_kl;
     HBuf<uint8_t> h_rb;
     uint32_t* h_ctr = nullptr;
@@ -1837,10 +1904,10 @@ struct Lat {
 
     ~Lat() {
         for (void* p : owned) if (p) (void)hipFree(p);
-        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release(); d_csr.release();
-        d_delta.release(); d_brec.release(); d_idx.release(); d_wp.release();
-        h_delta.release(); h_brec.release(); h_idx.release(); h_wp.release();
-        h_ctrbuf.release(); h_c0.release(); h_scstat.release(); h_kl.release(); h_rb.release(); h_sk[0].release(); h_sk[1].release();
+        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release();
+        d_brec.release(); d_idx.release(); d_wp.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release();
+        h_brec.release(); h_idx.release(); h_wp.release();
+        h_ctrbuf.release(); h_scstat.release(); h_kl.release(); h_rb.release(); h_sk[0].release(); h_sk[1].release();
         for (hipEvent_t e : {ev_sk[0], ev_sk[1], ev_stat}) if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -1869,13 +1936,13 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.m, N); r |= L->dalloc(D.im, N); r |= L->dalloc(D.sleep, N); r |= L->dalloc(D.rad, N);
     r |= L->dalloc(D.cool, N); r |= L->dalloc(D.lt, N); r |= L->dalloc(D.resid, N); r |= L->dalloc(D.maxr, N);
     r |= L->dalloc(D.nmax, N); r |= L->dalloc(D.nc, N); r |= L->dalloc(D.count, N); r |= L->dalloc(D.boff, N);
-    r |= L->dalloc(D.bcnt, N); r |= L->dalloc(D.ubody, N); r |= L->dalloc(D.bdflag, N);
+    r |= L->dalloc(D.bcnt, N); r |= L->dalloc(D.ubody, N);
     r |= L->dalloc(D.dyn, B); r |= L->dalloc(D.dyn_body, B);
     r |= L->dalloc(D.st_scene, NS); r |= L->dalloc(D.st_lo, NS); r |= L->dalloc(D.st_hi, NS); r |= L->dalloc(D.st_act, NS);
     r |= L->dalloc(D.st_on, NS); r |= L->dalloc(D.st_live, NS); r |= L->dalloc(D.st_nmin, NS); r |= L->dalloc(D.st_resid, NS);
     r |= L->dalloc(D.st_maxr, NS); r |= L->dalloc(D.st_flags, NS); r |= L->dalloc(D.st_red, B); r |= L->dalloc(D.st_rb, NS);
     r |= L->dalloc(D.sten_na, B); r |= L->dalloc(D.sten_nb, B); r |= L->dalloc(D.sten_ka, B); r |= L->dalloc(D.sten_kb, B); r |= L->dalloc(D.sten_s, B);
-    r |= L->dalloc(D.sc_c0, S + 1); r |= L->dalloc(D.sc_gmax, S); r |= L->dalloc(D.sc_fail, 4 * S);
+    r |= L->dalloc(D.sc_gmax, S); r |= L->dalloc(D.sc_fail, 4 * S);
     r |= L->dalloc(D.dslot, B); r |= L->dalloc(D.dkey, B); r |= L->dalloc(D.hent, B);
     r |= L->dalloc(D.ccnt, B); r |= L->dalloc(D.coff, B);
     r |= L->dalloc(D.mask, N); r |= L->dalloc(D.claim, N * (size_t)L->K);
@@ -1885,7 +1952,9 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.ltarget, B); r |= L->dalloc(D.lshare, B); r |= L->dalloc(D.bshare, N);
     r |= L->dalloc(D.bd_sor, N); r |= L->dalloc(D.bd_judge, N); r |= L->dalloc(D.bd_live, N); r |= L->dalloc(D.bd_dt, N);
     r |= L->dalloc(D.bd_dirty, N);
-    r |= L->dalloc(D.ctr, C_NCTR);
+    // the counters, then each scene's first contact: one copy brings back both
+    r |= L->dalloc(D.ctr, CTR_WORDS + S + 1);
+    D.sc_c0 = D.ctr + CTR_WORDS;
     D.pcap = (uint32_t)std::max<size_t>(4096, B);
     r |= L->dalloc(D.plist, D.pcap); r |= L->dalloc(D.ilist, D.pcap); r |= L->dalloc(D.klist, N);
     D.evcap = (uint32_t)std::max<size_t>(4096, B); D.bwcap = (uint32_t)std::max<size_t>(4096, 4 * B);
@@ -2399,15 +2468,17 @@ int rebuild_csr(Lat* L) {
     L->ndyn = k;
     L->D.ndyn = k;
     if (lat_hash_alloc(L, k)) return -1;
-    if (L->d_csr.n < csr.size() && L->d_csr.alloc(std::max<size_t>(csr.size(), 4096), L->st)) return -1;
-    hipStream_t s = L->st;
-    int r = 0;
-    r |= cp_up(L->D.dyn, L->dyn.data(), k, s); r |= cp_up(L->D.dyn_body, L->dyn_body.data(), k, s);
-    r |= cp_up(L->d_csr.p, csr.data(), csr.size(), s);
-    if (r) return r;
-    // live bodies' brick ranges; the ranges of dead bodies are never read
-    if (!csr.empty()) LAUNCH(k_csr_set, sf_grid(csr.size()), NT, L->D, (const int4*)L->d_csr.p, (uint32_t)csr.size());
-    SF_HIP(hipStreamSynchronize(s));
+    // the lists and the live bodies' brick ranges (the ranges of dead bodies are never read) in one upload
+    const size_t nc = csr.size(), o_csr = (8 * (size_t)k + 15) & ~(size_t)15, bytes = o_csr + sizeof(int4) * nc;
+    if (k || nc) {
+        uint8_t* blk = L->stg_csr.begin(bytes);
+        if (!blk) return -1;
+        memcpy(blk, L->dyn.data(), sizeof(uint32_t) * k);
+        memcpy(blk + 4 * (size_t)k, L->dyn_body.data(), sizeof(int32_t) * k);
+        memcpy(blk + o_csr, csr.data(), sizeof(int4) * nc);
+        if (L->stg_csr.end(bytes, L->st)) return -1;
+        LAUNCH(k_csr_set, sf_grid(std::max<size_t>(k, nc)), NT, L->D, (const uint8_t*)L->stg_csr.d.p, k, (uint32_t)nc, (uint32_t)o_csr);
+    }
     return upload_scenes(L);  // the scenes' first dynamic slots moved
 }
 
@@ -2448,9 +2519,10 @@ int fill_u32(Lat* L, uint32_t* p, uint32_t v, size_t n) { if (n) LAUNCH(k_fill_u
 __global__ void k_reset_ctr(uint32_t* c) { uint32_t i = threadIdx.x; if (i < (uint32_t)C_NCTR && i != (uint32_t)C_NKILL) c[i] = 0u; }
 void reset_ctr(Lat* L) { LAUNCH(k_reset_ctr, 1, 64, L->D.ctr); }
 int finish_stats(Lat* L);
-// the device counters into pinned memory; the wait also completes every read-back queued before it
-int read_ctr(Lat* L) {
-    SF_HIP(hipMemcpyAsync(L->h_ctr, L->D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, L->st));
+// the device counters into pinned memory (and with nsc, the first contact of each scene after them, at
+// h_ctr + CTR_WORDS); the wait also completes every read-back queued before it
+int read_ctr(Lat* L, size_t nsc = 0) {
+    SF_HIP(hipMemcpyAsync(L->h_ctr, L->D.ctr, sizeof(uint32_t) * (nsc ? CTR_WORDS + nsc : (size_t)C_NCTR), hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
     L->syncs++;
     return finish_stats(L);
@@ -2576,25 +2648,25 @@ int pull_body_wpos(Lat* L, const std::vector<int>& bodies) {
 int flush_delta(Lat* L) {
     size_t n = L->tlist.size(), nb = L->btlist.size();
     if (n) {
-        if (L->h_delta.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-        if (L->d_delta.n < n && L->d_delta.alloc(std::max<size_t>(n, 4096), L->st)) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
+        BrickDelta* hd = reinterpret_cast<BrickDelta*>(L->stg_delta.begin(sizeof(BrickDelta) * n));
+        if (!hd) return -1;
         for (size_t t = 0; t < n; t++) {
             uint32_t i = L->tlist[t];
-            BrickDelta& x = L->h_delta.p[t];
+            BrickDelta& x = hd[t];
             x.i = i; x.body = L->body[i]; x.mat = L->mat[i]; x.full = L->touched[i]; x.bond = L->bond[i]; x.local = L->local[i]; x.wpos = L->wpos[i];
             x.phi = L->phi[i]; x.ext = L->ext[i]; x.extn = L->extn[i]; x.pad2 = 0.0f;
             L->touched[i] = 0;
         }
         L->tlist.clear();
-        SF_HIP(hipMemcpyAsync(L->d_delta.p, L->h_delta.p, sizeof(BrickDelta) * n, hipMemcpyHostToDevice, L->st));
-        LAUNCH(k_brick_delta, sf_grid(n), NT, L->D, (const BrickDelta*)L->d_delta.p, (uint32_t)n);
+        if (L->stg_delta.end(sizeof(BrickDelta) * n, L->st)) return -1;
+        LAUNCH(k_brick_delta, sf_grid(n), NT, L->D, (const BrickDelta*)L->stg_delta.d.p, (uint32_t)n);
     }
     if (nb) {
-        if (L->h_brec.reserve(nb)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-        if (L->d_brec.n < nb && L->d_brec.alloc(std::max<size_t>(nb, 4096), L->st)) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
+        BodyRec* hb = reinterpret_cast<BodyRec*>(L->stg_brec.begin(sizeof(BodyRec) * nb));
+        if (!hb) return -1;
         for (size_t t = 0; t < nb; t++) {
             uint32_t b = L->btlist[t];
-            BodyRec& x = L->h_brec.p[t];
+            BodyRec& x = hb[t];
             x.b = b; x.alive = L->alive[b]; x.asleep = L->asleep[b]; x.wake = L->wake[b]; x.pad0 = 0; x.count = L->count[b]; x.nc = L->nc[b];
             x.pos = L->pos[b]; x.q = L->qq[b]; x.v = L->v[b]; x.w = L->w[b]; x.vlast = L->vlast[b]; x.wlast = L->wlast[b];
             for (int k = 0; k < 3; k++) { x.rot[k] = m_row(L->rot[b], k); x.iil[k] = m_row(L->iil[b], k); x.iiw[k] = m_row(L->iiw[b], k); }
@@ -2603,11 +2675,9 @@ int flush_delta(Lat* L) {
             L->btouched[b] = 0;
         }
         L->btlist.clear();
-        SF_HIP(hipMemcpyAsync(L->d_brec.p, L->h_brec.p, sizeof(BodyRec) * nb, hipMemcpyHostToDevice, L->st));
-        LAUNCH(k_body_unpack, sf_grid(nb), NT, L->D, (const BodyRec*)L->d_brec.p, (uint32_t)nb);
+        if (L->stg_brec.end(sizeof(BodyRec) * nb, L->st)) return -1;
+        LAUNCH(k_body_unpack, sf_grid(nb), NT, L->D, (const BodyRec*)L->stg_brec.d.p, (uint32_t)nb);
     }
-    // the staging buffers are reused by the next transfer
-    if (n || nb) { SF_HIP(hipStreamSynchronize(L->st)); L->syncs++; }
     return 0;
 }
 
@@ -2753,14 +2823,12 @@ int substep(Lat* L, float h) {
     // restart from zero, stamps and wake flags are idempotent)
     LAUNCH(k_contacts<false>, gw, NT, D);
     scan(L, D.ccnt, D.coff, D.ctr + C_NCS, L->ndyn);
-    if (L->h_c0.reserve(L->sc.size())) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-    const uint32_t* c0 = L->h_c0.p;
+    const uint32_t* c0 = L->h_ctr + CTR_WORDS;
     for (int pass = 0;; pass++) {
         if (pass > 0) LAUNCH(k_nc_clear, gb, NT, D);  // (the count pass cleared them the first time)
         LAUNCH(k_contacts<true>, gw, NT, D);
         LAUNCH(k_plough_flag, sf_grid(L->ccap, NT, 1024), NT, D);
-        SF_HIP(hipMemcpyAsync(L->h_c0.p, D.sc_c0, sizeof(uint32_t) * L->sc.size(), hipMemcpyDeviceToHost, L->st));
-        if (read_ctr(L)) return -1;
+        if (read_ctr(L, L->sc.size())) return -1;
         uint32_t need = L->h_ctr[C_NCS];
         if (need <= L->ccap) break;
         if (pass > 0 || lat_contacts_alloc(L, std::max<uint32_t>(need + need / 2, 1u << 16))) { sf::set_error(SF_E_CAP, "contact arrays"); return SF_E_CAP; }
@@ -2786,7 +2854,10 @@ int substep(Lat* L, float h) {
     }
     if (fused) {
         if (!L->slist.empty()) {
-            SF_HIP(hipMemcpyAsync(L->d_slist.p, L->slist.data(), sizeof(int32_t) * L->slist.size(), hipMemcpyHostToDevice, L->st));
+            if (L->slist != L->slist_up) {
+                SF_HIP(hipMemcpyAsync(L->d_slist.p, L->slist.data(), sizeof(int32_t) * L->slist.size(), hipMemcpyHostToDevice, L->st));
+                L->slist_up = L->slist;
+            }
             // bodies of scenes up to the LDS capacity (84 bytes each) are solved in LDS
             if (!L->lds_max) {
                 int v = 0;
@@ -2854,8 +2925,12 @@ void sor_scene_launch(Lat* L, int mode) {
 int structures_upload(Lat* L, bool warm) {
     size_t ns = L->sts.size();
     if (!ns) return 0;
-    std::vector<uint8_t> act(ns), on(ns), live(ns);
-    std::vector<int32_t> nmin(ns);
+    // one block (sweeps asked for, then the active, on and live flags) from one of two pinned buffers
+    const size_t bytes = 7 * ns;
+    uint8_t* blk = L->stg_ctl.begin(bytes);
+    if (!blk) return -1;
+    int32_t* nmin = reinterpret_cast<int32_t*>(blk);
+    uint8_t *act = blk + 4 * ns, *on = act + ns, *live = on + ns;
     for (size_t g = 0; g < ns; g++) {
         const HStruct& T = L->sts[g];
         const HScene& S = L->sc[T.scene];
@@ -2877,12 +2952,8 @@ int structures_upload(Lat* L, bool warm) {
             S.stats[13] = (float)na;
         }
     }
-    LD& D = L->D;
-    int r = 0;
-    r |= cp_up(D.st_act, act.data(), ns, L->st); r |= cp_up(D.st_on, on.data(), ns, L->st);
-    r |= cp_up(D.st_live, live.data(), ns, L->st); r |= cp_up(D.st_nmin, nmin.data(), ns, L->st);
-    if (r) return r;
-    SF_HIP(hipStreamSynchronize(L->st));
+    if (L->stg_ctl.end(bytes, L->st)) return -1;
+    LAUNCH(k_st_ctl, sf_grid(ns), NT, L->D, (const uint8_t*)L->stg_ctl.d.p, (uint32_t)ns);
     return 0;
 }
 
@@ -2920,33 +2991,59 @@ int structural(Lat* L, float dt) {
     const size_t nsc = L->sc.size();
     L->sp[8] += ms_since(t);
     t = std::chrono::steady_clock::now();
-    // into pinned memory; the counter read's wait completes them all
+    // packed on the device into one block, one copy into pinned memory (its wait completes every read-back
+    // queued before it)
     auto al = [](size_t x) { return (x + 63) & ~(size_t)63; };
-    const size_t o_mr = al(4 * ns), o_fl = o_mr + al(4 * ns), o_gm = o_fl + al(4 * ns), o_bd = o_gm + al(4 * nsc);
-    if (L->h_rb.reserve(o_bd + D.nub + 64)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-    float* resid = (float*)L->h_rb.p;
+    const size_t o_rs = al(4 * (size_t)C_NCTR), o_mr = o_rs + al(4 * ns), o_fl = o_mr + al(4 * ns), o_gm = o_fl + al(4 * ns);
+    const size_t o_bd = o_gm + al(4 * nsc), o_ek = al(o_bd + D.nub), o_ev = o_ek + al(8 * (size_t)L->kev_pack);
+    const size_t o_bi = o_ev + al(16 * (size_t)L->kev_pack), o_bv = o_bi + al(4 * (size_t)L->kbw_pack);
+    const size_t total = o_bv + 4 * (size_t)L->kbw_pack;
+    if (L->h_rb.reserve(total + 64)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    if (L->d_rbk.n < total && L->d_rbk.alloc(std::max<size_t>(total + total / 2, 4096), L->st)) return -1;
+    float* resid = (float*)(L->h_rb.p + o_rs);
     uint32_t* maxr = (uint32_t*)(L->h_rb.p + o_mr);
     uint32_t* flags = (uint32_t*)(L->h_rb.p + o_fl);
     uint32_t* gmax = (uint32_t*)(L->h_rb.p + o_gm);
     uint8_t* bdirty = L->h_rb.p + o_bd;
     int r = 0;
-    r |= cp_down(resid, D.st_resid, ns, L->st); r |= cp_down(maxr, D.st_maxr, ns, L->st);
-    r |= cp_down(flags, D.st_flags, ns, L->st); r |= cp_down(gmax, D.sc_gmax, nsc, L->st);
-    LAUNCH(k_take_dirty, gb, NT, D);
-    r |= cp_down(bdirty, D.bdflag, D.nub, L->st);
-    if (r) return r;
-    if (read_ctr(L)) return -1;
+    PackRB P{(uint32_t)ns, (uint32_t)nsc, (uint32_t)o_rs, (uint32_t)o_mr, (uint32_t)o_fl, (uint32_t)o_gm, (uint32_t)o_bd,
+             (uint32_t)o_ek, (uint32_t)o_ev, (uint32_t)o_bi, (uint32_t)o_bv, L->kev_pack, L->kbw_pack};
+    const size_t npk = std::max(std::max<size_t>(std::max<size_t>(D.nub, ns), std::max<size_t>(nsc, C_NCTR)),
+                                (size_t)std::max(L->kev_pack, L->kbw_pack));
+    LAUNCH(k_pack_rb, sf_grid(npk), NT, D, L->d_rbk.p, P);
+    SF_HIP(hipMemcpyAsync(L->h_rb.p, L->d_rbk.p, total, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    memcpy(L->h_ctr, L->h_rb.p, sizeof(uint32_t) * C_NCTR);
+    if (finish_stats(L)) return -1;
     if (drain_kills(L)) return -1;
     uint32_t nev = std::min(L->h_ctr[C_NEV], D.evcap), nbw = L->h_ctr[C_NBW];
-    std::vector<unsigned long long> ekey(nev);
-    std::vector<int4> ev(nev);
-    std::vector<uint32_t> bwi;
-    std::vector<float> bwv;
     bool bw_overflow = nbw > D.bwcap;
-    if (nev) { r |= cp_down(ekey.data(), D.evkey, nev, L->st); r |= cp_down(ev.data(), D.ev, nev, L->st); }
-    if (nbw && !bw_overflow) { bwi.resize(nbw); bwv.resize(nbw); r |= cp_down(bwi.data(), D.bwi, nbw, L->st); r |= cp_down(bwv.data(), D.bwv, nbw, L->st); }
+    // events and bond writes: from the packed block when they fit its share, else copied now
+    const unsigned long long* ekey = reinterpret_cast<const unsigned long long*>(L->h_rb.p + o_ek);
+    const int4* ev = reinterpret_cast<const int4*>(L->h_rb.p + o_ev);
+    const uint32_t* bwi = reinterpret_cast<const uint32_t*>(L->h_rb.p + o_bi);
+    const float* bwv = reinterpret_cast<const float*>(L->h_rb.p + o_bv);
+    std::vector<unsigned long long> ekey_x;
+    std::vector<int4> ev_x;
+    std::vector<uint32_t> bwi_x;
+    std::vector<float> bwv_x;
+    bool more = false;
+    if (nev > L->kev_pack) {
+        ekey_x.resize(nev); ev_x.resize(nev);
+        r |= cp_down(ekey_x.data(), D.evkey, nev, L->st); r |= cp_down(ev_x.data(), D.ev, nev, L->st);
+        ekey = ekey_x.data(); ev = ev_x.data(); more = true;
+    }
+    if (nbw && !bw_overflow && nbw > L->kbw_pack) {
+        bwi_x.resize(nbw); bwv_x.resize(nbw);
+        r |= cp_down(bwi_x.data(), D.bwi, nbw, L->st); r |= cp_down(bwv_x.data(), D.bwv, nbw, L->st);
+        bwi = bwi_x.data(); bwv = bwv_x.data(); more = true;
+    }
     if (r) return r;
-    SF_HIP(hipStreamSynchronize(L->st));
+    if (more) { SF_HIP(hipStreamSynchronize(L->st)); L->syncs++; }
+    // the next frame's shares follow this frame's counts
+    L->kev_pack = std::min<uint32_t>(std::max<uint32_t>(2 * nev + 64, 64u), 16384u);
+    L->kbw_pack = std::min<uint32_t>(std::max<uint32_t>(2 * std::min(nbw, D.bwcap) + 256, 256u), 65536u);
     L->sp[9] += ms_since(t);
     t = std::chrono::steady_clock::now();
     if (nev > 0 && L->h_ctr[C_NEV] > D.evcap) { sf::set_error(SF_E_CAP, "structural event buffer overflow"); return SF_E_CAP; }
@@ -3383,7 +3480,7 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     if (hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) != hipSuccess) { delete h; sf::set_error(SF_E_HIP, "stream"); return nullptr; }
     if (lat_alloc(L) || lat_contacts_alloc(L, (uint32_t)std::max<int>(1 << 16, 4 * brick_cap)) || lat_hash_alloc(L, 1024) || L->d_grid.alloc(1, L->st)) { delete h; return nullptr; }
     L->D.grid = L->d_grid.p;
-    if (L->h_ctrbuf.reserve(C_NCTR)) { delete h; sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return nullptr; }
+    if (L->h_ctrbuf.reserve(CTR_WORDS + (size_t)max_scenes + 1)) { delete h; sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return nullptr; }
     L->h_ctr = L->h_ctrbuf.p;
     memset(L->h_ctr, 0, sizeof(uint32_t) * C_NCTR);
     size_t B = brick_cap, N = body_cap;

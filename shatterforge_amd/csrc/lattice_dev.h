@@ -21,6 +21,8 @@ enum : int {
     C_COLCNT = 16,  // KMAX + 1 counts
     C_NCTR = C_COLCNT + KMAX + 1
 };
+constexpr int CTR_WORDS = 64;  // the counters' block; the scenes' first contacts follow it
+static_assert(C_NCTR <= CTR_WORDS, "counter block");
 
 // Per-scene values a substep or frame reads (uploaded by the host before each frame)
 struct SceneK {
@@ -122,7 +124,6 @@ struct LD {
     uint8_t* bd_live;
     float* bd_dt;
     uint8_t* bd_dirty;
-    uint8_t* bdflag;           // bd_dirty of the used slots, in list order, for the host
     // events
     uint32_t* ctr;
     uint32_t* plist;           // plough candidates
