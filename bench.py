@@ -267,7 +267,8 @@ def bench_box(args, sync):
     sync()
     el = time.perf_counter() - t0
     st = w.stats()
-    log(f"timed {args.steps} steps: {el * 1e3 / args.steps:.2f} ms/step, stats {st}")
+    graph = dict(zip(("captures", "replays"), w.graph_stats()))
+    log(f"timed {args.steps} steps: {el * 1e3 / args.steps:.2f} ms/step, stats {st}, sub-step graph {graph}")
     # the dominant kernel, timed live with HIP events on the library's stream over a second run of the same length
     w.set_kernel_timing(True)
     for k in range(args.steps):
@@ -305,6 +306,7 @@ def bench_box(args, sync):
                      "bytes_per_unit": SOLVE_BYTES_PER_MANIFOLD, "units": kitems},
         "stages_ms": stages,
         "stats": st,
+        "substep_graph": graph,
     }
     return out, (bodies, shape_body, shapes, prm)
 

@@ -89,6 +89,12 @@ int sf_world_stage_ms(sf_world* w, float* out5);
  * summed device ms, the launches and the manifold solves they performed since timing was switched on */
 int sf_world_set_kernel_timing(sf_world* w, int on);
 int sf_world_kernel_stats(sf_world* w, double* ms, int64_t* launches, int64_t* items);
+/* Launch mode of the sub-steps (no counterpart in the reference, whose sub-step dispatches are encoded one by one,
+ * gpu/gpu_world.js:2599-2604): on (the default), the chain of sub-step launches is captured once as a HIP graph and
+ * replayed each step while its launch shapes hold; off, the kernels are launched one by one. Results are the same.
+ * While kernel timing is on the launches go one by one. graph_stats: captures and replays so far. */
+int sf_world_set_graph(sf_world* w, int on);
+int sf_world_graph_stats(sf_world* w, int64_t* captures, int64_t* replays);
 
 /* ---- the reference's box-core exports (engine/src/wasm.rs:27-53) on one default world */
 void sf_pile_setup(uint32_t n, uint32_t substeps);              /* wasm.rs:27 pile_setup */

@@ -37,6 +37,8 @@ _PROTOS = {
     "sf_world_stage_ms": (c_int, [c_void_p, c_void_p]),
     "sf_world_set_kernel_timing": (c_int, [c_void_p, c_int]),
     "sf_world_kernel_stats": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_world_set_graph": (c_int, [c_void_p, c_int]),
+    "sf_world_graph_stats": (c_int, [c_void_p, c_void_p, c_void_p]),
     "sf_pile_setup": (None, [c_uint, c_uint]),
     "sf_stack_setup": (None, [c_uint, c_uint, c_uint]),
     "sf_step": (c_uint, [c_float]),

@@ -1108,9 +1108,13 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
 // its XCD's contiguous eighth of the bucket, so an XCD's L2 holds the bodies of one region of the bucket instead of
 // all of them. Within a colour no two manifolds share a moving body, so any assignment gives the same result.
 constexpr int SOLVE_NT = 64;
+// The grid may be larger than the bucket needs (a captured launch sized with headroom): the first G blocks, G from
+// the bucket's count, do the work, spread over the XCDs like a grid of exactly G.
 __global__ void __launch_bounds__(SOLVE_NT) k_solve_bucket(Dev D, StepK K, int c) {
     const uint32_t n = D.colcnt[c], o = D.coloff[c];
-    const uint32_t G = gridDim.x, b = blockIdx.x, x = b & 7u, q = G >> 3, r = G & 7u;
+    const uint32_t G = min(gridDim.x, max((n + SOLVE_NT - 1) / SOLVE_NT, 1u)), b = blockIdx.x;
+    if (b >= G) return;
+    const uint32_t x = b & 7u, q = G >> 3, r = G & 7u;
     const uint32_t lb = x * q + min(x, r) + (b >> 3);  // a bijection of [0, G)
     for (uint32_t i = lb * SOLVE_NT + threadIdx.x; i < n; i += G * SOLVE_NT) solve_one(D, o + i, K);
 }
@@ -1370,6 +1374,24 @@ struct HostWorld {
     uint32_t* h_ctr = nullptr;     // pinned
     uint32_t* h_col = nullptr;     // pinned colour counts
     bool timing = false;
+    // the sub-step launch chain as a captured graph, replayed while its launches keep their shape (the key: the
+    // device tables, the step constants, the colour plan and each launch's grid, sized with headroom)
+    // (two kept: the manifold buffers swap every step, so consecutive steps alternate between two device tables)
+    struct SubGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<uint8_t> key;
+        std::vector<unsigned> grid;  // per big colour, then the overflow pass
+        int64_t used = 0;
+        void clear() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr; graph = nullptr; key.clear(); grid.clear();
+        }
+    };
+    bool use_graph = true;
+    SubGraph sg[2];
+    int64_t graph_captures = 0, graph_replays = 0;
     // live timing of the solve-bucket kernel: an event pair around each launch
     bool ktime = false;
     std::vector<hipEvent_t> kev;
@@ -1388,6 +1410,7 @@ struct HostWorld {
         if (h_col) (void)hipHostFree(h_col);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         for (auto& e : kev) (void)hipEventDestroy(e);
+        for (auto& g : sg) g.clear();
         if (st) (void)hipStreamDestroy(st);
     }
     template <class T> int alloc(T*& p, size_t n) {
@@ -1570,6 +1593,76 @@ int launch_bucket(HostWorld* w, const StepK& K, int c) {
     return 0;
 }
 
+// The sub-steps' launches with the given grids (one per big colour, then the overflow pass); the kernels read the
+// counts on the device, so grids larger than the counts need are correct.
+void substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, int sub, const unsigned* grid) {
+    Dev& D = w->D;
+    hipStream_t s = w->st;
+    const unsigned gb = sf_grid(D.nb), gw = sf_grid((uint64_t)D.nb * 64, NT, 65536);
+    StepK Kb = K; Kb.use_bias = 1;
+    StepK Kr = K; Kr.use_bias = 0;
+    for (int it = 0; it < sub; it++) {
+        hipLaunchKernelGGL(k_warm, dim3(gw), dim3(NT), 0, s, D, K);
+        for (int c = 0; c < big; c++) hipLaunchKernelGGL(k_solve_bucket, dim3(grid[c]), dim3(SOLVE_NT), 0, s, D, Kb, c);
+        if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kb, big);
+        if (ovf) {
+            hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kb);
+            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kb);
+        }
+        hipLaunchKernelGGL(k_integrate, dim3(gb), dim3(NT), 0, s, D, K);
+        for (int c = 0; c < big; c++) hipLaunchKernelGGL(k_solve_bucket, dim3(grid[c]), dim3(SOLVE_NT), 0, s, D, Kr, c);
+        if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
+        if (ovf) {
+            hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kr);
+            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
+        }
+    }
+}
+
+// The sub-steps as one graph launch. A captured graph is replayed while the device tables, the step constants and
+// the colour plan are those it was captured with and every launch's grid still covers its count without being more
+// than 4x too large; otherwise the chain is captured again, with grids of 1.5x the counts, in place of the graph
+// used least recently.
+int solve_graph(HostWorld* w, const StepK& K, int big, bool tail, uint32_t novf, int sub) {
+    hipStream_t s = w->st;
+    std::vector<uint8_t> key(sizeof(Dev) + sizeof(StepK) + 4 * sizeof(int));
+    int plan[4] = {big, tail ? 1 : 0, novf ? 1 : 0, sub};
+    memcpy(key.data(), &w->D, sizeof(Dev));
+    memcpy(key.data() + sizeof(Dev), &K, sizeof(StepK));
+    memcpy(key.data() + sizeof(Dev) + sizeof(StepK), plan, sizeof(plan));
+    std::vector<unsigned> need(big + 1);
+    for (int c = 0; c < big; c++) need[c] = sf_grid(w->h_col[c], SOLVE_NT, 16384);
+    need[big] = sf_grid(novf ? novf : 1);
+    auto fits = [&](const HostWorld::SubGraph& g) {
+        if (!g.exec || g.key != key || g.grid.size() != need.size()) return false;
+        for (size_t k = 0; k < need.size(); k++) if (!(need[k] <= g.grid[k] && 4ull * need[k] >= g.grid[k])) return false;
+        return true;
+    };
+    HostWorld::SubGraph* use = fits(w->sg[0]) ? &w->sg[0] : (fits(w->sg[1]) ? &w->sg[1] : nullptr);
+    const int64_t tick = w->graph_captures + w->graph_replays + 1;
+    if (!use) {
+        use = w->sg[0].used <= w->sg[1].used ? &w->sg[0] : &w->sg[1];  // the one used least recently
+        use->clear();
+        use->grid.resize(need.size());
+        for (int c = 0; c < big; c++) use->grid[c] = std::min(16384u, need[c] + need[c] / 2);
+        use->grid[big] = std::min(4096u, need[big] + need[big] / 2);
+        SF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        substep_chain(w, K, big, tail, novf != 0, sub, use->grid.data());
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess || !g) { sf::set_error(SF_E_HIP, "graph capture failed"); return SF_E_HIP; }
+        use->graph = g;
+        SF_HIP(hipGraphInstantiate(&use->exec, use->graph, nullptr, nullptr, 0));
+        use->key = key;
+        w->graph_captures++;
+    } else {
+        w->graph_replays++;
+    }
+    use->used = tick;
+    SF_HIP(hipGraphLaunch(use->exec, s));
+    return 0;
+}
+
 int step_back(HostWorld* w, StepK K) {
     Dev& D = w->D;
     hipStream_t s = w->st;
@@ -1583,7 +1676,10 @@ int step_back(HostWorld* w, StepK K) {
     int sub = (int)w->prm[SF_P_SUBSTEPS]; if (sub < 1) sub = 1;
     StepK Kb = K; Kb.use_bias = 1;
     StepK Kr = K; Kr.use_bias = 0;
-    for (int it = 0; it < sub; it++) {
+    if (w->use_graph && !w->ktime) {
+        int r = solve_graph(w, K, big, tail, novf, sub);
+        if (r) return r;
+    } else for (int it = 0; it < sub; it++) {
         hipLaunchKernelGGL(k_warm, dim3(gw), dim3(NT), 0, s, D, K);
         for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kb, c); if (r) return r; }
         if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kb, big);
@@ -1872,6 +1968,13 @@ const float* sf_world_pose(sf_world* w) {
     return h->pose.data();
 }
 int sf_world_set_timing(sf_world* w, int on) { if (!w) return SF_E_ARG; w->h.timing = on != 0; return 0; }
+int sf_world_set_graph(sf_world* w, int on) { if (!w) return SF_E_ARG; w->h.use_graph = on != 0; return 0; }
+int sf_world_graph_stats(sf_world* w, int64_t* captures, int64_t* replays) {
+    if (!w) return SF_E_ARG;
+    if (captures) *captures = w->h.graph_captures;
+    if (replays) *replays = w->h.graph_replays;
+    return 0;
+}
 int sf_world_set_kernel_timing(sf_world* w, int on) {
     if (!w) return SF_E_ARG;
     w->h.ktime = on != 0;

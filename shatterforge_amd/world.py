@@ -169,6 +169,17 @@ class World:
         check(lib().sf_world_kernel_stats(self._h, ptr(ms), ptr(n), ptr(it)))
         return float(ms[0]), int(n[0]), int(it[0])
 
+    def set_graph(self, on=True):
+        """Replay the sub-steps' launches as one captured HIP graph (on by default; off: launched one by one)."""
+        check(lib().sf_world_set_graph(self._h, 1 if on else 0))
+
+    def graph_stats(self):
+        """(captures, replays) of the sub-step graph since the world was made."""
+        c = np.zeros(1, np.int64)
+        r = np.zeros(1, np.int64)
+        check(lib().sf_world_graph_stats(self._h, ptr(c), ptr(r)))
+        return int(c[0]), int(r[0])
+
     def stage_ms(self):
         out = np.zeros(5, np.float32)
         check(lib().sf_world_stage_ms(self._h, ptr(out)))

@@ -121,6 +121,29 @@ def test_determinism_run_twice():
         assert np.array_equal(a, c)
 
 
+def test_graph_replay_same_as_launches():
+    # the sub-steps replayed as a captured graph give the same bodies as launching them one by one, through the
+    # steps where the colour counts change and the graph is captured again
+    print("case: rubble(2000), 40 steps, graph on vs off", flush=True)
+    b, sb, s = scenes.rubble(2000, box=(14.0, 8.0, 14.0), seed=11)
+    prm = params(substeps=4)
+    wg = sf.World.from_scene(b, sb, s, prm)
+    wd = sf.World.from_scene(b, sb, s, prm)
+    wd.set_graph(False)
+    for k in range(40):
+        wg.step(DT)
+        wd.step(DT)
+        gb, db = wg.bodies(), wd.bodies()
+        assert np.array_equal(gb, db), f"step {k}: max diff {np.abs(gb - db).max()}"
+    assert np.array_equal(wg.colours(), wd.colours())
+    caps, reps = wg.graph_stats()
+    print(f"graph captures {caps}, replays {reps}", flush=True)
+    assert caps >= 1 and reps >= 1 and caps + reps == 40
+    assert wd.graph_stats() == (0, 0)
+    wg.close()
+    wd.close()
+
+
 def test_full_size_rubble_determinism_and_complete_candidates():
     # the full rubble pile, two worlds stepped side by side: fast bodies' swept boxes can have more partners than a
     # shape's candidate slots; the list must still be complete and sorted, whatever order the cell table was filled
