@@ -256,7 +256,21 @@ def bench_box(args, sync):
         prm[world.P["COL_LIMIT"]] = args.colours
     if args.tail_min is not None:
         prm[world.P["TAIL_MIN"]] = args.tail_min
+    tuned = None
+    if not args.no_autotune:
+        # launch shapes (solve workgroup size, small-colour threshold, graph replay) timed on this scene; none of
+        # them changes the results
+        from shatterforge_amd import autotune
+        t = time.perf_counter()
+        table = autotune.tune_box(bodies, shape_body, shapes, prm)
+        tuned = {"best": table["best"], "default": table["default"], "candidates": len(table["candidates"]),
+                 "seconds": time.perf_counter() - t}
+        log(f"autotune: best {table['best']} (default {table['default']['ms']:.2f} ms) in {tuned['seconds']:.1f} s")
+        prm[world.P["SOLVE_NT"]] = table["best"]["solve_nt"]
+        prm[world.P["TAIL_MIN"]] = table["best"]["tail_min"]
     w = sf.World.from_scene(bodies, shape_body, shapes, prm)
+    if tuned is not None:
+        w.set_graph(bool(tuned["best"]["graph"]))
     for k in range(args.warmup):
         w.step(1.0 / 60.0)
     log(f"warmup {args.warmup} steps done: {w.stats()}")
@@ -299,7 +313,8 @@ def bench_box(args, sync):
         "data": "synthetic (seeded rubble pile)",
         "config": {"workload": "rubble-pile stress, plain step on one scene", "bodies": len(bodies), "bricks": nbricks,
                    "substeps": args.substeps, "dt": "1/60", "mode": "plain",
-                   "colour_limit": int(prm[world.P["COL_LIMIT"]]), "tail_min": int(prm[world.P["TAIL_MIN"]])},
+                   "colour_limit": int(prm[world.P["COL_LIMIT"]]), "tail_min": int(prm[world.P["TAIL_MIN"]]),
+                   "solve_workgroup": int(prm[world.P["SOLVE_NT"]])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": SOLVE_TRAFFIC_BYTES,
                      "kernel": "k_solve_bucket", "launches": klaunch, "avg_us": ksum_ms / max(klaunch, 1) * 1e3,
@@ -307,6 +322,7 @@ def bench_box(args, sync):
         "stages_ms": stages,
         "stats": st,
         "substep_graph": graph,
+        "autotune": tuned,
     }
     return out, (bodies, shape_body, shapes, prm)
 
@@ -334,6 +350,7 @@ def main():
     ap.add_argument("--colours", type=int, default=None, help="colour cap (default: the library's)")
     ap.add_argument("--tail-min", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes for the box step")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
     ap.add_argument("--parts", default="box,lattice,whatif",

@@ -42,6 +42,7 @@ enum {
     SF_P_COL_ROUNDS,                /* colouring rounds (32) */
     SF_P_COL_SPREAD,                /* a contact picks among the lowest this-many free colours (4) */
     SF_P_TAIL_MIN,                  /* colour buckets smaller than this are solved in one workgroup (1024) */
+    SF_P_SOLVE_NT,                  /* threads per workgroup of the colour-bucket solve: 64, 128 or 256 (64); results do not depend on it */
     SF_NPARAM
 };
 /* body record, SF_BODY_F floats: p[3] q[4] v[3] w[3] inv_m inv_i[3] (principal, local) awake sleep_t reserved */

@@ -1,0 +1,75 @@
+"""Launch-shape tuner for the box step.
+
+The knobs it tunes change how the step's work is spread over the GPU, never what is computed: threads per
+workgroup of the colour-bucket solve (one manifold per thread; within a colour no two manifolds share a moving body,
+so any grouping gives the same results), the colour-bucket size below which the small colours are solved one after
+another in one workgroup instead of one launch each, and whether the sub-steps replay as a captured HIP graph.
+
+Every candidate is timed on one world built from the caller's own scene. The candidates take turns step by step, so
+they all see the same sequence of states (the trajectory does not depend on the knobs); each visit runs a few steps
+and times the last, so a graph capture on switching is not counted. The table (per candidate, the median step time)
+is returned and can be written as JSON and applied to a world later.
+"""
+import itertools
+import json
+import time
+
+import numpy as np
+
+CANDIDATES = {"solve_nt": (64, 128, 256), "tail_min": (256, 1024, 4096), "graph": (True, False)}
+
+
+def apply(world, best):
+    """Set a tuned configuration (a row of the table) on a World."""
+    from .world import P
+    prm = world.params()
+    prm[P["SOLVE_NT"]] = best["solve_nt"]
+    prm[P["TAIL_MIN"]] = best["tail_min"]
+    world.set_params(prm)
+    world.set_graph(bool(best["graph"]))
+
+
+def tune_box(bodies, shape_body, shapes, prm, rounds=3, visit_steps=3, warmup=2, dt=1.0 / 60.0, path=None):
+    """Time every candidate on the scene (bodies, shape_body, shapes) with parameters prm; returns the table
+    {"candidates": [...], "best": {...}, ...} and writes it to path as JSON when a path is given."""
+    import shatterforge_amd as sf
+    from .world import P
+    names = list(CANDIDATES)
+    combos = [dict(zip(names, v)) for v in itertools.product(*(CANDIDATES[k] for k in names))]
+    w = sf.World.from_scene(bodies, shape_body, shapes, prm)
+    try:
+        for _ in range(warmup):
+            w.step(dt)
+        times = [[] for _ in combos]
+        base = w.params()
+        for _ in range(rounds):
+            for k, c in enumerate(combos):
+                p = base.copy()
+                p[P["SOLVE_NT"]] = c["solve_nt"]
+                p[P["TAIL_MIN"]] = c["tail_min"]
+                w.set_params(p)
+                w.set_graph(c["graph"])
+                for _ in range(visit_steps - 1):
+                    w.step(dt)
+                t0 = time.perf_counter()
+                w.step(dt)  # (a step ends with the stream synchronised)
+                times[k].append(time.perf_counter() - t0)
+        rows = [dict(c, ms=float(np.median(t) * 1e3)) for c, t in zip(combos, times)]
+        default = next(r for r in rows if r["solve_nt"] == 64 and r["tail_min"] == 1024 and r["graph"])
+        best = min(rows, key=lambda r: r["ms"])
+        table = {"workload": {"bodies": int(len(bodies)), "shapes": int(len(shapes)),
+                              "substeps": int(prm[P["SUBSTEPS"]]), "colour_limit": int(prm[P["COL_LIMIT"]])},
+                 "rounds": rounds, "timed_steps_per_candidate": rounds, "candidates": rows, "best": best,
+                 "default": default}
+    finally:
+        w.close()
+    if path:
+        with open(path, "w") as f:
+            json.dump(table, f, indent=1)
+    return table
+
+
+def load(path):
+    """A table written by tune_box."""
+    with open(path) as f:
+        return json.load(f)

@@ -1109,14 +1109,22 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
 // all of them. Within a colour no two manifolds share a moving body, so any assignment gives the same result.
 constexpr int SOLVE_NT = 64;
 // The grid may be larger than the bucket needs (a captured launch sized with headroom): the first G blocks, G from
-// the bucket's count, do the work, spread over the XCDs like a grid of exactly G.
-__global__ void __launch_bounds__(SOLVE_NT) k_solve_bucket(Dev D, StepK K, int c) {
+// the bucket's count, do the work, spread over the XCDs like a grid of exactly G. NTB threads per workgroup (64 by
+// default; 128 and 256 for the launch-shape tuner): any value gives the same results.
+template <int NTB>
+__global__ void __launch_bounds__(NTB) k_solve_bucket(Dev D, StepK K, int c) {
     const uint32_t n = D.colcnt[c], o = D.coloff[c];
-    const uint32_t G = min(gridDim.x, max((n + SOLVE_NT - 1) / SOLVE_NT, 1u)), b = blockIdx.x;
+    const uint32_t G = min(gridDim.x, max((n + NTB - 1) / NTB, 1u)), b = blockIdx.x;
     if (b >= G) return;
     const uint32_t x = b & 7u, q = G >> 3, r = G & 7u;
     const uint32_t lb = x * q + min(x, r) + (b >> 3);  // a bijection of [0, G)
-    for (uint32_t i = lb * SOLVE_NT + threadIdx.x; i < n; i += G * SOLVE_NT) solve_one(D, o + i, K);
+    for (uint32_t i = lb * NTB + threadIdx.x; i < n; i += G * NTB) solve_one(D, o + i, K);
+}
+int solve_nt_of(const float* prm) { int v = (int)prm[SF_P_SOLVE_NT]; return v == 128 || v == 256 ? v : SOLVE_NT; }
+void launch_solve_bucket(int nt, unsigned grid, hipStream_t s, const Dev& D, const StepK& K, int c) {
+    if (nt == 256) hipLaunchKernelGGL(k_solve_bucket<256>, dim3(grid), dim3(256), 0, s, D, K, c);
+    else if (nt == 128) hipLaunchKernelGGL(k_solve_bucket<128>, dim3(grid), dim3(128), 0, s, D, K, c);
+    else hipLaunchKernelGGL(k_solve_bucket<SOLVE_NT>, dim3(grid), dim3(SOLVE_NT), 0, s, D, K, c);
 }
 
 // the small colours from c0 on, one after another in one workgroup
@@ -1429,7 +1437,7 @@ void default_params(float* p) {
     p[SF_P_SLEEP_LIN] = 0.05f; p[SF_P_SLEEP_ANG] = 0.08f; p[SF_P_SLEEP_TIME] = 0.5f; p[SF_P_SLEEP_DIST] = 0.01f;
     p[SF_P_SLEEP_ROT] = 0.02f; p[SF_P_ALLOW_SLEEP] = 1.0f; p[SF_P_CELL] = 1.0f; p[SF_P_LEAK] = 1.0f;
     p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 16;
-    p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_TAIL_MIN] = 1024;
+    p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_TAIL_MIN] = 1024; p[SF_P_SOLVE_NT] = 64;
 }
 
 int world_alloc(HostWorld* w) {
@@ -1583,7 +1591,8 @@ int launch_bucket(HostWorld* w, const StepK& K, int c) {
         while (w->kev.size() < w->kev_used + 2) { hipEvent_t e; SF_HIP(hipEventCreate(&e)); w->kev.push_back(e); }
         SF_HIP(hipEventRecord(w->kev[w->kev_used], s));
     }
-    hipLaunchKernelGGL(k_solve_bucket, dim3(sf_grid(w->h_col[c], SOLVE_NT, 16384)), dim3(SOLVE_NT), 0, s, w->D, K, c);
+    const int nt = solve_nt_of(w->prm);
+    launch_solve_bucket(nt, sf_grid(w->h_col[c], nt, 16384), s, w->D, K, c);
     if (w->ktime) {
         SF_HIP(hipEventRecord(w->kev[w->kev_used + 1], s));
         w->kev_used += 2;
@@ -1601,16 +1610,17 @@ void substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, i
     const unsigned gb = sf_grid(D.nb), gw = sf_grid((uint64_t)D.nb * 64, NT, 65536);
     StepK Kb = K; Kb.use_bias = 1;
     StepK Kr = K; Kr.use_bias = 0;
+    const int nt = solve_nt_of(w->prm);
     for (int it = 0; it < sub; it++) {
         hipLaunchKernelGGL(k_warm, dim3(gw), dim3(NT), 0, s, D, K);
-        for (int c = 0; c < big; c++) hipLaunchKernelGGL(k_solve_bucket, dim3(grid[c]), dim3(SOLVE_NT), 0, s, D, Kb, c);
+        for (int c = 0; c < big; c++) launch_solve_bucket(nt, grid[c], s, D, Kb, c);
         if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kb, big);
         if (ovf) {
             hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kb);
             hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kb);
         }
         hipLaunchKernelGGL(k_integrate, dim3(gb), dim3(NT), 0, s, D, K);
-        for (int c = 0; c < big; c++) hipLaunchKernelGGL(k_solve_bucket, dim3(grid[c]), dim3(SOLVE_NT), 0, s, D, Kr, c);
+        for (int c = 0; c < big; c++) launch_solve_bucket(nt, grid[c], s, D, Kr, c);
         if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
         if (ovf) {
             hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kr);
@@ -1625,13 +1635,14 @@ void substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, i
 // used least recently.
 int solve_graph(HostWorld* w, const StepK& K, int big, bool tail, uint32_t novf, int sub) {
     hipStream_t s = w->st;
-    std::vector<uint8_t> key(sizeof(Dev) + sizeof(StepK) + 4 * sizeof(int));
-    int plan[4] = {big, tail ? 1 : 0, novf ? 1 : 0, sub};
+    const int nt = solve_nt_of(w->prm);
+    std::vector<uint8_t> key(sizeof(Dev) + sizeof(StepK) + 5 * sizeof(int));
+    int plan[5] = {big, tail ? 1 : 0, novf ? 1 : 0, sub, nt};
     memcpy(key.data(), &w->D, sizeof(Dev));
     memcpy(key.data() + sizeof(Dev), &K, sizeof(StepK));
     memcpy(key.data() + sizeof(Dev) + sizeof(StepK), plan, sizeof(plan));
     std::vector<unsigned> need(big + 1);
-    for (int c = 0; c < big; c++) need[c] = sf_grid(w->h_col[c], SOLVE_NT, 16384);
+    for (int c = 0; c < big; c++) need[c] = sf_grid(w->h_col[c], nt, 16384);
     need[big] = sf_grid(novf ? novf : 1);
     auto fits = [&](const HostWorld::SubGraph& g) {
         if (!g.exec || g.key != key || g.grid.size() != need.size()) return false;

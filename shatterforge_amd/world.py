@@ -14,8 +14,8 @@ from ._native import lib, check, ptr
 # parameter indices (include/sf.h SF_P_*)
 P = dict(GX=0, GY=1, GZ=2, SUBSTEPS=3, HERTZ=4, DAMPING=5, MAXPUSH=6, FRICTION=7, SPEC=8, SLEEP_LIN=9, SLEEP_ANG=10,
          SLEEP_TIME=11, SLEEP_DIST=12, SLEEP_ROT=13, ALLOW_SLEEP=14, CELL=15, LEAK=16, ANCHOR=17, ANCHOR_PUSH=18,
-         ANCHOR_SLOP=19, COL_LIMIT=20, COL_ROUNDS=21, COL_SPREAD=22, TAIL_MIN=23)
-NPARAM = 24
+         ANCHOR_SLOP=19, COL_LIMIT=20, COL_ROUNDS=21, COL_SPREAD=22, TAIL_MIN=23, SOLVE_NT=24)
+NPARAM = 25
 # statistics (SF_S_*)
 S = dict(AWAKE=0, PAIRS=1, MANIFOLDS=2, POINTS=3, COLOURS=4, OVERFLOW=5, GROUND=6, WAKES=7, ENTRIES=8, CANDIDATES=9,
          DROPPED=10, ROUNDS=11)
@@ -46,7 +46,18 @@ def default_params():
     p[P["COL_ROUNDS"]] = 32
     p[P["COL_SPREAD"]] = 4
     p[P["TAIL_MIN"]] = 1024
+    p[P["SOLVE_NT"]] = 64
     return p
+
+
+def _full(prm):
+    """prm as NPARAM float32s, defaults after the ones given (parameters added later keep their defaults)"""
+    prm = np.ascontiguousarray(prm, np.float32)
+    if len(prm) < NPARAM:
+        full = default_params()
+        full[:len(prm)] = prm
+        prm = full
+    return prm
 
 
 def params(substeps=12, sleep=True, **kw):
@@ -63,7 +74,7 @@ class World:
 
     def __init__(self, body_cap, shape_cap, prm=None):
         L = lib()
-        prm = default_params() if prm is None else np.ascontiguousarray(prm, np.float32)
+        prm = default_params() if prm is None else _full(prm)
         self._prm = prm
         self._h = L.sf_world_create(int(body_cap), int(shape_cap), ptr(prm))
         if not self._h:
@@ -95,8 +106,12 @@ class World:
         except Exception:
             pass
 
+    def params(self):
+        """A copy of the world's parameters."""
+        return self._prm.copy()
+
     def set_params(self, prm):
-        prm = np.ascontiguousarray(prm, np.float32)
+        prm = _full(prm)
         self._prm = prm
         check(lib().sf_world_set_params(self._h, ptr(prm), len(prm)))
 

@@ -144,6 +144,28 @@ def test_graph_replay_same_as_launches():
     wd.close()
 
 
+def test_autotune_table_and_tuned_world_same_results(tmp_path):
+    # every launch shape the tuner may pick computes the same bodies as the defaults
+    from shatterforge_amd import autotune
+    print("case: rubble(1500) tuner, 1 round, then tuned vs default 12 steps", flush=True)
+    b, sb, s = scenes.rubble(1500, box=(12.0, 8.0, 12.0), seed=13)
+    prm = params(substeps=4)
+    table = autotune.tune_box(b, sb, s, prm, rounds=1, path=str(tmp_path / "tune.json"))
+    assert len(table["candidates"]) == 18 and all(r["ms"] > 0 for r in table["candidates"])
+    assert autotune.load(str(tmp_path / "tune.json"))["best"] == table["best"]
+    for cfg in (table["best"], dict(solve_nt=256, tail_min=4096, graph=False), dict(solve_nt=128, tail_min=256, graph=True)):
+        wt = sf.World.from_scene(b, sb, s, prm)
+        autotune.apply(wt, cfg)
+        wd = sf.World.from_scene(b, sb, s, prm)
+        for k in range(12):
+            wt.step(DT)
+            wd.step(DT)
+        assert np.array_equal(wt.bodies(), wd.bodies()), f"{cfg}: bodies differ"
+        assert np.array_equal(wt.colours(), wd.colours())
+        wt.close()
+        wd.close()
+
+
 def test_full_size_rubble_determinism_and_complete_candidates():
     # the full rubble pile, two worlds stepped side by side: fast bodies' swept boxes can have more partners than a
     # shape's candidate slots; the list must still be complete and sorted, whatever order the cell table was filled
