@@ -5,9 +5,10 @@ workgroup of the colour-bucket solve (one manifold per thread; within a colour n
 so any grouping gives the same results), the colour-bucket size below which the small colours are solved one after
 another in one workgroup instead of one launch each, and whether the sub-steps replay as a captured HIP graph.
 
-Every candidate is timed on one world built from the caller's own scene. The candidates take turns step by step, so
-they all see the same sequence of states (the trajectory does not depend on the knobs); each visit runs a few steps
-and times the last, so a graph capture on switching is not counted. The table (per candidate, the median step time)
+Every candidate is timed on one world built from the caller's own scene. The candidates take turns, a few steps each,
+on that one trajectory (it does not depend on the knobs), in the opposite order every other round so that a drift of
+the step cost along the trajectory does not favour the candidates that come last; each visit times only its last
+step, so a graph capture on switching is not counted. The table (per candidate, the median step time)
 is returned and can be written as JSON and applied to a world later.
 """
 import itertools
@@ -29,7 +30,7 @@ def apply(world, best):
     world.set_graph(bool(best["graph"]))
 
 
-def tune_box(bodies, shape_body, shapes, prm, rounds=3, visit_steps=3, warmup=2, dt=1.0 / 60.0, path=None):
+def tune_box(bodies, shape_body, shapes, prm, rounds=4, visit_steps=3, warmup=2, dt=1.0 / 60.0, path=None):
     """Time every candidate on the scene (bodies, shape_body, shapes) with parameters prm; returns the table
     {"candidates": [...], "best": {...}, ...} and writes it to path as JSON when a path is given."""
     import shatterforge_amd as sf
@@ -42,8 +43,10 @@ def tune_box(bodies, shape_body, shapes, prm, rounds=3, visit_steps=3, warmup=2,
             w.step(dt)
         times = [[] for _ in combos]
         base = w.params()
-        for _ in range(rounds):
-            for k, c in enumerate(combos):
+        for rnd in range(rounds):
+            order = range(len(combos)) if rnd % 2 == 0 else range(len(combos) - 1, -1, -1)
+            for k in order:
+                c = combos[k]
                 p = base.copy()
                 p[P["SOLVE_NT"]] = c["solve_nt"]
                 p[P["TAIL_MIN"]] = c["tail_min"]
