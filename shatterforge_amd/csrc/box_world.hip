@@ -82,14 +82,12 @@ struct Dev {
     float4* nar;  // 5 per candidate: n|count, points p|sep
     uint32_t *hit, *hoff;
     ManBuf cur, old;
-    float4* srow;  // rows as built, by canonical index (MR_ROWS x mcap), before the move into colour order
     float4* wrec;  // warm-start records by body contact-list entry: 8 float4 each (6 used)
     // colouring
     int32_t *mcol, *mcand;
     unsigned long long *mpri, *bmask, *claim;
     uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff, *ovcnt;
     uint32_t* chunkc;  // per colour, per chunk of NT manifolds: count, then its exclusive offset within the colour
-    int32_t* order;
     float4* ovl;   // overflow velocity changes by body contact-list entry: v, w (zero outside the overflow)
     // per-slot manifold lists (m * 2 + side), in manifold order
     uint32_t *blcnt, *bloff, *blfill, *blist;
@@ -105,7 +103,6 @@ struct Dev {
 __device__ __forceinline__ M3 load_m3(const float4* p) { M3 r; r.c[0] = v3f4(p[0]); r.c[1] = v3f4(p[1]); r.c[2] = v3f4(p[2]); return r; }
 __device__ __forceinline__ void store_m3(float4* p, const M3& r) { p[0] = f4(r.c[0], 0.0f); p[1] = f4(r.c[1], 0.0f); p[2] = f4(r.c[2], 0.0f); }
 __device__ __forceinline__ float4& MROW(const Dev& D, const ManBuf& M, int r, uint32_t p) { return M.row[(size_t)r * D.mcap + p]; }
-__device__ __forceinline__ float4& SROW(const Dev& D, int r, uint32_t m) { return D.srow[(size_t)r * D.mcap + m]; }
 __device__ __forceinline__ uint32_t cell_key(int x, int y, int z) { return ((uint32_t)x & 1023u) | (((uint32_t)y & 1023u) << 10) | (((uint32_t)z & 1023u) << 20); }
 __device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) { return mix32(key * 0x9E3779B1u) & mask; }
 
@@ -526,13 +523,49 @@ __device__ int find_old(const Dev& D, uint64_t key, uint32_t nold) {
 
 __device__ __forceinline__ float mix_mu(float a, float b) { return a == b ? a : sf_sqrt(a * b); }
 
-// push_manifold (phys.rs:450-498) and the constraint masses (:190-211), one manifold per lane
+// The warm-start record of one side of a manifold: the vectors the warm start adds to that side's body, w += t_k
+// (k < count) then v += tv, w += tf (signs by side), formed from the accumulated impulses with the true masses,
+// exactly as phys.rs:245-262 forms them. It goes to the body's contact-list entry (8 float4 each), so a body's warm
+// start reads its records in one contiguous run.
+__device__ __forceinline__ void write_warm_side(float4* w, int side, int cnt, float4 n4, float4 t14, float4 t24,
+                                                float j1, float j2, float jt, const float* jn, const float4* r2,
+                                                const float4* r3, float4 i14, float4 i24, float4 in4, float im) {
+    float pn = 0.0f;
+    V3 t[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k < cnt) { pn += jn[k]; t[k] = vmul(v3f4(side ? r3[k] : r2[k]), jn[k]); }
+        else t[k] = v3(0, 0, 0);
+    }
+    V3 pp = vadd(vadd(vmul(v3f4(n4), pn), vmul(v3f4(t14), j1)), vmul(v3f4(t24), j2));
+    V3 tf = vadd(vadd(vmul(v3f4(i14), j1), vmul(v3f4(i24), j2)), vmul(v3f4(in4), jt));
+    float c = (float)cnt;
+    if (side == 0) {  // side a subtracts: stored negated
+        w[0] = f4(vneg(t[0]), c); w[1] = f4(vneg(t[1]), 0.0f); w[2] = f4(vneg(t[2]), 0.0f); w[3] = f4(vneg(t[3]), 0.0f);
+        w[4] = f4(vneg(vmul(pp, im)), 0.0f); w[5] = f4(vneg(tf), 0.0f);
+    } else {
+        w[0] = f4(t[0], c); w[1] = f4(t[1], 0.0f); w[2] = f4(t[2], 0.0f); w[3] = f4(t[3], 0.0f);
+        w[4] = f4(vmul(pp, im), 0.0f); w[5] = f4(tf, 0.0f);
+    }
+}
+// both sides of the manifold at position p (ea, eb: the list entries of its two bodies, ~0u for the static world)
+__device__ __forceinline__ void write_warm(const Dev& D, uint32_t ea, uint32_t eb, int cnt, float4 n4, float4 t14,
+                                           float4 t24, float j1, float j2, float jt, const float* jn, const float4* r2,
+                                           const float4* r3, float4 ia14, float4 ia24, float4 ian4, float4 ib14,
+                                           float4 ib24, float4 ibn4, float ima, float imb) {
+    if (ea != ~0u) write_warm_side(D.wrec + (size_t)ea * 8, 0, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ia14, ia24, ian4, ima);
+    if (eb != ~0u) write_warm_side(D.wrec + (size_t)eb * 8, 1, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ib14, ib24, ibn4, imb);
+}
+
+// push_manifold (phys.rs:450-498) and the constraint masses (:190-211), one manifold per lane, written straight into
+// its place in colour order
 __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
     uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
     GRID_STRIDE(i, nc) {
         if (!D.hit[i]) continue;
         uint32_t m = D.hoff[i];
         if (m >= D.mcap) { atomicAdd(&D.ctr[C_DROP], 1u); continue; }
+        const uint32_t p = D.cur.pos[m];  // its place in colour order
         int2 c = D.cand[i];
         int a = c.x, b = c.y;
         int ba = D.shb[a];
@@ -591,6 +624,8 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         float ima = D.sv[sa].w, imb = D.sv[sb].w;
         M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
         const ManBuf& C = D.cur;
+        float jnk[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float4 r2k[4] = {}, r3k[4] = {};
         int ocnt = 0;
         if (oi >= 0) ocnt = min(max((int)MROW(D, D.old, MR_CB1, opos).w, 0), 4);
         for (int k = 0; k < cnt; k++) {
@@ -610,11 +645,12 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             V3 iaxn = mulv(iia, raxn), ibxn = mulv(iib, rbxn);
             float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn);
             float nm = kk > 0.0f ? 1.0f / kk : 0.0f;
-            SROW(D, MR_P0 + MR_PER_PT * k + 0, m) = f4(raxn, pts[k].s);
-            SROW(D, MR_P0 + MR_PER_PT * k + 1, m) = f4(rbxn, nm);
-            SROW(D, MR_P0 + MR_PER_PT * k + 2, m) = f4(iaxn, jn);
-            SROW(D, MR_P0 + MR_PER_PT * k + 3, m) = f4(ibxn, 0.0f);
-            SROW(D, MR_P0 + MR_PER_PT * k + 4, m) = f4(la, 0.0f);
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 0, p) = f4(raxn, pts[k].s);
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 1, p) = f4(rbxn, nm);
+            r2k[k] = f4(iaxn, jn); r3k[k] = f4(ibxn, 0.0f); jnk[k] = jn;
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 2, p) = r2k[k];
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 3, p) = r3k[k];
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 4, p) = f4(la, 0.0f);
         }
         V3 ca1 = vcross(fa, t1), ca2 = vcross(fa, t2), cb1 = vcross(fb, t1), cb2 = vcross(fb, t2);
         V3 ia1 = mulv(iia, ca1), ia2 = mulv(iia, ca2), ib1 = mulv(iib, cb1), ib2 = mulv(iib, cb2);
@@ -624,26 +660,50 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         V3 ian = mulv(iia, n), ibn = mulv(iib, n);
         float kt = vdot(n, ian) + vdot(n, ibn);
         float twm = kt > 0.0f ? 1.0f / kt : 0.0f;
-        C.hdr[m] = make_int4(a, b, sa, sb);
-        C.key[m] = key;
-        SROW(D, MR_N, m) = f4(n, mu);
-        SROW(D, MR_T1, m) = f4(t1, tm1);
-        SROW(D, MR_T2, m) = f4(t2, tm2);
-        SROW(D, MR_E0, m) = f4(e0, twm);
-        SROW(D, MR_J, m) = make_float4(jt1, jt2, jtw, rad);
-        SROW(D, MR_CA1, m) = f4(ca1, ima);
-        SROW(D, MR_CA2, m) = f4(ca2, imb);
-        SROW(D, MR_CB1, m) = f4(cb1, (float)cnt);
-        SROW(D, MR_CB2, m) = f4(cb2, stick ? 1.0f : 0.0f);
-        SROW(D, MR_IA1, m) = f4(ia1, 0.0f);
-        SROW(D, MR_IA2, m) = f4(ia2, 0.0f);
-        SROW(D, MR_IB1, m) = f4(ib1, 0.0f);
-        SROW(D, MR_IB2, m) = f4(ib2, 0.0f);
-        SROW(D, MR_IAN, m) = f4(ian, 0.0f);
-        SROW(D, MR_IBN, m) = f4(ibn, 0.0f);
-        SROW(D, MR_LAF, m) = f4(la_f, 0.0f);
-        SROW(D, MR_LBF, m) = f4(lb_f, 0.0f);
+        C.hdrp[p] = make_int4(a, b, sa, sb);
+        MROW(D, C, MR_N, p) = f4(n, mu);
+        MROW(D, C, MR_T1, p) = f4(t1, tm1);
+        MROW(D, C, MR_T2, p) = f4(t2, tm2);
+        MROW(D, C, MR_E0, p) = f4(e0, twm);
+        MROW(D, C, MR_J, p) = make_float4(jt1, jt2, jtw, rad);
+        MROW(D, C, MR_CA1, p) = f4(ca1, ima);
+        MROW(D, C, MR_CA2, p) = f4(ca2, imb);
+        MROW(D, C, MR_CB1, p) = f4(cb1, (float)cnt);
+        MROW(D, C, MR_CB2, p) = f4(cb2, stick ? 1.0f : 0.0f);
+        MROW(D, C, MR_IA1, p) = f4(ia1, 0.0f);
+        MROW(D, C, MR_IA2, p) = f4(ia2, 0.0f);
+        MROW(D, C, MR_IB1, p) = f4(ib1, 0.0f);
+        MROW(D, C, MR_IB2, p) = f4(ib2, 0.0f);
+        MROW(D, C, MR_IAN, p) = f4(ian, 0.0f);
+        MROW(D, C, MR_IBN, p) = f4(ibn, 0.0f);
+        MROW(D, C, MR_LAF, p) = f4(la_f, 0.0f);
+        MROW(D, C, MR_LBF, p) = f4(lb_f, 0.0f);
         atomicAdd(&D.ctr[C_NPTS], (uint32_t)cnt);
+        // the bodies' list entries of this position, their first warm-start records (from the impulses carried over
+        // from the last step) and their overflow changes cleared
+        const uint32_t ea = sa ? D.minv[2 * m] : ~0u, eb = sb ? D.minv[2 * m + 1] : ~0u;
+        D.blinv[2 * p] = ea;
+        D.blinv[2 * p + 1] = eb;
+        write_warm(D, ea, eb, cnt, f4(n, mu), f4(t1, tm1), f4(t2, tm2), jt1, jt2, jtw, jnk, r2k, r3k, f4(ia1, 0.0f),
+                   f4(ia2, 0.0f), f4(ian, 0.0f), f4(ib1, 0.0f), f4(ib2, 0.0f), f4(ibn, 0.0f), ima, imb);
+        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ea != ~0u) { D.ovl[2 * (size_t)ea] = z; D.ovl[2 * (size_t)ea + 1] = z; }
+        if (eb != ~0u) { D.ovl[2 * (size_t)eb] = z; D.ovl[2 * (size_t)eb + 1] = z; }
+    }
+}
+
+// the manifolds' headers (shapes, body slots) and keys, and the bodies' contact counts, before colouring
+__global__ void __launch_bounds__(NT) k_man_hdr(Dev D) {
+    uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
+    GRID_STRIDE(i, nc) {
+        if (!D.hit[i]) continue;
+        uint32_t m = D.hoff[i];
+        if (m >= D.mcap) continue;
+        int2 c = D.cand[i];
+        int ba = D.shb[c.x];
+        int sa = D.bslot[ba], sb = c.y >= 0 ? D.bslot[D.shb[c.y]] : 0;
+        D.cur.hdr[m] = make_int4(c.x, c.y, sa, sb);
+        D.cur.key[m] = man_key(c.x, c.y);
         if (sa) atomicAdd(&D.blcnt[sa], 1u);
         if (sb) atomicAdd(&D.blcnt[sb], 1u);
     }
@@ -773,75 +833,9 @@ __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
         if (col) {
             uint32_t before = 0;
             for (int k = 0; k < wv; k++) before += wc[k][c];
-            D.order[D.coloff[c] + D.chunkc[(size_t)c * nchunk + b0 / NT] + before + rank] = (int32_t)m;
-        } else if (c == K) D.order[D.coloff[K] + D.ovoff[m]] = (int32_t)m;  // the overflow in manifold order
+            D.cur.pos[m] = D.coloff[c] + D.chunkc[(size_t)c * nchunk + b0 / NT] + before + rank;
+        } else if (c == K) D.cur.pos[m] = D.coloff[K] + D.ovoff[m];  // the overflow in manifold order
         __syncthreads();
-    }
-}
-
-// The warm-start record of one side of a manifold: the vectors the warm start adds to that side's body, w += t_k
-// (k < count) then v += tv, w += tf (signs by side), formed from the accumulated impulses with the true masses,
-// exactly as phys.rs:245-262 forms them. It goes to the body's contact-list entry (8 float4 each), so a body's warm
-// start reads its records in one contiguous run.
-__device__ __forceinline__ void write_warm_side(float4* w, int side, int cnt, float4 n4, float4 t14, float4 t24,
-                                                float j1, float j2, float jt, const float* jn, const float4* r2,
-                                                const float4* r3, float4 i14, float4 i24, float4 in4, float im) {
-    float pn = 0.0f;
-    V3 t[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if (k < cnt) { pn += jn[k]; t[k] = vmul(v3f4(side ? r3[k] : r2[k]), jn[k]); }
-        else t[k] = v3(0, 0, 0);
-    }
-    V3 pp = vadd(vadd(vmul(v3f4(n4), pn), vmul(v3f4(t14), j1)), vmul(v3f4(t24), j2));
-    V3 tf = vadd(vadd(vmul(v3f4(i14), j1), vmul(v3f4(i24), j2)), vmul(v3f4(in4), jt));
-    float c = (float)cnt;
-    if (side == 0) {  // side a subtracts: stored negated
-        w[0] = f4(vneg(t[0]), c); w[1] = f4(vneg(t[1]), 0.0f); w[2] = f4(vneg(t[2]), 0.0f); w[3] = f4(vneg(t[3]), 0.0f);
-        w[4] = f4(vneg(vmul(pp, im)), 0.0f); w[5] = f4(vneg(tf), 0.0f);
-    } else {
-        w[0] = f4(t[0], c); w[1] = f4(t[1], 0.0f); w[2] = f4(t[2], 0.0f); w[3] = f4(t[3], 0.0f);
-        w[4] = f4(vmul(pp, im), 0.0f); w[5] = f4(tf, 0.0f);
-    }
-}
-// both sides of the manifold at position p (ea, eb: the list entries of its two bodies, ~0u for the static world)
-__device__ __forceinline__ void write_warm(const Dev& D, uint32_t ea, uint32_t eb, int cnt, float4 n4, float4 t14,
-                                           float4 t24, float j1, float j2, float jt, const float* jn, const float4* r2,
-                                           const float4* r3, float4 ia14, float4 ia24, float4 ian4, float4 ib14,
-                                           float4 ib24, float4 ibn4, float ima, float imb) {
-    if (ea != ~0u) write_warm_side(D.wrec + (size_t)ea * 8, 0, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ia14, ia24, ian4, ima);
-    if (eb != ~0u) write_warm_side(D.wrec + (size_t)eb * 8, 1, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ib14, ib24, ibn4, imb);
-}
-
-// Move the built rows into colour order (the manifold at position p of the colour-ordered list), note the bodies'
-// list entries of each position, write the first warm-start records into them and clear their overflow changes.
-__global__ void __launch_bounds__(NT) k_permute(Dev D) {
-    uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
-    GRID_STRIDE(p, nm) {
-        uint32_t m = (uint32_t)D.order[p];
-        D.cur.pos[m] = p;
-        int4 hd = D.cur.hdr[m];
-        D.cur.hdrp[p] = hd;
-        uint32_t ea = hd.z ? D.minv[2 * m] : ~0u, eb = hd.w ? D.minv[2 * m + 1] : ~0u;
-        D.blinv[2 * p] = ea;
-        D.blinv[2 * p + 1] = eb;
-#pragma unroll 4
-        for (int r = 0; r < MR_ROWS; r++) MROW(D, D.cur, r, p) = SROW(D, r, m);
-        // the first sub-step's warm start, from the impulses carried over from the last step
-        float4 r2[4], r3[4];
-        float jn[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            r2[k] = SROW(D, MR_P0 + MR_PER_PT * k + 2, m); r3[k] = SROW(D, MR_P0 + MR_PER_PT * k + 3, m); jn[k] = r2[k].w;
-        }
-        float4 j4 = SROW(D, MR_J, m);
-        write_warm(D, ea, eb, (int)SROW(D, MR_CB1, m).w, SROW(D, MR_N, m), SROW(D, MR_T1, m), SROW(D, MR_T2, m), j4.x, j4.y,
-                   j4.z, jn, r2, r3, SROW(D, MR_IA1, m), SROW(D, MR_IA2, m), SROW(D, MR_IAN, m), SROW(D, MR_IB1, m),
-                   SROW(D, MR_IB2, m), SROW(D, MR_IBN, m), SROW(D, MR_CA1, m).w, SROW(D, MR_CA2, m).w);
-        // overflow changes: zero for contacts outside the overflow (the overflow pass overwrites its own)
-        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (ea != ~0u) { D.ovl[2 * (size_t)ea] = z; D.ovl[2 * (size_t)ea + 1] = z; }
-        if (eb != ~0u) { D.ovl[2 * (size_t)eb] = z; D.ovl[2 * (size_t)eb + 1] = z; }
     }
 }
 
@@ -1466,13 +1460,12 @@ int world_alloc(HostWorld* w) {
         r |= w->alloc(M->hdr, D.mcap); r |= w->alloc(M->row, (size_t)MR_ROWS * D.mcap); r |= w->alloc(M->key, D.mcap);
         r |= w->alloc(M->pos, D.mcap); r |= w->alloc(M->hdrp, D.mcap);
     }
-    r |= w->alloc(D.srow, (size_t)MR_ROWS * D.mcap);
     r |= w->alloc(D.wrec, (size_t)16 * D.mcap);  // two list entries per manifold
     r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
     r |= w->alloc(D.chunkc, 72 * (size_t)((D.mcap + NT - 1) / NT + 1));
-    r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap); r |= w->alloc(D.order, D.mcap);
+    r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap);
     r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovl, 4 * (size_t)D.mcap);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blinv, 2 * (size_t)D.mcap); r |= w->alloc(D.minv, 2 * (size_t)D.mcap);
     r |= w->alloc(D.parent, nb); r |= w->alloc(D.sst, nb); r |= w->alloc(D.ctr, C_NCTR);
@@ -1547,7 +1540,7 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_narrow, dim3(gc), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.hit, D.hoff, D.ctr + C_NMAN, D.ctr + C_NCAND, 0, D.ccap, w->scan_tmp, s);
     hipLaunchKernelGGL(k_apply_wake, dim3(gb), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_build, dim3(gc), dim3(NT), 0, s, D, K, w->nold);
+    hipLaunchKernelGGL(k_man_hdr, dim3(gc), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[2], s));
     // colouring rounds (checked for progress every few rounds)
     int R = (int)w->prm[SF_P_COL_ROUNDS];
@@ -1571,13 +1564,13 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_ovf_count, dim3(gm), dim3(NT), 0, s, D, K.K);
-    // per-slot contact lists in manifold order, then the rows into colour order
+    // per-slot contact lists in manifold order, then the manifolds built straight into colour order
     sf::scan_u32(D.blcnt, D.bloff, nullptr, nullptr, (uint32_t)D.nb + 1, (uint32_t)D.nb + 1, w->scan_tmp, s);
     hipLaunchKernelGGL(k_bl_fill, dim3(gm), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort, dim3(sf_grid(((uint64_t)D.nb + 1) * 64, NT, 65536)), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort_big, dim3(1024), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_inv, dim3(sf_grid(2ull * D.mcap)), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_permute, dim3(gm), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_build, dim3(gc), dim3(NT), 0, s, D, K, w->nold);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
     SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
     SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
