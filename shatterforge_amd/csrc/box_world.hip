@@ -939,40 +939,43 @@ __device__ __forceinline__ V3 wave_tree(V3 x) {
 // Gravity, spin clamp, then the warm start (phys.rs:217-222, :245-262): one wavefront per body, each lane one contact
 // (the body's contacts in manifold order, 64 at a time); a contact's contribution is its record's terms summed in
 // point order, and the body adds the tree sum of its contacts' contributions.
+// one body's warm start (its wavefront; every lane holds the same v4 = sv, w = sw on entry)
+__device__ __forceinline__ void warm_body(const Dev& D, const StepK& K, int s, int lane, float4 v4, V3 w) {
+    const V3 gh = vmul(v3(K.gx, K.gy, K.gz), K.h);
+    V3 v = vadd(v3f4(v4), gh);
+    float wl = vlen2(w);
+    if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
+    uint32_t o = D.bloff[s], n = D.blcnt[s];
+    V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
+    for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+        uint32_t e = e0 + lane;
+        V3 dv = v3(0, 0, 0), dw = v3(0, 0, 0);
+        if (e < n) {
+            const float4* wr = D.wrec + (size_t)(o + e) * 8;
+            float4 w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4], w5 = wr[5];
+            int c = (int)w0.w;
+            dw = v3f4(w0);
+            if (c > 1) dw = vadd(dw, v3f4(w1));
+            if (c > 2) dw = vadd(dw, v3f4(w2));
+            if (c > 3) dw = vadd(dw, v3f4(w3));
+            dw = vadd(dw, v3f4(w5));
+            dv = v3f4(w4);
+        }
+        av = vadd(av, wave_tree(dv));
+        aw = vadd(aw, wave_tree(dw));
+    }
+    if (lane == 0) {
+        D.sv[s] = f4(vadd(v, av), v4.w);
+        D.sw[s] = f4(vadd(w, aw), 0.0f);
+    }
+}
 __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    const V3 gh = vmul(v3(K.gx, K.gy, K.gz), K.h);
     for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
         int s = D.bslot[i];
         if (!s) continue;
-        float4 v4 = D.sv[s];
-        V3 v = vadd(v3f4(v4), gh), w = v3f4(D.sw[s]);
-        float wl = vlen2(w);
-        if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
-        uint32_t o = D.bloff[s], n = D.blcnt[s];
-        V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
-        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-            uint32_t e = e0 + lane;
-            V3 dv = v3(0, 0, 0), dw = v3(0, 0, 0);
-            if (e < n) {
-                const float4* wr = D.wrec + (size_t)(o + e) * 8;
-                float4 w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4], w5 = wr[5];
-                int c = (int)w0.w;
-                dw = v3f4(w0);
-                if (c > 1) dw = vadd(dw, v3f4(w1));
-                if (c > 2) dw = vadd(dw, v3f4(w2));
-                if (c > 3) dw = vadd(dw, v3f4(w3));
-                dw = vadd(dw, v3f4(w5));
-                dv = v3f4(w4);
-            }
-            av = vadd(av, wave_tree(dv));
-            aw = vadd(aw, wave_tree(dw));
-        }
-        if (lane == 0) {
-            D.sv[s] = f4(vadd(v, av), v4.w);
-            D.sw[s] = f4(vadd(w, aw), 0.0f);
-        }
+        warm_body(D, K, s, lane, D.sv[s], v3f4(D.sw[s]));
     }
 }
 
@@ -1153,6 +1156,26 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
 }
 // each body takes the average of its copies: the tree sum of the changes over its contacts (zero for contacts not
 // in the overflow), one wavefront per body reading its list entries' changes in one contiguous run
+// one body's average of its overflow copies (its wavefront): v, w (every lane) take the tree sums of the changes
+// over its contacts (zero for contacts not in the overflow) over the count
+__device__ __forceinline__ void ovf_average(const Dev& D, int s, int lane, uint32_t cnt, V3& v, V3& w) {
+    V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
+    uint32_t o = D.bloff[s], n = D.blcnt[s];
+    for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+        uint32_t e = e0 + lane;
+        V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0);
+        if (e < n) {
+            const float4* dv = D.ovl + 2 * (size_t)(o + e);
+            dvv = v3f4(dv[0]);
+            dww = v3f4(dv[1]);
+        }
+        av = vadd(av, wave_tree(dvv));
+        aw = vadd(aw, wave_tree(dww));
+    }
+    float inv = 1.0f / (float)cnt;
+    v = vadd(v, vmul(av, inv));
+    w = vadd(w, vmul(aw, inv));
+}
 __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
@@ -1161,25 +1184,46 @@ __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
         if (!s) continue;
         uint32_t cnt = D.ovcnt[s];
         if (!cnt) continue;
-        V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
-        uint32_t o = D.bloff[s], n = D.blcnt[s];
-        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-            uint32_t e = e0 + lane;
-            V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0);
-            if (e < n) {
-                const float4* dv = D.ovl + 2 * (size_t)(o + e);
-                dvv = v3f4(dv[0]);
-                dww = v3f4(dv[1]);
-            }
-            av = vadd(av, wave_tree(dvv));
-            aw = vadd(aw, wave_tree(dww));
+        float4 v4 = D.sv[s];
+        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
+        ovf_average(D, s, lane, cnt, v, w);
+        if (lane == 0) { D.sv[s] = f4(v, v4.w); D.sw[s] = f4(w, 0.0f); }
+    }
+}
+// the bias pass's end: the overflow average, then the position integration (k_integrate's update), per body
+__device__ __forceinline__ void integrate_slot(const Dev& D, const StepK& K, int s, V3 v, V3 w) {
+    D.sdp[s] = f4(vadd(v3f4(D.sdp[s]), vmul(v, K.h)), 0.0f);
+    D.sth[s] = f4(vadd(v3f4(D.sth[s]), vmul(w, K.h)), 0.0f);
+    D.sq[s] = f4q(qintegrate(qf4(D.sq[s]), w, K.h));
+}
+__global__ void __launch_bounds__(NT) k_ovf_apply_integrate(Dev D, StepK K) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
+        int s = D.bslot[i];
+        if (!s) continue;
+        uint32_t cnt = D.ovcnt[s];
+        float4 v4 = D.sv[s];
+        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
+        if (cnt) {
+            ovf_average(D, s, lane, cnt, v, w);
+            if (lane == 0) { D.sv[s] = f4(v, v4.w); D.sw[s] = f4(w, 0.0f); }
         }
-        if (lane == 0) {
-            float inv = 1.0f / (float)cnt;
-            float4 v4 = D.sv[s];
-            D.sv[s] = f4(vadd(v3f4(v4), vmul(av, inv)), v4.w);
-            D.sw[s] = f4(vadd(v3f4(D.sw[s]), vmul(aw, inv)), 0.0f);
-        }
+        if (lane == 0) integrate_slot(D, K, s, v, w);
+    }
+}
+// the relax pass's end and the next sub-step's start: the overflow average, then the warm start, per body
+__global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
+        int s = D.bslot[i];
+        if (!s) continue;
+        uint32_t cnt = D.ovcnt[s];
+        float4 v4 = D.sv[s];
+        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
+        if (cnt) ovf_average(D, s, lane, cnt, v, w);
+        warm_body(D, K, s, lane, f4(v, v4.w), w);
     }
 }
 __global__ void __launch_bounds__(NT) k_ovf_count(Dev D, int K) {
@@ -1196,10 +1240,7 @@ __global__ void __launch_bounds__(NT) k_integrate(Dev D, StepK K) {
     GRID_STRIDE(i, (uint32_t)D.nb) {
         int s = D.bslot[i];
         if (!s) continue;
-        V3 v = v3f4(D.sv[s]), w = v3f4(D.sw[s]);
-        D.sdp[s] = f4(vadd(v3f4(D.sdp[s]), vmul(v, K.h)), 0.0f);
-        D.sth[s] = f4(vadd(v3f4(D.sth[s]), vmul(w, K.h)), 0.0f);
-        D.sq[s] = f4q(qintegrate(qf4(D.sq[s]), w, K.h));
+        integrate_slot(D, K, s, v3f4(D.sv[s]), v3f4(D.sw[s]));
     }
 }
 
@@ -1578,14 +1619,14 @@ int step_front(HostWorld* w, const StepK& K) {
     return 0;
 }
 
-int launch_bucket(HostWorld* w, const StepK& K, int c) {
+// one colour bucket's launch; with kernel timing on, an event pair around it
+int launch_bucket(HostWorld* w, const StepK& K, int c, unsigned grid) {
     hipStream_t s = w->st;
     if (w->ktime) {
         while (w->kev.size() < w->kev_used + 2) { hipEvent_t e; SF_HIP(hipEventCreate(&e)); w->kev.push_back(e); }
         SF_HIP(hipEventRecord(w->kev[w->kev_used], s));
     }
-    const int nt = solve_nt_of(w->prm);
-    launch_solve_bucket(nt, sf_grid(w->h_col[c], nt, 16384), s, w->D, K, c);
+    launch_solve_bucket(solve_nt_of(w->prm), grid, s, w->D, K, c);
     if (w->ktime) {
         SF_HIP(hipEventRecord(w->kev[w->kev_used + 1], s));
         w->kev_used += 2;
@@ -1597,29 +1638,32 @@ int launch_bucket(HostWorld* w, const StepK& K, int c) {
 
 // The sub-steps' launches with the given grids (one per big colour, then the overflow pass); the kernels read the
 // counts on the device, so grids larger than the counts need are correct.
-void substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, int sub, const unsigned* grid) {
+int substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, int sub, const unsigned* grid) {
     Dev& D = w->D;
     hipStream_t s = w->st;
     const unsigned gb = sf_grid(D.nb), gw = sf_grid((uint64_t)D.nb * 64, NT, 65536);
     StepK Kb = K; Kb.use_bias = 1;
     StepK Kr = K; Kr.use_bias = 0;
-    const int nt = solve_nt_of(w->prm);
     for (int it = 0; it < sub; it++) {
-        hipLaunchKernelGGL(k_warm, dim3(gw), dim3(NT), 0, s, D, K);
-        for (int c = 0; c < big; c++) launch_solve_bucket(nt, grid[c], s, D, Kb, c);
+        // (with an overflow pass, the warm start of sub-steps after the first runs with the last one's average)
+        if (it == 0 || !ovf) hipLaunchKernelGGL(k_warm, dim3(gw), dim3(NT), 0, s, D, K);
+        for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kb, c, grid[c]); if (r) return r; }
         if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kb, big);
         if (ovf) {
             hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kb);
-            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kb);
+            hipLaunchKernelGGL(k_ovf_apply_integrate, dim3(gw), dim3(NT), 0, s, D, K);
+        } else {
+            hipLaunchKernelGGL(k_integrate, dim3(gb), dim3(NT), 0, s, D, K);
         }
-        hipLaunchKernelGGL(k_integrate, dim3(gb), dim3(NT), 0, s, D, K);
-        for (int c = 0; c < big; c++) launch_solve_bucket(nt, grid[c], s, D, Kr, c);
+        for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kr, c, grid[c]); if (r) return r; }
         if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
         if (ovf) {
             hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kr);
-            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
+            if (it + 1 < sub) hipLaunchKernelGGL(k_ovf_apply_warm, dim3(gw), dim3(NT), 0, s, D, K);
+            else hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
         }
     }
+    return 0;
 }
 
 // The sub-steps as one graph launch. A captured graph is replayed while the device tables, the step constants and
@@ -1651,10 +1695,10 @@ int solve_graph(HostWorld* w, const StepK& K, int big, bool tail, uint32_t novf,
         for (int c = 0; c < big; c++) use->grid[c] = std::min(16384u, need[c] + need[c] / 2);
         use->grid[big] = std::min(4096u, need[big] + need[big] / 2);
         SF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        substep_chain(w, K, big, tail, novf != 0, sub, use->grid.data());
+        int rc = substep_chain(w, K, big, tail, novf != 0, sub, use->grid.data());
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
-        if (e != hipSuccess || !g) { sf::set_error(SF_E_HIP, "graph capture failed"); return SF_E_HIP; }
+        if (rc || e != hipSuccess || !g) { if (g) (void)hipGraphDestroy(g); sf::set_error(SF_E_HIP, "graph capture failed"); return SF_E_HIP; }
         use->graph = g;
         SF_HIP(hipGraphInstantiate(&use->exec, use->graph, nullptr, nullptr, 0));
         use->key = key;
@@ -1678,26 +1722,17 @@ int step_back(HostWorld* w, StepK K) {
     bool tail = big < ncol;
     uint32_t novf = w->h_col[K.K];
     int sub = (int)w->prm[SF_P_SUBSTEPS]; if (sub < 1) sub = 1;
-    StepK Kb = K; Kb.use_bias = 1;
-    StepK Kr = K; Kr.use_bias = 0;
     if (w->use_graph && !w->ktime) {
         int r = solve_graph(w, K, big, tail, novf, sub);
         if (r) return r;
-    } else for (int it = 0; it < sub; it++) {
-        hipLaunchKernelGGL(k_warm, dim3(gw), dim3(NT), 0, s, D, K);
-        for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kb, c); if (r) return r; }
-        if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kb, big);
-        if (novf) {
-            hipLaunchKernelGGL(k_ovf_solve, dim3(sf_grid(novf)), dim3(NT), 0, s, D, Kb);
-            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kb);
-        }
-        hipLaunchKernelGGL(k_integrate, dim3(gb), dim3(NT), 0, s, D, K);
-        for (int c = 0; c < big; c++) { int r = launch_bucket(w, Kr, c); if (r) return r; }
-        if (tail) hipLaunchKernelGGL(k_solve_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
-        if (novf) {
-            hipLaunchKernelGGL(k_ovf_solve, dim3(sf_grid(novf)), dim3(NT), 0, s, D, Kr);
-            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
-        }
+    } else {
+        // launched one by one, each grid sized to its count
+        const int nt = solve_nt_of(w->prm);
+        std::vector<unsigned> grid(big + 1);
+        for (int c = 0; c < big; c++) grid[c] = sf_grid(w->h_col[c], nt, 16384);
+        grid[big] = sf_grid(novf ? novf : 1);
+        int r = substep_chain(w, K, big, tail, novf != 0, sub, grid.data());
+        if (r) return r;
     }
     if (w->timing) SF_HIP(hipEventRecord(w->ev[4], s));
     hipLaunchKernelGGL(k_writeback, dim3(gb), dim3(NT), 0, s, D);
