@@ -37,9 +37,11 @@ enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCO
        C_ROUNDS, C_NOVF2, C_NCTR = 16 };
 
 // manifold rows (each row is a float4 array of mcap entries)
-enum { MR_N, MR_T1, MR_T2, MR_E0, MR_J, MR_CA1, MR_CA2, MR_CB1, MR_CB2, MR_IA1, MR_IA2, MR_IB1, MR_IB2, MR_IAN, MR_IBN,
-       MR_LAF, MR_LBF, MR_P0 };
-constexpr int MR_PER_PT = 5;  // raxn|adj, rbxn|nm, iaxn|jn, ibxn|-, la|-
+// (the inertia terms, a body's world inverse inertia times an arm or the normal, are not stored: the solve forms them
+// from the bodies' inverse inertia, fixed for the step, with the build's operations; MR_JN holds the four points'
+// normal impulses)
+enum { MR_N, MR_T1, MR_T2, MR_E0, MR_J, MR_CA1, MR_CA2, MR_CB1, MR_CB2, MR_JN, MR_LAF, MR_LBF, MR_P0 };
+constexpr int MR_PER_PT = 3;  // raxn|separation, rbxn|nm, la|-
 constexpr int MR_ROWS = MR_P0 + 4 * MR_PER_PT;
 
 // A step's manifolds. Canonical index m (candidate order): header, sort key, and pos[m], the manifold's position
@@ -528,17 +530,17 @@ __device__ __forceinline__ float mix_mu(float a, float b) { return a == b ? a : 
 // exactly as phys.rs:245-262 forms them. It goes to the body's contact-list entry (8 float4 each), so a body's warm
 // start reads its records in one contiguous run.
 __device__ __forceinline__ void write_warm_side(float4* w, int side, int cnt, float4 n4, float4 t14, float4 t24,
-                                                float j1, float j2, float jt, const float* jn, const float4* r2,
-                                                const float4* r3, float4 i14, float4 i24, float4 in4, float im) {
+                                                float j1, float j2, float jt, const float* jn, const V3* ixn, V3 i1,
+                                                V3 i2, V3 in, float im) {
     float pn = 0.0f;
     V3 t[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (k < cnt) { pn += jn[k]; t[k] = vmul(v3f4(side ? r3[k] : r2[k]), jn[k]); }
+        if (k < cnt) { pn += jn[k]; t[k] = vmul(ixn[k], jn[k]); }
         else t[k] = v3(0, 0, 0);
     }
     V3 pp = vadd(vadd(vmul(v3f4(n4), pn), vmul(v3f4(t14), j1)), vmul(v3f4(t24), j2));
-    V3 tf = vadd(vadd(vmul(v3f4(i14), j1), vmul(v3f4(i24), j2)), vmul(v3f4(in4), jt));
+    V3 tf = vadd(vadd(vmul(i1, j1), vmul(i2, j2)), vmul(in, jt));
     float c = (float)cnt;
     if (side == 0) {  // side a subtracts: stored negated
         w[0] = f4(vneg(t[0]), c); w[1] = f4(vneg(t[1]), 0.0f); w[2] = f4(vneg(t[2]), 0.0f); w[3] = f4(vneg(t[3]), 0.0f);
@@ -548,13 +550,14 @@ __device__ __forceinline__ void write_warm_side(float4* w, int side, int cnt, fl
         w[4] = f4(vmul(pp, im), 0.0f); w[5] = f4(tf, 0.0f);
     }
 }
-// both sides of the manifold at position p (ea, eb: the list entries of its two bodies, ~0u for the static world)
+// both sides of the manifold (ea, eb: the list entries of its two bodies, ~0u for the static world); iaxn, ibxn, ia*,
+// ib*: the inertia terms of side a and side b (unsplit)
 __device__ __forceinline__ void write_warm(const Dev& D, uint32_t ea, uint32_t eb, int cnt, float4 n4, float4 t14,
-                                           float4 t24, float j1, float j2, float jt, const float* jn, const float4* r2,
-                                           const float4* r3, float4 ia14, float4 ia24, float4 ian4, float4 ib14,
-                                           float4 ib24, float4 ibn4, float ima, float imb) {
-    if (ea != ~0u) write_warm_side(D.wrec + (size_t)ea * 8, 0, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ia14, ia24, ian4, ima);
-    if (eb != ~0u) write_warm_side(D.wrec + (size_t)eb * 8, 1, cnt, n4, t14, t24, j1, j2, jt, jn, r2, r3, ib14, ib24, ibn4, imb);
+                                           float4 t24, float j1, float j2, float jt, const float* jn, const V3* iaxn,
+                                           const V3* ibxn, V3 ia1, V3 ia2, V3 ian, V3 ib1, V3 ib2, V3 ibn, float ima,
+                                           float imb) {
+    if (ea != ~0u) write_warm_side(D.wrec + (size_t)ea * 8, 0, cnt, n4, t14, t24, j1, j2, jt, jn, iaxn, ia1, ia2, ian, ima);
+    if (eb != ~0u) write_warm_side(D.wrec + (size_t)eb * 8, 1, cnt, n4, t14, t24, j1, j2, jt, jn, ibxn, ib1, ib2, ibn, imb);
 }
 
 // push_manifold (phys.rs:450-498) and the constraint masses (:190-211), one manifold per lane, written straight into
@@ -625,7 +628,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
         const ManBuf& C = D.cur;
         float jnk[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        float4 r2k[4] = {}, r3k[4] = {};
+        V3 iaxk[4], ibxk[4];
         int ocnt = 0;
         if (oi >= 0) ocnt = min(max((int)MROW(D, D.old, MR_CB1, opos).w, 0), 4);
         for (int k = 0; k < cnt; k++) {
@@ -636,10 +639,10 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             if (oi >= 0) {
                 float best = 0.03f * 0.03f; int bi = -1;
                 for (int q = 0; q < ocnt; q++) {
-                    float d = vlen2(vsub(v3f4(MROW(D, D.old, MR_P0 + MR_PER_PT * q + 4, opos)), la));
+                    float d = vlen2(vsub(v3f4(MROW(D, D.old, MR_P0 + MR_PER_PT * q + 2, opos)), la));
                     if (d < best) { best = d; bi = q; }
                 }
-                if (bi >= 0) jn = MROW(D, D.old, MR_P0 + MR_PER_PT * bi + 2, opos).w;
+                if (bi >= 0) jn = reinterpret_cast<const float*>(&MROW(D, D.old, MR_JN, opos))[bi];
             }
             V3 raxn = vcross(rap, n), rbxn = vcross(rbp, n);
             V3 iaxn = mulv(iia, raxn), ibxn = mulv(iib, rbxn);
@@ -647,10 +650,8 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             float nm = kk > 0.0f ? 1.0f / kk : 0.0f;
             MROW(D, C, MR_P0 + MR_PER_PT * k + 0, p) = f4(raxn, pts[k].s);
             MROW(D, C, MR_P0 + MR_PER_PT * k + 1, p) = f4(rbxn, nm);
-            r2k[k] = f4(iaxn, jn); r3k[k] = f4(ibxn, 0.0f); jnk[k] = jn;
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 2, p) = r2k[k];
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 3, p) = r3k[k];
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 4, p) = f4(la, 0.0f);
+            iaxk[k] = iaxn; ibxk[k] = ibxn; jnk[k] = jn;
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 2, p) = f4(la, 0.0f);
         }
         V3 ca1 = vcross(fa, t1), ca2 = vcross(fa, t2), cb1 = vcross(fb, t1), cb2 = vcross(fb, t2);
         V3 ia1 = mulv(iia, ca1), ia2 = mulv(iia, ca2), ib1 = mulv(iib, cb1), ib2 = mulv(iib, cb2);
@@ -670,12 +671,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         MROW(D, C, MR_CA2, p) = f4(ca2, imb);
         MROW(D, C, MR_CB1, p) = f4(cb1, (float)cnt);
         MROW(D, C, MR_CB2, p) = f4(cb2, stick ? 1.0f : 0.0f);
-        MROW(D, C, MR_IA1, p) = f4(ia1, 0.0f);
-        MROW(D, C, MR_IA2, p) = f4(ia2, 0.0f);
-        MROW(D, C, MR_IB1, p) = f4(ib1, 0.0f);
-        MROW(D, C, MR_IB2, p) = f4(ib2, 0.0f);
-        MROW(D, C, MR_IAN, p) = f4(ian, 0.0f);
-        MROW(D, C, MR_IBN, p) = f4(ibn, 0.0f);
+        MROW(D, C, MR_JN, p) = make_float4(jnk[0], jnk[1], jnk[2], jnk[3]);
         MROW(D, C, MR_LAF, p) = f4(la_f, 0.0f);
         MROW(D, C, MR_LBF, p) = f4(lb_f, 0.0f);
         atomicAdd(&D.ctr[C_NPTS], (uint32_t)cnt);
@@ -684,8 +680,8 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         const uint32_t ea = sa ? D.minv[2 * m] : ~0u, eb = sb ? D.minv[2 * m + 1] : ~0u;
         D.blinv[2 * p] = ea;
         D.blinv[2 * p + 1] = eb;
-        write_warm(D, ea, eb, cnt, f4(n, mu), f4(t1, tm1), f4(t2, tm2), jt1, jt2, jtw, jnk, r2k, r3k, f4(ia1, 0.0f),
-                   f4(ia2, 0.0f), f4(ian, 0.0f), f4(ib1, 0.0f), f4(ib2, 0.0f), f4(ibn, 0.0f), ima, imb);
+        write_warm(D, ea, eb, cnt, f4(n, mu), f4(t1, tm1), f4(t2, tm2), jt1, jt2, jtw, jnk, iaxk, ibxk, ia1, ia2, ian, ib1,
+                   ib2, ibn, ima, imb);
         const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (ea != ~0u) { D.ovl[2 * (size_t)ea] = z; D.ovl[2 * (size_t)ea + 1] = z; }
         if (eb != ~0u) { D.ovl[2 * (size_t)eb] = z; D.ovl[2 * (size_t)eb + 1] = z; }
@@ -985,23 +981,22 @@ __device__ __forceinline__ float slop_fn(float e, float slop) { return e > slop 
 // reference solves it. fa, fb > 0 (the overflow pass): mass splitting, each side's inverse mass and inertia scaled
 // by the number of overflow manifolds at that body and the effective masses recomputed from the scaled ones.
 __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK& K, V3& va, V3& wa, V3& vb, V3& wb,
-                                           V3 tha, V3 thb, V3 dpab, float fa, float fb, uint32_t ea, uint32_t eb) {
+                                           V3 tha, V3 thb, V3 dpab, float fa, float fb, uint32_t ea, uint32_t eb,
+                                           const M3& iia, const M3& iib) {
     const ManBuf& C = D.cur;
     const bool split = fa > 0.0f;
     // every load of the record up front (they are independent; the stores come at the end)
     float4 n4 = MROW(D, C, MR_N, m), t14 = MROW(D, C, MR_T1, m), t24 = MROW(D, C, MR_T2, m);
     float4 ca14 = MROW(D, C, MR_CA1, m), ca24 = MROW(D, C, MR_CA2, m), cb14 = MROW(D, C, MR_CB1, m), cb24 = MROW(D, C, MR_CB2, m);
-    float4 e04 = MROW(D, C, MR_E0, m), j4 = MROW(D, C, MR_J, m);
-    float4 ia14 = MROW(D, C, MR_IA1, m), ia24 = MROW(D, C, MR_IA2, m), ib14 = MROW(D, C, MR_IB1, m), ib24 = MROW(D, C, MR_IB2, m);
-    float4 ian4 = MROW(D, C, MR_IAN, m), ibn4 = MROW(D, C, MR_IBN, m);
-    float4 r0[4], r1[4], r2[4], r3[4];
+    float4 e04 = MROW(D, C, MR_E0, m), j4 = MROW(D, C, MR_J, m), jn4 = MROW(D, C, MR_JN, m);
+    float4 r0[4], r1[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         r0[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 0, m);
         r1[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 1, m);
-        r2[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m);
-        r3[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 3, m);
     }
+    const float jnv[4] = {jn4.x, jn4.y, jn4.z, jn4.w};
+    V3 iax0[4], ibx0[4];  // the points' inertia terms, unsplit
     V3 n = v3f4(n4), t1 = v3f4(t14), t2 = v3f4(t24);
     float dn = vdot(dpab, n);
     float ima = split ? ca14.w * fa : ca14.w, imb = split ? ca24.w * fb : ca24.w;
@@ -1010,12 +1005,13 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     float njs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (k >= cnt) continue;
+        if (k >= cnt) { iax0[k] = v3(0, 0, 0); ibx0[k] = v3(0, 0, 0); continue; }
         V3 raxn = v3f4(r0[k]), rbxn = v3f4(r1[k]);
-        V3 iaxn = split ? vmul(v3f4(r2[k]), fa) : v3f4(r2[k]), ibxn = split ? vmul(v3f4(r3[k]), fb) : v3f4(r3[k]);
+        iax0[k] = mulv(iia, raxn); ibx0[k] = mulv(iib, rbxn);
+        V3 iaxn = split ? vmul(iax0[k], fa) : iax0[k], ibxn = split ? vmul(ibx0[k], fb) : ibx0[k];
         float nm = r1[k].w;
         if (split) { float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
-        float jn = r2[k].w;
+        float jn = jnv[k];
         float s = r0[k].w + dn + vdot(thb, rbxn) - vdot(tha, raxn);
         float bias, ms, is;
         if (s > 0.0f) { bias = s * K.inv_h; ms = 1.0f; is = 0.0f; }
@@ -1030,8 +1026,10 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
         va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(iaxn, imp));
         vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(ibxn, imp));
     }
-    V3 ia1 = v3f4(ia14), ia2 = v3f4(ia24), ib1 = v3f4(ib14), ib2 = v3f4(ib24), ian = v3f4(ian4), ibn = v3f4(ibn4);
     V3 ca1 = v3f4(ca14), ca2 = v3f4(ca24), cb1 = v3f4(cb14), cb2 = v3f4(cb24);
+    const V3 ia10 = mulv(iia, ca1), ia20 = mulv(iia, ca2), ib10 = mulv(iib, cb1), ib20 = mulv(iib, cb2);
+    const V3 ian0 = mulv(iia, n), ibn0 = mulv(iib, n);
+    V3 ia1 = ia10, ia2 = ia20, ib1 = ib10, ib2 = ib20, ian = ian0, ibn = ibn0;
     float tm1 = t14.w, tm2 = t24.w, twm = e04.w;
     if (split) {
         ia1 = vmul(ia1, fa); ia2 = vmul(ia2, fa); ian = vmul(ian, fa); ib1 = vmul(ib1, fb); ib2 = vmul(ib2, fb); ibn = vmul(ibn, fb);
@@ -1075,12 +1073,13 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     wb = vadd(wb, vmul(ibn, dtw));
     // stores: impulses and the stick flag
 #pragma unroll
-    for (int k = 0; k < 4; k++) if (k < cnt) reinterpret_cast<float*>(&MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m))[3] = njs[k];
+    for (int k = 0; k < 4; k++) if (k >= cnt) njs[k] = jnv[k];
+    MROW(D, C, MR_JN, m) = make_float4(njs[0], njs[1], njs[2], njs[3]);
     MROW(D, C, MR_J, m) = make_float4(j1, j2, jt, j4.w);
     if (unstick) reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;
     // the relax pass ends the sub-step: its impulses are the next sub-step's warm start
     if (!K.use_bias)
-        write_warm(D, ea, eb, cnt, n4, t14, t24, j1, j2, jt, njs, r2, r3, ia14, ia24, ian4, ib14, ib24, ibn4, ca14.w, ca24.w);
+        write_warm(D, ea, eb, cnt, n4, t14, t24, j1, j2, jt, njs, iax0, ibx0, ia10, ia20, ian0, ib10, ib20, ibn0, ca14.w, ca24.w);
 }
 
 // one manifold of the colour-ordered solve (p: its position in colour order)
@@ -1094,7 +1093,8 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
     V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
     // the relax pass writes the next warm start into the bodies' list entries
     uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
-    solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, 0.0f, 0.0f, ea, eb);
+    const M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
+    solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, 0.0f, 0.0f, ea, eb, iia, iib);
     if (sa) { D.sv[sa] = f4(va, va4.w); D.sw[sa] = f4(wa, 0.0f); }
     if (sb) { D.sv[sb] = f4(vb, vb4.w); D.sw[sb] = f4(wb, 0.0f); }
 }
@@ -1147,8 +1147,9 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
         V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
         float fa = sa ? (float)D.ovcnt[sa] : 1.0f, fb = sb ? (float)D.ovcnt[sb] : 1.0f;
         uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
+        const M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
         solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, fa, fb,
-                   K.use_bias ? ~0u : ea, K.use_bias ? ~0u : eb);
+                   K.use_bias ? ~0u : ea, K.use_bias ? ~0u : eb, iia, iib);
         // each side's change into its body's list entry
         if (ea != ~0u) { D.ovl[2 * (size_t)ea] = f4(vsub(va, va0), 0.0f); D.ovl[2 * (size_t)ea + 1] = f4(vsub(wa, wa0), 0.0f); }
         if (eb != ~0u) { D.ovl[2 * (size_t)eb] = f4(vsub(vb, vb0), 0.0f); D.ovl[2 * (size_t)eb + 1] = f4(vsub(wb, wb0), 0.0f); }
@@ -1951,7 +1952,8 @@ int sf_world_manifolds(sf_world* w, int32_t* keys, float* data, int cap) {
         float4 n4 = R(MR_N, m);
         f[0] = n4.x; f[1] = n4.y; f[2] = n4.z; f[3] = (float)cnt;
         for (int q = 0; q < cnt; q++) {
-            float4 a = R(MR_P0 + MR_PER_PT * q, m), b = R(MR_P0 + MR_PER_PT * q + 1, m), c = R(MR_P0 + MR_PER_PT * q + 2, m);
+            float4 a = R(MR_P0 + MR_PER_PT * q, m), b = R(MR_P0 + MR_PER_PT * q + 1, m), jn = R(MR_JN, m);
+            float4 c = make_float4(0.0f, 0.0f, 0.0f, q == 0 ? jn.x : (q == 1 ? jn.y : (q == 2 ? jn.z : jn.w)));
             float* o = f + 4 + 8 * q;
             o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = b.x; o[4] = b.y; o[5] = b.z; o[6] = a.w; o[7] = c.w;
         }
