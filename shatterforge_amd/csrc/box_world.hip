@@ -90,6 +90,7 @@ struct Dev {
     unsigned long long *mpri, *bmask, *claim;
     uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff, *ovcnt;
     uint32_t* chunkc;  // per colour, per chunk of NT manifolds: count, then its exclusive offset within the colour
+    uint32_t* ptpart;  // contact points per workgroup of the build (summed by k_sum_points)
     float4* ovl;   // overflow velocity changes by body contact-list entry: v, w (zero outside the overflow)
     // per-slot manifold lists (m * 2 + side), in manifold order
     uint32_t *blcnt, *bloff, *blfill, *blist;
@@ -563,7 +564,9 @@ __device__ __forceinline__ void write_warm(const Dev& D, uint32_t ea, uint32_t e
 // push_manifold (phys.rs:450-498) and the constraint masses (:190-211), one manifold per lane, written straight into
 // its place in colour order
 __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
+    __shared__ uint32_t wsum[NT / 64];
     uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
+    uint32_t npts = 0;  // this thread's contact points (the block's sum goes to ptpart, not to one global counter)
     GRID_STRIDE(i, nc) {
         if (!D.hit[i]) continue;
         uint32_t m = D.hoff[i];
@@ -674,7 +677,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         MROW(D, C, MR_JN, p) = make_float4(jnk[0], jnk[1], jnk[2], jnk[3]);
         MROW(D, C, MR_LAF, p) = f4(la_f, 0.0f);
         MROW(D, C, MR_LBF, p) = f4(lb_f, 0.0f);
-        atomicAdd(&D.ctr[C_NPTS], (uint32_t)cnt);
+        npts += (uint32_t)cnt;
         // the bodies' list entries of this position, their first warm-start records (from the impulses carried over
         // from the last step) and their overflow changes cleared
         const uint32_t ea = sa ? D.minv[2 * m] : ~0u, eb = sb ? D.minv[2 * m + 1] : ~0u;
@@ -685,6 +688,28 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (ea != ~0u) { D.ovl[2 * (size_t)ea] = z; D.ovl[2 * (size_t)ea + 1] = z; }
         if (eb != ~0u) { D.ovl[2 * (size_t)eb] = z; D.ovl[2 * (size_t)eb + 1] = z; }
+    }
+    for (int h = 32; h >= 1; h >>= 1) npts += __shfl_xor(npts, h, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = npts;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < NT / 64; k++) t += wsum[k];
+        D.ptpart[blockIdx.x] = t;
+    }
+}
+// the contact points of the step: the sum of k_build's per-block partials
+__global__ void __launch_bounds__(NT) k_sum_points(Dev D, uint32_t nparts) {
+    __shared__ uint32_t wsum[NT / 64];
+    uint32_t t = 0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += NT) t += D.ptpart[i];
+    for (int h = 32; h >= 1; h >>= 1) t += __shfl_xor(t, h, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a = 0;
+        for (int k = 0; k < NT / 64; k++) a += wsum[k];
+        D.ctr[C_NPTS] = a;
     }
 }
 
@@ -711,9 +736,12 @@ __global__ void __launch_bounds__(NT) k_col_init(Dev D) {
     GRID_STRIDE(m, nm) D.mcol[m] = -1;
 }
 
+// (C_ACTIVE is a flag, set by one plain store per wavefront that claimed: tens of thousands of atomic adds on one
+// address serialise)
 __global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int r) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     const unsigned long long all = K == 64 ? ~0ull : ((1ull << K) - 1ull);
+    bool claimed = false;
     GRID_STRIDE(m, nm) {
         if (D.mcol[m] != -1) continue;
         int4 hd = D.cur.hdr[m];
@@ -734,8 +762,9 @@ __global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int 
         D.mpri[m] = pri;
         if (sa) atomicMax(&D.claim[(size_t)sa * K + c], pri);
         if (sb) atomicMax(&D.claim[(size_t)sb * K + c], pri);
-        atomicAdd(&D.ctr[C_ACTIVE], 1u);
+        claimed = true;
     }
+    if (__ballot(claimed) && (threadIdx.x & 63) == 0) D.ctr[C_ACTIVE] = 1u;
 }
 
 __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
@@ -753,8 +782,9 @@ __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
     }
 }
 
-// colour histogram: per-block counts in LDS, one global add per colour per block (a few colours take every
-// manifold, so per-manifold global atomics would serialise on a handful of addresses)
+// colour histogram: per-block counts in LDS, stored per colour per chunk (a few colours take every manifold, so
+// global atomics, per manifold or per block, would serialise on a handful of addresses); the colour totals come
+// from the chunk scan
 __global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
     __shared__ uint32_t hist[72];
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
@@ -771,21 +801,16 @@ __global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
         __syncthreads();
         const uint32_t nchunk = (D.mcap + NT - 1) / NT, chunk = b0 / NT;
         for (int c = threadIdx.x; c < K; c += NT) D.chunkc[(size_t)c * nchunk + chunk] = hist[c];
-        for (int c = threadIdx.x; c <= K; c += NT) {
-            uint32_t h = hist[c];
-            if (!h) continue;
-            atomicAdd(&D.colcnt[c], h);
-            if (c < K) atomicMax(&D.ctr[C_NCOL], (uint32_t)c + 1u);
-            else atomicAdd(&D.ctr[C_NOVF], h);
-        }
         __syncthreads();
     }
 }
 
 // per colour, the exclusive scan of its chunk counts in chunk order (one block per colour)
+// block K: the overflow's total (from the scan of the overflow flags) into its bucket count
 __global__ void __launch_bounds__(NT) k_col_chunk_scan(Dev D, int K) {
     __shared__ uint32_t ws[NT / 64];
     const int c = blockIdx.x;
+    if (c == K && threadIdx.x == 0) D.colcnt[K] = D.ctr[C_NOVF];
     if (c >= K) return;
     const uint32_t nm = min(D.ctr[C_NMAN], D.mcap), nchunk = (D.mcap + NT - 1) / NT, used = (nm + NT - 1) / NT;
     uint32_t* a = D.chunkc + (size_t)c * nchunk;
@@ -801,6 +826,10 @@ __global__ void __launch_bounds__(NT) k_col_chunk_scan(Dev D, int K) {
         if (i < used) a[i] = carry + before + x - v;
         carry += all;
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        D.colcnt[c] = carry;
+        if (carry) atomicMax(&D.ctr[C_NCOL], (uint32_t)c + 1u);
     }
 }
 // colour-ordered list, stable: within a colour, manifolds keep their canonical order (chunk offsets from the scan,
@@ -1507,6 +1536,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
     r |= w->alloc(D.chunkc, 72 * (size_t)((D.mcap + NT - 1) / NT + 1));
+    r |= w->alloc(D.ptpart, 4096);  // k_build's grid: sf_grid(ccap) is at most 4096 workgroups
     r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap);
     r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovl, 4 * (size_t)D.mcap);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blinv, 2 * (size_t)D.mcap); r |= w->alloc(D.minv, 2 * (size_t)D.mcap);
@@ -1601,9 +1631,9 @@ int step_front(HostWorld* w, const StepK& K) {
     }
     w->stats[SF_S_ROUNDS] = rounds;
     hipLaunchKernelGGL(k_col_count, dim3(gm), dim3(NT), 0, s, D, K.K);
+    sf::scan_u32(D.ovflag, D.ovoff, D.ctr + C_NOVF, D.ctr + C_NMAN, 0, D.mcap, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K + 1), dim3(NT), 0, s, D, K.K);
     sf::scan_u32(D.colcnt, D.coloff, nullptr, nullptr, (uint32_t)K.K + 1, 72, w->scan_tmp, s);
-    sf::scan_u32(D.ovflag, D.ovoff, nullptr, D.ctr + C_NMAN, 0, D.mcap, w->scan_tmp, s);
-    hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_ovf_count, dim3(gm), dim3(NT), 0, s, D, K.K);
     // per-slot contact lists in manifold order, then the manifolds built straight into colour order
@@ -1613,6 +1643,7 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_bl_sort_big, dim3(1024), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_inv, dim3(sf_grid(2ull * D.mcap)), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_build, dim3(gc), dim3(NT), 0, s, D, K, w->nold);
+    hipLaunchKernelGGL(k_sum_points, dim3(1), dim3(NT), 0, s, D, gc);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
     SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
     SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
