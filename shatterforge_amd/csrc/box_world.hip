@@ -128,7 +128,23 @@ __device__ __forceinline__ CellRange cell_range(V3 lo, V3 hi, float inv_cell) {
 }
 
 // ---------------------------------------------------------------- prepare
+// A statistics count added from a whole workgroup at once (every thread calls it with its own share): one atomic per
+// workgroup, not one per wavefront on a single address.
+__device__ __forceinline__ void block_count(uint32_t v, uint32_t* dst) {
+    __shared__ uint32_t part[NT / 64];
+    for (int h = 32; h >= 1; h >>= 1) v += __shfl_xor(v, h, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < NT / 64; k++) t += part[k];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(NT) k_prep_bodies(Dev D) {
+    uint32_t nawake = 0;
     GRID_STRIDE(i, (uint32_t)D.nb) {
         float4 q4 = D.bq[i];
         M3 R = q_to_m3(qf4(q4));
@@ -148,9 +164,10 @@ __global__ void __launch_bounds__(NT) k_prep_bodies(Dev D) {
             D.sq[slot] = q4;
             D.sp0[slot] = make_float4(p.x, p.y, p.z, 0.0f);
             store_m3(D.sii + 3 * (size_t)slot, sandwich(R, v3f4(D.bii[i])));
-            atomicAdd(&D.ctr[C_AWAKE], 1u);
+            nawake++;
         }
     }
+    block_count(nawake, &D.ctr[C_AWAKE]);
 }
 
 __global__ void __launch_bounds__(NT) k_prep_shapes(Dev D, StepK K) {
@@ -257,6 +274,7 @@ __global__ void __launch_bounds__(NT) k_cand_count(Dev D, StepK K) {
 }
 // per shape: its candidate count (ground first) and each entry's offset within the shape's list
 __global__ void __launch_bounds__(NT) k_cand_shape(Dev D) {
+    uint32_t ng = 0, np = 0;
     GRID_STRIDE(a, (uint32_t)D.ns) {
         uint32_t g = (uint32_t)ground_cand(D, a), o = g, e0 = D.eoff[a], n = D.ecnt[a];
         uint32_t ne = min(D.ctr[C_NE], D.ecap);
@@ -268,9 +286,11 @@ __global__ void __launch_bounds__(NT) k_cand_shape(Dev D) {
             o += c;
         }
         D.ccnt[a] = o;
-        if (g) atomicAdd(&D.ctr[C_NGROUND], 1u);
-        if (o > g) atomicAdd(&D.ctr[C_NPAIR], o - g);
+        ng += g;
+        np += o - g;
     }
+    block_count(ng, &D.ctr[C_NGROUND]);
+    block_count(np, &D.ctr[C_NPAIR]);
 }
 __global__ void __launch_bounds__(NT) k_cand_write(Dev D, StepK K) {
     uint32_t ne = min(D.ctr[C_NE], D.ecap);
