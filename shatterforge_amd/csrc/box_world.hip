@@ -79,6 +79,7 @@ struct Dev {
     uint32_t *tcnt, *tstart, *tfill, *tkey;
     int32_t* tent;    // shape of each table entry
     uint32_t *epc, *erel;  // per cell entry: partners homed there, offset within the shape's candidate list
+    int32_t* ekeep;        // per cell entry: its first ENTRY_KEEP partners, as the count pass found them
     uint32_t *ccnt, *coff;
     int2* cand;
     float4* nar;  // 5 per candidate: n|count, points p|sep
@@ -263,13 +264,17 @@ __device__ __forceinline__ uint32_t cell_partners(const Dev& D, const StepK& K, 
     }
     return n;
 }
+// the count pass keeps up to ENTRY_KEEP partners of each entry, so the write pass walks the cell again only for
+// entries with more
+constexpr uint32_t ENTRY_KEEP = 8;
 __global__ void __launch_bounds__(NT) k_cand_count(Dev D, StepK K) {
     uint32_t ne = min(D.ctr[C_NE], D.ecap);
     GRID_STRIDE(e, ne) {
         uint32_t a = (uint32_t)D.eshape[e];
         int x, y, z;
         entry_cell(D, K, a, e, x, y, z);
-        D.epc[e] = cell_partners(D, K, (int)a, x, y, z, [](uint32_t, int) {});
+        int32_t* keep = D.ekeep + (size_t)e * ENTRY_KEEP;
+        D.epc[e] = cell_partners(D, K, (int)a, x, y, z, [&](uint32_t j, int b) { if (j < ENTRY_KEEP) keep[j] = b; });
     }
 }
 // per shape: its candidate count (ground first) and each entry's offset within the shape's list
@@ -299,10 +304,16 @@ __global__ void __launch_bounds__(NT) k_cand_write(Dev D, StepK K) {
         if (!D.epc[e]) continue;
         if (D.coff[a] + D.ccnt[a] > D.ccap) { atomicAdd(&D.ctr[C_DROP], D.epc[e]); continue; }  // the shape's list is dropped whole
         uint32_t o = D.coff[a] + D.erel[e];
-        int x, y, z;
-        entry_cell(D, K, a, e, x, y, z);
         int2* out = D.cand + o;
-        cell_partners(D, K, (int)a, x, y, z, [&](uint32_t j, int b) { out[j] = make_int2((int)a, b); });
+        const uint32_t n = D.epc[e];
+        if (n <= ENTRY_KEEP) {
+            const int32_t* keep = D.ekeep + (size_t)e * ENTRY_KEEP;
+            for (uint32_t j = 0; j < n; j++) out[j] = make_int2((int)a, keep[j]);
+        } else {
+            int x, y, z;
+            entry_cell(D, K, a, e, x, y, z);
+            cell_partners(D, K, (int)a, x, y, z, [&](uint32_t j, int b) { out[j] = make_int2((int)a, b); });
+        }
     }
 }
 // per shape (one wavefront each): the ground candidate first, the partners ascending after it
@@ -1544,7 +1555,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
     r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
     r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
-    r |= w->alloc(D.epc, D.ecap); r |= w->alloc(D.erel, D.ecap);
+    r |= w->alloc(D.epc, D.ecap); r |= w->alloc(D.erel, D.ecap); r |= w->alloc(D.ekeep, (size_t)ENTRY_KEEP * D.ecap);
     r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
     r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
     for (ManBuf* M : {&D.cur, &D.old}) {
