@@ -1063,9 +1063,12 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     int cnt = (int)cb14.w;
     float tot = 0.0f;
     float njs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // Every point is evaluated and only the first cnt are applied (selects, not a branch): a branch lets the compiler
+    // move each point's row loads after the previous point's updates, one memory round trip per point; this way the
+    // rows load with the rest of the record. For the points applied the operations are the same.
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (k >= cnt) { iax0[k] = v3(0, 0, 0); ibx0[k] = v3(0, 0, 0); continue; }
+        const bool on = k < cnt;
         V3 raxn = v3f4(r0[k]), rbxn = v3f4(r1[k]);
         iax0[k] = mulv(iia, raxn); ibx0[k] = mulv(iib, rbxn);
         V3 iaxn = split ? vmul(iax0[k], fa) : iax0[k], ibxn = split ? vmul(ibx0[k], fb) : ibx0[k];
@@ -1081,10 +1084,12 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
         float imp = -nm * ms * (vn + bias) - is * jn;
         float nj = sf_max(jn + imp, 0.0f);
         imp = nj - jn;
-        tot += nj;
-        njs[k] = nj;
-        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(iaxn, imp));
-        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(ibxn, imp));
+        const float tot1 = tot + nj;
+        const V3 va1 = vsub(va, vmul(n, imp * ima)), wa1 = vsub(wa, vmul(iaxn, imp));
+        const V3 vb1 = vadd(vb, vmul(n, imp * imb)), wb1 = vadd(wb, vmul(ibxn, imp));
+        tot = on ? tot1 : tot;
+        njs[k] = on ? nj : jnv[k];
+        va = vsel(on, va1, va); wa = vsel(on, wa1, wa); vb = vsel(on, vb1, vb); wb = vsel(on, wb1, wb);
     }
     V3 ca1 = v3f4(ca14), ca2 = v3f4(ca24), cb1 = v3f4(cb14), cb2 = v3f4(cb24);
     const V3 ia10 = mulv(iia, ca1), ia20 = mulv(iia, ca2), ib10 = mulv(iib, cb1), ib20 = mulv(iib, cb2);
@@ -1132,8 +1137,6 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     wa = vsub(wa, vmul(ian, dtw));
     wb = vadd(wb, vmul(ibn, dtw));
     // stores: impulses and the stick flag
-#pragma unroll
-    for (int k = 0; k < 4; k++) if (k >= cnt) njs[k] = jnv[k];
     MROW(D, C, MR_JN, m) = make_float4(njs[0], njs[1], njs[2], njs[3]);
     MROW(D, C, MR_J, m) = make_float4(j1, j2, jt, j4.w);
     if (unstick) reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;

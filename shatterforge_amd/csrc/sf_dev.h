@@ -16,6 +16,7 @@ SF_HD V3 v3(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return
 SF_HD V3 v3f4(float4 a) { return v3(a.x, a.y, a.z); }
 SF_HD float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 SF_HD V3 vadd(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+SF_HD V3 vsel(bool c, V3 a, V3 b) { return v3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
 SF_HD V3 vsub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 SF_HD V3 vmul(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
 SF_HD V3 vneg(V3 a) { return v3(-a.x, -a.y, -a.z); }
