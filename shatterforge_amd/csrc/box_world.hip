@@ -1276,18 +1276,58 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_integrate(Dev D, StepK K) {
         if (lane == 0) integrate_slot(D, K, s, v, w);
     }
 }
-// the relax pass's end and the next sub-step's start: the overflow average, then the warm start, per body
+// The relax pass's end and the next sub-step's start: the overflow average, then the warm start, per body. Both
+// sets of per-contact records are loaded in one pass over the body's list (they do not depend on each other), and
+// each tree sums its records as ovf_average and warm_body do.
 __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    const V3 gh = vmul(v3(K.gx, K.gy, K.gz), K.h);
     for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
         int s = D.bslot[i];
         if (!s) continue;
-        uint32_t cnt = D.ovcnt[s];
-        float4 v4 = D.sv[s];
+        const uint32_t cnt = D.ovcnt[s], o = D.bloff[s], n = D.blcnt[s];
+        const float4 v4 = D.sv[s];
         V3 v = v3f4(v4), w = v3f4(D.sw[s]);
-        if (cnt) ovf_average(D, s, lane, cnt, v, w);
-        warm_body(D, K, s, lane, f4(v, v4.w), w);
+        V3 ov = v3(0, 0, 0), ow = v3(0, 0, 0), av = v3(0, 0, 0), aw = v3(0, 0, 0);
+        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+            uint32_t e = e0 + lane;
+            V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0), dv = v3(0, 0, 0), dw = v3(0, 0, 0);
+            if (e < n) {
+                const float4* ol = D.ovl + 2 * (size_t)(o + e);
+                const float4* wr = D.wrec + (size_t)(o + e) * 8;
+                float4 o0 = cnt ? ol[0] : make_float4(0, 0, 0, 0), o1 = cnt ? ol[1] : make_float4(0, 0, 0, 0);
+                float4 w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4], w5 = wr[5];
+                dvv = v3f4(o0);
+                dww = v3f4(o1);
+                int c = (int)w0.w;
+                dw = v3f4(w0);
+                if (c > 1) dw = vadd(dw, v3f4(w1));
+                if (c > 2) dw = vadd(dw, v3f4(w2));
+                if (c > 3) dw = vadd(dw, v3f4(w3));
+                dw = vadd(dw, v3f4(w5));
+                dv = v3f4(w4);
+            }
+            if (cnt) {
+                ov = vadd(ov, wave_tree(dvv));
+                ow = vadd(ow, wave_tree(dww));
+            }
+            av = vadd(av, wave_tree(dv));
+            aw = vadd(aw, wave_tree(dw));
+        }
+        if (cnt) {
+            float inv = 1.0f / (float)cnt;
+            v = vadd(v, vmul(ov, inv));
+            w = vadd(w, vmul(ow, inv));
+        }
+        // the warm start (warm_body's operations)
+        V3 vw = vadd(v, gh);
+        float wl = vlen2(w);
+        if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
+        if (lane == 0) {
+            D.sv[s] = f4(vadd(vw, av), v4.w);
+            D.sw[s] = f4(vadd(w, aw), 0.0f);
+        }
     }
 }
 __global__ void __launch_bounds__(NT) k_ovf_count(Dev D, int K) {
