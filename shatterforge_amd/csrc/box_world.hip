@@ -69,7 +69,10 @@ struct Dev {
     int32_t *bflag, *bslot;
     uint32_t* wake;
     // solver slots (nb + 1, slot 0 = the static dummy, all zero)
-    float4 *sv, *sw, *sdp, *sth, *sq, *sp0, *sii;  // sv: v|inv_m, sii: 3 per slot
+    // solver slots: one 128-byte record each (SLOT_F4 float4: v|inv_m, w, position change, rotation change, q,
+    // world inverse inertia rows), so the two bodies a manifold gathers are two cache lines, not a dozen
+    float4* sbod;
+    float4* sp0;
     // shapes
     int32_t* shb;
     float4 *slp, *slr, *shh, *swp, *swr, *slo, *shi;  // slp: lp|reach, slr/swr: 3 per shape, shh: h|mu
@@ -102,6 +105,13 @@ struct Dev {
     uint32_t* ctr;
 };
 
+constexpr int SLOT_F4 = 8;
+__device__ __forceinline__ float4& SV(const Dev& D, int s) { return D.sbod[(size_t)s * SLOT_F4 + 0]; }
+__device__ __forceinline__ float4& SW(const Dev& D, int s) { return D.sbod[(size_t)s * SLOT_F4 + 1]; }
+__device__ __forceinline__ float4& SDP(const Dev& D, int s) { return D.sbod[(size_t)s * SLOT_F4 + 2]; }
+__device__ __forceinline__ float4& STH(const Dev& D, int s) { return D.sbod[(size_t)s * SLOT_F4 + 3]; }
+__device__ __forceinline__ float4& SQ(const Dev& D, int s) { return D.sbod[(size_t)s * SLOT_F4 + 4]; }
+__device__ __forceinline__ float4* SII(const Dev& D, int s) { return D.sbod + (size_t)s * SLOT_F4 + 5; }
 #define GRID_STRIDE(i, n) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 __device__ __forceinline__ M3 load_m3(const float4* p) { M3 r; r.c[0] = v3f4(p[0]); r.c[1] = v3f4(p[1]); r.c[2] = v3f4(p[2]); return r; }
@@ -158,13 +168,13 @@ __global__ void __launch_bounds__(NT) k_prep_bodies(Dev D) {
         D.sst[i] = 0u;
         if (awake) {
             float4 p = D.bp[i];
-            D.sv[slot] = make_float4(D.bv[i].x, D.bv[i].y, D.bv[i].z, p.w);
-            D.sw[slot] = make_float4(D.bw[i].x, D.bw[i].y, D.bw[i].z, 0.0f);
-            D.sdp[slot] = make_float4(0, 0, 0, 0);
-            D.sth[slot] = make_float4(0, 0, 0, 0);
-            D.sq[slot] = q4;
+            SV(D, slot) = make_float4(D.bv[i].x, D.bv[i].y, D.bv[i].z, p.w);
+            SW(D, slot) = make_float4(D.bw[i].x, D.bw[i].y, D.bw[i].z, 0.0f);
+            SDP(D, slot) = make_float4(0, 0, 0, 0);
+            STH(D, slot) = make_float4(0, 0, 0, 0);
+            SQ(D, slot) = q4;
             D.sp0[slot] = make_float4(p.x, p.y, p.z, 0.0f);
-            store_m3(D.sii + 3 * (size_t)slot, sandwich(R, v3f4(D.bii[i])));
+            store_m3(SII(D, slot), sandwich(R, v3f4(D.bii[i])));
             nawake++;
         }
     }
@@ -658,8 +668,8 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             e0 = v3(0, 0, 0);
         }
         // masses (the static side: slot 0, zero inverse mass and inertia)
-        float ima = D.sv[sa].w, imb = D.sv[sb].w;
-        M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
+        float ima = SV(D, sa).w, imb = SV(D, sb).w;
+        M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
         const ManBuf& C = D.cur;
         float jnk[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         V3 iaxk[4], ibxk[4];
@@ -1021,8 +1031,8 @@ __device__ __forceinline__ void warm_body(const Dev& D, const StepK& K, int s, i
         aw = vadd(aw, wave_tree(dw));
     }
     if (lane == 0) {
-        D.sv[s] = f4(vadd(v, av), v4.w);
-        D.sw[s] = f4(vadd(w, aw), 0.0f);
+        SV(D, s) = f4(vadd(v, av), v4.w);
+        SW(D, s) = f4(vadd(w, aw), 0.0f);
     }
 }
 __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
@@ -1031,7 +1041,7 @@ __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
     for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
         int s = D.bslot[i];
         if (!s) continue;
-        warm_body(D, K, s, lane, D.sv[s], v3f4(D.sw[s]));
+        warm_body(D, K, s, lane, SV(D, s), v3f4(SW(D, s)));
     }
 }
 
@@ -1151,15 +1161,15 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
     uint2 e2 = K.use_bias ? make_uint2(~0u, ~0u) : reinterpret_cast<const uint2*>(D.blinv)[m];
     int4 hd = D.cur.hdrp[m];
     int sa = hd.z, sb = hd.w;
-    float4 va4 = D.sv[sa], vb4 = D.sv[sb];
-    V3 va = v3f4(va4), wa = v3f4(D.sw[sa]), vb = v3f4(vb4), wb = v3f4(D.sw[sb]);
-    V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
+    float4 va4 = SV(D, sa), vb4 = SV(D, sb);
+    V3 va = v3f4(va4), wa = v3f4(SW(D, sa)), vb = v3f4(vb4), wb = v3f4(SW(D, sb));
+    V3 dpab = vsub(v3f4(SDP(D, sb)), v3f4(SDP(D, sa)));
     // the relax pass writes the next warm start into the bodies' list entries
     uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
-    const M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
-    solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, 0.0f, 0.0f, ea, eb, iia, iib);
-    if (sa) { D.sv[sa] = f4(va, va4.w); D.sw[sa] = f4(wa, 0.0f); }
-    if (sb) { D.sv[sb] = f4(vb, vb4.w); D.sw[sb] = f4(wb, 0.0f); }
+    const M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
+    solve_core(D, m, K, va, wa, vb, wb, v3f4(STH(D, sa)), v3f4(STH(D, sb)), dpab, 0.0f, 0.0f, ea, eb, iia, iib);
+    if (sa) { SV(D, sa) = f4(va, va4.w); SW(D, sa) = f4(wa, 0.0f); }
+    if (sb) { SV(D, sb) = f4(vb, vb4.w); SW(D, sb) = f4(wb, 0.0f); }
 }
 
 // one colour bucket: its manifolds share no moving body, so every lane solves one
@@ -1205,13 +1215,13 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
         uint2 e2 = reinterpret_cast<const uint2*>(D.blinv)[m];
         int4 hd = D.cur.hdrp[m];
         int sa = hd.z, sb = hd.w;
-        V3 va0 = v3f4(D.sv[sa]), wa0 = v3f4(D.sw[sa]), vb0 = v3f4(D.sv[sb]), wb0 = v3f4(D.sw[sb]);
+        V3 va0 = v3f4(SV(D, sa)), wa0 = v3f4(SW(D, sa)), vb0 = v3f4(SV(D, sb)), wb0 = v3f4(SW(D, sb));
         V3 va = va0, wa = wa0, vb = vb0, wb = wb0;
-        V3 dpab = vsub(v3f4(D.sdp[sb]), v3f4(D.sdp[sa]));
+        V3 dpab = vsub(v3f4(SDP(D, sb)), v3f4(SDP(D, sa)));
         float fa = sa ? (float)D.ovcnt[sa] : 1.0f, fb = sb ? (float)D.ovcnt[sb] : 1.0f;
         uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
-        const M3 iia = load_m3(D.sii + 3 * (size_t)sa), iib = load_m3(D.sii + 3 * (size_t)sb);
-        solve_core(D, m, K, va, wa, vb, wb, v3f4(D.sth[sa]), v3f4(D.sth[sb]), dpab, fa, fb,
+        const M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
+        solve_core(D, m, K, va, wa, vb, wb, v3f4(STH(D, sa)), v3f4(STH(D, sb)), dpab, fa, fb,
                    K.use_bias ? ~0u : ea, K.use_bias ? ~0u : eb, iia, iib);
         // each side's change into its body's list entry
         if (ea != ~0u) { D.ovl[2 * (size_t)ea] = f4(vsub(va, va0), 0.0f); D.ovl[2 * (size_t)ea + 1] = f4(vsub(wa, wa0), 0.0f); }
@@ -1248,17 +1258,17 @@ __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
         if (!s) continue;
         uint32_t cnt = D.ovcnt[s];
         if (!cnt) continue;
-        float4 v4 = D.sv[s];
-        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
+        float4 v4 = SV(D, s);
+        V3 v = v3f4(v4), w = v3f4(SW(D, s));
         ovf_average(D, s, lane, cnt, v, w);
-        if (lane == 0) { D.sv[s] = f4(v, v4.w); D.sw[s] = f4(w, 0.0f); }
+        if (lane == 0) { SV(D, s) = f4(v, v4.w); SW(D, s) = f4(w, 0.0f); }
     }
 }
 // the bias pass's end: the overflow average, then the position integration (k_integrate's update), per body
 __device__ __forceinline__ void integrate_slot(const Dev& D, const StepK& K, int s, V3 v, V3 w) {
-    D.sdp[s] = f4(vadd(v3f4(D.sdp[s]), vmul(v, K.h)), 0.0f);
-    D.sth[s] = f4(vadd(v3f4(D.sth[s]), vmul(w, K.h)), 0.0f);
-    D.sq[s] = f4q(qintegrate(qf4(D.sq[s]), w, K.h));
+    SDP(D, s) = f4(vadd(v3f4(SDP(D, s)), vmul(v, K.h)), 0.0f);
+    STH(D, s) = f4(vadd(v3f4(STH(D, s)), vmul(w, K.h)), 0.0f);
+    SQ(D, s) = f4q(qintegrate(qf4(SQ(D, s)), w, K.h));
 }
 __global__ void __launch_bounds__(NT) k_ovf_apply_integrate(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
@@ -1267,11 +1277,11 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_integrate(Dev D, StepK K) {
         int s = D.bslot[i];
         if (!s) continue;
         uint32_t cnt = D.ovcnt[s];
-        float4 v4 = D.sv[s];
-        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
+        float4 v4 = SV(D, s);
+        V3 v = v3f4(v4), w = v3f4(SW(D, s));
         if (cnt) {
             ovf_average(D, s, lane, cnt, v, w);
-            if (lane == 0) { D.sv[s] = f4(v, v4.w); D.sw[s] = f4(w, 0.0f); }
+            if (lane == 0) { SV(D, s) = f4(v, v4.w); SW(D, s) = f4(w, 0.0f); }
         }
         if (lane == 0) integrate_slot(D, K, s, v, w);
     }
@@ -1287,8 +1297,8 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
         int s = D.bslot[i];
         if (!s) continue;
         const uint32_t cnt = D.ovcnt[s], o = D.bloff[s], n = D.blcnt[s];
-        const float4 v4 = D.sv[s];
-        V3 v = v3f4(v4), w = v3f4(D.sw[s]);
+        const float4 v4 = SV(D, s);
+        V3 v = v3f4(v4), w = v3f4(SW(D, s));
         V3 ov = v3(0, 0, 0), ow = v3(0, 0, 0), av = v3(0, 0, 0), aw = v3(0, 0, 0);
         for (uint32_t e0 = 0; e0 < n; e0 += 64) {
             uint32_t e = e0 + lane;
@@ -1325,8 +1335,8 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
         float wl = vlen2(w);
         if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
         if (lane == 0) {
-            D.sv[s] = f4(vadd(vw, av), v4.w);
-            D.sw[s] = f4(vadd(w, aw), 0.0f);
+            SV(D, s) = f4(vadd(vw, av), v4.w);
+            SW(D, s) = f4(vadd(w, aw), 0.0f);
         }
     }
 }
@@ -1344,7 +1354,7 @@ __global__ void __launch_bounds__(NT) k_integrate(Dev D, StepK K) {
     GRID_STRIDE(i, (uint32_t)D.nb) {
         int s = D.bslot[i];
         if (!s) continue;
-        integrate_slot(D, K, s, v3f4(D.sv[s]), v3f4(D.sw[s]));
+        integrate_slot(D, K, s, v3f4(SV(D, s)), v3f4(SW(D, s)));
     }
 }
 
@@ -1354,11 +1364,11 @@ __global__ void __launch_bounds__(NT) k_writeback(Dev D) {
         int s = D.bslot[i];
         if (!s) continue;
         float4 p = D.bp[i];
-        V3 np = vadd(v3f4(D.sp0[s]), v3f4(D.sdp[s]));
+        V3 np = vadd(v3f4(D.sp0[s]), v3f4(SDP(D, s)));
         D.bp[i] = f4(np, p.w);
-        D.bv[i] = f4(v3f4(D.sv[s]), 0.0f);
-        D.bw[i] = f4(v3f4(D.sw[s]), 0.0f);
-        D.bq[i] = D.sq[s];
+        D.bv[i] = f4(v3f4(SV(D, s)), 0.0f);
+        D.bw[i] = f4(v3f4(SW(D, s)), 0.0f);
+        D.bq[i] = SQ(D, s);
     }
 }
 
@@ -1592,8 +1602,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.bp, nb); r |= w->alloc(D.bq, nb); r |= w->alloc(D.bv, nb); r |= w->alloc(D.bw, nb);
     r |= w->alloc(D.bii, nb); r |= w->alloc(D.bsp, nb); r |= w->alloc(D.bsq, nb); r |= w->alloc(D.brot, 3 * (size_t)nb);
     r |= w->alloc(D.bflag, nb); r |= w->alloc(D.bslot, nb); r |= w->alloc(D.wake, nb);
-    r |= w->alloc(D.sv, n1); r |= w->alloc(D.sw, n1); r |= w->alloc(D.sdp, n1); r |= w->alloc(D.sth, n1);
-    r |= w->alloc(D.sq, n1); r |= w->alloc(D.sp0, n1); r |= w->alloc(D.sii, 3 * n1);
+    r |= w->alloc(D.sbod, SLOT_F4 * n1); r |= w->alloc(D.sp0, n1);
     r |= w->alloc(D.shb, ns); r |= w->alloc(D.slp, ns); r |= w->alloc(D.slr, 3 * (size_t)ns); r |= w->alloc(D.shh, ns);
     r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
     r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
