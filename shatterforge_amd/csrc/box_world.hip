@@ -88,14 +88,14 @@ struct Dev {
     float4* nar;  // 5 per candidate: n|count, points p|sep
     uint32_t *hit, *hoff;
     ManBuf cur, old;
-    float4* wrec;  // warm-start records by body contact-list entry: 8 float4 each (6 used)
+    float4* wrec;  // by body contact-list entry, WREC_F4 float4: the warm start's velocity and spin change, then the
+                   // overflow velocity change v, w (zero outside the overflow)
     // colouring
     int32_t *mcol, *mcand;
     unsigned long long *mpri, *bmask, *claim;
     uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff, *ovcnt;
     uint32_t* chunkc;  // per colour, per chunk of NT manifolds: count, then its exclusive offset within the colour
     uint32_t* ptpart;  // contact points per workgroup of the build (summed by k_sum_points)
-    float4* ovl;   // overflow velocity changes by body contact-list entry: v, w (zero outside the overflow)
     // per-slot manifold lists (m * 2 + side), in manifold order
     uint32_t *blcnt, *bloff, *blfill, *blist;
     uint32_t *blinv, *minv;  // the list entry of (colour position, side), of (manifold, side); ~0u: the static world
@@ -112,6 +112,9 @@ __device__ __forceinline__ float4& SDP(const Dev& D, int s) { return D.sbod[(siz
 __device__ __forceinline__ float4& STH(const Dev& D, int s) { return D.sbod[(size_t)s * SLOT_F4 + 3]; }
 __device__ __forceinline__ float4& SQ(const Dev& D, int s) { return D.sbod[(size_t)s * SLOT_F4 + 4]; }
 __device__ __forceinline__ float4* SII(const Dev& D, int s) { return D.sbod + (size_t)s * SLOT_F4 + 5; }
+constexpr int WREC_F4 = 4;
+__device__ __forceinline__ float4* WREC(const Dev& D, uint32_t e) { return D.wrec + (size_t)e * WREC_F4; }
+__device__ __forceinline__ float4* OVL(const Dev& D, uint32_t e) { return D.wrec + (size_t)e * WREC_F4 + 2; }
 #define GRID_STRIDE(i, n) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 __device__ __forceinline__ M3 load_m3(const float4* p) { M3 r; r.c[0] = v3f4(p[0]); r.c[1] = v3f4(p[1]); r.c[2] = v3f4(p[2]); return r; }
@@ -583,14 +586,16 @@ __device__ __forceinline__ void write_warm_side(float4* w, int side, int cnt, fl
     }
     V3 pp = vadd(vadd(vmul(v3f4(n4), pn), vmul(v3f4(t14), j1)), vmul(v3f4(t24), j2));
     V3 tf = vadd(vadd(vmul(i1, j1), vmul(i2, j2)), vmul(in, jt));
-    float c = (float)cnt;
-    if (side == 0) {  // side a subtracts: stored negated
-        w[0] = f4(vneg(t[0]), c); w[1] = f4(vneg(t[1]), 0.0f); w[2] = f4(vneg(t[2]), 0.0f); w[3] = f4(vneg(t[3]), 0.0f);
-        w[4] = f4(vneg(vmul(pp, im)), 0.0f); w[5] = f4(vneg(tf), 0.0f);
-    } else {
-        w[0] = f4(t[0], c); w[1] = f4(t[1], 0.0f); w[2] = f4(t[2], 0.0f); w[3] = f4(t[3], 0.0f);
-        w[4] = f4(vmul(pp, im), 0.0f); w[5] = f4(tf, 0.0f);
-    }
+    // the body's spin change is the point terms then tf, added in point order; side a subtracts (its terms negated,
+    // which is exact)
+    auto sg = [&](V3 x) { return side ? x : vneg(x); };
+    V3 dw = sg(t[0]);
+    if (cnt > 1) dw = vadd(dw, sg(t[1]));
+    if (cnt > 2) dw = vadd(dw, sg(t[2]));
+    if (cnt > 3) dw = vadd(dw, sg(t[3]));
+    dw = vadd(dw, sg(tf));
+    w[0] = f4(sg(vmul(pp, im)), 0.0f);
+    w[1] = f4(dw, 0.0f);
 }
 // both sides of the manifold (ea, eb: the list entries of its two bodies, ~0u for the static world); iaxn, ibxn, ia*,
 // ib*: the inertia terms of side a and side b (unsplit)
@@ -598,8 +603,8 @@ __device__ __forceinline__ void write_warm(const Dev& D, uint32_t ea, uint32_t e
                                            float4 t24, float j1, float j2, float jt, const float* jn, const V3* iaxn,
                                            const V3* ibxn, V3 ia1, V3 ia2, V3 ian, V3 ib1, V3 ib2, V3 ibn, float ima,
                                            float imb) {
-    if (ea != ~0u) write_warm_side(D.wrec + (size_t)ea * 8, 0, cnt, n4, t14, t24, j1, j2, jt, jn, iaxn, ia1, ia2, ian, ima);
-    if (eb != ~0u) write_warm_side(D.wrec + (size_t)eb * 8, 1, cnt, n4, t14, t24, j1, j2, jt, jn, ibxn, ib1, ib2, ibn, imb);
+    if (ea != ~0u) write_warm_side(WREC(D, ea), 0, cnt, n4, t14, t24, j1, j2, jt, jn, iaxn, ia1, ia2, ian, ima);
+    if (eb != ~0u) write_warm_side(WREC(D, eb), 1, cnt, n4, t14, t24, j1, j2, jt, jn, ibxn, ib1, ib2, ibn, imb);
 }
 
 // push_manifold (phys.rs:450-498) and the constraint masses (:190-211), one manifold per lane, written straight into
@@ -727,8 +732,8 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         write_warm(D, ea, eb, cnt, f4(n, mu), f4(t1, tm1), f4(t2, tm2), jt1, jt2, jtw, jnk, iaxk, ibxk, ia1, ia2, ian, ib1,
                    ib2, ibn, ima, imb);
         const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (ea != ~0u) { D.ovl[2 * (size_t)ea] = z; D.ovl[2 * (size_t)ea + 1] = z; }
-        if (eb != ~0u) { D.ovl[2 * (size_t)eb] = z; D.ovl[2 * (size_t)eb + 1] = z; }
+        if (ea != ~0u) { OVL(D, ea)[0] = z; OVL(D, ea)[1] = z; }
+        if (eb != ~0u) { OVL(D, eb)[0] = z; OVL(D, eb)[1] = z; }
     }
     for (int h = 32; h >= 1; h >>= 1) npts += __shfl_xor(npts, h, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = npts;
@@ -1017,15 +1022,9 @@ __device__ __forceinline__ void warm_body(const Dev& D, const StepK& K, int s, i
         uint32_t e = e0 + lane;
         V3 dv = v3(0, 0, 0), dw = v3(0, 0, 0);
         if (e < n) {
-            const float4* wr = D.wrec + (size_t)(o + e) * 8;
-            float4 w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4], w5 = wr[5];
-            int c = (int)w0.w;
-            dw = v3f4(w0);
-            if (c > 1) dw = vadd(dw, v3f4(w1));
-            if (c > 2) dw = vadd(dw, v3f4(w2));
-            if (c > 3) dw = vadd(dw, v3f4(w3));
-            dw = vadd(dw, v3f4(w5));
-            dv = v3f4(w4);
+            const float4* wr = WREC(D, o + e);
+            dv = v3f4(wr[0]);
+            dw = v3f4(wr[1]);
         }
         av = vadd(av, wave_tree(dv));
         aw = vadd(aw, wave_tree(dw));
@@ -1224,8 +1223,8 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
         solve_core(D, m, K, va, wa, vb, wb, v3f4(STH(D, sa)), v3f4(STH(D, sb)), dpab, fa, fb,
                    K.use_bias ? ~0u : ea, K.use_bias ? ~0u : eb, iia, iib);
         // each side's change into its body's list entry
-        if (ea != ~0u) { D.ovl[2 * (size_t)ea] = f4(vsub(va, va0), 0.0f); D.ovl[2 * (size_t)ea + 1] = f4(vsub(wa, wa0), 0.0f); }
-        if (eb != ~0u) { D.ovl[2 * (size_t)eb] = f4(vsub(vb, vb0), 0.0f); D.ovl[2 * (size_t)eb + 1] = f4(vsub(wb, wb0), 0.0f); }
+        if (ea != ~0u) { OVL(D, ea)[0] = f4(vsub(va, va0), 0.0f); OVL(D, ea)[1] = f4(vsub(wa, wa0), 0.0f); }
+        if (eb != ~0u) { OVL(D, eb)[0] = f4(vsub(vb, vb0), 0.0f); OVL(D, eb)[1] = f4(vsub(wb, wb0), 0.0f); }
     }
 }
 // each body takes the average of its copies: the tree sum of the changes over its contacts (zero for contacts not
@@ -1239,7 +1238,7 @@ __device__ __forceinline__ void ovf_average(const Dev& D, int s, int lane, uint3
         uint32_t e = e0 + lane;
         V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0);
         if (e < n) {
-            const float4* dv = D.ovl + 2 * (size_t)(o + e);
+            const float4* dv = OVL(D, o + e);
             dvv = v3f4(dv[0]);
             dww = v3f4(dv[1]);
         }
@@ -1304,19 +1303,13 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
             uint32_t e = e0 + lane;
             V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0), dv = v3(0, 0, 0), dw = v3(0, 0, 0);
             if (e < n) {
-                const float4* ol = D.ovl + 2 * (size_t)(o + e);
-                const float4* wr = D.wrec + (size_t)(o + e) * 8;
-                float4 o0 = cnt ? ol[0] : make_float4(0, 0, 0, 0), o1 = cnt ? ol[1] : make_float4(0, 0, 0, 0);
-                float4 w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4], w5 = wr[5];
+                const float4* wr = WREC(D, o + e);
+                float4 w0 = wr[0], w1 = wr[1];
+                float4 o0 = cnt ? wr[2] : make_float4(0, 0, 0, 0), o1 = cnt ? wr[3] : make_float4(0, 0, 0, 0);
                 dvv = v3f4(o0);
                 dww = v3f4(o1);
-                int c = (int)w0.w;
-                dw = v3f4(w0);
-                if (c > 1) dw = vadd(dw, v3f4(w1));
-                if (c > 2) dw = vadd(dw, v3f4(w2));
-                if (c > 3) dw = vadd(dw, v3f4(w3));
-                dw = vadd(dw, v3f4(w5));
-                dv = v3f4(w4);
+                dv = v3f4(w0);
+                dw = v3f4(w1);
             }
             if (cnt) {
                 ov = vadd(ov, wave_tree(dvv));
@@ -1614,14 +1607,14 @@ int world_alloc(HostWorld* w) {
         r |= w->alloc(M->hdr, D.mcap); r |= w->alloc(M->row, (size_t)MR_ROWS * D.mcap); r |= w->alloc(M->key, D.mcap);
         r |= w->alloc(M->pos, D.mcap); r |= w->alloc(M->hdrp, D.mcap);
     }
-    r |= w->alloc(D.wrec, (size_t)16 * D.mcap);  // two list entries per manifold
+    r |= w->alloc(D.wrec, (size_t)2 * WREC_F4 * D.mcap);  // two list entries per manifold
     r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
     r |= w->alloc(D.chunkc, 72 * (size_t)((D.mcap + NT - 1) / NT + 1));
     r |= w->alloc(D.ptpart, 4096);  // k_build's grid: sf_grid(ccap) is at most 4096 workgroups
     r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap);
-    r |= w->alloc(D.ovcnt, n1); r |= w->alloc(D.ovl, 4 * (size_t)D.mcap);
+    r |= w->alloc(D.ovcnt, n1);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blinv, 2 * (size_t)D.mcap); r |= w->alloc(D.minv, 2 * (size_t)D.mcap);
     r |= w->alloc(D.parent, nb); r |= w->alloc(D.sst, nb); r |= w->alloc(D.ctr, C_NCTR);
     uint32_t big = std::max<uint32_t>(std::max<uint32_t>(D.tcap, D.ccap), (uint32_t)n1);
