@@ -21,7 +21,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_solve_bucket launch on this workload, from FETCH_SIZE / WRITE_SIZE counter runs calibrated by a copy
 # of known size (profiles/k_solve_bucket_rubble50k_traffic.md)
-SOLVE_TRAFFIC_BYTES = 24_388_096  # HBM bytes per k_solve_bucket dispatch, profiles/k_solve_bucket_rubble50k_traffic.md
+SOLVE_TRAFFIC_BYTES = 13_240_832  # HBM bytes per k_solve_bucket dispatch, profiles/k_solve_bucket_rubble50k_traffic.md
 # algorithmic bytes per manifold solve in one colour pass (box path): the hot contact record read, impulses written,
 # both bodies' v and w read and written
 SOLVE_BYTES_PER_MANIFOLD = 392
