@@ -259,21 +259,36 @@ __device__ __forceinline__ uint32_t cell_partners(const Dev& D, const StepK& K, 
     int ba = __float_as_int(lo4.w), fa = __float_as_int(hi4.w);
     uint32_t key = cell_key(x, y, z), sl = slot_of(key, D.tmask);
     uint32_t s0 = D.tstart[sl], s1 = s0 + D.tcnt[sl], n = 0;
-    for (uint32_t q = s0; q < s1; q++) {
-        if (D.tkey[q] != key) continue;
-        int b = D.tent[q];
-        if (b <= a) continue;
-        float4 lob4 = D.slo[b], hib4 = D.shi[b];
-        int bb = __float_as_int(lob4.w), fb = __float_as_int(hib4.w);
-        if (bb == ba) continue;
-        if (!((fa | fb) & 1)) continue;          // both resting
-        if ((fa & fb & 2)) continue;             // both static
-        V3 lob = v3f4(lob4), hib = v3f4(hib4);
-        if (lo.x > hib.x || lob.x > hi.x || lo.y > hib.y || lob.y > hi.y || lo.z > hib.z || lob.z > hi.z) continue;
-        V3 m = vmaxv(lo, lob);
-        if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
-        emit(n, b);
-        n++;
+    // four table entries at a time: their keys and shapes, then the boxes of all four (from a valid index whatever
+    // they hold, so the loads are not held behind the tests), then the tests and the emits in table order
+    constexpr int U = 4;
+    for (uint32_t q0 = s0; q0 < s1; q0 += U) {
+        uint32_t kq[U];
+        int bq[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool in = q0 + u < s1;
+            kq[u] = in ? D.tkey[q0 + u] : ~key;
+            bq[u] = in ? D.tent[q0 + u] : a;
+        }
+        float4 lq[U], hq[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) { lq[u] = D.slo[bq[u]]; hq[u] = D.shi[bq[u]]; }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int b = bq[u];
+            if (kq[u] != key || b <= a) continue;
+            int bb = __float_as_int(lq[u].w), fb = __float_as_int(hq[u].w);
+            if (bb == ba) continue;
+            if (!((fa | fb) & 1)) continue;          // both resting
+            if ((fa & fb & 2)) continue;             // both static
+            V3 lob = v3f4(lq[u]), hib = v3f4(hq[u]);
+            if (lo.x > hib.x || lob.x > hi.x || lo.y > hib.y || lob.y > hi.y || lo.z > hib.z || lob.z > hi.z) continue;
+            V3 m = vmaxv(lo, lob);
+            if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
+            emit(n, b);
+            n++;
+        }
     }
     return n;
 }
