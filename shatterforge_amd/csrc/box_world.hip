@@ -34,7 +34,7 @@ constexpr int NT = 256;  // threads per block of the grid-stride kernels
 
 // device counters
 enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_ACTIVE,
-       C_ROUNDS, C_NOVF2, C_NCTR = 16 };
+       C_ROUNDS, C_NOVF2, C_NSLOT, C_NCTR = 16 };
 
 // manifold rows (each row is a float4 array of mcap entries)
 // (the inertia terms, a body's world inverse inertia times an arm or the normal, are not stored: the solve forms them
@@ -81,6 +81,8 @@ struct Dev {
     int32_t* eshape;
     uint32_t *tcnt, *tstart, *tfill, *tkey;
     int32_t* tent;    // shape of each table entry
+    uint32_t* tidx;   // cell entry of each table entry
+    uint32_t *soff, *slist;  // occupied table slots: offset in the list (scan of the flags), the list
     uint32_t *epc, *erel;  // per cell entry: partners homed there, offset within the shape's candidate list
     int32_t* ekeep;        // per cell entry: its first ENTRY_KEEP partners, as the count pass found them
     uint32_t *ccnt, *coff;
@@ -115,6 +117,7 @@ __device__ __forceinline__ float4* SII(const Dev& D, int s) { return D.sbod + (s
 constexpr int WREC_F4 = 4;
 __device__ __forceinline__ float4* WREC(const Dev& D, uint32_t e) { return D.wrec + (size_t)e * WREC_F4; }
 __device__ __forceinline__ float4* OVL(const Dev& D, uint32_t e) { return D.wrec + (size_t)e * WREC_F4 + 2; }
+__device__ __forceinline__ float rdl(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
 #define GRID_STRIDE(i, n) for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 __device__ __forceinline__ M3 load_m3(const float4* p) { M3 r; r.c[0] = v3f4(p[0]); r.c[1] = v3f4(p[1]); r.c[2] = v3f4(p[2]); return r; }
@@ -235,8 +238,12 @@ __global__ void __launch_bounds__(NT) k_fill_table(Dev D) {
         uint32_t pos = D.tstart[sl] + atomicAdd(&D.tfill[sl], 1u);
         D.tkey[pos] = key;
         D.tent[pos] = D.eshape[e];
+        D.tidx[pos] = e;
     }
 }
+// the table's occupied slots in slot order (flags into tfill, free once the table is filled, then scanned)
+__global__ void __launch_bounds__(NT) k_slot_flag(Dev D) { GRID_STRIDE(sl, D.tcap) D.tfill[sl] = D.tcnt[sl] ? 1u : 0u; }
+__global__ void __launch_bounds__(NT) k_slot_list(Dev D) { GRID_STRIDE(sl, D.tcap) if (D.tcnt[sl]) D.slist[D.soff[sl]] = sl; }
 
 // Candidates by cell entry: one thread per (shape, cell) entry finds the shape's partners whose home is that cell
 // (count, then write), one thread per shape gathers its entries' counts and puts its ground candidate first, then
@@ -305,6 +312,60 @@ __global__ void __launch_bounds__(NT) k_cand_count(Dev D, StepK K) {
         D.epc[e] = cell_partners(D, K, (int)a, x, y, z, [&](uint32_t j, int b) { if (j < ENTRY_KEEP) keep[j] = b; });
     }
 }
+// The count pass by table slot, one wavefront each: the slot's entries (a cell's, and any cell hashed to the same
+// slot) go to the lanes once, and each entry in turn is broadcast to all lanes, every lane testing its own pair; a lane
+// keeps its entry's partners in slot order, as cell_partners finds them. Slots of more than 64 entries take the
+// per-entry walk.
+__global__ void __launch_bounds__(NT) k_cand_slot(Dev D, StepK K) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    const uint32_t nslot = D.ctr[C_NSLOT];
+    for (uint32_t w = wave; w < nslot; w += nwaves) {
+        const uint32_t sl = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.slist[w]);  // (the wave's own slot)
+        const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.tstart[sl]);
+        const uint32_t n = (uint32_t)__builtin_amdgcn_readfirstlane((int)D.tcnt[sl]);
+        if (n > 64) {
+            for (uint32_t q = s0 + lane; q < s0 + n; q += 64) {
+                uint32_t e = D.tidx[q];
+                uint32_t a = (uint32_t)D.eshape[e];
+                int x, y, z;
+                entry_cell(D, K, a, e, x, y, z);
+                int32_t* keep = D.ekeep + (size_t)e * ENTRY_KEEP;
+                D.epc[e] = cell_partners(D, K, (int)a, x, y, z, [&](uint32_t j, int b) { if (j < ENTRY_KEEP) keep[j] = b; });
+            }
+            continue;
+        }
+        const bool mine = (uint32_t)lane < n;
+        const uint32_t q = s0 + (mine ? (uint32_t)lane : 0u);
+        const uint32_t key = D.tkey[q], e = D.tidx[q];
+        const int a = D.tent[q];
+        const float4 lo4 = D.slo[a], hi4 = D.shi[a];
+        const V3 lo = v3f4(lo4), hi = v3f4(hi4);
+        const int ba = __float_as_int(lo4.w), fa = __float_as_int(hi4.w);
+        int x, y, z;
+        entry_cell(D, K, (uint32_t)a, e, x, y, z);
+        int32_t* keep = D.ekeep + (size_t)e * ENTRY_KEEP;
+        uint32_t np = 0;
+        for (uint32_t j = 0; j < n; j++) {
+            const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)j);
+            const int b = __builtin_amdgcn_readlane(a, (int)j);
+            const float lbx = rdl(lo.x, (int)j), lby = rdl(lo.y, (int)j), lbz = rdl(lo.z, (int)j);
+            const float hbx = rdl(hi.x, (int)j), hby = rdl(hi.y, (int)j), hbz = rdl(hi.z, (int)j);
+            const int bb = __builtin_amdgcn_readlane(ba, (int)j), fb = __builtin_amdgcn_readlane(fa, (int)j);
+            if (!mine || kj != key || b <= a) continue;
+            if (bb == ba) continue;
+            if (!((fa | fb) & 1)) continue;          // both resting
+            if ((fa & fb & 2)) continue;             // both static
+            if (lo.x > hbx || lbx > hi.x || lo.y > hby || lby > hi.y || lo.z > hbz || lbz > hi.z) continue;
+            V3 m = vmaxv(lo, v3(lbx, lby, lbz));
+            if ((int32_t)sf_floor(m.x * K.inv_cell) != x || (int32_t)sf_floor(m.y * K.inv_cell) != y || (int32_t)sf_floor(m.z * K.inv_cell) != z) continue;
+            if (np < ENTRY_KEEP) keep[np] = b;
+            np++;
+        }
+        if (mine) D.epc[e] = np;
+    }
+}
+
 // per shape: its candidate count (ground first) and each entry's offset within the shape's list
 __global__ void __launch_bounds__(NT) k_cand_shape(Dev D) {
     uint32_t ng = 0, np = 0;
@@ -1007,7 +1068,6 @@ __global__ void __launch_bounds__(NT) k_bl_inv(Dev D) {
 // ---------------------------------------------------------------- solver
 // gravity, spin clamp, then this body's warm start gathered over its contacts in manifold order: the same
 // sequence of f32 updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262)
-__device__ __forceinline__ float rdl(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
 __device__ __forceinline__ V3 rdl3(V3 a, int l) { return v3(rdl(a.x, l), rdl(a.y, l), rdl(a.z, l)); }
 
 // Gravity, spin clamp, then this body's warm start over its contacts in manifold order: the same sequence of f32
@@ -1615,6 +1675,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
     r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
     r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
+    r |= w->alloc(D.tidx, D.ecap); r |= w->alloc(D.soff, D.tcap); r |= w->alloc(D.slist, D.ecap);
     r |= w->alloc(D.epc, D.ecap); r |= w->alloc(D.erel, D.ecap); r |= w->alloc(D.ekeep, (size_t)ENTRY_KEEP * D.ecap);
     r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
     r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
@@ -1694,7 +1755,10 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_emit_entries, dim3(gs), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.tcnt, D.tstart, nullptr, nullptr, D.tcap, D.tcap, w->scan_tmp, s);
     hipLaunchKernelGGL(k_fill_table, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_cand_count, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_slot_flag, dim3(sf_grid(D.tcap)), dim3(NT), 0, s, D);
+    sf::scan_u32(D.tfill, D.soff, D.ctr + C_NSLOT, nullptr, D.tcap, D.tcap, w->scan_tmp, s);
+    hipLaunchKernelGGL(k_slot_list, dim3(sf_grid(D.tcap)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_cand_slot, dim3(sf_grid((uint64_t)D.ecap * 64, NT, 65536)), dim3(NT), 0, s, D, K);
     hipLaunchKernelGGL(k_cand_shape, dim3(gs), dim3(NT), 0, s, D);
     sf::scan_u32(D.ccnt, D.coff, D.ctr + C_NCAND, nullptr, (uint32_t)D.ns, (uint32_t)D.ns, w->scan_tmp, s);
     hipLaunchKernelGGL(k_cand_write, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D, K);
