@@ -34,7 +34,7 @@ constexpr int NT = 256;  // threads per block of the grid-stride kernels
 
 // device counters
 enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_ACTIVE,
-       C_ROUNDS, C_NOVF2, C_NSLOT, C_NCTR = 16 };
+       C_ROUNDS, C_NOVF2, C_NSLOT, C_NBIGC, C_NCTR = 16 };
 
 // manifold rows (each row is a float4 array of mcap entries)
 // (the inertia terms, a body's world inverse inertia times an arm or the normal, are not stored: the solve forms them
@@ -85,6 +85,7 @@ struct Dev {
     uint32_t *soff, *slist;  // occupied table slots: offset in the list (scan of the flags), the list
     uint32_t *epc, *erel;  // per cell entry: partners homed there, offset within the shape's candidate list
     int32_t* ekeep;        // per cell entry: its first ENTRY_KEEP partners, as the count pass found them
+    uint32_t* bigc;        // shapes with more than 16 partners this step
     uint32_t *ccnt, *coff;
     int2* cand;
     float4* nar;  // 5 per candidate: n|count, points p|sep
@@ -405,35 +406,74 @@ __global__ void __launch_bounds__(NT) k_cand_write(Dev D, StepK K) {
         }
     }
 }
-// per shape (one wavefront each): the ground candidate first, the partners ascending after it
+// per shape: the ground candidate first, the partners ascending after it. Up to 16 partners: one thread, a bitonic
+// network in registers; more: the shape goes to a list sorted by k_cand_finish_big, a wavefront per shape.
+__device__ __forceinline__ void cas_i(int& x, int& y) { int lo = min(x, y), hi = max(x, y); x = lo; y = hi; }
 __global__ void __launch_bounds__(NT) k_cand_finish(Dev D) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    for (uint32_t a = wave; a < (uint32_t)D.ns; a += nwaves) {
+    GRID_STRIDE(a, (uint32_t)D.ns) {
         uint32_t o = D.coff[a], n = D.ccnt[a];
         uint32_t g = (uint32_t)ground_cand(D, a);
         if (!n) continue;
-        if (o + n > D.ccap) { if (g && lane == 0) atomicAdd(&D.ctr[C_DROP], 1u); continue; }
-        if (g && lane == 0) D.cand[o] = make_int2((int)a, -1);
+        if (o + n > D.ccap) { if (g) atomicAdd(&D.ctr[C_DROP], 1u); continue; }
+        if (g) D.cand[o] = make_int2((int)a, -1);
         o += g; n -= g;
         if (n < 2) continue;
-        if (n <= 64) {
-            int v = (uint32_t)lane < n ? D.cand[o + lane].y : 0x7FFFFFFF;
-            for (int k = 2; k <= 64; k <<= 1)
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    int u = __shfl_xor(v, j, 64);
-                    bool up = (lane & k) == 0, lower = (lane & j) == 0;
-                    v = (lower == up) ? min(v, u) : max(v, u);
+        if (n > 16) { D.bigc[atomicAdd(&D.ctr[C_NBIGC], 1u)] = a; continue; }
+        int v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = (uint32_t)k < n ? D.cand[o + k].y : 0x7FFFFFFF;
+#pragma unroll
+        for (int k = 2; k <= 16; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    int l = i ^ j;
+                    if (l > i) { if ((i & k) == 0) cas_i(v[i], v[l]); else cas_i(v[l], v[i]); }
                 }
-            if ((uint32_t)lane < n) D.cand[o + lane].y = v;
-        } else if (lane == 0) {
-            for (uint32_t i = 1; i < n; i++) {
-                int2 v = D.cand[o + i];
-                uint32_t j = i;
-                while (j > 0 && D.cand[o + j - 1].y > v.y) { D.cand[o + j] = D.cand[o + j - 1]; j--; }
-                D.cand[o + j] = v;
+#pragma unroll
+        for (int k = 0; k < 16; k++) if ((uint32_t)k < n) D.cand[o + k].y = v[k];
+    }
+}
+// the shapes with more than 16 partners: one workgroup each, a bitonic sort in LDS (lists past 4,096 entries, which
+// no sane scene has, by insertion)
+__global__ void __launch_bounds__(NT) k_cand_finish_big(Dev D) {
+    __shared__ int sh[4096];
+    const uint32_t nbig = min(D.ctr[C_NBIGC], (uint32_t)D.ns);
+    for (uint32_t t = blockIdx.x; t < nbig; t += gridDim.x) {
+        const uint32_t a = D.bigc[t];
+        const uint32_t g = (uint32_t)ground_cand(D, a);
+        const uint32_t o = D.coff[a] + g, n = D.ccnt[a] - g;
+        if (n > 4096) {
+            if (threadIdx.x == 0) {
+                for (uint32_t i = 1; i < n; i++) {
+                    int2 v = D.cand[o + i];
+                    uint32_t j = i;
+                    while (j > 0 && D.cand[o + j - 1].y > v.y) { D.cand[o + j] = D.cand[o + j - 1]; j--; }
+                    D.cand[o + j] = v;
+                }
+            }
+            continue;
+        }
+        uint32_t n2 = 32;
+        while (n2 < n) n2 <<= 1;
+        for (uint32_t i = threadIdx.x; i < n2; i += NT) sh[i] = i < n ? D.cand[o + i].y : 0x7FFFFFFF;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < n2; i += NT) {
+                    uint32_t l = i ^ j;
+                    if (l > i) {
+                        int x = sh[i], y = sh[l];
+                        bool up = (i & k) == 0;
+                        if ((x > y) == up) { sh[i] = y; sh[l] = x; }
+                    }
+                }
+                __syncthreads();
             }
         }
+        for (uint32_t i = threadIdx.x; i < n; i += NT) D.cand[o + i].y = sh[i];
+        __syncthreads();
     }
 }
 
@@ -1677,6 +1717,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
     r |= w->alloc(D.tidx, D.ecap); r |= w->alloc(D.soff, D.tcap); r |= w->alloc(D.slist, D.ecap);
     r |= w->alloc(D.epc, D.ecap); r |= w->alloc(D.erel, D.ecap); r |= w->alloc(D.ekeep, (size_t)ENTRY_KEEP * D.ecap);
+    r |= w->alloc(D.bigc, (size_t)ns);
     r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
     r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
     for (ManBuf* M : {&D.cur, &D.old}) {
@@ -1762,7 +1803,8 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_cand_shape, dim3(gs), dim3(NT), 0, s, D);
     sf::scan_u32(D.ccnt, D.coff, D.ctr + C_NCAND, nullptr, (uint32_t)D.ns, (uint32_t)D.ns, w->scan_tmp, s);
     hipLaunchKernelGGL(k_cand_write, dim3(sf_grid(D.ecap)), dim3(NT), 0, s, D, K);
-    hipLaunchKernelGGL(k_cand_finish, dim3(sf_grid((uint64_t)D.ns * 64, NT, 65536)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_cand_finish, dim3(gs), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_cand_finish_big, dim3(1024), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[1], s));
     hipLaunchKernelGGL(k_narrow, dim3(gc), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.hit, D.hoff, D.ctr + C_NMAN, D.ctr + C_NCAND, 0, D.ccap, w->scan_tmp, s);
