@@ -1121,6 +1121,54 @@ __device__ __forceinline__ V3 wave_tree(V3 x) {
     for (int h = 32; h >= 1; h >>= 1) x = vadd(x, v3(__shfl_down(x.x, h, 64), __shfl_down(x.y, h, 64), __shfl_down(x.z, h, 64)));
     return rdl3(x, 0);
 }
+// The same tree for a list of at most 32 on a half wavefront: its first step adds the zeros of lanes 32-63 (x + 0,
+// kept: it turns -0 into +0 as that step does), then halving within the 32 lanes; every lane of the half gets the total.
+__device__ __forceinline__ V3 half_tree(V3 x) {
+    x = vadd(x, v3(0.0f, 0.0f, 0.0f));
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1) x = vadd(x, v3(__shfl_down(x.x, h, 32), __shfl_down(x.y, h, 32), __shfl_down(x.z, h, 32)));
+    return v3(__shfl(x.x, 0, 32), __shfl(x.y, 0, 32), __shfl(x.z, 0, 32));
+}
+// Per-body wavefront work over every body, two bodies at once when both have at most 32 contacts (a half wavefront
+// each, one chunk, half_tree), else one after the other on the whole wavefront (wave_tree): body(s, lane, half) for
+// slot s (s = 0, a resting body, in a half: it takes part in the shuffles and writes nothing).
+template <class Body>
+__device__ __forceinline__ void body_pairs(const Dev& D, Body body) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    const uint32_t nb = (uint32_t)D.nb, npair = (nb + 1) / 2;
+    for (uint32_t pi = wave; pi < npair; pi += nwaves) {
+        const uint32_t i0 = 2 * pi, i1 = i0 + 1;
+        const int s0 = D.bslot[i0], s1 = i1 < nb ? D.bslot[i1] : 0;
+        const uint32_t n0 = s0 ? D.blcnt[s0] : 0u, n1 = s1 ? D.blcnt[s1] : 0u;
+        if (n0 <= 32 && n1 <= 32) {
+            body((lane >> 5) ? s1 : s0, lane & 31, true);
+        } else {
+            if (s0) body(s0, lane, false);
+            if (s1) body(s1, lane, false);
+        }
+    }
+}
+// the tree sums of a body's list entries: f(e) gives the two vectors of entry e; half: one chunk of 32 lanes
+template <class F>
+__device__ __forceinline__ void list_sums(const Dev& D, int s, int lane, bool half, F f, V3& a, V3& b) {
+    a = v3(0, 0, 0); b = v3(0, 0, 0);
+    const uint32_t o = D.bloff[s], n = s ? D.blcnt[s] : 0u;
+    if (half) {
+        V3 x = v3(0, 0, 0), y = v3(0, 0, 0);
+        if ((uint32_t)lane < n) f(o + lane, x, y);
+        a = vadd(a, half_tree(x));
+        b = vadd(b, half_tree(y));
+        return;
+    }
+    for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+        uint32_t e = e0 + lane;
+        V3 x = v3(0, 0, 0), y = v3(0, 0, 0);
+        if (e < n) f(o + e, x, y);
+        a = vadd(a, wave_tree(x));
+        b = vadd(b, wave_tree(y));
+    }
+}
 
 // Gravity, spin clamp, then the warm start (phys.rs:217-222, :245-262): one wavefront per body, each lane one contact
 // (the body's contacts in manifold order, 64 at a time); a contact's contribution is its record's terms summed in
@@ -1385,54 +1433,35 @@ __device__ __forceinline__ void integrate_slot(const Dev& D, const StepK& K, int
     SQ(D, s) = f4q(qintegrate(qf4(SQ(D, s)), w, K.h));
 }
 __global__ void __launch_bounds__(NT) k_ovf_apply_integrate(Dev D, StepK K) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
-        int s = D.bslot[i];
-        if (!s) continue;
-        uint32_t cnt = D.ovcnt[s];
-        float4 v4 = SV(D, s);
+    body_pairs(D, [&](int s, int l, bool half) {
+        const uint32_t cnt = s ? D.ovcnt[s] : 0u;
+        const float4 v4 = SV(D, s);
         V3 v = v3f4(v4), w = v3f4(SW(D, s));
+        V3 ov, ow;
+        list_sums(D, s, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = OVL(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, ov, ow);
+        if (!s) return;
         if (cnt) {
-            ovf_average(D, s, lane, cnt, v, w);
-            if (lane == 0) { SV(D, s) = f4(v, v4.w); SW(D, s) = f4(w, 0.0f); }
+            float inv = 1.0f / (float)cnt;
+            v = vadd(v, vmul(ov, inv));
+            w = vadd(w, vmul(ow, inv));
+            if (l == 0) { SV(D, s) = f4(v, v4.w); SW(D, s) = f4(w, 0.0f); }
         }
-        if (lane == 0) integrate_slot(D, K, s, v, w);
-    }
+        if (l == 0) integrate_slot(D, K, s, v, w);
+    });
 }
 // The relax pass's end and the next sub-step's start: the overflow average, then the warm start, per body. Both
 // sets of per-contact records are loaded in one pass over the body's list (they do not depend on each other), and
 // each tree sums its records as ovf_average and warm_body do.
 __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     const V3 gh = vmul(v3(K.gx, K.gy, K.gz), K.h);
-    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
-        int s = D.bslot[i];
-        if (!s) continue;
-        const uint32_t cnt = D.ovcnt[s], o = D.bloff[s], n = D.blcnt[s];
+    body_pairs(D, [&](int s, int l, bool half) {
+        const uint32_t cnt = s ? D.ovcnt[s] : 0u;
         const float4 v4 = SV(D, s);
         V3 v = v3f4(v4), w = v3f4(SW(D, s));
-        V3 ov = v3(0, 0, 0), ow = v3(0, 0, 0), av = v3(0, 0, 0), aw = v3(0, 0, 0);
-        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-            uint32_t e = e0 + lane;
-            V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0), dv = v3(0, 0, 0), dw = v3(0, 0, 0);
-            if (e < n) {
-                const float4* wr = WREC(D, o + e);
-                float4 w0 = wr[0], w1 = wr[1];
-                float4 o0 = cnt ? wr[2] : make_float4(0, 0, 0, 0), o1 = cnt ? wr[3] : make_float4(0, 0, 0, 0);
-                dvv = v3f4(o0);
-                dww = v3f4(o1);
-                dv = v3f4(w0);
-                dw = v3f4(w1);
-            }
-            if (cnt) {
-                ov = vadd(ov, wave_tree(dvv));
-                ow = vadd(ow, wave_tree(dww));
-            }
-            av = vadd(av, wave_tree(dv));
-            aw = vadd(aw, wave_tree(dw));
-        }
+        V3 ov, ow, av, aw;
+        list_sums(D, s, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = OVL(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, ov, ow);
+        list_sums(D, s, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = WREC(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, av, aw);
+        if (!s) return;
         if (cnt) {
             float inv = 1.0f / (float)cnt;
             v = vadd(v, vmul(ov, inv));
@@ -1442,11 +1471,11 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
         V3 vw = vadd(v, gh);
         float wl = vlen2(w);
         if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
-        if (lane == 0) {
+        if (l == 0) {
             SV(D, s) = f4(vadd(vw, av), v4.w);
             SW(D, s) = f4(vadd(w, aw), 0.0f);
         }
-    }
+    });
 }
 __global__ void __launch_bounds__(NT) k_ovf_count(Dev D, int K) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
