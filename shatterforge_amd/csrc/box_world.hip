@@ -749,7 +749,12 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         V3 n = v3f4(n4);
         int cnt = (int)n4.w;
         Pt pts[4];
-        for (int k = 0; k < cnt; k++) { pts[k].p = v3f4(nr[1 + k]); pts[k].s = nr[1 + k].w; }
+        // (point loops run to the fixed four with k < cnt tests, so the per-point arrays stay in registers)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k < cnt) { float4 q4 = nr[1 + k]; pts[k].p = v3f4(q4); pts[k].s = q4.w; }
+            else { pts[k].p = v3(0, 0, 0); pts[k].s = 0.0f; }
+        }
         V3 t1, t2;
         tangents(n, t1, t2);
         float mu = mix_mu(D.shh[a].w, b >= 0 ? D.shh[b].w : K.friction);
@@ -758,10 +763,12 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         int oi = find_old(D, key, nold);
         uint32_t opos = oi >= 0 ? D.old.pos[oi] : 0u;
         V3 cen = v3(0, 0, 0);
-        for (int k = 0; k < cnt; k++) cen = vadd(cen, pts[k].p);
+#pragma unroll
+        for (int k = 0; k < 4; k++) if (k < cnt) cen = vadd(cen, pts[k].p);
         cen = vmul(cen, 1.0f / (float)cnt);
         float rad = 0.0f;
-        for (int k = 0; k < cnt; k++) rad += vlen(vsub(pts[k].p, cen));
+#pragma unroll
+        for (int k = 0; k < 4; k++) if (k < cnt) rad += vlen(vsub(pts[k].p, cen));
         rad = rad / (float)cnt;
         V3 fa, fb, la_f, lb_f, e0 = v3(0, 0, 0);
         float jt1 = 0.0f, jt2 = 0.0f, jtw = 0.0f;
@@ -795,16 +802,25 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         float jnk[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         V3 iaxk[4], ibxk[4];
         int ocnt = 0;
-        if (oi >= 0) ocnt = min(max((int)MROW(D, D.old, MR_CB1, opos).w, 0), 4);
-        for (int k = 0; k < cnt; k++) {
+        V3 ola[4];  // last step's point anchors, loaded once for all the new points
+        if (oi >= 0) {
+            ocnt = min(max((int)MROW(D, D.old, MR_CB1, opos).w, 0), 4);
+#pragma unroll
+            for (int q = 0; q < 4; q++) ola[q] = v3f4(MROW(D, D.old, MR_P0 + MR_PER_PT * q + 2, opos));
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k >= cnt) { iaxk[k] = v3(0, 0, 0); ibxk[k] = v3(0, 0, 0); continue; }
             V3 rap = vsub(pts[k].p, pa);
             V3 rbp = bb >= 0 ? vsub(pts[k].p, pb) : pts[k].p;
             V3 la = tmulv(ra, rap);
             float jn = 0.0f;
             if (oi >= 0) {
                 float best = 0.03f * 0.03f; int bi = -1;
-                for (int q = 0; q < ocnt; q++) {
-                    float d = vlen2(vsub(v3f4(MROW(D, D.old, MR_P0 + MR_PER_PT * q + 2, opos)), la));
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (q >= ocnt) continue;
+                    float d = vlen2(vsub(ola[q], la));
                     if (d < best) { best = d; bi = q; }
                 }
                 if (bi >= 0) jn = reinterpret_cast<const float*>(&MROW(D, D.old, MR_JN, opos))[bi];
