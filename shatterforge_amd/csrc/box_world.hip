@@ -486,33 +486,64 @@ __device__ __forceinline__ void tangents(V3 n, V3& t1, V3& t2) {
     t2 = vcross(n, t1);
 }
 
+// Eight-slot arrays kept in registers: every loop over them runs to eight with a guard and every access at a
+// computed position is an unrolled select (a dynamically indexed private array lives in scratch memory).
+// (bitwise picks: a select chain over an array is folded back into an indexed access, which needs the array in memory)
+__device__ __forceinline__ float pick(bool c, float a, float b) {
+    const int m = -(int)c;
+    return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+}
+__device__ __forceinline__ V3 pick(bool c, V3 a, V3 b) { return v3(pick(c, a.x, b.x), pick(c, a.y, b.y), pick(c, a.z, b.z)); }
+__device__ __forceinline__ Pt pick(bool c, Pt a, Pt b) { Pt r; r.p = pick(c, a.p, b.p); r.s = pick(c, a.s, b.s); return r; }
+template <class T> __device__ __forceinline__ void put8(T (&a)[8], int n, const T& v) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = pick(j == n, v, a[j]);
+}
+template <class T> __device__ __forceinline__ T get8(const T (&a)[8], int n) {
+    T r = a[0];
+#pragma unroll
+    for (int j = 1; j < 8; j++) r = pick(j == n, a[j], r);
+    return r;
+}
+__device__ __forceinline__ float sel3f(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ V3 row3(M3 m, int i) {
+    V3 a = m.c[0], b = m.c[1], c = m.c[2];
+    return v3(i == 0 ? a.x : (i == 1 ? b.x : c.x), i == 0 ? a.y : (i == 1 ? b.y : c.y), i == 0 ? a.z : (i == 1 ? b.z : c.z));
+}
+
 // keep the deepest point, the farthest from it and the two spanning the most area (phys.rs:507-527)
-__device__ int reduce4(Pt* pts, int cnt, V3 n) {
+__device__ __forceinline__ int reduce4(Pt (&pts)[8], int cnt, V3 n) {
     if (cnt <= 4) return cnt;
     int i0 = 0;
-    for (int k = 1; k < cnt; k++) if (pts[k].s < pts[i0].s) i0 = k;
-    V3 p0 = pts[i0].p;
+#pragma unroll
+    for (int k = 1; k < 8; k++) if (k < cnt && pts[k].s < get8(pts, i0).s) i0 = k;
+    V3 p0 = get8(pts, i0).p;
     int i1 = 0; float best = -1.0f;
-    for (int k = 0; k < cnt; k++) { float d = vlen2(vsub(pts[k].p, p0)); if (d > best) { best = d; i1 = k; } }
-    V3 e = vsub(pts[i1].p, p0);
+#pragma unroll
+    for (int k = 0; k < 8; k++) { if (k >= cnt) continue; float d = vlen2(vsub(pts[k].p, p0)); if (d > best) { best = d; i1 = k; } }
+    V3 e = vsub(get8(pts, i1).p, p0);
     int i2 = 0; float b2 = -3.40282347e38f;
-    for (int k = 0; k < cnt; k++) { float a = vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b2) { b2 = a; i2 = k; } }
+#pragma unroll
+    for (int k = 0; k < 8; k++) { if (k >= cnt) continue; float a = vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b2) { b2 = a; i2 = k; } }
     int i3 = 0; float b3 = -3.40282347e38f;
-    for (int k = 0; k < cnt; k++) { float a = -vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b3) { b3 = a; i3 = k; } }
-    int idx[4] = {i0, i1, i2, i3};
-    Pt sel[4] = {pts[i0], pts[i1], pts[i2], pts[i3]};
+#pragma unroll
+    for (int k = 0; k < 8; k++) { if (k >= cnt) continue; float a = -vdot(vcross(vsub(pts[k].p, p0), e), n); if (a > b3) { b3 = a; i3 = k; } }
+    const int idx[4] = {i0, i1, i2, i3};
+    const Pt sel[4] = {get8(pts, i0), get8(pts, i1), get8(pts, i2), get8(pts, i3)};
     int out = 0;
+#pragma unroll
     for (int k = 0; k < 4; k++) {
         bool dup = false;
+#pragma unroll
         for (int q = 0; q < k; q++) dup |= idx[q] == idx[k];
-        if (!dup) pts[out++] = sel[k];
+        if (!dup) { put8(pts, out, sel[k]); out++; }
     }
     return out;
 }
 
 // 15-axis SAT; face contact clips the incident face against the reference face's side planes; edge contact gives
 // the closest points of the two edges (phys.rs:531-628). Returns the point count, normal from A to B.
-__device__ int collide_boxes(V3 pa, const M3& ra, V3 ha, V3 pb, const M3& rb, V3 hb, float margin, Pt* out, V3& n_out) {
+__device__ __forceinline__ int collide_boxes(V3 pa, M3 ra, V3 ha, V3 pb, M3 rb, V3 hb, float margin, Pt (&out)[8], V3& n_out) {
     V3 t = vsub(pb, pa);
     float ac[3][3];
     for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) ac[i][j] = sf_abs(vdot(ra.c[i], rb.c[j])) + 1e-6f;
@@ -547,19 +578,21 @@ __device__ int collide_boxes(V3 pa, const M3& ra, V3 ha, V3 pb, const M3& rb, V3
     if (se > face_best + 0.01f) {
         V3 n = vdot(t, el) < 0.0f ? vneg(el) : el;
         V3 ca = pa, cb = pb;
+#pragma unroll
         for (int k = 0; k < 3; k++) {
             if (k != ei) { float s = vdot(ra.c[k], n) > 0.0f ? 1.0f : -1.0f; ca = vadd(ca, vmul(ra.c[k], s * hav[k])); }
             if (k != ej) { float s = vdot(rb.c[k], n) > 0.0f ? -1.0f : 1.0f; cb = vadd(cb, vmul(rb.c[k], s * hbv[k])); }
         }
-        V3 d1 = ra.c[ei], d2 = rb.c[ej];
+        V3 d1 = row3(ra, ei), d2 = row3(rb, ej);
+        const float hae = sel3f(ha, ei), hbe = sel3f(hb, ej);
         V3 r = vsub(ca, cb);
         float bb = vdot(d1, d2), c = vdot(d1, r), f = vdot(d2, r);
         float den = 1.0f - bb * bb;
         float s = den > 1e-6f ? (bb * f - c) / den : 0.0f;
-        s = sf_min(sf_max(s, -hav[ei]), hav[ei]);
+        s = sf_min(sf_max(s, -hae), hae);
         float tt = bb * s + f;
-        tt = sf_min(sf_max(tt, -hbv[ej]), hbv[ej]);
-        s = sf_min(sf_max(bb * tt - c, -hav[ei]), hav[ei]);
+        tt = sf_min(sf_max(tt, -hbe), hbe);
+        s = sf_min(sf_max(bb * tt - c, -hae), hae);
         V3 p1 = vadd(ca, vmul(d1, s)), p2 = vadd(cb, vmul(d2, tt));
         out[0].p = vmul(vadd(p1, p2), 0.5f);
         out[0].s = vdot(vsub(p2, p1), n);
@@ -567,47 +600,59 @@ __device__ int collide_boxes(V3 pa, const M3& ra, V3 ha, V3 pb, const M3& rb, V3
         return 1;
     }
     V3 pr = use_b ? pb : pa, pi = use_b ? pa : pb;
-    const M3& rr = use_b ? rb : ra;
-    const M3& ri = use_b ? ra : rb;
-    float hr[3], hi[3];
-    for (int q = 0; q < 3; q++) { hr[q] = use_b ? hbv[q] : hav[q]; hi[q] = use_b ? hav[q] : hbv[q]; }
+    M3 rr, ri;
+#pragma unroll
+    for (int q = 0; q < 3; q++) { rr.c[q] = use_b ? rb.c[q] : ra.c[q]; ri.c[q] = use_b ? ra.c[q] : rb.c[q]; }
+    const V3 hr = use_b ? hb : ha, hi = use_b ? ha : hb;
     int k = use_b ? jb : ia;
-    V3 n = rr.c[k];
+    V3 n = row3(rr, k);
     if (vdot(vsub(pi, pr), n) < 0.0f) n = vneg(n);
     int j = 0; float bd = -1.0f;
+#pragma unroll
     for (int q = 0; q < 3; q++) { float d = sf_abs(vdot(ri.c[q], n)); if (d > bd) { bd = d; j = q; } }
-    float sgn = vdot(ri.c[j], n) > 0.0f ? -1.0f : 1.0f;
-    V3 fc = vadd(pi, vmul(ri.c[j], sgn * hi[j]));
+    const V3 rj = row3(ri, j);
+    float sgn = vdot(rj, n) > 0.0f ? -1.0f : 1.0f;
+    V3 fc = vadd(pi, vmul(rj, sgn * sel3f(hi, j)));
     int u = (j + 1) % 3, v = (j + 2) % 3;
-    V3 eu = vmul(ri.c[u], hi[u]), ev = vmul(ri.c[v], hi[v]);
+    V3 eu = vmul(row3(ri, u), sel3f(hi, u)), ev = vmul(row3(ri, v), sel3f(hi, v));
     V3 poly[8], tmp[8];
+#pragma unroll
+    for (int q = 4; q < 8; q++) { poly[q] = v3(0, 0, 0); tmp[q] = v3(0, 0, 0); }
     poly[0] = vadd(vadd(fc, eu), ev); poly[1] = vadd(vsub(fc, eu), ev); poly[2] = vsub(vsub(fc, eu), ev); poly[3] = vsub(vadd(fc, eu), ev);
+#pragma unroll
+    for (int q = 0; q < 4; q++) tmp[q] = v3(0, 0, 0);
     int pn = 4;
     int u2 = (k + 1) % 3, v2 = (k + 2) % 3;
+#pragma unroll
     for (int pl = 0; pl < 4; pl++) {
         int ax = pl < 2 ? u2 : v2;
         float sg = (pl & 1) ? -1.0f : 1.0f;
-        V3 dir = vmul(rr.c[ax], sg);
-        float off = vdot(dir, pr) + hr[ax];
+        V3 dir = vmul(row3(rr, ax), sg);
+        float off = vdot(dir, pr) + sel3f(hr, ax);
         int tn = 0;
-        for (int q = 0; q < pn; q++) {
-            V3 p0 = poly[q], p1 = poly[(q + 1) % pn];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (q >= pn) continue;
+            V3 p0 = poly[q], p1 = (q + 1 < pn && q < 7) ? poly[q < 7 ? q + 1 : 0] : poly[0];
             float d0 = vdot(p0, dir) - off, d1 = vdot(p1, dir) - off;
-            if (d0 <= 0.0f) { if (tn < 8) tmp[tn++] = p0; }
+            if (d0 <= 0.0f) { if (tn < 8) { put8(tmp, tn, p0); tn++; } }
             if ((d0 <= 0.0f) != (d1 <= 0.0f)) {
                 float tt = d0 / (d0 - d1);
-                if (tn < 8) tmp[tn++] = vadd(p0, vmul(vsub(p1, p0), tt));
+                if (tn < 8) { put8(tmp, tn, vadd(p0, vmul(vsub(p1, p0), tt))); tn++; }
             }
         }
         pn = tn;
-        for (int q = 0; q < pn; q++) poly[q] = tmp[q];
+#pragma unroll
+        for (int q = 0; q < 8; q++) poly[q] = tmp[q];
         if (pn == 0) return 0;
     }
-    V3 rc = vadd(pr, vmul(n, hr[k]));
+    V3 rc = vadd(pr, vmul(n, sel3f(hr, k)));
     int cnt = 0;
-    for (int q = 0; q < pn; q++) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (q >= pn) continue;
         float sep = vdot(vsub(poly[q], rc), n);
-        if (sep <= margin) { out[cnt].p = vsub(poly[q], vmul(n, 0.5f * sep)); out[cnt].s = sep; cnt++; }
+        if (sep <= margin) { Pt t; t.p = vsub(poly[q], vmul(n, 0.5f * sep)); t.s = sep; put8(out, cnt, t); cnt++; }
     }
     if (cnt == 0) return 0;
     n_out = use_b ? vneg(n) : n;
@@ -629,10 +674,11 @@ __global__ void __launch_bounds__(NT) k_narrow(Dev D, StepK K) {
             M3 R = load_m3(D.swr + 3 * (size_t)c.x);
             V3 h = v3f4(D.shh[c.x]);
             float margin = K.spec + sf_max(-D.bv[ba].y, 0.0f) * (1.0f / 60.0f);
+#pragma unroll
             for (int k = 0; k < 8; k++) {
                 V3 sg = v3((k & 1) ? 1.0f : -1.0f, (k & 2) ? 1.0f : -1.0f, (k & 4) ? 1.0f : -1.0f);
                 V3 cp = vadd(wp, mulv(R, v3(sg.x * h.x, sg.y * h.y, sg.z * h.z)));
-                if (cp.y < margin) { pts[cnt].p = v3(cp.x, 0.5f * cp.y, cp.z); pts[cnt].s = cp.y; cnt++; }
+                if (cp.y < margin) { Pt t; t.p = v3(cp.x, 0.5f * cp.y, cp.z); t.s = cp.y; put8(pts, cnt, t); cnt++; }
             }
             n = v3(0.0f, -1.0f, 0.0f);
         } else {
@@ -657,7 +703,8 @@ __global__ void __launch_bounds__(NT) k_narrow(Dev D, StepK K) {
         D.hit[i] = hit ? 1u : 0u;
         float4* o = D.nar + 5 * (size_t)i;
         o[0] = f4(n, (float)cnt);
-        for (int k = 0; k < cnt; k++) o[1 + k] = f4(pts[k].p, pts[k].s);
+#pragma unroll
+        for (int k = 0; k < 4; k++) if (k < cnt) o[1 + k] = f4(pts[k].p, pts[k].s);  // (at most 4 after reduce4)
     }
 }
 
