@@ -33,8 +33,9 @@ namespace {
 constexpr int NT = 256;  // threads per block of the grid-stride kernels
 
 // device counters
-enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_ACTIVE,
-       C_ROUNDS, C_NOVF2, C_NSLOT, C_NBIGC, C_NCTR = 16 };
+// (C_ACTIVE0 + g: set when a manifold claimed in claim-round group g, rounds 4g to 4g + 3)
+enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_NSLOT, C_NBIGC,
+       C_ACTIVE0, C_NCTR = C_ACTIVE0 + 8 };
 
 // manifold rows (each row is a float4 array of mcap entries)
 // (the inertia terms, a body's world inverse inertia times an arm or the normal, are not stored: the solve forms them
@@ -961,8 +962,8 @@ __global__ void __launch_bounds__(NT) k_col_init(Dev D) {
     GRID_STRIDE(m, nm) D.mcol[m] = -1;
 }
 
-// (C_ACTIVE is a flag, set by one plain store per wavefront that claimed: tens of thousands of atomic adds on one
-// address serialise)
+// (the activity flag of the round's group is set by one plain store per wavefront that claimed: tens of thousands of
+// atomic adds on one address serialise)
 __global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int r) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     const unsigned long long all = K == 64 ? ~0ull : ((1ull << K) - 1ull);
@@ -989,7 +990,7 @@ __global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int 
         if (sb) atomicMax(&D.claim[(size_t)sb * K + c], pri);
         claimed = true;
     }
-    if (__ballot(claimed) && (threadIdx.x & 63) == 0) D.ctr[C_ACTIVE] = 1u;
+    if (__ballot(claimed) && (threadIdx.x & 63) == 0) D.ctr[C_ACTIVE0 + min(r >> 2, 7)] = 1u;
 }
 
 __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
@@ -1713,6 +1714,17 @@ __global__ void __launch_bounds__(NT) k_sleep_apply(Dev D) {
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) { for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v; }
+// the step's counters and per-step tables cleared in one launch
+__global__ void __launch_bounds__(NT) k_step_clear(Dev D, int K) {
+    const size_t nb1 = (size_t)D.nb + 1, ncl = nb1 * (size_t)K, n = max((size_t)D.tcap, ncl);
+    for (size_t i = blockIdx.x * (size_t)NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
+        if (i < (size_t)C_NCTR) D.ctr[i] = 0u;
+        if (i < (size_t)D.tcap) { D.tcnt[i] = 0u; D.tfill[i] = 0u; }
+        if (i < nb1) { D.blcnt[i] = 0u; D.blfill[i] = 0u; D.bmask[i] = 0ull; D.ovcnt[i] = 0u; }
+        if (i < ncl) D.claim[i] = 0ull;
+        if (i < 72) { D.colcnt[i] = 0u; D.colfill[i] = 0u; }
+    }
+}
 __global__ void k_fill_u64(unsigned long long* p, size_t n) { for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull; }
 
 // ---------------------------------------------------------------- host
@@ -1871,16 +1883,7 @@ int step_front(HostWorld* w, const StepK& K) {
     // old manifolds = last step's
     std::swap(D.cur, D.old);
     // counters and per-step tables
-    SF_HIP(hipMemsetAsync(D.ctr, 0, sizeof(uint32_t) * C_NCTR, s));
-    SF_HIP(hipMemsetAsync(D.tcnt, 0, sizeof(uint32_t) * D.tcap, s));
-    SF_HIP(hipMemsetAsync(D.tfill, 0, sizeof(uint32_t) * D.tcap, s));
-    SF_HIP(hipMemsetAsync(D.blcnt, 0, sizeof(uint32_t) * ((size_t)D.nb + 1), s));
-    SF_HIP(hipMemsetAsync(D.blfill, 0, sizeof(uint32_t) * ((size_t)D.nb + 1), s));
-    SF_HIP(hipMemsetAsync(D.bmask, 0, sizeof(unsigned long long) * ((size_t)D.nb + 1), s));
-    SF_HIP(hipMemsetAsync(D.claim, 0, sizeof(unsigned long long) * ((size_t)D.nb + 1) * K.K, s));
-    SF_HIP(hipMemsetAsync(D.colcnt, 0, sizeof(uint32_t) * 72, s));
-    SF_HIP(hipMemsetAsync(D.colfill, 0, sizeof(uint32_t) * 72, s));
-    SF_HIP(hipMemsetAsync(D.ovcnt, 0, sizeof(uint32_t) * ((size_t)D.nb + 1), s));
+    hipLaunchKernelGGL(k_step_clear, dim3(sf_grid(std::max<size_t>(D.tcap, ((size_t)D.nb + 1) * K.K))), dim3(NT), 0, s, D, K.K);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[0], s));
     hipLaunchKernelGGL(k_prep_bodies, dim3(gb), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_prep_shapes, dim3(gs), dim3(NT), 0, s, D, K);
@@ -1914,8 +1917,7 @@ int step_front(HostWorld* w, const StepK& K) {
         if ((r & 3) == 3 && r + 1 < R) {
             SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
             SF_HIP(hipStreamSynchronize(s));
-            if (w->h_ctr[C_ACTIVE] == 0) break;
-            SF_HIP(hipMemsetAsync(D.ctr + C_ACTIVE, 0, sizeof(uint32_t), s));
+            if (w->h_ctr[C_ACTIVE0 + std::min(r >> 2, 7)] == 0) break;
         }
     }
     w->stats[SF_S_ROUNDS] = rounds;
