@@ -1148,11 +1148,15 @@ __device__ float sweep_struct(LD& D, uint32_t g, float om, const Phi<LDSP>& phi,
             float4 ka[U];
             float2 kb[U];
             uint32_t ii[U];
+            // (every row of the batch loads at once: loading the rest only after the diagonal says the brick is in the
+            // structure would cost a second round trip)
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                uint32_t q = t + u * ST_NT, p = o + q;
-                sr[u] = q < n ? D.sten_s[p] : make_float2(-1.0f, 0.0f);
-                if (sr[u].x >= 0.0f) { na[u] = D.sten_na[p]; nb[u] = D.sten_nb[p]; ka[u] = D.sten_ka[p]; kb[u] = D.sten_kb[p]; ii[u] = D.st_red[p]; }
+                uint32_t q = t + u * ST_NT;
+                const bool in = q < n;
+                uint32_t p = o + (in ? q : 0u);
+                sr[u] = in ? D.sten_s[p] : make_float2(-1.0f, 0.0f);
+                na[u] = D.sten_na[p]; nb[u] = D.sten_nb[p]; ka[u] = D.sten_ka[p]; kb[u] = D.sten_kb[p]; ii[u] = D.st_red[p];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {

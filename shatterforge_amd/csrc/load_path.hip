@@ -135,15 +135,19 @@ __device__ void solve_struct(const LoadDev& L, int s, float* lds, double* sh) {
                     int jj[U][6];
                     float kk[U][6], rr[U], ss[U];
                     bool on[U];
+                    // (the rows load whether or not the brick is active, from a valid index, so they are in flight
+                    // with the activity flag instead of after it)
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         int ii = i + u * LT;
-                        size_t g = (size_t)base + ii;
-                        on[u] = ii < i1 && L.act[g];
+                        const bool in = ii < i1;
+                        size_t g = (size_t)base + (in ? ii : i);
+                        on[u] = in && L.act[g];
 #pragma unroll
-                        for (int d = 0; d < 6; d++) { jj[u][d] = on[u] ? L.nbr[d * N + g] : -1; kk[u][d] = on[u] ? L.kc[d * N + g] : 0.0f; }
-                        rr[u] = on[u] ? L.r32[g] : 0.0f;
-                        ss[u] = on[u] ? L.sk32[g] : 1.0f;
+                        for (int d = 0; d < 6; d++) { int jv = L.nbr[d * N + g]; float kv = L.kc[d * N + g]; jj[u][d] = on[u] ? jv : -1; kk[u][d] = on[u] ? kv : 0.0f; }
+                        float rv = L.r32[g], sv = L.sk32[g];
+                        rr[u] = on[u] ? rv : 0.0f;
+                        ss[u] = on[u] ? sv : 1.0f;
                     }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
