@@ -97,7 +97,7 @@ struct Dev {
     // colouring
     int32_t *mcol, *mcand;
     unsigned long long *mpri, *bmask, *claim;
-    uint32_t *colcnt, *coloff, *colfill, *ovflag, *ovoff, *ovcnt;
+    uint32_t *colcnt, *coloff, *colfill, *mbin, *subcnt, *suboff, *ovcnt;
     uint32_t* chunkc;  // per colour, per chunk of NT manifolds: count, then its exclusive offset within the colour
     uint32_t* ptpart;  // contact points per workgroup of the build (summed by k_sum_points)
     // per-slot manifold lists (m * 2 + side), in manifold order
@@ -1008,36 +1008,49 @@ __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
     }
 }
 
-// colour histogram: per-block counts in LDS, stored per colour per chunk (a few colours take every manifold, so
+// The overflow bucket is placed in OVS spatial sub-buckets: a Morton index of its first moving body's 4 m cell, each
+// axis taken modulo 4, so the contiguous eighth of the bucket that one XCD solves covers a compact region and its
+// bodies' records are reused from that XCD's L2. Within the overflow (a Jacobi pass) the order changes no result.
+constexpr int OVS = 64;
+constexpr int NBIN = 64 + OVS;  // colour bins (up to 64 colours) and the overflow's sub-buckets
+__device__ __forceinline__ uint32_t ovf_key(const Dev& D, int sa, int sb) {
+    const float4 p = D.sp0[sa ? sa : sb];
+    const uint32_t x = (uint32_t)(int)floorf(p.x * 0.25f) & 3u, y = (uint32_t)(int)floorf(p.y * 0.25f) & 3u,
+                   z = (uint32_t)(int)floorf(p.z * 0.25f) & 3u;
+    return (x & 1u) | ((y & 1u) << 1) | ((z & 1u) << 2) | ((x & 2u) << 2) | ((y & 2u) << 3) | ((z & 2u) << 4);
+}
+// colour histogram: per-block counts in LDS, stored per bin per chunk (a few colours take every manifold, so
 // global atomics, per manifold or per block, would serialise on a handful of addresses); the colour totals come
-// from the chunk scan
+// from the chunk scan. Bins 0..K-1 are the colours, K + k the overflow's sub-bucket k; mbin keeps each manifold's bin.
 __global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
-    __shared__ uint32_t hist[72];
+    __shared__ uint32_t hist[NBIN];
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     for (uint32_t b0 = blockIdx.x * NT; b0 < nm; b0 += gridDim.x * NT) {
-        for (int c = threadIdx.x; c <= K; c += NT) hist[c] = 0u;
+        for (int c = threadIdx.x; c < K + OVS; c += NT) hist[c] = 0u;
         __syncthreads();
         uint32_t m = b0 + threadIdx.x;
         if (m < nm) {
             int c = D.mcol[m];
             if (c < 0) { c = K; D.mcol[m] = K; }
+            if (c == K) {
+                const int4 hd = D.cur.hdr[m];
+                c = K + (int)ovf_key(D, hd.z, hd.w);
+            }
             atomicAdd(&hist[c], 1u);
-            D.ovflag[m] = c == K ? 1u : 0u;
+            D.mbin[m] = (uint32_t)c;
         }
         __syncthreads();
         const uint32_t nchunk = (D.mcap + NT - 1) / NT, chunk = b0 / NT;
-        for (int c = threadIdx.x; c < K; c += NT) D.chunkc[(size_t)c * nchunk + chunk] = hist[c];
+        for (int c = threadIdx.x; c < K + OVS; c += NT) D.chunkc[(size_t)c * nchunk + chunk] = hist[c];
         __syncthreads();
     }
 }
 
-// per colour, the exclusive scan of its chunk counts in chunk order (one block per colour)
-// block K: the overflow's total (from the scan of the overflow flags) into its bucket count
+// per bin, the exclusive scan of its chunk counts in chunk order (one block per bin): a colour's total into its
+// bucket count, an overflow sub-bucket's into subcnt
 __global__ void __launch_bounds__(NT) k_col_chunk_scan(Dev D, int K) {
     __shared__ uint32_t ws[NT / 64];
     const int c = blockIdx.x;
-    if (c == K && threadIdx.x == 0) D.colcnt[K] = D.ctr[C_NOVF];
-    if (c >= K) return;
     const uint32_t nm = min(D.ctr[C_NMAN], D.mcap), nchunk = (D.mcap + NT - 1) / NT, used = (nm + NT - 1) / NT;
     uint32_t* a = D.chunkc + (size_t)c * nchunk;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1054,24 +1067,26 @@ __global__ void __launch_bounds__(NT) k_col_chunk_scan(Dev D, int K) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        D.colcnt[c] = carry;
-        if (carry) atomicMax(&D.ctr[C_NCOL], (uint32_t)c + 1u);
+        if (c < K) {
+            D.colcnt[c] = carry;
+            if (carry) atomicMax(&D.ctr[C_NCOL], (uint32_t)c + 1u);
+        } else D.subcnt[c - K] = carry;
     }
 }
-// colour-ordered list, stable: within a colour, manifolds keep their canonical order (chunk offsets from the scan,
-// ranks inside a chunk from wavefront ballots), so a bucket is spatially ordered like the pair list
+// colour-ordered list, stable: within a colour (and within an overflow sub-bucket) manifolds keep their canonical
+// order (chunk offsets from the scan, ranks inside a chunk from wavefront ballots)
 __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
-    __shared__ uint32_t wc[NT / 64][72];
+    __shared__ uint32_t wc[NT / 64][NBIN];
     const uint32_t nm = min(D.ctr[C_NMAN], D.mcap), nchunk = (D.mcap + NT - 1) / NT;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.ctr[C_NOVF] = D.colcnt[K];
     for (uint32_t b0 = blockIdx.x * NT; b0 < nm; b0 += gridDim.x * NT) {
-        for (int q = threadIdx.x; q < (NT / 64) * 72; q += NT) wc[q / 72][q % 72] = 0u;
+        for (int q = threadIdx.x; q < (NT / 64) * NBIN; q += NT) wc[q / NBIN][q % NBIN] = 0u;
         __syncthreads();
         uint32_t m = b0 + threadIdx.x;
-        int c = m < nm ? D.mcol[m] : -1;
-        bool col = c >= 0 && c < K;
+        int c = m < nm ? (int)D.mbin[m] : -1;
         uint32_t rank = 0;
-        unsigned long long todo = __ballot(col);
+        unsigned long long todo = __ballot(c >= 0);
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             int cc = __shfl(c, l, 64);
@@ -1081,11 +1096,12 @@ __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
             todo &= ~same;
         }
         __syncthreads();
-        if (col) {
+        if (c >= 0) {
             uint32_t before = 0;
             for (int k = 0; k < wv; k++) before += wc[k][c];
-            D.cur.pos[m] = D.coloff[c] + D.chunkc[(size_t)c * nchunk + b0 / NT] + before + rank;
-        } else if (c == K) D.cur.pos[m] = D.coloff[K] + D.ovoff[m];  // the overflow in manifold order
+            const uint32_t base = c < K ? D.coloff[c] : D.coloff[K] + D.suboff[c - K];
+            D.cur.pos[m] = base + D.chunkc[(size_t)c * nchunk + b0 / NT] + before + rank;
+        }
         __syncthreads();
     }
 }
@@ -1174,50 +1190,90 @@ __global__ void __launch_bounds__(NT) k_bl_inv(Dev D) {
 // sequence of f32 updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262)
 __device__ __forceinline__ V3 rdl3(V3 a, int l) { return v3(rdl(a.x, l), rdl(a.y, l), rdl(a.z, l)); }
 
-// Gravity, spin clamp, then this body's warm start over its contacts in manifold order: the same sequence of f32
-// updates the reference's manifold loop applies to it (phys.rs:217-222, :245-262). One wavefront per body: the lanes
-// load up to 64 contacts at once and form their update vectors; every lane then applies them in order (values read
-// across lanes), so the loads overlap and the order of the additions is the reference's.
+// Cross-lane steps of the trees, on the vector ALU where the hardware allows: lane i takes lane i + H within its row
+// of 16 lanes (a DPP row shift, H = 1, 2, 4, 8), or lane i ^ 16 within its 32 lanes (a swizzle). Lanes whose source
+// lies outside the row keep their own value; in a halving tree they never feed lane 0 of a row.
+template <int H>
+__device__ __forceinline__ float row_down(float x) {
+    static_assert(H == 1 || H == 2 || H == 4 || H == 8, "row shift of 1, 2, 4 or 8 lanes");
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x100 + H, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float xor16(float x) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x1F | (0x10 << 10)));
+}
+template <int H>
+__device__ __forceinline__ V3 v3_row_down(V3 x) { return v3(row_down<H>(x.x), row_down<H>(x.y), row_down<H>(x.z)); }
+__device__ __forceinline__ V3 v3_xor16(V3 x) { return v3(xor16(x.x), xor16(x.y), xor16(x.z)); }
+// the tree's steps h = 16 ... 1 within each 32 lanes (lane 0 and lane 32 end with their half's total)
+__device__ __forceinline__ V3 tree32(V3 x) {
+    x = vadd(x, v3_xor16(x));
+    x = vadd(x, v3_row_down<8>(x));
+    x = vadd(x, v3_row_down<4>(x));
+    x = vadd(x, v3_row_down<2>(x));
+    return vadd(x, v3_row_down<1>(x));
+}
 // Fixed halving tree across a wavefront (x_i += x_{i+h}, h = 32 ... 1); lane 0's total is returned to every lane.
 // The oracle sums in exactly this tree.
 __device__ __forceinline__ V3 wave_tree(V3 x) {
-#pragma unroll
-    for (int h = 32; h >= 1; h >>= 1) x = vadd(x, v3(__shfl_down(x.x, h, 64), __shfl_down(x.y, h, 64), __shfl_down(x.z, h, 64)));
-    return rdl3(x, 0);
+    x = vadd(x, v3(__shfl_down(x.x, 32, 64), __shfl_down(x.y, 32, 64), __shfl_down(x.z, 32, 64)));
+    return rdl3(tree32(x), 0);
 }
 // The same tree for a list of at most 32 on a half wavefront: its first step adds the zeros of lanes 32-63 (x + 0,
 // kept: it turns -0 into +0 as that step does), then halving within the 32 lanes; every lane of the half gets the total.
 __device__ __forceinline__ V3 half_tree(V3 x) {
-    x = vadd(x, v3(0.0f, 0.0f, 0.0f));
-#pragma unroll
-    for (int h = 16; h >= 1; h >>= 1) x = vadd(x, v3(__shfl_down(x.x, h, 32), __shfl_down(x.y, h, 32), __shfl_down(x.z, h, 32)));
-    return v3(__shfl(x.x, 0, 32), __shfl(x.y, 0, 32), __shfl(x.z, 0, 32));
+    x = tree32(vadd(x, v3(0.0f, 0.0f, 0.0f)));
+    const V3 lo = rdl3(x, 0), hi = rdl3(x, 32);
+    return vsel((threadIdx.x & 32) != 0, hi, lo);
+}
+// What the per-body kernels read of a body's solver slot. A body's slot is its index + 1 while it is awake and 0
+// otherwise (k_prep_bodies), so these loads are issued on index + 1 in the same round trip as bslot, which then only
+// says whether the body takes part (s = 0: it does not; its list then counts as empty).
+struct BodyIn {
+    int s;                // the slot, 0 for a body that is not awake
+    uint32_t o, n, cnt;   // contact-list offset and length, overflow manifolds
+    float4 v4, w4;        // SV, SW
+};
+__device__ __forceinline__ BodyIn body_in(const Dev& D, uint32_t i) {
+    const int q = (int)i + 1;
+    BodyIn b;
+    const int s = D.bslot[i];
+    const uint32_t o = D.bloff[q], n = D.blcnt[q], cnt = D.ovcnt[q];
+    b.v4 = SV(D, q); b.w4 = SW(D, q);
+    b.s = s; b.o = o; b.n = s ? n : 0u; b.cnt = s ? cnt : 0u;
+    return b;
+}
+__device__ __forceinline__ BodyIn body_pick(bool hi, const BodyIn& b0, const BodyIn& b1) {
+    BodyIn b;
+    b.s = hi ? b1.s : b0.s; b.o = hi ? b1.o : b0.o; b.n = hi ? b1.n : b0.n; b.cnt = hi ? b1.cnt : b0.cnt;
+    b.v4 = hi ? b1.v4 : b0.v4; b.w4 = hi ? b1.w4 : b0.w4;
+    return b;
 }
 // Per-body wavefront work over every body, two bodies at once when both have at most 32 contacts (a half wavefront
-// each, one chunk, half_tree), else one after the other on the whole wavefront (wave_tree): body(s, lane, half) for
-// slot s (s = 0, a resting body, in a half: it takes part in the shuffles and writes nothing).
+// each, one chunk, half_tree), else one after the other on the whole wavefront (wave_tree): body(b, lane, half)
+// (b.s = 0, a resting body, in a half: it takes part in the shuffles and writes nothing).
 template <class Body>
 __device__ __forceinline__ void body_pairs(const Dev& D, Body body) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     const uint32_t nb = (uint32_t)D.nb, npair = (nb + 1) / 2;
     for (uint32_t pi = wave; pi < npair; pi += nwaves) {
-        const uint32_t i0 = 2 * pi, i1 = i0 + 1;
-        const int s0 = D.bslot[i0], s1 = i1 < nb ? D.bslot[i1] : 0;
-        const uint32_t n0 = s0 ? D.blcnt[s0] : 0u, n1 = s1 ? D.blcnt[s1] : 0u;
-        if (n0 <= 32 && n1 <= 32) {
-            body((lane >> 5) ? s1 : s0, lane & 31, true);
+        const uint32_t i0 = 2 * pi, i1 = i0 + 1 < nb ? i0 + 1 : i0;
+        const BodyIn b0 = body_in(D, i0);
+        BodyIn b1 = body_in(D, i1);
+        if (i1 == i0) { b1.s = 0; b1.n = 0u; b1.cnt = 0u; }  // an odd count: the last wavefront's second half is empty
+        if (b0.n <= 32 && b1.n <= 32) {
+            body(body_pick(lane >> 5, b0, b1), lane & 31, true);
         } else {
-            if (s0) body(s0, lane, false);
-            if (s1) body(s1, lane, false);
+            if (b0.s) body(b0, lane, false);
+            if (b1.s) body(b1, lane, false);
         }
     }
 }
-// the tree sums of a body's list entries: f(e) gives the two vectors of entry e; half: one chunk of 32 lanes
+// the tree sums of a body's list entries (offset o, length n): f(e) gives the two vectors of entry e; half: one
+// chunk of 32 lanes
 template <class F>
-__device__ __forceinline__ void list_sums(const Dev& D, int s, int lane, bool half, F f, V3& a, V3& b) {
+__device__ __forceinline__ void list_sums(uint32_t o, uint32_t n, int lane, bool half, F f, V3& a, V3& b) {
     a = v3(0, 0, 0); b = v3(0, 0, 0);
-    const uint32_t o = D.bloff[s], n = s ? D.blcnt[s] : 0u;
     if (half) {
         V3 x = v3(0, 0, 0), y = v3(0, 0, 0);
         if ((uint32_t)lane < n) f(o + lane, x, y);
@@ -1237,13 +1293,15 @@ __device__ __forceinline__ void list_sums(const Dev& D, int s, int lane, bool ha
 // Gravity, spin clamp, then the warm start (phys.rs:217-222, :245-262): one wavefront per body, each lane one contact
 // (the body's contacts in manifold order, 64 at a time); a contact's contribution is its record's terms summed in
 // point order, and the body adds the tree sum of its contacts' contributions.
-// one body's warm start (its wavefront; every lane holds the same v4 = sv, w = sw on entry)
-__device__ __forceinline__ void warm_body(const Dev& D, const StepK& K, int s, int lane, float4 v4, V3 w) {
+// one body's warm start (its wavefront; every lane holds the same b on entry; nothing is written for b.s = 0)
+__device__ __forceinline__ void warm_body(const Dev& D, const StepK& K, const BodyIn& b, int lane) {
     const V3 gh = vmul(v3(K.gx, K.gy, K.gz), K.h);
-    V3 v = vadd(v3f4(v4), gh);
+    const float4 v4 = b.v4;
+    V3 v = vadd(v3f4(v4), gh), w = v3f4(b.w4);
     float wl = vlen2(w);
     if (wl > K.wmax * K.wmax) w = vmul(w, K.wmax / sf_sqrt(wl));
-    uint32_t o = D.bloff[s], n = D.blcnt[s];
+    const int s = b.s;
+    const uint32_t o = b.o, n = b.n;
     V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
     for (uint32_t e0 = 0; e0 < n; e0 += 64) {
         uint32_t e = e0 + lane;
@@ -1256,7 +1314,7 @@ __device__ __forceinline__ void warm_body(const Dev& D, const StepK& K, int s, i
         av = vadd(av, wave_tree(dv));
         aw = vadd(aw, wave_tree(dw));
     }
-    if (lane == 0) {
+    if (s && lane == 0) {
         SV(D, s) = f4(vadd(v, av), v4.w);
         SW(D, s) = f4(vadd(w, aw), 0.0f);
     }
@@ -1264,11 +1322,7 @@ __device__ __forceinline__ void warm_body(const Dev& D, const StepK& K, int s, i
 __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
-    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
-        int s = D.bslot[i];
-        if (!s) continue;
-        warm_body(D, K, s, lane, SV(D, s), v3f4(SW(D, s)));
-    }
+    for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) warm_body(D, K, body_in(D, i), lane);
 }
 
 __device__ __forceinline__ float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
@@ -1433,10 +1487,16 @@ __global__ void __launch_bounds__(NT) k_solve_tail(Dev D, StepK K, int c0) {
     }
 }
 
-// overflow (Jacobi with mass splitting): every overflow manifold against copies of its bodies taken before the pass
+// overflow (Jacobi with mass splitting): every overflow manifold against copies of its bodies taken before the pass.
+// Block b works on a slice of its XCD's contiguous eighth of the bucket, as in k_solve_bucket, so an XCD's manifolds
+// lie in a few of the bucket's spatial sub-buckets and share their bodies' records in that XCD's L2.
 __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
-    uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
-    GRID_STRIDE(i, n) {
+    const uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
+    const uint32_t G = min(gridDim.x, max((n + NT - 1) / NT, 1u)), b = blockIdx.x;
+    if (b >= G) return;
+    const uint32_t x = b & 7u, q = G >> 3, r = G & 7u;
+    const uint32_t lb = x * q + min(x, r) + (b >> 3);  // a bijection of [0, G)
+    for (uint32_t i = lb * NT + threadIdx.x; i < n; i += G * NT) {
         uint32_t m = o + i;
         uint2 e2 = reinterpret_cast<const uint2*>(D.blinv)[m];
         int4 hd = D.cur.hdrp[m];
@@ -1458,9 +1518,8 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
 // in the overflow), one wavefront per body reading its list entries' changes in one contiguous run
 // one body's average of its overflow copies (its wavefront): v, w (every lane) take the tree sums of the changes
 // over its contacts (zero for contacts not in the overflow) over the count
-__device__ __forceinline__ void ovf_average(const Dev& D, int s, int lane, uint32_t cnt, V3& v, V3& w) {
+__device__ __forceinline__ void ovf_average(const Dev& D, uint32_t o, uint32_t n, int lane, uint32_t cnt, V3& v, V3& w) {
     V3 av = v3(0, 0, 0), aw = v3(0, 0, 0);
-    uint32_t o = D.bloff[s], n = D.blcnt[s];
     for (uint32_t e0 = 0; e0 < n; e0 += 64) {
         uint32_t e = e0 + lane;
         V3 dvv = v3(0, 0, 0), dww = v3(0, 0, 0);
@@ -1480,14 +1539,11 @@ __global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
-        int s = D.bslot[i];
-        if (!s) continue;
-        uint32_t cnt = D.ovcnt[s];
-        if (!cnt) continue;
-        float4 v4 = SV(D, s);
-        V3 v = v3f4(v4), w = v3f4(SW(D, s));
-        ovf_average(D, s, lane, cnt, v, w);
-        if (lane == 0) { SV(D, s) = f4(v, v4.w); SW(D, s) = f4(w, 0.0f); }
+        const BodyIn b = body_in(D, i);
+        if (!b.cnt) continue;  // (zero for a body that is not awake)
+        V3 v = v3f4(b.v4), w = v3f4(b.w4);
+        ovf_average(D, b.o, b.n, lane, b.cnt, v, w);
+        if (lane == 0) { SV(D, b.s) = f4(v, b.v4.w); SW(D, b.s) = f4(w, 0.0f); }
     }
 }
 // the bias pass's end: the overflow average, then the position integration (k_integrate's update), per body
@@ -1497,18 +1553,17 @@ __device__ __forceinline__ void integrate_slot(const Dev& D, const StepK& K, int
     SQ(D, s) = f4q(qintegrate(qf4(SQ(D, s)), w, K.h));
 }
 __global__ void __launch_bounds__(NT) k_ovf_apply_integrate(Dev D, StepK K) {
-    body_pairs(D, [&](int s, int l, bool half) {
-        const uint32_t cnt = s ? D.ovcnt[s] : 0u;
-        const float4 v4 = SV(D, s);
-        V3 v = v3f4(v4), w = v3f4(SW(D, s));
+    body_pairs(D, [&](const BodyIn& b, int l, bool half) {
+        const int s = b.s;
+        V3 v = v3f4(b.v4), w = v3f4(b.w4);
         V3 ov, ow;
-        list_sums(D, s, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = OVL(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, ov, ow);
+        list_sums(b.o, b.n, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = OVL(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, ov, ow);
         if (!s) return;
-        if (cnt) {
-            float inv = 1.0f / (float)cnt;
+        if (b.cnt) {
+            float inv = 1.0f / (float)b.cnt;
             v = vadd(v, vmul(ov, inv));
             w = vadd(w, vmul(ow, inv));
-            if (l == 0) { SV(D, s) = f4(v, v4.w); SW(D, s) = f4(w, 0.0f); }
+            if (l == 0) { SV(D, s) = f4(v, b.v4.w); SW(D, s) = f4(w, 0.0f); }
         }
         if (l == 0) integrate_slot(D, K, s, v, w);
     });
@@ -1518,16 +1573,16 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_integrate(Dev D, StepK K) {
 // each tree sums its records as ovf_average and warm_body do.
 __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
     const V3 gh = vmul(v3(K.gx, K.gy, K.gz), K.h);
-    body_pairs(D, [&](int s, int l, bool half) {
-        const uint32_t cnt = s ? D.ovcnt[s] : 0u;
-        const float4 v4 = SV(D, s);
-        V3 v = v3f4(v4), w = v3f4(SW(D, s));
+    body_pairs(D, [&](const BodyIn& b, int l, bool half) {
+        const int s = b.s;
+        const float4 v4 = b.v4;
+        V3 v = v3f4(v4), w = v3f4(b.w4);
         V3 ov, ow, av, aw;
-        list_sums(D, s, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = OVL(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, ov, ow);
-        list_sums(D, s, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = WREC(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, av, aw);
+        list_sums(b.o, b.n, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = OVL(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, ov, ow);
+        list_sums(b.o, b.n, l, half, [&](uint32_t e, V3& x, V3& y) { const float4* d = WREC(D, e); x = v3f4(d[0]); y = v3f4(d[1]); }, av, aw);
         if (!s) return;
-        if (cnt) {
-            float inv = 1.0f / (float)cnt;
+        if (b.cnt) {
+            float inv = 1.0f / (float)b.cnt;
             v = vadd(v, vmul(ov, inv));
             w = vadd(w, vmul(ow, inv));
         }
@@ -1832,9 +1887,9 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.mcol, D.mcap); r |= w->alloc(D.mcand, D.mcap); r |= w->alloc(D.mpri, D.mcap);
     r |= w->alloc(D.bmask, n1); r |= w->alloc(D.claim, n1 * 64);
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
-    r |= w->alloc(D.chunkc, 72 * (size_t)((D.mcap + NT - 1) / NT + 1));
+    r |= w->alloc(D.chunkc, NBIN * (size_t)((D.mcap + NT - 1) / NT + 1));
     r |= w->alloc(D.ptpart, 4096);  // k_build's grid: sf_grid(ccap) is at most 4096 workgroups
-    r |= w->alloc(D.ovflag, D.mcap); r |= w->alloc(D.ovoff, D.mcap);
+    r |= w->alloc(D.mbin, D.mcap); r |= w->alloc(D.subcnt, OVS); r |= w->alloc(D.suboff, OVS);
     r |= w->alloc(D.ovcnt, n1);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blinv, 2 * (size_t)D.mcap); r |= w->alloc(D.minv, 2 * (size_t)D.mcap);
     r |= w->alloc(D.parent, nb); r |= w->alloc(D.sst, nb); r |= w->alloc(D.ctr, C_NCTR);
@@ -1922,8 +1977,8 @@ int step_front(HostWorld* w, const StepK& K) {
     }
     w->stats[SF_S_ROUNDS] = rounds;
     hipLaunchKernelGGL(k_col_count, dim3(gm), dim3(NT), 0, s, D, K.K);
-    sf::scan_u32(D.ovflag, D.ovoff, D.ctr + C_NOVF, D.ctr + C_NMAN, 0, D.mcap, w->scan_tmp, s);
-    hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K + 1), dim3(NT), 0, s, D, K.K);
+    hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K + OVS), dim3(NT), 0, s, D, K.K);
+    sf::scan_u32(D.subcnt, D.suboff, D.colcnt + K.K, nullptr, OVS, OVS, w->scan_tmp, s);
     sf::scan_u32(D.colcnt, D.coloff, nullptr, nullptr, (uint32_t)K.K + 1, 72, w->scan_tmp, s);
     hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
     hipLaunchKernelGGL(k_ovf_count, dim3(gm), dim3(NT), 0, s, D, K.K);
