@@ -34,7 +34,7 @@ constexpr int NT = 256;  // threads per block of the grid-stride kernels
 
 // device counters
 // (C_ACTIVE0 + g: set when a manifold claimed in claim-round group g, rounds 4g to 4g + 3)
-enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_NSLOT, C_NBIGC,
+enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_NSLOT, C_NBIGC, C_NBIGL,
        C_ACTIVE0, C_NCTR = C_ACTIVE0 + 8 };
 
 // manifold rows (each row is a float4 array of mcap entries)
@@ -87,6 +87,7 @@ struct Dev {
     uint32_t *epc, *erel;  // per cell entry: partners homed there, offset within the shape's candidate list
     int32_t* ekeep;        // per cell entry: its first ENTRY_KEEP partners, as the count pass found them
     uint32_t* bigc;        // shapes with more than 16 partners this step
+    uint32_t* bigl;        // solver slots whose contact lists are longer than a wavefront this step
     uint32_t *ccnt, *coff;
     int2* cand;
     float4* nar;  // 5 per candidate: n|count, points p|sep
@@ -1137,12 +1138,27 @@ __global__ void __launch_bounds__(NT) k_bl_sort(Dev D) {
     }
 }
 
-// lists longer than a wavefront (the largest bodies): one workgroup each, bitonic sort in LDS
+// the slots whose lists are longer than a wavefront, into bigl (one thread per slot, one atomic per wavefront)
+__global__ void __launch_bounds__(NT) k_bl_big(Dev D) {
+    const int lane = threadIdx.x & 63;
+    for (uint32_t s0 = (blockIdx.x * NT + threadIdx.x) & ~63u; s0 < (uint32_t)D.nb + 1u; s0 += gridDim.x * NT) {
+        const uint32_t s = s0 + lane;
+        const bool big = s < (uint32_t)D.nb + 1u && D.blcnt[s] > 64u;
+        const unsigned long long m = __ballot(big);
+        if (!m) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&D.ctr[C_NBIGL], (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0, 64);
+        if (big) D.bigl[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = s;
+    }
+}
+// lists longer than a wavefront (the largest bodies, listed by k_bl_big): one workgroup each, bitonic sort in LDS
 __global__ void __launch_bounds__(NT) k_bl_sort_big(Dev D) {
     __shared__ uint32_t sh[2048];
-    for (uint32_t s = blockIdx.x; s < (uint32_t)D.nb + 1u; s += gridDim.x) {
+    const uint32_t nbig = min(D.ctr[C_NBIGL], (uint32_t)D.nb + 1u);
+    for (uint32_t t = blockIdx.x; t < nbig; t += gridDim.x) {
+        const uint32_t s = D.bigl[t];
         uint32_t o = D.bloff[s], n = D.blcnt[s];
-        if (n <= 64) continue;
         if (n > 2048) {
             if (threadIdx.x == 0) {
                 for (uint32_t i = 1; i < n; i++) {
@@ -1876,7 +1892,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
     r |= w->alloc(D.tidx, D.ecap); r |= w->alloc(D.soff, D.tcap); r |= w->alloc(D.slist, D.ecap);
     r |= w->alloc(D.epc, D.ecap); r |= w->alloc(D.erel, D.ecap); r |= w->alloc(D.ekeep, (size_t)ENTRY_KEEP * D.ecap);
-    r |= w->alloc(D.bigc, (size_t)ns);
+    r |= w->alloc(D.bigc, (size_t)ns); r |= w->alloc(D.bigl, (size_t)nb + 1);
     r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
     r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
     for (ManBuf* M : {&D.cur, &D.old}) {
@@ -1986,6 +2002,7 @@ int step_front(HostWorld* w, const StepK& K) {
     sf::scan_u32(D.blcnt, D.bloff, nullptr, nullptr, (uint32_t)D.nb + 1, (uint32_t)D.nb + 1, w->scan_tmp, s);
     hipLaunchKernelGGL(k_bl_fill, dim3(gm), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort, dim3(sf_grid(((uint64_t)D.nb + 1) * 64, NT, 65536)), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_bl_big, dim3(sf_grid((uint64_t)D.nb + 1)), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_sort_big, dim3(1024), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_bl_inv, dim3(sf_grid(2ull * D.mcap)), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_build, dim3(gc), dim3(NT), 0, s, D, K, w->nold);
