@@ -161,6 +161,10 @@ int sf_load_topology(sf_load* L, int sid, int32_t* nb6, float* bond4);
 int sf_load_count(sf_load* L);
 /* device time of the last sf_load_solve in ms and the FP32 / FP64 sweep counts summed over structures */
 int sf_load_last_timing(sf_load* L, float* ms, int64_t* sweeps32, int64_t* sweeps64, int64_t* bricks);
+/* FP32 sweeps from packed 16-byte rows (on = 1, the default) or from the full neighbour and conductance arrays
+ * (on = 0); both give the same field bit for bit. No counterpart in the reference: a switch for tests and
+ * measurement. */
+int sf_load_set_packed(sf_load* L, int on);
 
 /* Frame/wall building for load tests: footprint wx x wz bricks, storeys of 7 brick layers, columns every
  * `bay` bricks, slabs, perimeter walls of masonry with glass infill; `remove` columns removed at random.

@@ -65,6 +65,7 @@ _PROTOS = {
     "sf_load_verdicts": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "sf_load_topology": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "sf_load_count": (c_int, [c_void_p]),
+    "sf_load_set_packed": (c_int, [c_void_p, c_int]),
     "sf_load_last_timing": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_scene_building": (c_int, [c_int, c_int, c_int, c_int, c_int, c_uint, c_void_p, c_void_p, c_void_p]),
     "sf_scene_tower": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -67,7 +67,16 @@ struct LoadDev {
     double tol;
     int inner;             // FP32 sweeps between FP64 residual evaluations
     int lds_floats;        // structures of up to this many bricks keep their correction in LDS
+    // Packed sweep rows, used by the structures whose correction is in LDS when pk_codes is set: per brick, x/y/z hold
+    // the six neighbours' indices within the structure as 16-bit halves (0xFFFF: no bond), w the six bonds'
+    // conductances as 5-bit codes into ktab (bits 5d..5d+4) and the brick's grounded flag in bit 30. A sweep then
+    // reads 16 B of stencil per brick instead of 49 (six indices, six conductances, the flag).
+    const uint4* row;
+    const float* ktab;     // the batch's distinct conductances (code 0: 0.0f); PK_CODES entries
+    int pk_codes;          // 1: rows and table are valid
 };
+constexpr int PK_CODES = 32;
+constexpr uint32_t PK_NONE = 0xFFFFu;
 
 __device__ __forceinline__ double block_max(double v, double* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -92,9 +101,37 @@ template <> struct Corr<false> {
     __device__ __forceinline__ float& operator()(size_t g) const { return p[g]; }
 };
 
+// One half-sweep of the packed rows (the correction in LDS, neighbour k at lds[k]): the operations of the unpacked
+// half-sweep below, on the same f32 values, with U bricks per thread whose rows load together.
+__device__ __forceinline__ void half_sweep_packed(const LoadDev& L, int base, int i0, int i1, float om, float* del,
+                                                  const float* ktab) {
+    constexpr int U = 8;
+    for (int i = i0 + threadIdx.x; i < i1; i += LT * U) {
+        uint4 q[U];
+        float rr[U], ss[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ii = i + u * LT;
+            const size_t g = (size_t)base + (ii < i1 ? ii : i);
+            q[u] = L.row[g]; rr[u] = L.r32[g]; ss[u] = L.sk32[g];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int ii = i + u * LT;
+            if (ii >= i1 || !((q[u].w >> 30) & 1u)) continue;
+            const uint32_t rel[6] = {q[u].x & 0xFFFFu, q[u].x >> 16, q[u].y & 0xFFFFu, q[u].y >> 16, q[u].z & 0xFFFFu, q[u].z >> 16};
+            float skp = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 6; d++) if (rel[d] != PK_NONE) skp += ktab[(q[u].w >> (5 * d)) & 31u] * del[rel[d]];
+            float target = (rr[u] + skp) / ss[u];
+            del[ii] += om * (target - del[ii]);
+        }
+    }
+}
+
 // the whole solve of one structure by one workgroup
 template <bool LDSD>
-__device__ void solve_struct(const LoadDev& L, int s, float* lds, double* sh) {
+__device__ void solve_struct(const LoadDev& L, int s, float* lds, double* sh, const float* ktab) {
     const DevInfo I = L.info[s];
     const int base = I.base, n = I.n, nred = I.nred;
     const size_t N = L.N;
@@ -128,6 +165,11 @@ __device__ void solve_struct(const LoadDev& L, int s, float* lds, double* sh) {
         for (int it = 0; it < L.inner && sweeps < I.max_sweeps; it++, sweeps++) {
             for (int half = 0; half < 2; half++) {
                 int i0 = half ? nred : 0, i1 = half ? n : nred;
+                if (LDSD && L.pk_codes) {
+                    half_sweep_packed(L, base, i0, i1, om, lds, ktab);
+                    __syncthreads();
+                    continue;
+                }
                 // U bricks of one colour per thread at a time: their stencils are loaded together, then the
                 // neighbours (all of the other colour, so no brick of this batch is read after it is written)
                 constexpr int U = 4;
@@ -173,11 +215,14 @@ __device__ void solve_struct(const LoadDev& L, int s, float* lds, double* sh) {
 }
 __global__ void __launch_bounds__(LT) k_load_solve(LoadDev L) {
     __shared__ double sh[LT / 64];
+    __shared__ float ktab[PK_CODES];
     extern __shared__ float lds[];
     const int s = blockIdx.x;
     if (s >= L.nstruct) return;
-    if (L.info[s].n <= L.lds_floats) solve_struct<true>(L, s, lds, sh);
-    else solve_struct<false>(L, s, lds, sh);
+    if (L.pk_codes && threadIdx.x < PK_CODES) ktab[threadIdx.x] = L.ktab[threadIdx.x];
+    __syncthreads();
+    if (L.info[s].n <= L.lds_floats) solve_struct<true>(L, s, lds, sh, ktab);
+    else solve_struct<false>(L, s, lds, sh, ktab);
 }
 
 struct Verd { double* ratio; uint8_t* mode; uint8_t* fail; };
@@ -269,6 +314,7 @@ struct HostLoad {
     uint8_t *act = nullptr, *mat = nullptr, *foot = nullptr; float* hp = nullptr; float* gk = nullptr;
     double* phi = nullptr; float* delta = nullptr; float* r32 = nullptr;
     DevInfo* info = nullptr; int* osw = nullptr; double* ores = nullptr;
+    uint4* row = nullptr; float* ktab = nullptr; bool pk = false;  // packed sweep rows (LoadDev::row)
     double* vratio = nullptr; uint8_t *vmode = nullptr, *vfail = nullptr;
     bool verdicts_fresh = false;
     float last_ms = 0.0f;
@@ -280,6 +326,7 @@ struct HostLoad {
     bool pending = false;
     double tol = 0.0;
     int inner = 32;
+    bool use_packed = true;  // sf_load_set_packed
     std::vector<DevInfo> hinfo;
     ~HostLoad() {
         if (pending && e1) (void)hipEventSynchronize(e1);
@@ -309,6 +356,10 @@ int upload(HostLoad* L) {
     std::vector<double> sk64(N, 0.0);
     std::vector<uint8_t> act(N, 0), mat(N, 0), foot(N, 0);
     std::vector<DevInfo> info(L->st.size());
+    std::vector<uint4> row(N, make_uint4(0u, 0u, 0u, 0u));
+    std::vector<float> ktab(1, 0.0f);  // code 0: no bond
+    std::unordered_map<uint32_t, uint32_t> kcode{{0u, 0u}};
+    bool pk = true;
     for (size_t s = 0; s < L->st.size(); s++) {
         Struct& S = L->st[s];
         S.perm.clear(); S.inv.assign(S.n, 0);
@@ -344,6 +395,27 @@ int upload(HostLoad* L) {
             }
             for (int a = 0; a < 3; a++) xmax = std::max(xmax, S.xyz[3 * i + a] + 1);
         }
+        // packed rows of a structure whose indices fit 16 bits, conductances coded in the batch's table
+        if (S.n < (int)PK_NONE) {
+            for (int k = 0; k < S.n; k++) {
+                size_t g = (size_t)S.base + k;
+                uint32_t rel[6], codes = act[g] ? (1u << 30) : 0u;
+                for (int d = 0; d < 6; d++) {
+                    int j = nbr[d * N + g];
+                    rel[d] = j < 0 ? PK_NONE : (uint32_t)(j - S.base);
+                    if (j < 0) continue;
+                    uint32_t bits;
+                    memcpy(&bits, &kc[d * N + g], 4);
+                    auto it = kcode.find(bits);
+                    uint32_t c;
+                    if (it != kcode.end()) c = it->second;
+                    else if (ktab.size() < (size_t)PK_CODES) { c = (uint32_t)ktab.size(); kcode[bits] = c; ktab.push_back(kc[d * N + g]); }
+                    else { pk = false; c = 0; }
+                    codes |= c << (5 * d);
+                }
+                row[g] = make_uint4(rel[0] | rel[1] << 16, rel[2] | rel[3] << 16, rel[4] | rel[5] << 16, codes);
+            }
+        }
         info[s].base = S.base; info[s].n = S.n; info[s].nred = S.nred; info[s].bmax = bmax;
         info[s].omega = S.omega;
         info[s].max_sweeps = 0;
@@ -354,7 +426,12 @@ int upload(HostLoad* L) {
     r |= L->alloc(L->hp, 4 * N); r |= L->alloc(L->gk, N); r |= L->alloc(L->phi, N); r |= L->alloc(L->delta, N);
     r |= L->alloc(L->r32, N); r |= L->alloc(L->info, info.size()); r |= L->alloc(L->osw, info.size());
     r |= L->alloc(L->ores, info.size()); r |= L->alloc(L->vratio, 4 * N); r |= L->alloc(L->vmode, 4 * N); r |= L->alloc(L->vfail, 4 * N);
+    r |= L->alloc(L->row, N); r |= L->alloc(L->ktab, PK_CODES);
     if (r) return r;
+    ktab.resize(PK_CODES, 0.0f);
+    L->pk = pk;
+    SF_HIP(hipMemcpy(L->row, row.data(), sizeof(uint4) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->ktab, ktab.data(), sizeof(float) * PK_CODES, hipMemcpyHostToDevice));
     SF_HIP(hipMemcpy(L->nbr, nbr.data(), sizeof(int32_t) * 6 * N, hipMemcpyHostToDevice));
     SF_HIP(hipMemcpy(L->kc, kc.data(), sizeof(float) * 6 * N, hipMemcpyHostToDevice));
     SF_HIP(hipMemcpy(L->sk32, sk32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
@@ -385,6 +462,12 @@ sf_load* sf_load_create(void) {
 }
 void sf_load_destroy(sf_load* L) { delete L; }
 int sf_load_count(sf_load* L) { return L ? (int)L->h.st.size() : SF_E_ARG; }
+int sf_load_set_packed(sf_load* L, int on) {
+    if (!L) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    if (L->h.pending) { int r = sf_load_solve_end(L); if (r < 0) return r; }
+    L->h.use_packed = on != 0;
+    return 0;
+}
 
 int sf_load_add(sf_load* L, int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, const float* ext) {
     if (!L || n <= 0 || !mat || !xyz || !infill) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
@@ -438,11 +521,13 @@ int sf_load_solve_begin(sf_load* L, float omega, double rel_tol, int max_sweeps)
     D.nstruct = (int)H->st.size(); D.info = H->info; D.nbr = H->nbr; D.kc = H->kc; D.sk32 = H->sk32; D.sk64 = H->sk64;
     D.b = H->b; D.act = H->act; D.phi = H->phi; D.delta = H->delta; D.r32 = H->r32; D.out_sweeps = H->osw;
     D.out_resid = H->ores; D.N = H->N; D.tol = rel_tol; D.inner = H->inner;
+    D.row = H->row; D.ktab = H->ktab; D.pk_codes = H->pk && H->use_packed ? 1 : 0;
     H->tol = rel_tol;
     // dynamic LDS for the corrections: the largest structure that fits the workgroup limit
     int lds_max = 0;
     SF_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, H->dev));
-    int fit = std::max(0, (lds_max - 1024) / (int)sizeof(float)), nmax = 0;
+    // (at most PK_NONE - 1 bricks: the packed rows' 16-bit indices cover every structure with its correction in LDS)
+    int fit = std::min((int)PK_NONE - 1, std::max(0, (lds_max - 1024 - PK_CODES * (int)sizeof(float)) / (int)sizeof(float))), nmax = 0;
     for (const Struct& S : H->st) if (S.n <= fit) nmax = std::max(nmax, S.n);
     D.lds_floats = nmax;
     size_t lds_bytes = sizeof(float) * (size_t)nmax;

@@ -39,6 +39,11 @@ class LoadBatch:
         self.sizes.append(len(mat))
         return sid
 
+    def set_packed(self, on=True):
+        """FP32 sweeps from packed 16-byte rows (default) or from the full neighbour and conductance arrays; the
+        field is the same bit for bit"""
+        check(lib().sf_load_set_packed(self._h, 1 if on else 0))
+
     def solve(self, rel_tol=1e-6, omega=0.0, max_sweeps=200000):
         """FP32 sweeps with FP64 residual and refinement to max|b - A phi| / max|b| <= rel_tol; returns the number
         of structures that converged."""

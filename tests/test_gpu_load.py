@@ -70,3 +70,29 @@ def test_batch_of_buildings_in_one_launch(oracle):
     assert L.solve(rel_tol=1e-6) == len(sts)
     errs = [_check(L, sid, *st, oracle, f"building {sid}") for sid, st in sts]
     print("per-structure field error vs FP64 oracle:", ["%.1e" % e for e in errs])
+
+
+def test_packed_rows_same_field_bit_for_bit():
+    """The sweeps from packed 16-byte rows (16-bit neighbour indices, coded conductances) against the sweeps from
+    the full neighbour and conductance arrays: same sweeps, same FP64 field and verdicts, bit for bit."""
+    print("case: packed vs full rows, 6 buildings 2k-20k bricks, one with external loads", flush=True)
+    rng = np.random.default_rng(11)
+    sts = []
+    for k in range(6):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(20000))))
+        mat, xyz, inf = scenes.building_for(n, seed=300 + k)
+        ext = rng.uniform(-50.0, 400.0, len(mat)).astype(np.float32) if k == 2 else None
+        sts.append((mat, xyz, inf, ext))
+    out = []
+    for packed in (True, False):
+        L = sf.LoadBatch()
+        L.set_packed(packed)
+        sids = [L.add(mat, xyz, inf, ext=ext) for mat, xyz, inf, ext in sts]
+        assert L.solve(rel_tol=1e-6) == len(sts)
+        out.append([(L.field(s), L.info(s), L.verdicts(s)) for s in sids])
+        L.close()
+    for k, (a, b) in enumerate(zip(*out)):
+        assert a[1] == b[1], (k, a[1], b[1])
+        assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)), k
+        for x, y in zip(a[2], b[2]):
+            assert np.array_equal(x, y), k
