@@ -74,6 +74,10 @@ struct LoadDev {
     const uint4* row;
     const float* ktab;     // the batch's distinct conductances (code 0: 0.0f); PK_CODES entries
     int pk_codes;          // 1: rows and table are valid
+    // Block b solves structure order[b]: largest first. A CU holds one structure workgroup when the corrections of
+    // the largest structures fill its LDS, so the biggest solves start in the first round of workgroups and the
+    // small ones fill in behind them instead of a big one starting last.
+    const int32_t* order;
 };
 constexpr int PK_CODES = 32;
 constexpr uint32_t PK_NONE = 0xFFFFu;
@@ -217,8 +221,8 @@ __global__ void __launch_bounds__(LT) k_load_solve(LoadDev L) {
     __shared__ double sh[LT / 64];
     __shared__ float ktab[PK_CODES];
     extern __shared__ float lds[];
-    const int s = blockIdx.x;
-    if (s >= L.nstruct) return;
+    if ((int)blockIdx.x >= L.nstruct) return;
+    const int s = L.order[blockIdx.x];
     if (L.pk_codes && threadIdx.x < PK_CODES) ktab[threadIdx.x] = L.ktab[threadIdx.x];
     __syncthreads();
     if (L.info[s].n <= L.lds_floats) solve_struct<true>(L, s, lds, sh, ktab);
@@ -315,6 +319,7 @@ struct HostLoad {
     double* phi = nullptr; float* delta = nullptr; float* r32 = nullptr;
     DevInfo* info = nullptr; int* osw = nullptr; double* ores = nullptr;
     uint4* row = nullptr; float* ktab = nullptr; bool pk = false;  // packed sweep rows (LoadDev::row)
+    int32_t* order = nullptr;                                       // LoadDev::order
     double* vratio = nullptr; uint8_t *vmode = nullptr, *vfail = nullptr;
     bool verdicts_fresh = false;
     float last_ms = 0.0f;
@@ -426,8 +431,12 @@ int upload(HostLoad* L) {
     r |= L->alloc(L->hp, 4 * N); r |= L->alloc(L->gk, N); r |= L->alloc(L->phi, N); r |= L->alloc(L->delta, N);
     r |= L->alloc(L->r32, N); r |= L->alloc(L->info, info.size()); r |= L->alloc(L->osw, info.size());
     r |= L->alloc(L->ores, info.size()); r |= L->alloc(L->vratio, 4 * N); r |= L->alloc(L->vmode, 4 * N); r |= L->alloc(L->vfail, 4 * N);
-    r |= L->alloc(L->row, N); r |= L->alloc(L->ktab, PK_CODES);
+    r |= L->alloc(L->row, N); r |= L->alloc(L->ktab, PK_CODES); r |= L->alloc(L->order, info.size());
     if (r) return r;
+    std::vector<int32_t> order(L->st.size());
+    for (size_t s = 0; s < order.size(); s++) order[s] = (int32_t)s;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return L->st[a].n > L->st[b].n; });
+    SF_HIP(hipMemcpy(L->order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
     ktab.resize(PK_CODES, 0.0f);
     L->pk = pk;
     SF_HIP(hipMemcpy(L->row, row.data(), sizeof(uint4) * N, hipMemcpyHostToDevice));
@@ -521,7 +530,7 @@ int sf_load_solve_begin(sf_load* L, float omega, double rel_tol, int max_sweeps)
     D.nstruct = (int)H->st.size(); D.info = H->info; D.nbr = H->nbr; D.kc = H->kc; D.sk32 = H->sk32; D.sk64 = H->sk64;
     D.b = H->b; D.act = H->act; D.phi = H->phi; D.delta = H->delta; D.r32 = H->r32; D.out_sweeps = H->osw;
     D.out_resid = H->ores; D.N = H->N; D.tol = rel_tol; D.inner = H->inner;
-    D.row = H->row; D.ktab = H->ktab; D.pk_codes = H->pk && H->use_packed ? 1 : 0;
+    D.row = H->row; D.ktab = H->ktab; D.pk_codes = H->pk && H->use_packed ? 1 : 0; D.order = H->order;
     H->tol = rel_tol;
     // dynamic LDS for the corrections: the largest structure that fits the workgroup limit
     int lds_max = 0;
