@@ -1083,13 +1083,17 @@ __device__ __forceinline__ void block_max2_tail(float& a, float& b, float* sh) {
 constexpr int ST_NT = 1024;
 // the stencil of every brick of the structures that sweep this frame (mode 0: live ones, mode 1: warming ones);
 // links and conductances do not change during the sweeps, so each brick update becomes a gather of six potentials
-__global__ void __launch_bounds__(NT) k_stencil(LD D, int mode) {
+// Structures of at most pk_fit bricks (their potential then sits in the sweep's LDS) get packed rows; their linked
+// neighbours lie in the structure's own index range, as bonds join bricks of one structure only.
+__global__ void __launch_bounds__(NT) k_stencil(LD D, int mode, uint32_t pk_fit) {
     const Tables& T = tables();
     for (uint32_t g = blockIdx.y; g < D.nst; g += gridDim.y) {
         if (D.st_nmin[g] <= 0) continue;
         if (mode == 0 && !D.st_act[g]) continue;
         int sc = D.st_scene[g];
         int sl = (int)g - D.sc[sc].st0;
+        const int lo = D.st_lo[g];
+        const bool pk = (uint32_t)(D.st_hi[g] - lo) <= pk_fit;
         int4 rb = D.st_rb[g];
         uint32_t o = rb.x, n = (uint32_t)(rb.y + rb.w);
         for (uint32_t t = blockIdx.x * NT + threadIdx.x; t < n; t += gridDim.x * NT) {
@@ -1109,10 +1113,19 @@ __global__ void __launch_bounds__(NT) k_stencil(LD D, int mode) {
             }
             int mi = D.mat[i];
             if (ok && D.bxyz[i].y == 0 && D.bond[i].w > 0.0f) sk += GROUNDK * T.stiff[mi];
-            D.sten_na[p] = make_int4(nbr[0], nbr[1], nbr[2], nbr[3]);
-            D.sten_nb[p] = make_int2(nbr[4], nbr[5]);
-            D.sten_ka[p] = make_float4(kk[0], kk[1], kk[2], kk[3]);
-            D.sten_kb[p] = make_float2(kk[4], kk[5]);
+            if (pk) {
+                uint32_t rel[6], w = (uint32_t)mi & 15u;
+                for (int d = 0; d < 6; d++) {
+                    rel[d] = nbr[d] < 0 ? 0xFFFFu : (uint32_t)(nbr[d] - lo);
+                    if (nbr[d] >= 0) w |= ((uint32_t)D.mat[nbr[d]] & 15u) << (4 + 4 * d);
+                }
+                D.sten_pk[p] = make_uint4(rel[0] | rel[1] << 16, rel[2] | rel[3] << 16, rel[4] | rel[5] << 16, w);
+            } else {
+                D.sten_na[p] = make_int4(nbr[0], nbr[1], nbr[2], nbr[3]);
+                D.sten_nb[p] = make_int2(nbr[4], nbr[5]);
+                D.sten_ka[p] = make_float4(kk[0], kk[1], kk[2], kk[3]);
+                D.sten_kb[p] = make_float2(kk[4], kk[5]);
+            }
             D.sten_s[p] = make_float2(ok ? sk : -1.0f, ok ? T.mass[mi] + D.ext[i] : 0.0f);
         }
     }
@@ -1134,13 +1147,59 @@ template <> struct Phi<false> {
 // one red-black sweep over a structure's stencil (sor_brick in order within a colour, src/lib.rs:1005-1026); returns
 // max|change| / max(phi, 1) over its bricks. U stencil rows per thread are loaded together before any is applied:
 // rows of one colour only read bricks of the other colour, so the order of application within a colour is free.
+// The packed rows' half-sweep (LDS potential): the same operations on the same values, each conductance read from
+// ktab[(mat_i * 9 + mat_j) * 2 + vertical].
+__device__ __forceinline__ void half_sweep_pk(const LD& D, uint32_t o, uint32_t n, float om, const Phi<true>& phi,
+                                              const float* ktab, float& md, float& mp) {
+    constexpr int U = 8;
+    for (uint32_t t = threadIdx.x; t < n; t += ST_NT * U) {
+        float2 sr[U];
+        uint4 q[U];
+        uint32_t ii[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint32_t qq = t + u * ST_NT;
+            const bool in = qq < n;
+            uint32_t p = o + (in ? qq : 0u);
+            sr[u] = in ? D.sten_s[p] : make_float2(-1.0f, 0.0f);
+            q[u] = D.sten_pk[p]; ii[u] = D.st_red[p];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (sr[u].x < 0.0f) continue;
+            const uint32_t rel[6] = {q[u].x & 0xFFFFu, q[u].x >> 16, q[u].y & 0xFFFFu, q[u].y >> 16, q[u].z & 0xFFFFu, q[u].z >> 16};
+            const uint32_t w = q[u].w, row = (w & 15u) * 18u;
+            float skp = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 6; d++)
+                if (rel[d] != 0xFFFFu) skp += ktab[row + ((w >> (4 + 4 * d)) & 15u) * 2u + ((d >> 1) == 1 ? 1u : 0u)] * phi.p[rel[d]];
+            const uint32_t k = ii[u] - (uint32_t)phi.lo;
+            float pi = phi.p[k], po = pi;
+            if (sr[u].x > 0.0f) {
+                float target = (sr[u].y + skp) / sr[u].x;
+                float d = om * (target - pi);
+                po = pi + d;
+                phi.p[k] = po;
+                md = fmaxf(md, fabsf(d));
+            } else {
+                md = fmaxf(md, 0.0f);
+            }
+            mp = fmaxf(mp, po);
+        }
+    }
+}
 template <bool LDSP>
-__device__ float sweep_struct(LD& D, uint32_t g, float om, const Phi<LDSP>& phi, float* sh) {
+__device__ float sweep_struct(LD& D, uint32_t g, float om, const Phi<LDSP>& phi, float* sh, const float* ktab) {
     int4 rb = D.st_rb[g];
     float md = 0.0f, mp = 1.0f;
     constexpr int U = 4;
     for (int par = 0; par < 2; par++) {
         uint32_t o = par == 0 ? rb.x : rb.z, n = par == 0 ? rb.y : rb.w;
+        if constexpr (LDSP) {
+            half_sweep_pk(D, o, n, om, phi, ktab, md, mp);
+            if (par == 0) __syncthreads();
+            continue;
+        }
         for (uint32_t t = threadIdx.x; t < n; t += ST_NT * U) {
             float2 sr[U];
             int4 na[U];
@@ -1187,7 +1246,8 @@ __device__ float sweep_struct(LD& D, uint32_t g, float om, const Phi<LDSP>& phi,
 }
 // one structure's sweeps of a frame: adaptive (mode 0, structural :1268-1281) or warm-up (mode 1, warm_step :1068)
 template <bool LDSP>
-__device__ void sor_struct_frame(LD& D, uint32_t g, int mode, const Phi<LDSP>& phi, long long& budget, float* sh) {
+__device__ void sor_struct_frame(LD& D, uint32_t g, int mode, const Phi<LDSP>& phi, long long& budget, float* sh,
+                                 const float* ktab) {
     int nmin = D.st_nmin[g];
     long long size = D.st_hi[g] - D.st_lo[g];
     float resid = D.st_resid[g];
@@ -1198,7 +1258,7 @@ __device__ void sor_struct_frame(LD& D, uint32_t g, int mode, const Phi<LDSP>& p
     if (mode == 0) {
         int k = 0;
         for (;;) {
-            resid = sweep_struct<LDSP>(D, g, 1.85f, phi, sh);
+            resid = sweep_struct<LDSP>(D, g, 1.85f, phi, sh, ktab);
             k++;
             budget -= size;
             if (k >= nmin && (resid < 4e-4f || k >= 90 || budget <= 0)) break;
@@ -1206,7 +1266,7 @@ __device__ void sor_struct_frame(LD& D, uint32_t g, int mode, const Phi<LDSP>& p
     } else {
         int warm = nmin;
         while (warm > 0 && budget > 0) {
-            resid = sweep_struct<LDSP>(D, g, warm > 60 ? 1.9f : 1.0f, phi, sh);
+            resid = sweep_struct<LDSP>(D, g, warm > 60 ? 1.9f : 1.0f, phi, sh, ktab);
             warm--;
             budget -= size;
         }
@@ -1220,7 +1280,17 @@ __device__ void sor_struct_frame(LD& D, uint32_t g, int mode, const Phi<LDSP>& p
 // dynamic LDS hold the potential of a structure of up to that many bricks
 __global__ void __launch_bounds__(ST_NT) k_sor_scene(LD D, int mode, uint32_t lds_cap) {
     __shared__ float sh[2 * ST_NT / 64];
+    __shared__ float ktab[9 * 9 * 2];  // cond_w of two static bricks by (mat_i, mat_j, vertical)
     extern __shared__ float lph[];
+    {
+        const Tables& T = tables();
+        for (int t = threadIdx.x; t < 9 * 9 * 2; t += ST_NT) {
+            const int mi = t / 18, mj = (t / 2) % 9;
+            float k = minf(T.stiff[mi], T.stiff[mj]);
+            ktab[t] = k * ((t & 1) ? 1.0f : 0.3f);
+        }
+        __syncthreads();
+    }
     for (uint32_t sc = blockIdx.x; sc < D.nscene; sc += gridDim.x) {
         const SceneK S = D.sc[sc];
         long long budget = mode == 0 ? 160000 : 120000;
@@ -1230,8 +1300,8 @@ __global__ void __launch_bounds__(ST_NT) k_sor_scene(LD D, int mode, uint32_t ld
             if (mode == 0 && !D.st_act[g]) continue;
             int lo = D.st_lo[g];
             uint32_t size = (uint32_t)(D.st_hi[g] - lo);
-            if (size <= lds_cap) sor_struct_frame<true>(D, g, mode, Phi<true>{lph, lo, size, D.phi}, budget, sh);
-            else sor_struct_frame<false>(D, g, mode, Phi<false>{lph, lo, 0u, D.phi}, budget, sh);
+            if (size <= lds_cap) sor_struct_frame<true>(D, g, mode, Phi<true>{lph, lo, size, D.phi}, budget, sh, ktab);
+            else sor_struct_frame<false>(D, g, mode, Phi<false>{lph, lo, 0u, D.phi}, budget, sh, ktab);
         }
     }
 }
@@ -1946,6 +2016,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.st_on, NS); r |= L->dalloc(D.st_live, NS); r |= L->dalloc(D.st_nmin, NS); r |= L->dalloc(D.st_resid, NS);
     r |= L->dalloc(D.st_maxr, NS); r |= L->dalloc(D.st_flags, NS); r |= L->dalloc(D.st_red, B); r |= L->dalloc(D.st_rb, NS);
     r |= L->dalloc(D.sten_na, B); r |= L->dalloc(D.sten_nb, B); r |= L->dalloc(D.sten_ka, B); r |= L->dalloc(D.sten_kb, B); r |= L->dalloc(D.sten_s, B);
+    r |= L->dalloc(D.sten_pk, B);
     r |= L->dalloc(D.sc_gmax, S); r |= L->dalloc(D.sc_fail, 4 * S);
     r |= L->dalloc(D.dslot, B); r |= L->dalloc(D.dkey, B); r |= L->dalloc(D.hent, B);
     r |= L->dalloc(D.ccnt, B); r |= L->dalloc(D.coff, B);
@@ -2913,13 +2984,18 @@ int substep(Lat* L, float h) {
 }
 
 // the scenes' structure sweeps; the largest structure that fits the workgroup's LDS sizes the potential copy
-void sor_scene_launch(Lat* L, int mode) {
+// the largest structure whose potential fits the sweep workgroup's LDS (beside its own arrays); below 0xFFFF, so
+// the packed rows' 16-bit indices cover it
+uint32_t sor_fit(Lat* L) {
     if (!L->lds_max) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, L->dev) != hipSuccess || v <= 0) v = 65536;
         L->lds_max = v;
     }
-    uint32_t fit = (uint32_t)std::max(0, (L->lds_max - 1024) / (int)sizeof(float)), cap = 0;
+    return (uint32_t)std::min(0xFFFE, std::max(0, (L->lds_max - 1024) / (int)sizeof(float)));
+}
+void sor_scene_launch(Lat* L, int mode) {
+    uint32_t fit = sor_fit(L), cap = 0;
     for (const HStruct& T : L->sts) { uint32_t n = (uint32_t)(T.hi - T.lo); if (n <= fit) cap = std::max(cap, n); }
     size_t bytes = sizeof(float) * (size_t)cap;
     if (bytes > 65536 && !L->lds_attr) { (void)hipFuncSetAttribute((const void*)k_sor_scene, hipFuncAttributeMaxDynamicSharedMemorySize, L->lds_max - 1024); L->lds_attr = true; }
@@ -2979,7 +3055,7 @@ int structural(Lat* L, float dt) {
     LAUNCH(k_rest_sum_mix, gbr, NT, D);
     LAUNCH(k_unsupported, gw, NT, D);
     if (ns) {
-        hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(ns, 4096)), dim3(NT), 0, L->st, D, 0);
+        hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(ns, 4096)), dim3(NT), 0, L->st, D, 0, sor_fit(L));
         sor_scene_launch(L, 0);
     }
     LAUNCH(k_ga, gw, NT, D, dt);
@@ -3232,7 +3308,7 @@ int frame(Lat* L, float dt) {
     for (const HStruct& T : L->sts) if (T.warm > 0) { any_warm = true; break; }
     if (any_warm) {
         if (structures_upload(L, true)) return -1;
-        hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(L->sts.size(), 4096)), dim3(NT), 0, L->st, D, 1);
+        hipLaunchKernelGGL(k_stencil, dim3(64, (unsigned)std::min<size_t>(L->sts.size(), 4096)), dim3(NT), 0, L->st, D, 1, sor_fit(L));
         sor_scene_launch(L, 1);
         std::vector<float> resid(L->sts.size());
         std::vector<int32_t> left(L->sts.size());

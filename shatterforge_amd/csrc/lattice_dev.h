@@ -89,6 +89,11 @@ struct LD {
     float4* sten_ka;
     float2* sten_kb;
     float2* sten_s;            // sk, rhs
+    // Packed rows of the structures whose potential fits the sweep workgroup's LDS (the full rows above serve the
+    // others): x/y/z the six neighbours' indices relative to the structure's first brick as 16-bit halves (0xFFFF:
+    // not linked), w the brick's material in bits 0-3 and neighbour d's in bits 4+4d..7+4d (the conductance is
+    // min(stiff_i, stiff_j) x 1 vertical / 0.3 lateral, rebuilt from a table)
+    uint4* sten_pk;
     uint32_t* sc_c0;           // first contact of each scene this substep
     uint32_t* sc_gmax;         // f32 bits per scene
     uint32_t* sc_fail;         // per scene: joints failed in tension, shear, at the footing (4 slots, [0] unused)
