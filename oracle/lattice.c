@@ -1448,3 +1448,49 @@ int ll_contacts(const LW* w, float* out, int cap) {
     return w->ncs;
 }
 uint32_t ll_rng(const LW* w) { return w->rng; }
+
+/* ------------------------------------------------------------------ test hooks
+ * Pieces of a substep run on their own, so the tests can pin each against what geometry and mechanics fix
+ * (a sphere on a cube face, edge or corner; a single resting contact; the friction box; the pinch limits; a crush's
+ * energy budget; the g - a load of a piece in free fall). None of these is part of a frame. */
+void ll_set_body(LW* w, int b, const float* p3, const float* q4, const float* v3, const float* w3) {
+    if (b < 0 || b >= w->hw || !w->alive[b]) return;
+    if (p3) w->pos[b] = lv(p3[0], p3[1], p3[2]);
+    if (q4) { w->qq[b].x = q4[0]; w->qq[b].y = q4[1]; w->qq[b].z = q4[2]; w->qq[b].w = q4[3]; }
+    if (v3) w->v[b] = lv(v3[0], v3[1], v3[2]);
+    if (w3) w->w[b] = lv(w3[0], w3[1], w3[2]);
+    w->asleep[b] = 0;
+    update_cache(w);
+}
+/* update_cache, build_hash and gen_contacts; returns the contact count (read them with ll_contacts) */
+int ll_probe_contacts(LW* w) { update_cache(w); build_hash(w); gen_contacts(w); return w->ncs; }
+/* prep_contacts for substep length h */
+void ll_probe_prep(LW* w, float h) { w->hsub = h; prep_contacts(w, h); }
+/* `passes` Gauss-Seidel passes over the contacts in generation order (solve_one, src/lib.rs:1579-1601) */
+void ll_probe_solve(LW* w, int passes) {
+    for (int it = 0; it < passes; it++) for (int k = 0; k < w->ncs; k++) if (w->cs[k].on) solve_one(w, &w->cs[k]);
+}
+/* body b's velocity and spin before the solve are set to vpre/wpre, then pinch_check runs (src/lib.rs:1746-1768) */
+void ll_probe_pinch(LW* w, int b, const float* vpre3, const float* wpre3) {
+    if (b < 0 || b >= w->hw) return;
+    w->vpre[b] = lv(vpre3[0], vpre3[1], vpre3[2]);
+    w->wpre[b] = lv(wpre3[0], wpre3[1], wpre3[2]);
+    pinch_check(w);
+}
+/* crush_at from brick `start` with energy e at point p3 (src/lib.rs:1643-1702); returns the bricks crushed */
+uint32_t ll_probe_crush(LW* w, int start, float e, const float* p3, uint32_t cap) {
+    if (start < 0 || start >= w->nb) return 0;
+    return crush_at(w, start, e, lv(p3[0], p3[1], p3[2]), cap);
+}
+/* the body velocity and spin a frame ago (the g - a loading differentiates against them) */
+void ll_set_last(LW* w, int b, const float* v3, const float* w3) {
+    if (b < 0 || b >= w->hw) return;
+    w->vlast[b] = lv(v3[0], v3[1], v3[2]);
+    w->wlast[b] = lv(w3[0], w3[1], w3[2]);
+}
+/* structural(dt) alone: resting loads, g - a, sweeps, joints (src/lib.rs:1228-1365) */
+void ll_probe_structural(LW* w, float dt) { update_cache(w); structural(w, dt); }
+/* per body slot: contacts counted for it in the last contact pass */
+void ll_set_nc(LW* w, int b, uint32_t nc) { if (b >= 0 && b < w->hw) w->nc[b] = nc; }
+/* put a body to sleep where it is (its bricks keep their world positions) */
+void ll_set_asleep(LW* w, int b) { if (b >= 0 && b < w->hw && w->alive[b]) { w->asleep[b] = 1; w->v[b] = lv(0, 0, 0); w->w[b] = lv(0, 0, 0); } }

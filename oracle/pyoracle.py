@@ -195,6 +195,18 @@ def _lattice_protos():
     L.ll_rng.restype = u32
     L.ll_set_field.argtypes = [vp, i32, vp]
     L.ll_failures.argtypes = [vp, vp]
+    L.ll_set_body.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.ll_probe_contacts.argtypes = [vp]
+    L.ll_probe_contacts.restype = i32
+    L.ll_probe_prep.argtypes = [vp, f32]
+    L.ll_probe_solve.argtypes = [vp, i32]
+    L.ll_probe_pinch.argtypes = [vp, i32, vp, vp]
+    L.ll_probe_crush.argtypes = [vp, i32, f32, vp, u32]
+    L.ll_probe_crush.restype = u32
+    L.ll_set_last.argtypes = [vp, i32, vp, vp]
+    L.ll_probe_structural.argtypes = [vp, f32]
+    L.ll_set_nc.argtypes = [vp, i32, u32]
+    L.ll_set_asleep.argtypes = [vp, i32]
     L._ll_done = True
     return L
 
@@ -290,3 +302,40 @@ class LatticeOracle:
         out = np.zeros(4, np.uint32)
         lib().ll_failures(self._h, _p(out))
         return out
+
+    # ---- test hooks: one piece of a substep on its own
+    @staticmethod
+    def _f(v, n):
+        return None if v is None else np.ascontiguousarray(np.asarray(v, np.float32).reshape(n))
+
+    def set_body(self, b, pos=None, q=None, v=None, w=None):
+        a = [self._f(pos, 3), self._f(q, 4), self._f(v, 3), self._f(w, 3)]
+        lib().ll_set_body(self._h, int(b), *(_p(x) for x in a))
+
+    def probe_contacts(self):
+        lib().ll_probe_contacts(self._h)
+        return self.contacts()
+
+    def probe_prep(self, h):
+        lib().ll_probe_prep(self._h, ctypes.c_float(h))
+
+    def probe_solve(self, passes=1):
+        lib().ll_probe_solve(self._h, int(passes))
+
+    def probe_pinch(self, b, vpre, wpre):
+        lib().ll_probe_pinch(self._h, int(b), _p(self._f(vpre, 3)), _p(self._f(wpre, 3)))
+
+    def probe_crush(self, brick, e, p, cap=2500):
+        return lib().ll_probe_crush(self._h, int(brick), ctypes.c_float(e), _p(self._f(p, 3)), int(cap))
+
+    def set_last(self, b, v, w):
+        lib().ll_set_last(self._h, int(b), _p(self._f(v, 3)), _p(self._f(w, 3)))
+
+    def probe_structural(self, dt=1.0 / 60.0):
+        lib().ll_probe_structural(self._h, ctypes.c_float(dt))
+
+    def set_nc(self, b, n):
+        lib().ll_set_nc(self._h, int(b), int(n))
+
+    def set_asleep(self, b):
+        lib().ll_set_asleep(self._h, int(b))

@@ -117,6 +117,17 @@ uint32_t ll_rng(const LW* w);
 void ll_set_field(LW* w, int s, const float* phi);
 /* joints failed under load since creation: compression (crushed), tension, shear, footing */
 void ll_failures(const LW* w, uint32_t* out4);
+/* test hooks: a piece of a substep run on its own (see lattice.c) */
+void ll_set_body(LW* w, int b, const float* p3, const float* q4, const float* v3, const float* w3);
+int ll_probe_contacts(LW* w);
+void ll_probe_prep(LW* w, float h);
+void ll_probe_solve(LW* w, int passes);
+void ll_probe_pinch(LW* w, int b, const float* vpre3, const float* wpre3);
+uint32_t ll_probe_crush(LW* w, int start, float e, const float* p3, uint32_t cap);
+void ll_set_last(LW* w, int b, const float* v3, const float* w3);
+void ll_probe_structural(LW* w, float dt);
+void ll_set_nc(LW* w, int b, uint32_t nc);
+void ll_set_asleep(LW* w, int b);
 
 #ifdef __cplusplus
 }
