@@ -328,15 +328,24 @@ def bench_box(args, sync):
 
 
 def bench_copy(gib=1.0, reps=5):
-    """a device-to-device copy of known size (the library's copy probe): the reference reading for the HBM byte
-    counters (FETCH_SIZE and WRITE_SIZE per byte moved)"""
-    from shatterforge_amd._native import lib, check
+    """a device-to-device copy of known size (torch's copy kernel on two 1 GiB buffers): the reference reading for
+    the HBM byte counters (FETCH_SIZE and WRITE_SIZE per byte moved)"""
+    import torch
     nbytes = int(gib * (1 << 30))
-    ms = np.zeros(1, np.float32)
-    check(lib().sf_probe_copy(nbytes, reps, ms.ctypes.data))
-    res = {"kernel": "k_copy_probe", "bytes_per_copy": nbytes, "copies": reps,
-           "GBs_read_plus_write": 2.0 * nbytes * reps / (float(ms[0]) * 1e-3) / 1e9}
-    log(f"copy probe: {res['GBs_read_plus_write']:.0f} GB/s (read + write)")
+    a = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    res = {"kernel": "torch copy_ (device to device)", "bytes_per_copy": nbytes, "copies": reps,
+           "GBs_read_plus_write": 2.0 * nbytes * reps / (ms * 1e-3) / 1e9}
+    del a, b
+    log(f"copy: {res['GBs_read_plus_write']:.0f} GB/s (read + write)")
     return res
 
 

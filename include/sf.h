@@ -238,12 +238,13 @@ int sf_lat_subphase_ms(sf_lat* L, double* out16);
  * (default 32768), else as colour-by-colour launches over all scenes; 0: always the latter. Both give the same
  * bits (no counterpart in the reference: the solve route is a device choice) */
 int sf_lat_set_fused(sf_lat* L, int max_contacts);
+/* structures of at most max_bricks bricks are swept with their potential in the workgroup's LDS from packed 16-byte
+ * stencil rows; larger ones from the full neighbour and conductance rows with the potential in global memory.
+ * -1 (the default) takes the device's LDS limit (about 40k bricks on gfx950); 0 sends every structure down the
+ * global path. Both give the same bits (no counterpart in the reference: a storage choice, a switch for tests). */
+int sf_lat_set_sor_lds(sf_lat* L, int max_bricks);
 
 
-/* ---------------------------------------------------------------- measurement
- * a device copy of `bytes` (rounded down to 16), `reps` times; ms_out: device time of all copies. The reference
- * reading for HBM byte counters (no counterpart in the reference). */
-int sf_probe_copy(int64_t bytes, int reps, float* ms_out);
 
 #ifdef __cplusplus
 }

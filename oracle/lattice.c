@@ -18,7 +18,9 @@
  * Two orders:
  *   LL_ORDER_SEQ  the reference's orders: contacts solved one after another in generation order (10 Gauss-Seidel
  *                 passes), SOR sweeps in place in brick order, stress judged brick after brick.
- *   LL_ORDER_DEV  the GPU's orders: contacts coloured by claim rounds and solved colour after colour, the rest by
+ *   LL_ORDER_DEV  the GPU's orders: in each of the 10 passes, first every body's static chain (its contacts against
+ *                 static bricks, the ground and sleeping bodies, in generation order, as the reference solves them),
+ *                 then the body-body contacts coloured by claim rounds and solved colour after colour, the rest by
  *                 a Jacobi pass with mass splitting (per-body sums as a 64-lane halving tree in contact order); SOR
  *                 sweeps red-black by lattice parity; stress judged on the field at the start of the pass, crushes
  *                 applied afterwards in brick order.
@@ -33,6 +35,7 @@
 #include "sf_oracle.h"
 
 #define NONE 0xFFFFFFFFu
+#define LL_CHAIN (-2)      /* colour of a contact solved in its body's static chain */
 
 typedef struct {
     int a, b, bo;          /* body a; dynamic partner body or -1; body of the other brick (-1: static lattice or ground) */
@@ -85,6 +88,11 @@ typedef struct LW {
 } LW;
 
 static void* xcalloc(size_t n, size_t s) { void* p = calloc(n ? n : 1, s); if (!p) abort(); return p; }
+/* whether one part of the frame runs in the device's order: all of them for LL_ORDER_DEV, none for LL_ORDER_SEQ,
+ * the flagged ones for LL_ORDER_MIX | flags (mixed orders attribute a difference between the two to its part) */
+static inline int dev_part(const LW* w, int part) {
+    return w->order == LL_ORDER_DEV || ((w->order & LL_ORDER_MIX) && (w->order & part));
+}
 
 /* ------------------------------------------------------------------ creation */
 LW* ll_create(int order, int brick_cap, int body_cap, int ncolour, int rounds, uint32_t seed) {
@@ -547,7 +555,8 @@ static void push_contact(LW* w, int a, int b, int bo, uint32_t ba, uint32_t bb, 
     if (w->ncs >= w->ccap) { w->ccap *= 2; w->cs = realloc(w->cs, sizeof(LC) * (size_t)w->ccap); if (!w->cs) abort(); }
     LC* c = &w->cs[w->ncs++];
     memset(c, 0, sizeof *c);
-    c->a = a; c->b = b; c->bo = bo; c->ba = ba; c->bb = bb; c->p = p; c->n = n; c->depth = depth; c->on = 1; c->col = -1;
+    c->a = a; c->b = b; c->bo = bo; c->ba = ba; c->bb = bb; c->p = p; c->n = n; c->depth = depth; c->on = 1;
+    c->col = b < 0 ? LL_CHAIN : -1;
     w->nc[a]++;
     if (b >= 0) w->nc[b]++;
 }
@@ -746,7 +755,7 @@ static void colour_contacts(LW* w, uint32_t* mask, uint64_t* claim, int* prop) {
         for (int k = 0; k < w->ncs; k++) {
             LC* c = &w->cs[k];
             prop[k] = -1;
-            if (!c->on || c->col >= 0) continue;
+            if (!c->on || c->col != -1) continue;
             uint32_t used = mask[c->a] | (c->b >= 0 ? mask[c->b] : 0u);
             uint32_t fr = ~used & full;
             if (!fr) continue;
@@ -790,7 +799,7 @@ static void build_ovf_lists(LW* w, OvfLists* L) {
     for (int b = 0; b <= nbd; b++) L->off[b] = 0;
     for (int k = 0; k < w->ncs; k++) {
         LC* c = &w->cs[k];
-        if (!c->on || c->col >= 0) continue;
+        if (!c->on || c->col != -1) continue;
         L->off[c->a + 1]++;
         if (c->b >= 0) L->off[c->b + 1]++;
     }
@@ -798,14 +807,14 @@ static void build_ovf_lists(LW* w, OvfLists* L) {
     int* fill = (int*)xcalloc((size_t)nbd + 1, sizeof(int));
     for (int k = 0; k < w->ncs; k++) {
         LC* c = &w->cs[k];
-        if (!c->on || c->col >= 0) continue;
+        if (!c->on || c->col != -1) continue;
         L->ent[L->off[c->a] + fill[c->a]++] = 2 * k;
         if (c->b >= 0) L->ent[L->off[c->b] + fill[c->b]++] = 2 * k + 1;
     }
     free(fill);
     for (int k = 0; k < w->ncs; k++) {
         LC* c = &w->cs[k];
-        if (!c->on || c->col >= 0) continue;
+        if (!c->on || c->col != -1) continue;
         float na = (float)L->novf[c->a], nb = c->b >= 0 ? (float)L->novf[c->b] : 0.0f;
         c->smn = eff_mass_split(w, c->a, na, c->b, nb, c->ra, c->rb, c->n);
         c->smt1 = eff_mass_split(w, c->a, na, c->b, nb, c->ra, c->rb, c->t1);
@@ -818,7 +827,7 @@ static void build_ovf_lists(LW* w, OvfLists* L) {
 static void overflow_pass(LW* w, OvfLists* L) {
     for (int k = 0; k < w->ncs; k++) {
         LC* c = &w->cs[k];
-        if (!c->on || c->col >= 0) continue;
+        if (!c->on || c->col != -1) continue;
         int a = c->a, b = c->b;
         float na = (float)L->novf[a], nb = b >= 0 ? (float)L->novf[b] : 0.0f;
         LV va = w->v[a], wa = w->w[a], vb = lv(0, 0, 0), wb = lv(0, 0, 0);
@@ -866,7 +875,7 @@ static void overflow_pass(LW* w, OvfLists* L) {
 }
 
 static void solve_contacts(LW* w) {
-    if (w->order == LL_ORDER_SEQ) {
+    if (!dev_part(w, LL_MIX_SOLVE)) {
         for (int it = 0; it < 10; it++) for (int k = 0; k < w->ncs; k++) if (w->cs[k].on) solve_one(w, &w->cs[k]);
         return;
     }
@@ -883,6 +892,9 @@ static void solve_contacts(LW* w) {
     int any_ovf = 0;
     for (int b = 0; b < w->hw; b++) if (L.novf[b]) { any_ovf = 1; break; }
     for (int it = 0; it < 10; it++) {
+        /* static chains: each body's contacts against static bricks, the ground and sleeping bodies, in generation
+         * order (bodies do not meet here, so this loop equals one chain per body run side by side) */
+        for (int k = 0; k < w->ncs; k++) if (w->cs[k].on && w->cs[k].col == LL_CHAIN) solve_one(w, &w->cs[k]);
         for (int col = 0; col < w->K; col++)
             for (int k = 0; k < w->ncs; k++) if (w->cs[k].on && w->cs[k].col == col) solve_one(w, &w->cs[k]);
         if (any_ovf) overflow_pass(w, &L);
@@ -1061,7 +1073,7 @@ static float sor_brick(LW* w, int i, float omega) {
 /* one sweep over a list of bricks; returns max |update| / max(1, max phi) */
 static float sweep_list(LW* w, const uint32_t* list, int n, float omega) {
     float md = 0.0f, mp = 1.0f;
-    if (w->order == LL_ORDER_SEQ) {
+    if (!dev_part(w, LL_MIX_SWEEP)) {
         for (int t = 0; t < n; t++) {
             int i = (int)list[t];
             float d = sor_brick(w, i, omega);
@@ -1161,7 +1173,7 @@ static float stress_brick(LW* w, int i, float dt, int live, int* ev_crush) {
  * crushes are applied afterwards in list order */
 static float stress_list(LW* w, const uint32_t* list, int n, float dt, int live) {
     float mr = 0.0f;
-    if (w->order == LL_ORDER_SEQ) {
+    if (!dev_part(w, LL_MIX_STRESS)) {
         for (int t = 0; t < n; t++) { float r = stress_brick(w, (int)list[t], dt, live, NULL); if (r > mr) mr = r; }
         return mr;
     }

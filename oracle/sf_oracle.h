@@ -86,6 +86,9 @@ void ol_finalize(int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* i
  * engine's physics frame (src/lib.rs), in the reference's sequential orders or the GPU's orders. */
 typedef struct LW LW;
 enum { LL_ORDER_SEQ = 0, LL_ORDER_DEV = 1 };
+/* mixed orders (test use): LL_ORDER_MIX | any of the part flags runs those parts in the device's order and the rest
+ * in the reference's: the contact solve, the SOR sweeps, the joint judgement */
+enum { LL_ORDER_MIX = 0x10, LL_MIX_SOLVE = 0x1, LL_MIX_SWEEP = 0x2, LL_MIX_STRESS = 0x4 };
 #define LL_BODY_F 22
 #define LL_BRICK_F 13
 #define LL_CONTACT_F 18

@@ -93,8 +93,8 @@ _PROTOS = {
     "sf_lat_timing": (c_int, [c_void_p, c_void_p, c_void_p]),
     "sf_lat_phase_ms": (c_int, [c_void_p, c_void_p]),
     "sf_lat_set_fused": (c_int, [c_void_p, c_int]),
+    "sf_lat_set_sor_lds": (c_int, [c_void_p, c_int]),
     "sf_lat_subphase_ms": (c_int, [c_void_p, c_void_p]),
-    "sf_probe_copy": (c_int, [ctypes.c_int64, c_int, c_void_p]),
 }
 
 _lib = None

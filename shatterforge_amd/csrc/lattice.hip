@@ -198,7 +198,8 @@ __device__ __forceinline__ int static_brick(const LD& D, const SceneK& S, int x,
 // stamps, wake requests, per-body contact counts).
 template <bool WRITE>
 __global__ void __launch_bounds__(NT) k_contacts(LD D) {
-    if (!WRITE) GS(ub, D.nub) D.nc[D.ubody[ub]] = 0u;  // the write pass counts contacts per body from zero
+    // the write pass counts contacts per body from zero and records each body's contact range
+    if (!WRITE) GS(ub, D.nub) { const uint32_t b = D.ubody[ub]; D.nc[b] = 0u; D.cfirst[b] = 0xFFFFFFFFu; D.cend[b] = 0u; }
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     for (uint32_t k = wave; k < D.ndyn; k += nwaves) {
@@ -282,13 +283,14 @@ __global__ void __launch_bounds__(NT) k_contacts(LD D) {
             if (lane == 63) D.ccnt[k] = x;
             continue;
         }
+        if (lane == 63 && x > 0) { atomicMin(&D.cfirst[a], D.coff[k]); atomicMax(&D.cend[a], D.coff[k] + x); }
         uint32_t out = D.coff[k] + x - n, m = 0;
         probe([&](int b, int bo, uint32_t bb, LV cp, LV nn, float depth) {
             uint32_t c = out + m;
             if (c < D.ccap) {
                 D.ca[c] = a; D.cb[c] = b; D.cbo[c] = bo; D.cba[c] = i; D.cbb[c] = bb; D.cscene[c] = sc;
                 D.cp[c] = f4(cp); D.cn[c] = f4(nn); D.cdepth[c] = depth;
-                D.con[c] = 1; D.ccol[c] = -1; D.cjn[c] = 0.0f; D.cjt1[c] = 0.0f; D.cjt2[c] = 0.0f; D.cvn0[c] = 0.0f;
+                D.con[c] = 1; D.ccol[c] = b < 0 ? CHAIN : -1; D.cjn[c] = 0.0f; D.cjt1[c] = 0.0f; D.cjt2[c] = 0.0f; D.cvn0[c] = 0.0f;
             }
             atomicAdd(&D.nc[a], 1u);
             if (b >= 0) atomicAdd(&D.nc[b], 1u);
@@ -388,7 +390,7 @@ __global__ void __launch_bounds__(NT) k_col_claim(LD D, int r) {
     const uint32_t full = D.K >= 32 ? 0xFFFFFFFFu : ((1u << D.K) - 1u);
     GS(c, ncs) {
         D.cprop[c] = -1;
-        if (!D.con[c] || D.ccol[c] >= 0) continue;
+        if (!D.con[c] || D.ccol[c] != -1) continue;
         int a = D.ca[c], b = D.cb[c];
         uint32_t used = D.mask[a] | (b >= 0 ? D.mask[b] : 0u);
         uint32_t fr = ~used & full;
@@ -420,7 +422,7 @@ __global__ void __launch_bounds__(NT) k_col_count(LD D) {
     uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
     for (uint32_t t = threadIdx.x; t <= D.K; t += NT) h[t] = 0;
     __syncthreads();
-    GS(c, ncs) if (D.con[c]) atomicAdd(&h[D.ccol[c] < 0 ? D.K : (uint32_t)D.ccol[c]], 1u);
+    GS(c, ncs) if (D.con[c] && D.ccol[c] != CHAIN) atomicAdd(&h[D.ccol[c] < 0 ? D.K : (uint32_t)D.ccol[c]], 1u);
     __syncthreads();
     for (uint32_t t = threadIdx.x; t <= D.K; t += NT) if (h[t]) atomicAdd(&D.ctr[C_COLCNT + t], h[t]);
 }
@@ -437,7 +439,7 @@ __global__ void __launch_bounds__(NT) k_col_fill(LD D) {
         for (uint32_t t = threadIdx.x; t <= D.K; t += NT) cnt[t] = 0;
         __syncthreads();
         uint32_t c = c0 + threadIdx.x;
-        bool ok = c < ncs && D.con[c];
+        bool ok = c < ncs && D.con[c] && D.ccol[c] != CHAIN;
         uint32_t bin = 0, r = 0;
         if (ok) { bin = D.ccol[c] < 0 ? D.K : (uint32_t)D.ccol[c]; r = atomicAdd(&cnt[bin], 1u); }
         __syncthreads();
@@ -512,6 +514,55 @@ __device__ __forceinline__ void solve_contact(LD& D, uint32_t c) {
 __global__ void __launch_bounds__(NT) k_solve_colour(LD D, int col) {
     uint32_t n = D.ctr[C_COLCNT + col], o = D.coloff[col];
     GS(t, n) solve_contact(D, D.collist[o + t]);
+}
+// A body's static chain: its contacts against static bricks, the ground and sleeping bodies, in generation order
+// (the reference's order for them, src/lib.rs:1580), solved one after another by one lane. They touch no other
+// moving body, so every body's chain runs beside the others. A body's contacts (as side a) are the contiguous run
+// its bricks' slots wrote, [cfirst, cend) (recorded by the write pass; a body made after it, by a plough, has none:
+// the body check keeps a stale range from reaching another body's contacts).
+__device__ __forceinline__ void chain_range(const LD& D, uint32_t b, uint32_t& c0, uint32_t& c1) {
+    c0 = D.cfirst[b];
+    c1 = D.cend[b];
+}
+// solve_contact with b < 0 on a body held in registers (the same operations in the same order)
+__device__ __forceinline__ void chain_contact(LD& D, uint32_t c, LV& va, LV& wa, const LM& ia, float ima) {
+    const LV ra = lv4(D.cra[c]), nn = lv4(D.cn[c]), t1 = lv4(D.ct1[c]), t2 = lv4(D.ct2[c]);
+    const LV zero = lv(0.0f, 0.0f, 0.0f);
+    LV dv = sub(add(va, cross(wa, ra)), zero);
+    float vn = dot(dv, nn);
+    float lam = D.cmn[c] * (-vn + D.cbias[c]);
+    float old = D.cjn[c];
+    float jn = maxf(old + lam, 0.0f);
+    float d = jn - old;
+    LV p = mul(nn, d);
+    va = madd(va, p, ima); wa = add(wa, mv(ia, cross(ra, p)));
+    float lim = MU * jn;
+    dv = sub(add(va, cross(wa, ra)), zero);
+    float l1 = -D.cmt1[c] * dot(dv, t1);
+    float o1 = D.cjt1[c], j1 = clampf(o1 + l1, -lim, lim);
+    float l2 = -D.cmt2[c] * dot(dv, t2);
+    float o2 = D.cjt2[c], j2 = clampf(o2 + l2, -lim, lim);
+    p = add(mul(t1, j1 - o1), mul(t2, j2 - o2));
+    va = madd(va, p, ima); wa = add(wa, mv(ia, cross(ra, p)));
+    D.cjn[c] = jn; D.cjt1[c] = j1; D.cjt2[c] = j2;
+}
+// one body's chain on its state in registers; false when the body has no chain contact (its state untouched)
+__device__ __forceinline__ bool chain_body(LD& D, uint32_t b, LV& v, LV& w, const LM& ia, float ima) {
+    uint32_t c0, c1;
+    chain_range(D, b, c0, c1);
+    bool any = false;
+    for (uint32_t c = c0; c < c1; c++)
+        if (D.con[c] && D.ccol[c] == CHAIN && D.ca[c] == (int)b) { chain_contact(D, c, v, w, ia, ima); any = true; }
+    return any;
+}
+__global__ void __launch_bounds__(NT) k_chain(LD D) {
+    GS(ub, D.nub) {
+        const uint32_t b = D.ubody[ub];
+        if (D.cfirst[b] >= D.cend[b]) continue;
+        LV v = lv4(D.v[b]), w = lv4(D.w[b]);
+        const LM ia = load_m(D.iiw + 3 * (size_t)b);
+        if (chain_body(D, b, v, w, ia, D.im[b])) { D.v[b] = f4(v); D.w[b] = f4(w); }
+    }
 }
 // one overflow contact, Jacobi: private copies of its bodies with inverse masses scaled by their overflow counts
 // na, nb; the velocity changes go to odv/odw for the per-body sums
@@ -716,7 +767,7 @@ __device__ __forceinline__ void lbody_ovf(const LD& D, const LBody& B, uint32_t 
 
 __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list, uint32_t lds_cap) {
     __shared__ uint32_t cnt[KMAX + 1], off[KMAX + 2], fil[KMAX + 1], wsum[FS_NT / 64 + 1], nov;
-    __shared__ uint32_t ps_off[KMAX + 2], ps_cnt[KMAX + 2], ps_type[KMAX + 2], npass;
+    __shared__ uint32_t ps_off[KMAX + 3], ps_cnt[KMAX + 3], ps_type[KMAX + 3], npass, nchain;
     extern __shared__ float4 lbody[];
     const int s = list[blockIdx.x];
     const SceneK S = D.sc[s];
@@ -731,13 +782,13 @@ __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list
     for (uint32_t lb = tid; lb < hw; lb += FS_NT) { uint32_t b = bd0 + lb; D.mask[b] = 0u; D.ovcnt[b] = 0u; D.ovfill[b] = 0u; }
     for (uint32_t x = tid; x < hw * K; x += FS_NT) D.claim[(size_t)bd0 * K + x] = 0ull;
     if (tid <= K) { cnt[tid] = 0u; fil[tid] = 0u; }
-    if (tid == 0) nov = 0u;
+    if (tid == 0) { nov = 0u; nchain = 0u; }
     __syncthreads();
     // colouring: claim rounds (k_col_claim, k_col_resolve)
     for (int r = 0; r < (int)D.R; r++) {
         for (uint32_t c = c0 + tid; c < c1; c += FS_NT) {
             int prop = -1;
-            if (D.con[c] && D.ccol[c] < 0) {
+            if (D.con[c] && D.ccol[c] == -1) {
                 int a = D.ca[c], b = D.cb[c];
                 uint32_t used = ld_dev(&D.mask[a]) | (b >= 0 ? ld_dev(&D.mask[b]) : 0u);
                 uint32_t fr = ~used & full;
@@ -766,12 +817,16 @@ __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list
         __syncthreads();
     }
     // colour buckets in the scene's own stretch of collist, overflow last
-    for (uint32_t c = c0 + tid; c < c1; c += FS_NT) if (D.con[c]) atomicAdd(&cnt[D.ccol[c] < 0 ? K : (uint32_t)D.ccol[c]], 1u);
+    for (uint32_t c = c0 + tid; c < c1; c += FS_NT) {
+        if (!D.con[c]) continue;
+        if (D.ccol[c] == CHAIN) nchain = 1u;
+        else atomicAdd(&cnt[D.ccol[c] < 0 ? K : (uint32_t)D.ccol[c]], 1u);
+    }
     __syncthreads();
     if (tid == 0) { uint32_t o = 0; for (uint32_t t = 0; t <= K; t++) { off[t] = o; o += cnt[t]; } off[K + 1] = o; }
     __syncthreads();
     for (uint32_t c = c0 + tid; c < c1; c += FS_NT) {
-        if (!D.con[c]) continue;
+        if (!D.con[c] || D.ccol[c] == CHAIN) continue;
         uint32_t bin = D.ccol[c] < 0 ? K : (uint32_t)D.ccol[c];
         D.collist[c0 + off[bin] + atomicAdd(&fil[bin], 1u)] = c;
     }
@@ -853,6 +908,7 @@ __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list
         }
         if (tid == 0) {
             uint32_t np = 0;
+            if (nchain) { ps_type[np] = 3; ps_off[np] = 0; ps_cnt[np] = hw; np++; }
             for (uint32_t col = 0; col < K; col++) if (cnt[col]) { ps_type[np] = 0; ps_off[np] = c0 + off[col]; ps_cnt[np] = cnt[col]; np++; }
             if (no) { ps_type[np] = 1; ps_off[np] = oo; ps_cnt[np] = no; np++; ps_type[np] = 2; ps_off[np] = 0; ps_cnt[np] = nov; np++; }
             npass = np;
@@ -862,13 +918,13 @@ __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list
         const uint32_t np = npass, total = 10 * np;
         SRec cur, nxt;
         bool has = false, hasn = false;
-        if (np && ps_type[0] != 2 && tid < ps_cnt[0]) { cur = srec_load(D, ps_off[0] + tid); has = true; }
+        if (np && ps_type[0] < 2 && tid < ps_cnt[0]) { cur = srec_load(D, ps_off[0] + tid); has = true; }
         for (uint32_t x = 0; x < total; x++) {
             const uint32_t ps = x % np, ty = ps_type[ps], po = ps_off[ps], pn = ps_cnt[ps];
             hasn = false;
             if (x + 1 < total && np > 1) {  // (with one pass, its records are the ones being updated: kept in cur)
                 uint32_t q = (x + 1) % np;
-                if (ps_type[q] != 2 && tid < ps_cnt[q]) { nxt = srec_load(D, ps_off[q] + tid); hasn = true; }
+                if (ps_type[q] < 2 && tid < ps_cnt[q]) { nxt = srec_load(D, ps_off[q] + tid); hasn = true; }
             }
             if (ty == 0) {
                 if (has) srec_solve(D, B, cur, po + tid);
@@ -876,10 +932,16 @@ __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list
             } else if (ty == 1) {
                 if (has) srec_ovf(D, B, cur, po + tid);
                 for (uint32_t t = tid + FS_NT; t < pn; t += FS_NT) { SRec r = srec_load(D, po + t); srec_ovf(D, B, r, po + t); }
-            } else {
+            } else if (ty == 2) {
                 for (uint32_t k = tid >> 6; k < pn; k += FS_NT / 64) {
                     uint32_t b = D.ovbody[bd0 + k];
                     lbody_ovf(D, B, b - bd0, b, ld_dev(&D.ovcnt[b]), lane);
+                }
+            } else {
+                // the static chains: a lane per body, its state in registers for the whole chain
+                for (uint32_t lb = tid; lb < pn; lb += FS_NT) {
+                    LV v = lv4(B.v[lb]), w = lv4(B.w[lb]);
+                    if (chain_body(D, bd0 + lb, v, w, B.iiw(lb), B.im[lb])) { B.v[lb] = f4(v); B.w[lb] = f4(w); }
                 }
             }
             __syncthreads();
@@ -894,8 +956,16 @@ __global__ void __launch_bounds__(FS_NT) k_scene_solve(LD D, const int32_t* list
         for (uint32_t lb = tid; lb < hw; lb += FS_NT) { D.v[bd0 + lb] = B.v[lb]; D.w[bd0 + lb] = B.w[lb]; }
         return;
     }
-    // 10 iterations: colour after colour, then the overflow pass
+    // 10 iterations: the static chains, colour after colour, then the overflow pass
     for (int it = 0; it < 10; it++) {
+        if (nchain) {
+            for (uint32_t lb = tid; lb < hw; lb += FS_NT) {
+                const uint32_t b = bd0 + lb;
+                LV v = lv4(D.v[b]), w = lv4(D.w[b]);
+                if (chain_body(D, b, v, w, load_m(D.iiw + 3 * (size_t)b), D.im[b])) { D.v[b] = f4(v); D.w[b] = f4(w); }
+            }
+            __syncthreads();
+        }
         for (uint32_t col = 0; col < K; col++) {
             const uint32_t n = cnt[col], o = c0 + off[col];
             if (!n) continue;
@@ -1894,6 +1964,7 @@ struct Lat {
     int K = 16, R = 8;
     int fuse_max = 32768;
     int lds_max = 0;           // LDS bytes a workgroup may have (device attribute)
+    int64_t sor_lds = -1;      // caller's limit on the structures swept with their potential in LDS (-1: none)
     bool lds_attr = false, lds_attr_fs = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
     std::vector<int32_t> slist, slist_up;  // scenes with contacts this substep; the list on the device
     Stage stg_ctl, stg_csr, stg_delta, stg_brec;  // uploads: sweep controls, dynamic brick lists, brick and body deltas
@@ -2009,7 +2080,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.rot, 3 * N); r |= L->dalloc(D.iil, 3 * N); r |= L->dalloc(D.iiw, 3 * N);
     r |= L->dalloc(D.m, N); r |= L->dalloc(D.im, N); r |= L->dalloc(D.sleep, N); r |= L->dalloc(D.rad, N);
     r |= L->dalloc(D.cool, N); r |= L->dalloc(D.lt, N); r |= L->dalloc(D.resid, N); r |= L->dalloc(D.maxr, N);
-    r |= L->dalloc(D.nmax, N); r |= L->dalloc(D.nc, N); r |= L->dalloc(D.count, N); r |= L->dalloc(D.boff, N);
+    r |= L->dalloc(D.nmax, N); r |= L->dalloc(D.nc, N); r |= L->dalloc(D.cfirst, N); r |= L->dalloc(D.cend, N); r |= L->dalloc(D.count, N); r |= L->dalloc(D.boff, N);
     r |= L->dalloc(D.bcnt, N); r |= L->dalloc(D.ubody, N);
     r |= L->dalloc(D.dyn, B); r |= L->dalloc(D.dyn_body, B);
     r |= L->dalloc(D.st_scene, NS); r |= L->dalloc(D.st_lo, NS); r |= L->dalloc(D.st_hi, NS); r |= L->dalloc(D.st_act, NS);
@@ -2966,6 +3037,7 @@ int substep(Lat* L, float h) {
         if (read_ctr(L)) return -1;
         const uint32_t* cc = L->h_ctr + C_COLCNT;
         for (int it = 0; it < 10; it++) {
+            LAUNCH(k_chain, gb, NT, D);
             for (int c = 0; c < L->K; c++) if (cc[c]) LAUNCH(k_solve_colour, sf_grid(cc[c]), NT, D, c);
             if (cc[L->K]) {
                 LAUNCH(k_ovf_solve, sf_grid(cc[L->K]), NT, D);
@@ -2992,7 +3064,8 @@ uint32_t sor_fit(Lat* L) {
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, L->dev) != hipSuccess || v <= 0) v = 65536;
         L->lds_max = v;
     }
-    return (uint32_t)std::min(0xFFFE, std::max(0, (L->lds_max - 1024) / (int)sizeof(float)));
+    uint32_t fit = (uint32_t)std::min(0xFFFE, std::max(0, (L->lds_max - 1024) / (int)sizeof(float)));
+    return L->sor_lds >= 0 ? (uint32_t)std::min<int64_t>(fit, L->sor_lds) : fit;
 }
 void sor_scene_launch(Lat* L, int mode) {
     uint32_t fit = sor_fit(L), cap = 0;
@@ -3378,11 +3451,18 @@ int finalize(Lat* L, int si) {
     size_t gsz = (size_t)S.gnx * S.gny * S.gnz;
     if (gsz > S.gcap) { S.goff = L->grid.size(); S.gcap = gsz; L->grid.resize(S.goff + gsz, 0); }
     std::fill(L->grid.begin() + S.goff, L->grid.begin() + S.goff + S.gcap, 0);
+    // one brick per lattice cell: the neighbour table, the bonds and the sweeps' packed rows (16-bit indices within
+    // a structure) all rest on it
     for (int i = S.b0; i < S.b0 + S.nb; i++) {
         if (!L->mat[i]) continue;
         int4 c = L->bxyz[i];
         size_t idx = S.goff + ((size_t)(c.y - S.gy0) * S.gnz + (c.z - S.gz0)) * S.gnx + (c.x - S.gx0);
-        if (!L->grid[idx]) L->grid[idx] = i + 1;
+        if (L->grid[idx]) {
+            sf::set_error(SF_E_ARG, "sf_lat_finalize: two bricks in lattice cell (" + std::to_string(c.x) + ", " +
+                                        std::to_string(c.y) + ", " + std::to_string(c.z) + ")");
+            return SF_E_ARG;
+        }
+        L->grid[idx] = i + 1;
     }
     for (int i = S.b0; i < S.b0 + S.nb; i++) finalize_brick(L, S, i);
     int red_at = S.b0;
@@ -3652,7 +3732,8 @@ int sf_lat_add_body(sf_lat* h, int scene, int n, const int32_t* xyz, const uint8
 }
 int sf_lat_finalize(sf_lat* h, int scene) {
     if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
-    return finalize(&h->h.L, scene) ? SF_E_HIP : 0;
+    int r = finalize(&h->h.L, scene);
+    return r == SF_E_ARG ? SF_E_ARG : (r ? SF_E_HIP : 0);
 }
 int sf_lat_warm_all(sf_lat* h, int scene) {
     if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
@@ -3844,6 +3925,11 @@ int sf_lat_subphase_ms(sf_lat* h, double* out16) {
 int sf_lat_set_fused(sf_lat* h, int max_contacts) {
     if (!h || max_contacts < 0) return SF_E_ARG;
     h->h.L.fuse_max = max_contacts;
+    return 0;
+}
+int sf_lat_set_sor_lds(sf_lat* h, int max_bricks) {
+    if (!h || max_bricks < -1) { sf::set_error(SF_E_ARG, "sf_lat_set_sor_lds: max_bricks must be >= -1"); return SF_E_ARG; }
+    h->h.L.sor_lds = max_bricks;
     return 0;
 }
 

@@ -7,6 +7,8 @@ namespace lat {
 
 constexpr int NT = 256;
 constexpr int KMAX = 32;  // colour limit
+constexpr int CHAIN = -2;  // colour of a contact solved in its body's static chain (against static bricks, the ground
+                           // or a sleeping body; generation order, one lane per body)
 
 // device counters
 enum : int {
@@ -65,6 +67,7 @@ struct LD {
     float4 *rot, *iil, *iiw;   // three rows each
     float *m, *im, *sleep, *rad, *cool, *lt, *resid, *maxr, *nmax;
     uint32_t *nc, *count, *boff, *bcnt;
+    uint32_t *cfirst, *cend;   // a body's contacts as side a in the current substep: [cfirst, cend), contiguous
     uint32_t* dyn;             // dynamic slot -> brick (body-major, brick-list order)
     int32_t* dyn_body;
     // scenes and structures

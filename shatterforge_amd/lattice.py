@@ -144,6 +144,11 @@ class Lattice:
         """scenes of at most max_contacts contacts are solved by one workgroup each; 0: colour-by-colour launches"""
         check(lib().sf_lat_set_fused(self._h, int(max_contacts)))
 
+    def set_sor_lds(self, max_bricks):
+        """structures of at most max_bricks bricks are swept with their potential in LDS (packed rows); -1: the
+        device's limit, 0: every structure from the full rows with the potential in global memory"""
+        check(lib().sf_lat_set_sor_lds(self._h, int(max_bricks)))
+
     def timing(self):
         ms = np.zeros(1, np.float64)
         syncs = np.zeros(1, np.int32)

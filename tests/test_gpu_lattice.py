@@ -32,12 +32,13 @@ def _assert_same(G, s, O, tag):
     assert G.rng(s) == O.rng(), f"{tag}: random stream"
 
 
-def _run(oracle, case, frames, check_every=1, fused=True):
+def _run(oracle, case, frames, check_every=1, fused=True, sor_lds=-1):
     nb = lc.brick_count(case) + 8
     nd = 4096
     G = Lattice(nb, nd, scenes=1)
     if not fused:
         G.set_fused(0)
+    G.set_sor_lds(sor_lds)
     s = G.add_scene(nb, nd, structs=4)
     O = oracle.LatticeOracle(order=oracle.ORDER_DEV, brick_cap=nb, body_cap=nd)
     lc.build(G, s, case)
@@ -72,10 +73,27 @@ def test_tower_blast_bit_exact(oracle, route):
     assert O.stats()[2] > 0
 
 
-def test_building_collapse_bit_exact(oracle):
-    print("case: 2500-brick frame building, 4 charges r=3, 150 frames", flush=True)
-    G, s, O = _run(oracle, dict(kind="building", n=2500, seed=4, nch=4, r=3.0, p=1.4), 150, check_every=5)
+@pytest.mark.parametrize("sweeps", ["potential in LDS", "potential in global memory"])
+def test_building_collapse_bit_exact(oracle, sweeps):
+    # the structure sweeps from packed rows with the potential in LDS (structures up to the LDS limit) and from the
+    # full rows with the potential in global memory (larger structures; forced here): both equal the oracle
+    print(f"case: 2500-brick frame building, 4 charges r=3, 150 frames, sweeps with the {sweeps}", flush=True)
+    G, s, O = _run(oracle, dict(kind="building", n=2500, seed=4, nch=4, r=3.0, p=1.4), 150, check_every=5,
+                   sor_lds=-1 if sweeps.endswith("LDS") else 0)
     assert O.stats()[2] > 0
+
+
+def test_overlapping_structures_are_rejected():
+    # two structures sharing a lattice cell: the neighbour table would be ambiguous, so finalize refuses the scene
+    from shatterforge_amd._native import SfError
+    G = Lattice(64, 64, scenes=1)
+    s = G.add_scene(64, 64, structs=2)
+    col = np.stack([np.full(4, lc.C), np.arange(4), np.full(4, lc.C)], 1).astype(np.int32)
+    G.add_structure(s, np.full(4, 2, np.uint8), col)
+    G.add_structure(s, np.full(2, 1, np.uint8), col[2:4] + [0, 0, 0])
+    with pytest.raises(SfError, match="two bricks in lattice cell"):
+        G.finalize(s)
+    G.close()
 
 
 MIX = [dict(kind="drop", seed=3, nbody=24), dict(kind="tower", w=8, h=16, nch=4, r=2.2, p=1.4),

@@ -1,7 +1,9 @@
 """Load path on the GPU against the FP64 oracle field (oracle/load_path.c conjugate gradients) and the oracle's
 verdicts. Tolerance: the device reaches max|b - A phi| / max|b| <= 1e-6 (checked by the oracle's own FP64 residual);
 the field's relative error to the oracle's is stated per structure and must stay under 1e-5; bond modes equal
-exactly and pass/fail verdicts equal exactly for every bond whose ratio is not within 1e-5 of 1."""
+exactly; every bond's ratio is within 1e-6 of the oracle's, and pass/fail equals the oracle's on every bond whose
+oracle ratio is farther from 1 than the structure's own measured ratio error. Bonds closer to 1 than that are ties:
+each is listed with its margin, and the minimum margin |r - 1| over all bonds is reported per structure."""
 import numpy as np
 import pytest
 
@@ -13,7 +15,9 @@ pytestmark = pytest.mark.gpu
 
 def _check(L, sid, mat, xyz, inf, oracle, tag):
     o = oracle.LoadOracle(mat, xyz, inf)
-    ref, it = o.solve64(rel_tol=1e-12)
+    # the reference field: FP64 conjugate gradients to a relative residual of 1e-11, five orders below the device's
+    # target (at 40k bricks and 56 layers FP64 CG stalls near 5e-11, so 1e-12 is out of its reach)
+    ref, it = o.solve64(rel_tol=1e-11)
     assert it >= 0
     phi = L.field(sid)
     info = L.info(sid)
@@ -25,11 +29,18 @@ def _check(L, sid, mat, xyz, inf, oracle, tag):
     r, m, f = L.verdicts(sid)
     ro, mo, fo = o.verdicts64(ref)
     assert np.array_equal(m, mo), tag
-    clear = np.abs(ro - 1.0) > 1e-5
-    assert np.array_equal(f[clear], fo[clear]), tag
     # ratios: within 1e-6 of the FP64 oracle's (lateral bonds carry differences of nearly equal potentials, so a
     # relative bound on tiny ratios means nothing; 1e-6 absolute is far below any verdict margin)
-    assert np.abs(r - ro).max() < 1e-6, (tag, np.abs(r - ro).max())
+    rerr = float(np.abs(r - ro).max())
+    assert rerr < 1e-6, (tag, rerr)
+    live = mo != 0
+    margin = np.abs(ro - 1.0)
+    tie = live & (margin <= rerr)
+    ties = [(int(i), int(k), float(ro[i, k]), float(r[i, k])) for i, k in np.argwhere(tie)]
+    print(f"{tag}: {int(live.sum())} bonds, field err {err:.1e}, ratio err {rerr:.1e}, "
+          f"min margin |r-1| {float(margin[live].min()) if live.any() else float('nan'):.2e}, ties {ties}", flush=True)
+    assert np.array_equal(f[~tie], fo[~tie]), (tag, np.argwhere((f != fo) & ~tie)[:5].tolist())
+    assert len(ties) <= 2, (tag, ties)
     return err
 
 
@@ -70,6 +81,31 @@ def test_batch_of_buildings_in_one_launch(oracle):
     assert L.solve(rel_tol=1e-6) == len(sts)
     errs = [_check(L, sid, *st, oracle, f"building {sid}") for sid, st in sts]
     print("per-structure field error vs FP64 oracle:", ["%.1e" % e for e in errs])
+
+
+def test_large_structures_of_the_1024_batch(oracle):
+    """The 1,024-structure batch of the load-path workload (2k-40k bricks log-uniform, 1-8 ground columns removed)
+    solved in one launch; every structure converges, and the eight largest (all over 12k bricks) are checked against
+    the FP64 oracle field and verdicts bond by bond."""
+    from shatterforge_amd import serve
+    print("case: 1,024 structures 2k-40k bricks in one launch; the 8 largest against the FP64 oracle", flush=True)
+    rng = np.random.default_rng(4)
+    L = sf.LoadBatch()
+    sts = []
+    for k in range(1024):
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(40000))))
+        st = serve.building_near(n, int(rng.integers(1, 1 << 30)))
+        sts.append(st)
+        L.add(*st)
+    print(f"built {sum(len(s[0]) for s in sts)} bricks", flush=True)
+    assert L.solve(rel_tol=1e-6) == 1024
+    res = np.array([L.info(k)["resid"] for k in range(1024)])
+    assert res.max() <= 1e-6
+    big = np.argsort([-len(s[0]) for s in sts])[:8]
+    assert min(len(sts[k][0]) for k in big) > 12000
+    for k in big:
+        _check(L, int(k), *sts[k], oracle, f"structure {k} ({len(sts[k][0])} bricks)")
+    L.close()
 
 
 def test_packed_rows_same_field_bit_for_bit():
