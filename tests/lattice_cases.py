@@ -73,6 +73,33 @@ def frame_building(n_target=2500, seed=4):
     return centred(*scenes.building_for(n_target, seed=seed))
 
 
+def office_all_columns(storeys=6, w=16, seed=3):
+    """a concrete-frame office of `storeys` storeys on a w x w footprint, with a charge of radius 3 m at every fourth
+    ground-floor column cell (the reference's office collapse fixture brings down all ground columns,
+    tests/office_collapse.json)"""
+    mat, xyz, inf = centred(*scenes.building(w, w, storeys, 6, 0, seed))
+    g = xyz[(xyz[:, 1] == 0) & (mat == 1)]
+    cols = sorted(set((int(a), int(b)) for a, b in g[:, [0, 2]]))
+    charges = [((x - C) * 0.5 + 0.5, 0.6, (z - C) * 0.5 + 0.5, 3.0, 1.4) for x, z in cols[::4]]
+    return (mat, xyz, inf), charges
+
+
+def masonry_shaft():
+    """a hollow 10 x 10 masonry shaft 40 bricks high with timber floors every 8 layers and two charges of radius
+    4 m at opposite corners of its base (the reference's masonry collapse fixture: a church tower base,
+    tests/masonry_collapse.json)"""
+    cells, m = [], []
+    for y in range(40):
+        for z in range(10):
+            for x in range(10):
+                if x in (0, 9) or z in (0, 9):
+                    cells.append((x, y, z)); m.append(2)
+                elif y % 8 == 7:
+                    cells.append((x, y, z)); m.append(5)
+    st = centred(np.array(m, np.uint8), np.array(cells, np.int32), np.zeros(len(m), np.uint8))
+    return st, [(-2.25, 0.6, -2.5, 4.0, 1.4), (2.25, 0.6, 2.5, 4.0, 1.4)]
+
+
 def column_bases(mat, xyz, rng, k):
     g = xyz[(xyz[:, 1] == 0) & (mat == 1)]
     cols = sorted(set((int(a), int(b)) for a, b in g[:, [0, 2]]))
@@ -102,6 +129,10 @@ def build(world, scene_or_none, case):
         add_struct(*small_tower(case.get("w", 8), case.get("h", 16)))
     elif kind == "building":
         add_struct(*frame_building(case.get("n", 2500), case.get("seed", 4)))
+    elif kind == "office":
+        add_struct(*office_all_columns()[0])
+    elif kind == "shaft":
+        add_struct(*masonry_shaft()[0])
     if scene_or_none is None:
         world.finalize()
         world.warm_all()
@@ -116,6 +147,10 @@ def charges(case):
         mat, xyz, _ = small_tower(case.get("w", 8), case.get("h", 16))
     elif kind == "building":
         mat, xyz, _ = frame_building(case.get("n", 2500), case.get("seed", 4))
+    elif kind == "office":
+        return office_all_columns()[1]
+    elif kind == "shaft":
+        return masonry_shaft()[1]
     else:
         return []
     rng = np.random.default_rng(case.get("cseed", 1))
@@ -129,4 +164,8 @@ def brick_count(case):
         return len(col[0]) + sum(len(b[0]) for b in bodies)
     if kind == "tower":
         return len(small_tower(case.get("w", 8), case.get("h", 16))[0])
+    if kind == "office":
+        return len(office_all_columns()[0][0])
+    if kind == "shaft":
+        return len(masonry_shaft()[0][0])
     return len(frame_building(case.get("n", 2500), case.get("seed", 4))[0])

@@ -83,6 +83,14 @@ def test_building_collapse_bit_exact(oracle, sweeps):
     assert O.stats()[2] > 0
 
 
+def test_office_collapse_bit_exact(oracle):
+    # every fourth ground column cell of a six-storey concrete frame blown: the floors come down over the columns,
+    # so large pieces run long static chains (hundreds of contacts against the standing columns) every substep
+    print("case: six-storey office, 16 charges at the ground columns, 420 frames", flush=True)
+    G, s, O = _run(oracle, dict(kind="office"), 420, check_every=20)
+    assert O.stats()[2] > 100
+
+
 def test_overlapping_structures_are_rejected():
     # two structures sharing a lattice cell: the neighbour table would be ambiguous, so finalize refuses the scene
     from shatterforge_amd._native import SfError
