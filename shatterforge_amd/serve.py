@@ -106,6 +106,12 @@ class WhatIfServer:
         return q["bricks"] * BYTES_PER_BRICK + 4096 * BYTES_PER_BODY
 
     def submit(self, q, t_arrival):
+        """queue a query; one that can never be admitted (more bricks than a slot holds, or more resident bytes than
+        the whole budget) is refused here with ValueError, before it reaches the queue"""
+        if q["bricks"] > self.slot_bricks:
+            raise ValueError(f"query {q.get('id')}: {q['bricks']} bricks, a slot holds {self.slot_bricks}")
+        if self.query_bytes(q) > self.budget:
+            raise ValueError(f"query {q.get('id')}: {self.query_bytes(q)} resident bytes, the budget is {self.budget}")
         q = dict(q)
         q["t_arrival"] = t_arrival
         self.queue.append(q)
@@ -119,8 +125,6 @@ class WhatIfServer:
         batch = []
         while self.queue and self.free and self.in_use + self.query_bytes(self.queue[0]) <= self.budget:
             q = self.queue.popleft()
-            if q["bricks"] > self.slot_bricks:
-                raise ValueError("query larger than a scene slot")
             s = self.free.popleft()
             self.in_use += self.query_bytes(q)
             batch.append((s, q))
