@@ -165,6 +165,18 @@ int sf_load_last_timing(sf_load* L, float* ms, int64_t* sweeps32, int64_t* sweep
  * (on = 0); both give the same field bit for bit. No counterpart in the reference: a switch for tests and
  * measurement. */
 int sf_load_set_packed(sf_load* L, int on);
+/* One structure's solve and verdicts without allocating device memory. No counterpart in the reference: it is for
+ * callers that cannot allocate (serving loops and harnesses that forbid hipMalloc while they run). Inputs as
+ * sf_load_add, outputs as sf_load_field / sf_load_verdicts / sf_load_info (ratio4, mode4, fail4, sweeps, resid may be
+ * NULL); omega, rel_tol and max_sweeps as sf_load_solve. The work goes on `stream` (a hipStream_t; NULL: the null
+ * stream) and the call returns once it is done: 1 converged, 0 not, or a negative code. Every device array lives in
+ * one fixed scratch array per GPU (about 10 MB, held from the first call until the process ends), so a structure
+ * may have at most sf_load_static_capacity() bricks (65,000); a larger one is refused with SF_E_CAP. Calls on one
+ * GPU share its scratch: the library runs them one at a time. Results equal the handle path's bit for bit. */
+int sf_load_solve_static(int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, const float* ext,
+                         float omega, double rel_tol, int max_sweeps, void* stream, double* phi, double* ratio4,
+                         uint8_t* mode4, uint8_t* fail4, int32_t* sweeps, double* resid);
+int64_t sf_load_static_capacity(void);
 
 /* Frame/wall building for load tests: footprint wx x wz bricks, storeys of 7 brick layers, columns every
  * `bay` bricks, slabs, perimeter walls of masonry with glass infill; `remove` columns removed at random.

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libsf_hip.so")
 c_int, c_uint, c_float, c_double, c_void_p, c_char_p = (ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.c_double,
                                                         ctypes.c_void_p, ctypes.c_char_p)
 c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_int64 = ctypes.c_int64
 
 # name: (restype, argtypes)
 _PROTOS = {
@@ -66,6 +67,9 @@ _PROTOS = {
     "sf_load_topology": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "sf_load_count": (c_int, [c_void_p]),
     "sf_load_set_packed": (c_int, [c_void_p, c_int]),
+    "sf_load_solve_static": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_double, c_int,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_load_static_capacity": (c_int64, []),
     "sf_load_last_timing": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_scene_building": (c_int, [c_int, c_int, c_int, c_int, c_int, c_uint, c_void_p, c_void_p, c_void_p]),
     "sf_scene_tower": (c_int, [c_void_p, c_void_p, c_void_p]),

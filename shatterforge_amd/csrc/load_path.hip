@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <unordered_map>
 #include <chrono>
+#include <mutex>
 
 namespace {
 
@@ -350,39 +351,52 @@ struct HostLoad {
     }
 };
 
-// lay every structure out on the device: red bricks then black, neighbour indices remapped
-int upload(HostLoad* L) {
-    L->free_dev();
+// Every structure laid out for the device: red bricks then black, neighbour indices remapped, packed rows coded
+// against the batch's table of distinct conductances. Host arrays only; the handle path and the allocation-free path
+// copy them to their own device memory.
+struct Layout {
     size_t N = 0;
-    for (auto& S : L->st) { S.base = (int)N; N += (size_t)S.n; }
-    L->N = N;
-    std::vector<int32_t> nbr(6 * N, -1);
-    std::vector<float> kc(6 * N, 0.0f), sk32(N, 0.0f), b(N, 0.0f), hp(4 * N, 0.0f), gk(N, 0.0f);
-    std::vector<double> sk64(N, 0.0);
-    std::vector<uint8_t> act(N, 0), mat(N, 0), foot(N, 0);
-    std::vector<DevInfo> info(L->st.size());
-    std::vector<uint4> row(N, make_uint4(0u, 0u, 0u, 0u));
-    std::vector<float> ktab(1, 0.0f);  // code 0: no bond
-    std::unordered_map<uint32_t, uint32_t> kcode{{0u, 0u}};
+    std::vector<int32_t> nbr;
+    std::vector<float> kc, sk32, b, hp, gk;
+    std::vector<double> sk64;
+    std::vector<uint8_t> act, mat, foot;
+    std::vector<DevInfo> info;
+    std::vector<uint4> row;
+    std::vector<float> ktab;
+    std::vector<int32_t> order;
     bool pk = true;
-    for (size_t s = 0; s < L->st.size(); s++) {
-        Struct& S = L->st[s];
+};
+void make_layout(std::vector<Struct>& st, Layout& O) {
+    size_t N = 0;
+    for (auto& S : st) { S.base = (int)N; N += (size_t)S.n; }
+    O.N = N;
+    O.nbr.assign(6 * N, -1);
+    O.kc.assign(6 * N, 0.0f); O.sk32.assign(N, 0.0f); O.b.assign(N, 0.0f); O.hp.assign(4 * N, 0.0f); O.gk.assign(N, 0.0f);
+    O.sk64.assign(N, 0.0);
+    O.act.assign(N, 0); O.mat.assign(N, 0); O.foot.assign(N, 0);
+    O.info.assign(st.size(), DevInfo{});
+    O.row.assign(N, make_uint4(0u, 0u, 0u, 0u));
+    O.ktab.assign(1, 0.0f);  // code 0: no bond
+    std::unordered_map<uint32_t, uint32_t> kcode{{0u, 0u}};
+    O.pk = true;
+    auto& nbr = O.nbr; auto& kc = O.kc; auto& row = O.row; auto& ktab = O.ktab;
+    for (size_t s = 0; s < st.size(); s++) {
+        Struct& S = st[s];
         S.perm.clear(); S.inv.assign(S.n, 0);
         for (int i = 0; i < S.n; i++) if (((S.xyz[3 * i] + S.xyz[3 * i + 1] + S.xyz[3 * i + 2]) & 1) == 0) S.perm.push_back(i);
         S.nred = (int)S.perm.size();
         for (int i = 0; i < S.n; i++) if (((S.xyz[3 * i] + S.xyz[3 * i + 1] + S.xyz[3 * i + 2]) & 1) != 0) S.perm.push_back(i);
         for (int k = 0; k < S.n; k++) S.inv[S.perm[k]] = k;
         double bmax = 0.0;
-        int xmax = 0;
         for (int k = 0; k < S.n; k++) {
             int i = S.perm[k];
             size_t g = (size_t)S.base + k;
             int mi = S.mat[i];
-            mat[g] = (uint8_t)mi;
-            act[g] = S.grounded[i];
-            for (int a = 0; a < 4; a++) hp[4 * g + a] = S.bond[4 * i + a];
-            foot[g] = footing(S, i);
-            gk[g] = GROUNDK * STIFF[mi];
+            O.mat[g] = (uint8_t)mi;
+            O.act[g] = S.grounded[i];
+            for (int a = 0; a < 4; a++) O.hp[4 * g + a] = S.bond[4 * i + a];
+            O.foot[g] = footing(S, i);
+            O.gk[g] = GROUNDK * STIFF[mi];
             float s32 = 0.0f; double s64 = 0.0;
             for (int d = 0; d < 6; d++) {
                 int j = linked(S, i, d);
@@ -392,19 +406,18 @@ int upload(HostLoad* L) {
                 kc[d * N + g] = k;
                 s32 += k; s64 += (double)k;
             }
-            if (foot[g]) { s32 += GROUNDK * STIFF[mi]; s64 += (double)(GROUNDK * STIFF[mi]); }
-            sk32[g] = s32; sk64[g] = s64;
-            if (act[g]) {
-                b[g] = MASS[mi] + (S.ext.empty() ? 0.0f : S.ext[i]);
-                bmax = std::max(bmax, (double)std::fabs(b[g]));
+            if (O.foot[g]) { s32 += GROUNDK * STIFF[mi]; s64 += (double)(GROUNDK * STIFF[mi]); }
+            O.sk32[g] = s32; O.sk64[g] = s64;
+            if (O.act[g]) {
+                O.b[g] = MASS[mi] + (S.ext.empty() ? 0.0f : S.ext[i]);
+                bmax = std::max(bmax, (double)std::fabs(O.b[g]));
             }
-            for (int a = 0; a < 3; a++) xmax = std::max(xmax, S.xyz[3 * i + a] + 1);
         }
         // packed rows of a structure whose indices fit 16 bits, conductances coded in the batch's table
         if (S.n < (int)PK_NONE) {
             for (int k = 0; k < S.n; k++) {
                 size_t g = (size_t)S.base + k;
-                uint32_t rel[6], codes = act[g] ? (1u << 30) : 0u;
+                uint32_t rel[6], codes = O.act[g] ? (1u << 30) : 0u;
                 for (int d = 0; d < 6; d++) {
                     int j = nbr[d * N + g];
                     rel[d] = j < 0 ? PK_NONE : (uint32_t)(j - S.base);
@@ -415,46 +428,132 @@ int upload(HostLoad* L) {
                     uint32_t c;
                     if (it != kcode.end()) c = it->second;
                     else if (ktab.size() < (size_t)PK_CODES) { c = (uint32_t)ktab.size(); kcode[bits] = c; ktab.push_back(kc[d * N + g]); }
-                    else { pk = false; c = 0; }
+                    else { O.pk = false; c = 0; }
                     codes |= c << (5 * d);
                 }
                 row[g] = make_uint4(rel[0] | rel[1] << 16, rel[2] | rel[3] << 16, rel[4] | rel[5] << 16, codes);
             }
         }
-        info[s].base = S.base; info[s].n = S.n; info[s].nred = S.nred; info[s].bmax = bmax;
-        info[s].omega = S.omega;
-        info[s].max_sweeps = 0;
+        O.info[s].base = S.base; O.info[s].n = S.n; O.info[s].nred = S.nred; O.info[s].bmax = bmax;
+        O.info[s].omega = S.omega;
+        O.info[s].max_sweeps = 0;
     }
+    O.order.resize(st.size());
+    for (size_t s = 0; s < O.order.size(); s++) O.order[s] = (int32_t)s;
+    std::stable_sort(O.order.begin(), O.order.end(), [&](int32_t a, int32_t b) { return st[a].n > st[b].n; });
+    ktab.resize(PK_CODES, 0.0f);
+}
+
+// a standing structure from its bricks: topology (finalize_brick), grounded bricks (conn_struct's search) and the
+// over-relaxation for a lattice of its height and span, 2 / (1 + sin(pi / L)) held to [1.6, 1.95]
+int make_struct(int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, const float* ext, Struct& S) {
+    if (n <= 0 || !mat || !xyz || !infill) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    S.n = n;
+    S.mat.assign(mat, mat + n); S.infill.assign(infill, infill + n); S.xyz.assign(xyz, xyz + 3 * (size_t)n);
+    for (int i = 0; i < n; i++) if (S.mat[i] < 1 || S.mat[i] > 5) { sf::set_error(SF_E_ARG, "material must be 1..5"); return SF_E_ARG; }
+    if (ext) {
+        S.ext.assign(ext, ext + n);
+        for (float e : S.ext) if (!std::isfinite(e)) { sf::set_error(SF_E_ARG, "non-finite load"); return SF_E_ARG; }
+    }
+    build_topology(S);
+    grounded(S);
+    int ext_max = 1;
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (int i = 0; i < n; i++) for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], S.xyz[3 * i + a]); hi[a] = std::max(hi[a], S.xyz[3 * i + a]); }
+    for (int a = 0; a < 3; a++) ext_max = std::max(ext_max, hi[a] - lo[a] + 1);
+    double om = 2.0 / (1.0 + std::sin(3.14159265358979 / (double)ext_max));
+    S.omega = (float)std::min(1.95, std::max(1.6, om));
+    return 0;
+}
+// the largest right-hand side over a structure's grounded bricks (the residual's scale)
+double struct_bmax(const Struct& S) {
+    double bmax = 0.0;
+    for (int i = 0; i < S.n; i++) if (S.grounded[i]) bmax = std::max(bmax, (double)std::fabs(MASS[S.mat[i]] + (S.ext.empty() ? 0.0f : S.ext[i])));
+    return bmax;
+}
+// dynamic LDS for the corrections: the largest structure of the batch that fits the workgroup's limit (at most
+// PK_NONE - 1 bricks, so the packed rows' 16-bit indices cover every structure with its correction in LDS)
+int lds_floats_for(const std::vector<Struct>& st, int dev, size_t* bytes) {
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds_max <= 0) lds_max = 65536;
+    int fit = std::min((int)PK_NONE - 1, std::max(0, (lds_max - 1024 - PK_CODES * (int)sizeof(float)) / (int)sizeof(float))), nmax = 0;
+    for (const Struct& S : st) if (S.n <= fit) nmax = std::max(nmax, S.n);
+    *bytes = sizeof(float) * (size_t)nmax;
+    if (*bytes > 65536) (void)hipFuncSetAttribute((const void*)k_load_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*bytes);
+    return nmax;
+}
+
+// the handle's device arrays, allocated for the batch and filled from its layout
+int upload(HostLoad* L) {
+    L->free_dev();
+    Layout O;
+    make_layout(L->st, O);
+    const size_t N = O.N;
+    L->N = N;
     int r = 0;
     r |= L->alloc(L->nbr, 6 * N); r |= L->alloc(L->kc, 6 * N); r |= L->alloc(L->sk32, N); r |= L->alloc(L->sk64, N);
     r |= L->alloc(L->b, N); r |= L->alloc(L->act, N); r |= L->alloc(L->mat, N); r |= L->alloc(L->foot, N);
     r |= L->alloc(L->hp, 4 * N); r |= L->alloc(L->gk, N); r |= L->alloc(L->phi, N); r |= L->alloc(L->delta, N);
-    r |= L->alloc(L->r32, N); r |= L->alloc(L->info, info.size()); r |= L->alloc(L->osw, info.size());
-    r |= L->alloc(L->ores, info.size()); r |= L->alloc(L->vratio, 4 * N); r |= L->alloc(L->vmode, 4 * N); r |= L->alloc(L->vfail, 4 * N);
-    r |= L->alloc(L->row, N); r |= L->alloc(L->ktab, PK_CODES); r |= L->alloc(L->order, info.size());
+    r |= L->alloc(L->r32, N); r |= L->alloc(L->info, O.info.size()); r |= L->alloc(L->osw, O.info.size());
+    r |= L->alloc(L->ores, O.info.size()); r |= L->alloc(L->vratio, 4 * N); r |= L->alloc(L->vmode, 4 * N); r |= L->alloc(L->vfail, 4 * N);
+    r |= L->alloc(L->row, N); r |= L->alloc(L->ktab, PK_CODES); r |= L->alloc(L->order, O.info.size());
     if (r) return r;
-    std::vector<int32_t> order(L->st.size());
-    for (size_t s = 0; s < order.size(); s++) order[s] = (int32_t)s;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return L->st[a].n > L->st[b].n; });
-    SF_HIP(hipMemcpy(L->order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
-    ktab.resize(PK_CODES, 0.0f);
-    L->pk = pk;
-    SF_HIP(hipMemcpy(L->row, row.data(), sizeof(uint4) * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->ktab, ktab.data(), sizeof(float) * PK_CODES, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->nbr, nbr.data(), sizeof(int32_t) * 6 * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->kc, kc.data(), sizeof(float) * 6 * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->sk32, sk32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->sk64, sk64.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->b, b.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->act, act.data(), N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->mat, mat.data(), N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->foot, foot.data(), N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->hp, hp.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->gk, gk.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(L->info, info.data(), sizeof(DevInfo) * info.size(), hipMemcpyHostToDevice));
+    L->pk = O.pk;
+    SF_HIP(hipMemcpy(L->order, O.order.data(), sizeof(int32_t) * O.order.size(), hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->row, O.row.data(), sizeof(uint4) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->ktab, O.ktab.data(), sizeof(float) * PK_CODES, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->nbr, O.nbr.data(), sizeof(int32_t) * 6 * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->kc, O.kc.data(), sizeof(float) * 6 * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->sk32, O.sk32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->sk64, O.sk64.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->b, O.b.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->act, O.act.data(), N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->mat, O.mat.data(), N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->foot, O.foot.data(), N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->hp, O.hp.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->gk, O.gk.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(L->info, O.info.data(), sizeof(DevInfo) * O.info.size(), hipMemcpyHostToDevice));
     L->dirty = false;
     return 0;
 }
+
+// ---------------------------------------------------------------- the allocation-free solve
+// One structure's solve with every device array in a fixed, statically allocated scratch array instead of memory
+// allocated for the call: the layout (stencil rows, diagonals, right-hand side, flags), the FP64 field, the FP32
+// residual and correction, the verdicts and the result words. Each GPU holds its own copy of `scratch` from the
+// first use until the process ends. Host code never passes `scratch` itself to a kernel: its name is a host-side
+// stand-in; the device address comes from hipGetSymbolAddress, once per device, and is kept in g_scratch.
+constexpr int STATIC_MAX_BRICKS = 65000;
+constexpr size_t STATIC_BYTES_PER_BRICK = 160;  // 159 bytes of arrays per brick (see carve_static), rounded up
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)STATIC_MAX_BRICKS * STATIC_BYTES_PER_BRICK + (64u << 10);
+static __device__ __align__(256) uint8_t scratch[STATIC_SCRATCH_BYTES];
+
+constexpr int STATIC_MAX_DEVICES = 64;
+struct ScratchDev { std::mutex mu; void* ptr = nullptr; };
+ScratchDev g_scratch[STATIC_MAX_DEVICES];
+
+__global__ void k_zero64(double* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+// the arrays of one structure of n bricks, 256-byte aligned, out of the scratch array at `base`; returns the bytes
+struct StaticArrays {
+    int32_t* nbr; float *kc, *sk32, *b, *hp, *gk, *delta, *r32, *ktab; double *sk64, *phi, *vratio, *ores;
+    uint8_t *act, *mat, *foot, *vmode, *vfail; uint4* row; DevInfo* info; int* osw; int32_t* order;
+};
+size_t carve_static(uint8_t* base, size_t n, StaticArrays& A) {
+    size_t off = 0;
+    auto take = [&](auto*& p, size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + off);
+        off += sizeof(*p) * (count ? count : 1);
+    };
+    take(A.row, n); take(A.nbr, 6 * n); take(A.kc, 6 * n); take(A.hp, 4 * n); take(A.vratio, 4 * n);
+    take(A.sk64, n); take(A.phi, n); take(A.sk32, n); take(A.b, n); take(A.gk, n); take(A.delta, n); take(A.r32, n);
+    take(A.vmode, 4 * n); take(A.vfail, 4 * n); take(A.act, n); take(A.mat, n); take(A.foot, n);
+    take(A.ktab, PK_CODES); take(A.info, 1); take(A.osw, 1); take(A.ores, 1); take(A.order, 1);
+    return off;
+}
+static_assert((size_t)STATIC_MAX_BRICKS * 159 + 23 * 256 + PK_CODES * 4 + 64 <= STATIC_SCRATCH_BYTES, "scratch too small");
 
 }  // namespace
 
@@ -479,24 +578,10 @@ int sf_load_set_packed(sf_load* L, int on) {
 }
 
 int sf_load_add(sf_load* L, int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, const float* ext) {
-    if (!L || n <= 0 || !mat || !xyz || !infill) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    if (!L) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
     Struct S;
-    S.n = n;
-    S.mat.assign(mat, mat + n); S.infill.assign(infill, infill + n); S.xyz.assign(xyz, xyz + 3 * (size_t)n);
-    for (int i = 0; i < n; i++) if (S.mat[i] < 1 || S.mat[i] > 5) { sf::set_error(SF_E_ARG, "material must be 1..5"); return SF_E_ARG; }
-    if (ext) {
-        S.ext.assign(ext, ext + n);
-        for (float e : S.ext) if (!std::isfinite(e)) { sf::set_error(SF_E_ARG, "non-finite load"); return SF_E_ARG; }
-    }
-    build_topology(S);
-    grounded(S);
-    // over-relaxation for a lattice of this height and span: 2 / (1 + sin(pi / L)), held to [1.6, 1.95]
-    int ext_max = 1;
-    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-    for (int i = 0; i < n; i++) for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], S.xyz[3 * i + a]); hi[a] = std::max(hi[a], S.xyz[3 * i + a]); }
-    for (int a = 0; a < 3; a++) ext_max = std::max(ext_max, hi[a] - lo[a] + 1);
-    double om = 2.0 / (1.0 + std::sin(3.14159265358979 / (double)ext_max));
-    S.omega = (float)std::min(1.95, std::max(1.6, om));
+    int r = make_struct(n, mat, xyz, infill, ext, S);
+    if (r) return r;
     L->h.st.push_back(std::move(S));
     L->h.dirty = true;
     return (int)L->h.st.size() - 1;
@@ -520,9 +605,7 @@ int sf_load_solve_begin(sf_load* L, float omega, double rel_tol, int max_sweeps)
     H->hinfo.resize(H->st.size());
     for (size_t s = 0; s < H->st.size(); s++) {
         const Struct& S = H->st[s];
-        double bmax = 0.0;
-        for (int i = 0; i < S.n; i++) if (S.grounded[i]) bmax = std::max(bmax, (double)std::fabs(MASS[S.mat[i]] + (S.ext.empty() ? 0.0f : S.ext[i])));
-        H->hinfo[s] = DevInfo{S.base, S.n, S.nred, max_sweeps, S.omega, bmax};
+        H->hinfo[s] = DevInfo{S.base, S.n, S.nred, max_sweeps, S.omega, struct_bmax(S)};
     }
     SF_HIP(hipMemcpy(H->info, H->hinfo.data(), sizeof(DevInfo) * H->hinfo.size(), hipMemcpyHostToDevice));
     SF_HIP(hipMemsetAsync(H->phi, 0, sizeof(double) * H->N, H->stream));
@@ -532,15 +615,8 @@ int sf_load_solve_begin(sf_load* L, float omega, double rel_tol, int max_sweeps)
     D.out_resid = H->ores; D.N = H->N; D.tol = rel_tol; D.inner = H->inner;
     D.row = H->row; D.ktab = H->ktab; D.pk_codes = H->pk && H->use_packed ? 1 : 0; D.order = H->order;
     H->tol = rel_tol;
-    // dynamic LDS for the corrections: the largest structure that fits the workgroup limit
-    int lds_max = 0;
-    SF_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, H->dev));
-    // (at most PK_NONE - 1 bricks: the packed rows' 16-bit indices cover every structure with its correction in LDS)
-    int fit = std::min((int)PK_NONE - 1, std::max(0, (lds_max - 1024 - PK_CODES * (int)sizeof(float)) / (int)sizeof(float))), nmax = 0;
-    for (const Struct& S : H->st) if (S.n <= fit) nmax = std::max(nmax, S.n);
-    D.lds_floats = nmax;
-    size_t lds_bytes = sizeof(float) * (size_t)nmax;
-    if (lds_bytes > 65536) (void)hipFuncSetAttribute((const void*)k_load_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    size_t lds_bytes = 0;
+    D.lds_floats = lds_floats_for(H->st, H->dev, &lds_bytes);
     SF_HIP(hipEventRecord(H->e0, H->stream));
     hipLaunchKernelGGL(k_load_solve, dim3(D.nstruct), dim3(LT), lds_bytes, H->stream, D);
     SF_HIP(hipGetLastError());
@@ -648,5 +724,94 @@ int sf_load_last_timing(sf_load* L, float* ms, int64_t* sweeps32, int64_t* sweep
     if (bricks) *bricks = (int64_t)L->h.N;
     return 0;
 }
+
+int sf_load_solve_static(int n, const uint8_t* mat, const int32_t* xyz, const uint8_t* infill, const float* ext,
+                         float omega, double rel_tol, int max_sweeps, void* stream, double* phi, double* ratio4,
+                         uint8_t* mode4, uint8_t* fail4, int32_t* sweeps, double* resid) {
+    if (!phi || !(rel_tol > 0.0) || max_sweeps <= 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    if (n > STATIC_MAX_BRICKS) {
+        sf::set_error(SF_E_CAP, "sf_load_solve_static: " + std::to_string(n) + " bricks, at most " + std::to_string(STATIC_MAX_BRICKS));
+        return SF_E_CAP;
+    }
+    std::vector<Struct> st(1);
+    int r = make_struct(n, mat, xyz, infill, ext, st[0]);
+    if (r) return r;
+    if (omega > 0.0f) st[0].omega = omega;
+    Layout O;
+    make_layout(st, O);
+    int ndev = 0, dev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { sf::set_error(SF_E_NODEV, "no GPU visible"); return SF_E_NODEV; }
+    SF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) { sf::set_error(SF_E_ARG, "device index beyond the scratch table"); return SF_E_ARG; }
+    ScratchDev& G = g_scratch[dev];
+    std::lock_guard<std::mutex> lock(G.mu);  // one call at a time per GPU: they share its scratch
+    if (!G.ptr) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) { sf::set_error(SF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)); return SF_E_HIP; }
+        G.ptr = p;
+    }
+    const size_t N = O.N;
+    StaticArrays A;
+    if (carve_static(static_cast<uint8_t*>(G.ptr), N, A) > STATIC_SCRATCH_BYTES) { sf::set_error(SF_E_CAP, "scratch layout"); return SF_E_CAP; }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevInfo info = O.info[0];
+    info.max_sweeps = max_sweeps;
+    info.bmax = struct_bmax(st[0]);
+    const int32_t order0 = 0;
+    SF_HIP(hipMemcpyAsync(A.row, O.row.data(), sizeof(uint4) * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.ktab, O.ktab.data(), sizeof(float) * PK_CODES, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.nbr, O.nbr.data(), sizeof(int32_t) * 6 * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.kc, O.kc.data(), sizeof(float) * 6 * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.sk32, O.sk32.data(), sizeof(float) * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.sk64, O.sk64.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.b, O.b.data(), sizeof(float) * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.act, O.act.data(), N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.mat, O.mat.data(), N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.foot, O.foot.data(), N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.hp, O.hp.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.gk, O.gk.data(), sizeof(float) * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.info, &info, sizeof(DevInfo), hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(A.order, &order0, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    // the field starts at zero (a fill kernel, so a caller may capture the call in a graph)
+    hipLaunchKernelGGL(k_zero64, dim3(sf_grid(N)), dim3(256), 0, s, A.phi, N);
+    LoadDev D;
+    D.nstruct = 1; D.info = A.info; D.nbr = A.nbr; D.kc = A.kc; D.sk32 = A.sk32; D.sk64 = A.sk64;
+    D.b = A.b; D.act = A.act; D.phi = A.phi; D.delta = A.delta; D.r32 = A.r32; D.out_sweeps = A.osw;
+    D.out_resid = A.ores; D.N = N; D.tol = rel_tol; D.inner = 32;
+    D.row = A.row; D.ktab = A.ktab; D.pk_codes = O.pk ? 1 : 0; D.order = A.order;
+    size_t lds_bytes = 0;
+    D.lds_floats = lds_floats_for(st, dev, &lds_bytes);
+    hipLaunchKernelGGL(k_load_solve, dim3(1), dim3(LT), lds_bytes, s, D);
+    SF_HIP(hipGetLastError());
+    Verd V{A.vratio, A.vmode, A.vfail};
+    hipLaunchKernelGGL(k_verdicts, dim3(sf_grid(N)), dim3(256), 0, s, A.nbr, A.kc, A.hp, A.mat, A.foot, A.phi, A.gk, N, V);
+    SF_HIP(hipGetLastError());
+    std::vector<double> d(N), vr(ratio4 ? 4 * N : 0);
+    std::vector<uint8_t> vm(mode4 ? 4 * N : 0), vf(fail4 ? 4 * N : 0);
+    int osw = 0;
+    double ores = 0.0;
+    SF_HIP(hipMemcpyAsync(d.data(), A.phi, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    if (ratio4) SF_HIP(hipMemcpyAsync(vr.data(), A.vratio, sizeof(double) * 4 * N, hipMemcpyDeviceToHost, s));
+    if (mode4) SF_HIP(hipMemcpyAsync(vm.data(), A.vmode, 4 * N, hipMemcpyDeviceToHost, s));
+    if (fail4) SF_HIP(hipMemcpyAsync(vf.data(), A.vfail, 4 * N, hipMemcpyDeviceToHost, s));
+    SF_HIP(hipMemcpyAsync(&osw, A.osw, sizeof(int), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipMemcpyAsync(&ores, A.ores, sizeof(double), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+    const Struct& S = st[0];
+    for (int k = 0; k < S.n; k++) {
+        const int i = S.perm[k];
+        phi[i] = d[k];
+        for (int a = 0; a < 4; a++) {
+            if (ratio4) ratio4[4 * i + a] = vr[4 * k + a];
+            if (mode4) mode4[4 * i + a] = vm[4 * k + a];
+            if (fail4) fail4[4 * i + a] = vf[4 * k + a];
+        }
+    }
+    if (sweeps) *sweeps = osw;
+    if (resid) *resid = ores;
+    return ores <= rel_tol ? 1 : 0;
+}
+int64_t sf_load_static_capacity(void) { return STATIC_MAX_BRICKS; }
 
 }  // extern "C"
