@@ -185,6 +185,11 @@ int sf_scene_building(int wx, int wz, int storeys, int bay, int remove, uint32_t
 /* the tower of the plain-method workload: 12x12 footprint x 40 high, 2x2 concrete corner columns, a concrete
  * slab every 6th layer, masonry elsewhere */
 int sf_scene_tower(uint8_t* mat, int32_t* xyz, uint8_t* infill);
+/* The site-scale scene: 37 structures on a 320 x 320 m site (16 frame towers of 8-16 storeys, 12 three-storey row
+ * blocks, a 56 m ring chimney, an arched viaduct, 7 steel halls), about 380k bricks, seeded. k = -1 returns the
+ * structure count; k >= 0 returns structure k's brick count and, with non-NULL arrays, its bricks (lattice cells,
+ * materials 1..5, infill flags). Stands in for the reference's hand-built map (build_map, src/lib.rs:3528-3590). */
+int sf_scene_site(uint32_t seed, int k, uint8_t* mat, int32_t* xyz, uint8_t* infill);
 
 
 /* ---------------------------------------------------------------- lattice world (the legacy brick engine)
@@ -255,6 +260,32 @@ int sf_lat_set_fused(sf_lat* L, int max_contacts);
  * -1 (the default) takes the device's LDS limit (about 40k bricks on gfx950); 0 sends every structure down the
  * global path. Both give the same bits (no counterpart in the reference: a storage choice, a switch for tests). */
 int sf_lat_set_sor_lds(sf_lat* L, int max_bricks);
+
+/* ---- the legacy engine's own exports (src/lib.rs, tools/debug_exports.rs) on one default brick world. Prefixed
+ * because `step` also names the box core's export. Pointer results are library buffers valid until the next call.
+ * init builds the seeded site scene (sf_scene_site; the reference's build_map is game content) with every
+ * structure's load field converged; charges are placed where thrown (their flight is game behaviour); events
+ * (sound, particles, HUD) are not produced, so n_events is 0. */
+uint32_t sf_legacy_init(void);                          /* init() -> bricks, src/lib.rs:3614-3638 */
+void sf_legacy_step(float dt);                          /* step(dt), :3639-3721 (dt clamped to [0, 1/30]) */
+void sf_legacy_explode(float x, float y, float z, float r, float p);  /* explode, :3855-3856 */
+uint32_t sf_legacy_throw_charge(float ox, float oy, float oz, float vx, float vy, float vz);  /* :3933-3941; 1 held, 0 full (8) */
+uint32_t sf_legacy_detonate(void);                      /* detonate() -> charges fired (r 3, power 1.4), :3942-3954 */
+uint32_t sf_legacy_n_structs(void);                     /* n_structs, :4051 */
+const float* sf_legacy_ptr_structs(void);               /* ptr_structs: 12 f32 per structure, :3733-3748, :4053 */
+const float* sf_legacy_ptr_stats(void);                 /* ptr_stats: 16 f32, :3718-3720 */
+void sf_legacy_clear_events(void);                      /* clear_events, :4026 */
+uint32_t sf_legacy_n_events(void);                      /* n_events, :4027 */
+const float* sf_legacy_ptr_events(void);                /* ptr_events, :4028 */
+const float* sf_legacy_dbg_hist(uint32_t s);            /* tools/debug_exports.rs:3-19 */
+const float* sf_legacy_dbg_bodies(void);                /* :20-40 */
+const float* sf_legacy_dbg_matstats(void);              /* :41-56 */
+const float* sf_legacy_dbg_speed(void);                 /* :57-71 */
+const float* sf_legacy_dbg_bigbody(void);               /* :72-87 (12 values) */
+const float* sf_legacy_dbg_top(uint32_t s);             /* :89-105 */
+void sf_legacy_dbg_wake_all(void);                      /* :107 */
+/* the default world's handle and its scene, to mix both APIs (no counterpart in the reference) */
+sf_lat* sf_legacy_world(int* scene);
 
 
 

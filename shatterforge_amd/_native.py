@@ -73,6 +73,7 @@ _PROTOS = {
     "sf_load_last_timing": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_scene_building": (c_int, [c_int, c_int, c_int, c_int, c_int, c_uint, c_void_p, c_void_p, c_void_p]),
     "sf_scene_tower": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "sf_scene_site": (c_int, [c_uint, c_int, c_void_p, c_void_p, c_void_p]),
     "sf_lat_create": (c_void_p, [c_int, c_int, c_int, c_int, c_int]),
     "sf_lat_destroy": (None, [c_void_p]),
     "sf_lat_scene_add": (c_int, [c_void_p, c_int, c_int, c_int, c_uint]),
@@ -98,6 +99,25 @@ _PROTOS = {
     "sf_lat_phase_ms": (c_int, [c_void_p, c_void_p]),
     "sf_lat_set_fused": (c_int, [c_void_p, c_int]),
     "sf_lat_set_sor_lds": (c_int, [c_void_p, c_int]),
+    "sf_legacy_init": (c_uint, []),
+    "sf_legacy_step": (None, [c_float]),
+    "sf_legacy_explode": (None, [c_float, c_float, c_float, c_float, c_float]),
+    "sf_legacy_throw_charge": (c_uint, [c_float, c_float, c_float, c_float, c_float, c_float]),
+    "sf_legacy_detonate": (c_uint, []),
+    "sf_legacy_n_structs": (c_uint, []),
+    "sf_legacy_ptr_structs": (c_void_p, []),
+    "sf_legacy_ptr_stats": (c_void_p, []),
+    "sf_legacy_clear_events": (None, []),
+    "sf_legacy_n_events": (c_uint, []),
+    "sf_legacy_ptr_events": (c_void_p, []),
+    "sf_legacy_dbg_hist": (c_void_p, [c_uint]),
+    "sf_legacy_dbg_bodies": (c_void_p, []),
+    "sf_legacy_dbg_matstats": (c_void_p, []),
+    "sf_legacy_dbg_speed": (c_void_p, []),
+    "sf_legacy_dbg_bigbody": (c_void_p, []),
+    "sf_legacy_dbg_top": (c_void_p, [c_uint]),
+    "sf_legacy_dbg_wake_all": (None, []),
+    "sf_legacy_world": (c_void_p, [c_void_p]),
     "sf_lat_subphase_ms": (c_int, [c_void_p, c_void_p]),
 }
 

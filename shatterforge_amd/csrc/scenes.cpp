@@ -136,6 +136,97 @@ void build_compound(int nb, const int32_t* xyz, const uint8_t* mat, const double
     }
 }
 
+
+// ---- the site-scale scene: a 320 x 320 m site (lattice indices 0..639 around 320) of 37 structures, one per 40 m
+// lot: 16 frame towers of 8-16 storeys (3.5 m each), 12 three-storey masonry row blocks with concrete floors, a 56 m
+// single-brick-ring masonry chimney, an arched masonry viaduct and 7 steel-frame halls with glass walls.
+struct SiteStruct { std::vector<uint8_t> mat, infill; std::vector<int32_t> xyz; };
+void site_add(SiteStruct& S, int x, int y, int z, uint8_t m, uint8_t inf = 0) {
+    S.xyz.push_back(x); S.xyz.push_back(y); S.xyz.push_back(z); S.mat.push_back(m); S.infill.push_back(inf);
+}
+std::vector<SiteStruct> make_site(uint32_t seed) {
+    Rng rng(seed);
+    std::vector<std::array<int, 2>> lots;
+    for (int cx = 40; cx < 600; cx += 80) for (int cz = 40; cz < 600; cz += 80) lots.push_back({cx, cz});
+    for (size_t k = lots.size(); k > 1; k--) std::swap(lots[k - 1], lots[rng.below((uint32_t)k)]);  // Fisher-Yates
+    std::vector<SiteStruct> out;
+    size_t lot = 0;
+    for (int t = 0; t < 16; t++) {  // frame towers
+        const int cx = lots[lot][0], cz = lots[lot][1]; lot++;
+        const int w = 22 + (int)rng.below(10), storeys = 8 + (int)rng.below(9);
+        const uint32_t bseed = 1 + (uint32_t)(rng.next() % 0x3FFFFFFF);
+        const int n = sf_scene_building(w, w, storeys, 6, 0, bseed, nullptr, nullptr, nullptr);
+        std::vector<uint8_t> m(n), inf(n);
+        std::vector<int32_t> xyz(3 * (size_t)n);
+        sf_scene_building(w, w, storeys, 6, 0, bseed, m.data(), xyz.data(), inf.data());
+        SiteStruct S;
+        for (int i = 0; i < n; i++) site_add(S, xyz[3 * i] + cx - w / 2, xyz[3 * i + 1], xyz[3 * i + 2] + cz - w / 2, m[i], inf[i]);
+        out.push_back(std::move(S));
+    }
+    for (int t = 0; t < 12; t++) {  // row blocks: per storey a 6-layer masonry wall ring and a concrete slab
+        const int cx = lots[lot][0], cz = lots[lot][1]; lot++;
+        const int lx = 30 + (int)rng.below(18), lz = 8 + (int)rng.below(4);
+        const int x0 = cx - lx / 2, z0 = cz - lz / 2;
+        SiteStruct S;
+        for (int st = 0; st < 3; st++) {
+            for (int y = 7 * st; y < 7 * st + 6; y++)
+                for (int x = 0; x < lx; x++) for (int z = 0; z < lz; z++)
+                    if (x == 0 || x == lx - 1 || z == 0 || z == lz - 1) site_add(S, x0 + x, y, z0 + z, 2);
+            for (int x = 0; x < lx; x++) for (int z = 0; z < lz; z++) site_add(S, x0 + x, 7 * st + 6, z0 + z, 1);
+        }
+        out.push_back(std::move(S));
+    }
+    {  // the chimney: a ring of radius 6 bricks, 112 bricks (56 m) tall
+        const int cx = lots[lot][0], cz = lots[lot][1]; lot++;
+        std::vector<std::array<int, 2>> ring;
+        for (int k = 0; k < 400; k++) {
+            const double a = 2.0 * 3.14159265358979 * k / 399.0;
+            std::array<int, 2> c{(int)std::lround(cx + 6.0 * std::cos(a)), (int)std::lround(cz + 6.0 * std::sin(a))};
+            if (std::find(ring.begin(), ring.end(), c) == ring.end()) ring.push_back(c);
+        }
+        std::sort(ring.begin(), ring.end());
+        SiteStruct S;
+        for (int y = 0; y < 112; y++) for (auto& c : ring) site_add(S, c[0], y, c[1], 2);
+        out.push_back(std::move(S));
+    }
+    {  // the viaduct: 9 masonry piers of 2 x 4 bricks, 20 tall, 8 arches between them and a deck, along x
+        const int cx = lots[lot][0], cz = lots[lot][1]; lot++;
+        const int span = 8, x0 = cx - 4 * span;
+        std::vector<std::array<int, 3>> cells;
+        for (int p = 0; p < 9; p++) for (int y = 0; y < 20; y++) for (int dx = 0; dx < 2; dx++) for (int dz = 0; dz < 4; dz++)
+            cells.push_back({x0 + p * span + dx, y, cz + dz});
+        for (int a = 0; a < 8; a++) {
+            const int ax0 = x0 + a * span + 2;
+            for (int x = ax0; x < ax0 + span - 2; x++) {
+                const double u = (x - ax0 + 0.5) / (span - 2);
+                const int ytop = 14 + (int)std::lround(5.0 * std::sin(3.14159265358979 * u));
+                for (int y = ytop; y < 20; y++) for (int dz = 0; dz < 4; dz++) cells.push_back({x, y, cz + dz});
+            }
+        }
+        for (int x = x0; x < x0 + 8 * span + 2; x++) for (int dz = 0; dz < 4; dz++) cells.push_back({x, 20, cz + dz});
+        std::sort(cells.begin(), cells.end());
+        cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+        SiteStruct S;
+        for (auto& c : cells) site_add(S, c[0], c[1], c[2], 2);
+        out.push_back(std::move(S));
+    }
+    for (int t = 0; t < 7; t++) {  // steel halls: columns every 6 bricks and at the far edges, a roof grid, glass walls
+        const int cx = lots[lot][0], cz = lots[lot][1]; lot++;
+        const int w = 20 + (int)rng.below(10), h = 14;
+        SiteStruct S;
+        for (int x = 0; x < w; x++) for (int z = 0; z < w; z++) {
+            const bool col = (x % 6 == 0 || x == w - 1) && (z % 6 == 0 || z == w - 1);
+            const bool per = x == 0 || x == w - 1 || z == 0 || z == w - 1;
+            for (int y = 0; y < h; y++) {
+                if (col) site_add(S, cx - w / 2 + x, y, cz - w / 2 + z, 3);
+                else if (per && y < h - 1) site_add(S, cx - w / 2 + x, y, cz - w / 2 + z, 4, 1);
+            }
+            if (x % 3 == 0 || z % 3 == 0) site_add(S, cx - w / 2 + x, h, cz - w / 2 + z, 3);
+        }
+        out.push_back(std::move(S));
+    }
+    return out;
+}
 }  // namespace
 
 extern "C" {
@@ -306,6 +397,20 @@ int sf_scene_tower(uint8_t* mat, int32_t* xyz, uint8_t* infill) {
         if (mat) { mat[n] = (col || slab) ? 1 : 2; xyz[3 * n] = x; xyz[3 * n + 1] = y; xyz[3 * n + 2] = z; infill[n] = 0; }
         n++;
     }
+    return n;
+}
+
+int sf_scene_site(uint32_t seed, int k, uint8_t* mat, int32_t* xyz, uint8_t* infill) {
+    static uint32_t cached_seed = 0;
+    static std::vector<SiteStruct> cached;
+    if (cached.empty() || cached_seed != seed) { cached = make_site(seed); cached_seed = seed; }
+    if (k < 0) return (int)cached.size();
+    if (k >= (int)cached.size()) { sf::set_error(SF_E_ARG, "no such structure"); return SF_E_ARG; }
+    const SiteStruct& S = cached[k];
+    const int n = (int)S.mat.size();
+    if (mat) std::memcpy(mat, S.mat.data(), n);
+    if (xyz) std::memcpy(xyz, S.xyz.data(), sizeof(int32_t) * 3 * (size_t)n);
+    if (infill) std::memcpy(infill, S.infill.data(), n);
     return n;
 }
 

@@ -140,90 +140,21 @@ def _box_shell(x0, z0, wx, wz, h, mat, y0=0):
 
 
 def site(seed=1):
-    """Site-scale scene: a 320 x 320 m site (lattice indices 0..639 around 320) with 37 structures of about 370k
-    bricks: 16 frame towers of 8-16 storeys (3.5 m each, concrete frame, glass infill band, masonry walls), 12
-    three-storey masonry row blocks, one 56 m single-brick-ring chimney, one arched masonry viaduct and 7 steel-frame
-    halls with glass walls, one structure per 40 m lot.
+    """Site-scale scene (sf_scene_site): a 320 x 320 m site (lattice indices 0..639 around 320) of 37 structures of
+    about 380k bricks: 16 frame towers of 8-16 storeys (3.5 m each, concrete frame, glass infill band, masonry walls),
+    12 three-storey masonry row blocks, one 56 m single-brick-ring chimney, one arched masonry viaduct and 7
+    steel-frame halls with glass walls, one structure per 40 m lot.
     Returns a list of (mat, xyz, infill) per structure, lattice coordinates."""
-    rng = np.random.default_rng(seed)
-    structs = []
-    lots = [(cx, cz) for cx in range(40, 600, 80) for cz in range(40, 600, 80)]
-    rng.shuffle(lots)
-    k = 0
-    # 16 frame towers (8-16 storeys)
-    for t in range(16):
-        cx, cz = lots[k]; k += 1
-        w = int(rng.integers(22, 32))
-        storeys = int(rng.integers(8, 17))
-        mat, xyz, infill = building(w, w, storeys, 6, 0, int(rng.integers(1, 1 << 30)))
-        xyz = np.array(xyz, np.int32)
-        xyz[:, 0] += cx - w // 2
-        xyz[:, 2] += cz - w // 2
-        structs.append((mat, xyz, infill))
-    # 12 row blocks: 3 storeys of masonry walls with concrete floor slabs, long and narrow
-    for t in range(12):
-        cx, cz = lots[k]; k += 1
-        lx, lz = int(rng.integers(30, 48)), int(rng.integers(8, 12))
-        parts_x, parts_m = [], []
-        for st in range(3):
-            wx_, wm = _box_shell(cx - lx // 2, cz - lz // 2, lx, lz, 6, 2, y0=7 * st)
-            parts_x.append(wx_); parts_m.append(wm)
-            sl = np.array([(cx - lx // 2 + x, 7 * st + 6, cz - lz // 2 + z) for x in range(lx) for z in range(lz)], np.int32)
-            parts_x.append(sl); parts_m.append(np.full(len(sl), 1, np.uint8))
-        xyz = np.concatenate(parts_x)
-        mat = np.concatenate(parts_m)
-        structs.append((mat, xyz, np.zeros(len(mat), np.uint8)))
-    # the chimney: a ring of radius 6 bricks, 112 bricks (56 m) tall, masonry
-    cx, cz = lots[k]; k += 1
-    ring = sorted(set((int(round(cx + 6 * np.cos(a))), int(round(cz + 6 * np.sin(a)))) for a in np.linspace(0, 2 * np.pi, 400)))
-    xyz = np.array([(x, y, z) for y in range(112) for (x, z) in ring], np.int32)
-    structs.append((np.full(len(xyz), 2, np.uint8), xyz, np.zeros(len(xyz), np.uint8)))
-    # the viaduct: 9 masonry piers 2 x 4 bricks, 20 bricks tall, 8 arches and a deck, along x
-    cx, cz = lots[k]; k += 1
-    cells = set()
-    span = 8
-    x0 = cx - 4 * span
-    for p in range(9):
-        px = x0 + p * span
-        for y in range(20):
-            for dx in range(2):
-                for dz in range(4):
-                    cells.add((px + dx, y, cz + dz))
-    for a in range(8):
-        ax0 = x0 + a * span + 2
-        for x in range(ax0, ax0 + span - 2):
-            u = (x - ax0 + 0.5) / (span - 2)
-            ytop = 14 + int(round(5 * np.sin(np.pi * u)))
-            for y in range(ytop, 20):
-                for dz in range(4):
-                    cells.add((x, y, cz + dz))
-    for x in range(x0, x0 + 8 * span + 2):
-        for dz in range(4):
-            cells.add((x, 20, cz + dz))
-    xyz = np.array(sorted(cells), np.int32)
-    structs.append((np.full(len(xyz), 2, np.uint8), xyz, np.zeros(len(xyz), np.uint8)))
-    # 7 steel-frame halls: steel columns every 6 bricks, steel roof grid, glass walls
-    for t in range(7):
-        cx, cz = lots[k]; k += 1
-        w = int(rng.integers(20, 30))
-        h = 14
-        cells = {}
-        for x in range(w):
-            for z in range(w):
-                col = (x % 6 == 0 or x == w - 1) and (z % 6 == 0 or z == w - 1)
-                for y in range(h):
-                    per = x in (0, w - 1) or z in (0, w - 1)
-                    if col:
-                        cells[(x, y, z)] = 3
-                    elif per and y < h - 1:
-                        cells[(x, y, z)] = 4
-                if x % 3 == 0 or z % 3 == 0:
-                    cells[(x, h, z)] = 3
-        xyz = np.array([(cx - w // 2 + x, y, cz - w // 2 + z) for (x, y, z) in cells], np.int32)
-        mat = np.array(list(cells.values()), np.uint8)
-        infill = (mat == 4).astype(np.uint8)
-        structs.append((mat, xyz, infill))
-    return structs
+    L = lib()
+    out = []
+    for k in range(check(L.sf_scene_site(int(seed), -1, None, None, None))):
+        n = check(L.sf_scene_site(int(seed), k, None, None, None))
+        mat = np.zeros(n, np.uint8)
+        xyz = np.zeros((n, 3), np.int32)
+        infill = np.zeros(n, np.uint8)
+        check(L.sf_scene_site(int(seed), k, ptr(mat), ptr(xyz), ptr(infill)))
+        out.append((mat, xyz, infill))
+    return out
 
 
 def site_charges(structs, seed=1, sets=8, per_set=4, r=2.2, power=1.4):
