@@ -209,6 +209,17 @@ void sf_lat_destroy(sf_lat* L);
 int sf_lat_scene_add(sf_lat* L, int brick_cap, int body_cap, int max_structs, uint32_t seed);
 /* empty a scene for reuse (same capacities) */
 int sf_lat_scene_clear(sf_lat* L, int scene, uint32_t seed);
+/* A scene's brick and body ranges go back to the handle's pool (the scene stays, empty, without capacity), and a
+ * released scene takes new ranges of the given capacities from the pool (first fit; SF_E_CAP when none is free that
+ * large). Lets a server keep scenes of uneven size in one handle. (No counterpart in the reference.) */
+int sf_lat_scene_release(sf_lat* L, int scene);
+int sf_lat_scene_place(sf_lat* L, int scene, int brick_cap, int body_cap, uint32_t seed);
+/* device bytes a scene holds in the handle's arrays: its share of every per-brick, per-body, per-contact (four per
+ * brick) and per-structure array, and its static grid. What a server admits against its HBM budget. */
+int64_t sf_lat_bytes_resident(sf_lat* L, int scene);
+/* the same rule for a scene not placed yet: its brick, body and structure capacities (without the grid, which
+ * finalize sizes) */
+int64_t sf_lat_bytes_for(sf_lat* L, int brick_cap, int body_cap, int max_structs);
 /* a standing structure (add_brick, src/lib.rs:629-645): lattice cells xyz[3n], materials 1..8, infill flags */
 int sf_lat_add_structure(sf_lat* L, int scene, int n, const int32_t* xyz, const uint8_t* mat, const uint8_t* infill);
 /* a loose body of lattice bricks with velocity v3 and spin w3 (setup_body, src/lib.rs:844-891); returns its slot */
@@ -288,6 +299,44 @@ void sf_legacy_dbg_wake_all(void);                      /* :107 */
 sf_lat* sf_legacy_world(int* scene);
 
 
+
+/* ---------------------------------------------------------------- served what-if stream (no counterpart in the
+ * reference, which answers one scene at a time on the player's machine). A query is a standing structure plus charges:
+ * load-path solve of the intact structure (its converged field becomes the scene's load field), the charges, then
+ * `frames` frames of the brick step; the answer gives joints failed by mode, the fragment table, bricks standing and
+ * pieces. Continuous batching: live queries are scenes of one lattice handle stepping in one batched frame; a query is
+ * admitted when a slot is free and its scene's resident bytes (sf_lat_bytes_for at its size) fit the HBM budget beside
+ * the live ones, its scene then takes ranges of its own size from the handle's pool and gives them back when it
+ * retires; admitted queries' load solves run as one batch on their own stream while the frames go on. An answer is
+ * the same alone and in any mix (every order in a frame is scene local). Calls on one server are not thread safe. */
+typedef struct sf_serve sf_serve;
+typedef struct {
+    uint64_t id;
+    int32_t bricks, standing, pieces, load_sweeps;
+    uint32_t failed[4];         /* joints failed: compression (crushed), tension, shear, footing */
+    float fragments[25];        /* per dominant material 1..5: pieces, bricks, largest, pieces of 50+, pieces of 1-2 */
+    double load_resid;          /* the load solve's final relative FP64 residual */
+    double latency_ms, queue_ms; /* submit to answer, submit to admission */
+} sf_answer;
+/* slots: scenes in flight at most; pool_bricks: the handle's brick pool shared by the live scenes; slot_bodies: body
+ * slots per scene; hbm_budget: bytes the live scenes may hold (<= 0: the whole pool's); max_solving: load-solve
+ * batches in flight */
+sf_serve* sf_serve_create(int slots, int pool_bricks, int slot_bodies, int frames, int64_t hbm_budget, int max_solving);
+void sf_serve_destroy(sf_serve* S);
+/* resident bytes a query of this many bricks would hold (what admission charges) */
+int64_t sf_serve_query_bytes(sf_serve* S, int bricks);
+/* queue a query (charges: 5 floats each, x y z radius power); SF_E_CAP at once for one that could never fit */
+int sf_serve_submit(sf_serve* S, uint64_t id, int n, const int32_t* xyz, const uint8_t* mat, const uint8_t* infill,
+                    int ncharge, const float* charges5, uint32_t seed);
+/* admit, start the admitted queries' load solves, bring in those solved, advance every live scene one frame, retire
+ * the finished; returns the live scenes */
+int sf_serve_tick(sf_serve* S, float dt);
+/* move up to cap finished answers out (oldest first); returns how many */
+int sf_serve_poll(sf_serve* S, sf_answer* out, int cap);
+/* queued, solving, live, answers waiting, bytes in use, budget, brick-steps, batched frames */
+int sf_serve_status(sf_serve* S, int64_t* out8);
+/* the server's lattice handle (read-only use: statistics, timing) */
+sf_lat* sf_serve_world(sf_serve* S);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,18 @@ _PROTOS = {
     "sf_lat_destroy": (None, [c_void_p]),
     "sf_lat_scene_add": (c_int, [c_void_p, c_int, c_int, c_int, c_uint]),
     "sf_lat_scene_clear": (c_int, [c_void_p, c_int, c_uint]),
+    "sf_lat_scene_release": (c_int, [c_void_p, c_int]),
+    "sf_lat_scene_place": (c_int, [c_void_p, c_int, c_int, c_int, c_uint]),
+    "sf_lat_bytes_resident": (c_int64, [c_void_p, c_int]),
+    "sf_lat_bytes_for": (c_int64, [c_void_p, c_int, c_int, c_int]),
+    "sf_serve_create": (c_void_p, [c_int, c_int, c_int, c_int, c_int64, c_int]),
+    "sf_serve_destroy": (None, [c_void_p]),
+    "sf_serve_query_bytes": (c_int64, [c_void_p, c_int]),
+    "sf_serve_submit": (c_int, [c_void_p, ctypes.c_uint64, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_uint]),
+    "sf_serve_tick": (c_int, [c_void_p, c_float]),
+    "sf_serve_poll": (c_int, [c_void_p, c_void_p, c_int]),
+    "sf_serve_status": (c_int, [c_void_p, c_void_p]),
+    "sf_serve_world": (c_void_p, [c_void_p]),
     "sf_lat_add_structure": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "sf_lat_add_body": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_lat_finalize": (c_int, [c_void_p, c_int]),

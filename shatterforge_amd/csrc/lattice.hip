@@ -1965,6 +1965,8 @@ struct Lat {
     int fuse_max = 32768;
     int lds_max = 0;           // LDS bytes a workgroup may have (device attribute)
     int64_t sor_lds = -1;      // caller's limit on the structures swept with their potential in LDS (-1: none)
+    size_t per_brick = 0, per_body = 0, per_struct = 0, per_contact = 0;  // device bytes per unit (dalloc tallies)
+    std::vector<std::pair<int, int>> free_b, free_d;  // unplaced brick and body ranges (start, length), by start
     bool lds_attr = false, lds_attr_fs = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
     std::vector<int32_t> slist, slist_up;  // scenes with contacts this substep; the list on the device
     Stage stg_ctl, stg_csr, stg_delta, stg_brec;  // uploads: sweep controls, dynamic brick lists, brick and body deltas
@@ -2056,7 +2058,10 @@ struct Lat {
         for (hipEvent_t e : {ev_sk[0], ev_sk[1], ev_stat}) if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
-    template <class T> int dalloc(T*& p, size_t n) {
+    // `tally` += the array's bytes per unit of its size (per brick, body, structure or contact): the resident bytes
+    // of a scene follow from its capacities (sf_lat_bytes_resident)
+    template <class T> int dalloc(T*& p, size_t n, size_t* tally = nullptr, size_t per = 0) {
+        if (tally) *tally += sizeof(T) * per;
         p = nullptr;
         if (hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)) != hipSuccess) { sf::set_error(SF_E_HIP, "hipMalloc failed"); return -1; }
         owned.push_back((void*)p);
@@ -2070,41 +2075,41 @@ int lat_alloc(Lat* L) {
     const size_t B = L->bcap, N = L->dcap, S = L->scap, NS = 256 * (size_t)S;
     LD& D = L->D;
     int r = 0;
-    r |= L->dalloc(D.mat, B); r |= L->dalloc(D.sid, B); r |= L->dalloc(D.body, B); r |= L->dalloc(D.bxyz, B);
-    r |= L->dalloc(D.nb6, 6 * B); r |= L->dalloc(D.bond, B); r |= L->dalloc(D.bmax, B); r |= L->dalloc(D.local, B);
-    r |= L->dalloc(D.wpos, B); r |= L->dalloc(D.phi, B); r |= L->dalloc(D.ext, B); r |= L->dalloc(D.extn, B);
-    r |= L->dalloc(D.stress, B); r |= L->dalloc(D.sup, B);
-    r |= L->dalloc(D.bd_scene, N); r |= L->dalloc(D.alive, N); r |= L->dalloc(D.asleep, N); r |= L->dalloc(D.wake, N);
-    r |= L->dalloc(D.pos, N); r |= L->dalloc(D.q, N); r |= L->dalloc(D.v, N); r |= L->dalloc(D.w, N);
-    r |= L->dalloc(D.vpre, N); r |= L->dalloc(D.wpre, N); r |= L->dalloc(D.vlast, N); r |= L->dalloc(D.wlast, N);
-    r |= L->dalloc(D.rot, 3 * N); r |= L->dalloc(D.iil, 3 * N); r |= L->dalloc(D.iiw, 3 * N);
-    r |= L->dalloc(D.m, N); r |= L->dalloc(D.im, N); r |= L->dalloc(D.sleep, N); r |= L->dalloc(D.rad, N);
-    r |= L->dalloc(D.cool, N); r |= L->dalloc(D.lt, N); r |= L->dalloc(D.resid, N); r |= L->dalloc(D.maxr, N);
-    r |= L->dalloc(D.nmax, N); r |= L->dalloc(D.nc, N); r |= L->dalloc(D.cfirst, N); r |= L->dalloc(D.cend, N); r |= L->dalloc(D.count, N); r |= L->dalloc(D.boff, N);
-    r |= L->dalloc(D.bcnt, N); r |= L->dalloc(D.ubody, N);
-    r |= L->dalloc(D.dyn, B); r |= L->dalloc(D.dyn_body, B);
-    r |= L->dalloc(D.st_scene, NS); r |= L->dalloc(D.st_lo, NS); r |= L->dalloc(D.st_hi, NS); r |= L->dalloc(D.st_act, NS);
-    r |= L->dalloc(D.st_on, NS); r |= L->dalloc(D.st_live, NS); r |= L->dalloc(D.st_nmin, NS); r |= L->dalloc(D.st_resid, NS);
-    r |= L->dalloc(D.st_maxr, NS); r |= L->dalloc(D.st_flags, NS); r |= L->dalloc(D.st_red, B); r |= L->dalloc(D.st_rb, NS);
-    r |= L->dalloc(D.sten_na, B); r |= L->dalloc(D.sten_nb, B); r |= L->dalloc(D.sten_ka, B); r |= L->dalloc(D.sten_kb, B); r |= L->dalloc(D.sten_s, B);
-    r |= L->dalloc(D.sten_pk, B);
+    r |= L->dalloc(D.mat, B, &L->per_brick, 1); r |= L->dalloc(D.sid, B, &L->per_brick, 1); r |= L->dalloc(D.body, B, &L->per_brick, 1); r |= L->dalloc(D.bxyz, B, &L->per_brick, 1);
+    r |= L->dalloc(D.nb6, 6 * B, &L->per_brick, 6); r |= L->dalloc(D.bond, B, &L->per_brick, 1); r |= L->dalloc(D.bmax, B, &L->per_brick, 1); r |= L->dalloc(D.local, B, &L->per_brick, 1);
+    r |= L->dalloc(D.wpos, B, &L->per_brick, 1); r |= L->dalloc(D.phi, B, &L->per_brick, 1); r |= L->dalloc(D.ext, B, &L->per_brick, 1); r |= L->dalloc(D.extn, B, &L->per_brick, 1);
+    r |= L->dalloc(D.stress, B, &L->per_brick, 1); r |= L->dalloc(D.sup, B, &L->per_brick, 1);
+    r |= L->dalloc(D.bd_scene, N, &L->per_body, 1); r |= L->dalloc(D.alive, N, &L->per_body, 1); r |= L->dalloc(D.asleep, N, &L->per_body, 1); r |= L->dalloc(D.wake, N, &L->per_body, 1);
+    r |= L->dalloc(D.pos, N, &L->per_body, 1); r |= L->dalloc(D.q, N, &L->per_body, 1); r |= L->dalloc(D.v, N, &L->per_body, 1); r |= L->dalloc(D.w, N, &L->per_body, 1);
+    r |= L->dalloc(D.vpre, N, &L->per_body, 1); r |= L->dalloc(D.wpre, N, &L->per_body, 1); r |= L->dalloc(D.vlast, N, &L->per_body, 1); r |= L->dalloc(D.wlast, N, &L->per_body, 1);
+    r |= L->dalloc(D.rot, 3 * N, &L->per_body, 3); r |= L->dalloc(D.iil, 3 * N, &L->per_body, 3); r |= L->dalloc(D.iiw, 3 * N, &L->per_body, 3);
+    r |= L->dalloc(D.m, N, &L->per_body, 1); r |= L->dalloc(D.im, N, &L->per_body, 1); r |= L->dalloc(D.sleep, N, &L->per_body, 1); r |= L->dalloc(D.rad, N, &L->per_body, 1);
+    r |= L->dalloc(D.cool, N, &L->per_body, 1); r |= L->dalloc(D.lt, N, &L->per_body, 1); r |= L->dalloc(D.resid, N, &L->per_body, 1); r |= L->dalloc(D.maxr, N, &L->per_body, 1);
+    r |= L->dalloc(D.nmax, N, &L->per_body, 1); r |= L->dalloc(D.nc, N, &L->per_body, 1); r |= L->dalloc(D.cfirst, N, &L->per_body, 1); r |= L->dalloc(D.cend, N, &L->per_body, 1); r |= L->dalloc(D.count, N, &L->per_body, 1); r |= L->dalloc(D.boff, N, &L->per_body, 1);
+    r |= L->dalloc(D.bcnt, N, &L->per_body, 1); r |= L->dalloc(D.ubody, N, &L->per_body, 1);
+    r |= L->dalloc(D.dyn, B, &L->per_brick, 1); r |= L->dalloc(D.dyn_body, B, &L->per_brick, 1);
+    r |= L->dalloc(D.st_scene, NS, &L->per_struct, 1); r |= L->dalloc(D.st_lo, NS, &L->per_struct, 1); r |= L->dalloc(D.st_hi, NS, &L->per_struct, 1); r |= L->dalloc(D.st_act, NS, &L->per_struct, 1);
+    r |= L->dalloc(D.st_on, NS, &L->per_struct, 1); r |= L->dalloc(D.st_live, NS, &L->per_struct, 1); r |= L->dalloc(D.st_nmin, NS, &L->per_struct, 1); r |= L->dalloc(D.st_resid, NS, &L->per_struct, 1);
+    r |= L->dalloc(D.st_maxr, NS, &L->per_struct, 1); r |= L->dalloc(D.st_flags, NS, &L->per_struct, 1); r |= L->dalloc(D.st_red, B, &L->per_brick, 1); r |= L->dalloc(D.st_rb, NS, &L->per_struct, 1);
+    r |= L->dalloc(D.sten_na, B, &L->per_brick, 1); r |= L->dalloc(D.sten_nb, B, &L->per_brick, 1); r |= L->dalloc(D.sten_ka, B, &L->per_brick, 1); r |= L->dalloc(D.sten_kb, B, &L->per_brick, 1); r |= L->dalloc(D.sten_s, B, &L->per_brick, 1);
+    r |= L->dalloc(D.sten_pk, B, &L->per_brick, 1);
     r |= L->dalloc(D.sc_gmax, S); r |= L->dalloc(D.sc_fail, 4 * S);
-    r |= L->dalloc(D.dslot, B); r |= L->dalloc(D.dkey, B); r |= L->dalloc(D.hent, B);
-    r |= L->dalloc(D.ccnt, B); r |= L->dalloc(D.coff, B);
-    r |= L->dalloc(D.mask, N); r |= L->dalloc(D.claim, N * (size_t)L->K);
+    r |= L->dalloc(D.dslot, B, &L->per_brick, 1); r |= L->dalloc(D.dkey, B, &L->per_brick, 1); r |= L->dalloc(D.hent, B, &L->per_brick, 1);
+    r |= L->dalloc(D.ccnt, B, &L->per_brick, 1); r |= L->dalloc(D.coff, B, &L->per_brick, 1);
+    r |= L->dalloc(D.mask, N, &L->per_body, 1); r |= L->dalloc(D.claim, N * (size_t)L->K, &L->per_body, (size_t)L->K);
     r |= L->dalloc(D.colfill, KMAX + 1); r |= L->dalloc(D.coloff, KMAX + 1);
-    r |= L->dalloc(D.ovcnt, N); r |= L->dalloc(D.ovoff, N); r |= L->dalloc(D.ovfill, N); r |= L->dalloc(D.ovbody, N);
-    r |= L->dalloc(D.lcnt, B); r |= L->dalloc(D.loff, B); r |= L->dalloc(D.lfill, B); r |= L->dalloc(D.lent, B);
-    r |= L->dalloc(D.ltarget, B); r |= L->dalloc(D.lshare, B); r |= L->dalloc(D.bshare, N);
-    r |= L->dalloc(D.bd_sor, N); r |= L->dalloc(D.bd_judge, N); r |= L->dalloc(D.bd_live, N); r |= L->dalloc(D.bd_dt, N);
-    r |= L->dalloc(D.bd_dirty, N);
+    r |= L->dalloc(D.ovcnt, N, &L->per_body, 1); r |= L->dalloc(D.ovoff, N, &L->per_body, 1); r |= L->dalloc(D.ovfill, N, &L->per_body, 1); r |= L->dalloc(D.ovbody, N, &L->per_body, 1);
+    r |= L->dalloc(D.lcnt, B, &L->per_brick, 1); r |= L->dalloc(D.loff, B, &L->per_brick, 1); r |= L->dalloc(D.lfill, B, &L->per_brick, 1); r |= L->dalloc(D.lent, B, &L->per_brick, 1);
+    r |= L->dalloc(D.ltarget, B, &L->per_brick, 1); r |= L->dalloc(D.lshare, B, &L->per_brick, 1); r |= L->dalloc(D.bshare, N, &L->per_body, 1);
+    r |= L->dalloc(D.bd_sor, N, &L->per_body, 1); r |= L->dalloc(D.bd_judge, N, &L->per_body, 1); r |= L->dalloc(D.bd_live, N, &L->per_body, 1); r |= L->dalloc(D.bd_dt, N, &L->per_body, 1);
+    r |= L->dalloc(D.bd_dirty, N, &L->per_body, 1);
     // the counters, then each scene's first contact: one copy brings back both
     r |= L->dalloc(D.ctr, CTR_WORDS + S + 1);
     D.sc_c0 = D.ctr + CTR_WORDS;
     D.pcap = (uint32_t)std::max<size_t>(4096, B);
-    r |= L->dalloc(D.plist, D.pcap); r |= L->dalloc(D.ilist, D.pcap); r |= L->dalloc(D.klist, N);
+    r |= L->dalloc(D.plist, D.pcap, &L->per_brick, 1); r |= L->dalloc(D.ilist, D.pcap, &L->per_brick, 1); r |= L->dalloc(D.klist, N, &L->per_body, 1);
     D.evcap = (uint32_t)std::max<size_t>(4096, B); D.bwcap = (uint32_t)std::max<size_t>(4096, 4 * B);
-    r |= L->dalloc(D.evkey, D.evcap); r |= L->dalloc(D.ev, D.evcap); r |= L->dalloc(D.bwi, D.bwcap); r |= L->dalloc(D.bwv, D.bwcap);
+    r |= L->dalloc(D.evkey, D.evcap, &L->per_brick, 1); r |= L->dalloc(D.ev, D.evcap, &L->per_brick, 1); r |= L->dalloc(D.bwi, D.bwcap, &L->per_brick, 4); r |= L->dalloc(D.bwv, D.bwcap, &L->per_brick, 4);
     r |= L->dalloc(D.scan_tmp, sf::scan_scratch_words((uint32_t)std::max<size_t>(std::max(2 * B + 4096, N), 1u << 16)));
     r |= L->d_sc.alloc(S, L->st); r |= L->d_scstat.alloc(3 * S, L->st); r |= L->d_slist.alloc(S, L->st); r |= L->d_rec.alloc(D.pcap, L->st); r |= L->d_dis.alloc(D.pcap, L->st);
     D.sc = L->d_sc.p;
@@ -2122,18 +2127,19 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     fr(D.csmn); fr(D.csmt1); fr(D.csmt2); fr(D.collist); fr(D.ovent); fr(D.odv); fr(D.odw);
     fr(D.rq_c); fr(D.rq_ab); fr(D.rq_0); fr(D.rq_1); fr(D.rq_2); fr(D.rq_3); fr(D.rq_4); fr(D.rq_5);
     int r = 0;
-    r |= L->dalloc(D.ca, cap); r |= L->dalloc(D.cb, cap); r |= L->dalloc(D.cbo, cap); r |= L->dalloc(D.cscene, cap);
-    r |= L->dalloc(D.ccol, cap); r |= L->dalloc(D.cprop, cap); r |= L->dalloc(D.cba, cap); r |= L->dalloc(D.cbb, cap);
-    r |= L->dalloc(D.con, cap);
-    r |= L->dalloc(D.cn, cap); r |= L->dalloc(D.cp, cap); r |= L->dalloc(D.cra, cap); r |= L->dalloc(D.crb, cap);
-    r |= L->dalloc(D.ct1, cap); r |= L->dalloc(D.ct2, cap);
-    r |= L->dalloc(D.cdepth, cap); r |= L->dalloc(D.cmn, cap); r |= L->dalloc(D.cmt1, cap); r |= L->dalloc(D.cmt2, cap);
-    r |= L->dalloc(D.cjn, cap); r |= L->dalloc(D.cjt1, cap); r |= L->dalloc(D.cjt2, cap); r |= L->dalloc(D.cbias, cap);
-    r |= L->dalloc(D.cvn0, cap); r |= L->dalloc(D.csmn, cap); r |= L->dalloc(D.csmt1, cap); r |= L->dalloc(D.csmt2, cap);
-    r |= L->dalloc(D.collist, cap); r |= L->dalloc(D.ovent, 2 * (size_t)cap);
-    r |= L->dalloc(D.odv, 2 * (size_t)cap); r |= L->dalloc(D.odw, 2 * (size_t)cap);
-    r |= L->dalloc(D.rq_c, cap); r |= L->dalloc(D.rq_ab, cap); r |= L->dalloc(D.rq_0, cap); r |= L->dalloc(D.rq_1, cap);
-    r |= L->dalloc(D.rq_2, cap); r |= L->dalloc(D.rq_3, cap); r |= L->dalloc(D.rq_4, cap); r |= L->dalloc(D.rq_5, cap);
+    L->per_contact = 0;
+    r |= L->dalloc(D.ca, cap, &L->per_contact, 1); r |= L->dalloc(D.cb, cap, &L->per_contact, 1); r |= L->dalloc(D.cbo, cap, &L->per_contact, 1); r |= L->dalloc(D.cscene, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.ccol, cap, &L->per_contact, 1); r |= L->dalloc(D.cprop, cap, &L->per_contact, 1); r |= L->dalloc(D.cba, cap, &L->per_contact, 1); r |= L->dalloc(D.cbb, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.con, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.cn, cap, &L->per_contact, 1); r |= L->dalloc(D.cp, cap, &L->per_contact, 1); r |= L->dalloc(D.cra, cap, &L->per_contact, 1); r |= L->dalloc(D.crb, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.ct1, cap, &L->per_contact, 1); r |= L->dalloc(D.ct2, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.cdepth, cap, &L->per_contact, 1); r |= L->dalloc(D.cmn, cap, &L->per_contact, 1); r |= L->dalloc(D.cmt1, cap, &L->per_contact, 1); r |= L->dalloc(D.cmt2, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.cjn, cap, &L->per_contact, 1); r |= L->dalloc(D.cjt1, cap, &L->per_contact, 1); r |= L->dalloc(D.cjt2, cap, &L->per_contact, 1); r |= L->dalloc(D.cbias, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.cvn0, cap, &L->per_contact, 1); r |= L->dalloc(D.csmn, cap, &L->per_contact, 1); r |= L->dalloc(D.csmt1, cap, &L->per_contact, 1); r |= L->dalloc(D.csmt2, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.collist, cap, &L->per_contact, 1); r |= L->dalloc(D.ovent, 2 * (size_t)cap, &L->per_contact, 2);
+    r |= L->dalloc(D.odv, 2 * (size_t)cap, &L->per_contact, 2); r |= L->dalloc(D.odw, 2 * (size_t)cap, &L->per_contact, 2);
+    r |= L->dalloc(D.rq_c, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_ab, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_0, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_1, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.rq_2, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_3, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_4, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_5, cap, &L->per_contact, 1);
     D.ccap = cap;
     L->ccap = cap;
     return r ? -1 : 0;
@@ -3595,6 +3601,42 @@ int add_body(Lat* L, int si, int n, const int32_t* xyz, const uint8_t* mat, cons
     return b - S.bd0;
 }
 
+// First-fit placement of a scene's brick or body range in the handle's pool of free ranges (sorted by start) and
+// the return of a range, merged with its free neighbours. The handle's arrays are sized once; scenes come and go.
+int pool_take(std::vector<std::pair<int, int>>& fl, int n) {
+    for (size_t k = 0; k < fl.size(); k++) {
+        if (fl[k].second < n) continue;
+        const int s = fl[k].first;
+        fl[k].first += n; fl[k].second -= n;
+        if (!fl[k].second) fl.erase(fl.begin() + (long)k);
+        return s;
+    }
+    return -1;
+}
+void pool_give(std::vector<std::pair<int, int>>& fl, int s, int n) {
+    if (n <= 0) return;
+    auto it = std::lower_bound(fl.begin(), fl.end(), std::make_pair(s, 0));
+    it = fl.insert(it, std::make_pair(s, n));
+    if (it + 1 != fl.end() && it->first + it->second == (it + 1)->first) { it->second += (it + 1)->second; fl.erase(it + 1); }
+    if (it != fl.begin() && (it - 1)->first + (it - 1)->second == it->first) { (it - 1)->second += it->second; fl.erase(it); }
+}
+// a scene's ranges marked as its own on the device and the host: scene of each body slot, scene of each brick,
+// no structure yet
+int claim_ranges(Lat* L, int si) {
+    HScene& S = L->sc[si];
+    LD& D = L->D;
+    L->nb = std::max(L->nb, S.b0 + S.bcap); L->nbd = std::max(L->nbd, S.bd0 + S.dcap);
+    D.nb = (uint32_t)L->nb; D.nbd = (uint32_t)L->nbd;
+    std::vector<int32_t> bsc(S.dcap, si);
+    if (S.dcap && cp_up(D.bd_scene + S.bd0, bsc.data(), S.dcap, L->st)) return -1;
+    std::vector<int4> bx(S.bcap, make_int4(0, 0, 0, si));
+    if (S.bcap && cp_up(D.bxyz + S.b0, bx.data(), S.bcap, L->st)) return -1;
+    std::vector<uint8_t> sids(S.bcap, 255);
+    if (S.bcap && cp_up(D.sid + S.b0, sids.data(), S.bcap, L->st)) return -1;
+    for (int k = 0; k < S.bcap; k++) { L->bxyz[S.b0 + k] = make_int4(0, 0, 0, si); L->sid[S.b0 + k] = 255; }
+    return 0;
+}
+
 void clear_scene(Lat* L, int si, uint32_t seed) {
     HScene& S = L->sc[si];
     // the search stamps restart with the scene's stamp counter, so the old ones must go
@@ -3637,6 +3679,8 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     Lat* L = &h->h.L;
     SF_HIP_NULL(hipGetDevice(&L->dev));
     L->bcap = brick_cap; L->dcap = body_cap; L->scap = max_scenes; L->K = colours; L->R = rounds;
+    L->free_b.assign(1, std::make_pair(0, brick_cap));
+    L->free_d.assign(1, std::make_pair(0, body_cap));
     if (hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) != hipSuccess) { delete h; sf::set_error(SF_E_HIP, "stream"); return nullptr; }
     if (lat_alloc(L) || lat_contacts_alloc(L, (uint32_t)std::max<int>(1 << 16, 4 * brick_cap)) || lat_hash_alloc(L, 1024) || L->d_grid.alloc(1, L->st)) { delete h; return nullptr; }
     L->D.grid = L->d_grid.p;
@@ -3665,29 +3709,69 @@ void sf_lat_destroy(sf_lat* h) { delete h; }
 int sf_lat_scene_add(sf_lat* h, int brick_cap, int body_cap, int max_structs, uint32_t seed) {
     if (!h || brick_cap <= 0 || body_cap <= 0 || max_structs < 0 || max_structs > 255) { sf::set_error(SF_E_ARG, "bad scene capacities"); return SF_E_ARG; }
     Lat* L = &h->h.L;
-    if ((int)L->sc.size() >= L->scap || L->nb + brick_cap > L->bcap || L->nbd + body_cap > L->dcap || (L->sts.size() + max_structs) > 256 * (size_t)L->scap) {
+    if ((int)L->sc.size() >= L->scap || (L->sts.size() + max_structs) > 256 * (size_t)L->scap) {
         sf::set_error(SF_E_CAP, "world capacity exceeded"); return SF_E_CAP;
     }
+    const int b0 = pool_take(L->free_b, brick_cap);
+    if (b0 < 0) { sf::set_error(SF_E_CAP, "world brick capacity exceeded"); return SF_E_CAP; }
+    const int bd0 = pool_take(L->free_d, body_cap);
+    if (bd0 < 0) { pool_give(L->free_b, b0, brick_cap); sf::set_error(SF_E_CAP, "world body capacity exceeded"); return SF_E_CAP; }
     HScene S;
-    S.b0 = L->nb; S.bcap = brick_cap; S.bd0 = L->nbd; S.dcap = body_cap; S.st0 = (int)L->sts.size();
+    S.b0 = b0; S.bcap = brick_cap; S.bd0 = bd0; S.dcap = body_cap; S.st0 = (int)L->sts.size();
     S.rng = seed ? seed : 0x9E3779B9u;
     int si = (int)L->sc.size();
     L->sc.push_back(S);
     h->h.smax.push_back(max_structs);
     for (int k = 0; k < max_structs; k++) { HStruct T; T.scene = si; L->sts.push_back(T); }
-    L->nb += brick_cap; L->nbd += body_cap;
     LD& D = L->D;
-    D.nb = (uint32_t)L->nb; D.nbd = (uint32_t)L->nbd; D.nscene = (uint32_t)L->sc.size(); D.nst = (uint32_t)L->sts.size();
-    std::vector<int32_t> bsc(body_cap, si);
-    if (cp_up(D.bd_scene + S.bd0, bsc.data(), body_cap, L->st)) return SF_E_HIP;
-    std::vector<int4> bx(brick_cap, make_int4(0, 0, 0, si));
-    if (cp_up(D.bxyz + S.b0, bx.data(), brick_cap, L->st)) return SF_E_HIP;
-    std::vector<uint8_t> sids(brick_cap, 255);
-    if (cp_up(D.sid + S.b0, sids.data(), brick_cap, L->st)) return SF_E_HIP;
-    for (int k = 0; k < brick_cap; k++) L->bxyz[S.b0 + k] = make_int4(0, 0, 0, si);
+    D.nscene = (uint32_t)L->sc.size(); D.nst = (uint32_t)L->sts.size();
+    if (claim_ranges(L, si)) return SF_E_HIP;
     SF_HIP(hipStreamSynchronize(L->st));
     if (upload_scenes(L)) return SF_E_HIP;
     return si;
+}
+int sf_lat_scene_release(sf_lat* h, int scene) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    int r = sf_lat_scene_clear(h, scene, 1);
+    if (r < 0) return r;
+    HScene& S = L->sc[scene];
+    pool_give(L->free_b, S.b0, S.bcap);
+    pool_give(L->free_d, S.bd0, S.dcap);
+    S.bcap = 0; S.dcap = 0;
+    L->ub_dirty = true;
+    L->lists_dirty = true;
+    if (rebuild_csr(L) || upload_scenes(L)) return SF_E_HIP;
+    return 0;
+}
+int sf_lat_scene_place(sf_lat* h, int scene, int brick_cap, int body_cap, uint32_t seed) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size() || brick_cap <= 0 || body_cap <= 0) { sf::set_error(SF_E_ARG, "bad scene or capacities"); return SF_E_ARG; }
+    Lat* L = &h->h.L;
+    HScene& S = L->sc[scene];
+    if (S.bcap || S.dcap) { sf::set_error(SF_E_ARG, "scene still placed: release it first"); return SF_E_ARG; }
+    const int b0 = pool_take(L->free_b, brick_cap);
+    if (b0 < 0) { sf::set_error(SF_E_CAP, "no free brick range that large"); return SF_E_CAP; }
+    const int bd0 = pool_take(L->free_d, body_cap);
+    if (bd0 < 0) { pool_give(L->free_b, b0, brick_cap); sf::set_error(SF_E_CAP, "no free body range that large"); return SF_E_CAP; }
+    S.b0 = b0; S.bcap = brick_cap; S.bd0 = bd0; S.dcap = body_cap;
+    if (claim_ranges(L, scene)) return SF_E_HIP;
+    SF_HIP(hipStreamSynchronize(L->st));
+    return sf_lat_scene_clear(h, scene, seed);
+}
+int64_t sf_lat_bytes_for(sf_lat* h, int brick_cap, int body_cap, int max_structs) {
+    if (!h || brick_cap < 0 || body_cap < 0 || max_structs < 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    const Lat* L = &h->h.L;
+    return (int64_t)brick_cap * (int64_t)(L->per_brick + 4 * L->per_contact) + (int64_t)body_cap * (int64_t)L->per_body +
+           (int64_t)max_structs * (int64_t)L->per_struct;
+}
+int64_t sf_lat_bytes_resident(sf_lat* h, int scene) {
+    if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }
+    const Lat* L = &h->h.L;
+    const HScene& S = L->sc[scene];
+    // its bricks' and bodies' shares of every per-brick and per-body array, four contacts per brick (the contact
+    // arrays' sizing), its structures' rows and its static grid
+    return (int64_t)S.bcap * (int64_t)(L->per_brick + 4 * L->per_contact) + (int64_t)S.dcap * (int64_t)L->per_body +
+           (int64_t)h->h.smax[scene] * (int64_t)L->per_struct + (int64_t)S.gcap * (int64_t)sizeof(int32_t);
 }
 int sf_lat_scene_clear(sf_lat* h, int scene, uint32_t seed) {
     if (!h || scene < 0 || scene >= (int)h->h.L.sc.size()) { sf::set_error(SF_E_ARG, "bad scene"); return SF_E_ARG; }

@@ -1,7 +1,8 @@
 """Served what-if stream: demolition queries answered by continuous batching on one GPU.
 
-A query is a seeded frame/wall building of 2k-40k bricks plus 1-8 charges at its ground-floor column bases. It is
-answered in three parts, all on the GPU:
+The server itself is the library's (sf_serve_*, shatterforge_amd/csrc/serve.cpp); this module generates the queries
+(seeded frame/wall buildings of 2k-40k bricks with 1-8 charges at ground-floor column bases), drives a Poisson
+arrival stream in real time and binds the server. A query is answered in three parts, all on the GPU:
   1. the load-path solve of the intact building (FP32 red-black sweeps, FP64 residual and refinement, sf_load);
      the converged field becomes the scene's load field, so the structure starts solved and settled;
   2. the charges (explode_at, src/lib.rs:2697-2716);
@@ -10,26 +11,22 @@ The answer: joints failed by mode (compression, tension, shear, footing), the fr
 (pieces, bricks, largest, pieces of 50+ bricks, crumbs of 1-2 bricks; tools/debug_exports.rs:41-56), bricks still
 standing and pieces made.
 
-Continuous batching: every query lives in a scene slot of one lattice world; all live slots advance together in one
+Continuous batching: every query lives in a scene of one lattice world; all live scenes advance together in one
 batched frame. Queries that arrive while others are on the GPU are admitted as soon as a slot is free and their
-resident bytes fit the HBM budget, and leave the batch the frame they finish, so scenes of uneven size and age share
-every launch. Each scene's result does not depend on the others in the batch (every order and priority in a frame
-is scene local), which the tests check.
+scene's resident bytes fit the HBM budget, take brick and body ranges of their own size from the world's pool, and
+leave the batch the frame they finish. Each scene's result does not depend on the others in the batch (every order
+and priority in a frame is scene local), which the tests check.
 """
+import ctypes
 import time
 from collections import deque
 
 import numpy as np
 
 from . import scenes
-from .lattice import Lattice
-from .load import LoadBatch
+from ._native import check, lib, ptr
 
 C = 320  # lattice index of x = 0 and z = 0
-
-# device bytes a resident scene holds per brick and per body slot (lattice world arrays, contacts at 4 per brick)
-BYTES_PER_BRICK = 176 + 4 * 196
-BYTES_PER_BODY = 420
 
 _SIZE_TABLE = None
 
@@ -80,120 +77,90 @@ def make_query(qid, seed, n_min=2000, n_max=40000):
                 rng_seed=bseed & 0xFFFFFFFF or 1)
 
 
+class _Answer(ctypes.Structure):
+    """sf_answer (include/sf.h)"""
+    _fields_ = [("id", ctypes.c_uint64), ("bricks", ctypes.c_int32), ("standing", ctypes.c_int32),
+                ("pieces", ctypes.c_int32), ("load_sweeps", ctypes.c_int32), ("failed", ctypes.c_uint32 * 4),
+                ("fragments", ctypes.c_float * 25), ("load_resid", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("queue_ms", ctypes.c_double)]
+
+
 class WhatIfServer:
-    """Continuous-batching server over `slots` resident scene slots of `slot_bricks` bricks each."""
+    """The library's continuous-batching server (sf_serve_*): `slots` scenes in flight at most, a brick pool of
+    slots x slot_bricks shared by them, admission against `hbm_budget` bytes of resident scenes (default: the
+    whole pool's)."""
 
     def __init__(self, slots=64, slot_bricks=40960, slot_bodies=8192, frames=300, hbm_budget=None, dt=1.0 / 60.0,
                  max_solving=3):
         self.frames = frames
-        self.max_solving = max_solving
         self.dt = dt
         self.slot_bricks = slot_bricks
-        self.lat = Lattice(slots * slot_bricks, slots * slot_bodies, scenes=slots)
-        self.slot = [self.lat.add_scene(slot_bricks, slot_bodies, structs=2) for _ in range(slots)]
-        self.free = deque(range(slots))
-        self.budget = hbm_budget if hbm_budget is not None else slots * (slot_bricks * BYTES_PER_BRICK + slot_bodies * BYTES_PER_BODY)
-        self.in_use = 0
-        self.live = {}          # slot -> query state
-        self.queue = deque()    # admitted to the server, waiting for a slot
-        self.pending = []       # (slot and query list, load batch) whose load-path solve is running
+        self._h = lib().sf_serve_create(int(slots), int(slots * slot_bricks), int(slot_bodies), int(frames),
+                                        int(hbm_budget or 0), int(max_solving))
+        if not self._h:
+            check(-1)
         self.done = []
-        self.brick_steps = 0
-        self.frames_run = 0
+        self._by_id = {}
 
-    @staticmethod
-    def query_bytes(q):
-        return q["bricks"] * BYTES_PER_BRICK + 4096 * BYTES_PER_BODY
+    def query_bytes(self, q):
+        """resident bytes the query's scene would hold: what admission charges against the budget"""
+        return check(lib().sf_serve_query_bytes(self._h, int(q["bricks"])))
 
-    def submit(self, q, t_arrival):
-        """queue a query; one that can never be admitted (more bricks than a slot holds, or more resident bytes than
-        the whole budget) is refused here with ValueError, before it reaches the queue"""
-        if q["bricks"] > self.slot_bricks:
-            raise ValueError(f"query {q.get('id')}: {q['bricks']} bricks, a slot holds {self.slot_bricks}")
-        if self.query_bytes(q) > self.budget:
-            raise ValueError(f"query {q.get('id')}: {self.query_bytes(q)} resident bytes, the budget is {self.budget}")
-        q = dict(q)
-        q["t_arrival"] = t_arrival
-        self.queue.append(q)
+    def status(self):
+        out = np.zeros(8, np.int64)
+        check(lib().sf_serve_status(self._h, ptr(out)))
+        return dict(zip(["queued", "solving", "live", "answers", "bytes_in_use", "budget", "brick_steps", "frames"],
+                        out.tolist()))
 
-    def _admit(self, now):
-        """slots and HBM bytes permitting, queued queries take a slot and their load-path solves start on the load
-        batch's own stream; the scenes already in flight keep stepping meanwhile. At most `max_solving` solve
-        batches run at once (each on a stream of its own beside the frame's); later arrivals form the next batch"""
-        if len(self.pending) >= self.max_solving:
-            return
-        batch = []
-        while self.queue and self.free and self.in_use + self.query_bytes(self.queue[0]) <= self.budget:
-            q = self.queue.popleft()
-            s = self.free.popleft()
-            self.in_use += self.query_bytes(q)
-            batch.append((s, q))
-        if not batch:
-            return
-        load = LoadBatch()
-        for s, q in batch:
-            load.add(q["mat"], q["xyz"], q["infill"])
-        load.solve_begin(rel_tol=1e-6)
-        self.pending.append((batch, load))
+    @property
+    def brick_steps(self):
+        return self.status()["brick_steps"]
 
-    def _activate(self, now, wait=False):
-        """queries whose load-path solve has finished join the batch: scene, converged field, charges"""
-        keep = []
-        for batch, load in self.pending:
-            if not (wait or load.ready()):
-                keep.append((batch, load))
-                continue
-            load.solve_end()
-            for k, (s, q) in enumerate(batch):
-                self.lat.clear_scene(s, q["rng_seed"])
-                q["structure"] = self.lat.add_structure(s, q["mat"], q["xyz"], q["infill"])
-                self.lat.finalize(s)
-                inf = load.info(k)
-                q["load_resid"] = inf["resid"]
-                q["load_sweeps"] = inf["sweeps"]
-                self.lat.set_field(s, q["structure"], load.field(k).astype(np.float32))
-                for ch in q["charges"]:
-                    self.lat.explode(s, *ch)
-                q["t_start"] = now
-                q["frame"] = 0
-                self.live[s] = q
-            load.close()
-        self.pending = keep
+    @property
+    def frames_run(self):
+        return self.status()["frames"]
 
-    def _retire(self, now):
-        for s in [s for s, q in self.live.items() if q["frame"] >= self.frames]:
-            q = self.live.pop(s)
-            st = self.lat.stats(s)
-            ms = self.lat.matstats(s).reshape(10, 5)
-            q["result"] = dict(failed=self.lat.failures(s).tolist(), fragments=ms[1:6].tolist(),
-                               standing=int(st[0]), pieces=int(st[2]))
-            q["t_done"] = time.perf_counter()
-            q["latency"] = q["t_done"] - q["t_arrival"]
-            for k in ("mat", "xyz", "infill"):
-                q.pop(k, None)
-            self.done.append(q)
-            self.in_use -= self.query_bytes(q)
-            self.lat.clear_scene(s, 1)  # an empty slot costs the batched frame nothing
-            self.free.append(s)
+    def submit(self, q, t_arrival=None):
+        """queue a query; one that could never be admitted (its scene's bytes beyond the budget) is refused here
+        with ValueError"""
+        xyz = np.ascontiguousarray(q["xyz"], np.int32)
+        mat = np.ascontiguousarray(q["mat"], np.uint8)
+        inf = np.ascontiguousarray(q["infill"], np.uint8)
+        ch = np.ascontiguousarray(np.array(q["charges"], np.float32).reshape(-1, 5))
+        rc = lib().sf_serve_submit(self._h, int(q["id"]), len(mat), ptr(xyz), ptr(mat), ptr(inf), len(ch), ptr(ch),
+                                   int(q.get("rng_seed", 1)))
+        if rc == -3:
+            raise ValueError(lib().sf_last_error_message().decode())
+        check(rc)
+        self._by_id[int(q["id"])] = {k: v for k, v in q.items() if k not in ("mat", "xyz", "infill")}
+
+    def _collect(self):
+        buf = (_Answer * 64)()
+        while True:
+            n = check(lib().sf_serve_poll(self._h, ctypes.cast(buf, ctypes.c_void_p), 64))
+            for a in buf[:n]:
+                q = dict(self._by_id.pop(int(a.id), {"id": int(a.id)}))
+                q["result"] = dict(failed=list(a.failed), fragments=np.array(a.fragments, np.float32).reshape(5, 5).tolist(),
+                                   standing=int(a.standing), pieces=int(a.pieces))
+                q["latency"] = a.latency_ms * 1e-3
+                q["queue"] = a.queue_ms * 1e-3
+                q["load_resid"] = a.load_resid
+                q["load_sweeps"] = int(a.load_sweeps)
+                self.done.append(q)
+            if n < 64:
+                return
 
     def busy(self):
         """queries waiting for a slot, solving their load path, or stepping"""
-        return bool(self.queue or self.pending or self.live)
+        st = self.status()
+        return bool(st["queued"] or st["solving"] or st["live"])
 
     def tick(self):
-        """admit what can be admitted, activate queries whose solve finished, advance every live scene one frame,
-        retire finished queries"""
-        now = time.perf_counter()
-        self._admit(now)
-        self._activate(now, wait=not self.live)
-        if self.live:
-            self.lat.step(self.dt)
-            self.frames_run += 1
-            for q in self.live.values():
-                q["frame"] += 1
-                self.brick_steps += q["bricks"]
-        self._retire(time.perf_counter())
-        return len(self.live)
+        """admit what can be admitted, bring in queries whose solve finished, advance every live scene one frame,
+        retire finished queries; returns the live scenes"""
+        n = check(lib().sf_serve_tick(self._h, ctypes.c_float(self.dt)))
+        self._collect()
+        return n
 
     def run_stream(self, queries, arrivals):
         """serve queries arriving at the given times (s, from now) in real time; returns the finished queries"""
@@ -202,8 +169,8 @@ class WhatIfServer:
         while pending or self.busy():
             now = time.perf_counter() - t0
             while pending and pending[0][0] <= now:
-                ta, i = pending.popleft()
-                self.submit(queries[i], t0 + ta)
+                _, i = pending.popleft()
+                self.submit(queries[i])
             if not self.busy() and pending:
                 time.sleep(max(0.0, min(pending[0][0] - now, 0.01)))
                 continue
@@ -211,10 +178,15 @@ class WhatIfServer:
         return self.done
 
     def close(self):
-        for _, load in self.pending:
-            load.close()
-        self.pending = []
-        self.lat.close()
+        if getattr(self, "_h", None):
+            lib().sf_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def poisson_arrivals(rate, n, seed=0):

@@ -43,6 +43,47 @@ def test_whatif_same_alone_and_in_mix():
     assert any(sum(m["failed"]) > 0 or m["pieces"] > 0 for m in mixed.values())
 
 
+def test_oversized_query_is_refused_and_the_rest_complete():
+    print("case: 3 queries of 2k-5k bricks and one of 40k against a 3 x 6,000-brick pool", flush=True)
+    qs = [serve.make_query(i, 3000 + i, n_min=2000, n_max=5000) for i in range(3)]
+    big = serve.make_query(99, 3099, n_min=39000, n_max=40000)
+    srv = serve.WhatIfServer(slots=3, slot_bricks=6000, slot_bodies=2048, frames=30)
+    srv.submit(qs[0], 0.0)
+    with pytest.raises(ValueError, match="budget"):
+        srv.submit(big, 0.0)
+    srv.submit(qs[1], 0.0)
+    srv.submit(qs[2], 0.0)
+    while srv.busy():
+        srv.tick()
+    assert sorted(q["id"] for q in srv.done) == [0, 1, 2]
+    assert srv.status()["bytes_in_use"] == 0
+    srv.close()
+
+
+def test_hbm_budget_holds_a_query_while_a_slot_is_free():
+    print("case: budget for one scene, two slots, two queries", flush=True)
+    qs = [serve.make_query(i, 4000 + i, n_min=3000, n_max=4000) for i in range(2)]
+    probe = serve.WhatIfServer(slots=2, slot_bricks=6000, slot_bodies=2048, frames=40)
+    need = max(probe.query_bytes(q) for q in qs)
+    probe.close()
+    srv = serve.WhatIfServer(slots=2, slot_bricks=6000, slot_bodies=2048, frames=40, hbm_budget=need + need // 4)
+    for q in qs:
+        srv.submit(q, 0.0)
+    srv.tick()
+    st = srv.status()
+    # one scene admitted; the other waits for bytes although a slot is free
+    assert st["queued"] == 1 and st["solving"] + st["live"] == 1, st
+    assert st["bytes_in_use"] <= st["budget"]
+    while srv.busy():
+        srv.tick()
+        st = srv.status()
+        assert st["bytes_in_use"] <= st["budget"]
+        assert st["solving"] + st["live"] <= 1
+    assert [q["id"] for q in srv.done] == [0, 1]
+    assert srv.done[1]["queue"] >= srv.done[0]["latency"] * 0.5
+    srv.close()
+
+
 def test_whatif_poisson_stream_completes():
     print("case: 16 queries of 2k-6k bricks at 200 q/s, 300 frames each", flush=True)
     qs = [serve.make_query(i, 2000 + i, n_min=2000, n_max=6000) for i in range(16)]
