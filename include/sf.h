@@ -66,8 +66,8 @@ int sf_world_set_params(sf_world* w, const float* params, int n);
 int sf_world_load(sf_world* w, int nbody, const float* bodies, int nshape, const int32_t* shape_body, const float* shapes);
 /* One 60 Hz step; returns the number of bodies in the solver (phys.rs World::step :152, wasm step :39) */
 int sf_world_step(sf_world* w, float dt);
-/* Enqueue `nsteps` steps without host synchronisation in between (the colour plan of each step is read on the
- * device); returns 0 or an error */
+/* `nsteps` steps, one after another, exactly as that many sf_world_step calls (each step reads its counters and colour
+ * plan back to the host once, so the steps are not enqueued ahead of each other); returns 0 or an error */
 int sf_world_step_many(sf_world* w, float dt, int nsteps);
 int sf_world_sync(sf_world* w);
 int sf_world_read_bodies(sf_world* w, float* out);

@@ -33,3 +33,19 @@ def test_no_gpu_world_creation_fails_cleanly():
     w = L.sf_world_create(10, 10, None)
     assert not w
     assert L.sf_last_error() == -4
+
+
+def test_rust_binding_names_only_exported_symbols():
+    """The Rust crate (shatterforge_amd/rust/sf-hip, source only here) declares every function it binds with a name
+    the header declares and the library exports, and binds every legacy, serving and load entry point."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = open(os.path.join(root, "shatterforge_amd", "rust", "sf-hip", "src", "lib.rs")).read()
+    block = src[src.index('extern "C" {'):src.index("/// A failed call")]
+    names = re.findall(r"pub fn (sf_\w+)\(", block)
+    declared = set(_native.declared_symbols())
+    L = _native.lib()
+    assert names and all(n in declared and hasattr(L, n) for n in names), [n for n in names if n not in declared]
+    want = {n for n in declared if n.startswith(("sf_legacy_", "sf_serve_s", "sf_serve_t", "sf_serve_p", "sf_load_solve"))}
+    assert want <= set(names), want - set(names)
