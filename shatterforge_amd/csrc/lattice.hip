@@ -181,6 +181,44 @@ __global__ void __launch_bounds__(NT) k_segsort_all(const uint32_t* off, const u
     }
 }
 
+// ---------------------------------------------------------------- finalize (src/lib.rs:668-696)
+// Each brick of a scene (one thread each): its six neighbours from the scene's lattice grid, its three positive-axis
+// bond capacities (finalize_brick's rule: a neighbour of the same body and structure, not across an infill gap,
+// min CAPV vertical / min CAPH lateral) and the footing bond of a standing brick on the ground (2 CAPV).
+struct GridK { int32_t gx0, gy0, gz0, gnx, gny, gnz; uint32_t goff; };
+__device__ __forceinline__ int32_t grid_at(const LD& D, const GridK& G, int x, int y, int z) {
+    x -= G.gx0; y -= G.gy0; z -= G.gz0;
+    if (x < 0 || y < 0 || z < 0 || x >= G.gnx || y >= G.gny || z >= G.gnz) return 0;
+    return D.grid[G.goff + ((size_t)y * G.gnz + z) * G.gnx + x];
+}
+__global__ void __launch_bounds__(NT) k_finalize(LD D, GridK G, uint32_t b0, uint32_t nb) {
+    const Tables& T = tables();
+    GS(t, nb) {
+        const uint32_t i = b0 + (uint32_t)t;
+        const int mi = D.mat[i];
+        if (mi == 0) continue;
+        const int4 c = D.bxyz[i];
+        uint32_t nbr[6];
+        for (int d = 0; d < 6; d++) {
+            const int32_t g = grid_at(D, G, c.x + DIRS[d][0], c.y + DIRS[d][1], c.z + DIRS[d][2]);
+            nbr[d] = g ? (uint32_t)(g - 1) : NONE;
+            D.nb6[6 * (size_t)i + d] = nbr[d];
+        }
+        float bd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int bi = D.body[i], si = D.sid[i], fi = D.infill[i];
+        for (int a = 0; a < 3; a++) {
+            const uint32_t j = nbr[2 * a];
+            if (j == NONE) continue;
+            const bool gap = fi != D.infill[j] && (a != 1 || fi == 1);
+            if (!gap && D.mat[j] != 0 && D.body[j] == bi && D.sid[j] == si) bd[a] = bond_cap(mi, D.mat[j], a);
+        }
+        if (c.y == 0 && bi < 0) bd[3] = T.capv[mi] * 2.0f;
+        const float4 b4 = make_float4(bd[0], bd[1], bd[2], bd[3]);
+        D.bond[i] = b4;
+        D.bmax[i] = b4;
+    }
+}
+
 // ---------------------------------------------------------------- contacts
 __device__ __forceinline__ int static_brick(const LD& D, const SceneK& S, int x, int y, int z) {
     x -= S.gx0; y -= S.gy0; z -= S.gz0;
@@ -2075,7 +2113,7 @@ int lat_alloc(Lat* L) {
     const size_t B = L->bcap, N = L->dcap, S = L->scap, NS = 256 * (size_t)S;
     LD& D = L->D;
     int r = 0;
-    r |= L->dalloc(D.mat, B, &L->per_brick, 1); r |= L->dalloc(D.sid, B, &L->per_brick, 1); r |= L->dalloc(D.body, B, &L->per_brick, 1); r |= L->dalloc(D.bxyz, B, &L->per_brick, 1);
+    r |= L->dalloc(D.mat, B, &L->per_brick, 1); r |= L->dalloc(D.sid, B, &L->per_brick, 1); r |= L->dalloc(D.infill, B, &L->per_brick, 1); r |= L->dalloc(D.body, B, &L->per_brick, 1); r |= L->dalloc(D.bxyz, B, &L->per_brick, 1);
     r |= L->dalloc(D.nb6, 6 * B, &L->per_brick, 6); r |= L->dalloc(D.bond, B, &L->per_brick, 1); r |= L->dalloc(D.bmax, B, &L->per_brick, 1); r |= L->dalloc(D.local, B, &L->per_brick, 1);
     r |= L->dalloc(D.wpos, B, &L->per_brick, 1); r |= L->dalloc(D.phi, B, &L->per_brick, 1); r |= L->dalloc(D.ext, B, &L->per_brick, 1); r |= L->dalloc(D.extn, B, &L->per_brick, 1);
     r |= L->dalloc(D.stress, B, &L->per_brick, 1); r |= L->dalloc(D.sup, B, &L->per_brick, 1);
@@ -3426,24 +3464,6 @@ int add_brick(Lat* L, HScene& S, int x, int y, int z, int m, int sid, int infill
     L->phi[i] = 0.0f; L->ext[i] = 0.0f; L->extn[i] = 0.0f;
     return i;
 }
-void finalize_brick(Lat* L, const HScene& S, int i) {
-    if (L->mat[i] == 0) return;
-    int4 c = L->bxyz[i];
-    for (int d = 0; d < 6; d++) {
-        int32_t g = hgget(L, S, c.x + DIRS[d][0], c.y + DIRS[d][1], c.z + DIRS[d][2]);
-        L->nb6[6 * (size_t)i + d] = g ? (uint32_t)(g - 1) : NONE;
-    }
-    float bd[4] = {0, 0, 0, 0};
-    for (int a = 0; a < 3; a++) {
-        int j = hnbr(L, i, a * 2);
-        bool gap = j >= 0 && L->infill[i] != L->infill[j] && (a != 1 || L->infill[i] == 1);
-        if (j >= 0 && !gap && L->mat[j] != 0 && L->body[j] == L->body[i] && L->sid[j] == L->sid[i]) bd[a] = bond_cap(L->mat[i], L->mat[j], a);
-    }
-    if (c.y == 0 && L->body[i] < 0) bd[3] = tables().capv[L->mat[i]] * 2.0f;
-    L->bond[i] = make_float4(bd[0], bd[1], bd[2], bd[3]);
-    L->bmax[i] = L->bond[i];
-}
-
 int finalize(Lat* L, int si) {
     HScene& S = L->sc[si];
     int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
@@ -3470,7 +3490,6 @@ int finalize(Lat* L, int si) {
         }
         L->grid[idx] = i + 1;
     }
-    for (int i = S.b0; i < S.b0 + S.nb; i++) finalize_brick(L, S, i);
     int red_at = S.b0;
     std::vector<int32_t> stsc(S.nst), stlo(S.nst), sthi(S.nst);
     std::vector<int4> strb(S.nst);
@@ -3506,8 +3525,7 @@ int finalize(Lat* L, int si) {
     size_t b0 = S.b0, nb = S.nb;
     int r = 0;
     r |= cp_up(D.mat + b0, &L->mat[b0], nb, s); r |= cp_up(D.sid + b0, &L->sid[b0], nb, s); r |= cp_up(D.body + b0, &L->body[b0], nb, s);
-    r |= cp_up(D.bxyz + b0, &L->bxyz[b0], nb, s); r |= cp_up(D.nb6 + 6 * b0, &L->nb6[6 * b0], 6 * nb, s);
-    r |= cp_up(D.bond + b0, &L->bond[b0], nb, s); r |= cp_up(D.bmax + b0, &L->bmax[b0], nb, s);
+    r |= cp_up(D.bxyz + b0, &L->bxyz[b0], nb, s); r |= cp_up(D.infill + b0, &L->infill[b0], nb, s);
     r |= cp_up(D.local + b0, &L->local[b0], nb, s); r |= cp_up(D.wpos + b0, &L->wpos[b0], nb, s);
     r |= cp_up(D.phi + b0, &L->phi[b0], nb, s); r |= cp_up(D.ext + b0, &L->ext[b0], nb, s); r |= cp_up(D.extn + b0, &L->extn[b0], nb, s);
     std::vector<uint32_t> zero(nb, 0u);
@@ -3531,6 +3549,15 @@ int finalize(Lat* L, int si) {
     }
     SF_HIP(hipStreamSynchronize(s));
     D.grid = L->d_grid.p;
+    // neighbours and bonds on the device, then mirrored to the host for the fracture bookkeeping
+    if (nb) {
+        const GridK G{S.gx0, S.gy0, S.gz0, S.gnx, S.gny, S.gnz, (uint32_t)S.goff};
+        LAUNCH(k_finalize, sf_grid(nb), NT, D, G, (uint32_t)b0, (uint32_t)nb);
+        r |= cp_down(&L->nb6[6 * b0], D.nb6 + 6 * b0, 6 * nb, s);
+        r |= cp_down(&L->bond[b0], D.bond + b0, nb, s); r |= cp_down(&L->bmax[b0], D.bmax + b0, nb, s);
+        if (r) return -1;
+        SF_HIP(hipStreamSynchronize(s));
+    }
     std::vector<int32_t> bsc(S.dcap, si);
     r |= cp_up(D.bd_scene + S.bd0, bsc.data(), S.dcap, s);
     if (r) return -1;

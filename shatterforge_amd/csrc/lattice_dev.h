@@ -46,6 +46,7 @@ struct LD {
     // bricks
     uint8_t* mat;
     uint8_t* sid;
+    uint8_t* infill;       // glass or marked infill (finalize's gap rule)
     int32_t* body;
     int4* bxyz;            // lattice x, y, z, scene
     uint32_t* nb6;
