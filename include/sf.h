@@ -26,7 +26,9 @@ const char* sf_last_error_message(void);
 int sf_device_count(void);
 
 /* ---------------------------------------------------------------- box world */
-/* Parameter block (floats). Defaults: engine/src/phys.rs:101-105 (Params::default) plus the colouring limits. */
+/* Parameter block (floats). Defaults: engine/src/phys.rs:101-105 (Params::default) plus the colouring limits; the GPU
+ * world's solver extensions (gpu/gpu_world.js:1152-1203, :954-985) are off by default, so the defaults are the
+ * reference's CPU model. The GPU world's values: SF_P_RATIO 20, SF_P_SHOCK 6, SF_P_COL_SUPPORT 2. */
 enum {
     SF_P_GX, SF_P_GY, SF_P_GZ,      /* gravity (0, -9.81, 0) */
     SF_P_SUBSTEPS,                  /* 12 */
@@ -43,12 +45,18 @@ enum {
     SF_P_COL_SPREAD,                /* a contact picks among the lowest this-many free colours (4) */
     SF_P_TAIL_MIN,                  /* colour buckets smaller than this are solved in one workgroup (1024) */
     SF_P_SOLVE_NT,                  /* threads per workgroup of the colour-bucket solve: 64, 128 or 256 (64); results do not depend on it */
+    SF_P_RATIO,                     /* mass-ratio limit: across a contact the heavier side acts at most this many times heavier (0 = off) */
+    SF_P_SHOCK,                     /* shock-propagation level passes: the relax pass holds the lower body of a separating vertical contact (0 = off) */
+    SF_P_REST_THRESHOLD,            /* restitution acts on points that approached faster than this (1 m/s) */
+    SF_P_REST_ITERS,                /* restitution iterations per manifold (8) */
+    SF_P_COL_SUPPORT,               /* colouring rounds for supports alone before the rounds for all; supports outrank the rest (0 = off) */
     SF_NPARAM
 };
 /* body record, SF_BODY_F floats: p[3] q[4] v[3] w[3] inv_m inv_i[3] (principal, local) awake sleep_t reserved */
 #define SF_BODY_F 20
-/* shape record, SF_SHAPE_F floats: local position[3] local rotation q[4] half extents[3] friction */
-#define SF_SHAPE_F 11
+/* shape record, SF_SHAPE_F floats: local position[3] local rotation q[4] half extents[3] friction restitution
+ * (a pair bounces with the larger of its two restitutions, the ground's being 0: gpu/gpu_world.js:601-603) */
+#define SF_SHAPE_F 12
 /* statistics of the last step */
 enum { SF_S_AWAKE, SF_S_PAIRS, SF_S_MANIFOLDS, SF_S_POINTS, SF_S_COLOURS, SF_S_OVERFLOW, SF_S_GROUND, SF_S_WAKES,
        SF_S_ENTRIES, SF_S_CANDIDATES, SF_S_DROPPED, SF_S_ROUNDS, SF_NSTAT };

@@ -13,7 +13,7 @@
 
 #define NONE_U 0xFFFFFFFFu
 
-typedef struct { V3 ra, rb, la, raxn, rbxn, iaxn, ibxn; float adj, nm, jn; } Cp;
+typedef struct { V3 ra, rb, la, raxn, rbxn, iaxn, ibxn; float adj, nm, jn, rv; } Cp;  /* rv: approach speed at build */
 typedef struct {
     int32_t a, b;   /* shapes (b = -1: ground) */
     int32_t ba, bb; /* bodies (bb = -1: ground) */
@@ -22,13 +22,14 @@ typedef struct {
     V3 fa, fb, la_f, lb_f, e0;
     float tm1, tm2, twm, rad, jt1, jt2, jtw; int stick;
     V3 ca1, ca2, cb1, cb2, ia1, ia2, ib1, ib2, ian, ibn; float ima, imb;
+    float rest; int pushed;  /* the pair's restitution; set once a relax pass pushed (gpu_world.js M_OUT .x, .y) */
 } Man;
 
 typedef struct {
     V3 p; Quat q; V3 v, w; float inv_m; V3 inv_i; int awake; float sleep_t; V3 sleep_p; Quat sleep_q;
     M3 rot;
 } Body;
-typedef struct { int body; V3 lp; Quat lq; M3 lrot; V3 h; float mu; V3 wp; M3 wrot; V3 lo, hi; } Shape;
+typedef struct { int body; V3 lp; Quat lq; M3 lrot; V3 h; float mu, e; V3 wp; M3 wrot; V3 lo, hi; } Shape;
 
 struct ob_world {
     int order;
@@ -45,12 +46,15 @@ struct ob_world {
     int32_t stats[8];
     uint8_t* wake_flag;
     int32_t *bl_off, *bl_ent, *ovf_index;  /* device order: per-slot contact lists, overflow position per manifold */
+    uint32_t *lvl, *lvl2;  /* shock-propagation heights per slot (the two copies the passes alternate between) */
+    int any_rest;          /* some shape has a restitution above zero */
 };
 
 /* parameter indices (same as SF_P_* in include/sf.h) */
 enum { P_GX, P_GY, P_GZ, P_SUBSTEPS, P_HERTZ, P_DAMPING, P_MAXPUSH, P_FRICTION, P_SPEC, P_SLEEP_LIN, P_SLEEP_ANG,
        P_SLEEP_TIME, P_SLEEP_DIST, P_SLEEP_ROT, P_ALLOW_SLEEP, P_CELL, P_LEAK, P_ANCHOR, P_ANCHOR_PUSH, P_ANCHOR_SLOP,
-       P_COL_LIMIT, P_COL_ROUNDS, P_COL_SPREAD, P_RESERVED };
+       P_COL_LIMIT, P_COL_ROUNDS, P_COL_SPREAD, P_TAIL_MIN, P_SOLVE_NT, P_RATIO, P_SHOCK, P_REST_THR, P_REST_ITERS,
+       P_COL_SUPPORT };
 
 static void default_params(float* p) {
     /* phys.rs:101-105 */
@@ -58,7 +62,10 @@ static void default_params(float* p) {
     p[P_MAXPUSH] = 3.0f; p[P_FRICTION] = 0.6f; p[P_SPEC] = 0.02f; p[P_SLEEP_LIN] = 0.05f; p[P_SLEEP_ANG] = 0.08f;
     p[P_SLEEP_TIME] = 0.5f; p[P_SLEEP_DIST] = 0.01f; p[P_SLEEP_ROT] = 0.02f; p[P_ALLOW_SLEEP] = 1.0f; p[P_CELL] = 1.0f;
     p[P_LEAK] = 1.0f; p[P_ANCHOR] = 1.0f; p[P_ANCHOR_PUSH] = 3.0f; p[P_ANCHOR_SLOP] = 0.0f;
-    p[P_COL_LIMIT] = 16; p[P_COL_ROUNDS] = 32; p[P_COL_SPREAD] = 4; p[P_RESERVED] = 0;
+    p[P_COL_LIMIT] = 16; p[P_COL_ROUNDS] = 32; p[P_COL_SPREAD] = 4; p[P_TAIL_MIN] = 1024; p[P_SOLVE_NT] = 64;
+    /* the GPU world's extensions (gpu_world.js), off: ratio 20, 6 level passes and 2 support rounds turn them on;
+     * restitution acts where shapes carry one (threshold 1 m/s, 8 iterations, docs/GPU.md:41) */
+    p[P_RATIO] = 0.0f; p[P_SHOCK] = 0.0f; p[P_REST_THR] = 1.0f; p[P_REST_ITERS] = 8.0f; p[P_COL_SUPPORT] = 0.0f;
 }
 
 ob_world* ob_create(int order) {
@@ -71,7 +78,8 @@ void ob_destroy(ob_world* w) {
     if (!w) return;
     free(w->bd); free(w->sh); free(w->sv); free(w->sw); free(w->sdp); free(w->sth); free(w->sp0); free(w->sim);
     free(w->sii); free(w->sq); free(w->sidx); free(w->man); free(w->old); free(w->old_sorted); free(w->cand);
-    free(w->colour); free(w->island); free(w->wake_flag); free(w->bl_off); free(w->bl_ent); free(w->ovf_index); free(w);
+    free(w->colour); free(w->island); free(w->wake_flag); free(w->bl_off); free(w->bl_ent); free(w->ovf_index);
+    free(w->lvl); free(w->lvl2); free(w);
 }
 void ob_set_params(ob_world* w, const float* prm, int n) { for (int i = 0; i < n && i < OB_NPARAM; i++) w->prm[i] = prm[i]; }
 int ob_nbody(const ob_world* w) { return w->nb; }
@@ -95,8 +103,11 @@ int ob_load(ob_world* w, int nbody, const float* bodies, int nshape, const int32
         if (sh->body < 0 || sh->body >= nbody) return -1;
         if (s > 0 && shape_body[s] < shape_body[s - 1]) return -2;
         sh->lp = v3(f[0], f[1], f[2]); sh->lq = quat(f[3], f[4], f[5], f[6]); sh->h = v3(f[7], f[8], f[9]); sh->mu = f[10];
+        sh->e = f[11];
         sh->lrot = q_to_m3(sh->lq);
     }
+    w->any_rest = 0;
+    for (int s = 0; s < nshape; s++) if (w->sh[s].e > 0.0f) w->any_rest = 1;
     size_t n1 = (size_t)nbody + 1;
     w->sv = (V3*)realloc(w->sv, sizeof(V3) * n1); w->sw = (V3*)realloc(w->sw, sizeof(V3) * n1);
     w->sdp = (V3*)realloc(w->sdp, sizeof(V3) * n1); w->sth = (V3*)realloc(w->sth, sizeof(V3) * n1);
@@ -105,6 +116,8 @@ int ob_load(ob_world* w, int nbody, const float* bodies, int nshape, const int32
     w->sidx = (int32_t*)realloc(w->sidx, sizeof(int32_t) * (size_t)(nbody > 0 ? nbody : 1));
     w->island = (int32_t*)realloc(w->island, sizeof(int32_t) * (size_t)(nbody > 0 ? nbody : 1));
     w->wake_flag = (uint8_t*)realloc(w->wake_flag, (size_t)(nbody > 0 ? nbody : 1));
+    w->lvl = (uint32_t*)realloc(w->lvl, sizeof(uint32_t) * n1); w->lvl2 = (uint32_t*)realloc(w->lvl2, sizeof(uint32_t) * n1);
+    for (size_t i = 0; i < n1; i++) { w->lvl[i] = 0; w->lvl2[i] = 0; }
     for (int i = 0; i < nbody; i++) w->island[i] = i;
     w->nman = 0; w->nold = 0;
     return 0;
@@ -387,6 +400,8 @@ static void push_manifold(ob_world* w, int a, int b, V3 n, const Pt* pts, int cn
     m->a = a; m->b = b; m->ba = sa_->body; m->bb = bb ? w->sh[b].body : -1; m->sa = sa; m->sb = sb;
     m->n = n; tangents(n, &m->t1, &m->t2);
     m->mu = mix_mu(sa_->mu, b >= 0 ? w->sh[b].mu : w->prm[P_FRICTION]);
+    /* the larger restitution of the two (the ground's is 0), gpu_world.js:601-603 */
+    m->rest = om_max(sa_->e, b >= 0 ? w->sh[b].e : 0.0f); m->pushed = 0;
     m->cnt = cnt; m->stick = w->prm[P_ANCHOR] > 0.5f;
     int oi = find_old(w, a, b);
     const Man* o = oi >= 0 ? &w->old[oi] : NULL;
@@ -417,6 +432,11 @@ static void push_manifold(ob_world* w, int a, int b, V3 n, const Pt* pts, int cn
         c->rb = bb ? vsub(pts[k].p, pb) : pts[k].p;
         c->adj = pts[k].s;
         c->la = m3_tmulv(ra, c->ra);
+        /* the point's normal velocity at the step start (negative: approaching), gpu_world.js:662 */
+        {
+            V3 pva = vadd(w->sv[sa], vcross(w->sw[sa], c->ra)), pvb = vadd(w->sv[sb], vcross(w->sw[sb], c->rb));
+            c->rv = vdot(vsub(pvb, pva), n);
+        }
         if (o) {
             float best = 0.03f * 0.03f; int bi = -1;
             for (int q = 0; q < o->cnt; q++) { float d = vlen2(vsub(o->cp[q].la, c->la)); if (d < best) { best = d; bi = q; } }
@@ -475,13 +495,18 @@ static void collide_pair(ob_world* w, int ia, int ib, const uint8_t* snap) {
 /* Deterministic claim rounds. Each round every uncoloured manifold picks one of the lowest `spread` colours free at
  * both its moving bodies (by a hash of its key and the round) and claims it at each with a unique priority; it takes
  * the colour where its claim is the highest at both. Manifolds with no free colour, or none after the last round,
- * go to the overflow (solved one by one in manifold order). */
+ * go to the overflow (solved one by one in manifold order).
+ * Supports first (gpu_world.js:954-985): with RS = P_COL_SUPPORT > 0 the first RS rounds are for supports alone
+ * (manifolds with a side that is not moving: the ground, a static or sleeping body), and supports outrank the
+ * others in every round. A round in which nothing claims changes nothing more in its phase: the support phase then
+ * goes on to the first round for all, the last phase ends. */
 static inline uint32_t mix32(uint32_t h) { h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16; return h; }
 static inline uint32_t key_hash(int32_t a, int32_t b) { return mix32((uint32_t)a * 0x9E3779B9u + mix32((uint32_t)b ^ 0x5BD1E995u)); }
 
 static void colour_manifolds(ob_world* w) {
     int K = (int)w->prm[P_COL_LIMIT]; if (K < 1) K = 1; if (K > 64) K = 64;
     int R = (int)w->prm[P_COL_ROUNDS]; int spread = (int)w->prm[P_COL_SPREAD]; if (spread < 1) spread = 1;
+    int RS = (int)w->prm[P_COL_SUPPORT]; if (RS < 0) RS = 0;
     int M = w->nman;
     if (w->capcolour < M) { w->capcolour = M; w->colour = (int32_t*)realloc(w->colour, sizeof(int32_t) * (size_t)M); }
     uint64_t* mask = (uint64_t*)calloc((size_t)w->nb + 1, sizeof(uint64_t));
@@ -497,6 +522,8 @@ static void colour_manifolds(ob_world* w) {
             cand[m] = -1;
             if (w->colour[m] != -1) continue;
             const Man* mm = &w->man[m];
+            const int support = !mm->sa || !mm->sb;
+            if (r < RS && !support) continue;
             uint64_t used = (mm->sa ? mask[mm->sa] : 0) | (mm->sb ? mask[mm->sb] : 0);
             uint64_t fr = all & ~used;
             if (!fr) { w->colour[m] = K; continue; }
@@ -507,12 +534,13 @@ static void colour_manifolds(ob_world* w) {
             for (int q = 0; q < pick; q++) fr &= fr - 1;
             int c = __builtin_ctzll(fr);
             cand[m] = c;
-            pri[m] = ((uint64_t)(mix32(kh ^ ((uint32_t)r * 0x27D4EB2Fu)) >> 8) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)m);
+            pri[m] = ((uint64_t)(RS > 0 && support) << 56) | ((uint64_t)(mix32(kh ^ ((uint32_t)r * 0x27D4EB2Fu)) >> 8) << 32) |
+                     (uint64_t)(0xFFFFFFFFu - (uint32_t)m);
             if (mm->sa) { uint64_t* cl = &claim[(size_t)mm->sa * K + c]; if (pri[m] > *cl) *cl = pri[m]; }
             if (mm->sb) { uint64_t* cl = &claim[(size_t)mm->sb * K + c]; if (pri[m] > *cl) *cl = pri[m]; }
             any = 1;
         }
-        if (!any) break;
+        if (!any) { if (r < RS) { r = RS - 1; continue; } break; }
         for (int m = 0; m < M; m++) {
             if (cand[m] < 0) continue;
             const Man* mm = &w->man[m]; int c = cand[m];
@@ -621,42 +649,80 @@ static void warm_start_tree(ob_world* w, const int32_t* off, const int32_t* ent)
     }
 }
 
-typedef struct { int use_bias; float inv_h, bias_rate, mass_scale, imp_scale, max_push, anchor_push, anchor_slop; } SolveK;
+typedef struct {
+    int use_bias; float inv_h, bias_rate, mass_scale, imp_scale, max_push, anchor_push, anchor_slop;
+    float ratio; int shock; float rest_thr; int rest_iters;
+} SolveK;
 
 static inline float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
+
+/* Mass-ratio limit (gpu_world.js:1155-1160): across one contact the heavier side responds as if at most R times
+ * heavier than the lighter; the factors multiply the two inverse masses (1 when either side is static or R is 0). */
+static inline void ratio_scale(float ia, float ib, float R, float* sa, float* sb) {
+    *sa = 1.0f; *sb = 1.0f;
+    if (R <= 0.0f || ia <= 0.0f || ib <= 0.0f) return;
+    if (ia * R < ib) *sa = ib / (R * ia);
+    else if (ib * R < ia) *sb = ia / (R * ib);
+}
+/* The limited side keeps the part of a change d to its velocity x that brings |x| down to its least, and takes the
+ * rest divided by sc: the limit only takes energy from the heavy body (gpu_world.js:1166-1177). */
+static inline V3 slowing(V3 x, V3 d, float sc) {
+    float dd = vdot(d, d);
+    if (dd == 0.0f) return d;
+    float t = om_min(om_max(-vdot(x, d) / dd, 0.0f), 1.0f);
+    return vmul(d, t + (1.0f - t) / sc);
+}
+/* a slot's shock-propagation height: 255 for the static world (slot 0, or no inverse mass) */
+static inline uint32_t slot_height(const ob_world* w, int s) { return s && w->sim[s] > 0.0f ? w->lvl[s] : 255u; }
 
 /* phys.rs:268-323 on velocity copies of the two bodies. fa, fb = 0: the manifold as the reference solves it.
  * fa, fb > 0: mass splitting for the overflow pass: each side's inverse mass and inertia are scaled by the number of
  * overflow manifolds at that body (each manifold acts on its own copy of the body), and the effective masses are
- * recomputed from the scaled ones. */
-static void solve_core(Man* m, const SolveK* K, V3* pva, V3* pwa, V3* pvb, V3* pwb, V3 tha, V3 thb, V3 dpab, float fa, float fb) {
+ * recomputed from the scaled ones.
+ * The GPU world's extensions (gpu_world.js:1207-1329): with a mass-ratio limit the inverse masses (as split) are
+ * scaled by ratio_scale as well, and each limited side keeps its change only as `slowing` allows. In the relax pass
+ * with levels (ha, hb: the sides' heights), a contact within 45 degrees of vertical whose lower body sits on fewer
+ * levels holds that body where a point is separating: the upper side alone takes the point's impulse. */
+static void solve_core(Man* m, const SolveK* K, V3* pva, V3* pwa, V3* pvb, V3* pwb, V3 tha, V3 thb, V3 dpab, float fa, float fb,
+                       uint32_t ha, uint32_t hb) {
     V3 va = *pva, wa = *pwa, vb = *pvb, wb = *pwb;
+    const V3 va0 = va, wa0 = wa, vb0 = vb, wb0 = wb;
     const int split = fa > 0.0f;
+    float sca, scb;
+    ratio_scale(split ? m->ima * fa : m->ima, split ? m->imb * fb : m->imb, K->ratio, &sca, &scb);
+    const int rec = split || sca != 1.0f || scb != 1.0f;
+    const float xa = (split ? fa : 1.0f) * sca, xb = (split ? fb : 1.0f) * scb;
     V3 n = m->n, t1 = m->t1, t2 = m->t2;
     float dn = vdot(dpab, n);
-    float ima = split ? m->ima * fa : m->ima, imb = split ? m->imb * fb : m->imb;
+    int hold_a = 0, hold_b = 0;
+    if (!K->use_bias && K->shock) { hold_a = n.y > 0.7f && ha > hb; hold_b = n.y < -0.7f && hb > ha; }
+    float ima = rec ? m->ima * xa : m->ima, imb = rec ? m->imb * xb : m->imb;
     float tot = 0.0f;
     for (int k = 0; k < m->cnt; k++) {
         Cp* c = &m->cp[k];
-        V3 iaxn = split ? vmul(c->iaxn, fa) : c->iaxn, ibxn = split ? vmul(c->ibxn, fb) : c->ibxn;
+        V3 iaxn = rec ? vmul(c->iaxn, xa) : c->iaxn, ibxn = rec ? vmul(c->ibxn, xb) : c->ibxn;
         float nm = c->nm;
-        if (split) { float kk = ima + imb + vdot(c->raxn, iaxn) + vdot(c->rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
+        if (rec) { float kk = ima + imb + vdot(c->raxn, iaxn) + vdot(c->rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
         float s = c->adj + dn + vdot(thb, c->rbxn) - vdot(tha, c->raxn);
         float bias, ms, is;
         if (s > 0.0f) { bias = s * K->inv_h; ms = 1.0f; is = 0.0f; }
         else if (K->use_bias) { bias = om_max(K->bias_rate * s, -K->max_push); ms = K->mass_scale; is = K->imp_scale; }
         else { bias = 0.0f; ms = 1.0f; is = 0.0f; }
         float vn = vdot(vsub(vb, va), n) + vdot(wb, c->rbxn) - vdot(wa, c->raxn);
+        const int ga = hold_a && vn > 0.0f, gb = hold_b && vn > 0.0f;  /* this point holds side a (b) */
+        if (ga) { float kk = imb + vdot(c->rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
+        if (gb) { float kk = ima + vdot(c->raxn, iaxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
         float imp = -nm * ms * (vn + bias) - is * c->jn;
         float nj = om_max(c->jn + imp, 0.0f);
         imp = nj - c->jn; c->jn = nj; tot += nj;
-        va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(iaxn, imp));
-        vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(ibxn, imp));
+        if (!ga) { va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(iaxn, imp)); }
+        if (!gb) { vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(ibxn, imp)); }
     }
+    if (!K->use_bias && tot > 0.0f) m->pushed = 1;
     V3 ia1 = m->ia1, ia2 = m->ia2, ib1 = m->ib1, ib2 = m->ib2, ian = m->ian, ibn = m->ibn;
     float tm1 = m->tm1, tm2 = m->tm2, twm = m->twm;
-    if (split) {
-        ia1 = vmul(ia1, fa); ia2 = vmul(ia2, fa); ian = vmul(ian, fa); ib1 = vmul(ib1, fb); ib2 = vmul(ib2, fb); ibn = vmul(ibn, fb);
+    if (rec) {
+        ia1 = vmul(ia1, xa); ia2 = vmul(ia2, xa); ian = vmul(ian, xa); ib1 = vmul(ib1, xb); ib2 = vmul(ib2, xb); ibn = vmul(ibn, xb);
         float k1 = ima + imb + vdot(m->ca1, ia1) + vdot(m->cb1, ib1);
         float k2 = ima + imb + vdot(m->ca2, ia2) + vdot(m->cb2, ib2);
         tm1 = k1 > 0.0f ? 1.0f / k1 : 0.0f; tm2 = k2 > 0.0f ? 1.0f / k2 : 0.0f;
@@ -693,6 +759,45 @@ static void solve_core(Man* m, const SolveK* K, V3* pva, V3* pwa, V3* pvb, V3* p
     m->jtw = jt;
     float dtw = jt - ot;
     wa = vsub(wa, vmul(ian, dtw)); wb = vadd(wb, vmul(ibn, dtw));
+    if (sca != 1.0f) { va = vadd(va0, slowing(va0, vsub(va, va0), sca)); wa = vadd(wa0, slowing(wa0, vsub(wa, wa0), sca)); }
+    if (scb != 1.0f) { vb = vadd(vb0, slowing(vb0, vsub(vb, vb0), scb)); wb = vadd(wb0, slowing(wb0, vsub(wb, wb0), scb)); }
+    *pva = va; *pwa = wa; *pvb = vb; *pwb = wb;
+}
+
+/* Restitution after the sub-steps (gpu_world.js:1333-1363): in a manifold with a restitution that pushed during the
+ * step, the points that approached faster than the threshold at the step start have their normal velocity driven to
+ * -rest times that approach, the points iterated together (a point's bounce moves the others). fa, fb as in
+ * solve_core (mass splitting for the overflow), with the mass-ratio limit. */
+static void rest_core(Man* m, const SolveK* K, V3* pva, V3* pwa, V3* pvb, V3* pwb, float fa, float fb) {
+    if (m->rest <= 0.0f || !m->pushed) return;
+    int any = 0;
+    for (int k = 0; k < m->cnt; k++) if (!(m->cp[k].rv > -K->rest_thr)) any = 1;
+    if (!any) return;
+    V3 va = *pva, wa = *pwa, vb = *pvb, wb = *pwb;
+    const V3 va0 = va, wa0 = wa, vb0 = vb, wb0 = wb;
+    const int split = fa > 0.0f;
+    float sca, scb;
+    ratio_scale(split ? m->ima * fa : m->ima, split ? m->imb * fb : m->imb, K->ratio, &sca, &scb);
+    const int rec = split || sca != 1.0f || scb != 1.0f;
+    const float xa = (split ? fa : 1.0f) * sca, xb = (split ? fb : 1.0f) * scb;
+    float ima = rec ? m->ima * xa : m->ima, imb = rec ? m->imb * xb : m->imb;
+    V3 n = m->n;
+    for (int it = 0; it < K->rest_iters; it++) {
+        for (int k = 0; k < m->cnt; k++) {
+            Cp* c = &m->cp[k];
+            if (c->rv > -K->rest_thr) continue;
+            V3 iaxn = rec ? vmul(c->iaxn, xa) : c->iaxn, ibxn = rec ? vmul(c->ibxn, xb) : c->ibxn;
+            float nm = c->nm;
+            if (rec) { float kk = ima + imb + vdot(c->raxn, iaxn) + vdot(c->rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
+            float vn = vdot(vsub(vb, va), n) + vdot(wb, c->rbxn) - vdot(wa, c->raxn);
+            float nj = om_max(c->jn - nm * (vn + m->rest * c->rv), 0.0f);
+            float imp = nj - c->jn; c->jn = nj;
+            va = vsub(va, vmul(n, imp * ima)); wa = vsub(wa, vmul(iaxn, imp));
+            vb = vadd(vb, vmul(n, imp * imb)); wb = vadd(wb, vmul(ibxn, imp));
+        }
+    }
+    if (sca != 1.0f) { va = vadd(va0, slowing(va0, vsub(va, va0), sca)); wa = vadd(wa0, slowing(wa0, vsub(wa, wa0), sca)); }
+    if (scb != 1.0f) { vb = vadd(vb0, slowing(vb0, vsub(vb, vb0), scb)); wb = vadd(wb0, slowing(wb0, vsub(wb, wb0), scb)); }
     *pva = va; *pwa = wa; *pvb = vb; *pwb = wb;
 }
 
@@ -701,7 +806,14 @@ static void solve_one(ob_world* w, Man* m, const SolveK* K) {
     int sa = m->sa, sb = m->sb;
     V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
     V3 dpab = vsub(w->sdp[sb], w->sdp[sa]);
-    solve_core(m, K, &va, &wa, &vb, &wb, w->sth[sa], w->sth[sb], dpab, 0.0f, 0.0f);
+    solve_core(m, K, &va, &wa, &vb, &wb, w->sth[sa], w->sth[sb], dpab, 0.0f, 0.0f, slot_height(w, sa), slot_height(w, sb));
+    w->sv[sa] = va; w->sw[sa] = wa; w->sv[sb] = vb; w->sw[sb] = wb;
+    w->sv[0] = v3(0, 0, 0); w->sw[0] = v3(0, 0, 0);
+}
+static void rest_one(ob_world* w, Man* m, const SolveK* K) {
+    int sa = m->sa, sb = m->sb;
+    V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
+    rest_core(m, K, &va, &wa, &vb, &wb, 0.0f, 0.0f);
     w->sv[sa] = va; w->sw[sa] = wa; w->sv[sb] = vb; w->sw[sb] = wb;
     w->sv[0] = v3(0, 0, 0); w->sw[0] = v3(0, 0, 0);
 }
@@ -709,7 +821,7 @@ static void solve_one(ob_world* w, Man* m, const SolveK* K) {
 /* The overflow pass (manifolds the colouring left over): Jacobi with mass splitting. Every overflow manifold solves
  * against its own copies of its bodies, all starting from the velocities before the pass; each body then takes the
  * average of its copies, its copies' changes summed in manifold order. */
-static void overflow_pass(ob_world* w, const SolveK* K, const int32_t* ovf, int novf, const int32_t* ocnt, V3* dv) {
+static void overflow_pass(ob_world* w, const SolveK* K, const int32_t* ovf, int novf, const int32_t* ocnt, V3* dv, int rest) {
     size_t n1 = (size_t)w->nb + 1;
     V3* v0 = (V3*)malloc(sizeof(V3) * n1);
     V3* w0 = (V3*)malloc(sizeof(V3) * n1);
@@ -720,7 +832,8 @@ static void overflow_pass(ob_world* w, const SolveK* K, const int32_t* ovf, int 
         V3 va = v0[sa], wa = w0[sa], vb = v0[sb], wb = w0[sb];
         V3 dpab = vsub(w->sdp[sb], w->sdp[sa]);
         float fa = sa ? (float)ocnt[sa] : 1.0f, fb = sb ? (float)ocnt[sb] : 1.0f;
-        solve_core(m, K, &va, &wa, &vb, &wb, w->sth[sa], w->sth[sb], dpab, fa, fb);
+        if (rest) rest_core(m, K, &va, &wa, &vb, &wb, fa, fb);
+        else solve_core(m, K, &va, &wa, &vb, &wb, w->sth[sa], w->sth[sb], dpab, fa, fb, slot_height(w, sa), slot_height(w, sb));
         dv[4 * i + 0] = vsub(va, v0[sa]); dv[4 * i + 1] = vsub(wa, w0[sa]);
         dv[4 * i + 2] = vsub(vb, v0[sb]); dv[4 * i + 3] = vsub(wb, w0[sb]);
     }
@@ -754,7 +867,37 @@ static void overflow_pass(ob_world* w, const SolveK* K, const int32_t* ovf, int 
 static void solve_pass(ob_world* w, const SolveK* K, const int32_t* order, int ncol_order, const int32_t* ovf, int novf,
                        const int32_t* ocnt, V3* dv) {
     for (int i = 0; i < ncol_order; i++) solve_one(w, &w->man[order ? order[i] : i], K);
-    if (novf) overflow_pass(w, K, ovf, novf, ocnt, dv);
+    if (novf) overflow_pass(w, K, ovf, novf, ocnt, dv, 0);
+}
+/* the restitution pass: the colours in order, then the overflow (Jacobi with mass splitting, as the solve's) */
+static void rest_pass(ob_world* w, const SolveK* K, const int32_t* order, int ncol_order, const int32_t* ovf, int novf,
+                      const int32_t* ocnt, V3* dv) {
+    for (int i = 0; i < ncol_order; i++) rest_one(w, &w->man[order ? order[i] : i], K);
+    if (novf) overflow_pass(w, K, ovf, novf, ocnt, dv, 1);
+}
+
+/* Shock-propagation levels (gpu_world.js:1178-1203): a body's height is 255 minus its number of near-vertical
+ * contacts above the static world (static and sleeping bodies 255, a body on them 254, ...; 0: no chain reached it).
+ * Each pass reads one copy and raises the other with maxima over the manifolds (carrying both bodies' read heights,
+ * and lifting the upper body of a contact within 45 degrees of vertical to one below its support's), so the result
+ * does not depend on the manifold order. The count of passes is even, so the heights end in lvl. */
+static void level_passes(ob_world* w, int passes) {
+    size_t n1 = (size_t)w->nb + 1;
+    for (size_t i = 0; i < n1; i++) { w->lvl[i] = 0; w->lvl2[i] = 0; }
+    for (int k = 0; k < passes; k++) {
+        uint32_t* rd = (k & 1) ? w->lvl2 : w->lvl;
+        uint32_t* wr = (k & 1) ? w->lvl : w->lvl2;
+        for (int mi = 0; mi < w->nman; mi++) {
+            const Man* m = &w->man[mi];
+            int sa = m->sa, sb = m->sb;
+            int da = sa && w->sim[sa] > 0.0f, db = sb && w->sim[sb] > 0.0f;
+            uint32_t ha = da ? rd[sa] : 255u, hb = db ? rd[sb] : 255u;
+            if (da && rd[sa] > wr[sa]) wr[sa] = rd[sa];
+            if (db && rd[sb] > wr[sb]) wr[sb] = rd[sb];
+            if (m->n.y > 0.7f && ha > 1u && db && ha - 1u > wr[sb]) wr[sb] = ha - 1u;
+            if (m->n.y < -0.7f && hb > 1u && da && hb - 1u > wr[sa]) wr[sa] = hb - 1u;
+        }
+    }
 }
 
 /* phys.rs:326-354, plus island labels = minimum body index of each island */
@@ -934,8 +1077,13 @@ void ob_step(ob_world* w, float dt) {
     /* ---- sub-steps (phys.rs:214-230) */
     V3 g = v3(P[P_GX], P[P_GY], P[P_GZ]);
     float wmax = 0.25f * 3.14159265358979323846f * inv_h;
-    SolveK kb = {1, inv_h, bias_rate, mass_scale, imp_scale, P[P_MAXPUSH], P[P_ANCHOR_PUSH], P[P_ANCHOR_SLOP]};
+    /* (level passes: an even count, at most 64; restitution iterations at most 64) */
+    int shock = (int)P[P_SHOCK]; if (shock < 0) shock = 0; if (shock > 64) shock = 64; shock = (shock + 1) & ~1;
+    int riters = (int)P[P_REST_ITERS]; if (riters < 0) riters = 0; if (riters > 64) riters = 64;
+    SolveK kb = {1, inv_h, bias_rate, mass_scale, imp_scale, P[P_MAXPUSH], P[P_ANCHOR_PUSH], P[P_ANCHOR_SLOP],
+                 P[P_RATIO] > 0.0f ? P[P_RATIO] : 0.0f, shock > 0, P[P_REST_THR], riters};
     SolveK kr = kb; kr.use_bias = 0;
+    if (shock) level_passes(w, shock);
     for (int s = 0; s < sub; s++) {
         for (int i = 0; i < n; i++) {
             int si = w->sidx[i]; if (!si) continue;
@@ -953,6 +1101,7 @@ void ob_step(ob_world* w, float dt) {
         }
         solve_pass(w, &kr, order, ncol_order, ovf, novf, ocnt, dv);
     }
+    if (w->any_rest) rest_pass(w, &kr, order, ncol_order, ovf, novf, ocnt, dv);
     free(order); free(ovf); free(ocnt); free(dv);
     /* ---- write back */
     for (int i = 0; i < n; i++) {
@@ -993,3 +1142,6 @@ int ob_colours(const ob_world* w, int32_t* out, int cap) {
 }
 
 void ob_islands(const ob_world* w, int32_t* out) { for (int i = 0; i < w->nb; i++) out[i] = w->island[i]; }
+void ob_levels(const ob_world* w, int32_t* out) {
+    for (int i = 0; i < w->nb; i++) { int s = w->sidx[i]; out[i] = (int32_t)slot_height(w, s); }
+}

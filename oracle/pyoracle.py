@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 ORDER_REFERENCE, ORDER_COLOURED = 0, 1
-BODY_F, SHAPE_F, MAN_F = 20, 11, 40
+BODY_F, SHAPE_F, MAN_F = 20, 12, 40
 
 _lib = None
 vp, ci, cf, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
@@ -31,7 +31,7 @@ def lib():
             "ob_create": (vp, [ci]), "ob_destroy": (None, [vp]), "ob_set_params": (None, [vp, vp, ci]),
             "ob_load": (ci, [vp, ci, vp, ci, vp, vp]), "ob_step": (None, [vp, cf]), "ob_nbody": (ci, [vp]),
             "ob_read_bodies": (None, [vp, vp]), "ob_stats": (None, [vp, vp]), "ob_candidates": (ci, [vp, vp, ci]),
-            "ob_manifolds": (ci, [vp, vp, vp, ci]), "ob_colours": (ci, [vp, vp, ci]), "ob_islands": (None, [vp, vp]),
+            "ob_manifolds": (ci, [vp, vp, vp, ci]), "ob_colours": (ci, [vp, vp, ci]), "ob_islands": (None, [vp, vp]), "ob_levels": (None, [vp, vp]),
             "ob_bruteforce_pairs": (ci, [vp, cf, vp, ci]),
             "ol_sor_sweep": (cf, [vp, vp, cf]), "ol_solve64": (ci, [vp, vp, cd, ci]), "ol_residual64": (cd, [vp, vp]),
             "ol_verdicts64": (None, [vp, vp, vp, vp, vp]), "ol_verdicts32": (None, [vp, vp, vp, vp]),
@@ -106,6 +106,11 @@ class BoxOracle:
     def islands(self):
         out = np.zeros(self.nb, np.int32)
         lib().ob_islands(self._h, _p(out))
+        return out
+
+    def levels(self):
+        out = np.zeros(self.nb, np.int32)
+        lib().ob_levels(self._h, _p(out))
         return out
 
     def bruteforce_pairs(self, dt=1.0 / 60.0, cap=4_000_000):

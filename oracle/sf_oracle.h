@@ -21,13 +21,15 @@ typedef struct ob_world ob_world;
  * colour-bucketed solve in colour order then the overflow in manifold order, wakes from the step-start snapshot. */
 enum { OB_ORDER_REFERENCE = 0, OB_ORDER_COLOURED = 1 };
 
-/* Parameter block, phys.rs:74-106 defaults plus the colouring limits. Index names in sf.h (SF_P_*). */
-#define OB_NPARAM 24
+/* Parameter block, phys.rs:74-106 defaults plus the colouring limits and the GPU world's solver extensions (mass-ratio
+ * limit, shock-propagation level passes, restitution threshold and iterations, supports-first colouring rounds; all
+ * off by default). Index names in sf.h (SF_P_*). */
+#define OB_NPARAM 30
 
 /* body record: p[3] q[4] v[3] w[3] inv_m inv_i[3] awake sleep_t reserved  (20 floats) */
 #define OB_BODY_F 20
-/* shape record: lp[3] lq[4] h[3] mu  (11 floats); shape_body[] gives the owning body */
-#define OB_SHAPE_F 11
+/* shape record: lp[3] lq[4] h[3] mu e  (12 floats: friction, restitution); shape_body[] gives the owning body */
+#define OB_SHAPE_F 12
 
 ob_world* ob_create(int order);
 void ob_destroy(ob_world* w);
@@ -50,6 +52,9 @@ int ob_manifolds(const ob_world* w, int32_t* keys, float* data, int cap);
 int ob_colours(const ob_world* w, int32_t* out, int cap);
 /* island label per body (min body index of its island, -1 for bodies not awake at the step start) */
 void ob_islands(const ob_world* w, int32_t* out);
+/* shock-propagation heights of the last step per body (255 - level; 255 for bodies not moving, 0 for bodies no
+ * support chain reached), all 0 when the passes are off */
+void ob_levels(const ob_world* w, int32_t* out);
 /* brute-force O(n^2) candidate pairs for the current pose (test pin for the broadphase) */
 int ob_bruteforce_pairs(ob_world* w, float dt, int32_t* out, int cap);
 

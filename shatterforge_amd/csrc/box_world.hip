@@ -33,14 +33,15 @@ namespace {
 constexpr int NT = 256;  // threads per block of the grid-stride kernels
 
 // device counters
-// (C_ACTIVE0 + g: set when a manifold claimed in claim-round group g, rounds 4g to 4g + 3)
+// (C_ACTIVE0 + g: set when a manifold claimed in the rounds the host checks as group g)
 enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_NSLOT, C_NBIGC, C_NBIGL,
-       C_ACTIVE0, C_NCTR = C_ACTIVE0 + 8 };
+       C_ACTIVE0, C_NACTIVE = 16, C_NCTR = C_ACTIVE0 + C_NACTIVE };
 
 // manifold rows (each row is a float4 array of mcap entries)
 // (the inertia terms, a body's world inverse inertia times an arm or the normal, are not stored: the solve forms them
 // from the bodies' inverse inertia, fixed for the step, with the build's operations; MR_JN holds the four points'
-// normal impulses)
+// normal impulses. Spare lanes: MR_LAF.w the pair's restitution, MR_LBF.w set once a relax pass pushed, each point's
+// third row .w its normal velocity at the build, for the restitution pass.)
 enum { MR_N, MR_T1, MR_T2, MR_E0, MR_J, MR_CA1, MR_CA2, MR_CB1, MR_CB2, MR_JN, MR_LAF, MR_LBF, MR_P0 };
 constexpr int MR_PER_PT = 3;  // raxn|separation, rbxn|nm, la|-
 constexpr int MR_ROWS = MR_P0 + 4 * MR_PER_PT;
@@ -60,6 +61,10 @@ struct StepK {
     float spec, inv_cell, friction, sleep_lin2, sleep_ang2, sleep_time, sleep_dist2, sleep_rot, wmax;
     float gx, gy, gz;
     int anchor, allow_sleep, K, spread, use_bias;
+    // the GPU world's extensions: mass-ratio limit (0 off), shock-propagation passes (0 off, else even), restitution
+    // threshold and iterations, whether any shape bounces, support-only colouring rounds
+    float ratio, rest_thr;
+    int shock, rest_iters, any_rest, col_support;
 };
 
 struct Dev {
@@ -77,6 +82,7 @@ struct Dev {
     // shapes
     int32_t* shb;
     float4 *slp, *slr, *shh, *swp, *swr, *slo, *shi;  // slp: lp|reach, slr/swr: 3 per shape, shh: h|mu
+    float* srest;                                     // restitution per shape
     // broadphase
     uint32_t *ecnt, *eoff, *ekey;
     int32_t* eshape;
@@ -104,6 +110,8 @@ struct Dev {
     // per-slot manifold lists (m * 2 + side), in manifold order
     uint32_t *blcnt, *bloff, *blfill, *blist;
     uint32_t *blinv, *minv;  // the list entry of (colour position, side), of (manifold, side); ~0u: the static world
+    // shock-propagation heights per solver slot (255 - level), the two copies the level passes alternate between
+    uint32_t *lvl, *lvl2;
     // islands
     int32_t* parent;
     uint32_t* sst;     // sleep state bits per body (SS_*)
@@ -807,6 +815,8 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         V3 t1, t2;
         tangents(n, t1, t2);
         float mu = mix_mu(D.shh[a].w, b >= 0 ? D.shh[b].w : K.friction);
+        // the larger restitution of the two, the ground's being 0 (gpu_world.js:601-603)
+        const float rest = sf_max(D.srest[a], b >= 0 ? D.srest[b] : 0.0f);
         bool stick = K.anchor != 0;
         uint64_t key = man_key(a, b);
         int oi = find_old(D, key, nold);
@@ -845,7 +855,9 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             e0 = v3(0, 0, 0);
         }
         // masses (the static side: slot 0, zero inverse mass and inertia)
-        float ima = SV(D, sa).w, imb = SV(D, sb).w;
+        const float4 sva = SV(D, sa), svb = SV(D, sb);
+        const V3 swa = v3f4(SW(D, sa)), swb = v3f4(SW(D, sb));
+        float ima = sva.w, imb = svb.w;
         M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
         const ManBuf& C = D.cur;
         float jnk[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -878,10 +890,13 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             V3 iaxn = mulv(iia, raxn), ibxn = mulv(iib, rbxn);
             float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn);
             float nm = kk > 0.0f ? 1.0f / kk : 0.0f;
+            // the point's normal velocity at the step start (negative: approaching), gpu_world.js:662
+            const V3 pva = vadd(v3f4(sva), vcross(swa, rap)), pvb = vadd(v3f4(svb), vcross(swb, rbp));
+            const float rv = vdot(vsub(pvb, pva), n);
             MROW(D, C, MR_P0 + MR_PER_PT * k + 0, p) = f4(raxn, pts[k].s);
             MROW(D, C, MR_P0 + MR_PER_PT * k + 1, p) = f4(rbxn, nm);
             iaxk[k] = iaxn; ibxk[k] = ibxn; jnk[k] = jn;
-            MROW(D, C, MR_P0 + MR_PER_PT * k + 2, p) = f4(la, 0.0f);
+            MROW(D, C, MR_P0 + MR_PER_PT * k + 2, p) = f4(la, rv);
         }
         V3 ca1 = vcross(fa, t1), ca2 = vcross(fa, t2), cb1 = vcross(fb, t1), cb2 = vcross(fb, t2);
         V3 ia1 = mulv(iia, ca1), ia2 = mulv(iia, ca2), ib1 = mulv(iib, cb1), ib2 = mulv(iib, cb2);
@@ -902,8 +917,8 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         MROW(D, C, MR_CB1, p) = f4(cb1, (float)cnt);
         MROW(D, C, MR_CB2, p) = f4(cb2, stick ? 1.0f : 0.0f);
         MROW(D, C, MR_JN, p) = make_float4(jnk[0], jnk[1], jnk[2], jnk[3]);
-        MROW(D, C, MR_LAF, p) = f4(la_f, 0.0f);
-        MROW(D, C, MR_LBF, p) = f4(lb_f, 0.0f);
+        MROW(D, C, MR_LAF, p) = f4(la_f, rest);
+        MROW(D, C, MR_LBF, p) = f4(lb_f, 0.0f);  // (.w: not pushed yet)
         npts += (uint32_t)cnt;
         // the bodies' list entries of this position, their first warm-start records (from the impulses carried over
         // from the last step) and their overflow changes cleared
@@ -965,7 +980,11 @@ __global__ void __launch_bounds__(NT) k_col_init(Dev D) {
 
 // (the activity flag of the round's group is set by one plain store per wavefront that claimed: tens of thousands of
 // atomic adds on one address serialise)
-__global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int r) {
+// Supports first (gpu/gpu_world.js:954-985): in rounds r < rs only supports claim (manifolds with a side that is not
+// moving: the ground, a static or sleeping body), and with rs > 0 supports outrank the others in every round. A
+// claim's priority: the round + 1 (bits 57-62: a claim left from an earlier round never wins), the support bit (56),
+// a hash of the key and round, the manifold index.
+__global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int r, int rs, int flag) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     const unsigned long long all = K == 64 ? ~0ull : ((1ull << K) - 1ull);
     bool claimed = false;
@@ -973,6 +992,8 @@ __global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int 
         if (D.mcol[m] != -1) continue;
         int4 hd = D.cur.hdr[m];
         int sa = hd.z, sb = hd.w;
+        const bool support = !sa || !sb;
+        if (r < rs && !support) { D.mcand[m] = -1; continue; }  // (no claim this round: resolve passes it over)
         unsigned long long used = (sa ? D.bmask[sa] : 0ull) | (sb ? D.bmask[sb] : 0ull);
         unsigned long long fr = all & ~used;
         if (!fr) { D.mcol[m] = K; continue; }
@@ -982,7 +1003,7 @@ __global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int 
         int pick = (int)(mix32(kh + (uint32_t)r * 0x165667B1u) % (uint32_t)lim);
         for (int q = 0; q < pick; q++) fr &= fr - 1;
         int c = __ffsll((long long)fr) - 1;
-        unsigned long long pri = ((unsigned long long)(r + 1) << 56) |
+        unsigned long long pri = ((unsigned long long)(r + 1) << 57) | ((unsigned long long)(rs > 0 && support) << 56) |
                                  ((unsigned long long)(mix32(kh ^ ((uint32_t)r * 0x27D4EB2Fu)) >> 8) << 32) |
                                  (unsigned long long)(0xFFFFFFFFu - m);
         D.mcand[m] = c;
@@ -991,7 +1012,7 @@ __global__ void __launch_bounds__(NT) k_col_claim(Dev D, int K, int spread, int 
         if (sb) atomicMax(&D.claim[(size_t)sb * K + c], pri);
         claimed = true;
     }
-    if (__ballot(claimed) && (threadIdx.x & 63) == 0) D.ctr[C_ACTIVE0 + min(r >> 2, 7)] = 1u;
+    if (__ballot(claimed) && (threadIdx.x & 63) == 0) D.ctr[C_ACTIVE0 + flag] = 1u;
 }
 
 __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
@@ -1000,6 +1021,7 @@ __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
         if (D.mcol[m] != -1) continue;
         int4 hd = D.cur.hdr[m];
         int sa = hd.z, sb = hd.w, c = D.mcand[m];
+        if (c < 0) continue;
         unsigned long long pri = D.mpri[m];
         if (sa && D.claim[(size_t)sa * K + c] != pri) continue;
         if (sb && D.claim[(size_t)sb * K + c] != pri) continue;
@@ -1343,14 +1365,38 @@ __global__ void __launch_bounds__(NT) k_warm(Dev D, StepK K) {
 
 __device__ __forceinline__ float slop_fn(float e, float slop) { return e > slop ? e - slop : (e < -slop ? e + slop : 0.0f); }
 
+// Mass-ratio limit (gpu_world.js:1155-1160): across one contact the heavier side responds as if at most R times heavier
+// than the lighter; the factors multiply the two inverse masses (1 when either side is static or R is 0).
+__device__ __forceinline__ void ratio_scale(float ia, float ib, float R, float& sa, float& sb) {
+    sa = 1.0f; sb = 1.0f;
+    if (R <= 0.0f || ia <= 0.0f || ib <= 0.0f) return;
+    if (ia * R < ib) sa = ib / (R * ia);
+    else if (ib * R < ia) sb = ia / (R * ib);
+}
+// the limited side keeps the part of a change d to its velocity x that brings |x| down to its least and takes the rest
+// divided by sc: the limit only takes energy from the heavy body (gpu_world.js:1166-1177)
+__device__ __forceinline__ V3 slowing(V3 x, V3 d, float sc) {
+    const float dd = vdot(d, d);
+    if (dd == 0.0f) return d;
+    const float t = sf_min(sf_max(-vdot(x, d) / dd, 0.0f), 1.0f);
+    return vmul(d, t + (1.0f - t) / sc);
+}
+// a slot's shock-propagation height, 255 for the static world (v4: the slot's v|inv_m)
+__device__ __forceinline__ uint32_t slot_height(const Dev& D, int s, const float4& v4) { return s && v4.w > 0.0f ? D.lvl[s] : 255u; }
+
 // The soft-step solve of one manifold (phys.rs:268-323) on velocity copies of its two bodies. fa, fb = 0: as the
 // reference solves it. fa, fb > 0 (the overflow pass): mass splitting, each side's inverse mass and inertia scaled
 // by the number of overflow manifolds at that body and the effective masses recomputed from the scaled ones.
+// The GPU world's extensions (gpu_world.js:1207-1329): with a mass-ratio limit the inverse masses (as split) are
+// scaled by ratio_scale as well and each limited side keeps its change only as `slowing` allows; in the relax pass
+// with levels (ha, hb: the sides' heights) a contact within 45 degrees of vertical whose lower body sits on fewer
+// levels holds that body where a point is separating, the upper side alone taking the point's impulse.
 __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK& K, V3& va, V3& wa, V3& vb, V3& wb,
                                            V3 tha, V3 thb, V3 dpab, float fa, float fb, uint32_t ea, uint32_t eb,
-                                           const M3& iia, const M3& iib) {
+                                           const M3& iia, const M3& iib, uint32_t ha, uint32_t hb) {
     const ManBuf& C = D.cur;
     const bool split = fa > 0.0f;
+    const V3 va0 = va, wa0 = wa, vb0 = vb, wb0 = wb;
     // every load of the record up front (they are independent; the stores come at the end)
     float4 n4 = MROW(D, C, MR_N, m), t14 = MROW(D, C, MR_T1, m), t24 = MROW(D, C, MR_T2, m);
     float4 ca14 = MROW(D, C, MR_CA1, m), ca24 = MROW(D, C, MR_CA2, m), cb14 = MROW(D, C, MR_CB1, m), cb24 = MROW(D, C, MR_CB2, m);
@@ -1365,7 +1411,13 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     V3 iax0[4], ibx0[4];  // the points' inertia terms, unsplit
     V3 n = v3f4(n4), t1 = v3f4(t14), t2 = v3f4(t24);
     float dn = vdot(dpab, n);
-    float ima = split ? ca14.w * fa : ca14.w, imb = split ? ca24.w * fb : ca24.w;
+    float sca, scb;
+    ratio_scale(split ? ca14.w * fa : ca14.w, split ? ca24.w * fb : ca24.w, K.ratio, sca, scb);
+    const bool rec = split || sca != 1.0f || scb != 1.0f;
+    const float xa = (split ? fa : 1.0f) * sca, xb = (split ? fb : 1.0f) * scb;
+    bool hold_a = false, hold_b = false;
+    if (!K.use_bias && K.shock) { hold_a = n.y > 0.7f && ha > hb; hold_b = n.y < -0.7f && hb > ha; }
+    float ima = rec ? ca14.w * xa : ca14.w, imb = rec ? ca24.w * xb : ca24.w;
     int cnt = (int)cb14.w;
     float tot = 0.0f;
     float njs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -1377,9 +1429,9 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
         const bool on = k < cnt;
         V3 raxn = v3f4(r0[k]), rbxn = v3f4(r1[k]);
         iax0[k] = mulv(iia, raxn); ibx0[k] = mulv(iib, rbxn);
-        V3 iaxn = split ? vmul(iax0[k], fa) : iax0[k], ibxn = split ? vmul(ibx0[k], fb) : ibx0[k];
+        V3 iaxn = rec ? vmul(iax0[k], xa) : iax0[k], ibxn = rec ? vmul(ibx0[k], xb) : ibx0[k];
         float nm = r1[k].w;
-        if (split) { float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
+        if (rec) { float kk = ima + imb + vdot(raxn, iaxn) + vdot(rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
         float jn = jnv[k];
         float s = r0[k].w + dn + vdot(thb, rbxn) - vdot(tha, raxn);
         float bias, ms, is;
@@ -1387,12 +1439,15 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
         else if (K.use_bias) { bias = sf_max(K.bias_rate * s, -K.max_push); ms = K.mass_scale; is = K.imp_scale; }
         else { bias = 0.0f; ms = 1.0f; is = 0.0f; }
         float vn = vdot(vsub(vb, va), n) + vdot(wb, rbxn) - vdot(wa, raxn);
+        const bool ga = hold_a && vn > 0.0f, gb = hold_b && vn > 0.0f;  // this point holds side a (b)
+        if (ga) { float kk = imb + vdot(rbxn, ibxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
+        if (gb) { float kk = ima + vdot(raxn, iaxn); nm = kk > 0.0f ? 1.0f / kk : 0.0f; }
         float imp = -nm * ms * (vn + bias) - is * jn;
         float nj = sf_max(jn + imp, 0.0f);
         imp = nj - jn;
         const float tot1 = tot + nj;
-        const V3 va1 = vsub(va, vmul(n, imp * ima)), wa1 = vsub(wa, vmul(iaxn, imp));
-        const V3 vb1 = vadd(vb, vmul(n, imp * imb)), wb1 = vadd(wb, vmul(ibxn, imp));
+        const V3 va1 = ga ? va : vsub(va, vmul(n, imp * ima)), wa1 = ga ? wa : vsub(wa, vmul(iaxn, imp));
+        const V3 vb1 = gb ? vb : vadd(vb, vmul(n, imp * imb)), wb1 = gb ? wb : vadd(wb, vmul(ibxn, imp));
         tot = on ? tot1 : tot;
         njs[k] = on ? nj : jnv[k];
         va = vsel(on, va1, va); wa = vsel(on, wa1, wa); vb = vsel(on, vb1, vb); wb = vsel(on, wb1, wb);
@@ -1402,8 +1457,8 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     const V3 ian0 = mulv(iia, n), ibn0 = mulv(iib, n);
     V3 ia1 = ia10, ia2 = ia20, ib1 = ib10, ib2 = ib20, ian = ian0, ibn = ibn0;
     float tm1 = t14.w, tm2 = t24.w, twm = e04.w;
-    if (split) {
-        ia1 = vmul(ia1, fa); ia2 = vmul(ia2, fa); ian = vmul(ian, fa); ib1 = vmul(ib1, fb); ib2 = vmul(ib2, fb); ibn = vmul(ibn, fb);
+    if (rec) {
+        ia1 = vmul(ia1, xa); ia2 = vmul(ia2, xa); ian = vmul(ian, xa); ib1 = vmul(ib1, xb); ib2 = vmul(ib2, xb); ibn = vmul(ibn, xb);
         float k1 = ima + imb + vdot(ca1, ia1) + vdot(cb1, ib1);
         float k2 = ima + imb + vdot(ca2, ia2) + vdot(cb2, ib2);
         tm1 = k1 > 0.0f ? 1.0f / k1 : 0.0f; tm2 = k2 > 0.0f ? 1.0f / k2 : 0.0f;
@@ -1442,10 +1497,13 @@ __device__ __forceinline__ void solve_core(const Dev& D, uint32_t m, const StepK
     float dtw = jt - ot;
     wa = vsub(wa, vmul(ian, dtw));
     wb = vadd(wb, vmul(ibn, dtw));
-    // stores: impulses and the stick flag
+    if (sca != 1.0f) { va = vadd(va0, slowing(va0, vsub(va, va0), sca)); wa = vadd(wa0, slowing(wa0, vsub(wa, wa0), sca)); }
+    if (scb != 1.0f) { vb = vadd(vb0, slowing(vb0, vsub(vb, vb0), scb)); wb = vadd(wb0, slowing(wb0, vsub(wb, wb0), scb)); }
+    // stores: impulses and the stick flag (and, for the restitution pass, that a relax pass pushed)
     MROW(D, C, MR_JN, m) = make_float4(njs[0], njs[1], njs[2], njs[3]);
     MROW(D, C, MR_J, m) = make_float4(j1, j2, jt, j4.w);
     if (unstick) reinterpret_cast<float*>(&MROW(D, C, MR_CB2, m))[3] = 0.0f;
+    if (K.any_rest && !K.use_bias && tot > 0.0f) reinterpret_cast<float*>(&MROW(D, C, MR_LBF, m))[3] = 1.0f;
     // the relax pass ends the sub-step: its impulses are the next sub-step's warm start
     if (!K.use_bias)
         write_warm(D, ea, eb, cnt, n4, t14, t24, j1, j2, jt, njs, iax0, ibx0, ia10, ia20, ian0, ib10, ib20, ibn0, ca14.w, ca24.w);
@@ -1463,7 +1521,9 @@ __device__ void solve_one(const Dev& D, uint32_t m, const StepK& K) {
     // the relax pass writes the next warm start into the bodies' list entries
     uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
     const M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
-    solve_core(D, m, K, va, wa, vb, wb, v3f4(STH(D, sa)), v3f4(STH(D, sb)), dpab, 0.0f, 0.0f, ea, eb, iia, iib);
+    uint32_t ha = 255u, hb = 255u;
+    if (K.shock && !K.use_bias) { ha = slot_height(D, sa, va4); hb = slot_height(D, sb, vb4); }
+    solve_core(D, m, K, va, wa, vb, wb, v3f4(STH(D, sa)), v3f4(STH(D, sb)), dpab, 0.0f, 0.0f, ea, eb, iia, iib, ha, hb);
     if (sa) { SV(D, sa) = f4(va, va4.w); SW(D, sa) = f4(wa, 0.0f); }
     if (sb) { SV(D, sb) = f4(vb, vb4.w); SW(D, sb) = f4(wb, 0.0f); }
 }
@@ -1517,14 +1577,17 @@ __global__ void __launch_bounds__(NT) k_ovf_solve(Dev D, StepK K) {
         uint2 e2 = reinterpret_cast<const uint2*>(D.blinv)[m];
         int4 hd = D.cur.hdrp[m];
         int sa = hd.z, sb = hd.w;
-        V3 va0 = v3f4(SV(D, sa)), wa0 = v3f4(SW(D, sa)), vb0 = v3f4(SV(D, sb)), wb0 = v3f4(SW(D, sb));
+        const float4 va4 = SV(D, sa), vb4 = SV(D, sb);
+        V3 va0 = v3f4(va4), wa0 = v3f4(SW(D, sa)), vb0 = v3f4(vb4), wb0 = v3f4(SW(D, sb));
         V3 va = va0, wa = wa0, vb = vb0, wb = wb0;
         V3 dpab = vsub(v3f4(SDP(D, sb)), v3f4(SDP(D, sa)));
         float fa = sa ? (float)D.ovcnt[sa] : 1.0f, fb = sb ? (float)D.ovcnt[sb] : 1.0f;
         uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
         const M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
+        uint32_t ha = 255u, hb = 255u;
+        if (K.shock && !K.use_bias) { ha = slot_height(D, sa, va4); hb = slot_height(D, sb, vb4); }
         solve_core(D, m, K, va, wa, vb, wb, v3f4(STH(D, sa)), v3f4(STH(D, sb)), dpab, fa, fb,
-                   K.use_bias ? ~0u : ea, K.use_bias ? ~0u : eb, iia, iib);
+                   K.use_bias ? ~0u : ea, K.use_bias ? ~0u : eb, iia, iib, ha, hb);
         // each side's change into its body's list entry
         if (ea != ~0u) { OVL(D, ea)[0] = f4(vsub(va, va0), 0.0f); OVL(D, ea)[1] = f4(vsub(wa, wa0), 0.0f); }
         if (eb != ~0u) { OVL(D, eb)[0] = f4(vsub(vb, vb0), 0.0f); OVL(D, eb)[1] = f4(vsub(wb, wb0), 0.0f); }
@@ -1612,6 +1675,148 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
         }
     });
 }
+// ---------------------------------------------------------------- shock propagation and restitution
+// One level pass (gpu_world.js:1178-1203) over the manifolds (by position): heights are read from rd and raised in wr
+// with atomic maxima (both bodies' read heights carried over; the upper body of a contact within 45 degrees of
+// vertical lifted to one below its support's), so the result does not depend on the order threads run in.
+__global__ void __launch_bounds__(NT) k_level_pass(Dev D, const uint32_t* rd, uint32_t* wr) {
+    const uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    GRID_STRIDE(p, nm) {
+        const int4 hd = D.cur.hdrp[p];
+        const int sa = hd.z, sb = hd.w;
+        const bool da = sa && SV(D, sa).w > 0.0f, db = sb && SV(D, sb).w > 0.0f;
+        const uint32_t ha = da ? rd[sa] : 255u, hb = db ? rd[sb] : 255u;
+        const float ny = MROW(D, D.cur, MR_N, p).y;
+        if (da) atomicMax(&wr[sa], ha);
+        if (db) atomicMax(&wr[sb], hb);
+        if (ny > 0.7f && ha > 1u && db) atomicMax(&wr[sb], ha - 1u);
+        if (ny < -0.7f && hb > 1u && da) atomicMax(&wr[sa], hb - 1u);
+    }
+}
+
+// Restitution after the sub-steps (gpu_world.js:1333-1363): in a manifold with a restitution that pushed during the
+// step, the points that approached faster than the threshold at the step start have their normal velocity driven to
+// -rest times that approach, the points iterated together. rest_live: whether manifold m (a position) takes part
+// (loads only the flags and the points' approach speeds).
+__device__ __forceinline__ bool rest_live(const Dev& D, uint32_t m, const StepK& K) {
+    const ManBuf& C = D.cur;
+    const float rest = MROW(D, C, MR_LAF, m).w, pushed = MROW(D, C, MR_LBF, m).w;
+    if (!(rest > 0.0f) || pushed == 0.0f) return false;
+    const int cnt = (int)MROW(D, C, MR_CB1, m).w;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) if (k < cnt && !(MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m).w > -K.rest_thr)) any = true;
+    return any;
+}
+// fa, fb as in solve_core (mass splitting in the overflow), with the mass-ratio limit
+__device__ __forceinline__ void rest_core(const Dev& D, uint32_t m, const StepK& K, V3& va, V3& wa, V3& vb, V3& wb,
+                                          float fa, float fb, const M3& iia, const M3& iib) {
+    const ManBuf& C = D.cur;
+    const bool split = fa > 0.0f;
+    const V3 va0 = va, wa0 = wa, vb0 = vb, wb0 = wb;
+    const float4 n4 = MROW(D, C, MR_N, m), jn4 = MROW(D, C, MR_JN, m);
+    const float ima0 = MROW(D, C, MR_CA1, m).w, imb0 = MROW(D, C, MR_CA2, m).w, rest = MROW(D, C, MR_LAF, m).w;
+    const int cnt = (int)MROW(D, C, MR_CB1, m).w;
+    float4 r0[4], r1[4];
+    float rv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        r0[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 0, m);
+        r1[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 1, m);
+        rv[k] = MROW(D, C, MR_P0 + MR_PER_PT * k + 2, m).w;
+    }
+    float sca, scb;
+    ratio_scale(split ? ima0 * fa : ima0, split ? imb0 * fb : imb0, K.ratio, sca, scb);
+    const bool rec = split || sca != 1.0f || scb != 1.0f;
+    const float xa = (split ? fa : 1.0f) * sca, xb = (split ? fb : 1.0f) * scb;
+    const float ima = rec ? ima0 * xa : ima0, imb = rec ? imb0 * xb : imb0;
+    const V3 n = v3f4(n4);
+    float jn[4] = {jn4.x, jn4.y, jn4.z, jn4.w};
+    V3 iaxn[4], ibxn[4];
+    float nm[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const V3 raxn = v3f4(r0[k]), rbxn = v3f4(r1[k]);
+        iaxn[k] = mulv(iia, raxn); ibxn[k] = mulv(iib, rbxn);
+        if (rec) { iaxn[k] = vmul(iaxn[k], xa); ibxn[k] = vmul(ibxn[k], xb); }
+        nm[k] = r1[k].w;
+        if (rec) { float kk = ima + imb + vdot(raxn, iaxn[k]) + vdot(rbxn, ibxn[k]); nm[k] = kk > 0.0f ? 1.0f / kk : 0.0f; }
+    }
+    for (int it = 0; it < K.rest_iters; it++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool on = k < cnt && !(rv[k] > -K.rest_thr);
+            const V3 raxn = v3f4(r0[k]), rbxn = v3f4(r1[k]);
+            const float vn = vdot(vsub(vb, va), n) + vdot(wb, rbxn) - vdot(wa, raxn);
+            const float nj = sf_max(jn[k] - nm[k] * (vn + rest * rv[k]), 0.0f);
+            const float imp = nj - jn[k];
+            const V3 va1 = vsub(va, vmul(n, imp * ima)), wa1 = vsub(wa, vmul(iaxn[k], imp));
+            const V3 vb1 = vadd(vb, vmul(n, imp * imb)), wb1 = vadd(wb, vmul(ibxn[k], imp));
+            jn[k] = on ? nj : jn[k];
+            va = vsel(on, va1, va); wa = vsel(on, wa1, wa); vb = vsel(on, vb1, vb); wb = vsel(on, wb1, wb);
+        }
+    }
+    if (sca != 1.0f) { va = vadd(va0, slowing(va0, vsub(va, va0), sca)); wa = vadd(wa0, slowing(wa0, vsub(wa, wa0), sca)); }
+    if (scb != 1.0f) { vb = vadd(vb0, slowing(vb0, vsub(vb, vb0), scb)); wb = vadd(wb0, slowing(wb0, vsub(wb, wb0), scb)); }
+    MROW(D, C, MR_JN, m) = make_float4(jn[0], jn[1], jn[2], jn[3]);
+}
+__device__ void rest_one(const Dev& D, uint32_t m, const StepK& K) {
+    if (!rest_live(D, m, K)) return;
+    const int4 hd = D.cur.hdrp[m];
+    const int sa = hd.z, sb = hd.w;
+    const float4 va4 = SV(D, sa), vb4 = SV(D, sb);
+    V3 va = v3f4(va4), wa = v3f4(SW(D, sa)), vb = v3f4(vb4), wb = v3f4(SW(D, sb));
+    const M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
+    rest_core(D, m, K, va, wa, vb, wb, 0.0f, 0.0f, iia, iib);
+    if (sa) { SV(D, sa) = f4(va, va4.w); SW(D, sa) = f4(wa, 0.0f); }
+    if (sb) { SV(D, sb) = f4(vb, vb4.w); SW(D, sb) = f4(wb, 0.0f); }
+}
+// one colour bucket of the restitution pass, spread over the XCDs as k_solve_bucket's
+__global__ void __launch_bounds__(NT) k_rest_bucket(Dev D, StepK K, int c) {
+    const uint32_t n = D.colcnt[c], o = D.coloff[c];
+    const uint32_t G = min(gridDim.x, max((n + NT - 1) / NT, 1u)), b = blockIdx.x;
+    if (b >= G) return;
+    const uint32_t x = b & 7u, q = G >> 3, r = G & 7u;
+    const uint32_t lb = x * q + min(x, r) + (b >> 3);
+    for (uint32_t i = lb * NT + threadIdx.x; i < n; i += G * NT) rest_one(D, o + i, K);
+}
+// the small colours from c0 on, one after another in one workgroup
+__global__ void __launch_bounds__(NT) k_rest_tail(Dev D, StepK K, int c0) {
+    int ncol = (int)D.ctr[C_NCOL];
+    for (int c = c0; c < ncol; c++) {
+        uint32_t n = D.colcnt[c], o = D.coloff[c];
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) rest_one(D, o + i, K);
+        __syncthreads();
+    }
+}
+// the overflow's restitution: Jacobi with mass splitting as its solve (every overflow manifold writes its changes,
+// zero when it takes no part, and k_ovf_apply averages them)
+__global__ void __launch_bounds__(NT) k_rest_ovf(Dev D, StepK K) {
+    const uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
+    const uint32_t G = min(gridDim.x, max((n + NT - 1) / NT, 1u)), b = blockIdx.x;
+    if (b >= G) return;
+    const uint32_t x = b & 7u, q = G >> 3, r = G & 7u;
+    const uint32_t lb = x * q + min(x, r) + (b >> 3);
+    for (uint32_t i = lb * NT + threadIdx.x; i < n; i += G * NT) {
+        const uint32_t m = o + i;
+        const uint2 e2 = reinterpret_cast<const uint2*>(D.blinv)[m];
+        const int4 hd = D.cur.hdrp[m];
+        const int sa = hd.z, sb = hd.w;
+        const uint32_t ea = sa ? e2.x : ~0u, eb = sb ? e2.y : ~0u;
+        V3 dva = v3(0, 0, 0), dwa = v3(0, 0, 0), dvb = v3(0, 0, 0), dwb = v3(0, 0, 0);
+        if (rest_live(D, m, K)) {
+            const V3 va0 = v3f4(SV(D, sa)), wa0 = v3f4(SW(D, sa)), vb0 = v3f4(SV(D, sb)), wb0 = v3f4(SW(D, sb));
+            V3 va = va0, wa = wa0, vb = vb0, wb = wb0;
+            const float fa = sa ? (float)D.ovcnt[sa] : 1.0f, fb = sb ? (float)D.ovcnt[sb] : 1.0f;
+            const M3 iia = load_m3(SII(D, sa)), iib = load_m3(SII(D, sb));
+            rest_core(D, m, K, va, wa, vb, wb, fa, fb, iia, iib);
+            dva = vsub(va, va0); dwa = vsub(wa, wa0); dvb = vsub(vb, vb0); dwb = vsub(wb, wb0);
+        }
+        if (ea != ~0u) { OVL(D, ea)[0] = f4(dva, 0.0f); OVL(D, ea)[1] = f4(dwa, 0.0f); }
+        if (eb != ~0u) { OVL(D, eb)[0] = f4(dvb, 0.0f); OVL(D, eb)[1] = f4(dwb, 0.0f); }
+    }
+}
+
 __global__ void __launch_bounds__(NT) k_ovf_count(Dev D, int K) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(m, nm) {
@@ -1791,7 +1996,7 @@ __global__ void __launch_bounds__(NT) k_step_clear(Dev D, int K) {
     for (size_t i = blockIdx.x * (size_t)NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
         if (i < (size_t)C_NCTR) D.ctr[i] = 0u;
         if (i < (size_t)D.tcap) { D.tcnt[i] = 0u; D.tfill[i] = 0u; }
-        if (i < nb1) { D.blcnt[i] = 0u; D.blfill[i] = 0u; D.bmask[i] = 0ull; D.ovcnt[i] = 0u; }
+        if (i < nb1) { D.blcnt[i] = 0u; D.blfill[i] = 0u; D.bmask[i] = 0ull; D.ovcnt[i] = 0u; D.lvl[i] = 0u; D.lvl2[i] = 0u; }
         if (i < ncl) D.claim[i] = 0ull;
         if (i < 72) { D.colcnt[i] = 0u; D.colfill[i] = 0u; }
     }
@@ -1805,6 +2010,7 @@ struct HostWorld {
     hipStream_t st = nullptr;
     int body_cap = 0, shape_cap = 0;
     float prm[SF_NPARAM];
+    bool any_rest = false;  // some loaded shape has a restitution above zero
     Dev D{};
     std::vector<void*> allocs;
     uint32_t nold = 0;
@@ -1870,6 +2076,8 @@ void default_params(float* p) {
     p[SF_P_SLEEP_ROT] = 0.02f; p[SF_P_ALLOW_SLEEP] = 1.0f; p[SF_P_CELL] = 1.0f; p[SF_P_LEAK] = 1.0f;
     p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 16;
     p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_TAIL_MIN] = 1024; p[SF_P_SOLVE_NT] = 64;
+    p[SF_P_RATIO] = 0.0f; p[SF_P_SHOCK] = 0.0f; p[SF_P_REST_THRESHOLD] = 1.0f; p[SF_P_REST_ITERS] = 8.0f;
+    p[SF_P_COL_SUPPORT] = 0.0f;
 }
 
 int world_alloc(HostWorld* w) {
@@ -1888,6 +2096,7 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.sbod, SLOT_F4 * n1); r |= w->alloc(D.sp0, n1);
     r |= w->alloc(D.shb, ns); r |= w->alloc(D.slp, ns); r |= w->alloc(D.slr, 3 * (size_t)ns); r |= w->alloc(D.shh, ns);
     r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
+    r |= w->alloc(D.srest, ns); r |= w->alloc(D.lvl, n1); r |= w->alloc(D.lvl2, n1);
     r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
     r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
     r |= w->alloc(D.tidx, D.ecap); r |= w->alloc(D.soff, D.tcap); r |= w->alloc(D.slist, D.ecap);
@@ -1942,6 +2151,13 @@ StepK make_step(const float* P, float dt) {
     K.K = (int)P[SF_P_COL_LIMIT]; if (K.K < 1) K.K = 1; if (K.K > 64) K.K = 64;
     K.spread = (int)P[SF_P_COL_SPREAD]; if (K.spread < 1) K.spread = 1;
     K.use_bias = 1;
+    K.ratio = P[SF_P_RATIO] > 0.0f ? P[SF_P_RATIO] : 0.0f;
+    K.shock = std::min(std::max((int)P[SF_P_SHOCK], 0), 64);
+    K.shock = (K.shock + 1) & ~1;  // an even count: the heights end in lvl
+    K.rest_thr = P[SF_P_REST_THRESHOLD];
+    K.rest_iters = std::min(std::max((int)P[SF_P_REST_ITERS], 0), 64);
+    K.col_support = std::max((int)P[SF_P_COL_SUPPORT], 0);
+    K.any_rest = 0;  // (set from the loaded shapes)
     return K;
 }
 
@@ -1977,18 +2193,29 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_apply_wake, dim3(gb), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_man_hdr, dim3(gc), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[2], s));
-    // colouring rounds (checked for progress every few rounds)
+    // colouring rounds (checked for progress every few rounds and at the end of the support rounds: rounds in which
+    // nothing claimed change nothing more in their phase, so a quiet group ends the support phase or the colouring)
     int R = (int)w->prm[SF_P_COL_ROUNDS];
+    const int RS = K.col_support;
     hipLaunchKernelGGL(k_col_init, dim3(gm), dim3(NT), 0, s, D);
-    int rounds = 0;
+    int rounds = 0, flag = 0;
     for (int r = 0; r < R; r++) {
-        hipLaunchKernelGGL(k_col_claim, dim3(gm), dim3(NT), 0, s, D, K.K, K.spread, r);
+        hipLaunchKernelGGL(k_col_claim, dim3(gm), dim3(NT), 0, s, D, K.K, K.spread, r, RS, flag);
         hipLaunchKernelGGL(k_col_resolve, dim3(gm), dim3(NT), 0, s, D, K.K);
         rounds++;
-        if ((r & 3) == 3 && r + 1 < R) {
+        if (((r & 3) == 3 || r == RS - 1) && r + 1 < R) {
             SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
             SF_HIP(hipStreamSynchronize(s));
-            if (w->h_ctr[C_ACTIVE0 + std::min(r >> 2, 7)] == 0) break;
+            const bool active = w->h_ctr[C_ACTIVE0 + flag] != 0;
+            flag = std::min(flag + 1, C_NACTIVE - 1);
+            if (flag == C_NACTIVE - 1) {  // (the last flag is reused: cleared before its next group)
+                w->h_ctr[C_ACTIVE0 + flag] = 0;
+                SF_HIP(hipMemcpyAsync(D.ctr + C_ACTIVE0 + flag, w->h_ctr + C_ACTIVE0 + flag, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            }
+            if (!active) {
+                if (r < RS) { r = RS - 1; continue; }
+                break;
+            }
         }
     }
     w->stats[SF_S_ROUNDS] = rounds;
@@ -2007,6 +2234,9 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_bl_inv, dim3(sf_grid(2ull * D.mcap)), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_build, dim3(gc), dim3(NT), 0, s, D, K, w->nold);
     hipLaunchKernelGGL(k_sum_points, dim3(1), dim3(NT), 0, s, D, gc);
+    // shock-propagation heights: an even number of passes alternating between the two copies, ending in lvl
+    for (int k = 0; k < K.shock; k++)
+        hipLaunchKernelGGL(k_level_pass, dim3(gm), dim3(NT), 0, s, D, (k & 1) ? D.lvl2 : D.lvl, (k & 1) ? D.lvl : D.lvl2);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
     SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
     SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
@@ -2056,6 +2286,15 @@ int substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, in
             hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kr);
             if (it + 1 < sub) hipLaunchKernelGGL(k_ovf_apply_warm, dim3(gw), dim3(NT), 0, s, D, K);
             else hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
+        }
+    }
+    // restitution after the sub-steps: the colours in order, then the overflow (Jacobi with mass splitting)
+    if (K.any_rest && K.rest_iters > 0) {
+        for (int c = 0; c < big; c++) hipLaunchKernelGGL(k_rest_bucket, dim3(sf_grid(grid[c] * (unsigned)solve_nt_of(w->prm), NT, 16384)), dim3(NT), 0, s, D, Kr, c);
+        if (tail) hipLaunchKernelGGL(k_rest_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
+        if (ovf) {
+            hipLaunchKernelGGL(k_rest_ovf, dim3(grid[big]), dim3(NT), 0, s, D, Kr);
+            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
         }
     }
     return 0;
@@ -2165,6 +2404,7 @@ int world_step(HostWorld* w, float dt) {
     if (!(dt >= 0.0f) || dt > 1.0f) { sf::set_error(SF_E_ARG, "dt out of range"); return SF_E_ARG; }
     SF_HIP(hipSetDevice(w->dev));
     StepK K = make_step(w->prm, dt);
+    K.any_rest = w->any_rest ? 1 : 0;
     int r = step_front(w, K);
     if (r) return r;
     r = step_back(w, K);
@@ -2208,6 +2448,8 @@ int world_load(HostWorld* w, int nb, const float* bodies, int ns, const int32_t*
         fl[i] = f[17] > 0.5f ? 1 : 0;
     }
     std::vector<float4> slp(ns), slr(3 * (size_t)ns), shh(ns);
+    std::vector<float> srest(ns);
+    w->any_rest = false;
     w->h_h.assign(3 * (size_t)nb, 0.0f);
     std::vector<int> seen(nb, 0);
     for (int s = 0; s < ns; s++) {
@@ -2222,6 +2464,8 @@ int world_load(HostWorld* w, int nb, const float* bodies, int ns, const int32_t*
         slr[3 * s + 1] = make_float4(2.0f * (x * y - z * ww), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + x * ww), 0.0f);
         slr[3 * s + 2] = make_float4(2.0f * (x * z + y * ww), 2.0f * (y * z - x * ww), 1.0f - 2.0f * (x * x + y * y), 0.0f);
         shh[s] = make_float4(hx, hy, hz, f[10]);
+        srest[s] = f[11];
+        if (f[11] > 0.0f) w->any_rest = true;
         if (!seen[shb[s]]) { seen[shb[s]] = 1; w->h_h[3 * shb[s]] = hx; w->h_h[3 * shb[s] + 1] = hy; w->h_h[3 * shb[s] + 2] = hz; }
     }
     hipStream_t s = w->st;
@@ -2237,6 +2481,7 @@ int world_load(HostWorld* w, int nb, const float* bodies, int ns, const int32_t*
     SF_HIP(hipMemcpyAsync(D.slp, slp.data(), sizeof(float4) * ns, hipMemcpyHostToDevice, s));
     SF_HIP(hipMemcpyAsync(D.slr, slr.data(), sizeof(float4) * 3 * ns, hipMemcpyHostToDevice, s));
     SF_HIP(hipMemcpyAsync(D.shh, shh.data(), sizeof(float4) * ns, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.srest, srest.data(), sizeof(float) * ns, hipMemcpyHostToDevice, s));
     SF_HIP(hipStreamSynchronize(s));
     w->nold = 0;
     w->h_shb.assign(shb, shb + ns);

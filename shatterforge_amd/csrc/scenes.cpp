@@ -57,9 +57,11 @@ struct Rng {
     double normal() { double u = uni(), v = uni(); if (u < 1e-300) u = 1e-300; return std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * v); }
 };
 
-// material table of the GPU engine (gpu/gpu_world.js:15-25): density, friction; codes 1..5 as in src/lib.rs
+// material table of the GPU engine (gpu/gpu_world.js:15-25): density, friction, restitution; codes 1..5 as in
+// src/lib.rs, 0 the ground
 const double DENSITY[6] = {0.0, 2400.0, 1900.0, 7850.0, 2500.0, 500.0};
 const float FRICTION[6] = {0.6f, 0.6f, 0.6f, 0.45f, 0.5f, 0.5f};
+const float RESTITUTION[6] = {0.0f, 0.1f, 0.05f, 0.25f, 0.2f, 0.3f};
 
 // cyclic Jacobi eigen-decomposition of a symmetric 3x3 (columns of V are the eigenvectors)
 void eig_sym(double A[3][3], double vals[3], double V[3][3]) {
@@ -133,6 +135,7 @@ void build_compound(int nb, const int32_t* xyz, const uint8_t* mat, const double
         s[3] = (float)-q[0]; s[4] = (float)-q[1]; s[5] = (float)-q[2]; s[6] = (float)q[3];
         s[7] = s[8] = s[9] = (float)h;
         s[10] = FRICTION[mat[k]];
+        s[11] = RESTITUTION[mat[k]];
     }
 }
 

@@ -1,7 +1,7 @@
 """Seeded scenes as numpy arrays (thin wrappers over the library's host-side generators).
 
 A box scene is ``(bodies, shape_body, shapes)``: ``bodies`` float32 [nb, 20] (p, q, v, w, inv_m, inv_i, awake,
-sleep_t, reserved), ``shape_body`` int32 [ns], ``shapes`` float32 [ns, 11] (local p, local q, half extents, mu).
+sleep_t, reserved), ``shape_body`` int32 [ns], ``shapes`` float32 [ns, 12] (local p, local q, half extents, mu, restitution).
 A structure is ``(mat, xyz, infill)`` on the 0.5 m lattice.
 """
 import numpy as np
@@ -9,7 +9,7 @@ import numpy as np
 from ._native import lib, check, ptr
 
 BODY_F = 20
-SHAPE_F = 11
+SHAPE_F = 12
 
 
 def _box_scene(fn, *args):
@@ -41,7 +41,7 @@ def rubble(nbody=50000, p=0.3, maxsize=64, box=(40.0, 20.0, 40.0), vsig=3.0, wsi
 
 
 def compound(xyz, mat):
-    """One compound body from bricks at lattice cells: returns (body[20], shapes[n, 11])."""
+    """One compound body from bricks at lattice cells: returns (body[20], shapes[n, 12])."""
     xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
     mat = np.ascontiguousarray(mat, np.uint8).reshape(-1)
     body = np.zeros(BODY_F, np.float32)
@@ -50,7 +50,7 @@ def compound(xyz, mat):
     return body, shapes
 
 
-def box(p, h, q=(0.0, 0.0, 0.0, 1.0), density=1900.0, mu=0.6, v=(0, 0, 0), w=(0, 0, 0)):
+def box(p, h, q=(0.0, 0.0, 0.0, 1.0), density=1900.0, mu=0.6, v=(0, 0, 0), w=(0, 0, 0), e=0.0):
     """A single box body as World::add_box builds it (engine/src/phys.rs:139-148), in f32 arithmetic."""
     f = np.float32
     hx, hy, hz = f(h[0]), f(h[1]), f(h[2])
@@ -69,6 +69,7 @@ def box(p, h, q=(0.0, 0.0, 0.0, 1.0), density=1900.0, mu=0.6, v=(0, 0, 0), w=(0,
     shape[6] = 1.0
     shape[7:10] = h
     shape[10] = mu
+    shape[11] = e
     return body, shape
 
 

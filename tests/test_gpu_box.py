@@ -256,3 +256,63 @@ def test_island_sleep_bit_exact(oracle):
     n = len(b)
     assert max(awake) == n and min(awake) < n, awake
     assert len(set(awake)) >= 3, awake
+
+
+# ---------------------------------------------------------------- the GPU world's solver extensions
+def _ext_case(name):
+    """Small scenes that drive each extension: the reference's landing (shock propagation, ratio limit, restitution),
+    a bouncy drop on a static floor (restitution of the pair), a head-on meeting at 100:1 (ratio limit, no gravity)."""
+    if name == "landing":
+        hl, hh, v = 0.135, 0.5, 7.7
+        items = [scenes.box((0, hl + 0.05, 0), (hl, hl, hl), density=2400.0, e=0.1, v=(0, -v, 0)),
+                 scenes.box((0, 0.05 + 2 * hl + hh * 0.4 + 0.0005, 0), (hh, hh * 0.4, hh), density=2400.0, e=0.1, v=(0, -v, 0))]
+        return items, params(sleep=False, **world.GPU_WORLD), 40
+    if name == "bounce":
+        items = [scenes.box((0, 2.25, 0), (0.25, 0.25, 0.25), e=0.3),
+                 scenes.box((0, -0.5, 0), (3.0, 0.5, 3.0), density=0.0, e=0.5)]
+        return items, params(sleep=False, **world.GPU_WORLD), 100
+    items = [scenes.box((0, 10, 0), (0.5, 0.5, 0.5), density=12500.0, v=(2.0, 0, 0)),
+             scenes.box((0.755, 10, 0), (0.25, 0.25, 0.25), density=1000.0)]
+    return items, params(sleep=False, ratio=20.0, gy=0.0), 10
+
+
+@pytest.mark.parametrize("name", ["landing", "bounce", "headon"])
+def test_gpu_world_extensions_bit_exact_small(oracle, name):
+    print(f"case: {name} with the GPU world's extensions", flush=True)
+    items, prm, frames = _ext_case(name)
+    b, sb, s = scenes.boxes(items)
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    for k in range(frames):
+        w.step(DT)
+        o.step(DT)
+        _assert_same_step(w, o, f"{name} step {k}")
+
+
+@pytest.mark.parametrize("limit", [3, 16])
+def test_gpu_world_extensions_rubble_bit_exact(oracle, limit):
+    # mixed materials (restitution 0.05-0.3, densities 500-7850, 1-64 bricks: mass ratios past 20), supports-first
+    # colouring, level passes; a small colour cap puts restitution and the ratio limit through the split overflow pass
+    print(f"case: rubble(500) GPU world extensions, colour limit {limit}, 30 steps", flush=True)
+    b, sb, s = scenes.rubble(500, box=(9.0, 6.0, 9.0), seed=17)
+    prm = params(substeps=4, col_limit=limit, tail_min=64, **world.GPU_WORLD)
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    for k in range(30):
+        w.step(DT)
+        o.step(DT)
+        gb, ob = w.bodies(), o.bodies()
+        assert np.array_equal(gb, ob), f"step {k}: max diff {np.abs(gb - ob).max()}"
+    _assert_same_step(w, o, "final")
+    assert w.stats()["overflow"] == o.stats()["overflow"]
+
+
+def test_full_size_rubble_step_with_the_gpu_world_extensions(oracle):
+    print("case: rubble 50k bodies with the GPU world's extensions, 1 step vs oracle", flush=True)
+    b, sb, s = scenes.rubble(50000, seed=1)
+    prm = params(**world.GPU_WORLD)
+    w = sf.World.from_scene(b, sb, s, prm)
+    o = oracle.BoxOracle(b, sb, s, prm, oracle.ORDER_COLOURED)
+    w.step(DT)
+    o.step(DT)
+    _assert_same_step(w, o, "rubble 50k extensions step 0")
