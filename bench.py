@@ -1,9 +1,10 @@
 """Benchmark of the physics hot path on one MI355X.
 
 Workload (default): the rubble-pile stress scene, plain step on one scene: 50,000 compound bodies of 1-64 bricks
-(geometric p = 0.3, truncated at 64), five materials, packed in a 40 x 20 x 40 m box, v ~ N(0, 3 m/s) per axis,
-w ~ N(0, 2 rad/s), FP32, 60 Hz, 12 sub-steps, synthetic seeded input. A step is one World::step of the whole
-scene; the metric counts every live brick each step (brick-steps/s).
+(geometric p = 0.3, truncated at 64), five materials (with their friction and restitution), packed in a 40 x 20 x 40 m
+box, v ~ N(0, 3 m/s) per axis, w ~ N(0, 2 rad/s), FP32, 60 Hz, 12 sub-steps, the GPU world's solver (mass-ratio limit
+20, six shock-propagation level passes, two supports-first colouring rounds), synthetic seeded input. A step is one
+World::step of the whole scene; the metric counts every live brick each step (brick-steps/s).
 
 Prints one JSON line (see the keys below). Progress goes to stderr.
 """
@@ -19,21 +20,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_solve_bucket launch on this workload, from FETCH_SIZE / WRITE_SIZE counter runs calibrated by a copy
-# of known size (profiles/k_solve_bucket_rubble50k_traffic.md)
-SOLVE_TRAFFIC_BYTES = 13_240_832  # HBM bytes per k_solve_bucket dispatch, profiles/k_solve_bucket_rubble50k_traffic.md
+# HBM bytes per k_solve_bucket dispatch on this workload, measured by FETCH_SIZE / WRITE_SIZE counter runs calibrated by
+# a copy of known size: the json block of this profile (with the commit and launch shape it was measured at)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solve_bucket_rubble50k_traffic.md")
 # algorithmic bytes per manifold solve in one colour pass (box path): the hot contact record read, impulses written,
 # both bodies' v and w read and written
 SOLVE_BYTES_PER_MANIFOLD = 392
+
+
+def solve_traffic(solve_nt):
+    """the counter-measured HBM bytes per k_solve_bucket dispatch, if the profile was taken at this launch shape"""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        rec = json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return None, None
+    if int(rec.get("solve_nt", -1)) != int(solve_nt):
+        return None, rec
+    return int(rec["bytes_per_dispatch"]), rec
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(bodies, shape_body, shapes, prm, seconds=15.0, max_steps=20):
+def cpu_baseline(bodies, shape_body, shapes, prm, seconds=15.0, max_steps=20, sync=lambda: None):
     """The oracle (the reference's box step restated in C, reference order, one thread) on the same scene: as many
-    steps as fit in about `seconds`, starting from the same state."""
+    steps as fit in about `seconds`, starting from the same state; then the GPU on the same steps (a fresh world
+    from the same state, the same launch shapes), so the two rates cover the same window."""
+    import shatterforge_amd as sf
     from oracle import pyoracle
     o = pyoracle.BoxOracle(bodies, shape_body, shapes, prm, pyoracle.ORDER_REFERENCE)
     nbricks = len(shapes)
@@ -45,8 +61,64 @@ def cpu_baseline(bodies, shape_body, shapes, prm, seconds=15.0, max_steps=20):
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
+    w = sf.World.from_scene(bodies, shape_body, shapes, prm)
+    sync()
+    t1 = time.perf_counter()
+    for _ in range(n):
+        w.step(1.0 / 60.0)
+    sync()
+    gdt = time.perf_counter() - t1
+    w.close()
+    gpu = nbricks * n / gdt
     return {"value": nbricks * n / dt, "unit": "brick-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle box step (reference order), same scene, first {n} steps from the same state, {dt:.1f} s"}
+            "sample": f"oracle box step (reference order), same scene and parameters, steps 1-{n} from the same state, "
+                      f"{dt:.1f} s",
+            "gpu_same_steps": gpu, "gpu_over_cpu_same_steps": gpu / (nbricks * n / dt)}
+
+
+def pile_anchor(n=20000, substeps=4, steps=3):
+    """the harness check against the reference's published single-thread figure: pile(20000) at 4 sub-steps took
+    118 ms per step on a 2-core 2.8 GHz Xeon VM (docs/PHASE0.md:12); the oracle in the reference order, steps 2-4"""
+    from oracle import pyoracle
+    from shatterforge_amd import scenes
+    from shatterforge_amd.world import params
+    b, sb, s = scenes.pile(n)
+    o = pyoracle.BoxOracle(b, sb, s, params(substeps=substeps, sleep=False), pyoracle.ORDER_REFERENCE)
+    o.step(np.float32(1.0 / 60.0))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(np.float32(1.0 / 60.0))
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    log(f"pile({n}) oracle, {substeps} sub-steps: {ms:.1f} ms/step (reference: 118 ms)")
+    return {"bricks": n, "substeps": substeps, "ms_per_step": ms, "reference_ms_per_step": 118.0,
+            "reference_machine": "2-core 2.8 GHz Xeon VM, one thread (docs/PHASE0.md:3,12)", "steps_timed": steps}
+
+
+def tower_cpu(frames, seeds_warm):
+    """the C1 tower on the oracle in the reference's sequential order (one thread), the same frames as the GPU's plain
+    tower line: the load field by the reference's warm-up, the same frames before the charges, then `frames` timed"""
+    from oracle import pyoracle
+    from shatterforge_amd import scenes
+    mat, xyz, infill = scenes.tower()
+    xyz = np.array(xyz, np.int32)
+    xyz[:, 0] += 320 - 6
+    xyz[:, 2] += 320 - 6
+    O = pyoracle.LatticeOracle(order=pyoracle.ORDER_SEQ, brick_cap=len(mat) + 64, body_cap=8192)
+    O.add_structure(mat, xyz, infill)
+    O.finalize()
+    O.warm_all()
+    for _ in range(seeds_warm):
+        O.step()
+    for cx, cz in [(1, 1), (1, 11), (11, 1), (11, 11)]:
+        O.explode((cx - 6) * 0.5, 0.6, (cz - 6) * 0.5, 2.2, 1.4)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        O.step()
+    el = time.perf_counter() - t0
+    st = O.stats()
+    return {"value": len(mat) * frames / el, "unit": "brick-steps/s", "cores": 1, "kind": "port",
+            "ms_per_frame": el * 1e3 / frames, "pieces": int(st[2]), "standing": int(st[0]),
+            "sample": f"oracle brick frame (sequential order), C1 tower, the {frames} frames after the charges"}
 
 
 def _lattice_mix(n_scenes, seed=11):
@@ -55,9 +127,27 @@ def _lattice_mix(n_scenes, seed=11):
     return [serve.make_query(k, seed * 1000 + k) for k in range(n_scenes)]
 
 
-def bench_lattice(frames=120, batches=(32, 256), plain_scenes=8, plain_frames=30):
-    """legacy brick frame: the plain C1 tower collapse; batches of what-if scenes in one world; a sample of the same
-    scenes stepped one world (one launch sequence) each"""
+def _frame_split(L, frames):
+    """step `frames` frames, summing the library's own per-frame split (wall ms of the host phases and the waits on
+    the device: sf_lat_phase_ms / sf_lat_subphase_ms); returns (elapsed s, mean split per frame)"""
+    acc, sub = {}, {}
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        L.step(1.0 / 60.0)
+        for k, v in L.phase_ms().items():
+            acc[k] = acc.get(k, 0.0) + v
+        for k, v in L.subphase_ms().items():
+            sub[k] = sub.get(k, 0.0) + v
+    el = time.perf_counter() - t0
+    split = {k: v / frames for k, v in acc.items()}
+    split.update({k: v / frames for k, v in sub.items()})
+    return el, split
+
+
+def bench_lattice(frames=120, batches=(32, 256), cpu=True):
+    """legacy brick frame: the plain C1 tower collapse (with the oracle on the same frames); batches of what-if scenes
+    in one world; every scene of the batches also stepped in a world of its own (one launch sequence per scene) over
+    the same frames, so batched and plain cover the same scenes and the same frame window"""
     from shatterforge_amd import scenes, serve
     from shatterforge_amd.lattice import Lattice
     from shatterforge_amd.load import LoadBatch
@@ -80,17 +170,19 @@ def bench_lattice(frames=120, batches=(32, 256), plain_scenes=8, plain_frames=30
             break
     for cx, cz in [(1, 1), (1, 11), (11, 1), (11, 11)]:  # the corners of the four 2 x 2 columns
         L.explode(s, (cx - 6) * 0.5, 0.6, (cz - 6) * 0.5, 2.2, 1.4)
-    t0 = time.perf_counter()
-    for f in range(frames):
-        L.step(1.0 / 60.0)
-    el = time.perf_counter() - t0
+    el, split = _frame_split(L, frames)
     st = L.stats(s)
     out["plain_tower"] = {"bricks": int(len(mat)), "frames": frames, "ms_per_frame": el * 1e3 / frames,
                           "brick_steps_per_s": len(mat) * frames / el, "warmup_frames": nwarm,
-                          "pieces": int(st[2]), "standing": int(st[0])}
+                          "pieces": int(st[2]), "standing": int(st[0]), "split_ms": split}
     L.close()
     log(f"lattice plain tower: {el * 1e3 / frames:.2f} ms/frame")
-    # batches of what-if scenes in one world vs a sample of the same scenes one world each
+    if cpu:
+        c = tower_cpu(frames, nwarm)
+        out["plain_tower"]["cpu_baseline"] = c
+        out["plain_tower"]["gpu_over_cpu"] = out["plain_tower"]["brick_steps_per_s"] / c["value"]
+        log(f"lattice tower on the oracle (sequential order, one thread): {c['ms_per_frame']:.2f} ms/frame")
+    # batches of what-if scenes in one world vs the same scenes one world each
     n_max = max(batches)
     qs_all = _lattice_mix(n_max)
     def setup(world, slots, qq):
@@ -105,6 +197,18 @@ def bench_lattice(frames=120, batches=(32, 256), plain_scenes=8, plain_frames=30
             for ch in q["charges"]:
                 world.explode(sl, *ch)
         lb.close()
+    # plain: each scene in its own world, one untimed frame, then the timed frames
+    plain_t = []
+    for q in qs_all:
+        P = Lattice(q["bricks"] + 64, 4096)
+        ps = P.add_scene(q["bricks"] + 64, 4096, structs=2, seed=q["rng_seed"])
+        setup(P, [ps], [q])
+        P.step(1.0 / 60.0)
+        t0 = time.perf_counter()
+        for f in range(frames):
+            P.step(1.0 / 60.0)
+        plain_t.append(time.perf_counter() - t0)
+        P.close()
     rows = []
     for n_scenes in batches:
         qs = qs_all[:n_scenes]
@@ -113,37 +217,20 @@ def bench_lattice(frames=120, batches=(32, 256), plain_scenes=8, plain_frames=30
         slots = [B.add_scene(caps[k], 4096, structs=2, seed=qs[k]["rng_seed"]) for k in range(n_scenes)]
         setup(B, slots, qs)
         B.step(1.0 / 60.0)
-        t0 = time.perf_counter()
-        for f in range(frames):
-            B.step(1.0 / 60.0)
-        el = time.perf_counter() - t0
+        el, split = _frame_split(B, frames)
         nbr = sum(q["bricks"] for q in qs)
+        plain = nbr * frames / sum(plain_t[:n_scenes])
         rows.append({"scenes": n_scenes, "bricks": nbr, "frames": frames, "ms_per_frame": el * 1e3 / frames,
-                     "brick_steps_per_s": nbr * frames / el})
+                     "brick_steps_per_s": nbr * frames / el, "plain_same_scenes_brick_steps_per_s": plain,
+                     "batched_over_plain": nbr * frames / el / plain, "split_ms": split})
         B.close()
         log(f"lattice batched {n_scenes} scenes ({nbr} bricks): {el * 1e3 / frames:.2f} ms/frame, "
-            f"{nbr * frames / el:.3e} brick-steps/s")
-    tot_steps, tot_t = 0, 0.0
-    sample = qs_all[::max(1, n_max // plain_scenes)][:plain_scenes]
-    for q in sample:
-        P = Lattice(q["bricks"] + 64, 4096)
-        ps = P.add_scene(q["bricks"] + 64, 4096, structs=2, seed=q["rng_seed"])
-        setup(P, [ps], [q])
-        P.step(1.0 / 60.0)
-        t0 = time.perf_counter()
-        for f in range(plain_frames):
-            P.step(1.0 / 60.0)
-        tot_t += time.perf_counter() - t0
-        tot_steps += q["bricks"] * plain_frames
-        P.close()
-    plain = tot_steps / tot_t
-    for r in rows:
-        r["batched_over_plain"] = r["brick_steps_per_s"] / plain
+            f"{nbr * frames / el:.3e} brick-steps/s; the same scenes one world each {plain:.3e}")
     best = max(rows, key=lambda r: r["brick_steps_per_s"])
-    out["batched"] = dict(best, plain_same_scenes_brick_steps_per_s=plain, plain_scenes=len(sample),
-                          plain_frames=plain_frames, by_batch=rows)
-    log(f"lattice plain (one world per scene, {len(sample)} scenes): {plain:.3e} brick-steps/s; "
-        f"batched/plain {best['batched_over_plain']:.1f} at {best['scenes']} scenes")
+    out["batched"] = dict(best, by_batch=rows,
+                          plain_note="every scene of the batch stepped alone in its own world over the same frames "
+                                     "(frame 1 untimed, frames 2 to 1 + frames timed)")
+    log(f"lattice batched/plain {best['batched_over_plain']:.1f} at {best['scenes']} scenes")
     return out
 
 
@@ -251,13 +338,15 @@ def bench_box(args, sync):
     bodies, shape_body, shapes = scenes.rubble(args.bodies, seed=1)
     nbricks = len(shapes)
     log(f"scene: {len(bodies)} bodies, {nbricks} bricks ({time.perf_counter() - t:.1f} s)")
-    prm = params(substeps=args.substeps, sleep=True)
+    prm = params(substeps=args.substeps, sleep=True, **({} if args.phys else world.GPU_WORLD))
     if args.colours is not None:
         prm[world.P["COL_LIMIT"]] = args.colours
     if args.tail_min is not None:
         prm[world.P["TAIL_MIN"]] = args.tail_min
     tuned = None
-    if not args.no_autotune:
+    if args.solve_nt is not None:
+        prm[world.P["SOLVE_NT"]] = args.solve_nt
+    if not args.no_autotune and args.solve_nt is None:
         # launch shapes (solve workgroup size, small-colour threshold, graph replay) timed on this scene; none of
         # them changes the results
         from shatterforge_amd import autotune
@@ -298,6 +387,7 @@ def bench_box(args, sync):
     log(f"stages (ms): {stages}")
 
     value = nbricks * args.steps / el
+    traffic, trec = solve_traffic(int(prm[world.P["SOLVE_NT"]]))
     out = {
         "metric": "brick-steps/s (batched vs plain GPU step) + p50/p99 what-if query latency",
         "value": value,
@@ -314,9 +404,14 @@ def bench_box(args, sync):
         "config": {"workload": "rubble-pile stress, plain step on one scene", "bodies": len(bodies), "bricks": nbricks,
                    "substeps": args.substeps, "dt": "1/60", "mode": "plain",
                    "colour_limit": int(prm[world.P["COL_LIMIT"]]), "tail_min": int(prm[world.P["TAIL_MIN"]]),
-                   "solve_workgroup": int(prm[world.P["SOLVE_NT"]])},
+                   "solve_workgroup": int(prm[world.P["SOLVE_NT"]]),
+                   "solver": "reference CPU model (phys.rs)" if args.phys else
+                             "GPU world (mass-ratio limit 20, 6 level passes, 2 supports-first rounds)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": SOLVE_TRAFFIC_BYTES,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": None if trec is None else
+                                       f"profiles/k_solve_bucket_rubble50k_traffic.md (commit {trec.get('commit')}, "
+                                       f"workgroup {trec.get('solve_nt')})",
                      "kernel": "k_solve_bucket", "launches": klaunch, "avg_us": ksum_ms / max(klaunch, 1) * 1e3,
                      "bytes_per_unit": SOLVE_BYTES_PER_MANIFOLD, "units": kitems},
         "stages_ms": stages,
@@ -328,23 +423,13 @@ def bench_box(args, sync):
 
 
 def bench_copy(gib=1.0, reps=5):
-    """a device-to-device copy of known size (torch's copy kernel on two 1 GiB buffers): the reference reading for
-    the HBM byte counters (FETCH_SIZE and WRITE_SIZE per byte moved)"""
-    import torch
-    nbytes = int(gib * (1 << 30))
-    a = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    res = {"kernel": "torch copy_ (device to device)", "bytes_per_copy": nbytes, "copies": reps,
-           "GBs_read_plus_write": 2.0 * nbytes * reps / (ms * 1e-3) / 1e9}
-    del a, b
+    """a device-to-device copy of known size (the bench-only program shatterforge_amd/sfb_copy, built apart from the
+    library: 16-byte loads and stores between two 1 GiB buffers), run as a child process: the reference reading for the
+    HBM byte counters (FETCH_SIZE and WRITE_SIZE per byte moved), which is profiled in a run of its own"""
+    import subprocess
+    exe = os.path.join(ROOT, "shatterforge_amd", "sfb_copy")
+    r = subprocess.run([exe, str(gib), str(reps)], capture_output=True, text=True, check=True)
+    res = json.loads(r.stdout.strip().splitlines()[-1])
     log(f"copy: {res['GBs_read_plus_write']:.0f} GB/s (read + write)")
     return res
 
@@ -360,11 +445,14 @@ def main():
     ap.add_argument("--tail-min", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes for the box step")
+    ap.add_argument("--solve-nt", type=int, default=None, help="box solve workgroup size (skips the autotuner)")
+    ap.add_argument("--phys", action="store_true", help="the box step with the reference CPU model's solver (the GPU "
+                    "world's extensions off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
     ap.add_argument("--parts", default="box,lattice,whatif",
-                    help="comma list of box, lattice, whatif, load, site, copy (a device copy of known size, the "
-                         "reference for HBM byte counters)")
+                    help="comma list of box, lattice, whatif, load, site, copy (a device copy of known size by the "
+                         "bench-only program shatterforge_amd/sfb_copy)")
     ap.add_argument("--whatif-queries", type=int, default=64)
     ap.add_argument("--load-batch", type=int, default=0, help="also run the load-path batch of this many structures")
     ap.add_argument("--site", type=int, default=0, help="also run the site-scale collapse for this many frames")
@@ -397,7 +485,7 @@ def main():
     if "copy" in parts:
         out["copy"] = bench_copy()
     if "lattice" in parts:
-        out["lattice"] = bench_lattice()
+        out["lattice"] = bench_lattice(cpu=not args.no_cpu)
     if "whatif" in parts:
         out["whatif"] = bench_whatif(n_queries=args.whatif_queries)
     if "load" in parts and not args.load_batch:
@@ -410,7 +498,8 @@ def main():
         out["site"] = bench_site(args.site)
     if not args.no_cpu and scene is not None:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(*scene, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(*scene, args.cpu_seconds, sync=sync)
+        out["cpu_baseline"]["pile_anchor"] = pile_anchor()
     print(json.dumps(out), flush=True)
 
 

@@ -1101,7 +1101,14 @@ void ob_step(ob_world* w, float dt) {
         }
         solve_pass(w, &kr, order, ncol_order, ovf, novf, ocnt, dv);
     }
-    if (w->any_rest) rest_pass(w, &kr, order, ncol_order, ovf, novf, ocnt, dv);
+    /* (the pass is skipped when no manifold may bounce: a restitution and a point approaching past the threshold) */
+    int bounce = 0;
+    if (w->any_rest)
+        for (int mi = 0; mi < w->nman && !bounce; mi++) {
+            const Man* m = &w->man[mi];
+            if (m->rest > 0.0f) for (int k = 0; k < m->cnt; k++) if (!(m->cp[k].rv > -kr.rest_thr)) bounce = 1;
+        }
+    if (bounce) rest_pass(w, &kr, order, ncol_order, ovf, novf, ocnt, dv);
     free(order); free(ovf); free(ocnt); free(dv);
     /* ---- write back */
     for (int i = 0; i < n; i++) {

@@ -34,8 +34,9 @@ constexpr int NT = 256;  // threads per block of the grid-stride kernels
 
 // device counters
 // (C_ACTIVE0 + g: set when a manifold claimed in the rounds the host checks as group g)
+// (C_NREST: manifolds of the step that may bounce, a restitution and a point approaching past the threshold)
 enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_NSLOT, C_NBIGC, C_NBIGL,
-       C_ACTIVE0, C_NACTIVE = 16, C_NCTR = C_ACTIVE0 + C_NACTIVE };
+       C_NREST, C_ACTIVE0, C_NACTIVE = 16, C_NCTR = C_ACTIVE0 + C_NACTIVE };
 
 // manifold rows (each row is a float4 array of mcap entries)
 // (the inertia terms, a body's world inverse inertia times an arm or the normal, are not stored: the solve forms them
@@ -786,6 +787,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
     __shared__ uint32_t wsum[NT / 64];
     uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
     uint32_t npts = 0;  // this thread's contact points (the block's sum goes to ptpart, not to one global counter)
+    uint32_t nbounce = 0;  // this thread's manifolds that may bounce
     GRID_STRIDE(i, nc) {
         if (!D.hit[i]) continue;
         uint32_t m = D.hoff[i];
@@ -863,6 +865,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         float jnk[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         V3 iaxk[4], ibxk[4];
         int ocnt = 0;
+        bool bounce = false;
         V3 ola[4];  // last step's point anchors, loaded once for all the new points
         if (oi >= 0) {
             ocnt = min(max((int)MROW(D, D.old, MR_CB1, opos).w, 0), 4);
@@ -893,6 +896,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             // the point's normal velocity at the step start (negative: approaching), gpu_world.js:662
             const V3 pva = vadd(v3f4(sva), vcross(swa, rap)), pvb = vadd(v3f4(svb), vcross(swb, rbp));
             const float rv = vdot(vsub(pvb, pva), n);
+            if (rest > 0.0f && !(rv > -K.rest_thr)) bounce = true;
             MROW(D, C, MR_P0 + MR_PER_PT * k + 0, p) = f4(raxn, pts[k].s);
             MROW(D, C, MR_P0 + MR_PER_PT * k + 1, p) = f4(rbxn, nm);
             iaxk[k] = iaxn; ibxk[k] = ibxn; jnk[k] = jn;
@@ -920,6 +924,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         MROW(D, C, MR_LAF, p) = f4(la_f, rest);
         MROW(D, C, MR_LBF, p) = f4(lb_f, 0.0f);  // (.w: not pushed yet)
         npts += (uint32_t)cnt;
+        nbounce += bounce ? 1u : 0u;
         // the bodies' list entries of this position, their first warm-start records (from the impulses carried over
         // from the last step) and their overflow changes cleared
         const uint32_t ea = sa ? D.minv[2 * m] : ~0u, eb = sb ? D.minv[2 * m + 1] : ~0u;
@@ -932,6 +937,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         if (eb != ~0u) { OVL(D, eb)[0] = z; OVL(D, eb)[1] = z; }
     }
     for (int h = 32; h >= 1; h >>= 1) npts += __shfl_xor(npts, h, 64);
+    if (__ballot(nbounce != 0) && (threadIdx.x & 63) == 0) atomicAdd(&D.ctr[C_NREST], 1u);  // (a flag: any is enough)
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = npts;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1614,7 +1620,10 @@ __device__ __forceinline__ void ovf_average(const Dev& D, uint32_t o, uint32_t n
     v = vadd(v, vmul(av, inv));
     w = vadd(w, vmul(aw, inv));
 }
-__global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K) {
+// (rest_gate: after the restitution pass, which left the changes of the last relax pass in place when no manifold
+// of the step may bounce: then nothing to apply)
+__global__ void __launch_bounds__(NT) k_ovf_apply(Dev D, StepK K, int rest_gate) {
+    if (rest_gate && !D.ctr[C_NREST]) return;
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
     for (uint32_t i = wave; i < (uint32_t)D.nb; i += nwaves) {
@@ -1676,21 +1685,30 @@ __global__ void __launch_bounds__(NT) k_ovf_apply_warm(Dev D, StepK K) {
     });
 }
 // ---------------------------------------------------------------- shock propagation and restitution
-// One level pass (gpu_world.js:1178-1203) over the manifolds (by position): heights are read from rd and raised in wr
-// with atomic maxima (both bodies' read heights carried over; the upper body of a contact within 45 degrees of
-// vertical lifted to one below its support's), so the result does not depend on the order threads run in.
-__global__ void __launch_bounds__(NT) k_level_pass(Dev D, const uint32_t* rd, uint32_t* wr) {
+// One level pass (gpu_world.js:1178-1203): heights are read from rd and raised in wr, in two launches. Each pass's
+// output is at least its input at every body a manifold touches, so carrying the read heights over (the maximum of
+// the two copies) is a copy, k_level_copy; then, over the manifolds, the upper body of a contact within 45 degrees of
+// vertical is lifted to one below its support's with atomic maxima (k_level_lift; a lift a plain load shows to be no
+// higher is skipped: heights only rise), so the result does not depend on the order threads run in.
+__global__ void __launch_bounds__(NT) k_level_copy(Dev D, const uint32_t* rd, uint32_t* wr) {
+    GRID_STRIDE(s, (uint32_t)D.nb + 1u) wr[s] = rd[s];
+}
+__global__ void __launch_bounds__(NT) k_level_lift(Dev D, const uint32_t* rd, uint32_t* wr) {
     const uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(p, nm) {
+        const float ny = MROW(D, D.cur, MR_N, p).y;
+        if (!(ny > 0.7f) && !(ny < -0.7f)) continue;
         const int4 hd = D.cur.hdrp[p];
         const int sa = hd.z, sb = hd.w;
         const bool da = sa && SV(D, sa).w > 0.0f, db = sb && SV(D, sb).w > 0.0f;
-        const uint32_t ha = da ? rd[sa] : 255u, hb = db ? rd[sb] : 255u;
-        const float ny = MROW(D, D.cur, MR_N, p).y;
-        if (da) atomicMax(&wr[sa], ha);
-        if (db) atomicMax(&wr[sb], hb);
-        if (ny > 0.7f && ha > 1u && db) atomicMax(&wr[sb], ha - 1u);
-        if (ny < -0.7f && hb > 1u && da) atomicMax(&wr[sa], hb - 1u);
+        if (ny > 0.7f && db) {
+            const uint32_t ha = da ? rd[sa] : 255u;
+            if (ha > 1u && ha - 1u > wr[sb]) atomicMax(&wr[sb], ha - 1u);
+        }
+        if (ny < -0.7f && da) {
+            const uint32_t hb = db ? rd[sb] : 255u;
+            if (hb > 1u && hb - 1u > wr[sa]) atomicMax(&wr[sa], hb - 1u);
+        }
     }
 }
 
@@ -1773,6 +1791,7 @@ __device__ void rest_one(const Dev& D, uint32_t m, const StepK& K) {
 }
 // one colour bucket of the restitution pass, spread over the XCDs as k_solve_bucket's
 __global__ void __launch_bounds__(NT) k_rest_bucket(Dev D, StepK K, int c) {
+    if (!D.ctr[C_NREST]) return;  // (no manifold of the step may bounce)
     const uint32_t n = D.colcnt[c], o = D.coloff[c];
     const uint32_t G = min(gridDim.x, max((n + NT - 1) / NT, 1u)), b = blockIdx.x;
     if (b >= G) return;
@@ -1782,6 +1801,7 @@ __global__ void __launch_bounds__(NT) k_rest_bucket(Dev D, StepK K, int c) {
 }
 // the small colours from c0 on, one after another in one workgroup
 __global__ void __launch_bounds__(NT) k_rest_tail(Dev D, StepK K, int c0) {
+    if (!D.ctr[C_NREST]) return;
     int ncol = (int)D.ctr[C_NCOL];
     for (int c = c0; c < ncol; c++) {
         uint32_t n = D.colcnt[c], o = D.coloff[c];
@@ -1792,6 +1812,7 @@ __global__ void __launch_bounds__(NT) k_rest_tail(Dev D, StepK K, int c0) {
 // the overflow's restitution: Jacobi with mass splitting as its solve (every overflow manifold writes its changes,
 // zero when it takes no part, and k_ovf_apply averages them)
 __global__ void __launch_bounds__(NT) k_rest_ovf(Dev D, StepK K) {
+    if (!D.ctr[C_NREST]) return;  // (then k_ovf_apply, gated the same way, applies nothing)
     const uint32_t n = D.colcnt[K.K], o = D.coloff[K.K];
     const uint32_t G = min(gridDim.x, max((n + NT - 1) / NT, 1u)), b = blockIdx.x;
     if (b >= G) return;
@@ -2235,8 +2256,12 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_build, dim3(gc), dim3(NT), 0, s, D, K, w->nold);
     hipLaunchKernelGGL(k_sum_points, dim3(1), dim3(NT), 0, s, D, gc);
     // shock-propagation heights: an even number of passes alternating between the two copies, ending in lvl
-    for (int k = 0; k < K.shock; k++)
-        hipLaunchKernelGGL(k_level_pass, dim3(gm), dim3(NT), 0, s, D, (k & 1) ? D.lvl2 : D.lvl, (k & 1) ? D.lvl : D.lvl2);
+    for (int k = 0; k < K.shock; k++) {
+        const uint32_t* rd = (k & 1) ? D.lvl2 : D.lvl;
+        uint32_t* wr = (k & 1) ? D.lvl : D.lvl2;
+        hipLaunchKernelGGL(k_level_copy, dim3(sf_grid((uint64_t)D.nb + 1)), dim3(NT), 0, s, D, rd, wr);
+        hipLaunchKernelGGL(k_level_lift, dim3(gm), dim3(NT), 0, s, D, rd, wr);
+    }
     if (w->timing) SF_HIP(hipEventRecord(w->ev[3], s));
     SF_HIP(hipMemcpyAsync(w->h_col, D.colcnt, sizeof(uint32_t) * 72, hipMemcpyDeviceToHost, s));
     SF_HIP(hipMemcpyAsync(w->h_ctr, D.ctr, sizeof(uint32_t) * C_NCTR, hipMemcpyDeviceToHost, s));
@@ -2285,7 +2310,7 @@ int substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, in
         if (ovf) {
             hipLaunchKernelGGL(k_ovf_solve, dim3(grid[big]), dim3(NT), 0, s, D, Kr);
             if (it + 1 < sub) hipLaunchKernelGGL(k_ovf_apply_warm, dim3(gw), dim3(NT), 0, s, D, K);
-            else hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
+            else hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr, 0);
         }
     }
     // restitution after the sub-steps: the colours in order, then the overflow (Jacobi with mass splitting)
@@ -2294,7 +2319,7 @@ int substep_chain(HostWorld* w, const StepK& K, int big, bool tail, bool ovf, in
         if (tail) hipLaunchKernelGGL(k_rest_tail, dim3(1), dim3(NT), 0, s, D, Kr, big);
         if (ovf) {
             hipLaunchKernelGGL(k_rest_ovf, dim3(grid[big]), dim3(NT), 0, s, D, Kr);
-            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr);
+            hipLaunchKernelGGL(k_ovf_apply, dim3(gw), dim3(NT), 0, s, D, Kr, 1);
         }
     }
     return 0;
