@@ -29,16 +29,16 @@ SOLVE_BYTES_PER_MANIFOLD = 392
 
 
 def solve_traffic(solve_nt):
-    """the counter-measured HBM bytes per k_solve_bucket dispatch, if the profile was taken at this launch shape"""
+    """the counter-measured HBM bytes per coloured manifold solve of k_solve_bucket, if the profile was taken at this
+    launch shape (the profile's json block: bytes per manifold solve by workgroup size)"""
     try:
         text = open(TRAFFIC_PROFILE).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
         rec = json.loads(block)
     except (OSError, IndexError, ValueError):
         return None, None
-    if int(rec.get("solve_nt", -1)) != int(solve_nt):
-        return None, rec
-    return int(rec["bytes_per_dispatch"]), rec
+    per = rec.get("bytes_per_manifold_solve", {}).get(str(int(solve_nt)))
+    return (float(per) if per is not None else None), rec
 
 
 def log(*a):
@@ -360,13 +360,21 @@ def bench_box(args, sync):
     w = sf.World.from_scene(bodies, shape_body, shapes, prm)
     if tuned is not None:
         w.set_graph(bool(tuned["best"]["graph"]))
+    # coloured manifold solves of every step of this world (both passes of every sub-step): with the counter
+    # profile's per-dispatch bytes they give the bytes per manifold solve
+    coloured = [0]
+    def counted():
+        st_ = w.stats()
+        coloured[0] += (st_["manifolds"] - st_["overflow"]) * 2 * args.substeps
     for k in range(args.warmup):
         w.step(1.0 / 60.0)
+        counted()
     log(f"warmup {args.warmup} steps done: {w.stats()}")
     sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
         w.step(1.0 / 60.0)
+        counted()  # (a host read of the step's counters: the step has already synchronised)
     sync()
     el = time.perf_counter() - t0
     st = w.stats()
@@ -376,18 +384,22 @@ def bench_box(args, sync):
     w.set_kernel_timing(True)
     for k in range(args.steps):
         w.step(1.0 / 60.0)
+        counted()
     ksum_ms, klaunch, kitems = w.kernel_stats()
     w.set_kernel_timing(False)
     achieved = SOLVE_BYTES_PER_MANIFOLD * kitems / (ksum_ms * 1e-3) / 1e9 if ksum_ms > 0 else 0.0
     log(f"solve-bucket kernel: {klaunch} launches, {kitems} manifold solves, avg {ksum_ms / max(klaunch, 1) * 1e3:.1f} us")
     w.set_timing(True)
     w.step(1.0 / 60.0)
+    counted()
     stages = w.stage_ms()
     w.set_timing(False)
     log(f"stages (ms): {stages}")
+    log(f"all {args.warmup + 2 * args.steps + 1} steps of this world: {coloured[0]} coloured manifold solves")
 
     value = nbricks * args.steps / el
-    traffic, trec = solve_traffic(int(prm[world.P["SOLVE_NT"]]))
+    per_solve, trec = solve_traffic(int(prm[world.P["SOLVE_NT"]]))
+    traffic = per_solve * kitems / max(klaunch, 1) if per_solve is not None else None
     out = {
         "metric": "brick-steps/s (batched vs plain GPU step) + p50/p99 what-if query latency",
         "value": value,
@@ -409,9 +421,9 @@ def bench_box(args, sync):
                              "GPU world (mass-ratio limit 20, 6 level passes, 2 supports-first rounds)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": None if trec is None else
-                                       f"profiles/k_solve_bucket_rubble50k_traffic.md (commit {trec.get('commit')}, "
-                                       f"workgroup {trec.get('solve_nt')})",
+                     "traffic_source": None if per_solve is None else
+                                       f"profiles/k_solve_bucket_rubble50k_traffic.md (commit {trec.get('commit')}): "
+                                       f"{per_solve:.0f} HBM bytes per manifold solve x this run's solves per launch",
                      "kernel": "k_solve_bucket", "launches": klaunch, "avg_us": ksum_ms / max(klaunch, 1) * 1e3,
                      "bytes_per_unit": SOLVE_BYTES_PER_MANIFOLD, "units": kitems},
         "stages_ms": stages,
@@ -450,7 +462,7 @@ def main():
                     "world's extensions off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
-    ap.add_argument("--parts", default="box,lattice,whatif",
+    ap.add_argument("--parts", default="box,lattice,whatif,load,site",
                     help="comma list of box, lattice, whatif, load, site, copy (a device copy of known size by the "
                          "bench-only program shatterforge_amd/sfb_copy)")
     ap.add_argument("--whatif-queries", type=int, default=64)
