@@ -470,7 +470,8 @@ def main():
     ap.add_argument("--site", type=int, default=0, help="also run the site-scale collapse for this many frames")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard by scene, see DESIGN.md)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the scenes would shard whole across "
+              f"GPUs; nothing here runs on more than one)",
               file=sys.stderr)
         sys.exit(2)
 
