@@ -328,6 +328,42 @@ def bench_site(frames=1200, seed=1):
     return out
 
 
+def bench_particles(n=100_000, bodies=5000, steps=90, seed=2):
+    """particle debris: n crumbs (r 0.1 m) raining onto a rubble pile of `bodies` compound bodies (GPU world solver),
+    `steps` steps; the particle stage's device time per step (HIP events on the world's stream) and the moving
+    particle-sub-steps it performed"""
+    import shatterforge_amd as sf
+    from shatterforge_amd import scenes, world
+    from shatterforge_amd.world import params
+    b, sb, s = scenes.rubble(bodies, box=(20.0, 10.0, 20.0), seed=seed)
+    w = sf.World.from_scene(b, sb, s, params(**world.GPU_WORLD))
+    w.particles_capacity(n)
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((n, 8), np.float32)
+    rec[:, 0] = rng.uniform(-10.0, 10.0, n)
+    rec[:, 1] = rng.uniform(12.0, 20.0, n)
+    rec[:, 2] = rng.uniform(-10.0, 10.0, n)
+    rec[:, 6] = 0.1
+    rec[:, 7] = rng.choice([0.5, 0.6], n)
+    w.add_particles(rec)
+    w.set_timing(True)
+    pms, moving, t0 = 0.0, 0, time.perf_counter()
+    for _ in range(steps):
+        w.step(1.0 / 60.0)
+        pms += w.particle_ms()
+        moving += int(w.particle_stats()[1])
+    el = time.perf_counter() - t0
+    st = w.particle_stats()
+    w.close()
+    psub = int(params()[world.P["PSUB"]])
+    res = {"particles": n, "bodies": bodies, "steps": steps, "ms_per_step": el * 1e3 / steps,
+           "particle_stage_ms_per_step": pms / steps, "moving_per_step": moving / steps, "asleep_at_end": int(st[2]),
+           "moving_particle_substeps_per_s": moving * psub / (pms * 1e-3) if pms > 0 else 0.0}
+    log(f"particles: {n} on {bodies} bodies, stage {pms / steps:.3f} ms/step, {moving / steps:.0f} moving, "
+        f"whole step {el * 1e3 / steps:.2f} ms")
+    return res
+
+
 def bench_box(args, sync):
     """the plain box step on the rubble pile: the JSON line's value, roofline line and stages"""
     import shatterforge_amd as sf
@@ -462,9 +498,9 @@ def main():
                     "world's extensions off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
-    ap.add_argument("--parts", default="box,lattice,whatif,load,site",
-                    help="comma list of box, lattice, whatif, load, site, copy (a device copy of known size by the "
-                         "bench-only program shatterforge_amd/sfb_copy)")
+    ap.add_argument("--parts", default="box,lattice,whatif,load,site,particles",
+                    help="comma list of box, lattice, whatif, load, site, particles, copy (a device copy of known size by "
+                         "the bench-only program shatterforge_amd/sfb_copy)")
     ap.add_argument("--whatif-queries", type=int, default=64)
     ap.add_argument("--load-batch", type=int, default=0, help="also run the load-path batch of this many structures")
     ap.add_argument("--site", type=int, default=0, help="also run the site-scale collapse for this many frames")
@@ -509,6 +545,8 @@ def main():
         out["load_batch"] = bench_load_batch(args.load_batch)
     if args.site:
         out["site"] = bench_site(args.site)
+    if "particles" in parts:
+        out["particles"] = bench_particles()
     if not args.no_cpu and scene is not None:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(*scene, args.cpu_seconds, sync=sync)

@@ -50,6 +50,8 @@ enum {
     SF_P_REST_THRESHOLD,            /* restitution acts on points that approached faster than this (1 m/s) */
     SF_P_REST_ITERS,                /* restitution iterations per manifold (8) */
     SF_P_COL_SUPPORT,               /* colouring rounds for supports alone before the rounds for all; supports outrank the rest (0 = off) */
+    SF_P_PSUB,                      /* particle debris: sub-steps per step (4) */
+    SF_P_PCELL,                     /* particle debris: grid cell, twice the largest radius + 0.1 m (0.7) */
     SF_NPARAM
 };
 /* body record, SF_BODY_F floats: p[3] q[4] v[3] w[3] inv_m inv_i[3] (principal, local) awake sleep_t reserved */
@@ -91,6 +93,22 @@ int sf_world_colours(sf_world* w, int32_t* out, int cap);
 int sf_world_islands(sf_world* w, int32_t* out);
 /* 10 floats per body (position, rotation, half extents of the first shape), like engine/src/wasm.rs pose (:47-53) */
 const float* sf_world_pose(sf_world* w);
+/* Particle debris (GpuWorld addParticles / killParticle / blastParticles / wakeAll / readParticles,
+ * gpu/gpu_world.js:2863-2896, stages :1671-2038): position-based crumbs after each step, pushed out of each other, the
+ * ground and the world's shapes (one way: they never push a body), asleep after 0.5 s at rest. Capacity first (slots
+ * in use are kept); records of 8 floats: p[3] v[3] radius friction, the slot of each into `slots` (-1: full or not
+ * finite), the count added returned; a killed slot is reused; blasts apply at the next step, three a step; reading
+ * gives 12 floats per slot in use: x y z vx, vy vz qx qy, qz qw state radius (state 0 free, 1 moving, 2 asleep);
+ * stats: slots in use, moving at the last step's start, asleep at its end, free slots. */
+int sf_world_particles_capacity(sf_world* w, int cap);
+int sf_world_add_particles(sf_world* w, int n, const float* rec8, int32_t* slots);
+int sf_world_kill_particle(sf_world* w, int i);
+int sf_world_blast_particles(sf_world* w, float cx, float cy, float cz, float reach, float speed);
+int sf_world_wake_particles(sf_world* w);
+int sf_world_read_particles(sf_world* w, float* out12);
+int sf_world_particle_stats(sf_world* w, int32_t* out4);
+/* device ms of the last step's particle stage (with sf_world_set_timing on; 0 otherwise) */
+int sf_world_particle_ms(sf_world* w, float* ms);
 /* device time of the last step's stages in ms: broad, narrow, colour, solve, sleep (when timing is on) */
 int sf_world_set_timing(sf_world* w, int on);
 int sf_world_stage_ms(sf_world* w, float* out5);

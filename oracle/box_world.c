@@ -31,6 +31,16 @@ typedef struct {
 } Body;
 typedef struct { int body; V3 lp; Quat lq; M3 lrot; V3 h; float mu, e; V3 wp; M3 wrot; V3 lo, hi; } Shape;
 
+/* a debris particle (gpu_world.js:1676-1684 record): position, radius; velocity, state (0 free, 1 moving, 2 asleep);
+ * drawing rotation; sub-step start position, rest timer; spin, friction; corrected position, touch (1 touching, 2
+ * bearing a moving body); where the rest timer last restarted; ground flags; candidate lists */
+#define OP_NN 24
+#define OP_NS 12
+typedef struct {
+    V3 x; float r; V3 v; float st; Quat q; V3 xs; float t; V3 w; float mu; V3 xc; float touch; V3 anchor;
+    uint32_t g_last, g_now; int nn, ns; int32_t nb[OP_NN], sh[OP_NS]; uint32_t cell;
+} Particle;
+
 struct ob_world {
     int order;
     float prm[OB_NPARAM];
@@ -48,13 +58,17 @@ struct ob_world {
     int32_t *bl_off, *bl_ent, *ovf_index;  /* device order: per-slot contact lists, overflow position per manifold */
     uint32_t *lvl, *lvl2;  /* shock-propagation heights per slot (the two copies the passes alternate between) */
     int any_rest;          /* some shape has a restitution above zero */
+    /* particle debris */
+    Particle* pt; int np, capp; int32_t* pfree; int nfree;
+    float blast[16][5]; int nblast; int pwake;
+    int32_t pstats[4];
 };
 
 /* parameter indices (same as SF_P_* in include/sf.h) */
 enum { P_GX, P_GY, P_GZ, P_SUBSTEPS, P_HERTZ, P_DAMPING, P_MAXPUSH, P_FRICTION, P_SPEC, P_SLEEP_LIN, P_SLEEP_ANG,
        P_SLEEP_TIME, P_SLEEP_DIST, P_SLEEP_ROT, P_ALLOW_SLEEP, P_CELL, P_LEAK, P_ANCHOR, P_ANCHOR_PUSH, P_ANCHOR_SLOP,
        P_COL_LIMIT, P_COL_ROUNDS, P_COL_SPREAD, P_TAIL_MIN, P_SOLVE_NT, P_RATIO, P_SHOCK, P_REST_THR, P_REST_ITERS,
-       P_COL_SUPPORT };
+       P_COL_SUPPORT, P_PSUB, P_PCELL };
 
 static void default_params(float* p) {
     /* phys.rs:101-105 */
@@ -66,6 +80,8 @@ static void default_params(float* p) {
     /* the GPU world's extensions (gpu_world.js), off: ratio 20, 6 level passes and 2 support rounds turn them on;
      * restitution acts where shapes carry one (threshold 1 m/s, 8 iterations, docs/GPU.md:41) */
     p[P_RATIO] = 0.0f; p[P_SHOCK] = 0.0f; p[P_REST_THR] = 1.0f; p[P_REST_ITERS] = 8.0f; p[P_COL_SUPPORT] = 0.0f;
+    /* particle debris (gpu_world.js:2411-2412): 4 sub-steps, grid cell 2 x 0.3 m + 0.1 m */
+    p[P_PSUB] = 4.0f; p[P_PCELL] = 0.7f;
 }
 
 ob_world* ob_create(int order) {
@@ -79,7 +95,7 @@ void ob_destroy(ob_world* w) {
     free(w->bd); free(w->sh); free(w->sv); free(w->sw); free(w->sdp); free(w->sth); free(w->sp0); free(w->sim);
     free(w->sii); free(w->sq); free(w->sidx); free(w->man); free(w->old); free(w->old_sorted); free(w->cand);
     free(w->colour); free(w->island); free(w->wake_flag); free(w->bl_off); free(w->bl_ent); free(w->ovf_index);
-    free(w->lvl); free(w->lvl2); free(w);
+    free(w->lvl); free(w->lvl2); free(w->pt); free(w->pfree); free(w);
 }
 void ob_set_params(ob_world* w, const float* prm, int n) { for (int i = 0; i < n && i < OB_NPARAM; i++) w->prm[i] = prm[i]; }
 int ob_nbody(const ob_world* w) { return w->nb; }
@@ -947,6 +963,8 @@ static int cmp_old(const void* x, const void* y) {
     return ka < kb ? -1 : (ka > kb ? 1 : 0);
 }
 
+static void particle_stage(ob_world* w, float dt);
+
 /* phys.rs:152-243 */
 void ob_step(ob_world* w, float dt) {
     const float* P = w->prm;
@@ -1118,6 +1136,7 @@ void ob_step(ob_world* w, float dt) {
     }
     sleep_islands(w, dt);
     w->stats[0] = nsolver;
+    particle_stage(w, dt);
 }
 
 int ob_candidates(const ob_world* w, int32_t* out, int cap) {
@@ -1151,4 +1170,279 @@ int ob_colours(const ob_world* w, int32_t* out, int cap) {
 void ob_islands(const ob_world* w, int32_t* out) { for (int i = 0; i < w->nb; i++) out[i] = w->island[i]; }
 void ob_levels(const ob_world* w, int32_t* out) {
     for (int i = 0; i < w->nb; i++) { int s = w->sidx[i]; out[i] = (int32_t)slot_height(w, s); }
+}
+
+/* ---------------------------------------------------------------- particle debris (gpu_world.js:1671-2038) */
+/* Position-based debris: each sub-step predicts, pushes each moving particle out of the other particles (Jacobi:
+ * the pushes averaged and over-relaxed, against the predicted positions of all), then out of the ground and every
+ * candidate rigid shape in turn, each in full, with Coulomb friction from the push; the velocity is the move over the
+ * sub-step. Particles never push bodies. A particle at rest for 0.5 s sleeps; a blast, a wake request or a moving shape
+ * near it wakes it. Candidates are gathered once per step: other particles from a hashed grid (each slot's particles
+ * in index order, the 27 cells round the particle's in z-y-x order, the first 24 in reach), shapes from the box
+ * step's cells (the 12 lowest indices whose step-start box meets the particle's reach). Shapes are seen at their
+ * step-start pose, their bodies' velocities at the end of the step. */
+static float op_pcell(const ob_world* w) { return w->prm[P_PCELL] > 0.1f ? w->prm[P_PCELL] : 0.7f; }
+static uint32_t op_grid_size(int np) { uint32_t g = 256; while (g < 2u * (uint32_t)np) g <<= 1; return g; }
+static uint32_t op_hash(int x, int y, int z, uint32_t mask) {
+    return (((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u)) & mask;
+}
+static int op_cell(float v, float pc) { return (int32_t)om_floor(v / pc); }
+
+int ob_particles_capacity(ob_world* w, int cap) {
+    if (cap < w->np) return -1;
+    w->pt = (Particle*)realloc(w->pt, sizeof(Particle) * (size_t)(cap > 0 ? cap : 1));
+    w->pfree = (int32_t*)realloc(w->pfree, sizeof(int32_t) * (size_t)(cap > 0 ? cap : 1));
+    w->capp = cap;
+    return 0;
+}
+/* records of 8 floats: p[3] v[3] radius friction; the slot of each (-1: full, or not finite); returns the count added */
+int ob_add_particles(ob_world* w, int n, const float* rec, int32_t* slots) {
+    int added = 0;
+    float rmax = op_pcell(w) * 0.5f - 0.05f;
+    for (int k = 0; k < n; k++) {
+        const float* f = rec + (size_t)8 * k;
+        int fin = 1;
+        for (int a = 0; a < 8; a++) if (!isfinite(f[a])) fin = 0;
+        int i = -1;
+        if (fin) { if (w->nfree) i = w->pfree[--w->nfree]; else if (w->np < w->capp) i = w->np++; }
+        slots[k] = i;
+        if (i < 0) continue;
+        Particle* p = &w->pt[i];
+        memset(p, 0, sizeof(Particle));
+        p->x = v3(f[0], f[1], f[2]); p->v = v3(f[3], f[4], f[5]); p->r = om_min(f[6], rmax); p->mu = f[7];
+        p->st = 1.0f; p->q = quat(0, 0, 0, 1); p->xs = p->x; p->anchor = p->x; p->xc = p->x;
+        added++;
+    }
+    return added;
+}
+void ob_kill_particle(ob_world* w, int i) {
+    if (i < 0 || i >= w->np || w->pt[i].st == 0.0f) return;
+    w->pt[i].st = 0.0f;
+    w->pfree[w->nfree++] = i;
+}
+void ob_blast_particles(ob_world* w, float cx, float cy, float cz, float reach, float speed) {
+    if (w->nblast >= 16) return;
+    float* b = w->blast[w->nblast++];
+    b[0] = cx; b[1] = cy; b[2] = cz; b[3] = reach; b[4] = speed;
+}
+void ob_wake_particles(ob_world* w) { w->pwake = 1; }
+/* 12 floats per slot: x y z vx, vy vz qx qy, qz qw state radius (gpu_world.js:2891-2896) */
+int ob_read_particles(const ob_world* w, float* out) {
+    for (int i = 0; i < w->np; i++) {
+        const Particle* p = &w->pt[i]; float* o = out + (size_t)12 * i;
+        o[0] = p->x.x; o[1] = p->x.y; o[2] = p->x.z; o[3] = p->v.x; o[4] = p->v.y; o[5] = p->v.z;
+        o[6] = p->q.x; o[7] = p->q.y; o[8] = p->q.z; o[9] = p->q.w; o[10] = p->st; o[11] = p->r;
+    }
+    return w->np;
+}
+/* [slots in use (high-water), moving at the stage's start, asleep at its end, free slots] */
+void ob_particle_stats(const ob_world* w, int32_t* out4) { for (int k = 0; k < 4; k++) out4[k] = w->pstats[k]; }
+
+/* whether shape s is in the box step's cell grid: at most 8 cells along each axis of its step-start box, finite */
+static int op_in_grid(const ob_world* w, int s) {
+    const float ic = 1.0f / w->prm[P_CELL];
+    const Shape* sh = &w->sh[s];
+    float fl[3] = {sh->lo.x * ic, sh->lo.y * ic, sh->lo.z * ic}, fh[3] = {sh->hi.x * ic, sh->hi.y * ic, sh->hi.z * ic};
+    for (int a = 0; a < 3; a++) {
+        if (!(om_fabs(fl[a]) < 1.0e8f) || !(om_fabs(fh[a]) < 1.0e8f) || !(fh[a] - fl[a] < 8.0f)) return 0;
+        int n = (int32_t)om_floor(fh[a]) - (int32_t)om_floor(fl[a]) + 1;
+        if (n < 1 || n > 8) return 0;
+    }
+    return 1;
+}
+/* a moving shape (awake body with mass) whose step-start box meets [lo, hi] */
+static int op_near_moving(const ob_world* w, V3 lo, V3 hi) {
+    for (int s = 0; s < w->ns; s++) {
+        const Shape* sh = &w->sh[s]; const Body* b = &w->bd[sh->body];
+        if (!b->awake || b->inv_m == 0.0f) continue;
+        if (sh->lo.x > hi.x || lo.x > sh->hi.x || sh->lo.y > hi.y || lo.y > sh->hi.y || sh->lo.z > hi.z || lo.z > sh->hi.z) continue;
+        if (!op_in_grid(w, s)) continue;
+        return 1;
+    }
+    return 0;
+}
+
+/* friction: take back the sliding of this sub-step, up to mu times the push (gpu_world.js:1765-1769) */
+static V3 op_fric(V3 disp, V3 n, float pen, float mu) {
+    V3 t = vsub(disp, vmul(n, vdot(disp, n)));
+    float tl = vlen(t);
+    if (tl < 1e-9f) return v3(0, 0, 0);
+    return vmul(t, -om_min(mu * pen / tl, 1.0f));
+}
+static V3 op_cap(V3 v) { float l = vlen(v); return l > 40.0f ? vmul(v, 40.0f / l) : v; }
+
+static void particle_stage(ob_world* w, float dt) {
+    const float* P = w->prm;
+    int np = w->np;
+    w->pstats[0] = np; w->pstats[1] = 0; w->pstats[2] = 0; w->pstats[3] = w->nfree;
+    int nbl = w->nblast < 3 ? w->nblast : 3;
+    if (np == 0) { w->nblast -= nbl; memmove(w->blast, w->blast + nbl, sizeof(w->blast[0]) * (size_t)w->nblast); w->pwake = 0; return; }
+    int psub = (int)P[P_PSUB]; if (psub < 1) psub = 1; if (psub > 16) psub = 16;
+    const float h = dt / (float)psub, pc = op_pcell(w);
+    const V3 g = v3(P[P_GX], P[P_GY], P[P_GZ]);
+    const float mug = P[P_FRICTION];
+    const uint32_t G = op_grid_size(np), mask = G - 1u;
+    /* ---- blasts, wakes, grid (gpu_world.js:1696-1713) */
+    uint32_t* cnt = (uint32_t*)calloc(G + 1, sizeof(uint32_t));
+    int moving = 0;
+    for (int i = 0; i < np; i++) {
+        Particle* p = &w->pt[i];
+        if (p->st == 0.0f || p->st >= 3.0f) continue;
+        V3 v = p->v; int hit = 0;
+        for (int b = 0; b < nbl; b++) {
+            const float* bl = w->blast[b];
+            V3 d = vsub(p->x, v3(bl[0], bl[1], bl[2])); float l = vlen(d);
+            if (l < bl[3]) { hit = 1; v = vadd(v, vmul(vadd(vmul(d, 1.0f / om_max(l, 1e-3f)), v3(0.0f, 0.3f, 0.0f)), bl[4] * (1.0f - l / bl[3]))); }
+        }
+        if (p->st == 2.0f && (hit || w->pwake || op_near_moving(w, vsub(p->x, v3(0.5f, 0.5f, 0.5f)), vadd(p->x, v3(0.5f, 0.5f, 0.5f))))) { p->st = 1.0f; p->t = 0.0f; }
+        if (p->st == 1.0f) { moving++; p->v = v; }
+        p->cell = op_hash(op_cell(p->x.x, pc), op_cell(p->x.y, pc), op_cell(p->x.z, pc), mask);
+        cnt[p->cell + 1]++;
+    }
+    w->nblast -= nbl; memmove(w->blast, w->blast + nbl, sizeof(w->blast[0]) * (size_t)w->nblast); w->pwake = 0;
+    w->pstats[1] = moving;
+    for (uint32_t k = 0; k < G; k++) cnt[k + 1] += cnt[k];
+    int32_t* list = (int32_t*)malloc(sizeof(int32_t) * (size_t)np);
+    uint32_t* fill = (uint32_t*)malloc(sizeof(uint32_t) * G);
+    memcpy(fill, cnt, sizeof(uint32_t) * G);
+    for (int i = 0; i < np; i++) {  /* index order: each slot's list ascending */
+        const Particle* p = &w->pt[i];
+        if (p->st == 0.0f || p->st >= 3.0f) continue;
+        list[fill[p->cell]++] = i;
+    }
+    if (moving) {
+        /* ---- candidates (gpu_world.js:1774-1816) */
+        for (int i = 0; i < np; i++) {
+            Particle* p = &w->pt[i];
+            if (p->st != 1.0f) continue;
+            const V3 xi = p->x; const float ri = p->r;
+            const float m = vlen(p->v) * dt + 0.02f;
+            int cx = op_cell(xi.x, pc), cy = op_cell(xi.y, pc), cz = op_cell(xi.z, pc);
+            uint32_t seen[27]; int ns = 0; int nn = 0;
+            for (int z = cz - 1; z <= cz + 1; z++) for (int y = cy - 1; y <= cy + 1; y++) for (int x = cx - 1; x <= cx + 1; x++) {
+                uint32_t hc = op_hash(x, y, z, mask);
+                int dup = 0; for (int q = 0; q < ns; q++) if (seen[q] == hc) dup = 1;
+                if (dup) continue;
+                seen[ns++] = hc;
+                for (uint32_t e = cnt[hc]; e < cnt[hc + 1]; e++) {
+                    int j = list[e]; if (j == i) continue;
+                    const Particle* q = &w->pt[j];
+                    float rr = ri + q->r + m + vlen(q->v) * dt + 0.01f;
+                    V3 d = vsub(xi, q->x);
+                    if (vdot(d, d) < rr * rr && nn < OP_NN) p->nb[nn++] = j;
+                }
+            }
+            p->nn = nn;
+            /* shapes: the 12 lowest indices whose step-start box meets the particle's reach */
+            float e = ri + m + 0.01f;
+            int nsh = 0;
+            for (int sidx = 0; sidx < w->ns && nsh < OP_NS; sidx++) {
+                const Shape* sh = &w->sh[sidx];
+                if (xi.x + e < sh->lo.x || xi.y + e < sh->lo.y || xi.z + e < sh->lo.z ||
+                    xi.x - e > sh->hi.x || xi.y - e > sh->hi.y || xi.z - e > sh->hi.z) continue;
+                if (!op_in_grid(w, sidx)) continue;
+                p->sh[nsh++] = sidx;
+            }
+            p->ns = nsh;
+        }
+        /* ---- sub-steps */
+        const float cap = 2.0f * h;
+        for (int sub = 0; sub < psub; sub++) {
+            for (int i = 0; i < np; i++) {  /* predict (:1755-1763) */
+                Particle* p = &w->pt[i];
+                if (p->st != 1.0f) continue;
+                p->v = op_cap(vadd(p->v, vmul(g, h)));
+                p->xs = p->x;
+                p->x = vadd(p->x, vmul(p->v, h));
+            }
+            for (int i = 0; i < np; i++) {  /* push out (:1818-1924) */
+                Particle* p = &w->pt[i];
+                if (p->st != 1.0f) continue;
+                const V3 xi = p->x, xs = p->xs; const float ri = p->r, mui = p->mu;
+                V3 corr = v3(0, 0, 0); float n = 0.0f, touch = 0.0f; int bearing = 0, grounded = 0;
+                for (int q = 0; q < p->nn; q++) {
+                    const Particle* o = &w->pt[p->nb[q]];
+                    V3 d = vsub(xi, o->x); float d2 = vdot(d, d), rr = ri + o->r;
+                    if (d2 >= (rr + 0.01f) * (rr + 0.01f) || d2 < 1e-12f) continue;
+                    float dist = om_sqrt(d2); V3 nn = vmul(d, 1.0f / dist);
+                    if (o->g_last && nn.y > 0.5f) grounded = 1;
+                    touch = 1.0f;
+                    if (dist >= rr) continue;
+                    const int om = o->st == 1.0f;
+                    const float wgt = om ? 0.5f : 1.0f;
+                    const V3 dj = om ? vsub(o->x, o->xs) : v3(0, 0, 0);
+                    const V3 rel = vsub(vsub(xi, xs), dj);
+                    float pen = om_min((rr - dist) * wgt, (om_max(-vdot(rel, nn), 0.0f) + cap) * wgt);
+                    corr = vadd(corr, vadd(vmul(nn, pen), vmul(op_fric(rel, nn, pen, mui), wgt)));
+                    n += 1.0f;
+                }
+                V3 x = vadd(xi, vmul(corr, n > 1.5f ? 1.5f / n : 1.0f));
+                float pg0 = ri - x.y;
+                if (pg0 > -0.01f) { touch = 1.0f; grounded = 1; }
+                if (pg0 > 0.0f) {
+                    V3 nn = v3(0.0f, 1.0f, 0.0f);
+                    x = vadd(x, vadd(vmul(nn, pg0), op_fric(vsub(x, xs), nn, pg0, 0.5f * (mui + mug))));
+                }
+                for (int q = 0; q < p->ns; q++) {
+                    const Shape* sh = &w->sh[p->sh[q]];
+                    const Body* bd = &w->bd[sh->body];
+                    V3 dl = vsub(x, sh->wp);
+                    V3 lc = m3_tmulv(&sh->wrot, dl);
+                    V3 hh = sh->h;
+                    V3 cl = v3(om_min(om_max(lc.x, -hh.x), hh.x), om_min(om_max(lc.y, -hh.y), hh.y), om_min(om_max(lc.z, -hh.z), hh.z));
+                    V3 df = vsub(lc, cl); float d2 = vdot(df, df);
+                    V3 nl; float pen;
+                    if (d2 > 1e-12f) {
+                        float dist = om_sqrt(d2);
+                        if (dist >= ri + 0.01f) continue;
+                        touch = 1.0f;
+                        if (dist >= ri) continue;
+                        nl = vmul(df, 1.0f / dist); pen = ri - dist;
+                    } else {
+                        V3 dd = v3(hh.x - om_fabs(lc.x), hh.y - om_fabs(lc.y), hh.z - om_fabs(lc.z));
+                        if (dd.x <= dd.y && dd.x <= dd.z) { nl = v3(lc.x >= 0.0f ? 1.0f : -1.0f, 0.0f, 0.0f); pen = dd.x + ri; }
+                        else if (dd.y <= dd.z) { nl = v3(0.0f, lc.y >= 0.0f ? 1.0f : -1.0f, 0.0f); pen = dd.y + ri; }
+                        else { nl = v3(0.0f, 0.0f, lc.z >= 0.0f ? 1.0f : -1.0f); pen = dd.z + ri; }
+                    }
+                    V3 nn = m3_mulv(&sh->wrot, nl);
+                    V3 vb = vadd(bd->v, vcross(bd->w, vsub(x, bd->p)));
+                    V3 vbh = vmul(vb, h);
+                    float push = om_min(pen, om_max(vdot(vsub(vbh, vsub(x, xs)), nn), 0.0f) + cap);
+                    float mu = 0.5f * (mui + sh->mu);
+                    x = vadd(x, vadd(vmul(nn, push), op_fric(vsub(vsub(x, xs), vbh), nn, push, mu)));
+                    touch = 1.0f;
+                    const int fixed = !(bd->inv_m > 0.0f) || !bd->awake;
+                    if (fixed && nn.y > 0.5f) grounded = 1;
+                    if (!fixed && nn.y < -0.3f) bearing = 1;
+                }
+                p->xc = x; p->touch = bearing ? 2.0f : touch; p->g_now = (uint32_t)grounded;
+            }
+            for (int i = 0; i < np; i++) {  /* velocity from the move (:1994-2001) */
+                Particle* p = &w->pt[i];
+                if (p->st != 1.0f) continue;
+                p->g_last = p->g_now;
+                p->v = op_cap(vmul(vsub(p->xc, p->xs), 1.0f / h));
+                p->x = p->xc;
+            }
+        }
+        /* ---- rest timers, spin, drawing rotation (:2002-2033) */
+        for (int i = 0; i < np; i++) {
+            Particle* p = &w->pt[i];
+            if (p->st != 1.0f) continue;
+            V3 v = p->v, ww = p->w; float t = 0.0f, st = 1.0f;
+            if (p->touch > 0.0f) ww = vmul(vcross(v3(0.0f, 1.0f, 0.0f), v), 1.0f / om_max(p->r, 0.01f));
+            float dr = vlen(vsub(p->x, p->anchor));
+            if (p->touch < 1.5f && dr < 0.01f) t = p->t + dt;
+            else if (p->touch < 1.5f && dr < 2.0f * p->r &&
+                     !op_near_moving(w, vsub(p->x, v3(p->r, p->r, p->r)), vadd(p->x, v3(p->r, p->r, p->r)))) t = p->t + 0.25f * dt;
+            else p->anchor = p->x;
+            if (t > 0.5f || p->x.y < -50.0f) { st = 2.0f; v = v3(0, 0, 0); ww = v3(0, 0, 0); }
+            p->q = qintegrate(p->q, ww, dt);
+            p->v = v; p->st = st; p->t = t; p->w = ww;
+        }
+    }
+    int asleep = 0;
+    for (int i = 0; i < np; i++) if (w->pt[i].st == 2.0f) asleep++;
+    w->pstats[2] = asleep;
+    free(cnt); free(list); free(fill);
 }

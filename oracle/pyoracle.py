@@ -32,6 +32,9 @@ def lib():
             "ob_load": (ci, [vp, ci, vp, ci, vp, vp]), "ob_step": (None, [vp, cf]), "ob_nbody": (ci, [vp]),
             "ob_read_bodies": (None, [vp, vp]), "ob_stats": (None, [vp, vp]), "ob_candidates": (ci, [vp, vp, ci]),
             "ob_manifolds": (ci, [vp, vp, vp, ci]), "ob_colours": (ci, [vp, vp, ci]), "ob_islands": (None, [vp, vp]), "ob_levels": (None, [vp, vp]),
+            "ob_particles_capacity": (ci, [vp, ci]), "ob_add_particles": (ci, [vp, ci, vp, vp]),
+            "ob_kill_particle": (None, [vp, ci]), "ob_blast_particles": (None, [vp, cf, cf, cf, cf, cf]),
+            "ob_wake_particles": (None, [vp]), "ob_read_particles": (ci, [vp, vp]), "ob_particle_stats": (None, [vp, vp]),
             "ob_bruteforce_pairs": (ci, [vp, cf, vp, ci]),
             "ol_sor_sweep": (cf, [vp, vp, cf]), "ol_solve64": (ci, [vp, vp, cd, ci]), "ol_residual64": (cd, [vp, vp]),
             "ol_verdicts64": (None, [vp, vp, vp, vp, vp]), "ol_verdicts32": (None, [vp, vp, vp, vp]),
@@ -111,6 +114,38 @@ class BoxOracle:
     def levels(self):
         out = np.zeros(self.nb, np.int32)
         lib().ob_levels(self._h, _p(out))
+        return out
+
+    # particle debris
+    def particles_capacity(self, cap):
+        assert lib().ob_particles_capacity(self._h, int(cap)) == 0
+
+    def add_particles(self, rec):
+        rec = np.ascontiguousarray(rec, np.float32).reshape(-1, 8)
+        slots = np.zeros(len(rec), np.int32)
+        lib().ob_add_particles(self._h, len(rec), _p(rec), _p(slots))
+        return slots
+
+    def kill_particle(self, i):
+        lib().ob_kill_particle(self._h, int(i))
+
+    def blast_particles(self, c, reach, speed):
+        lib().ob_blast_particles(self._h, ctypes.c_float(c[0]), ctypes.c_float(c[1]), ctypes.c_float(c[2]),
+                                 ctypes.c_float(reach), ctypes.c_float(speed))
+
+    def wake_particles(self):
+        lib().ob_wake_particles(self._h)
+
+    def particles(self):
+        """12 floats per slot: x y z vx, vy vz qx qy, qz qw state radius"""
+        st = self.particle_stats()
+        out = np.zeros((max(st[0], 1), 12), np.float32)
+        lib().ob_read_particles(self._h, _p(out))
+        return out[:st[0]]
+
+    def particle_stats(self):
+        out = np.zeros(4, np.int32)
+        lib().ob_particle_stats(self._h, _p(out))
         return out
 
     def bruteforce_pairs(self, dt=1.0 / 60.0, cap=4_000_000):

@@ -24,7 +24,7 @@ enum { OB_ORDER_REFERENCE = 0, OB_ORDER_COLOURED = 1 };
 /* Parameter block, phys.rs:74-106 defaults plus the colouring limits and the GPU world's solver extensions (mass-ratio
  * limit, shock-propagation level passes, restitution threshold and iterations, supports-first colouring rounds; all
  * off by default). Index names in sf.h (SF_P_*). */
-#define OB_NPARAM 30
+#define OB_NPARAM 32
 
 /* body record: p[3] q[4] v[3] w[3] inv_m inv_i[3] awake sleep_t reserved  (20 floats) */
 #define OB_BODY_F 20
@@ -55,6 +55,17 @@ void ob_islands(const ob_world* w, int32_t* out);
 /* shock-propagation heights of the last step per body (255 - level; 255 for bodies not moving, 0 for bodies no
  * support chain reached), all 0 when the passes are off */
 void ob_levels(const ob_world* w, int32_t* out);
+/* particle debris (gpu/gpu_world.js:1671-2038, addParticles/killParticle/blastParticles/readParticles :2870-2896):
+ * capacity, records of 8 floats (p[3] v[3] radius friction) returning their slots, kill, blast (applied at the next
+ * step, three a step), wake all, read 12 floats per slot (x y z vx, vy vz qx qy, qz qw state radius), stats
+ * [slots, moving at the stage's start, asleep at its end, free slots] */
+int ob_particles_capacity(ob_world* w, int cap);
+int ob_add_particles(ob_world* w, int n, const float* rec, int32_t* slots);
+void ob_kill_particle(ob_world* w, int i);
+void ob_blast_particles(ob_world* w, float cx, float cy, float cz, float reach, float speed);
+void ob_wake_particles(ob_world* w);
+int ob_read_particles(const ob_world* w, float* out);
+void ob_particle_stats(const ob_world* w, int32_t* out4);
 /* brute-force O(n^2) candidate pairs for the current pose (test pin for the broadphase) */
 int ob_bruteforce_pairs(ob_world* w, float dt, int32_t* out, int cap);
 

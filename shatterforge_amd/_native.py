@@ -34,6 +34,14 @@ _PROTOS = {
     "sf_world_colours": (c_int, [c_void_p, c_void_p, c_int]),
     "sf_world_islands": (c_int, [c_void_p, c_void_p]),
     "sf_world_pose": (ctypes.POINTER(c_float), [c_void_p]),
+    "sf_world_particles_capacity": (c_int, [c_void_p, c_int]),
+    "sf_world_add_particles": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "sf_world_kill_particle": (c_int, [c_void_p, c_int]),
+    "sf_world_blast_particles": (c_int, [c_void_p, c_float, c_float, c_float, c_float, c_float]),
+    "sf_world_wake_particles": (c_int, [c_void_p]),
+    "sf_world_read_particles": (c_int, [c_void_p, c_void_p]),
+    "sf_world_particle_stats": (c_int, [c_void_p, c_void_p]),
+    "sf_world_particle_ms": (c_int, [c_void_p, c_void_p]),
     "sf_world_set_timing": (c_int, [c_void_p, c_int]),
     "sf_world_stage_ms": (c_int, [c_void_p, c_void_p]),
     "sf_world_set_kernel_timing": (c_int, [c_void_p, c_int]),

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <cmath>
 #include <algorithm>
+#include <array>
 
 namespace {
 
@@ -2024,6 +2025,293 @@ __global__ void __launch_bounds__(NT) k_step_clear(Dev D, int K) {
 }
 __global__ void k_fill_u64(unsigned long long* p, size_t n) { for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull; }
 
+// ---------------------------------------------------------------- particle debris (gpu/gpu_world.js:1671-2038)
+// Position-based debris, after the body step: every sub-step predicts, pushes each moving particle out of the other
+// particles (Jacobi: the pushes averaged and over-relaxed, against the predicted positions of all), then out of the
+// ground and every candidate shape in turn, each in full, with Coulomb friction from the push; the velocity is the
+// move over the sub-step. Particles never push bodies. A particle at rest for 0.5 s sleeps; a blast, a wake request
+// or a moving shape near it wakes it. Candidates are gathered once per step: particles from a hashed grid (each
+// slot's particles in index order, the 27 cells round the particle's in z-y-x order, the first 24 in reach), shapes
+// from this step's cell table (the 12 lowest indices whose step-start box meets the particle's reach). Shapes are
+// seen at their step-start pose, their bodies' velocities at the end of the step. One lane per particle; records
+// are struct-of-arrays of float4, candidate lists stored by rank ([k][capacity]) so a wavefront's loads coalesce.
+constexpr int P_NN = 24, P_NS = 12;
+struct PDev {
+    int np, cap;
+    uint32_t G, gmask;
+    float4 *x, *v, *q, *s, *w, *c, *a;  // x|r, v|state, q, xs|t, w|mu, xc|touch, anchor|-
+    uint32_t *gl, *gn;                  // on the ground (directly or through grounded particles): last sub-step, this one
+    int2* cnt;                          // neighbours, shapes
+    int32_t *nb, *sh;                   // [P_NN][cap], [P_NS][cap]
+    uint32_t* cell;                     // grid slot of each particle
+    uint32_t *gcnt, *gstart, *gfill;    // [G]
+    int32_t* list;                      // [cap]
+    uint32_t* ctr;                      // moving at the stage's start, asleep at its end
+};
+struct PStepK {
+    float dt, h, pc, mug, gx, gy, gz, cell;
+    int psub, nblast, wake;
+    float4 bc[3];  // blast centre | reach
+    float bs[3];   // blast speed
+};
+
+__device__ __forceinline__ uint32_t p_hash(int x, int y, int z, uint32_t mask) {
+    return (((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u)) & mask;
+}
+__device__ __forceinline__ int p_cellc(float v, float pc) { return (int32_t)sf_floor(v / pc); }
+__device__ __forceinline__ V3 p_fric(V3 disp, V3 n, float pen, float mu) {
+    const V3 t = vsub(disp, vmul(n, vdot(disp, n)));
+    const float tl = vlen(t);
+    if (tl < 1e-9f) return v3(0, 0, 0);
+    return vmul(t, -sf_min(mu * pen / tl, 1.0f));
+}
+__device__ __forceinline__ V3 p_capv(V3 v) { const float l = vlen(v); return l > 40.0f ? vmul(v, 40.0f / l) : v; }
+// Each shape of this step's cell table whose step-start box meets [lo, hi], once (a shape sits in every cell its box
+// covers; it is visited in the first of the box's cells it shares): f(shape).
+template <class F>
+__device__ __forceinline__ void p_table_shapes(const Dev& D, const PStepK& K, V3 lo, V3 hi, F f) {
+    const float ic = 1.0f / K.cell;
+    const int x0 = (int32_t)sf_floor(lo.x * ic), y0 = (int32_t)sf_floor(lo.y * ic), z0 = (int32_t)sf_floor(lo.z * ic);
+    const int x1 = (int32_t)sf_floor(hi.x * ic), y1 = (int32_t)sf_floor(hi.y * ic), z1 = (int32_t)sf_floor(hi.z * ic);
+    for (int x = x0; x <= x1; x++) for (int y = y0; y <= y1; y++) for (int z = z0; z <= z1; z++) {
+        const uint32_t key = cell_key(x, y, z), sl = slot_of(key, D.tmask);
+        const uint32_t e0 = D.tstart[sl], e1 = e0 + D.tcnt[sl];
+        for (uint32_t e = e0; e < e1; e++) {
+            if (D.tkey[e] != key) continue;
+            const int s = D.tent[e];
+            const float4 slo = D.slo[s], shi = D.shi[s];
+            if (slo.x > hi.x || lo.x > shi.x || slo.y > hi.y || lo.y > shi.y || slo.z > hi.z || lo.z > shi.z) continue;
+            // the first cell of [lo, hi] the shape's box covers: visit it there only
+            const int fx = max(x0, (int32_t)sf_floor(slo.x * ic)), fy = max(y0, (int32_t)sf_floor(slo.y * ic)), fz = max(z0, (int32_t)sf_floor(slo.z * ic));
+            if (fx != x || fy != y || fz != z) continue;
+            f(s);
+        }
+    }
+}
+// a moving shape (awake body with mass, as of now) whose step-start box meets [lo, hi]
+__device__ __forceinline__ bool p_near_moving(const Dev& D, const PStepK& K, V3 lo, V3 hi) {
+    bool found = false;
+    p_table_shapes(D, K, lo, hi, [&](int s) {
+        const int b = D.shb[s];
+        if ((D.bflag[b] & 1) && D.bp[b].w > 0.0f) found = true;
+    });
+    return found;
+}
+
+__global__ void __launch_bounds__(NT) k_p_prep(Dev D, PDev P, PStepK K) {
+    uint32_t nmov = 0;
+    GRID_STRIDE(i, (uint32_t)P.np) {
+        float4 v4 = P.v[i];
+        if (v4.w == 0.0f || v4.w >= 3.0f) continue;
+        const float4 x4 = P.x[i];
+        const V3 x = v3f4(x4);
+        V3 v = v3f4(v4);
+        bool hit = false;
+        for (int b = 0; b < K.nblast; b++) {
+            const V3 d = vsub(x, v3f4(K.bc[b]));
+            const float l = vlen(d);
+            if (l < K.bc[b].w) { hit = true; v = vadd(v, vmul(vadd(vmul(d, 1.0f / sf_max(l, 1e-3f)), v3(0.0f, 0.3f, 0.0f)), K.bs[b] * (1.0f - l / K.bc[b].w))); }
+        }
+        if (v4.w == 2.0f && (hit || K.wake || p_near_moving(D, K, vsub(x, v3(0.5f, 0.5f, 0.5f)), vadd(x, v3(0.5f, 0.5f, 0.5f))))) {
+            v4.w = 1.0f;
+            const float4 s4 = P.s[i];
+            P.s[i] = make_float4(s4.x, s4.y, s4.z, 0.0f);
+        }
+        if (v4.w == 1.0f) { nmov++; P.v[i] = f4(v, 1.0f); }
+        const uint32_t hc = p_hash(p_cellc(x.x, K.pc), p_cellc(x.y, K.pc), p_cellc(x.z, K.pc), P.gmask);
+        P.cell[i] = hc;
+        atomicAdd(&P.gcnt[hc], 1u);
+    }
+    block_count(nmov, &P.ctr[0]);
+}
+__global__ void __launch_bounds__(NT) k_p_scatter(PDev P) {
+    GRID_STRIDE(i, (uint32_t)P.np) {
+        const float st = P.v[i].w;
+        if (st == 0.0f || st >= 3.0f) continue;
+        const uint32_t hc = P.cell[i];
+        P.list[P.gstart[hc] + atomicAdd(&P.gfill[hc], 1u)] = (int32_t)i;
+    }
+}
+// each grid slot's particles in index order (the lists are a few long: insertion sort, a thread per slot)
+__global__ void __launch_bounds__(NT) k_p_sort(PDev P) {
+    GRID_STRIDE(hc, P.G) {
+        const uint32_t s0 = P.gstart[hc], n = P.gcnt[hc];
+        for (uint32_t a = 1; a < n; a++) {
+            const int32_t v = P.list[s0 + a];
+            uint32_t b = a;
+            while (b > 0 && P.list[s0 + b - 1] > v) { P.list[s0 + b] = P.list[s0 + b - 1]; b--; }
+            P.list[s0 + b] = v;
+        }
+    }
+}
+__global__ void __launch_bounds__(NT) k_p_gather(Dev D, PDev P, PStepK K) {
+    GRID_STRIDE(i, (uint32_t)P.np) {
+        const float4 v4 = P.v[i];
+        if (v4.w != 1.0f) continue;
+        const float4 x4 = P.x[i];
+        const V3 xi = v3f4(x4);
+        const float ri = x4.w;
+        const float m = vlen(v3f4(v4)) * K.dt + 0.02f;
+        const int cx = p_cellc(xi.x, K.pc), cy = p_cellc(xi.y, K.pc), cz = p_cellc(xi.z, K.pc);
+        int nn = 0;
+        for (int k = 0; k < 27; k++) {
+            const int z = cz - 1 + k / 9, y = cy - 1 + (k / 3) % 3, x = cx - 1 + k % 3;
+            const uint32_t hc = p_hash(x, y, z, P.gmask);
+            bool dup = false;  // (a slot already visited through another cell)
+            for (int k2 = 0; k2 < k; k2++) dup |= p_hash(cx - 1 + k2 % 3, cy - 1 + (k2 / 3) % 3, cz - 1 + k2 / 9, P.gmask) == hc;
+            if (dup) continue;
+            const uint32_t e0 = P.gstart[hc], e1 = e0 + P.gcnt[hc];
+            for (uint32_t e = e0; e < e1; e++) {
+                const int j = P.list[e];
+                if (j == (int)i) continue;
+                const float4 xj = P.x[j];
+                const float rr = ri + xj.w + m + vlen(v3f4(P.v[j])) * K.dt + 0.01f;
+                const V3 d = vsub(xi, v3f4(xj));
+                if (vdot(d, d) < rr * rr && nn < P_NN) { P.nb[(size_t)nn * P.cap + i] = j; nn++; }
+            }
+        }
+        // the 12 lowest shape indices whose step-start box meets the particle's reach (kept sorted in registers)
+        const float e = ri + m + 0.01f;
+        int best[P_NS];
+#pragma unroll
+        for (int k = 0; k < P_NS; k++) best[k] = 0x7fffffff;
+        p_table_shapes(D, K, vsub(xi, v3(e, e, e)), vadd(xi, v3(e, e, e)), [&](int s) {
+            if (s >= best[P_NS - 1]) return;
+            int c = s;
+#pragma unroll
+            for (int k = 0; k < P_NS; k++) { const int lo = min(c, best[k]), hi = max(c, best[k]); best[k] = lo; c = hi; }
+        });
+        int ns = 0;
+#pragma unroll
+        for (int k = 0; k < P_NS; k++) if (best[k] != 0x7fffffff) { P.sh[(size_t)k * P.cap + i] = best[k]; ns = k + 1; }
+        P.cnt[i] = make_int2(nn, ns);
+    }
+}
+__global__ void __launch_bounds__(NT) k_p_integrate(PDev P, PStepK K) {
+    GRID_STRIDE(i, (uint32_t)P.np) {
+        const float4 v4 = P.v[i];
+        if (v4.w != 1.0f) continue;
+        const V3 v = p_capv(vadd(v3f4(v4), vmul(v3(K.gx, K.gy, K.gz), K.h)));
+        const float4 x4 = P.x[i];
+        P.s[i] = make_float4(x4.x, x4.y, x4.z, P.s[i].w);
+        P.x[i] = f4(vadd(v3f4(x4), vmul(v, K.h)), x4.w);
+        P.v[i] = f4(v, 1.0f);
+    }
+}
+__global__ void __launch_bounds__(NT) k_p_contact(Dev D, PDev P, PStepK K) {
+    GRID_STRIDE(i, (uint32_t)P.np) {
+        if (P.v[i].w != 1.0f) continue;
+        const float4 x4 = P.x[i];
+        const V3 xi = v3f4(x4), xs = v3f4(P.s[i]);
+        const float ri = x4.w, mui = P.w[i].w;
+        const int2 cn = P.cnt[i];
+        const float cap = 2.0f * K.h;
+        V3 corr = v3(0, 0, 0);
+        float n = 0.0f, touch = 0.0f;
+        bool bearing = false, grounded = false;
+        for (int q = 0; q < cn.x; q++) {
+            const int j = P.nb[(size_t)q * P.cap + i];
+            const float4 xj = P.x[j];
+            const V3 d = vsub(xi, v3f4(xj));
+            const float d2 = vdot(d, d), rr = ri + xj.w;
+            if (d2 >= (rr + 0.01f) * (rr + 0.01f) || d2 < 1e-12f) continue;
+            const float dist = sf_sqrt(d2);
+            const V3 nn = vmul(d, 1.0f / dist);
+            if (P.gl[j] && nn.y > 0.5f) grounded = true;
+            touch = 1.0f;
+            if (dist >= rr) continue;
+            const bool om = P.v[j].w == 1.0f;
+            const float wgt = om ? 0.5f : 1.0f;
+            const V3 dj = om ? vsub(v3f4(xj), v3f4(P.s[j])) : v3(0, 0, 0);
+            const V3 rel = vsub(vsub(xi, xs), dj);
+            const float pen = sf_min((rr - dist) * wgt, (sf_max(-vdot(rel, nn), 0.0f) + cap) * wgt);
+            corr = vadd(corr, vadd(vmul(nn, pen), vmul(p_fric(rel, nn, pen, mui), wgt)));
+            n += 1.0f;
+        }
+        V3 x = vadd(xi, vmul(corr, n > 1.5f ? 1.5f / n : 1.0f));
+        const float pg0 = ri - x.y;
+        if (pg0 > -0.01f) { touch = 1.0f; grounded = true; }
+        if (pg0 > 0.0f) {
+            const V3 nn = v3(0.0f, 1.0f, 0.0f);
+            x = vadd(x, vadd(vmul(nn, pg0), p_fric(vsub(x, xs), nn, pg0, 0.5f * (mui + K.mug))));
+        }
+        for (int q = 0; q < cn.y; q++) {
+            const int sj = P.sh[(size_t)q * P.cap + i];
+            const int b = D.shb[sj];
+            const V3 wp = v3f4(D.swp[sj]);
+            const M3 Rm = load_m3(D.swr + 3 * (size_t)sj);
+            const float4 hh4 = D.shh[sj];
+            const V3 hh = v3f4(hh4);
+            const V3 lc = tmulv(Rm, vsub(x, wp));
+            const V3 cl = v3(sf_min(sf_max(lc.x, -hh.x), hh.x), sf_min(sf_max(lc.y, -hh.y), hh.y), sf_min(sf_max(lc.z, -hh.z), hh.z));
+            const V3 df = vsub(lc, cl);
+            const float d2 = vdot(df, df);
+            V3 nl;
+            float pen;
+            if (d2 > 1e-12f) {
+                const float dist = sf_sqrt(d2);
+                if (dist >= ri + 0.01f) continue;
+                touch = 1.0f;
+                if (dist >= ri) continue;
+                nl = vmul(df, 1.0f / dist); pen = ri - dist;
+            } else {
+                const V3 dd = v3(hh.x - sf_abs(lc.x), hh.y - sf_abs(lc.y), hh.z - sf_abs(lc.z));
+                if (dd.x <= dd.y && dd.x <= dd.z) { nl = v3(lc.x >= 0.0f ? 1.0f : -1.0f, 0.0f, 0.0f); pen = dd.x + ri; }
+                else if (dd.y <= dd.z) { nl = v3(0.0f, lc.y >= 0.0f ? 1.0f : -1.0f, 0.0f); pen = dd.y + ri; }
+                else { nl = v3(0.0f, 0.0f, lc.z >= 0.0f ? 1.0f : -1.0f); pen = dd.z + ri; }
+            }
+            const V3 nn = mulv(Rm, nl);
+            const float4 bp = D.bp[b];
+            const V3 vb = vadd(v3f4(D.bv[b]), vcross(v3f4(D.bw[b]), vsub(x, v3f4(bp))));
+            const V3 vbh = vmul(vb, K.h);
+            const float push = sf_min(pen, sf_max(vdot(vsub(vbh, vsub(x, xs)), nn), 0.0f) + cap);
+            const float mu = 0.5f * (mui + hh4.w);
+            x = vadd(x, vadd(vmul(nn, push), p_fric(vsub(vsub(x, xs), vbh), nn, push, mu)));
+            touch = 1.0f;
+            const bool fixed = !(bp.w > 0.0f) || !(D.bflag[b] & 1);
+            if (fixed && nn.y > 0.5f) grounded = true;
+            if (!fixed && nn.y < -0.3f) bearing = true;
+        }
+        P.c[i] = f4(x, bearing ? 2.0f : touch);
+        P.gn[i] = grounded ? 1u : 0u;
+    }
+}
+__global__ void __launch_bounds__(NT) k_p_apply(PDev P, PStepK K) {
+    GRID_STRIDE(i, (uint32_t)P.np) {
+        if (P.v[i].w != 1.0f) continue;
+        P.gl[i] = P.gn[i];
+        const float4 c4 = P.c[i], x4 = P.x[i];
+        P.v[i] = f4(p_capv(vmul(vsub(v3f4(c4), v3f4(P.s[i])), 1.0f / K.h)), 1.0f);
+        P.x[i] = f4(v3f4(c4), x4.w);
+    }
+}
+__global__ void __launch_bounds__(NT) k_p_finish(Dev D, PDev P, PStepK K) {
+    uint32_t nasleep = 0;
+    GRID_STRIDE(i, (uint32_t)P.np) {
+        float4 v4 = P.v[i];
+        if (v4.w == 1.0f) {
+            const float4 x4 = P.x[i], s4 = P.s[i], w4 = P.w[i], a4 = P.a[i];
+            const float touch = P.c[i].w, r = x4.w;
+            const V3 x = v3f4(x4);
+            V3 v = v3f4(v4), ww = v3f4(w4);
+            float t = 0.0f, st = 1.0f;
+            if (touch > 0.0f) ww = vmul(vcross(v3(0.0f, 1.0f, 0.0f), v), 1.0f / sf_max(r, 0.01f));
+            const float dr = vlen(vsub(x, v3f4(a4)));
+            if (touch < 1.5f && dr < 0.01f) t = s4.w + K.dt;
+            else if (touch < 1.5f && dr < 2.0f * r && !p_near_moving(D, K, vsub(x, v3(r, r, r)), vadd(x, v3(r, r, r)))) t = s4.w + 0.25f * K.dt;
+            else P.a[i] = f4(x, 0.0f);
+            if (t > 0.5f || x.y < -50.0f) { st = 2.0f; v = v3(0, 0, 0); ww = v3(0, 0, 0); }
+            P.q[i] = f4q(qintegrate(qf4(P.q[i]), ww, K.dt));
+            v4 = f4(v, st);
+            P.v[i] = v4;
+            P.s[i] = make_float4(s4.x, s4.y, s4.z, t);
+            P.w[i] = f4(ww, w4.w);
+        }
+        if (v4.w == 2.0f) nasleep++;
+    }
+    block_count(nasleep, &P.ctr[1]);
+}
+
 // ---------------------------------------------------------------- host
 struct HostWorld {
     int dev = 0;
@@ -2069,13 +2357,29 @@ struct HostWorld {
     float stage_ms[5] = {0};
     std::vector<int32_t> h_shb;
     std::vector<float> h_h;  // half extents of the first shape of each body (pose())
+    // particle debris: device records, free slots (reused last in, first out), queued blasts, wake request, stats
+    PDev P{};
+    uint32_t Gmax = 0;
+    uint32_t* pscan = nullptr;
+    std::vector<void*> pallocs;
+    std::vector<int32_t> pfree;
+    std::vector<std::array<float, 5>> pblast;
+    bool pwake = false;
+    uint32_t* h_pctr = nullptr;  // pinned: moving, asleep
+    int32_t pstats[4] = {0, 0, 0, 0};
+    hipEvent_t pev[2] = {};      // around the particle stage, when timing is on
+    bool ptimed = false;
+    float pstage_ms = 0.0f;
 
     ~HostWorld() {
         if (st) (void)hipStreamSynchronize(st);
         for (void* p : allocs) (void)hipFree(p);
+        for (void* p : pallocs) (void)hipFree(p);
+        if (h_pctr) (void)hipHostFree(h_pctr);
         if (h_ctr) (void)hipHostFree(h_ctr);
         if (h_col) (void)hipHostFree(h_col);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        for (auto& e : pev) if (e) (void)hipEventDestroy(e);
         for (auto& e : kev) (void)hipEventDestroy(e);
         for (auto& g : sg) g.clear();
         if (st) (void)hipStreamDestroy(st);
@@ -2098,7 +2402,7 @@ void default_params(float* p) {
     p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 16;
     p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_TAIL_MIN] = 1024; p[SF_P_SOLVE_NT] = 64;
     p[SF_P_RATIO] = 0.0f; p[SF_P_SHOCK] = 0.0f; p[SF_P_REST_THRESHOLD] = 1.0f; p[SF_P_REST_ITERS] = 8.0f;
-    p[SF_P_COL_SUPPORT] = 0.0f;
+    p[SF_P_COL_SUPPORT] = 0.0f; p[SF_P_PSUB] = 4.0f; p[SF_P_PCELL] = 0.7f;
 }
 
 int world_alloc(HostWorld* w) {
@@ -2425,6 +2729,146 @@ int finish_stats(HostWorld* w) {
     return 0;
 }
 
+// ---- particle debris (host side)
+uint32_t p_grid_size(int np) { uint32_t g = 256; while (g < 2u * (uint32_t)np) g <<= 1; return g; }
+int particles_alloc(HostWorld* w, int cap) {
+    if (cap < w->P.np) { sf::set_error(SF_E_ARG, "capacity below the particles in use"); return SF_E_ARG; }
+    SF_HIP(hipStreamSynchronize(w->st));
+    PDev old = w->P;
+    std::vector<void*> old_allocs;
+    old_allocs.swap(w->pallocs);
+    uint32_t* old_scan = w->pscan;
+    PDev& P = w->P;
+    const uint32_t G = p_grid_size(cap);
+    auto get = [&](auto*& ptr, size_t n) -> int {
+        void* q = nullptr;
+        SF_HIP(hipMalloc(&q, sizeof(*ptr) * (n ? n : 1)));
+        SF_HIP(hipMemset(q, 0, sizeof(*ptr) * (n ? n : 1)));
+        w->pallocs.push_back(q);
+        ptr = (std::remove_reference_t<decltype(ptr)>)q;
+        return 0;
+    };
+    const size_t c = (size_t)cap;
+    int r = 0;
+    r |= get(P.x, c); r |= get(P.v, c); r |= get(P.q, c); r |= get(P.s, c); r |= get(P.w, c); r |= get(P.c, c); r |= get(P.a, c);
+    r |= get(P.gl, c); r |= get(P.gn, c); r |= get(P.cnt, c); r |= get(P.nb, (size_t)P_NN * c); r |= get(P.sh, (size_t)P_NS * c);
+    r |= get(P.cell, c); r |= get(P.gcnt, G); r |= get(P.gstart, G); r |= get(P.gfill, G); r |= get(P.list, c); r |= get(P.ctr, 4);
+    r |= get(w->pscan, sf::scan_scratch_words(G));
+    if (r) return r;
+    if (!w->h_pctr) SF_HIP(hipHostMalloc((void**)&w->h_pctr, sizeof(uint32_t) * 4, hipHostMallocDefault));
+    SF_HIP(hipDeviceSynchronize());  // (the zero fills ran on the null stream)
+    P.cap = cap;
+    w->Gmax = G;
+    if (old.np) {  // keep the slots in use
+        const size_t n = (size_t)old.np;
+        float4* src[7] = {old.x, old.v, old.q, old.s, old.w, old.c, old.a};
+        float4* dst[7] = {P.x, P.v, P.q, P.s, P.w, P.c, P.a};
+        for (int k = 0; k < 7; k++) SF_HIP(hipMemcpy(dst[k], src[k], sizeof(float4) * n, hipMemcpyDeviceToDevice));
+        SF_HIP(hipMemcpy(P.gl, old.gl, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice));
+        SF_HIP(hipMemcpy(P.gn, old.gn, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice));
+    }
+    P.np = old.np;
+    for (void* q : old_allocs) (void)hipFree(q);
+    (void)old_scan;
+    return 0;
+}
+// Records of 8 floats: p[3] v[3] radius friction (addParticles, gpu/gpu_world.js:2870-2883: moving, at rest
+// rotation, the radius capped at half a grid cell less 5 cm). Slots of the new particles into `slots` (-1: full or
+// not finite).
+int particles_add(HostWorld* w, int n, const float* rec, int32_t* slots) {
+    PDev& P = w->P;
+    float pc = w->prm[SF_P_PCELL] > 0.1f ? w->prm[SF_P_PCELL] : 0.7f;
+    const float rmax = pc * 0.5f - 0.05f;
+    std::vector<int32_t> at;
+    std::vector<float4> rx, rv, rq, rs, rw, rc, ra;
+    int added = 0;
+    for (int k = 0; k < n; k++) {
+        const float* f = rec + (size_t)8 * k;
+        bool fin = true;
+        for (int a = 0; a < 8; a++) if (!std::isfinite(f[a])) fin = false;
+        int i = -1;
+        if (fin) {
+            if (!w->pfree.empty()) { i = w->pfree.back(); w->pfree.pop_back(); }
+            else if (P.np < P.cap) i = P.np++;
+        }
+        slots[k] = i;
+        if (i < 0) continue;
+        const float r = rmax < f[6] ? rmax : f[6];
+        at.push_back(i);
+        rx.push_back(make_float4(f[0], f[1], f[2], r));
+        rv.push_back(make_float4(f[3], f[4], f[5], 1.0f));
+        rq.push_back(make_float4(0.0f, 0.0f, 0.0f, 1.0f));
+        rs.push_back(make_float4(f[0], f[1], f[2], 0.0f));
+        rw.push_back(make_float4(0.0f, 0.0f, 0.0f, f[7]));
+        rc.push_back(make_float4(f[0], f[1], f[2], 0.0f));
+        ra.push_back(make_float4(f[0], f[1], f[2], 0.0f));
+        added++;
+    }
+    SF_HIP(hipStreamSynchronize(w->st));
+    const uint32_t zero = 0u;
+    for (size_t t = 0; t < at.size(); t++) {
+        const int i = at[t];
+        SF_HIP(hipMemcpy(P.x + i, &rx[t], sizeof(float4), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.v + i, &rv[t], sizeof(float4), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.q + i, &rq[t], sizeof(float4), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.s + i, &rs[t], sizeof(float4), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.w + i, &rw[t], sizeof(float4), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.c + i, &rc[t], sizeof(float4), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.a + i, &ra[t], sizeof(float4), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.gl + i, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P.gn + i, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    return added;
+}
+// the stage's launches, after the body step (on the world's stream)
+int particle_stage(HostWorld* w, float dt) {
+    PDev& P = w->P;
+    const int nbl = (int)std::min<size_t>(w->pblast.size(), 3);
+    PStepK K{};
+    int psub = (int)w->prm[SF_P_PSUB]; if (psub < 1) psub = 1; if (psub > 16) psub = 16;
+    K.dt = dt; K.h = dt / (float)psub; K.psub = psub;
+    K.pc = w->prm[SF_P_PCELL] > 0.1f ? w->prm[SF_P_PCELL] : 0.7f;
+    K.mug = w->prm[SF_P_FRICTION]; K.gx = w->prm[SF_P_GX]; K.gy = w->prm[SF_P_GY]; K.gz = w->prm[SF_P_GZ];
+    K.cell = w->prm[SF_P_CELL];
+    K.nblast = nbl; K.wake = w->pwake ? 1 : 0;
+    for (int b = 0; b < nbl; b++) {
+        const auto& e = w->pblast[b];
+        K.bc[b] = make_float4(e[0], e[1], e[2], e[3]);
+        K.bs[b] = e[4];
+    }
+    w->pblast.erase(w->pblast.begin(), w->pblast.begin() + nbl);
+    w->pwake = false;
+    w->pstats[0] = P.np; w->pstats[3] = (int32_t)w->pfree.size();
+    if (!P.np) { w->pstats[1] = 0; w->pstats[2] = 0; return 0; }
+    P.G = p_grid_size(P.np); P.gmask = P.G - 1u;
+    hipStream_t s = w->st;
+    const Dev& D = w->D;
+    w->ptimed = false;
+    if (w->timing) {
+        for (auto& e : w->pev) if (!e) SF_HIP(hipEventCreate(&e));
+        SF_HIP(hipEventRecord(w->pev[0], s));
+    }
+    const unsigned gp = sf_grid((uint64_t)P.np);
+    hipLaunchKernelGGL(k_fill_u32, dim3(sf_grid(P.G)), dim3(NT), 0, s, P.gcnt, 0u, (size_t)P.G);
+    hipLaunchKernelGGL(k_fill_u32, dim3(sf_grid(P.G)), dim3(NT), 0, s, P.gfill, 0u, (size_t)P.G);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, P.ctr, 0u, (size_t)4);
+    hipLaunchKernelGGL(k_p_prep, dim3(gp), dim3(NT), 0, s, D, P, K);
+    sf::scan_u32(P.gcnt, P.gstart, nullptr, nullptr, P.G, P.G, w->pscan, s);
+    hipLaunchKernelGGL(k_p_scatter, dim3(gp), dim3(NT), 0, s, P);
+    hipLaunchKernelGGL(k_p_sort, dim3(sf_grid(P.G)), dim3(NT), 0, s, P);
+    hipLaunchKernelGGL(k_p_gather, dim3(gp), dim3(NT), 0, s, D, P, K);
+    for (int k = 0; k < psub; k++) {
+        hipLaunchKernelGGL(k_p_integrate, dim3(gp), dim3(NT), 0, s, P, K);
+        hipLaunchKernelGGL(k_p_contact, dim3(gp), dim3(NT), 0, s, D, P, K);
+        hipLaunchKernelGGL(k_p_apply, dim3(gp), dim3(NT), 0, s, P, K);
+    }
+    hipLaunchKernelGGL(k_p_finish, dim3(gp), dim3(NT), 0, s, D, P, K);
+    if (w->timing) { SF_HIP(hipEventRecord(w->pev[1], s)); w->ptimed = true; }
+    SF_HIP(hipMemcpyAsync(w->h_pctr, P.ctr, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, s));
+    SF_HIP(hipGetLastError());
+    return 1;  // (stats read after the step's synchronisation)
+}
+
 int world_step(HostWorld* w, float dt) {
     if (!(dt >= 0.0f) || dt > 1.0f) { sf::set_error(SF_E_ARG, "dt out of range"); return SF_E_ARG; }
     SF_HIP(hipSetDevice(w->dev));
@@ -2434,7 +2878,12 @@ int world_step(HostWorld* w, float dt) {
     if (r) return r;
     r = step_back(w, K);
     if (r) return r;
+    const int pst = particle_stage(w, dt);
+    if (pst < 0) return pst;
     SF_HIP(hipStreamSynchronize(w->st));
+    if (pst > 0) { w->pstats[1] = (int32_t)w->h_pctr[0]; w->pstats[2] = (int32_t)w->h_pctr[1]; }
+    w->pstage_ms = 0.0f;
+    if (w->ptimed) SF_HIP(hipEventElapsedTime(&w->pstage_ms, w->pev[0], w->pev[1]));
     w->steps++;
     finish_stats(w);
     if (w->timing) {
@@ -2671,6 +3120,64 @@ const float* sf_world_pose(sf_world* w) {
         o[7] = h->h_h[3 * i]; o[8] = h->h_h[3 * i + 1]; o[9] = h->h_h[3 * i + 2];
     }
     return h->pose.data();
+}
+// ---- particle debris (gpu/gpu_world.js addParticles/killParticle/blastParticles/readParticles :2870-2896)
+int sf_world_particles_capacity(sf_world* w, int cap) {
+    if (!w || cap < 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    SF_HIP(hipSetDevice(w->h.dev));
+    return particles_alloc(&w->h, cap);
+}
+int sf_world_add_particles(sf_world* w, int n, const float* rec8, int32_t* slots) {
+    if (!w || n < 0 || (n && (!rec8 || !slots))) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    SF_HIP(hipSetDevice(w->h.dev));
+    return particles_add(&w->h, n, rec8, slots);
+}
+int sf_world_kill_particle(sf_world* w, int i) {
+    if (!w || i < 0 || i >= w->h.P.np) { sf::set_error(SF_E_ARG, "no such particle"); return SF_E_ARG; }
+    HostWorld* h = &w->h;
+    SF_HIP(hipSetDevice(h->dev));
+    SF_HIP(hipStreamSynchronize(h->st));
+    float4 v4;
+    SF_HIP(hipMemcpy(&v4, h->P.v + i, sizeof(float4), hipMemcpyDeviceToHost));
+    if (v4.w == 0.0f) return 0;
+    v4.w = 0.0f;
+    SF_HIP(hipMemcpy(h->P.v + i, &v4, sizeof(float4), hipMemcpyHostToDevice));
+    h->pfree.push_back(i);
+    return 0;
+}
+int sf_world_blast_particles(sf_world* w, float cx, float cy, float cz, float reach, float speed) {
+    if (!w) return SF_E_ARG;
+    if (w->h.pblast.size() < 16) w->h.pblast.push_back({cx, cy, cz, reach, speed});
+    return 0;
+}
+int sf_world_wake_particles(sf_world* w) { if (!w) return SF_E_ARG; w->h.pwake = true; return 0; }
+int sf_world_read_particles(sf_world* w, float* out12) {
+    if (!w) return SF_E_ARG;
+    HostWorld* h = &w->h;
+    const int n = h->P.np;
+    if (!out12 || !n) return n;
+    SF_HIP(hipSetDevice(h->dev));
+    SF_HIP(hipStreamSynchronize(h->st));
+    std::vector<float4> x(n), v(n), q(n);
+    SF_HIP(hipMemcpy(x.data(), h->P.x, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(v.data(), h->P.v, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    SF_HIP(hipMemcpy(q.data(), h->P.q, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        float* o = out12 + (size_t)12 * i;
+        o[0] = x[i].x; o[1] = x[i].y; o[2] = x[i].z; o[3] = v[i].x; o[4] = v[i].y; o[5] = v[i].z;
+        o[6] = q[i].x; o[7] = q[i].y; o[8] = q[i].z; o[9] = q[i].w; o[10] = v[i].w; o[11] = x[i].w;
+    }
+    return n;
+}
+int sf_world_particle_ms(sf_world* w, float* ms) {
+    if (!w || !ms) return SF_E_ARG;
+    *ms = w->h.pstage_ms;
+    return 0;
+}
+int sf_world_particle_stats(sf_world* w, int32_t* out4) {
+    if (!w || !out4) return SF_E_ARG;
+    for (int k = 0; k < 4; k++) out4[k] = w->h.pstats[k];
+    return 0;
 }
 int sf_world_set_timing(sf_world* w, int on) { if (!w) return SF_E_ARG; w->h.timing = on != 0; return 0; }
 int sf_world_set_graph(sf_world* w, int on) { if (!w) return SF_E_ARG; w->h.use_graph = on != 0; return 0; }

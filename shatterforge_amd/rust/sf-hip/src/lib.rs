@@ -67,6 +67,14 @@ pub mod ffi {
         pub fn sf_world_read_bodies(w: *mut SfWorld, out: *mut f32) -> c_int;
         pub fn sf_world_nbody(w: *mut SfWorld) -> c_int;
         pub fn sf_world_pose(w: *mut SfWorld) -> *const f32;
+        pub fn sf_world_particles_capacity(w: *mut SfWorld, cap: c_int) -> c_int;
+        pub fn sf_world_add_particles(w: *mut SfWorld, n: c_int, rec8: *const f32, slots: *mut i32) -> c_int;
+        pub fn sf_world_kill_particle(w: *mut SfWorld, i: c_int) -> c_int;
+        pub fn sf_world_blast_particles(w: *mut SfWorld, cx: f32, cy: f32, cz: f32, reach: f32, speed: f32) -> c_int;
+        pub fn sf_world_wake_particles(w: *mut SfWorld) -> c_int;
+        pub fn sf_world_read_particles(w: *mut SfWorld, out12: *mut f32) -> c_int;
+        pub fn sf_world_particle_stats(w: *mut SfWorld, out4: *mut i32) -> c_int;
+        pub fn sf_world_particle_ms(w: *mut SfWorld, ms: *mut f32) -> c_int;
         pub fn sf_pile_setup(n: u32, substeps: u32);
         pub fn sf_stack_setup(n: u32, substeps: u32, sleep: u32);
         pub fn sf_step(dt: f32) -> u32;

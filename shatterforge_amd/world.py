@@ -15,8 +15,8 @@ from ._native import lib, check, ptr
 P = dict(GX=0, GY=1, GZ=2, SUBSTEPS=3, HERTZ=4, DAMPING=5, MAXPUSH=6, FRICTION=7, SPEC=8, SLEEP_LIN=9, SLEEP_ANG=10,
          SLEEP_TIME=11, SLEEP_DIST=12, SLEEP_ROT=13, ALLOW_SLEEP=14, CELL=15, LEAK=16, ANCHOR=17, ANCHOR_PUSH=18,
          ANCHOR_SLOP=19, COL_LIMIT=20, COL_ROUNDS=21, COL_SPREAD=22, TAIL_MIN=23, SOLVE_NT=24, RATIO=25, SHOCK=26,
-         REST_THRESHOLD=27, REST_ITERS=28, COL_SUPPORT=29)
-NPARAM = 30
+         REST_THRESHOLD=27, REST_ITERS=28, COL_SUPPORT=29, PSUB=30, PCELL=31)
+NPARAM = 32
 # the GPU world's solver extensions (gpu/gpu_world.js:1152-1203, :954-985): mass-ratio limit, shock-propagation level
 # passes, supports-first colouring rounds; params(**GPU_WORLD) turns them on
 GPU_WORLD = dict(ratio=20.0, shock=6, col_support=2)
@@ -53,6 +53,8 @@ def default_params():
     p[P["SOLVE_NT"]] = 64
     p[P["REST_THRESHOLD"]] = 1.0
     p[P["REST_ITERS"]] = 8
+    p[P["PSUB"]] = 4
+    p[P["PCELL"]] = 0.7
     return p
 
 
@@ -205,6 +207,46 @@ class World:
         out = np.zeros(5, np.float32)
         check(lib().sf_world_stage_ms(self._h, ptr(out)))
         return dict(zip(["broad", "narrow", "colour", "solve", "sleep"], out.tolist()))
+
+    # ---- particle debris (gpu/gpu_world.js addParticles/killParticle/blastParticles/readParticles :2863-2896)
+    def particles_capacity(self, cap):
+        check(lib().sf_world_particles_capacity(self._h, int(cap)))
+
+    def add_particles(self, rec):
+        """rec: [n, 8] float32 (p, v, radius, friction); returns the slots (-1: full or not finite)"""
+        rec = np.ascontiguousarray(rec, np.float32).reshape(-1, 8)
+        slots = np.zeros(len(rec), np.int32)
+        check(lib().sf_world_add_particles(self._h, len(rec), ptr(rec), ptr(slots)))
+        return slots
+
+    def kill_particle(self, i):
+        check(lib().sf_world_kill_particle(self._h, int(i)))
+
+    def blast_particles(self, c, reach, speed):
+        check(lib().sf_world_blast_particles(self._h, float(c[0]), float(c[1]), float(c[2]), float(reach), float(speed)))
+
+    def wake_particles(self):
+        check(lib().sf_world_wake_particles(self._h))
+
+    def particles(self):
+        """[slots, 12]: x y z vx, vy vz qx qy, qz qw state radius"""
+        n = check(lib().sf_world_read_particles(self._h, None))
+        out = np.zeros((max(n, 1), 12), np.float32)
+        if n:
+            check(lib().sf_world_read_particles(self._h, ptr(out)))
+        return out[:n]
+
+    def particle_ms(self):
+        """device ms of the last step's particle stage (timing on)"""
+        ms = np.zeros(1, np.float32)
+        check(lib().sf_world_particle_ms(self._h, ptr(ms)))
+        return float(ms[0])
+
+    def particle_stats(self):
+        """[slots in use, moving at the last step's start, asleep at its end, free slots]"""
+        out = np.zeros(4, np.int32)
+        check(lib().sf_world_particle_stats(self._h, ptr(out)))
+        return out
 
 
 # ---- the reference's singleton exports (engine/src/wasm.rs:27-53)
