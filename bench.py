@@ -148,7 +148,7 @@ def bench_lattice(frames=120, batches=(32, 256), cpu=True):
     """legacy brick frame: the plain C1 tower collapse (with the oracle on the same frames); batches of what-if scenes
     in one world; every scene of the batches also stepped in a world of its own (one launch sequence per scene) over
     the same frames, so batched and plain cover the same scenes and the same frame window"""
-    from shatterforge_amd import scenes, serve
+    from shatterforge_amd import autotune, scenes, serve
     from shatterforge_amd.lattice import Lattice
     from shatterforge_amd.load import LoadBatch
     out = {}
@@ -213,16 +213,28 @@ def bench_lattice(frames=120, batches=(32, 256), cpu=True):
     for n_scenes in batches:
         qs = qs_all[:n_scenes]
         caps = [q["bricks"] + 64 for q in qs]
-        B = Lattice(sum(caps), 4096 * n_scenes, scenes=n_scenes)
-        slots = [B.add_scene(caps[k], 4096, structs=2, seed=qs[k]["rng_seed"]) for k in range(n_scenes)]
-        setup(B, slots, qs)
-        B.step(1.0 / 60.0)
+        def batch():
+            W = Lattice(sum(caps), 4096 * n_scenes, scenes=n_scenes)
+            sl = [W.add_scene(caps[k], 4096, structs=2, seed=qs[k]["rng_seed"]) for k in range(n_scenes)]
+            setup(W, sl, qs)
+            W.step(1.0 / 60.0)
+            return W
+        # the launch-shape tuner on a copy of the batch (the knobs change no result), its best applied to the
+        # measured world, which then steps the same frames as the plain worlds
+        T = batch()
+        tune = autotune.tune_lattice(T, rounds=8, apply=False)
+        T.close()
+        B = batch()
+        autotune.apply_lattice(B, tune["best"])
         el, split = _frame_split(B, frames)
         nbr = sum(q["bricks"] for q in qs)
         plain = nbr * frames / sum(plain_t[:n_scenes])
         rows.append({"scenes": n_scenes, "bricks": nbr, "frames": frames, "ms_per_frame": el * 1e3 / frames,
                      "brick_steps_per_s": nbr * frames / el, "plain_same_scenes_brick_steps_per_s": plain,
-                     "batched_over_plain": nbr * frames / el / plain, "split_ms": split})
+                     "batched_over_plain": nbr * frames / el / plain, "split_ms": split,
+                     "tuned": {"best": tune["best"], "default_ms": next(
+                         (r["ms"] for r in tune["candidates"] if r["fused"] == 32768 and r["sor_lds"] == -1), None),
+                               "candidates": len(tune["candidates"]), "frames": tune["frames"]}})
         B.close()
         log(f"lattice batched {n_scenes} scenes ({nbr} bricks): {el * 1e3 / frames:.2f} ms/frame, "
             f"{nbr * frames / el:.3e} brick-steps/s; the same scenes one world each {plain:.3e}")

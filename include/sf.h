@@ -292,6 +292,9 @@ int sf_lat_subphase_ms(sf_lat* L, double* out16);
  * (default 32768), else as colour-by-colour launches over all scenes; 0: always the latter. Both give the same
  * bits (no counterpart in the reference: the solve route is a device choice) */
 int sf_lat_set_fused(sf_lat* L, int max_contacts);
+/* worker threads of the frame's per-scene host bookkeeping (body rebuilds, connectivity), 1-16 (0: SF_HOST_THREADS,
+ * else the cores, at most 16); results do not depend on it (no counterpart in the reference: a host choice) */
+int sf_lat_set_host_threads(sf_lat* L, int n);
 /* structures of at most max_bricks bricks are swept with their potential in the workgroup's LDS from packed 16-byte
  * stencil rows; larger ones from the full neighbour and conductance rows with the potential in global memory.
  * -1 (the default) takes the device's LDS limit (about 40k bricks on gfx950); 0 sends every structure down the

@@ -1,4 +1,4 @@
-"""Launch-shape tuner for the box step.
+"""Launch-shape tuners for the box step and the batched brick frame.
 
 The knobs it tunes change how the step's work is spread over the GPU, never what is computed: threads per
 workgroup of the colour-bucket solve (one manifold per thread; within a colour no two manifolds share a moving body,
@@ -10,6 +10,11 @@ on that one trajectory (it does not depend on the knobs), in the opposite order 
 the step cost along the trajectory does not favour the candidates that come last; each visit times only its last
 step, so a graph capture on switching is not counted. The table (per candidate, the median step time)
 is returned and can be written as JSON and applied to a world later.
+
+The brick frame's tuner works on the caller's own batched world (its scenes as they are): the contact solve's route
+(one workgroup per scene up to a contact count, else colour-by-colour launches over all scenes) and the structure
+sweeps' route (potential in LDS from packed rows up to the device's limit, or every structure from the full rows in
+global memory). Neither changes the results.
 """
 import itertools
 import json
@@ -76,3 +81,42 @@ def load(path):
     """A table written by tune_box."""
     with open(path) as f:
         return json.load(f)
+
+
+LATTICE_CANDIDATES = {"fused": (0, 32768, 1 << 30), "sor_lds": (-1, 0)}
+LATTICE_DEFAULT = {"fused": 32768, "sor_lds": -1}
+
+
+def apply_lattice(lat, best):
+    """Set a tuned configuration (a row of tune_lattice's table) on a Lattice."""
+    lat.set_fused(int(best["fused"]))
+    lat.set_sor_lds(int(best["sor_lds"]))
+
+
+def tune_lattice(lat, rounds=8, dt=1.0 / 60.0, path=None, apply=True):
+    """Time every candidate on the Lattice `lat` as it is: each round steps one frame per candidate (in the opposite
+    order every other round), so every candidate samples frames spread over the same stretch of the trajectory (a
+    frame's cost follows the fracture going on, which one frame per candidate would mistake for the candidate's); the
+    median over the rounds decides. The scenes advance rounds x candidates frames, the same frames whatever the
+    candidates. Returns the table and leaves the best applied (the default when apply is false). The worker threads
+    of the host bookkeeping (set_host_threads) are not tuned: they matter only on frames with fracture work, and a
+    change rebuilds the thread pool."""
+    names = list(LATTICE_CANDIDATES)
+    combos = [dict(zip(names, v)) for v in itertools.product(*(LATTICE_CANDIDATES[k] for k in names))]
+    times = [[] for _ in combos]
+    for rnd in range(rounds):
+        order = range(len(combos)) if rnd % 2 == 0 else range(len(combos) - 1, -1, -1)
+        for k in order:
+            apply_lattice(lat, combos[k])
+            t0 = time.perf_counter()
+            lat.step(dt)
+            times[k].append(time.perf_counter() - t0)
+    rows = [dict(c, ms=float(np.median(t) * 1e3)) for c, t in zip(combos, times)]
+    best = min(rows, key=lambda r: r["ms"])
+    table = {"rounds": rounds, "frames": rounds * len(combos), "candidates": rows, "best": best,
+             "default": dict(LATTICE_DEFAULT)}
+    apply_lattice(lat, best if apply else LATTICE_DEFAULT)
+    if path:
+        with open(path, "w") as f:
+            json.dump(table, f, indent=1)
+    return table

@@ -2039,6 +2039,7 @@ struct Lat {
     std::atomic<bool> ub_dirty{true};  // a scene's body high-water mark moved: the used-slot list is rebuilt
     // host workers: per-slot scratch (slot 0 is the calling thread's)
     std::unique_ptr<Pool> pool;
+    int host_threads = 0;  // worker pool size (0: SF_HOST_THREADS, else the cores, at most 16)
     std::vector<std::unique_ptr<Scratch>> scr;
     std::vector<uint32_t> ubody;
     // device
@@ -3334,6 +3335,7 @@ Pool* host_pool(Lat* L) {
     if (!L->pool) {
         int n = (int)std::thread::hardware_concurrency();
         if (const char* e = getenv("SF_HOST_THREADS")) n = atoi(e);
+        if (L->host_threads > 0) n = L->host_threads;
         n = std::max(1, std::min(n, 16));
         if (n < 2) return nullptr;
         L->pool.reset(new Pool(n - 1));
@@ -4031,6 +4033,15 @@ int sf_lat_timing(sf_lat* h, double* ms_frame, int32_t* syncs) {
 int sf_lat_subphase_ms(sf_lat* h, double* out16) {
     if (!h || !out16) return SF_E_ARG;
     for (int k = 0; k < 16; k++) out16[k] = h->h.L.sp[k];
+    return 0;
+}
+int sf_lat_set_host_threads(sf_lat* h, int n) {
+    if (!h || n < 0 || n > 16) { sf::set_error(SF_E_ARG, "host threads: 0 (default) to 16"); return SF_E_ARG; }
+    Lat& L = h->h.L;
+    if (n == L.host_threads) return 0;
+    L.pool.reset();
+    L.scr.clear();
+    L.host_threads = n;
     return 0;
 }
 int sf_lat_set_fused(sf_lat* h, int max_contacts) {

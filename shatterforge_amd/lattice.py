@@ -144,6 +144,10 @@ class Lattice:
         """scenes of at most max_contacts contacts are solved by one workgroup each; 0: colour-by-colour launches"""
         check(lib().sf_lat_set_fused(self._h, int(max_contacts)))
 
+    def set_host_threads(self, n):
+        """worker threads of the per-scene host bookkeeping (0: the default); results do not depend on it"""
+        check(lib().sf_lat_set_host_threads(self._h, int(n)))
+
     def set_sor_lds(self, max_bricks):
         """structures of at most max_bricks bricks are swept with their potential in LDS (packed rows); -1: the
         device's limit, 0: every structure from the full rows with the potential in global memory"""

@@ -147,3 +147,44 @@ def test_batched_scenes_same_alone_and_in_mix(oracle):
                 A, a = alone[k]
                 assert np.array_equal(G.bodies(sc[k]), A.bodies(a), equal_nan=True), f"frame {f} scene {k}: mix vs alone bodies"
                 assert np.array_equal(G.bricks(sc[k]), A.bricks(a), equal_nan=True), f"frame {f} scene {k}: mix vs alone bricks"
+
+
+def test_lattice_tuner_changes_no_result(oracle):
+    # the batched frame's tuner steps one world through every candidate (solve route, sweep route), one frame each
+    # per round; the world it leaves equals, bit for bit, the oracle and a world stepped the same frames with the
+    # defaults
+    from shatterforge_amd import autotune
+    print("case: 4 scenes batched, the tuner's 6 candidates x 4 rounds, vs the defaults and the oracle", flush=True)
+    caps = [lc.brick_count(c) + 8 for c in MIX]
+    worlds = []
+    for _ in range(2):
+        W = Lattice(sum(caps), 4096 * len(MIX), scenes=len(MIX))
+        sc = [W.add_scene(caps[k], 4096, structs=4, seed=0x9E3779B9) for k in range(len(MIX))]
+        for k, c in enumerate(MIX):
+            lc.build(W, sc[k], c)
+        worlds.append((W, sc))
+    Os = [oracle.LatticeOracle(order=oracle.ORDER_DEV, brick_cap=caps[k], body_cap=4096) for k in range(len(MIX))]
+    for k, c in enumerate(MIX):
+        lc.build(Os[k], None, c)
+    for W, sc in worlds:
+        W.step(DT)
+        for k, c in enumerate(MIX):
+            for ch in lc.charges(c):
+                W.explode(sc[k], *ch)
+    for k, c in enumerate(MIX):
+        Os[k].step(DT)
+        for ch in lc.charges(c):
+            Os[k].explode(*ch)
+    (T, st), (D, sd) = worlds
+    table = autotune.tune_lattice(T, rounds=4)
+    n = table["frames"]
+    assert n == 24
+    for _ in range(n):
+        D.step(DT)
+        for O in Os:
+            O.step(DT)
+    for k in range(len(MIX)):
+        assert np.array_equal(T.bodies(st[k]), D.bodies(sd[k]), equal_nan=True), f"scene {k}: tuned vs default bodies"
+        assert np.array_equal(T.bricks(st[k]), D.bricks(sd[k]), equal_nan=True), f"scene {k}: tuned vs default bricks"
+        _assert_same(T, st[k], Os[k], f"scene {k} vs oracle after the tuner")
+    assert table["best"]["ms"] > 0.0
