@@ -39,6 +39,16 @@ def lib():
             "ol_sor_sweep": (cf, [vp, vp, cf]), "ol_solve64": (ci, [vp, vp, cd, ci]), "ol_residual64": (cd, [vp, vp]),
             "ol_verdicts64": (None, [vp, vp, vp, vp, vp]), "ol_verdicts32": (None, [vp, vp, vp, vp]),
             "ol_grounded": (ci, [vp, vp]), "ol_finalize": (None, [ci, vp, vp, vp, vp, vp]),
+            "os_create": (vp, []), "os_destroy": (None, [vp]), "os_len": (ci, [vp]),
+            "os_add": (ci, [vp, vp, vp, ci, ci, ci]), "os_set_flags": (None, [vp, ci, ci]), "os_flags": (ci, [vp, ci]),
+            "os_remove": (None, [vp, ci]), "os_build_joints": (ci, [vp]), "os_njoint_slots": (ci, [vp]),
+            "os_live_joints": (ci, [vp]), "os_joints": (ci, [vp, vp, ci]), "os_set_joint_damage": (None, [vp, ci, ci]),
+            "os_mass": (cf, [vp, ci]), "os_warm_create": (vp, []), "os_warm_destroy": (None, [vp]),
+            "os_warm_size": (ci, [vp]),
+            "os_analyse": (vp, [vp, ci, vp, vp, vp, vp, ci, vp, ci, vp]), "os_result_free": (None, [vp]),
+            "os_result_scalars": (None, [vp, vp]), "os_result_levels": (None, [vp, vp]),
+            "os_result_joints": (ci, [vp, vp, vp, ci]), "os_result_inner": (ci, [vp, vp, vp, ci]),
+            "os_result_nodes": (ci, [vp, vp, vp, vp, vp, ci]), "os_result_springs": (ci, [vp, vp, vp, ci]),
         }
         for k, (r, a) in protos.items():
             f = getattr(L, k)
@@ -379,3 +389,105 @@ class LatticeOracle:
 
     def set_asleep(self, b):
         lib().ll_set_asleep(self._h, int(b))
+
+
+ORDER_DEVICE = 1
+
+
+class BricksOracle:
+    """The brick store of engine/src/brick.rs on the CPU (whole boxes on the 0.125 m tick grid, joints as shared
+    faces), for the structural analysis oracle."""
+
+    def __init__(self):
+        self._h = lib().os_create()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().os_destroy(self._h)
+            self._h = None
+
+    def __len__(self):
+        return lib().os_len(self._h)
+
+    def add(self, mn, size, mat, paint=0, structure=0):
+        mn = np.ascontiguousarray(mn, np.int32)
+        size = np.ascontiguousarray(size, np.int32)
+        i = lib().os_add(self._h, _p(mn), _p(size), int(mat), int(paint), int(structure))
+        if i < 0:
+            raise ValueError(f"brick size {size.tolist()} outside 1..255 ticks")
+        return i
+
+    def set_flags(self, i, flags):
+        lib().os_set_flags(self._h, int(i), int(flags))
+
+    def flags(self, i):
+        return lib().os_flags(self._h, int(i))
+
+    def remove(self, i):
+        lib().os_remove(self._h, int(i))
+
+    def build_joints(self):
+        return lib().os_build_joints(self._h)
+
+    def joints(self):
+        n = lib().os_njoint_slots(self._h)
+        out = np.zeros((max(n, 1), 5), np.int32)
+        lib().os_joints(self._h, _p(out), n)
+        return out[:n]
+
+    def set_joint_damage(self, j, dmg):
+        lib().os_set_joint_damage(self._h, int(j), int(dmg))
+
+    def mass(self, i):
+        return np.float32(lib().os_mass(self._h, int(i)))
+
+
+class WarmOracle:
+    def __init__(self):
+        self._h = lib().os_warm_create()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().os_warm_destroy(self._h)
+            self._h = None
+
+    def __len__(self):
+        return lib().os_warm_size(self._h)
+
+
+def analyse_oracle(b, set_, load, sink, dir3=(0.0, -1.0, 0.0), point=(), order=ORDER_REFERENCE, warm=None):
+    """structure.rs analyse_warm on the CPU; a dict of the analysis, plus the cells, displacements and springs."""
+    set_ = np.ascontiguousarray(set_, np.uint32)
+    load = np.ascontiguousarray(load, np.float32)
+    sink = np.ascontiguousarray(sink, np.uint8)
+    d = np.ascontiguousarray(dir3, np.float32)
+    pt = np.ascontiguousarray(point, np.float32).reshape(-1, 4)
+    L = lib()
+    R = L.os_analyse(b._h, len(set_), _p(set_), _p(load), _p(sink), _p(d), len(pt), _p(pt) if len(pt) else None,
+                     int(order), warm._h if warm is not None else None)
+    try:
+        sc = np.zeros(12, np.float64)
+        L.os_result_scalars(R, _p(sc))
+        lv = np.zeros(8, np.int32)
+        L.os_result_levels(R, _p(lv))
+        nj, ni, n, nl = int(sc[10]), int(sc[11]), int(sc[2]), int(sc[3])
+        jid = np.zeros(max(nj, 1), np.int32)
+        jr = np.zeros(max(nj, 1), np.float32)
+        L.os_result_joints(R, _p(jid), _p(jr), nj)
+        bat = np.zeros((max(ni, 1), 3), np.int32)
+        ir = np.zeros(max(ni, 1), np.float32)
+        L.os_result_inner(R, _p(bat), _p(ir), ni)
+        node = np.zeros((max(n, 1), 7), np.int32)
+        nload = np.zeros(max(n, 1), np.float32)
+        fixed = np.zeros(max(n, 1), np.uint8)
+        x = np.zeros((max(n, 1), 6), np.float64)
+        L.os_result_nodes(R, _p(node), _p(nload), _p(fixed), _p(x), n)
+        ends = np.zeros((max(nl, 1), 2), np.uint32)
+        sp = np.zeros((max(nl, 1), 12), np.float64)
+        L.os_result_springs(R, _p(ends), _p(sp), nl)
+    finally:
+        L.os_result_free(R)
+    return dict(max_ratio=np.float32(sc[0]), iterations=int(sc[1]), nodes=n, links=nl, levels=lv[:int(sc[4])].tolist(),
+                worst_in=(np.float32(sc[5]), int(sc[6]), int(sc[7]), np.float32(sc[8]), np.float32(sc[9])),
+                joint_ids=jid[:nj], joint_ratios=jr[:nj], inner=bat[:ni], inner_ratios=ir[:ni],
+                node=node[:n], node_load=nload[:n], fixed=fixed[:n].astype(bool), x=x[:n], ends=ends[:nl], springs=sp[:nl])

@@ -148,6 +148,46 @@ void ll_probe_structural(LW* w, float dt);
 void ll_set_nc(LW* w, int b, uint32_t nc);
 void ll_set_asleep(LW* w, int b);
 
+/* ---- spring-network structural analysis (engine/src/structure.rs:191-783) over a brick store (engine/src/brick.rs) */
+typedef struct os_store os_store;
+typedef struct os_result os_result;
+typedef struct os_warm os_warm;
+/* dot-product order: element after element (the reference), or as the device's one-workgroup solve sums */
+enum { OS_ORDER_REFERENCE = 0, OS_ORDER_DEVICE = 1 };
+os_store* os_create(void);
+void os_destroy(os_store* b);
+int os_len(const os_store* b);
+/* a brick: min corner and size in ticks (0.125 m, sizes 1..255), material (0 concrete, 1 masonry, 2 steel, 3 glass,
+ * 4 timber), paint, structure; returns its index (-1: bad size) */
+int os_add(os_store* b, const int32_t* min3, const int32_t* size3, int mat, int paint, int structure);
+/* flags: 1 alive, 2 infill, 4 dynamic, 8 cast monolithically */
+void os_set_flags(os_store* b, int i, int flags);
+int os_flags(const os_store* b, int i);
+void os_remove(os_store* b, int i);
+int os_build_joints(os_store* b);
+int os_njoint_slots(const os_store* b);
+int os_live_joints(const os_store* b);
+/* per joint slot: ja, jb (-1 for a free slot), axis, area in tick^2, damage */
+int os_joints(const os_store* b, int32_t* out5, int cap);
+void os_set_joint_damage(os_store* b, int j, int dmg);
+float os_mass(const os_store* b, int i);
+os_warm* os_warm_create(void);
+void os_warm_destroy(os_warm* w);
+int os_warm_size(const os_warm* w);
+/* analyse the bricks `set` under per-brick loads (N) along dir, supported at the `sink` bricks; point loads as
+ * (index into set, x, z in ticks, N); warm: the last solution to start from and to keep (may be NULL) */
+os_result* os_analyse(const os_store* b, int n, const uint32_t* set, const float* load, const uint8_t* sink,
+                      const float* dir3, int npoint, const float* point4, int order, os_warm* warm);
+void os_result_free(os_result* r);
+/* [max_ratio, iterations, nodes, links, levels, worst-in ratio, brick, axis, flow, moment^2, joints checked, inner
+ *  faces checked] */
+void os_result_scalars(const os_result* r, double* out12);
+void os_result_levels(const os_result* r, int32_t* out8);
+int os_result_joints(const os_result* r, int32_t* jid, float* ratio, int cap);
+int os_result_inner(const os_result* r, int32_t* bat3, float* ratio, int cap);
+int os_result_nodes(const os_result* r, int32_t* node7, float* load, uint8_t* fixed, double* x6, int cap);
+int os_result_springs(const os_result* r, uint32_t* ends2, double* sp12, int cap);
+
 #ifdef __cplusplus
 }
 #endif
