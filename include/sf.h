@@ -367,6 +367,69 @@ int sf_serve_status(sf_serve* S, int64_t* out8);
 /* the server's lattice handle (read-only use: statistics, timing) */
 sf_lat* sf_serve_world(sf_serve* S);
 
+/* ---- structural analysis: the spring-network solve of the new engine (engine/src/structure.rs:191-783) over its
+ * brick store (engine/src/brick.rs). A brick is a box on the 0.125 m tick grid (min corner and size in ticks, sizes
+ * 1..255); joints are the faces two bricks share, with exact areas. Materials: 0 concrete, 1 masonry, 2 steel,
+ * 3 glass, 4 timber; paint as the reference's elements (0 slab, 1 column, 3 core, 8 timber floor, ...); flags:
+ * 1 alive, 2 infill, 4 dynamic, 8 cast monolithically. Whole boxes only: the fracture cells (convex hulls) of the
+ * reference's store are not represented. */
+typedef struct sf_bricks sf_bricks;
+typedef struct sf_warm sf_warm;
+typedef struct sf_net sf_net;
+/* Bricks::new / add (brick.rs:147-170); add returns the brick's index (SF_E_ARG for a size outside 1..255, a corner
+ * outside 16 bits or a material outside 0..4) */
+sf_bricks* sf_bricks_create(void);
+void sf_bricks_destroy(sf_bricks* b);
+int sf_bricks_len(const sf_bricks* b);
+int sf_bricks_add(sf_bricks* b, const int32_t* min3, const int32_t* size3, int mat, int paint, int structure);
+int sf_bricks_set_flags(sf_bricks* b, int i, int flags);
+int sf_bricks_flags(const sf_bricks* b, int i);
+/* structure.rs mass (:66-71): the brick's mass in kg (a steel member weighs its section) */
+float sf_bricks_mass(const sf_bricks* b, int i);
+/* Bricks::remove (brick.rs:182-190): its joints go, the record stays dead */
+int sf_bricks_remove(sf_bricks* b, int i);
+/* Bricks::build_joints (brick.rs:447-458): every joint from geometry, numbered as the reference numbers them; returns
+ * the live joints */
+int sf_bricks_build_joints(sf_bricks* b);
+/* per joint slot: ja (min side), jb (-1 for a removed joint), axis, area in tick^2, damage; returns the slots */
+int sf_bricks_joints(const sf_bricks* b, int32_t* out5, int cap);
+/* a joint's damage byte (bit 0 cracked, bits 1..7 wear), as the game layer sets it */
+int sf_bricks_set_joint_damage(sf_bricks* b, int j, int dmg);
+/* structure.rs Warm (:178-189): the last solution of a load case, per cell; a case given one starts from it and keeps
+ * its own solution in it */
+sf_warm* sf_warm_create(void);
+void sf_warm_destroy(sf_warm* w);
+int sf_warm_size(const sf_warm* w);
+/* Analyses queued, solved together: each case is analyse_warm (structure.rs:191) of the bricks `set` (live, once
+ * each) under per-brick loads (N) along dir3, supported at the bricks whose sink is set, plus point loads of
+ * (index into set, x, z in ticks, N). sf_net_add returns the case's index; sf_net_solve builds every case's cells,
+ * links, springs and multigrid levels on host threads and runs all their conjugate-gradient solves in one launch
+ * (a 1,024-thread workgroup per case), then the face checks. The cases and their results stay until sf_net_clear; the
+ * bricks a case names must not change until then. */
+sf_net* sf_net_create(void);
+void sf_net_destroy(sf_net* N);
+int sf_net_add(sf_net* N, const sf_bricks* b, int n, const uint32_t* set, const float* load, const uint8_t* sink,
+               const float* dir3, int npoint, const float* point4, sf_warm* warm);
+int sf_net_solve(sf_net* N);
+int sf_net_clear(sf_net* N);
+int sf_net_count(const sf_net* N);
+/* host threads that build the cases (0: up to 16) */
+int sf_net_set_host_threads(sf_net* N, int n);
+/* Analysis (structure.rs:119-133): [max_ratio, iterations, cells, links, levels, worst inner face: ratio, brick, axis,
+ * flow N, moment^2, joints checked, inner faces checked] */
+int sf_net_result(const sf_net* N, int c, double* out12);
+/* cells per multigrid level; returns the levels */
+int sf_net_levels(const sf_net* N, int c, int32_t* out8);
+/* every joint checked, in the reference's order, with its stress over strength (> 1: it fails); returns the count */
+int sf_net_joints(const sf_net* N, int c, int32_t* jid, float* ratio, int cap);
+/* every face inside a brick checked: (brick, axis, plane in ticks) and its ratio (> 1: it cracks there, > 0.7: it
+ * wears); returns the count */
+int sf_net_inner(const sf_net* N, int c, int32_t* bat3, float* ratio, int cap);
+/* the cells (brick, lo[3], hi[3] in ticks) and their displacements and rotations (6 per cell) */
+int sf_net_cells(const sf_net* N, int c, int32_t* cell7, double* x6, int cap);
+/* [host build ms, upload and read-back ms, device solve ms, device checks ms] of the last sf_net_solve */
+int sf_net_ms(const sf_net* N, double* out4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,6 +138,33 @@ _PROTOS = {
     "sf_legacy_dbg_bigbody": (c_void_p, []),
     "sf_legacy_dbg_top": (c_void_p, [c_uint]),
     "sf_legacy_dbg_wake_all": (None, []),
+    "sf_bricks_create": (c_void_p, []),
+    "sf_bricks_destroy": (None, [c_void_p]),
+    "sf_bricks_len": (c_int, [c_void_p]),
+    "sf_bricks_add": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int]),
+    "sf_bricks_set_flags": (c_int, [c_void_p, c_int, c_int]),
+    "sf_bricks_flags": (c_int, [c_void_p, c_int]),
+    "sf_bricks_mass": (c_float, [c_void_p, c_int]),
+    "sf_bricks_remove": (c_int, [c_void_p, c_int]),
+    "sf_bricks_build_joints": (c_int, [c_void_p]),
+    "sf_bricks_joints": (c_int, [c_void_p, c_void_p, c_int]),
+    "sf_bricks_set_joint_damage": (c_int, [c_void_p, c_int, c_int]),
+    "sf_warm_create": (c_void_p, []),
+    "sf_warm_destroy": (None, [c_void_p]),
+    "sf_warm_size": (c_int, [c_void_p]),
+    "sf_net_create": (c_void_p, []),
+    "sf_net_destroy": (None, [c_void_p]),
+    "sf_net_add": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "sf_net_solve": (c_int, [c_void_p]),
+    "sf_net_clear": (c_int, [c_void_p]),
+    "sf_net_count": (c_int, [c_void_p]),
+    "sf_net_set_host_threads": (c_int, [c_void_p, c_int]),
+    "sf_net_result": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_net_levels": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_net_joints": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
+    "sf_net_inner": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
+    "sf_net_cells": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
+    "sf_net_ms": (c_int, [c_void_p, c_void_p]),
     "sf_legacy_world": (c_void_p, [c_void_p]),
     "sf_lat_subphase_ms": (c_int, [c_void_p, c_void_p]),
 }

@@ -174,3 +174,74 @@ def site_charges(structs, seed=1, sets=8, per_set=4, r=2.2, power=1.4):
             ch.append(((x - 320) * 0.5 + 0.25, 0.6, (z - 320) * 0.5 + 0.25, r, power))
         out.append(ch)
     return out
+
+
+def frame_tower(storeys=12, bays=3, kind=0, remove=(), bay=60, storey=30, slab=2, col=5):
+    """A frame building for the structural analysis (the new engine's brick store, ticks of 0.125 m), laid out after
+    engine/src/tower.rs: `bays` x `bays` bays of `bay` ticks, storeys of `storey` ticks (a `slab`-tick slab, then
+    columns `col` ticks square on the bay grid), a concrete core of 0.25 m walls round the centre bay with a door each
+    storey, glass infill on the facade between the columns, a roof slab. kind 0: concrete frame; 1: masonry columns;
+    2: timber frame. `remove`: ground-storey columns (i, k) on the grid left out. Slabs wider than 255 ticks are laid
+    as one brick per bay. Returns [(min, size, material, paint, flags)]; the ground slab rests on y = 0."""
+    CONCRETE, MASONRY, GLASS, TIMBER = 0, 1, 3, 4
+    INFILL = 2 | 1
+    w = bays * bay
+    colm = {0: CONCRETE, 1: MASONRY, 2: TIMBER}[kind]
+    slabm = TIMBER if kind == 2 else CONCRETE
+    out = []
+
+    def put(mn, size, m, paint=0, flags=1):
+        out.append((list(mn), list(size), m, paint, flags))
+
+    def slab_at(y0):
+        if w <= 255:
+            put([0, y0, 0], [w, slab, w], slabm, 0)
+        else:
+            for i in range(bays):
+                for k in range(bays):
+                    put([i * bay, y0, k * bay], [bay, slab, bay], slabm, 0)
+
+    grid = [min(i * bay, w - col) for i in range(bays + 1)]
+    c0 = (bays // 2) * bay
+    c1 = c0 + bay
+    for s in range(storeys):
+        y0 = s * storey
+        slab_at(y0)
+        ya, h = y0 + slab, storey - slab
+        for i, gx in enumerate(grid):
+            for k, gz in enumerate(grid):
+                if s == 0 and (i, k) in remove:
+                    continue
+                put([gx, ya, gz], [col, h, col], colm, 1)
+        # the core: south wall with a door, north wall, west and east walls between them
+        door = 8
+        mid = (c0 + col + c1) // 2
+        put([c0 + col, ya, c0], [mid - door // 2 - (c0 + col), h, 2], CONCRETE, 3)
+        put([mid + door // 2, ya, c0], [c1 - (mid + door // 2), h, 2], CONCRETE, 3)
+        put([c0 + col, ya, c1 - 2], [c1 - (c0 + col), h, 2], CONCRETE, 3)
+        put([c0, ya, c0 + col], [2, h, c1 - 2 - (c0 + col)], CONCRETE, 3)
+        put([c1 - 2, ya, c0 + col], [2, h, c1 - 2 - (c0 + col)], CONCRETE, 3)
+        # glass between the perimeter columns
+        for i in range(bays):
+            x0, x1 = grid[i] + col, grid[i + 1]
+            put([x0, ya, 0], [x1 - x0, h, 1], GLASS, 0, INFILL)
+            put([x0, ya, w - 1], [x1 - x0, h, 1], GLASS, 0, INFILL)
+            put([0, ya, x0], [1, h, x1 - x0], GLASS, 0, INFILL)
+            put([w - 1, ya, x0], [1, h, x1 - x0], GLASS, 0, INFILL)
+    slab_at(storeys * storey)
+    return out
+
+
+def frame_towers(count, seed=1, storeys=(4, 16), bays=(2, 3, 4)):
+    """`count` frame buildings of seeded sizes (storeys uniform in the range, bays from the list, kind from the three),
+    some with ground-storey columns left out; a list of (bricks, description)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        st = int(rng.integers(storeys[0], storeys[1] + 1))
+        nb = int(rng.choice(bays))
+        kind = int(rng.integers(0, 3))
+        nrem = int(rng.integers(0, 4))
+        rem = {(int(rng.integers(0, nb + 1)), int(rng.integers(0, nb + 1))) for _ in range(nrem)}
+        out.append((frame_tower(st, nb, kind, remove=rem), dict(storeys=st, bays=nb, kind=kind, removed=sorted(rem))))
+    return out
