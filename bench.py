@@ -376,6 +376,95 @@ def bench_particles(n=100_000, bodies=5000, steps=90, seed=2):
     return res
 
 
+def _bricks_of(bricks):
+    from shatterforge_amd import structure
+    b = structure.Bricks()
+    for mn, size, m, paint, fl in bricks:
+        i = b.add(mn, size, m, paint)
+        if fl != 1:
+            b.set_flags(i, fl)
+    b.build_joints()
+    return b
+
+
+def bench_structure(cpu=True, batch=256, seed=2, reps=3):
+    """structural analysis (the spring-network solve): a 12-storey frame tower of 3 x 3 bays analysed alone, then a
+    batch of `batch` seeded frame buildings (4-16 storeys, 2-4 bays, some ground columns left out) in one launch, a
+    workgroup per building; the oracle (reference summation order, one thread) on the tower and on a sample of the
+    batch"""
+    from shatterforge_amd import scenes, structure
+    tower = _bricks_of(scenes.frame_tower(12, 3))
+    case = structure.gravity_case(tower)
+    net = structure.Net()
+    walls, splits = [], []
+    for _ in range(reps):
+        net.clear()
+        net.add(tower, *case)
+        t0 = time.perf_counter()
+        net.solve()
+        walls.append((time.perf_counter() - t0) * 1e3)
+        splits.append(net.ms().tolist())
+    k = int(np.argsort(walls)[len(walls) // 2])
+    r = net.result(0)
+    res = {"tower": {"storeys": 12, "bays": 3, "bricks": len(tower), "cells": r.nodes, "links": r.links,
+                     "levels": r.levels, "iterations": r.iterations, "max_ratio": float(r.max_ratio),
+                     "ms": walls[k], "host_build_ms": splits[k][0], "upload_ms": splits[k][1],
+                     "device_solve_ms": splits[k][2], "device_checks_ms": splits[k][3]}}
+    log(f"structure: tower {r.nodes} cells, {r.iterations} iterations, {walls[k]:.1f} ms (device {splits[k][2]:.1f})")
+    towers = scenes.frame_towers(batch, seed=seed)
+    stores = [_bricks_of(t) for t, _ in towers]
+    cases = [(b, *structure.gravity_case(b)) for b in stores]
+    walls, splits = [], []
+    for _ in range(2):
+        net.clear()
+        for c in cases:
+            net.add(*c)
+        t0 = time.perf_counter()
+        net.solve()
+        walls.append((time.perf_counter() - t0) * 1e3)
+        splits.append(net.ms().tolist())
+    k = int(np.argmin(walls))
+    cells = sum(net.result(i).nodes for i in range(batch))
+    its = [net.result(i).iterations for i in range(batch)]
+    res["batch"] = {"buildings": batch, "cells": cells, "iterations_mean": float(np.mean(its)),
+                    "iterations_max": int(max(its)), "ms": walls[k], "analyses_per_s": batch / (walls[k] * 1e-3),
+                    "host_build_ms": splits[k][0], "upload_ms": splits[k][1], "device_solve_ms": splits[k][2],
+                    "device_checks_ms": splits[k][3]}
+    log(f"structure: batch of {batch}, {cells} cells, {walls[k]:.0f} ms ({batch / (walls[k] * 1e-3):.0f} analyses/s, "
+        f"device {splits[k][2]:.0f} ms)")
+    if cpu:
+        from oracle import pyoracle as po  # the checker: timed as the CPU baseline only
+
+        def oracle_case(bricks):
+            o = po.BricksOracle()
+            for mn, size, m, paint, fl in bricks:
+                i = o.add(mn, size, m, paint)
+                if fl != 1:
+                    o.set_flags(i, fl)
+            o.build_joints()
+            n = len(bricks)
+            load = np.array([o.mass(i) * np.float32(9.81) for i in range(n)], np.float32)
+            sink = np.array([bk[0][1] <= 0 for bk in bricks], np.uint8)
+            return o, np.arange(n, dtype=np.uint32), load, sink
+
+        o, set_, load, sink = oracle_case(scenes.frame_tower(12, 3))
+        t0 = time.perf_counter()
+        po.analyse_oracle(o, set_, load, sink, order=po.ORDER_REFERENCE)
+        tower_cpu = (time.perf_counter() - t0) * 1e3
+        sample = 8
+        t0 = time.perf_counter()
+        for t, _ in towers[:sample]:
+            po.analyse_oracle(*oracle_case(t), order=po.ORDER_REFERENCE)
+        per = (time.perf_counter() - t0) * 1e3 / sample
+        res["cpu_baseline"] = {"kind": "port", "cores": 1, "tower_ms": tower_cpu, "batch_ms_per_analysis": per,
+                               "batch_analyses_per_s": 1e3 / per, "sample": f"the tower, and the first {sample} "
+                               "buildings of the batch (with their store built)",
+                               "tower_speedup": tower_cpu / res["tower"]["ms"],
+                               "batch_speedup": res["batch"]["analyses_per_s"] / (1e3 / per)}
+        log(f"structure cpu: tower {tower_cpu:.0f} ms, {per:.0f} ms per building of the batch")
+    return res
+
+
 def bench_box(args, sync):
     """the plain box step on the rubble pile: the JSON line's value, roofline line and stages"""
     import shatterforge_amd as sf
@@ -510,9 +599,9 @@ def main():
                     "world's extensions off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
-    ap.add_argument("--parts", default="box,lattice,whatif,load,site,particles",
-                    help="comma list of box, lattice, whatif, load, site, particles, copy (a device copy of known size by "
-                         "the bench-only program shatterforge_amd/sfb_copy)")
+    ap.add_argument("--parts", default="box,lattice,whatif,load,site,particles,structure",
+                    help="comma list of box, lattice, whatif, load, site, particles, structure, copy (a device copy of "
+                         "known size by the bench-only program shatterforge_amd/sfb_copy)")
     ap.add_argument("--whatif-queries", type=int, default=64)
     ap.add_argument("--load-batch", type=int, default=0, help="also run the load-path batch of this many structures")
     ap.add_argument("--site", type=int, default=0, help="also run the site-scale collapse for this many frames")
@@ -559,6 +648,8 @@ def main():
         out["site"] = bench_site(args.site)
     if "particles" in parts:
         out["particles"] = bench_particles()
+    if "structure" in parts:
+        out["structure"] = bench_structure(cpu=not args.no_cpu)
     if not args.no_cpu and scene is not None:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(*scene, args.cpu_seconds, sync=sync)

@@ -415,6 +415,11 @@ int sf_net_clear(sf_net* N);
 int sf_net_count(const sf_net* N);
 /* host threads that build the cases (0: up to 16) */
 int sf_net_set_host_threads(sf_net* N, int n);
+/* The route of a case: in a batch of at most max_cases cases, one of at least min_cells cells is solved a kernel per
+ * step over the whole chip, block_its iterations queued between checks (min_cells -1: every case in its own
+ * workgroup). Default -1, 16, 6: the kernel-per-step route is switched off, having made a GPU memory fault (a
+ * 1,760-cell case with a warm start) whose cause is not yet found; its tests are skipped. */
+int sf_net_set_route(sf_net* N, int min_cells, int max_cases, int block_its);
 /* Analysis (structure.rs:119-133): [max_ratio, iterations, cells, links, levels, worst inner face: ratio, brick, axis,
  * flow N, moment^2, joints checked, inner faces checked] */
 int sf_net_result(const sf_net* N, int c, double* out12);
@@ -429,6 +434,9 @@ int sf_net_inner(const sf_net* N, int c, int32_t* bat3, float* ratio, int cap);
 int sf_net_cells(const sf_net* N, int c, int32_t* cell7, double* x6, int cap);
 /* [host build ms, upload and read-back ms, device solve ms, device checks ms] of the last sf_net_solve */
 int sf_net_ms(const sf_net* N, double* out4);
+/* the host build of the last sf_net_solve summed over its cases (thread ms): cells, links and springs; multigrid
+ * levels; loads, warm starts and checks; then packing (wall ms) */
+int sf_net_host_ms(const sf_net* N, double* out4);
 
 #ifdef __cplusplus
 }

@@ -159,12 +159,14 @@ _PROTOS = {
     "sf_net_clear": (c_int, [c_void_p]),
     "sf_net_count": (c_int, [c_void_p]),
     "sf_net_set_host_threads": (c_int, [c_void_p, c_int]),
+    "sf_net_set_route": (c_int, [c_void_p, c_int, c_int, c_int]),
     "sf_net_result": (c_int, [c_void_p, c_int, c_void_p]),
     "sf_net_levels": (c_int, [c_void_p, c_int, c_void_p]),
     "sf_net_joints": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     "sf_net_inner": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     "sf_net_cells": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     "sf_net_ms": (c_int, [c_void_p, c_void_p]),
+    "sf_net_host_ms": (c_int, [c_void_p, c_void_p]),
     "sf_legacy_world": (c_void_p, [c_void_p]),
     "sf_lat_subphase_ms": (c_int, [c_void_p, c_void_p]),
 }

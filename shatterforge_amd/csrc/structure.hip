@@ -631,26 +631,32 @@ void build_rest(const Store& S, Case& C) {
 }
 
 // ---------------------------------------------------------------- the device solve
+// A level on the device. An apply (y = K x) runs in two phases: every spring (finest level) or coupling (coarse
+// levels) writes what it adds to each of its two cells into that cell's run of the contribution buffer (`cpos`: the
+// two slots; the runs are in link or coupling order), then every cell sums its own run in order — the additions the
+// reference makes, in its order.
 struct DLevel {
-    int n;
-    const double* diag;     // coarse levels
-    const double* inv;
-    const double* reg;      // finest level
-    const uint8_t* fixed;   // finest level
-    const int32_t* ioff;    // per cell: its springs (finest: link * 2 + end) or couplings (coarse: coupling * 2 + end)
-    const int32_t* inc;
-    const double* om;       // couplings (coarse)
-    const int32_t* oends;
+    int n, nc;              // cells; springs (finest) or couplings (coarse)
+    const double* diag;     // (coarse) 36 per cell
+    const double* inv;      // 36 per cell
+    const double* reg;      // (finest) the diagonal's small addition per cell
+    const uint8_t* fixed;   // (finest)
+    const int32_t* ioff;    // per cell: its run in the contribution buffer
+    const int32_t* cpos;    // per spring or coupling: its slots for end a and end b
+    const double* om;       // (coarse) 36 per coupling
+    const int32_t* oends;   // (coarse) the couplings' cells
     const int32_t* choff;   // (coarse) per cell: the finer level's cells in it, in order
     const int32_t* ch;
     const int32_t* upf;     // (coarse) per finer cell: its block here (-1: fixed)
     const double* dupf;     // (coarse) per finer cell: its centre minus its block's
-    double *r, *z, *kz;
+    double *r, *z, *kz, *buf;
 };
 struct DCase {
     int nl, n0;
+    int lds;                // bytes of the coarsest level kept in LDS (0: it stays in global memory)
+    int route;              // 0: solved by one workgroup (k_net_solve); 1: a kernel per step over the chip
     DLevel lv[MAX_LEVELS];
-    const double* sp;       // 15 per link
+    const double* sp;       // 15 per link: k[3] kr[3] ra[3] rb[3] face centre[3]
     const int32_t* ends;
     const double* f;
     const double* x0;       // warm start or null
@@ -672,165 +678,258 @@ __device__ inline void spring_load(const double* s, const double* xa, const doub
     f[0] = s[0] * (pb0 - pa0); f[1] = s[1] * (pb1 - pa1); f[2] = s[2] * (pb2 - pa2);
     m[0] = s[3] * (xb[3] - xa[3]); m[1] = s[4] * (xb[4] - xa[4]); m[2] = s[5] * (xb[5] - xa[5]);
 }
-// y = K x on the finest level, gathered per cell over its springs in link order
-__device__ void apply_fine(const DCase& C, const double* x, double* y) {
-    const DLevel& L = C.lv[0];
-    for (int k = threadIdx.x; k < L.n; k += NT_SOLVE) {
-        double acc[6];
+// The threads sharing a loop: one workgroup's (the one-workgroup solve) or a whole grid's (the solve of one large case
+// spread over the chip, a kernel per step).
+struct Span { int first, stride; };
+__device__ inline Span wg_span() { return {(int)threadIdx.x, NT_SOLVE}; }
+__device__ inline Span grid_span() { return {(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x)}; }
+
+// y = K x on the finest level (structure.rs:636-650), first half: each spring's force on end b and its opposite on
+// end a, with the moments of the arms, into the two cells' runs
+__device__ void fine_pull(const DCase& C, const DLevel& L, const double* x, Span t) {
+    for (int s = t.first; s < L.nc; s += t.stride) {
+        const double* S = C.sp + 15 * (size_t)s;
+        double f[3], m[3];
+        spring_load(S, x + 6 * (size_t)C.ends[2 * s], x + 6 * (size_t)C.ends[2 * s + 1], f, m);
+        const double *ra = S + 6, *rb = S + 9;
+        double* eb = L.buf + 6 * (size_t)L.cpos[2 * s + 1];
+        double* ea = L.buf + 6 * (size_t)L.cpos[2 * s];
+        eb[0] = f[0]; eb[1] = f[1]; eb[2] = f[2];
+        eb[3] = m[0] + rb[1] * f[2] - rb[2] * f[1];
+        eb[4] = m[1] + rb[2] * f[0] - rb[0] * f[2];
+        eb[5] = m[2] + rb[0] * f[1] - rb[1] * f[0];
+        ea[0] = -f[0]; ea[1] = -f[1]; ea[2] = -f[2];
+        ea[3] = -(m[0] + ra[1] * f[2] - ra[2] * f[1]);
+        ea[4] = -(m[1] + ra[2] * f[0] - ra[0] * f[2]);
+        ea[5] = -(m[2] + ra[0] * f[1] - ra[1] * f[0]);
+    }
+}
+// second half: each cell's reg x plus its run in order (fixed cells: 0)
+__device__ void fine_sum(const DLevel& L, const double* x, double* y, Span t) {
+    for (int k = t.first; k < L.n; k += t.stride) {
+        double* yk = y + 6 * (size_t)k;
         if (L.fixed[k]) {
-            for (int i = 0; i < 6; i++) y[6 * k + i] = 0.0;
+            for (int i = 0; i < 6; i++) yk[i] = 0.0;
             continue;
         }
-        double rg = L.reg[k];
-        for (int i = 0; i < 6; i++) acc[i] = rg * x[6 * k + i];
+        double rg = L.reg[k], acc[6];
+        for (int i = 0; i < 6; i++) acc[i] = rg * x[6 * (size_t)k + i];
         for (int e = L.ioff[k]; e < L.ioff[k + 1]; e++) {
-            int q = L.inc[e], s = q >> 1;
-            const double* S = C.sp + 15 * (size_t)s;
-            int a = C.ends[2 * s], b = C.ends[2 * s + 1];
-            double f[3], m[3];
-            spring_load(S, x + 6 * (size_t)a, x + 6 * (size_t)b, f, m);
-            const double* r = (q & 1) ? S + 9 : S + 6;
-            double mm0 = m[0] + r[1] * f[2] - r[2] * f[1], mm1 = m[1] + r[2] * f[0] - r[0] * f[2], mm2 = m[2] + r[0] * f[1] - r[1] * f[0];
-            if (q & 1) { acc[0] += f[0]; acc[1] += f[1]; acc[2] += f[2]; acc[3] += mm0; acc[4] += mm1; acc[5] += mm2; }
-            else { acc[0] -= f[0]; acc[1] -= f[1]; acc[2] -= f[2]; acc[3] -= mm0; acc[4] -= mm1; acc[5] -= mm2; }
+            const double* b = L.buf + 6 * (size_t)e;
+            for (int i = 0; i < 6; i++) acc[i] += b[i];
         }
-        for (int i = 0; i < 6; i++) y[6 * k + i] = acc[i];
+        for (int i = 0; i < 6; i++) yk[i] = acc[i];
     }
 }
-// y = K x on a coarse level: the block's own 6 x 6, then its couplings in order
-__device__ void apply_coarse(const DLevel& L, const double* x, double* y) {
-    for (int k = threadIdx.x; k < L.n; k += NT_SOLVE) {
+// y = K x on a coarse level (structure.rs:651-660): each coupling's products for its two blocks, then each block's own
+// 6 x 6 and its run in order
+__device__ void coarse_pull(const DLevel& L, const double* x, Span t) {
+    for (int o = t.first; o < L.nc; o += t.stride) {
+        const double* M = L.om + 36 * (size_t)o;
+        const double* xa = x + 6 * (size_t)L.oends[2 * o];
+        const double* xc = x + 6 * (size_t)L.oends[2 * o + 1];
+        double* ea = L.buf + 6 * (size_t)L.cpos[2 * o];
+        double* ec = L.buf + 6 * (size_t)L.cpos[2 * o + 1];
+        for (int i = 0; i < 6; i++) {
+            double s = 0.0, u = 0.0;
+            for (int j = 0; j < 6; j++) { s += M[6 * i + j] * xc[j]; u += M[6 * j + i] * xa[j]; }
+            ea[i] = s;
+            ec[i] = u;
+        }
+    }
+}
+__device__ void coarse_sum(const DLevel& L, const double* x, double* y, Span t) {
+    for (int k = t.first; k < L.n; k += t.stride) {
         const double* m = L.diag + 36 * (size_t)k;
-        double v[6], acc[6];
-        for (int j = 0; j < 6; j++) v[j] = x[6 * k + j];
+        const double* v = x + 6 * (size_t)k;
+        double acc[6];
         for (int i = 0; i < 6; i++) { double s = 0.0; for (int j = 0; j < 6; j++) s += m[6 * i + j] * v[j]; acc[i] = s; }
         for (int e = L.ioff[k]; e < L.ioff[k + 1]; e++) {
-            int q = L.inc[e], o = q >> 1;
-            const double* M = L.om + 36 * (size_t)o;
-            if ((q & 1) == 0) {
-                const double* xc = x + 6 * (size_t)L.oends[2 * o + 1];
-                for (int i = 0; i < 6; i++) { double s = 0.0; for (int j = 0; j < 6; j++) s += M[6 * i + j] * xc[j]; acc[i] += s; }
-            } else {
-                const double* xa = x + 6 * (size_t)L.oends[2 * o];
-                for (int i = 0; i < 6; i++) { double t = 0.0; for (int j = 0; j < 6; j++) t += M[6 * j + i] * xa[j]; acc[i] += t; }
-            }
+            const double* b = L.buf + 6 * (size_t)e;
+            for (int i = 0; i < 6; i++) acc[i] += b[i];
         }
-        for (int i = 0; i < 6; i++) y[6 * k + i] = acc[i];
+        for (int i = 0; i < 6; i++) y[6 * (size_t)k + i] = acc[i];
     }
 }
-__device__ inline void apply_level(const DCase& C, int l, const double* x, double* y) {
-    if (l == 0) apply_fine(C, x, y); else apply_coarse(C.lv[l], x, y);
-}
-// z = 0 + w inv r (first sweep from zero) or z += w inv (r - kz)
-__device__ void jacobi(const DLevel& L, bool fine, const double* r, const double* kz, double* z, bool first) {
-    for (int k = threadIdx.x; k < L.n; k += NT_SOLVE) {
+// z = 0 + w inv r (the first sweep from zero) or z += w inv (r - kz) (structure.rs:661-663)
+__device__ void jacobi(const DLevel& L, bool fine, const double* kz, bool first, Span t) {
+    for (int k = t.first; k < L.n; k += t.stride) {
+        double* z = L.z + 6 * (size_t)k;
         if (fine && L.fixed[k]) {
-            if (first) for (int i = 0; i < 6; i++) z[6 * k + i] = 0.0;
+            if (first) for (int i = 0; i < 6; i++) z[i] = 0.0;
             continue;
         }
         double rr[6];
-        if (first) for (int j = 0; j < 6; j++) rr[j] = r[6 * k + j];
-        else for (int j = 0; j < 6; j++) rr[j] = r[6 * k + j] - kz[6 * k + j];
+        const double* r = L.r + 6 * (size_t)k;
+        if (first) for (int j = 0; j < 6; j++) rr[j] = r[j];
+        else for (int j = 0; j < 6; j++) rr[j] = r[j] - kz[6 * (size_t)k + j];
         const double* iv = L.inv + 36 * (size_t)k;
         for (int i = 0; i < 6; i++) {
             double s = 0.0;
             for (int j = 0; j < 6; j++) s += iv[6 * i + j] * rr[j];
             double w = OMEGA * s;
-            z[6 * k + i] = (first ? 0.0 : z[6 * k + i]) + w;
+            z[i] = (first ? 0.0 : z[i]) + w;
         }
     }
 }
-__device__ double block_dot(const double* a, const double* b, int n6, double* red) {
-    double s = 0.0;
-    for (int e = threadIdx.x; e < n6; e += NT_SOLVE) s += a[e] * b[e];
-    red[threadIdx.x] = s;
+// the residual r - K z of level L restricted to its blocks on level N: forces summed, their moments about the block's
+// centre added
+__device__ void restrict_to(const DLevel& L, const DLevel& N, Span t) {
+    for (int c = t.first; c < N.n; c += t.stride) {
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int e = N.choff[c]; e < N.choff[c + 1]; e++) {
+            int k = N.ch[e];
+            double v[6];
+            for (int i = 0; i < 6; i++) v[i] = L.r[6 * (size_t)k + i] - L.kz[6 * (size_t)k + i];
+            const double* d = N.dupf + 3 * (size_t)k;
+            acc[0] += v[0]; acc[1] += v[1]; acc[2] += v[2];
+            acc[3] += v[3] + d[1] * v[2] - d[2] * v[1];
+            acc[4] += v[4] + d[2] * v[0] - d[0] * v[2];
+            acc[5] += v[5] + d[0] * v[1] - d[1] * v[0];
+        }
+        for (int i = 0; i < 6; i++) N.r[6 * (size_t)c + i] = acc[i];
+    }
+}
+// each cell of level L moves with its block on level N
+__device__ void prolong_from(const DLevel& L, const DLevel& N, Span t) {
+    for (int k = t.first; k < L.n; k += t.stride) {
+        int c = N.upf[k];
+        if (c < 0) continue;
+        const double* d = N.dupf + 3 * (size_t)k;
+        const double* zc = N.z + 6 * (size_t)c;
+        double* z = L.z + 6 * (size_t)k;
+        z[0] += zc[0] + zc[4] * d[2] - zc[5] * d[1];
+        z[1] += zc[1] + zc[5] * d[0] - zc[3] * d[2];
+        z[2] += zc[2] + zc[3] * d[1] - zc[4] * d[0];
+        for (int i = 3; i < 6; i++) z[i] += zc[i];
+    }
+}
+__device__ inline void wg_apply(const DCase& C, const DLevel* lv, int l, const double* x, double* y) {
+    if (l == 0) { fine_pull(C, lv[0], x, wg_span()); __syncthreads(); fine_sum(lv[0], x, y, wg_span()); }
+    else { coarse_pull(lv[l], x, wg_span()); __syncthreads(); coarse_sum(lv[l], x, y, wg_span()); }
+}
+// the coarsest level's smoothing: a first sweep from zero, then 29 more (structure.rs:733-741)
+__device__ void coarsest_sweeps(const DCase& C, const DLevel* lv) {
+    int l = C.nl - 1;
+    const DLevel& L = lv[l];
+    jacobi(L, l == 0, nullptr, true, wg_span());
     __syncthreads();
-    for (int st = NT_SOLVE / 2; st > 0; st >>= 1) {
+    for (int s = 1; s < COARSE_SWEEPS; s++) {
+        wg_apply(C, lv, l, L.z, L.kz);
+        __syncthreads();
+        jacobi(L, l == 0, L.kz, false, wg_span());
+        __syncthreads();
+    }
+}
+// 1,024 strided partial sums, then a halving tree (the last six steps within the first wavefront)
+__device__ double tree_sum(double* red) {
+    __syncthreads();
+    for (int st = NT_SOLVE / 2; st >= 64; st >>= 1) {
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
+    if (threadIdx.x < 64) {
+        double v = red[threadIdx.x];
+        for (int st = 32; st > 0; st >>= 1) {
+            double o = __shfl_down(v, st, 64);
+            if ((int)threadIdx.x < st) v += o;
+        }
+        if (threadIdx.x == 0) red[0] = v;
+    }
+    __syncthreads();
     double v = red[0];
     __syncthreads();
     return v;
 }
+__device__ inline double strided_sum(const double* a, const double* b, int n6, int t) {
+    double s = 0.0;
+    for (int e = t; e < n6; e += NT_SOLVE) s += a[e] * b[e];
+    return s;
+}
+__device__ double block_dot(const double* a, const double* b, int n6, double* red) {
+    red[threadIdx.x] = strided_sum(a, b, n6, threadIdx.x);
+    return tree_sum(red);
+}
 // one V-cycle: smooth, correct on the next level, smooth (structure.rs:725-764), as a loop down and up the levels
-__device__ void vcycle(const DCase& C) {
+__device__ void vcycle(const DCase& C, const DLevel* lv) {
     int nl = C.nl;
-    for (int l = 0; l < nl; l++) {
-        const DLevel& L = C.lv[l];
-        jacobi(L, l == 0, L.r, nullptr, L.z, true);
+    for (int l = 0; l < nl - 1; l++) {
+        const DLevel& L = lv[l];
+        jacobi(L, l == 0, nullptr, true, wg_span());
         __syncthreads();
-        if (l == nl - 1) {
-            for (int s = 1; s < COARSE_SWEEPS; s++) {
-                apply_level(C, l, L.z, L.kz);
-                __syncthreads();
-                jacobi(L, l == 0, L.r, L.kz, L.z, false);
-                __syncthreads();
-            }
-            break;
-        }
-        apply_level(C, l, L.z, L.kz);
+        wg_apply(C, lv, l, L.z, L.kz);
         __syncthreads();
-        // restrict the residual: forces summed, their moments about the block's centre added
-        const DLevel& N = C.lv[l + 1];
-        for (int c = threadIdx.x; c < N.n; c += NT_SOLVE) {
-            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int e = N.choff[c]; e < N.choff[c + 1]; e++) {
-                int k = N.ch[e];
-                double v[6];
-                for (int i = 0; i < 6; i++) v[i] = L.r[6 * k + i] - L.kz[6 * k + i];
-                const double* d = N.dupf + 3 * (size_t)k;
-                acc[0] += v[0]; acc[1] += v[1]; acc[2] += v[2];
-                acc[3] += v[3] + d[1] * v[2] - d[2] * v[1];
-                acc[4] += v[4] + d[2] * v[0] - d[0] * v[2];
-                acc[5] += v[5] + d[0] * v[1] - d[1] * v[0];
-            }
-            for (int i = 0; i < 6; i++) N.r[6 * c + i] = acc[i];
-        }
+        restrict_to(L, lv[l + 1], wg_span());
         __syncthreads();
     }
+    coarsest_sweeps(C, lv);
     for (int l = nl - 2; l >= 0; l--) {
-        const DLevel& L = C.lv[l];
-        const DLevel& N = C.lv[l + 1];
-        // prolong: each cell moves with its block
-        for (int k = threadIdx.x; k < L.n; k += NT_SOLVE) {
-            int c = N.upf[k];
-            if (c < 0) continue;
-            const double* d = N.dupf + 3 * (size_t)k;
-            const double* zc = N.z + 6 * (size_t)c;
-            double* z = L.z + 6 * (size_t)k;
-            z[0] += zc[0] + zc[4] * d[2] - zc[5] * d[1];
-            z[1] += zc[1] + zc[5] * d[0] - zc[3] * d[2];
-            z[2] += zc[2] + zc[3] * d[1] - zc[4] * d[0];
-            for (int i = 3; i < 6; i++) z[i] += zc[i];
-        }
+        const DLevel& L = lv[l];
+        prolong_from(L, lv[l + 1], wg_span());
         __syncthreads();
-        apply_level(C, l, L.z, L.kz);
+        wg_apply(C, lv, l, L.z, L.kz);
         __syncthreads();
-        jacobi(L, l == 0, L.r, L.kz, L.z, false);
+        jacobi(L, l == 0, L.kz, false, wg_span());
         __syncthreads();
     }
 }
-// the whole conjugate-gradient solve of one case (structure.rs:566-617)
+// copy 8-byte words into LDS with the whole workgroup
+__device__ inline void lds_copy(void* dst, const void* src, size_t bytes) {
+    const uint64_t* s = (const uint64_t*)src;
+    uint64_t* d = (uint64_t*)dst;
+    for (size_t i = threadIdx.x; i < bytes / 8; i += NT_SOLVE) d[i] = s[i];
+}
+// the level descriptors into LDS; the coarsest level's blocks, couplings and vectors too when the case has room for
+// them (its 30 sweeps a cycle then run from LDS). with_r: bring the coarsest level's right-hand side along.
+__device__ void load_levels(const DCase& C, DLevel* lv, uint8_t* dyn, bool with_r) {
+    if (threadIdx.x < MAX_LEVELS) lv[threadIdx.x] = C.lv[threadIdx.x];
+    __syncthreads();
+    if (!C.lds) return;
+    const DLevel& G = C.lv[C.nl - 1];
+    size_t n = (size_t)G.n, nc = (size_t)G.nc;
+    double* dp = (double*)dyn;
+    double *diag = dp, *inv = diag + 36 * n, *om = inv + 36 * n, *buf = om + 36 * nc, *r = buf + 12 * nc, *z = r + 6 * n,
+           *kz = z + 6 * n;
+    int32_t* ip = (int32_t*)(kz + 6 * n);
+    int32_t *ioff = ip, *cpos = ioff + (n + 1), *oends = cpos + 2 * nc;
+    lds_copy(diag, G.diag, 8 * 36 * n);
+    lds_copy(inv, G.inv, 8 * 36 * n);
+    lds_copy(om, G.om, 8 * 36 * nc);
+    if (with_r) lds_copy(r, G.r, 8 * 6 * n);
+    for (size_t i = threadIdx.x; i < n + 1; i += NT_SOLVE) ioff[i] = G.ioff[i];
+    for (size_t i = threadIdx.x; i < 2 * nc; i += NT_SOLVE) { cpos[i] = G.cpos[i]; oends[i] = G.oends[i]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DLevel& L = lv[C.nl - 1];
+        L.diag = diag; L.inv = inv; L.om = om; L.buf = buf; L.r = r; L.z = z; L.kz = kz;
+        L.ioff = ioff; L.cpos = cpos; L.oends = oends;
+    }
+    __syncthreads();
+}
+// the whole conjugate-gradient solve of one case in one workgroup (structure.rs:566-617)
 __global__ __launch_bounds__(NT_SOLVE) void k_net_solve(const DCase* cases) {
     __shared__ double red[NT_SOLVE];
+    __shared__ DLevel lv[MAX_LEVELS];
+    extern __shared__ __align__(16) uint8_t dyn[];
     const DCase& C = cases[blockIdx.x];
-    const DLevel& F = C.lv[0];
+    if (C.route) return;   // solved a kernel per step
+    load_levels(C, lv, dyn, false);
+    const DLevel& F = lv[0];
     int n6 = 6 * C.n0;
     double *r = F.r, *z = F.z;
     for (int k = threadIdx.x; k < C.n0; k += NT_SOLVE)
         for (int i = 0; i < 6; i++) r[6 * k + i] = F.fixed[k] ? 0.0 : C.f[6 * k + i];
     __syncthreads();
-    vcycle(C);
+    vcycle(C, lv);
     double r0 = fmax(block_dot(r, z, n6, red), 1e-30);
     if (C.x0) {
         for (int e = threadIdx.x; e < n6; e += NT_SOLVE) C.x[e] = C.x0[e];
         __syncthreads();
-        apply_fine(C, C.x, C.ap);
+        wg_apply(C, lv, 0, C.x, C.ap);
         __syncthreads();
         for (int e = threadIdx.x; e < n6; e += NT_SOLVE) r[e] -= C.ap[e];
         __syncthreads();
-        vcycle(C);
+        vcycle(C, lv);
     } else {
         for (int e = threadIdx.x; e < n6; e += NT_SOLVE) C.x[e] = 0.0;
     }
@@ -840,7 +939,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_net_solve(const DCase* cases) {
     int it = 0;
     if (!(rz < TOL * r0)) {
         while (it < MAX_IT) {
-            apply_fine(C, C.p, C.ap);
+            wg_apply(C, lv, 0, C.p, C.ap);
             __syncthreads();
             double pap = block_dot(C.p, C.ap, n6, red);
             if (pap <= 0.0) break;
@@ -848,7 +947,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_net_solve(const DCase* cases) {
             for (int e = threadIdx.x; e < n6; e += NT_SOLVE) { C.x[e] += alpha * C.p[e]; r[e] -= alpha * C.ap[e]; }
             it++;
             __syncthreads();
-            vcycle(C);
+            vcycle(C, lv);
             double rz2 = block_dot(r, z, n6, red);
             if (rz2 < TOL * r0) break;
             double beta = rz2 / rz;
@@ -859,6 +958,101 @@ __global__ __launch_bounds__(NT_SOLVE) void k_net_solve(const DCase* cases) {
     }
     if (threadIdx.x == 0) *C.out_it = it;
 }
+
+// ---- the solve of one large case spread over the chip: the same steps as k_net_solve, each a kernel over a grid,
+// the scalars of the iteration in a small device record. Every kernel returns at once after the solve has ended
+// (converged, pap <= 0 or MAX_IT), so the host may queue iterations in blocks and stop on the first block that ends.
+struct NetState { int32_t done, it, pad[2]; double r0, rz, alpha, beta; double part[NT_SOLVE]; };
+enum : int { DOT_R0 = 0, DOT_RZ = 1, DOT_PAP = 2, DOT_RZ2 = 3 };
+#define MK_GUARD(S) do { if ((S)->done) return; } while (0)
+__global__ void k_mk_init_r(const DCase* C, const NetState* S) {
+    const DLevel& F = C->lv[0];
+    Span t = grid_span();
+    for (int k = t.first; k < C->n0; k += t.stride)
+        for (int i = 0; i < 6; i++) F.r[6 * (size_t)k + i] = F.fixed[k] ? 0.0 : C->f[6 * (size_t)k + i];
+}
+__global__ void k_mk_pull(const DCase* C, const NetState* S, int l, int which) {
+    MK_GUARD(S);
+    const DLevel& L = C->lv[l];
+    const double* x = which == 0 ? L.z : (which == 1 ? C->p : C->x);
+    if (l == 0) fine_pull(*C, L, x, grid_span()); else coarse_pull(L, x, grid_span());
+}
+__global__ void k_mk_sum(const DCase* C, const NetState* S, int l, int which) {
+    MK_GUARD(S);
+    const DLevel& L = C->lv[l];
+    const double* x = which == 0 ? L.z : (which == 1 ? C->p : C->x);
+    double* y = which == 0 ? L.kz : C->ap;
+    if (l == 0) fine_sum(L, x, y, grid_span()); else coarse_sum(L, x, y, grid_span());
+}
+__global__ void k_mk_jacobi(const DCase* C, const NetState* S, int l, int first) {
+    MK_GUARD(S);
+    const DLevel& L = C->lv[l];
+    jacobi(L, l == 0, L.kz, first != 0, grid_span());
+}
+__global__ void k_mk_restrict(const DCase* C, const NetState* S, int l) {
+    MK_GUARD(S);
+    restrict_to(C->lv[l], C->lv[l + 1], grid_span());
+}
+__global__ void k_mk_prolong(const DCase* C, const NetState* S, int l) {
+    MK_GUARD(S);
+    prolong_from(C->lv[l], C->lv[l + 1], grid_span());
+}
+__global__ __launch_bounds__(NT_SOLVE) void k_mk_coarsest(const DCase* C, const NetState* S) {
+    MK_GUARD(S);
+    __shared__ DLevel lv[MAX_LEVELS];
+    extern __shared__ __align__(16) uint8_t dyn[];
+    load_levels(*C, lv, dyn, true);
+    coarsest_sweeps(*C, lv);
+    if (C->lds) {
+        const DLevel& G = C->lv[C->nl - 1];
+        for (int e = threadIdx.x; e < 6 * G.n; e += NT_SOLVE) G.z[e] = lv[C->nl - 1].z[e];
+    }
+}
+// the 1,024 strided partial sums of a dot product (4 x 256 threads)
+__global__ void k_mk_dot_part(const DCase* C, NetState* S, int which) {
+    MK_GUARD(S);
+    const DLevel& F = C->lv[0];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *a = which == 1 ? C->p : F.r, *b = which == 1 ? C->ap : F.z;
+    if (t < NT_SOLVE) S->part[t] = strided_sum(a, b, 6 * C->n0, t);
+}
+// the tree over the partial sums, then the iteration's scalar step
+__global__ __launch_bounds__(NT_SOLVE) void k_mk_dot_end(NetState* S, int mode) {
+    if (S->done) return;
+    __shared__ double red[NT_SOLVE];
+    red[threadIdx.x] = S->part[threadIdx.x];
+    double v = tree_sum(red);
+    if (threadIdx.x != 0) return;
+    if (mode == DOT_R0) S->r0 = fmax(v, 1e-30);
+    else if (mode == DOT_RZ) { S->rz = v; if (v < TOL * S->r0) S->done = 1; }
+    else if (mode == DOT_PAP) { if (v <= 0.0) S->done = 1; else { S->alpha = S->rz / v; S->it += 1; } }
+    else {
+        if (v < TOL * S->r0) S->done = 1;
+        else { S->beta = v / S->rz; S->rz = v; if (S->it >= MAX_IT) S->done = 1; }
+    }
+}
+// x = x0 (or 0) with r -= K x0 to follow; p = z; x += alpha p, r -= alpha ap; p = z + beta p
+__global__ void k_mk_vec(const DCase* C, const NetState* S, int op) {
+    MK_GUARD(S);
+    const DLevel& F = C->lv[0];
+    Span t = grid_span();
+    int n6 = 6 * C->n0;
+    for (int e = t.first; e < n6; e += t.stride) {
+        if (op == 0) C->x[e] = C->x0 ? C->x0[e] : 0.0;
+        else if (op == 1) F.r[e] -= C->ap[e];
+        else if (op == 2) C->p[e] = F.z[e];
+        else if (op == 3) { C->x[e] += S->alpha * C->p[e]; F.r[e] -= S->alpha * C->ap[e]; }
+        else C->p[e] = F.z[e] + S->beta * C->p[e];
+    }
+}
+__global__ void k_mk_state_init(NetState* S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) { S->done = 0; S->it = 0; }
+}
+__global__ void k_mk_finish(const DCase* C, const NetState* S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *C->out_it = S->it;
+}
+// bytes of LDS the coarsest level takes (the layout load_levels uses)
+size_t coarse_lds_bytes(int n, int nc) { return 8 * (90 * (size_t)n + 48 * (size_t)nc) + 4 * ((size_t)n + 1 + 4 * (size_t)nc); }
 // stress over strength of a face under force f and moment m in its frame (structure.rs:532-550)
 __device__ float section_ratio(const double* s, const double* f, const double* m, const float* cap, float flex) {
     double area = s[0], iu = s[1], iw = s[2], zu = s[3], zw = s[4], j = s[5], rmax = s[6];
@@ -908,83 +1102,80 @@ __global__ void k_net_checks(const DCheck* checks, int n, const DCase* cases, fl
 }
 
 // ---------------------------------------------------------------- packing a batch into one device arena
+// The arena holds the cases' data (uploaded) then their work vectors (never uploaded: every one is written before it
+// is read). A first pass only measures, so the host buffer is allocated once.
 struct Arena {
     std::vector<uint8_t> host;
+    size_t size = 0;
+    bool measure = true;
     size_t put(const void* p, size_t bytes) {
-        size_t o = (host.size() + 255) & ~(size_t)255;
-        host.resize(o + bytes);
-        if (bytes && p) std::memcpy(host.data() + o, p, bytes);
+        size_t o = (size + 255) & ~(size_t)255;
+        size = o + bytes;
+        if (!measure && bytes && p) std::memcpy(host.data() + o, p, bytes);
         return o;
     }
-    size_t zeros(size_t bytes) { return put(nullptr, bytes); }
+    size_t reserve(size_t bytes) { size_t o = (size + 255) & ~(size_t)255; size = o + bytes; return o; }
 };
 template <class T> size_t put_vec(Arena& A, const std::vector<T>& v) { return A.put(v.data(), sizeof(T) * v.size()); }
 
 struct CaseOffsets {                           // arena offsets of one case (device pointers = base + offset)
-    size_t sp, ends, f, x0, x, p, ap, it;
-    size_t reg0, fixed0;
-    size_t diag[MAX_LEVELS], inv[MAX_LEVELS], ioff[MAX_LEVELS], inc[MAX_LEVELS], om[MAX_LEVELS], oends[MAX_LEVELS];
-    size_t choff[MAX_LEVELS], ch[MAX_LEVELS], upf[MAX_LEVELS], dupf[MAX_LEVELS], r[MAX_LEVELS], z[MAX_LEVELS], kz[MAX_LEVELS];
-    size_t glinks;
+    size_t sp, ends, f, x0, reg0, fixed0, glinks;
+    size_t diag[MAX_LEVELS], inv[MAX_LEVELS], ioff[MAX_LEVELS], cpos[MAX_LEVELS], om[MAX_LEVELS], oends[MAX_LEVELS];
+    size_t choff[MAX_LEVELS], ch[MAX_LEVELS], upf[MAX_LEVELS], dupf[MAX_LEVELS];
+    // work vectors
+    size_t x, p, ap, it, buf, state;
+    size_t r[MAX_LEVELS], z[MAX_LEVELS], kz[MAX_LEVELS];
 };
-
-void pack_case(const Case& C, Arena& A, CaseOffsets& O) {
-    size_t n0 = C.cbrick.size(), nl = C.la.size();
+// per cell, the run of its ends' contributions (end a: 2 e, end b: 2 e + 1 in `cpos`), in the order of the edges
+void contribution_runs(int n, const std::vector<uint32_t>& a, const std::vector<uint32_t>& b, std::vector<int32_t>& off,
+                       std::vector<int32_t>& cpos) {
+    size_t ne = a.size();
+    off.assign((size_t)n + 1, 0);
+    cpos.assign(2 * std::max<size_t>(ne, 1), 0);
+    for (size_t e = 0; e < ne; e++) { off[a[e] + 1]++; off[b[e] + 1]++; }
+    for (int c = 0; c < n; c++) off[c + 1] += off[c];
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (size_t e = 0; e < ne; e++) { cpos[2 * e] = fill[a[e]]++; cpos[2 * e + 1] = fill[b[e]]++; }
+}
+void pack_data(const Case& C, Arena& A, CaseOffsets& O) {
+    size_t nl = C.la.size();
     O.sp = put_vec(A, C.spring);
-    std::vector<int32_t> ends(2 * nl);
+    std::vector<int32_t> ends(2 * std::max<size_t>(nl, 1), 0);
     for (size_t l = 0; l < nl; l++) { ends[2 * l] = (int32_t)C.la[l]; ends[2 * l + 1] = (int32_t)C.lb[l]; }
     O.ends = put_vec(A, ends);
     O.f = put_vec(A, C.f);
     O.x0 = C.has_x0 ? put_vec(A, C.x0) : 0;
-    O.x = A.zeros(sizeof(double) * 6 * std::max<size_t>(n0, 1));
-    O.p = A.zeros(sizeof(double) * 6 * std::max<size_t>(n0, 1));
-    O.ap = A.zeros(sizeof(double) * 6 * std::max<size_t>(n0, 1));
-    O.it = A.zeros(sizeof(int32_t));
     const Level& F = C.lv[0];
     O.reg0 = put_vec(A, F.reg);
     O.fixed0 = put_vec(A, F.fixed);
-    // the finest level's incidence: per cell, its springs in link order (end a: 0, end b: 1)
-    {
-        std::vector<int32_t> off(n0 + 1, 0), inc(2 * nl);
-        for (size_t l = 0; l < nl; l++) { off[C.la[l] + 1]++; off[C.lb[l] + 1]++; }
-        for (size_t c = 0; c < n0; c++) off[c + 1] += off[c];
-        std::vector<int32_t> fill(off.begin(), off.end() - 1);
-        for (size_t l = 0; l < nl; l++) { inc[fill[C.la[l]]++] = (int32_t)(2 * l); inc[fill[C.lb[l]]++] = (int32_t)(2 * l + 1); }
-        O.ioff[0] = put_vec(A, off);
-        O.inc[0] = put_vec(A, inc);
-    }
+    std::vector<int32_t> off, cpos;
+    contribution_runs(F.n, C.la, C.lb, off, cpos);
+    O.ioff[0] = put_vec(A, off);
+    O.cpos[0] = put_vec(A, cpos);
     O.inv[0] = put_vec(A, F.inv);
-    O.diag[0] = 0; O.om[0] = 0; O.oends[0] = 0; O.choff[0] = 0; O.ch[0] = 0; O.upf[0] = 0; O.dupf[0] = 0;
-    for (size_t l = 0; l < C.lv.size(); l++) {
+    for (size_t l = 1; l < C.lv.size(); l++) {
         const Level& L = C.lv[l];
-        size_t n = std::max(L.n, 1);
-        O.r[l] = A.zeros(sizeof(double) * 6 * n);
-        O.z[l] = A.zeros(sizeof(double) * 6 * n);
-        O.kz[l] = A.zeros(sizeof(double) * 6 * n);
-        if (l == 0) continue;
         const Level& P = C.lv[l - 1];
+        size_t no = L.oa.size();
         O.diag[l] = put_vec(A, L.diag);
         O.inv[l] = put_vec(A, L.inv);
-        size_t no = L.oa.size();
-        std::vector<int32_t> off((size_t)L.n + 1, 0), inc(2 * no), oends(2 * std::max<size_t>(no, 1), 0);
-        for (size_t o = 0; o < no; o++) { off[L.oa[o] + 1]++; off[L.oc[o] + 1]++; oends[2 * o] = (int32_t)L.oa[o]; oends[2 * o + 1] = (int32_t)L.oc[o]; }
-        for (int c = 0; c < L.n; c++) off[c + 1] += off[c];
-        std::vector<int32_t> fill(off.begin(), off.end() - 1);
-        for (size_t o = 0; o < no; o++) { inc[fill[L.oa[o]]++] = (int32_t)(2 * o); inc[fill[L.oc[o]]++] = (int32_t)(2 * o + 1); }
+        contribution_runs(L.n, L.oa, L.oc, off, cpos);
         O.ioff[l] = put_vec(A, off);
-        O.inc[l] = put_vec(A, inc);
+        O.cpos[l] = put_vec(A, cpos);
         O.om[l] = put_vec(A, L.om);
+        std::vector<int32_t> oends(2 * std::max<size_t>(no, 1), 0);
+        for (size_t o = 0; o < no; o++) { oends[2 * o] = (int32_t)L.oa[o]; oends[2 * o + 1] = (int32_t)L.oc[o]; }
         O.oends[l] = put_vec(A, oends);
-        std::vector<int32_t> choff((size_t)L.n + 1, 0), ch, upf(P.n);
-        std::vector<double> dupf(3 * (size_t)P.n, 0.0);
+        std::vector<int32_t> choff((size_t)L.n + 1, 0), upf(std::max(P.n, 1), -1);
+        std::vector<double> dupf(3 * (size_t)std::max(P.n, 1), 0.0);
         for (int k = 0; k < P.n; k++) if (L.up[k] != NONE) choff[L.up[k] + 1]++;
         for (int c = 0; c < L.n; c++) choff[c + 1] += choff[c];
-        ch.resize(std::max(choff[L.n], 1));
+        std::vector<int32_t> ch(std::max(choff[L.n], 1), 0);
         std::vector<int32_t> cf(choff.begin(), choff.end() - 1);
         for (int k = 0; k < P.n; k++) {
             uint32_t c = L.up[k];
-            upf[k] = c == NONE ? -1 : (int32_t)c;
             if (c == NONE) continue;
+            upf[k] = (int32_t)c;
             ch[cf[c]++] = k;
             for (int d = 0; d < 3; d++) dupf[3 * k + d] = P.pos[3 * k + d] - L.pos[3 * c + d];
         }
@@ -995,8 +1186,26 @@ void pack_case(const Case& C, Arena& A, CaseOffsets& O) {
     }
     O.glinks = put_vec(A, C.glinks);
 }
+void pack_work(const Case& C, Arena& A, CaseOffsets& O) {
+    size_t n0 = std::max<size_t>(C.cbrick.size(), 1);
+    O.x = A.reserve(sizeof(double) * 6 * n0);
+    O.p = A.reserve(sizeof(double) * 6 * n0);
+    O.ap = A.reserve(sizeof(double) * 6 * n0);
+    O.it = A.reserve(sizeof(int32_t));
+    O.state = A.reserve(sizeof(NetState));
+    size_t nb = 2 * std::max<size_t>(C.la.size(), 1);
+    for (size_t l = 1; l < C.lv.size(); l++) nb = std::max(nb, 2 * C.lv[l].oa.size());
+    O.buf = A.reserve(sizeof(double) * 6 * nb);
+    for (size_t l = 0; l < C.lv.size(); l++) {
+        size_t n = std::max(C.lv[l].n, 1);
+        O.r[l] = A.reserve(sizeof(double) * 6 * n);
+        O.z[l] = A.reserve(sizeof(double) * 6 * n);
+        O.kz[l] = A.reserve(sizeof(double) * 6 * n);
+    }
+}
 
 template <class T> T* at(uint8_t* base, size_t off) { return reinterpret_cast<T*>(base + off); }
+double since(std::chrono::steady_clock::time_point t) { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
 
 void parallel_for(int n, int threads, const std::function<void(int)>& fn) {
     if (n <= 1 || threads <= 1) { for (int i = 0; i < n; i++) fn(i); return; }
@@ -1018,8 +1227,16 @@ struct sf_net {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int host_threads = 0;
     double ms[4] = {0, 0, 0, 0};   // host build, upload, device solve, device checks
+    double host_ms[4] = {0, 0, 0, 0};  // summed over cases: cells and links, levels, loads and checks; packing
+    size_t lds_budget = 0;             // bytes of dynamic LDS a workgroup may take for the coarsest level
+    int route_min_cells = -1;          // a case of at least this many cells, in a batch of at most route_max_cases,
+    int route_max_cases = 16;          // is solved a kernel per step over the whole chip (-1: never; the default, that
+                                       // route being switched off: it made a GPU memory fault whose cause is not found)
+    int block_its = 6;                 // iterations queued between checks of a per-step solve
+    int32_t* pinned = nullptr;         // the per-step solve's state read-back
     ~sf_net() {
         if (dev) (void)hipFree(dev);
+        if (pinned) (void)hipHostFree(pinned);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -1030,6 +1247,23 @@ int ensure_device(sf_net* N, size_t bytes) {
     if (!N->st) {
         SF_HIP(hipStreamCreateWithFlags(&N->st, hipStreamNonBlocking));
         for (auto& e : N->ev) SF_HIP(hipEventCreate(&e));
+        // dynamic LDS for the coarsest level: what a workgroup may hold beyond the kernel's own 10 KB, at most 128 KB
+        int dev = 0, lds = 0;
+        SF_HIP(hipGetDevice(&dev));
+        SF_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        // (both kernels that take the coarsest level into LDS must be allowed the same dynamic size: a launch beyond
+        // a kernel's allowance leaves the kernel's LDS accesses outside its allocation)
+        size_t budget = lds > 16384 ? std::min<size_t>((size_t)lds - 16384, 131072) : 0;
+        if (budget > 49152) {
+            const void* ks[2] = {(const void*)k_net_solve, (const void*)k_mk_coarsest};
+            for (const void* k : ks)
+                if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)budget) != hipSuccess) {
+                    (void)hipGetLastError();
+                    budget = 49152;
+                }
+        }
+        N->lds_budget = budget;
+        SF_HIP(hipHostMalloc((void**)&N->pinned, 64, hipHostMallocDefault));
     }
     if (bytes > N->dev_bytes) {
         if (N->dev) SF_HIP(hipFree(N->dev));
@@ -1040,7 +1274,75 @@ int ensure_device(sf_net* N, size_t bytes) {
     }
     return 0;
 }
-double since(std::chrono::steady_clock::time_point t) { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
+}  // namespace
+
+namespace {
+dim3 mk_grid(size_t n) { return dim3((unsigned)std::min<size_t>(std::max<size_t>((n + 255) / 256, 1), 1024)); }
+// one V-cycle of a case solved a kernel per step (the order of vcycle's loops)
+int mk_vcycle(sf_net* N, const DCase& h, const DCase* C, const NetState* S) {
+    hipStream_t st = N->st;
+    int nl = h.nl;
+    auto apply = [&](int l, int which) {
+        size_t np = h.lv[l].nc, ns = h.lv[l].n;
+        hipLaunchKernelGGL(k_mk_pull, mk_grid(np), dim3(256), 0, st, C, S, l, which);
+        hipLaunchKernelGGL(k_mk_sum, mk_grid(ns), dim3(256), 0, st, C, S, l, which);
+    };
+    for (int l = 0; l < nl - 1; l++) {
+        hipLaunchKernelGGL(k_mk_jacobi, mk_grid(h.lv[l].n), dim3(256), 0, st, C, S, l, 1);
+        apply(l, 0);
+        hipLaunchKernelGGL(k_mk_restrict, mk_grid(h.lv[l + 1].n), dim3(256), 0, st, C, S, l);
+    }
+    hipLaunchKernelGGL(k_mk_coarsest, dim3(1), dim3(NT_SOLVE), (size_t)h.lds, st, C, S);
+    for (int l = nl - 2; l >= 0; l--) {
+        hipLaunchKernelGGL(k_mk_prolong, mk_grid(h.lv[l].n), dim3(256), 0, st, C, S, l);
+        apply(l, 0);
+        hipLaunchKernelGGL(k_mk_jacobi, mk_grid(h.lv[l].n), dim3(256), 0, st, C, S, l, 0);
+    }
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+// the conjugate-gradient solve of one case, a kernel per step (the order of k_net_solve)
+int mk_solve(sf_net* N, const DCase& h, const DCase* C, NetState* S) {
+    hipStream_t st = N->st;
+    size_t n6 = 6 * (size_t)h.n0;
+    auto dot = [&](int which, int mode) {
+        hipLaunchKernelGGL(k_mk_dot_part, dim3(NT_SOLVE / 256), dim3(256), 0, st, C, S, which);
+        hipLaunchKernelGGL(k_mk_dot_end, dim3(1), dim3(NT_SOLVE), 0, st, S, mode);
+    };
+    auto vec = [&](int op) { hipLaunchKernelGGL(k_mk_vec, mk_grid(n6), dim3(256), 0, st, C, S, op); };
+    auto apply0 = [&](int which) {
+        hipLaunchKernelGGL(k_mk_pull, mk_grid(h.lv[0].nc), dim3(256), 0, st, C, S, 0, which);
+        hipLaunchKernelGGL(k_mk_sum, mk_grid(h.lv[0].n), dim3(256), 0, st, C, S, 0, which);
+    };
+    hipLaunchKernelGGL(k_mk_state_init, dim3(1), dim3(64), 0, st, S);
+    hipLaunchKernelGGL(k_mk_init_r, mk_grid(h.n0), dim3(256), 0, st, C, S);
+    if (mk_vcycle(N, h, C, S)) return -1;
+    dot(0, DOT_R0);
+    vec(0);
+    if (h.x0) {
+        apply0(2);
+        vec(1);
+        if (mk_vcycle(N, h, C, S)) return -1;
+    }
+    vec(2);
+    dot(0, DOT_RZ);
+    for (;;) {
+        for (int k = 0; k < N->block_its; k++) {
+            apply0(1);
+            dot(1, DOT_PAP);
+            vec(3);
+            if (mk_vcycle(N, h, C, S)) return -1;
+            dot(0, DOT_RZ2);
+            vec(4);
+        }
+        SF_HIP(hipMemcpyAsync(N->pinned, S, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        SF_HIP(hipStreamSynchronize(st));
+        if (N->pinned[0]) break;
+    }
+    hipLaunchKernelGGL(k_mk_finish, dim3(1), dim3(64), 0, st, C, S);
+    SF_HIP(hipGetLastError());
+    return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -1119,6 +1421,13 @@ int sf_net_clear(sf_net* N) {
     N->stores.clear();
     return 0;
 }
+int sf_net_set_route(sf_net* N, int min_cells, int max_cases, int block_its) {
+    if (!N || max_cases < 0 || block_its < 1 || block_its > 64) return SF_E_ARG;
+    N->route_min_cells = min_cells;
+    N->route_max_cases = max_cases;
+    N->block_its = block_its;
+    return 0;
+}
 int sf_net_set_host_threads(sf_net* N, int n) {
     if (!N || n < 0) return SF_E_ARG;
     N->host_threads = n;
@@ -1157,25 +1466,45 @@ int sf_net_solve(sf_net* N) {
     auto t0 = std::chrono::steady_clock::now();
     int nc = (int)N->cases.size();
     int threads = N->host_threads > 0 ? N->host_threads : (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::array<double, 3>> part(nc);
     parallel_for(nc, threads, [&](int c) {
         Case& C = N->cases[c];
+        auto t = std::chrono::steady_clock::now();
         build_graph(*N->stores[c], C);
+        part[c][0] = since(t);
+        t = std::chrono::steady_clock::now();
         build_levels(C);
+        part[c][1] = since(t);
+        t = std::chrono::steady_clock::now();
         build_rest(*N->stores[c], C);
+        part[c][2] = since(t);
     });
-    Arena A;
-    std::vector<CaseOffsets> O(nc);
-    for (int c = 0; c < nc; c++) pack_case(N->cases[c], A, O[c]);
+    for (int q = 0; q < 4; q++) N->host_ms[q] = 0.0;
+    for (int c = 0; c < nc; c++) for (int q = 0; q < 3; q++) N->host_ms[q] += part[c][q];
+    auto tp = std::chrono::steady_clock::now();
     size_t ncheck = 0;
     for (auto& C : N->cases) ncheck += C.checks.size();
-    size_t o_cases = A.zeros(sizeof(DCase) * nc);
-    size_t o_checks = A.zeros(sizeof(DCheck) * std::max<size_t>(ncheck, 1));
-    size_t o_out = A.zeros(3 * sizeof(float) * std::max<size_t>(ncheck, 1));
-    if (ensure_device(N, A.host.size())) return SF_E_HIP;
+    // the data (uploaded), then the case and check records (uploaded), then the work vectors
+    std::vector<CaseOffsets> O(nc);
+    Arena A;
+    size_t o_cases = 0, o_checks = 0, up = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        A.size = 0;
+        A.measure = pass == 0;
+        for (int c = 0; c < nc; c++) pack_data(N->cases[c], A, O[c]);
+        o_cases = A.reserve(sizeof(DCase) * nc);
+        o_checks = A.reserve(sizeof(DCheck) * std::max<size_t>(ncheck, 1));
+        up = A.size;
+        if (pass == 0) A.host.resize(up);
+    }
+    for (int c = 0; c < nc; c++) pack_work(N->cases[c], A, O[c]);
+    size_t o_out = A.reserve(3 * sizeof(float) * std::max<size_t>(ncheck, 1));
+    N->host_ms[3] = since(tp);
+    if (ensure_device(N, A.size)) return SF_E_HIP;
     uint8_t* D = N->dev;
     DCase* hc = at<DCase>(A.host.data(), o_cases);
     DCheck* hk = at<DCheck>(A.host.data(), o_checks);
-    size_t kk = 0;
+    size_t kk = 0, dyn = 0;
     for (int c = 0; c < nc; c++) {
         const Case& C = N->cases[c];
         const CaseOffsets& P = O[c];
@@ -1188,14 +1517,20 @@ int sf_net_solve(sf_net* N) {
         for (int l = 0; l < d.nl; l++) {
             DLevel& L = d.lv[l];
             L.n = C.lv[l].n;
+            L.nc = l == 0 ? (int)C.la.size() : (int)C.lv[l].oa.size();
             L.inv = at<double>(D, P.inv[l]);
-            L.ioff = at<int32_t>(D, P.ioff[l]); L.inc = at<int32_t>(D, P.inc[l]);
-            L.r = at<double>(D, P.r[l]); L.z = at<double>(D, P.z[l]); L.kz = at<double>(D, P.kz[l]);
+            L.ioff = at<int32_t>(D, P.ioff[l]); L.cpos = at<int32_t>(D, P.cpos[l]);
+            L.r = at<double>(D, P.r[l]); L.z = at<double>(D, P.z[l]); L.kz = at<double>(D, P.kz[l]); L.buf = at<double>(D, P.buf);
             if (l == 0) { L.reg = at<double>(D, P.reg0); L.fixed = at<uint8_t>(D, P.fixed0); continue; }
             L.diag = at<double>(D, P.diag[l]); L.om = at<double>(D, P.om[l]); L.oends = at<int32_t>(D, P.oends[l]);
             L.choff = at<int32_t>(D, P.choff[l]); L.ch = at<int32_t>(D, P.ch[l]);
             L.upf = at<int32_t>(D, P.upf[l]); L.dupf = at<double>(D, P.dupf[l]);
         }
+        if (d.nl >= 2) {
+            size_t b = coarse_lds_bytes(d.lv[d.nl - 1].n, d.lv[d.nl - 1].nc);
+            if (b <= N->lds_budget) { d.lds = (int)b; dyn = std::max(dyn, b); }
+        }
+        d.route = (N->route_min_cells >= 0 && nc <= N->route_max_cases && d.n0 >= N->route_min_cells && d.nl >= 2) ? 1 : 0;
         for (const Check& K : C.checks) {
             DCheck& q = hk[kk++];
             q.cs = c; q.kind = K.kind; q.axis = K.axis; q.hinge = K.hinge; q.l0 = K.l0; q.l1 = K.l1;
@@ -1208,10 +1543,12 @@ int sf_net_solve(sf_net* N) {
     }
     N->ms[0] = since(t0);
     t0 = std::chrono::steady_clock::now();
-    SF_HIP(hipMemcpyAsync(D, A.host.data(), A.host.size(), hipMemcpyHostToDevice, N->st));
+    SF_HIP(hipMemcpyAsync(D, A.host.data(), up, hipMemcpyHostToDevice, N->st));
     SF_HIP(hipEventRecord(N->ev[0], N->st));
-    hipLaunchKernelGGL(k_net_solve, dim3(nc), dim3(NT_SOLVE), 0, N->st, at<DCase>(D, o_cases));
+    hipLaunchKernelGGL(k_net_solve, dim3(nc), dim3(NT_SOLVE), dyn, N->st, at<DCase>(D, o_cases));
     SF_HIP(hipGetLastError());
+    for (int c = 0; c < nc; c++)
+        if (hc[c].route && mk_solve(N, hc[c], at<DCase>(D, o_cases) + c, at<NetState>(D, O[c].state))) return SF_E_HIP;
     SF_HIP(hipEventRecord(N->ev[1], N->st));
     float* out = at<float>(D, o_out);
     if (ncheck) {
@@ -1319,6 +1656,13 @@ int sf_net_cells(const sf_net* N, int c, int32_t* cell7, double* x6, int cap) {
 int sf_net_ms(const sf_net* N, double* out4) {
     if (!N || !out4) return SF_E_ARG;
     for (int q = 0; q < 4; q++) out4[q] = N->ms[q];
+    return 0;
+}
+// the host build of the last sf_net_solve, summed over its cases (thread ms): cells, links and springs; levels; loads,
+// warm starts and checks; packing into the arena (wall ms)
+int sf_net_host_ms(const sf_net* N, double* out4) {
+    if (!N || !out4) return SF_E_ARG;
+    for (int q = 0; q < 4; q++) out4[q] = N->host_ms[q];
     return 0;
 }
 

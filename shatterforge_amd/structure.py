@@ -197,10 +197,22 @@ class Net:
     def set_host_threads(self, n):
         check(lib().sf_net_set_host_threads(self._h, int(n)))
 
+    def set_route(self, min_cells=-1, max_cases=16, block_its=6):
+        """a case of at least min_cells cells in a batch of at most max_cases is solved a kernel per step over the
+        whole chip (min_cells -1, the default: every case in one workgroup). The kernel-per-step route is switched
+        off: it made a GPU memory fault whose cause is not found yet."""
+        check(lib().sf_net_set_route(self._h, int(min_cells), int(max_cases), int(block_its)))
+
     def ms(self):
         """[host build, upload and read-back, device solve, device checks] of the last solve, ms"""
         out = np.zeros(4, np.float64)
         check(lib().sf_net_ms(self._h, ptr(out)))
+        return out
+
+    def host_ms(self):
+        """the host build of the last solve summed over its cases: cells and links, levels, loads and checks; packing"""
+        out = np.zeros(4, np.float64)
+        check(lib().sf_net_host_ms(self._h, ptr(out)))
         return out
 
     def result(self, c):

@@ -52,12 +52,29 @@ def test_validation_structures_bit_exact(oracle, name):
     _same(a, po.analyse_oracle(o, set_, load, sink, order=po.ORDER_DEVICE), name)
 
 
+ROUTES = {"one workgroup": (-1, 16), "a kernel per step": (0, 16)}
+# The kernel-per-step route made a GPU memory fault (HSA_STATUS_ERROR_MEMORY_APERTURE_VIOLATION) twice, in the
+# warm-start test on a 4-storey tower; its cause is not found, so the route is switched off in the library and its
+# tests do not run until it is.
+PER_STEP_OFF = pytest.mark.skip(reason="kernel-per-step route switched off: it faulted the GPU (cause not found yet)")
+ROUTE_PARAMS = [pytest.param("one workgroup"), pytest.param("a kernel per step", marks=PER_STEP_OFF)]
+
+
+def _solve(b, set_, load, sink, route, dir3=(0.0, -1.0, 0.0), point=(), warm=None):
+    net = structure.Net()
+    net.set_route(*ROUTES[route])
+    net.add(b, set_, load, sink, dir3, point, warm)
+    net.solve()
+    return net.result(0)
+
+
+@pytest.mark.parametrize("route", ROUTE_PARAMS)
 @pytest.mark.parametrize("storeys,remove", [(12, ()), (6, ((0, 0), (1, 1), (2, 1), (1, 0)))])
-def test_frame_tower_bit_exact_and_verdicts_of_the_reference_order(oracle, storeys, remove):
-    print(f"case: frame tower, {storeys} storeys, ground columns removed {remove}", flush=True)
+def test_frame_tower_bit_exact_and_verdicts_of_the_reference_order(oracle, storeys, remove, route):
+    print(f"case: frame tower, {storeys} storeys, ground columns removed {remove}, {route}", flush=True)
     b, o, ys = both(scenes.frame_tower(storeys, 3, remove=set(remove)))
     set_, load, sink = gravity(o, ys)
-    a = structure.analyse(b, set_, load, sink)
+    a = _solve(b, set_, load, sink, route)
     _same(a, po.analyse_oracle(o, set_, load, sink, order=po.ORDER_DEVICE), "tower")
     r = po.analyse_oracle(o, set_, load, sink, order=po.ORDER_REFERENCE)
     assert np.array_equal(a.joint_ratios > 1.0, r["joint_ratios"] > 1.0)
@@ -81,19 +98,24 @@ def test_batch_in_one_launch_equals_each_alone(oracle):
         assert np.array_equal(alone.x, got.x) and np.array_equal(alone.joint_ratios, got.joint_ratios)
 
 
-def test_warm_start_point_loads_and_a_sideways_load(oracle):
-    print("case: warm start twice, a point load, a sideways load", flush=True)
-    b, o, ys = both(scenes.frame_tower(4, 2))
+@pytest.mark.parametrize("route", ROUTE_PARAMS)
+def test_warm_start_point_loads_and_a_sideways_load(oracle, route):
+    print(f"case: warm start twice, a point load, a sideways load, {route}", flush=True)
+    b, o, ys = both(scenes.frame_tower(4, 3))
     set_, load, sink = gravity(o, ys)
     wg, wo = structure.Warm(), po.WarmOracle()
     for rnd in range(2):
-        a = structure.analyse_warm(b, set_, load, (), sink, (0.0, -1.0, 0.0), wg)
+        a = _solve(b, set_, load, sink, route, warm=wg)
         _same(a, po.analyse_oracle(o, set_, load, sink, order=po.ORDER_DEVICE, warm=wo), f"warm {rnd}")
     assert a.iterations == 0 and len(wg) == len(wo)
+    # a warm start from a different load: iterations from a non-zero start
+    a = _solve(b, set_, load * np.float32(1.5), sink, route, warm=wg)
+    _same(a, po.analyse_oracle(o, set_, load * np.float32(1.5), sink, order=po.ORDER_DEVICE, warm=wo), "warm, heavier")
+    assert a.iterations > 0
     roof = int(np.argmax(ys))
     pt = [(roof, 30.0, 50.0, 4.0e5)]
-    a = structure.analyse_at(b, set_, load, pt, sink, (0.0, -1.0, 0.0))
+    a = _solve(b, set_, load, sink, route, point=pt)
     _same(a, po.analyse_oracle(o, set_, load, sink, point=pt, order=po.ORDER_DEVICE), "point")
     d = (0.6, -0.8, 0.0)
-    a = structure.analyse(b, set_, load * np.float32(3.0), sink, d)
+    a = _solve(b, set_, load * np.float32(3.0), sink, route, dir3=d)
     _same(a, po.analyse_oracle(o, set_, load * np.float32(3.0), sink, dir3=d, order=po.ORDER_DEVICE), "sideways")
