@@ -31,6 +31,18 @@ pub mod ffi {
     pub struct SfServe {
         _p: [u8; 0],
     }
+    #[repr(C)]
+    pub struct SfBricks {
+        _p: [u8; 0],
+    }
+    #[repr(C)]
+    pub struct SfWarm {
+        _p: [u8; 0],
+    }
+    #[repr(C)]
+    pub struct SfNet {
+        _p: [u8; 0],
+    }
 
     /// `sf_answer`: one served query's answer
     #[repr(C)]
@@ -155,6 +167,37 @@ pub mod ffi {
         pub fn sf_legacy_dbg_top(s: u32) -> *const f32;
         pub fn sf_legacy_dbg_wake_all();
         pub fn sf_legacy_world(scene: *mut c_int) -> *mut SfLat;
+        // structural analysis (engine/src/structure.rs over engine/src/brick.rs)
+        pub fn sf_bricks_create() -> *mut SfBricks;
+        pub fn sf_bricks_destroy(b: *mut SfBricks);
+        pub fn sf_bricks_len(b: *const SfBricks) -> c_int;
+        pub fn sf_bricks_add(b: *mut SfBricks, min3: *const i32, size3: *const i32, mat: c_int, paint: c_int, structure: c_int) -> c_int;
+        pub fn sf_bricks_set_flags(b: *mut SfBricks, i: c_int, flags: c_int) -> c_int;
+        pub fn sf_bricks_flags(b: *const SfBricks, i: c_int) -> c_int;
+        pub fn sf_bricks_mass(b: *const SfBricks, i: c_int) -> f32;
+        pub fn sf_bricks_remove(b: *mut SfBricks, i: c_int) -> c_int;
+        pub fn sf_bricks_build_joints(b: *mut SfBricks) -> c_int;
+        pub fn sf_bricks_joints(b: *const SfBricks, out5: *mut i32, cap: c_int) -> c_int;
+        pub fn sf_bricks_set_joint_damage(b: *mut SfBricks, j: c_int, dmg: c_int) -> c_int;
+        pub fn sf_warm_create() -> *mut SfWarm;
+        pub fn sf_warm_destroy(w: *mut SfWarm);
+        pub fn sf_warm_size(w: *const SfWarm) -> c_int;
+        pub fn sf_net_create() -> *mut SfNet;
+        pub fn sf_net_destroy(n: *mut SfNet);
+        pub fn sf_net_add(n: *mut SfNet, b: *const SfBricks, count: c_int, set: *const u32, load: *const f32, sink: *const u8,
+                          dir3: *const f32, npoint: c_int, point4: *const f32, warm: *mut SfWarm) -> c_int;
+        pub fn sf_net_solve(n: *mut SfNet) -> c_int;
+        pub fn sf_net_clear(n: *mut SfNet) -> c_int;
+        pub fn sf_net_count(n: *const SfNet) -> c_int;
+        pub fn sf_net_set_host_threads(n: *mut SfNet, threads: c_int) -> c_int;
+        pub fn sf_net_set_route(n: *mut SfNet, min_cells: c_int, max_cases: c_int, block_its: c_int) -> c_int;
+        pub fn sf_net_result(n: *const SfNet, c: c_int, out12: *mut f64) -> c_int;
+        pub fn sf_net_levels(n: *const SfNet, c: c_int, out8: *mut i32) -> c_int;
+        pub fn sf_net_joints(n: *const SfNet, c: c_int, jid: *mut i32, ratio: *mut f32, cap: c_int) -> c_int;
+        pub fn sf_net_inner(n: *const SfNet, c: c_int, bat3: *mut i32, ratio: *mut f32, cap: c_int) -> c_int;
+        pub fn sf_net_cells(n: *const SfNet, c: c_int, cell7: *mut i32, x6: *mut f64, cap: c_int) -> c_int;
+        pub fn sf_net_ms(n: *const SfNet, out4: *mut f64) -> c_int;
+        pub fn sf_net_host_ms(n: *const SfNet, out4: *mut f64) -> c_int;
     }
 }
 
