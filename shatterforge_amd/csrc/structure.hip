@@ -40,7 +40,12 @@ constexpr int MAX_LEVELS = 8;
 constexpr int MAX_IT = 2000;
 constexpr double TOL = 1e-5;
 constexpr double OMEGA = 0.8;
-constexpr int NT_SOLVE = 1024;
+#ifndef SF_NET_THREADS
+#define SF_NET_THREADS 1024
+#endif
+constexpr int NT_SOLVE = SF_NET_THREADS;  // threads of the one-workgroup solve (1024 or 512)
+constexpr int NPART = 1024;               // a dot product's strided partial sums (the order the results depend on)
+static_assert(NT_SOLVE == 1024 || NT_SOLVE == 512, "the dot's partial sums assume 1,024 or 512 threads");
 constexpr int COARSE_SWEEPS = 30;
 enum : int { CONCRETE = 0, MASONRY = 1, STEEL = 2, GLASS = 3, TIMBER = 4 };
 enum : int { F_ALIVE = 1, F_INFILL = 2, F_DYNAMIC = 4, F_MONO = 8 };
@@ -671,6 +676,12 @@ struct DCheck {
     const int32_t* glinks;
 };
 
+// A spring's 15 numbers (k[3] kr[3] ra[3] rb[3] face centre[3]) are stored field after field over all the case's
+// springs (field q of spring s at q * ld + s), and a 6 x 6 block's 36 entries entry after entry over the level's
+// blocks, so the lanes of a wavefront read consecutive words.
+__device__ inline void spring_fields(const double* sp, size_t ld, int s, double* S) {
+    for (int q = 0; q < 12; q++) S[q] = sp[(size_t)q * ld + s];
+}
 __device__ inline void spring_load(const double* s, const double* xa, const double* xb, double* f, double* m) {
     const double *A = s + 6, *B = s + 9;
     double pa0 = xa[0] + xa[4] * A[2] - xa[5] * A[1], pa1 = xa[1] + xa[5] * A[0] - xa[3] * A[2], pa2 = xa[2] + xa[3] * A[1] - xa[4] * A[0];
@@ -688,8 +699,8 @@ __device__ inline Span grid_span() { return {(int)(blockIdx.x * blockDim.x + thr
 // end a, with the moments of the arms, into the two cells' runs
 __device__ void fine_pull(const DCase& C, const DLevel& L, const double* x, Span t) {
     for (int s = t.first; s < L.nc; s += t.stride) {
-        const double* S = C.sp + 15 * (size_t)s;
-        double f[3], m[3];
+        double S[12], f[3], m[3];
+        spring_fields(C.sp, (size_t)L.nc, s, S);
         spring_load(S, x + 6 * (size_t)C.ends[2 * s], x + 6 * (size_t)C.ends[2 * s + 1], f, m);
         const double *ra = S + 6, *rb = S + 9;
         double* eb = L.buf + 6 * (size_t)L.cpos[2 * s + 1];
@@ -725,14 +736,15 @@ __device__ void fine_sum(const DLevel& L, const double* x, double* y, Span t) {
 // 6 x 6 and its run in order
 __device__ void coarse_pull(const DLevel& L, const double* x, Span t) {
     for (int o = t.first; o < L.nc; o += t.stride) {
-        const double* M = L.om + 36 * (size_t)o;
+        const double* M = L.om + o;
+        const size_t ld = (size_t)L.nc;
         const double* xa = x + 6 * (size_t)L.oends[2 * o];
         const double* xc = x + 6 * (size_t)L.oends[2 * o + 1];
         double* ea = L.buf + 6 * (size_t)L.cpos[2 * o];
         double* ec = L.buf + 6 * (size_t)L.cpos[2 * o + 1];
         for (int i = 0; i < 6; i++) {
             double s = 0.0, u = 0.0;
-            for (int j = 0; j < 6; j++) { s += M[6 * i + j] * xc[j]; u += M[6 * j + i] * xa[j]; }
+            for (int j = 0; j < 6; j++) { s += M[(size_t)(6 * i + j) * ld] * xc[j]; u += M[(size_t)(6 * j + i) * ld] * xa[j]; }
             ea[i] = s;
             ec[i] = u;
         }
@@ -740,10 +752,11 @@ __device__ void coarse_pull(const DLevel& L, const double* x, Span t) {
 }
 __device__ void coarse_sum(const DLevel& L, const double* x, double* y, Span t) {
     for (int k = t.first; k < L.n; k += t.stride) {
-        const double* m = L.diag + 36 * (size_t)k;
+        const double* m = L.diag + k;
+        const size_t ld = (size_t)L.n;
         const double* v = x + 6 * (size_t)k;
         double acc[6];
-        for (int i = 0; i < 6; i++) { double s = 0.0; for (int j = 0; j < 6; j++) s += m[6 * i + j] * v[j]; acc[i] = s; }
+        for (int i = 0; i < 6; i++) { double s = 0.0; for (int j = 0; j < 6; j++) s += m[(size_t)(6 * i + j) * ld] * v[j]; acc[i] = s; }
         for (int e = L.ioff[k]; e < L.ioff[k + 1]; e++) {
             const double* b = L.buf + 6 * (size_t)e;
             for (int i = 0; i < 6; i++) acc[i] += b[i];
@@ -763,10 +776,11 @@ __device__ void jacobi(const DLevel& L, bool fine, const double* kz, bool first,
         const double* r = L.r + 6 * (size_t)k;
         if (first) for (int j = 0; j < 6; j++) rr[j] = r[j];
         else for (int j = 0; j < 6; j++) rr[j] = r[j] - kz[6 * (size_t)k + j];
-        const double* iv = L.inv + 36 * (size_t)k;
+        const double* iv = L.inv + k;
+        const size_t ld = (size_t)L.n;
         for (int i = 0; i < 6; i++) {
             double s = 0.0;
-            for (int j = 0; j < 6; j++) s += iv[6 * i + j] * rr[j];
+            for (int j = 0; j < 6; j++) s += iv[(size_t)(6 * i + j) * ld] * rr[j];
             double w = OMEGA * s;
             z[i] = (first ? 0.0 : z[i]) + w;
         }
@@ -821,7 +835,7 @@ __device__ void coarsest_sweeps(const DCase& C, const DLevel* lv) {
         __syncthreads();
     }
 }
-// 1,024 strided partial sums, then a halving tree (the last six steps within the first wavefront)
+// a halving tree over the workgroup's values in red (the last six steps within the first wavefront)
 __device__ double tree_sum(double* red) {
     __syncthreads();
     for (int st = NT_SOLVE / 2; st >= 64; st >>= 1) {
@@ -843,11 +857,14 @@ __device__ double tree_sum(double* red) {
 }
 __device__ inline double strided_sum(const double* a, const double* b, int n6, int t) {
     double s = 0.0;
-    for (int e = t; e < n6; e += NT_SOLVE) s += a[e] * b[e];
+    for (int e = t; e < n6; e += NPART) s += a[e] * b[e];
     return s;
 }
+// (with 512 threads each holds two of the 1,024 partial sums and adds them: the tree's first step)
 __device__ double block_dot(const double* a, const double* b, int n6, double* red) {
-    red[threadIdx.x] = strided_sum(a, b, n6, threadIdx.x);
+    double v = strided_sum(a, b, n6, threadIdx.x);
+    if (NT_SOLVE < NPART) v += strided_sum(a, b, n6, threadIdx.x + NT_SOLVE);
+    red[threadIdx.x] = v;
     return tree_sum(red);
 }
 // one V-cycle: smooth, correct on the next level, smooth (structure.rs:725-764), as a loop down and up the levels
@@ -962,7 +979,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_net_solve(const DCase* cases) {
 // ---- the solve of one large case spread over the chip: the same steps as k_net_solve, each a kernel over a grid,
 // the scalars of the iteration in a small device record. Every kernel returns at once after the solve has ended
 // (converged, pap <= 0 or MAX_IT), so the host may queue iterations in blocks and stop on the first block that ends.
-struct NetState { int32_t done, it, pad[2]; double r0, rz, alpha, beta; double part[NT_SOLVE]; };
+struct NetState { int32_t done, it, pad[2]; double r0, rz, alpha, beta; double part[NPART]; };
 enum : int { DOT_R0 = 0, DOT_RZ = 1, DOT_PAP = 2, DOT_RZ2 = 3 };
 #define MK_GUARD(S) do { if ((S)->done) return; } while (0)
 __global__ void k_mk_init_r(const DCase* C, const NetState* S) {
@@ -1014,13 +1031,15 @@ __global__ void k_mk_dot_part(const DCase* C, NetState* S, int which) {
     const DLevel& F = C->lv[0];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     const double *a = which == 1 ? C->p : F.r, *b = which == 1 ? C->ap : F.z;
-    if (t < NT_SOLVE) S->part[t] = strided_sum(a, b, 6 * C->n0, t);
+    if (t < NPART) S->part[t] = strided_sum(a, b, 6 * C->n0, t);
 }
 // the tree over the partial sums, then the iteration's scalar step
 __global__ __launch_bounds__(NT_SOLVE) void k_mk_dot_end(NetState* S, int mode) {
     if (S->done) return;
     __shared__ double red[NT_SOLVE];
-    red[threadIdx.x] = S->part[threadIdx.x];
+    double v0 = S->part[threadIdx.x];
+    if (NT_SOLVE < NPART) v0 += S->part[threadIdx.x + NT_SOLVE];
+    red[threadIdx.x] = v0;
     double v = tree_sum(red);
     if (threadIdx.x != 0) return;
     if (mode == DOT_R0) S->r0 = fmax(v, 1e-30);
@@ -1081,17 +1100,20 @@ __global__ void k_net_checks(const DCheck* checks, int n, const DCase* cases, fl
         if (K.kind == 0) {
             for (int e = K.l0; e < K.l1; e++) {
                 int l = K.glinks[e];
-                const double* S = C.sp + 15 * (size_t)l;
-                double fo[3], mo[3];
+                const size_t ld = (size_t)C.lv[0].nc;
+                double S[12], fo[3], mo[3];
+                spring_fields(C.sp, ld, l, S);
                 spring_load(S, C.x + 6 * (size_t)C.ends[2 * l], C.x + 6 * (size_t)C.ends[2 * l + 1], fo, mo);
-                double rr[3] = {S[12] - K.c[0], S[13] - K.c[1], S[14] - K.c[2]};
+                double rr[3] = {C.sp[12 * ld + l] - K.c[0], C.sp[13 * ld + l] - K.c[1], C.sp[14 * ld + l] - K.c[2]};
                 for (int q = 0; q < 3; q++) { ft[q] += fo[q]; mt[q] += mo[q]; }
                 double rx[3] = {rr[1] * fo[2] - rr[2] * fo[1], rr[2] * fo[0] - rr[0] * fo[2], rr[0] * fo[1] - rr[1] * fo[0]};
                 for (int q = 0; q < 3; q++) mt[q] += rx[q];
             }
         } else {
             int l = K.l0;
-            spring_load(C.sp + 15 * (size_t)l, C.x + 6 * (size_t)C.ends[2 * l], C.x + 6 * (size_t)C.ends[2 * l + 1], ft, mt);
+            double S[12];
+            spring_fields(C.sp, (size_t)C.lv[0].nc, l, S);
+            spring_load(S, C.x + 6 * (size_t)C.ends[2 * l], C.x + 6 * (size_t)C.ends[2 * l + 1], ft, mt);
         }
         double fl[3] = {ft[ax], ft[u], ft[w]}, ml[3] = {mt[ax], mt[u], mt[w]};
         if (K.hinge) { ml[0] = 0.0; ml[1] = 0.0; ml[2] = 0.0; }
@@ -1117,6 +1139,17 @@ struct Arena {
     size_t reserve(size_t bytes) { size_t o = (size + 255) & ~(size_t)255; size = o + bytes; return o; }
 };
 template <class T> size_t put_vec(Arena& A, const std::vector<T>& v) { return A.put(v.data(), sizeof(T) * v.size()); }
+// records of `rec` numbers each, stored field after field (field q of record i at q * n + i)
+size_t put_fields(Arena& A, const std::vector<double>& aos, size_t rec) {
+    size_t n = aos.size() / rec;
+    size_t o = A.put(nullptr, sizeof(double) * aos.size());
+    if (!A.measure) {
+        double* d = reinterpret_cast<double*>(A.host.data() + o);
+        for (size_t i = 0; i < n; i++)
+            for (size_t q = 0; q < rec; q++) d[q * n + i] = aos[rec * i + q];
+    }
+    return o;
+}
 
 struct CaseOffsets {                           // arena offsets of one case (device pointers = base + offset)
     size_t sp, ends, f, x0, reg0, fixed0, glinks;
@@ -1139,7 +1172,7 @@ void contribution_runs(int n, const std::vector<uint32_t>& a, const std::vector<
 }
 void pack_data(const Case& C, Arena& A, CaseOffsets& O) {
     size_t nl = C.la.size();
-    O.sp = put_vec(A, C.spring);
+    O.sp = put_fields(A, C.spring, 15);
     std::vector<int32_t> ends(2 * std::max<size_t>(nl, 1), 0);
     for (size_t l = 0; l < nl; l++) { ends[2 * l] = (int32_t)C.la[l]; ends[2 * l + 1] = (int32_t)C.lb[l]; }
     O.ends = put_vec(A, ends);
@@ -1152,17 +1185,17 @@ void pack_data(const Case& C, Arena& A, CaseOffsets& O) {
     contribution_runs(F.n, C.la, C.lb, off, cpos);
     O.ioff[0] = put_vec(A, off);
     O.cpos[0] = put_vec(A, cpos);
-    O.inv[0] = put_vec(A, F.inv);
+    O.inv[0] = put_fields(A, F.inv, 36);
     for (size_t l = 1; l < C.lv.size(); l++) {
         const Level& L = C.lv[l];
         const Level& P = C.lv[l - 1];
         size_t no = L.oa.size();
-        O.diag[l] = put_vec(A, L.diag);
-        O.inv[l] = put_vec(A, L.inv);
+        O.diag[l] = put_fields(A, L.diag, 36);
+        O.inv[l] = put_fields(A, L.inv, 36);
         contribution_runs(L.n, L.oa, L.oc, off, cpos);
         O.ioff[l] = put_vec(A, off);
         O.cpos[l] = put_vec(A, cpos);
-        O.om[l] = put_vec(A, L.om);
+        O.om[l] = put_fields(A, L.om, 36);
         std::vector<int32_t> oends(2 * std::max<size_t>(no, 1), 0);
         for (size_t o = 0; o < no; o++) { oends[2 * o] = (int32_t)L.oa[o]; oends[2 * o + 1] = (int32_t)L.oc[o]; }
         O.oends[l] = put_vec(A, oends);
@@ -1306,7 +1339,7 @@ int mk_solve(sf_net* N, const DCase& h, const DCase* C, NetState* S) {
     hipStream_t st = N->st;
     size_t n6 = 6 * (size_t)h.n0;
     auto dot = [&](int which, int mode) {
-        hipLaunchKernelGGL(k_mk_dot_part, dim3(NT_SOLVE / 256), dim3(256), 0, st, C, S, which);
+        hipLaunchKernelGGL(k_mk_dot_part, dim3(NPART / 256), dim3(256), 0, st, C, S, which);
         hipLaunchKernelGGL(k_mk_dot_end, dim3(1), dim3(NT_SOLVE), 0, st, S, mode);
     };
     auto vec = [&](int op) { hipLaunchKernelGGL(k_mk_vec, mk_grid(n6), dim3(256), 0, st, C, S, op); };
