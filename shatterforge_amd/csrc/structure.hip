@@ -194,6 +194,35 @@ void rigid6(const double* d, double* t) {
     for (int i = 0; i < 36; i++) t[i] = (i % 7 == 0) ? 1.0 : 0.0;
     t[4] = d[2]; t[5] = -d[1]; t[9] = -d[2]; t[11] = d[0]; t[15] = d[1]; t[16] = -d[0];
 }
+// Products with a rigid map t = rigid6(d), whose only non-zero entries are its unit diagonal and the six of [d]x. Each
+// sum keeps mm6's order of k and its start from +0.0; a term with a zero factor adds +-0 and never changes such a sum,
+// so leaving those terms out gives mm6's result bit for bit (and mm6's own skips of zero left factors are such terms).
+// RK[j]: the rows k of t with a non-zero in column j, in increasing order.
+const int RN[6] = {1, 1, 1, 3, 3, 3};
+const int RK[6][3] = {{0, 0, 0}, {1, 0, 0}, {2, 0, 0}, {1, 2, 3}, {0, 2, 4}, {0, 1, 5}};
+// c = m t
+void mul_rigid(const double* m, const double* t, double* c) {
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int r = 0; r < RN[j]; r++) { int k = RK[j][r]; s += m[6 * i + k] * t[6 * k + j]; }
+            c[6 * i + j] = s;
+        }
+}
+// c = t' m
+void rigid_t_mul(const double* t, const double* m, double* c) {
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int r = 0; r < RN[i]; r++) { int k = RK[i][r]; s += t[6 * k + i] * m[6 * k + j]; }
+            c[6 * i + j] = s;
+        }
+}
+// c = diag(d6) t
+void diag_rigid(const double* d6, const double* t, double* c) {
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) c[6 * i + j] = 0.0 + d6[i] * t[6 * i + j];
+}
 // Gauss-Jordan with partial pivoting (structure.rs:765-779)
 void inv6(const double* a, double* out) {
     double m[36];
@@ -470,15 +499,14 @@ void fine_level(Case& C, Level& L) {
     }
     for (size_t l = 0; l < nl; l++) {
         const double* s = &C.spring[15 * l];
-        double D[36] = {0}, ba[36], bb[36], dba[36], dbb[36], t[36], m[36];
-        for (int i = 0; i < 3; i++) { D[7 * i] = s[i]; D[7 * (3 + i)] = s[3 + i]; }
+        double ba[36], bb[36], dba[36], dbb[36], m[36];
         rigid6(s + 6, ba); rigid6(s + 9, bb);
-        mm6(D, ba, dba); mm6(D, bb, dbb);
+        diag_rigid(s, ba, dba); diag_rigid(s, bb, dbb);   // s[0..5]: the spring's k and kr, its diagonal D
         uint32_t a = C.la[l], b = C.lb[l];
-        if (!L.fixed[a]) { tr6(ba, t); mm6(t, dba, m); for (int q = 0; q < 36; q++) L.diag[36 * (size_t)a + q] += m[q]; }
-        if (!L.fixed[b]) { tr6(bb, t); mm6(t, dbb, m); for (int q = 0; q < 36; q++) L.diag[36 * (size_t)b + q] += m[q]; }
+        if (!L.fixed[a]) { rigid_t_mul(ba, dba, m); for (int q = 0; q < 36; q++) L.diag[36 * (size_t)a + q] += m[q]; }
+        if (!L.fixed[b]) { rigid_t_mul(bb, dbb, m); for (int q = 0; q < 36; q++) L.diag[36 * (size_t)b + q] += m[q]; }
         if (!L.fixed[a] && !L.fixed[b]) {
-            tr6(ba, t); mm6(t, dbb, m);
+            rigid_t_mul(ba, dbb, m);
             L.oa.push_back(a); L.oc.push_back(b);
             for (int q = 0; q < 36; q++) L.om.push_back(-m[q]);
         }
@@ -515,20 +543,18 @@ void coarse_level(const Level& F, double bin, Level& L) {
     L.diag.assign(36 * (size_t)std::max(L.n, 1), 0.0);
     for (int k = 0; k < F.n; k++) {
         if (L.up[k] == NONE) continue;
-        double a[36], t[36], m[36];
-        mm6(&F.diag[36 * (size_t)k], &T[36 * (size_t)k], a);
-        tr6(&T[36 * (size_t)k], t);
-        mm6(t, a, m);
+        double a[36], m[36];
+        mul_rigid(&F.diag[36 * (size_t)k], &T[36 * (size_t)k], a);
+        rigid_t_mul(&T[36 * (size_t)k], a, m);
         for (int q = 0; q < 36; q++) L.diag[36 * (size_t)L.up[k] + q] += m[q];
     }
     std::unordered_map<uint64_t, uint32_t> pairs;
     for (size_t o = 0; o < F.oa.size(); o++) {
         uint32_t ua = L.up[F.oa[o]], uc = L.up[F.oc[o]];
         if (ua == NONE || uc == NONE) continue;
-        double a[36], t[36], q[36], qt[36];
-        mm6(&F.om[36 * o], &T[36 * (size_t)F.oc[o]], a);
-        tr6(&T[36 * (size_t)F.oa[o]], t);
-        mm6(t, a, q);
+        double a[36], q[36], qt[36];
+        mul_rigid(&F.om[36 * o], &T[36 * (size_t)F.oc[o]], a);
+        rigid_t_mul(&T[36 * (size_t)F.oa[o]], a, q);
         tr6(q, qt);
         if (ua == uc) {
             double* d = &L.diag[36 * (size_t)ua];
