@@ -28,7 +28,7 @@ int sf_device_count(void);
 /* ---------------------------------------------------------------- box world */
 /* Parameter block (floats). Defaults: engine/src/phys.rs:101-105 (Params::default) plus the colouring limits; the GPU
  * world's solver extensions (gpu/gpu_world.js:1152-1203, :954-985) are off by default, so the defaults are the
- * reference's CPU model. The GPU world's values: SF_P_RATIO 20, SF_P_SHOCK 6, SF_P_COL_SUPPORT 2. */
+ * reference's CPU model. The GPU world's values: SF_P_RATIO 20, SF_P_SHOCK 6, SF_P_COL_SUPPORT 2, SF_P_RED_K 32. */
 enum {
     SF_P_GX, SF_P_GY, SF_P_GZ,      /* gravity (0, -9.81, 0) */
     SF_P_SUBSTEPS,                  /* 12 */
@@ -52,16 +52,20 @@ enum {
     SF_P_COL_SUPPORT,               /* colouring rounds for supports alone before the rounds for all; supports outrank the rest (0 = off) */
     SF_P_PSUB,                      /* particle debris: sub-steps per step (4) */
     SF_P_PCELL,                     /* particle debris: grid cell, twice the largest radius + 0.1 m (0.7) */
+    SF_P_RED_K,                     /* contact reduction: a body pair with more manifolds than this keeps the 26 furthest out and the deepest (0 = off; 1-7 act as 8) */
     SF_NPARAM
 };
 /* body record, SF_BODY_F floats: p[3] q[4] v[3] w[3] inv_m inv_i[3] (principal, local) awake sleep_t reserved */
 #define SF_BODY_F 20
-/* shape record, SF_SHAPE_F floats: local position[3] local rotation q[4] half extents[3] friction restitution
- * (a pair bounces with the larger of its two restitutions, the ground's being 0: gpu/gpu_world.js:601-603) */
-#define SF_SHAPE_F 12
+/* shape record, SF_SHAPE_F floats: local position[3] local rotation q[4] half extents[3] friction restitution hull
+ * (a pair bounces with the larger of its two restitutions, the ground's being 0: gpu/gpu_world.js:601-603; hull 0 is
+ * the whole box, hull k > 0 a convex cell of the chunk library in that box, gpu/gpu_world.js:402-410) */
+#define SF_SHAPE_F 13
+/* a shape given in the world, for the body builder: position[3] rotation q[4] half extents[3] material (1..5) hull */
+#define SF_SHAPE_IN 12
 /* statistics of the last step */
 enum { SF_S_AWAKE, SF_S_PAIRS, SF_S_MANIFOLDS, SF_S_POINTS, SF_S_COLOURS, SF_S_OVERFLOW, SF_S_GROUND, SF_S_WAKES,
-       SF_S_ENTRIES, SF_S_CANDIDATES, SF_S_DROPPED, SF_S_ROUNDS, SF_NSTAT };
+       SF_S_ENTRIES, SF_S_CANDIDATES, SF_S_DROPPED, SF_S_ROUNDS, SF_S_REDUCED, SF_NSTAT };
 /* per-manifold record from sf_world_manifolds, SF_MAN_F floats: n[3] count, then 4 points of
  * (raxn[3], rbxn[3], separation, normal impulse), then jt1 jt2 jtw stick */
 #define SF_MAN_F 40
@@ -87,7 +91,7 @@ int sf_world_nbody(sf_world* w);
 int sf_world_candidates(sf_world* w, int32_t* out, int cap);
 /* last step's manifolds: keys (a, b, count) and SF_MAN_F floats each */
 int sf_world_manifolds(sf_world* w, int32_t* keys, float* data, int cap);
-/* colour of each manifold of the last step (SF_P_COL_LIMIT = overflow) */
+/* colour of each manifold of the last step (SF_P_COL_LIMIT = overflow, -2 = reduced) */
 int sf_world_colours(sf_world* w, int32_t* out, int cap);
 /* island label (minimum body index of its island) of each body after the last step */
 int sf_world_islands(sf_world* w, int32_t* out);
@@ -149,6 +153,33 @@ int sf_scene_rubble(int nbody, float p, int maxsize, float bx, float by, float b
 /* Compound body from bricks: nb bricks at lattice cells xyz (0.5 m), materials 1..5; writes one body record and nb
  * shape records (principal frame, gpu/gpu_world.js buildBodies :2212-2249). Returns 0. */
 int sf_build_compound(int nb, const int32_t* xyz, const uint8_t* mat, float* body, float* shapes);
+/* A body from n shapes given in the world (SF_SHAPE_IN floats each), as gpu/gpu_world.js buildBodies (:2212-2249)
+ * builds it: masses by material density and hull volume (none when fixed), centre of mass over the shapes' centroids,
+ * inertia from each shape's covariance, principal frame; v, w may be NULL. Writes the body and n shape records. */
+int sf_build_body(int n, const float* in, int fixed, const float* v, const float* w, float* body, float* shapes);
+/* Convex-chunk library (engine/src/hull.rs, gpu/gpu_world.js hullLibrary :2113-2208): 16 fracture patterns of the
+ * cube [-1,1]^3 from seed 7, each a Voronoi partition of 2-6 seeds; hull 0 is the whole cube. */
+int sf_hull_count(void);
+int sf_hull_pattern_count(void);
+/* the hull ids of pattern p's cells (ids may be NULL); returns their count */
+int sf_hull_pattern(int p, int32_t* ids);
+/* 26 doubles: nv nf nseg ndir, box centre[3] and half size[3] in the cube, volume share of its box and of the cube,
+ * centroid[3] and covariance per unit volume[9] in box coordinates, pattern, slot */
+int sf_hull_cell(int h, double* out);
+/* vertices (3 doubles each) in box coordinates, or (cube != 0) in the cube's; returns the count */
+int sf_hull_vertices(int h, int cube, double* out);
+/* face f: its polygon in order (3 doubles per vertex; box or cube coordinates), its plane (n, d of n.x <= d, box
+ * coordinates) and what lies across it (a cube face 0..5: +x -x +y -y +z -z, or 6 + the slot of a cell of the
+ * pattern); returns the vertex count */
+int sf_hull_face(int h, int f, int cube, double* xyz, double* plane4, int32_t* other);
+/* edges (pairs of vertex indices) and distinct edge directions (3 floats each, box coordinates); returns the edges */
+int sf_hull_edges(int h, int32_t* pairs, float* dirs);
+/* the narrow phase's f32 tables of every hull (sizes per hull: 16 x 4, 12 x 4, 12 x 4 floats, 96, 12, 48 bytes, 4
+ * ints); returns the hull count */
+int sf_hull_tables(float* v4, float* pl4, float* dir4, uint8_t* fidx, uint8_t* fcnt, uint8_t* seg, int32_t* head);
+/* the pieces of a brick (centre p, rotation q, half size h) broken by pattern `pattern`: one SF_SHAPE_IN record per
+ * cell, tiling the brick (gpu/gpu_world.js GpuWorld.hullShapes :2257-2263); out may be NULL; returns the count */
+int sf_hull_shapes(const float* p, const float* q, const float* h, int pattern, int mat, float* out);
 
 /* ---------------------------------------------------------------- load path (src/lib.rs structures, SOR) */
 /* A standing structure of bricks on the 0.5 m lattice: material 1..5 per brick, lattice cell, infill flag.

@@ -23,13 +23,23 @@ typedef struct {
     float tm1, tm2, twm, rad, jt1, jt2, jtw; int stick;
     V3 ca1, ca2, cb1, cb2, ia1, ia2, ib1, ib2, ian, ibn; float ima, imb;
     float rest; int pushed;  /* the pair's restitution; set once a relax pass pushed (gpu_world.js M_OUT .x, .y) */
+    V3 cen; float smin;      /* centre of its points, least separation (contact reduction) */
+    int red;                 /* reduced: kept for the next step's warm start, not coloured, solved or warm-started */
 } Man;
 
 typedef struct {
     V3 p; Quat q; V3 v, w; float inv_m; V3 inv_i; int awake; float sleep_t; V3 sleep_p; Quat sleep_q;
     M3 rot;
 } Body;
-typedef struct { int body; V3 lp; Quat lq; M3 lrot; V3 h; float mu, e; V3 wp; M3 wrot; V3 lo, hi; } Shape;
+typedef struct { int body; V3 lp; Quat lq; M3 lrot; V3 h; float mu, e; int hull; V3 wp; M3 wrot; V3 lo, hi; } Shape;
+
+/* the chunk library's f32 tables, given by the caller (field by field, hull h at h times each stride) */
+#define OH_V 16
+#define OH_F 12
+#define OH_FV 8
+#define OH_S 24
+#define OH_D 12
+typedef struct { int count; float *v4, *pl4, *dir4; uint8_t *fidx, *fcnt, *seg; int32_t* head; } Hulls;
 
 /* a debris particle (gpu_world.js:1676-1684 record): position, radius; velocity, state (0 free, 1 moving, 2 asleep);
  * drawing rotation; sub-step start position, rest timer; spin, friction; corrected position, touch (1 touching, 2
@@ -58,6 +68,8 @@ struct ob_world {
     int32_t *bl_off, *bl_ent, *ovf_index;  /* device order: per-slot contact lists, overflow position per manifold */
     uint32_t *lvl, *lvl2;  /* shock-propagation heights per slot (the two copies the passes alternate between) */
     int any_rest;          /* some shape has a restitution above zero */
+    int nred;              /* manifolds reduced in the last step */
+    Hulls hl;              /* convex-chunk tables (count 0: none given) */
     /* particle debris */
     Particle* pt; int np, capp; int32_t* pfree; int nfree;
     float blast[16][5]; int nblast; int pwake;
@@ -68,7 +80,7 @@ struct ob_world {
 enum { P_GX, P_GY, P_GZ, P_SUBSTEPS, P_HERTZ, P_DAMPING, P_MAXPUSH, P_FRICTION, P_SPEC, P_SLEEP_LIN, P_SLEEP_ANG,
        P_SLEEP_TIME, P_SLEEP_DIST, P_SLEEP_ROT, P_ALLOW_SLEEP, P_CELL, P_LEAK, P_ANCHOR, P_ANCHOR_PUSH, P_ANCHOR_SLOP,
        P_COL_LIMIT, P_COL_ROUNDS, P_COL_SPREAD, P_TAIL_MIN, P_SOLVE_NT, P_RATIO, P_SHOCK, P_REST_THR, P_REST_ITERS,
-       P_COL_SUPPORT, P_PSUB, P_PCELL };
+       P_COL_SUPPORT, P_PSUB, P_PCELL, P_RED_K };
 
 static void default_params(float* p) {
     /* phys.rs:101-105 */
@@ -82,6 +94,8 @@ static void default_params(float* p) {
     p[P_RATIO] = 0.0f; p[P_SHOCK] = 0.0f; p[P_REST_THR] = 1.0f; p[P_REST_ITERS] = 8.0f; p[P_COL_SUPPORT] = 0.0f;
     /* particle debris (gpu_world.js:2411-2412): 4 sub-steps, grid cell 2 x 0.3 m + 0.1 m */
     p[P_PSUB] = 4.0f; p[P_PCELL] = 0.7f;
+    /* contact reduction (gpu_world.js:841): off; the GPU world reduces body pairs with more than 32 manifolds */
+    p[P_RED_K] = 0.0f;
 }
 
 ob_world* ob_create(int order) {
@@ -90,7 +104,28 @@ ob_world* ob_create(int order) {
     default_params(w->prm);
     return w;
 }
+static void free_hulls(Hulls* h) {
+    free(h->v4); free(h->pl4); free(h->dir4); free(h->fidx); free(h->fcnt); free(h->seg); free(h->head);
+    memset(h, 0, sizeof(Hulls));
+}
+int ob_set_hulls(ob_world* w, int count, const float* v4, const float* pl4, const float* dir4, const uint8_t* fidx,
+                 const uint8_t* fcnt, const uint8_t* seg, const int32_t* head) {
+    free_hulls(&w->hl);
+    if (count <= 0) return 0;
+    Hulls* h = &w->hl;
+    h->count = count;
+    h->v4 = (float*)malloc(sizeof(float) * 4 * OH_V * (size_t)count); memcpy(h->v4, v4, sizeof(float) * 4 * OH_V * (size_t)count);
+    h->pl4 = (float*)malloc(sizeof(float) * 4 * OH_F * (size_t)count); memcpy(h->pl4, pl4, sizeof(float) * 4 * OH_F * (size_t)count);
+    h->dir4 = (float*)malloc(sizeof(float) * 4 * OH_D * (size_t)count); memcpy(h->dir4, dir4, sizeof(float) * 4 * OH_D * (size_t)count);
+    h->fidx = (uint8_t*)malloc((size_t)OH_F * OH_FV * count); memcpy(h->fidx, fidx, (size_t)OH_F * OH_FV * count);
+    h->fcnt = (uint8_t*)malloc((size_t)OH_F * count); memcpy(h->fcnt, fcnt, (size_t)OH_F * count);
+    h->seg = (uint8_t*)malloc((size_t)OH_S * 2 * count); memcpy(h->seg, seg, (size_t)OH_S * 2 * count);
+    h->head = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)count); memcpy(h->head, head, sizeof(int32_t) * 4 * (size_t)count);
+    return count;
+}
+
 void ob_destroy(ob_world* w) {
+    free_hulls(&w->hl);
     if (!w) return;
     free(w->bd); free(w->sh); free(w->sv); free(w->sw); free(w->sdp); free(w->sth); free(w->sp0); free(w->sim);
     free(w->sii); free(w->sq); free(w->sidx); free(w->man); free(w->old); free(w->old_sorted); free(w->cand);
@@ -120,6 +155,8 @@ int ob_load(ob_world* w, int nbody, const float* bodies, int nshape, const int32
         if (s > 0 && shape_body[s] < shape_body[s - 1]) return -2;
         sh->lp = v3(f[0], f[1], f[2]); sh->lq = quat(f[3], f[4], f[5], f[6]); sh->h = v3(f[7], f[8], f[9]); sh->mu = f[10];
         sh->e = f[11];
+        sh->hull = (int)f[12];
+        if (sh->hull < 0 || (sh->hull > 0 && sh->hull >= w->hl.count)) return -3;
         sh->lrot = q_to_m3(sh->lq);
     }
     w->any_rest = 0;
@@ -149,6 +186,7 @@ void ob_read_bodies(const ob_world* w, float* out) {
     }
 }
 void ob_stats(const ob_world* w, int32_t* out8) { for (int i = 0; i < 8; i++) out8[i] = w->stats[i]; }
+int ob_reduced(const ob_world* w) { return w->nred; }
 
 /* ---------------------------------------------------------------- geometry helpers */
 static void tangents(V3 n, V3* t1, V3* t2) {
@@ -379,7 +417,8 @@ int ob_bruteforce_pairs(ob_world* w, float dt, int32_t* out, int cap) {
 }
 
 /* ---------------------------------------------------------------- manifolds */
-static uint64_t mkey64(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint64_t)((uint32_t)b + 1u); }
+/* sort key: shape a, then the ground (b = -1), a chunk's second ground patch (b = -2), partners by index */
+static uint64_t mkey64(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint64_t)(b >= 0 ? (uint32_t)b + 3u : (uint32_t)(-b)); }
 
 static int find_old(const ob_world* w, int32_t a, int32_t b) {
     uint64_t k = mkey64(a, b);
@@ -423,6 +462,8 @@ static void push_manifold(ob_world* w, int a, int b, V3 n, const Pt* pts, int cn
     const Man* o = oi >= 0 ? &w->old[oi] : NULL;
     M3 rbm = bb ? bb->rot : m3_identity();
     V3 cen = v3(0, 0, 0); for (int k = 0; k < cnt; k++) cen = vadd(cen, pts[k].p); cen = vmul(cen, 1.0f / (float)cnt);
+    m->cen = cen; m->smin = pts[0].s;
+    for (int k = 1; k < cnt; k++) m->smin = pts[k].s < m->smin ? pts[k].s : m->smin;
     float rad = 0.0f; for (int k = 0; k < cnt; k++) rad += vlen(vsub(pts[k].p, cen)); m->rad = rad / (float)cnt;
     int anchored = 0;
     if (o) {
@@ -461,6 +502,196 @@ static void push_manifold(ob_world* w, int a, int b, V3 n, const Pt* pts, int cn
     }
 }
 
+
+/* ---------------------------------------------------------------- convex chunks (gpu/gpu_world.js:402-530) */
+/* A shape with hull t > 0 is cell t of the chunk library in its box stretched by its half size h: vertex k in the
+ * world is p + R (h * v_k); a face's world plane is R (n / h) normalised. A box paired with a chunk is the whole-cube
+ * cell (t = 0). */
+typedef struct { V3 p; M3 R; V3 h; int t; } HS;
+static V3 hull_w(const Hulls* H, const HS* s, int k) {
+    const float* v = H->v4 + 4 * ((size_t)s->t * OH_V + k);
+    return vadd(s->p, m3_mulv(&s->R, v3(s->h.x * v[0], s->h.y * v[1], s->h.z * v[2])));
+}
+static void hull_face(const Hulls* H, const HS* s, int f, V3* n, float* d) {
+    const float* pl = H->pl4 + 4 * ((size_t)s->t * OH_F + f);
+    V3 nw = m3_mulv(&s->R, v3(pl[0] / s->h.x, pl[1] / s->h.y, pl[2] / s->h.z));
+    float l = vlen(nw);
+    *n = vmul(nw, 1.0f / l);
+    *d = pl[3] / l + vdot(*n, s->p);
+}
+static int hull_fv(const Hulls* H, const HS* s, int f, int i) { return H->fidx[((size_t)s->t * OH_F + f) * OH_FV + i]; }
+static int hull_fn(const Hulls* H, const HS* s, int f) { return H->fcnt[(size_t)s->t * OH_F + f]; }
+static V3 hull_dir(const Hulls* H, const HS* s, int k) {
+    const float* d = H->dir4 + 4 * ((size_t)s->t * OH_D + k);
+    return vnorm(m3_mulv(&s->R, v3(s->h.x * d[0], s->h.y * d[1], s->h.z * d[2])));
+}
+/* up to 8 points; past 8 a deeper point replaces the shallowest */
+static void keep8(Pt* pts, int* cnt, Pt t) {
+    if (*cnt < 8) { pts[(*cnt)++] = t; return; }
+    int hi = 0;
+    for (int q = 1; q < 8; q++) if (pts[q].s > pts[hi].s) hi = q;
+    if (t.s < pts[hi].s) pts[hi] = t;
+}
+/* gpu_world.js hull_ground :421-435: the corners below the margin */
+static int hull_ground(const Hulls* H, const HS* s, float margin, Pt* pts) {
+    int nv = H->head[4 * s->t], cnt = 0;
+    for (int k = 0; k < nv; k++) {
+        V3 x = hull_w(H, s, k);
+        if (!(x.y < margin)) continue;
+        Pt t; t.p = v3(x.x, 0.5f * x.y, x.z); t.s = x.y;
+        keep8(pts, &cnt, t);
+    }
+    return cnt;
+}
+/* gpu_world.js hull_hull :437-530: face normals of both, then edge-direction cross products; a clipped reference
+ * face (B's only if better by 0.005) or, for an edge axis better than both faces by 0.01, the closest points of the
+ * support edges. Normal from A to B. */
+static int hull_hull(const Hulls* H, const HS* A, const HS* B, float margin, Pt* pts, V3* n_out) {
+    const int32_t *ha = H->head + 4 * A->t, *hb = H->head + 4 * B->t;
+    V3 VA[OH_V], VB[OH_V], DA[OH_D], DB[OH_D];
+    for (int k = 0; k < ha[0]; k++) VA[k] = hull_w(H, A, k);
+    for (int k = 0; k < hb[0]; k++) VB[k] = hull_w(H, B, k);
+    for (int k = 0; k < ha[3]; k++) DA[k] = hull_dir(H, A, k);
+    for (int k = 0; k < hb[3]; k++) DB[k] = hull_dir(H, B, k);
+    float sa = -3.40282347e38f, sb = -3.40282347e38f, se = -3.40282347e38f;
+    int fa = 0, fb = 0, ei = 0, ej = 0;
+    V3 el = v3(0, 0, 0);
+    for (int f = 0; f < ha[1]; f++) {
+        V3 pn; float pd; hull_face(H, A, f, &pn, &pd);
+        float lo = 3.40282347e38f;
+        for (int k = 0; k < hb[0]; k++) lo = om_min(lo, vdot(VB[k], pn));
+        float sp = lo - pd;
+        if (sp > margin) return 0;
+        if (sp > sa) { sa = sp; fa = f; }
+    }
+    for (int f = 0; f < hb[1]; f++) {
+        V3 pn; float pd; hull_face(H, B, f, &pn, &pd);
+        float lo = 3.40282347e38f;
+        for (int k = 0; k < ha[0]; k++) lo = om_min(lo, vdot(VA[k], pn));
+        float sp = lo - pd;
+        if (sp > margin) return 0;
+        if (sp > sb) { sb = sp; fb = f; }
+    }
+    for (int i = 0; i < ha[3]; i++)
+        for (int j = 0; j < hb[3]; j++) {
+            V3 l0 = vcross(DA[i], DB[j]);
+            float l2 = vdot(l0, l0);
+            if (l2 < 1e-6f) continue;
+            V3 l = vmul(l0, 1.0f / om_sqrt(l2));
+            float alo = 3.40282347e38f, ahi = -3.40282347e38f, blo = 3.40282347e38f, bhi = -3.40282347e38f;
+            for (int k = 0; k < ha[0]; k++) { float d = vdot(VA[k], l); alo = om_min(alo, d); ahi = om_max(ahi, d); }
+            for (int k = 0; k < hb[0]; k++) { float d = vdot(VB[k], l); blo = om_min(blo, d); bhi = om_max(bhi, d); }
+            float s1 = blo - ahi, s2 = alo - bhi, sp = s1;
+            if (s2 > s1) { sp = s2; l = vneg(l); }
+            if (sp > margin) return 0;
+            if (sp > se) { se = sp; el = l; ei = i; ej = j; }
+        }
+    int use_b = sb > sa + 0.005f;
+    float fbest = use_b ? sb : sa;
+    if (se > fbest + 0.01f) {
+        V3 da = DA[ei], db = DB[ej];
+        V3 a0 = A->p, a1 = A->p, b0 = B->p, b1 = B->p;
+        float best = -3.40282347e38f;
+        for (int k = 0; k < ha[2]; k++) {
+            int x = H->seg[((size_t)A->t * OH_S + k) * 2], y = H->seg[((size_t)A->t * OH_S + k) * 2 + 1];
+            V3 p0 = VA[x], p1 = VA[y];
+            if (fabsf(vdot(vnorm(vsub(p1, p0)), da)) < 0.999f) continue;
+            float m = vdot(vadd(p0, p1), el);
+            if (m > best) { best = m; a0 = p0; a1 = p1; }
+        }
+        best = 3.40282347e38f;
+        for (int k = 0; k < hb[2]; k++) {
+            int x = H->seg[((size_t)B->t * OH_S + k) * 2], y = H->seg[((size_t)B->t * OH_S + k) * 2 + 1];
+            V3 p0 = VB[x], p1 = VB[y];
+            if (fabsf(vdot(vnorm(vsub(p1, p0)), db)) < 0.999f) continue;
+            float m = vdot(vadd(p0, p1), el);
+            if (m < best) { best = m; b0 = p0; b1 = p1; }
+        }
+        V3 d1 = vsub(a1, a0), d2 = vsub(b1, b0), r = vsub(a0, b0);
+        float aa = vdot(d1, d1), ee = vdot(d2, d2), ff = vdot(d2, r), cc = vdot(d1, r), bb = vdot(d1, d2);
+        float den = aa * ee - bb * bb;
+        float s1 = 0.0f;
+        if (den > 1e-9f) s1 = om_min(om_max((bb * ff - cc * ee) / den, 0.0f), 1.0f);
+        float t1 = (bb * s1 + ff) / om_max(ee, 1e-9f);
+        if (t1 < 0.0f) { t1 = 0.0f; s1 = om_min(om_max(-cc / om_max(aa, 1e-9f), 0.0f), 1.0f); }
+        else if (t1 > 1.0f) { t1 = 1.0f; s1 = om_min(om_max((bb - cc) / om_max(aa, 1e-9f), 0.0f), 1.0f); }
+        V3 p1 = vadd(a0, vmul(d1, s1)), p2 = vadd(b0, vmul(d2, t1));
+        pts[0].p = vmul(vadd(p1, p2), 0.5f);
+        pts[0].s = vdot(vsub(p2, p1), el);
+        *n_out = el;
+        return 1;
+    }
+    const HS* Rs = use_b ? B : A;
+    const HS* Is = use_b ? A : B;
+    int fr = use_b ? fb : fa, nfi = use_b ? ha[1] : hb[1];
+    V3 n; float rw;
+    hull_face(H, Rs, fr, &n, &rw);
+    int fi = 0;
+    float bd = 3.40282347e38f;
+    for (int f = 0; f < nfi; f++) { V3 fnv; float fdv; hull_face(H, Is, f, &fnv, &fdv); float d = vdot(fnv, n); if (d < bd) { bd = d; fi = f; } }
+    V3 poly[16], tmp[16];
+    int pn = hull_fn(H, Is, fi);
+    for (int q = 0; q < pn; q++) poly[q] = hull_w(H, Is, hull_fv(H, Is, fi, q));
+    int rn = hull_fn(H, Rs, fr);
+    V3 rc = v3(0, 0, 0);
+    for (int q = 0; q < rn; q++) rc = vadd(rc, hull_w(H, Rs, hull_fv(H, Rs, fr, q)));
+    rc = vmul(rc, 1.0f / (float)rn);
+    for (int e = 0; e < rn; e++) {
+        V3 v0 = hull_w(H, Rs, hull_fv(H, Rs, fr, e)), v1 = hull_w(H, Rs, hull_fv(H, Rs, fr, (e + 1) % rn));
+        V3 sn = vnorm(vcross(vsub(v1, v0), n));
+        if (vdot(sn, vsub(rc, v0)) > 0.0f) sn = vneg(sn);
+        int tn = 0;
+        for (int q = 0; q < pn; q++) {
+            V3 p0 = poly[q], p1 = poly[(q + 1) % pn];
+            float d0 = vdot(vsub(p0, v0), sn), d1 = vdot(vsub(p1, v0), sn);
+            if (d0 <= 0.0f && tn < 16) tmp[tn++] = p0;
+            if ((d0 <= 0.0f) != (d1 <= 0.0f) && tn < 16) tmp[tn++] = vadd(p0, vmul(vsub(p1, p0), d0 / (d0 - d1)));
+        }
+        pn = tn;
+        for (int q = 0; q < pn; q++) poly[q] = tmp[q];
+        if (pn == 0) return 0;
+    }
+    int cnt = 0;
+    for (int q = 0; q < pn; q++) {
+        float sep = vdot(poly[q], n) - rw;
+        if (sep > margin) continue;
+        Pt t; t.p = vsub(poly[q], vmul(n, 0.5f * sep)); t.s = sep;
+        keep8(pts, &cnt, t);
+    }
+    *n_out = use_b ? vneg(n) : n;
+    return cnt;
+}
+static HS hshape(const Shape* s) { HS h; h.p = s->wp; h.R = s->wrot; h.h = s->h; h.t = s->hull; return h; }
+
+/* One pair's contact by the chunk path (hull_hull, or hull_ground when b is NULL) or by the box SAT (collide_boxes;
+ * the ground's eight corners), before reduce4: shapes as p[3] q[4] h[3] hull; out: n[3], count, then up to 8 x (p[3],
+ * separation). For the checks that pin the chunk path against the box path and closed forms. */
+static HS hs_of(const float* f) {
+    HS h; h.p = v3(f[0], f[1], f[2]); h.R = q_to_m3(quat(f[3], f[4], f[5], f[6])); h.h = v3(f[7], f[8], f[9]); h.t = (int)f[10];
+    return h;
+}
+int ob_pair_contact(const ob_world* w, int chunk_path, const float* a, const float* b, float margin, float* out) {
+    Pt pts[8]; V3 n = v3(0, 0, 0); int cnt = 0;
+    HS A = hs_of(a);
+    if (chunk_path) {
+        if (b) { HS B = hs_of(b); cnt = hull_hull(&w->hl, &A, &B, margin, pts, &n); }
+        else { cnt = hull_ground(&w->hl, &A, margin, pts); n = v3(0.0f, -1.0f, 0.0f); }
+    } else {
+        if (b) { HS B = hs_of(b); cnt = collide_boxes(A.p, &A.R, A.h, B.p, &B.R, B.h, margin, pts, &n); }
+        else {
+            for (int k = 0; k < 8; k++) {
+                V3 sg = v3((k & 1) ? 1.0f : -1.0f, (k & 2) ? 1.0f : -1.0f, (k & 4) ? 1.0f : -1.0f);
+                V3 c = vadd(A.p, m3_mulv(&A.R, v3(sg.x * A.h.x, sg.y * A.h.y, sg.z * A.h.z)));
+                if (c.y < margin) { pts[cnt].p = v3(c.x, 0.5f * c.y, c.z); pts[cnt].s = c.y; cnt++; }
+            }
+            n = v3(0.0f, -1.0f, 0.0f);
+        }
+    }
+    out[0] = n.x; out[1] = n.y; out[2] = n.z; out[3] = (float)cnt;
+    for (int k = 0; k < cnt; k++) { out[4 + 4 * k] = pts[k].p.x; out[5 + 4 * k] = pts[k].p.y; out[6 + 4 * k] = pts[k].p.z; out[7 + 4 * k] = pts[k].s; }
+    return cnt;
+}
+
 /* phys.rs:411-416 */
 static void wake_body(ob_world* w, int bi) {
     Body* b = &w->bd[bi];
@@ -473,6 +704,19 @@ static void collide_ground(ob_world* w, int i) {
     const Shape* s = &w->sh[i]; const Body* b = &w->bd[s->body];
     Pt pts[8]; int cnt = 0;
     float margin = w->prm[P_SPEC] + om_max(-b->v.y, 0.0f) * (1.0f / 60.0f);
+    if (s->hull) {
+        /* a chunk's margin also covers its corners' sweep from spin (gpu_world.js ground_margin :776-778) */
+        HS hs = hshape(s);
+        margin = w->prm[P_SPEC] + (om_max(-b->v.y, 0.0f) + vlen(b->w) * shape_reach(s)) * (1.0f / 60.0f);
+        cnt = hull_ground(&w->hl, &hs, margin, pts);
+        if (cnt == 0) return;
+        /* up to 8 corners as two manifolds of 4 in the order found (gpu_world.js narrow_hull :813-823); the second
+         * keyed as b = -2 */
+        V3 n = v3(0.0f, -1.0f, 0.0f);
+        push_manifold(w, i, -1, n, pts, cnt < 4 ? cnt : 4);
+        if (cnt > 4) push_manifold(w, i, -2, n, pts + 4, cnt - 4);
+        return;
+    }
     for (int k = 0; k < 8; k++) {
         V3 sg = v3((k & 1) ? 1.0f : -1.0f, (k & 2) ? 1.0f : -1.0f, (k & 4) ? 1.0f : -1.0f);
         V3 c = vadd(s->wp, m3_mulv(&s->wrot, v3(sg.x * s->h.x, sg.y * s->h.y, sg.z * s->h.z)));
@@ -494,7 +738,9 @@ static void collide_pair(ob_world* w, int ia, int ib, const uint8_t* snap) {
     float rel = vlen(vsub(b->v, a->v)) + vlen(a->w) * shape_reach(sa) + vlen(b->w) * shape_reach(sb);
     float margin = w->prm[P_SPEC] + rel * (1.0f / 60.0f);
     Pt pts[8]; V3 n;
-    int cnt = collide_boxes(sa->wp, &sa->wrot, sa->h, sb->wp, &sb->wrot, sb->h, margin, pts, &n);
+    int cnt;
+    if (sa->hull || sb->hull) { HS A = hshape(sa), B = hshape(sb); cnt = hull_hull(&w->hl, &A, &B, margin, pts, &n); }
+    else cnt = collide_boxes(sa->wp, &sa->wrot, sa->h, sb->wp, &sb->wrot, sb->h, margin, pts, &n);
     if (!cnt) return;
     if (snap) {
         if (a_awake && !b_awake && !a_still) w->wake_flag[sb->body] = 1;
@@ -505,6 +751,63 @@ static void collide_pair(ob_world* w, int ia, int ib, const uint8_t* snap) {
     }
     cnt = reduce4(pts, cnt, n);
     push_manifold(w, ia, ib, n, pts, cnt);
+}
+
+/* ---------------------------------------------------------------- contact reduction */
+/* gpu_world.js:834-920. A body pair (the ground counting as one body) touching through more than K manifolds keeps
+ * only representative ones: for each of the 26 directions d to the neighbours of a cube cell (corners, edges, faces)
+ * the manifold whose points' centre c lies furthest out (c . d largest), and the deepest (least separation). Ties go
+ * to the later manifold (manifold order is the canonical candidate order, so later = larger shape key). The others
+ * are marked reduced: colouring, the solve, the warm start and the level and island passes skip them, while their
+ * points and impulses stay in the step's manifolds for the next step's warm-start match. */
+static inline uint32_t f2o(float x) { uint32_t b; memcpy(&b, &x, 4); return (b & 0x80000000u) ? ~b : (b | 0x80000000u); }
+/* direction k of 26 (the cell offsets j = 0..26 in x-fastest order, the centre j = 13 skipped) */
+static inline V3 red_dir(int k) {
+    int j = k >= 13 ? k + 1 : k;
+    return v3((float)(j % 3) - 1.0f, (float)((j / 3) % 3) - 1.0f, (float)(j / 9) - 1.0f);
+}
+static inline uint64_t pair_key(const Man* m) { return ((uint64_t)(uint32_t)m->ba << 32) | (uint64_t)(m->bb >= 0 ? (uint32_t)m->bb : 0xFFFFFFFFu); }
+static const Man* g_red_base;
+static int cmp_red(const void* x, const void* y) {
+    int a = *(const int32_t*)x, b = *(const int32_t*)y;
+    uint64_t ka = pair_key(&g_red_base[a]), kb = pair_key(&g_red_base[b]);
+    if (ka != kb) return ka < kb ? -1 : 1;
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+static int reduce_pairs(ob_world* w) {
+    int M = w->nman, K = w->prm[P_RED_K] >= 1.0f ? (int)w->prm[P_RED_K] : 0, nred = 0;
+    if (K > 0 && K < 8) K = 8;  /* (smaller thresholds act as 8) */
+    for (int m = 0; m < M; m++) w->man[m].red = 0;
+    if (K <= 0 || M <= K) return 0;
+    int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * (size_t)M);
+    for (int m = 0; m < M; m++) idx[m] = m;
+    g_red_base = w->man;
+    qsort(idx, (size_t)M, sizeof(int32_t), cmp_red);
+    for (int r0 = 0; r0 < M;) {
+        int r1 = r0 + 1;
+        uint64_t key = pair_key(&w->man[idx[r0]]);
+        while (r1 < M && pair_key(&w->man[idx[r1]]) == key) r1++;
+        if (r1 - r0 > K) {
+            uint64_t best[27] = {0};
+            for (int t = r0; t < r1; t++) {
+                const Man* m = &w->man[idx[t]];
+                for (int k = 0; k < 26; k++) {
+                    V3 d = red_dir(k);
+                    float sc = m->cen.x * d.x + m->cen.y * d.y + m->cen.z * d.z;
+                    uint64_t v = ((uint64_t)f2o(sc) << 32) | (uint32_t)idx[t];
+                    if (v > best[k]) best[k] = v;
+                }
+                uint64_t v = ((uint64_t)f2o(-m->smin) << 32) | (uint32_t)idx[t];
+                if (v > best[26]) best[26] = v;
+            }
+            for (int t = r0; t < r1; t++) w->man[idx[t]].red = 1;
+            for (int k = 0; k < 27; k++) w->man[(uint32_t)best[k]].red = 0;
+            for (int t = r0; t < r1; t++) nred += w->man[idx[t]].red;
+        }
+        r0 = r1;
+    }
+    free(idx);
+    return nred;
 }
 
 /* ---------------------------------------------------------------- colouring (device order) */
@@ -530,7 +833,7 @@ static void colour_manifolds(ob_world* w) {
     int32_t* cand = (int32_t*)malloc(sizeof(int32_t) * (size_t)(M ? M : 1));
     uint64_t* pri = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(M ? M : 1));
     const uint64_t all = K == 64 ? ~0ull : ((1ull << K) - 1ull);
-    for (int m = 0; m < M; m++) w->colour[m] = -1;
+    for (int m = 0; m < M; m++) w->colour[m] = w->man[m].red ? -2 : -1;
     for (int r = 0; r < R; r++) {
         int any = 0;
         memset(claim, 0, sizeof(uint64_t) * ((size_t)w->nb + 1) * (size_t)K);
@@ -585,6 +888,7 @@ static void colour_manifolds(ob_world* w) {
 static void warm_start(ob_world* w) {
     for (int mi = 0; mi < w->nman; mi++) {
         const Man* m = &w->man[mi];
+        if (m->red) continue;
         int sa = m->sa, sb = m->sb;
         V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
         float pn = 0.0f;
@@ -613,11 +917,16 @@ static V3 tree64(V3* x) {
 static void build_lists(const ob_world* w, int32_t* off, int32_t* ent) {
     size_t n1 = (size_t)w->nb + 1;
     for (size_t s = 0; s <= n1; s++) off[s] = 0;
-    for (int m = 0; m < w->nman; m++) { if (w->man[m].sa) off[w->man[m].sa + 1]++; if (w->man[m].sb) off[w->man[m].sb + 1]++; }
+    for (int m = 0; m < w->nman; m++) {
+        if (w->man[m].red) continue;
+        if (w->man[m].sa) off[w->man[m].sa + 1]++;
+        if (w->man[m].sb) off[w->man[m].sb + 1]++;
+    }
     for (size_t s = 0; s < n1; s++) off[s + 1] += off[s];
     int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * n1);
     for (size_t s = 0; s < n1; s++) fill[s] = off[s];
     for (int m = 0; m < w->nman; m++) {
+        if (w->man[m].red) continue;
         if (w->man[m].sa) ent[fill[w->man[m].sa]++] = 2 * m;
         if (w->man[m].sb) ent[fill[w->man[m].sb]++] = 2 * m + 1;
     }
@@ -819,6 +1128,7 @@ static void rest_core(Man* m, const SolveK* K, V3* pva, V3* pwa, V3* pvb, V3* pw
 
 /* one manifold of the sequential (or colour-ordered) solve: the reference's manifold loop body */
 static void solve_one(ob_world* w, Man* m, const SolveK* K) {
+    if (m->red) return;
     int sa = m->sa, sb = m->sb;
     V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
     V3 dpab = vsub(w->sdp[sb], w->sdp[sa]);
@@ -827,6 +1137,7 @@ static void solve_one(ob_world* w, Man* m, const SolveK* K) {
     w->sv[0] = v3(0, 0, 0); w->sw[0] = v3(0, 0, 0);
 }
 static void rest_one(ob_world* w, Man* m, const SolveK* K) {
+    if (m->red) return;
     int sa = m->sa, sb = m->sb;
     V3 va = w->sv[sa], wa = w->sw[sa], vb = w->sv[sb], wb = w->sw[sb];
     rest_core(m, K, &va, &wa, &vb, &wb, 0.0f, 0.0f);
@@ -905,6 +1216,7 @@ static void level_passes(ob_world* w, int passes) {
         uint32_t* wr = (k & 1) ? w->lvl : w->lvl2;
         for (int mi = 0; mi < w->nman; mi++) {
             const Man* m = &w->man[mi];
+            if (m->red) continue;
             int sa = m->sa, sb = m->sb;
             int da = sa && w->sim[sa] > 0.0f, db = sb && w->sim[sb] > 0.0f;
             uint32_t ha = da ? rd[sa] : 255u, hb = db ? rd[sb] : 255u;
@@ -924,7 +1236,7 @@ static void sleep_islands(ob_world* w, float dt) {
     for (int i = 0; i < n; i++) uf[i] = i;
     for (int mi = 0; mi < w->nman; mi++) {
         const Man* m = &w->man[mi];
-        if (m->sa == 0 || m->sb == 0) continue;
+        if (m->sa == 0 || m->sb == 0 || m->red) continue;
         int ra = uf_find(uf, m->ba), rb = uf_find(uf, m->bb);
         if (ra != rb) uf[ra] = rb;
     }
@@ -1030,6 +1342,7 @@ void ob_step(ob_world* w, float dt) {
     }
     w->stats[6] = ngc;
     free(pairs);
+    w->nred = reduce_pairs(w);
     /* ---- constraint masses (phys.rs:180-211) */
     int sub = (int)P[P_SUBSTEPS]; if (sub < 1) sub = 1;
     float h = dt / (float)sub;
@@ -1124,7 +1437,7 @@ void ob_step(ob_world* w, float dt) {
     if (w->any_rest)
         for (int mi = 0; mi < w->nman && !bounce; mi++) {
             const Man* m = &w->man[mi];
-            if (m->rest > 0.0f) for (int k = 0; k < m->cnt; k++) if (!(m->cp[k].rv > -kr.rest_thr)) bounce = 1;
+            if (m->rest > 0.0f && !m->red) for (int k = 0; k < m->cnt; k++) if (!(m->cp[k].rv > -kr.rest_thr)) bounce = 1;
         }
     if (bounce) rest_pass(w, &kr, order, ncol_order, ovf, novf, ocnt, dv);
     free(order); free(ovf); free(ocnt); free(dv);

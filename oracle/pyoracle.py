@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 ORDER_REFERENCE, ORDER_COLOURED = 0, 1
-BODY_F, SHAPE_F, MAN_F = 20, 12, 40
+BODY_F, SHAPE_F, MAN_F = 20, 13, 40
 
 _lib = None
 vp, ci, cf, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
@@ -30,7 +30,7 @@ def lib():
         protos = {
             "ob_create": (vp, [ci]), "ob_destroy": (None, [vp]), "ob_set_params": (None, [vp, vp, ci]),
             "ob_load": (ci, [vp, ci, vp, ci, vp, vp]), "ob_step": (None, [vp, cf]), "ob_nbody": (ci, [vp]),
-            "ob_read_bodies": (None, [vp, vp]), "ob_stats": (None, [vp, vp]), "ob_candidates": (ci, [vp, vp, ci]),
+            "ob_read_bodies": (None, [vp, vp]), "ob_stats": (None, [vp, vp]), "ob_reduced": (ci, [vp]), "ob_set_hulls": (ci, [vp, ci, vp, vp, vp, vp, vp, vp, vp]), "ob_pair_contact": (ci, [vp, ci, vp, vp, cf, vp]), "ob_candidates": (ci, [vp, vp, ci]),
             "ob_manifolds": (ci, [vp, vp, vp, ci]), "ob_colours": (ci, [vp, vp, ci]), "ob_islands": (None, [vp, vp]), "ob_levels": (None, [vp, vp]),
             "ob_particles_capacity": (ci, [vp, ci]), "ob_add_particles": (ci, [vp, ci, vp, vp]),
             "ob_kill_particle": (None, [vp, ci]), "ob_blast_particles": (None, [vp, cf, cf, cf, cf, cf]),
@@ -65,12 +65,17 @@ def _p(a):
 class BoxOracle:
     """The reference box step on the CPU (engine/src/phys.rs restated, compound shapes)."""
 
-    def __init__(self, bodies, shape_body, shapes, prm=None, order=ORDER_COLOURED):
+    def __init__(self, bodies, shape_body, shapes, prm=None, order=ORDER_COLOURED, hulls=None):
+        """hulls: the convex-chunk tables (v4, pl4, dir4, fidx, fcnt, seg, head), needed when a shape has a hull"""
         L = lib()
         self._h = L.ob_create(order)
         if prm is not None:
             prm = np.ascontiguousarray(prm, np.float32)
             L.ob_set_params(self._h, _p(prm), len(prm))
+        if hulls is not None:
+            t = [np.ascontiguousarray(x) for x in hulls]
+            self._hulls = t
+            L.ob_set_hulls(self._h, len(t[6]) // 4, *[_p(x) for x in t])
         b = np.ascontiguousarray(bodies, np.float32)
         sb = np.ascontiguousarray(shape_body, np.int32)
         s = np.ascontiguousarray(shapes, np.float32)
@@ -87,6 +92,14 @@ class BoxOracle:
     def step(self, dt=1.0 / 60.0):
         lib().ob_step(self._h, ctypes.c_float(dt))
 
+    def pair_contact(self, a, b, margin, chunk_path=True):
+        """(normal, points [k, 4]: p, separation) of one pair before reduce4; a, b: p[3] q[4] h[3] hull (b None: ground)"""
+        fa = np.ascontiguousarray(a, np.float32)
+        fb = None if b is None else np.ascontiguousarray(b, np.float32)
+        out = np.zeros(4 + 32, np.float32)
+        n = lib().ob_pair_contact(self._h, 1 if chunk_path else 0, _p(fa), _p(fb), ctypes.c_float(margin), _p(out))
+        return out[0:3].copy(), out[4:4 + 4 * n].reshape(n, 4).copy()
+
     def bodies(self):
         out = np.zeros((self.nb, BODY_F), np.float32)
         lib().ob_read_bodies(self._h, _p(out))
@@ -95,7 +108,9 @@ class BoxOracle:
     def stats(self):
         out = np.zeros(8, np.int32)
         lib().ob_stats(self._h, _p(out))
-        return dict(zip(["awake", "pairs", "manifolds", "points", "colours", "overflow", "ground", "wakes"], out.tolist()))
+        d = dict(zip(["awake", "pairs", "manifolds", "points", "colours", "overflow", "ground", "wakes"], out.tolist()))
+        d["reduced"] = int(lib().ob_reduced(self._h))
+        return d
 
     def candidates(self):
         n = lib().ob_candidates(self._h, None, 0)

@@ -24,24 +24,34 @@ enum { OB_ORDER_REFERENCE = 0, OB_ORDER_COLOURED = 1 };
 /* Parameter block, phys.rs:74-106 defaults plus the colouring limits and the GPU world's solver extensions (mass-ratio
  * limit, shock-propagation level passes, restitution threshold and iterations, supports-first colouring rounds; all
  * off by default). Index names in sf.h (SF_P_*). */
-#define OB_NPARAM 32
+#define OB_NPARAM 33
 
 /* body record: p[3] q[4] v[3] w[3] inv_m inv_i[3] awake sleep_t reserved  (20 floats) */
 #define OB_BODY_F 20
-/* shape record: lp[3] lq[4] h[3] mu e  (12 floats: friction, restitution); shape_body[] gives the owning body */
-#define OB_SHAPE_F 12
+/* shape record: lp[3] lq[4] h[3] mu e hull  (13 floats: friction, restitution, convex cell 0 = whole box); shape_body[] gives the owning body */
+#define OB_SHAPE_F 13
 
 ob_world* ob_create(int order);
 void ob_destroy(ob_world* w);
 void ob_set_params(ob_world* w, const float* prm, int n);
 /* Bodies and shapes; shapes of one body must be contiguous and in body order. */
+/* The convex-chunk tables (f32, field by field as the library's narrow phase reads them: per hull 16 vertices x 4,
+ * 12 planes x 4, 12 directions x 4 floats, 12 x 8 face vertex indices, 12 face counts, 24 x 2 edge ends, 4 counts
+ * nv nf nseg ndir), copied; before ob_load of shapes whose hull (record float 12) is above 0. */
+int ob_set_hulls(ob_world* w, int count, const float* v4, const float* pl4, const float* dir4, const uint8_t* fidx,
+                 const uint8_t* fcnt, const uint8_t* seg, const int32_t* head);
 int ob_load(ob_world* w, int nbody, const float* bodies, int nshape, const int32_t* shape_body, const float* shapes);
+/* one pair's contact points before reduce4, by the chunk path (chunk_path != 0) or the box SAT; shapes p[3] q[4] h[3]
+ * hull (b NULL: the ground); out: n[3] count, then up to 8 x (p[3], separation); returns the count */
+int ob_pair_contact(const ob_world* w, int chunk_path, const float* a, const float* b, float margin, float* out);
 void ob_step(ob_world* w, float dt);
 int ob_nbody(const ob_world* w);
 /* read back body state (OB_BODY_F floats per body) */
 void ob_read_bodies(const ob_world* w, float* out);
 /* stats: [awake solver bodies, pairs, manifolds, points, colours used, overflow, ground candidates, wakes] */
 void ob_stats(const ob_world* w, int32_t* out8);
+/* manifolds the last step's contact reduction marked (SF_P_RED_K > 0; gpu/gpu_world.js:834-920) */
+int ob_reduced(const ob_world* w);
 /* last step's candidate list in canonical order: (a, b) with b = -1 for the ground; returns the count */
 int ob_candidates(const ob_world* w, int32_t* out, int cap);
 /* last step's manifolds: keys (a, b) and point counts; plus per manifold: normal[3], npts, and up to 4 points of

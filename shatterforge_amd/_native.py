@@ -58,6 +58,16 @@ _PROTOS = {
     "sf_body_pos": (c_float, [c_uint, c_uint]),
     "sf_pose": (ctypes.POINTER(c_float), []),
     "sf_scene_pile": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_build_body": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sf_hull_count": (c_int, []),
+    "sf_hull_pattern_count": (c_int, []),
+    "sf_hull_pattern": (c_int, [c_int, c_void_p]),
+    "sf_hull_cell": (c_int, [c_int, c_void_p]),
+    "sf_hull_vertices": (c_int, [c_int, c_int, c_void_p]),
+    "sf_hull_face": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "sf_hull_edges": (c_int, [c_int, c_void_p, c_void_p]),
+    "sf_hull_tables": (c_int, [c_void_p] * 7),
+    "sf_hull_shapes": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "sf_scene_stack": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sf_scene_rubble": (c_int, [c_int, c_float, c_int, c_float, c_float, c_float, c_float, c_float, c_uint,
                                 c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -21,6 +21,7 @@
 #include "sf_dev.h"
 #include "sf_host.h"
 #include "sf_scan.h"
+#include "hulls.h"
 #include "../../include/sf.h"
 #include <vector>
 #include <string>
@@ -32,12 +33,14 @@
 namespace {
 
 constexpr int NT = 256;  // threads per block of the grid-stride kernels
+constexpr int sf_hull_maxv = sf::HULL_MAXV, sf_hull_maxf = sf::HULL_MAXF, sf_hull_maxfv = sf::HULL_MAXFV,
+              sf_hull_maxs = sf::HULL_MAXS, sf_hull_maxd = sf::HULL_MAXD;
 
 // device counters
 // (C_ACTIVE0 + g: set when a manifold claimed in the rounds the host checks as group g)
 // (C_NREST: manifolds of the step that may bounce, a restitution and a point approaching past the threshold)
 enum { C_NE, C_NCAND, C_NMAN, C_NPTS, C_NPAIR, C_NGROUND, C_NWAKE, C_NOVF, C_NCOL, C_DROP, C_AWAKE, C_NSLOT, C_NBIGC, C_NBIGL,
-       C_NREST, C_ACTIVE0, C_NACTIVE = 16, C_NCTR = C_ACTIVE0 + C_NACTIVE };
+       C_NREST, C_NRED, C_NREDP, C_NHULL, C_ACTIVE0, C_NACTIVE = 16, C_NCTR = C_ACTIVE0 + C_NACTIVE };
 
 // manifold rows (each row is a float4 array of mcap entries)
 // (the inertia terms, a body's world inverse inertia times an arm or the normal, are not stored: the solve forms them
@@ -67,7 +70,11 @@ struct StepK {
     // threshold and iterations, whether any shape bounces, support-only colouring rounds
     float ratio, rest_thr;
     int shock, rest_iters, any_rest, col_support;
+    int red_k;  // contact reduction: body pairs with more manifolds than this are reduced (0 off, else at least 8)
 };
+
+// the chunk library's f32 tables on the device (hull h's entries at h times each field's stride, as sf::HullTables)
+struct HullDev { const float4 *v, *pl, *dir; const uint8_t *fidx, *fcnt, *seg; const int4* head; };
 
 struct Dev {
     int nb, ns;
@@ -85,6 +92,9 @@ struct Dev {
     int32_t* shb;
     float4 *slp, *slr, *shh, *swp, *swr, *slo, *shi;  // slp: lp|reach, slr/swr: 3 per shape, shh: h|mu
     float* srest;                                     // restitution per shape
+    int32_t* shull;                                   // hull id per shape (0: the whole box)
+    HullDev hl;
+    uint32_t* hlist;                                  // this step's candidates with a convex chunk
     // broadphase
     uint32_t *ecnt, *eoff, *ekey;
     int32_t* eshape;
@@ -99,6 +109,7 @@ struct Dev {
     uint32_t *ccnt, *coff;
     int2* cand;
     float4* nar;  // 5 per candidate: n|count, points p|sep
+    float4* nar2; // 5 per shape: a chunk's second patch of ground corners (its ground candidate has two manifolds)
     uint32_t *hit, *hoff;
     ManBuf cur, old;
     float4* wrec;  // by body contact-list entry, WREC_F4 float4: the warm start's velocity and spin change, then the
@@ -114,6 +125,12 @@ struct Dev {
     uint32_t *blinv, *minv;  // the list entry of (colour position, side), of (manifold, side); ~0u: the static world
     // shock-propagation heights per solver slot (255 - level), the two copies the level passes alternate between
     uint32_t *lvl, *lvl2;
+    // contact reduction: body-pair table (key, manifold count, the pair's place among the reduced pairs + 1), each
+    // manifold's table slot and reduced flag, and per reduced pair the RED_D extreme picks (score << 32 | manifold)
+    unsigned long long* rkey;
+    uint32_t *rcnt, *rbig, *rslot, *rflag;
+    unsigned long long* rext;
+    uint32_t rmask;
     // islands
     int32_t* parent;
     uint32_t* sst;     // sleep state bits per body (SS_*)
@@ -671,10 +688,12 @@ __device__ __forceinline__ int collide_boxes(V3 pa, M3 ra, V3 ha, V3 pb, M3 rb, 
     return cnt;
 }
 
-__global__ void __launch_bounds__(NT) k_narrow(Dev D, StepK K) {
+// (with chunks in the world, candidates with one go to the list k_narrow_hull takes; their outputs come from there)
+__global__ void __launch_bounds__(NT) k_narrow(Dev D, StepK K, int any_hull) {
     uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
     GRID_STRIDE(i, nc) {
         int2 c = D.cand[i];
+        if (any_hull && (D.shull[c.x] || (c.y >= 0 && D.shull[c.y]))) { D.hlist[atomicAdd(&D.ctr[C_NHULL], 1u)] = i; continue; }
         Pt pts[8];
         V3 n = v3(0, 0, 0);
         int cnt = 0;
@@ -720,6 +739,229 @@ __global__ void __launch_bounds__(NT) k_narrow(Dev D, StepK K) {
     }
 }
 
+
+// ---------------------------------------------------------------- convex chunks (gpu/gpu_world.js:402-530)
+// A shape with hull t > 0 is cell t of the chunk library in its box, stretched by the shape's half size: vertex k in
+// the world is p + R (h * v_k); a face's world plane is R (n / h) normalised, offset by its d and the position. A box
+// (t = 0) paired with a chunk takes the same path as the whole-cube cell. Per-lane arrays (corners, directions, clip
+// polygons) are indexed at run time: this path serves the chunk pairs only, listed apart from the box pairs.
+struct HS { V3 p; M3 R; V3 h; int t; };
+__device__ __forceinline__ V3 hull_w(const HullDev& H, const HS& s, int k) {
+    const V3 v = v3f4(H.v[s.t * sf_hull_maxv + k]);
+    return vadd(s.p, mulv(s.R, v3(s.h.x * v.x, s.h.y * v.y, s.h.z * v.z)));
+}
+__device__ __forceinline__ float4 hull_face(const HullDev& H, const HS& s, int f) {
+    const float4 pl = H.pl[s.t * sf_hull_maxf + f];
+    const V3 nw = mulv(s.R, v3(pl.x / s.h.x, pl.y / s.h.y, pl.z / s.h.z));
+    const float l = vlen(nw);
+    const V3 n = vmul(nw, 1.0f / l);
+    return f4(n, pl.w / l + vdot(n, s.p));
+}
+__device__ __forceinline__ int hull_fv(const HullDev& H, const HS& s, int f, int i) { return H.fidx[(s.t * sf_hull_maxf + f) * sf_hull_maxfv + i]; }
+__device__ __forceinline__ int hull_fn(const HullDev& H, const HS& s, int f) { return H.fcnt[s.t * sf_hull_maxf + f]; }
+__device__ __forceinline__ V3 hull_dir(const HullDev& H, const HS& s, int k) {
+    const V3 d = v3f4(H.dir[s.t * sf_hull_maxd + k]);
+    return vnorm(mulv(s.R, v3(s.h.x * d.x, s.h.y * d.y, s.h.z * d.z)));
+}
+// keep up to 8 points; past 8, a deeper point replaces the shallowest
+__device__ __forceinline__ void keep8(Pt* pts, int& cnt, Pt t) {
+    if (cnt < 8) { pts[cnt++] = t; return; }
+    int hi = 0;
+    for (int q = 1; q < 8; q++) if (pts[q].s > pts[hi].s) hi = q;
+    if (t.s < pts[hi].s) pts[hi] = t;
+}
+// the chunk's corners below the margin (normal from the chunk into the ground)
+__device__ int hull_ground(const HullDev& H, const HS& s, float margin, Pt* pts) {
+    const int nv = H.head[s.t].x;
+    int cnt = 0;
+    for (int k = 0; k < nv; k++) {
+        const V3 x = hull_w(H, s, k);
+        if (!(x.y < margin)) continue;
+        Pt t; t.p = v3(x.x, 0.5f * x.y, x.z); t.s = x.y;
+        keep8(pts, cnt, t);
+    }
+    return cnt;
+}
+// Separating axes: both hulls' face normals, then the cross products of their edge directions (the sign that gives
+// the larger separation). The best face (B's only if clearly better) gives a reference face whose side planes clip
+// the other hull's face most against it; an edge axis clearly better than both gives the closest points of the two
+// support edges. Normal from A to B; a point's separation is negative in overlap.
+__device__ int hull_hull(const HullDev& H, const HS& A, const HS& B, float margin, Pt* pts, V3& n_out) {
+    const int4 ha = H.head[A.t], hb = H.head[B.t];
+    V3 VA[sf_hull_maxv], VB[sf_hull_maxv], DA[sf_hull_maxd], DB[sf_hull_maxd];
+    for (int k = 0; k < ha.x; k++) VA[k] = hull_w(H, A, k);
+    for (int k = 0; k < hb.x; k++) VB[k] = hull_w(H, B, k);
+    for (int k = 0; k < ha.w; k++) DA[k] = hull_dir(H, A, k);
+    for (int k = 0; k < hb.w; k++) DB[k] = hull_dir(H, B, k);
+    float sa = -3.40282347e38f, sb = -3.40282347e38f, se = -3.40282347e38f;
+    int fa = 0, fb = 0, ei = 0, ej = 0;
+    V3 el = v3(0, 0, 0);
+    for (int f = 0; f < ha.y; f++) {
+        const float4 pl = hull_face(H, A, f);
+        float lo = 3.40282347e38f;
+        for (int k = 0; k < hb.x; k++) lo = sf_min(lo, vdot(VB[k], v3f4(pl)));
+        const float sp = lo - pl.w;
+        if (sp > margin) return 0;
+        if (sp > sa) { sa = sp; fa = f; }
+    }
+    for (int f = 0; f < hb.y; f++) {
+        const float4 pl = hull_face(H, B, f);
+        float lo = 3.40282347e38f;
+        for (int k = 0; k < ha.x; k++) lo = sf_min(lo, vdot(VA[k], v3f4(pl)));
+        const float sp = lo - pl.w;
+        if (sp > margin) return 0;
+        if (sp > sb) { sb = sp; fb = f; }
+    }
+    for (int i = 0; i < ha.w; i++)
+        for (int j = 0; j < hb.w; j++) {
+            const V3 l0 = vcross(DA[i], DB[j]);
+            const float l2 = vdot(l0, l0);
+            if (l2 < 1e-6f) continue;
+            V3 l = vmul(l0, 1.0f / sf_sqrt(l2));
+            float alo = 3.40282347e38f, ahi = -3.40282347e38f, blo = 3.40282347e38f, bhi = -3.40282347e38f;
+            for (int k = 0; k < ha.x; k++) { const float d = vdot(VA[k], l); alo = sf_min(alo, d); ahi = sf_max(ahi, d); }
+            for (int k = 0; k < hb.x; k++) { const float d = vdot(VB[k], l); blo = sf_min(blo, d); bhi = sf_max(bhi, d); }
+            const float s1 = blo - ahi, s2 = alo - bhi;
+            float sp = s1;
+            if (s2 > s1) { sp = s2; l = vneg(l); }
+            if (sp > margin) return 0;
+            if (sp > se) { se = sp; el = l; ei = i; ej = j; }
+        }
+    const bool use_b = sb > sa + 0.005f;
+    const float fbest = use_b ? sb : sa;
+    if (se > fbest + 0.01f) {
+        // edge against edge: each hull's edge along the axis's direction furthest out (A) / in (B) along the axis
+        const V3 da = DA[ei], db = DB[ej];
+        V3 a0 = A.p, a1 = A.p, b0 = B.p, b1 = B.p;
+        float best = -3.40282347e38f;
+        for (int k = 0; k < ha.z; k++) {
+            const int x = H.seg[(A.t * sf_hull_maxs + k) * 2], y = H.seg[(A.t * sf_hull_maxs + k) * 2 + 1];
+            const V3 p0 = VA[x], p1 = VA[y];
+            if (sf_abs(vdot(vnorm(vsub(p1, p0)), da)) < 0.999f) continue;
+            const float m = vdot(vadd(p0, p1), el);
+            if (m > best) { best = m; a0 = p0; a1 = p1; }
+        }
+        best = 3.40282347e38f;
+        for (int k = 0; k < hb.z; k++) {
+            const int x = H.seg[(B.t * sf_hull_maxs + k) * 2], y = H.seg[(B.t * sf_hull_maxs + k) * 2 + 1];
+            const V3 p0 = VB[x], p1 = VB[y];
+            if (sf_abs(vdot(vnorm(vsub(p1, p0)), db)) < 0.999f) continue;
+            const float m = vdot(vadd(p0, p1), el);
+            if (m < best) { best = m; b0 = p0; b1 = p1; }
+        }
+        const V3 d1 = vsub(a1, a0), d2 = vsub(b1, b0), r = vsub(a0, b0);
+        const float aa = vdot(d1, d1), ee = vdot(d2, d2), ff = vdot(d2, r), cc = vdot(d1, r), bb = vdot(d1, d2);
+        const float den = aa * ee - bb * bb;
+        float s1 = 0.0f;
+        if (den > 1e-9f) s1 = sf_min(sf_max((bb * ff - cc * ee) / den, 0.0f), 1.0f);
+        float t1 = (bb * s1 + ff) / sf_max(ee, 1e-9f);
+        if (t1 < 0.0f) { t1 = 0.0f; s1 = sf_min(sf_max(-cc / sf_max(aa, 1e-9f), 0.0f), 1.0f); }
+        else if (t1 > 1.0f) { t1 = 1.0f; s1 = sf_min(sf_max((bb - cc) / sf_max(aa, 1e-9f), 0.0f), 1.0f); }
+        const V3 p1 = vadd(a0, vmul(d1, s1)), p2 = vadd(b0, vmul(d2, t1));
+        pts[0].p = vmul(vadd(p1, p2), 0.5f);
+        pts[0].s = vdot(vsub(p2, p1), el);
+        n_out = el;
+        return 1;
+    }
+    // a face: reference Rs (the hull whose face it is), incident Is (the other's face most against its normal)
+    const HS& Rs = use_b ? B : A;
+    const HS& Is = use_b ? A : B;
+    const int fr = use_b ? fb : fa, nfi = use_b ? ha.y : hb.y;
+    const float4 rpl = hull_face(H, Rs, fr);
+    const V3 n = v3f4(rpl);
+    int fi = 0;
+    float bd = 3.40282347e38f;
+    for (int f = 0; f < nfi; f++) { const float d = vdot(v3f4(hull_face(H, Is, f)), n); if (d < bd) { bd = d; fi = f; } }
+    V3 poly[16], tmp[16];
+    int pn = hull_fn(H, Is, fi);
+    for (int q = 0; q < pn; q++) poly[q] = hull_w(H, Is, hull_fv(H, Is, fi, q));
+    const int rn = hull_fn(H, Rs, fr);
+    V3 rc = v3(0, 0, 0);
+    for (int q = 0; q < rn; q++) rc = vadd(rc, hull_w(H, Rs, hull_fv(H, Rs, fr, q)));
+    rc = vmul(rc, 1.0f / (float)rn);
+    for (int e = 0; e < rn; e++) {
+        const V3 v0 = hull_w(H, Rs, hull_fv(H, Rs, fr, e)), v1 = hull_w(H, Rs, hull_fv(H, Rs, fr, (e + 1) % rn));
+        V3 sn = vnorm(vcross(vsub(v1, v0), n));
+        if (vdot(sn, vsub(rc, v0)) > 0.0f) sn = vneg(sn);
+        int tn = 0;
+        for (int q = 0; q < pn; q++) {
+            const V3 p0 = poly[q], p1 = poly[(q + 1) % pn];
+            const float d0 = vdot(vsub(p0, v0), sn), d1 = vdot(vsub(p1, v0), sn);
+            if (d0 <= 0.0f && tn < 16) tmp[tn++] = p0;
+            if ((d0 <= 0.0f) != (d1 <= 0.0f) && tn < 16) tmp[tn++] = vadd(p0, vmul(vsub(p1, p0), d0 / (d0 - d1)));
+        }
+        pn = tn;
+        for (int q = 0; q < pn; q++) poly[q] = tmp[q];
+        if (pn == 0) return 0;
+    }
+    int cnt = 0;
+    for (int q = 0; q < pn; q++) {
+        const float sep = vdot(poly[q], n) - rpl.w;
+        if (sep > margin) continue;
+        Pt t; t.p = vsub(poly[q], vmul(n, 0.5f * sep)); t.s = sep;
+        keep8(pts, cnt, t);
+    }
+    n_out = use_b ? vneg(n) : n;
+    return cnt;
+}
+
+// the listed chunk candidates, one lane each: the box narrow phase's margins, wakes and reduce4, chunk geometry
+__global__ void __launch_bounds__(NT) k_narrow_hull(Dev D, StepK K) {
+    const uint32_t nl = min(D.ctr[C_NHULL], D.ccap);
+    GRID_STRIDE(li, nl) {
+        const uint32_t i = D.hlist[li];
+        const int2 c = D.cand[i];
+        Pt pts[8];
+        V3 n = v3(0, 0, 0);
+        int cnt = 0;
+        const int ba = D.shb[c.x];
+        int slot_b = 0;
+        HS A;
+        A.p = v3f4(D.swp[c.x]); A.R = load_m3(D.swr + 3 * (size_t)c.x); A.h = v3f4(D.shh[c.x]); A.t = D.shull[c.x];
+        if (c.y < 0) {
+            // (a chunk's margin also covers its corners' sweep from spin, gpu/gpu_world.js ground_margin :776-778)
+            const float margin = K.spec + (sf_max(-D.bv[ba].y, 0.0f) + vlen(v3f4(D.bw[ba])) * D.slp[c.x].w) * (1.0f / 60.0f);
+            cnt = hull_ground(D.hl, A, margin, pts);
+            n = v3(0.0f, -1.0f, 0.0f);
+            // up to 8 corners as two manifolds of 4 in the order found (gpu/gpu_world.js narrow_hull :813-823: one
+            // patch of 4 left the other corners free to swing into the ground); no reduce4
+            const bool hit = cnt > 0 && D.bslot[ba] != 0;
+            D.hit[i] = hit ? (cnt > 4 ? 2u : 1u) : 0u;
+            float4* o = D.nar + 5 * (size_t)i;
+            o[0] = f4(n, (float)min(cnt, 4));
+            for (int k = 0; k < 4; k++) if (k < cnt) o[1 + k] = f4(pts[k].p, pts[k].s);
+            if (cnt > 4) {
+                float4* o2 = D.nar2 + 5 * (size_t)c.x;
+                o2[0] = f4(n, (float)(cnt - 4));
+                for (int k = 4; k < 8; k++) if (k < cnt) o2[k - 3] = f4(pts[k].p, pts[k].s);
+            }
+            continue;
+        } else {
+            const int bb = D.shb[c.y];
+            slot_b = D.bslot[bb];
+            const V3 va = v3f4(D.bv[ba]), vb = v3f4(D.bv[bb]);
+            const float rel = vlen(vsub(vb, va)) + vlen(v3f4(D.bw[ba])) * D.slp[c.x].w + vlen(v3f4(D.bw[bb])) * D.slp[c.y].w;
+            const float margin = K.spec + rel * (1.0f / 60.0f);
+            HS B;
+            B.p = v3f4(D.swp[c.y]); B.R = load_m3(D.swr + 3 * (size_t)c.y); B.h = v3f4(D.shh[c.y]); B.t = D.shull[c.y];
+            cnt = hull_hull(D.hl, A, B, margin, pts, n);
+            if (cnt) {
+                const int fa = D.bflag[ba], fb = D.bflag[bb];
+                const bool a_awake = fa & 1, b_awake = fb & 1;
+                const bool a_still = D.bii[ba].w != 0.0f, b_still = D.bii[bb].w != 0.0f;
+                if (a_awake && !b_awake && !a_still) D.wake[bb] = 1u;
+                if (b_awake && !a_awake && !b_still) D.wake[ba] = 1u;
+            }
+        }
+        if (cnt) cnt = reduce4(pts, cnt, n);
+        const bool hit = cnt > 0 && !(D.bslot[ba] == 0 && slot_b == 0);
+        D.hit[i] = hit ? 1u : 0u;
+        float4* o = D.nar + 5 * (size_t)i;
+        o[0] = f4(n, (float)cnt);
+        for (int k = 0; k < 4; k++) if (k < cnt) o[1 + k] = f4(pts[k].p, pts[k].s);
+    }
+}
+
 __global__ void __launch_bounds__(NT) k_apply_wake(Dev D) {
     GRID_STRIDE(i, (uint32_t)D.nb) {
         if (!D.wake[i]) continue;
@@ -730,6 +972,12 @@ __global__ void __launch_bounds__(NT) k_apply_wake(Dev D) {
         D.bsq[i] = D.bq[i];
         atomicAdd(&D.ctr[C_NWAKE], 1u);
     }
+}
+
+// the j-th manifold of candidate i (hit[i] of them): its normal|count and 4 points (j = 1: a chunk's second patch of
+// ground corners, kept by shape)
+__device__ __forceinline__ const float4* cand_pts(const Dev& D, uint32_t i, uint32_t j) {
+    return j ? D.nar2 + 5 * (size_t)D.cand[i].x : D.nar + 5 * (size_t)i;
 }
 
 __device__ int find_old(const Dev& D, uint64_t key, uint32_t nold) {
@@ -789,13 +1037,13 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
     uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
     uint32_t npts = 0;  // this thread's contact points (the block's sum goes to ptpart, not to one global counter)
     uint32_t nbounce = 0;  // this thread's manifolds that may bounce
-    GRID_STRIDE(i, nc) {
-        if (!D.hit[i]) continue;
-        uint32_t m = D.hoff[i];
+    GRID_STRIDE(i, nc) for (uint32_t j = 0; j < D.hit[i]; j++) {
+        uint32_t m = D.hoff[i] + j;
         if (m >= D.mcap) { atomicAdd(&D.ctr[C_DROP], 1u); continue; }
         const uint32_t p = D.cur.pos[m];  // its place in colour order
+        const bool red = D.mcol[m] == -2;  // reduced: built for the next step's warm start only
         int2 c = D.cand[i];
-        int a = c.x, b = c.y;
+        int a = c.x, b = j ? -2 : c.y;  // (-2: a chunk's second patch of ground corners)
         int ba = D.shb[a];
         int bb = b >= 0 ? D.shb[b] : -1;
         V3 pa = v3f4(D.bp[ba]);
@@ -804,7 +1052,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         M3 rbm;
         if (bb >= 0) rbm = load_m3(D.brot + 3 * (size_t)bb);
         int sa = D.bslot[ba], sb = bb >= 0 ? D.bslot[bb] : 0;
-        const float4* nr = D.nar + 5 * (size_t)i;
+        const float4* nr = cand_pts(D, i, j);
         float4 n4 = nr[0];
         V3 n = v3f4(n4);
         int cnt = (int)n4.w;
@@ -897,7 +1145,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
             // the point's normal velocity at the step start (negative: approaching), gpu_world.js:662
             const V3 pva = vadd(v3f4(sva), vcross(swa, rap)), pvb = vadd(v3f4(svb), vcross(swb, rbp));
             const float rv = vdot(vsub(pvb, pva), n);
-            if (rest > 0.0f && !(rv > -K.rest_thr)) bounce = true;
+            if (rest > 0.0f && !(rv > -K.rest_thr) && !red) bounce = true;
             MROW(D, C, MR_P0 + MR_PER_PT * k + 0, p) = f4(raxn, pts[k].s);
             MROW(D, C, MR_P0 + MR_PER_PT * k + 1, p) = f4(rbxn, nm);
             iaxk[k] = iaxn; ibxk[k] = ibxn; jnk[k] = jn;
@@ -928,7 +1176,7 @@ __global__ void __launch_bounds__(NT) k_build(Dev D, StepK K, uint32_t nold) {
         nbounce += bounce ? 1u : 0u;
         // the bodies' list entries of this position, their first warm-start records (from the impulses carried over
         // from the last step) and their overflow changes cleared
-        const uint32_t ea = sa ? D.minv[2 * m] : ~0u, eb = sb ? D.minv[2 * m + 1] : ~0u;
+        const uint32_t ea = sa && !red ? D.minv[2 * m] : ~0u, eb = sb && !red ? D.minv[2 * m + 1] : ~0u;
         D.blinv[2 * p] = ea;
         D.blinv[2 * p + 1] = eb;
         write_warm(D, ea, eb, cnt, f4(n, mu), f4(t1, tm1), f4(t2, tm2), jt1, jt2, jtw, jnk, iaxk, ibxk, ia1, ia2, ian, ib1,
@@ -962,27 +1210,119 @@ __global__ void __launch_bounds__(NT) k_sum_points(Dev D, uint32_t nparts) {
     }
 }
 
-// the manifolds' headers (shapes, body slots) and keys, and the bodies' contact counts, before colouring
-__global__ void __launch_bounds__(NT) k_man_hdr(Dev D) {
+// the manifolds' headers (shapes, body slots) and keys, and the bodies' contact counts (reduced manifolds take no
+// place in the bodies' lists), before colouring
+__global__ void __launch_bounds__(NT) k_man_hdr(Dev D, StepK K) {
     uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
-    GRID_STRIDE(i, nc) {
-        if (!D.hit[i]) continue;
-        uint32_t m = D.hoff[i];
+    GRID_STRIDE(i, nc) for (uint32_t j = 0; j < D.hit[i]; j++) {
+        uint32_t m = D.hoff[i] + j;
         if (m >= D.mcap) continue;
         int2 c = D.cand[i];
+        const int cb = j ? -2 : c.y;
         int ba = D.shb[c.x];
-        int sa = D.bslot[ba], sb = c.y >= 0 ? D.bslot[D.shb[c.y]] : 0;
-        D.cur.hdr[m] = make_int4(c.x, c.y, sa, sb);
-        D.cur.key[m] = man_key(c.x, c.y);
+        int sa = D.bslot[ba], sb = cb >= 0 ? D.bslot[D.shb[cb]] : 0;
+        D.cur.hdr[m] = make_int4(c.x, cb, sa, sb);
+        D.cur.key[m] = man_key(c.x, cb);
+        if (K.red_k && D.rflag[m]) continue;
         if (sa) atomicAdd(&D.blcnt[sa], 1u);
         if (sb) atomicAdd(&D.blcnt[sb], 1u);
     }
 }
 
+// ---------------------------------------------------------------- contact reduction (gpu/gpu_world.js:834-920)
+// A body pair (the ground counting as one body) touching through more than red_k manifolds keeps only representative
+// ones: for each of the 26 directions to the neighbours of a cube cell the manifold whose points' centre lies furthest
+// out, and the deepest (least separation); ties go to the later manifold. The rest are reduced: not coloured, not
+// solved, not in the warm start, but built into the step's records (after the overflow) so the next step's warm
+// start still finds their impulses. Three passes over the manifolds: count per pair in an open-addressing table
+// (the pair's (red_k + 1)-th manifold hands the pair its place among the reduced pairs), score with 64-bit atomic
+// maxima of (ordered score << 32 | manifold), pick. Which thread wins a race changes no result.
+constexpr int RED_D = 27;  // 26 directions and the deepest
+constexpr int RED_KMIN = 8;  // smaller thresholds act as this (it bounds the reduced pairs at a ninth of the manifolds)
+__device__ __forceinline__ uint64_t red_pair(const Dev& D, int2 c) {
+    const uint32_t ba = (uint32_t)D.shb[c.x], bb = c.y >= 0 ? (uint32_t)D.shb[c.y] : 0xFFFFFFFFu;
+    return ((uint64_t)ba << 32) | bb;
+}
+// a float's bits as an unsigned integer in the float's order
+__device__ __forceinline__ uint32_t f2o(float x) { const uint32_t b = __float_as_uint(x); return (b & 0x80000000u) ? ~b : (b | 0x80000000u); }
+// direction k of 26: the cell offsets j = 0..26 in x-fastest order, the centre (j = 13) skipped
+__device__ __forceinline__ V3 red_dir(int k) {
+    const int j = k >= 13 ? k + 1 : k;
+    return v3((float)(j % 3) - 1.0f, (float)((j / 3) % 3) - 1.0f, (float)(j / 9) - 1.0f);
+}
+__global__ void __launch_bounds__(NT) k_red_count(Dev D, StepK K) {
+    const uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
+    GRID_STRIDE(i, nc) for (uint32_t j = 0; j < D.hit[i]; j++) {
+        const uint32_t m = D.hoff[i] + j;
+        if (m >= D.mcap) continue;
+        D.rflag[m] = 0u;
+        const uint64_t key = red_pair(D, D.cand[i]);
+        // (the table has at least twice as many slots as there are manifolds, so a free slot is always found)
+        uint32_t h = mix32((uint32_t)key * 0x9E3779B1u ^ mix32((uint32_t)(key >> 32))) & D.rmask;
+        while (true) {
+            const unsigned long long old = atomicCAS(&D.rkey[h], ~0ull, (unsigned long long)key);
+            if (old == ~0ull || old == key) break;
+            h = (h + 1u) & D.rmask;
+        }
+        D.rslot[m] = h;
+        if (atomicAdd(&D.rcnt[h], 1u) == (uint32_t)K.red_k) {
+            const uint32_t b = atomicAdd(&D.ctr[C_NREDP], 1u);
+            D.rbig[h] = b + 1u;
+            for (int k = 0; k < RED_D; k++) D.rext[(size_t)b * RED_D + k] = 0ull;
+        }
+    }
+}
+__global__ void __launch_bounds__(NT) k_red_score(Dev D) {
+    if (!D.ctr[C_NREDP]) return;
+    const uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
+    GRID_STRIDE(i, nc) for (uint32_t j = 0; j < D.hit[i]; j++) {
+        const uint32_t m = D.hoff[i] + j;
+        if (m >= D.mcap) continue;
+        const uint32_t b = D.rbig[D.rslot[m]];
+        if (!b) continue;
+        const float4* nr = cand_pts(D, i, j);
+        const int cnt = (int)nr[0].w;
+        V3 c = v3(0, 0, 0);
+        float smin = nr[1].w;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k >= cnt) continue;
+            const float4 q = nr[1 + k];
+            c = vadd(c, v3f4(q));
+            if (k > 0) smin = q.w < smin ? q.w : smin;
+        }
+        c = vmul(c, 1.0f / (float)cnt);
+        unsigned long long* e = D.rext + (size_t)(b - 1u) * RED_D;
+        for (int k = 0; k < RED_D - 1; k++) atomicMax(&e[k], ((unsigned long long)f2o(vdot(c, red_dir(k))) << 32) | m);
+        atomicMax(&e[RED_D - 1], ((unsigned long long)f2o(-smin) << 32) | m);
+    }
+}
+__global__ void __launch_bounds__(NT) k_red_pick(Dev D) {
+    if (!D.ctr[C_NREDP]) return;
+    const uint32_t nc = min(D.ctr[C_NCAND], D.ccap);
+    uint32_t nred = 0;
+    GRID_STRIDE(i, nc) for (uint32_t j = 0; j < D.hit[i]; j++) {
+        const uint32_t m = D.hoff[i] + j;
+        if (m >= D.mcap) continue;
+        const uint32_t b = D.rbig[D.rslot[m]];
+        if (!b) continue;
+        const unsigned long long* e = D.rext + (size_t)(b - 1u) * RED_D;
+        bool kept = false;
+        for (int k = 0; k < RED_D; k++) kept |= (uint32_t)e[k] == m;
+        if (!kept) { D.rflag[m] = 1u; nred++; }
+    }
+    block_count(nred, &D.ctr[C_NRED]);
+}
+
 // ---------------------------------------------------------------- colouring
-__global__ void __launch_bounds__(NT) k_col_init(Dev D) {
+// (with reduction on, the pair table's slots the step used are cleared here, after the last pass that reads them)
+__global__ void __launch_bounds__(NT) k_col_init(Dev D, StepK K) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
-    GRID_STRIDE(m, nm) D.mcol[m] = -1;
+    GRID_STRIDE(m, nm) {
+        const bool red = K.red_k && D.rflag[m];
+        D.mcol[m] = red ? -2 : -1;
+        if (K.red_k) { const uint32_t h = D.rslot[m]; D.rkey[h] = ~0ull; D.rcnt[h] = 0u; D.rbig[h] = 0u; }
+    }
 }
 
 // (the activity flag of the round's group is set by one plain store per wavefront that claimed: tens of thousands of
@@ -1042,7 +1382,7 @@ __global__ void __launch_bounds__(NT) k_col_resolve(Dev D, int K) {
 // axis taken modulo 4, so the contiguous eighth of the bucket that one XCD solves covers a compact region and its
 // bodies' records are reused from that XCD's L2. Within the overflow (a Jacobi pass) the order changes no result.
 constexpr int OVS = 64;
-constexpr int NBIN = 64 + OVS;  // colour bins (up to 64 colours) and the overflow's sub-buckets
+constexpr int NBIN = 64 + OVS + 1;  // colour bins (up to 64 colours), the overflow's sub-buckets, the reduced manifolds
 __device__ __forceinline__ uint32_t ovf_key(const Dev& D, int sa, int sb) {
     const float4 p = D.sp0[sa ? sa : sb];
     const uint32_t x = (uint32_t)(int)floorf(p.x * 0.25f) & 3u, y = (uint32_t)(int)floorf(p.y * 0.25f) & 3u,
@@ -1056,28 +1396,31 @@ __global__ void __launch_bounds__(NT) k_col_count(Dev D, int K) {
     __shared__ uint32_t hist[NBIN];
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     for (uint32_t b0 = blockIdx.x * NT; b0 < nm; b0 += gridDim.x * NT) {
-        for (int c = threadIdx.x; c < K + OVS; c += NT) hist[c] = 0u;
+        for (int c = threadIdx.x; c < K + OVS + 1; c += NT) hist[c] = 0u;
         __syncthreads();
         uint32_t m = b0 + threadIdx.x;
         if (m < nm) {
             int c = D.mcol[m];
-            if (c < 0) { c = K; D.mcol[m] = K; }
-            if (c == K) {
-                const int4 hd = D.cur.hdr[m];
-                c = K + (int)ovf_key(D, hd.z, hd.w);
+            if (c == -2) c = K + OVS;  // (reduced: after the overflow)
+            else {
+                if (c < 0) { c = K; D.mcol[m] = K; }
+                if (c == K) {
+                    const int4 hd = D.cur.hdr[m];
+                    c = K + (int)ovf_key(D, hd.z, hd.w);
+                }
             }
             atomicAdd(&hist[c], 1u);
             D.mbin[m] = (uint32_t)c;
         }
         __syncthreads();
         const uint32_t nchunk = (D.mcap + NT - 1) / NT, chunk = b0 / NT;
-        for (int c = threadIdx.x; c < K + OVS; c += NT) D.chunkc[(size_t)c * nchunk + chunk] = hist[c];
+        for (int c = threadIdx.x; c < K + OVS + 1; c += NT) D.chunkc[(size_t)c * nchunk + chunk] = hist[c];
         __syncthreads();
     }
 }
 
 // per bin, the exclusive scan of its chunk counts in chunk order (one block per bin): a colour's total into its
-// bucket count, an overflow sub-bucket's into subcnt
+// bucket count, an overflow sub-bucket's into subcnt, the reduced manifolds' into subcnt[OVS]
 __global__ void __launch_bounds__(NT) k_col_chunk_scan(Dev D, int K) {
     __shared__ uint32_t ws[NT / 64];
     const int c = blockIdx.x;
@@ -1129,7 +1472,7 @@ __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
         if (c >= 0) {
             uint32_t before = 0;
             for (int k = 0; k < wv; k++) before += wc[k][c];
-            const uint32_t base = c < K ? D.coloff[c] : D.coloff[K] + D.suboff[c - K];
+            const uint32_t base = c < K ? D.coloff[c] : (c < K + OVS ? D.coloff[K] + D.suboff[c - K] : D.coloff[K] + D.colcnt[K]);
             D.cur.pos[m] = base + D.chunkc[(size_t)c * nchunk + b0 / NT] + before + rank;
         }
         __syncthreads();
@@ -1140,6 +1483,7 @@ __global__ void __launch_bounds__(NT) k_col_fill(Dev D, int K) {
 __global__ void __launch_bounds__(NT) k_bl_fill(Dev D) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(m, nm) {
+        if (D.mcol[m] == -2) continue;
         int4 hd = D.cur.hdr[m];
         if (hd.z) D.blist[D.bloff[hd.z] + atomicAdd(&D.blfill[hd.z], 1u)] = m * 2u;
         if (hd.w) D.blist[D.bloff[hd.w] + atomicAdd(&D.blfill[hd.w], 1u)] = m * 2u + 1u;
@@ -1695,7 +2039,7 @@ __global__ void __launch_bounds__(NT) k_level_copy(Dev D, const uint32_t* rd, ui
     GRID_STRIDE(s, (uint32_t)D.nb + 1u) wr[s] = rd[s];
 }
 __global__ void __launch_bounds__(NT) k_level_lift(Dev D, const uint32_t* rd, uint32_t* wr) {
-    const uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
+    const uint32_t nm = min(D.ctr[C_NMAN], D.mcap) - D.ctr[C_NRED];  // (the reduced manifolds: the last positions)
     GRID_STRIDE(p, nm) {
         const float ny = MROW(D, D.cur, MR_N, p).y;
         if (!(ny > 0.7f) && !(ny < -0.7f)) continue;
@@ -1969,6 +2313,7 @@ __global__ void __launch_bounds__(NT) k_uf_jump(Dev D, int all) {
 __global__ void __launch_bounds__(NT) k_uf_hook(Dev D, int all) {
     uint32_t nm = min(D.ctr[C_NMAN], D.mcap);
     GRID_STRIDE(m, nm) {
+        if (D.mcol[m] == -2) continue;
         int4 hd = D.cur.hdr[m];
         if (hd.z == 0 || hd.w == 0) continue;
         int a = hd.z - 1, b = hd.w - 1;
@@ -2320,6 +2665,7 @@ struct HostWorld {
     int body_cap = 0, shape_cap = 0;
     float prm[SF_NPARAM];
     bool any_rest = false;  // some loaded shape has a restitution above zero
+    bool any_hull = false;  // some loaded shape is a convex chunk
     Dev D{};
     std::vector<void*> allocs;
     uint32_t nold = 0;
@@ -2402,7 +2748,7 @@ void default_params(float* p) {
     p[SF_P_ANCHOR] = 1.0f; p[SF_P_ANCHOR_PUSH] = 3.0f; p[SF_P_ANCHOR_SLOP] = 0.0f; p[SF_P_COL_LIMIT] = 16;
     p[SF_P_COL_ROUNDS] = 32; p[SF_P_COL_SPREAD] = 4; p[SF_P_TAIL_MIN] = 1024; p[SF_P_SOLVE_NT] = 64;
     p[SF_P_RATIO] = 0.0f; p[SF_P_SHOCK] = 0.0f; p[SF_P_REST_THRESHOLD] = 1.0f; p[SF_P_REST_ITERS] = 8.0f;
-    p[SF_P_COL_SUPPORT] = 0.0f; p[SF_P_PSUB] = 4.0f; p[SF_P_PCELL] = 0.7f;
+    p[SF_P_COL_SUPPORT] = 0.0f; p[SF_P_PSUB] = 4.0f; p[SF_P_PCELL] = 0.7f; p[SF_P_RED_K] = 0.0f;
 }
 
 int world_alloc(HostWorld* w) {
@@ -2422,13 +2768,34 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.shb, ns); r |= w->alloc(D.slp, ns); r |= w->alloc(D.slr, 3 * (size_t)ns); r |= w->alloc(D.shh, ns);
     r |= w->alloc(D.swp, ns); r |= w->alloc(D.swr, 3 * (size_t)ns); r |= w->alloc(D.slo, ns); r |= w->alloc(D.shi, ns);
     r |= w->alloc(D.srest, ns); r |= w->alloc(D.lvl, n1); r |= w->alloc(D.lvl2, n1);
+    r |= w->alloc(D.shull, ns);
     r |= w->alloc(D.ecnt, ns); r |= w->alloc(D.eoff, ns); r |= w->alloc(D.ekey, D.ecap); r |= w->alloc(D.eshape, D.ecap);
     r |= w->alloc(D.tcnt, D.tcap); r |= w->alloc(D.tstart, D.tcap); r |= w->alloc(D.tfill, D.tcap); r |= w->alloc(D.tent, D.ecap); r |= w->alloc(D.tkey, D.ecap);
     r |= w->alloc(D.tidx, D.ecap); r |= w->alloc(D.soff, D.tcap); r |= w->alloc(D.slist, D.ecap);
     r |= w->alloc(D.epc, D.ecap); r |= w->alloc(D.erel, D.ecap); r |= w->alloc(D.ekeep, (size_t)ENTRY_KEEP * D.ecap);
     r |= w->alloc(D.bigc, (size_t)ns); r |= w->alloc(D.bigl, (size_t)nb + 1);
     r |= w->alloc(D.ccnt, ns); r |= w->alloc(D.coff, ns); r |= w->alloc(D.cand, D.ccap); r |= w->alloc(D.nar, 5 * (size_t)D.ccap);
-    r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap);
+    r |= w->alloc(D.hit, D.ccap); r |= w->alloc(D.hoff, D.ccap); r |= w->alloc(D.hlist, D.ccap);
+    r |= w->alloc(D.nar2, 5 * (size_t)ns);
+    {
+        // the chunk library's tables, once per world
+        const sf::HullTables& T = sf::hull_tables();
+        float4 *v = nullptr, *pl = nullptr, *dir = nullptr;
+        uint8_t *fidx = nullptr, *fcnt = nullptr, *seg = nullptr;
+        int4* head = nullptr;
+        r |= w->alloc(v, T.v4.size() / 4); r |= w->alloc(pl, T.pl4.size() / 4); r |= w->alloc(dir, T.dir4.size() / 4);
+        r |= w->alloc(fidx, T.fidx.size()); r |= w->alloc(fcnt, T.fcnt.size()); r |= w->alloc(seg, T.seg.size());
+        r |= w->alloc(head, T.head.size() / 4);
+        if (r) return r;
+        SF_HIP(hipMemcpy(v, T.v4.data(), sizeof(float) * T.v4.size(), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(pl, T.pl4.data(), sizeof(float) * T.pl4.size(), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(dir, T.dir4.data(), sizeof(float) * T.dir4.size(), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(fidx, T.fidx.data(), T.fidx.size(), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(fcnt, T.fcnt.data(), T.fcnt.size(), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(seg, T.seg.data(), T.seg.size(), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(head, T.head.data(), sizeof(int32_t) * T.head.size(), hipMemcpyHostToDevice));
+        D.hl = HullDev{v, pl, dir, fidx, fcnt, seg, head};
+    }
     for (ManBuf* M : {&D.cur, &D.old}) {
         r |= w->alloc(M->hdr, D.mcap); r |= w->alloc(M->row, (size_t)MR_ROWS * D.mcap); r |= w->alloc(M->key, D.mcap);
         r |= w->alloc(M->pos, D.mcap); r |= w->alloc(M->hdrp, D.mcap);
@@ -2439,7 +2806,15 @@ int world_alloc(HostWorld* w) {
     r |= w->alloc(D.colcnt, 72); r |= w->alloc(D.coloff, 72); r |= w->alloc(D.colfill, 72);
     r |= w->alloc(D.chunkc, NBIN * (size_t)((D.mcap + NT - 1) / NT + 1));
     r |= w->alloc(D.ptpart, 4096);  // k_build's grid: sf_grid(ccap) is at most 4096 workgroups
-    r |= w->alloc(D.mbin, D.mcap); r |= w->alloc(D.subcnt, OVS); r |= w->alloc(D.suboff, OVS);
+    r |= w->alloc(D.mbin, D.mcap); r |= w->alloc(D.subcnt, OVS + 1); r |= w->alloc(D.suboff, OVS + 1);
+    {
+        uint32_t rt = 1; while (rt < 2 * D.mcap) rt <<= 1;
+        D.rmask = rt - 1;
+        r |= w->alloc(D.rkey, rt); r |= w->alloc(D.rcnt, rt); r |= w->alloc(D.rbig, rt);
+        r |= w->alloc(D.rslot, D.mcap); r |= w->alloc(D.rflag, D.mcap);
+        r |= w->alloc(D.rext, (size_t)RED_D * (D.mcap / (RED_KMIN + 1) + 1));  // (a reduced pair has more than RED_KMIN)
+        if (!r) SF_HIP(hipMemset(D.rkey, 0xFF, sizeof(unsigned long long) * rt));  // (free slots hold ~0)
+    }
     r |= w->alloc(D.ovcnt, n1);
     r |= w->alloc(D.blcnt, n1); r |= w->alloc(D.bloff, n1); r |= w->alloc(D.blfill, n1); r |= w->alloc(D.blist, 2 * (size_t)D.mcap); r |= w->alloc(D.blinv, 2 * (size_t)D.mcap); r |= w->alloc(D.minv, 2 * (size_t)D.mcap);
     r |= w->alloc(D.parent, nb); r |= w->alloc(D.sst, nb); r |= w->alloc(D.ctr, C_NCTR);
@@ -2482,6 +2857,7 @@ StepK make_step(const float* P, float dt) {
     K.rest_thr = P[SF_P_REST_THRESHOLD];
     K.rest_iters = std::min(std::max((int)P[SF_P_REST_ITERS], 0), 64);
     K.col_support = std::max((int)P[SF_P_COL_SUPPORT], 0);
+    K.red_k = P[SF_P_RED_K] >= 1.0f ? std::max((int)P[SF_P_RED_K], RED_KMIN) : 0;
     K.any_rest = 0;  // (set from the loaded shapes)
     return K;
 }
@@ -2513,16 +2889,22 @@ int step_front(HostWorld* w, const StepK& K) {
     hipLaunchKernelGGL(k_cand_finish, dim3(gs), dim3(NT), 0, s, D);
     hipLaunchKernelGGL(k_cand_finish_big, dim3(1024), dim3(NT), 0, s, D);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[1], s));
-    hipLaunchKernelGGL(k_narrow, dim3(gc), dim3(NT), 0, s, D, K);
+    hipLaunchKernelGGL(k_narrow, dim3(gc), dim3(NT), 0, s, D, K, w->any_hull ? 1 : 0);
+    if (w->any_hull) hipLaunchKernelGGL(k_narrow_hull, dim3(std::min(gc, 1024u)), dim3(NT), 0, s, D, K);
     sf::scan_u32(D.hit, D.hoff, D.ctr + C_NMAN, D.ctr + C_NCAND, 0, D.ccap, w->scan_tmp, s);
     hipLaunchKernelGGL(k_apply_wake, dim3(gb), dim3(NT), 0, s, D);
-    hipLaunchKernelGGL(k_man_hdr, dim3(gc), dim3(NT), 0, s, D);
+    if (K.red_k) {
+        hipLaunchKernelGGL(k_red_count, dim3(gc), dim3(NT), 0, s, D, K);
+        hipLaunchKernelGGL(k_red_score, dim3(gc), dim3(NT), 0, s, D);
+        hipLaunchKernelGGL(k_red_pick, dim3(gc), dim3(NT), 0, s, D);
+    }
+    hipLaunchKernelGGL(k_man_hdr, dim3(gc), dim3(NT), 0, s, D, K);
     if (w->timing) SF_HIP(hipEventRecord(w->ev[2], s));
     // colouring rounds (checked for progress every few rounds and at the end of the support rounds: rounds in which
     // nothing claimed change nothing more in their phase, so a quiet group ends the support phase or the colouring)
     int R = (int)w->prm[SF_P_COL_ROUNDS];
     const int RS = K.col_support;
-    hipLaunchKernelGGL(k_col_init, dim3(gm), dim3(NT), 0, s, D);
+    hipLaunchKernelGGL(k_col_init, dim3(gm), dim3(NT), 0, s, D, K);
     int rounds = 0, flag = 0;
     for (int r = 0; r < R; r++) {
         hipLaunchKernelGGL(k_col_claim, dim3(gm), dim3(NT), 0, s, D, K.K, K.spread, r, RS, flag);
@@ -2545,7 +2927,7 @@ int step_front(HostWorld* w, const StepK& K) {
     }
     w->stats[SF_S_ROUNDS] = rounds;
     hipLaunchKernelGGL(k_col_count, dim3(gm), dim3(NT), 0, s, D, K.K);
-    hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K + OVS), dim3(NT), 0, s, D, K.K);
+    hipLaunchKernelGGL(k_col_chunk_scan, dim3(K.K + OVS + 1), dim3(NT), 0, s, D, K.K);
     sf::scan_u32(D.subcnt, D.suboff, D.colcnt + K.K, nullptr, OVS, OVS, w->scan_tmp, s);
     sf::scan_u32(D.colcnt, D.coloff, nullptr, nullptr, (uint32_t)K.K + 1, 72, w->scan_tmp, s);
     hipLaunchKernelGGL(k_col_fill, dim3(gm), dim3(NT), 0, s, D, K.K);
@@ -2726,6 +3108,7 @@ int finish_stats(HostWorld* w) {
     w->stats[SF_S_ENTRIES] = (int32_t)c[C_NE];
     w->stats[SF_S_CANDIDATES] = (int32_t)c[C_NCAND];
     w->stats[SF_S_DROPPED] = (int32_t)c[C_DROP];
+    w->stats[SF_S_REDUCED] = (int32_t)c[C_NRED];
     return 0;
 }
 
@@ -2923,7 +3306,16 @@ int world_load(HostWorld* w, int nb, const float* bodies, int ns, const int32_t*
     }
     std::vector<float4> slp(ns), slr(3 * (size_t)ns), shh(ns);
     std::vector<float> srest(ns);
+    std::vector<int32_t> shull(ns);
     w->any_rest = false;
+    w->any_hull = false;
+    const int nhull = (int)sf::hull_lib().cells.size();
+    for (int s = 0; s < ns; s++) {
+        const float hf = shapes[(size_t)s * SF_SHAPE_F + 12];
+        if (!(hf >= 0.0f && hf < (float)nhull && hf == (float)(int)hf)) { sf::set_error(SF_E_ARG, "a shape's hull is not in the chunk library"); return SF_E_ARG; }
+        shull[s] = (int32_t)hf;
+        if (shull[s]) w->any_hull = true;
+    }
     w->h_h.assign(3 * (size_t)nb, 0.0f);
     std::vector<int> seen(nb, 0);
     for (int s = 0; s < ns; s++) {
@@ -2956,6 +3348,7 @@ int world_load(HostWorld* w, int nb, const float* bodies, int ns, const int32_t*
     SF_HIP(hipMemcpyAsync(D.slr, slr.data(), sizeof(float4) * 3 * ns, hipMemcpyHostToDevice, s));
     SF_HIP(hipMemcpyAsync(D.shh, shh.data(), sizeof(float4) * ns, hipMemcpyHostToDevice, s));
     SF_HIP(hipMemcpyAsync(D.srest, srest.data(), sizeof(float) * ns, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(D.shull, shull.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
     SF_HIP(hipStreamSynchronize(s));
     w->nold = 0;
     w->h_shb.assign(shb, shb + ns);

@@ -2,6 +2,7 @@
 // builder, and the synthetic workloads (rubble pile, frame/wall buildings, the brick tower).
 #include "../../include/sf.h"
 #include "sf_host.h"
+#include "hulls.h"
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -136,6 +137,7 @@ void build_compound(int nb, const int32_t* xyz, const uint8_t* mat, const double
         s[7] = s[8] = s[9] = (float)h;
         s[10] = FRICTION[mat[k]];
         s[11] = RESTITUTION[mat[k]];
+        s[12] = 0.0f;  // (a whole box)
     }
 }
 
@@ -272,6 +274,128 @@ int sf_scene_stack(int n, float* bodies, int32_t* shape_body, float* shapes, int
         shape_body[k] = k;
     }
     return n;
+}
+
+// A body from shapes given in the world (gpu/gpu_world.js buildBodies :2212-2249), FP64: each shape's mass is the
+// density times its box volume times its hull's share of the box (none on a fixed body); the centre of mass is taken
+// over the shapes' centroids (a chunk's centroid, not its box centre); each shape's inertia about its centroid comes
+// from its covariance (a box: h^2 / 3 per axis) stretched by h, turned into the world and moved to the centre of mass;
+// the principal frame by Jacobi rotations; the shapes at their offsets in that frame, rotations relative to it.
+// in: n records of SF_SHAPE_IN floats (p[3] q[4] h[3] material 1..5, hull id); out: body[SF_BODY_F], n shapes.
+int sf_build_body(int n, const float* in, int fixed, const float* v, const float* w, float* body, float* shapes) {
+    if (n <= 0 || !in || !body || !shapes) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    const sf::HullLib& HL = sf::hull_lib();
+    for (int k = 0; k < n; k++) {
+        const float* f = in + (size_t)k * SF_SHAPE_IN;
+        const int mat = (int)f[10], hull = (int)f[11];
+        if (mat < 1 || mat > 5 || hull < 0 || hull >= (int)HL.cells.size()) { sf::set_error(SF_E_ARG, "material 1..5 and a hull of the library"); return SF_E_ARG; }
+        for (int q = 0; q < SF_SHAPE_IN; q++) if (!std::isfinite(f[q])) { sf::set_error(SF_E_ARG, "NaN or inf in the input"); return SF_E_ARG; }
+    }
+    auto q2m = [](const float* q, double R[3][3]) {
+        const double x = q[0], y = q[1], z = q[2], ww = q[3];
+        R[0][0] = 1 - 2 * (y * y + z * z); R[0][1] = 2 * (x * y - z * ww); R[0][2] = 2 * (x * z + y * ww);
+        R[1][0] = 2 * (x * y + z * ww); R[1][1] = 1 - 2 * (x * x + z * z); R[1][2] = 2 * (y * z - x * ww);
+        R[2][0] = 2 * (x * z - y * ww); R[2][1] = 2 * (y * z + x * ww); R[2][2] = 1 - 2 * (x * x + y * y);
+    };
+    std::vector<double> ms(n);
+    std::vector<std::array<double, 3>> mc(n);
+    double M = 0, com[3] = {0, 0, 0};
+    for (int k = 0; k < n; k++) {
+        const float* f = in + (size_t)k * SF_SHAPE_IN;
+        const sf::HullCell& C = HL.cells[(int)f[11]];
+        ms[k] = (fixed ? 0.0 : DENSITY[(int)f[10]]) * 8.0 * (double)f[7] * (double)f[8] * (double)f[9] * C.box_vol;
+        double R[3][3];
+        q2m(f + 3, R);
+        const double o[3] = {C.cen[0] * f[7], C.cen[1] * f[8], C.cen[2] * f[9]};
+        for (int i = 0; i < 3; i++) mc[k][i] = (double)f[i] + R[i][0] * o[0] + R[i][1] * o[1] + R[i][2] * o[2];
+        M += ms[k];
+        for (int i = 0; i < 3; i++) com[i] += ms[k] * mc[k][i];
+    }
+    if (M > 0) { for (int i = 0; i < 3; i++) com[i] /= M; }
+    else {
+        for (int i = 0; i < 3; i++) com[i] = 0.0;
+        for (int k = 0; k < n; k++) for (int i = 0; i < 3; i++) com[i] += (double)in[(size_t)k * SF_SHAPE_IN + i] / n;
+    }
+    double qb[4] = {0, 0, 0, 1}, il[3] = {0, 0, 0};
+    if (M > 0) {
+        double I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int k = 0; k < n; k++) {
+            const float* f = in + (size_t)k * SF_SHAPE_IN;
+            const sf::HullCell& C = HL.cells[(int)f[11]];
+            const double h[3] = {f[7], f[8], f[9]};
+            double Cm[3][3], Il[3][3], R[3][3];
+            for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) Cm[r][c] = ms[k] * C.cov[r][c] * h[r] * h[c];
+            const double tr = Cm[0][0] + Cm[1][1] + Cm[2][2];
+            for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) Il[r][c] = (r == c ? tr : 0.0) - Cm[r][c];
+            q2m(f + 3, R);
+            const double d[3] = {mc[k][0] - com[0], mc[k][1] - com[1], mc[k][2] - com[2]};
+            const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    double t = 0.0;
+                    for (int a2 = 0; a2 < 3; a2++) for (int b2 = 0; b2 < 3; b2++) t += R[r][a2] * Il[a2][b2] * R[c][b2];
+                    I[r][c] += t + ms[k] * ((r == c ? dd : 0.0) - d[r] * d[c]);
+                }
+        }
+        double vals[3], V[3][3];
+        eig_sym(I, vals, V);
+        m2q(V, qb);
+        for (int i = 0; i < 3; i++) il[i] = vals[i] > 1e-12 ? 1.0 / vals[i] : 0.0;
+    }
+    float qf[4] = {(float)qb[0], (float)qb[1], (float)qb[2], (float)qb[3]};
+    double Rb[3][3];
+    q2m(qf, Rb);
+    std::memset(body, 0, sizeof(float) * SF_BODY_F);
+    for (int i = 0; i < 3; i++) body[i] = (float)com[i];
+    for (int i = 0; i < 4; i++) body[3 + i] = qf[i];
+    if (v) for (int i = 0; i < 3; i++) body[7 + i] = v[i];
+    if (w) for (int i = 0; i < 3; i++) body[10 + i] = w[i];
+    body[13] = M > 0 ? (float)(1.0 / M) : 0.0f;
+    for (int i = 0; i < 3; i++) body[14 + i] = (float)il[i];
+    body[17] = M > 0 ? 1.0f : 0.0f;
+    // shapes: local position Rb^T (p - com), local rotation conj(qb) * q
+    for (int k = 0; k < n; k++) {
+        const float* f = in + (size_t)k * SF_SHAPE_IN;
+        float* o = shapes + (size_t)k * SF_SHAPE_F;
+        std::memset(o, 0, sizeof(float) * SF_SHAPE_F);
+        const double d[3] = {f[0] - com[0], f[1] - com[1], f[2] - com[2]};
+        for (int a2 = 0; a2 < 3; a2++) o[a2] = (float)(Rb[0][a2] * d[0] + Rb[1][a2] * d[1] + Rb[2][a2] * d[2]);
+        const double ax = -qb[0], ay = -qb[1], az = -qb[2], aw = qb[3], bx = f[3], by = f[4], bz = f[5], bw = f[6];
+        o[3] = (float)(aw * bx + ax * bw + ay * bz - az * by);
+        o[4] = (float)(aw * by - ax * bz + ay * bw + az * bx);
+        o[5] = (float)(aw * bz + ax * by - ay * bx + az * bw);
+        o[6] = (float)(aw * bw - ax * bx - ay * by - az * bz);
+        o[7] = f[7]; o[8] = f[8]; o[9] = f[9];
+        o[10] = FRICTION[(int)f[10]];
+        o[11] = RESTITUTION[(int)f[10]];
+        o[12] = f[11];
+    }
+    return 0;
+}
+
+// The pieces of a brick (centre p, rotation q, half size h) broken by a pattern of the hull library, one shape record
+// (SF_SHAPE_IN) per cell, tiling it (gpu/gpu_world.js GpuWorld.hullShapes :2257-2263): each cell's box centre turned
+// with the brick, its half size the cell box's times the brick's. Returns the piece count.
+int sf_hull_shapes(const float* p, const float* q, const float* h, int pattern, int mat, float* out) {
+    const sf::HullLib& HL = sf::hull_lib();
+    if (!p || !q || !h || pattern < 0) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    const std::vector<int>& P = HL.patterns[pattern % (int)HL.patterns.size()];
+    if (!out) return (int)P.size();
+    const double x = q[0], y = q[1], z = q[2], ww = q[3];
+    const double R[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y - z * ww), 2 * (x * z + y * ww)},
+                            {2 * (x * y + z * ww), 1 - 2 * (x * x + z * z), 2 * (y * z - x * ww)},
+                            {2 * (x * z - y * ww), 2 * (y * z + x * ww), 1 - 2 * (x * x + y * y)}};
+    for (size_t k = 0; k < P.size(); k++) {
+        const sf::HullCell& C = HL.cells[P[k]];
+        const double o[3] = {C.bc[0] * h[0], C.bc[1] * h[1], C.bc[2] * h[2]};
+        float* r = out + k * SF_SHAPE_IN;
+        for (int i = 0; i < 3; i++) r[i] = (float)(p[i] + R[i][0] * o[0] + R[i][1] * o[1] + R[i][2] * o[2]);
+        for (int i = 0; i < 4; i++) r[3 + i] = q[i];
+        for (int i = 0; i < 3; i++) r[7 + i] = (float)(C.bh[i] * h[i]);
+        r[10] = (float)mat;
+        r[11] = (float)P[k];
+    }
+    return (int)P.size();
 }
 
 int sf_build_compound(int nb, const int32_t* xyz, const uint8_t* mat, float* body, float* shapes) {

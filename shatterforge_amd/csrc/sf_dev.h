@@ -77,4 +77,5 @@ SF_HD M3 m3mul(const M3& a, const M3& b) { M3 o; for (int j = 0; j < 3; j++) o.c
 SF_HD uint32_t mix32(uint32_t h) { h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16; return h; }
 // hash of a manifold's key (shape a, shape b or -1 for the ground): colour choices and priorities go by it
 SF_HD uint32_t key_hash(int32_t a, int32_t b) { return mix32((uint32_t)a * 0x9E3779B9u + mix32((uint32_t)b ^ 0x5BD1E995u)); }
-SF_HD uint64_t man_key(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint64_t)((uint32_t)b + 1u); }
+// a manifold's sort key: shape a, then the ground (b = -1), a chunk's second ground patch (b = -2), partners by index
+SF_HD uint64_t man_key(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint64_t)(b >= 0 ? (uint32_t)b + 3u : (uint32_t)(-b)); }

@@ -61,7 +61,7 @@ pub mod ffi {
     }
 
     pub const SF_BODY_F: usize = 20;
-    pub const SF_SHAPE_F: usize = 12;
+    pub const SF_SHAPE_F: usize = 13;
     pub const SF_LAT_BODY_F: usize = 22;
     pub const SF_LAT_BRICK_F: usize = 13;
 

@@ -1,7 +1,8 @@
 """Seeded scenes as numpy arrays (thin wrappers over the library's host-side generators).
 
 A box scene is ``(bodies, shape_body, shapes)``: ``bodies`` float32 [nb, 20] (p, q, v, w, inv_m, inv_i, awake,
-sleep_t, reserved), ``shape_body`` int32 [ns], ``shapes`` float32 [ns, 12] (local p, local q, half extents, mu, restitution).
+sleep_t, reserved), ``shape_body`` int32 [ns], ``shapes`` float32 [ns, 13] (local p, local q, half extents, mu, restitution,
+hull: 0 the whole box, else a convex cell of the chunk library).
 A structure is ``(mat, xyz, infill)`` on the 0.5 m lattice.
 """
 import numpy as np
@@ -9,7 +10,7 @@ import numpy as np
 from ._native import lib, check, ptr
 
 BODY_F = 20
-SHAPE_F = 12
+SHAPE_F = 13
 
 
 def _box_scene(fn, *args):
@@ -48,6 +49,40 @@ def compound(xyz, mat):
     shapes = np.zeros((len(mat), SHAPE_F), np.float32)
     check(lib().sf_build_compound(len(mat), ptr(xyz), ptr(mat), ptr(body), ptr(shapes)))
     return body, shapes
+
+
+SHAPE_IN = 12  # a shape in the world for the body builder: p[3] q[4] h[3] material (1..5) hull
+
+
+def shape(p, h, q=(0.0, 0.0, 0.0, 1.0), mat=1, hull=0):
+    """A shape given in the world: centre, half size, rotation, material code (1 concrete .. 5 timber), hull id."""
+    r = np.zeros(SHAPE_IN, np.float32)
+    r[0:3] = p
+    r[3:7] = q
+    r[7:10] = h
+    r[10] = mat
+    r[11] = hull
+    return r
+
+
+def body(shapes, fixed=False, v=(0, 0, 0), w=(0, 0, 0)):
+    """One body from shapes given in the world, as gpu/gpu_world.js buildBodies (:2212-2249) builds it: returns
+    (body[20], shapes[n, 13])."""
+    shp = np.ascontiguousarray(np.stack([np.asarray(x, np.float32) for x in shapes]), np.float32)
+    b = np.zeros(BODY_F, np.float32)
+    out = np.zeros((len(shp), SHAPE_F), np.float32)
+    vv = np.asarray(v, np.float32)
+    ww = np.asarray(w, np.float32)
+    check(lib().sf_build_body(len(shp), ptr(shp), 1 if fixed else 0, ptr(vv), ptr(ww), ptr(b), ptr(out)))
+    return b, out
+
+
+def bodies(items):
+    """A scene from (body, shapes[n, 13]) pairs (body() results): (bodies, shape_body, shapes)."""
+    bs = np.stack([b for b, _ in items]).astype(np.float32)
+    sb = np.concatenate([np.full(len(s), k, np.int32) for k, (_, s) in enumerate(items)])
+    ss = np.concatenate([s for _, s in items]).astype(np.float32)
+    return bs, sb, ss
 
 
 def box(p, h, q=(0.0, 0.0, 0.0, 1.0), density=1900.0, mu=0.6, v=(0, 0, 0), w=(0, 0, 0), e=0.0):

@@ -15,15 +15,15 @@ from ._native import lib, check, ptr
 P = dict(GX=0, GY=1, GZ=2, SUBSTEPS=3, HERTZ=4, DAMPING=5, MAXPUSH=6, FRICTION=7, SPEC=8, SLEEP_LIN=9, SLEEP_ANG=10,
          SLEEP_TIME=11, SLEEP_DIST=12, SLEEP_ROT=13, ALLOW_SLEEP=14, CELL=15, LEAK=16, ANCHOR=17, ANCHOR_PUSH=18,
          ANCHOR_SLOP=19, COL_LIMIT=20, COL_ROUNDS=21, COL_SPREAD=22, TAIL_MIN=23, SOLVE_NT=24, RATIO=25, SHOCK=26,
-         REST_THRESHOLD=27, REST_ITERS=28, COL_SUPPORT=29, PSUB=30, PCELL=31)
-NPARAM = 32
-# the GPU world's solver extensions (gpu/gpu_world.js:1152-1203, :954-985): mass-ratio limit, shock-propagation level
-# passes, supports-first colouring rounds; params(**GPU_WORLD) turns them on
-GPU_WORLD = dict(ratio=20.0, shock=6, col_support=2)
+         REST_THRESHOLD=27, REST_ITERS=28, COL_SUPPORT=29, PSUB=30, PCELL=31, RED_K=32)
+NPARAM = 33
+# the GPU world's solver extensions (gpu/gpu_world.js:1152-1203, :954-985, :834-920): mass-ratio limit,
+# shock-propagation level passes, supports-first colouring rounds, contact reduction; params(**GPU_WORLD) turns them on
+GPU_WORLD = dict(ratio=20.0, shock=6, col_support=2, red_k=32)
 # statistics (SF_S_*)
 S = dict(AWAKE=0, PAIRS=1, MANIFOLDS=2, POINTS=3, COLOURS=4, OVERFLOW=5, GROUND=6, WAKES=7, ENTRIES=8, CANDIDATES=9,
-         DROPPED=10, ROUNDS=11)
-NSTAT = 12
+         DROPPED=10, ROUNDS=11, REDUCED=12)
+NSTAT = 13
 MAN_F = 40
 
 
