@@ -426,6 +426,16 @@ int sf_bricks_build_joints(sf_bricks* b);
 int sf_bricks_joints(const sf_bricks* b, int32_t* out5, int cap);
 /* a joint's damage byte (bit 0 cracked, bits 1..7 wear), as the game layer sets it */
 int sf_bricks_set_joint_damage(sf_bricks* b, int j, int dmg);
+/* Bricks::save / Bricks::load (brick.rs:116-145): the store as bytes, each array as its u32 length and elements padded
+ * to 8 bytes, in the reference's order and types (the spatial index's keys ascending); save returns the byte count
+ * (out may be NULL; SF_E_CAP when cap is smaller), load returns a new store or NULL for malformed bytes */
+int64_t sf_bricks_save(const sf_bricks* b, uint8_t* out, int64_t cap);
+sf_bricks* sf_bricks_load(const uint8_t* src, int64_t n);
+/* the per-brick fields the store carries for its format: owning body (-1 none), damage byte, convex cell */
+int sf_bricks_set_cell(sf_bricks* b, int i, int body, int damage, int hull);
+int sf_bricks_cell(const sf_bricks* b, int i, int32_t* out3);
+/* a brick's box: min corner[3] and size[3] in ticks */
+int sf_bricks_box(const sf_bricks* b, int i, int32_t* out6);
 /* structure.rs Warm (:178-189): the last solution of a load case, per cell; a case given one starts from it and keeps
  * its own solution in it */
 sf_warm* sf_warm_create(void);

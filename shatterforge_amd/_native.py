@@ -159,6 +159,11 @@ _PROTOS = {
     "sf_bricks_build_joints": (c_int, [c_void_p]),
     "sf_bricks_joints": (c_int, [c_void_p, c_void_p, c_int]),
     "sf_bricks_set_joint_damage": (c_int, [c_void_p, c_int, c_int]),
+    "sf_bricks_save": (c_int64, [c_void_p, c_void_p, c_int64]),
+    "sf_bricks_load": (c_void_p, [c_void_p, c_int64]),
+    "sf_bricks_set_cell": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "sf_bricks_cell": (c_int, [c_void_p, c_int, c_void_p]),
+    "sf_bricks_box": (c_int, [c_void_p, c_int, c_void_p]),
     "sf_warm_create": (c_void_p, []),
     "sf_warm_destroy": (None, [c_void_p]),
     "sf_warm_size": (c_int, [c_void_p]),

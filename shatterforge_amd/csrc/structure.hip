@@ -26,6 +26,7 @@
 #include <thread>
 #include <unordered_map>
 #include <vector>
+#include <map>
 
 #include "../../include/sf.h"
 #include "sf_host.h"
@@ -75,6 +76,10 @@ struct Store {
     std::vector<uint8_t> jaxis, jdmg;
     std::vector<std::vector<uint32_t>> inc;  // per brick: its live joints, the newest first
     int live = 0;
+    // carried for the store's save format (brick.rs:116-145); the solve does not read them
+    std::vector<uint32_t> body;              // owning body (NONE: none)
+    std::vector<uint8_t> damage, hull;       // damage byte, convex cell (0: the whole box)
+    std::vector<uint32_t> jfree;             // removed joint slots, in the order they were freed
     size_t size() const { return mat.size(); }
     int32_t mn(size_t i, int k) const { return lo[3 * i + k]; }
     int32_t mx(size_t i, int k) const { return lo[3 * i + k] + ext[3 * i + k]; }
@@ -143,7 +148,7 @@ bool shared_face(const Store& S, uint32_t p, uint32_t q, int* ax, uint32_t* area
 // brick's rank in the neighbour query.
 int build_joints(Store& S) {
     size_t n = S.size();
-    S.ja.clear(); S.jb.clear(); S.jarea.clear(); S.jaxis.clear(); S.jdmg.clear();
+    S.ja.clear(); S.jb.clear(); S.jarea.clear(); S.jaxis.clear(); S.jdmg.clear(); S.jfree.clear();
     S.inc.assign(n, {});
     S.live = 0;
     std::vector<uint32_t> ord;
@@ -1404,9 +1409,192 @@ int mk_solve(sf_net* N, const DCase& h, const DCase* C, NetState* S) {
 }
 }  // namespace
 
+
+// ---------------------------------------------------------------- the store's byte format (brick.rs:100-145)
+// Each array as its length (u32, little endian) and its elements, padded with zeros to 8 bytes: x y z (i16), sx sy sz
+// mat paint flags (u8), structure (u16), body jhead (u32), damage hull (u8), ja jb jna jnb (u32), jarea (u16), jaxis
+// jdmg (u8), the free joint slots (u32), the live joint count (one u32), the spatial index's cell keys (u64) and first
+// bricks (u32), and each brick's next in its cell (u32). The joint lists (jhead, jna, jnb) are each brick's joints
+// newest first; the index puts each brick in the cell of its min corner at the finest of four levels (1, 4, 16, 64 m)
+// as large as the brick, later bricks first in a cell. A map's entry order carries no meaning: keys are written in
+// ascending order, and any order is read.
+namespace {
+struct Out {
+    std::vector<uint8_t> b;
+    template <class T> void put(const T* v, size_t n) {
+        const uint32_t len = (uint32_t)n;
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(&len);
+        b.insert(b.end(), p, p + 4);
+        const uint8_t* q = reinterpret_cast<const uint8_t*>(v);
+        b.insert(b.end(), q, q + n * sizeof(T));
+        while (b.size() % 8) b.push_back(0);
+    }
+    template <class T> void put(const std::vector<T>& v) { put(v.data(), v.size()); }
+};
+struct In {
+    const uint8_t* src;
+    size_t n, at = 0;
+    bool ok = true;
+    template <class T> std::vector<T> get() {
+        std::vector<T> v;
+        if (!ok || at + 4 > n) { ok = false; return v; }
+        uint32_t len;
+        std::memcpy(&len, src + at, 4);
+        at += 4;
+        if ((uint64_t)len * sizeof(T) > n - at) { ok = false; return v; }
+        v.resize(len);
+        if (len) std::memcpy(v.data(), src + at, (size_t)len * sizeof(T));
+        at += (size_t)len * sizeof(T);
+        at = (at + 7) & ~(size_t)7;
+        return v;
+    }
+};
+constexpr int IDX_LEVELS = 4;
+inline int32_t lcell(int l) { return 8 << (2 * l); }
+inline int32_t fdiv(int32_t a, int32_t c) { int32_t q = a / c; return (a % c != 0 && ((a < 0) != (c < 0))) ? q - 1 : q; }
+inline uint64_t ckey(int l, int32_t x, int32_t y, int32_t z) {
+    return ((uint64_t)l << 60) | (((uint64_t)(x + (1 << 19)) & 0xfffffull) << 40) | (((uint64_t)(y + (1 << 19)) & 0xfffffull) << 20) |
+           ((uint64_t)(z + (1 << 19)) & 0xfffffull);
+}
+std::vector<uint8_t> save_store(const Store& S) {
+    const size_t n = S.size(), nj = S.ja.size();
+    std::vector<int16_t> x(n), y(n), z(n);
+    std::vector<uint8_t> sx(n), sy(n), sz(n);
+    std::vector<uint32_t> jhead(n, NONE), jna(nj, NONE), jnb(nj, NONE);
+    for (size_t i = 0; i < n; i++) {
+        x[i] = (int16_t)S.lo[3 * i]; y[i] = (int16_t)S.lo[3 * i + 1]; z[i] = (int16_t)S.lo[3 * i + 2];
+        sx[i] = S.ext[3 * i]; sy[i] = S.ext[3 * i + 1]; sz[i] = S.ext[3 * i + 2];
+        const std::vector<uint32_t>& L = S.inc[i];
+        if (!L.empty()) jhead[i] = L[0];
+        for (size_t k = 0; k < L.size(); k++) {
+            const uint32_t j = L[k], nx = k + 1 < L.size() ? L[k + 1] : NONE;
+            if (S.ja[j] == (uint32_t)i) jna[j] = nx; else jnb[j] = nx;
+        }
+    }
+    // (free slots keep the values they had: ends NONE; their next pointers are never read)
+    std::map<uint64_t, uint32_t> cells;
+    std::vector<uint32_t> cnext(n, NONE);
+    for (size_t i = 0; i < n; i++) {
+        int32_t m = std::max<int32_t>(std::max<int32_t>(S.ext[3 * i], S.ext[3 * i + 1]), S.ext[3 * i + 2]);
+        int l = 0;
+        while (l < IDX_LEVELS - 1 && lcell(l) < m) l++;
+        const int32_t c = lcell(l);
+        const uint64_t key = ckey(l, fdiv(S.lo[3 * i], c), fdiv(S.lo[3 * i + 1], c), fdiv(S.lo[3 * i + 2], c));
+        auto it = cells.find(key);
+        cnext[i] = it == cells.end() ? NONE : it->second;
+        cells[key] = (uint32_t)i;
+    }
+    std::vector<uint64_t> keys;
+    std::vector<uint32_t> vals;
+    for (const auto& kv : cells) { keys.push_back(kv.first); vals.push_back(kv.second); }
+    const uint32_t live = (uint32_t)S.live;
+    Out o;
+    o.put(x); o.put(y); o.put(z); o.put(sx); o.put(sy); o.put(sz);
+    o.put(S.mat); o.put(S.paint); o.put(S.flags);
+    o.put(S.sid); o.put(S.body); o.put(jhead); o.put(S.damage); o.put(S.hull);
+    o.put(S.ja); o.put(S.jb); o.put(jna); o.put(jnb);
+    o.put(S.jarea); o.put(S.jaxis); o.put(S.jdmg);
+    o.put(S.jfree); o.put(&live, 1);
+    o.put(keys); o.put(vals); o.put(cnext);
+    return o.b;
+}
+// the store from its bytes; false (with the reason) on a malformed or inconsistent input
+bool load_store(const uint8_t* src, size_t nbytes, Store& S, const char** why) {
+    In r{src, nbytes};
+    auto x = r.get<int16_t>(), y = r.get<int16_t>(), z = r.get<int16_t>();
+    auto sx = r.get<uint8_t>(), sy = r.get<uint8_t>(), sz = r.get<uint8_t>();
+    S.mat = r.get<uint8_t>(); S.paint = r.get<uint8_t>(); S.flags = r.get<uint8_t>();
+    S.sid = r.get<uint16_t>(); S.body = r.get<uint32_t>();
+    auto jhead = r.get<uint32_t>();
+    S.damage = r.get<uint8_t>(); S.hull = r.get<uint8_t>();
+    S.ja = r.get<uint32_t>(); S.jb = r.get<uint32_t>();
+    auto jna = r.get<uint32_t>(), jnb = r.get<uint32_t>();
+    S.jarea = r.get<uint16_t>(); S.jaxis = r.get<uint8_t>(); S.jdmg = r.get<uint8_t>();
+    S.jfree = r.get<uint32_t>();
+    auto live = r.get<uint32_t>();
+    auto keys = r.get<uint64_t>();
+    auto vals = r.get<uint32_t>();
+    auto cnext = r.get<uint32_t>();
+    if (!r.ok) { *why = "truncated"; return false; }
+    const size_t n = x.size(), nj = S.ja.size();
+    for (size_t m : {y.size(), z.size(), sx.size(), sy.size(), sz.size(), S.mat.size(), S.paint.size(), S.flags.size(),
+                     S.sid.size(), S.body.size(), jhead.size(), S.damage.size(), S.hull.size(), cnext.size()})
+        if (m != n) { *why = "brick arrays of unequal lengths"; return false; }
+    for (size_t m : {S.jb.size(), jna.size(), jnb.size(), S.jarea.size(), S.jaxis.size(), S.jdmg.size()})
+        if (m != nj) { *why = "joint arrays of unequal lengths"; return false; }
+    if (live.size() != 1 || keys.size() != vals.size()) { *why = "malformed counts"; return false; }
+    S.lo.resize(3 * n); S.ext.resize(3 * n);
+    for (size_t i = 0; i < n; i++) {
+        S.lo[3 * i] = x[i]; S.lo[3 * i + 1] = y[i]; S.lo[3 * i + 2] = z[i];
+        S.ext[3 * i] = sx[i]; S.ext[3 * i + 1] = sy[i]; S.ext[3 * i + 2] = sz[i];
+        if (!sx[i] || !sy[i] || !sz[i] || S.mat[i] > 4) { *why = "a brick's size or material out of range"; return false; }
+    }
+    // each brick's joint list from its linked list (bounded by the joint count: a cycle is malformed)
+    S.inc.assign(n, {});
+    int nlive = 0;
+    for (size_t j = 0; j < nj; j++) {
+        if (S.ja[j] == NONE) continue;
+        if (S.ja[j] >= n || S.jb[j] >= n) { *why = "a joint's end out of range"; return false; }
+        nlive++;
+    }
+    for (size_t i = 0; i < n; i++) {
+        uint32_t j = jhead[i];
+        size_t steps = 0;
+        while (j != NONE) {
+            if (j >= nj || steps++ > nj || (S.ja[j] != (uint32_t)i && S.jb[j] != (uint32_t)i)) { *why = "a joint list that does not close"; return false; }
+            S.inc[i].push_back(j);
+            j = S.ja[j] == (uint32_t)i ? jna[j] : jnb[j];
+        }
+    }
+    for (uint32_t j : S.jfree) if (j >= nj || S.ja[j] != NONE) { *why = "a free joint slot in use"; return false; }
+    if (live[0] != (uint32_t)nlive) { *why = "the live joint count disagrees"; return false; }
+    S.live = nlive;
+    return true;
+}
+}  // namespace
+
 extern "C" {
 
 sf_bricks* sf_bricks_create(void) { return new sf_bricks(); }
+int64_t sf_bricks_save(const sf_bricks* b, uint8_t* out, int64_t cap) {
+    if (!b) { sf::set_error(SF_E_ARG, "sf_bricks_save: null store"); return SF_E_ARG; }
+    const std::vector<uint8_t> v = save_store(b->s);
+    if (out) {
+        if (cap < (int64_t)v.size()) { sf::set_error(SF_E_CAP, "sf_bricks_save: buffer too small"); return SF_E_CAP; }
+        std::memcpy(out, v.data(), v.size());
+    }
+    return (int64_t)v.size();
+}
+sf_bricks* sf_bricks_load(const uint8_t* src, int64_t n) {
+    if (!src || n < 0) { sf::set_error(SF_E_ARG, "sf_bricks_load: null input"); return nullptr; }
+    sf_bricks* b = new sf_bricks();
+    const char* why = "";
+    if (!load_store(src, (size_t)n, b->s, &why)) {
+        delete b;
+        sf::set_error(SF_E_ARG, std::string("sf_bricks_load: ") + why);
+        return nullptr;
+    }
+    return b;
+}
+int sf_bricks_set_cell(sf_bricks* b, int i, int body, int damage, int hull) {
+    if (!b || i < 0 || (size_t)i >= b->s.size() || damage < 0 || damage > 255 || hull < 0 || hull > 255) return SF_E_ARG;
+    b->s.body[i] = body < 0 ? NONE : (uint32_t)body;
+    b->s.damage[i] = (uint8_t)damage;
+    b->s.hull[i] = (uint8_t)hull;
+    return 0;
+}
+int sf_bricks_box(const sf_bricks* b, int i, int32_t* out6) {
+    if (!b || !out6 || i < 0 || (size_t)i >= b->s.size()) return SF_E_ARG;
+    for (int k = 0; k < 3; k++) { out6[k] = b->s.lo[3 * (size_t)i + k]; out6[3 + k] = b->s.ext[3 * (size_t)i + k]; }
+    return 0;
+}
+int sf_bricks_cell(const sf_bricks* b, int i, int32_t* out3) {
+    if (!b || !out3 || i < 0 || (size_t)i >= b->s.size()) return SF_E_ARG;
+    out3[0] = b->s.body[i] == NONE ? -1 : (int32_t)b->s.body[i];
+    out3[1] = b->s.damage[i];
+    out3[2] = b->s.hull[i];
+    return 0;
+}
 void sf_bricks_destroy(sf_bricks* b) { delete b; }
 int sf_bricks_len(const sf_bricks* b) { return b ? (int)b->s.size() : SF_E_ARG; }
 int sf_bricks_add(sf_bricks* b, const int32_t* min3, const int32_t* size3, int mat, int paint, int structure) {
@@ -1421,6 +1609,7 @@ int sf_bricks_add(sf_bricks* b, const int32_t* min3, const int32_t* size3, int m
     for (int k = 0; k < 3; k++) { S.lo.push_back(min3[k]); S.ext.push_back((uint8_t)size3[k]); }
     S.mat.push_back((uint8_t)mat); S.paint.push_back((uint8_t)paint); S.flags.push_back(F_ALIVE); S.sid.push_back((uint16_t)structure);
     S.inc.emplace_back();
+    S.body.push_back(NONE); S.damage.push_back(0); S.hull.push_back(0);
     return (int)S.size() - 1;
 }
 int sf_bricks_set_flags(sf_bricks* b, int i, int flags) {
@@ -1444,6 +1633,7 @@ int sf_bricks_remove(sf_bricks* b, int i) {
         auto& v = S.inc[o];
         v.erase(std::find(v.begin(), v.end(), j));
         S.ja[j] = NONE; S.jb[j] = NONE;
+        S.jfree.push_back(j);
         S.live--;
     }
     S.inc[i].clear();

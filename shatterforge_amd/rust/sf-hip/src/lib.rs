@@ -179,6 +179,22 @@ pub mod ffi {
         pub fn sf_bricks_build_joints(b: *mut SfBricks) -> c_int;
         pub fn sf_bricks_joints(b: *const SfBricks, out5: *mut i32, cap: c_int) -> c_int;
         pub fn sf_bricks_set_joint_damage(b: *mut SfBricks, j: c_int, dmg: c_int) -> c_int;
+        // brick.rs Bricks::save / Bricks::load (:116-145)
+        pub fn sf_bricks_save(b: *const SfBricks, out: *mut u8, cap: i64) -> i64;
+        pub fn sf_bricks_load(src: *const u8, n: i64) -> *mut SfBricks;
+        pub fn sf_bricks_set_cell(b: *mut SfBricks, i: c_int, body: c_int, damage: c_int, hull: c_int) -> c_int;
+        pub fn sf_bricks_cell(b: *const SfBricks, i: c_int, out3: *mut i32) -> c_int;
+        pub fn sf_bricks_box(b: *const SfBricks, i: c_int, out6: *mut i32) -> c_int;
+        // convex chunks (engine/src/hull.rs; gpu/gpu_world.js hullLibrary, buildBodies, GpuWorld.hullShapes)
+        pub fn sf_build_body(n: c_int, shapes_in: *const f32, fixed: c_int, v: *const f32, w: *const f32, body: *mut f32, shapes: *mut f32) -> c_int;
+        pub fn sf_hull_count() -> c_int;
+        pub fn sf_hull_pattern_count() -> c_int;
+        pub fn sf_hull_pattern(p: c_int, ids: *mut i32) -> c_int;
+        pub fn sf_hull_cell(h: c_int, out26: *mut f64) -> c_int;
+        pub fn sf_hull_vertices(h: c_int, cube: c_int, out: *mut f64) -> c_int;
+        pub fn sf_hull_face(h: c_int, f: c_int, cube: c_int, xyz: *mut f64, plane4: *mut f64, other: *mut i32) -> c_int;
+        pub fn sf_hull_edges(h: c_int, pairs: *mut i32, dirs: *mut f32) -> c_int;
+        pub fn sf_hull_shapes(p: *const f32, q: *const f32, h: *const f32, pattern: c_int, mat: c_int, out: *mut f32) -> c_int;
         pub fn sf_warm_create() -> *mut SfWarm;
         pub fn sf_warm_destroy(w: *mut SfWarm);
         pub fn sf_warm_size(w: *const SfWarm) -> c_int;

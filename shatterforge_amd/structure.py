@@ -107,6 +107,39 @@ class Bricks:
         """structure.rs mass (:66-71), kg"""
         return np.float32(lib().sf_bricks_mass(self._h, int(i)))
 
+    def set_cell(self, i, body=-1, damage=0, hull=0):
+        """the per-brick fields the store carries for its format: owning body (-1 none), damage byte, convex cell"""
+        check(lib().sf_bricks_set_cell(self._h, int(i), int(body), int(damage), int(hull)))
+
+    def cell(self, i):
+        out = np.zeros(3, np.int32)
+        check(lib().sf_bricks_cell(self._h, int(i), ptr(out)))
+        return tuple(int(v) for v in out)
+
+    def save(self):
+        """Bricks::save (brick.rs:116-127): the store as bytes"""
+        n = check(lib().sf_bricks_save(self._h, None, 0))
+        out = np.zeros(n, np.uint8)
+        check(lib().sf_bricks_save(self._h, ptr(out), n))
+        return out.tobytes()
+
+    @classmethod
+    def load(cls, data):
+        """Bricks::load (brick.rs:128-145): a store from bytes (SfError for malformed bytes)"""
+        buf = np.frombuffer(bytes(data), np.uint8).copy()
+        h = lib().sf_bricks_load(ptr(buf), len(buf))
+        if not h:
+            check(-1)
+        b = cls.__new__(cls)
+        b._h = h
+        b.min, b.size = [], []
+        box = np.zeros(6, np.int32)
+        for i in range(check(lib().sf_bricks_len(h))):
+            check(lib().sf_bricks_box(h, i, ptr(box)))
+            b.min.append(box[0:3].tolist())
+            b.size.append(box[3:6].tolist())
+        return b
+
 
 class Warm:
     """The last solution of a load case, per cell (structure.rs Warm, :178-189)."""
