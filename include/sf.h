@@ -317,7 +317,8 @@ int sf_lat_timing(sf_lat* L, double* ms_frame, int32_t* syncs);
  * connectivity), structural, warm-up, statistics, all */
 int sf_lat_phase_ms(sf_lat* L, double* out8);
 /* finer split of the last frame's host work, ms: topology (body pull, rebuilds, connectivity, delta flush, slot
- * lists, scene upload, scenes visited), structural (upload, launches, read-back wait, event application, flush) */
+ * lists, scene upload, scenes visited), structural (upload, launches, read-back wait, event application, flush),
+ * then connectivity's device search (launch and wait) */
 int sf_lat_subphase_ms(sf_lat* L, double* out16);
 /* the contact solve of a substep runs as one workgroup per scene when no scene has more than max_contacts contacts
  * (default 32768), else as colour-by-colour launches over all scenes; 0: always the latter. Both give the same
@@ -326,6 +327,11 @@ int sf_lat_set_fused(sf_lat* L, int max_contacts);
 /* worker threads of the frame's per-scene host bookkeeping (body rebuilds, connectivity), 1-16 (0: SF_HOST_THREADS,
  * else the cores, at most 16); results do not depend on it (no counterpart in the reference: a host choice) */
 int sf_lat_set_host_threads(sf_lat* L, int n);
+/* connectivity (src/lib.rs conn_struct :1085-1141): the search for the bricks of a structure that still reach the
+ * ground runs on the device (on, the default: a breadth-first search per structure, its bits copied back once per
+ * pass) or on the host (off); the detached parts are then made bodies on the host in the reference's order. Results
+ * do not depend on it (no counterpart in the reference: a placement choice, a switch for tests) */
+int sf_lat_set_conn_device(sf_lat* L, int on);
 /* structures of at most max_bricks bricks are swept with their potential in the workgroup's LDS from packed 16-byte
  * stencil rows; larger ones from the full neighbour and conductance rows with the potential in global memory.
  * -1 (the default) takes the device's LDS limit (about 40k bricks on gfx950); 0 sends every structure down the

@@ -129,6 +129,7 @@ _PROTOS = {
     "sf_lat_phase_ms": (c_int, [c_void_p, c_void_p]),
     "sf_lat_set_fused": (c_int, [c_void_p, c_int]),
     "sf_lat_set_host_threads": (c_int, [c_void_p, c_int]),
+    "sf_lat_set_conn_device": (c_int, [c_void_p, c_int]),
     "sf_lat_set_sor_lds": (c_int, [c_void_p, c_int]),
     "sf_legacy_init": (c_uint, []),
     "sf_legacy_step": (None, [c_float]),

@@ -1162,6 +1162,56 @@ __device__ __forceinline__ bool in_struct(const LD& D, uint32_t i, int s_local) 
     int m = D.mat[i];
     return D.sid[i] == s_local && m != 0 && m < SLUG && D.body[i] < 0;
 }
+
+// ---------------------------------------------------------------- structure connectivity (src/lib.rs:1085-1100)
+// The bricks of a structure that reach the ground through linked bonds: a breadth-first search from its grounded
+// footings, one workgroup per listed structure (list[k] = global index, scene-local index), level by level; a brick
+// joins the queue once, through the lane whose exchange on its mark claimed it, so the set reached does not depend on
+// the order lanes run in. Marks are this search's token (nothing to clear). The result goes back as one bit per brick
+// of the structure's range at out + nlist + woff[k], and out[k] = 1 when a linked brick lay outside the range (bonds
+// never leave a structure; the host then searches that structure itself).
+__global__ void __launch_bounds__(NT) k_conn_reach(LD D, const int2* list, const uint32_t* woff, uint32_t nlist, uint32_t* out, uint32_t token) {
+    __shared__ uint32_t s_tail;
+    const int2 e = list[blockIdx.x];
+    const uint32_t lo = (uint32_t)D.st_lo[e.x], hi = (uint32_t)D.st_hi[e.x], n = hi - lo;
+    uint32_t* q = D.cq + lo;
+    if (threadIdx.x == 0) s_tail = 0u;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += NT)
+        if (in_struct(D, i, e.y) && D.bxyz[i].y == 0 && D.bond[i].w > 0.0f) { D.creach[i] = token; q[atomicAdd(&s_tail, 1u)] = i; }
+    __syncthreads();
+    uint32_t head = 0, tail = s_tail;
+    int outside = 0;
+    while (head < tail) {
+        for (uint32_t k = head + threadIdx.x; k < tail; k += NT) {
+            const uint32_t i = q[k];
+            for (int d = 0; d < 6; d++) {
+                const int j = linked(D, i, d, nullptr);
+                if (j < 0) continue;
+                if ((uint32_t)j < lo || (uint32_t)j >= hi) { outside = 1; continue; }
+                if (atomicExch(&D.creach[j], token) != token) q[atomicAdd(&s_tail, 1u)] = (uint32_t)j;
+            }
+        }
+        __syncthreads();
+        head = tail;
+        tail = s_tail;
+        __syncthreads();
+    }
+    outside = __syncthreads_or(outside);
+    if (threadIdx.x == 0) out[blockIdx.x] = outside ? 1u : 0u;
+    // the reached bits, 64 bricks per wavefront at a time
+    const int lane = threadIdx.x & 63;
+    const uint32_t words = (n + 31u) / 32u;
+    uint32_t* bits = out + nlist + woff[blockIdx.x];
+    for (uint32_t c = (threadIdx.x >> 6) * 64u; c < n; c += NT) {
+        const uint32_t k = c + (uint32_t)lane;
+        const unsigned long long m = __ballot(k < n && D.creach[lo + k] == token);
+        if (lane == 0) {
+            bits[c / 32u] = (uint32_t)m;
+            if (c / 32u + 1u < words) bits[c / 32u + 1u] = (uint32_t)(m >> 32);
+        }
+    }
+}
 // block max of two values; every thread gets the result
 template <int BT>
 __device__ __forceinline__ void block_max2(float& a, float& b, float* sh) {
@@ -2008,6 +2058,16 @@ struct Lat {
     bool lds_attr = false, lds_attr_fs = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
     std::vector<int32_t> slist, slist_up;  // scenes with contacts this substep; the list on the device
     Stage stg_ctl, stg_csr, stg_delta, stg_brec;  // uploads: sweep controls, dynamic brick lists, brick and body deltas
+    // structure connectivity on the device: on (default) the grounded search runs as k_conn_reach, off the host
+    // searches; the list of structures searched this pass, per structure its place in it (-1: none), the results
+    bool conn_device = true;
+    uint32_t ctoken = 0;
+    Stage stg_conn;
+    DBuf<uint32_t> d_creach_out;
+    HBuf<uint32_t> h_creach_out;
+    std::vector<int> reach_idx;
+    std::vector<uint32_t> reach_woff;
+    size_t reach_n = 0;
     DBuf<uint8_t> d_rbk;     // the structural results packed for one copy
     uint32_t kev_pack = 64, kbw_pack = 256;  // crush events and bond writes the packed block carries
     int bcap = 0, dcap = 0, scap = 0;
@@ -2118,6 +2178,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.nb6, 6 * B, &L->per_brick, 6); r |= L->dalloc(D.bond, B, &L->per_brick, 1); r |= L->dalloc(D.bmax, B, &L->per_brick, 1); r |= L->dalloc(D.local, B, &L->per_brick, 1);
     r |= L->dalloc(D.wpos, B, &L->per_brick, 1); r |= L->dalloc(D.phi, B, &L->per_brick, 1); r |= L->dalloc(D.ext, B, &L->per_brick, 1); r |= L->dalloc(D.extn, B, &L->per_brick, 1);
     r |= L->dalloc(D.stress, B, &L->per_brick, 1); r |= L->dalloc(D.sup, B, &L->per_brick, 1);
+    r |= L->dalloc(D.creach, B, &L->per_brick, 1); r |= L->dalloc(D.cq, B, &L->per_brick, 1);
     r |= L->dalloc(D.bd_scene, N, &L->per_body, 1); r |= L->dalloc(D.alive, N, &L->per_body, 1); r |= L->dalloc(D.asleep, N, &L->per_body, 1); r |= L->dalloc(D.wake, N, &L->per_body, 1);
     r |= L->dalloc(D.pos, N, &L->per_body, 1); r |= L->dalloc(D.q, N, &L->per_body, 1); r |= L->dalloc(D.v, N, &L->per_body, 1); r |= L->dalloc(D.w, N, &L->per_body, 1);
     r |= L->dalloc(D.vpre, N, &L->per_body, 1); r |= L->dalloc(D.wpre, N, &L->per_body, 1); r |= L->dalloc(D.vlast, N, &L->per_body, 1); r |= L->dalloc(D.wlast, N, &L->per_body, 1);
@@ -2443,14 +2504,22 @@ void conn_struct(Lat* L, HScene& S, int sl) {
     ScrRef W = scr(L);
     HStruct& T = L->sts[S.st0 + sl];
     uint32_t sid = ++S.stamp_id;
-    int qh = 0, qt = 0;
-    for (int i = T.lo; i < T.hi; i++)
-        if (hin_struct(L, i, sl) && L->bxyz[i].y == 0 && L->bond[i].w > 0.0f) { L->stamp[i] = sid; W.q[qt++] = (uint32_t)i; }
-    while (qh < qt) {
-        int i = (int)W.q[qh++];
-        for (int d = 0; d < 6; d++) {
-            int j = hlinked(L, i, d);
-            if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; W.q[qt++] = (uint32_t)j; }
+    const int g = S.st0 + sl;
+    const int ri = g < (int)L->reach_idx.size() ? L->reach_idx[g] : -1;
+    if (ri >= 0 && L->h_creach_out.p[ri] == 0u) {
+        // the grounded bricks as the device's search found them (the same set the search below finds)
+        const uint32_t* bits = L->h_creach_out.p + L->reach_n + L->reach_woff[ri];
+        for (int i = T.lo; i < T.hi; i++) { const uint32_t k = (uint32_t)(i - T.lo); if ((bits[k >> 5] >> (k & 31u)) & 1u) L->stamp[i] = sid; }
+    } else {
+        int qh = 0, qt = 0;
+        for (int i = T.lo; i < T.hi; i++)
+            if (hin_struct(L, i, sl) && L->bxyz[i].y == 0 && L->bond[i].w > 0.0f) { L->stamp[i] = sid; W.q[qt++] = (uint32_t)i; }
+        while (qh < qt) {
+            int i = (int)W.q[qh++];
+            for (int d = 0; d < 6; d++) {
+                int j = hlinked(L, i, d);
+                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; W.q[qt++] = (uint32_t)j; }
+            }
         }
     }
     bool retry = false;
@@ -3352,6 +3421,51 @@ void merge_touched(Lat* L) {
     }
 }
 
+// The grounded search of every structure the host will check for detached parts (connectivity), on the device: the
+// host's changes so far go down first, the structures' bits come back in one copy. Off (conn_device false) or with
+// nothing to check, nothing runs and conn_struct searches on the host.
+int conn_reach(Lat* L, const std::vector<int>& need) {
+    L->reach_idx.clear();
+    if (!L->conn_device) return 0;
+    std::vector<int2> list;
+    L->reach_woff.clear();
+    uint32_t words = 0;
+    for (int si : need) {
+        const HScene& S = L->sc[si];
+        if (!S.conn_any) continue;
+        for (int sl = 0; sl < S.nst; sl++) {
+            const HStruct& T = L->sts[S.st0 + sl];
+            if (!T.conn || T.hi <= T.lo) continue;
+            if (L->reach_idx.empty()) L->reach_idx.assign(L->sts.size(), -1);
+            L->reach_idx[S.st0 + sl] = (int)list.size();
+            list.push_back(make_int2(S.st0 + sl, sl));
+            L->reach_woff.push_back(words);
+            words += (uint32_t)((T.hi - T.lo + 31) / 32);
+        }
+    }
+    if (list.empty()) return 0;
+    if (flush_delta(L)) return -1;
+    const size_t nl = list.size(), bytes = sizeof(int2) * nl + sizeof(uint32_t) * nl;
+    uint8_t* hs = L->stg_conn.begin(bytes);
+    if (!hs) return -1;
+    std::memcpy(hs, list.data(), sizeof(int2) * nl);
+    std::memcpy(hs + sizeof(int2) * nl, L->reach_woff.data(), sizeof(uint32_t) * nl);
+    if (L->stg_conn.end(bytes, L->st)) return -1;
+    const size_t outn = nl + words;
+    if (L->d_creach_out.n < outn && L->d_creach_out.alloc(outn + outn / 2, L->st)) return -1;
+    if (L->h_creach_out.reserve(outn)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    const uint32_t token = ++L->ctoken;
+    const int2* dl = reinterpret_cast<const int2*>(L->stg_conn.d.p);
+    const uint32_t* dw = reinterpret_cast<const uint32_t*>(L->stg_conn.d.p + sizeof(int2) * nl);
+    hipLaunchKernelGGL(k_conn_reach, dim3((unsigned)nl), dim3(NT), 0, L->st, L->D, dl, dw, (uint32_t)nl, L->d_creach_out.p, token);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipMemcpyAsync(L->h_creach_out.p, L->d_creach_out.p, sizeof(uint32_t) * outn, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    L->reach_n = nl;
+    return 0;
+}
+
 int host_topology(Lat* L, bool rebuild) {
     struct Acc { Lat* L; std::chrono::steady_clock::time_point t; ~Acc() { L->ph[3] += ms_since(t); } } acc{L, std::chrono::steady_clock::now()};
     std::vector<int> need;
@@ -3365,25 +3479,36 @@ int host_topology(Lat* L, bool rebuild) {
     if (pull_bodies(L, need)) return -1;
     L->sp[0] += ms_since(t);
     t = std::chrono::steady_clock::now();
-    auto scene_work = [&](int si) {
+    // per scene: rebuild split bodies, then (after the device has searched the structures to check) connectivity
+    auto rebuild_work = [&](int si) {
         HScene& S = L->sc[si];
         if (rebuild) for (int lb = 0; lb < S.hw; lb++) { int b = S.bd0 + lb; if (L->alive[b] && L->dirty[b]) rebuild_body(L, S, b); }
-        connectivity(L, S);
     };
+    auto conn_work = [&](int si) { connectivity(L, L->sc[si]); };
     Pool* pool = need.size() >= 2 ? host_pool(L) : nullptr;
-    if (pool) {
-        size_t maxb = 0;
-        for (int si : need) maxb = std::max<size_t>(maxb, (size_t)L->sc[si].bcap);
-        for (auto& w : L->scr) w->reserve(maxb);
-        pool->run((int)need.size(), [&](int k, int slot) {
-            t_scr = L->scr[slot].get();
-            scene_work(need[k]);
-            t_scr = nullptr;
-        });
-        merge_touched(L);
-    } else {
-        for (int si : need) scene_work(si);
-    }
+    auto run = [&](const std::function<void(int)>& fn) {
+        if (pool) {
+            size_t maxb = 0;
+            for (int si : need) maxb = std::max<size_t>(maxb, (size_t)L->sc[si].bcap);
+            for (auto& w : L->scr) w->reserve(maxb);
+            pool->run((int)need.size(), [&](int k, int slot) {
+                t_scr = L->scr[slot].get();
+                fn(need[k]);
+                t_scr = nullptr;
+            });
+            merge_touched(L);
+        } else {
+            for (int si : need) fn(si);
+        }
+    };
+    if (rebuild) run(rebuild_work);
+    L->sp[1] += ms_since(t);
+    t = std::chrono::steady_clock::now();
+    if (conn_reach(L, need)) return -1;
+    L->sp[12] += ms_since(t);
+    t = std::chrono::steady_clock::now();
+    run(conn_work);
+    L->reach_idx.clear();
     L->sp[2] += ms_since(t);
     t = std::chrono::steady_clock::now();
     if (flush_delta(L)) return -1;
@@ -4047,6 +4172,11 @@ int sf_lat_set_host_threads(sf_lat* h, int n) {
 int sf_lat_set_fused(sf_lat* h, int max_contacts) {
     if (!h || max_contacts < 0) return SF_E_ARG;
     h->h.L.fuse_max = max_contacts;
+    return 0;
+}
+int sf_lat_set_conn_device(sf_lat* h, int on) {
+    if (!h) return SF_E_ARG;
+    h->h.L.conn_device = on != 0;
     return 0;
 }
 int sf_lat_set_sor_lds(sf_lat* h, int max_bricks) {

@@ -143,6 +143,10 @@ struct LD {
     uint32_t* bwi;             // bond writes: brick * 4 + axis
     float* bwv;
     uint32_t evcap, bwcap, pcap;
+    // structure connectivity: per brick the token of the last search that reached it, and the search queue (each
+    // structure's part at its own brick range)
+    uint32_t* creach;
+    uint32_t* cq;
     // scan scratch
     uint32_t* scan_tmp;
 };

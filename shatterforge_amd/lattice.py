@@ -137,8 +137,12 @@ class Lattice:
         out = np.zeros(16, np.float64)
         check(lib().sf_lat_subphase_ms(self._h, ptr(out)))
         keys = ["topo_pull", "topo_rebuild", "topo_conn", "topo_flush", "topo_csr", "topo_upload", "topo_scenes",
-                "st_upload", "st_launch", "st_wait", "st_apply", "st_flush"]
+                "st_upload", "st_launch", "st_wait", "st_apply", "st_flush", "topo_reach"]
         return dict(zip(keys, out.tolist()))
+
+    def set_conn_device(self, on):
+        """connectivity's grounded search on the device (default) or on the host; results do not depend on it"""
+        check(lib().sf_lat_set_conn_device(self._h, 1 if on else 0))
 
     def set_fused(self, max_contacts):
         """scenes of at most max_contacts contacts are solved by one workgroup each; 0: colour-by-colour launches"""

@@ -32,12 +32,13 @@ def _assert_same(G, s, O, tag):
     assert G.rng(s) == O.rng(), f"{tag}: random stream"
 
 
-def _run(oracle, case, frames, check_every=1, fused=True, sor_lds=-1):
+def _run(oracle, case, frames, check_every=1, fused=True, sor_lds=-1, conn_device=True):
     nb = lc.brick_count(case) + 8
     nd = 4096
     G = Lattice(nb, nd, scenes=1)
     if not fused:
         G.set_fused(0)
+    G.set_conn_device(conn_device)
     G.set_sor_lds(sor_lds)
     s = G.add_scene(nb, nd, structs=4)
     O = oracle.LatticeOracle(order=oracle.ORDER_DEV, brick_cap=nb, body_cap=nd)
@@ -70,6 +71,16 @@ def test_tower_blast_bit_exact(oracle, route):
     # the contact solve as one workgroup per scene and as colour-by-colour launches: both equal the oracle
     print(f"case: 8x8x16 tower, 4 charges at the column bases, 120 frames, solve as {route}", flush=True)
     G, s, O = _run(oracle, dict(kind="tower", w=8, h=16, nch=4, r=2.2, p=1.4), 120, fused=route.startswith("work"))
+    assert O.stats()[2] > 0
+
+
+@pytest.mark.parametrize("search", ["device", "host"])
+def test_connectivity_search_on_device_and_host_bit_exact(oracle, search):
+    # connectivity's grounded search as the device's breadth-first search per structure (default) and as the host's
+    # search: the detached parts, their bodies and everything after equal the oracle either way
+    print(f"case: 2500-brick frame building, 4 charges r=3, 150 frames, grounded search on the {search}", flush=True)
+    G, s, O = _run(oracle, dict(kind="building", n=2500, seed=7, nch=4, r=3.0, p=1.4), 150, check_every=5,
+                   conn_device=search == "device")
     assert O.stats()[2] > 0
 
 
