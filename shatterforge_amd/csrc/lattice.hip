@@ -1164,48 +1164,81 @@ __device__ __forceinline__ bool in_struct(const LD& D, uint32_t i, int s_local) 
 }
 
 // ---------------------------------------------------------------- structure connectivity (src/lib.rs:1085-1100)
-// The bricks of a structure that reach the ground through linked bonds: a breadth-first search from its grounded
-// footings, one workgroup per listed structure (list[k] = global index, scene-local index), level by level; a brick
-// joins the queue once, through the lane whose exchange on its mark claimed it, so the set reached does not depend on
-// the order lanes run in. Marks are this search's token (nothing to clear). The result goes back as one bit per brick
-// of the structure's range at out + nlist + woff[k], and out[k] = 1 when a linked brick lay outside the range (bonds
-// never leave a structure; the host then searches that structure itself).
-__global__ void __launch_bounds__(NT) k_conn_reach(LD D, const int2* list, const uint32_t* woff, uint32_t nlist, uint32_t* out, uint32_t token) {
-    __shared__ uint32_t s_tail;
-    const int2 e = list[blockIdx.x];
-    const uint32_t lo = (uint32_t)D.st_lo[e.x], hi = (uint32_t)D.st_hi[e.x], n = hi - lo;
-    uint32_t* q = D.cq + lo;
-    if (threadIdx.x == 0) s_tail = 0u;
-    __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += NT)
-        if (in_struct(D, i, e.y) && D.bxyz[i].y == 0 && D.bond[i].w > 0.0f) { D.creach[i] = token; q[atomicAdd(&s_tail, 1u)] = i; }
-    __syncthreads();
-    uint32_t head = 0, tail = s_tail;
-    int outside = 0;
-    while (head < tail) {
-        for (uint32_t k = head + threadIdx.x; k < tail; k += NT) {
-            const uint32_t i = q[k];
-            for (int d = 0; d < 6; d++) {
-                const int j = linked(D, i, d, nullptr);
-                if (j < 0) continue;
-                if ((uint32_t)j < lo || (uint32_t)j >= hi) { outside = 1; continue; }
-                if (atomicExch(&D.creach[j], token) != token) q[atomicAdd(&s_tail, 1u)] = (uint32_t)j;
+// The bricks of a structure that reach the ground through linked bonds are the connected components (under "linked":
+// same body, bond above zero; symmetric between a brick and its neighbour) that hold a grounded footing. Found for every
+// listed structure at once (list[k] = global index, scene-local index; blockIdx.y = k) by union-find over the
+// structures' bricks: each brick hooks, for its three positive directions, the larger root under the smaller with
+// atomic minima (parents only move to smaller indices of the same component, so whatever a racing store leaves is still
+// an ancestor); once the forest is final the footings mark their roots with this pass's token, and a brick is reached
+// when its root is marked (roots found by walking up without stores: a path-halving store racing a root store could
+// leave a brick pointing at an ancestor below its root). The set equals the host's breadth-first search, whatever order lanes run
+// in. The result: one bit per brick of each structure's range at out + nlist + woff[k]; out[k] = 1 when a linked brick
+// lay outside the range (bonds never leave a structure; the host then searches that structure itself).
+__device__ __forceinline__ uint32_t cc_load(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void cc_store(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ uint32_t cc_find(uint32_t* par, uint32_t x) {
+    while (true) {
+        const uint32_t p = cc_load(&par[x]);
+        if (p == x) return x;
+        const uint32_t g = cc_load(&par[p]);
+        if (g != p) cc_store(&par[x], g);
+        x = g;
+    }
+}
+// (each pass: blockIdx.y the listed structure, blockIdx.x and the lanes stride its brick range)
+#define CC_RANGE(k, lo, hi)                                                                   \
+    const int2 e_ = list[blockIdx.y];                                                         \
+    const uint32_t lo = (uint32_t)D.st_lo[e_.x], hi = (uint32_t)D.st_hi[e_.x];              \
+    const uint32_t k = blockIdx.y;                                                           \
+    (void)k;
+__global__ void __launch_bounds__(NT) k_cc_init(LD D, const int2* list) {
+    CC_RANGE(k, lo, hi)
+    for (uint32_t i = lo + blockIdx.x * NT + threadIdx.x; i < hi; i += gridDim.x * NT) D.cq[i] = i;
+}
+__global__ void __launch_bounds__(NT) k_cc_hook(LD D, const int2* list, uint32_t* out) {
+    CC_RANGE(k, lo, hi)
+    bool outside = false;
+    for (uint32_t i = lo + blockIdx.x * NT + threadIdx.x; i < hi; i += gridDim.x * NT) {
+        if (D.mat[i] == 0) continue;
+        for (int d = 0; d < 6; d += 2) {
+            const int jj = linked(D, i, d, nullptr);
+            if (jj < 0) continue;
+            const uint32_t j = (uint32_t)jj;
+            if (j < lo || j >= hi) { outside = true; continue; }
+            uint32_t a = i, b = j;
+            while (true) {
+                a = cc_find(D.cq, a);
+                b = cc_find(D.cq, b);
+                if (a == b) break;
+                if (a < b) { const uint32_t t = a; a = b; b = t; }
+                const uint32_t old = atomicMin(&D.cq[a], b);
+                if (old == a) break;
+                a = old;
             }
         }
-        __syncthreads();
-        head = tail;
-        tail = s_tail;
-        __syncthreads();
+        // (the other direction of a lateral or vertical link is the neighbour's positive one: d odd adds nothing)
     }
-    outside = __syncthreads_or(outside);
-    if (threadIdx.x == 0) out[blockIdx.x] = outside ? 1u : 0u;
-    // the reached bits, 64 bricks per wavefront at a time
+    if (outside) out[k] = 1u;
+}
+// the forest is final after k_cc_hook: a root found by walking up without stores is the component's
+__device__ __forceinline__ uint32_t cc_root(const uint32_t* par, uint32_t x) {
+    while (true) { const uint32_t p = cc_load(&par[x]); if (p == x) return x; x = p; }
+}
+__global__ void __launch_bounds__(NT) k_cc_root(LD D, const int2* list, uint32_t token) {
+    CC_RANGE(k, lo, hi)
+    for (uint32_t i = lo + blockIdx.x * NT + threadIdx.x; i < hi; i += gridDim.x * NT)
+        if (in_struct(D, i, e_.y) && D.bxyz[i].y == 0 && D.bond[i].w > 0.0f) cc_store(&D.creach[cc_root(D.cq, i)], token);
+}
+__global__ void __launch_bounds__(NT) k_cc_bits(LD D, const int2* list, const uint32_t* woff, uint32_t nlist, uint32_t* out, uint32_t token) {
+    CC_RANGE(k, lo, hi)
+    const uint32_t n = hi - lo, words = (n + 31u) / 32u;
+    uint32_t* bits = out + nlist + woff[k];
     const int lane = threadIdx.x & 63;
-    const uint32_t words = (n + 31u) / 32u;
-    uint32_t* bits = out + nlist + woff[blockIdx.x];
-    for (uint32_t c = (threadIdx.x >> 6) * 64u; c < n; c += NT) {
-        const uint32_t k = c + (uint32_t)lane;
-        const unsigned long long m = __ballot(k < n && D.creach[lo + k] == token);
+    const uint32_t wave = (blockIdx.x * NT + threadIdx.x) >> 6, nwaves = (gridDim.x * NT) >> 6;
+    for (uint32_t c = wave * 64u; c < n; c += nwaves * 64u) {
+        const uint32_t i = lo + c + (uint32_t)lane;
+        const bool r = c + (uint32_t)lane < n && D.mat[i] != 0 && cc_load(&D.creach[cc_root(D.cq, i)]) == token;
+        const unsigned long long m = __ballot(r);
         if (lane == 0) {
             bits[c / 32u] = (uint32_t)m;
             if (c / 32u + 1u < words) bits[c / 32u + 1u] = (uint32_t)(m >> 32);
@@ -2058,8 +2091,8 @@ struct Lat {
     bool lds_attr = false, lds_attr_fs = false;      // scenes of at most this many contacts are solved by one workgroup each (0: never)
     std::vector<int32_t> slist, slist_up;  // scenes with contacts this substep; the list on the device
     Stage stg_ctl, stg_csr, stg_delta, stg_brec;  // uploads: sweep controls, dynamic brick lists, brick and body deltas
-    // structure connectivity on the device: on (default) the grounded search runs as k_conn_reach, off the host
-    // searches; the list of structures searched this pass, per structure its place in it (-1: none), the results
+    // structure connectivity on the device: on (default) the grounded search runs as union-find kernels (k_cc_*),
+    // off the host searches; per structure its place in this pass's list (-1: none), the list's bit offsets, results
     bool conn_device = true;
     uint32_t ctoken = 0;
     Stage stg_conn;
@@ -3430,17 +3463,19 @@ int conn_reach(Lat* L, const std::vector<int>& need) {
     std::vector<int2> list;
     L->reach_woff.clear();
     uint32_t words = 0;
+    int maxn = 0;
     for (int si : need) {
         const HScene& S = L->sc[si];
         if (!S.conn_any) continue;
         for (int sl = 0; sl < S.nst; sl++) {
             const HStruct& T = L->sts[S.st0 + sl];
-            if (!T.conn || T.hi <= T.lo) continue;
+            if (!T.conn || T.hi <= T.lo || list.size() >= 65535) continue;  // (past a grid's height: the host searches)
             if (L->reach_idx.empty()) L->reach_idx.assign(L->sts.size(), -1);
             L->reach_idx[S.st0 + sl] = (int)list.size();
             list.push_back(make_int2(S.st0 + sl, sl));
             L->reach_woff.push_back(words);
             words += (uint32_t)((T.hi - T.lo + 31) / 32);
+            maxn = std::max(maxn, T.hi - T.lo);
         }
     }
     if (list.empty()) return 0;
@@ -3457,7 +3492,12 @@ int conn_reach(Lat* L, const std::vector<int>& need) {
     const uint32_t token = ++L->ctoken;
     const int2* dl = reinterpret_cast<const int2*>(L->stg_conn.d.p);
     const uint32_t* dw = reinterpret_cast<const uint32_t*>(L->stg_conn.d.p + sizeof(int2) * nl);
-    hipLaunchKernelGGL(k_conn_reach, dim3((unsigned)nl), dim3(NT), 0, L->st, L->D, dl, dw, (uint32_t)nl, L->d_creach_out.p, token);
+    fill_u32(L, L->d_creach_out.p, 0u, nl);
+    const dim3 grid((unsigned)std::min<int>((maxn + NT - 1) / NT, 64), (unsigned)nl);
+    hipLaunchKernelGGL(k_cc_init, grid, dim3(NT), 0, L->st, L->D, dl);
+    hipLaunchKernelGGL(k_cc_hook, grid, dim3(NT), 0, L->st, L->D, dl, L->d_creach_out.p);
+    hipLaunchKernelGGL(k_cc_root, grid, dim3(NT), 0, L->st, L->D, dl, token);
+    hipLaunchKernelGGL(k_cc_bits, grid, dim3(NT), 0, L->st, L->D, dl, dw, (uint32_t)nl, L->d_creach_out.p, token);
     SF_HIP(hipGetLastError());
     SF_HIP(hipMemcpyAsync(L->h_creach_out.p, L->d_creach_out.p, sizeof(uint32_t) * outn, hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
