@@ -316,3 +316,42 @@ def test_full_size_rubble_step_with_the_gpu_world_extensions(oracle):
     w.step(DT)
     o.step(DT)
     _assert_same_step(w, o, "rubble 50k extensions step 0")
+
+
+def test_full_size_rubble_300_steps_sane(oracle):
+    # the rubble pile (50,000 bodies) over its 300 steps with the GPU world's solver: the 20 m pile collapses under
+    # gravity, so kinetic energy grows, but kinetic plus potential energy never rises (no energy from nowhere: the
+    # solve, restitution and the mass-ratio limit only take energy out), nothing is NaN, no candidate or manifold is
+    # dropped, and no body sinks through the ground
+    print("case: rubble 50k bodies, 300 steps, energy bound and invariants", flush=True)
+    b, sb, s = scenes.rubble(50000, seed=1)
+    w = sf.World.from_scene(b, sb, s, params(**world.GPU_WORLD))
+    m = np.where(b[:, 13] > 0, 1.0 / np.maximum(b[:, 13].astype(np.float64), 1e-30), 0.0)
+
+    def energy(B):
+        v2 = (B[:, 7:10].astype(np.float64) ** 2).sum(1)
+        w3 = B[:, 10:13].astype(np.float64)
+        # rotational energy in the body frame: R^T w, principal inverse inertia B[:, 14:17]
+        q = B[:, 3:7].astype(np.float64)
+        x, y, z, ww = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+        R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y + z * ww), 2 * (x * z - y * ww)], 1),
+                      np.stack([2 * (x * y - z * ww), 1 - 2 * (x * x + z * z), 2 * (y * z + x * ww)], 1),
+                      np.stack([2 * (x * z + y * ww), 2 * (y * z - x * ww), 1 - 2 * (x * x + y * y)], 1)], 2)
+        wl = np.einsum("nij,ni->nj", R, w3)
+        ii = B[:, 14:17].astype(np.float64)
+        rot = 0.5 * (wl ** 2 / np.where(ii > 0, ii, np.inf)).sum(1)
+        return float((0.5 * m * v2 + rot + m * 9.81 * B[:, 1].astype(np.float64)).sum())
+
+    e0 = energy(w.bodies())
+    for k in range(300):
+        w.step(DT)
+        if k % 30 == 29:
+            B = w.bodies()
+            st = w.stats()
+            assert not np.isnan(B).any(), f"step {k}: NaN"
+            assert st["dropped"] == 0, f"step {k}: dropped"
+            assert B[:, 1].min() > 0.15, f"step {k}: a body at y {B[:, 1].min()}"
+            e = energy(B)
+            assert e <= e0 * (1.0 + 1e-3), f"step {k}: energy {e:.4e} above the start's {e0:.4e}"
+    assert e < e0
+

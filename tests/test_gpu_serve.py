@@ -43,6 +43,28 @@ def test_whatif_same_alone_and_in_mix():
     assert any(sum(m["failed"]) > 0 or m["pieces"] > 0 for m in mixed.values())
 
 
+def test_large_queries_same_alone_and_in_mix():
+    # queries up to the 40k bricks the served stream allows, three sharing frames against each alone
+    print("case: 3 queries of 20k-40k bricks, 60 frames, 3 slots vs each alone", flush=True)
+    qs = [serve.make_query(i, 5000 + i, n_min=20000, n_max=40000) for i in range(3)]
+    srv = serve.WhatIfServer(slots=3, slot_bricks=42000, slot_bodies=8192, frames=60)
+    for q in qs:
+        srv.submit(q, 0.0)
+    while srv.busy():
+        srv.tick()
+    mixed = _answers(srv.done)
+    assert len(mixed) == 3 and all(q["load_resid"] <= 1e-6 for q in srv.done)
+    srv.close()
+    for q in qs:
+        alone = serve.WhatIfServer(slots=1, slot_bricks=42000, slot_bodies=8192, frames=60)
+        alone.submit(q, 0.0)
+        while alone.busy():
+            alone.tick()
+        a = _answers(alone.done)[q["id"]]
+        alone.close()
+        assert a == mixed[q["id"]], f"query {q['id']}: alone {a} vs in the mix {mixed[q['id']]}"
+
+
 def test_oversized_query_is_refused_and_the_rest_complete():
     print("case: 3 queries of 2k-5k bricks and one of 40k against a 3 x 6,000-brick pool", flush=True)
     qs = [serve.make_query(i, 3000 + i, n_min=2000, n_max=5000) for i in range(3)]
