@@ -259,7 +259,8 @@ def bench_whatif(n_queries=64, rate=200.0, frames=300, slots=64):
     res = {"queries": len(done), "rate_per_s": rate, "frames_per_query": frames, "slots": slots,
            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
            "answered_per_s": len(done) / el, "brick_steps_per_s": srv.brick_steps / el,
-           "mean_bricks": float(np.mean([q["bricks"] for q in qs])), "batched_frames": srv.frames_run}
+           "mean_bricks": float(np.mean([q["bricks"] for q in qs])), "batched_frames": srv.frames_run,
+           "server_ms": srv.timing()}
     srv.close()
     log(f"what-if: p50 {res['p50_ms']:.0f} ms p99 {res['p99_ms']:.0f} ms, {res['answered_per_s']:.1f} q/s")
     return res

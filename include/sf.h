@@ -401,6 +401,9 @@ int sf_serve_tick(sf_serve* S, float dt);
 int sf_serve_poll(sf_serve* S, sf_answer* out, int cap);
 /* queued, solving, live, answers waiting, bytes in use, budget, brick-steps, batched frames */
 int sf_serve_status(sf_serve* S, int64_t* out8);
+/* wall-clock ms the server has spent in admission (placing scenes, starting load solves), activation (scenes built,
+ * converged fields set, charges), frames and retirement (answers read, ranges released), since it was created */
+int sf_serve_timing(sf_serve* S, double* out4);
 /* the server's lattice handle (read-only use: statistics, timing) */
 sf_lat* sf_serve_world(sf_serve* S);
 

@@ -107,6 +107,7 @@ _PROTOS = {
     "sf_serve_tick": (c_int, [c_void_p, c_float]),
     "sf_serve_poll": (c_int, [c_void_p, c_void_p, c_int]),
     "sf_serve_status": (c_int, [c_void_p, c_void_p]),
+    "sf_serve_timing": (c_int, [c_void_p, c_void_p]),
     "sf_serve_world": (c_void_p, [c_void_p]),
     "sf_lat_add_structure": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "sf_lat_add_body": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -51,6 +51,7 @@ struct sf_serve {
     sf_lat* lat = nullptr;
     int slots = 0, slot_bodies = 0, frames = 0, max_solving = 3, margin = 64;
     int64_t budget = 0, in_use = 0, brick_steps = 0, frames_run = 0;
+    double ms[4] = {0, 0, 0, 0};   // wall ms in admission, activation, frames, retirement
     std::deque<int> free_slots;
     std::deque<std::unique_ptr<Query>> queue;
     std::vector<Batch> pending;
@@ -225,16 +226,22 @@ int sf_serve_submit(sf_serve* S, uint64_t id, int n, const int32_t* xyz, const u
 
 int sf_serve_tick(sf_serve* S, float dt) {
     if (!S) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    double t0 = now_ms();
     int r = admit(S);
     if (r < 0) return r;
+    double t1 = now_ms();
     if ((r = activate(S, nlive(S) == 0)) < 0) return r;
+    double t2 = now_ms();
     const int n = nlive(S);
     if (n) {
         if ((r = sf_lat_step(S->lat, dt)) < 0) return r;
         S->frames_run++;
         for (auto& q : S->live) if (q) { q->frame++; S->brick_steps += q->n; }
     }
+    double t3 = now_ms();
     if ((r = retire(S)) < 0) return r;
+    double t4 = now_ms();
+    S->ms[0] += t1 - t0; S->ms[1] += t2 - t1; S->ms[2] += t3 - t2; S->ms[3] += t4 - t3;
     return nlive(S);
 }
 
@@ -248,6 +255,11 @@ int sf_serve_poll(sf_serve* S, sf_answer* out, int cap) {
     return n;
 }
 
+int sf_serve_timing(sf_serve* S, double* out4) {
+    if (!S || !out4) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
+    for (int k = 0; k < 4; k++) out4[k] = S->ms[k];
+    return 0;
+}
 int sf_serve_status(sf_serve* S, int64_t* out8) {
     if (!S || !out8) { sf::set_error(SF_E_ARG, "bad argument"); return SF_E_ARG; }
     int64_t solving = 0;

@@ -112,6 +112,12 @@ class WhatIfServer:
         return dict(zip(["queued", "solving", "live", "answers", "bytes_in_use", "budget", "brick_steps", "frames"],
                         out.tolist()))
 
+    def timing(self):
+        """wall ms spent in admission, activation, frames and retirement since the server was made"""
+        out = np.zeros(4, np.float64)
+        check(lib().sf_serve_timing(self._h, ptr(out)))
+        return dict(zip(["admit", "activate", "frames", "retire"], out.tolist()))
+
     @property
     def brick_steps(self):
         return self.status()["brick_steps"]
