@@ -146,6 +146,7 @@ pub mod ffi {
         pub fn sf_serve_tick(s: *mut SfServe, dt: f32) -> c_int;
         pub fn sf_serve_poll(s: *mut SfServe, out: *mut SfAnswer, cap: c_int) -> c_int;
         pub fn sf_serve_status(s: *mut SfServe, out8: *mut i64) -> c_int;
+        pub fn sf_serve_timing(s: *mut SfServe, out4: *mut f64) -> c_int;
 
         // the legacy engine's own exports on the default world
         pub fn sf_legacy_init() -> u32;
