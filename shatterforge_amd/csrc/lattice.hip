@@ -1961,6 +1961,18 @@ struct Scratch {
 };
 thread_local Scratch* t_scr = nullptr;  // set while a pool worker (or the caller, as slot 0) runs a scene's work
 
+// A body to set up on the device (setup_body, src/lib.rs:844-929): its slot, its bricks [s, e) of the job brick list
+// in the reference's order, and the parent's pose and motion it inherits. flags: 1 a part detached from a structure
+// (its bricks' load field cleared), 2 the slot keeps its cooldown and life timer (the first part of a split body).
+struct SetupJob {
+    int32_t b, s, e, flags;
+    float4 ppos, pq, pv, pw;
+    float4 prot[3];
+    float keep_cool, keep_lt, pad0, pad1;
+};
+// what comes back of a set-up body: position, velocity, inverse inertia (body and world frames), mass, radius
+struct SetupOut { float4 pos, v; float4 iil[3], iiw[3]; float m, im, rad, pad; };
+
 struct HScene {
     int b0 = 0, bcap = 0, nb = 0;          // bricks [b0, b0 + nb), capacity bcap
     int bd0 = 0, dcap = 0, hw = 0, cursor = 0;
@@ -1977,6 +1989,12 @@ struct HScene {
     uint32_t dyn0 = 0, ndyn = 0;
     uint32_t c0 = 0, ncs = 0;              // contact range of the last substep
     bool host_dirty = false;               // host mirror changed since the last push
+    // bodies to set up on the device (decided by the host in the reference's order), their bricks, and the bodies a
+    // connectivity pass detached (blasted and their neighbours woken once they are set up)
+    std::vector<SetupJob> sjobs;
+    std::vector<uint32_t> sbricks;
+    std::vector<float4> sin;
+    std::vector<int> conn_new;
 };
 struct HStruct {
     int scene = 0, lo = 0, hi = 0, mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, top0 = 0, top = 0, warm = 0;
@@ -2095,7 +2113,9 @@ struct Lat {
     // off the host searches; per structure its place in this pass's list (-1: none), the list's bit offsets, results
     bool conn_device = true;
     uint32_t ctoken = 0;
-    Stage stg_conn;
+    Stage stg_conn, stg_setup;
+    DBuf<uint8_t> d_setup_out;
+    HBuf<uint8_t> h_setup_out;
     DBuf<uint32_t> d_creach_out;
     HBuf<uint32_t> h_creach_out;
     std::vector<int> reach_idx;
@@ -2184,7 +2204,7 @@ struct Lat {
     ~Lat() {
         for (void* p : owned) if (p) (void)hipFree(p);
         d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release();
-        d_brec.release(); d_idx.release(); d_wp.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release();
+        d_brec.release(); d_idx.release(); d_wp.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release(); stg_conn.release(); stg_setup.release(); d_setup_out.release(); h_setup_out.release();
         h_brec.release(); h_idx.release(); h_wp.release();
         h_ctrbuf.release(); h_scstat.release(); h_kl.release(); h_rb.release(); h_sk[0].release(); h_sk[1].release();
         for (hipEvent_t e : {ev_sk[0], ev_sk[1], ev_stat}) if (e) (void)hipEventDestroy(e);
@@ -2434,51 +2454,158 @@ int alloc_body(Lat* L, HScene& S) {
     return -1;
 }
 const LM IDM = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-void setup_body(Lat* L, int b, int s, int e, LV ppos, const LM& prot, LQ pq, LV pv, LV pw) {
-    ScrRef W = scr(L);
-    const Tables& T = tables();
-    L->lt[b] = 0.0f; L->resid[b] = 1.0f; L->maxr[b] = 0.0f;
-    float mass = 0.0f;
-    LV cl = lv(0.0f, 0.0f, 0.0f);
-    for (int t = s; t < e; t++) { int i = W.lst2[t]; float mm = T.mass[L->mat[i]]; mass += mm; cl = madd(cl, hv(L->local[i]), mm); }
-    cl = mul(cl, 1.0f / mass);
-    float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    float radv = 0.0f;
-    const float cube = S * S / 6.0f;
-    for (int t = s; t < e; t++) {
-        int i = W.lst2[t];
-        float mm = T.mass[L->mat[i]];
-        LV r = sub(hv(L->local[i]), cl);
-        L->local[i] = hf(r);
-        float rr = dot(r, r);
-        radv = maxf(radv, rr);
-        it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
-        it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
-        L->body[i] = b;
-        L->next[i] = t + 1 < e ? W.lst2[t + 1] : NONE;
+// Bodies set up on the device, one wavefront each (setup_body, src/lib.rs:844-929). Lane 0 sums the masses and the
+// mass-weighted positions brick after brick in the job's order (the reference's f32 order), then the inertia about
+// the centre with each brick's own S^2/6 term, and forms the body: inverse inertia, world inertia, radius, the
+// velocity of the new centre. Then every lane writes its share of the bricks: position in the body frame, world
+// position, body. `in` holds each brick's position in the parent's frame (xyz) and its mass (w).
+__global__ void __launch_bounds__(64) k_setup_bodies(LD D, const SetupJob* jobs, const uint32_t* idx, const float4* in,
+                                                     SetupOut* out, float4* bout) {
+    const SetupJob J = jobs[blockIdx.x];
+    __shared__ float sh[16];
+    const LM prot = load_m(J.prot);
+    if (threadIdx.x == 0) {
+        float mass = 0.0f;
+        LV cl = lv(0.0f, 0.0f, 0.0f);
+        for (int t = J.s; t < J.e; t++) { float4 a = in[t]; mass += a.w; cl = madd(cl, lv(a.x, a.y, a.z), a.w); }
+        cl = mul(cl, 1.0f / mass);
+        float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        float radv = 0.0f;
+        const float cube = S * S / 6.0f;
+        for (int t = J.s; t < J.e; t++) {
+            float4 a = in[t];
+            float mm = a.w;
+            LV r = sub(lv(a.x, a.y, a.z), cl);
+            float rr = dot(r, r);
+            radv = maxf(radv, rr);
+            it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
+            it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
+        }
+        it[3] = it[1]; it[6] = it[2]; it[7] = it[5];
+        LM itm;
+        for (int k = 0; k < 9; k++) itm.m[k] = it[k];
+        const LM iilm = minv(itm);
+        const LV off = mv(prot, cl);
+        const float4 pos = f4(add(lv4(J.ppos), off));
+        const float4 v = f4(add(lv4(J.pv), cross(lv4(J.pw), off)));
+        const LM iiw = mmul(mmul(prot, iilm), mt(prot));
+        const float rad = sqrtf(radv) + H * 1.8f;
+        const int b = J.b;
+        const bool keep = (J.flags & 2) != 0;
+        D.alive[b] = 1; D.count[b] = (uint32_t)(J.e - J.s);
+        D.pos[b] = pos; D.q[b] = J.pq; D.v[b] = v; D.w[b] = f4(lv4(J.pw)); D.vlast[b] = v; D.wlast[b] = f4(lv4(J.pw));
+        store_m(D.rot + 3 * (size_t)b, prot); store_m(D.iil + 3 * (size_t)b, iilm); store_m(D.iiw + 3 * (size_t)b, iiw);
+        D.m[b] = mass; D.im[b] = 1.0f / mass; D.rad[b] = rad; D.nmax[b] = 1.0f;
+        D.lt[b] = keep ? J.keep_lt : 0.0f; D.resid[b] = 1.0f; D.maxr[b] = 0.0f;
+        D.sleep[b] = 0.0f; D.asleep[b] = 0; D.wake[b] = 0; D.cool[b] = keep ? J.keep_cool : 0.15f;
+        SetupOut o;
+        o.pos = pos; o.v = v;
+        store_m(o.iil, iilm); store_m(o.iiw, iiw);
+        o.m = mass; o.im = 1.0f / mass; o.rad = rad; o.pad = 0.0f;
+        out[blockIdx.x] = o;
+        sh[0] = cl.x; sh[1] = cl.y; sh[2] = cl.z; sh[3] = pos.x; sh[4] = pos.y; sh[5] = pos.z;
     }
-    it[3] = it[1]; it[6] = it[2]; it[7] = it[5];
-    LM itm;
-    for (int k = 0; k < 9; k++) itm.m[k] = it[k];
-    LM iilm = minv(itm);
-    LV off = mv(prot, cl);
-    L->alive[b] = 1;
-    L->first[b] = W.lst2[s];
-    L->count[b] = (uint32_t)(e - s);
-    L->pos[b] = hf(add(ppos, off));
-    L->qq[b] = f4q(pq); L->rot[b] = prot;
-    L->v[b] = hf(add(pv, cross(pw, off)));
-    L->w[b] = hf(pw);
-    L->vlast[b] = L->v[b]; L->wlast[b] = hf(pw); L->nmax[b] = 1.0f;
-    L->m[b] = mass; L->im[b] = 1.0f / mass;
-    L->iil[b] = iilm;
-    LM t1 = mmul(prot, iilm), pt = mt(prot);
-    L->iiw[b] = mmul(t1, pt);
-    L->sleep[b] = 0.0f; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0; L->cool[b] = 0.15f;
-    L->rad[b] = sqrtf(radv) + H * 1.8f;
-    for (int t = s; t < e; t++) { int i = W.lst2[t]; L->wpos[i] = hf(add(hv(L->pos[b]), mv(prot, hv(L->local[i])))); L->touch(i, 3); }
-    L->btouch(b);
+    __syncthreads();
+    const LV cl = lv(sh[0], sh[1], sh[2]), pos = lv(sh[3], sh[4], sh[5]);
+    for (int t = J.s + (int)threadIdx.x; t < J.e; t += 64) {
+        float4 a = in[t];
+        const LV r = sub(lv(a.x, a.y, a.z), cl);
+        const uint32_t i = idx[t];
+        const float4 loc = f4(r), wp = f4(add(pos, mv(prot, r)));
+        D.local[i] = loc; D.wpos[i] = wp; D.body[i] = J.b;
+        if (J.flags & 1) { D.phi[i] = 0.0f; D.ext[i] = 0.0f; D.extn[i] = 0.0f; }
+        bout[2 * (size_t)t] = loc; bout[2 * (size_t)t + 1] = wp;
+    }
+}
+float4 m_row(const LM& m, int k) { return make_float4(m.m[3 * k], m.m[3 * k + 1], m.m[3 * k + 2], 0.0f); }
+void row_m(LM& m, int k, float4 r) { m.m[3 * k] = r.x; m.m[3 * k + 1] = r.y; m.m[3 * k + 2] = r.z; }
+// A new body in slot b from the bricks list[0, n) (in the reference's order), queued for the device's set-up
+// (k_setup_bodies; setup_body, src/lib.rs:844-929). The host keeps the slot's list structure and the constants the
+// set-up writes; the arithmetic comes back with run_setups. lattice: the bricks leave a structure, their positions in
+// the parent's frame are their lattice cells and their load field is cleared. keep: the slot keeps its cooldown and
+// life timer.
+void setup_queue(Lat* L, HScene& S, int b, const uint32_t* list, int n, bool lattice, LV ppos, const LM& prot, LQ pq,
+                 LV pv, LV pw, bool keep) {
+    const Tables& T = tables();
+    SetupJob J;
+    std::memset(&J, 0, sizeof(J));
+    J.b = b; J.s = (int)S.sbricks.size(); J.e = J.s + n; J.flags = (lattice ? 1 : 0) | (keep ? 2 : 0);
+    J.ppos = hf(ppos); J.pq = f4q(pq); J.pv = hf(pv); J.pw = hf(pw);
+    for (int k = 0; k < 3; k++) J.prot[k] = m_row(prot, k);
+    J.keep_cool = L->cool[b]; J.keep_lt = L->lt[b];
+    for (int t = 0; t < n; t++) {
+        const int i = (int)list[t];
+        const LV p = lattice ? lat_pos(L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z) : hv(L->local[i]);
+        S.sbricks.push_back((uint32_t)i);
+        S.sin.push_back(make_float4(p.x, p.y, p.z, T.mass[L->mat[i]]));
+        L->body[i] = b;
+        L->next[i] = t + 1 < n ? list[t + 1] : NONE;
+        if (lattice) { L->phi[i] = 0.0f; L->ext[i] = 0.0f; L->extn[i] = 0.0f; }
+        L->touch(i);
+    }
+    L->alive[b] = 1; L->first[b] = list[0]; L->count[b] = (uint32_t)n;
+    L->qq[b] = f4q(pq); L->rot[b] = prot; L->w[b] = hf(pw); L->wlast[b] = hf(pw);
+    if (!keep) { L->lt[b] = 0.0f; L->cool[b] = 0.15f; }
+    L->resid[b] = 1.0f; L->maxr[b] = 0.0f; L->nmax[b] = 1.0f;
+    L->sleep[b] = 0.0f; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0;
     L->lists_dirty = true;
+    S.sjobs.push_back(J);
+}
+// The queued set-ups of the given scenes: one upload, one launch (a wavefront per body), one copy back into the host
+// mirror, one wait. The device holds the bodies and bricks once the launch has run; the mirrored bodies are marked
+// for the next flush like every body the host has written.
+int run_setups(Lat* L, const std::vector<int>& scenes) {
+    size_t nj = 0, nbk = 0;
+    for (int si : scenes) { nj += L->sc[si].sjobs.size(); nbk += L->sc[si].sbricks.size(); }
+    if (!nj) return 0;
+    const size_t bj = sizeof(SetupJob) * nj, bin = sizeof(float4) * nbk, bytes = bj + bin + sizeof(uint32_t) * nbk;
+    uint8_t* hs = L->stg_setup.begin(bytes);
+    if (!hs) return -1;
+    SetupJob* hj = reinterpret_cast<SetupJob*>(hs);
+    float4* hin = reinterpret_cast<float4*>(hs + bj);
+    uint32_t* hidx = reinterpret_cast<uint32_t*>(hs + bj + bin);
+    size_t kj = 0, kb = 0;
+    for (int si : scenes) {
+        HScene& S = L->sc[si];
+        for (const SetupJob& J : S.sjobs) { hj[kj] = J; hj[kj].s += (int)kb; hj[kj].e += (int)kb; kj++; }
+        std::memcpy(hin + kb, S.sin.data(), sizeof(float4) * S.sin.size());
+        std::memcpy(hidx + kb, S.sbricks.data(), sizeof(uint32_t) * S.sbricks.size());
+        kb += S.sbricks.size();
+    }
+    if (L->stg_setup.end(bytes, L->st)) return -1;
+    const size_t outb = sizeof(SetupOut) * nj + 2 * sizeof(float4) * nbk;
+    if (L->d_setup_out.n < outb && L->d_setup_out.alloc(outb + outb / 2, L->st)) return -1;
+    if (L->h_setup_out.reserve(outb)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    const uint8_t* d = L->stg_setup.d.p;
+    SetupOut* dout = reinterpret_cast<SetupOut*>(L->d_setup_out.p);
+    float4* dbout = reinterpret_cast<float4*>(L->d_setup_out.p + sizeof(SetupOut) * nj);
+    LAUNCH(k_setup_bodies, (unsigned)nj, 64, L->D, reinterpret_cast<const SetupJob*>(d), reinterpret_cast<const uint32_t*>(d + bj + bin),
+           reinterpret_cast<const float4*>(d + bj), dout, dbout);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipMemcpyAsync(L->h_setup_out.p, L->d_setup_out.p, outb, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    const SetupOut* ho = reinterpret_cast<const SetupOut*>(L->h_setup_out.p);
+    const float4* hb = reinterpret_cast<const float4*>(L->h_setup_out.p + sizeof(SetupOut) * nj);
+    kj = 0; kb = 0;
+    for (int si : scenes) {
+        HScene& S = L->sc[si];
+        for (const SetupJob& J : S.sjobs) {
+            const SetupOut& o = ho[kj++];
+            const int b = J.b;
+            L->pos[b] = o.pos; L->v[b] = o.v; L->vlast[b] = o.v;
+            for (int k = 0; k < 3; k++) { row_m(L->iil[b], k, o.iil[k]); row_m(L->iiw[b], k, o.iiw[k]); }
+            L->m[b] = o.m; L->im[b] = o.im; L->rad[b] = o.rad;
+            L->btouch(b);
+        }
+        for (size_t t = 0; t < S.sbricks.size(); t++) {
+            const uint32_t i = S.sbricks[t];
+            L->local[i] = hb[2 * (kb + t)]; L->wpos[i] = hb[2 * (kb + t) + 1];
+        }
+        kb += S.sbricks.size();
+        S.sjobs.clear(); S.sbricks.clear(); S.sin.clear();
+    }
+    return 0;
 }
 void rebuild_body(Lat* L, HScene& S, int id) {
     ScrRef W = scr(L);
@@ -2528,9 +2655,7 @@ void rebuild_body(Lat* L, HScene& S, int id) {
         }
         int b = c == 0 ? id : alloc_body(L, S);
         if (b < 0) { for (int t = s0; t < e; t++) remove_brick_raw(L, W.lst2[t]); continue; }
-        float keep_cool = L->cool[id], keep_lt = L->lt[id];
-        setup_body(L, b, s0, e, ppos, prot, pq, pv, pw);
-        if (c == 0) { L->cool[b] = keep_cool; L->lt[b] = keep_lt; }
+        setup_queue(L, S, b, W.lst2.data() + s0, e - s0, false, ppos, prot, pq, pv, pw, c == 0);
     }
 }
 void conn_struct(Lat* L, HScene& S, int sl) {
@@ -2570,16 +2695,9 @@ void conn_struct(Lat* L, HScene& S, int sl) {
         }
         int b = alloc_body(L, S);
         if (b < 0) { retry = true; continue; }
-        for (int t = 0; t < n; t++) {
-            int i = W.lst2[t];
-            L->local[i] = hf(lat_pos(L->bxyz[i].x, L->bxyz[i].y, L->bxyz[i].z));
-            L->phi[i] = 0.0f; L->ext[i] = 0.0f; L->extn[i] = 0.0f;
-            L->touch(i, 7);
-        }
         LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
-        setup_body(L, b, 0, n, lv(0.0f, 0.0f, 0.0f), IDM, idq, lv(0.0f, 0.0f, 0.0f), lv(0.0f, 0.0f, 0.0f));
-        apply_blast_to_body(L, S, b);
-        wake_near(L, S, hv(L->pos[b]), L->rad[b] + 1.5f);
+        setup_queue(L, S, b, W.lst2.data(), n, true, lv(0.0f, 0.0f, 0.0f), IDM, idq, lv(0.0f, 0.0f, 0.0f), lv(0.0f, 0.0f, 0.0f), false);
+        S.conn_new.push_back(b);   // blasted, and its neighbours woken, once it is set up (conn_finish)
     }
     if (retry) { T.conn = true; S.conn_any = true; }
     uint32_t cntv = 0; int top = 0;
@@ -2590,6 +2708,15 @@ void connectivity(Lat* L, HScene& S) {
     if (!S.conn_any) return;
     S.conn_any = false;
     for (int sl = 0; sl < S.nst; sl++) { HStruct& T = L->sts[S.st0 + sl]; if (T.conn) { T.conn = false; conn_struct(L, S, sl); } }
+}
+// the detached parts, once set up, in the order they were found: the blast's impulse, then their neighbours woken
+// (src/lib.rs:1118-1121; a later part, already set up, is awake and is blasted on its own turn, so the order holds)
+void conn_finish(Lat* L, HScene& S) {
+    for (int b : S.conn_new) {
+        apply_blast_to_body(L, S, b);
+        wake_near(L, S, hv(L->pos[b]), L->rad[b] + 1.5f);
+    }
+    S.conn_new.clear();
 }
 uint32_t crush_at(Lat* L, HScene& S, int start, float e, LV p, uint32_t cap) {
     ScrRef W = scr(L);
@@ -2883,8 +3010,6 @@ __global__ void k_body_unpack(LD D, const BodyRec* in, uint32_t n) {
         D.resid[b] = r.resid; D.maxr[b] = r.maxr; D.nmax[b] = r.nmax;
     }
 }
-float4 m_row(const LM& m, int k) { return make_float4(m.m[3 * k], m.m[3 * k + 1], m.m[3 * k + 2], 0.0f); }
-void row_m(LM& m, int k, float4 r) { m.m[3 * k] = r.x; m.m[3 * k + 1] = r.y; m.m[3 * k + 2] = r.z; }
 
 // the scene's body slots in use, from the device into the host mirror (one packing kernel, one copy)
 // the body records of the given scenes (every slot up to each scene's high-water mark) into the host mirror: one
@@ -2990,6 +3115,7 @@ int plough_phase(Lat* L, uint32_t n) {
     }
     if (pull_bodies(L, scenes)) return -1;
     const Tables& T = tables();
+    std::vector<std::pair<int, LV>> spun;
     for (const CRec& c : rec) {
         int si = scene_of_contact(L, c.c);
         if (si < 0) continue;
@@ -3016,21 +3142,21 @@ int plough_phase(Lat* L, uint32_t n) {
         st_static_change(L, S, j);
         int b = alloc_body(L, S);
         if (b < 0) { dis.push_back(make_int2((int)c.c, -1)); continue; }
-        L->local[j] = hf(lat_pos(L->bxyz[j].x, L->bxyz[j].y, L->bxyz[j].z));
-        L->lst2[0] = (uint32_t)j;
         LV rv = rndv(S);
         LV vv = madd(mul(hv(L->v[a]), 0.75f), rv, 2.0f);
         LV ww = mul(rndv(S), 6.0f);
         LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
-        setup_body(L, b, 0, 1, lv(0.0f, 0.0f, 0.0f), IDM, idq, vv, lv(0.0f, 0.0f, 0.0f));
-        L->w[b] = hf(ww);
-        L->btouch(b);
+        const uint32_t one = (uint32_t)j;
+        setup_queue(L, S, b, &one, 1, true, lv(0.0f, 0.0f, 0.0f), IDM, idq, vv, lv(0.0f, 0.0f, 0.0f), false);
+        spun.push_back(std::make_pair(b, ww));   // its spin once it is set up
         float mm = L->m[a];
         float f = mm / (mm + T.mass[mj] * 1.5f);
         L->v[a] = hf(mul(hv(L->v[a]), f));
         L->btouch(a);
         dis.push_back(make_int2((int)c.c, j));
     }
+    if (run_setups(L, scenes)) return -1;
+    for (const auto& bw : spun) { L->w[bw.first] = hf(bw.second); L->btouch(bw.first); }
     if (flush_delta(L)) return -1;
     if (!dis.empty()) {
         if (dis.size() > L->D.pcap) { sf::set_error(SF_E_CAP, "too many plough events"); return SF_E_CAP; }
@@ -3542,6 +3668,7 @@ int host_topology(Lat* L, bool rebuild) {
         }
     };
     if (rebuild) run(rebuild_work);
+    if (run_setups(L, need)) return -1;
     L->sp[1] += ms_since(t);
     t = std::chrono::steady_clock::now();
     if (conn_reach(L, need)) return -1;
@@ -3549,6 +3676,8 @@ int host_topology(Lat* L, bool rebuild) {
     t = std::chrono::steady_clock::now();
     run(conn_work);
     L->reach_idx.clear();
+    if (run_setups(L, need)) return -1;
+    run([&](int si) { conn_finish(L, L->sc[si]); });
     L->sp[2] += ms_since(t);
     t = std::chrono::steady_clock::now();
     if (flush_delta(L)) return -1;
@@ -3791,7 +3920,8 @@ int add_body(Lat* L, int si, int n, const int32_t* xyz, const uint8_t* mat, cons
         L->lst2[k] = (uint32_t)i;
     }
     LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
-    setup_body(L, b, 0, n, lv(0.0f, 0.0f, 0.0f), IDM, idq, lv(v3[0], v3[1], v3[2]), lv(w3[0], w3[1], w3[2]));
+    setup_queue(L, S, b, L->lst2.data(), n, true, lv(0.0f, 0.0f, 0.0f), IDM, idq, lv(v3[0], v3[1], v3[2]), lv(w3[0], w3[1], w3[2]), false);
+    if (run_setups(L, std::vector<int>{si})) return -1;
     return b - S.bd0;
 }
 
