@@ -1185,6 +1185,19 @@ __device__ __forceinline__ uint32_t cc_find(uint32_t* par, uint32_t x) {
         x = g;
     }
 }
+// joins the trees of a and b: the larger root is hooked under the smaller by an atomic minimum, retried from the
+// value found when another hook got there first (the forest ends the same whatever order the hooks land in)
+__device__ __forceinline__ void cc_union(uint32_t* par, uint32_t a, uint32_t b) {
+    while (true) {
+        a = cc_find(par, a);
+        b = cc_find(par, b);
+        if (a == b) return;
+        if (a < b) { const uint32_t t = a; a = b; b = t; }
+        const uint32_t old = atomicMin(&par[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
 // (each pass: blockIdx.y the listed structure, blockIdx.x and the lanes stride its brick range)
 #define CC_RANGE(k, lo, hi)                                                                   \
     const int2 e_ = list[blockIdx.y];                                                         \
@@ -1205,16 +1218,7 @@ __global__ void __launch_bounds__(NT) k_cc_hook(LD D, const int2* list, uint32_t
             if (jj < 0) continue;
             const uint32_t j = (uint32_t)jj;
             if (j < lo || j >= hi) { outside = true; continue; }
-            uint32_t a = i, b = j;
-            while (true) {
-                a = cc_find(D.cq, a);
-                b = cc_find(D.cq, b);
-                if (a == b) break;
-                if (a < b) { const uint32_t t = a; a = b; b = t; }
-                const uint32_t old = atomicMin(&D.cq[a], b);
-                if (old == a) break;
-                a = old;
-            }
+            cc_union(D.cq, i, j);
         }
         // (the other direction of a lateral or vertical link is the neighbour's positive one: d odd adds nothing)
     }
@@ -1244,6 +1248,22 @@ __global__ void __launch_bounds__(NT) k_cc_bits(LD D, const int2* list, const ui
             if (c / 32u + 1u < words) bits[c / 32u + 1u] = (uint32_t)(m >> 32);
         }
     }
+}
+// The parts of split bodies (rebuild_body, src/lib.rs:930-1002): the live bricks of every body to rebuild, listed by
+// the host, joined over their linked bonds by the same union-find; each listed brick's root (the smallest brick index
+// of its part) comes back. A linked neighbour of a listed brick is a live brick of the same body, so it is listed too.
+__global__ void __launch_bounds__(NT) k_split_init(LD D, const uint32_t* list, uint32_t n) { GS(t, n) D.cq[list[t]] = list[t]; }
+__global__ void __launch_bounds__(NT) k_split_hook(LD D, const uint32_t* list, uint32_t n) {
+    GS(t, n) {
+        const uint32_t i = list[t];
+        for (int d = 0; d < 6; d += 2) {
+            const int j = linked(D, i, d, nullptr);
+            if (j >= 0) cc_union(D.cq, i, (uint32_t)j);
+        }
+    }
+}
+__global__ void __launch_bounds__(NT) k_split_root(LD D, const uint32_t* list, uint32_t n, uint32_t* out) {
+    GS(t, n) out[t] = cc_root(D.cq, list[t]);
 }
 // block max of two values; every thread gets the result
 template <int BT>
@@ -1995,6 +2015,9 @@ struct HScene {
     std::vector<uint32_t> sbricks;
     std::vector<float4> sin;
     std::vector<int> conn_new;
+    // bodies to rebuild: their live bricks in list order (rlist) and per body its slot, first entry and count
+    std::vector<uint32_t> rlist;
+    std::vector<int3> rbody;
 };
 struct HStruct {
     int scene = 0, lo = 0, hi = 0, mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, top0 = 0, top = 0, warm = 0;
@@ -2120,7 +2143,8 @@ struct Lat {
     HBuf<uint32_t> h_creach_out;
     std::vector<int> reach_idx;
     std::vector<uint32_t> reach_woff;
-    size_t reach_n = 0;
+    size_t reach_n = 0, split_off = 0;   // the read-back's structure statuses, then bits, then split roots from split_off
+    std::vector<size_t> sc_split;        // per scene: its first split root
     DBuf<uint8_t> d_rbk;     // the structural results packed for one copy
     uint32_t kev_pack = 64, kbw_pack = 256;  // crush events and bond writes the packed block carries
     int bcap = 0, dcap = 0, scap = 0;
@@ -2607,41 +2631,41 @@ int run_setups(Lat* L, const std::vector<int>& scenes) {
     }
     return 0;
 }
-void rebuild_body(Lat* L, HScene& S, int id) {
-    ScrRef W = scr(L);
+// rebuild_body (src/lib.rs:930-1002), first half: the body's live bricks in list order, its removed ones unlinked.
+// The parts are found on the device (k_split_*) for every listed body at once.
+void rebuild_list(Lat* L, HScene& S, int id) {
     L->dirty[id] = 0;
-    int n = 0;
+    const int off = (int)S.rlist.size();
     uint32_t k = L->first[id];
     while (k != NONE) {
         uint32_t nx = L->next[k];
-        if (L->body[k] == id) { if (L->mat[k] != 0) W.lst[n++] = k; else { L->body[k] = -1; L->touch(k); } }
+        if (L->body[k] == id) { if (L->mat[k] != 0) S.rlist.push_back(k); else { L->body[k] = -1; L->touch(k); } }
         k = nx;
     }
     L->lists_dirty = true;
     L->btouch(id);
+    const int n = (int)S.rlist.size() - off;
     if (n == 0) { L->alive[id] = 0; L->first[id] = NONE; return; }
+    S.rbody.push_back(make_int3(id, off, n));
+}
+// second half, from the parts' roots: parts numbered in the order their first brick comes in the list (the order the
+// reference's search finds them), each part's bricks in list order; single bricks of a heavily loaded scene dusted,
+// each part a body (the first keeps the slot), set up on the device
+void rebuild_split(Lat* L, HScene& S, int id, const uint32_t* list, const uint32_t* root, int n) {
+    ScrRef W = scr(L);
     uint32_t sid = ++S.stamp_id;
     uint32_t ncomp = 0;
     for (int t = 0; t < n; t++) {
-        uint32_t s0 = W.lst[t];
-        if (L->stamp[s0] == sid) continue;
-        L->stamp[s0] = sid; L->comp[s0] = ncomp;
-        int qh = 0, qt = 0;
-        W.q[qt++] = s0;
-        while (qh < qt) {
-            int i = (int)W.q[qh++];
-            for (int d = 0; d < 6; d++) {
-                int j = hlinked(L, i, d);
-                if (j >= 0 && L->stamp[j] != sid) { L->stamp[j] = sid; L->comp[j] = ncomp; W.q[qt++] = (uint32_t)j; }
-            }
-        }
-        ncomp++;
+        const uint32_t r = root[t];
+        if (L->stamp[r] != sid) { L->stamp[r] = sid; L->comp[r] = ncomp++; }
+        W.lst[t] = list[t];
+        W.q[t] = L->comp[r];
     }
     int ncmp = (int)ncomp;
     for (int c = 0; c <= ncmp; c++) W.cnt[c] = 0;
-    for (int t = 0; t < n; t++) W.cnt[L->comp[W.lst[t]] + 1]++;
+    for (int t = 0; t < n; t++) W.cnt[W.q[t] + 1]++;
     for (int c = 0; c < ncmp; c++) W.cnt[c + 1] += W.cnt[c];
-    for (int t = 0; t < n; t++) { int i = W.lst[t]; uint32_t c = L->comp[i]; W.lst2[W.cnt[c]++] = (uint32_t)i; }
+    for (int t = 0; t < n; t++) { int i = W.lst[t]; uint32_t c = W.q[t]; W.lst2[W.cnt[c]++] = (uint32_t)i; }
     LV ppos = hv(L->pos[id]), pv = hv(L->v[id]), pw = hv(L->w[id]);
     LM prot = L->rot[id];
     LQ pq = lq4(L->qq[id]);
@@ -3580,19 +3604,22 @@ void merge_touched(Lat* L) {
     }
 }
 
-// The grounded search of every structure the host will check for detached parts (connectivity), on the device: the
-// host's changes so far go down first, the structures' bits come back in one copy. Off (conn_device false) or with
-// nothing to check, nothing runs and conn_struct searches on the host.
-int conn_reach(Lat* L, const std::vector<int>& need) {
+// The device's searches of a topology pass, in one upload, one copy back and one wait: the parts of every body to
+// rebuild (k_split_*, over the bricks rebuild_list listed) and the grounded bricks of every structure the host will
+// check for detached parts (k_cc_*). The host's changes so far go down first. With the grounded search off
+// (conn_device false) the host searches the structures itself (conn_struct).
+int device_search(Lat* L, const std::vector<int>& need) {
     L->reach_idx.clear();
-    if (!L->conn_device) return 0;
+    L->reach_n = 0;
     std::vector<int2> list;
     L->reach_woff.clear();
     uint32_t words = 0;
     int maxn = 0;
+    size_t nsplit = 0;
     for (int si : need) {
         const HScene& S = L->sc[si];
-        if (!S.conn_any) continue;
+        nsplit += S.rlist.size();
+        if (!L->conn_device || !S.conn_any) continue;
         for (int sl = 0; sl < S.nst; sl++) {
             const HStruct& T = L->sts[S.st0 + sl];
             if (!T.conn || T.hi <= T.lo || list.size() >= 65535) continue;  // (past a grid's height: the host searches)
@@ -3604,31 +3631,48 @@ int conn_reach(Lat* L, const std::vector<int>& need) {
             maxn = std::max(maxn, T.hi - T.lo);
         }
     }
-    if (list.empty()) return 0;
+    if (list.empty() && !nsplit) return 0;
     if (flush_delta(L)) return -1;
-    const size_t nl = list.size(), bytes = sizeof(int2) * nl + sizeof(uint32_t) * nl;
+    const size_t nl = list.size(), bytes = sizeof(int2) * nl + sizeof(uint32_t) * (nl + nsplit);
     uint8_t* hs = L->stg_conn.begin(bytes);
     if (!hs) return -1;
     std::memcpy(hs, list.data(), sizeof(int2) * nl);
     std::memcpy(hs + sizeof(int2) * nl, L->reach_woff.data(), sizeof(uint32_t) * nl);
+    uint32_t* hsplit = reinterpret_cast<uint32_t*>(hs + sizeof(int2) * nl + sizeof(uint32_t) * nl);
+    for (int si : need) {
+        const HScene& S = L->sc[si];
+        std::memcpy(hsplit, S.rlist.data(), sizeof(uint32_t) * S.rlist.size());
+        hsplit += S.rlist.size();
+    }
     if (L->stg_conn.end(bytes, L->st)) return -1;
-    const size_t outn = nl + words;
+    // out: per listed structure its status, then the structures' bits, then the split roots
+    const size_t outn = nl + words + nsplit;
     if (L->d_creach_out.n < outn && L->d_creach_out.alloc(outn + outn / 2, L->st)) return -1;
     if (L->h_creach_out.reserve(outn)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-    const uint32_t token = ++L->ctoken;
     const int2* dl = reinterpret_cast<const int2*>(L->stg_conn.d.p);
     const uint32_t* dw = reinterpret_cast<const uint32_t*>(L->stg_conn.d.p + sizeof(int2) * nl);
-    fill_u32(L, L->d_creach_out.p, 0u, nl);
-    const dim3 grid((unsigned)std::min<int>((maxn + NT - 1) / NT, 64), (unsigned)nl);
-    hipLaunchKernelGGL(k_cc_init, grid, dim3(NT), 0, L->st, L->D, dl);
-    hipLaunchKernelGGL(k_cc_hook, grid, dim3(NT), 0, L->st, L->D, dl, L->d_creach_out.p);
-    hipLaunchKernelGGL(k_cc_root, grid, dim3(NT), 0, L->st, L->D, dl, token);
-    hipLaunchKernelGGL(k_cc_bits, grid, dim3(NT), 0, L->st, L->D, dl, dw, (uint32_t)nl, L->d_creach_out.p, token);
+    const uint32_t* ds = dw + nl;
+    if (nsplit) {
+        const unsigned g = sf_grid(nsplit);
+        LAUNCH(k_split_init, g, NT, L->D, ds, (uint32_t)nsplit);
+        LAUNCH(k_split_hook, g, NT, L->D, ds, (uint32_t)nsplit);
+        LAUNCH(k_split_root, g, NT, L->D, ds, (uint32_t)nsplit, L->d_creach_out.p + nl + words);
+    }
+    if (nl) {
+        const uint32_t token = ++L->ctoken;
+        fill_u32(L, L->d_creach_out.p, 0u, nl);
+        const dim3 grid((unsigned)std::min<int>((maxn + NT - 1) / NT, 64), (unsigned)nl);
+        hipLaunchKernelGGL(k_cc_init, grid, dim3(NT), 0, L->st, L->D, dl);
+        hipLaunchKernelGGL(k_cc_hook, grid, dim3(NT), 0, L->st, L->D, dl, L->d_creach_out.p);
+        hipLaunchKernelGGL(k_cc_root, grid, dim3(NT), 0, L->st, L->D, dl, token);
+        hipLaunchKernelGGL(k_cc_bits, grid, dim3(NT), 0, L->st, L->D, dl, dw, (uint32_t)nl, L->d_creach_out.p, token);
+    }
     SF_HIP(hipGetLastError());
     SF_HIP(hipMemcpyAsync(L->h_creach_out.p, L->d_creach_out.p, sizeof(uint32_t) * outn, hipMemcpyDeviceToHost, L->st));
     SF_HIP(hipStreamSynchronize(L->st));
     L->syncs++;
     L->reach_n = nl;
+    L->split_off = nl + words;
     return 0;
 }
 
@@ -3645,10 +3689,18 @@ int host_topology(Lat* L, bool rebuild) {
     if (pull_bodies(L, need)) return -1;
     L->sp[0] += ms_since(t);
     t = std::chrono::steady_clock::now();
-    // per scene: rebuild split bodies, then (after the device has searched the structures to check) connectivity
-    auto rebuild_work = [&](int si) {
+    // per scene: list the bodies to rebuild; the device then finds their parts and the structures' grounded bricks; per
+    // scene the split bodies and the detached parts are decided in the reference's order and set up on the device;
+    // last, per scene, the detached parts are blasted and their neighbours woken
+    auto list_work = [&](int si) {
         HScene& S = L->sc[si];
-        if (rebuild) for (int lb = 0; lb < S.hw; lb++) { int b = S.bd0 + lb; if (L->alive[b] && L->dirty[b]) rebuild_body(L, S, b); }
+        if (rebuild) for (int lb = 0; lb < S.hw; lb++) { int b = S.bd0 + lb; if (L->alive[b] && L->dirty[b]) rebuild_list(L, S, b); }
+    };
+    auto split_work = [&](int si) {
+        HScene& S = L->sc[si];
+        const uint32_t* roots = L->h_creach_out.p + L->split_off + L->sc_split[si];
+        for (const int3& r : S.rbody) rebuild_split(L, S, r.x, S.rlist.data() + r.y, roots + r.y, r.z);
+        S.rlist.clear(); S.rbody.clear();
     };
     auto conn_work = [&](int si) { connectivity(L, L->sc[si]); };
     Pool* pool = need.size() >= 2 ? host_pool(L) : nullptr;
@@ -3667,14 +3719,18 @@ int host_topology(Lat* L, bool rebuild) {
             for (int si : need) fn(si);
         }
     };
-    if (rebuild) run(rebuild_work);
-    if (run_setups(L, need)) return -1;
+    if (rebuild) run(list_work);
+    L->sc_split.assign(L->sc.size(), 0);
+    {
+        size_t o = 0;
+        for (int si : need) { L->sc_split[si] = o; o += L->sc[si].rlist.size(); }
+    }
     L->sp[1] += ms_since(t);
     t = std::chrono::steady_clock::now();
-    if (conn_reach(L, need)) return -1;
+    if (device_search(L, need)) return -1;
     L->sp[12] += ms_since(t);
     t = std::chrono::steady_clock::now();
-    run(conn_work);
+    run([&](int si) { split_work(si); conn_work(si); });
     L->reach_idx.clear();
     if (run_setups(L, need)) return -1;
     run([&](int si) { conn_finish(L, L->sc[si]); });
