@@ -2485,26 +2485,42 @@ const LM IDM = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
 // position, body. `in` holds each brick's position in the parent's frame (xyz) and its mass (w).
 __global__ void __launch_bounds__(64) k_setup_bodies(LD D, const SetupJob* jobs, const uint32_t* idx, const float4* in,
                                                      SetupOut* out, float4* bout) {
+    constexpr int CH = 256;            // bricks staged in LDS at a time: the lanes load a chunk, lane 0 sums it
     const SetupJob J = jobs[blockIdx.x];
     __shared__ float sh[16];
+    __shared__ float4 buf[CH];
     const LM prot = load_m(J.prot);
+    float mass = 0.0f;
+    LV cl = lv(0.0f, 0.0f, 0.0f);
+    for (int c0 = J.s; c0 < J.e; c0 += CH) {
+        const int m = min(CH, J.e - c0);
+        for (int k = (int)threadIdx.x; k < m; k += 64) buf[k] = in[c0 + k];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < m; k++) { float4 a = buf[k]; mass += a.w; cl = madd(cl, lv(a.x, a.y, a.z), a.w); }
+        __syncthreads();
+    }
+    cl = mul(cl, 1.0f / mass);
+    float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float radv = 0.0f;
+    const float cube = S * S / 6.0f;
+    for (int c0 = J.s; c0 < J.e; c0 += CH) {
+        const int m = min(CH, J.e - c0);
+        for (int k = (int)threadIdx.x; k < m; k += 64) buf[k] = in[c0 + k];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < m; k++) {
+                float4 a = buf[k];
+                float mm = a.w;
+                LV r = sub(lv(a.x, a.y, a.z), cl);
+                float rr = dot(r, r);
+                radv = maxf(radv, rr);
+                it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
+                it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
+            }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        float mass = 0.0f;
-        LV cl = lv(0.0f, 0.0f, 0.0f);
-        for (int t = J.s; t < J.e; t++) { float4 a = in[t]; mass += a.w; cl = madd(cl, lv(a.x, a.y, a.z), a.w); }
-        cl = mul(cl, 1.0f / mass);
-        float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        float radv = 0.0f;
-        const float cube = S * S / 6.0f;
-        for (int t = J.s; t < J.e; t++) {
-            float4 a = in[t];
-            float mm = a.w;
-            LV r = sub(lv(a.x, a.y, a.z), cl);
-            float rr = dot(r, r);
-            radv = maxf(radv, rr);
-            it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
-            it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
-        }
         it[3] = it[1]; it[6] = it[2]; it[7] = it[5];
         LM itm;
         for (int k = 0; k < 9; k++) itm.m[k] = it[k];
@@ -2530,12 +2546,12 @@ __global__ void __launch_bounds__(64) k_setup_bodies(LD D, const SetupJob* jobs,
         sh[0] = cl.x; sh[1] = cl.y; sh[2] = cl.z; sh[3] = pos.x; sh[4] = pos.y; sh[5] = pos.z;
     }
     __syncthreads();
-    const LV cl = lv(sh[0], sh[1], sh[2]), pos = lv(sh[3], sh[4], sh[5]);
+    const LV cen = lv(sh[0], sh[1], sh[2]), bpos = lv(sh[3], sh[4], sh[5]);
     for (int t = J.s + (int)threadIdx.x; t < J.e; t += 64) {
         float4 a = in[t];
-        const LV r = sub(lv(a.x, a.y, a.z), cl);
+        const LV r = sub(lv(a.x, a.y, a.z), cen);
         const uint32_t i = idx[t];
-        const float4 loc = f4(r), wp = f4(add(pos, mv(prot, r)));
+        const float4 loc = f4(r), wp = f4(add(bpos, mv(prot, r)));
         D.local[i] = loc; D.wpos[i] = wp; D.body[i] = J.b;
         if (J.flags & 1) { D.phi[i] = 0.0f; D.ext[i] = 0.0f; D.extn[i] = 0.0f; }
         bout[2 * (size_t)t] = loc; bout[2 * (size_t)t + 1] = wp;
