@@ -1265,6 +1265,131 @@ __global__ void __launch_bounds__(NT) k_split_hook(LD D, const uint32_t* list, u
 __global__ void __launch_bounds__(NT) k_split_root(LD D, const uint32_t* list, uint32_t n, uint32_t* out) {
     GS(t, n) out[t] = cc_root(D.cq, list[t]);
 }
+// ---------------------------------------------------------------- a charge (explode_at, src/lib.rs:2679-2716)
+// The bricks a charge may reach, listed by the host in the reference's order (each structure near the charge brick by
+// brick, then each body near it along its brick list) with their structure slot (-1: a body's brick). k_blast_event
+// gives each listed brick its event (blast_brick, :2679-2696): removed within half its material's blast radius,
+// damaged by power (1 - d / 1.35 rr)^2 1.3 within 1.35 rr, and marks the brick as listed this pass (creach = token,
+// cq = its rank in the order). k_blast_bonds then applies the damage to every bond a listed brick owns (the positive
+// x, y, z links and the footing): a link's two ends each damage it in rank order, each only while the other end is
+// not yet removed and the bond is still above zero (damage_brick, :725-771), so the f32 result is the sequential one.
+// k_blast_remove clears removed bricks; k_blast_bodies gives each body near the charge the blast impulse brick by brick
+// along its list, wakes it and limits its speeds (apply_blast_to_body :898-916, clamp_vel :917-929).
+enum : uint32_t { BL_NONE = 0, BL_REMOVE = 1, BL_DAMAGE = 2, BL_BROKE = 4 };
+struct BlastParams { float cx, cy, cz, r, power, br, bp, pad; uint32_t token, n, pad1, pad2; };
+__global__ void __launch_bounds__(NT) k_blast_event(LD D, BlastParams P, const int2* ent, uint32_t* ev, float* frac) {
+    const Tables& T = tables();
+    GS(t, P.n) {
+        const uint32_t i = (uint32_t)ent[t].x;
+        const int sl = ent[t].y;
+        const int m = D.mat[i];
+        uint32_t e = BL_NONE;
+        float fr = 0.0f;
+        const bool active = sl >= 0 ? in_struct(D, i, sl) : (m != 0 && m < SLUG);
+        if (active) {
+            const LV p = D.body[i] < 0 ? lat_pos(D.bxyz[i].x, D.bxyz[i].y, D.bxyz[i].z) : lv4(D.wpos[i]);
+            const float d = len(sub(p, lv(P.cx, P.cy, P.cz)));
+            const float rr = P.r * T.blast[m];
+            if (d < rr * 0.5f) e = BL_REMOVE;
+            else if (d < rr * 1.35f) { const float f = 1.0f - d / (rr * 1.35f); e = BL_DAMAGE; fr = P.power * f * f * 1.3f; }
+        }
+        // the entry acts for the brick (its events, the bonds it owns): a live brick of the structure, or of the body
+        if (m != 0 && (sl < 0 || (D.sid[i] == sl && D.body[i] < 0))) { D.creach[i] = P.token; D.cq[i] = t; }
+        ev[t] = e;
+        frac[t] = fr;
+    }
+}
+__device__ __forceinline__ float& f4_ref(float4& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : (a == 2 ? v.z : v.w)); }
+__device__ __forceinline__ float f4_get(const float4& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : (a == 2 ? v.z : v.w)); }
+__global__ void __launch_bounds__(NT) k_blast_bonds(LD D, BlastParams P, const int2* ent, uint32_t* ev, const float* frac, float4* bond_out) {
+    GS(t, P.n) {
+        const uint32_t o = (uint32_t)ent[t].x;
+        float4 bd = D.bond[o];
+        if (D.creach[o] == P.token && D.cq[o] == t) {   // (the entry that acts for the brick)
+            const float4 bm = D.bmax[o];
+            const uint32_t eo = ev[t] & 3u;
+            // alive at rank r: not empty, and not removed by its own event before r
+            auto alive = [&](uint32_t y, uint32_t r) {
+                if (D.mat[y] == 0) return false;
+                if (D.creach[y] != P.token) return true;
+                const uint32_t ry = D.cq[y];
+                return !((ev[ry] & 3u) == BL_REMOVE && ry < r);
+            };
+            for (int a = 0; a < 3; a++) {
+                const uint32_t j = D.nb6[6 * (size_t)o + 2 * a];
+                if (j == NONE) continue;
+                const bool jl = D.creach[j] == P.token;
+                const uint32_t tj = jl ? D.cq[j] : 0u;
+                const uint32_t ej = jl ? (ev[tj] & 3u) : BL_NONE;
+                // the link's damage events in rank order: (rank, acting entry, other end)
+                uint32_t r0 = t, r1 = tj;
+                bool h0 = eo == BL_DAMAGE, h1 = ej == BL_DAMAGE;
+                uint32_t x0 = o, x1 = j, y0 = j, y1 = o;
+                if (h0 && h1 && r1 < r0) { uint32_t q = r0; r0 = r1; r1 = q; q = x0; x0 = x1; x1 = q; q = y0; y0 = y1; y1 = q; bool hh = h0; h0 = h1; h1 = hh; }
+                else if (!h0 && h1) { r0 = r1; x0 = x1; y0 = y1; h0 = true; h1 = false; }
+                float v = f4_get(bd, a);
+                const float vm = f4_get(bm, a);
+                for (int k = 0; k < 2; k++) {
+                    const bool h = k == 0 ? h0 : h1;
+                    if (!h) continue;
+                    const uint32_t rk = k == 0 ? r0 : r1, xk = k == 0 ? x0 : x1, yk = k == 0 ? y0 : y1;
+                    if (!alive(yk, rk)) continue;
+                    if (v > 0.0f) {
+                        v -= vm * frac[rk] + 0.0f;
+                        if (v <= vm * 0.03f) { v = 0.0f; atomicOr(&ev[rk], BL_BROKE); }
+                    }
+                    (void)xk;
+                }
+                f4_ref(bd, a) = v;
+            }
+            if (eo == BL_DAMAGE && bd.w > 0.0f) {   // the footing: only the brick's own event
+                bd.w -= bm.w * frac[t] * 0.5f + 0.0f * 0.5f;
+                if (bd.w <= 0.0f) { bd.w = 0.0f; atomicOr(&ev[t], BL_BROKE); }
+            }
+            D.bond[o] = bd;
+        }
+        bond_out[t] = bd;
+    }
+}
+__global__ void __launch_bounds__(NT) k_blast_remove(LD D, BlastParams P, const int2* ent, const uint32_t* ev) {
+    GS(t, P.n) if ((ev[t] & 3u) == BL_REMOVE) D.mat[ent[t].x] = 0;
+}
+// one wavefront per body (lane 0): its list is body slot, first and last entry of `list`
+__global__ void __launch_bounds__(64) k_blast_bodies(LD D, BlastParams P, const int4* bodies, const uint32_t* list, float4* vw_out) {
+    if (threadIdx.x != 0) return;
+    const Tables& T = tables();
+    const int4 B = bodies[blockIdx.x];
+    const int b = B.x;
+    D.asleep[b] = 0; D.sleep[b] = 0.0f;
+    LV v = lv4(D.v[b]), w = lv4(D.w[b]);
+    const LV bp = lv4(D.pos[b]), c = lv(P.cx, P.cy, P.cz);
+    const LM iiw = load_m(D.iiw + 3 * (size_t)b);
+    const float im = D.im[b];
+    for (int t = B.y; t < B.z; t++) {
+        const uint32_t k = list[t];
+        const LV p = lv4(D.wpos[k]);
+        const LV d = sub(p, c);
+        const float dist = len(d);
+        if (dist < P.br) {
+            const float f = P.bp * 1600.0f * (1.0f - dist / P.br);
+            LV dir = dist > 1e-3f ? mul(d, 1.0f / dist) : lv(0.0f, 1.0f, 0.0f);
+            dir = add(dir, lv(0.0f, 0.35f, 0.0f));
+            const float mm = maxf(T.mass[D.mat[k]], 25.0f);
+            const LV j = mul(dir, f * mm / 300.0f);
+            v = madd(v, j, im);
+            const LV r = sub(p, bp);
+            w = add(w, mv(iiw, cross(r, j)));
+        }
+    }
+    const float vl = len(v);
+    if (vl > 60.0f) v = mul(v, 60.0f / vl);
+    const float a = len(w);
+    const float amax = minf(25.0f, 16.0f / maxf(D.rad[b], 0.3f));
+    if (a > amax) w = mul(w, amax / a);
+    D.v[b] = f4(v); D.w[b] = f4(w);
+    vw_out[2 * (size_t)blockIdx.x] = f4(v); vw_out[2 * (size_t)blockIdx.x + 1] = f4(w);
+}
+
 // block max of two values; every thread gets the result
 template <int BT>
 __device__ __forceinline__ void block_max2(float& a, float& b, float* sh) {
@@ -3951,31 +4076,102 @@ void blast_brick(Lat* L, HScene& S, int i, LV c, float r, float power) {
     if (d < rr * 0.5f) remove_brick(L, S, i);
     else if (d < rr * 1.35f) { float f = 1.0f - d / (rr * 1.35f); damage_brick(L, S, i, power * f * f * 1.3f, 0.0f); }
 }
+// A charge (explode_at, src/lib.rs:2697-2716), its bricks and bodies on the device (k_blast_*): the host lists what
+// the charge may reach in the reference's order, one launch per stage, one copy back into the host mirror, one wait;
+// the structures and bodies it disturbed are then marked in that order (for connectivity, the structural solve and
+// the next rebuild).
 int explode(Lat* L, int si, float x, float y, float z, float r, float power) {
     HScene& S = L->sc[si];
-    if (pull_bodies(L, si) || pull(L, si, G_WPOS)) return -1;
+    if (pull_bodies(L, si) || flush_delta(L)) return -1;
     LV c = lv(x, y, z);
     S.blast_c = c; S.blast_r = r * 2.2f; S.blast_p = power; S.blast_t = 0.25f;
     float reach = r * 1.4f * 1.35f + 0.5f;
+    std::vector<int2> ent;      // (brick, structure slot; -1: a body's brick) in the order the reference visits them
     for (int sl = 0; sl < S.nst; sl++) {
         HStruct& T = L->sts[S.st0 + sl];
         LV a = sub(c, lv(reach, reach, reach)), b = add(c, lv(reach, reach, reach));
         int x0 = fcell(a.x) + NX2, y0 = fcell(a.y), z0 = fcell(a.z) + NX2;
         int x1 = fcell(b.x) + NX2, y1 = fcell(b.y), z1 = fcell(b.z) + NX2;
         if (x1 < T.mn[0] || x0 > T.mx[0] || y1 < T.mn[1] || y0 > T.mx[1] || z1 < T.mn[2] || z0 > T.mx[2]) continue;
-        for (int i = T.lo; i < T.hi; i++) if (hin_struct(L, i, sl)) blast_brick(L, S, i, c, r, power);
+        for (int i = T.lo; i < T.hi; i++) ent.push_back(make_int2(i, sl));
     }
     for (int lb = 0; lb < S.hw; lb++) {
         int b = S.bd0 + lb;
         if (!(L->alive[b] && len(sub(hv(L->pos[b]), c)) < reach + L->rad[b])) continue;
-        uint32_t k = L->first[b];
-        while (k != NONE) { uint32_t nx = L->next[k]; blast_brick(L, S, (int)k, c, r, power); k = nx; }
+        for (uint32_t k = L->first[b]; k != NONE; k = L->next[k]) ent.push_back(make_int2((int)k, -1));
     }
+    std::vector<int4> bl;       // bodies the blast pushes: slot, their bricks [y, z) of `chain`
+    std::vector<uint32_t> chain;
     for (int lb = 0; lb < S.hw; lb++) {
         int b = S.bd0 + lb;
         if (!(L->alive[b] && len(sub(hv(L->pos[b]), c)) < S.blast_r + L->rad[b])) continue;
-        wake_body(L, b);
-        apply_blast_to_body(L, S, b);
+        const int s0 = (int)chain.size();
+        for (uint32_t k = L->first[b]; k != NONE; k = L->next[k]) chain.push_back(k);
+        bl.push_back(make_int4(b, s0, (int)chain.size(), 0));
+    }
+    const size_t n = ent.size(), nbl = bl.size(), nch = chain.size();
+    if (n || nbl) {
+        const size_t bytes = sizeof(int4) * nbl + sizeof(int2) * n + sizeof(uint32_t) * nch;
+        uint8_t* hs = L->stg_conn.begin(bytes);
+        if (!hs) return -1;
+        std::memcpy(hs, bl.data(), sizeof(int4) * nbl);
+        std::memcpy(hs + sizeof(int4) * nbl, ent.data(), sizeof(int2) * n);
+        std::memcpy(hs + sizeof(int4) * nbl + sizeof(int2) * n, chain.data(), sizeof(uint32_t) * nch);
+        if (L->stg_conn.end(bytes, L->st)) return -1;
+        // device out: bonds (float4 per entry), the bodies' velocities and spins, events (u32 per entry), fractions
+        const size_t outb = sizeof(float4) * (n + 2 * nbl) + sizeof(uint32_t) * n, devb = outb + sizeof(float) * n;
+        if (L->d_setup_out.n < devb && L->d_setup_out.alloc(devb + devb / 2, L->st)) return -1;
+        if (L->h_setup_out.reserve(outb)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+        float4* dbond = reinterpret_cast<float4*>(L->d_setup_out.p);
+        float4* dvw = dbond + n;
+        uint32_t* dev = reinterpret_cast<uint32_t*>(dvw + 2 * nbl);
+        float* dfr = reinterpret_cast<float*>(dev + n);
+        const uint8_t* d = L->stg_conn.d.p;
+        const int4* dbl = reinterpret_cast<const int4*>(d);
+        const int2* dent = reinterpret_cast<const int2*>(d + sizeof(int4) * nbl);
+        const uint32_t* dch = reinterpret_cast<const uint32_t*>(d + sizeof(int4) * nbl + sizeof(int2) * n);
+        BlastParams P;
+        std::memset(&P, 0, sizeof(P));
+        P.cx = x; P.cy = y; P.cz = z; P.r = r; P.power = power; P.br = S.blast_r; P.bp = S.blast_p;
+        P.token = ++L->ctoken; P.n = (uint32_t)n;
+        if (n) {
+            const unsigned g = sf_grid(n);
+            LAUNCH(k_blast_event, g, NT, L->D, P, dent, dev, dfr);
+            LAUNCH(k_blast_bonds, g, NT, L->D, P, dent, dev, (const float*)dfr, dbond);
+            LAUNCH(k_blast_remove, g, NT, L->D, P, dent, (const uint32_t*)dev);
+        }
+        if (nbl) LAUNCH(k_blast_bodies, (unsigned)nbl, 64, L->D, P, dbl, dch, dvw);
+        SF_HIP(hipGetLastError());
+        SF_HIP(hipMemcpyAsync(L->h_setup_out.p, L->d_setup_out.p, outb, hipMemcpyDeviceToHost, L->st));
+        SF_HIP(hipStreamSynchronize(L->st));
+        L->syncs++;
+        const float4* hbond = reinterpret_cast<const float4*>(L->h_setup_out.p);
+        const float4* hvw = hbond + n;
+        const uint32_t* hev = reinterpret_cast<const uint32_t*>(hvw + 2 * nbl);
+        // the mirror: bonds and removed bricks; then what each event disturbed, in order (blast_brick, remove_brick,
+        // damage_brick: the structure to check and to solve again, the body to rebuild)
+        for (size_t t = 0; t < n; t++) L->bond[ent[t].x] = hbond[t];
+        for (size_t t = 0; t < n; t++) {
+            const int i = ent[t].x;
+            const uint32_t e = hev[t] & 3u;
+            const bool broke = (hev[t] & BL_BROKE) != 0;
+            if (e == BL_NONE) continue;
+            if (e == BL_REMOVE) L->mat[i] = 0;
+            if (L->body[i] < 0) {
+                const int sl = L->sid[i];
+                if (sl < S.nst && (e == BL_REMOVE || broke)) {
+                    HStruct& T = L->sts[S.st0 + sl];
+                    T.conn = true; S.conn_any = true;
+                    if (e == BL_DAMAGE) T.broke = true;
+                }
+                st_static_change(L, S, i);
+            } else if (e == BL_REMOVE || broke) L->dirty[L->body[i]] = 1;
+        }
+        for (size_t k = 0; k < nbl; k++) {
+            const int b = bl[k].x;
+            L->v[b] = hvw[2 * k]; L->w[b] = hvw[2 * k + 1];
+            L->asleep[b] = 0; L->sleep[b] = 0.0f;
+        }
     }
     if (flush_delta(L)) return -1;
     return upload_scenes(L);

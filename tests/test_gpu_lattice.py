@@ -59,6 +59,48 @@ def _run(oracle, case, frames, check_every=1, fused=True, sor_lds=-1, conn_devic
     return G, s, O
 
 
+def test_charges_among_bodies_bit_exact(oracle):
+    """Charges set off after the tower has come apart: the blast reaches bricks of moving bodies (removed, their
+    bonds damaged, the bodies split at the next rebuild) and pushes every body near it, beside the standing
+    structure's bricks; the dropped bodies of the drop case take a charge of their own."""
+    print("case: 8x8x16 tower, 4 charges, 60 frames, then charges at its two largest bodies, 60 frames", flush=True)
+    G, s, O = _run(oracle, dict(kind="tower", w=8, h=16), 60, check_every=10)
+    ob = O.bodies()
+    live = np.flatnonzero(ob[:, 0] > 0)
+    assert live.size >= 2
+    big = live[np.argsort(-ob[live, 2], kind="stable")[:2]]
+    for k in big:
+        x, y, z = (float(v) for v in ob[k, 4:7])
+        G.explode(s, x, y, z, 2.0, 1.2)
+        O.explode(x, y, z, 2.0, 1.2)
+        _assert_same(G, s, O, f"after the charge at body {k}")
+    for f in range(60):
+        G.step(DT)
+        O.step(DT)
+        if f % 5 == 0 or f == 59:
+            _assert_same(G, s, O, f"frame {f} after the body charges")
+    print("case: 24 loose bodies dropped around a column, 30 frames, charges at the two largest, 30 frames", flush=True)
+    G, s, O = _run(oracle, dict(kind="drop", seed=3, nbody=24), 30, check_every=10)
+    ob, bk0 = O.bodies(), O.bricks()
+    live = np.flatnonzero(ob[:, 0] > 0)
+    for k in live[np.argsort(-ob[live, 2], kind="stable")[:2]]:
+        x, y, z = (float(v) for v in ob[k, 4:7])
+        G.explode(s, x, y, z, 1.5, 1.0)
+        O.explode(x, y, z, 1.5, 1.0)
+        _assert_same(G, s, O, f"after the charge at dropped body {k}")
+    bk1 = O.bricks()
+    in_body = bk0[:, 1] >= 0
+    assert ((bk0[:, 0] != 0) & (bk1[:, 0] == 0) & in_body).sum() > 0      # bricks of bodies blown away
+    assert ((bk0 != bk1).any(1) & in_body & (bk1[:, 0] != 0)).sum() > 0  # and bonds inside bodies damaged
+    most = 0
+    for f in range(30):
+        G.step(DT)
+        O.step(DT)
+        _assert_same(G, s, O, f"frame {f} after the charges at the dropped bodies")
+        most = max(most, int((O.bodies()[:, 0] > 0).sum()))
+    assert most > live.size                                                # the damaged bodies split
+
+
 def test_drop_bodies_bit_exact(oracle):
     print("case: 24 loose bodies dropped around a column, 90 frames", flush=True)
     G, s, O = _run(oracle, dict(kind="drop", seed=3, nbody=24), 90)
