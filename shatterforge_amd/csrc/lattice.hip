@@ -1068,6 +1068,7 @@ __global__ void __launch_bounds__(NT) k_impact_flag(LD D) {
         if (!hit && !crush) continue;
         uint32_t slot = atomicAdd(&D.ctr[C_NIMPACT], 1u);
         if (slot < D.pcap) D.ilist[slot] = c;
+        D.iflag[c] = 1;
     }
 }
 __device__ __forceinline__ void integrate_body(LD& D, uint32_t b, float h);
@@ -1388,6 +1389,231 @@ __global__ void __launch_bounds__(64) k_blast_bodies(LD D, BlastParams P, const 
     if (a > amax) w = mul(w, amax / a);
     D.v[b] = f4(v); D.w[b] = f4(w);
     vw_out[2 * (size_t)blockIdx.x] = f4(v); vw_out[2 * (size_t)blockIdx.x + 1] = f4(w);
+}
+
+// ---------------------------------------------------------------- impacts: the crush walk on the device
+// (impact_damage, src/lib.rs:1610-1642; crush_at, :1643-1702) One wavefront per scene takes the scene's impact
+// candidates in contact order (the flags k_impact_flag set, cleared as they are read) with the scene's budget of 2,500
+// crushed bricks. A crush walks from the hit brick: of the next 24 queued bricks the one nearest the contact point is
+// crushed next (its energy cost paid), its linked neighbours of the same body or structure queued; when the energy left
+// is below a brick's cost that brick is damaged instead and the walk ends. Each crushed brick is then pulverised (with
+// the material's share, from the scene's random stream) or severed. The queue (at most 1 + 6 x 600 bricks) and the
+// queued bricks' distances live in LDS; lane 0 makes the walk's decisions and writes, the lanes pick the nearest of
+// the window and test the six neighbours. What changed is logged for the host mirror: bricks, bodies to rebuild,
+// structures to activate, check and mark broken.
+constexpr int XQ = 4096, XCR = 600;
+enum : uint32_t { XL_BRICK = 0, XL_DIRTY = 1, XL_ACT = 2, XL_CONN = 3, XL_BROKE = 4 };
+struct CrushP { uint32_t* log; uint32_t* nlog; uint32_t* rng; uint32_t logcap, pad; };
+__device__ __forceinline__ void x_log(const CrushP& P, uint32_t kind, uint32_t idx) {
+    const uint32_t k = atomicAdd(P.nlog, 1u);
+    if (k < P.logcap) { P.log[2 * (size_t)k] = kind; P.log[2 * (size_t)k + 1] = idx; }
+}
+__device__ __forceinline__ void x_static(const LD& D, const CrushP& P, const SceneK& K, uint32_t i) {
+    const int sl = D.sid[i];
+    if (sl < K.nst) x_log(P, XL_ACT, (uint32_t)(K.st0 + sl));
+}
+// damage_brick (src/lib.rs:725-771) with no absolute part
+__device__ void x_damage(LD& D, const CrushP& P, const SceneK& K, uint32_t i, float frac) {
+    uint32_t broke = 0;
+    for (int d = 0; d < 6; d++) {
+        const uint32_t j = D.nb6[6 * (size_t)i + d];
+        if (j == NONE || D.mat[j] == 0) continue;
+        const uint32_t o = (d & 1) == 0 ? i : j;
+        const int a = d >> 1;
+        float4 b = D.bond[o];
+        float& bd = f4_ref(b, a);
+        if (bd > 0.0f) {
+            const float bm = f4_get(D.bmax[o], a);
+            bd -= bm * frac + 0.0f;
+            if (bd <= bm * 0.03f) { bd = 0.0f; broke++; }
+            D.bond[o] = b;
+            x_log(P, XL_BRICK, o);
+        }
+    }
+    float4 b = D.bond[i];
+    if (b.w > 0.0f) {
+        b.w -= D.bmax[i].w * frac * 0.5f + 0.0f * 0.5f;
+        if (b.w <= 0.0f) { b.w = 0.0f; broke++; }
+        D.bond[i] = b;
+        x_log(P, XL_BRICK, i);
+    }
+    if (D.body[i] < 0) {
+        x_static(D, P, K, i);
+        const int sl = D.sid[i];
+        if (broke > 0 && sl < K.nst) { x_log(P, XL_CONN, (uint32_t)(K.st0 + sl)); x_log(P, XL_BROKE, (uint32_t)(K.st0 + sl)); }
+    } else if (broke > 0) x_log(P, XL_DIRTY, (uint32_t)D.body[i]);
+}
+__device__ __forceinline__ float x_rnd(uint32_t& x) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; return (float)(x >> 8) / 16777216.0f; }
+__device__ __forceinline__ float x_dist2(const LD& D, uint32_t i, LV p) {
+    const LV q = D.body[i] < 0 ? lat_pos(D.bxyz[i].x, D.bxyz[i].y, D.bxyz[i].z) : lv4(D.wpos[i]);
+    const LV d = sub(q, p);
+    return dot(d, d);
+}
+struct XWalk { uint32_t* Q; float* DD; uint32_t* CR; uint32_t* shu; };
+// crush_at from brick `start` with energy e at point p, at most cap bricks; every lane calls it (uniform arguments)
+__device__ uint32_t x_crush(LD& D, const CrushP& P, const SceneK& K, const XWalk& W, uint32_t& rng, uint32_t start, float e, LV p,
+                            uint32_t cap) {
+    const Tables& T = tables();
+    const int lane = threadIdx.x;
+    const int m0 = D.mat[start];
+    if (m0 == 0 || m0 >= SLUG) return 0;
+    if (e < T.eb[m0] * 0.15f) return 0;
+    const int bodyv = D.body[start];
+    const int sidv = D.sid[start];
+    if (lane == 0) {
+        const uint32_t st0 = D.xtok[blockIdx.x] + 1u;
+        D.xtok[blockIdx.x] = st0;
+        W.shu[0] = st0;
+        W.Q[0] = start; W.DD[0] = x_dist2(D, start, p);
+        D.xstamp[start] = st0;
+    }
+    __syncthreads();
+    const uint32_t st = W.shu[0];
+    int qn = 1, head = 0, ncr = 0;
+    uint32_t crushed = 0;
+    const uint32_t maxn = cap < 600u ? cap : 600u;
+    while (head < qn && crushed < maxn) {
+        const int lim = min(head + 24, qn);
+        // the nearest of the window: the first of the smallest distances below 1e18 (none: the window's first)
+        unsigned long long key = ~0ull;
+        if (lane < lim - head) {
+            const float v = W.DD[head + lane];
+            if (v < 1e18f) key = ((unsigned long long)__float_as_uint(v) << 32) | (unsigned)lane;
+        }
+        for (int h = 32; h >= 1; h >>= 1) { const unsigned long long o = __shfl_xor(key, h, 64); key = o < key ? o : key; }
+        const int best = key == ~0ull ? head : head + (int)(key & 0xffffffffull);
+        __syncthreads();
+        if (lane == 0 && best != head) {
+            const uint32_t q = W.Q[head]; W.Q[head] = W.Q[best]; W.Q[best] = q;
+            const float f = W.DD[head]; W.DD[head] = W.DD[best]; W.DD[best] = f;
+        }
+        __syncthreads();
+        const uint32_t i = W.Q[head];
+        head++;
+        const int mm = D.mat[i];
+        if (mm == 0) continue;
+        const float cost = T.eb[mm];
+        if (e < cost) {
+            if (lane == 0) x_damage(D, P, K, i, 0.6f * e / cost);
+            __syncthreads();
+            break;
+        }
+        e -= cost;
+        if (lane == 0) W.CR[ncr] = i;
+        ncr++;
+        crushed++;
+        // the six neighbours in order: linked, not queued by this walk, of the same body and structure
+        int j = -1;
+        if (lane < 6) {
+            const int jj = linked(D, i, lane, nullptr);
+            if (jj >= 0 && D.xstamp[jj] != st && D.body[jj] == bodyv && D.sid[jj] == sidv) j = jj;
+        }
+        const unsigned long long am = __ballot(j >= 0);
+        if (j >= 0) {
+            const int pos = qn + __popcll(am & ((1ull << lane) - 1ull));
+            if (pos < XQ) { W.Q[pos] = (uint32_t)j; W.DD[pos] = x_dist2(D, (uint32_t)j, p); D.xstamp[j] = st; }
+        }
+        qn = min(qn + __popcll(am), XQ);
+        __syncthreads();
+    }
+    if (ncr == 0) return 0;
+    if (lane == 0) {
+        for (int t = 0; t < ncr; t++) {
+            const uint32_t i = W.CR[t];
+            const int mm = D.mat[i];
+            if (mm == 0) continue;
+            if (x_rnd(rng) < T.pulv[mm]) {
+                // remove_brick
+                D.mat[i] = 0;
+                x_log(P, XL_BRICK, i);
+                if (D.body[i] < 0) {
+                    const int sl = D.sid[i];
+                    if (sl < K.nst) x_log(P, XL_CONN, (uint32_t)(K.st0 + sl));
+                    x_static(D, P, K, i);
+                } else x_log(P, XL_DIRTY, (uint32_t)D.body[i]);
+            } else {
+                // sever_brick
+                D.bond[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                x_log(P, XL_BRICK, i);
+                for (int d = 1; d < 6; d += 2) {
+                    const uint32_t jn = D.nb6[6 * (size_t)i + d];
+                    if (jn == NONE) continue;
+                    float4 b = D.bond[jn];
+                    f4_ref(b, d >> 1) = 0.0f;
+                    D.bond[jn] = b;
+                    x_log(P, XL_BRICK, jn);
+                }
+                if (bodyv < 0) {
+                    const int sl = D.sid[i];
+                    if (sl < K.nst) x_log(P, XL_CONN, (uint32_t)(K.st0 + sl));
+                    x_static(D, P, K, i);
+                } else x_log(P, XL_DIRTY, (uint32_t)bodyv);
+            }
+        }
+    }
+    __syncthreads();
+    return crushed;
+}
+__global__ void __launch_bounds__(64) k_crush(LD D, CrushP P) {
+    __shared__ uint32_t Q[XQ];
+    __shared__ float DD[XQ];
+    __shared__ uint32_t CR[XCR];
+    __shared__ uint32_t shu[4];
+    const XWalk W{Q, DD, CR, shu};
+    const Tables& T = tables();
+    const int s = blockIdx.x, lane = threadIdx.x;
+    if (s >= (int)D.nscene) return;
+    const SceneK K = D.sc[s];
+    const uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    const uint32_t c0 = min(D.sc_c0[s], ncs), c1 = s + 1 < (int)D.nscene ? min(D.sc_c0[s + 1], ncs) : ncs;
+    uint32_t rng = P.rng[s];
+    uint32_t bud = 2500u;
+    for (uint32_t cb = c0; cb < c1; cb += 64) {
+        const uint32_t cl = cb + (uint32_t)lane;
+        const bool f = cl < c1 && D.iflag[cl] != 0;
+        unsigned long long m = __ballot(f);
+        if (f) D.iflag[cl] = 0;
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t c = cb + (uint32_t)l;
+            if (!D.con[c]) continue;
+            const uint32_t ia = D.cba[c], bb = D.cbb[c];
+            const int ma = D.mat[ia];
+            if (ma == 0 || ma >= SLUG) continue;
+            const float jn = D.cjn[c], vn0 = D.cvn0[c];
+            if (bb != NONE && D.cbo[c] < 0 && jn * vn0 > 4000.0f) {
+                if (lane == 0 && D.mat[bb] != 0 && D.body[bb] < 0) {
+                    const int sl = D.sid[bb];
+                    if (sl >= 0 && sl < K.nst) x_log(P, XL_ACT, (uint32_t)(K.st0 + sl));
+                }
+            }
+            if (D.mat[ia] == 0) continue;
+            if (vn0 < 1.0f) continue;
+            const float e = 0.5f * jn * vn0;
+            if (e < 150.0f) continue;
+            const int jb = bb != NONE ? (int)bb : -1;
+            const int mb = jb >= 0 ? D.mat[jb] : 0;
+            const float soft_a = 1.0f / T.eb[ma];
+            const float soft_b = (jb < 0 || mb == 0 || mb >= SLUG) ? 0.0f : 1.0f / T.eb[mb];
+            if (soft_a + soft_b <= 0.0f) continue;
+            const float ea = e * soft_a / (soft_a + soft_b);
+            const float4 p4 = D.cp[c];
+            const LV p = lv(p4.x, p4.y, p4.z);
+            if (ea > 0.0f && bud > 0) { const uint32_t k = x_crush(D, P, K, W, rng, ia, ea, p, bud); bud -= k < bud ? k : bud; }
+            if (soft_b > 0.0f && bud > 0) { const uint32_t k = x_crush(D, P, K, W, rng, (uint32_t)jb, e - ea, p, bud); bud -= k < bud ? k : bud; }
+        }
+    }
+    if (lane == 0) P.rng[s] = rng;
+}
+__global__ void __launch_bounds__(NT) k_crush_gather(LD D, const uint32_t* log, const uint32_t* nlog, uint32_t cap, float4* bond_out, uint32_t* mat_out) {
+    const uint32_t n = min(*nlog, cap);
+    GS(k, n) {
+        if (log[2 * (size_t)k] != XL_BRICK) continue;
+        const uint32_t i = log[2 * (size_t)k + 1];
+        bond_out[k] = D.bond[i];
+        mat_out[k] = D.mat[i];
+    }
 }
 
 // block max of two values; every thread gets the result
@@ -2261,9 +2487,9 @@ struct Lat {
     // off the host searches; per structure its place in this pass's list (-1: none), the list's bit offsets, results
     bool conn_device = true;
     uint32_t ctoken = 0;
-    Stage stg_conn, stg_setup;
-    DBuf<uint8_t> d_setup_out;
-    HBuf<uint8_t> h_setup_out;
+    Stage stg_conn, stg_setup, stg_crush;
+    DBuf<uint8_t> d_setup_out, d_crush;
+    HBuf<uint8_t> h_setup_out, h_crush;
     DBuf<uint32_t> d_creach_out;
     HBuf<uint32_t> h_creach_out;
     std::vector<int> reach_idx;
@@ -2353,7 +2579,7 @@ struct Lat {
     ~Lat() {
         for (void* p : owned) if (p) (void)hipFree(p);
         d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release();
-        d_brec.release(); d_idx.release(); d_wp.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release(); stg_conn.release(); stg_setup.release(); d_setup_out.release(); h_setup_out.release();
+        d_brec.release(); d_idx.release(); d_wp.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release(); stg_conn.release(); stg_setup.release(); d_setup_out.release(); h_setup_out.release(); stg_crush.release(); d_crush.release(); h_crush.release();
         h_brec.release(); h_idx.release(); h_wp.release();
         h_ctrbuf.release(); h_scstat.release(); h_kl.release(); h_rb.release(); h_sk[0].release(); h_sk[1].release();
         for (hipEvent_t e : {ev_sk[0], ev_sk[1], ev_stat}) if (e) (void)hipEventDestroy(e);
@@ -2381,6 +2607,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.wpos, B, &L->per_brick, 1); r |= L->dalloc(D.phi, B, &L->per_brick, 1); r |= L->dalloc(D.ext, B, &L->per_brick, 1); r |= L->dalloc(D.extn, B, &L->per_brick, 1);
     r |= L->dalloc(D.stress, B, &L->per_brick, 1); r |= L->dalloc(D.sup, B, &L->per_brick, 1);
     r |= L->dalloc(D.creach, B, &L->per_brick, 1); r |= L->dalloc(D.cq, B, &L->per_brick, 1);
+    r |= L->dalloc(D.xstamp, B, &L->per_brick, 1); r |= L->dalloc(D.xtok, S);
     r |= L->dalloc(D.bd_scene, N, &L->per_body, 1); r |= L->dalloc(D.alive, N, &L->per_body, 1); r |= L->dalloc(D.asleep, N, &L->per_body, 1); r |= L->dalloc(D.wake, N, &L->per_body, 1);
     r |= L->dalloc(D.pos, N, &L->per_body, 1); r |= L->dalloc(D.q, N, &L->per_body, 1); r |= L->dalloc(D.v, N, &L->per_body, 1); r |= L->dalloc(D.w, N, &L->per_body, 1);
     r |= L->dalloc(D.vpre, N, &L->per_body, 1); r |= L->dalloc(D.wpre, N, &L->per_body, 1); r |= L->dalloc(D.vlast, N, &L->per_body, 1); r |= L->dalloc(D.wlast, N, &L->per_body, 1);
@@ -2427,7 +2654,7 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     fr(D.cn); fr(D.cp); fr(D.cra); fr(D.crb); fr(D.ct1); fr(D.ct2);
     fr(D.cdepth); fr(D.cmn); fr(D.cmt1); fr(D.cmt2); fr(D.cjn); fr(D.cjt1); fr(D.cjt2); fr(D.cbias); fr(D.cvn0);
     fr(D.csmn); fr(D.csmt1); fr(D.csmt2); fr(D.collist); fr(D.ovent); fr(D.odv); fr(D.odw);
-    fr(D.rq_c); fr(D.rq_ab); fr(D.rq_0); fr(D.rq_1); fr(D.rq_2); fr(D.rq_3); fr(D.rq_4); fr(D.rq_5);
+    fr(D.rq_c); fr(D.rq_ab); fr(D.rq_0); fr(D.rq_1); fr(D.rq_2); fr(D.rq_3); fr(D.rq_4); fr(D.rq_5); fr(D.iflag);
     int r = 0;
     L->per_contact = 0;
     r |= L->dalloc(D.ca, cap, &L->per_contact, 1); r |= L->dalloc(D.cb, cap, &L->per_contact, 1); r |= L->dalloc(D.cbo, cap, &L->per_contact, 1); r |= L->dalloc(D.cscene, cap, &L->per_contact, 1);
@@ -2442,6 +2669,7 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     r |= L->dalloc(D.odv, 2 * (size_t)cap, &L->per_contact, 2); r |= L->dalloc(D.odw, 2 * (size_t)cap, &L->per_contact, 2);
     r |= L->dalloc(D.rq_c, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_ab, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_0, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_1, cap, &L->per_contact, 1);
     r |= L->dalloc(D.rq_2, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_3, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_4, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_5, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.iflag, cap, &L->per_contact, 1);
     D.ccap = cap;
     L->ccap = cap;
     return r ? -1 : 0;
@@ -3332,50 +3560,74 @@ int plough_phase(Lat* L, uint32_t n) {
     return rebuild_csr(L);
 }
 
-// (src/lib.rs:1610-1642) in contact order, a budget of 2500 crushed bricks per scene and substep
+// (src/lib.rs:1610-1642) in contact order, a budget of 2500 crushed bricks per scene and substep: the walks run on the
+// device (k_crush, a wavefront per scene, with the scenes' random streams), their log comes back in one copy (two when
+// it is long) and the host mirror follows it: bricks' materials and bonds, bodies to rebuild, structures to activate,
+// check and mark broken.
 int impact_phase(Lat* L, uint32_t n) {
     struct Acc { Lat* L; std::chrono::steady_clock::time_point t; ~Acc() { L->ph[2] += ms_since(t); } } acc{L, std::chrono::steady_clock::now()};
-    n = std::min(n, L->D.pcap);
-    std::vector<CRec> rec;
-    if (gather(L, L->D.ilist, n, rec)) return -1;
-    // a crush walks the bricks of the body it starts in: the world positions of the candidates' bodies, all at once
-    std::vector<int> bodies;
-    auto mark = [&](uint32_t i) {
-        if (i == NONE) return;
-        int b = L->body[i];
-        if (b >= 0 && !L->bmark[b]) { L->bmark[b] = 1; bodies.push_back(b); }
-    };
-    for (const CRec& c : rec) if (c.on && scene_of_contact(L, c.c) >= 0) { mark(c.ba); mark(c.bb); }
-    for (int b : bodies) L->bmark[b] = 0;
-    if (pull_body_wpos(L, bodies)) return -1;
-    std::vector<uint32_t> budget(L->sc.size(), 2500u);
-    const Tables& T = tables();
-    for (const CRec& c : rec) {
-        if (!c.on) continue;
-        int si = scene_of_contact(L, c.c);
-        if (si < 0) continue;
-        HScene& S = L->sc[si];
-        int ia = (int)c.ba;
-        int ma = L->mat[ia];
-        if (ma == 0 || ma >= SLUG) continue;
-        if (c.bb != NONE && c.bo < 0 && c.jn * c.vn0 > 4000.0f) {
-            int j = (int)c.bb;
-            if (L->mat[j] != 0 && L->body[j] < 0) st_activate(L, S, L->sid[j]);
-        }
-        if (L->mat[ia] == 0) continue;
-        if (c.vn0 < 1.0f) continue;
-        float e = 0.5f * c.jn * c.vn0;
-        if (e < 150.0f) continue;
-        int jb = c.bb != NONE ? (int)c.bb : -1;
-        int mb = jb >= 0 ? L->mat[jb] : 0;
-        float soft_a = 1.0f / T.eb[ma];
-        float soft_b = (jb < 0 || mb == 0 || mb >= SLUG) ? 0.0f : 1.0f / T.eb[mb];
-        if (soft_a + soft_b <= 0.0f) continue;
-        float ea = e * soft_a / (soft_a + soft_b);
-        LV p = lv(c.p[0], c.p[1], c.p[2]);
-        uint32_t& bud = budget[si];
-        if (ea > 0.0f && bud > 0) { uint32_t k = crush_at(L, S, ia, ea, p, bud); bud -= k < bud ? k : bud; }
-        if (soft_b > 0.0f && bud > 0) { uint32_t k = crush_at(L, S, jb, e - ea, p, bud); bud -= k < bud ? k : bud; }
+    if (!n) return 0;
+    const size_t ns = L->sc.size();
+    // log entries: per scene at most 2,500 crushed bricks a phase, each 6 entries (brick and three neighbours severed,
+    // a structure or body mark), and per candidate two damaged walk ends (7 bricks and 4 marks each) and a mark
+    const size_t cap = 15016 * ns + 23 * (size_t)std::min(n, L->D.pcap) + 1024;
+    const size_t o_rng = 0, o_n = (ns * 4 + 15) / 16 * 16, o_log = o_n + 16, o_bond = o_log + 8 * cap, o_mat = o_bond + 16 * cap,
+                 bytes = o_mat + 4 * cap;
+    if (L->d_crush.n < bytes && L->d_crush.alloc(bytes + bytes / 4, L->st)) return -1;
+    const size_t pref = 2048;   // log entries brought back with the first copy
+    const size_t hb = o_log + pref * (8 + 16 + 4);
+    if (L->h_crush.reserve(std::max(hb, o_log + 8))) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    uint8_t* d = L->d_crush.p;
+    uint32_t* hr = reinterpret_cast<uint32_t*>(L->stg_crush.begin(o_log));
+    if (!hr) return -1;
+    std::memset(hr, 0, o_log);
+    for (size_t si = 0; si < ns; si++) hr[si] = L->sc[si].rng;
+    if (L->stg_crush.end(o_log, L->st)) return -1;
+    SF_HIP(hipMemcpyAsync(d, L->stg_crush.d.p, o_log, hipMemcpyDeviceToDevice, L->st));
+    CrushP P;
+    P.log = reinterpret_cast<uint32_t*>(d + o_log); P.nlog = reinterpret_cast<uint32_t*>(d + o_n);
+    P.rng = reinterpret_cast<uint32_t*>(d + o_rng); P.logcap = (uint32_t)cap; P.pad = 0;
+    float4* dbond = reinterpret_cast<float4*>(d + o_bond);
+    uint32_t* dmat = reinterpret_cast<uint32_t*>(d + o_mat);
+    LAUNCH(k_crush, (unsigned)ns, 64, L->D, P);
+    LAUNCH(k_crush_gather, sf_grid(std::min<size_t>(cap, 1u << 20)), NT, L->D, (const uint32_t*)P.log, (const uint32_t*)P.nlog, (uint32_t)cap, dbond, dmat);
+    SF_HIP(hipGetLastError());
+    uint8_t* h = L->h_crush.p;
+    const size_t h_log = o_log, h_bond = h_log + 8 * pref, h_mat = h_bond + 16 * pref;
+    SF_HIP(hipMemcpyAsync(h, d, o_log, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipMemcpyAsync(h + h_log, d + o_log, 8 * pref, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipMemcpyAsync(h + h_bond, d + o_bond, 16 * pref, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipMemcpyAsync(h + h_mat, d + o_mat, 4 * pref, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    const uint32_t* hrng = reinterpret_cast<const uint32_t*>(h);
+    const uint32_t nlog = reinterpret_cast<const uint32_t*>(h + o_n)[0];
+    if (nlog > cap) { sf::set_error(SF_E_CAP, "crush log overflow"); return SF_E_CAP; }
+    for (size_t si = 0; si < ns; si++) L->sc[si].rng = hrng[si];
+    std::vector<uint32_t> xlog;
+    std::vector<float4> xbond;
+    std::vector<uint32_t> xmat;
+    const uint32_t* lg = reinterpret_cast<const uint32_t*>(h + h_log);
+    const float4* bd = reinterpret_cast<const float4*>(h + h_bond);
+    const uint32_t* mt_ = reinterpret_cast<const uint32_t*>(h + h_mat);
+    if (nlog > pref) {
+        xlog.resize(2 * (size_t)nlog); xbond.resize(nlog); xmat.resize(nlog);
+        SF_HIP(hipMemcpyAsync(xlog.data(), d + o_log, 8 * (size_t)nlog, hipMemcpyDeviceToHost, L->st));
+        SF_HIP(hipMemcpyAsync(xbond.data(), d + o_bond, 16 * (size_t)nlog, hipMemcpyDeviceToHost, L->st));
+        SF_HIP(hipMemcpyAsync(xmat.data(), d + o_mat, 4 * (size_t)nlog, hipMemcpyDeviceToHost, L->st));
+        SF_HIP(hipStreamSynchronize(L->st));
+        L->syncs++;
+        lg = xlog.data(); bd = xbond.data(); mt_ = xmat.data();
+    }
+    for (uint32_t k = 0; k < nlog; k++) {
+        const uint32_t kind = lg[2 * (size_t)k], x = lg[2 * (size_t)k + 1];
+        if (kind == XL_BRICK) { L->mat[x] = (uint8_t)mt_[k]; L->bond[x] = bd[k]; continue; }
+        if (kind == XL_DIRTY) { L->dirty[x] = 1; continue; }
+        HStruct& T = L->sts[x];
+        HScene& S = L->sc[T.scene];
+        if (kind == XL_ACT) st_activate(L, S, (int)x - S.st0);
+        else if (kind == XL_CONN) { T.conn = true; S.conn_any = true; }
+        else if (kind == XL_BROKE) T.broke = true;
     }
     if (flush_delta(L)) return -1;
     return rebuild_csr(L);

@@ -147,6 +147,11 @@ struct LD {
     // structure's part at its own brick range)
     uint32_t* creach;
     uint32_t* cq;
+    // impacts: per contact the candidate flag (cleared as the crush walk visits it); per brick the stamp of the last
+    // crush walk that queued it; per scene the walks' counter (the stamps' source)
+    uint8_t* iflag;
+    uint32_t* xstamp;
+    uint32_t* xtok;
     // scan scratch
     uint32_t* scan_tmp;
 };
