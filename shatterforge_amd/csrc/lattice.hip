@@ -2542,11 +2542,8 @@ struct Lat {
     // host phases' transfers: brick deltas and body records up, body records and brick positions down
     DBuf<BodyRec> d_brec;
     DBuf<uint32_t> d_idx;
-    DBuf<float4> d_wp;
     HBuf<BodyRec> h_brec;
     HBuf<uint32_t> h_idx;
-    HBuf<float4> h_wp;
-    std::vector<uint8_t> bmark;
     // pinned read-back: counters, scene first contacts, killed bodies, the structural pass's results, scene
     // statistics (read at the next wait), and the scene table (two buffers, each reused once its copy is done)
     HBuf<uint32_t> h_ctrbuf, h_scstat;
@@ -2579,8 +2576,8 @@ struct Lat {
     ~Lat() {
         for (void* p : owned) if (p) (void)hipFree(p);
         d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release();
-        d_brec.release(); d_idx.release(); d_wp.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release(); stg_conn.release(); stg_setup.release(); d_setup_out.release(); h_setup_out.release(); stg_crush.release(); d_crush.release(); h_crush.release();
-        h_brec.release(); h_idx.release(); h_wp.release();
+        d_brec.release(); d_idx.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release(); stg_conn.release(); stg_setup.release(); d_setup_out.release(); h_setup_out.release(); stg_crush.release(); d_crush.release(); h_crush.release();
+        h_brec.release(); h_idx.release();
         h_ctrbuf.release(); h_scstat.release(); h_kl.release(); h_rb.release(); h_sk[0].release(); h_sk[1].release();
         for (hipEvent_t e : {ev_sk[0], ev_sk[1], ev_stat}) if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
@@ -3111,54 +3108,6 @@ void conn_finish(Lat* L, HScene& S) {
     }
     S.conn_new.clear();
 }
-uint32_t crush_at(Lat* L, HScene& S, int start, float e, LV p, uint32_t cap) {
-    ScrRef W = scr(L);
-    const Tables& T = tables();
-    int m0 = L->mat[start];
-    if (m0 == 0 || m0 >= SLUG) return 0;
-    if (e < T.eb[m0] * 0.15f) return 0;
-    int bodyv = L->body[start];
-    int sidv = L->sid[start];
-    uint32_t st = ++S.stamp_id;
-    W.q[0] = (uint32_t)start;
-    int qn = 1; L->stamp[start] = st;
-    int head = 0;
-    uint32_t crushed = 0;
-    uint32_t maxn = cap < 600 ? cap : 600;
-    int ncr = 0;
-    while (head < qn && crushed < maxn) {
-        int best = head; float bd = 1e18f;
-        int lim = head + 24 < qn ? head + 24 : qn;
-        for (int t = head; t < lim; t++) { int i = (int)W.q[t]; LV d = sub(hbpos(L, i), p); float dd = dot(d, d); if (dd < bd) { bd = dd; best = t; } }
-        std::swap(W.q[head], W.q[best]);
-        int i = (int)W.q[head]; head++;
-        int mm = L->mat[i];
-        if (mm == 0) continue;
-        float cost = T.eb[mm];
-        if (e < cost) { damage_brick(L, S, i, 0.6f * e / cost, 0.0f); break; }
-        e -= cost;
-        W.lst2[ncr++] = (uint32_t)i; crushed++;
-        for (int d = 0; d < 6; d++) {
-            int j = hlinked(L, i, d);
-            if (j < 0 || L->stamp[j] == st || L->body[j] != bodyv || L->sid[j] != sidv) continue;
-            if (qn >= S.bcap) break;
-            L->stamp[j] = st; W.q[qn++] = (uint32_t)j;
-        }
-    }
-    if (ncr == 0) return 0;
-    for (int t = 0; t < ncr; t++) {
-        int i = (int)W.lst2[t];
-        int mm = L->mat[i];
-        if (mm == 0) continue;
-        if (rnd(S) < T.pulv[mm]) remove_brick(L, S, i);
-        else {
-            sever_brick(L, i);
-            if (bodyv < 0) { int s = L->sid[i]; if (s < S.nst) { L->sts[S.st0 + s].conn = true; S.conn_any = true; } st_static_change(L, S, i); }
-            else L->dirty[bodyv] = 1;
-        }
-    }
-    return crushed;
-}
 }  // namespace
 
 namespace {
@@ -3391,7 +3340,6 @@ __global__ void k_body_pack(LD D, const uint32_t* idx, uint32_t n, BodyRec* out)
         out[t] = r;
     }
 }
-__global__ void k_gather_wpos(LD D, const uint32_t* idx, uint32_t n, float4* out) { GS(t, n) out[t] = D.wpos[idx[t]]; }
 __global__ void k_body_unpack(LD D, const BodyRec* in, uint32_t n) {
     GS(t, n) {
         BodyRec r = in[t];
@@ -3434,27 +3382,6 @@ int pull_bodies(Lat* L, const std::vector<int>& scenes) {
     return 0;
 }
 int pull_bodies(Lat* L, int si) { return pull_bodies(L, std::vector<int>{si}); }
-// world positions of the bricks of the given bodies into the host mirror: one gather, one copy, one wait
-int pull_body_wpos(Lat* L, const std::vector<int>& bodies) {
-    std::vector<uint32_t>& list = L->lst;
-    size_t n = 0;
-    for (int b : bodies)
-        for (uint32_t i = L->first[b]; i != NONE; i = L->next[i]) list[n++] = i;
-    if (!n) return 0;
-    if (L->h_idx.reserve(n) || L->h_wp.reserve(n)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
-    if ((L->d_idx.n < n && L->d_idx.alloc(std::max<size_t>(n, 4096), L->st)) || (L->d_wp.n < n && L->d_wp.alloc(std::max<size_t>(n, 4096), L->st))) {
-        sf::set_error(SF_E_HIP, "hipMalloc failed");
-        return -1;
-    }
-    memcpy(L->h_idx.p, list.data(), sizeof(uint32_t) * n);
-    SF_HIP(hipMemcpyAsync(L->d_idx.p, L->h_idx.p, sizeof(uint32_t) * n, hipMemcpyHostToDevice, L->st));
-    LAUNCH(k_gather_wpos, sf_grid(n), NT, L->D, (const uint32_t*)L->d_idx.p, (uint32_t)n, L->d_wp.p);
-    SF_HIP(hipMemcpyAsync(L->h_wp.p, L->d_wp.p, sizeof(float4) * n, hipMemcpyDeviceToHost, L->st));
-    SF_HIP(hipStreamSynchronize(L->st));
-    L->syncs++;
-    for (size_t t = 0; t < n; t++) L->wpos[L->h_idx.p[t]] = L->h_wp.p[t];
-    return 0;
-}
 // upload what the host changed: bricks (material, body, bonds, local and world position, load field) and bodies
 int flush_delta(Lat* L) {
     size_t n = L->tlist.size(), nb = L->btlist.size();
@@ -4543,7 +4470,7 @@ sf_lat* sf_lat_create(int brick_cap, int body_cap, int max_scenes, int colours, 
     L->rot.assign(N, z); L->iil = L->rot; L->iiw = L->rot;
     L->m.assign(N, 0.0f); L->im = L->m; L->sleep = L->m; L->rad = L->m; L->cool = L->m; L->lt = L->m; L->resid = L->m; L->maxr = L->m; L->nmax = L->m;
     L->touched.assign(B, 0);
-    L->btouched.assign(N, 0); L->bmark.assign(N, 0);
+    L->btouched.assign(N, 0);
     L->lst.assign(B + 1, 0); L->lst2.assign(B + 1, 0); L->comp.assign(B + 1, 0); L->cnt.assign(B + 2, 0);
     L->dyn.assign(B, 0); L->dyn_body.assign(B, 0);
     return h;
