@@ -3497,7 +3497,7 @@ int impact_phase(Lat* L, uint32_t n) {
     const size_t ns = L->sc.size();
     // log entries: per scene at most 2,500 crushed bricks a phase, each 6 entries (brick and three neighbours severed,
     // a structure or body mark), and per candidate two damaged walk ends (7 bricks and 4 marks each) and a mark
-    const size_t cap = 15016 * ns + 23 * (size_t)std::min(n, L->D.pcap) + 1024;
+    const size_t cap = 15016 * ns + 23 * (size_t)n + 1024;
     const size_t o_rng = 0, o_n = (ns * 4 + 15) / 16 * 16, o_log = o_n + 16, o_bond = o_log + 8 * cap, o_mat = o_bond + 16 * cap,
                  bytes = o_mat + 4 * cap;
     if (L->d_crush.n < bytes && L->d_crush.alloc(bytes + bytes / 4, L->st)) return -1;
