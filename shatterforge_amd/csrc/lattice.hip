@@ -362,8 +362,8 @@ __global__ void __launch_bounds__(NT) k_plough_flag(LD D) {
             int mj = D.mat[j];
             if (mj >= SLUG || mom < T.capv[mj] * 0.35f) continue;
         }
-        uint32_t slot = atomicAdd(&D.ctr[C_NPLOUGH], 1u);
-        if (slot < D.pcap) D.plist[slot] = c;
+        atomicAdd(&D.ctr[C_NPLOUGH], 1u);
+        D.pflag[c] = 1;
     }
 }
 
@@ -1066,8 +1066,7 @@ __global__ void __launch_bounds__(NT) k_impact_flag(LD D) {
         bool hit = D.cbb[c] != NONE && D.cbo[c] < 0 && jn * vn0 > 4000.0f;
         bool crush = !(vn0 < 1.0f) && !(0.5f * jn * vn0 < 150.0f);
         if (!hit && !crush) continue;
-        uint32_t slot = atomicAdd(&D.ctr[C_NIMPACT], 1u);
-        if (slot < D.pcap) D.ilist[slot] = c;
+        atomicAdd(&D.ctr[C_NIMPACT], 1u);
         D.iflag[c] = 1;
     }
 }
@@ -1402,7 +1401,7 @@ __global__ void __launch_bounds__(64) k_blast_bodies(LD D, BlastParams P, const 
 // the window and test the six neighbours. What changed is logged for the host mirror: bricks, bodies to rebuild,
 // structures to activate, check and mark broken.
 constexpr int XQ = 4096, XCR = 600;
-enum : uint32_t { XL_BRICK = 0, XL_DIRTY = 1, XL_ACT = 2, XL_CONN = 3, XL_BROKE = 4 };
+enum : uint32_t { XL_BRICK = 0, XL_DIRTY = 1, XL_ACT = 2, XL_CONN = 3, XL_BROKE = 4, XL_VEL = 5 };
 struct CrushP { uint32_t* log; uint32_t* nlog; uint32_t* rng; uint32_t logcap, pad; };
 __device__ __forceinline__ void x_log(const CrushP& P, uint32_t kind, uint32_t idx) {
     const uint32_t k = atomicAdd(P.nlog, 1u);
@@ -2378,29 +2377,6 @@ struct HStruct {
     int4 rb{0, 0, 0, 0};
 };
 
-// candidate contact records gathered on the device (plough / impact)
-struct CRec { uint32_t c; int32_t a, b, bo; uint32_t ba, bb; int32_t on; float p[3], n[3], jn, vn0; };
-
-__global__ void k_gather(LD D, const uint32_t* list, uint32_t n, CRec* out) {
-    GS(t, n) {
-        uint32_t c = list[t];
-        CRec r;
-        r.c = c; r.a = D.ca[c]; r.b = D.cb[c]; r.bo = D.cbo[c]; r.ba = D.cba[c]; r.bb = D.cbb[c]; r.on = D.con[c];
-        float4 p = D.cp[c], nn = D.cn[c];
-        r.p[0] = p.x; r.p[1] = p.y; r.p[2] = p.z; r.n[0] = nn.x; r.n[1] = nn.y; r.n[2] = nn.z;
-        r.jn = D.cjn[c]; r.vn0 = D.cvn0[c];
-        out[t] = r;
-    }
-}
-// contacts switched off by the host: contact k itself, and (j >= 0) every later contact of the scene against brick j
-__global__ void k_disable(LD D, const int2* kj, uint32_t n, uint32_t c_end) {
-    GS(c, c_end) {
-        for (uint32_t t = 0; t < n; t++) {
-            int2 e = kj[t];
-            if (c == (uint32_t)e.x || (e.y >= 0 && c > (uint32_t)e.x && D.cbb[c] == (uint32_t)e.y && D.cscene[c] == D.cscene[e.x])) D.con[c] = 0;
-        }
-    }
-}
 
 // a device buffer zero-filled on the world's stream. Growing it keeps the old block until release: freeing memory
 // waits for the whole device, including other streams' work (a load-path solve running beside the frame).
@@ -2535,8 +2511,6 @@ struct Lat {
     std::vector<void*> owned;
     DBuf<SceneK> d_sc;
     DBuf<int32_t> d_grid;
-    DBuf<CRec> d_rec;
-    DBuf<int2> d_dis;
     DBuf<uint32_t> d_scstat;
     DBuf<int32_t> d_slist;
     // host phases' transfers: brick deltas and body records up, body records and brick positions down
@@ -2575,7 +2549,7 @@ struct Lat {
 
     ~Lat() {
         for (void* p : owned) if (p) (void)hipFree(p);
-        d_sc.release(); d_grid.release(); d_rec.release(); d_dis.release(); d_scstat.release(); d_slist.release();
+        d_sc.release(); d_grid.release(); d_scstat.release(); d_slist.release();
         d_brec.release(); d_idx.release(); d_rbk.release(); stg_ctl.release(); stg_csr.release(); stg_delta.release(); stg_brec.release(); stg_conn.release(); stg_setup.release(); d_setup_out.release(); h_setup_out.release(); stg_crush.release(); d_crush.release(); h_crush.release();
         h_brec.release(); h_idx.release();
         h_ctrbuf.release(); h_scstat.release(); h_kl.release(); h_rb.release(); h_sk[0].release(); h_sk[1].release();
@@ -2633,11 +2607,11 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.ctr, CTR_WORDS + S + 1);
     D.sc_c0 = D.ctr + CTR_WORDS;
     D.pcap = (uint32_t)std::max<size_t>(4096, B);
-    r |= L->dalloc(D.plist, D.pcap, &L->per_brick, 1); r |= L->dalloc(D.ilist, D.pcap, &L->per_brick, 1); r |= L->dalloc(D.klist, N, &L->per_body, 1);
+    r |= L->dalloc(D.klist, N, &L->per_body, 1);
     D.evcap = (uint32_t)std::max<size_t>(4096, B); D.bwcap = (uint32_t)std::max<size_t>(4096, 4 * B);
     r |= L->dalloc(D.evkey, D.evcap, &L->per_brick, 1); r |= L->dalloc(D.ev, D.evcap, &L->per_brick, 1); r |= L->dalloc(D.bwi, D.bwcap, &L->per_brick, 4); r |= L->dalloc(D.bwv, D.bwcap, &L->per_brick, 4);
     r |= L->dalloc(D.scan_tmp, sf::scan_scratch_words((uint32_t)std::max<size_t>(std::max(2 * B + 4096, N), 1u << 16)));
-    r |= L->d_sc.alloc(S, L->st); r |= L->d_scstat.alloc(3 * S, L->st); r |= L->d_slist.alloc(S, L->st); r |= L->d_rec.alloc(D.pcap, L->st); r |= L->d_dis.alloc(D.pcap, L->st);
+    r |= L->d_sc.alloc(S, L->st); r |= L->d_scstat.alloc(3 * S, L->st); r |= L->d_slist.alloc(S, L->st);
     D.sc = L->d_sc.p;
     D.K = (uint32_t)L->K; D.R = (uint32_t)L->R;
     return r ? -1 : 0;
@@ -2651,7 +2625,7 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     fr(D.cn); fr(D.cp); fr(D.cra); fr(D.crb); fr(D.ct1); fr(D.ct2);
     fr(D.cdepth); fr(D.cmn); fr(D.cmt1); fr(D.cmt2); fr(D.cjn); fr(D.cjt1); fr(D.cjt2); fr(D.cbias); fr(D.cvn0);
     fr(D.csmn); fr(D.csmt1); fr(D.csmt2); fr(D.collist); fr(D.ovent); fr(D.odv); fr(D.odw);
-    fr(D.rq_c); fr(D.rq_ab); fr(D.rq_0); fr(D.rq_1); fr(D.rq_2); fr(D.rq_3); fr(D.rq_4); fr(D.rq_5); fr(D.iflag);
+    fr(D.rq_c); fr(D.rq_ab); fr(D.rq_0); fr(D.rq_1); fr(D.rq_2); fr(D.rq_3); fr(D.rq_4); fr(D.rq_5); fr(D.iflag); fr(D.pflag);
     int r = 0;
     L->per_contact = 0;
     r |= L->dalloc(D.ca, cap, &L->per_contact, 1); r |= L->dalloc(D.cb, cap, &L->per_contact, 1); r |= L->dalloc(D.cbo, cap, &L->per_contact, 1); r |= L->dalloc(D.cscene, cap, &L->per_contact, 1);
@@ -2666,7 +2640,7 @@ int lat_contacts_alloc(Lat* L, uint32_t cap) {
     r |= L->dalloc(D.odv, 2 * (size_t)cap, &L->per_contact, 2); r |= L->dalloc(D.odw, 2 * (size_t)cap, &L->per_contact, 2);
     r |= L->dalloc(D.rq_c, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_ab, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_0, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_1, cap, &L->per_contact, 1);
     r |= L->dalloc(D.rq_2, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_3, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_4, cap, &L->per_contact, 1); r |= L->dalloc(D.rq_5, cap, &L->per_contact, 1);
-    r |= L->dalloc(D.iflag, cap, &L->per_contact, 1);
+    r |= L->dalloc(D.iflag, cap, &L->per_contact, 1); r |= L->dalloc(D.pflag, cap, &L->per_contact, 1);
     D.ccap = cap;
     L->ccap = cap;
     return r ? -1 : 0;
@@ -2833,6 +2807,52 @@ const LM IDM = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
 // the centre with each brick's own S^2/6 term, and forms the body: inverse inertia, world inertia, radius, the
 // velocity of the new centre. Then every lane writes its share of the bricks: position in the body frame, world
 // position, body. `in` holds each brick's position in the parent's frame (xyz) and its mass (w).
+// the set-up's sums, one brick at a time in the job's order: mass and mass-weighted position; then, about the
+// centre, the inertia with each brick's own S^2/6 term and the largest squared distance
+__device__ __forceinline__ void setup_acc_mass(float4 a, float& mass, LV& cl) { mass += a.w; cl = madd(cl, lv(a.x, a.y, a.z), a.w); }
+__device__ __forceinline__ void setup_acc_inertia(float4 a, LV cl, float* it, float& radv) {
+    const float cube = S * S / 6.0f;
+    float mm = a.w;
+    LV r = sub(lv(a.x, a.y, a.z), cl);
+    float rr = dot(r, r);
+    radv = maxf(radv, rr);
+    it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
+    it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
+}
+// the body from its sums: inverse inertia, world inertia, radius, the new centre's position and velocity; written to
+// the body's slot and to `o`; returns the body's position
+__device__ LV setup_write(LD& D, const SetupJob& J, const LM& prot, float mass, LV cl, float* it, float radv, SetupOut& o) {
+    it[3] = it[1]; it[6] = it[2]; it[7] = it[5];
+    LM itm;
+    for (int k = 0; k < 9; k++) itm.m[k] = it[k];
+    const LM iilm = minv(itm);
+    const LV off = mv(prot, cl);
+    const float4 pos = f4(add(lv4(J.ppos), off));
+    const float4 v = f4(add(lv4(J.pv), cross(lv4(J.pw), off)));
+    const LM iiw = mmul(mmul(prot, iilm), mt(prot));
+    const float rad = sqrtf(radv) + H * 1.8f;
+    const int b = J.b;
+    const bool keep = (J.flags & 2) != 0;
+    D.alive[b] = 1; D.count[b] = (uint32_t)(J.e - J.s);
+    D.pos[b] = pos; D.q[b] = J.pq; D.v[b] = v; D.w[b] = f4(lv4(J.pw)); D.vlast[b] = v; D.wlast[b] = f4(lv4(J.pw));
+    store_m(D.rot + 3 * (size_t)b, prot); store_m(D.iil + 3 * (size_t)b, iilm); store_m(D.iiw + 3 * (size_t)b, iiw);
+    D.m[b] = mass; D.im[b] = 1.0f / mass; D.rad[b] = rad; D.nmax[b] = 1.0f;
+    D.lt[b] = keep ? J.keep_lt : 0.0f; D.resid[b] = 1.0f; D.maxr[b] = 0.0f;
+    D.sleep[b] = 0.0f; D.asleep[b] = 0; D.wake[b] = 0; D.cool[b] = keep ? J.keep_cool : 0.15f;
+    o.pos = pos; o.v = v;
+    store_m(o.iil, iilm); store_m(o.iiw, iiw);
+    o.m = mass; o.im = 1.0f / mass; o.rad = rad; o.pad = 0.0f;
+    return lv4(pos);
+}
+// one brick of a set-up body: its position in the body frame and in the world, its body (and, leaving a structure,
+// its load field cleared)
+__device__ __forceinline__ void setup_brick(LD& D, const SetupJob& J, const LM& prot, LV cl, LV bpos, uint32_t i, float4 a, float4* bout) {
+    const LV r = sub(lv(a.x, a.y, a.z), cl);
+    const float4 loc = f4(r), wp = f4(add(bpos, mv(prot, r)));
+    D.local[i] = loc; D.wpos[i] = wp; D.body[i] = J.b;
+    if (J.flags & 1) { D.phi[i] = 0.0f; D.ext[i] = 0.0f; D.extn[i] = 0.0f; }
+    if (bout) { bout[0] = loc; bout[1] = wp; }
+}
 __global__ void __launch_bounds__(64) k_setup_bodies(LD D, const SetupJob* jobs, const uint32_t* idx, const float4* in,
                                                      SetupOut* out, float4* bout) {
     constexpr int CH = 256;            // bricks staged in LDS at a time: the lanes load a chunk, lane 0 sums it
@@ -2847,68 +2867,177 @@ __global__ void __launch_bounds__(64) k_setup_bodies(LD D, const SetupJob* jobs,
         for (int k = (int)threadIdx.x; k < m; k += 64) buf[k] = in[c0 + k];
         __syncthreads();
         if (threadIdx.x == 0)
-            for (int k = 0; k < m; k++) { float4 a = buf[k]; mass += a.w; cl = madd(cl, lv(a.x, a.y, a.z), a.w); }
+            for (int k = 0; k < m; k++) setup_acc_mass(buf[k], mass, cl);
         __syncthreads();
     }
     cl = mul(cl, 1.0f / mass);
     float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     float radv = 0.0f;
-    const float cube = S * S / 6.0f;
     for (int c0 = J.s; c0 < J.e; c0 += CH) {
         const int m = min(CH, J.e - c0);
         for (int k = (int)threadIdx.x; k < m; k += 64) buf[k] = in[c0 + k];
         __syncthreads();
         if (threadIdx.x == 0)
-            for (int k = 0; k < m; k++) {
-                float4 a = buf[k];
-                float mm = a.w;
-                LV r = sub(lv(a.x, a.y, a.z), cl);
-                float rr = dot(r, r);
-                radv = maxf(radv, rr);
-                it[0] += mm * (rr - r.x * r.x + cube); it[4] += mm * (rr - r.y * r.y + cube); it[8] += mm * (rr - r.z * r.z + cube);
-                it[1] -= mm * r.x * r.y; it[2] -= mm * r.x * r.z; it[5] -= mm * r.y * r.z;
-            }
+            for (int k = 0; k < m; k++) setup_acc_inertia(buf[k], cl, it, radv);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        it[3] = it[1]; it[6] = it[2]; it[7] = it[5];
-        LM itm;
-        for (int k = 0; k < 9; k++) itm.m[k] = it[k];
-        const LM iilm = minv(itm);
-        const LV off = mv(prot, cl);
-        const float4 pos = f4(add(lv4(J.ppos), off));
-        const float4 v = f4(add(lv4(J.pv), cross(lv4(J.pw), off)));
-        const LM iiw = mmul(mmul(prot, iilm), mt(prot));
-        const float rad = sqrtf(radv) + H * 1.8f;
-        const int b = J.b;
-        const bool keep = (J.flags & 2) != 0;
-        D.alive[b] = 1; D.count[b] = (uint32_t)(J.e - J.s);
-        D.pos[b] = pos; D.q[b] = J.pq; D.v[b] = v; D.w[b] = f4(lv4(J.pw)); D.vlast[b] = v; D.wlast[b] = f4(lv4(J.pw));
-        store_m(D.rot + 3 * (size_t)b, prot); store_m(D.iil + 3 * (size_t)b, iilm); store_m(D.iiw + 3 * (size_t)b, iiw);
-        D.m[b] = mass; D.im[b] = 1.0f / mass; D.rad[b] = rad; D.nmax[b] = 1.0f;
-        D.lt[b] = keep ? J.keep_lt : 0.0f; D.resid[b] = 1.0f; D.maxr[b] = 0.0f;
-        D.sleep[b] = 0.0f; D.asleep[b] = 0; D.wake[b] = 0; D.cool[b] = keep ? J.keep_cool : 0.15f;
         SetupOut o;
-        o.pos = pos; o.v = v;
-        store_m(o.iil, iilm); store_m(o.iiw, iiw);
-        o.m = mass; o.im = 1.0f / mass; o.rad = rad; o.pad = 0.0f;
+        const LV bp = setup_write(D, J, prot, mass, cl, it, radv, o);
         out[blockIdx.x] = o;
-        sh[0] = cl.x; sh[1] = cl.y; sh[2] = cl.z; sh[3] = pos.x; sh[4] = pos.y; sh[5] = pos.z;
+        sh[0] = cl.x; sh[1] = cl.y; sh[2] = cl.z; sh[3] = bp.x; sh[4] = bp.y; sh[5] = bp.z;
     }
     __syncthreads();
     const LV cen = lv(sh[0], sh[1], sh[2]), bpos = lv(sh[3], sh[4], sh[5]);
-    for (int t = J.s + (int)threadIdx.x; t < J.e; t += 64) {
-        float4 a = in[t];
-        const LV r = sub(lv(a.x, a.y, a.z), cen);
-        const uint32_t i = idx[t];
-        const float4 loc = f4(r), wp = f4(add(bpos, mv(prot, r)));
-        D.local[i] = loc; D.wpos[i] = wp; D.body[i] = J.b;
-        if (J.flags & 1) { D.phi[i] = 0.0f; D.ext[i] = 0.0f; D.extn[i] = 0.0f; }
-        bout[2 * (size_t)t] = loc; bout[2 * (size_t)t + 1] = wp;
-    }
+    for (int t = J.s + (int)threadIdx.x; t < J.e; t += 64) setup_brick(D, J, prot, cen, bpos, idx[t], in[t], bout + 2 * (size_t)t);
 }
 float4 m_row(const LM& m, int k) { return make_float4(m.m[3 * k], m.m[3 * k + 1], m.m[3 * k + 2], 0.0f); }
 void row_m(LM& m, int k, float4 r) { m.m[3 * k] = r.x; m.m[3 * k + 1] = r.y; m.m[3 * k + 2] = r.z; }
+// ---------------------------------------------------------------- the plough on the device (plough, src/lib.rs:1493-1550)
+// One wavefront per scene takes its plough candidates in contact order (the flags k_plough_flag set, cleared as read),
+// with the scene's random stream and body-slot cursor. A fast, heavy body striking a static brick: a large body
+// damages the brick; otherwise the brick is severed and leaves as a body of its own (a new slot from the scene's
+// cursor, set up as setup_body does, thrown forward with a random push and spin) and the body slows by the momentum it
+// hands over. Contacts made void (the brick no longer static, no slot left, or the brick ploughed away: every later
+// contact of the scene against it) are switched off. Lane 0 decides and writes; the lanes switch contacts off. What
+// changed is logged for the host mirror, and each new body's record comes back whole.
+struct NewRec { int32_t b, j, pad0, pad1; SetupOut o; float4 loc, wp, w; };
+__device__ LV x_rndv(uint32_t& x) {
+    for (int k = 0; k < (1 << 20); k++) {
+        const float a = x_rnd(x) * 2.0f - 1.0f, b = x_rnd(x) * 2.0f - 1.0f, c = x_rnd(x) * 2.0f - 1.0f;
+        const LV v = lv(a, b, c);
+        const float l = dot(v, v);
+        if (l > 0.01f && l <= 1.0f) return mul(v, 1.0f / sqrtf(l));
+    }
+    return lv(0.0f, 1.0f, 0.0f);   // (a stream stuck at zero: never drawn from a seeded scene)
+}
+__global__ void __launch_bounds__(64) k_plough(LD D, CrushP P, uint32_t* cur, uint32_t* hwv, NewRec* nrec, uint32_t* nnew, uint32_t ncap) {
+    const int s = blockIdx.x, lane = threadIdx.x;
+    if (s >= (int)D.nscene) return;
+    const Tables& T = tables();
+    const SceneK K = D.sc[s];
+    const uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
+    const uint32_t c0 = min(D.sc_c0[s], ncs), c1 = s + 1 < (int)D.nscene ? min(D.sc_c0[s + 1], ncs) : ncs;
+    uint32_t rng = P.rng[s], cursor = cur[s], hw = hwv[s];
+    const LM idm = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    for (uint32_t cb = c0; cb < c1; cb += 64) {
+        const uint32_t cl = cb + (uint32_t)lane;
+        const bool f = cl < c1 && D.pflag[cl] != 0;
+        unsigned long long m = __ballot(f);
+        if (f) D.pflag[cl] = 0;
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t c = cb + (uint32_t)l;
+            int off = -2;   // -2: the contact stays; -1: it alone is switched off; j >= 0: it and later ones against j
+            if (lane == 0) {
+                const uint32_t j = D.cbb[c];
+                if (D.mat[j] == 0 || D.body[j] >= 0) off = -1;
+                else {
+                    const int a = D.ca[c];
+                    const float4 p4 = D.cp[c], n4 = D.cn[c];
+                    const LV p = lv(p4.x, p4.y, p4.z), nn = lv(n4.x, n4.y, n4.z);
+                    const LV ra = sub(p, lv4(D.pos[a]));
+                    const float vn = -dot(add(lv4(D.v[a]), cross(lv4(D.w[a]), ra)), nn);
+                    const int mj = D.mat[j];
+                    float mom = 0.0f;
+                    bool go = !(vn < 2.0f);
+                    if (go) {
+                        const float meff = D.m[a] * 3.0f / maxf((float)D.nc[a], 3.0f);
+                        mom = minf(meff, D.m[a]) * vn;
+                        go = !(mj >= SLUG || mom < T.capv[mj] * 0.35f);
+                    }
+                    if (go && D.count[a] > 40u) {
+                        const float fr = clampf(mom / (T.capv[mj] * 1.5f), 0.15f, 0.9f);
+                        x_damage(D, P, K, j, fr);
+                        go = false;
+                    }
+                    if (go) {
+                        // sever_brick; the brick's structure to check and to solve again
+                        D.bond[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        x_log(P, XL_BRICK, j);
+                        for (int d = 1; d < 6; d += 2) {
+                            const uint32_t jn = D.nb6[6 * (size_t)j + d];
+                            if (jn == NONE) continue;
+                            float4 b4 = D.bond[jn];
+                            f4_ref(b4, d >> 1) = 0.0f;
+                            D.bond[jn] = b4;
+                            x_log(P, XL_BRICK, jn);
+                        }
+                        const int sl = D.sid[j];
+                        if (sl < K.nst) x_log(P, XL_CONN, (uint32_t)(K.st0 + sl));
+                        x_static(D, P, K, j);
+                        // alloc_body: the first free slot from the scene's cursor
+                        int b = -1;
+                        const int dcap = K.nbd;
+                        for (int k = 0; k < dcap; k++) {
+                            const int lb = (int)((cursor + (uint32_t)k) % (uint32_t)dcap);
+                            if (!D.alive[K.bd0 + lb]) {
+                                b = K.bd0 + lb;
+                                cursor = (uint32_t)((lb + 1) % dcap);
+                                if ((uint32_t)(lb + 1) > hw) hw = (uint32_t)(lb + 1);
+                                break;
+                            }
+                        }
+                        if (b < 0) off = -1;
+                        else {
+                            const LV rv = x_rndv(rng);
+                            const LV vv = madd(mul(lv4(D.v[a]), 0.75f), rv, 2.0f);
+                            const LV ww = mul(x_rndv(rng), 6.0f);
+                            SetupJob J;
+                            J.b = b; J.s = 0; J.e = 1; J.flags = 0;   // (the brick keeps its load field, as the reference's plough leaves it)
+                            J.ppos = make_float4(0.0f, 0.0f, 0.0f, 0.0f); J.pq = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                            J.pv = f4(vv); J.pw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            J.prot[0] = make_float4(1.0f, 0.0f, 0.0f, 0.0f); J.prot[1] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+                            J.prot[2] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+                            J.keep_cool = 0.0f; J.keep_lt = 0.0f; J.pad0 = 0.0f; J.pad1 = 0.0f;
+                            const LV loc = lat_pos(D.bxyz[j].x, D.bxyz[j].y, D.bxyz[j].z);
+                            const float4 a4 = make_float4(loc.x, loc.y, loc.z, T.mass[D.mat[j]]);
+                            float mass = 0.0f;
+                            LV cen = lv(0.0f, 0.0f, 0.0f);
+                            setup_acc_mass(a4, mass, cen);
+                            cen = mul(cen, 1.0f / mass);
+                            float it[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                            float radv = 0.0f;
+                            setup_acc_inertia(a4, cen, it, radv);
+                            NewRec r;
+                            const LV bp = setup_write(D, J, idm, mass, cen, it, radv, r.o);
+                            float4 bo[2];
+                            setup_brick(D, J, idm, cen, bp, j, a4, bo);
+                            D.w[b] = f4(ww);
+                            r.b = b; r.j = (int)j; r.pad0 = r.pad1 = 0; r.loc = bo[0]; r.wp = bo[1]; r.w = f4(ww);
+                            const uint32_t k = atomicAdd(nnew, 1u);
+                            if (k < ncap) nrec[k] = r;
+                            // the striking body hands over the brick's momentum
+                            const float mm = D.m[a];
+                            const float fa = mm / (mm + T.mass[mj] * 1.5f);
+                            D.v[a] = f4(mul(lv4(D.v[a]), fa));
+                            x_log(P, XL_VEL, (uint32_t)a);
+                            off = (int)j;
+                        }
+                    }
+                }
+            }
+            off = __shfl(off, 0, 64);
+            if (off != -2) {
+                if (lane == 0) D.con[c] = 0;
+                if (off >= 0)
+                    for (uint32_t cc = c + 1u + (uint32_t)lane; cc < c1; cc += 64)
+                        if (D.cbb[cc] == (uint32_t)off) D.con[cc] = 0;
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) { P.rng[s] = rng; cur[s] = cursor; hwv[s] = hw; }
+}
+__global__ void __launch_bounds__(NT) k_plough_gather(LD D, const uint32_t* log, const uint32_t* nlog, uint32_t cap, float4* v_out, uint32_t* mat_out) {
+    const uint32_t n = min(*nlog, cap);
+    GS(k, n) {
+        const uint32_t kind = log[2 * (size_t)k], i = log[2 * (size_t)k + 1];
+        if (kind == XL_BRICK) { v_out[k] = D.bond[i]; mat_out[k] = D.mat[i]; }
+        else if (kind == XL_VEL) v_out[k] = D.v[i];
+    }
+}
 // A new body in slot b from the bricks list[0, n) (in the reference's order), queued for the device's set-up
 // (k_setup_bodies; setup_body, src/lib.rs:844-929). The host keeps the slot's list structure and the constants the
 // set-up writes; the arithmetic comes back with run_setups. lattice: the bricks leave a structure, their positions in
@@ -3304,21 +3433,6 @@ int drain_kills(Lat* L) {
     return fill_u32(L, L->D.ctr + C_NKILL, 0, 1);
 }
 
-int scene_of_contact(Lat* L, uint32_t c) {
-    for (size_t si = 0; si < L->sc.size(); si++) { const HScene& S = L->sc[si]; if (c >= S.c0 && c < S.c0 + S.ncs) return (int)si; }
-    return -1;
-}
-
-int gather(Lat* L, const uint32_t* list, uint32_t n, std::vector<CRec>& out) {
-    out.resize(n);
-    if (!n) return 0;
-    LAUNCH(k_gather, sf_grid(n), NT, L->D, list, n, L->d_rec.p);
-    SF_HIP(hipMemcpyAsync(out.data(), L->d_rec.p, sizeof(CRec) * n, hipMemcpyDeviceToHost, L->st));
-    SF_HIP(hipStreamSynchronize(L->st));
-    L->syncs++;
-    std::sort(out.begin(), out.end(), [](const CRec& a, const CRec& b) { return a.c < b.c; });
-    return 0;
-}
 
 __global__ void k_brick_delta(LD D, const BrickDelta* d, uint32_t n) {
     GS(t, n) {
@@ -3419,71 +3533,95 @@ int flush_delta(Lat* L) {
     return 0;
 }
 
-// (src/lib.rs:1493-1550) applied in contact order on the host mirror
 double ms_since(std::chrono::steady_clock::time_point t) { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
+// (plough, src/lib.rs:1493-1550) on the device (k_plough, a wavefront per scene with the scenes' random streams and
+// slot cursors); its log and the new bodies' records come back and the host mirror follows them
 int plough_phase(Lat* L, uint32_t n) {
     auto t0 = std::chrono::steady_clock::now();
     struct Acc { Lat* L; std::chrono::steady_clock::time_point t; ~Acc() { L->ph[1] += ms_since(t); } } acc{L, t0};
-    n = std::min(n, L->D.pcap);
-    std::vector<CRec> rec;
-    if (gather(L, L->D.plist, n, rec)) return -1;
-    std::vector<int2> dis;
-    std::vector<int> scenes;
-    for (const CRec& c : rec) {
-        int si = scene_of_contact(L, c.c);
-        if (si >= 0 && (scenes.empty() || scenes.back() != si)) scenes.push_back(si);  // records are in contact order
+    if (!n) return 0;
+    const size_t ns = L->sc.size();
+    // log entries per candidate: a damaged brick (7 bricks and 4 marks) or a severed one (4 bricks, 2 marks) and the
+    // striking body's velocity; new bodies: at most one per candidate
+    const size_t cap = 12 * (size_t)n + 1024, ncap = (size_t)n + 16;
+    const size_t o_cur = (ns * 4 + 15) / 16 * 16, o_hw = o_cur + o_cur, o_cnt = o_hw + o_cur, o_log = o_cnt + 16,
+                 o_v = o_log + 8 * cap, o_mat = o_v + 16 * cap, o_new = (o_mat + 4 * cap + 15) / 16 * 16,
+                 bytes = o_new + sizeof(NewRec) * ncap;
+    if (L->d_crush.n < bytes && L->d_crush.alloc(bytes + bytes / 4, L->st)) return -1;
+    uint8_t* d = L->d_crush.p;
+    uint32_t* hs = reinterpret_cast<uint32_t*>(L->stg_crush.begin(o_log));
+    if (!hs) return -1;
+    std::memset(hs, 0, o_log);
+    for (size_t si = 0; si < ns; si++) {
+        const HScene& S = L->sc[si];
+        hs[si] = S.rng;
+        hs[o_cur / 4 + si] = (uint32_t)S.cursor;
+        hs[o_hw / 4 + si] = (uint32_t)S.hw;
     }
-    if (pull_bodies(L, scenes)) return -1;
-    const Tables& T = tables();
-    std::vector<std::pair<int, LV>> spun;
-    for (const CRec& c : rec) {
-        int si = scene_of_contact(L, c.c);
-        if (si < 0) continue;
+    if (L->stg_crush.end(o_log, L->st)) return -1;
+    SF_HIP(hipMemcpyAsync(d, L->stg_crush.d.p, o_log, hipMemcpyDeviceToDevice, L->st));
+    CrushP P;
+    P.log = reinterpret_cast<uint32_t*>(d + o_log); P.nlog = reinterpret_cast<uint32_t*>(d + o_cnt);
+    P.rng = reinterpret_cast<uint32_t*>(d); P.logcap = (uint32_t)cap; P.pad = 0;
+    uint32_t* dnnew = reinterpret_cast<uint32_t*>(d + o_cnt) + 1;
+    NewRec* dnew = reinterpret_cast<NewRec*>(d + o_new);
+    LAUNCH(k_plough, (unsigned)ns, 64, L->D, P, reinterpret_cast<uint32_t*>(d + o_cur), reinterpret_cast<uint32_t*>(d + o_hw), dnew, dnnew, (uint32_t)ncap);
+    LAUNCH(k_plough_gather, sf_grid(std::min<size_t>(cap, 1u << 20)), NT, L->D, (const uint32_t*)P.log, (const uint32_t*)P.nlog, (uint32_t)cap,
+           reinterpret_cast<float4*>(d + o_v), reinterpret_cast<uint32_t*>(d + o_mat));
+    SF_HIP(hipGetLastError());
+    if (L->h_crush.reserve(o_log)) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
+    uint8_t* h = L->h_crush.p;
+    SF_HIP(hipMemcpyAsync(h, d, o_log, hipMemcpyDeviceToHost, L->st));
+    SF_HIP(hipStreamSynchronize(L->st));
+    L->syncs++;
+    const uint32_t* hh = reinterpret_cast<const uint32_t*>(h);
+    const uint32_t nlog = hh[o_cnt / 4], nnew = hh[o_cnt / 4 + 1];
+    if (nlog > cap || nnew > ncap) { sf::set_error(SF_E_CAP, "plough log overflow"); return SF_E_CAP; }
+    for (size_t si = 0; si < ns; si++) {
         HScene& S = L->sc[si];
-        int j = (int)c.bb;
-        if (L->mat[j] == 0 || L->body[j] >= 0) { dis.push_back(make_int2((int)c.c, -1)); continue; }
-        int a = c.a;
-        LV p = lv(c.p[0], c.p[1], c.p[2]), nn = lv(c.n[0], c.n[1], c.n[2]);
-        LV ra = sub(p, hv(L->pos[a]));
-        float vn = -dot(add(hv(L->v[a]), cross(hv(L->w[a]), ra)), nn);
-        if (vn < 2.0f) continue;
-        float meff = L->m[a] * 3.0f / maxf((float)L->nc[a], 3.0f);
-        float mom = minf(meff, L->m[a]) * vn;
-        int mj = L->mat[j];
-        if (mj >= SLUG || mom < T.capv[mj] * 0.35f) continue;
-        if (L->count[a] > 40) {
-            float f = clampf(mom / (T.capv[mj] * 1.5f), 0.15f, 0.9f);
-            damage_brick(L, S, j, f, 0.0f);
-            continue;
+        S.rng = hh[si];
+        S.cursor = (int)hh[o_cur / 4 + si];
+        const int hw = (int)hh[o_hw / 4 + si];
+        if (hw > S.hw) { S.hw = hw; L->ub_dirty = true; }
+    }
+    if (nlog || nnew) {
+        std::vector<uint32_t> lg(2 * (size_t)nlog), mt_(nlog);
+        std::vector<float4> vv(nlog);
+        std::vector<NewRec> nr(nnew);
+        if (nlog) {
+            SF_HIP(hipMemcpyAsync(lg.data(), d + o_log, 8 * (size_t)nlog, hipMemcpyDeviceToHost, L->st));
+            SF_HIP(hipMemcpyAsync(vv.data(), d + o_v, 16 * (size_t)nlog, hipMemcpyDeviceToHost, L->st));
+            SF_HIP(hipMemcpyAsync(mt_.data(), d + o_mat, 4 * (size_t)nlog, hipMemcpyDeviceToHost, L->st));
         }
-        sever_brick(L, j);
-        int sl = L->sid[j];
-        if (sl < S.nst) { L->sts[S.st0 + sl].conn = true; S.conn_any = true; }
-        st_static_change(L, S, j);
-        int b = alloc_body(L, S);
-        if (b < 0) { dis.push_back(make_int2((int)c.c, -1)); continue; }
-        LV rv = rndv(S);
-        LV vv = madd(mul(hv(L->v[a]), 0.75f), rv, 2.0f);
-        LV ww = mul(rndv(S), 6.0f);
-        LQ idq; idq.x = 0.0f; idq.y = 0.0f; idq.z = 0.0f; idq.w = 1.0f;
-        const uint32_t one = (uint32_t)j;
-        setup_queue(L, S, b, &one, 1, true, lv(0.0f, 0.0f, 0.0f), IDM, idq, vv, lv(0.0f, 0.0f, 0.0f), false);
-        spun.push_back(std::make_pair(b, ww));   // its spin once it is set up
-        float mm = L->m[a];
-        float f = mm / (mm + T.mass[mj] * 1.5f);
-        L->v[a] = hf(mul(hv(L->v[a]), f));
-        L->btouch(a);
-        dis.push_back(make_int2((int)c.c, j));
-    }
-    if (run_setups(L, scenes)) return -1;
-    for (const auto& bw : spun) { L->w[bw.first] = hf(bw.second); L->btouch(bw.first); }
-    if (flush_delta(L)) return -1;
-    if (!dis.empty()) {
-        if (dis.size() > L->D.pcap) { sf::set_error(SF_E_CAP, "too many plough events"); return SF_E_CAP; }
-        SF_HIP(hipMemcpyAsync(L->d_dis.p, dis.data(), sizeof(int2) * dis.size(), hipMemcpyHostToDevice, L->st));
-        LAUNCH(k_disable, sf_grid(L->h_ctr[C_NCS]), NT, L->D, L->d_dis.p, (uint32_t)dis.size(), L->h_ctr[C_NCS]);
+        if (nnew) SF_HIP(hipMemcpyAsync(nr.data(), dnew, sizeof(NewRec) * nnew, hipMemcpyDeviceToHost, L->st));
         SF_HIP(hipStreamSynchronize(L->st));
+        L->syncs++;
+        for (uint32_t k = 0; k < nlog; k++) {
+            const uint32_t kind = lg[2 * (size_t)k], x = lg[2 * (size_t)k + 1];
+            if (kind == XL_BRICK) { L->mat[x] = (uint8_t)mt_[k]; L->bond[x] = vv[k]; continue; }
+            if (kind == XL_VEL) { L->v[x] = vv[k]; continue; }
+            if (kind == XL_DIRTY) { L->dirty[x] = 1; continue; }
+            HStruct& T = L->sts[x];
+            HScene& S = L->sc[T.scene];
+            if (kind == XL_ACT) st_activate(L, S, (int)x - S.st0);
+            else if (kind == XL_CONN) { T.conn = true; S.conn_any = true; }
+            else if (kind == XL_BROKE) T.broke = true;
+        }
+        // the new bodies in the mirror: the slot's list and constants as setup_body leaves them, its numbers from the device
+        for (const NewRec& r : nr) {
+            const int b = r.b, j = r.j;
+            L->body[j] = b; L->next[j] = NONE; L->local[j] = r.loc; L->wpos[j] = r.wp;
+            L->alive[b] = 1; L->first[b] = (uint32_t)j; L->count[b] = 1;
+            L->qq[b] = make_float4(0.0f, 0.0f, 0.0f, 1.0f); L->rot[b] = IDM;
+            L->pos[b] = r.o.pos; L->v[b] = r.o.v; L->vlast[b] = r.o.v; L->w[b] = r.w; L->wlast[b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (int k = 0; k < 3; k++) { row_m(L->iil[b], k, r.o.iil[k]); row_m(L->iiw[b], k, r.o.iiw[k]); }
+            L->m[b] = r.o.m; L->im[b] = r.o.im; L->rad[b] = r.o.rad;
+            L->lt[b] = 0.0f; L->cool[b] = 0.15f; L->resid[b] = 1.0f; L->maxr[b] = 0.0f; L->nmax[b] = 1.0f;
+            L->sleep[b] = 0.0f; L->asleep[b] = 0; L->dirty[b] = 0; L->wake[b] = 0;
+            L->lists_dirty = true;
+        }
     }
+    if (flush_delta(L)) return -1;
     return rebuild_csr(L);
 }
 

@@ -135,8 +135,6 @@ struct LD {
     uint8_t* bd_dirty;
     // events
     uint32_t* ctr;
-    uint32_t* plist;           // plough candidates
-    uint32_t* ilist;           // impact candidates
     int32_t* klist;            // killed bodies
     unsigned long long* evkey; // crush events: order key
     int4* ev;                  // brick, axis, target, kind
@@ -150,6 +148,7 @@ struct LD {
     // impacts: per contact the candidate flag (cleared as the crush walk visits it); per brick the stamp of the last
     // crush walk that queued it; per scene the walks' counter (the stamps' source)
     uint8_t* iflag;
+    uint8_t* pflag;            // plough candidates, likewise
     uint32_t* xstamp;
     uint32_t* xtok;
     // scan scratch

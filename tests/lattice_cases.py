@@ -100,6 +100,19 @@ def masonry_shaft():
     return st, [(-2.25, 0.6, -2.5, 4.0, 1.4), (2.25, 0.6, 2.5, 4.0, 1.4)]
 
 
+def ram_scene(speed=14.0):
+    """a masonry wall 10 wide, 6 high and 2 deep, struck broadside by two concrete rams: a small one (2 x 2 x 2, which
+    ploughs bricks out of the wall) and a large one (4 x 4 x 3, which only damages the bricks it strikes)"""
+    wall = [(x, y, z) for y in range(6) for z in range(2) for x in range(10)]
+    mat = np.full(len(wall), 2, np.uint8)
+    xyz = np.array(wall, np.int32) + np.array([C - 5, 0, C - 1], np.int32)
+    small = np.array([(x, y, z) for x in range(2) for y in range(2) for z in range(2)], np.int32) + np.array([C - 3, 2, C - 6], np.int32)
+    large = np.array([(x, y, z) for x in range(4) for y in range(4) for z in range(3)], np.int32) + np.array([C, 1, C + 4], np.int32)
+    bodies = [(np.full(len(small), 1, np.uint8), small, (0.0, 0.0, speed), (0.0, 0.0, 0.0)),
+              (np.full(len(large), 1, np.uint8), large, (0.0, 0.0, -speed), (0.0, 0.0, 0.0))]
+    return (mat, xyz, np.zeros(len(mat), np.uint8)), bodies
+
+
 def column_bases(mat, xyz, rng, k):
     g = xyz[(xyz[:, 1] == 0) & (mat == 1)]
     cols = sorted(set((int(a), int(b)) for a, b in g[:, [0, 2]]))
@@ -123,6 +136,11 @@ def build(world, scene_or_none, case):
     if kind == "drop":
         col, bodies = drop_scene(case.get("seed", 3), case.get("nbody", 24))
         add_struct(*col)
+        for mat, xyz, v, w in bodies:
+            add_body(mat, xyz, v, w)
+    elif kind == "ram":
+        wall, bodies = ram_scene(case.get("speed", 14.0))
+        add_struct(*wall)
         for mat, xyz, v, w in bodies:
             add_body(mat, xyz, v, w)
     elif kind == "tower":
@@ -162,6 +180,9 @@ def brick_count(case):
     if kind == "drop":
         col, bodies = drop_scene(case.get("seed", 3), case.get("nbody", 24))
         return len(col[0]) + sum(len(b[0]) for b in bodies)
+    if kind == "ram":
+        wall, bodies = ram_scene(case.get("speed", 14.0))
+        return len(wall[0]) + sum(len(b[0]) for b in bodies)
     if kind == "tower":
         return len(small_tower(case.get("w", 8), case.get("h", 16))[0])
     if kind == "office":
