@@ -3547,7 +3547,7 @@ int plough_phase(Lat* L, uint32_t n) {
     const size_t o_cur = (ns * 4 + 15) / 16 * 16, o_hw = o_cur + o_cur, o_cnt = o_hw + o_cur, o_log = o_cnt + 16,
                  o_v = o_log + 8 * cap, o_mat = o_v + 16 * cap, o_new = (o_mat + 4 * cap + 15) / 16 * 16,
                  bytes = o_new + sizeof(NewRec) * ncap;
-    if (L->d_crush.n < bytes && L->d_crush.alloc(bytes + bytes / 4, L->st)) return -1;
+    if (L->d_crush.n < bytes && L->d_crush.alloc(2 * bytes, L->st)) return -1;
     uint8_t* d = L->d_crush.p;
     uint32_t* hs = reinterpret_cast<uint32_t*>(L->stg_crush.begin(o_log));
     if (!hs) return -1;
@@ -3635,10 +3635,10 @@ int impact_phase(Lat* L, uint32_t n) {
     const size_t ns = L->sc.size();
     // log entries: per scene at most 2,500 crushed bricks a phase, each 6 entries (brick and three neighbours severed,
     // a structure or body mark), and per candidate two damaged walk ends (7 bricks and 4 marks each) and a mark
-    const size_t cap = 15016 * ns + 23 * (size_t)n + 1024;
+    const size_t cap = 15016 * ns + 23 * (size_t)std::max<uint32_t>(n, 4096) + 1024;
     const size_t o_rng = 0, o_n = (ns * 4 + 15) / 16 * 16, o_log = o_n + 16, o_bond = o_log + 8 * cap, o_mat = o_bond + 16 * cap,
                  bytes = o_mat + 4 * cap;
-    if (L->d_crush.n < bytes && L->d_crush.alloc(bytes + bytes / 4, L->st)) return -1;
+    if (L->d_crush.n < bytes && L->d_crush.alloc(2 * bytes, L->st)) return -1;   // (doubled: growing allocates and waits)
     const size_t pref = 2048;   // log entries brought back with the first copy
     const size_t hb = o_log + pref * (8 + 16 + 4);
     if (L->h_crush.reserve(std::max(hb, o_log + 8))) { sf::set_error(SF_E_HIP, "hipHostMalloc failed"); return -1; }
