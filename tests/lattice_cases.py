@@ -113,6 +113,17 @@ def ram_scene(speed=14.0):
     return (mat, xyz, np.zeros(len(mat), np.uint8)), bodies
 
 
+def heavy_ram_scene():
+    """a masonry wall 16 wide, 10 high and 3 deep struck broadside at 40 m/s by a 6 x 6 x 4 steel ram: long crush walks
+    (over 300 bricks crushed in one walk, its queue near 300 bricks) and a wall torn into pieces"""
+    wall = [(x, y, z) for y in range(10) for z in range(3) for x in range(16)]
+    mat = np.full(len(wall), 2, np.uint8)
+    xyz = np.array(wall, np.int32) + np.array([C - 8, 0, C - 1], np.int32)
+    ram = np.array([(x, y, z) for x in range(6) for y in range(6) for z in range(4)], np.int32) + np.array([C - 3, 1, C - 6], np.int32)
+    bodies = [(np.full(len(ram), 3, np.uint8), ram, (0.0, 0.0, 40.0), (0.0, 0.0, 0.0))]
+    return (mat, xyz, np.zeros(len(mat), np.uint8)), bodies
+
+
 def column_bases(mat, xyz, rng, k):
     g = xyz[(xyz[:, 1] == 0) & (mat == 1)]
     cols = sorted(set((int(a), int(b)) for a, b in g[:, [0, 2]]))
@@ -136,6 +147,11 @@ def build(world, scene_or_none, case):
     if kind == "drop":
         col, bodies = drop_scene(case.get("seed", 3), case.get("nbody", 24))
         add_struct(*col)
+        for mat, xyz, v, w in bodies:
+            add_body(mat, xyz, v, w)
+    elif kind == "heavy_ram":
+        wall, bodies = heavy_ram_scene()
+        add_struct(*wall)
         for mat, xyz, v, w in bodies:
             add_body(mat, xyz, v, w)
     elif kind == "ram":
@@ -182,6 +198,9 @@ def brick_count(case):
         return len(col[0]) + sum(len(b[0]) for b in bodies)
     if kind == "ram":
         wall, bodies = ram_scene(case.get("speed", 14.0))
+        return len(wall[0]) + sum(len(b[0]) for b in bodies)
+    if kind == "heavy_ram":
+        wall, bodies = heavy_ram_scene()
         return len(wall[0]) + sum(len(b[0]) for b in bodies)
     if kind == "tower":
         return len(small_tower(case.get("w", 8), case.get("h", 16))[0])

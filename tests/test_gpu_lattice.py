@@ -111,6 +111,15 @@ def test_rams_plough_a_wall_bit_exact(oracle):
     assert (O.bodies()[:, 0] > 0).sum() > 2
 
 
+def test_heavy_ram_long_crush_walks_bit_exact(oracle):
+    """A steel ram at 40 m/s through a masonry wall: crush walks of over 300 bricks (the nearest of each 24-brick window,
+    queues near 300 long), pulverised and severed bricks from the scene's random stream, the wall in pieces. Every
+    frame equal to the oracle bit for bit."""
+    print("case: 16x10x3 masonry wall, a 6x6x4 steel ram at 40 m/s, 60 frames", flush=True)
+    G, s, O = _run(oracle, dict(kind="heavy_ram"), 60)
+    assert (O.bodies()[:, 0] > 0).sum() > 50
+
+
 def test_drop_bodies_bit_exact(oracle):
     print("case: 24 loose bodies dropped around a column, 90 frames", flush=True)
     G, s, O = _run(oracle, dict(kind="drop", seed=3, nbody=24), 90)
