@@ -119,6 +119,12 @@ pub mod ffi {
         pub fn sf_lat_set_field(l: *mut SfLat, scene: c_int, structure: c_int, phi: *const f32) -> c_int;
         pub fn sf_lat_failures(l: *mut SfLat, scene: c_int, out4: *mut u32) -> c_int;
         pub fn sf_lat_bytes_resident(l: *mut SfLat, scene: c_int) -> i64;
+        pub fn sf_lat_phase_ms(l: *mut SfLat, out8: *mut f64) -> c_int;
+        pub fn sf_lat_subphase_ms(l: *mut SfLat, out16: *mut f64) -> c_int;
+        pub fn sf_lat_set_fused(l: *mut SfLat, max_contacts: c_int) -> c_int;
+        pub fn sf_lat_set_host_threads(l: *mut SfLat, n: c_int) -> c_int;
+        pub fn sf_lat_set_conn_device(l: *mut SfLat, on: c_int) -> c_int;
+        pub fn sf_lat_set_sor_lds(l: *mut SfLat, max_bricks: c_int) -> c_int;
 
         // load query (src/lib.rs sor_struct / stress_brick)
         pub fn sf_load_create() -> *mut SfLoad;
