@@ -177,14 +177,15 @@ def test_overlapping_structures_are_rejected():
 
 
 MIX = [dict(kind="drop", seed=3, nbody=24), dict(kind="tower", w=8, h=16, nch=4, r=2.2, p=1.4),
-       dict(kind="building", n=2500, seed=4, nch=4, r=3.0, p=1.4), dict(kind="drop", seed=11, nbody=30)]
+       dict(kind="building", n=2500, seed=4, nch=4, r=3.0, p=1.4), dict(kind="drop", seed=11, nbody=30),
+       dict(kind="ram", speed=20.0), dict(kind="heavy_ram")]
 
 
 def test_batched_scenes_same_alone_and_in_mix(oracle):
-    # four scenes of uneven size stepped together in one world: each equals the oracle run of that scene alone and
+    # six scenes of uneven size stepped together in one world: each equals the oracle run of that scene alone and
     # the GPU run of that scene alone, bit for bit (the batch solves one workgroup per scene, the lone worlds by
-    # colour launches)
-    print("case: 4 scenes batched, 100 frames, vs each alone (GPU and oracle)", flush=True)
+    # colour launches; two of the scenes plough and crush at once, each with its own random stream and slot cursor)
+    print("case: 6 scenes batched, 100 frames, vs each alone (GPU and oracle)", flush=True)
     caps = [lc.brick_count(c) + 8 for c in MIX]
     G = Lattice(sum(caps), 4096 * len(MIX), scenes=len(MIX))
     sc = [G.add_scene(caps[k], 4096, structs=4, seed=0x9E3779B9) for k in range(len(MIX))]
