@@ -377,6 +377,62 @@ def bench_particles(n=100_000, bodies=5000, steps=90, seed=2):
     return res
 
 
+def bench_chunks(n=20000, steps=30, warm=30, cpu=True, seed=5):
+    """convex chunks: n bodies, each one cell of the chunk library (random pattern and cell, random orientation),
+    dropped in a loose grid onto the ground and stepped with the GPU world's solver and contact reduction; the GPU's
+    ms per step over steps warm+1 .. warm+steps, and the oracle (coloured order, one thread) on the same scene and
+    the same steps (at most 20 s of them)"""
+    import math
+    import shatterforge_amd as sf
+    from shatterforge_amd import hulls, scenes, world
+    from shatterforge_amd.world import params
+    rng = np.random.default_rng(seed)
+    side = int(math.ceil((n / 6) ** 0.5))
+    items = []
+    for k in range(n):
+        pat = int(rng.integers(0, hulls.pattern_count()))
+        ids = hulls.pattern(pat)
+        h = ids[int(rng.integers(0, len(ids)))]
+        hh = hulls.cell(h)["box_h"] * 0.25 + 0.04
+        gx, gz, gy = k % side, (k // side) % side, k // (side * side)
+        p = ((gx - side / 2) * 0.9, 0.5 + gy * 0.9, (gz - side / 2) * 0.9)
+        ax = rng.random(3) - 0.5
+        ax = ax / np.linalg.norm(ax)
+        ang = rng.random() * 6.0
+        q = np.array([ax[0] * math.sin(ang / 2), ax[1] * math.sin(ang / 2), ax[2] * math.sin(ang / 2), math.cos(ang / 2)], np.float32)
+        items.append(scenes.body([scenes.shape(p, hh, q=q, mat=1 + k % 5, hull=h)], v=(0, -1, 0)))
+    b, sb, s = scenes.bodies(items)
+    prm = params(**world.GPU_WORLD)
+    w = sf.World.from_scene(b, sb, s, prm)
+    for _ in range(warm):
+        w.step(1.0 / 60.0)
+    w.stats()   # (a read-back: the warm-up has finished)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.step(1.0 / 60.0)
+    st = w.stats()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    w.close()
+    res = {"bodies": n, "steps": steps, "warmup": warm, "ms_per_step": ms, "chunk_steps_per_s": n / (ms * 1e-3),
+           "manifolds": st["manifolds"], "reduced": st["reduced"], "awake": st["awake"]}
+    log(f"chunks: {n} bodies, steps {warm + 1}-{warm + steps}: {ms:.3f} ms/step, {st['manifolds']} manifolds")
+    if cpu:
+        from oracle import pyoracle
+        o = pyoracle.BoxOracle(b, sb, s, prm, pyoracle.ORDER_COLOURED, hulls=hulls.tables())
+        for _ in range(warm):
+            o.step(1.0 / 60.0)
+        k, t1 = 0, time.perf_counter()
+        while k < steps and time.perf_counter() - t1 < 20.0:
+            o.step(1.0 / 60.0)
+            k += 1
+        cms = (time.perf_counter() - t1) * 1e3 / k
+        res["cpu_baseline"] = {"kind": "port", "cores": 1, "ms_per_step": cms, "steps": k,
+                               "sample": f"oracle box step with chunks (coloured order), the same scene, steps {warm + 1}-{warm + k}",
+                               "gpu_over_cpu": cms / ms}
+        log(f"chunks cpu: {cms:.1f} ms/step ({cms / ms:.1f}x)")
+    return res
+
+
 def _bricks_of(bricks):
     from shatterforge_amd import structure
     b = structure.Bricks()
@@ -600,8 +656,8 @@ def main():
                     "world's extensions off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-lattice", action="store_true", help="skip the legacy brick frame and what-if measurements")
-    ap.add_argument("--parts", default="box,lattice,whatif,load,site,particles,structure",
-                    help="comma list of box, lattice, whatif, load, site, particles, structure, copy (a device copy of "
+    ap.add_argument("--parts", default="box,lattice,whatif,load,site,particles,structure,chunks",
+                    help="comma list of box, lattice, whatif, load, site, particles, structure, chunks, copy (a device copy of "
                          "known size by the bench-only program shatterforge_amd/sfb_copy)")
     ap.add_argument("--whatif-queries", type=int, default=64)
     ap.add_argument("--load-batch", type=int, default=0, help="also run the load-path batch of this many structures")
@@ -651,6 +707,8 @@ def main():
         out["particles"] = bench_particles()
     if "structure" in parts:
         out["structure"] = bench_structure(cpu=not args.no_cpu)
+    if "chunks" in parts:
+        out["chunks"] = bench_chunks(cpu=not args.no_cpu)
     if not args.no_cpu and scene is not None:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(*scene, args.cpu_seconds, sync=sync)
