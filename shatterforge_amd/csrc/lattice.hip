@@ -363,6 +363,7 @@ __global__ void __launch_bounds__(NT) k_plough_flag(LD D) {
             if (mj >= SLUG || mom < T.capv[mj] * 0.35f) continue;
         }
         atomicAdd(&D.ctr[C_NPLOUGH], 1u);
+        atomicAdd(&D.xpcnt[D.cscene[c]], 1u);
         D.pflag[c] = 1;
     }
 }
@@ -1067,6 +1068,7 @@ __global__ void __launch_bounds__(NT) k_impact_flag(LD D) {
         bool crush = !(vn0 < 1.0f) && !(0.5f * jn * vn0 < 150.0f);
         if (!hit && !crush) continue;
         atomicAdd(&D.ctr[C_NIMPACT], 1u);
+        atomicAdd(&D.xicnt[D.cscene[c]], 1u);
         D.iflag[c] = 1;
     }
 }
@@ -1565,13 +1567,16 @@ __global__ void __launch_bounds__(64) k_crush(LD D, CrushP P) {
     const SceneK K = D.sc[s];
     const uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
     const uint32_t c0 = min(D.sc_c0[s], ncs), c1 = s + 1 < (int)D.nscene ? min(D.sc_c0[s + 1], ncs) : ncs;
+    uint32_t left = D.xicnt[s];   // (the scene's candidates; at least as many as it flagged)
+    if (left == 0) return;
     uint32_t rng = P.rng[s];
     uint32_t bud = 2500u;
-    for (uint32_t cb = c0; cb < c1; cb += 64) {
+    for (uint32_t cb = c0; cb < c1 && left > 0; cb += 64) {
         const uint32_t cl = cb + (uint32_t)lane;
         const bool f = cl < c1 && D.iflag[cl] != 0;
         unsigned long long m = __ballot(f);
         if (f) D.iflag[cl] = 0;
+        left -= min(left, (uint32_t)__popcll(m));
         while (m) {
             const int l = __ffsll((long long)m) - 1;
             m &= m - 1;
@@ -1603,7 +1608,7 @@ __global__ void __launch_bounds__(64) k_crush(LD D, CrushP P) {
             if (soft_b > 0.0f && bud > 0) { const uint32_t k = x_crush(D, P, K, W, rng, (uint32_t)jb, e - ea, p, bud); bud -= k < bud ? k : bud; }
         }
     }
-    if (lane == 0) P.rng[s] = rng;
+    if (lane == 0) { P.rng[s] = rng; D.xicnt[s] = 0u; }
 }
 __global__ void __launch_bounds__(NT) k_crush_gather(LD D, const uint32_t* log, const uint32_t* nlog, uint32_t cap, float4* bond_out, uint32_t* mat_out) {
     const uint32_t n = min(*nlog, cap);
@@ -2578,7 +2583,7 @@ int lat_alloc(Lat* L) {
     r |= L->dalloc(D.wpos, B, &L->per_brick, 1); r |= L->dalloc(D.phi, B, &L->per_brick, 1); r |= L->dalloc(D.ext, B, &L->per_brick, 1); r |= L->dalloc(D.extn, B, &L->per_brick, 1);
     r |= L->dalloc(D.stress, B, &L->per_brick, 1); r |= L->dalloc(D.sup, B, &L->per_brick, 1);
     r |= L->dalloc(D.creach, B, &L->per_brick, 1); r |= L->dalloc(D.cq, B, &L->per_brick, 1);
-    r |= L->dalloc(D.xstamp, B, &L->per_brick, 1); r |= L->dalloc(D.xtok, S);
+    r |= L->dalloc(D.xstamp, B, &L->per_brick, 1); r |= L->dalloc(D.xtok, S); r |= L->dalloc(D.xicnt, S); r |= L->dalloc(D.xpcnt, S);
     r |= L->dalloc(D.bd_scene, N, &L->per_body, 1); r |= L->dalloc(D.alive, N, &L->per_body, 1); r |= L->dalloc(D.asleep, N, &L->per_body, 1); r |= L->dalloc(D.wake, N, &L->per_body, 1);
     r |= L->dalloc(D.pos, N, &L->per_body, 1); r |= L->dalloc(D.q, N, &L->per_body, 1); r |= L->dalloc(D.v, N, &L->per_body, 1); r |= L->dalloc(D.w, N, &L->per_body, 1);
     r |= L->dalloc(D.vpre, N, &L->per_body, 1); r |= L->dalloc(D.wpre, N, &L->per_body, 1); r |= L->dalloc(D.vlast, N, &L->per_body, 1); r |= L->dalloc(D.wlast, N, &L->per_body, 1);
@@ -2918,13 +2923,16 @@ __global__ void __launch_bounds__(64) k_plough(LD D, CrushP P, uint32_t* cur, ui
     const SceneK K = D.sc[s];
     const uint32_t ncs = min(D.ctr[C_NCS], D.ccap);
     const uint32_t c0 = min(D.sc_c0[s], ncs), c1 = s + 1 < (int)D.nscene ? min(D.sc_c0[s + 1], ncs) : ncs;
+    uint32_t left = D.xpcnt[s];   // (the scene's candidates; at least as many as it flagged)
+    if (left == 0) return;
     uint32_t rng = P.rng[s], cursor = cur[s], hw = hwv[s];
     const LM idm = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-    for (uint32_t cb = c0; cb < c1; cb += 64) {
+    for (uint32_t cb = c0; cb < c1 && left > 0; cb += 64) {
         const uint32_t cl = cb + (uint32_t)lane;
         const bool f = cl < c1 && D.pflag[cl] != 0;
         unsigned long long m = __ballot(f);
         if (f) D.pflag[cl] = 0;
+        left -= min(left, (uint32_t)__popcll(m));
         while (m) {
             const int l = __ffsll((long long)m) - 1;
             m &= m - 1;
@@ -3028,7 +3036,7 @@ __global__ void __launch_bounds__(64) k_plough(LD D, CrushP P, uint32_t* cur, ui
             __syncthreads();
         }
     }
-    if (lane == 0) { P.rng[s] = rng; cur[s] = cursor; hwv[s] = hw; }
+    if (lane == 0) { P.rng[s] = rng; cur[s] = cursor; hwv[s] = hw; D.xpcnt[s] = 0u; }
 }
 __global__ void __launch_bounds__(NT) k_plough_gather(LD D, const uint32_t* log, const uint32_t* nlog, uint32_t cap, float4* v_out, uint32_t* mat_out) {
     const uint32_t n = min(*nlog, cap);

@@ -151,6 +151,8 @@ struct LD {
     uint8_t* pflag;            // plough candidates, likewise
     uint32_t* xstamp;
     uint32_t* xtok;
+    uint32_t* xicnt;           // per scene: impact and plough candidates flagged (zeroed by the walks)
+    uint32_t* xpcnt;
     // scan scratch
     uint32_t* scan_tmp;
 };
