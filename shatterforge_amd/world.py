@@ -78,7 +78,7 @@ def params(substeps=12, sleep=True, **kw):
 
 
 class World:
-    """One GPU box world (a handle of the C ABI)."""
+    """One box-physics world on the GPU (a handle of the C ABI)."""
 
     def __init__(self, body_cap, shape_cap, prm=None):
         L = lib()
